@@ -1,0 +1,115 @@
+"""ctypes binding of ``libarwen_dycore.so`` (see include/arwen_dycore.h).
+
+The shared library is built in-tree by :mod:`arwen_amd.build`.  Loading
+fails loudly when it is missing: there is no CPU fallback for the step.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_NAME = "libarwen_dycore.so"
+LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
+
+ABI_VERSION = 1
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+_F = ctypes.c_float
+_D = ctypes.c_double
+
+DOMAIN_ARRAYS = (
+    "u", "v", "w", "thp", "php", "mup",
+    "p", "al", "alt",
+    "u0", "v0", "w0", "thp0", "php0", "mup0",
+    "ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t",
+    "u_pp", "v_pp", "w_pp", "th_pp", "ph_pp", "mu_pp",
+    "p_pp", "p_pp_old", "ww_pp", "al_pp",
+    "thb", "pb", "alb", "phb", "mub2d", "ht",
+    "c1h", "c2h", "c1f", "c2f", "dnw", "rdnw", "dn", "rdn", "fnm", "fnp",
+)
+SCRATCH_ARRAYS = ("ww", "c2a", "a", "alpha", "gam", "dmdt_rf")
+DIFF_ARRAYS = ("rdzf_h", "rdzc_h", "rdzf_w", "rdzc_w")
+
+
+class ArwDomain(ctypes.Structure):
+    _fields_ = ([("nz", _I), ("ny", _I), ("nx", _I), ("base3d", _I)]
+                + [(n, _P) for n in DOMAIN_ARRAYS]
+                + [(n, _F) for n in ("cf1", "cf2", "cf3", "cfn", "cfn1")]
+                + [(n, _P) for n in SCRATCH_ARRAYS]
+                + [(n, _P) for n in DIFF_ARRAYS])
+
+
+class ArwParams(ctypes.Structure):
+    _fields_ = [("dx", _D), ("dy", _D), ("dt", _D),
+                ("time_step_sound", _I),
+                ("epssm", _D), ("smdiv", _D),
+                ("khdif", _D), ("kvdif", _D),
+                ("damp_opt", _I), ("zdamp", _D), ("dampcoef", _D),
+                ("top_lid", _I)]
+
+
+class ArwenError(RuntimeError):
+    """A library entry point returned a nonzero status."""
+
+
+_lib = None
+
+
+def load():
+    """Load (once) and return the library handle."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ArwenError(
+            f"{LIB_NAME} is not built ({LIB_PATH}); run "
+            "`python -c 'import arwen_amd.build as b; b.build()'` first -- "
+            "the dycore step has no CPU fallback")
+    lib = ctypes.CDLL(os.fspath(LIB_PATH))
+    dp, pp = ctypes.POINTER(ArwDomain), ctypes.POINTER(ArwParams)
+    sigs = {
+        "arw_abi_version": ([], _I),
+        "arw_last_error": ([], ctypes.c_char_p),
+        "arw_step": ([dp, pp, _P], _I),
+        "arw_calc_p_alpha": ([dp, _P], _I),
+        "arw_stage_fluxes": ([dp, pp, _P, _P, _P], _I),
+        "arw_stage_prepare": ([dp, pp, _D, _I, _P], _I),
+        "arw_acoustic_substep": ([dp, pp, _D, _I, _P], _I),
+        "arw_calc_coefs": ([dp, pp, _D, _P], _I),
+        "arw_small_step_finish": ([dp, _P], _I),
+        "arw_set_w_surface": ([dp, pp, _P], _I),
+        "arw_health": ([dp, _P, _P], _I),
+        "arw_domain_mass": ([dp, _P, _P], _I),
+    }
+    flux = [_P, _P, _P, _P, _P, _P, _P, _P, _F, _F, _I, _I, _I, _P]
+    for name in ("arw_flux_div_scalar", "arw_flux_div_u", "arw_flux_div_v",
+                 "arw_flux_div_w"):
+        sigs[name] = (flux, _I)
+    for name, (args, res) in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if lib.arw_abi_version() != ABI_VERSION:
+        raise ArwenError(f"{LIB_NAME} ABI {lib.arw_abi_version()} != "
+                         f"expected {ABI_VERSION}; rebuild the library")
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().arw_last_error()
+        raise ArwenError(f"{what} failed ({rc}): "
+                         f"{msg.decode() if msg else ''}")
+
+
+def exported_symbols() -> list[str]:
+    """Names of the C entry points this binding declares."""
+    return ["arw_abi_version", "arw_last_error", "arw_step",
+            "arw_calc_p_alpha", "arw_stage_fluxes", "arw_stage_prepare",
+            "arw_acoustic_substep", "arw_calc_coefs", "arw_small_step_finish",
+            "arw_set_w_surface", "arw_health", "arw_domain_mass",
+            "arw_flux_div_scalar", "arw_flux_div_u", "arw_flux_div_v",
+            "arw_flux_div_w"]

@@ -1,0 +1,233 @@
+"""Host-side (FP64 NumPy) initial conditions for the dycore step.
+
+A :class:`HostCase` bundles a :class:`~arwen_amd.config.RunConfig`, the
+vertical coordinate, the base state and the initial prognostic fields.
+``arwen_amd.state.DomainState.from_host`` uploads it to the GPU; the test
+oracle builds its own state from the same object.
+
+Builders:
+
+* :func:`init_theta_perturbation` -- at-rest state plus a hydrostatically
+  rebalanced theta perturbation (reference state.py:961-993);
+* :func:`straka` -- Straka et al. (1993) density current (reference
+  verify/cases/straka.py:42-75), at the resolution and time step given;
+* :func:`stratified_hill` -- 3-D dry stratified flow over a Gaussian hill
+  with seeded smooth random u/v/theta' (250x200x49, dx=12 km);
+* :func:`warm_bubbles` -- stratified dry state with seeded +3 K cosine^2
+  warm bubbles (1000x1000x50, dx=1 km);
+* :func:`moist_tracer` -- stratified state carrying a seeded qv tracer.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import constants as c
+from .config import RunConfig
+from .grid import (BaseState, VerticalCoord, gaussian_hill, make_base_state,
+                   make_vertical_coord, rebalance_hydrostatic,
+                   rebalance_hydrostatic_terrain, surface_extrapolation)
+
+
+@dataclass
+class HostCase:
+    cfg: RunConfig
+    coord: VerticalCoord
+    base: BaseState
+    fields: dict = field(default_factory=dict)   # u v w thp php mup [qv..]
+
+    @property
+    def cf(self) -> dict:
+        return surface_extrapolation(self.coord)
+
+    def zeros_like_state(self):
+        nz, ny, nx = self.cfg.nz, self.cfg.ny, self.cfg.nx
+        return {"u": np.zeros((nz, ny, nx + 1)), "v": np.zeros((nz, ny + 1, nx)),
+                "w": np.zeros((nz + 1, ny, nx)), "thp": np.zeros((nz, ny, nx)),
+                "php": np.zeros((nz + 1, ny, nx)), "mup": np.zeros((ny, nx))}
+
+
+def height_half(base: BaseState) -> np.ndarray:
+    """Base-state half-level heights (m): (nz,) flat, (nz,ny,nx) terrain
+    (reference state.py:314-316)."""
+    return 0.5 * (base.phb[:-1] + base.phb[1:]) / c.G
+
+
+def cell_x(cfg: RunConfig) -> np.ndarray:
+    """Domain-centred cell-centre x coordinates (nx,)."""
+    return (np.arange(cfg.nx) + 0.5) * cfg.dx - 0.5 * cfg.nx * cfg.dx
+
+
+def cell_y(cfg: RunConfig) -> np.ndarray:
+    return (np.arange(cfg.ny) + 0.5) * cfg.dy - 0.5 * cfg.ny * cfg.dy
+
+
+def init_at_rest(cfg: RunConfig, coord: VerticalCoord,
+                 base: BaseState) -> HostCase:
+    case = HostCase(cfg, coord, base)
+    case.fields = case.zeros_like_state()
+    return case
+
+
+def init_theta_perturbation(cfg: RunConfig, coord: VerticalCoord,
+                            base: BaseState, thp_func) -> HostCase:
+    """At-rest state plus theta' = thp_func(x, z), with phi' chosen so
+    every column is discretely hydrostatic (reference state.py:961-993);
+    winds zero, mu' = 0."""
+    case = init_at_rest(cfg, coord, base)
+    z = height_half(base)
+    thp = np.asarray(thp_func(cell_x(cfg), z), dtype=np.float64)
+    thp = np.broadcast_to(thp, (cfg.nz, cfg.ny, cfg.nx)).copy()
+    if base.terrain_z is None:
+        th_total = base.thb[:, None, None] + thp
+        ph = rebalance_hydrostatic(th_total, base.mub, coord, cfg.p_surf)
+        php = ph - base.phb[:, None, None]
+    else:
+        th_total = base.thb + thp
+        php = rebalance_hydrostatic_terrain(th_total, base, coord) - base.phb
+    case.fields["thp"] = thp
+    case.fields["php"] = php
+    return case
+
+
+def set_periodic_duplicates(fields: dict) -> None:
+    """Make the staggered duplicate face column/row equal to face 0."""
+    fields["u"][..., -1] = fields["u"][..., 0]
+    fields["v"][:, -1, :] = fields["v"][:, 0, :]
+
+
+def stratified_sounding(n_bv: float = 0.01, theta0: float = 300.0):
+    """theta(z) = theta0 exp(N^2 z / g): constant Brunt-Vaisala N."""
+    return lambda z: theta0 * np.exp(n_bv * n_bv * np.asarray(z, float) / c.G)
+
+
+def isentropic_sounding(theta0: float = 300.0):
+    return lambda z: np.full_like(np.asarray(z, dtype=np.float64), theta0)
+
+
+# ---------------------------------------------------------------------------
+# Benchmark workloads
+# ---------------------------------------------------------------------------
+
+def straka(nx=512, nz=64, dx=100.0, dt=1.0, time_step_sound=6,
+           k=75.0) -> HostCase:
+    """Straka density current: 300 K isentropic, cold blob T' = -15 K
+    (cos(pi L)+1)/2 inside L = sqrt((x/4 km)^2 + ((z-3 km)/2 km)^2) <= 1,
+    theta' = T'/Exner_b (reference verify/cases/straka.py:42-75)."""
+    cfg = RunConfig(nx=nx, ny=1, nz=nz, dx=dx, dy=dx, ztop=nz * dx, dt=dt,
+                    run_seconds=900.0, khdif=k, kvdif=k,
+                    time_step_sound=time_step_sound, case="straka")
+    coord = make_vertical_coord(nz)
+    base = make_base_state(coord, isentropic_sounding(), cfg.p_surf,
+                           cfg.ztop)
+    exner = (np.asarray(base.pb, dtype=np.float64) / c.P0) ** c.RCP
+
+    def thp(x, z):
+        L = np.sqrt((x[None, None, :] / 4000.0) ** 2
+                    + ((z[:, None, None] - 3000.0) / 2000.0) ** 2)
+        tp = np.where(L <= 1.0, -15.0 * (np.cos(np.pi * L) + 1.0) / 2.0, 0.0)
+        return tp / exner[:, None, None]
+
+    return init_theta_perturbation(cfg, coord, base, thp)
+
+
+def smooth_noise(rng, shape, dx, dy, scale, sigma, dz_levels=None):
+    """Seeded white noise smoothed with a Gaussian spectral filter of
+    length ``scale`` (m) in x/y (and ``dz_levels`` levels in z), then
+    scaled to standard deviation ``sigma``.  Periodic in x and y."""
+    noise = rng.standard_normal(shape)
+    ny, nx = shape[-2], shape[-1]
+    kx = 2.0 * np.pi * np.fft.fftfreq(nx, d=dx)
+    ky = 2.0 * np.pi * np.fft.fftfreq(ny, d=dy)
+    filt = np.exp(-0.5 * scale ** 2 * (ky[:, None] ** 2 + kx[None, :] ** 2))
+    spec = np.fft.fft2(noise, axes=(-2, -1)) * filt
+    out = np.real(np.fft.ifft2(spec, axes=(-2, -1)))
+    if dz_levels and out.ndim == 3 and out.shape[0] > 1:
+        kz = 2.0 * np.pi * np.fft.fftfreq(out.shape[0])
+        fz = np.exp(-0.5 * (dz_levels * kz) ** 2)
+        out = np.real(np.fft.ifft(np.fft.fft(out, axis=0)
+                                  * fz[:, None, None], axis=0))
+    out -= out.mean()
+    std = out.std()
+    return out * (sigma / std if std > 0 else 0.0)
+
+
+def stratified_hill(nx=250, ny=200, nz=49, dx=12000.0, dt=60.0,
+                    time_step_sound=4, seed=20260731, ztop=20000.0,
+                    u0=10.0, n_bv=0.01, hill=1000.0, radius=50000.0,
+                    damp_opt=3) -> HostCase:
+    """3-D dry stratified flow over a Gaussian hill: theta = 300
+    exp(N^2 z/g), u = u0 + u', v = v', theta' smooth seeded noise (sigma
+    1 m/s, 1 m/s, 0.5 K at a 200 km scale), hybrid coordinate, terrain
+    base state, rigid lid with the w damper over the top 5 km."""
+    cfg = RunConfig(nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, ztop=ztop, dt=dt,
+                    time_step_sound=time_step_sound, terrain_opt=1,
+                    hybrid_opt=2, damp_opt=damp_opt, zdamp=5000.0,
+                    dampcoef=0.2, case="stratified_hill")
+    coord = make_vertical_coord(nz, hybrid_opt=2, etac=0.2)
+    tz = gaussian_hill(nx, ny, dx, dx, hill, radius)
+    base = make_base_state(coord, stratified_sounding(n_bv), cfg.p_surf,
+                           ztop, terrain_z=tz)
+    rng = np.random.default_rng(seed)
+    th_noise = smooth_noise(rng, (nz, ny, nx), dx, dx, 200e3, 0.5, 3)
+    case = init_theta_perturbation(cfg, coord, base, lambda x, z: th_noise)
+    u = u0 + smooth_noise(rng, (nz, ny, nx), dx, dx, 200e3, 1.0, 3)
+    v = smooth_noise(rng, (nz, ny, nx), dx, dx, 200e3, 1.0, 3)
+    case.fields["u"][..., :nx] = u
+    case.fields["v"][:, :ny, :] = v
+    set_periodic_duplicates(case.fields)
+    set_w_surface_host(case)
+    return case
+
+
+def warm_bubbles(nx=1000, ny=1000, nz=50, dx=1000.0, dt=6.0,
+                 time_step_sound=4, seed=20260731, ztop=15000.0,
+                 n_bubbles=16, amp=3.0, n_bv=0.01, u0=5.0,
+                 v0=2.0) -> HostCase:
+    """Flat stratified dry state with ``n_bubbles`` seeded +3 K cosine^2
+    bubbles (horizontal radius 10 km, vertical 1.5 km, centred at 2 km)
+    and a uniform background wind (u0, v0)."""
+    cfg = RunConfig(nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, ztop=ztop, dt=dt,
+                    time_step_sound=time_step_sound, case="warm_bubbles")
+    coord = make_vertical_coord(nz)
+    base = make_base_state(coord, stratified_sounding(n_bv), cfg.p_surf, ztop)
+    rng = np.random.default_rng(seed)
+    xs = rng.uniform(-0.5, 0.5, n_bubbles) * nx * dx
+    ys = rng.uniform(-0.5, 0.5, n_bubbles) * ny * dx
+    x, y = cell_x(cfg), cell_y(cfg)
+    Lx, Lz = 10000.0, 1500.0
+
+    def thp(_x, z):
+        out = np.zeros((nz, ny, nx))
+        zz = z[:, None, None]
+        for bx, by in zip(xs, ys):
+            ddx = (x[None, None, :] - bx + 0.5 * nx * dx) % (nx * dx) \
+                - 0.5 * nx * dx
+            ddy = (y[None, :, None] - by + 0.5 * ny * dx) % (ny * dx) \
+                - 0.5 * ny * dx
+            L = np.sqrt((ddx / Lx) ** 2 + (ddy / Lx) ** 2
+                        + ((zz - 2000.0) / Lz) ** 2)
+            out += np.where(L < 1.0, amp * np.cos(0.5 * np.pi * L) ** 2, 0.0)
+        return out
+
+    case = init_theta_perturbation(cfg, coord, base, thp)
+    case.fields["u"][...] = u0
+    case.fields["v"][...] = v0
+    return case
+
+
+def set_w_surface_host(case: HostCase) -> None:
+    """w(sfc) = u . grad(ht) on the host (reference dycore.py:1999-2026)."""
+    cfg, f = case.cfg, case.fields
+    cf = case.cf
+    ht = (np.zeros((cfg.ny, cfg.nx)) if case.base.terrain_z is None
+          else case.base.terrain_z)
+    uc = cf["cf1"] * f["u"][0] + cf["cf2"] * f["u"][1] + cf["cf3"] * f["u"][2]
+    vc = cf["cf1"] * f["v"][0] + cf["cf2"] * f["v"][1] + cf["cf3"] * f["v"][2]
+    dyn = np.roll(ht, -1, 0) - ht
+    dys = ht - np.roll(ht, 1, 0)
+    dxe = np.roll(ht, -1, 1) - ht
+    dxw = ht - np.roll(ht, 1, 1)
+    f["w"][0] = ((0.5 / cfg.dy) * (dyn * vc[1:] + dys * vc[:-1])
+                 + (0.5 / cfg.dx) * (dxe * uc[:, 1:] + dxw * uc[:, :-1]))

@@ -1,0 +1,121 @@
+"""Run configuration: the subset of the reference ``RunConfig`` the dycore
+step reads (gpuwm/config.py:27-190), with identical field names and
+defaults, plus the validation the step performs before any device work.
+
+Fields for subsystems this build does not implement (physics schemes,
+nesting, open lateral boundaries, Smagorinsky, sixth-order filter) are
+accepted with their reference defaults so a reference config constructs
+unchanged; :func:`validate_for_step` refuses any non-default value with a
+``NotImplementedError`` instead of silently ignoring it.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, fields, replace
+
+
+@dataclass(frozen=True)
+class RunConfig:
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    ztop: float
+    dt: float
+    run_seconds: float = 0.0
+    clock_dt: float = 0.0
+    p_surf: float = 1.0e5
+    time_step_sound: int = 4
+    epssm: float = 0.1
+    smdiv: float = 0.1
+    khdif: float = 0.0
+    kvdif: float = 0.0
+    damp_opt: int = 0
+    zdamp: float = 5000.0
+    dampcoef: float = 0.2
+    output_interval_s: float = 300.0
+    case: str = ""
+    hybrid_opt: int = 0
+    etac: float = 0.2
+    moist: bool = False
+    mp_physics: int = 0
+    moist_adv_opt: int = 1
+    diff_6th_opt: int = 0
+    diff_6th_factor: float = 0.12
+    km_opt: int = 1
+    w_damping: int = 0
+    open_x: bool = False
+    open_y: bool = False
+    specified: bool = False
+    spec_zone: int = 1
+    emdiv: float = 0.0
+    h_sca_adv_order: int = 2
+    terrain_opt: int = 0
+    hill_height: float = 100.0
+    hill_halfwidth: float = 10000.0
+    hypsometric_opt: int = 1
+    nested: bool = False
+    top_lid: bool = True
+    moist_cq: bool = False
+    bl_pbl_physics: int = 0
+    ra_physics: int = 0
+    cu_physics: int = 0
+    sf_sfclay_physics: int = 0
+    sf_surface_physics: int = 0
+    nwp_diagnostics: int = 0
+
+    def with_(self, **kw) -> "RunConfig":
+        """Copy with fields replaced (frozen dataclass convenience)."""
+        return replace(self, **kw)
+
+
+def config_fields() -> tuple[str, ...]:
+    return tuple(f.name for f in fields(RunConfig))
+
+
+def validate_for_step(cfg: RunConfig, *, acoustic: bool = True) -> None:
+    """Fail-closed checks made before any device work.
+
+    Mirrors the guards of gpuwm/core/dycore.py:2238-2293 for the features
+    this build carries, and refuses (NotImplementedError) the reference
+    features that are outside the ported path.
+    """
+    if acoustic and cfg.time_step_sound % 2 != 0:
+        raise ValueError(
+            f"time_step_sound must be even, got {cfg.time_step_sound}: RK3 "
+            "stage 2 runs time_step_sound//2 acoustic substeps of "
+            "dt/time_step_sound, which mis-times the dt/2 stage for odd "
+            "values.")
+    if cfg.time_step_sound < 1:
+        raise ValueError("time_step_sound must be positive")
+    if cfg.km_opt not in (0, 1):
+        raise NotImplementedError(
+            f"km_opt={cfg.km_opt} (Smagorinsky) is not part of this port; "
+            "only km_opt=1 constant-K diffusion is implemented")
+    if not 0.0 < cfg.epssm <= 1.0:
+        raise ValueError(f"epssm must lie in (0, 1], got {cfg.epssm}")
+    if cfg.smdiv < 0.0:
+        raise ValueError(f"smdiv must be >= 0, got {cfg.smdiv}")
+    if cfg.damp_opt not in (0, 3):
+        raise ValueError(f"damp_opt must be 0 or 3, got {cfg.damp_opt}")
+    if cfg.damp_opt == 3 and cfg.zdamp <= 0.0:
+        raise ValueError("damp_opt=3 needs zdamp > 0")
+    if cfg.h_sca_adv_order != 2:
+        raise NotImplementedError(
+            "only h_sca_adv_order=2 (the reference default) is ported")
+    if cfg.hypsometric_opt != 1:
+        raise NotImplementedError(
+            "only hypsometric_opt=1 (the reference default) is ported")
+    for name in ("open_x", "open_y", "specified", "nested", "moist_cq"):
+        if getattr(cfg, name):
+            raise NotImplementedError(f"{name}=True is not part of this port")
+    if cfg.emdiv > 0.0 or cfg.w_damping or cfg.diff_6th_opt:
+        raise NotImplementedError(
+            "emdiv / w_damping / diff_6th_opt are not part of this port")
+    if cfg.mp_physics != 0 or cfg.bl_pbl_physics or cfg.ra_physics \
+            or cfg.cu_physics or cfg.sf_sfclay_physics \
+            or cfg.sf_surface_physics:
+        raise NotImplementedError(
+            "physics schemes are outside the dycore step port")
+    if cfg.khdif < 0.0 or cfg.kvdif < 0.0:
+        raise ValueError("khdif/kvdif must be non-negative")

@@ -1,0 +1,197 @@
+"""The dycore step: the reference's Python call surface over the HIP library.
+
+``step(state, cfg, ...)`` keeps the signature of gpuwm/core/dycore.py:
+2161-2163 and mutates ``state`` in place; ``run_steps`` follows
+dycore.py:2506-2510.  All arithmetic runs in ``libarwen_dycore.so`` on the
+current PyTorch stream; this module validates the configuration (raising
+before any device work, like dycore.py:2238-2293), builds the raw-pointer
+views and calls the C-ABI.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import _lib
+from .config import RunConfig, validate_for_step
+from .state import DomainState, mu_at_u_faces, mu_at_v_faces
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def params(cfg: RunConfig) -> _lib.ArwParams:
+    p = _lib.ArwParams()
+    p.dx, p.dy, p.dt = float(cfg.dx), float(cfg.dy), float(cfg.dt)
+    p.time_step_sound = int(cfg.time_step_sound)
+    p.epssm, p.smdiv = float(cfg.epssm), float(cfg.smdiv)
+    if cfg.km_opt == 1:
+        p.khdif, p.kvdif = float(cfg.khdif), float(cfg.kvdif)
+    else:
+        p.khdif = p.kvdif = 0.0
+    p.damp_opt = int(cfg.damp_opt)
+    p.zdamp, p.dampcoef = float(cfg.zdamp), float(cfg.dampcoef)
+    p.top_lid = int(bool(cfg.top_lid))
+    return p
+
+
+def _uses_diffusion(cfg: RunConfig) -> bool:
+    return cfg.km_opt == 1 and (cfg.khdif > 0.0 or cfg.kvdif > 0.0)
+
+
+def _check_state(state: DomainState) -> None:
+    if not state.u.is_cuda:
+        raise RuntimeError("the dycore step runs on the GPU: state tensors "
+                           "must live on a ROCm device")
+    if state.has_msf or state.rotational:
+        raise NotImplementedError(
+            "map factors / Coriolis are not part of this port")
+
+
+def step(state: DomainState, cfg: RunConfig, *, acoustic: bool = True,
+         mass_flux_observer=None, mass_flux_accumulator=None,
+         refl_10cm_due: bool = False) -> None:
+    """Advance ``state`` one RK3 split-explicit step of length cfg.dt."""
+    if mass_flux_observer is not None and mass_flux_accumulator is not None:
+        raise ValueError(
+            "mass_flux_observer and mass_flux_accumulator are mutually "
+            "exclusive")
+    if not acoustic:
+        raise NotImplementedError(
+            "step(acoustic=False) (the advection-only test path) is not "
+            "part of this port")
+    if mass_flux_observer is not None or mass_flux_accumulator is not None:
+        raise NotImplementedError(
+            "boundary mass-flux observers apply to open boundaries, which "
+            "are not part of this port (periodic domains conserve mass)")
+    validate_for_step(cfg, acoustic=acoustic)
+    if cfg.moist or state.qv is not None:
+        raise NotImplementedError("moist transport is not wired yet")
+    _check_state(state)
+    lib = _lib.load()
+    d = state.domain_struct(diffusion=_uses_diffusion(cfg))
+    p = params(cfg)
+    _lib.check(lib.arw_step(d, p, _stream()), "arw_step")
+    state.elapsed_seconds += cfg.dt
+
+
+def run_steps(state: DomainState, cfg: RunConfig, n: int, *,
+              acoustic: bool = True) -> None:
+    for _ in range(n):
+        step(state, cfg, acoustic=acoustic)
+
+
+def update_diagnostics(state: DomainState, hypsometric_opt: int = 1) -> None:
+    """Recompute p, al, alt from (thp, php, mup) (reference
+    diagnostics.py:14-57)."""
+    if hypsometric_opt != 1:
+        raise NotImplementedError("only hypsometric_opt=1 is ported")
+    _check_state(state)
+    _lib.check(_lib.load().arw_calc_p_alpha(state.domain_struct(), _stream()),
+               "arw_calc_p_alpha")
+
+
+def stage_fluxes(state: DomainState, cfg: RunConfig):
+    """(ru, rv, ww) at the current state (reference dycore.py:162-206)."""
+    _check_state(state)
+    nz, ny, nx = state.shape
+    ru = state.scratch((nz, ny, nx + 1), "rk_ru")
+    rv = state.scratch((nz, ny + 1, nx), "rk_rv")
+    d = state.domain_struct()
+    _lib.check(_lib.load().arw_stage_fluxes(d, params(cfg), ru.data_ptr(),
+                                            rv.data_ptr(), _stream()),
+               "arw_stage_fluxes")
+    return ru, rv, state.scratch((nz + 1, ny, nx), "arw_ww")
+
+
+def set_w_surface(state: DomainState, cfg: RunConfig) -> None:
+    """w(sfc) = u . grad(ht) (reference dycore.py:1999-2026)."""
+    _check_state(state)
+    _lib.check(_lib.load().arw_set_w_surface(state.domain_struct(),
+                                             params(cfg), _stream()),
+               "arw_set_w_surface")
+
+
+def prepare_acoustic_coefficients(state: DomainState, cfg: RunConfig,
+                                  dtau: float) -> None:
+    """Implicit-w coefficients for the current reference state (reference
+    acoustic.py:299-327)."""
+    _check_state(state)
+    _lib.check(_lib.load().arw_calc_coefs(state.domain_struct(), params(cfg),
+                                          float(dtau), _stream()),
+               "arw_calc_coefs")
+
+
+def acoustic_substep(state: DomainState, cfg: RunConfig, dtau: float,
+                     first: bool) -> None:
+    """One acoustic substep with the prepared coefficients (reference
+    acoustic.py:482-567)."""
+    _check_state(state)
+    _lib.check(_lib.load().arw_acoustic_substep(
+        state.domain_struct(), params(cfg), float(dtau), int(bool(first)),
+        _stream()), "arw_acoustic_substep")
+
+
+def run_acoustic_only(state: DomainState, cfg: RunConfig, dtau: float,
+                      n: int) -> None:
+    """n substeps about the frozen reference state (reference
+    acoustic.py:570-583)."""
+    if n <= 0:
+        return
+    prepare_acoustic_coefficients(state, cfg, dtau)
+    for i in range(n):
+        acoustic_substep(state, cfg, dtau, first=(i == 0))
+
+
+def domain_mass_measure(state: DomainState) -> float:
+    """FP64 sum of the total dry column mass (reference dycore.py:209-218;
+    identity map factors)."""
+    _check_state(state)
+    out = torch.zeros(1, dtype=torch.float64, device=state.device)
+    _lib.check(_lib.load().arw_domain_mass(state.domain_struct(),
+                                           out.data_ptr(), _stream()),
+               "arw_domain_mass")
+    return float(out.item())
+
+
+def stability_report(state: DomainState, cfg: RunConfig | None = None) -> dict:
+    """max |u|, |w|, |theta'| and the co-located CFL with one device
+    reduction and one readback (reference dycore.py:2513-2582)."""
+    _check_state(state)
+    if state.u.numel() == 0 or state.w.numel() == 0:
+        raise ValueError("zero-size array to reduction operation maximum "
+                         "which has no identity")
+    out = torch.zeros(4, dtype=torch.float32, device=state.device)
+    _lib.check(_lib.load().arw_health(state.domain_struct(), out.data_ptr(),
+                                      _stream()), "arw_health")
+    u_max, w_max, th_max, wcfl = (float(x) for x in out.cpu().tolist())
+    nan = not (math.isfinite(u_max) and math.isfinite(w_max)
+               and math.isfinite(th_max))
+    rep = {"u_max": u_max, "w_max": w_max, "th_max": th_max, "cfl": None,
+           "horizontal_cfl": None, "vertical_cfl": None, "nan": nan}
+    if cfg is not None and not nan:
+        rep["horizontal_cfl"] = cfg.dt * u_max / cfg.dx
+        rep["vertical_cfl"] = cfg.dt * wcfl
+        rep["cfl"] = max(rep["horizontal_cfl"], rep["vertical_cfl"])
+    return rep
+
+
+def stability_gate_failed(report: dict, *, max_cfl: float,
+                          max_w_ms: float) -> bool:
+    """True when a health sample crosses a safety limit (reference
+    dycore.py:2585-2603)."""
+    cfl = report.get("cfl")
+    w_max = report.get("w_max")
+    return (bool(report.get("nan")) or cfl is None
+            or not math.isfinite(float(cfl)) or float(cfl) > max_cfl
+            or w_max is None or not math.isfinite(float(w_max))
+            or float(w_max) > max_w_ms)
+
+
+__all__ = ["step", "run_steps", "update_diagnostics", "stage_fluxes",
+           "set_w_surface", "stability_report", "stability_gate_failed",
+           "domain_mass_measure", "mu_at_u_faces", "mu_at_v_faces",
+           "run_acoustic_only", "acoustic_substep",
+           "prepare_acoustic_coefficients"]

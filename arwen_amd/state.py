@@ -1,0 +1,222 @@
+"""Domain state on PyTorch-ROCm tensors, laid out like the reference.
+
+``DomainState`` owns every array the step touches, with the reference's
+names, shapes and dtype (gpuwm/core/state.py:359-695): layout ``(nz, ny,
+nx)`` x-fastest, ``u (nz,ny,nx+1)``, ``v (nz,ny+1,nx)``, ``w, php
+(nz+1,ny,nx)``, FP32.  The arrays live in HBM as plain contiguous tensors;
+the HIP library receives raw pointers through :meth:`domain_struct`.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import constants as c
+from ._lib import DIFF_ARRAYS, DOMAIN_ARRAYS, SCRATCH_ARRAYS, ArwDomain
+from .config import RunConfig
+from .grid import BaseState, VerticalCoord, surface_extrapolation
+
+DTYPE = torch.float32
+
+_COORD = ("dnw", "rdnw", "dn", "rdn", "fnp", "fnm", "znu", "znw",
+          "c1h", "c2h", "c1f", "c2f", "c3h", "c4h", "c3f", "c4f")
+
+
+def mu_at_u_faces(mu: torch.Tensor) -> torch.Tensor:
+    """(ny, nx) -> (ny, nx+1) periodic face average (face f between cells
+    f-1 and f; face nx duplicates face 0) (reference state.py:339-346)."""
+    mux = 0.5 * (mu + torch.roll(mu, 1, dims=1))
+    return torch.cat([mux, mux[:, :1]], dim=1)
+
+
+def mu_at_v_faces(mu: torch.Tensor) -> torch.Tensor:
+    """(ny, nx) -> (ny+1, nx) (reference state.py:349-356)."""
+    muy = 0.5 * (mu + torch.roll(mu, 1, dims=0))
+    return torch.cat([muy, muy[:1]], dim=0)
+
+
+class DomainState:
+    """All prognostic, diagnostic, tendency, acoustic, base and coordinate
+    arrays of one domain (reference state.py:359-695), on ``device``."""
+
+    def __init__(self, cfg: RunConfig, device="cuda"):
+        nz, ny, nx = cfg.nz, cfg.ny, cfg.nx
+        self.cfg = cfg
+        self.device = torch.device(device)
+        z = lambda *sh: torch.zeros(sh, dtype=DTYPE, device=self.device)
+        self.u, self.v, self.w = z(nz, ny, nx + 1), z(nz, ny + 1, nx), \
+            z(nz + 1, ny, nx)
+        self.thp, self.php, self.mup = z(nz, ny, nx), z(nz + 1, ny, nx), \
+            z(ny, nx)
+        self.p, self.al, self.alt = z(nz, ny, nx), z(nz, ny, nx), \
+            z(nz, ny, nx)
+        if cfg.moist:
+            for q in ("qv", "qc", "qr"):
+                setattr(self, q, z(nz, ny, nx))
+                setattr(self, q + "0", z(nz, ny, nx))
+            self.h_diabatic = z(nz, ny, nx)
+        else:
+            self.qv = self.qc = self.qr = None
+            self.qv0 = self.qc0 = self.qr0 = None
+            self.h_diabatic = None
+        self.u0, self.v0, self.w0 = z(nz, ny, nx + 1), z(nz, ny + 1, nx), \
+            z(nz + 1, ny, nx)
+        self.thp0, self.php0, self.mup0 = z(nz, ny, nx), z(nz + 1, ny, nx), \
+            z(ny, nx)
+        self.ru_t, self.rv_t, self.rw_t = z(nz, ny, nx + 1), \
+            z(nz, ny + 1, nx), z(nz + 1, ny, nx)
+        self.rth_t, self.rph_t, self.rmu_t = z(nz, ny, nx), \
+            z(nz + 1, ny, nx), z(ny, nx)
+        self.u_pp, self.v_pp, self.w_pp = z(nz, ny, nx + 1), \
+            z(nz, ny + 1, nx), z(nz + 1, ny, nx)
+        self.th_pp, self.ph_pp, self.mu_pp = z(nz, ny, nx), \
+            z(nz + 1, ny, nx), z(ny, nx)
+        self.p_pp, self.p_pp_old = z(nz, ny, nx), z(nz, ny, nx)
+        self.ww_pp, self.al_pp = z(nz + 1, ny, nx), z(nz, ny, nx)
+        if cfg.terrain_opt == 0:
+            self.thb, self.pb, self.alb = z(nz), z(nz), z(nz)
+            self.phb = z(nz + 1)
+        else:
+            self.thb, self.pb, self.alb = z(nz, ny, nx), z(nz, ny, nx), \
+                z(nz, ny, nx)
+            self.phb = z(nz + 1, ny, nx)
+        self.mub = 0.0
+        self.p_top = None
+        self.mub2d, self.ht = z(ny, nx), z(ny, nx)
+        for name in _COORD:
+            setattr(self, name, z(nz + 1 if name.endswith("f")
+                                  or name == "znw" else nz))
+        self.msft = torch.ones((ny, nx), dtype=DTYPE, device=self.device)
+        self.msfu = torch.ones((ny, nx + 1), dtype=DTYPE, device=self.device)
+        self.msfv = torch.ones((ny + 1, nx), dtype=DTYPE, device=self.device)
+        self.f, self.e = z(ny, nx), z(ny, nx)
+        self.has_msf = False
+        self.rotational = False
+        self.cf1 = self.cf2 = self.cf3 = self.cfn = self.cfn1 = 0.0
+        self.elapsed_seconds = 0.0
+        self._scratch: dict[str, torch.Tensor] = {}
+        self._phb_host: np.ndarray | None = None
+
+    # -- setup --------------------------------------------------------------
+
+    def load_base(self, coord: VerticalCoord, base: BaseState) -> None:
+        """Copy the FP64 host coordinate/base arrays to FP32 device arrays
+        (reference state.py:722-771)."""
+        put = lambda t, a: t.copy_(torch.as_tensor(
+            np.asarray(a, dtype=np.float32)))
+        for name in _COORD:
+            put(getattr(self, name), getattr(coord, name))
+        for name in ("thb", "pb", "alb", "phb"):
+            host = np.asarray(getattr(base, name), dtype=np.float64)
+            dev = getattr(self, name)
+            if host.ndim != dev.ndim:
+                raise ValueError(
+                    f"base state {name} is {host.ndim}-D but the state was "
+                    f"allocated for {dev.ndim}-D profiles: cfg.terrain_opt "
+                    "must match the terrain the base state was built with")
+            put(dev, host)
+        self.p_top = float(np.float32(base.p_top))
+        if np.ndim(base.mub) == 0:
+            self.mub = float(np.float32(base.mub))
+            self.mub2d.fill_(self.mub)
+        else:
+            self.mub = None
+            put(self.mub2d, base.mub)
+        if base.terrain_z is not None:
+            put(self.ht, base.terrain_z)
+        else:
+            self.ht.zero_()
+        self._phb_host = np.array(base.phb, dtype=np.float64, copy=True)
+        for name, val in surface_extrapolation(coord).items():
+            setattr(self, name, float(np.float32(val)))
+
+    @classmethod
+    def from_host(cls, case, device="cuda") -> "DomainState":
+        """Upload a host initial condition (``arwen_amd.cases.HostCase``)."""
+        s = cls(case.cfg, device=device)
+        s.load_base(case.coord, case.base)
+        for name, arr in case.fields.items():
+            getattr(s, name).copy_(torch.as_tensor(
+                np.asarray(arr, dtype=np.float32)))
+        return s
+
+    # -- reference helpers ----------------------------------------------------
+
+    def scratch(self, shape, slot: str) -> torch.Tensor:
+        """Persistent named scratch buffer (reference state.py:805-830)."""
+        shape = tuple(shape)
+        buf = self._scratch.get(slot)
+        if buf is None:
+            buf = torch.zeros(shape, dtype=DTYPE, device=self.device)
+            self._scratch[slot] = buf
+        elif tuple(buf.shape) != shape:
+            raise ValueError(f"scratch slot {slot!r} has shape "
+                             f"{tuple(buf.shape)}, requested {shape}")
+        return buf
+
+    def total_theta(self) -> torch.Tensor:
+        thb = self.thb
+        return (thb if thb.dim() == 3 else thb[:, None, None]) + self.thp
+
+    def total_mu(self) -> torch.Tensor:
+        return self.mub2d + self.mup
+
+    def height_half(self) -> np.ndarray:
+        if self._phb_host is None:
+            raise RuntimeError("height_half() called before load_base()")
+        return 0.5 * (self._phb_host[:-1] + self._phb_host[1:]) / c.G
+
+    @property
+    def shape(self):
+        return tuple(self.p.shape)
+
+    # -- C-ABI view -----------------------------------------------------------
+
+    def _diff_spacings(self):
+        """Constant-K vertical spacings from the FP32 base geopotential,
+        formed in double (reference diffusion.py:34-61, 117)."""
+        cached = getattr(self, "_diff_rdz", None)
+        if cached is not None:
+            return cached
+        if self.phb.dim() != 1:
+            raise NotImplementedError(
+                "constant-K diffusion assumes a flat base state")
+        zf = self.phb.detach().cpu().numpy().astype(np.float64) / c.G
+        zh = 0.5 * (zf[:-1] + zf[1:])
+        rdzf_w = 1.0 / np.diff(zf)
+        rdzc_w = np.zeros(zf.size)
+        rdzc_w[1:-1] = 1.0 / (zh[1:] - zh[:-1])
+        rdzf_h = 1.0 / np.diff(zh)
+        rdzc_h = 1.0 / np.diff(zf)
+        self._diff_rdz = [
+            torch.as_tensor(np.asarray(a, dtype=np.float32)).to(self.device)
+            for a in (rdzf_h, rdzc_h, rdzf_w, rdzc_w)]
+        return self._diff_rdz
+
+    def domain_struct(self, diffusion: bool = False) -> ArwDomain:
+        """Raw-pointer view of the state for the C-ABI."""
+        nz, ny, nx = self.shape
+        d = ArwDomain()
+        d.nz, d.ny, d.nx = nz, ny, nx
+        d.base3d = int(self.thb.dim() == 3)
+        for name in DOMAIN_ARRAYS:
+            t = getattr(self, name)
+            if not t.is_contiguous() or t.dtype != DTYPE:
+                raise ValueError(f"state.{name} must be contiguous float32")
+            setattr(d, name, t.data_ptr())
+        for name in ("cf1", "cf2", "cf3", "cfn", "cfn1"):
+            setattr(d, name, float(getattr(self, name)))
+        shapes = {"ww": (nz + 1, ny, nx), "c2a": (nz, ny, nx),
+                  "a": (nz + 1, ny, nx), "alpha": (nz + 1, ny, nx),
+                  "gam": (nz + 1, ny, nx), "dmdt_rf": (ny, nx)}
+        for name in SCRATCH_ARRAYS:
+            setattr(d, name, self.scratch(shapes[name],
+                                          "arw_" + name).data_ptr())
+        if diffusion:
+            for n, t in zip(DIFF_ARRAYS, self._diff_spacings()):
+                setattr(d, n, t.data_ptr())
+        return d
+
+    def numpy_fields(self, names=("u", "v", "w", "thp", "php", "mup")):
+        return {n: getattr(self, n).detach().cpu().numpy().copy()
+                for n in names}

@@ -1,0 +1,170 @@
+/*
+ * arwen_dycore.h -- C-ABI of the MI355X (gfx950) dycore-step library
+ * libarwen_dycore.so.
+ *
+ * The reference binds its device code through one seam,
+ * gpuwm/core/kernels/__init__.py:140-143 get_kernel(module, func), which
+ * returns a CuPy RawFunction called as kern(grid, block, args).  This
+ * library replaces that seam: every entry point takes plain device
+ * pointers (FP32 arrays in the reference layout (nz, ny, nx), x fastest,
+ * C-grid staggering u (nz,ny,nx+1), v (nz,ny+1,nx), w/php (nz+1,ny,nx)),
+ * integer extents, scalars and a hipStream_t passed as void*.  Nothing
+ * allocates: all storage is owned by the caller (the Python DomainState).
+ * Every entry point returns ARW_OK (0) or a negative ARW_E* code and never
+ * throws across the ABI; arw_last_error() returns a message for the last
+ * failure on the calling thread.
+ */
+#ifndef ARWEN_DYCORE_H
+#define ARWEN_DYCORE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ARW_ABI_VERSION 1
+
+#define ARW_OK 0
+#define ARW_E_ARG (-1)      /* invalid argument (null pointer, bad extent) */
+#define ARW_E_LAUNCH (-2)   /* a kernel launch or runtime call failed */
+#define ARW_E_UNSUPPORTED (-3)
+
+/*
+ * One domain's device arrays.  Field names follow the reference
+ * DomainState (gpuwm/core/state.py:389-695); scratch arrays replace the
+ * reference's named scratch slots (acoustic_c2a/_a/_alpha/_gamma,
+ * rk_ww, state.py:805-830).
+ */
+typedef struct arw_domain {
+    int32_t nz, ny, nx;
+    int32_t base3d;            /* thb/pb/alb/phb are (nz[+1],ny,nx) */
+    /* prognostics */
+    float *u, *v, *w, *thp, *php, *mup;
+    /* EOS diagnostics */
+    float *p, *al, *alt;
+    /* RK time-t copies */
+    float *u0, *v0, *w0, *thp0, *php0, *mup0;
+    /* coupled slow tendencies */
+    float *ru_t, *rv_t, *rw_t, *rth_t, *rph_t, *rmu_t;
+    /* acoustic perturbations */
+    float *u_pp, *v_pp, *w_pp, *th_pp, *ph_pp, *mu_pp;
+    float *p_pp, *p_pp_old, *ww_pp, *al_pp;
+    /* base state, terrain, base column mass */
+    const float *thb, *pb, *alb, *phb, *mub2d, *ht;
+    /* vertical coordinate */
+    const float *c1h, *c2h, *c1f, *c2f, *dnw, *rdnw, *dn, *rdn, *fnm, *fnp;
+    float cf1, cf2, cf3, cfn, cfn1;
+    /* scratch: ww (nz+1,ny,nx), c2a (nz,ny,nx), a/alpha/gam (nz+1,ny,nx),
+     * dmdt_rf (ny,nx) */
+    float *ww, *c2a, *a, *alpha, *gam, *dmdt_rf;
+    /* constant-K diffusion inverse spacings (reference diffusion.py:34-61):
+     * half-level fields rdzf_h (nz-1), rdzc_h (nz); w rdzf_w (nz),
+     * rdzc_w (nz+1).  May be NULL when khdif = kvdif = 0. */
+    const float *rdzf_h, *rdzc_h, *rdzf_w, *rdzc_w;
+} arw_domain;
+
+/* Step configuration (the RunConfig fields the step reads,
+ * gpuwm/config.py:28-190).  Scalars that the reference derives in
+ * double precision on the host and hands to its kernels as float32
+ * (rdx = float32(1/dx), dtau = float32(dt/ns), ...) are derived here the
+ * same way from these doubles. */
+typedef struct arw_params {
+    double dx, dy, dt;
+    int32_t time_step_sound;
+    double epssm, smdiv;
+    double khdif, kvdif;
+    int32_t damp_opt;
+    double zdamp, dampcoef;
+    int32_t top_lid;
+} arw_params;
+
+int32_t arw_abi_version(void);
+const char *arw_last_error(void);
+
+/* Whole RK3 split-explicit step, in place (replaces gpuwm/core/dycore.py:
+ * 2161-2503 ``step`` for periodic, msf = 1, dry states).  Leaves p/al/alt
+ * consistent with the new state, like the reference's closing
+ * update_diagnostics (dycore.py:2482). */
+int32_t arw_step(const arw_domain *d, const arw_params *prm, void *stream);
+
+/* EOS p, al, alt from thp, php, mup (replaces diagnostics.cu:37-89
+ * calc_p_alpha, launched by diagnostics.py:14-57; hypsometric_opt = 1). */
+int32_t arw_calc_p_alpha(const arw_domain *d, void *stream);
+
+/* Coupled mass fluxes and Omega at the current state (replaces
+ * dycore.py:162-206 stage_fluxes: couple_momentum + _omega_ref).  ru is
+ * (nz,ny,nx+1), rv (nz,ny+1,nx); Omega goes to d->ww. */
+int32_t arw_stage_fluxes(const arw_domain *d, const arw_params *prm,
+                         float *ru, float *rv, void *stream);
+
+/* Flux-form advection kernels, periodic, msf = 1 (replace advection.cu:
+ * 206-416 flux_div_scalar, 424-611 flux_div_u, 616-794 flux_div_v,
+ * 803-1051 flux_div_w, launched by advection.py:31-129).  They ADD into
+ * tend.  rdz is rdnw (scalar/u/v) or rdn (w). */
+int32_t arw_flux_div_scalar(const float *q, const float *ru, const float *rv,
+                            const float *rw, float *tend, const float *rdnw,
+                            const float *fnm, const float *fnp, float dx_inv,
+                            float dy_inv, int32_t nz, int32_t ny, int32_t nx,
+                            void *stream);
+int32_t arw_flux_div_u(const float *u, const float *ru, const float *rv,
+                       const float *rw, float *tend, const float *rdnw,
+                       const float *fnm, const float *fnp, float dx_inv,
+                       float dy_inv, int32_t nz, int32_t ny, int32_t nx,
+                       void *stream);
+int32_t arw_flux_div_v(const float *v, const float *ru, const float *rv,
+                       const float *rw, float *tend, const float *rdnw,
+                       const float *fnm, const float *fnp, float dx_inv,
+                       float dy_inv, int32_t nz, int32_t ny, int32_t nx,
+                       void *stream);
+int32_t arw_flux_div_w(const float *w, const float *ru, const float *rv,
+                       const float *rw, float *tend, const float *rdn,
+                       const float *fnm, const float *fnp, float dx_inv,
+                       float dy_inv, int32_t nz, int32_t ny, int32_t nx,
+                       void *stream);
+
+/* Stage preparation: EOS (when do_eos), Omega, the slow tendencies
+ * ru_t..rph_t (advection + PGF + buoyancy + rhs_ph + constant-K
+ * diffusion; replaces dycore.py:502-569 _add_slow_tendencies and
+ * diffusion.py:100-130), small-step initialisation (dycore.cu:947-1085)
+ * and the implicit-w coefficients (acoustic.cu:675-759 calc_coefs).
+ * dtau is the stage's acoustic step. */
+int32_t arw_stage_prepare(const arw_domain *d, const arw_params *prm,
+                          double dtau, int32_t do_eos, void *stream);
+
+/* One acoustic substep (replaces acoustic.py:330-479: advance_uv
+ * acoustic.cu:244-314, advance_mu_th :329-456, advance_w_phi :774-982).
+ * Uses the coefficients of the last arw_stage_prepare (or
+ * arw_calc_coefs).  first != 0 drops divergence damping (smdiv = 0). */
+int32_t arw_acoustic_substep(const arw_domain *d, const arw_params *prm,
+                             double dtau, int32_t first, void *stream);
+
+/* Implicit-w coefficients c2a/a/alpha/gam and dmdt_rf for the current
+ * reference state (acoustic.cu:675-759; acoustic.py:299-327). */
+int32_t arw_calc_coefs(const arw_domain *d, const arw_params *prm,
+                       double dtau, void *stream);
+
+/* Small-step finish (dycore.cu:1087-1198) followed by the EOS of the new
+ * state. */
+int32_t arw_small_step_finish(const arw_domain *d, void *stream);
+
+/* w(sfc) = u . grad(ht) (replaces dycore.py:1999-2026 set_w_surface). */
+int32_t arw_set_w_surface(const arw_domain *d, const arw_params *prm,
+                          void *stream);
+
+/* Health maxima (replaces health.cu:23-222 health_partial/health_final as
+ * used by dycore.py:2513-2582 stability_report): out[0..3] = max|u|,
+ * max|w|, max|thp|, max |w[k>=1]| / dz(k) with dz from the live
+ * geopotential.  NaNs propagate.  out is a device array of 4 floats. */
+int32_t arw_health(const arw_domain *d, float *out, void *stream);
+
+/* FP64 domain dry-mass sum(mub2d + mup) (replaces dycore.py:209-218
+ * domain_mass_measure, identity map factors).  out is one device
+ * double. */
+int32_t arw_domain_mass(const arw_domain *d, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARWEN_DYCORE_H */
