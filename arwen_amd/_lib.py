@@ -82,6 +82,10 @@ def load():
         "arw_set_w_surface": ([dp, pp, _P], _I),
         "arw_health": ([dp, _P, _P], _I),
         "arw_domain_mass": ([dp, _P, _P], _I),
+        "arw_profile_begin": ([_I], _I),
+        "arw_profile_end": ([_P, _P, _I], _I),
+        "arw_kernel_name": ([_I], ctypes.c_char_p),
+        "arw_kernel_count": ([], _I),
     }
     flux = [_P, _P, _P, _P, _P, _P, _P, _P, _F, _F, _I, _I, _I, _P]
     for name in ("arw_flux_div_scalar", "arw_flux_div_u", "arw_flux_div_v",
@@ -112,4 +116,31 @@ def exported_symbols() -> list[str]:
             "arw_acoustic_substep", "arw_calc_coefs", "arw_small_step_finish",
             "arw_set_w_surface", "arw_health", "arw_domain_mass",
             "arw_flux_div_scalar", "arw_flux_div_u", "arw_flux_div_v",
-            "arw_flux_div_w"]
+            "arw_flux_div_w", "arw_profile_begin", "arw_profile_end",
+            "arw_kernel_name", "arw_kernel_count"]
+
+
+class KernelTimer:
+    """Context manager collecting per-kernel HIP-event time of arw_step."""
+
+    def __init__(self, max_events: int = 1 << 16):
+        self.max_events = max_events
+        self.ms = {}
+        self.launches = {}
+
+    def __enter__(self):
+        check(load().arw_profile_begin(self.max_events), "arw_profile_begin")
+        return self
+
+    def __exit__(self, *exc):
+        lib = load()
+        n = lib.arw_kernel_count()
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int32 * n)()
+        check(lib.arw_profile_end(ctypes.cast(ms, _P), ctypes.cast(cnt, _P),
+                                  n), "arw_profile_end")
+        for i in range(n):
+            name = lib.arw_kernel_name(i).decode()
+            self.ms[name] = ms[i]
+            self.launches[name] = cnt[i]
+        return False

@@ -201,14 +201,22 @@ def warm_bubbles(nx=1000, ny=1000, nz=50, dx=1000.0, dt=6.0,
     def thp(_x, z):
         out = np.zeros((nz, ny, nx))
         zz = z[:, None, None]
+        hw = int(np.ceil(Lx / dx)) + 1          # half-width of the window
         for bx, by in zip(xs, ys):
-            ddx = (x[None, None, :] - bx + 0.5 * nx * dx) % (nx * dx) \
+            ii = np.arange(nx) if 2 * hw + 1 >= nx else (
+                np.round((bx + 0.5 * nx * dx) / dx).astype(int)
+                + np.arange(-hw, hw + 1)) % nx
+            jj = np.arange(ny) if 2 * hw + 1 >= ny else (
+                np.round((by + 0.5 * ny * dx) / dx).astype(int)
+                + np.arange(-hw, hw + 1)) % ny
+            ddx = (x[ii][None, None, :] - bx + 0.5 * nx * dx) % (nx * dx) \
                 - 0.5 * nx * dx
-            ddy = (y[None, :, None] - by + 0.5 * ny * dx) % (ny * dx) \
+            ddy = (y[jj][None, :, None] - by + 0.5 * ny * dx) % (ny * dx) \
                 - 0.5 * ny * dx
             L = np.sqrt((ddx / Lx) ** 2 + (ddy / Lx) ** 2
                         + ((zz - 2000.0) / Lz) ** 2)
-            out += np.where(L < 1.0, amp * np.cos(0.5 * np.pi * L) ** 2, 0.0)
+            out[:, jj[:, None], ii[None, :]] += np.where(
+                L < 1.0, amp * np.cos(0.5 * np.pi * L) ** 2, 0.0)
         return out
 
     case = init_theta_perturbation(cfg, coord, base, thp)

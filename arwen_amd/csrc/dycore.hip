@@ -20,6 +20,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <new>
+
 typedef arw_domain Dom;
 
 // Float32 scalars derived from the double-precision configuration exactly
@@ -1209,6 +1211,38 @@ __global__ void k_mass(Dom d, double* out)
 
 namespace {
 
+// Optional per-kernel timing: when active, an event is recorded on the
+// launch stream after every kernel of arw_step, and the interval since the
+// previous event is attributed to that kernel.  Events come from a pool
+// created by arw_profile_begin, so the timed step allocates nothing.
+enum KernelId {
+    KID_STAGE_COL = 0, KID_SLOW_TEND, KID_ADVANCE_UV, KID_ACOUSTIC_COL,
+    KID_FINISH_UV, KID_FINISH_COL, KID_SET_W_SURFACE, KID_COPY, KID_COUNT
+};
+const char* const kKernelNames[KID_COUNT] = {
+    "k_stage_col", "k_slow_tend", "k_advance_uv", "k_acoustic_col",
+    "k_finish_uv", "k_finish_col", "k_set_w_surface", "time_t_copies"};
+
+struct Profiler {
+    bool active = false;
+    int cap = 0, n = 0;
+    hipEvent_t* ev = nullptr;
+    int* kid = nullptr;
+    bool overflow = false;
+} g_prof;
+
+void prof_mark(int id, hipStream_t st)
+{
+    if (!g_prof.active) return;
+    if (g_prof.n >= g_prof.cap) {
+        g_prof.overflow = true;
+        return;
+    }
+    g_prof.kid[g_prof.n] = id;
+    hipEventRecord(g_prof.ev[g_prof.n], st);
+    ++g_prof.n;
+}
+
 int32_t check_domain(const Dom* d)
 {
     if (!d) return fail(ARW_E_ARG, "null domain");
@@ -1267,9 +1301,12 @@ int32_t stage_prepare(const Dom& d, const Sc& s, int do_eos, hipStream_t st)
                        do_eos, 1, 1);
     int32_t rc = launched("k_stage_col");
     if (rc) return rc;
+    prof_mark(KID_STAGE_COL, st);
     hipLaunchKernelGGL(k_slow_tend, union_grid(d, d.nz + 1), dim3(64, 4), 0,
                        st, d, s, 1);
-    return launched("k_slow_tend");
+    rc = launched("k_slow_tend");
+    prof_mark(KID_SLOW_TEND, st);
+    return rc;
 }
 
 int32_t substep(const Dom& d, const Sc& s, float* pout, hipStream_t st)
@@ -1278,9 +1315,12 @@ int32_t substep(const Dom& d, const Sc& s, float* pout, hipStream_t st)
                        d, s);
     int32_t rc = launched("k_advance_uv");
     if (rc) return rc;
+    prof_mark(KID_ADVANCE_UV, st);
     hipLaunchKernelGGL(k_acoustic_col, col_grid(d), dim3(256), 0, st, d, s,
                        pout);
-    return launched("k_acoustic_col");
+    rc = launched("k_acoustic_col");
+    prof_mark(KID_ACOUSTIC_COL, st);
+    return rc;
 }
 
 int32_t finish(const Dom& d, hipStream_t st)
@@ -1289,8 +1329,11 @@ int32_t finish(const Dom& d, hipStream_t st)
                        d);
     int32_t rc = launched("k_finish_uv");
     if (rc) return rc;
+    prof_mark(KID_FINISH_UV, st);
     hipLaunchKernelGGL(k_finish_col, col_grid(d), dim3(256), 0, st, d);
-    return launched("k_finish_col");
+    rc = launched("k_finish_col");
+    prof_mark(KID_FINISH_COL, st);
+    return rc;
 }
 
 }  // namespace
@@ -1456,10 +1499,12 @@ int32_t arw_step(const arw_domain* d, const arw_params* prm, void* stream)
         {d->u0, d->u, nz * ny * (nx + 1)}, {d->v0, d->v, nz * (ny + 1) * nx},
         {d->w0, d->w, (nz + 1) * ny * nx}, {d->thp0, d->thp, nz * ny * nx},
         {d->php0, d->php, (nz + 1) * ny * nx}, {d->mup0, d->mup, ny * nx}};
+    prof_mark(KID_COUNT, st);                    // opening event
     for (auto& cp : copies)
         if (hipMemcpyAsync(cp.dst, cp.src, cp.n * sizeof(float),
                            hipMemcpyDeviceToDevice, st) != hipSuccess)
             return fail(ARW_E_LAUNCH, "time-t copy failed");
+    prof_mark(KID_COPY, st);
     const int ns = prm->time_step_sound;
     const int nsub[3] = {1, ns / 2 > 1 ? ns / 2 : 1, ns};
     const double dtaus[3] = {prm->dt / 3.0, prm->dt / ns, prm->dt / ns};
@@ -1481,8 +1526,67 @@ int32_t arw_step(const arw_domain* d, const arw_params* prm, void* stream)
     Sc s = make_sc(prm, prm->dt, 1);
     hipLaunchKernelGGL(k_set_w_surface, col_grid(*d), dim3(256), 0, st, *d,
                        s);
-    return launched("k_set_w_surface");
+    rc = launched("k_set_w_surface");
+    prof_mark(KID_SET_W_SURFACE, st);
+    return rc;
 }
+
+int32_t arw_profile_begin(int32_t max_events)
+{
+    if (g_prof.active) return fail(ARW_E_ARG, "profiling already active");
+    if (max_events < 2) return fail(ARW_E_ARG, "max_events must be >= 2");
+    g_prof.ev = new (std::nothrow) hipEvent_t[max_events];
+    g_prof.kid = new (std::nothrow) int[max_events];
+    if (!g_prof.ev || !g_prof.kid) return fail(ARW_E_ARG, "out of memory");
+    for (int i = 0; i < max_events; ++i)
+        if (hipEventCreate(&g_prof.ev[i]) != hipSuccess) {
+            for (int j = 0; j < i; ++j) hipEventDestroy(g_prof.ev[j]);
+            delete[] g_prof.ev;
+            delete[] g_prof.kid;
+            g_prof = Profiler();
+            return fail(ARW_E_LAUNCH, "hipEventCreate failed");
+        }
+    g_prof.cap = max_events;
+    g_prof.n = 0;
+    g_prof.overflow = false;
+    g_prof.active = true;
+    return ARW_OK;
+}
+
+int32_t arw_profile_end(double* ms, int32_t* launches, int32_t n_kernels)
+{
+    if (!g_prof.active) return fail(ARW_E_ARG, "profiling not active");
+    int32_t rc = ARW_OK;
+    for (int k = 0; k < n_kernels; ++k) {
+        if (ms) ms[k] = 0.0;
+        if (launches) launches[k] = 0;
+    }
+    if (g_prof.n > 0 && hipEventSynchronize(g_prof.ev[g_prof.n - 1]) != hipSuccess)
+        rc = fail(ARW_E_LAUNCH, "hipEventSynchronize failed");
+    for (int i = 1; rc == ARW_OK && i < g_prof.n; ++i) {
+        int id = g_prof.kid[i];
+        if (g_prof.kid[i - 1] == KID_COUNT && id == KID_COUNT) continue;
+        if (id >= KID_COUNT || id >= n_kernels) continue;
+        float t = 0.0f;
+        hipEventElapsedTime(&t, g_prof.ev[i - 1], g_prof.ev[i]);
+        if (ms) ms[id] += t;
+        if (launches) launches[id] += 1;
+    }
+    if (g_prof.overflow && rc == ARW_OK)
+        rc = fail(ARW_E_ARG, "profile event pool overflowed");
+    for (int i = 0; i < g_prof.cap; ++i) hipEventDestroy(g_prof.ev[i]);
+    delete[] g_prof.ev;
+    delete[] g_prof.kid;
+    g_prof = Profiler();
+    return rc;
+}
+
+const char* arw_kernel_name(int32_t id)
+{
+    return (id >= 0 && id < KID_COUNT) ? kKernelNames[id] : "";
+}
+
+int32_t arw_kernel_count(void) { return KID_COUNT; }
 
 int32_t arw_health(const arw_domain* d, float* out, void* stream)
 {

@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X dycore step (one JSON line on stdout).
+
+Workload (default): the plain FP32 RK3 split-explicit step on a
+1000x1000x50 dry stratified domain (dx = 1 km, dt = 6 s, 4 acoustic
+substeps; seeded +3 K warm bubbles), the grid the throughput/roofline
+target is stated on.  Metric: Mpoint-steps/s (mass points x steps / s).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2]
+
+The timed region runs K steps with the state resident in HBM, bracketed
+by torch.cuda.synchronize().  Per-kernel durations over the same region
+come from HIP events the library records on its launch stream.  The CPU
+baseline (rank 0 only) times the float32 NumPy oracle on a bounded strip
+of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+
+#: Algorithmic FP32 words per mass point of one pass of the algorithm
+#: (each distinct 3-D input read once, each output written once, one HBM
+#: pass per required global synchronisation): stage pass S (19 words on
+#: stage 1, 24 on stages 2-3), acoustic substep pass U (24), finish F (15).
+WORDS_S1, WORDS_S23, WORDS_U, WORDS_F = 19, 24, 24, 15
+
+
+def step_words(ns: int) -> int:
+    """Algorithmic words per point-step: 67 + 24 (1 + ns/2 + ns) + 45."""
+    return WORDS_S1 + 2 * WORDS_S23 + WORDS_U * (1 + ns // 2 + ns) \
+        + 3 * WORDS_F
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=("c3", "c2"), default="c3")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=160,
+                    help="y rows of the CPU-baseline strip")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also launch one 1 GiB device copy (for PMC "
+                         "byte-counter calibration under rocprofv3)")
+    ap.add_argument("--traffic-file", default=None,
+                    help="PMC summary giving HBM bytes per launch")
+    return ap.parse_args()
+
+
+def build_case(workload: str, ny_override: int | None = None):
+    from arwen_amd import cases
+    if workload == "c3":
+        kw = {} if ny_override is None else {"ny": ny_override}
+        return cases.warm_bubbles(**kw), (
+            "dry stratified 1000x1000x50, dx=1 km, dt=6 s, 4 acoustic "
+            "substeps, 16 seeded +3 K warm bubbles, u=5 v=2 m/s")
+    kw = {} if ny_override is None else {"ny": ny_override}
+    return cases.stratified_hill(**kw), (
+        "3-D dry stratified flow 250x200x49 over a 1 km Gaussian hill, "
+        "dx=12 km, dt=60 s, 4 acoustic substeps, seeded smooth u/v/theta'")
+
+
+def read_traffic(path, kernel):
+    """Bytes per launch of ``kernel`` from a PMC summary text file with
+    lines ``<kernel> fetch_bytes=<F> write_bytes=<W>``."""
+    if not path or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            parts = line.split()
+            if parts and parts[0] == kernel:
+                kv = dict(p.split("=", 1) for p in parts[1:] if "=" in p)
+                if "fetch_bytes" in kv and "write_bytes" in kv:
+                    return float(kv["fetch_bytes"]) + float(kv["write_bytes"])
+    return None
+
+
+def cpu_baseline(workload: str, rows: int, steps: int):
+    """Float32 NumPy oracle on a (nz, rows, nx) strip of the workload."""
+    import numpy as np
+    from oracle import dycore_np as O
+    case, _ = build_case(workload, ny_override=rows)
+    s = O.from_case(case, np.float32)
+    cfg = case.cfg
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        O.step(s, cfg)
+    dt = time.perf_counter() - t0
+    pts = cfg.nx * cfg.ny * cfg.nz
+    return {"value": pts * steps / dt / 1e6, "unit": "Mpoint-steps/s",
+            "cores": 1, "kind": "port",
+            "sample": f"float32 NumPy oracle, {steps} steps on a "
+                      f"{cfg.nx}x{cfg.ny}x{cfg.nz} strip of the same "
+                      f"workload ({dt:.1f} s, single-threaded ufuncs)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus "
+              f"{args.gpus}); the step is a single-GPU kernel set and the "
+              "multi-GPU sharding is design-only", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from arwen_amd import _lib, dycore
+    from arwen_amd.state import DomainState
+
+    case, desc = build_case(args.workload)
+    cfg = case.cfg
+    state = DomainState.from_host(case)
+    del case
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        dycore.step(state, cfg)
+    torch.cuda.synchronize()
+
+    timer = _lib.KernelTimer(max_events=64 * max(args.steps, 1) + 64)
+    t0 = time.perf_counter()
+    with timer:
+        for _ in range(args.steps):
+            dycore.step(state, cfg)
+        torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    if args.copy_calibration:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+    rep = dycore.stability_report(state, cfg)
+    if rep["nan"]:
+        raise SystemExit("bench.py: the state went NaN")
+
+    pts = cfg.nx * cfg.ny * cfg.nz
+    sec = (t1 - t0) / args.steps
+    value = pts / sec / 1e6
+    ns = cfg.time_step_sound
+    wstep = step_words(ns)
+    step_gbs = wstep * 4 * pts / sec / 1e9
+
+    # the acoustic substep pass (U) dominates: 7 launches per step at ns=4
+    n_sub = timer.launches.get("k_acoustic_col", 0)
+    u_ms = (timer.ms.get("k_advance_uv", 0.0)
+            + timer.ms.get("k_acoustic_col", 0.0))
+    u_avg_ms = u_ms / n_sub if n_sub else float("nan")
+    u_bytes = WORDS_U * 4 * pts
+    u_gbs = u_bytes / (u_avg_ms * 1e-3) / 1e9 if n_sub else None
+    traffic = None
+    t_uv = read_traffic(args.traffic_file, "k_advance_uv")
+    t_col = read_traffic(args.traffic_file, "k_acoustic_col")
+    if t_uv is not None and t_col is not None:
+        traffic = t_uv + t_col
+
+    out = {
+        "metric": "RK3 dycore Mpoint-steps/s, % HBM roofline; "
+                  "adjoint:forward cost at stated err",
+        "value": round(value, 3),
+        "unit": "Mpoint-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded initial state)",
+        "config": {"workload": desc, "grid": [cfg.nx, cfg.ny, cfg.nz],
+                   "dt": cfg.dt, "time_step_sound": ns,
+                   "parallelism": "single GPU"},
+        "step_roofline": {"alg_bytes_per_point_step": wstep * 4,
+                          "achieved_GBs": round(step_gbs, 1),
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "acoustic substep pass (k_advance_uv + "
+                      "k_acoustic_col)",
+            "avg_launch_ms": round(u_avg_ms, 4),
+            "alg_bytes_per_launch": u_bytes,
+            "achieved": None if u_gbs is None else round(u_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": None if u_gbs is None else round(u_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "kernel_ms_per_step": {k: round(v / args.steps, 4)
+                               for k, v in timer.ms.items() if v > 0},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_rows,
+                                           args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -163,6 +163,18 @@ int32_t arw_health(const arw_domain *d, float *out, void *stream);
  * double. */
 int32_t arw_domain_mass(const arw_domain *d, double *out, void *stream);
 
+/* Per-kernel timing of arw_step with HIP events on the launch stream.
+ * Not part of the reference interface (the reference's equivalent is the
+ * CUDA-event section profiler of tools/benchmark_seeded_step.py:384-420).
+ * arw_profile_begin creates a pool of max_events events; every kernel
+ * issued by arw_step afterwards records one.  arw_profile_end waits for
+ * the last event, writes the summed milliseconds and launch counts per
+ * kernel id (0 .. arw_kernel_count()-1) and frees the pool. */
+int32_t arw_profile_begin(int32_t max_events);
+int32_t arw_profile_end(double *ms, int32_t *launches, int32_t n_kernels);
+const char *arw_kernel_name(int32_t id);
+int32_t arw_kernel_count(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -589,8 +589,6 @@ def calc_coefs(s, dtau, epssm, top_lid=True):
     c2a = f(GAMMA) * s.p / s.alt
     shp = (nz + 1,) + mut.shape
     a = np.zeros(shp, s.dtype)
-    alpha = np.zeros(shp, s.dtype)
-    gam = np.zeros(shp, s.dtype)
     chm = [s.c1h[k] * mut + s.c2h[k] for k in range(nz)]
     cfm = [s.c1f[k] * mut + s.c2f[k] for k in range(nz + 1)]
     for k in range(2, nz):
@@ -599,18 +597,45 @@ def calc_coefs(s, dtau, epssm, top_lid=True):
     if not top_lid:
         a[nz] = (f(-2.0) * cof * s.rdnw[nz - 1] * s.rdnw[nz - 1]
                  * c2a[nz - 1] / (chm[nz - 1] * cfm[nz - 1]))
+    b = np.zeros(shp, s.dtype)
+    cc = np.zeros(shp, s.dtype)
     for k in range(1, nz):
-        b = f(1.0) + cof * s.rdn[k] * (
+        b[k] = f(1.0) + cof * s.rdn[k] * (
             s.rdnw[k] * c2a[k] / (chm[k] * cfm[k])
             + s.rdnw[k - 1] * c2a[k - 1] / (chm[k - 1] * cfm[k]))
-        cc = -cof * s.rdn[k] * s.rdnw[k] * c2a[k] / (chm[k] * cfm[k + 1])
-        alpha[k] = f(1.0) / (b - a[k] * gam[k - 1])
-        gam[k] = cc * alpha[k]
-    b_top = f(1.0) + f(2.0) * cof * s.rdnw[nz - 1] * s.rdnw[nz - 1] \
+        cc[k] = -cof * s.rdn[k] * s.rdnw[k] * c2a[k] / (chm[k] * cfm[k + 1])
+    b[nz] = f(1.0) + f(2.0) * cof * s.rdnw[nz - 1] * s.rdnw[nz - 1] \
         * c2a[nz - 1] / (chm[nz - 1] * cfm[nz])
-    alpha[nz] = f(1.0) / (b_top - a[nz] * gam[nz - 1])
-    gam[nz] = 0.0
+    alpha, gam = lu_factors(a, b, cc)
     return {"c2a": c2a, "a": a, "alpha": alpha, "gam": gam}
+
+
+def lu_factors(a, b, c):
+    """Thomas factors of the tridiagonal rows 1..n with sub-diagonal a,
+    diagonal b, super-diagonal c (row n has no super-diagonal):
+    alpha[k] = 1/(b[k] - a[k] gam[k-1]), gam[k] = c[k] alpha[k], gam[0] =
+    gam[n] = 0 (reference acoustic.cu:739-758)."""
+    n = a.shape[0] - 1
+    one = np.asarray(1.0, dtype=a.dtype)
+    alpha = np.zeros_like(a)
+    gam = np.zeros_like(a)
+    for k in range(1, n):
+        alpha[k] = one / (b[k] - a[k] * gam[k - 1])
+        gam[k] = c[k] * alpha[k]
+    alpha[n] = one / (b[n] - a[n] * gam[n - 1])
+    gam[n] = 0.0
+    return alpha, gam
+
+
+def thomas_solve(w, a, alpha, gam):
+    """Solve the factored system in place for rows 1..n (row 0 holds the
+    boundary value, coupled through a[1]) (reference acoustic.cu:941-948)."""
+    n = w.shape[0] - 1
+    for k in range(1, n + 1):
+        w[k] = (w[k] - a[k] * w[k - 1]) * alpha[k]
+    for k in range(n - 1, 0, -1):
+        w[k] = w[k] - gam[k] * w[k + 1]
+    return w
 
 
 def _pdmp(p, pold, smdiv):
@@ -789,10 +814,7 @@ def advance_w_phi(s, coefs, dtau, epssm, dx, dy, top_lid=True,
         cterm = dtau * g * (f(2.0) * s.rdnw[nz - 1] * c2a[nz - 1]
                             * s.alt[nz - 1] * t2[nz - 1] + s.c1f[nz] * muave)
         w[nz] = w[nz] + ((dtau * s.rw_t[nz] - bterm) - cterm)
-    for k in range(1, nz + 1):
-        w[k] = (w[k] - a[k] * w[k - 1]) * alpha[k]
-    for k in range(nz - 1, 0, -1):
-        w[k] = w[k] - gam[k] * w[k + 1]
+    thomas_solve(w, a, alpha, gam)
     if dampmag > 0.0:
         phf = (s.phb if s.base3d else s.phb[:, None, None]) + s.php
         htop = phf[nz] / g
