@@ -12,7 +12,7 @@ from pathlib import Path
 LIB_NAME = "libarwen_dycore.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -29,7 +29,7 @@ DOMAIN_ARRAYS = (
     "thb", "pb", "alb", "phb", "mub2d", "ht",
     "c1h", "c2h", "c1f", "c2f", "dnw", "rdnw", "dn", "rdn", "fnm", "fnp",
 )
-SCRATCH_ARRAYS = ("ww", "c2a", "a", "alpha", "gam", "dmdt_rf")
+SCRATCH_ARRAYS = ("ww", "gam", "dmdt_rf", "mup_next")
 DIFF_ARRAYS = ("rdzf_h", "rdzc_h", "rdzf_w", "rdzc_w")
 
 

@@ -14,8 +14,9 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
-SOURCES = [PKG / "csrc" / "dycore.hip"]
-HEADERS = [PKG / "csrc" / "common.h", ROOT / "include" / "arwen_dycore.h"]
+SOURCES = [PKG / "csrc" / "dycore.hip", PKG / "csrc" / "march.hip"]
+HEADERS = [PKG / "csrc" / "common.h", PKG / "csrc" / "march.h",
+           ROOT / "include" / "arwen_dycore.h"]
 OUT = PKG / "libarwen_dycore.so"
 
 HIPCC_FLAGS = [

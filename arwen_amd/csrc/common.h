@@ -10,6 +10,52 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "arwen_dycore.h"
+
+typedef arw_domain Dom;
+
+// Float32 scalars derived from the double-precision configuration exactly
+// as the reference's host code derives them before a launch
+// (rdx = float32(1/dx), dtau = float32(dt/ns), dampmag =
+// float32(dtau_double * dampcoef), ...).
+struct Sc {
+    float rdx, rdy;          // float32(1/dx)
+    float hrdx, hrdy;        // float32(0.5/dx)
+    float qrdx, qrdy;        // float32(0.25/dx)
+    float dxi2, dyi2;        // float32(1/dx**2)
+    float kh, kv;
+    float dtau, epssm, smdiv;
+    float dampmag, zdamp;
+    int top_lid, diff, damp, first, use_pold;
+};
+
+static inline Sc make_sc(const arw_params* p, double dtau, int first)
+{
+    Sc s;
+    double rdx = 1.0 / p->dx, rdy = 1.0 / p->dy;
+    s.rdx = (float)rdx;
+    s.rdy = (float)rdy;
+    s.hrdx = (float)(0.5 * rdx);
+    s.hrdy = (float)(0.5 * rdy);
+    s.qrdx = (float)(0.25 * rdx);
+    s.qrdy = (float)(0.25 * rdy);
+    s.dxi2 = (float)(1.0 / (p->dx * p->dx));
+    s.dyi2 = (float)(1.0 / (p->dy * p->dy));
+    s.kh = (float)p->khdif;
+    s.kv = (float)p->kvdif;
+    s.dtau = (float)dtau;
+    s.epssm = (float)p->epssm;
+    s.smdiv = first ? 0.0f : (float)p->smdiv;
+    s.first = first != 0;
+    s.use_pold = 1;
+    s.damp = p->damp_opt == 3;
+    s.dampmag = s.damp ? (float)(dtau * p->dampcoef) : 0.0f;
+    s.zdamp = (float)p->zdamp;
+    s.top_lid = p->top_lid != 0;
+    s.diff = (p->khdif > 0.0 || p->kvdif > 0.0);
+    return s;
+}
+
 #define G_F 9.81f
 #define RD_F 287.0f
 #define P0_F 100000.0f

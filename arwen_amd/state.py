@@ -206,9 +206,8 @@ class DomainState:
             setattr(d, name, t.data_ptr())
         for name in ("cf1", "cf2", "cf3", "cfn", "cfn1"):
             setattr(d, name, float(getattr(self, name)))
-        shapes = {"ww": (nz + 1, ny, nx), "c2a": (nz, ny, nx),
-                  "a": (nz + 1, ny, nx), "alpha": (nz + 1, ny, nx),
-                  "gam": (nz + 1, ny, nx), "dmdt_rf": (ny, nx)}
+        shapes = {"ww": (nz + 1, ny, nx), "gam": (nz + 1, ny, nx),
+                  "dmdt_rf": (ny, nx), "mup_next": (ny, nx)}
         for name in SCRATCH_ARRAYS:
             setattr(d, name, self.scratch(shapes[name],
                                           "arw_" + name).data_ptr())

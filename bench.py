@@ -146,15 +146,14 @@ def main():
     step_gbs = wstep * 4 * pts / sec / 1e9
 
     # the acoustic substep pass (U) dominates: 7 launches per step at ns=4
-    n_sub = timer.launches.get("k_acoustic_col", 0)
-    u_ms = (timer.ms.get("k_advance_uv", 0.0)
-            + timer.ms.get("k_acoustic_col", 0.0))
+    n_sub = timer.launches.get("k_column", 0)
+    u_ms = timer.ms.get("k_uv", 0.0) + timer.ms.get("k_column", 0.0)
     u_avg_ms = u_ms / n_sub if n_sub else float("nan")
     u_bytes = WORDS_U * 4 * pts
     u_gbs = u_bytes / (u_avg_ms * 1e-3) / 1e9 if n_sub else None
     traffic = None
-    t_uv = read_traffic(args.traffic_file, "k_advance_uv")
-    t_col = read_traffic(args.traffic_file, "k_acoustic_col")
+    t_uv = read_traffic(args.traffic_file, "k_uv")
+    t_col = read_traffic(args.traffic_file, "k_column")
     if t_uv is not None and t_col is not None:
         traffic = t_uv + t_col
 
@@ -180,8 +179,7 @@ def main():
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "roofline": {
             "bound": "hbm",
-            "kernel": "acoustic substep pass (k_advance_uv + "
-                      "k_acoustic_col)",
+            "kernel": "acoustic substep pass (k_uv + k_column)",
             "avg_launch_ms": round(u_avg_ms, 4),
             "alg_bytes_per_launch": u_bytes,
             "achieved": None if u_gbs is None else round(u_gbs, 1),

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define ARW_ABI_VERSION 1
+#define ARW_ABI_VERSION 2
 
 #define ARW_OK 0
 #define ARW_E_ARG (-1)      /* invalid argument (null pointer, bad extent) */
@@ -55,9 +55,9 @@ typedef struct arw_domain {
     /* vertical coordinate */
     const float *c1h, *c2h, *c1f, *c2f, *dnw, *rdnw, *dn, *rdn, *fnm, *fnp;
     float cf1, cf2, cf3, cfn, cfn1;
-    /* scratch: ww (nz+1,ny,nx), c2a (nz,ny,nx), a/alpha/gam (nz+1,ny,nx),
-     * dmdt_rf (ny,nx) */
-    float *ww, *c2a, *a, *alpha, *gam, *dmdt_rf;
+    /* scratch: ww (Omega, nz+1,ny,nx), gam (LU factor, nz+1,ny,nx),
+     * dmdt_rf (reference mass divergence, ny,nx), mup_next (ny,nx) */
+    float *ww, *gam, *dmdt_rf, *mup_next;
     /* constant-K diffusion inverse spacings (reference diffusion.py:34-61):
      * half-level fields rdzf_h (nz-1), rdzc_h (nz); w rdzf_w (nz),
      * rdzc_w (nz+1).  May be NULL when khdif = kvdif = 0. */
@@ -139,8 +139,11 @@ int32_t arw_stage_prepare(const arw_domain *d, const arw_params *prm,
 int32_t arw_acoustic_substep(const arw_domain *d, const arw_params *prm,
                              double dtau, int32_t first, void *stream);
 
-/* Implicit-w coefficients c2a/a/alpha/gam and dmdt_rf for the current
- * reference state (acoustic.cu:675-759; acoustic.py:299-327). */
+/* Per-stage acoustic preparation for the current reference state
+ * (replaces acoustic.py:299-327 prepare_acoustic_coefficients): Omega and
+ * the reference-momentum mass divergence.  The implicit-w LU factors
+ * (acoustic.cu:675-759 calc_coefs) are formed inside every substep from
+ * the stage reference p/alt, so none are stored. */
 int32_t arw_calc_coefs(const arw_domain *d, const arw_params *prm,
                        double dtau, void *stream);
 
