@@ -115,3 +115,71 @@ __device__ __forceinline__ float basev(const float* __restrict__ b, int k,
 {
     return base3d ? b[k * st + c] : b[k];
 }
+
+// ---------------------------------------------------------------------------
+// Column-march addressing.
+//
+// A column kernel touches the same few columns at every level, so each load
+// address splits into a per-thread column byte offset (fixed for the whole
+// march, one 32-bit VGPR) and a per-level offset that is the same for every
+// lane.  gld/gst form base + level in scalar registers and add the 32-bit
+// column offset in the load itself (global_load_dword v, voff, s[base:+1]),
+// so the inner loops carry no 64-bit per-lane address arithmetic.  The
+// compiler only selects that form when it sees the 32-bit offset being
+// zero-extended in the same block as the load; colb_opaque() at the top of
+// each loop iteration keeps the offsets from being widened once, outside
+// the loop.  Byte offsets are unsigned 32-bit: the host keeps every array
+// below 4 GiB.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float gld(const float* base, unsigned lev,
+                                     unsigned colb)
+{
+    const char* b = reinterpret_cast<const char*>(base + lev);
+    return *reinterpret_cast<const float*>(b + (uint64_t)colb);
+}
+
+__device__ __forceinline__ void gst(float* base, unsigned lev, unsigned colb,
+                                    float x)
+{
+    char* b = reinterpret_cast<char*>(base + lev);
+    *reinterpret_cast<float*>(b + (uint64_t)colb) = x;
+}
+
+__device__ __forceinline__ void colb_opaque(unsigned& a)
+{
+    asm volatile("" : "+v"(a));
+}
+
+// 1-D level coefficients (c1h, rdnw, fnm, ... and the 1-D base profiles)
+// are never written by any kernel.  Reading them through the constant
+// address space lets a uniform index become a scalar load: it is tracked
+// by lgkmcnt instead of the in-order vector-memory counter, so waiting for
+// a coefficient does not also wait for the next level's prefetched loads.
+typedef const __attribute__((address_space(4))) float kfloat;
+
+__device__ __forceinline__ kfloat* kview(const float* p)
+{
+    return (kfloat*)p;
+}
+
+// ---------------------------------------------------------------------------
+// XCD-aware workgroup numbering.  MI355X hands workgroups to its 8 XCDs
+// round-robin, and each XCD has its own L2.  Column kernels read rows j-1
+// and j+1 (or j-3..j+3) that other workgroups of the same launch also
+// read; renumbering so that XCD x runs the x-th contiguous chunk of the
+// domain keeps those neighbouring rows in one L2.  The host pads the grid
+// to a multiple of ARW_NXCD; padded workgroups fall outside the domain and
+// exit at the bounds check.
+// ---------------------------------------------------------------------------
+#define ARW_NXCD 8
+
+__device__ __forceinline__ int xcd_block()
+{
+    const int n = (int)gridDim.x, b = (int)blockIdx.x;
+    return (b % ARW_NXCD) * (n / ARW_NXCD) + b / ARW_NXCD;
+}
+
+static inline unsigned xcd_grid(unsigned nblocks)
+{
+    return (nblocks + ARW_NXCD - 1) / ARW_NXCD * ARW_NXCD;
+}

@@ -437,8 +437,9 @@ int32_t check_domain(const Dom* d)
     if (d->nz < 3 || d->ny < 1 || d->nx < 1)
         return fail(ARW_E_ARG, "need nz >= 3, ny >= 1, nx >= 1");
     long long big = (long long)(d->nz + 1) * (d->ny + 1) * (d->nx + 1);
-    if (big >= (1LL << 31))
-        return fail(ARW_E_ARG, "domain exceeds 2^31 points per field");
+    // column kernels address each array with 32-bit byte offsets
+    if (big >= (1LL << 30))
+        return fail(ARW_E_ARG, "domain exceeds 2^30 points per field");
     const void* req[] = {d->u, d->v, d->w, d->thp, d->php, d->mup, d->p,
                          d->al, d->alt, d->u0, d->v0, d->w0, d->thp0,
                          d->php0, d->mup0, d->ru_t, d->rv_t, d->rw_t,

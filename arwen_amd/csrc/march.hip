@@ -8,6 +8,8 @@
 // neighbouring lanes' rows through L1/L2.  Expressions keep the evaluation
 // order of the float32 restatement of the reference step (see common.h),
 // so results are bit-identical to it.
+#include <type_traits>
+
 #include "march.h"
 
 #define R __restrict__
@@ -22,13 +24,17 @@ __device__ __forceinline__ float mu_at(const float* R mub, const float* R mup,
 
 // Vertical face flux of a half-level field at w-level kf from the window
 // q(kf-2), q(kf-1), q(kf), q(kf+1) (reference advection.cu:189-201).
+template <class CP>
 __device__ __forceinline__ float zf(float qm2, float qm1, float q0, float qp1,
-                                    float vel, int kf, int nz,
-                                    const float* R fnm, const float* R fnp)
+                                    float vel, int kf, int nz, CP fnm, CP fnp)
 {
-    if (kf == 0 || kf == nz) return 0.0f;
-    if (kf == 1 || kf == nz - 1) return vel * (fnm[kf] * q0 + fnp[kf] * qm1);
-    return flux3(qm2, qm1, q0, qp1, vel);
+    // both forms are evaluated and one is selected, so a level loop built
+    // on zf stays one basic block (fnm/fnp have nz entries)
+    const int kc = kf < nz ? kf : nz - 1;
+    const float lin = vel * (fnm[kc] * q0 + fnp[kc] * qm1);
+    const float cub = flux3(qm2, qm1, q0, qp1, vel);
+    const float r = (kf == 1 || kf == nz - 1) ? lin : cub;
+    return (kf == 0 || kf == nz) ? 0.0f : r;
 }
 
 __device__ __forceinline__ float div3(float dfx, float dfy, float dfz,
@@ -63,253 +69,280 @@ __device__ __forceinline__ float pgf_val(
     return left + (rd * dphp) * vertical;
 }
 
-// Vertical window of one mass column for the large-step PGF.
-struct PgfCol {
-    int c;
-    float pp_m, pp_0, pp_p;   // p - pb at k-1, k, k+1
-    float ph_0, ph_p;         // php at k, k+1
-    float gb_0, gb_p;         // phb at k, k+1 (terrain)
-    float dpn_0, dpn_p;       // dpn at k, k+1
-};
-
-__device__ __forceinline__ float pp_of(const Dom& d, const float* R p,
-                                       const float* R pb, int k, int c,
-                                       int st)
-{
-    return p[k * st + c] - basev(pb, k, c, st, d.base3d);
-}
-
-__device__ __forceinline__ void pgf_init(PgfCol& X, const Dom& d, int c,
-                                         int st, const float* R p,
-                                         const float* R pb,
-                                         const float* R php,
-                                         const float* R phb)
-{
-    X.c = c;
-    float p0 = pp_of(d, p, pb, 0, c, st), p1 = pp_of(d, p, pb, 1, c, st);
-    float p2 = pp_of(d, p, pb, 2, c, st);
-    X.pp_m = 0.0f;
-    X.pp_0 = p0;
-    X.pp_p = p1;
-    X.ph_0 = php[c];
-    X.gb_0 = d.base3d ? phb[c] : 0.0f;
-    X.dpn_0 = (d.cf1 * p0 + d.cf2 * p1) + d.cf3 * p2;
-}
-
-// Values at level k+1 entering the window at iteration k.
-__device__ __forceinline__ void pgf_load(PgfCol& X, const Dom& d, int k,
-                                         int st, int top_lid,
-                                         const float* R php,
-                                         const float* R phb,
-                                         const float* R fnm,
-                                         const float* R fnp)
-{
-    const int nz = d.nz;
-    X.ph_p = php[(k + 1) * st + X.c];
-    X.gb_p = d.base3d ? phb[(k + 1) * st + X.c] : 0.0f;
-    if (k + 1 == nz)
-        X.dpn_p = top_lid ? d.cfn * X.pp_0 + d.cfn1 * X.pp_m : 0.0f;
-    else
-        X.dpn_p = fnm[k + 1] * X.pp_p + fnp[k + 1] * X.pp_0;
-}
-
-__device__ __forceinline__ void pgf_shift(PgfCol& X, const Dom& d, int k,
-                                          int st, const float* R p,
-                                          const float* R pb)
-{
-    X.pp_m = X.pp_0;
-    X.pp_0 = X.pp_p;
-    X.pp_p = (k + 2 < d.nz) ? pp_of(d, p, pb, k + 2, X.c, st) : 0.0f;
-    X.ph_0 = X.ph_p;
-    X.gb_0 = X.gb_p;
-    X.dpn_0 = X.dpn_p;
-}
-
-__device__ __forceinline__ float phh_of(const PgfCol& X, int base3d)
-{
-    if (base3d) return 0.5f * ((X.gb_0 + X.gb_p) + (X.ph_0 + X.ph_p));
-    return 0.5f * (X.ph_0 + X.ph_p);
-}
-
 // ---------------------------------------------------------------------------
 // Omega and the stage EOS
 // ---------------------------------------------------------------------------
-
-__device__ __forceinline__ void eos_at(const Dom& d, int k, int c, int st,
-                                       float mu, float thp_k, float php_k,
-                                       float php_kp)
-{
-    float th = basev(d.thb, k, c, st, d.base3d) + thp_k;
-    float dph = (basev(d.phb, k + 1, c, st, d.base3d) + php_kp)
-                - (basev(d.phb, k, c, st, d.base3d) + php_k);
-    float alt = (-dph * d.rdnw[k]) / (d.c1h[k] * mu + d.c2h[k]);
-    d.alt[k * st + c] = alt;
-    d.al[k * st + c] = alt - basev(d.alb, k, c, st, d.base3d);
-    d.p[k * st + c] = P0_F * pow_gamma((RD_F * th) / (P0_F * alt));
-}
 
 // WRF calc_ww_cp (reference dycore.py:102-131): the column sum of the
 // coupled-flux divergence, then Omega by the upward recurrence.  The sum
 // also serves the acoustic substeps as the reference-momentum mass
 // divergence (acoustic.cu:396-405): dnw*div(C(mu_face) u_t*) is the same
 // float32 expression, so one sum serves both.  Pass 1 parks each level's
-// divergence in ww; pass 2 turns it into Omega.
-__global__ __launch_bounds__(256) void k_omega(Dom d, Sc s, int do_eos)
+// divergence in ww; pass 2 turns it into Omega.  EOS: the stage-start
+// calc_p_alpha (diagnostics.cu:37-89) when the stage state is new.
+template <bool EOS, bool B3>
+__global__ __launch_bounds__(256) void k_omega(Dom d, Sc s)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stu = ny * (nx + 1), stv = (ny + 1) * nx;
+    const int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= st) return;
     const int j = c / nx, i = c - j * nx;
     const int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
     const int jp = (j + 1 == ny) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
-    const float* R u = d.u;
-    const float* R v = d.v;
-    const float* R c1h = d.c1h;
-    const float* R c2h = d.c2h;
-    const float* R dnw = d.dnw;
-    float* R ww = d.ww;
-    const float ms = mu_at(d.mub2d, d.mup, c);
-    if (do_eos) {
-        const float* R thp = d.thp;
-        const float* R php = d.php;
-        float ph_lo = php[c];
+    kfloat* c1h = kview(d.c1h);
+    kfloat* c2h = kview(d.c2h);
+    kfloat* dnw = kview(d.dnw);
+    kfloat* rdnw = kview(d.rdnw);
+    unsigned bc = 4u * c, bu = 4u * (j * (nx + 1) + i), bu1 = bu + 4u;
+    unsigned bvn = 4u * (c + nx);
+    const float ms = d.mub2d[c] + d.mup[c];
+    if (EOS) {
+        kfloat* thb1 = kview(d.thb);
+        kfloat* phb1 = kview(d.phb);
+        kfloat* alb1 = kview(d.alb);
+        float ph_lo = gld(d.php, 0, bc);
+        float gb_lo = B3 ? gld(d.phb, 0, bc) : phb1[0];
+#pragma unroll 2
         for (int k = 0; k < nz; ++k) {
-            float ph_hi = php[(k + 1) * st + c];
-            eos_at(d, k, c, st, ms, thp[k * st + c], ph_lo, ph_hi);
+            colb_opaque(bc);
+            const unsigned h = k * st;
+            const float ph_hi = gld(d.php, h + st, bc);
+            const float gb_hi = B3 ? gld(d.phb, h + st, bc) : phb1[k + 1];
+            const float th = (B3 ? gld(d.thb, h, bc) : thb1[k]) + gld(d.thp, h, bc);
+            const float albk = B3 ? gld(d.alb, h, bc) : alb1[k];
+            float dph = (gb_hi + ph_hi) - (gb_lo + ph_lo);
+            float alt = (-dph * rdnw[k]) / (c1h[k] * ms + c2h[k]);
+            gst(d.alt, h, bc, alt);
+            gst(d.al, h, bc, alt - albk);
+            gst(d.p, h, bc, P0_F * pow_gamma((RD_F * th) / (P0_F * alt)));
             ph_lo = ph_hi;
+            gb_lo = gb_hi;
         }
     }
-    const float mw = mu_at(d.mub2d, d.mup, j * nx + im);
-    const float me = mu_at(d.mub2d, d.mup, j * nx + ip);
-    const float mS = mu_at(d.mub2d, d.mup, jm * nx + i);
-    const float mN = mu_at(d.mub2d, d.mup, jp * nx + i);
+    const float mw = d.mub2d[j * nx + im] + d.mup[j * nx + im];
+    const float me = d.mub2d[j * nx + ip] + d.mup[j * nx + ip];
+    const float mS = d.mub2d[jm * nx + i] + d.mup[jm * nx + i];
+    const float mN = d.mub2d[jp * nx + i] + d.mup[jp * nx + i];
     const float muxw = 0.5f * (ms + mw), muxe = 0.5f * (me + ms);
     const float muys = 0.5f * (ms + mS), muyn = 0.5f * (mN + ms);
     float dmdt = 0.0f, rf = 0.0f;
-#pragma unroll 2
+#pragma unroll 4
     for (int k = 0; k < nz; ++k) {
-        float ruw = (c1h[k] * muxw + c2h[k]) * u[iU(k, j, i, ny, nx)];
-        float rue = (c1h[k] * muxe + c2h[k]) * u[iU(k, j, i + 1, ny, nx)];
-        float rvs = (c1h[k] * muys + c2h[k]) * v[iV(k, j, i, ny, nx)];
-        float rvn = (c1h[k] * muyn + c2h[k]) * v[iV(k, j + 1, i, ny, nx)];
+        colb_opaque(bc); colb_opaque(bu); colb_opaque(bu1); colb_opaque(bvn);
+        const float c1 = c1h[k], c2 = c2h[k];
+        float ruw = (c1 * muxw + c2) * gld(d.u, k * stu, bu);
+        float rue = (c1 * muxe + c2) * gld(d.u, k * stu, bu1);
+        float rvs = (c1 * muys + c2) * gld(d.v, k * stv, bc);
+        float rvn = (c1 * muyn + c2) * gld(d.v, k * stv, bvn);
         float divv = dnw[k] * (s.rdx * (rue - ruw) + s.rdy * (rvn - rvs));
         dmdt = (k == 0) ? divv : dmdt + divv;
         rf = rf + divv;
-        ww[(k + 1) * st + c] = divv;
+        gst(d.ww, (k + 1) * st, bc, divv);
     }
-    ww[c] = 0.0f;
+    gst(d.ww, 0, bc, 0.0f);
     float run = 0.0f;
+#pragma unroll 4
     for (int k = 0; k < nz - 1; ++k) {
-        float inc = (c1h[k] * dnw[k]) * dmdt + ww[(k + 1) * st + c];
+        colb_opaque(bc);
+        const unsigned h = (k + 1) * st;
+        float inc = (c1h[k] * dnw[k]) * dmdt + gld(d.ww, h, bc);
         run = run + inc;
-        ww[(k + 1) * st + c] = -run;
+        gst(d.ww, h, bc, -run);
     }
-    ww[nz * st + c] = 0.0f;
+    gst(d.ww, nz * st, bc, 0.0f);
     d.dmdt_rf[c] = rf;
     d.rmu_t[c] = 0.0f;
 }
 
 // ---------------------------------------------------------------------------
-// Slow tendencies on u and v faces (+ u''/v'' initialisation)
+// Row-segment waves.  The stencil kernels run 64x4 workgroups in which
+// each 64-lane wave covers 64 consecutive columns of ONE row, so the row
+// index is wave-uniform: every row offset (the y stencil rows j-3..j+3,
+// j+-1, the periodic wrap in y) is scalar arithmetic, and each lane keeps
+// only its column offsets (the x stencil i-3..i+3 with periodic wrap).
 // ---------------------------------------------------------------------------
+struct RowSeg {
+    int i, j;          // column (per lane), row (wave-uniform)
+    bool live;
+};
 
+__device__ __forceinline__ RowSeg row_seg(int nx, int ny)
+{
+    const int nbx = (nx + 63) / 64;
+    const int L = xcd_block();
+    const int by = L / nbx, bx = L - by * nbx;
+    RowSeg r;
+    r.i = bx * 64 + (int)threadIdx.x;
+    r.j = __builtin_amdgcn_readfirstlane(by * 4 + (int)threadIdx.y);
+    r.live = r.i < nx && r.j < ny;
+    return r;
+}
+
+static inline dim3 rowseg_grid(int nx, int ny)
+{
+    return dim3(xcd_grid((unsigned)((nx + 63) / 64) * ((ny + 3) / 4)));
+}
+
+// Vertical window of one mass column for the large-step PGF: p - pb at
+// k-1, k, k+1; php, phb and dpn at w levels k, k+1.
+struct PgfWin {
+    float pp_m = 0.0f, pp_0 = 0.0f, pp_p = 0.0f;
+    float ph_0 = 0.0f, ph_p = 0.0f, gb_0 = 0.0f, gb_p = 0.0f;
+    float dpn_0 = 0.0f, dpn_p = 0.0f;
+};
+
+// Slow tendencies on the u face (j, i) and v face (j, i): 5th/3rd-order
+// advection (reference advection.cu:424-794), large-step PGF
+// (dycore.cu:98-157), constant-K diffusion (diffusion.cu:22-51), and the
+// u''/v'' small-step initialisation (dycore.cu:947-1010).  The periodic
+// duplicate faces are finished by k_tend_faces_edge.
+template <bool B3, bool DIFF>
 __global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i > nx || j > ny) return;
-    const bool hu = j < ny, hv = i < nx;
-    const int ic = (i >= nx) ? i - nx : i, jc = (j >= ny) ? j - ny : j;
-    const int im = (ic == 0) ? nx - 1 : ic - 1;
-    const int jm = (jc == 0) ? ny - 1 : jc - 1;
-    const int ip = (ic + 1 == nx) ? 0 : ic + 1;
-    const int jp = (jc + 1 == ny) ? 0 : jc + 1;
-    int xo[7], yo[7];
+    const int pu = nx + 1;                       // u row pitch
+    const int stu = ny * pu, stv = (ny + 1) * nx;
+    const RowSeg rs = row_seg(nx, ny);
+    if (!rs.live) return;
+    const int i = rs.i, j = rs.j;
+    kfloat* c1h = kview(d.c1h);
+    kfloat* c2h = kview(d.c2h);
+    kfloat* fnm = kview(d.fnm);
+    kfloat* fnp = kview(d.fnp);
+    kfloat* rdnw = kview(d.rdnw);
+    kfloat* rdzf_h = kview(d.rdzf_h);
+    kfloat* rdzc_h = kview(d.rdzc_h);
+    // wave-uniform rows
+    int yo[7];
 #pragma unroll
-    for (int o = 0; o < 7; ++o) {
-        xo[o] = pmod(ic + o - 3, nx);
-        yo[o] = pmod(jc + o - 3, ny);
-    }
-    const float* R u = d.u;
-    const float* R v = d.v;
-    const float* R ww = d.ww;
-    const float* R p = d.p;
-    const float* R pb = d.pb;
-    const float* R php = d.php;
-    const float* R phb = d.phb;
-    const float* R alt = d.alt;
-    const float* R al = d.al;
-    const float* R c1h = d.c1h;
-    const float* R c2h = d.c2h;
-    const float* R fnm = d.fnm;
-    const float* R fnp = d.fnp;
-    const float* R rdnw = d.rdnw;
+    for (int o = 0; o < 7; ++o) yo[o] = pmod(j + o - 3, ny);
+    const int jm = yo[2], jp = yo[4];
+    const int jb = (j == 0) ? ny : j;            // v_b row (duplicate at j=0)
+    // per-lane column byte offsets
+    unsigned cx[7];
+#pragma unroll
+    for (int o = 0; o < 7; ++o) cx[o] = 4u * pmod(i + o - 3, nx);
+    unsigned ci1 = 4u * (i + 1);                 // unwrapped east face
+    unsigned cub = 4u * ((i == 0) ? nx : i);     // u_b column
+    auto opaque_all = [&]() {
+#pragma unroll
+        for (int o = 0; o < 7; ++o) colb_opaque(cx[o]);
+        colb_opaque(ci1); colb_opaque(cub);
+    };
+    unsigned& cC = cx[3];                        // centre, west
+    unsigned& cW = cx[2];
+    // mass columns: A = (j, i), W = (j, i-1), S = (j-1, i)
+    const int rA = j * nx, rS = jm * nx;
     const float* R mub = d.mub2d;
     const float* R mup = d.mup;
     const float* R mup0 = d.mup0;
-    const int base3d = d.base3d;
-
-    const int cA = jc * nx + ic, cW = jc * nx + im, cS = jm * nx + ic;
-    const float mA = mu_at(mub, mup, cA), mW = mu_at(mub, mup, cW);
-    const float mS = mu_at(mub, mup, cS);
-    const float mE = mu_at(mub, mup, jc * nx + ip);
-    const float mN = mu_at(mub, mup, jp * nx + ic);
+    auto mu2 = [&](int row, unsigned cb) {
+        return gld(mub, row, cb) + gld(mup, row, cb);
+    };
+    const float mA = mu2(rA, cC), mW = mu2(rA, cW), mS = mu2(rS, cC);
+    const float mE = mu2(rA, cx[4]), mN = mu2(jp * nx, cC);
     // u-face masses
-    const float mux_a = 0.5f * (mW + mu_at(mub, mup, jc * nx + pmod(im - 1, nx)));
+    const float mux_a = 0.5f * (mW + mu2(rA, cx[1]));
     const float mux_c = 0.5f * (mA + mW);
     const float mux_d = 0.5f * (mE + mA);
-    const float muy_jW = 0.5f * (mW + mu_at(mub, mup, jm * nx + im));
+    const float muy_jW = 0.5f * (mW + mu2(rS, cW));
     const float muy_jA = 0.5f * (mA + mS);
-    const float muy_pW = 0.5f * (mu_at(mub, mup, jp * nx + im) + mW);
+    const float muy_pW = 0.5f * (mu2(jp * nx, cW) + mW);
     const float muy_pA = 0.5f * (mN + mA);
     // v-face masses
-    const float muy_a = 0.5f * (mS + mu_at(mub, mup, pmod(jm - 1, ny) * nx + ic));
+    const float muy_a = 0.5f * (mS + mu2(yo[1] * nx, cC));
     const float muy_c = 0.5f * (mA + mS);
     const float muy_d = 0.5f * (mN + mA);
-    const float mux_jS = 0.5f * (mS + mu_at(mub, mup, jm * nx + im));
+    const float mux_jS = 0.5f * (mS + mu2(rS, cW));
     const float mux_jA = 0.5f * (mA + mW);
-    const float mux_pS = 0.5f * (mu_at(mub, mup, jm * nx + ip) + mS);
+    const float mux_pS = 0.5f * (mu2(rS, cx[4]) + mS);
     const float mux_pA = 0.5f * (mE + mA);
     // PGF and init masses
-    const float dmu_u = 0.5f * (mup[cA] + mup[cW]);
-    const float dmu_v = 0.5f * (mup[cA] + mup[cS]);
-    const float mtA = mub[cA] + mup0[cA];
-    const float mtf_u = 0.5f * (mtA + (mub[cW] + mup0[cW]));
-    const float mtf_v = 0.5f * (mtA + (mub[cS] + mup0[cS]));
+    const float dmu_u = 0.5f * (gld(mup, rA, cC) + gld(mup, rA, cW));
+    const float dmu_v = 0.5f * (gld(mup, rA, cC) + gld(mup, rS, cC));
+    const float mtA = gld(mub, rA, cC) + gld(mup0, rA, cC);
+    const float mtf_u = 0.5f * (mtA + (gld(mub, rA, cW) + gld(mup0, rA, cW)));
+    const float mtf_v = 0.5f * (mtA + (gld(mub, rS, cC) + gld(mup0, rS, cC)));
     const float msf_u = 0.5f * (mA + mW), msf_v = 0.5f * (mA + mS);
 
-    PgfCol A, W, S;
-    pgf_init(A, d, cA, st, p, pb, php, phb);
-    if (hu) pgf_init(W, d, cW, st, p, pb, php, phb);
-    if (hv) pgf_init(S, d, cS, st, p, pb, php, phb);
+    // PGF windows of A, W, S
+    auto pp_at = [&](int k, int row, unsigned cb) {
+        const int l = k * st + row;
+        return gld(d.p, l, cb) - (B3 ? gld(d.pb, l, cb) : kview(d.pb)[k]);
+    };
+    auto win_init = [&](PgfWin& X, int row, unsigned cb) {
+        float p0 = pp_at(0, row, cb), p1 = pp_at(1, row, cb);
+        float p2 = pp_at(2, row, cb);
+        X.pp_0 = p0;
+        X.pp_p = p1;
+        X.ph_0 = gld(d.php, row, cb);
+        X.gb_0 = B3 ? gld(d.phb, row, cb) : 0.0f;
+        X.dpn_0 = (d.cf1 * p0 + d.cf2 * p1) + d.cf3 * p2;
+    };
+    auto win_load = [&](PgfWin& X, int k, int row, unsigned cb, bool top) {
+        X.ph_p = gld(d.php, (k + 1) * st + row, cb);
+        X.gb_p = B3 ? gld(d.phb, (k + 1) * st + row, cb) : 0.0f;
+        const int kp = top ? nz - 1 : k + 1;
+        const float lid = s.top_lid ? d.cfn * X.pp_0 + d.cfn1 * X.pp_m : 0.0f;
+        const float mid = fnm[kp] * X.pp_p + fnp[kp] * X.pp_0;
+        X.dpn_p = top ? lid : mid;
+    };
+    auto win_shift = [&](PgfWin& X, int k, int row, unsigned cb) {
+        X.pp_m = X.pp_0;
+        X.pp_0 = X.pp_p;
+        const float np = pp_at(k + 2 < nz ? k + 2 : nz - 1, row, cb);
+        X.pp_p = (k + 2 < nz) ? np : 0.0f;
+        X.ph_0 = X.ph_p;
+        X.gb_0 = X.gb_p;
+        X.dpn_0 = X.dpn_p;
+    };
+    auto phh = [&](const PgfWin& X) {
+        if (B3) return 0.5f * ((X.gb_0 + X.gb_p) + (X.ph_0 + X.ph_p));
+        return 0.5f * (X.ph_0 + X.ph_p);
+    };
+    PgfWin A, W, S;
+    win_init(A, rA, cC);
+    win_init(W, rA, cW);
+    win_init(S, rS, cC);
 
     float uz[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     float vz[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     for (int kk = 0; kk < 3 && kk < nz; ++kk) {
-        if (hu) uz[2 + kk] = u[iU(kk, jc, ic, ny, nx)];
-        if (hv) vz[2 + kk] = v[iV(kk, jc, ic, ny, nx)];
+        uz[2 + kk] = gld(d.u, kk * stu + j * pu, cC);
+        vz[2 + kk] = gld(d.v, kk * stv + j * nx, cC);
     }
-    float wA0 = ww[cA], wW0 = hu ? ww[cW] : 0.0f, wS0 = hv ? ww[cS] : 0.0f;
+    float wA0 = gld(d.ww, rA, cC), wW0 = gld(d.ww, rA, cW);
+    float wS0 = gld(d.ww, rS, cC);
 
     for (int k = 0; k < nz; ++k) {
+        opaque_all();
+        const bool top = k + 1 == nz;
         const float c1 = c1h[k], c2 = c2h[k];
-        pgf_load(A, d, k, st, s.top_lid, php, phb, fnm, fnp);
-        const float wA1 = ww[(k + 1) * st + cA];
-        const float altA = alt[k * st + cA], alA = al[k * st + cA];
-        const float pbA = base3d ? pb[k * st + cA] : 0.0f;
-
-        if (hu) {
-            pgf_load(W, d, k, st, s.top_lid, php, phb, fnm, fnp);
-            const float wW1 = ww[(k + 1) * st + cW];
+        const int lm = k * st;
+        win_load(A, k, rA, cC, top);
+        win_load(W, k, rA, cW, top);
+        win_load(S, k, rS, cC, top);
+        const float wA1 = gld(d.ww, lm + st + rA, cC);
+        const float wW1 = gld(d.ww, lm + st + rA, cW);
+        const float wS1 = gld(d.ww, lm + st + rS, cC);
+        const float altA = gld(d.alt, lm + rA, cC), alA = gld(d.al, lm + rA, cC);
+        const float altW = gld(d.alt, lm + rA, cW), alW = gld(d.al, lm + rA, cW);
+        const float altS = gld(d.alt, lm + rS, cC), alS = gld(d.al, lm + rS, cC);
+        const float pbA = B3 ? gld(d.pb, lm + rA, cC) : 0.0f;
+        const float pbW = B3 ? gld(d.pb, lm + rA, cW) : 0.0f;
+        const float pbS = B3 ? gld(d.pb, lm + rS, cC) : 0.0f;
+        const int lu = k * stu, lv = k * stv;
+        // ---- u face ----
+        {
             float x[7], y[7];
 #pragma unroll
-            for (int o = 0; o < 7; ++o) x[o] = u[iU(k, jc, xo[o], ny, nx)];
+            for (int o = 0; o < 7; ++o) x[o] = gld(d.u, lu + j * pu, cx[o]);
 #pragma unroll
             for (int o = 0; o < 7; ++o)
-                y[o] = (o == 3) ? x[3] : u[iU(k, yo[o], ic, ny, nx)];
-            const float u_b = (ic == 0) ? u[iU(k, jc, nx, ny, nx)] : x[3];
-            const float u_d = (ic + 1 == nx) ? u[iU(k, jc, nx, ny, nx)] : x[4];
+                y[o] = (o == 3) ? x[3] : gld(d.u, lu + yo[o] * pu, cC);
+            const float u_b = gld(d.u, lu + j * pu, cub);
+            const float u_d = gld(d.u, lu + j * pu, ci1);
             float ru_a = (c1 * mux_a + c2) * x[2];
             float ru_b = (c1 * mux_c + c2) * u_b;
             float ru_c = (c1 * mux_c + c2) * x[3];
@@ -317,10 +350,10 @@ __global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
             float vl = 0.5f * (ru_a + ru_b), vr = 0.5f * (ru_c + ru_d);
             float fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], vl);
             float fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], vr);
-            float rv_jW = (c1 * muy_jW + c2) * v[iV(k, jc, im, ny, nx)];
-            float rv_jA = (c1 * muy_jA + c2) * v[iV(k, jc, ic, ny, nx)];
-            float rv_pW = (c1 * muy_pW + c2) * v[iV(k, jc + 1, im, ny, nx)];
-            float rv_pA = (c1 * muy_pA + c2) * v[iV(k, jc + 1, ic, ny, nx)];
+            float rv_jW = (c1 * muy_jW + c2) * gld(d.v, lv + j * nx, cW);
+            float rv_jA = (c1 * muy_jA + c2) * gld(d.v, lv + j * nx, cC);
+            float rv_pW = (c1 * muy_pW + c2) * gld(d.v, lv + (j + 1) * nx, cW);
+            float rv_pA = (c1 * muy_pA + c2) * gld(d.v, lv + (j + 1) * nx, cC);
             float v0 = 0.5f * (rv_jW + rv_jA), v1 = 0.5f * (rv_pW + rv_pA);
             float fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], v0);
             float fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], v1);
@@ -329,49 +362,47 @@ __global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
             float fz1 = zf(uz[1], uz[2], uz[3], uz[4], wz1, k + 1, nz, fnm, fnp);
             float t = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy,
                            rdnw[k]);
-            const float altW = alt[k * st + cW], alW = al[k * st + cW];
-            const float pbW = base3d ? pb[k * st + cW] : 0.0f;
             t = t - pgf_val(s.rdx, s.hrdx, c1, c2, rdnw[k], mux_c, dmu_u,
                             A.ph_0, A.ph_p, W.ph_0, W.ph_p, altA, altW,
-                            A.pp_0, W.pp_0, alA, alW, pbA, pbW, base3d,
-                            phh_of(A, base3d), phh_of(W, base3d), A.dpn_0,
-                            A.dpn_p, W.dpn_0, W.dpn_p);
-            if (s.diff) {
+                            A.pp_0, W.pp_0, alA, alW, pbA, pbW, B3,
+                            phh(A), phh(W), A.dpn_0, A.dpn_p, W.dpn_0,
+                            W.dpn_p);
+            if (DIFF) {
                 float fc = x[3];
                 float lap = s.kh * (s.dxi2 * ((x[4] - 2.0f * fc) + x[2])
                                     + s.dyi2 * ((y[4] - 2.0f * fc) + y[2]));
-                float fup = (k < nz - 1) ? (uz[3] - fc) * d.rdzf_h[k] : 0.0f;
-                float fdn = (k > 0) ? (fc - uz[1]) * d.rdzf_h[k - 1] : 0.0f;
-                lap = lap + (s.kv * (fup - fdn)) * d.rdzc_h[k];
+                const float gup = (uz[3] - fc) * rdzf_h[k < nz - 1 ? k : nz - 2];
+                const float gdn = (fc - uz[1]) * rdzf_h[k > 0 ? k - 1 : 0];
+                float fup = (k < nz - 1) ? gup : 0.0f;
+                float fdn = (k > 0) ? gdn : 0.0f;
+                lap = lap + (s.kv * (fup - fdn)) * rdzc_h[k];
                 t = t + (c1 * msf_u + c2) * lap;
             }
-            const int ix = iU(k, jc, i, ny, nx);
-            d.ru_t[ix] = t;
-            const float uo = (i == ic) ? x[3] : u[ix];
-            d.u_pp[ix] = (c1 * mtf_u + c2) * d.u0[ix] - (c1 * msf_u + c2) * uo;
-            wW0 = wW1;
-            pgf_shift(W, d, k, st, p, pb);
+            gst(d.ru_t, lu + j * pu, cC, t);
+            gst(d.u_pp, lu + j * pu, cC,
+                (c1 * mtf_u + c2) * gld(d.u0, lu + j * pu, cC)
+                    - (c1 * msf_u + c2) * x[3]);
             uz[0] = uz[1]; uz[1] = uz[2]; uz[2] = uz[3]; uz[3] = uz[4];
-            uz[4] = (k + 3 < nz) ? u[iU(k + 3, jc, ic, ny, nx)] : 0.0f;
+            const float un = gld(d.u, (k + 3 < nz ? k + 3 : nz - 1) * stu + j * pu, cC);
+            uz[4] = (k + 3 < nz) ? un : 0.0f;
         }
-        if (hv) {
-            pgf_load(S, d, k, st, s.top_lid, php, phb, fnm, fnp);
-            const float wS1 = ww[(k + 1) * st + cS];
+        // ---- v face ----
+        {
             float x[7], y[7];
 #pragma unroll
-            for (int o = 0; o < 7; ++o) y[o] = v[iV(k, yo[o], ic, ny, nx)];
+            for (int o = 0; o < 7; ++o) y[o] = gld(d.v, lv + yo[o] * nx, cC);
 #pragma unroll
             for (int o = 0; o < 7; ++o)
-                x[o] = (o == 3) ? y[3] : v[iV(k, jc, xo[o], ny, nx)];
-            float ru_jS = (c1 * mux_jS + c2) * u[iU(k, jm, ic, ny, nx)];
-            float ru_jA = (c1 * mux_jA + c2) * u[iU(k, jc, ic, ny, nx)];
-            float ru_pS = (c1 * mux_pS + c2) * u[iU(k, jm, ic + 1, ny, nx)];
-            float ru_pA = (c1 * mux_pA + c2) * u[iU(k, jc, ic + 1, ny, nx)];
+                x[o] = (o == 3) ? y[3] : gld(d.v, lv + j * nx, cx[o]);
+            float ru_jS = (c1 * mux_jS + c2) * gld(d.u, lu + jm * pu, cC);
+            float ru_jA = (c1 * mux_jA + c2) * gld(d.u, lu + j * pu, cC);
+            float ru_pS = (c1 * mux_pS + c2) * gld(d.u, lu + jm * pu, ci1);
+            float ru_pA = (c1 * mux_pA + c2) * gld(d.u, lu + j * pu, ci1);
             float u0c = 0.5f * (ru_jS + ru_jA), u1c = 0.5f * (ru_pS + ru_pA);
             float fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], u0c);
             float fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], u1c);
-            const float v_b = (jc == 0) ? v[iV(k, ny, ic, ny, nx)] : y[3];
-            const float v_d = (jc + 1 == ny) ? v[iV(k, ny, ic, ny, nx)] : y[4];
+            const float v_b = gld(d.v, lv + jb * nx, cC);
+            const float v_d = gld(d.v, lv + (j + 1) * nx, cC);
             float rv_a = (c1 * muy_a + c2) * y[2];
             float rv_b = (c1 * muy_c + c2) * v_b;
             float rv_c = (c1 * muy_c + c2) * y[3];
@@ -384,33 +415,72 @@ __global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
             float fz1 = zf(vz[1], vz[2], vz[3], vz[4], wz1, k + 1, nz, fnm, fnp);
             float t = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy,
                            rdnw[k]);
-            const float altS = alt[k * st + cS], alS = al[k * st + cS];
-            const float pbS = base3d ? pb[k * st + cS] : 0.0f;
             t = t - pgf_val(s.rdy, s.hrdy, c1, c2, rdnw[k], muy_c, dmu_v,
                             A.ph_0, A.ph_p, S.ph_0, S.ph_p, altA, altS,
-                            A.pp_0, S.pp_0, alA, alS, pbA, pbS, base3d,
-                            phh_of(A, base3d), phh_of(S, base3d), A.dpn_0,
-                            A.dpn_p, S.dpn_0, S.dpn_p);
-            if (s.diff) {
+                            A.pp_0, S.pp_0, alA, alS, pbA, pbS, B3,
+                            phh(A), phh(S), A.dpn_0, A.dpn_p, S.dpn_0,
+                            S.dpn_p);
+            if (DIFF) {
                 float fc = y[3];
                 float lap = s.kh * (s.dxi2 * ((x[4] - 2.0f * fc) + x[2])
                                     + s.dyi2 * ((y[4] - 2.0f * fc) + y[2]));
-                float fup = (k < nz - 1) ? (vz[3] - fc) * d.rdzf_h[k] : 0.0f;
-                float fdn = (k > 0) ? (fc - vz[1]) * d.rdzf_h[k - 1] : 0.0f;
-                lap = lap + (s.kv * (fup - fdn)) * d.rdzc_h[k];
+                const float gup = (vz[3] - fc) * rdzf_h[k < nz - 1 ? k : nz - 2];
+                const float gdn = (fc - vz[1]) * rdzf_h[k > 0 ? k - 1 : 0];
+                float fup = (k < nz - 1) ? gup : 0.0f;
+                float fdn = (k > 0) ? gdn : 0.0f;
+                lap = lap + (s.kv * (fup - fdn)) * rdzc_h[k];
                 t = t + (c1 * msf_v + c2) * lap;
             }
-            const int ix = iV(k, j, ic, ny, nx);
-            d.rv_t[ix] = t;
-            const float vo = (j == jc) ? y[3] : v[ix];
-            d.v_pp[ix] = (c1 * mtf_v + c2) * d.v0[ix] - (c1 * msf_v + c2) * vo;
-            wS0 = wS1;
-            pgf_shift(S, d, k, st, p, pb);
+            gst(d.rv_t, lv + j * nx, cC, t);
+            gst(d.v_pp, lv + j * nx, cC,
+                (c1 * mtf_v + c2) * gld(d.v0, lv + j * nx, cC)
+                    - (c1 * msf_v + c2) * y[3]);
             vz[0] = vz[1]; vz[1] = vz[2]; vz[2] = vz[3]; vz[3] = vz[4];
-            vz[4] = (k + 3 < nz) ? v[iV(k + 3, jc, ic, ny, nx)] : 0.0f;
+            const float vn = gld(d.v, (k + 3 < nz ? k + 3 : nz - 1) * stv + j * nx, cC);
+            vz[4] = (k + 3 < nz) ? vn : 0.0f;
         }
-        wA0 = wA1;
-        pgf_shift(A, d, k, st, p, pb);
+        wA0 = wA1; wW0 = wW1; wS0 = wS1;
+        win_shift(A, k, rA, cC);
+        win_shift(W, k, rA, cW);
+        win_shift(S, k, rS, cC);
+    }
+}
+
+// Periodic duplicate faces of the stage tendencies: the face-nx u tendency
+// equals face 0's (same stencil inputs), the face-ny v tendency equals
+// row 0's; u''/v'' init uses the duplicate's own u/u0 (v/v0).  One thread
+// per (k, j) duplicate u face and per (k, i) duplicate v face.
+__global__ __launch_bounds__(256) void k_tend_faces_edge(Dom d)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    const int pu = nx + 1, stu = ny * pu, stv = (ny + 1) * nx;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const float* R mub = d.mub2d;
+    const float* R mup = d.mup;
+    const float* R mup0 = d.mup0;
+    if (t < nz * ny) {
+        const int k = t / ny, j = t - k * ny;
+        const int cA = j * nx, cW = j * nx + nx - 1;
+        const float mA = mub[cA] + mup[cA], mW = mub[cW] + mup[cW];
+        const float mtA = mub[cA] + mup0[cA];
+        const float mtf = 0.5f * (mtA + (mub[cW] + mup0[cW]));
+        const float msf = 0.5f * (mA + mW);
+        const float c1 = d.c1h[k], c2 = d.c2h[k];
+        const int f0 = k * stu + j * pu, fd = f0 + nx;
+        d.ru_t[fd] = d.ru_t[f0];
+        d.u_pp[fd] = (c1 * mtf + c2) * d.u0[fd] - (c1 * msf + c2) * d.u[fd];
+    } else if (t < nz * ny + nz * nx) {
+        const int r = t - nz * ny;
+        const int k = r / nx, i = r - k * nx;
+        const int cA = i, cS = (ny - 1) * nx + i;
+        const float mA = mub[cA] + mup[cA], mS = mub[cS] + mup[cS];
+        const float mtA = mub[cA] + mup0[cA];
+        const float mtf = 0.5f * (mtA + (mub[cS] + mup0[cS]));
+        const float msf = 0.5f * (mA + mS);
+        const float c1 = d.c1h[k], c2 = d.c2h[k];
+        const int f0 = k * stv + i, fd = f0 + ny * nx;
+        d.rv_t[fd] = d.rv_t[f0];
+        d.v_pp[fd] = (c1 * mtf + c2) * d.v0[fd] - (c1 * msf + c2) * d.v[fd];
     }
 }
 
@@ -672,167 +742,232 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
 // ---------------------------------------------------------------------------
 
 // Damped-pressure window of one column for the acoustic PGF.
-struct AcCol {
-    int c;
-    float pd_0, pd_p;          // damped p'' at k, k+1 (own column)
-    float ph_0, ph_p;          // ph'' at k, k+1
-    float hp_0, hp_p;          // php at k, k+1
-    float gb_0, gb_p;          // phb at k, k+1 (terrain)
+// Loads of one u''/v'' level, issued one level ahead.
+struct UvIn {
+    float pA = 0.0f, oA = 0.0f, pW = 0.0f, oW = 0.0f, pS = 0.0f, oS = 0.0f;
+    float phA = 0.0f, phW = 0.0f, phS = 0.0f;   // ph'' at w level k+1
+    float hpA = 0.0f, hpW = 0.0f, hpS = 0.0f;   // php at w level k+1
+    float gbA = 0.0f, gbW = 0.0f, gbS = 0.0f;   // phb at w level k+1 (terrain)
+    float altA = 0.0f, altW = 0.0f, altS = 0.0f;
+    float aA = 0.0f, aW = 0.0f, aS = 0.0f;      // al'' (terrain)
+    float bA = 0.0f, bW = 0.0f, bS = 0.0f;      // pb (terrain)
+    float u = 0.0f, ru = 0.0f, v = 0.0f, rv = 0.0f;
 };
 
-__device__ __forceinline__ float pd_of(const float* R pp, const float* R po,
-                                       int ix, float smdiv, int use_pold)
-{
-    float a = pp[ix];
-    float b = use_pold ? po[ix] : a;
-    return a + smdiv * (a - b);
-}
-
-__global__ __launch_bounds__(256) void k_uv(Dom d, Sc s)
+// u'' at face (j, iu) and/or v'' at face (jv, i) through all levels
+// (reference acoustic.cu:244-314).  A is the mass column east of the u face
+// and north of the v face, W its west and S its south neighbour.  The
+// damped pressure p'' + smdiv (p'' - p''_old) rides in 4-level windows for
+// the surface and lid extrapolations of dpn.
+template <bool B3, bool USEP, bool DU, bool DV>
+__device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int cA,
+                                         int cW, int cS, int fu, int fv)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i > nx || j > ny) return;
-    const bool hu = j < ny, hv = i < nx;
-    const int ic = (i >= nx) ? i - nx : i, jc = (j >= ny) ? j - ny : j;
-    const int im = (ic == 0) ? nx - 1 : ic - 1;
-    const int jm = (jc == 0) ? ny - 1 : jc - 1;
-    const float* R p_pp = d.p_pp;
-    const float* R p_old = d.p_pp_old;
-    const float* R ph_pp = d.ph_pp;
-    const float* R php = d.php;
-    const float* R phb = d.phb;
-    const float* R alt = d.alt;
-    const float* R al_pp = d.al_pp;
-    const float* R pb = d.pb;
-    const float* R c1h = d.c1h;
-    const float* R c2h = d.c2h;
-    const float* R fnm = d.fnm;
-    const float* R fnp = d.fnp;
-    const float* R rdnw = d.rdnw;
-    const float smdiv = s.smdiv;
-    const int usep = s.use_pold;
-    const int b3 = d.base3d;
-    const float dtau = s.dtau;
-
-    const int cA = jc * nx + ic, cW = jc * nx + im, cS = jm * nx + ic;
-    const float mA = mu_at(d.mub2d, d.mup, cA);
-    const float muf_u = 0.5f * (mu_at(d.mub2d, d.mup, cW) + mA);
-    const float muf_v = 0.5f * (mu_at(d.mub2d, d.mup, cS) + mA);
+    const int stu = ny * (nx + 1), stv = (ny + 1) * nx;
+    const float smdiv = s.smdiv, dtau = s.dtau;
+    kfloat* c1h = kview(d.c1h);
+    kfloat* c2h = kview(d.c2h);
+    kfloat* fnm = kview(d.fnm);
+    kfloat* fnp = kview(d.fnp);
+    kfloat* rdnw = kview(d.rdnw);
+    unsigned bA = 4u * cA, bW = 4u * cW, bS = 4u * cS;
+    unsigned bu = 4u * fu, bv = 4u * fv;
+    auto opaque_all = [&]() {
+        colb_opaque(bA); colb_opaque(bW); colb_opaque(bS);
+        colb_opaque(bu); colb_opaque(bv);
+    };
+    auto pd = [&](float a, float o) {
+        float b = USEP ? o : a;
+        return a + smdiv * (a - b);
+    };
+    const float mA = d.mub2d[cA] + d.mup[cA];
+    const float muf_u = DU ? 0.5f * ((d.mub2d[cW] + d.mup[cW]) + mA) : 0.0f;
+    const float muf_v = DV ? 0.5f * ((d.mub2d[cS] + d.mup[cS]) + mA) : 0.0f;
     const float mppA = d.mu_pp[cA];
-    const float dmu_u = d.mu_pp[cW] + mppA, dmu_v = d.mu_pp[cS] + mppA;
+    const float dmu_u = DU ? d.mu_pp[cW] + mppA : 0.0f;
+    const float dmu_v = DV ? d.mu_pp[cS] + mppA : 0.0f;
 
-    // per-column vertical data: damped p'' sums per face need levels
-    // k-2..k+1 (surface and lid extrapolations)
-    auto pdA = [&](int k) { return pd_of(p_pp, p_old, k * st + cA, smdiv, usep); };
-    auto pdW = [&](int k) { return pd_of(p_pp, p_old, k * st + cW, smdiv, usep); };
-    auto pdS = [&](int k) { return pd_of(p_pp, p_old, k * st + cS, smdiv, usep); };
-    // S windows: S(k-2), S(k-1), S(k), S(k+1)
+    // windows S(k-2), S(k-1), S(k), S(k+1) of the damped pressure
+    opaque_all();
     float pa[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pw[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     float ps[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    pa[2] = pdA(0); pa[3] = pdA(1);
-    if (hu) { pw[2] = pdW(0); pw[3] = pdW(1); }
-    if (hv) { ps[2] = pdS(0); ps[3] = pdS(1); }
-    const float pa2 = pdA(2);
-    const float pw2 = hu ? pdW(2) : 0.0f, ps2 = hv ? pdS(2) : 0.0f;
-    // dpn at level 0
+    pa[2] = pd(gld(d.p_pp, 0, bA), gld(d.p_pp_old, 0, bA));
+    pa[3] = pd(gld(d.p_pp, st, bA), gld(d.p_pp_old, st, bA));
+    const float pa2 = pd(gld(d.p_pp, 2 * st, bA), gld(d.p_pp_old, 2 * st, bA));
     float dnu_0 = 0.0f, dnv_0 = 0.0f;
-    if (hu) {
+    if (DU) {
+        pw[2] = pd(gld(d.p_pp, 0, bW), gld(d.p_pp_old, 0, bW));
+        pw[3] = pd(gld(d.p_pp, st, bW), gld(d.p_pp_old, st, bW));
+        const float pw2 = pd(gld(d.p_pp, 2 * st, bW), gld(d.p_pp_old, 2 * st, bW));
         float S0 = pa[2] + pw[2], S1 = pa[3] + pw[3], S2 = pa2 + pw2;
         dnu_0 = 0.5f * ((d.cf1 * S0 + d.cf2 * S1) + d.cf3 * S2);
     }
-    if (hv) {
+    if (DV) {
+        ps[2] = pd(gld(d.p_pp, 0, bS), gld(d.p_pp_old, 0, bS));
+        ps[3] = pd(gld(d.p_pp, st, bS), gld(d.p_pp_old, st, bS));
+        const float ps2 = pd(gld(d.p_pp, 2 * st, bS), gld(d.p_pp_old, 2 * st, bS));
         float S0 = pa[2] + ps[2], S1 = pa[3] + ps[3], S2 = pa2 + ps2;
         dnv_0 = 0.5f * ((d.cf1 * S0 + d.cf2 * S1) + d.cf3 * S2);
     }
-    float phA0 = ph_pp[cA], phW0 = hu ? ph_pp[cW] : 0.0f, phS0 = hv ? ph_pp[cS] : 0.0f;
-    float hpA0 = php[cA], hpW0 = hu ? php[cW] : 0.0f, hpS0 = hv ? php[cS] : 0.0f;
-    float gbA0 = b3 ? phb[cA] : 0.0f, gbW0 = (b3 && hu) ? phb[cW] : 0.0f;
-    float gbS0 = (b3 && hv) ? phb[cS] : 0.0f;
+    float phA0 = gld(d.ph_pp, 0, bA), hpA0 = gld(d.php, 0, bA);
+    float phW0 = DU ? gld(d.ph_pp, 0, bW) : 0.0f, hpW0 = DU ? gld(d.php, 0, bW) : 0.0f;
+    float phS0 = DV ? gld(d.ph_pp, 0, bS) : 0.0f, hpS0 = DV ? gld(d.php, 0, bS) : 0.0f;
+    float gbA0 = B3 ? gld(d.phb, 0, bA) : 0.0f;
+    float gbW0 = (B3 && DU) ? gld(d.phb, 0, bW) : 0.0f;
+    float gbS0 = (B3 && DV) ? gld(d.phb, 0, bS) : 0.0f;
 
-    for (int k = 0; k < nz; ++k) {
+    auto load = [&](UvIn& L, int k) {
+        const unsigned lp = (k + 2 < nz ? k + 2 : nz - 1) * st;
+        const unsigned lw = (k + 1) * st, lm = k * st;
+        L.pA = gld(d.p_pp, lp, bA);
+        if (USEP) L.oA = gld(d.p_pp_old, lp, bA);
+        L.phA = gld(d.ph_pp, lw, bA);
+        L.hpA = gld(d.php, lw, bA);
+        L.altA = gld(d.alt, lm, bA);
+        if (B3) {
+            L.gbA = gld(d.phb, lw, bA);
+            L.aA = gld(d.al_pp, lm, bA);
+            L.bA = gld(d.pb, lm, bA);
+        }
+        if (DU) {
+            L.pW = gld(d.p_pp, lp, bW);
+            if (USEP) L.oW = gld(d.p_pp_old, lp, bW);
+            L.phW = gld(d.ph_pp, lw, bW);
+            L.hpW = gld(d.php, lw, bW);
+            L.altW = gld(d.alt, lm, bW);
+            if (B3) {
+                L.gbW = gld(d.phb, lw, bW);
+                L.aW = gld(d.al_pp, lm, bW);
+                L.bW = gld(d.pb, lm, bW);
+            }
+            L.u = gld(d.u_pp, k * stu, bu);
+            L.ru = gld(d.ru_t, k * stu, bu);
+        }
+        if (DV) {
+            L.pS = gld(d.p_pp, lp, bS);
+            if (USEP) L.oS = gld(d.p_pp_old, lp, bS);
+            L.phS = gld(d.ph_pp, lw, bS);
+            L.hpS = gld(d.php, lw, bS);
+            L.altS = gld(d.alt, lm, bS);
+            if (B3) {
+                L.gbS = gld(d.phb, lw, bS);
+                L.aS = gld(d.al_pp, lm, bS);
+                L.bS = gld(d.pb, lm, bS);
+            }
+            L.v = gld(d.v_pp, k * stv, bv);
+            L.rv = gld(d.rv_t, k * stv, bv);
+        }
+    };
+
+    UvIn nxt;
+    load(nxt, 0);
+    auto level = [&](int k, auto last_kind) {
+        constexpr bool LAST = decltype(last_kind)::value;
+        opaque_all();
+        const UvIn L = nxt;
+        if (!LAST) load(nxt, k + 1);
         const float c1 = c1h[k], c2 = c2h[k];
-        const int a = k * st + cA;
-        const float phA1 = ph_pp[a + st], hpA1 = php[a + st];
-        const float gbA1 = b3 ? phb[a + st] : 0.0f;
-        const float altA = alt[a];
-        if (hu) {
-            const int b = k * st + cW;
-            const float phW1 = ph_pp[b + st], hpW1 = php[b + st];
-            const float gbW1 = b3 ? phb[b + st] : 0.0f;
+        const float phA1 = L.phA, hpA1 = L.hpA, gbA1 = L.gbA;
+        const float altA = L.altA;
+        auto dn_top = [&](float Sk, float Sm1, float Sm2) {
+            return s.top_lid ? 0.5f * ((d.cf1 * Sk + d.cf2 * Sm1) + d.cf3 * Sm2)
+                             : 0.0f;
+        };
+        if (DU) {
             float Sk = pa[2] + pw[2];
             float dn_up;
-            if (k + 1 == nz) {
-                if (s.top_lid) {
-                    float Sm1 = pa[1] + pw[1], Sm2 = pa[0] + pw[0];
-                    dn_up = 0.5f * ((d.cf1 * Sk + d.cf2 * Sm1) + d.cf3 * Sm2);
-                } else {
-                    dn_up = 0.0f;
-                }
+            if (LAST) {
+                dn_up = dn_top(Sk, pa[1] + pw[1], pa[0] + pw[0]);
             } else {
                 float Sp = pa[3] + pw[3];
                 dn_up = 0.5f * (fnm[k + 1] * Sp + fnp[k + 1] * Sk);
             }
-            float dph = (phA1 - phW1) + (phA0 - phW0);
+            float dph = (phA1 - L.phW) + (phA0 - phW0);
             float dpe = pa[2] - pw[2];
-            float brk = dph + (altA + alt[b]) * dpe;
-            if (b3) brk = brk + (al_pp[a] + al_pp[b]) * (pb[a] - pb[b]);
+            float brk = dph + (altA + L.altW) * dpe;
+            if (B3) brk = brk + (L.aA + L.aW) * (L.bA - L.bW);
             float dpxy = ((0.5f * s.rdx) * (c1 * muf_u + c2)) * brk;
-            float phsA = hpA0 + hpA1, phsB = hpW0 + hpW1;
+            float phsA = hpA0 + hpA1, phsB = hpW0 + L.hpW;
             float dphp = 0.5f * (phsA - phsB);
-            if (b3) dphp = dphp + 0.5f * ((gbA0 + gbA1) - (gbW0 + gbW1));
+            if (B3) dphp = dphp + 0.5f * ((gbA0 + gbA1) - (gbW0 + L.gbW));
             dpxy = dpxy + (s.rdx * dphp) * (rdnw[k] * (dn_up - dnu_0)
                                             - (0.5f * c1) * dmu_u);
-            const int ix = iU(k, jc, i, ny, nx);
-            d.u_pp[ix] = d.u_pp[ix] + dtau * (d.ru_t[ix] - dpxy);
+            gst(d.u_pp, k * stu, bu, L.u + dtau * (L.ru - dpxy));
             dnu_0 = dn_up;
-            phW0 = phW1; hpW0 = hpW1; gbW0 = gbW1;
+            phW0 = L.phW; hpW0 = L.hpW; gbW0 = L.gbW;
         }
-        if (hv) {
-            const int b = k * st + cS;
-            const float phS1 = ph_pp[b + st], hpS1 = php[b + st];
-            const float gbS1 = b3 ? phb[b + st] : 0.0f;
+        if (DV) {
             float Sk = pa[2] + ps[2];
             float dn_up;
-            if (k + 1 == nz) {
-                if (s.top_lid) {
-                    float Sm1 = pa[1] + ps[1], Sm2 = pa[0] + ps[0];
-                    dn_up = 0.5f * ((d.cf1 * Sk + d.cf2 * Sm1) + d.cf3 * Sm2);
-                } else {
-                    dn_up = 0.0f;
-                }
+            if (LAST) {
+                dn_up = dn_top(Sk, pa[1] + ps[1], pa[0] + ps[0]);
             } else {
                 float Sp = pa[3] + ps[3];
                 dn_up = 0.5f * (fnm[k + 1] * Sp + fnp[k + 1] * Sk);
             }
-            float dph = (phA1 - phS1) + (phA0 - phS0);
+            float dph = (phA1 - L.phS) + (phA0 - phS0);
             float dpe = pa[2] - ps[2];
-            float brk = dph + (altA + alt[b]) * dpe;
-            if (b3) brk = brk + (al_pp[a] + al_pp[b]) * (pb[a] - pb[b]);
+            float brk = dph + (altA + L.altS) * dpe;
+            if (B3) brk = brk + (L.aA + L.aS) * (L.bA - L.bS);
             float dpxy = ((0.5f * s.rdy) * (c1 * muf_v + c2)) * brk;
-            float phsA = hpA0 + hpA1, phsB = hpS0 + hpS1;
+            float phsA = hpA0 + hpA1, phsB = hpS0 + L.hpS;
             float dphp = 0.5f * (phsA - phsB);
-            if (b3) dphp = dphp + 0.5f * ((gbA0 + gbA1) - (gbS0 + gbS1));
+            if (B3) dphp = dphp + 0.5f * ((gbA0 + gbA1) - (gbS0 + L.gbS));
             dpxy = dpxy + (s.rdy * dphp) * (rdnw[k] * (dn_up - dnv_0)
                                             - (0.5f * c1) * dmu_v);
-            const int ix = iV(k, j, ic, ny, nx);
-            d.v_pp[ix] = d.v_pp[ix] + dtau * (d.rv_t[ix] - dpxy);
+            gst(d.v_pp, k * stv, bv, L.v + dtau * (L.rv - dpxy));
             dnv_0 = dn_up;
-            phS0 = phS1; hpS0 = hpS1; gbS0 = gbS1;
+            phS0 = L.phS; hpS0 = L.hpS; gbS0 = L.gbS;
         }
         phA0 = phA1; hpA0 = hpA1; gbA0 = gbA1;
-        // advance the damped-pressure windows to k+1
+        // advance the damped-pressure windows to k+1 (level k+2 enters)
+        const bool in = k + 2 < nz;
         pa[0] = pa[1]; pa[1] = pa[2]; pa[2] = pa[3];
-        pa[3] = (k + 2 < nz) ? pdA(k + 2) : 0.0f;
-        if (hu) {
+        pa[3] = in ? pd(L.pA, L.oA) : 0.0f;
+        if (DU) {
             pw[0] = pw[1]; pw[1] = pw[2]; pw[2] = pw[3];
-            pw[3] = (k + 2 < nz) ? pdW(k + 2) : 0.0f;
+            pw[3] = in ? pd(L.pW, L.oW) : 0.0f;
         }
-        if (hv) {
+        if (DV) {
             ps[0] = ps[1]; ps[1] = ps[2]; ps[2] = ps[3];
-            ps[3] = (k + 2 < nz) ? pdS(k + 2) : 0.0f;
+            ps[3] = in ? pd(L.pS, L.oS) : 0.0f;
         }
+    };
+    for (int k = 0; k < nz - 1; ++k) level(k, std::false_type{});
+    level(nz - 1, std::true_type{});
+}
+
+// Main launch: one thread per mass column (i, j) advances u'' on face
+// (j, i) and v'' on face (j, i).  No role branches.
+template <bool B3, bool USEP>
+__global__ __launch_bounds__(256) void k_uv(Dom d, Sc s)
+{
+    const int ny = d.ny, nx = d.nx;
+    const int c = xcd_block() * blockDim.x + threadIdx.x;
+    if (c >= ny * nx) return;
+    const int j = c / nx, i = c - j * nx;
+    const int im = (i == 0) ? nx - 1 : i - 1;
+    const int jm = (j == 0) ? ny - 1 : j - 1;
+    uv_march<B3, USEP, true, true>(d, s, c, j * nx + im, jm * nx + i,
+                                   j * (nx + 1) + i, c);
+}
+
+// Periodic duplicate faces: u'' at i = nx (one per row) and v'' at j = ny
+// (one per column), each with its own u''/ru_t and the PGF of face 0.
+template <bool B3, bool USEP>
+__global__ __launch_bounds__(256) void k_uv_edge(Dom d, Sc s)
+{
+    const int ny = d.ny, nx = d.nx;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ny) {
+        const int j = t;
+        uv_march<B3, USEP, true, false>(d, s, j * nx, j * nx + nx - 1, 0,
+                                        j * (nx + 1) + nx, 0);
+    } else if (t < ny + nx) {
+        const int i = t - ny;
+        uv_march<B3, USEP, false, true>(d, s, i, 0, (ny - 1) * nx + i, 0,
+                                        ny * nx + i);
     }
 }
 
@@ -849,52 +984,78 @@ __global__ __launch_bounds__(256) void k_uv(Dom d, Sc s)
 // coefficient arrays stream in; gamma is parked for the way down.  Between
 // the sweeps w'' holds the forward-eliminated values and ph'' the phi''
 // right-hand side.  pout receives the new p''.
-__global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout,
-                                               int write_al, int write_ww)
+// Inputs of upward level m (raw loads; all arithmetic on them happens in
+// the iteration that consumes them, so the loads of level m+1 stay in
+// flight while level m computes).
+struct ColUp {
+    float ui = 0.0f, uip = 0.0f;   // u'' at faces i, i+1
+    float vj = 0.0f, vjp = 0.0f;   // v'' at faces j, j+1
+    float te = 0.0f, tw = 0.0f, tn = 0.0f, ts = 0.0f;  // thp, 4 neighbours
+    float be = 0.0f, bw = 0.0f, bn = 0.0f, bs = 0.0f;  // thb there (terrain)
+    float thu = 0.0f, bhu = 0.0f;  // thp, thb at (m+1, c)
+    float thpp = 0.0f, rth = 0.0f; // theta'' and its tendency
+    float phu = 0.0f, ghu = 0.0f;  // php, phb at w level m+1
+    float alt = 0.0f, p = 0.0f;    // stage reference alt, p
+    float phpp = 0.0f, wpp = 0.0f, rph = 0.0f;  // at w level m
+    float rw = 0.0f;               // w tendency at w level m-1
+};
+
+// Inputs of downward w level k (and mass level k when k < nz).
+struct ColDn {
+    float wpp = 0.0f, gam = 0.0f, w = 0.0f, php = 0.0f, phb = 0.0f;
+    float phpp = 0.0f, thp = 0.0f, thb = 0.0f, alt = 0.0f, p = 0.0f;
+    float thpp = 0.0f;
+};
+
+template <int V>
+using Kind = std::integral_constant<int, V>;
+enum { LV_FIRST = 0, LV_SECOND = 1, LV_THIRD = 2, LV_MID = 3, LV_LAST = 4 };
+
+// B3: terrain (3-D) base state; DAMP: damp_opt = 3 w damper; AL: store
+// alpha''; WW: store Omega'' (standalone substep API).  nz >= 3.
+template <bool B3, bool DAMP, bool AL, bool WW>
+__global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stu = ny * (nx + 1), stv = (ny + 1) * nx;
+    const int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= st) return;
     const int j = c / nx, i = c - j * nx;
     const int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
     const int jp = (j + 1 == ny) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
-    const int b3 = d.base3d;
     const float eps = s.epssm, dtau = s.dtau;
     const float one_p = 1.0f + eps, one_m = 1.0f - eps;
-    const float* R u_pp = d.u_pp;
-    const float* R v_pp = d.v_pp;
-    const float* R thp = d.thp;
-    const float* R thb = d.thb;
-    const float* R php = d.php;
-    const float* R phb = d.phb;
-    const float* R alt = d.alt;
-    const float* R p = d.p;
-    const float* R rth_t = d.rth_t;
-    const float* R rph_t = d.rph_t;
-    const float* R rw_t = d.rw_t;
-    const float* R c1h = d.c1h;
-    const float* R c2h = d.c2h;
-    const float* R c1f = d.c1f;
-    const float* R c2f = d.c2f;
-    const float* R dnw = d.dnw;
-    const float* R rdnw = d.rdnw;
-    const float* R rdn = d.rdn;
-    const float* R fnm = d.fnm;
-    const float* R fnp = d.fnp;
-    float* R th_pp = d.th_pp;
-    float* R w_pp = d.w_pp;
-    float* R ph_pp = d.ph_pp;
-    float* R gam = d.gam;
-
-    auto dvp_at = [&](int k) {
-        return s.rdx * (u_pp[iU(k, j, i + 1, ny, nx)] - u_pp[iU(k, j, i, ny, nx)])
-               + s.rdy * (v_pp[iV(k, j + 1, i, ny, nx)] - v_pp[iV(k, j, i, ny, nx)]);
+    kfloat* c1h = kview(d.c1h);
+    kfloat* c2h = kview(d.c2h);
+    kfloat* c1f = kview(d.c1f);
+    kfloat* c2f = kview(d.c2f);
+    kfloat* dnw = kview(d.dnw);
+    kfloat* rdnw = kview(d.rdnw);
+    kfloat* rdn = kview(d.rdn);
+    kfloat* fnm = kview(d.fnm);
+    kfloat* fnp = kview(d.fnp);
+    // per-thread column byte offsets
+    unsigned bc = 4u * c;
+    unsigned bE = 4u * (j * nx + ip), bW = 4u * (j * nx + im);
+    unsigned bN = 4u * (jp * nx + i), bS = 4u * (jm * nx + i);
+    unsigned bu = 4u * (j * (nx + 1) + i), bu1 = bu + 4u;
+    unsigned bvn = 4u * (c + nx);
+    auto opaque_all = [&]() {
+        colb_opaque(bc); colb_opaque(bE); colb_opaque(bW);
+        colb_opaque(bN); colb_opaque(bS); colb_opaque(bu);
+        colb_opaque(bu1); colb_opaque(bvn);
     };
-    const float mut = mu_at(d.mub2d, d.mup, c);
+
+    const float mut = d.mub2d[c] + d.mup[c];
     const float mupp_old = d.mu_pp[c];
     float dmdt_pp = 0.0f;
-#pragma unroll 2
-    for (int k = 0; k < nz; ++k) dmdt_pp = dmdt_pp + dnw[k] * dvp_at(k);
+#pragma unroll 5
+    for (int k = 0; k < nz; ++k) {
+        opaque_all();
+        const float a = gld(d.u_pp, k * stu, bu1), b = gld(d.u_pp, k * stu, bu);
+        const float e = gld(d.v_pp, k * stv, bvn), f = gld(d.v_pp, k * stv, bc);
+        dmdt_pp = dmdt_pp + dnw[k] * (s.rdx * (a - b) + s.rdy * (e - f));
+    }
     const float rmu = d.rmu_t[c];
     const float mupp = mupp_old + dtau * ((dmdt_pp + d.dmdt_rf[c]) + rmu);
     d.mu_pp[c] = mupp;
@@ -902,40 +1063,72 @@ __global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout,
     const float muave = 0.5f * (one_p * mupp + one_m * mupp_old);
     float cof = 0.5f * dtau * G_F * one_p;
     cof = cof * cof;
-
-    auto thr = [&](int kk, int cc) {
-        return basev(thb, kk, cc, st, b3) + thp[kk * st + cc];
-    };
-    auto phs = [&](int kk) { return basev(phb, kk, c, st, b3) + php[kk * st + c]; };
     auto cfm_at = [&](int kk) { return c1f[kk] * mut + c2f[kk]; };
+    kfloat* thb1 = kview(d.thb);     // 1-D base profiles (flat base)
+    kfloat* phb1 = kview(d.phb);
+    auto thbv = [&](float loaded, int kk) { return B3 ? loaded : thb1[kk]; };
+    auto phbv = [&](float loaded, int kk) { return B3 ? loaded : phb1[kk]; };
 
     // kinematic lower boundary on w'' from the updated u''/v''
     float w0;
     {
-        auto lw = [&](const float* R a, int ix0, int ix1, int ix2) {
-            return (d.cf1 * a[ix0] + d.cf2 * a[ix1]) + d.cf3 * a[ix2];
+        opaque_all();
+        auto lw = [&](const float* a, unsigned colb, int sl) {
+            return (d.cf1 * gld(a, 0, colb) + d.cf2 * gld(a, sl, colb))
+                   + d.cf3 * gld(a, 2 * sl, colb);
         };
-        float ue = lw(u_pp, iU(0, j, i + 1, ny, nx), iU(1, j, i + 1, ny, nx),
-                      iU(2, j, i + 1, ny, nx));
-        float uw = lw(u_pp, iU(0, j, i, ny, nx), iU(1, j, i, ny, nx),
-                      iU(2, j, i, ny, nx));
-        float vn = lw(v_pp, iV(0, j + 1, i, ny, nx), iV(1, j + 1, i, ny, nx),
-                      iV(2, j + 1, i, ny, nx));
-        float vs = lw(v_pp, iV(0, j, i, ny, nx), iV(1, j, i, ny, nx),
-                      iV(2, j, i, ny, nx));
+        float ue = lw(d.u_pp, bu1, stu), uw = lw(d.u_pp, bu, stu);
+        float vn = lw(d.v_pp, bvn, stv), vs = lw(d.v_pp, bc, stv);
         const float* R ht = d.ht;
         float h = ht[c];
         w0 = (0.5f * s.rdy) * ((ht[jp * nx + i] - h) * vn + (h - ht[jm * nx + i]) * vs)
              + (0.5f * s.rdx) * ((ht[j * nx + ip] - h) * ue + (h - ht[j * nx + im]) * uw);
-        w_pp[c] = w0;
+        gst(d.w_pp, 0, bc, w0);
     }
 
+    // q: level of the loads.  Mass and face loads clamp to nz-1 (their
+    // values are unused past the top); w-level loads use q <= nz.
+    auto load_up = [&](ColUp& L, int q) {
+        const int qm = q < nz ? q : nz - 1;
+        const int qu = (q + 1 < nz) ? q + 1 : nz - 1;
+        const int qw = q + 1 <= nz ? q + 1 : nz;
+        const unsigned lm = qm * st;
+        L.ui = gld(d.u_pp, qm * stu, bu);
+        L.uip = gld(d.u_pp, qm * stu, bu1);
+        L.vj = gld(d.v_pp, qm * stv, bc);
+        L.vjp = gld(d.v_pp, qm * stv, bvn);
+        L.te = gld(d.thp, lm, bE);
+        L.tw = gld(d.thp, lm, bW);
+        L.tn = gld(d.thp, lm, bN);
+        L.ts = gld(d.thp, lm, bS);
+        L.thu = gld(d.thp, qu * st, bc);
+        if (B3) {
+            L.be = gld(d.thb, lm, bE);
+            L.bw = gld(d.thb, lm, bW);
+            L.bn = gld(d.thb, lm, bN);
+            L.bs = gld(d.thb, lm, bS);
+            L.bhu = gld(d.thb, qu * st, bc);
+            L.ghu = gld(d.phb, qw * st, bc);
+        }
+        L.thpp = gld(d.th_pp, lm, bc);
+        L.rth = gld(d.rth_t, lm, bc);
+        L.phu = gld(d.php, qw * st, bc);
+        L.alt = gld(d.alt, lm, bc);
+        L.p = gld(d.p, lm, bc);
+        L.phpp = gld(d.ph_pp, q * st, bc);
+        L.wpp = gld(d.w_pp, q * st, bc);
+        L.rph = gld(d.rph_t, q * st, bc);
+        L.rw = gld(d.rw_t, (q >= 1 ? q - 1 : 0) * st, bc);
+    };
+
     // upward sweep
-    if (write_ww) d.ww_pp[c] = 0.0f;
+    if (WW) gst(d.ww_pp, 0, bc, 0.0f);
     float ww_k = 0.0f, wwk = 0.0f, wdtn_k = 0.0f;
-    float th_m = thr(0, c), ph_m = phs(0), wd_prev = 0.0f;
+    float th_m = (B3 ? gld(d.thb, 0, bc) : thb1[0]) + gld(d.thp, 0, bc);
+    float ph_m = (B3 ? gld(d.phb, 0, bc) : phb1[0]) + gld(d.php, 0, bc);
+    float wd_prev = 0.0f;
     float rhs_a = 0.0f, rhs_b = 0.0f;
-    float pho_a = 0.0f, pho_b = ph_pp[c];
+    float pho_a = 0.0f, pho_b = gld(d.ph_pp, 0, bc);
     float t2_a = 0.0f, t2_b = 0.0f;
     float c2a_a = 0.0f, c2a_b = 0.0f;
     float alt_a = 0.0f, alt_b = 0.0f;
@@ -947,13 +1140,14 @@ __global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout,
     const float c_force = (0.5f * dtau) * G_F;
     const float c_buoy = dtau * G_F;
 
-    auto force_row = [&](int r, float rhs_dn, float rhs_c, float rhs_up,
-                         float ph_dn, float ph_c, float ph_up, float wold,
-                         float t2dn, float t2c, float c2dn, float c2c,
-                         float altdn, float altc, float chmdn, float chmc) {
+    auto force_row = [&](int r, float rwr, float rhs_dn, float rhs_c,
+                         float rhs_up, float ph_dn, float ph_c, float ph_up,
+                         float wold, float t2dn, float t2c, float c2dn,
+                         float c2c, float altdn, float altc, float chmdn,
+                         float chmc) {
         float dph_up = one_p * (rhs_up - rhs_c) + one_m * (ph_up - ph_c);
         float dph_dn = one_p * (rhs_c - rhs_dn) + one_m * (ph_c - ph_dn);
-        float t1 = dtau * rw_t[r * st + c]
+        float t1 = dtau * rwr
                    + (c_force * rdn[r])
                          * ((c2c * rdnw[r]) / chmc * dph_up
                             - (c2dn * rdnw[r - 1]) / chmdn * dph_dn);
@@ -963,66 +1157,80 @@ __global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout,
     };
     // LU factors of row r (1 <= r <= nz-1) from the windowed c2a/chm
     auto lu_row = [&](int r, float c2dn, float c2c, float chmdn, float chmc,
-                      float& alpha_r) {
+                      float& alpha_r, bool r_ge2) {
         float a_r = 0.0f;
-        if (r >= 2)
+        if (r_ge2)
             a_r = -cof * rdn[r] * rdnw[r - 1] * c2dn / (chmdn * cfm_at(r - 1));
         float b = 1.0f + cof * rdn[r] * (rdnw[r] * c2c / (chmc * cfm_at(r))
                                          + rdnw[r - 1] * c2dn / (chmdn * cfm_at(r)));
         float cc = -cof * rdn[r] * rdnw[r] * c2c / (chmc * cfm_at(r + 1));
         alpha_r = 1.0f / (b - a_r * gam_prev);
         gam_prev = cc * alpha_r;
-        gam[r * st + c] = gam_prev;
+        gst(d.gam, r * st, bc, gam_prev);
         return a_r;
     };
 
-    for (int m = 0; m < nz; ++m) {
-        const int h = m * st + c;
-        float dvp = dvp_at(m);
+    ColUp nxt;
+    load_up(nxt, 0);
+    // One upward level; the kind fixes at compile time which parts exist:
+    // rhs of phi'' from level 1, an eliminated row from level 2, and no
+    // Omega/theta flux above the top level.
+    auto up_level = [&](int m, auto kind) {
+        constexpr int K = decltype(kind)::value;
+        constexpr bool has_up = K != LV_LAST;       // m + 1 < nz
+        constexpr bool has_rhs = K != LV_FIRST;     // m >= 1
+        constexpr bool has_row = K >= LV_THIRD;     // m >= 2
+        opaque_all();
+        const ColUp L = nxt;
+        load_up(nxt, m + 1);
+        const unsigned ms = m * st;
+        float dvp = s.rdx * (L.uip - L.ui) + s.rdy * (L.vjp - L.vj);
         float ww_kp1;
-        if (m + 1 < nz) {
+        if (has_up) {
             wwk = wwk - dnw[m] * (c1h[m] * (dmdt_pp + rmu) + dvp);
             ww_kp1 = wwk;
         } else {
             ww_kp1 = 0.0f;
         }
-        if (write_ww) d.ww_pp[h + st] = ww_kp1;
-        float th_up = (m + 1 < nz) ? thr(m + 1, c) : 0.0f;
-        float wdtn_kp1 = (m + 1 < nz)
+        if (WW) gst(d.ww_pp, ms + st, bc, ww_kp1);
+        float th_up = has_up ? thbv(L.bhu, m + 1) + L.thu : 0.0f;
+        float wdtn_kp1 = has_up
             ? ww_kp1 * (fnm[m + 1] * th_up + fnp[m + 1] * th_m) : 0.0f;
         float t_c = th_m;
-        float hx = (0.5f * s.rdx) * (u_pp[iU(m, j, i + 1, ny, nx)] * (thr(m, j * nx + ip) + t_c)
-                                     - u_pp[iU(m, j, i, ny, nx)] * (t_c + thr(m, j * nx + im)));
-        float hy = (0.5f * s.rdy) * (v_pp[iV(m, j + 1, i, ny, nx)] * (thr(m, jp * nx + i) + t_c)
-                                     - v_pp[iV(m, j, i, ny, nx)] * (t_c + thr(m, jm * nx + i)));
-        float tho = th_pp[h];
-        float thn = tho + dtau * (rth_t[h] - ((hx + hy) + rdnw[m] * (wdtn_kp1 - wdtn_k)));
-        th_pp[h] = thn;
+        float hx = (0.5f * s.rdx) * (L.uip * (thbv(L.be, m) + L.te + t_c)
+                                     - L.ui * (t_c + (thbv(L.bw, m) + L.tw)));
+        float hy = (0.5f * s.rdy) * (L.vjp * (thbv(L.bn, m) + L.tn + t_c)
+                                     - L.vj * (t_c + (thbv(L.bs, m) + L.ts)));
+        float tho = L.thpp;
+        float thn = tho + dtau * (L.rth - ((hx + hy) + rdnw[m] * (wdtn_kp1 - wdtn_k)));
+        gst(d.th_pp, ms, bc, thn);
         float chs = c1h[m] * muts + c2h[m];
         float t2 = (0.5f * (one_p * thn + one_m * tho)) / (chs * t_c);
-        float ph_up = phs(m + 1);
+        float ph_up = phbv(L.ghu, m + 1) + L.phu;
         float wd = ((0.5f * (ww_kp1 + ww_k)) * rdnw[m]) * (ph_up - ph_m);
-        float altm = alt[h];
-        float c2a = GAMMA_F * p[h] / altm;
+        float altm = L.alt;
+        float c2a = GAMMA_F * L.p / altm;
         float chm = c1h[m] * mut + c2h[m];
         float rhs_c = 0.0f, pho_c = pho_b, wold_c = 0.0f;
-        if (m >= 1) {
-            pho_c = ph_pp[h];
-            wold_c = w_pp[h];
-            float r = dtau * (rph_t[h] + ghalf_eps * wold_c);
+        if (has_rhs) {
+            pho_c = L.phpp;
+            wold_c = L.wpp;
+            float r = dtau * (L.rph + ghalf_eps * wold_c);
             r = r - dtau * (fnm[m] * wd + fnp[m] * wd_prev);
             rhs_c = pho_c + r / cfm_at(m);
-            ph_pp[h] = rhs_c;
+            gst(d.ph_pp, ms, bc, rhs_c);
         }
-        if (m >= 2) {
+        if (has_row) {
             const int r = m - 1;
             float alpha_r;
-            float a_r = lu_row(r, c2a_a, c2a_b, chm_a, chm_b, alpha_r);
-            float wr = force_row(r, rhs_a, rhs_b, rhs_c, pho_a, pho_b, pho_c,
-                                 wold_b, t2_a, t2_b, c2a_a, c2a_b, alt_a,
-                                 alt_b, chm_a, chm_b);
+            const bool r_ge2 = K == LV_MID ? true
+                               : K == LV_THIRD ? false : r >= 2;
+            float a_r = lu_row(r, c2a_a, c2a_b, chm_a, chm_b, alpha_r, r_ge2);
+            float wr = force_row(r, L.rw, rhs_a, rhs_b, rhs_c, pho_a, pho_b,
+                                 pho_c, wold_b, t2_a, t2_b, c2a_a, c2a_b,
+                                 alt_a, alt_b, chm_a, chm_b);
             float wf = (wr - a_r * wf_prev) * alpha_r;
-            w_pp[r * st + c] = wf;
+            gst(d.w_pp, r * st, bc, wf);
             wf_prev = wf;
         }
         rhs_a = rhs_b; rhs_b = rhs_c;
@@ -1037,23 +1245,29 @@ __global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout,
         th_m = th_up;
         ph_m = ph_up;
         wd_prev = wd;
-    }
+    };
+    up_level(0, Kind<LV_FIRST>{});
+    up_level(1, Kind<LV_SECOND>{});
+    if (nz > 3) up_level(2, Kind<LV_THIRD>{});
+    for (int m = 3; m < nz - 1; ++m) up_level(m, Kind<LV_MID>{});
+    up_level(nz - 1, Kind<LV_LAST>{});
     {
-        const int f = nz * st + c;
-        const float pho_c = ph_pp[f];
-        const float wold_c = w_pp[f];
-        float r = dtau * (rph_t[f] + ghalf_eps * wold_c);
+        // nxt holds the w-level nz loads (phi'', w'', rph_t) and rw_t at nz-1
+        const unsigned f = nz * st;
+        const float pho_c = nxt.phpp;
+        const float wold_c = nxt.wpp;
+        float r = dtau * (nxt.rph + ghalf_eps * wold_c);
         float rhs_c = pho_c + r / cfm_at(nz);
         if (s.top_lid) rhs_c = 0.0f;
-        ph_pp[f] = rhs_c;
+        gst(d.ph_pp, f, bc, rhs_c);
         const int rr = nz - 1;
         float alpha_r;
-        float a_r = lu_row(rr, c2a_a, c2a_b, chm_a, chm_b, alpha_r);
-        float wr = force_row(rr, rhs_a, rhs_b, rhs_c, pho_a, pho_b, pho_c,
-                             wold_b, t2_a, t2_b, c2a_a, c2a_b, alt_a, alt_b,
-                             chm_a, chm_b);
+        float a_r = lu_row(rr, c2a_a, c2a_b, chm_a, chm_b, alpha_r, rr >= 2);
+        float wr = force_row(rr, nxt.rw, rhs_a, rhs_b, rhs_c, pho_a, pho_b,
+                             pho_c, wold_b, t2_a, t2_b, c2a_a, c2a_b, alt_a,
+                             alt_b, chm_a, chm_b);
         float wfr = (wr - a_r * wf_prev) * alpha_r;
-        w_pp[rr * st + c] = wfr;
+        gst(d.w_pp, rr * st, bc, wfr);
         float a_top = 0.0f;
         float wtop;
         if (s.top_lid) {
@@ -1066,141 +1280,244 @@ __global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout,
                           / chm_b * dph_dn;
             float cterm = dtau * G_F * (2.0f * rdnw[nz - 1] * c2a_b * alt_b * t2_b
                                         + c1f[nz] * muave);
-            wtop = wold_c + ((dtau * rw_t[f] - bterm) - cterm);
+            wtop = wold_c + ((dtau * d.rw_t[f + c] - bterm) - cterm);
         }
         float b_top = 1.0f + 2.0f * cof * rdnw[nz - 1] * rdnw[nz - 1] * c2a_b
                              / (chm_b * cfm_at(nz));
         float alpha_top = 1.0f / (b_top - a_top * gam_prev);
-        w_pp[f] = (wtop - a_top * wfr) * alpha_top;
+        gst(d.w_pp, f, bc, (wtop - a_top * wfr) * alpha_top);
     }
 
     // downward sweep: back substitution, damper, phi'' update, EOS
     float htop = 0.0f, hbot = 0.0f;
-    if (s.damp) {
-        htop = phs(nz) / G_F;
+    if (DAMP) {
+        htop = ((B3 ? d.phb[nz * st + c] : phb1[nz]) + d.php[nz * st + c]) / G_F;
         hbot = htop - s.zdamp;
     }
     const float c_ph = ((0.5f * dtau) * G_F) * one_p;
+    auto load_dn = [&](ColDn& L, int k) {
+        const unsigned ks = k * st;
+        const unsigned km = (k < nz ? k : nz - 1) * st;
+        L.wpp = gld(d.w_pp, ks, bc);
+        L.gam = gld(d.gam, ks, bc);
+        L.phpp = gld(d.ph_pp, ks, bc);
+        if (DAMP) {
+            L.w = gld(d.w, ks, bc);
+            L.php = gld(d.php, ks, bc);
+            if (B3) L.phb = gld(d.phb, ks, bc);
+        }
+        L.thp = gld(d.thp, km, bc);
+        if (B3) L.thb = gld(d.thb, km, bc);
+        L.alt = gld(d.alt, km, bc);
+        L.p = gld(d.p, km, bc);
+        L.thpp = gld(d.th_pp, km, bc);
+    };
     float wb_next = 0.0f, phn_up = 0.0f;
-    for (int k = nz; k >= 0; --k) {
-        const int f = k * st + c;
+    ColDn dn;
+    load_dn(dn, nz);
+    // kinds: LV_FIRST = top w level (no mass level, no gamma), LV_MID,
+    // LV_LAST = surface (phi'' only)
+    auto dn_level = [&](int k, auto kind) {
+        constexpr int K = decltype(kind)::value;
+        colb_opaque(bc);
+        const ColDn L = dn;
+        load_dn(dn, k >= 1 ? k - 1 : 0);
+        const unsigned f = k * st;
         float phn;
-        if (k >= 1) {
-            float wb = w_pp[f];
-            if (k < nz) wb = wb - gam[f] * wb_next;
+        if (K != LV_LAST) {
+            float wb = L.wpp;
+            if (K != LV_FIRST) wb = wb - L.gam * wb_next;
             wb_next = wb;
             float wfin = wb;
-            if (s.damp) {
-                float hk = phs(k) / G_F;
+            if (DAMP) {
+                float hk = (phbv(L.phb, k) + L.php) / G_F;
                 if (hk >= hbot) {
                     float arg = (1.5707963f * (hk - hbot)) / s.zdamp;
                     float sn = (float)sin((double)arg);
                     float dw = (s.dampmag * sn) * sn;
-                    wfin = (wb - (dw * cfm_at(k)) * d.w[f]) / (1.0f + dw);
+                    wfin = (wb - (dw * cfm_at(k)) * L.w) / (1.0f + dw);
                 }
             }
-            w_pp[f] = wfin;
+            gst(d.w_pp, f, bc, wfin);
             float cfs = c1f[k] * muts + c2f[k];
-            phn = ph_pp[f] + (c_ph * wfin) / cfs;
-            ph_pp[f] = phn;
+            phn = L.phpp + (c_ph * wfin) / cfs;
+            gst(d.ph_pp, f, bc, phn);
         } else {
-            phn = ph_pp[f];
+            phn = L.phpp;
         }
-        if (k < nz) {
+        if (K != LV_FIRST) {
             float chm2 = c1h[k] * muts + c2h[k];
-            float th_ref = thr(k, c);
-            float altk = alt[f];
-            float c2a = GAMMA_F * p[f] / altk;
+            float th_ref = thbv(L.thb, k) + L.thp;
+            float altk = L.alt;
+            float c2a = GAMMA_F * L.p / altk;
             float al = -((altk * (c1h[k] * mupp)) + rdnw[k] * (phn_up - phn)) / chm2;
-            if (write_al) d.al_pp[f] = al;
-            pout[f] = c2a * ((altk * (th_pp[f] - (c1h[k] * mupp) * th_ref))
-                             / (chm2 * th_ref) - al);
+            if (AL) gst(d.al_pp, f, bc, al);
+            gst(pout, f, bc, c2a * ((altk * (L.thpp - (c1h[k] * mupp) * th_ref))
+                                    / (chm2 * th_ref) - al));
         }
         phn_up = phn;
-    }
+    };
+    dn_level(nz, Kind<LV_FIRST>{});
+    for (int k = nz - 1; k >= 1; --k) dn_level(k, Kind<LV_MID>{});
+    dn_level(0, Kind<LV_LAST>{});
 }
 
 // ---------------------------------------------------------------------------
 // Small-step finish (reference dycore.cu:1087-1198) + EOS of the new state
 // ---------------------------------------------------------------------------
 
+// Coupled-to-uncoupled fold of one face (reference dycore.cu:1087-1127):
+// new = (C(mu_t*) x_t* + x'') / C(mu_t* + mu'').
+template <int STRIDE_KIND>
+__device__ __forceinline__ void finish_face(const Dom& d, float* R out,
+                                            const float* R x,
+                                            const float* R xpp, int f,
+                                            float msf, float mnf)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    const int sf = STRIDE_KIND == 0 ? ny * (nx + 1) : (ny + 1) * nx;
+    kfloat* c1h = kview(d.c1h);
+    kfloat* c2h = kview(d.c2h);
+    unsigned bf = 4u * f;
+    for (int k = 0; k < nz; ++k) {
+        colb_opaque(bf);
+        const float c1 = c1h[k], c2 = c2h[k];
+        gst(out, k * sf, bf, ((c1 * msf + c2) * gld(x, k * sf, bf)
+                              + gld(xpp, k * sf, bf)) / (c1 * mnf + c2));
+    }
+}
+
+struct FinIn {
+    float u = 0.0f, upp = 0.0f, v = 0.0f, vpp = 0.0f;
+    float w = 0.0f, wpp = 0.0f, php = 0.0f, phpp = 0.0f;
+    float thp = 0.0f, thpp = 0.0f;
+    float thb = 0.0f, phb = 0.0f, alb = 0.0f;   // terrain base
+};
+
+// Small-step finish (reference dycore.cu:1087-1198) of the mass column
+// (j, i), its u face (j, i) and v face (j, i), then the EOS of the new
+// state (diagnostics.cu:37-89).
+template <bool B3>
 __global__ __launch_bounds__(256) void k_finish(Dom d, FinishDst o)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i > nx || j > ny) return;
-    const bool hu = j < ny, hv = i < nx, hc = hu && hv;
-    const int ic = (i >= nx) ? i - nx : i, jc = (j >= ny) ? j - ny : j;
-    const int im = (ic == 0) ? nx - 1 : ic - 1;
-    const int jm = (jc == 0) ? ny - 1 : jc - 1;
+    const int stu = ny * (nx + 1), stv = (ny + 1) * nx;
+    const int c = xcd_block() * blockDim.x + threadIdx.x;
+    if (c >= st) return;
+    const int j = c / nx, i = c - j * nx;
+    const int im = (i == 0) ? nx - 1 : i - 1;
+    const int jm = (j == 0) ? ny - 1 : j - 1;
+    kfloat* c1h = kview(d.c1h);
+    kfloat* c2h = kview(d.c2h);
+    kfloat* c1f = kview(d.c1f);
+    kfloat* c2f = kview(d.c2f);
+    kfloat* rdnw = kview(d.rdnw);
+    kfloat* thb1 = kview(d.thb);
+    kfloat* phb1 = kview(d.phb);
+    kfloat* alb1 = kview(d.alb);
     const float* R mub = d.mub2d;
     const float* R mup = d.mup;
     const float* R mu_pp = d.mu_pp;
-    const float* R c1h = d.c1h;
-    const float* R c2h = d.c2h;
-    const float* R c1f = d.c1f;
-    const float* R c2f = d.c2f;
-    const int b3 = d.base3d;
-    const int cA = jc * nx + ic;
-    const float msA = mub[cA] + mup[cA], mnA = msA + mu_pp[cA];
-    float msf_u = 0.0f, mnf_u = 0.0f, msf_v = 0.0f, mnf_v = 0.0f;
-    if (hu) {
-        const int cB = jc * nx + im;
-        const float msB = mub[cB] + mup[cB], mnB = msB + mu_pp[cB];
-        msf_u = 0.5f * (msA + msB);
-        mnf_u = 0.5f * (mnA + mnB);
-    }
-    if (hv) {
-        const int cB = jm * nx + ic;
-        const float msB = mub[cB] + mup[cB], mnB = msB + mu_pp[cB];
-        msf_v = 0.5f * (msA + msB);
-        mnf_v = 0.5f * (mnA + mnB);
-    }
-    const float mup_new = hc ? mup[cA] + mu_pp[cA] : 0.0f;
-    const float mu_eos = hc ? mub[cA] + mup_new : 0.0f;
-    float php_lo = 0.0f, thp_lo = 0.0f;
-    for (int k = 0; k <= nz; ++k) {
-        if (k < nz) {
+    const float msA = mub[c] + mup[c], mnA = msA + mu_pp[c];
+    const int cW = j * nx + im, cS = jm * nx + i;
+    const float msW = mub[cW] + mup[cW], mnW = msW + mu_pp[cW];
+    const float msS = mub[cS] + mup[cS], mnS = msS + mu_pp[cS];
+    const float msf_u = 0.5f * (msA + msW), mnf_u = 0.5f * (mnA + mnW);
+    const float msf_v = 0.5f * (msA + msS), mnf_v = 0.5f * (mnA + mnS);
+    const float mup_new = mup[c] + mu_pp[c];
+    const float mu_eos = mub[c] + mup_new;
+    unsigned bc = 4u * c, bu = 4u * (j * (nx + 1) + i);
+
+    auto load = [&](FinIn& L, int k) {
+        const int km = k < nz ? k : nz - 1;
+        L.u = gld(d.u, km * stu, bu);
+        L.upp = gld(d.u_pp, km * stu, bu);
+        L.v = gld(d.v, km * stv, bc);
+        L.vpp = gld(d.v_pp, km * stv, bc);
+        L.w = gld(d.w, k * st, bc);
+        L.wpp = gld(d.w_pp, k * st, bc);
+        L.php = gld(d.php, k * st, bc);
+        L.phpp = gld(d.ph_pp, k * st, bc);
+        L.thp = gld(d.thp, km * st, bc);
+        L.thpp = gld(d.th_pp, km * st, bc);
+        if (B3) {
+            L.thb = gld(d.thb, km * st, bc);
+            L.phb = gld(d.phb, k * st, bc);
+            L.alb = gld(d.alb, km * st, bc);
+        }
+    };
+    float php_lo = 0.0f, thp_lo = 0.0f, thb_lo = 0.0f, phb_lo = 0.0f;
+    float alb_lo = 0.0f;
+    FinIn nxt;
+    load(nxt, 0);
+    // kinds: 0 = surface (no EOS below), 1 = interior, 2 = top w level
+    auto level = [&](int k, auto kind) {
+        constexpr int K = decltype(kind)::value;
+        colb_opaque(bc);
+        colb_opaque(bu);
+        const FinIn L = nxt;
+        if (K != 2) load(nxt, k + 1);
+        const unsigned f = k * st;
+        if (K != 2) {
             const float c1 = c1h[k], c2 = c2h[k];
-            if (hu) {
-                const int ix = iU(k, jc, i, ny, nx);
-                o.u[ix] = ((c1 * msf_u + c2) * d.u[ix] + d.u_pp[ix])
-                          / (c1 * mnf_u + c2);
-            }
-            if (hv) {
-                const int ix = iV(k, j, ic, ny, nx);
-                o.v[ix] = ((c1 * msf_v + c2) * d.v[ix] + d.v_pp[ix])
-                          / (c1 * mnf_v + c2);
-            }
+            gst(o.u, k * stu, bu, ((c1 * msf_u + c2) * L.u + L.upp)
+                                      / (c1 * mnf_u + c2));
+            gst(o.v, k * stv, bc, ((c1 * msf_v + c2) * L.v + L.vpp)
+                                      / (c1 * mnf_v + c2));
         }
-        if (hc) {
-            const int f = k * st + cA;
-            o.w[f] = ((c1f[k] * msA + c2f[k]) * d.w[f] + d.w_pp[f])
-                     / (c1f[k] * mnA + c2f[k]);
-            const float php_new = d.php[f] + d.ph_pp[f];
-            o.php[f] = php_new;
-            if (k >= 1) {
-                const int h = f - st;
-                float th = basev(d.thb, k - 1, cA, st, b3) + thp_lo;
-                float dph = (basev(d.phb, k, cA, st, b3) + php_new)
-                            - (basev(d.phb, k - 1, cA, st, b3) + php_lo);
-                float alt = (-dph * d.rdnw[k - 1]) / (c1h[k - 1] * mu_eos + c2h[k - 1]);
-                d.alt[h] = alt;
-                d.al[h] = alt - basev(d.alb, k - 1, cA, st, b3);
-                d.p[h] = P0_F * pow_gamma((RD_F * th) / (P0_F * alt));
-            }
-            if (k < nz) {
-                float thbk = basev(d.thb, k, cA, st, b3);
-                float num = (c1h[k] * msA + c2h[k]) * (thbk + d.thp[f]) + d.th_pp[f];
-                float thn = num / (c1h[k] * mnA + c2h[k]) - thbk;
-                o.thp[f] = thn;
-                thp_lo = thn;
-            }
-            php_lo = php_new;
+        gst(o.w, f, bc, ((c1f[k] * msA + c2f[k]) * L.w + L.wpp)
+                            / (c1f[k] * mnA + c2f[k]));
+        const float php_new = L.php + L.phpp;
+        gst(o.php, f, bc, php_new);
+        const float phb_k = B3 ? L.phb : phb1[k];
+        if (K != 0) {
+            const unsigned h = f - st;
+            float th = thb_lo + thp_lo;
+            float dph = (phb_k + php_new) - (phb_lo + php_lo);
+            float alt = (-dph * rdnw[k - 1]) / (c1h[k - 1] * mu_eos + c2h[k - 1]);
+            gst(d.alt, h, bc, alt);
+            gst(d.al, h, bc, alt - alb_lo);
+            gst(d.p, h, bc, P0_F * pow_gamma((RD_F * th) / (P0_F * alt)));
         }
+        if (K != 2) {
+            const float thbk = B3 ? L.thb : thb1[k];
+            float num = (c1h[k] * msA + c2h[k]) * (thbk + L.thp) + L.thpp;
+            float thn = num / (c1h[k] * mnA + c2h[k]) - thbk;
+            gst(o.thp, f, bc, thn);
+            thp_lo = thn;
+            thb_lo = thbk;
+            alb_lo = B3 ? L.alb : alb1[k];
+        }
+        php_lo = php_new;
+        phb_lo = phb_k;
+    };
+    level(0, Kind<0>{});
+    for (int k = 1; k < nz; ++k) level(k, Kind<1>{});
+    level(nz, Kind<2>{});
+    o.mup[c] = mup_new;
+}
+
+// Periodic duplicate faces of the finish: u at i = nx, v at j = ny.
+__global__ __launch_bounds__(256) void k_finish_edge(Dom d, FinishDst o)
+{
+    const int ny = d.ny, nx = d.nx;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const float* R mub = d.mub2d;
+    const float* R mup = d.mup;
+    const float* R mu_pp = d.mu_pp;
+    if (t < ny) {
+        const int j = t, cA = j * nx, cB = j * nx + nx - 1;
+        const float msA = mub[cA] + mup[cA], mnA = msA + mu_pp[cA];
+        const float msB = mub[cB] + mup[cB], mnB = msB + mu_pp[cB];
+        finish_face<0>(d, o.u, d.u, d.u_pp, j * (nx + 1) + nx,
+                       0.5f * (msA + msB), 0.5f * (mnA + mnB));
+    } else if (t < ny + nx) {
+        const int i = t - ny, cA = i, cB = (ny - 1) * nx + i;
+        const float msA = mub[cA] + mup[cA], mnA = msA + mu_pp[cA];
+        const float msB = mub[cB] + mup[cB], mnB = msB + mu_pp[cB];
+        finish_face<1>(d, o.v, d.v, d.v_pp, ny * nx + i,
+                       0.5f * (msA + msB), 0.5f * (mnA + mnB));
     }
-    if (hc) o.mup[cA] = mup_new;
 }
 
 }  // namespace
@@ -1209,7 +1526,10 @@ __global__ __launch_bounds__(256) void k_finish(Dom d, FinishDst o)
 // Launchers
 // ---------------------------------------------------------------------------
 
-static inline dim3 cols(const Dom& d) { return dim3((d.ny * d.nx + 255) / 256); }
+static inline dim3 cols(const Dom& d)
+{
+    return dim3(xcd_grid((d.ny * d.nx + 255) / 256));
+}
 static inline dim3 faces(const Dom& d)
 {
     return dim3((d.nx + 1 + 63) / 64, (d.ny + 1 + 3) / 4);
@@ -1221,13 +1541,30 @@ static inline dim3 masses(const Dom& d)
 
 hipError_t run_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_omega, cols(d), dim3(256), 0, st, d, s, do_eos);
+    const bool b3 = d.base3d != 0;
+    if (do_eos && b3)
+        hipLaunchKernelGGL((k_omega<true, true>), cols(d), dim3(256), 0, st, d, s);
+    else if (do_eos)
+        hipLaunchKernelGGL((k_omega<true, false>), cols(d), dim3(256), 0, st, d, s);
+    else
+        hipLaunchKernelGGL((k_omega<false, false>), cols(d), dim3(256), 0, st, d, s);
     return hipGetLastError();
 }
 
 hipError_t run_tend_faces(const Dom& d, const Sc& s, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_tend_faces, faces(d), dim3(64, 4), 0, st, d, s);
+    const dim3 g = rowseg_grid(d.nx, d.ny), b(64, 4);
+    if (d.base3d && s.diff)
+        hipLaunchKernelGGL((k_tend_faces<true, true>), g, b, 0, st, d, s);
+    else if (d.base3d)
+        hipLaunchKernelGGL((k_tend_faces<true, false>), g, b, 0, st, d, s);
+    else if (s.diff)
+        hipLaunchKernelGGL((k_tend_faces<false, true>), g, b, 0, st, d, s);
+    else
+        hipLaunchKernelGGL((k_tend_faces<false, false>), g, b, 0, st, d, s);
+    const int n = d.nz * d.ny + d.nz * d.nx;
+    hipLaunchKernelGGL(k_tend_faces_edge, dim3((n + 255) / 256), dim3(256), 0,
+                       st, d);
     return hipGetLastError();
 }
 
@@ -1237,22 +1574,60 @@ hipError_t run_tend_cols(const Dom& d, const Sc& s, hipStream_t st)
     return hipGetLastError();
 }
 
+template <bool B3, bool USEP>
+static void launch_uv(const Dom& d, const Sc& s, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_uv<B3, USEP>), cols(d), dim3(256), 0, st, d, s);
+    hipLaunchKernelGGL((k_uv_edge<B3, USEP>), dim3((d.ny + d.nx + 255) / 256),
+                       dim3(256), 0, st, d, s);
+}
+
 hipError_t run_uv(const Dom& d, const Sc& s, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_uv, faces(d), dim3(64, 4), 0, st, d, s);
+    const bool b3 = d.base3d != 0, usep = s.use_pold != 0;
+    if (b3 && usep) launch_uv<true, true>(d, s, st);
+    else if (b3) launch_uv<true, false>(d, s, st);
+    else if (usep) launch_uv<false, true>(d, s, st);
+    else launch_uv<false, false>(d, s, st);
     return hipGetLastError();
+}
+
+template <bool B3, bool DAMP, bool AL, bool WW>
+static void launch_column(const Dom& d, const Sc& s, float* pout,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL((k_column<B3, DAMP, AL, WW>), cols(d), dim3(256), 0, st,
+                       d, s, pout);
+}
+
+template <bool B3, bool DAMP>
+static void launch_column_bd(const Dom& d, const Sc& s, float* pout, int al,
+                             int ww, hipStream_t st)
+{
+    if (al && ww) launch_column<B3, DAMP, true, true>(d, s, pout, st);
+    else if (al) launch_column<B3, DAMP, true, false>(d, s, pout, st);
+    else if (ww) launch_column<B3, DAMP, false, true>(d, s, pout, st);
+    else launch_column<B3, DAMP, false, false>(d, s, pout, st);
 }
 
 hipError_t run_column(const Dom& d, const Sc& s, float* pout, int write_al,
                       int write_ww, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_column, cols(d), dim3(256), 0, st, d, s, pout,
-                       write_al, write_ww);
+    const bool b3 = d.base3d != 0, damp = s.damp != 0;
+    if (b3 && damp) launch_column_bd<true, true>(d, s, pout, write_al, write_ww, st);
+    else if (b3) launch_column_bd<true, false>(d, s, pout, write_al, write_ww, st);
+    else if (damp) launch_column_bd<false, true>(d, s, pout, write_al, write_ww, st);
+    else launch_column_bd<false, false>(d, s, pout, write_al, write_ww, st);
     return hipGetLastError();
 }
 
 hipError_t run_finish(const Dom& d, const FinishDst& o, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_finish, faces(d), dim3(64, 4), 0, st, d, o);
+    if (d.base3d)
+        hipLaunchKernelGGL(k_finish<true>, cols(d), dim3(256), 0, st, d, o);
+    else
+        hipLaunchKernelGGL(k_finish<false>, cols(d), dim3(256), 0, st, d, o);
+    hipLaunchKernelGGL(k_finish_edge, dim3((d.ny + d.nx + 255) / 256),
+                       dim3(256), 0, st, d, o);
     return hipGetLastError();
 }
