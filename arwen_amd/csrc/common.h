@@ -183,3 +183,11 @@ static inline unsigned xcd_grid(unsigned nblocks)
 {
     return (nblocks + ARW_NXCD - 1) / ARW_NXCD * ARW_NXCD;
 }
+
+// Pin a value as computed unconditionally.  A select between a cheap and
+// an expensive operand (a division) is otherwise turned back into a
+// branch by the code generator, splitting a level loop into blocks.
+__device__ __forceinline__ void keep(float& x)
+{
+    asm volatile("" : "+v"(x));
+}

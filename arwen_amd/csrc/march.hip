@@ -16,6 +16,12 @@
 
 namespace {
 
+// Compile-time level kinds: loops peel their special levels so the
+// interior level body has no branches.
+template <int V>
+using Kind = std::integral_constant<int, V>;
+enum { LV_FIRST = 0, LV_SECOND = 1, LV_THIRD = 2, LV_MID = 3, LV_LAST = 4 };
+
 __device__ __forceinline__ float mu_at(const float* R mub, const float* R mup,
                                        int c)
 {
@@ -32,7 +38,8 @@ __device__ __forceinline__ float zf(float qm2, float qm1, float q0, float qp1,
     // on zf stays one basic block (fnm/fnp have nz entries)
     const int kc = kf < nz ? kf : nz - 1;
     const float lin = vel * (fnm[kc] * q0 + fnp[kc] * qm1);
-    const float cub = flux3(qm2, qm1, q0, qp1, vel);
+    float cub = flux3(qm2, qm1, q0, qp1, vel);
+    keep(cub);
     const float r = (kf == 1 || kf == nz - 1) ? lin : cub;
     return (kf == 0 || kf == nz) ? 0.0f : r;
 }
@@ -166,6 +173,21 @@ struct RowSeg {
     int i, j;          // column (per lane), row (wave-uniform)
     bool live;
 };
+
+// Row segment of workgroup (blockIdx.x - first) of a grid whose first
+// `first` workgroups (a multiple of ARW_NXCD) do other work.
+__device__ __forceinline__ RowSeg row_seg_from(int nx, int ny, int first)
+{
+    const int nbx = (nx + 63) / 64;
+    const int n = (int)gridDim.x - first, b = (int)blockIdx.x - first;
+    const int L = (b % ARW_NXCD) * (n / ARW_NXCD) + b / ARW_NXCD;
+    const int by = L / nbx, bx = L - by * nbx;
+    RowSeg r;
+    r.i = bx * 64 + (int)threadIdx.x;
+    r.j = __builtin_amdgcn_readfirstlane(by * 4 + (int)threadIdx.y);
+    r.live = r.i < nx && r.j < ny;
+    return r;
+}
 
 __device__ __forceinline__ RowSeg row_seg(int nx, int ny)
 {
@@ -488,103 +510,136 @@ __global__ __launch_bounds__(256) void k_tend_faces_edge(Dom d)
 // Slow tendencies in the mass column (theta, w, phi) + column init
 // ---------------------------------------------------------------------------
 
+// Slow tendencies in the mass column (j, i): rw_t (w advection, buoyancy
+// and vertical PGF, diffusion; reference advection.cu:803-1051,
+// dycore.cu:213-308, diffusion.cu:22-51), rph_t (WRF rhs_ph,
+// dycore.cu:541-750), rth_t (theta advection + diffusion), and the
+// w''/phi''/theta''/p''/alpha'' small-step initialisation
+// (dycore.cu:1012-1085).  Row-segment waves; level kinds peel the surface
+// (k = 0) and the lid (k = nz) so the interior level body is one block.
+template <bool B3, bool DIFF>
 __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= nx || j >= ny) return;
-    const int c = j * nx + i;
-    const int im = (i == 0) ? nx - 1 : i - 1, ip = (i + 1 == nx) ? 0 : i + 1;
-    const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j + 1 == ny) ? 0 : j + 1;
-    int xo[7], yo[7];
+    const int pu = nx + 1, stu = ny * pu, stv = (ny + 1) * nx;
+    const RowSeg rs = row_seg(nx, ny);
+    if (!rs.live) return;
+    const int i = rs.i, j = rs.j;
+    kfloat* c1h = kview(d.c1h);
+    kfloat* c2h = kview(d.c2h);
+    kfloat* c1f = kview(d.c1f);
+    kfloat* c2f = kview(d.c2f);
+    kfloat* fnm = kview(d.fnm);
+    kfloat* fnp = kview(d.fnp);
+    kfloat* rdn = kview(d.rdn);
+    kfloat* rdnw = kview(d.rdnw);
+    kfloat* thb1 = kview(d.thb);
+    kfloat* phb1 = kview(d.phb);
+    kfloat* pb1 = kview(d.pb);
+    kfloat* rdzf_h = kview(d.rdzf_h);
+    kfloat* rdzc_h = kview(d.rdzc_h);
+    kfloat* rdzf_w = kview(d.rdzf_w);
+    kfloat* rdzc_w = kview(d.rdzc_w);
+    int yo[7];
 #pragma unroll
-    for (int o = 0; o < 7; ++o) {
-        xo[o] = j * nx + pmod(i + o - 3, nx);
-        yo[o] = pmod(j + o - 3, ny) * nx + i;
-    }
-    const float* R u = d.u;
-    const float* R v = d.v;
-    const float* R w = d.w;
-    const float* R thp = d.thp;
-    const float* R thb = d.thb;
-    const float* R php = d.php;
-    const float* R phb = d.phb;
-    const float* R ww = d.ww;
-    const float* R p = d.p;
-    const float* R pb = d.pb;
-    const float* R c1h = d.c1h;
-    const float* R c2h = d.c2h;
-    const float* R c1f = d.c1f;
-    const float* R c2f = d.c2f;
-    const float* R fnm = d.fnm;
-    const float* R fnp = d.fnp;
-    const float* R rdn = d.rdn;
-    const float* R rdnw = d.rdnw;
+    for (int o = 0; o < 7; ++o) yo[o] = pmod(j + o - 3, ny) * nx;
+    const int rA = j * nx, rS = yo[2], rN = yo[4];
+    unsigned cx[7];
+#pragma unroll
+    for (int o = 0; o < 7; ++o) cx[o] = 4u * pmod(i + o - 3, nx);
+    unsigned ci1 = 4u * (i + 1);
+    auto opaque_all = [&]() {
+#pragma unroll
+        for (int o = 0; o < 7; ++o) colb_opaque(cx[o]);
+        colb_opaque(ci1);
+    };
+    unsigned& cC = cx[3];
     const float* R mub = d.mub2d;
     const float* R mup = d.mup;
-    const int b3 = d.base3d;
-
-    const float mA = mu_at(mub, mup, c), mW = mu_at(mub, mup, j * nx + im);
-    const float mE = mu_at(mub, mup, j * nx + ip);
-    const float mS = mu_at(mub, mup, jm * nx + i);
-    const float mN = mu_at(mub, mup, jp * nx + i);
+    auto mu2 = [&](int row, unsigned cb) {
+        return gld(mub, row, cb) + gld(mup, row, cb);
+    };
+    const float mA = mu2(rA, cC), mW = mu2(rA, cx[2]);
+    const float mE = mu2(rA, cx[4]), mS = mu2(rS, cC), mN = mu2(rN, cC);
     const float mux_i = 0.5f * (mA + mW), mux_ip = 0.5f * (mE + mA);
     const float muy_j = 0.5f * (mA + mS), muy_jp = 0.5f * (mN + mA);
-    const float mupc = mup[c];
-    const float mt = mub[c] + d.mup0[c];
-    const float mupp = d.mup0[c] - mupc;
-    d.mu_pp[c] = mupp;
+    const float mupc = gld(mup, rA, cC);
+    const float mt = gld(mub, rA, cC) + gld(d.mup0, rA, cC);
+    const float mupp = gld(d.mup0, rA, cC) - mupc;
+    gst(d.mu_pp, rA, cC, mupp);
     const float qx = s.qrdx, qy = s.qrdy;
 
-    auto th_at = [&](int k, int cc) {
-        return basev(thb, k, cc, st, b3) + thp[k * st + cc];
+    auto thb_at = [&](int k, int row, unsigned cb) {
+        return B3 ? gld(d.thb, k * st + row, cb) : thb1[k];
     };
-    auto ph_at = [&](int k, int cc) {
-        return basev(phb, k, cc, st, b3) + php[k * st + cc];
+    auto phb_at = [&](int k, int row, unsigned cb) {
+        return B3 ? gld(d.phb, k * st + row, cb) : phb1[k];
+    };
+    auto th_at = [&](int k, int row, unsigned cb) {
+        return thb_at(k, row, cb) + gld(d.thp, k * st + row, cb);
+    };
+    auto ph_at = [&](int k, int row, unsigned cb) {
+        return phb_at(k, row, cb) + gld(d.php, k * st + row, cb);
     };
 
     // windows
     float thz[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};   // theta k-2..k+2
     float wz[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};    // w k-2..k+2
     for (int kk = 0; kk < 3; ++kk) {
-        if (kk < nz) thz[2 + kk] = th_at(kk, c);
-        wz[2 + kk] = w[kk * st + c];
+        if (kk < nz) thz[2 + kk] = th_at(kk, rA, cC);
+        wz[2 + kk] = gld(d.w, kk * st + rA, cC);
     }
-    float ww_m = 0.0f, ww_0 = ww[c], ww_p = ww[st + c];
-    float ph_m = 0.0f, ph_0 = ph_at(0, c), ph_p = ph_at(1, c);
+    float ww_m = 0.0f, ww_0 = gld(d.ww, rA, cC), ww_p = gld(d.ww, st + rA, cC);
+    float ph_m = 0.0f, ph_0 = ph_at(0, rA, cC), ph_p = ph_at(1, rA, cC);
     float pp_m = 0.0f;
     float ru_i_m = 0.0f, ru_ip_m = 0.0f, rv_j_m = 0.0f, rv_jp_m = 0.0f;
     float u_i_m = 0.0f, u_i_mm = 0.0f, u_ip_m = 0.0f, u_ip_mm = 0.0f;
     float v_j_m = 0.0f, v_j_mm = 0.0f, v_jp_m = 0.0f, v_jp_mm = 0.0f;
-    float thp_m = 0.0f, thp_0 = thp[c], thp_p = (nz > 1) ? thp[st + c] : 0.0f;
-    float phpp_0 = d.php0[c] - php[c];      // ph'' init at level k
+    float thp_m = 0.0f, thp_0 = gld(d.thp, rA, cC);
+    float thp_p = gld(d.thp, st + rA, cC);
+    float phpp_0 = gld(d.php0, rA, cC) - gld(d.php, rA, cC);
 
-    for (int k = 0; k <= nz; ++k) {
-        const bool mass = k < nz;
-        float pp_0 = mass ? p[k * st + c] - basev(pb, k, c, st, b3) : 0.0f;
+    // kinds: 0 = surface level (w level 0 + mass level 0), 1 = interior,
+    // 2 = lid (w level nz only)
+    auto level = [&](int k, auto kind) {
+        constexpr int K = decltype(kind)::value;
+        constexpr bool mass = K != 2;
+        constexpr bool interior = K == 1;
+        constexpr bool top = K == 2;
+        opaque_all();
+        const int lm = k * st;
+        const int km = k < nz ? k : nz - 1;          // clamped mass level
+        float pp_0 = 0.0f;
+        float p_k = 0.0f;
+        if (mass) {
+            p_k = gld(d.p, lm + rA, cC);
+            pp_0 = p_k - (B3 ? gld(d.pb, lm + rA, cC) : pb1[k]);
+        }
         float ru_i = 0.0f, ru_ip = 0.0f, rv_j = 0.0f, rv_jp = 0.0f;
         float u_i = 0.0f, u_ip = 0.0f, v_j = 0.0f, v_jp = 0.0f;
         if (mass) {
-            ru_i = (c1h[k] * mux_i + c2h[k]) * u[iU(k, j, i, ny, nx)];
-            ru_ip = (c1h[k] * mux_ip + c2h[k]) * u[iU(k, j, i + 1, ny, nx)];
-            rv_j = (c1h[k] * muy_j + c2h[k]) * v[iV(k, j, i, ny, nx)];
-            rv_jp = (c1h[k] * muy_jp + c2h[k]) * v[iV(k, j + 1, i, ny, nx)];
-            u_i = u[iU(k, j, i, ny, nx)];
-            u_ip = u[iU(k, j, ip, ny, nx)];
-            v_j = v[iV(k, j, i, ny, nx)];
-            v_jp = v[iV(k, jp, i, ny, nx)];
+            const int lu = k * stu, lv = k * stv;
+            u_i = gld(d.u, lu + j * pu, cC);
+            const float u_i1 = gld(d.u, lu + j * pu, ci1);
+            u_ip = gld(d.u, lu + j * pu, cx[4]);
+            v_j = gld(d.v, lv + rA, cC);
+            const float v_j1 = gld(d.v, lv + (j + 1) * nx, cC);
+            v_jp = gld(d.v, lv + rN, cC);
+            ru_i = (c1h[k] * mux_i + c2h[k]) * u_i;
+            ru_ip = (c1h[k] * mux_ip + c2h[k]) * u_i1;
+            rv_j = (c1h[k] * muy_j + c2h[k]) * v_j;
+            rv_jp = (c1h[k] * muy_jp + c2h[k]) * v_j1;
         }
 
         // --- rw_t: advection + buoyancy + diffusion ---
         float t = 0.0f;
         float wx2 = 0.0f, wx4 = 0.0f, wy2 = 0.0f, wy4 = 0.0f;
-        if (k >= 1 && k < nz) {
+        if (interior) {
             float x[7], y[7];
 #pragma unroll
             for (int o = 0; o < 7; ++o) {
-                x[o] = (o == 3) ? wz[2] : w[k * st + xo[o]];
-                y[o] = (o == 3) ? wz[2] : w[k * st + yo[o]];
+                x[o] = (o == 3) ? wz[2] : gld(d.w, lm + rA, cx[o]);
+                y[o] = (o == 3) ? wz[2] : gld(d.w, lm + yo[o], cC);
             }
             wx2 = x[2]; wx4 = x[4]; wy2 = y[2]; wy4 = y[4];
             float u0v = fnm[k] * ru_i + fnp[k] * ru_i_m;
@@ -599,39 +654,38 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
             const float wwv[3] = {ww_m, ww_0, ww_p};
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                int m = k - 1 + q;
-                float vel = 0.5f * (wwv[q] + wwv[q + 1]);
-                if (m == 0 || m == nz - 1)
-                    fz[q] = 0.5f * vel * (wz[q + 2] + wz[q + 1]);
-                else
-                    fz[q] = flux3(wz[q], wz[q + 1], wz[q + 2], wz[q + 3], vel);
+                const int m = k - 1 + q;
+                const float vel = 0.5f * (wwv[q] + wwv[q + 1]);
+                const float lin = 0.5f * vel * (wz[q + 2] + wz[q + 1]);
+                float cub = flux3(wz[q], wz[q + 1], wz[q + 2], wz[q + 3], vel);
+                keep(cub);
+                fz[q] = (m == 0 || m == nz - 1) ? lin : cub;
             }
             t = div3(fx1 - fx0, fy1 - fy0, fz[1] - fz[0], s.rdx, s.rdy, rdn[k]);
             float term = rdn[k] * (pp_0 - pp_m) - c1f[k] * mupc;
             t = t + G_F * term;
-        } else if (k == nz) {
+        } else if (top) {
             float pressure = ((1.0f * 2.0f) * rdnw[nz - 1]) * (0.0f - pp_m);
             float term = pressure - c1f[nz] * mupc;
             t = t + G_F * term;
         }
-        if (s.diff) {
+        if (DIFF) {
             float lap = 0.0f;
-            if (k >= 1 && k < nz) {
+            if (interior) {
                 float fc = wz[2];
                 lap = s.kh * (s.dxi2 * ((wx4 - 2.0f * fc) + wx2)
                               + s.dyi2 * ((wy4 - 2.0f * fc) + wy2));
-                float fup = (wz[3] - fc) * d.rdzf_w[k];
-                float fdn = (fc - wz[1]) * d.rdzf_w[k - 1];
-                lap = lap + (s.kv * (fup - fdn)) * d.rdzc_w[k];
+                float fup = (wz[3] - fc) * rdzf_w[k];
+                float fdn = (fc - wz[1]) * rdzf_w[k - 1];
+                lap = lap + (s.kv * (fup - fdn)) * rdzc_w[k];
             }
             t = t + (c1f[k] * mA + c2f[k]) * lap;
         }
-        d.rw_t[k * st + c] = t;
+        gst(d.rw_t, lm + rA, cC, t);
 
         // --- rph_t: WRF rhs_ph ---
         float r = 0.0f;
-        if (k >= 1) {
-            const bool top = (k == nz);
+        if (K != 0) {
             if (!top) {
                 float wd_lo = ((0.5f * (ww_0 + ww_m)) * rdnw[k - 1]) * (ph_0 - ph_m);
                 float wd_hi = ((0.5f * (ww_p + ww_0)) * rdnw[k]) * (ph_p - ph_0);
@@ -658,29 +712,35 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
             float fcy_j = (c1f[k] * muy_j + c2f[k]) * wj;
             float fcy_jp = (c1f[k] * muy_jp + c2f[k]) * wjp;
             float p0 = ph_0;
-            float fx = fcx_i * (p0 - ph_at(k, j * nx + im));
-            float fxp = fcx_ip * (ph_at(k, j * nx + ip) - p0);
+            float fx = fcx_i * (p0 - ph_at(k, rA, cx[2]));
+            float fxp = fcx_ip * (ph_at(k, rA, cx[4]) - p0);
             r = r - qxk * (fxp + fx);
-            float fy = fcy_j * (p0 - ph_at(k, jm * nx + i));
-            float fyp = fcy_jp * (ph_at(k, jp * nx + i) - p0);
+            float fy = fcy_j * (p0 - ph_at(k, rS, cC));
+            float fyp = fcy_jp * (ph_at(k, rN, cC) - p0);
             r = r - qyk * (fyp + fy);
         }
-        d.rph_t[k * st + c] = r;
+        gst(d.rph_t, lm + rA, cC, r);
 
         // --- w'' and phi'' initialisation at w level k ---
-        d.w_pp[k * st + c] = (c1f[k] * mt + c2f[k]) * d.w0[k * st + c]
-                             - (c1f[k] * mA + c2f[k]) * wz[2];
-        d.ph_pp[k * st + c] = phpp_0;
+        gst(d.w_pp, lm + rA, cC,
+            (c1f[k] * mt + c2f[k]) * gld(d.w0, lm + rA, cC)
+                - (c1f[k] * mA + c2f[k]) * wz[2]);
+        gst(d.ph_pp, lm + rA, cC, phpp_0);
 
         // --- mass level k: rth_t and the theta''/p''/alpha'' init ---
         float phpp_p = 0.0f;
         if (mass) {
-            float qxv[7], qyv[7];
+            float qxv[7], qyv[7], tpx[7], tpy[7];
 #pragma unroll
             for (int o = 0; o < 7; ++o) {
-                qxv[o] = (o == 3) ? thz[2] : th_at(k, xo[o]);
-                qyv[o] = (o == 3) ? thz[2] : th_at(k, yo[o]);
+                if (o == 3) continue;
+                tpx[o] = gld(d.thp, lm + rA, cx[o]);
+                tpy[o] = gld(d.thp, lm + yo[o], cC);
+                qxv[o] = thb_at(k, rA, cx[o]) + tpx[o];
+                qyv[o] = thb_at(k, yo[o], cC) + tpy[o];
             }
+            qxv[3] = thz[2];
+            qyv[3] = thz[2];
             float fx0 = flux5(qxv[0], qxv[1], qxv[2], qxv[3], qxv[4], qxv[5], ru_i);
             float fx1 = flux5(qxv[1], qxv[2], qxv[3], qxv[4], qxv[5], qxv[6], ru_ip);
             float fy0 = flux5(qyv[0], qyv[1], qyv[2], qyv[3], qyv[4], qyv[5], rv_j);
@@ -688,53 +748,61 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
             float fz0 = zf(thz[0], thz[1], thz[2], thz[3], ww_0, k, nz, fnm, fnp);
             float fz1 = zf(thz[1], thz[2], thz[3], thz[4], ww_p, k + 1, nz, fnm, fnp);
             float t2 = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy, rdnw[k]);
-            if (s.diff) {
+            if (DIFF) {
                 float fc = thp_0;
-                float lap = s.kh * (s.dxi2 * ((thp[k * st + j * nx + ip] - 2.0f * fc)
-                                              + thp[k * st + j * nx + im])
-                                    + s.dyi2 * ((thp[k * st + jp * nx + i] - 2.0f * fc)
-                                                + thp[k * st + jm * nx + i]));
-                float fup = (k < nz - 1) ? (thp_p - fc) * d.rdzf_h[k] : 0.0f;
-                float fdn = (k > 0) ? (fc - thp_m) * d.rdzf_h[k - 1] : 0.0f;
-                lap = lap + (s.kv * (fup - fdn)) * d.rdzc_h[k];
+                float lap = s.kh * (s.dxi2 * ((tpx[4] - 2.0f * fc) + tpx[2])
+                                    + s.dyi2 * ((tpy[4] - 2.0f * fc) + tpy[2]));
+                const float gup = (thp_p - fc) * rdzf_h[k < nz - 1 ? k : nz - 2];
+                const float gdn = (fc - thp_m) * rdzf_h[k > 0 ? k - 1 : 0];
+                float fup = (k < nz - 1) ? gup : 0.0f;
+                float fdn = (k > 0) ? gdn : 0.0f;
+                lap = lap + (s.kv * (fup - fdn)) * rdzc_h[k];
                 t2 = t2 + (c1h[k] * mA + c2h[k]) * lap;
             }
-            d.rth_t[k * st + c] = t2;
+            gst(d.rth_t, lm + rA, cC, t2);
 
-            const int h = k * st + c;
-            phpp_p = d.php0[h + st] - php[h + st];
+            phpp_p = gld(d.php0, lm + st + rA, cC) - gld(d.php, lm + st + rA, cC);
             float cht = c1h[k] * mt + c2h[k];
             float chs = c1h[k] * mA + c2h[k];
-            float thbk = basev(thb, k, c, st, b3);
-            float tht = thbk + d.thp0[h];
+            float thbk = thb_at(k, rA, cC);
+            float tht = thbk + gld(d.thp0, lm + rA, cC);
             float ths = thbk + thp_0;
             float thpp = cht * tht - chs * ths;
-            d.th_pp[h] = thpp;
-            float altk = d.alt[h];
-            float c2a = (GAMMA_F * p[h]) / altk;
+            gst(d.th_pp, lm + rA, cC, thpp);
+            float altk = gld(d.alt, lm + rA, cC);
+            float c2a = (GAMMA_F * p_k) / altk;
             float al = -((altk * (c1h[k] * mupp)) + rdnw[k] * (phpp_p - phpp_0)) / cht;
-            if (b3) d.al_pp[h] = al;
+            if (B3) gst(d.al_pp, lm + rA, cC, al);
             float p5 = (altk * (thpp - (c1h[k] * mupp) * ths)) / (cht * ths);
-            d.p_pp[h] = c2a * (p5 - al);
+            gst(d.p_pp, lm + rA, cC, c2a * (p5 - al));
         }
+        (void)km;
 
-        // shift windows
+        // shift windows (refills are clamped loads and selects)
         phpp_0 = phpp_p;
+        const float th_n = th_at(k + 3 < nz ? k + 3 : nz - 1, rA, cC);
         thz[0] = thz[1]; thz[1] = thz[2]; thz[2] = thz[3]; thz[3] = thz[4];
-        thz[4] = (k + 3 < nz) ? th_at(k + 3, c) : 0.0f;
+        thz[4] = (k + 3 < nz) ? th_n : 0.0f;
+        const float w_n = gld(d.w, (k + 3 <= nz ? k + 3 : nz) * st + rA, cC);
         wz[0] = wz[1]; wz[1] = wz[2]; wz[2] = wz[3]; wz[3] = wz[4];
-        wz[4] = (k + 3 <= nz) ? w[(k + 3) * st + c] : 0.0f;
+        wz[4] = (k + 3 <= nz) ? w_n : 0.0f;
+        const float ww_n = gld(d.ww, (k + 2 <= nz ? k + 2 : nz) * st + rA, cC);
         ww_m = ww_0; ww_0 = ww_p;
-        ww_p = (k + 2 <= nz) ? ww[(k + 2) * st + c] : 0.0f;
+        ww_p = (k + 2 <= nz) ? ww_n : 0.0f;
+        const float ph_n = ph_at(k + 2 <= nz ? k + 2 : nz, rA, cC);
         ph_m = ph_0; ph_0 = ph_p;
-        ph_p = (k + 2 <= nz) ? ph_at(k + 2, c) : 0.0f;
+        ph_p = (k + 2 <= nz) ? ph_n : 0.0f;
         pp_m = pp_0;
         ru_i_m = ru_i; ru_ip_m = ru_ip; rv_j_m = rv_j; rv_jp_m = rv_jp;
         u_i_mm = u_i_m; u_i_m = u_i; u_ip_mm = u_ip_m; u_ip_m = u_ip;
         v_j_mm = v_j_m; v_j_m = v_j; v_jp_mm = v_jp_m; v_jp_m = v_jp;
+        const float thp_n = gld(d.thp, (k + 2 < nz ? k + 2 : nz - 1) * st + rA, cC);
         thp_m = thp_0; thp_0 = thp_p;
-        thp_p = (k + 2 < nz) ? thp[(k + 2) * st + c] : 0.0f;
-    }
+        thp_p = (k + 2 < nz) ? thp_n : 0.0f;
+    };
+    level(0, Kind<0>{});
+    for (int k = 1; k < nz; ++k) level(k, Kind<1>{});
+    level(nz, Kind<2>{});
 }
 
 // ---------------------------------------------------------------------------
@@ -760,8 +828,10 @@ struct UvIn {
 // damped pressure p'' + smdiv (p'' - p''_old) rides in 4-level windows for
 // the surface and lid extrapolations of dpn.
 template <bool B3, bool USEP, bool DU, bool DV>
-__device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int cA,
-                                         int cW, int cS, int fu, int fv)
+__device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
+                                         int rS, int rU, unsigned bA,
+                                         unsigned bW, unsigned bS, unsigned bu,
+                                         unsigned bv)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
     const int stu = ny * (nx + 1), stv = (ny + 1) * nx;
@@ -771,8 +841,6 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int cA,
     kfloat* fnm = kview(d.fnm);
     kfloat* fnp = kview(d.fnp);
     kfloat* rdnw = kview(d.rdnw);
-    unsigned bA = 4u * cA, bW = 4u * cW, bS = 4u * cS;
-    unsigned bu = 4u * fu, bv = 4u * fv;
     auto opaque_all = [&]() {
         colb_opaque(bA); colb_opaque(bW); colb_opaque(bS);
         colb_opaque(bu); colb_opaque(bv);
@@ -781,82 +849,84 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int cA,
         float b = USEP ? o : a;
         return a + smdiv * (a - b);
     };
-    const float mA = d.mub2d[cA] + d.mup[cA];
-    const float muf_u = DU ? 0.5f * ((d.mub2d[cW] + d.mup[cW]) + mA) : 0.0f;
-    const float muf_v = DV ? 0.5f * ((d.mub2d[cS] + d.mup[cS]) + mA) : 0.0f;
-    const float mppA = d.mu_pp[cA];
-    const float dmu_u = DU ? d.mu_pp[cW] + mppA : 0.0f;
-    const float dmu_v = DV ? d.mu_pp[cS] + mppA : 0.0f;
+    const float mA = gld(d.mub2d, rA, bA) + gld(d.mup, rA, bA);
+    const float muf_u = DU ? 0.5f * ((gld(d.mub2d, rA, bW) + gld(d.mup, rA, bW)) + mA)
+                           : 0.0f;
+    const float muf_v = DV ? 0.5f * ((gld(d.mub2d, rS, bS) + gld(d.mup, rS, bS)) + mA)
+                           : 0.0f;
+    const float mppA = gld(d.mu_pp, rA, bA);
+    const float dmu_u = DU ? gld(d.mu_pp, rA, bW) + mppA : 0.0f;
+    const float dmu_v = DV ? gld(d.mu_pp, rS, bS) + mppA : 0.0f;
 
     // windows S(k-2), S(k-1), S(k), S(k+1) of the damped pressure
     opaque_all();
     float pa[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pw[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     float ps[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    pa[2] = pd(gld(d.p_pp, 0, bA), gld(d.p_pp_old, 0, bA));
-    pa[3] = pd(gld(d.p_pp, st, bA), gld(d.p_pp_old, st, bA));
-    const float pa2 = pd(gld(d.p_pp, 2 * st, bA), gld(d.p_pp_old, 2 * st, bA));
+    pa[2] = pd(gld(d.p_pp, 0 + rA, bA), gld(d.p_pp_old, 0 + rA, bA));
+    pa[3] = pd(gld(d.p_pp, st + rA, bA), gld(d.p_pp_old, st + rA, bA));
+    const float pa2 = pd(gld(d.p_pp, 2 * st + rA, bA), gld(d.p_pp_old, 2 * st + rA, bA));
     float dnu_0 = 0.0f, dnv_0 = 0.0f;
     if (DU) {
-        pw[2] = pd(gld(d.p_pp, 0, bW), gld(d.p_pp_old, 0, bW));
-        pw[3] = pd(gld(d.p_pp, st, bW), gld(d.p_pp_old, st, bW));
-        const float pw2 = pd(gld(d.p_pp, 2 * st, bW), gld(d.p_pp_old, 2 * st, bW));
+        pw[2] = pd(gld(d.p_pp, 0 + rA, bW), gld(d.p_pp_old, 0 + rA, bW));
+        pw[3] = pd(gld(d.p_pp, st + rA, bW), gld(d.p_pp_old, st + rA, bW));
+        const float pw2 = pd(gld(d.p_pp, 2 * st + rA, bW), gld(d.p_pp_old, 2 * st + rA, bW));
         float S0 = pa[2] + pw[2], S1 = pa[3] + pw[3], S2 = pa2 + pw2;
         dnu_0 = 0.5f * ((d.cf1 * S0 + d.cf2 * S1) + d.cf3 * S2);
     }
     if (DV) {
-        ps[2] = pd(gld(d.p_pp, 0, bS), gld(d.p_pp_old, 0, bS));
-        ps[3] = pd(gld(d.p_pp, st, bS), gld(d.p_pp_old, st, bS));
-        const float ps2 = pd(gld(d.p_pp, 2 * st, bS), gld(d.p_pp_old, 2 * st, bS));
+        ps[2] = pd(gld(d.p_pp, 0 + rS, bS), gld(d.p_pp_old, 0 + rS, bS));
+        ps[3] = pd(gld(d.p_pp, st + rS, bS), gld(d.p_pp_old, st + rS, bS));
+        const float ps2 = pd(gld(d.p_pp, 2 * st + rS, bS), gld(d.p_pp_old, 2 * st + rS, bS));
         float S0 = pa[2] + ps[2], S1 = pa[3] + ps[3], S2 = pa2 + ps2;
         dnv_0 = 0.5f * ((d.cf1 * S0 + d.cf2 * S1) + d.cf3 * S2);
     }
-    float phA0 = gld(d.ph_pp, 0, bA), hpA0 = gld(d.php, 0, bA);
-    float phW0 = DU ? gld(d.ph_pp, 0, bW) : 0.0f, hpW0 = DU ? gld(d.php, 0, bW) : 0.0f;
-    float phS0 = DV ? gld(d.ph_pp, 0, bS) : 0.0f, hpS0 = DV ? gld(d.php, 0, bS) : 0.0f;
-    float gbA0 = B3 ? gld(d.phb, 0, bA) : 0.0f;
-    float gbW0 = (B3 && DU) ? gld(d.phb, 0, bW) : 0.0f;
-    float gbS0 = (B3 && DV) ? gld(d.phb, 0, bS) : 0.0f;
+    float phA0 = gld(d.ph_pp, 0 + rA, bA), hpA0 = gld(d.php, 0 + rA, bA);
+    float phW0 = DU ? gld(d.ph_pp, 0 + rA, bW) : 0.0f, hpW0 = DU ? gld(d.php, 0 + rA, bW) : 0.0f;
+    float phS0 = DV ? gld(d.ph_pp, 0 + rS, bS) : 0.0f, hpS0 = DV ? gld(d.php, 0 + rS, bS) : 0.0f;
+    float gbA0 = B3 ? gld(d.phb, 0 + rA, bA) : 0.0f;
+    float gbW0 = (B3 && DU) ? gld(d.phb, 0 + rA, bW) : 0.0f;
+    float gbS0 = (B3 && DV) ? gld(d.phb, 0 + rS, bS) : 0.0f;
 
     auto load = [&](UvIn& L, int k) {
         const unsigned lp = (k + 2 < nz ? k + 2 : nz - 1) * st;
         const unsigned lw = (k + 1) * st, lm = k * st;
-        L.pA = gld(d.p_pp, lp, bA);
-        if (USEP) L.oA = gld(d.p_pp_old, lp, bA);
-        L.phA = gld(d.ph_pp, lw, bA);
-        L.hpA = gld(d.php, lw, bA);
-        L.altA = gld(d.alt, lm, bA);
+        L.pA = gld(d.p_pp, lp + rA, bA);
+        if (USEP) L.oA = gld(d.p_pp_old, lp + rA, bA);
+        L.phA = gld(d.ph_pp, lw + rA, bA);
+        L.hpA = gld(d.php, lw + rA, bA);
+        L.altA = gld(d.alt, lm + rA, bA);
         if (B3) {
-            L.gbA = gld(d.phb, lw, bA);
-            L.aA = gld(d.al_pp, lm, bA);
-            L.bA = gld(d.pb, lm, bA);
+            L.gbA = gld(d.phb, lw + rA, bA);
+            L.aA = gld(d.al_pp, lm + rA, bA);
+            L.bA = gld(d.pb, lm + rA, bA);
         }
         if (DU) {
-            L.pW = gld(d.p_pp, lp, bW);
-            if (USEP) L.oW = gld(d.p_pp_old, lp, bW);
-            L.phW = gld(d.ph_pp, lw, bW);
-            L.hpW = gld(d.php, lw, bW);
-            L.altW = gld(d.alt, lm, bW);
+            L.pW = gld(d.p_pp, lp + rA, bW);
+            if (USEP) L.oW = gld(d.p_pp_old, lp + rA, bW);
+            L.phW = gld(d.ph_pp, lw + rA, bW);
+            L.hpW = gld(d.php, lw + rA, bW);
+            L.altW = gld(d.alt, lm + rA, bW);
             if (B3) {
-                L.gbW = gld(d.phb, lw, bW);
-                L.aW = gld(d.al_pp, lm, bW);
-                L.bW = gld(d.pb, lm, bW);
+                L.gbW = gld(d.phb, lw + rA, bW);
+                L.aW = gld(d.al_pp, lm + rA, bW);
+                L.bW = gld(d.pb, lm + rA, bW);
             }
-            L.u = gld(d.u_pp, k * stu, bu);
-            L.ru = gld(d.ru_t, k * stu, bu);
+            L.u = gld(d.u_pp, k * stu + rU, bu);
+            L.ru = gld(d.ru_t, k * stu + rU, bu);
         }
         if (DV) {
-            L.pS = gld(d.p_pp, lp, bS);
-            if (USEP) L.oS = gld(d.p_pp_old, lp, bS);
-            L.phS = gld(d.ph_pp, lw, bS);
-            L.hpS = gld(d.php, lw, bS);
-            L.altS = gld(d.alt, lm, bS);
+            L.pS = gld(d.p_pp, lp + rS, bS);
+            if (USEP) L.oS = gld(d.p_pp_old, lp + rS, bS);
+            L.phS = gld(d.ph_pp, lw + rS, bS);
+            L.hpS = gld(d.php, lw + rS, bS);
+            L.altS = gld(d.alt, lm + rS, bS);
             if (B3) {
-                L.gbS = gld(d.phb, lw, bS);
-                L.aS = gld(d.al_pp, lm, bS);
-                L.bS = gld(d.pb, lm, bS);
+                L.gbS = gld(d.phb, lw + rS, bS);
+                L.aS = gld(d.al_pp, lm + rS, bS);
+                L.bS = gld(d.pb, lm + rS, bS);
             }
-            L.v = gld(d.v_pp, k * stv, bv);
-            L.rv = gld(d.rv_t, k * stv, bv);
+            L.v = gld(d.v_pp, k * stv + rA, bv);
+            L.rv = gld(d.rv_t, k * stv + rA, bv);
         }
     };
 
@@ -893,7 +963,7 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int cA,
             if (B3) dphp = dphp + 0.5f * ((gbA0 + gbA1) - (gbW0 + L.gbW));
             dpxy = dpxy + (s.rdx * dphp) * (rdnw[k] * (dn_up - dnu_0)
                                             - (0.5f * c1) * dmu_u);
-            gst(d.u_pp, k * stu, bu, L.u + dtau * (L.ru - dpxy));
+            gst(d.u_pp, k * stu + rU, bu, L.u + dtau * (L.ru - dpxy));
             dnu_0 = dn_up;
             phW0 = L.phW; hpW0 = L.hpW; gbW0 = L.gbW;
         }
@@ -916,7 +986,7 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int cA,
             if (B3) dphp = dphp + 0.5f * ((gbA0 + gbA1) - (gbS0 + L.gbS));
             dpxy = dpxy + (s.rdy * dphp) * (rdnw[k] * (dn_up - dnv_0)
                                             - (0.5f * c1) * dmu_v);
-            gst(d.v_pp, k * stv, bv, L.v + dtau * (L.rv - dpxy));
+            gst(d.v_pp, k * stv + rA, bv, L.v + dtau * (L.rv - dpxy));
             dnv_0 = dn_up;
             phS0 = L.phS; hpS0 = L.hpS; gbS0 = L.gbS;
         }
@@ -938,37 +1008,37 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int cA,
     level(nz - 1, std::true_type{});
 }
 
-// Main launch: one thread per mass column (i, j) advances u'' on face
-// (j, i) and v'' on face (j, i).  No role branches.
+// One launch: the first ne workgroups advance the periodic duplicate faces
+// (u'' at i = nx, one per row; v'' at j = ny, one per column; each with its
+// own u''/ru_t and the PGF of face 0), dispatched first so they overlap the
+// main work; the rest are row-segment waves in which the thread of mass
+// column (j, i) advances u'' on face (j, i) and v'' on face (j, i).
 template <bool B3, bool USEP>
-__global__ __launch_bounds__(256) void k_uv(Dom d, Sc s)
+__global__ __launch_bounds__(256) void k_uv(Dom d, Sc s, int ne)
 {
-    const int ny = d.ny, nx = d.nx;
-    const int c = xcd_block() * blockDim.x + threadIdx.x;
-    if (c >= ny * nx) return;
-    const int j = c / nx, i = c - j * nx;
-    const int im = (i == 0) ? nx - 1 : i - 1;
-    const int jm = (j == 0) ? ny - 1 : j - 1;
-    uv_march<B3, USEP, true, true>(d, s, c, j * nx + im, jm * nx + i,
-                                   j * (nx + 1) + i, c);
-}
-
-// Periodic duplicate faces: u'' at i = nx (one per row) and v'' at j = ny
-// (one per column), each with its own u''/ru_t and the PGF of face 0.
-template <bool B3, bool USEP>
-__global__ __launch_bounds__(256) void k_uv_edge(Dom d, Sc s)
-{
-    const int ny = d.ny, nx = d.nx;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < ny) {
-        const int j = t;
-        uv_march<B3, USEP, true, false>(d, s, j * nx, j * nx + nx - 1, 0,
-                                        j * (nx + 1) + nx, 0);
-    } else if (t < ny + nx) {
-        const int i = t - ny;
-        uv_march<B3, USEP, false, true>(d, s, i, 0, (ny - 1) * nx + i, 0,
-                                        ny * nx + i);
+    const int ny = d.ny, nx = d.nx, pu = nx + 1;
+    if ((int)blockIdx.x < ne) {
+        const int t = blockIdx.x * 256 + threadIdx.y * 64 + threadIdx.x;
+        if (t < ny) {
+            const int j = t;
+            uv_march<B3, USEP, true, false>(
+                d, s, 0, 0, 0, 4u * (j * nx), 4u * (j * nx + nx - 1), 0u,
+                4u * (j * pu + nx), 0u);
+        } else if (t < ny + nx) {
+            const int i = t - ny;
+            uv_march<B3, USEP, false, true>(
+                d, s, 0, 0, 0, 4u * i, 0u, 4u * ((ny - 1) * nx + i), 0u,
+                4u * (ny * nx + i));
+        }
+        return;
     }
+    const RowSeg rs = row_seg_from(nx, ny, ne);
+    if (!rs.live) return;
+    const int i = rs.i, j = rs.j;
+    const int jm = (j == 0) ? ny - 1 : j - 1;
+    const unsigned bc = 4u * i, bw = 4u * ((i == 0) ? nx - 1 : i - 1);
+    uv_march<B3, USEP, true, true>(d, s, j * nx, jm * nx, j * pu, bc, bw, bc,
+                                   bc, bc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1007,9 +1077,6 @@ struct ColDn {
     float thpp = 0.0f;
 };
 
-template <int V>
-using Kind = std::integral_constant<int, V>;
-enum { LV_FIRST = 0, LV_SECOND = 1, LV_THIRD = 2, LV_MID = 3, LV_LAST = 4 };
 
 // B3: terrain (3-D) base state; DAMP: damp_opt = 3 w damper; AL: store
 // alpha''; WW: store Omega'' (standalone substep API).  nz >= 3.
@@ -1570,16 +1637,24 @@ hipError_t run_tend_faces(const Dom& d, const Sc& s, hipStream_t st)
 
 hipError_t run_tend_cols(const Dom& d, const Sc& s, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_tend_cols, masses(d), dim3(64, 4), 0, st, d, s);
+    const dim3 g = rowseg_grid(d.nx, d.ny), b(64, 4);
+    if (d.base3d && s.diff)
+        hipLaunchKernelGGL((k_tend_cols<true, true>), g, b, 0, st, d, s);
+    else if (d.base3d)
+        hipLaunchKernelGGL((k_tend_cols<true, false>), g, b, 0, st, d, s);
+    else if (s.diff)
+        hipLaunchKernelGGL((k_tend_cols<false, true>), g, b, 0, st, d, s);
+    else
+        hipLaunchKernelGGL((k_tend_cols<false, false>), g, b, 0, st, d, s);
     return hipGetLastError();
 }
 
 template <bool B3, bool USEP>
 static void launch_uv(const Dom& d, const Sc& s, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_uv<B3, USEP>), cols(d), dim3(256), 0, st, d, s);
-    hipLaunchKernelGGL((k_uv_edge<B3, USEP>), dim3((d.ny + d.nx + 255) / 256),
-                       dim3(256), 0, st, d, s);
+    const int ne = (int)xcd_grid((unsigned)((d.ny + d.nx + 255) / 256));
+    const dim3 g(rowseg_grid(d.nx, d.ny).x + ne);
+    hipLaunchKernelGGL((k_uv<B3, USEP>), g, dim3(64, 4), 0, st, d, s, ne);
 }
 
 hipError_t run_uv(const Dom& d, const Sc& s, hipStream_t st)
