@@ -15,7 +15,10 @@ Builders:
   with seeded smooth random u/v/theta' (250x200x49, dx=12 km);
 * :func:`warm_bubbles` -- stratified dry state with seeded +3 K cosine^2
   warm bubbles (1000x1000x50, dx=1 km);
-* :func:`moist_tracer` -- stratified state carrying a seeded qv tracer.
+* :func:`init_moist_balanced` -- discretely balanced moist at-rest state
+  (reference moist.py:700-785, WRF quarter_ss construction);
+* :func:`moist_tracer` -- 3-D stratified flow carrying a seeded qv field,
+  0-16 g/kg decaying with height (192x160x50, dx=3 km).
 """
 from __future__ import annotations
 
@@ -220,6 +223,82 @@ def warm_bubbles(nx=1000, ny=1000, nz=50, dx=1000.0, dt=6.0,
         return out
 
     case = init_theta_perturbation(cfg, coord, base, thp)
+    case.fields["u"][...] = u0
+    case.fields["v"][...] = v0
+    return case
+
+
+def init_moist_balanced(cfg: RunConfig, coord: VerticalCoord,
+                        base: BaseState, qv3: np.ndarray,
+                        thp: np.ndarray | None = None) -> HostCase:
+    """Discretely balanced moist at-rest state on a flat dry base
+    (reference moist.py:700-785, WRF module_initialize_ideal.F quarter_ss):
+    with mu' = 0, integrate p' down the column by inverting the moist
+    pg_buoy_w balance row by row, invert the moist EOS for the dry specific
+    volume, then integrate phi' up with the discrete hydrostatic
+    recurrence.  ``qv3`` is (nz, ny, nx); ``thp`` (optional) rides on top
+    without entering the pressure construction, as in WRF.  FP64."""
+    if not cfg.moist:
+        raise ValueError("init_moist_balanced requires cfg.moist=True")
+    if base.terrain_z is not None:
+        raise NotImplementedError("moist balance is built on flat bases only")
+    nz, ny, nx = cfg.nz, cfg.ny, cfg.nx
+    case = init_at_rest(cfg, coord, base)
+    qv3 = np.broadcast_to(np.asarray(qv3, np.float64), (nz, ny, nx))
+    thp = np.zeros((nz, ny, nx)) if thp is None else np.broadcast_to(
+        np.asarray(thp, np.float64), (nz, ny, nx))
+    th = base.thb[:, None, None] + thp
+    mub, mup = float(base.mub), 0.0
+    c1f, c2f, c1h, c2h = coord.c1f, coord.c2f, coord.c1h, coord.c2h
+    p = np.zeros((nz, ny, nx))
+    q1 = qv3[nz - 1]
+    q2 = 1.0 / (1.0 + q1)
+    q1 = q1 * q2
+    p[nz - 1] = (-0.5 * (c1f[nz] * mup + q1 * (c1f[nz] * mub + c2f[nz]))
+                 / coord.rdnw[nz - 1] / q2)
+    for kk in range(nz - 2, -1, -1):
+        kw = kk + 1
+        q1 = 0.5 * (qv3[kk] + qv3[kk + 1])
+        q2 = 1.0 / (1.0 + q1)
+        q1 = q1 * q2
+        p[kk] = p[kk + 1] - (c1f[kw] * mup + q1 * (c1f[kw] * mub + c2f[kw])) \
+            / q2 / coord.rdn[kw]
+    qvf = 1.0 + c.RVOVRD * qv3
+    alt = (c.RD / c.P0) * th * qvf * ((p + base.pb[:, None, None]) / c.P0) \
+        ** (-c.CV / c.CP)
+    al = alt - base.alb[:, None, None]
+    php = np.zeros((nz + 1, ny, nx))
+    for k in range(nz):
+        php[k + 1] = php[k] - coord.dnw[k] * (
+            ((c1h[k] * mub + c2h[k]) + c1h[k] * mup) * al[k]
+            + (c1h[k] * mup) * base.alb[k])
+    case.fields["thp"] = np.array(thp)
+    case.fields["php"] = php
+    for q in ("qv", "qc", "qr"):
+        case.fields[q] = np.zeros((nz, ny, nx))
+    case.fields["qv"] = np.array(qv3)
+    return case
+
+
+def moist_tracer(nx=192, ny=160, nz=50, dx=3000.0, dt=18.0,
+                 time_step_sound=4, seed=20260802, ztop=20000.0,
+                 qv_max=0.016, qv_scale_height=2500.0, n_bv=0.01,
+                 u0=10.0, v0=5.0, moist_adv_opt=1) -> HostCase:
+    """3-D stratified flow carrying a seeded water-vapour field: qv drawn
+    uniformly in [0, qv_max] per cell and decaying as exp(-z/H), balanced
+    by :func:`init_moist_balanced`, with a uniform wind (u0, v0).
+    dt = 18 s follows WRF's 6 s per km of dx."""
+    cfg = RunConfig(nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, ztop=ztop, dt=dt,
+                    time_step_sound=time_step_sound, moist=True,
+                    mp_physics=0, moist_adv_opt=moist_adv_opt,
+                    case="moist_tracer")
+    coord = make_vertical_coord(nz)
+    base = make_base_state(coord, stratified_sounding(n_bv), cfg.p_surf, ztop)
+    rng = np.random.default_rng(seed)
+    z = height_half(base)
+    qv = rng.uniform(0.0, qv_max, (nz, ny, nx)) \
+        * np.exp(-z / qv_scale_height)[:, None, None]
+    case = init_moist_balanced(cfg, coord, base, qv)
     case.fields["u"][...] = u0
     case.fields["v"][...] = v0
     return case

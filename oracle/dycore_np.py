@@ -915,6 +915,14 @@ def set_w_surface(s, dx, dy) -> None:
 # Moist scalar transport (qv/qc/qr) with time-averaged mass fluxes
 # ---------------------------------------------------------------------------
 
+def sumflux_finish(ru_m, rv_m, ww_m, ru, rv, ww, nsub):
+    """WRF sumflux at the last substep (reference acoustic.cu:1469-1489):
+    the substep mean of the perturbation fluxes plus the stage-reference
+    coupled fluxes."""
+    fn = ru_m.dtype.type(nsub)
+    return ru_m / fn + ru, rv_m / fn + rv, ww_m / fn + ww
+
+
 def advance_scalars_stage(s, ru_m, rv_m, ww_m, dt_stage, final, dx, dy,
                           moist_adv_opt=1):
     """Coupled qv/qc/qr stage update with the substep-averaged fluxes
@@ -967,11 +975,9 @@ def step(s, cfg) -> None:
                 ww_m += s.ww_pp
         small_step_finish(s)
         if moist:
-            rn = f(1.0) / f(nsub)
-            ru_m = ru_m * rn + ru
-            rv_m = rv_m * rn + rv
-            ww_m = ww_m * rn + ww
-            advance_scalars_stage(s, ru_m, rv_m, ww_m, nsub * dtau32,
+            ru_m, rv_m, ww_m = sumflux_finish(ru_m, rv_m, ww_m, ru, rv, ww,
+                                              nsub)
+            advance_scalars_stage(s, ru_m, rv_m, ww_m, nsub * dtau,
                                   istage == 2, cfg.dx, cfg.dy,
                                   getattr(cfg, "moist_adv_opt", 1))
     set_w_surface(s, cfg.dx, cfg.dy)

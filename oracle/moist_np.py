@@ -1,0 +1,171 @@
+"""Moist scalar transport for the oracle -- TEST INFRASTRUCTURE ONLY.
+
+NumPy restatement (float32 or float64) of the reference's qv/qc/qr stage
+update: the coupled update with the acoustic time-averaged mass fluxes
+(gpuwm/core/moist.py:434-698, fused update kernel :353-402) and WRF's
+positive-definite limiter on the final RK stage
+(gpuwm/core/kernels/pd_advection.cu, periodic path, msf = 1).
+
+Pinning: uniform-qv invariance (a constant mixing ratio stays constant to
+round-off, the reference's tests/test_moist.py:517-577 property), exact
+periodic conservation of sum(C(mu) q), positivity with the limiter on,
+the limiter reducing to the unlimited flux form (itself pinned by the
+reference's advection fixture) wherever no cell needs limiting, and
+planted mistakes in the limiter (tests/test_oracle_moist.py).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import dycore_np as D
+
+SPECIES = ("qv", "qc", "qr")
+
+
+def _pm(a, shift, axis):
+    """a[(i + shift) mod n] along axis (periodic neighbour)."""
+    return np.roll(a, -shift, axis=axis)
+
+
+def coupled_update(q0, tend, c1h, c2h, mu0, mu, dt_eff, clamp):
+    """WRF rk_update_scalar (reference moist.py:385-398): q = (C(mu_t) q_t
+    + dt tend) / C(mu_new); the final stage clamps at zero (NaN kept)."""
+    f = q0.dtype.type
+    c1, c2 = c1h[:, None, None], c2h[:, None, None]
+    chm0 = c1 * mu0[None] + c2
+    chm = c1 * mu[None] + c2
+    v = chm0 * q0
+    t = f(dt_eff) * tend
+    v = v + t
+    v = v / chm
+    if clamp:
+        v = np.where(np.isnan(v), v, np.maximum(v, f(0.0)))
+    return v
+
+
+def flux_upwind(qm1, q0v, cr):
+    """WRF flux_upwind (reference pd_advection.cu:101-108)."""
+    f = cr.dtype.type
+    half, one = f(0.5), f(1.0)
+    return ((half * np.fmin(one, cr + np.abs(cr))) * qm1
+            + (half * np.fmax(-one, cr - np.abs(cr))) * q0v)
+
+
+def pd_fluxes(q, q0, ru, rv, rw, mut, c1h, c2h, rdnw, fnm, fnp, dx, dy,
+              dt):
+    """Upwind fluxes of q0 and high-order corrections of q at every face
+    (reference pd_advection.cu pd_fluxes, periodic branches).  Returns
+    (fxl, fxc, fyl, fyc, fzl, fzc)."""
+    f = q.dtype.type
+    half, two = f(0.5), f(2.0)
+    nz, ny, nx = q.shape
+    dxf, dyf, dtf = f(dx), f(dy), f(dt)
+    c1, c2 = c1h[:, None, None], c2h[:, None, None]
+    # x faces 0..nx: cells i-1 and i (periodic), duplicate face nx == 0
+    mx = (c1 * half) * (mut + _pm(mut, -1, 1))[None] + c2
+    mx = np.concatenate([mx, mx[..., :1]], axis=-1)
+    cr = ((ru * dtf) / dxf) / mx
+    qm1 = np.concatenate([_pm(q0, -1, 2), _pm(q0, -1, 2)[..., :1]], axis=-1)
+    qc0 = np.concatenate([q0, q0[..., :1]], axis=-1)
+    fxl = (mx * (dxf / dtf)) * flux_upwind(qm1, qc0, cr)
+    fxc = D._xfaces(q, ru) - fxl
+    # y faces 0..ny
+    my = (c1 * half) * (mut + _pm(mut, -1, 0))[None] + c2
+    my = np.concatenate([my, my[:, :1]], axis=1)
+    cr = ((rv * dtf) / dyf) / my
+    qm1 = np.concatenate([_pm(q0, -1, 1), _pm(q0, -1, 1)[:, :1]], axis=1)
+    qc0 = np.concatenate([q0, q0[:, :1]], axis=1)
+    fyl = (my * (dyf / dtf)) * flux_upwind(qm1, qc0, cr)
+    fyc = D._yfaces(q, rv) - fyl
+    # eta faces 0..nz; boundary faces carry nothing
+    fzl = np.zeros((nz + 1, ny, nx), dtype=q.dtype)
+    fzc = np.zeros_like(fzl)
+    fh = D._zfaces_half(q, rw, fnm, fnp)
+    for k in range(1, nz):
+        dz = two / (rdnw[k] + rdnw[k - 1])
+        muf = c1h[k] * mut + c2h[k]
+        crz = (rw[k] * dtf) / (dz * muf)
+        fzl[k] = (muf * (dz / dtf)) * flux_upwind(q0[k - 1], q0[k], crz)
+        fzc[k] = fh[k] - fzl[k]
+    return fxl, fxc, fyl, fyc, fzl, fzc
+
+
+def pd_scale(q0, mu_old, c1h, c2h, rdnw, fl, dx_inv, dy_inv, dt):
+    """Per-cell renormalisation factor (reference pd_advection.cu
+    pd_scale, periodic no-msf branch): where the outgoing correction flux
+    exceeds the upwind-updated coupled scalar, scale it down to fit."""
+    fxl, fxc, fyl, fyc, fzl, fzc = fl
+    f = q0.dtype.type
+    zero, one = f(0.0), f(1.0)
+    dxi, dyi, dtf = f(dx_inv), f(dy_inv), f(dt)
+    rz = rdnw[:, None, None]
+    chm0 = c1h[:, None, None] * mu_old[None] + c2h[:, None, None]
+    dl = (dxi * (fxl[..., 1:] - fxl[..., :-1])
+          + dyi * (fyl[:, 1:] - fyl[:, :-1])) + rz * (fzl[1:] - fzl[:-1])
+    ph_low = chm0 * q0 - dtf * dl
+    # fminf/fmaxf semantics (a NaN operand yields the other one)
+    fo = dtf * ((dxi * (np.fmax(zero, fxc[..., 1:])
+                        - np.fmin(zero, fxc[..., :-1]))
+                 + dyi * (np.fmax(zero, fyc[:, 1:])
+                          - np.fmin(zero, fyc[:, :-1])))
+                + rz * (np.fmin(zero, fzc[1:]) - np.fmax(zero, fzc[:-1])))
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        r = np.fmin(one, np.fmax(zero, ph_low / (fo + f(1e-20))))
+    return np.where(fo > ph_low, r, one)
+
+
+def pd_apply(sc, fl, rdnw, dx_inv, dy_inv):
+    """Renormalised PD flux divergence (reference pd_advection.cu
+    pd_renorm_apply, periodic no-msf branch): each correction flux is
+    scaled by its donor cell's factor."""
+    fxl, fxc, fyl, fyc, fzl, fzc = fl
+    f = sc.dtype.type
+    zero, one = f(0.0), f(1.0)
+    dxi, dyi = f(dx_inv), f(dy_inv)
+    nz = sc.shape[0]
+    fxc_l, fxc_r = fxc[..., :-1], fxc[..., 1:]
+    sx_l = np.where(fxc_l > zero, _pm(sc, -1, 2),
+                    np.where(fxc_l < zero, sc, one))
+    sx_r = np.where(fxc_r > zero, sc,
+                    np.where(fxc_r < zero, _pm(sc, 1, 2), one))
+    fyc_l, fyc_r = fyc[:, :-1], fyc[:, 1:]
+    sy_l = np.where(fyc_l > zero, _pm(sc, -1, 1),
+                    np.where(fyc_l < zero, sc, one))
+    sy_r = np.where(fyc_r > zero, sc,
+                    np.where(fyc_r < zero, _pm(sc, 1, 1), one))
+    below = np.concatenate([sc[:1], sc[:-1]], axis=0)      # scale(k-1)
+    above = np.concatenate([sc[1:], sc[-1:]], axis=0)      # scale(k+1)
+    fzc_b, fzc_t = fzc[:nz], fzc[1:]
+    sz_b = np.where(fzc_b > zero, sc, np.where(fzc_b < zero, below, one))
+    sz_t = np.where(fzc_t < zero, sc, np.where(fzc_t > zero, above, one))
+    gx = (sx_r * fxc_r + fxl[..., 1:]) - (sx_l * fxc_l + fxl[..., :-1])
+    gy = (sy_r * fyc_r + fyl[:, 1:]) - (sy_l * fyc_l + fyl[:, :-1])
+    gz = (sz_t * fzc_t + fzl[1:]) - (sz_b * fzc_b + fzl[:nz])
+    return ((-dxi) * gx - dyi * gy) - rdnw[:, None, None] * gz
+
+
+def advance_scalars_stage(s, ru_m, rv_m, ww_m, dt_stage, final, dx, dy,
+                          moist_adv_opt=1):
+    """qv/qc/qr one RK stage from their time-t copies (reference moist.py:
+    434-698): unlimited flux form, or the PD limiter on the final stage
+    when moist_adv_opt = 1.  ``dt_stage`` is the host double nsub*dtau."""
+    f = s.dtype.type
+    mu0 = s.mub2d + s.mup0
+    mu = s.mub2d + s.mup
+    dx_inv, dy_inv = float(f(1.0 / dx)), float(f(1.0 / dy))
+    pd = final and moist_adv_opt == 1
+    for name in SPECIES:
+        q, q0 = getattr(s, name), getattr(s, name + "0")
+        if pd:
+            fl = pd_fluxes(q, q0, ru_m, rv_m, ww_m, mu, s.c1h, s.c2h,
+                           s.rdnw, s.fnm, s.fnp, dx, dy, dt_stage)
+            sc = pd_scale(q0, mu0, s.c1h, s.c2h, s.rdnw, fl, dx_inv,
+                          dy_inv, dt_stage)
+            tend = f(0.0) + pd_apply(sc, fl, s.rdnw, dx_inv, dy_inv)
+            q[...] = coupled_update(q0, tend, s.c1h, s.c2h, mu0, mu,
+                                    dt_stage, clamp=True)
+        else:
+            tend = f(0.0) + D.flux_div_scalar(q, ru_m, rv_m, ww_m, s.rdnw,
+                                              s.fnm, s.fnp, dx_inv, dy_inv)
+            q[...] = coupled_update(q0, tend, s.c1h, s.c2h, mu0, mu,
+                                    dt_stage, clamp=final)
