@@ -12,7 +12,7 @@ from pathlib import Path
 LIB_NAME = "libarwen_dycore.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -31,6 +31,11 @@ DOMAIN_ARRAYS = (
 )
 SCRATCH_ARRAYS = ("ww", "gam", "dmdt_rf", "mup_next")
 DIFF_ARRAYS = ("rdzf_h", "rdzc_h", "rdzf_w", "rdzc_w")
+#: moist scalars: state arrays, then the third stage buffer (scratch)
+MOIST_ARRAYS = ("qv", "qc", "qr", "qv0", "qc0", "qr0", "qsv", "qsc", "qsr")
+#: moist scratch: WRF sumflux averages and the PD limiter's buffers
+MOIST_SCRATCH = ("ru_m", "rv_m", "ww_m", "pd_fxl", "pd_fxc", "pd_fyl",
+                 "pd_fyc", "pd_fzl", "pd_fzc", "pd_sc")
 
 
 class ArwDomain(ctypes.Structure):
@@ -38,7 +43,9 @@ class ArwDomain(ctypes.Structure):
                 + [(n, _P) for n in DOMAIN_ARRAYS]
                 + [(n, _F) for n in ("cf1", "cf2", "cf3", "cfn", "cfn1")]
                 + [(n, _P) for n in SCRATCH_ARRAYS]
-                + [(n, _P) for n in DIFF_ARRAYS])
+                + [(n, _P) for n in DIFF_ARRAYS]
+                + [(n, _P) for n in MOIST_ARRAYS]
+                + [(n, _P) for n in MOIST_SCRATCH])
 
 
 class ArwParams(ctypes.Structure):
@@ -47,7 +54,7 @@ class ArwParams(ctypes.Structure):
                 ("epssm", _D), ("smdiv", _D),
                 ("khdif", _D), ("kvdif", _D),
                 ("damp_opt", _I), ("zdamp", _D), ("dampcoef", _D),
-                ("top_lid", _I)]
+                ("top_lid", _I), ("moist_adv_opt", _I)]
 
 
 class ArwenError(RuntimeError):

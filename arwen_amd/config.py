@@ -119,3 +119,8 @@ def validate_for_step(cfg: RunConfig, *, acoustic: bool = True) -> None:
             "physics schemes are outside the dycore step port")
     if cfg.khdif < 0.0 or cfg.kvdif < 0.0:
         raise ValueError("khdif/kvdif must be non-negative")
+    if cfg.moist_adv_opt not in (0, 1):
+        raise NotImplementedError(
+            f"moist_adv_opt={cfg.moist_adv_opt}: only 0 (unlimited) and 1 "
+            "(positive definite) are ported")
+

@@ -275,6 +275,7 @@ __device__ __forceinline__ void eos_level(const Dom& d, int k, int c, int st,
                                           float mu)
 {
     float th = basev(d.thb, k, c, st, d.base3d) + d.thp[k * st + c];
+    if (d.qv) th = th * (1.0f + RVOVRD_F * d.qv[k * st + c]);
     float ph_hi = basev(d.phb, k + 1, c, st, d.base3d)
                   + d.php[(k + 1) * st + c];
     float ph_lo = basev(d.phb, k, c, st, d.base3d) + d.php[k * st + c];
@@ -325,6 +326,246 @@ __global__ void k_faces(Dom d, float* ru, float* rv)
     if (i > d.nx || j > d.ny || k >= d.nz) return;
     if (j < d.ny) ru[iU(k, j, i, d.ny, d.nx)] = ru_at(d, k, j, i);
     if (i < d.nx) rv[iV(k, j, i, d.ny, d.nx)] = rv_at(d, k, j, i);
+}
+
+// ---------------------------------------------------------------------------
+// Moist scalar transport (qv/qc/qr), periodic, msf = 1
+// ---------------------------------------------------------------------------
+
+// WRF sumflux (reference acoustic.cu:1437-1467): after every substep the
+// perturbation fluxes u'', v'', Omega'' are summed; the first substep
+// starts from zero.  One thread per (k, j, i) of the union of staggers.
+__global__ void k_sumflux(Dom d, int first)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    if (j < ny && k < nz) {
+        const int x = iU(k, j, i, ny, nx);
+        d.ru_m[x] = (first ? 0.0f : d.ru_m[x]) + d.u_pp[x];
+    }
+    if (i < nx && k < nz) {
+        const int x = iV(k, j, i, ny, nx);
+        d.rv_m[x] = (first ? 0.0f : d.rv_m[x]) + d.v_pp[x];
+    }
+    if (i < nx && j < ny) {
+        const int x = iM(k, j, i, ny, nx);
+        d.ww_m[x] = (first ? 0.0f : d.ww_m[x]) + d.ww_pp[x];
+    }
+}
+
+// WRF sumflux at the last substep (reference acoustic.cu:1469-1489): the
+// substep mean plus the stage-reference coupled fluxes ru, rv (from the
+// stage state d.u/d.v/d.mup) and Omega (d.ww).
+__global__ void k_sumflux_finish(Dom d, float nsub)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    if (j < ny && k < nz) {
+        const int x = iU(k, j, i, ny, nx);
+        d.ru_m[x] = d.ru_m[x] / nsub + ru_at(d, k, j, i);
+    }
+    if (i < nx && k < nz) {
+        const int x = iV(k, j, i, ny, nx);
+        d.rv_m[x] = d.rv_m[x] / nsub + rv_at(d, k, j, i);
+    }
+    if (i < nx && j < ny) {
+        const int x = iM(k, j, i, ny, nx);
+        d.ww_m[x] = d.ww_m[x] / nsub + d.ww[x];
+    }
+}
+
+// Post-acoustic stage mu' of column c: the value small_step_finish
+// stores (mup + mu''), formed here so the scalar update can run before
+// the finish overwrites the time-t mass on the last stage.
+__device__ __forceinline__ float mup_new_at(const Dom& d, int c)
+{
+    return d.mup[c] + d.mu_pp[c];
+}
+
+// WRF rk_update_scalar (reference moist.py:385-398): (C(mu_t) q_t + dt
+// tend) / C(mu_new), clamped at zero on the final stage (NaN kept, -0
+// becomes +0).
+__device__ __forceinline__ float scalar_update(const Dom& d, int k, int c,
+                                               float q0, float tend,
+                                               const float* mup0, float dt,
+                                               int clamp)
+{
+    const float mu0 = d.mub2d[c] + mup0[c];
+    const float mu = d.mub2d[c] + mup_new_at(d, c);
+    const float chm0 = d.c1h[k] * mu0 + d.c2h[k];
+    const float chm = d.c1h[k] * mu + d.c2h[k];
+    float v = chm0 * q0;
+    const float t = dt * tend;
+    v = v + t;
+    v = v / chm;
+    if (clamp && !(v > 0.0f) && v == v) v = 0.0f;
+    return v;
+}
+
+// Unlimited stage update of one scalar: tend = div of qcur with the
+// averaged fluxes ru_m, rv_m, ww_m; qdst = update from q0.
+__global__ void k_scalar_update(Dom d, const float* __restrict__ qcur,
+                                const float* __restrict__ q0,
+                                float* __restrict__ qdst,
+                                const float* __restrict__ mup0, float dt,
+                                int clamp, float rdx, float rdy)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    Arr3 Q{qcur, ny, nx}, RU{d.ru_m, ny, nx + 1}, RV{d.rv_m, ny + 1, nx},
+        RW{d.ww_m, ny, nx};
+    const float tend = 0.0f + adv_scalar(Q, RU, RV, RW, k, j, i, nz, ny, nx,
+                                         d.rdnw, d.fnm, d.fnp, rdx, rdy);
+    const int x = iM(k, j, i, ny, nx);
+    qdst[x] = scalar_update(d, k, j * nx + i, q0[x], tend, mup0, dt, clamp);
+}
+
+// WRF flux_upwind (reference pd_advection.cu:101-108).
+__device__ __forceinline__ float flux_upwind(float qm1, float q0v, float cr)
+{
+    return (0.5f * fminf(1.0f, cr + fabsf(cr))) * qm1
+           + (0.5f * fmaxf(-1.0f, cr - fabsf(cr))) * q0v;
+}
+
+// PD face fluxes (reference pd_advection.cu pd_fluxes, periodic, no
+// msf): the CFL-clamped upwind flux of q0 and the correction to the
+// unlimited flux of qcur, at x faces (k, j, i<=nx), y faces (k, j<=ny, i)
+// and eta faces (k<=nz, j, i).  mut is the post-acoustic stage mass.
+__global__ void k_pd_fluxes(Dom d, const float* __restrict__ qcur,
+                            const float* __restrict__ q0, float dx, float dy,
+                            float dt)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    auto mut = [&](int jj, int ii) {
+        return d.mub2d[jj * nx + ii] + mup_new_at(d, jj * nx + ii);
+    };
+    auto q0a = [&](int kk, int jj, int ii) { return q0[iM(kk, jj, ii, ny, nx)]; };
+    auto qc = [&](int kk, int jj, int ii) { return qcur[iM(kk, jj, ii, ny, nx)]; };
+    if (j < ny && k < nz) {
+        const int a = (i >= nx) ? i - nx : i, b = (i == 0) ? nx - 1 : i - 1;
+        const float vel = d.ru_m[iU(k, j, i, ny, nx)];
+        const float muf = (d.c1h[k] * 0.5f) * (mut(j, a) + mut(j, b)) + d.c2h[k];
+        const float cr = ((vel * dt) / dx) / muf;
+        const float fl = (muf * (dx / dt)) * flux_upwind(q0a(k, j, b), q0a(k, j, a), cr);
+        float x[6];
+#pragma unroll
+        for (int o = 0; o < 6; ++o) x[o] = qc(k, j, pmod(i + o - 3, nx));
+        const float fh = flux5(x[0], x[1], x[2], x[3], x[4], x[5], vel);
+        d.pd_fxl[iU(k, j, i, ny, nx)] = fl;
+        d.pd_fxc[iU(k, j, i, ny, nx)] = fh - fl;
+    }
+    if (i < nx && k < nz) {
+        const int a = (j >= ny) ? j - ny : j, b = (j == 0) ? ny - 1 : j - 1;
+        const float vel = d.rv_m[iV(k, j, i, ny, nx)];
+        const float muf = (d.c1h[k] * 0.5f) * (mut(a, i) + mut(b, i)) + d.c2h[k];
+        const float cr = ((vel * dt) / dy) / muf;
+        const float fl = (muf * (dy / dt)) * flux_upwind(q0a(k, b, i), q0a(k, a, i), cr);
+        float y[6];
+#pragma unroll
+        for (int o = 0; o < 6; ++o) y[o] = qc(k, pmod(j + o - 3, ny), i);
+        const float fh = flux5(y[0], y[1], y[2], y[3], y[4], y[5], vel);
+        d.pd_fyl[iV(k, j, i, ny, nx)] = fl;
+        d.pd_fyc[iV(k, j, i, ny, nx)] = fh - fl;
+    }
+    if (i < nx && j < ny) {
+        float fl = 0.0f, fc = 0.0f;
+        if (k > 0 && k < nz) {
+            const float vel = d.ww_m[iM(k, j, i, ny, nx)];
+            const float dz = 2.0f / (d.rdnw[k] + d.rdnw[k - 1]);
+            const float muf = d.c1h[k] * mut(j, i) + d.c2h[k];
+            const float cr = (vel * dt) / (dz * muf);
+            fl = (muf * (dz / dt)) * flux_upwind(q0a(k - 1, j, i), q0a(k, j, i), cr);
+            auto qz = [&](int kk) { return qc(kk, j, i); };
+            fc = zface_half(qz, vel, k, nz, d.fnm, d.fnp) - fl;
+        }
+        d.pd_fzl[iM(k, j, i, ny, nx)] = fl;
+        d.pd_fzc[iM(k, j, i, ny, nx)] = fc;
+    }
+}
+
+// PD renormalisation factor of each cell (reference pd_advection.cu
+// pd_scale, periodic no-msf branch).
+__global__ void k_pd_scale(Dom d, const float* __restrict__ q0,
+                           const float* __restrict__ mup0, float dxi,
+                           float dyi, float dt)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    const int c = j * nx + i;
+    const float chm0 = d.c1h[k] * (d.mub2d[c] + mup0[c]) + d.c2h[k];
+    const int u0 = iU(k, j, i, ny, nx), v0 = iV(k, j, i, ny, nx);
+    const int z0 = iM(k, j, i, ny, nx), st = ny * nx;
+    const float dl = (dxi * (d.pd_fxl[u0 + 1] - d.pd_fxl[u0])
+                      + dyi * (d.pd_fyl[v0 + nx] - d.pd_fyl[v0]))
+                     + d.rdnw[k] * (d.pd_fzl[z0 + st] - d.pd_fzl[z0]);
+    const float ph_low = chm0 * q0[z0] - dt * dl;
+    const float fo = dt * ((dxi * (fmaxf(0.0f, d.pd_fxc[u0 + 1])
+                                   - fminf(0.0f, d.pd_fxc[u0]))
+                            + dyi * (fmaxf(0.0f, d.pd_fyc[v0 + nx])
+                                     - fminf(0.0f, d.pd_fyc[v0])))
+                           + d.rdnw[k] * (fminf(0.0f, d.pd_fzc[z0 + st])
+                                          - fmaxf(0.0f, d.pd_fzc[z0])));
+    float r = 1.0f;
+    if (fo > ph_low) r = fminf(1.0f, fmaxf(0.0f, ph_low / (fo + 1e-20f)));
+    d.pd_sc[z0] = r;
+}
+
+// Renormalised PD divergence and the coupled update (reference
+// pd_advection.cu pd_renorm_apply + moist.py:385-398, final stage, clamp):
+// each correction flux takes its donor cell's scale.
+__global__ void k_pd_apply(Dom d, const float* __restrict__ q0,
+                           float* __restrict__ qdst,
+                           const float* __restrict__ mup0, float dxi,
+                           float dyi, float dt)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    const int st = ny * nx;
+    const int u0 = iU(k, j, i, ny, nx), v0 = iV(k, j, i, ny, nx);
+    const int z0 = iM(k, j, i, ny, nx);
+    const int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
+    const int jp = (j + 1 == ny) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
+    const float* sc = d.pd_sc;
+    const float s0 = sc[z0];
+    const float fxc_l = d.pd_fxc[u0], fxc_r = d.pd_fxc[u0 + 1];
+    const float sx_l = (fxc_l > 0.0f) ? sc[iM(k, j, im, ny, nx)]
+                                      : ((fxc_l < 0.0f) ? s0 : 1.0f);
+    const float sx_r = (fxc_r > 0.0f) ? s0
+                       : ((fxc_r < 0.0f) ? sc[iM(k, j, ip, ny, nx)] : 1.0f);
+    const float fyc_l = d.pd_fyc[v0], fyc_r = d.pd_fyc[v0 + nx];
+    const float sy_l = (fyc_l > 0.0f) ? sc[iM(k, jm, i, ny, nx)]
+                                      : ((fyc_l < 0.0f) ? s0 : 1.0f);
+    const float sy_r = (fyc_r > 0.0f) ? s0
+                       : ((fyc_r < 0.0f) ? sc[iM(k, jp, i, ny, nx)] : 1.0f);
+    const float fzc_b = d.pd_fzc[z0], fzc_t = d.pd_fzc[z0 + st];
+    const float sz_b = (fzc_b > 0.0f) ? s0
+                       : ((fzc_b < 0.0f) ? sc[z0 - st] : 1.0f);
+    const float sz_t = (fzc_t < 0.0f) ? s0
+                       : ((fzc_t > 0.0f) ? sc[z0 + st] : 1.0f);
+    const float gx = (sx_r * fxc_r + d.pd_fxl[u0 + 1]) - (sx_l * fxc_l + d.pd_fxl[u0]);
+    const float gy = (sy_r * fyc_r + d.pd_fyl[v0 + nx]) - (sy_l * fyc_l + d.pd_fyl[v0]);
+    const float gz = (sz_t * fzc_t + d.pd_fzl[z0 + st]) - (sz_b * fzc_b + d.pd_fzl[z0]);
+    const float tend = 0.0f + (((-dxi) * gx - dyi * gy) - d.rdnw[k] * gz);
+    qdst[z0] = scalar_update(d, k, j * nx + i, q0[z0], tend, mup0, dt, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -405,11 +646,11 @@ namespace {
 // created by arw_profile_begin, so the timed step allocates nothing.
 enum KernelId {
     KID_OMEGA = 0, KID_TEND_FACES, KID_TEND_COLS, KID_UV, KID_COLUMN,
-    KID_FINISH, KID_SET_W_SURFACE, KID_COUNT
+    KID_FINISH, KID_SET_W_SURFACE, KID_SCALARS, KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {
     "k_omega", "k_tend_faces", "k_tend_cols", "k_uv", "k_column",
-    "k_finish", "k_set_w_surface"};
+    "k_finish", "k_set_w_surface", "moist_scalars"};
 
 struct Profiler {
     bool active = false;
@@ -452,6 +693,14 @@ int32_t check_domain(const Dom* d)
                          d->mup_next};
     for (const void* p : req)
         if (!p) return fail(ARW_E_ARG, "null array in arw_domain");
+    if (d->qv) {
+        const void* moist[] = {d->qc, d->qr, d->qv0, d->qc0, d->qr0, d->qsv,
+                               d->qsc, d->qsr, d->ru_m, d->rv_m, d->ww_m,
+                               d->pd_fxl, d->pd_fxc, d->pd_fyl, d->pd_fyc,
+                               d->pd_fzl, d->pd_fzc, d->pd_sc};
+        for (const void* p : moist)
+            if (!p) return fail(ARW_E_ARG, "moist domain: null scalar or scratch array");
+    }
     return ARW_OK;
 }
 
@@ -508,6 +757,37 @@ int32_t substep(const Dom& d, const Sc& s, float* pout, int write_al,
     if (rc) return rc;
     return launched(run_column(d, s, pout, write_al, write_ww, st), "k_column",
                     KID_COLUMN, st);
+}
+
+// qv/qc/qr one RK stage (reference moist.py:434-698): the unlimited flux
+// form, or the positive-definite limiter on the final stage when
+// moist_adv_opt = 1.  cur: stage-state scalars; q0: time t; dst: new.
+int32_t scalar_stage(const Dom& dd, const arw_params* prm, float* const* cur,
+                     float* const* q0, float* const* dst, const float* mup0,
+                     double dt_stage, bool final, hipStream_t st)
+{
+    const float dt = (float)dt_stage;
+    const float rdx = (float)(1.0 / prm->dx), rdy = (float)(1.0 / prm->dy);
+    const bool pd = final && prm->moist_adv_opt == 1;
+    const dim3 cells((dd.nx + 63) / 64, (dd.ny + 3) / 4, dd.nz);
+    for (int q = 0; q < 3; ++q) {
+        if (pd) {
+            hipLaunchKernelGGL(k_pd_fluxes, union_grid(dd, dd.nz + 1),
+                               dim3(64, 4), 0, st, dd, cur[q], q0[q],
+                               (float)prm->dx, (float)prm->dy, dt);
+            hipLaunchKernelGGL(k_pd_scale, cells, dim3(64, 4), 0, st, dd,
+                               q0[q], mup0, rdx, rdy, dt);
+            hipLaunchKernelGGL(k_pd_apply, cells, dim3(64, 4), 0, st, dd,
+                               q0[q], dst[q], mup0, rdx, rdy, dt);
+        } else {
+            hipLaunchKernelGGL(k_scalar_update, cells, dim3(64, 4), 0, st, dd,
+                               cur[q], q0[q], dst[q], mup0, dt, final ? 1 : 0,
+                               rdx, rdy);
+        }
+        int32_t rc = launched(hipGetLastError(), "scalar stage", KID_SCALARS, st);
+        if (rc) return rc;
+    }
+    return ARW_OK;
 }
 
 }  // namespace
@@ -693,6 +973,10 @@ int32_t arw_step(const arw_domain* d, const arw_params* prm, void* stream)
     float* A[6] = {d->u, d->v, d->w, d->thp, d->php, d->mup};
     float* B[6] = {d->u0, d->v0, d->w0, d->thp0, d->php0, d->mup0};
     float* mup_c = d->mup_next;
+    const bool moist = d->qv != nullptr;
+    float* QA[3] = {d->qv, d->qc, d->qr};
+    float* QB[3] = {d->qv0, d->qc0, d->qr0};
+    float* QC[3] = {d->qsv, d->qsc, d->qsr};
     for (int stage = 0; stage < 3; ++stage) {
         Dom dd = *d;
         // time-t fields: always A
@@ -707,6 +991,11 @@ int32_t arw_step(const arw_domain* d, const arw_params* prm, void* stream)
         o.u = D[0]; o.v = D[1]; o.w = D[2]; o.thp = D[3]; o.php = D[4];
         o.mup = (stage == 0) ? B[5] : (stage == 1 ? mup_c : A[5]);
 
+        if (moist) {
+            // scalars of the stage state: time t, then the stage buffers
+            float* const* Q = (stage == 0) ? QA : (stage == 1 ? QB : QC);
+            dd.qv = Q[0]; dd.qc = Q[1]; dd.qr = Q[2];
+        }
         Sc s = make_sc(prm, dtaus[stage], 1);
         if ((rc = stage_prepare(dd, s, stage == 0, st))) return rc;
         Dom ds = dd;
@@ -716,14 +1005,44 @@ int32_t arw_step(const arw_domain* d, const arw_params* prm, void* stream)
             const bool last = (n + 1 == nsub[stage]);
             // new p'' goes to the buffer holding the dead p''(n-1); the two
             // names swap afterwards
-            if ((rc = substep(ds, sn, ds.p_pp_old, ds.base3d || last, 0, st)))
+            if ((rc = substep(ds, sn, ds.p_pp_old, ds.base3d || last, moist,
+                              st)))
                 return rc;
+            if (moist) {
+                hipLaunchKernelGGL(k_sumflux, union_grid(*d, d->nz + 1),
+                                   dim3(64, 4), 0, st, ds, n == 0);
+                if ((rc = launched(hipGetLastError(), "k_sumflux", KID_SCALARS, st)))
+                    return rc;
+            }
             float* t = ds.p_pp;
             ds.p_pp = ds.p_pp_old;
             ds.p_pp_old = t;
         }
+        if (moist) {
+            // WRF sumflux finish, then the scalar stage update -- both
+            // before the finish, which on the last stage overwrites the
+            // time-t mass the update divides by
+            hipLaunchKernelGGL(k_sumflux_finish, union_grid(*d, d->nz + 1),
+                               dim3(64, 4), 0, st, dd, (float)nsub[stage]);
+            if ((rc = launched(hipGetLastError(), "k_sumflux_finish", KID_SCALARS, st)))
+                return rc;
+            float* const* Q = (stage == 0) ? QA : (stage == 1 ? QB : QC);
+            float* const* Dst = (stage == 0) ? QB : (stage == 1 ? QC : QA);
+            if ((rc = scalar_stage(dd, prm, Q, QA, Dst, A[5],
+                                   nsub[stage] * dtaus[stage], stage == 2, st)))
+                return rc;
+        }
         if ((rc = launched(run_finish(dd, o, st), "k_finish", KID_FINISH, st)))
             return rc;
+        if (moist) {
+            // moist EOS of the new stage state (needs its updated qv)
+            Dom de = dd;
+            de.thp = o.thp; de.php = o.php; de.mup = o.mup;
+            de.qv = (stage == 0) ? QB[0] : (stage == 1 ? QC[0] : QA[0]);
+            hipLaunchKernelGGL(k_eos, col_grid(*d), dim3(256), 0, st, de);
+            if ((rc = launched(hipGetLastError(), "k_eos", KID_SCALARS, st)))
+                return rc;
+        }
     }
     Sc s = make_sc(prm, prm->dt, 1);
     hipLaunchKernelGGL(k_set_w_surface, col_grid(*d), dim3(256), 0, st, *d,

@@ -116,7 +116,8 @@ __global__ __launch_bounds__(256) void k_omega(Dom d, Sc s)
             const unsigned h = k * st;
             const float ph_hi = gld(d.php, h + st, bc);
             const float gb_hi = B3 ? gld(d.phb, h + st, bc) : phb1[k + 1];
-            const float th = (B3 ? gld(d.thb, h, bc) : thb1[k]) + gld(d.thp, h, bc);
+            float th = (B3 ? gld(d.thb, h, bc) : thb1[k]) + gld(d.thp, h, bc);
+            if (d.qv) th = th * (1.0f + RVOVRD_F * gld(d.qv, h, bc));
             const float albk = B3 ? gld(d.alb, h, bc) : alb1[k];
             float dph = (gb_hi + ph_hi) - (gb_lo + ph_lo);
             float alt = (-dph * rdnw[k]) / (c1h[k] * ms + c2h[k]);
@@ -517,7 +518,7 @@ __global__ __launch_bounds__(256) void k_tend_faces_edge(Dom d)
 // w''/phi''/theta''/p''/alpha'' small-step initialisation
 // (dycore.cu:1012-1085).  Row-segment waves; level kinds peel the surface
 // (k = 0) and the lid (k = nz) so the interior level body is one block.
-template <bool B3, bool DIFF>
+template <bool B3, bool DIFF, bool MOIST>
 __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
@@ -598,6 +599,9 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
     float thp_m = 0.0f, thp_0 = gld(d.thp, rA, cC);
     float thp_p = gld(d.thp, st + rA, cC);
     float phpp_0 = gld(d.php0, rA, cC) - gld(d.php, rA, cC);
+    // moist: total water (qv + qc) + qr at the level below (buoyancy)
+    float qt_m = 0.0f, qt_mm = 0.0f;
+    const float mubc = MOIST ? gld(mub, rA, cC) : 0.0f;
 
     // kinds: 0 = surface level (w level 0 + mass level 0), 1 = interior,
     // 2 = lid (w level nz only)
@@ -611,9 +615,13 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
         const int km = k < nz ? k : nz - 1;          // clamped mass level
         float pp_0 = 0.0f;
         float p_k = 0.0f;
+        float qt_0 = 0.0f;
         if (mass) {
             p_k = gld(d.p, lm + rA, cC);
             pp_0 = p_k - (B3 ? gld(d.pb, lm + rA, cC) : pb1[k]);
+            if (MOIST)
+                qt_0 = (gld(d.qv, lm + rA, cC) + gld(d.qc, lm + rA, cC))
+                       + gld(d.qr, lm + rA, cC);
         }
         float ru_i = 0.0f, ru_ip = 0.0f, rv_j = 0.0f, rv_jp = 0.0f;
         float u_i = 0.0f, u_ip = 0.0f, v_j = 0.0f, v_jp = 0.0f;
@@ -662,12 +670,32 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
                 fz[q] = (m == 0 || m == nz - 1) ? lin : cub;
             }
             t = div3(fx1 - fx0, fy1 - fy0, fz[1] - fz[0], s.rdx, s.rdy, rdn[k]);
-            float term = rdn[k] * (pp_0 - pp_m) - c1f[k] * mupc;
+            float term;
+            if (MOIST) {
+                // moist pg_buoy_w (reference dycore.cu:279-289)
+                const float cqw = 0.5f * (qt_0 + qt_m);
+                const float cq1 = 1.0f / (1.0f + cqw);
+                term = (cq1 * rdn[k]) * (pp_0 - pp_m) - c1f[k] * mupc;
+                term = term - (cqw * cq1) * (c1f[k] * mubc + c2f[k]);
+            } else {
+                term = rdn[k] * (pp_0 - pp_m) - c1f[k] * mupc;
+            }
             t = t + G_F * term;
         } else if (top) {
-            float pressure = ((1.0f * 2.0f) * rdnw[nz - 1]) * (0.0f - pp_m);
-            float term = pressure - c1f[nz] * mupc;
-            t = t + G_F * term;
+            if (MOIST) {
+                // lid: the top two layers (reference dycore.cu:258-273);
+                // qt_m holds layer nz-1, qt_mm layer nz-2
+                const float cqw = 0.5f * (qt_m + qt_mm);
+                const float cq1 = 1.0f / (1.0f + cqw);
+                float pressure = ((cq1 * 2.0f) * rdnw[nz - 1]) * (0.0f - pp_m);
+                float term = pressure - c1f[nz] * mupc;
+                term = term - (cqw * cq1) * (c1f[nz] * mubc + c2f[nz]);
+                t = t + G_F * term;
+            } else {
+                float pressure = ((1.0f * 2.0f) * rdnw[nz - 1]) * (0.0f - pp_m);
+                float term = pressure - c1f[nz] * mupc;
+                t = t + G_F * term;
+            }
         }
         if (DIFF) {
             float lap = 0.0f;
@@ -793,6 +821,10 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
         ph_m = ph_0; ph_0 = ph_p;
         ph_p = (k + 2 <= nz) ? ph_n : 0.0f;
         pp_m = pp_0;
+        if (MOIST) {
+            qt_mm = qt_m;
+            qt_m = qt_0;
+        }
         ru_i_m = ru_i; ru_ip_m = ru_ip; rv_j_m = rv_j; rv_jp_m = rv_jp;
         u_i_mm = u_i_m; u_i_m = u_i; u_ip_mm = u_ip_m; u_ip_m = u_ip;
         v_j_mm = v_j_m; v_j_m = v_j; v_jp_mm = v_jp_m; v_jp_m = v_jp;
@@ -1464,7 +1496,7 @@ struct FinIn {
 // Small-step finish (reference dycore.cu:1087-1198) of the mass column
 // (j, i), its u face (j, i) and v face (j, i), then the EOS of the new
 // state (diagnostics.cu:37-89).
-template <bool B3>
+template <bool B3, bool EOS>
 __global__ __launch_bounds__(256) void k_finish(Dom d, FinishDst o)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
@@ -1537,7 +1569,7 @@ __global__ __launch_bounds__(256) void k_finish(Dom d, FinishDst o)
         const float php_new = L.php + L.phpp;
         gst(o.php, f, bc, php_new);
         const float phb_k = B3 ? L.phb : phb1[k];
-        if (K != 0) {
+        if (EOS && K != 0) {
             const unsigned h = f - st;
             float th = thb_lo + thp_lo;
             float dph = (phb_k + php_new) - (phb_lo + php_lo);
@@ -1635,17 +1667,24 @@ hipError_t run_tend_faces(const Dom& d, const Sc& s, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t run_tend_cols(const Dom& d, const Sc& s, hipStream_t st)
+template <bool MOIST>
+static void launch_tend_cols(const Dom& d, const Sc& s, hipStream_t st)
 {
     const dim3 g = rowseg_grid(d.nx, d.ny), b(64, 4);
     if (d.base3d && s.diff)
-        hipLaunchKernelGGL((k_tend_cols<true, true>), g, b, 0, st, d, s);
+        hipLaunchKernelGGL((k_tend_cols<true, true, MOIST>), g, b, 0, st, d, s);
     else if (d.base3d)
-        hipLaunchKernelGGL((k_tend_cols<true, false>), g, b, 0, st, d, s);
+        hipLaunchKernelGGL((k_tend_cols<true, false, MOIST>), g, b, 0, st, d, s);
     else if (s.diff)
-        hipLaunchKernelGGL((k_tend_cols<false, true>), g, b, 0, st, d, s);
+        hipLaunchKernelGGL((k_tend_cols<false, true, MOIST>), g, b, 0, st, d, s);
     else
-        hipLaunchKernelGGL((k_tend_cols<false, false>), g, b, 0, st, d, s);
+        hipLaunchKernelGGL((k_tend_cols<false, false, MOIST>), g, b, 0, st, d, s);
+}
+
+hipError_t run_tend_cols(const Dom& d, const Sc& s, hipStream_t st)
+{
+    if (d.qv) launch_tend_cols<true>(d, s, st);
+    else launch_tend_cols<false>(d, s, st);
     return hipGetLastError();
 }
 
@@ -1698,10 +1737,16 @@ hipError_t run_column(const Dom& d, const Sc& s, float* pout, int write_al,
 
 hipError_t run_finish(const Dom& d, const FinishDst& o, hipStream_t st)
 {
-    if (d.base3d)
-        hipLaunchKernelGGL(k_finish<true>, cols(d), dim3(256), 0, st, d, o);
+    // a moist state's EOS waits for the scalar update of the stage
+    const bool eos = d.qv == nullptr;
+    if (d.base3d && eos)
+        hipLaunchKernelGGL((k_finish<true, true>), cols(d), dim3(256), 0, st, d, o);
+    else if (d.base3d)
+        hipLaunchKernelGGL((k_finish<true, false>), cols(d), dim3(256), 0, st, d, o);
+    else if (eos)
+        hipLaunchKernelGGL((k_finish<false, true>), cols(d), dim3(256), 0, st, d, o);
     else
-        hipLaunchKernelGGL(k_finish<false>, cols(d), dim3(256), 0, st, d, o);
+        hipLaunchKernelGGL((k_finish<false, false>), cols(d), dim3(256), 0, st, d, o);
     hipLaunchKernelGGL(k_finish_edge, dim3((d.ny + d.nx + 255) / 256),
                        dim3(256), 0, st, d, o);
     return hipGetLastError();

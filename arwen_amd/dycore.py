@@ -34,6 +34,7 @@ def params(cfg: RunConfig) -> _lib.ArwParams:
     p.damp_opt = int(cfg.damp_opt)
     p.zdamp, p.dampcoef = float(cfg.zdamp), float(cfg.dampcoef)
     p.top_lid = int(bool(cfg.top_lid))
+    p.moist_adv_opt = int(cfg.moist_adv_opt)
     return p
 
 
@@ -67,8 +68,10 @@ def step(state: DomainState, cfg: RunConfig, *, acoustic: bool = True,
             "boundary mass-flux observers apply to open boundaries, which "
             "are not part of this port (periodic domains conserve mass)")
     validate_for_step(cfg, acoustic=acoustic)
-    if cfg.moist or state.qv is not None:
-        raise NotImplementedError("moist transport is not wired yet")
+    if bool(cfg.moist) != (state.qv is not None):
+        raise ValueError(
+            "cfg.moist must match the state: a moist step needs a state "
+            "allocated with the qv/qc/qr scalars, and a dry one none")
     _check_state(state)
     lib = _lib.load()
     d = state.domain_struct(diffusion=_uses_diffusion(cfg))

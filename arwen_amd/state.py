@@ -12,7 +12,8 @@ import numpy as np
 import torch
 
 from . import constants as c
-from ._lib import DIFF_ARRAYS, DOMAIN_ARRAYS, SCRATCH_ARRAYS, ArwDomain
+from ._lib import (DIFF_ARRAYS, DOMAIN_ARRAYS, MOIST_ARRAYS, MOIST_SCRATCH,
+                   SCRATCH_ARRAYS, ArwDomain)
 from .config import RunConfig
 from .grid import BaseState, VerticalCoord, surface_extrapolation
 
@@ -214,6 +215,21 @@ class DomainState:
         if diffusion:
             for n, t in zip(DIFF_ARRAYS, self._diff_spacings()):
                 setattr(d, n, t.data_ptr())
+        if self.qv is not None:
+            m3, w3 = (nz, ny, nx), (nz + 1, ny, nx)
+            u3, v3 = (nz, ny, nx + 1), (nz, ny + 1, nx)
+            for name in MOIST_ARRAYS:
+                t = (getattr(self, name) if not name.startswith("qs")
+                     else self.scratch(m3, "arw_" + name))
+                if not t.is_contiguous() or t.dtype != DTYPE:
+                    raise ValueError(f"state.{name} must be contiguous float32")
+                setattr(d, name, t.data_ptr())
+            shapes = {"ru_m": u3, "rv_m": v3, "ww_m": w3, "pd_fxl": u3,
+                      "pd_fxc": u3, "pd_fyl": v3, "pd_fyc": v3, "pd_fzl": w3,
+                      "pd_fzc": w3, "pd_sc": m3}
+            for name in MOIST_SCRATCH:
+                setattr(d, name, self.scratch(shapes[name],
+                                              "arw_" + name).data_ptr())
         return d
 
     def numpy_fields(self, names=("u", "v", "w", "thp", "php", "mup")):

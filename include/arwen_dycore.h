@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define ARW_ABI_VERSION 2
+#define ARW_ABI_VERSION 3
 
 #define ARW_OK 0
 #define ARW_E_ARG (-1)      /* invalid argument (null pointer, bad extent) */
@@ -62,6 +62,16 @@ typedef struct arw_domain {
      * half-level fields rdzf_h (nz-1), rdzc_h (nz); w rdzf_w (nz),
      * rdzc_w (nz+1).  May be NULL when khdif = kvdif = 0. */
     const float *rdzf_h, *rdzc_h, *rdzf_w, *rdzc_w;
+    /* moist scalars (reference state.py:408-427), all NULL for a dry
+     * domain.  qv/qc/qr hold time t on entry to arw_step and the new
+     * state on exit; qv0/qc0/qr0 and qsv/qsc/qsr are stage buffers. */
+    float *qv, *qc, *qr, *qv0, *qc0, *qr0, *qsv, *qsc, *qsr;
+    /* moist scratch: substep-averaged mass fluxes (WRF sumflux; ru_m
+     * (nz,ny,nx+1), rv_m (nz,ny+1,nx), ww_m (nz+1,ny,nx)) and the
+     * positive-definite limiter's face fluxes (fxl/fxc like u, fyl/fyc
+     * like v, fzl/fzc like w) and cell scales (nz,ny,nx) */
+    float *ru_m, *rv_m, *ww_m;
+    float *pd_fxl, *pd_fxc, *pd_fyl, *pd_fyc, *pd_fzl, *pd_fzc, *pd_sc;
 } arw_domain;
 
 /* Step configuration (the RunConfig fields the step reads,
@@ -77,15 +87,19 @@ typedef struct arw_params {
     int32_t damp_opt;
     double zdamp, dampcoef;
     int32_t top_lid;
+    int32_t moist_adv_opt;     /* 1: PD limiter on the final RK stage */
 } arw_params;
 
 int32_t arw_abi_version(void);
 const char *arw_last_error(void);
 
 /* Whole RK3 split-explicit step, in place (replaces gpuwm/core/dycore.py:
- * 2161-2503 ``step`` for periodic, msf = 1, dry states).  Leaves p/al/alt
- * consistent with the new state, like the reference's closing
- * update_diagnostics (dycore.py:2482). */
+ * 2161-2503 ``step`` for periodic, msf = 1 states, dry or carrying the
+ * qv/qc/qr scalars with mp_physics = 0: moist EOS and buoyancy, WRF
+ * sumflux, the coupled scalar stage update of moist.py:434-698 and the
+ * positive-definite limiter of pd_advection.cu on the final stage when
+ * moist_adv_opt = 1).  Leaves p/al/alt consistent with the new state,
+ * like the reference's closing update_diagnostics (dycore.py:2482). */
 int32_t arw_step(const arw_domain *d, const arw_params *prm, void *stream);
 
 /* EOS p, al, alt from thp, php, mup (replaces diagnostics.cu:37-89
