@@ -1,0 +1,82 @@
+"""The C-ABI library loads on a host without a GPU and exports every
+entry point include/arwen_dycore.h declares (no device calls here), and
+the ctypes struct layouts match the header's field order and sizes."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from arwen_amd import _lib
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "arwen_dycore.h"
+
+
+def _declared():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(
+        r"^(?:int32_t|const char \*|void|double)\s*\*?\s*(arw_\w+)\(",
+        text, flags=re.M)))
+
+
+def _lib_or_skip():
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built (run __graft_entry__.build())")
+    return ctypes.CDLL(str(_lib.LIB_PATH))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert "arw_step" in names and len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib_or_skip()
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_the_header():
+    assert sorted(_lib.exported_symbols()) == _declared()
+
+
+def test_abi_version_matches_binding():
+    lib = _lib_or_skip()
+    lib.arw_abi_version.restype = ctypes.c_int32
+    text = HEADER.read_text()
+    ver = int(re.search(r"#define ARW_ABI_VERSION (\d+)", text).group(1))
+    assert lib.arw_abi_version() == ver == _lib.ABI_VERSION
+
+
+def _struct_fields(name):
+    text = HEADER.read_text()
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name),
+                     text, flags=re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        m = re.match(r"(?:const\s+)?(int32_t|float|double)\s+(.*)", decl,
+                     flags=re.S)
+        base, rest = m.group(1), m.group(2)
+        for part in rest.split(","):
+            part = part.strip()
+            ptr = part.startswith("*")
+            fields.append((part.lstrip("*").strip(),
+                           "ptr" if ptr else base))
+    return fields
+
+
+@pytest.mark.parametrize("cname,pyt", [("arw_domain", _lib.ArwDomain),
+                                       ("arw_params", _lib.ArwParams)])
+def test_ctypes_layout_matches_header(cname, pyt):
+    kinds = {"ptr": ctypes.c_void_p, "int32_t": ctypes.c_int32,
+             "float": ctypes.c_float, "double": ctypes.c_double}
+    want = [(n, kinds[k]) for n, k in _struct_fields(cname)]
+    have = [(n, t) for n, t in pyt._fields_]
+    assert [n for n, _ in have] == [n for n, _ in want]
+    assert all(ctypes.sizeof(a) == ctypes.sizeof(b)
+               for (_, a), (_, b) in zip(have, want))
