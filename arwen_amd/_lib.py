@@ -45,7 +45,13 @@ class ArwDomain(ctypes.Structure):
                 + [(n, _P) for n in SCRATCH_ARRAYS]
                 + [(n, _P) for n in DIFF_ARRAYS]
                 + [(n, _P) for n in MOIST_ARRAYS]
-                + [(n, _P) for n in MOIST_SCRATCH])
+                + [(n, _P) for n in MOIST_SCRATCH]
+                + [("tri64", _P)])
+
+
+class ArwDomain64(ctypes.Structure):
+    """arw64_domain: the same fields with FP64 arrays and scalars."""
+    _fields_ = [(n, _D if t is _F else t) for n, t in ArwDomain._fields_]
 
 
 class ArwParams(ctypes.Structure):
@@ -54,7 +60,8 @@ class ArwParams(ctypes.Structure):
                 ("epssm", _D), ("smdiv", _D),
                 ("khdif", _D), ("kvdif", _D),
                 ("damp_opt", _I), ("zdamp", _D), ("dampcoef", _D),
-                ("top_lid", _I), ("moist_adv_opt", _I)]
+                ("top_lid", _I), ("moist_adv_opt", _I),
+                ("precision_form", _I)]
 
 
 class ArwenError(RuntimeError):
@@ -98,11 +105,21 @@ def load():
     for name in ("arw_flux_div_scalar", "arw_flux_div_u", "arw_flux_div_v",
                  "arw_flux_div_w"):
         sigs[name] = (flux, _I)
+    # the FP64 build of the same entry points
+    d64 = ctypes.POINTER(ArwDomain64)
+    for name in FP64_ENTRY_POINTS:
+        base = "arw_" + name[len("arw64_"):]
+        args, res = sigs[base]
+        args = [d64 if a is dp else a for a in args]
+        if name == "arw64_flux_div_scalar":
+            args = [_D if a is _F else a for a in args]
+        sigs[name] = (args, res)
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
-    if lib.arw_abi_version() != ABI_VERSION:
+    if lib.arw_abi_version() != ABI_VERSION \
+            or lib.arw64_abi_version() != ABI_VERSION:
         raise ArwenError(f"{LIB_NAME} ABI {lib.arw_abi_version()} != "
                          f"expected {ABI_VERSION}; rebuild the library")
     _lib = lib
@@ -111,14 +128,25 @@ def load():
 
 def check(rc: int, what: str) -> None:
     if rc != 0:
-        msg = load().arw_last_error()
+        lib = load()
+        msg = (lib.arw64_last_error() if what.startswith("arw64_")
+               else lib.arw_last_error())
         raise ArwenError(f"{what} failed ({rc}): "
                          f"{msg.decode() if msg else ''}")
 
 
+#: FP64 entry points (same discretisation in double precision)
+FP64_ENTRY_POINTS = (
+    "arw64_abi_version", "arw64_last_error", "arw64_step",
+    "arw64_calc_p_alpha", "arw64_stage_fluxes", "arw64_stage_prepare",
+    "arw64_acoustic_substep", "arw64_calc_coefs", "arw64_small_step_finish",
+    "arw64_set_w_surface", "arw64_health", "arw64_domain_mass",
+    "arw64_flux_div_scalar")
+
+
 def exported_symbols() -> list[str]:
     """Names of the C entry points this binding declares."""
-    return ["arw_abi_version", "arw_last_error", "arw_step",
+    return list(FP64_ENTRY_POINTS) + ["arw_abi_version", "arw_last_error", "arw_step",
             "arw_calc_p_alpha", "arw_stage_fluxes", "arw_stage_prepare",
             "arw_acoustic_substep", "arw_calc_coefs", "arw_small_step_finish",
             "arw_set_w_surface", "arw_health", "arw_domain_mass",

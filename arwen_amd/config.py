@@ -63,6 +63,10 @@ class RunConfig:
     sf_sfclay_physics: int = 0
     sf_surface_physics: int = 0
     nwp_diagnostics: int = 0
+    # --- extension of this build (no reference field): the precision form
+    # of the float32 step -- Kahan-compensated column sums and an FP64
+    # vertical implicit tridiagonal
+    precision_form: bool = False
 
     def with_(self, **kw) -> "RunConfig":
         """Copy with fields replaced (frozen dataclass convenience)."""

@@ -1,10 +1,17 @@
 // Shared device helpers for the gfx950 dycore kernels.
 //
 // Arithmetic contract: the library is compiled with -ffp-contract=off and
-// every expression below is written in the evaluation order of the
-// float32 NumPy restatement of the reference step, so the device result
-// rounds exactly like that restatement.  Powers and sines are evaluated
-// in double precision and rounded once to float.
+// every expression below is written in the evaluation order of the NumPy
+// restatement of the reference step, so the device result rounds exactly
+// like that restatement in the same precision.  In the float32 build,
+// powers and sines are evaluated in double precision and rounded once.
+//
+// Precision: every kernel source is compiled twice, with ARW_F64 = 0
+// (real = float, entry points arw_*) and ARW_F64 = 1 (real = double,
+// entry points arw64_*): the FP64 build is the same discretisation in
+// double precision, the reference the float32 step's error is measured
+// against.  R(x) spells a literal in the build's precision (R(0.5) is
+// 0.5f in the float32 build, so its instructions are unchanged).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -12,55 +19,69 @@
 
 #include "arwen_dycore.h"
 
+#if ARW_F64
+typedef double real;
+typedef arw64_domain Dom;
+#define R(x) x
+#define ARW_NS prec64
+#define ARW_API(name) arw64_##name
+#else
+typedef float real;
 typedef arw_domain Dom;
+#define R(x) x##f
+#define ARW_NS prec32
+#define ARW_API(name) arw_##name
+#endif
+// bytes per element: the unit of the column-march byte offsets
+#define RB ((unsigned)sizeof(real))
 
-// Float32 scalars derived from the double-precision configuration exactly
-// as the reference's host code derives them before a launch
-// (rdx = float32(1/dx), dtau = float32(dt/ns), dampmag =
-// float32(dtau_double * dampcoef), ...).
+// Scalars derived from the double-precision configuration exactly as the
+// reference's host code derives them before a launch (rdx = real(1/dx),
+// dtau = real(dt/ns), dampmag = real(dtau_double * dampcoef), ...).
 struct Sc {
-    float rdx, rdy;          // float32(1/dx)
-    float hrdx, hrdy;        // float32(0.5/dx)
-    float qrdx, qrdy;        // float32(0.25/dx)
-    float dxi2, dyi2;        // float32(1/dx**2)
-    float kh, kv;
-    float dtau, epssm, smdiv;
-    float dampmag, zdamp;
-    int top_lid, diff, damp, first, use_pold;
+    real rdx, rdy;          // float32(1/dx)
+    real hrdx, hrdy;        // float32(0.5/dx)
+    real qrdx, qrdy;        // float32(0.25/dx)
+    real dxi2, dyi2;        // float32(1/dx**2)
+    real kh, kv;
+    real dtau, epssm, smdiv;
+    real dampmag, zdamp;
+    int top_lid, diff, damp, first, use_pold, prec;
 };
 
 static inline Sc make_sc(const arw_params* p, double dtau, int first)
 {
     Sc s;
     double rdx = 1.0 / p->dx, rdy = 1.0 / p->dy;
-    s.rdx = (float)rdx;
-    s.rdy = (float)rdy;
-    s.hrdx = (float)(0.5 * rdx);
-    s.hrdy = (float)(0.5 * rdy);
-    s.qrdx = (float)(0.25 * rdx);
-    s.qrdy = (float)(0.25 * rdy);
-    s.dxi2 = (float)(1.0 / (p->dx * p->dx));
-    s.dyi2 = (float)(1.0 / (p->dy * p->dy));
-    s.kh = (float)p->khdif;
-    s.kv = (float)p->kvdif;
-    s.dtau = (float)dtau;
-    s.epssm = (float)p->epssm;
-    s.smdiv = first ? 0.0f : (float)p->smdiv;
+    s.rdx = (real)rdx;
+    s.rdy = (real)rdy;
+    s.hrdx = (real)(0.5 * rdx);
+    s.hrdy = (real)(0.5 * rdy);
+    s.qrdx = (real)(0.25 * rdx);
+    s.qrdy = (real)(0.25 * rdy);
+    s.dxi2 = (real)(1.0 / (p->dx * p->dx));
+    s.dyi2 = (real)(1.0 / (p->dy * p->dy));
+    s.kh = (real)p->khdif;
+    s.kv = (real)p->kvdif;
+    s.dtau = (real)dtau;
+    s.epssm = (real)p->epssm;
+    s.smdiv = first ? R(0.0) : (real)p->smdiv;
     s.first = first != 0;
     s.use_pold = 1;
     s.damp = p->damp_opt == 3;
-    s.dampmag = s.damp ? (float)(dtau * p->dampcoef) : 0.0f;
-    s.zdamp = (float)p->zdamp;
+    s.dampmag = s.damp ? (real)(dtau * p->dampcoef) : R(0.0);
+    s.zdamp = (real)p->zdamp;
     s.top_lid = p->top_lid != 0;
     s.diff = (p->khdif > 0.0 || p->kvdif > 0.0);
+    s.prec = (!ARW_F64) && p->precision_form != 0;
     return s;
 }
 
-#define G_F 9.81f
-#define RD_F 287.0f
-#define P0_F 100000.0f
-#define GAMMA_F 1.4f
-#define RVOVRD_F 1.6083624362945557f
+#define G_F R(9.81)
+#define RD_F R(287.0)
+#define P0_F R(100000.0)
+#define GAMMA_F R(1.4)
+#define RVOVRD_F R(1.6083624362945557)
 
 // Positive modulo for small offsets and any n >= 1.
 __device__ __forceinline__ int pmod(int i, int n)
@@ -85,32 +106,32 @@ __device__ __forceinline__ int iV(int k, int j, int i, int ny, int nx)
 }
 
 // WRF flux5 (upwind-biased 5th order); face between qm1 and q0.
-__device__ __forceinline__ float flux5(float qm3, float qm2, float qm1,
-                                       float q0, float qp1, float qp2,
-                                       float vel)
+__device__ __forceinline__ real flux5(real qm3, real qm2, real qm1,
+                                       real q0, real qp1, real qp2,
+                                       real vel)
 {
-    float a = 37.0f * (q0 + qm1) - 8.0f * (qp1 + qm2) + (qp2 + qm3);
-    float b = 10.0f * (q0 - qm1) - 5.0f * (qp1 - qm2) + (qp2 - qm3);
-    return (vel * a - fabsf(vel) * b) / 60.0f;
+    real a = R(37.0) * (q0 + qm1) - R(8.0) * (qp1 + qm2) + (qp2 + qm3);
+    real b = R(10.0) * (q0 - qm1) - R(5.0) * (qp1 - qm2) + (qp2 - qm3);
+    return (vel * a - fabs(vel) * b) / R(60.0);
 }
 
 // Omega-signed vertical flux3 (eta decreases upward).
-__device__ __forceinline__ float flux3(float qm2, float qm1, float q0,
-                                       float qp1, float vel)
+__device__ __forceinline__ real flux3(real qm2, real qm1, real q0,
+                                       real qp1, real vel)
 {
-    float a = 7.0f * (q0 + qm1) - (qp1 + qm2);
-    float b = 3.0f * (q0 - qm1) - (qp1 - qm2);
-    return (vel * a + fabsf(vel) * b) / 12.0f;
+    real a = R(7.0) * (q0 + qm1) - (qp1 + qm2);
+    real b = R(3.0) * (q0 - qm1) - (qp1 - qm2);
+    return (vel * a + fabs(vel) * b) / R(12.0);
 }
 
 // x**GAMMA evaluated in double and rounded once.
-__device__ __forceinline__ float pow_gamma(float x)
+__device__ __forceinline__ real pow_gamma(real x)
 {
-    return (float)pow((double)x, (double)GAMMA_F);
+    return (real)pow((double)x, (double)GAMMA_F);
 }
 
 // Base-profile value: 1-D column (base3d = 0) or per-point 3-D field.
-__device__ __forceinline__ float basev(const float* __restrict__ b, int k,
+__device__ __forceinline__ real basev(const real* __restrict__ b, int k,
                                        int c, int st, int base3d)
 {
     return base3d ? b[k * st + c] : b[k];
@@ -131,18 +152,18 @@ __device__ __forceinline__ float basev(const float* __restrict__ b, int k,
 // the loop.  Byte offsets are unsigned 32-bit: the host keeps every array
 // below 4 GiB.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float gld(const float* base, unsigned lev,
+__device__ __forceinline__ real gld(const real* base, unsigned lev,
                                      unsigned colb)
 {
     const char* b = reinterpret_cast<const char*>(base + lev);
-    return *reinterpret_cast<const float*>(b + (uint64_t)colb);
+    return *reinterpret_cast<const real*>(b + (uint64_t)colb);
 }
 
-__device__ __forceinline__ void gst(float* base, unsigned lev, unsigned colb,
-                                    float x)
+__device__ __forceinline__ void gst(real* base, unsigned lev, unsigned colb,
+                                    real x)
 {
     char* b = reinterpret_cast<char*>(base + lev);
-    *reinterpret_cast<float*>(b + (uint64_t)colb) = x;
+    *reinterpret_cast<real*>(b + (uint64_t)colb) = x;
 }
 
 __device__ __forceinline__ void colb_opaque(unsigned& a)
@@ -155,11 +176,11 @@ __device__ __forceinline__ void colb_opaque(unsigned& a)
 // address space lets a uniform index become a scalar load: it is tracked
 // by lgkmcnt instead of the in-order vector-memory counter, so waiting for
 // a coefficient does not also wait for the next level's prefetched loads.
-typedef const __attribute__((address_space(4))) float kfloat;
+typedef const __attribute__((address_space(4))) real kreal;
 
-__device__ __forceinline__ kfloat* kview(const float* p)
+__device__ __forceinline__ kreal* kview(const real* p)
 {
-    return (kfloat*)p;
+    return (kreal*)p;
 }
 
 // ---------------------------------------------------------------------------
@@ -187,7 +208,20 @@ static inline unsigned xcd_grid(unsigned nblocks)
 // Pin a value as computed unconditionally.  A select between a cheap and
 // an expensive operand (a division) is otherwise turned back into a
 // branch by the code generator, splitting a level loop into blocks.
-__device__ __forceinline__ void keep(float& x)
+__device__ __forceinline__ void keep(real& x)
 {
     asm volatile("" : "+v"(x));
 }
+
+// Compensated (Kahan) running sum of the precision form: s carries the
+// sum, e the low-order part lost by the last addition.
+struct Kahan {
+    real s = R(0.0), e = R(0.0);
+    __device__ __forceinline__ void add(real x)
+    {
+        const real y = x - e;
+        const real t = s + y;
+        e = (t - s) - y;
+        s = t;
+    }
+};

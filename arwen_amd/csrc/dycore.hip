@@ -22,6 +22,8 @@
 
 #include <new>
 
+namespace ARW_NS {
+
 static thread_local char g_err[512] = "";
 
 static int32_t fail(int32_t code, const char* msg)
@@ -34,28 +36,28 @@ static int32_t fail(int32_t code, const char* msg)
 // Shared device pieces
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ float mut_at(const Dom& d, int c)
+__device__ __forceinline__ real mut_at(const Dom& d, int c)
 {
     return d.mub2d[c] + d.mup[c];
 }
 
 // Coupled x mass flux at u face f (0..nx) of row j, level k:
 // (c1h*<mu>_x + c2h)*u with <mu>_x = 0.5*(mu[f] + mu[f-1]) (periodic).
-__device__ __forceinline__ float ru_at(const Dom& d, int k, int j, int f)
+__device__ __forceinline__ real ru_at(const Dom& d, int k, int j, int f)
 {
     const int nx = d.nx, ny = d.ny;
     int a = (f >= nx) ? f - nx : f;
     int b = (f == 0) ? nx - 1 : f - 1;
-    float mux = 0.5f * (mut_at(d, j * nx + a) + mut_at(d, j * nx + b));
+    real mux = R(0.5) * (mut_at(d, j * nx + a) + mut_at(d, j * nx + b));
     return (d.c1h[k] * mux + d.c2h[k]) * d.u[iU(k, j, f, ny, nx)];
 }
 
-__device__ __forceinline__ float rv_at(const Dom& d, int k, int g, int i)
+__device__ __forceinline__ real rv_at(const Dom& d, int k, int g, int i)
 {
     const int nx = d.nx, ny = d.ny;
     int a = (g >= ny) ? g - ny : g;
     int b = (g == 0) ? ny - 1 : g - 1;
-    float muy = 0.5f * (mut_at(d, a * nx + i) + mut_at(d, b * nx + i));
+    real muy = R(0.5) * (mut_at(d, a * nx + i) + mut_at(d, b * nx + i));
     return (d.c1h[k] * muy + d.c2h[k]) * d.v[iV(k, g, i, ny, nx)];
 }
 
@@ -63,20 +65,20 @@ __device__ __forceinline__ float rv_at(const Dom& d, int k, int g, int i)
 // advection.cu:189-201): zero at the boundaries, fnm/fnp 2nd order one
 // face in, Omega-signed flux3 inside.
 template <class Q>
-__device__ __forceinline__ float zface_half(Q q, float vel, int kf, int nz,
-                                            const float* __restrict__ fnm,
-                                            const float* __restrict__ fnp)
+__device__ __forceinline__ real zface_half(Q q, real vel, int kf, int nz,
+                                            const real* __restrict__ fnm,
+                                            const real* __restrict__ fnp)
 {
-    if (kf == 0 || kf == nz) return 0.0f;
+    if (kf == 0 || kf == nz) return R(0.0);
     if (kf == 1 || kf == nz - 1)
         return vel * (fnm[kf] * q(kf) + fnp[kf] * q(kf - 1));
     return flux3(q(kf - 2), q(kf - 1), q(kf), q(kf + 1), vel);
 }
 
-__device__ __forceinline__ float div3(float dfx, float dfy, float dfz,
-                                      float rdx, float rdy, float rdz)
+__device__ __forceinline__ real div3(real dfx, real dfy, real dfz,
+                                      real rdx, real rdy, real rdz)
 {
-    float t = -dfx * rdx;
+    real t = -dfx * rdx;
     t = t - dfy * rdy;
     return t - dfz * rdz;
 }
@@ -88,126 +90,126 @@ __device__ __forceinline__ float div3(float dfx, float dfy, float dfz,
 // Scalar q at mass point (k,j,i); q(kk,jj,ii) accessor with wrapped
 // indices; RU(k,j,f), RV(k,g,i), RW(kf,j,i) advecting fluxes.
 template <class Q, class RU, class RV, class RW>
-__device__ float adv_scalar(Q q, RU ru, RV rv, RW rw, int k, int j, int i,
+__device__ real adv_scalar(Q q, RU ru, RV rv, RW rw, int k, int j, int i,
                             int nz, int ny, int nx,
-                            const float* __restrict__ rdnw,
-                            const float* __restrict__ fnm,
-                            const float* __restrict__ fnp,
-                            float rdx, float rdy)
+                            const real* __restrict__ rdnw,
+                            const real* __restrict__ fnm,
+                            const real* __restrict__ fnp,
+                            real rdx, real rdy)
 {
-    float x[7], y[7];
+    real x[7], y[7];
 #pragma unroll
     for (int o = 0; o < 7; ++o) {
         x[o] = q(k, j, pmod(i + o - 3, nx));
         y[o] = q(k, pmod(j + o - 3, ny), i);
     }
-    float fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], ru(k, j, i));
-    float fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], ru(k, j, i + 1));
-    float fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], rv(k, j, i));
-    float fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], rv(k, j + 1, i));
+    real fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], ru(k, j, i));
+    real fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], ru(k, j, i + 1));
+    real fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], rv(k, j, i));
+    real fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], rv(k, j + 1, i));
     auto qz = [&](int kk) { return q(kk, j, i); };
-    float fz0 = zface_half(qz, rw(k, j, i), k, nz, fnm, fnp);
-    float fz1 = zface_half(qz, rw(k + 1, j, i), k + 1, nz, fnm, fnp);
+    real fz0 = zface_half(qz, rw(k, j, i), k, nz, fnm, fnp);
+    real fz1 = zface_half(qz, rw(k + 1, j, i), k + 1, nz, fnm, fnp);
     return div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, rdx, rdy, rdnw[k]);
 }
 
 // u momentum at u point (k, j, i), i = 0..nx (reference advection.cu:
 // 572-611).  u(kk,jj,ii) reads the u array at a wrapped core column.
 template <class U, class RU, class RV, class RW>
-__device__ float adv_u(U u, RU ru, RV rv, RW rw, int k, int j, int i,
+__device__ real adv_u(U u, RU ru, RV rv, RW rw, int k, int j, int i,
                        int nz, int ny, int nx,
-                       const float* __restrict__ rdnw,
-                       const float* __restrict__ fnm,
-                       const float* __restrict__ fnp, float rdx, float rdy)
+                       const real* __restrict__ rdnw,
+                       const real* __restrict__ fnm,
+                       const real* __restrict__ fnp, real rdx, real rdy)
 {
     int ic = (i >= nx) ? i - nx : i;
     int im = (ic == 0) ? nx - 1 : ic - 1;
     // x faces at mass centres im (left) and ic (right)
-    float x[7];
+    real x[7];
 #pragma unroll
     for (int o = 0; o < 7; ++o) x[o] = u(k, j, pmod(ic + o - 3, nx));
-    float vl = 0.5f * (ru(k, j, im) + ru(k, j, im + 1));
-    float vr = 0.5f * (ru(k, j, ic) + ru(k, j, ic + 1));
-    float fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], vl);
-    float fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], vr);
-    float y[7];
+    real vl = R(0.5) * (ru(k, j, im) + ru(k, j, im + 1));
+    real vr = R(0.5) * (ru(k, j, ic) + ru(k, j, ic + 1));
+    real fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], vl);
+    real fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], vr);
+    real y[7];
 #pragma unroll
     for (int o = 0; o < 7; ++o) y[o] = u(k, pmod(j + o - 3, ny), ic);
-    float v0 = 0.5f * (rv(k, j, im) + rv(k, j, ic));
-    float v1 = 0.5f * (rv(k, j + 1, im) + rv(k, j + 1, ic));
-    float fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], v0);
-    float fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], v1);
+    real v0 = R(0.5) * (rv(k, j, im) + rv(k, j, ic));
+    real v1 = R(0.5) * (rv(k, j + 1, im) + rv(k, j + 1, ic));
+    real fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], v0);
+    real fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], v1);
     auto qz = [&](int kk) { return u(kk, j, ic); };
-    float w0 = 0.5f * (rw(k, j, im) + rw(k, j, ic));
-    float w1 = 0.5f * (rw(k + 1, j, im) + rw(k + 1, j, ic));
-    float fz0 = zface_half(qz, w0, k, nz, fnm, fnp);
-    float fz1 = zface_half(qz, w1, k + 1, nz, fnm, fnp);
+    real w0 = R(0.5) * (rw(k, j, im) + rw(k, j, ic));
+    real w1 = R(0.5) * (rw(k + 1, j, im) + rw(k + 1, j, ic));
+    real fz0 = zface_half(qz, w0, k, nz, fnm, fnp);
+    real fz1 = zface_half(qz, w1, k + 1, nz, fnm, fnp);
     return div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, rdx, rdy, rdnw[k]);
 }
 
 // v momentum at v point (k, j, i), j = 0..ny (reference advection.cu:
 // 754-794).
 template <class V, class RU, class RV, class RW>
-__device__ float adv_v(V v, RU ru, RV rv, RW rw, int k, int j, int i,
+__device__ real adv_v(V v, RU ru, RV rv, RW rw, int k, int j, int i,
                        int nz, int ny, int nx,
-                       const float* __restrict__ rdnw,
-                       const float* __restrict__ fnm,
-                       const float* __restrict__ fnp, float rdx, float rdy)
+                       const real* __restrict__ rdnw,
+                       const real* __restrict__ fnm,
+                       const real* __restrict__ fnp, real rdx, real rdy)
 {
     int jc = (j >= ny) ? j - ny : j;
     int jm = (jc == 0) ? ny - 1 : jc - 1;
-    float x[7];
+    real x[7];
 #pragma unroll
     for (int o = 0; o < 7; ++o) x[o] = v(k, jc, pmod(i + o - 3, nx));
-    float u0 = 0.5f * (ru(k, jm, i) + ru(k, jc, i));
-    float u1 = 0.5f * (ru(k, jm, i + 1) + ru(k, jc, i + 1));
-    float fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], u0);
-    float fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], u1);
-    float y[7];
+    real u0 = R(0.5) * (ru(k, jm, i) + ru(k, jc, i));
+    real u1 = R(0.5) * (ru(k, jm, i + 1) + ru(k, jc, i + 1));
+    real fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], u0);
+    real fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], u1);
+    real y[7];
 #pragma unroll
     for (int o = 0; o < 7; ++o) y[o] = v(k, pmod(jc + o - 3, ny), i);
-    float vl = 0.5f * (rv(k, jm, i) + rv(k, jm + 1, i));
-    float vr = 0.5f * (rv(k, jc, i) + rv(k, jc + 1, i));
-    float fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], vl);
-    float fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], vr);
+    real vl = R(0.5) * (rv(k, jm, i) + rv(k, jm + 1, i));
+    real vr = R(0.5) * (rv(k, jc, i) + rv(k, jc + 1, i));
+    real fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], vl);
+    real fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], vr);
     auto qz = [&](int kk) { return v(kk, jc, i); };
-    float w0 = 0.5f * (rw(k, jm, i) + rw(k, jc, i));
-    float w1 = 0.5f * (rw(k + 1, jm, i) + rw(k + 1, jc, i));
-    float fz0 = zface_half(qz, w0, k, nz, fnm, fnp);
-    float fz1 = zface_half(qz, w1, k + 1, nz, fnm, fnp);
+    real w0 = R(0.5) * (rw(k, jm, i) + rw(k, jc, i));
+    real w1 = R(0.5) * (rw(k + 1, jm, i) + rw(k + 1, jc, i));
+    real fz0 = zface_half(qz, w0, k, nz, fnm, fnp);
+    real fz1 = zface_half(qz, w1, k + 1, nz, fnm, fnp);
     return div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, rdx, rdy, rdnw[k]);
 }
 
 // w momentum at interior w level k = 1..nz-1 (reference advection.cu:
 // 1016-1050).
 template <class W, class RU, class RV, class RW>
-__device__ float adv_w(W w, RU ru, RV rv, RW rw, int k, int j, int i,
+__device__ real adv_w(W w, RU ru, RV rv, RW rw, int k, int j, int i,
                        int nz, int ny, int nx,
-                       const float* __restrict__ rdn,
-                       const float* __restrict__ fnm,
-                       const float* __restrict__ fnp, float rdx, float rdy)
+                       const real* __restrict__ rdn,
+                       const real* __restrict__ fnm,
+                       const real* __restrict__ fnp, real rdx, real rdy)
 {
-    float x[7], y[7];
+    real x[7], y[7];
 #pragma unroll
     for (int o = 0; o < 7; ++o) {
         x[o] = w(k, j, pmod(i + o - 3, nx));
         y[o] = w(k, pmod(j + o - 3, ny), i);
     }
-    float u0 = fnm[k] * ru(k, j, i) + fnp[k] * ru(k - 1, j, i);
-    float u1 = fnm[k] * ru(k, j, i + 1) + fnp[k] * ru(k - 1, j, i + 1);
-    float v0 = fnm[k] * rv(k, j, i) + fnp[k] * rv(k - 1, j, i);
-    float v1 = fnm[k] * rv(k, j + 1, i) + fnp[k] * rv(k - 1, j + 1, i);
-    float fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], u0);
-    float fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], u1);
-    float fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], v0);
-    float fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], v1);
-    float fz[2];
+    real u0 = fnm[k] * ru(k, j, i) + fnp[k] * ru(k - 1, j, i);
+    real u1 = fnm[k] * ru(k, j, i + 1) + fnp[k] * ru(k - 1, j, i + 1);
+    real v0 = fnm[k] * rv(k, j, i) + fnp[k] * rv(k - 1, j, i);
+    real v1 = fnm[k] * rv(k, j + 1, i) + fnp[k] * rv(k - 1, j + 1, i);
+    real fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], u0);
+    real fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], u1);
+    real fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], v0);
+    real fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], v1);
+    real fz[2];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         int m = k - 1 + s;
-        float vel = 0.5f * (rw(m, j, i) + rw(m + 1, j, i));
+        real vel = R(0.5) * (rw(m, j, i) + rw(m + 1, j, i));
         if (m == 0 || m == nz - 1)
-            fz[s] = 0.5f * vel * (w(m + 1, j, i) + w(m, j, i));
+            fz[s] = R(0.5) * vel * (w(m + 1, j, i) + w(m, j, i));
         else
             fz[s] = flux3(w(m - 1, j, i), w(m, j, i), w(m + 1, j, i),
                           w(m + 2, j, i), vel);
@@ -220,23 +222,23 @@ __device__ float adv_w(W w, RU ru, RV rv, RW rw, int k, int j, int i,
 // ---------------------------------------------------------------------------
 
 struct Arr3 {
-    const float* p;
+    const real* p;
     int ny, nx;      // row count and row width of the array
-    __device__ float operator()(int k, int j, int i) const
+    __device__ real operator()(int k, int j, int i) const
     {
         return p[(k * ny + j) * nx + i];
     }
 };
 
-__global__ void k_flux_div(int which, const float* __restrict__ q,
-                           const float* __restrict__ ru,
-                           const float* __restrict__ rv,
-                           const float* __restrict__ rw,
-                           float* __restrict__ tend,
-                           const float* __restrict__ rdz,
-                           const float* __restrict__ fnm,
-                           const float* __restrict__ fnp, float rdx,
-                           float rdy, int nz, int ny, int nx)
+__global__ void k_flux_div(int which, const real* __restrict__ q,
+                           const real* __restrict__ ru,
+                           const real* __restrict__ rv,
+                           const real* __restrict__ rw,
+                           real* __restrict__ tend,
+                           const real* __restrict__ rdz,
+                           const real* __restrict__ fnm,
+                           const real* __restrict__ fnp, real rdx,
+                           real rdy, int nz, int ny, int nx)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -272,15 +274,15 @@ __global__ void k_flux_div(int which, const float* __restrict__ q,
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void eos_level(const Dom& d, int k, int c, int st,
-                                          float mu)
+                                          real mu)
 {
-    float th = basev(d.thb, k, c, st, d.base3d) + d.thp[k * st + c];
-    if (d.qv) th = th * (1.0f + RVOVRD_F * d.qv[k * st + c]);
-    float ph_hi = basev(d.phb, k + 1, c, st, d.base3d)
+    real th = basev(d.thb, k, c, st, d.base3d) + d.thp[k * st + c];
+    if (d.qv) th = th * (R(1.0) + RVOVRD_F * d.qv[k * st + c]);
+    real ph_hi = basev(d.phb, k + 1, c, st, d.base3d)
                   + d.php[(k + 1) * st + c];
-    float ph_lo = basev(d.phb, k, c, st, d.base3d) + d.php[k * st + c];
-    float dph = ph_hi - ph_lo;
-    float alt = (-dph * d.rdnw[k]) / (d.c1h[k] * mu + d.c2h[k]);
+    real ph_lo = basev(d.phb, k, c, st, d.base3d) + d.php[k * st + c];
+    real dph = ph_hi - ph_lo;
+    real alt = (-dph * d.rdnw[k]) / (d.c1h[k] * mu + d.c2h[k]);
     d.alt[k * st + c] = alt;
     d.al[k * st + c] = alt - basev(d.alb, k, c, st, d.base3d);
     d.p[k * st + c] = P0_F * pow_gamma((RD_F * th) / (P0_F * alt));
@@ -291,7 +293,7 @@ __global__ void k_eos(Dom d)
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     int st = d.ny * d.nx;
     if (c >= st) return;
-    float mu = mut_at(d, c);
+    real mu = mut_at(d, c);
     for (int k = 0; k < d.nz; ++k) eos_level(d, k, c, st, mu);
 }
 
@@ -311,14 +313,14 @@ __global__ void k_set_w_surface(Dom d, Sc s)
         return (d.cf1 * d.v[iV(0, g, i, ny, nx)] + d.cf2 * d.v[iV(1, g, i, ny, nx)])
                + d.cf3 * d.v[iV(2, g, i, ny, nx)];
     };
-    float h = d.ht[c];
-    float dyn = d.ht[jp * nx + i] - h, dys = h - d.ht[jm * nx + i];
-    float dxe = d.ht[j * nx + ip] - h, dxw = h - d.ht[j * nx + im];
+    real h = d.ht[c];
+    real dyn = d.ht[jp * nx + i] - h, dys = h - d.ht[jm * nx + i];
+    real dxe = d.ht[j * nx + ip] - h, dxw = h - d.ht[j * nx + im];
     d.w[c] = s.hrdy * (dyn * vc(j + 1) + dys * vc(j))
              + s.hrdx * (dxe * uc(i + 1) + dxw * uc(i));
 }
 
-__global__ void k_faces(Dom d, float* ru, float* rv)
+__global__ void k_faces(Dom d, real* ru, real* rv)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -344,22 +346,22 @@ __global__ void k_sumflux(Dom d, int first)
     if (i > nx || j > ny || k > nz) return;
     if (j < ny && k < nz) {
         const int x = iU(k, j, i, ny, nx);
-        d.ru_m[x] = (first ? 0.0f : d.ru_m[x]) + d.u_pp[x];
+        d.ru_m[x] = (first ? R(0.0) : d.ru_m[x]) + d.u_pp[x];
     }
     if (i < nx && k < nz) {
         const int x = iV(k, j, i, ny, nx);
-        d.rv_m[x] = (first ? 0.0f : d.rv_m[x]) + d.v_pp[x];
+        d.rv_m[x] = (first ? R(0.0) : d.rv_m[x]) + d.v_pp[x];
     }
     if (i < nx && j < ny) {
         const int x = iM(k, j, i, ny, nx);
-        d.ww_m[x] = (first ? 0.0f : d.ww_m[x]) + d.ww_pp[x];
+        d.ww_m[x] = (first ? R(0.0) : d.ww_m[x]) + d.ww_pp[x];
     }
 }
 
 // WRF sumflux at the last substep (reference acoustic.cu:1469-1489): the
 // substep mean plus the stage-reference coupled fluxes ru, rv (from the
 // stage state d.u/d.v/d.mup) and Omega (d.ww).
-__global__ void k_sumflux_finish(Dom d, float nsub)
+__global__ void k_sumflux_finish(Dom d, real nsub)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -383,7 +385,7 @@ __global__ void k_sumflux_finish(Dom d, float nsub)
 // Post-acoustic stage mu' of column c: the value small_step_finish
 // stores (mup + mu''), formed here so the scalar update can run before
 // the finish overwrites the time-t mass on the last stage.
-__device__ __forceinline__ float mup_new_at(const Dom& d, int c)
+__device__ __forceinline__ real mup_new_at(const Dom& d, int c)
 {
     return d.mup[c] + d.mu_pp[c];
 }
@@ -391,30 +393,30 @@ __device__ __forceinline__ float mup_new_at(const Dom& d, int c)
 // WRF rk_update_scalar (reference moist.py:385-398): (C(mu_t) q_t + dt
 // tend) / C(mu_new), clamped at zero on the final stage (NaN kept, -0
 // becomes +0).
-__device__ __forceinline__ float scalar_update(const Dom& d, int k, int c,
-                                               float q0, float tend,
-                                               const float* mup0, float dt,
+__device__ __forceinline__ real scalar_update(const Dom& d, int k, int c,
+                                               real q0, real tend,
+                                               const real* mup0, real dt,
                                                int clamp)
 {
-    const float mu0 = d.mub2d[c] + mup0[c];
-    const float mu = d.mub2d[c] + mup_new_at(d, c);
-    const float chm0 = d.c1h[k] * mu0 + d.c2h[k];
-    const float chm = d.c1h[k] * mu + d.c2h[k];
-    float v = chm0 * q0;
-    const float t = dt * tend;
+    const real mu0 = d.mub2d[c] + mup0[c];
+    const real mu = d.mub2d[c] + mup_new_at(d, c);
+    const real chm0 = d.c1h[k] * mu0 + d.c2h[k];
+    const real chm = d.c1h[k] * mu + d.c2h[k];
+    real v = chm0 * q0;
+    const real t = dt * tend;
     v = v + t;
     v = v / chm;
-    if (clamp && !(v > 0.0f) && v == v) v = 0.0f;
+    if (clamp && !(v > R(0.0)) && v == v) v = R(0.0);
     return v;
 }
 
 // Unlimited stage update of one scalar: tend = div of qcur with the
 // averaged fluxes ru_m, rv_m, ww_m; qdst = update from q0.
-__global__ void k_scalar_update(Dom d, const float* __restrict__ qcur,
-                                const float* __restrict__ q0,
-                                float* __restrict__ qdst,
-                                const float* __restrict__ mup0, float dt,
-                                int clamp, float rdx, float rdy)
+__global__ void k_scalar_update(Dom d, const real* __restrict__ qcur,
+                                const real* __restrict__ q0,
+                                real* __restrict__ qdst,
+                                const real* __restrict__ mup0, real dt,
+                                int clamp, real rdx, real rdy)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -423,26 +425,26 @@ __global__ void k_scalar_update(Dom d, const float* __restrict__ qcur,
     if (i >= nx || j >= ny || k >= nz) return;
     Arr3 Q{qcur, ny, nx}, RU{d.ru_m, ny, nx + 1}, RV{d.rv_m, ny + 1, nx},
         RW{d.ww_m, ny, nx};
-    const float tend = 0.0f + adv_scalar(Q, RU, RV, RW, k, j, i, nz, ny, nx,
+    const real tend = R(0.0) + adv_scalar(Q, RU, RV, RW, k, j, i, nz, ny, nx,
                                          d.rdnw, d.fnm, d.fnp, rdx, rdy);
     const int x = iM(k, j, i, ny, nx);
     qdst[x] = scalar_update(d, k, j * nx + i, q0[x], tend, mup0, dt, clamp);
 }
 
 // WRF flux_upwind (reference pd_advection.cu:101-108).
-__device__ __forceinline__ float flux_upwind(float qm1, float q0v, float cr)
+__device__ __forceinline__ real flux_upwind(real qm1, real q0v, real cr)
 {
-    return (0.5f * fminf(1.0f, cr + fabsf(cr))) * qm1
-           + (0.5f * fmaxf(-1.0f, cr - fabsf(cr))) * q0v;
+    return (R(0.5) * fmin(R(1.0), cr + fabs(cr))) * qm1
+           + (R(0.5) * fmax(-R(1.0), cr - fabs(cr))) * q0v;
 }
 
 // PD face fluxes (reference pd_advection.cu pd_fluxes, periodic, no
 // msf): the CFL-clamped upwind flux of q0 and the correction to the
 // unlimited flux of qcur, at x faces (k, j, i<=nx), y faces (k, j<=ny, i)
 // and eta faces (k<=nz, j, i).  mut is the post-acoustic stage mass.
-__global__ void k_pd_fluxes(Dom d, const float* __restrict__ qcur,
-                            const float* __restrict__ q0, float dx, float dy,
-                            float dt)
+__global__ void k_pd_fluxes(Dom d, const real* __restrict__ qcur,
+                            const real* __restrict__ q0, real dx, real dy,
+                            real dt)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -456,37 +458,37 @@ __global__ void k_pd_fluxes(Dom d, const float* __restrict__ qcur,
     auto qc = [&](int kk, int jj, int ii) { return qcur[iM(kk, jj, ii, ny, nx)]; };
     if (j < ny && k < nz) {
         const int a = (i >= nx) ? i - nx : i, b = (i == 0) ? nx - 1 : i - 1;
-        const float vel = d.ru_m[iU(k, j, i, ny, nx)];
-        const float muf = (d.c1h[k] * 0.5f) * (mut(j, a) + mut(j, b)) + d.c2h[k];
-        const float cr = ((vel * dt) / dx) / muf;
-        const float fl = (muf * (dx / dt)) * flux_upwind(q0a(k, j, b), q0a(k, j, a), cr);
-        float x[6];
+        const real vel = d.ru_m[iU(k, j, i, ny, nx)];
+        const real muf = (d.c1h[k] * R(0.5)) * (mut(j, a) + mut(j, b)) + d.c2h[k];
+        const real cr = ((vel * dt) / dx) / muf;
+        const real fl = (muf * (dx / dt)) * flux_upwind(q0a(k, j, b), q0a(k, j, a), cr);
+        real x[6];
 #pragma unroll
         for (int o = 0; o < 6; ++o) x[o] = qc(k, j, pmod(i + o - 3, nx));
-        const float fh = flux5(x[0], x[1], x[2], x[3], x[4], x[5], vel);
+        const real fh = flux5(x[0], x[1], x[2], x[3], x[4], x[5], vel);
         d.pd_fxl[iU(k, j, i, ny, nx)] = fl;
         d.pd_fxc[iU(k, j, i, ny, nx)] = fh - fl;
     }
     if (i < nx && k < nz) {
         const int a = (j >= ny) ? j - ny : j, b = (j == 0) ? ny - 1 : j - 1;
-        const float vel = d.rv_m[iV(k, j, i, ny, nx)];
-        const float muf = (d.c1h[k] * 0.5f) * (mut(a, i) + mut(b, i)) + d.c2h[k];
-        const float cr = ((vel * dt) / dy) / muf;
-        const float fl = (muf * (dy / dt)) * flux_upwind(q0a(k, b, i), q0a(k, a, i), cr);
-        float y[6];
+        const real vel = d.rv_m[iV(k, j, i, ny, nx)];
+        const real muf = (d.c1h[k] * R(0.5)) * (mut(a, i) + mut(b, i)) + d.c2h[k];
+        const real cr = ((vel * dt) / dy) / muf;
+        const real fl = (muf * (dy / dt)) * flux_upwind(q0a(k, b, i), q0a(k, a, i), cr);
+        real y[6];
 #pragma unroll
         for (int o = 0; o < 6; ++o) y[o] = qc(k, pmod(j + o - 3, ny), i);
-        const float fh = flux5(y[0], y[1], y[2], y[3], y[4], y[5], vel);
+        const real fh = flux5(y[0], y[1], y[2], y[3], y[4], y[5], vel);
         d.pd_fyl[iV(k, j, i, ny, nx)] = fl;
         d.pd_fyc[iV(k, j, i, ny, nx)] = fh - fl;
     }
     if (i < nx && j < ny) {
-        float fl = 0.0f, fc = 0.0f;
+        real fl = R(0.0), fc = R(0.0);
         if (k > 0 && k < nz) {
-            const float vel = d.ww_m[iM(k, j, i, ny, nx)];
-            const float dz = 2.0f / (d.rdnw[k] + d.rdnw[k - 1]);
-            const float muf = d.c1h[k] * mut(j, i) + d.c2h[k];
-            const float cr = (vel * dt) / (dz * muf);
+            const real vel = d.ww_m[iM(k, j, i, ny, nx)];
+            const real dz = R(2.0) / (d.rdnw[k] + d.rdnw[k - 1]);
+            const real muf = d.c1h[k] * mut(j, i) + d.c2h[k];
+            const real cr = (vel * dt) / (dz * muf);
             fl = (muf * (dz / dt)) * flux_upwind(q0a(k - 1, j, i), q0a(k, j, i), cr);
             auto qz = [&](int kk) { return qc(kk, j, i); };
             fc = zface_half(qz, vel, k, nz, d.fnm, d.fnp) - fl;
@@ -498,9 +500,9 @@ __global__ void k_pd_fluxes(Dom d, const float* __restrict__ qcur,
 
 // PD renormalisation factor of each cell (reference pd_advection.cu
 // pd_scale, periodic no-msf branch).
-__global__ void k_pd_scale(Dom d, const float* __restrict__ q0,
-                           const float* __restrict__ mup0, float dxi,
-                           float dyi, float dt)
+__global__ void k_pd_scale(Dom d, const real* __restrict__ q0,
+                           const real* __restrict__ mup0, real dxi,
+                           real dyi, real dt)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -508,31 +510,31 @@ __global__ void k_pd_scale(Dom d, const float* __restrict__ q0,
     int k = blockIdx.z;
     if (i >= nx || j >= ny || k >= nz) return;
     const int c = j * nx + i;
-    const float chm0 = d.c1h[k] * (d.mub2d[c] + mup0[c]) + d.c2h[k];
+    const real chm0 = d.c1h[k] * (d.mub2d[c] + mup0[c]) + d.c2h[k];
     const int u0 = iU(k, j, i, ny, nx), v0 = iV(k, j, i, ny, nx);
     const int z0 = iM(k, j, i, ny, nx), st = ny * nx;
-    const float dl = (dxi * (d.pd_fxl[u0 + 1] - d.pd_fxl[u0])
+    const real dl = (dxi * (d.pd_fxl[u0 + 1] - d.pd_fxl[u0])
                       + dyi * (d.pd_fyl[v0 + nx] - d.pd_fyl[v0]))
                      + d.rdnw[k] * (d.pd_fzl[z0 + st] - d.pd_fzl[z0]);
-    const float ph_low = chm0 * q0[z0] - dt * dl;
-    const float fo = dt * ((dxi * (fmaxf(0.0f, d.pd_fxc[u0 + 1])
-                                   - fminf(0.0f, d.pd_fxc[u0]))
-                            + dyi * (fmaxf(0.0f, d.pd_fyc[v0 + nx])
-                                     - fminf(0.0f, d.pd_fyc[v0])))
-                           + d.rdnw[k] * (fminf(0.0f, d.pd_fzc[z0 + st])
-                                          - fmaxf(0.0f, d.pd_fzc[z0])));
-    float r = 1.0f;
-    if (fo > ph_low) r = fminf(1.0f, fmaxf(0.0f, ph_low / (fo + 1e-20f)));
+    const real ph_low = chm0 * q0[z0] - dt * dl;
+    const real fo = dt * ((dxi * (fmax(R(0.0), d.pd_fxc[u0 + 1])
+                                   - fmin(R(0.0), d.pd_fxc[u0]))
+                            + dyi * (fmax(R(0.0), d.pd_fyc[v0 + nx])
+                                     - fmin(R(0.0), d.pd_fyc[v0])))
+                           + d.rdnw[k] * (fmin(R(0.0), d.pd_fzc[z0 + st])
+                                          - fmax(R(0.0), d.pd_fzc[z0])));
+    real r = R(1.0);
+    if (fo > ph_low) r = fmin(R(1.0), fmax(R(0.0), ph_low / (fo + R(1e-20))));
     d.pd_sc[z0] = r;
 }
 
 // Renormalised PD divergence and the coupled update (reference
 // pd_advection.cu pd_renorm_apply + moist.py:385-398, final stage, clamp):
 // each correction flux takes its donor cell's scale.
-__global__ void k_pd_apply(Dom d, const float* __restrict__ q0,
-                           float* __restrict__ qdst,
-                           const float* __restrict__ mup0, float dxi,
-                           float dyi, float dt)
+__global__ void k_pd_apply(Dom d, const real* __restrict__ q0,
+                           real* __restrict__ qdst,
+                           const real* __restrict__ mup0, real dxi,
+                           real dyi, real dt)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -544,27 +546,27 @@ __global__ void k_pd_apply(Dom d, const float* __restrict__ q0,
     const int z0 = iM(k, j, i, ny, nx);
     const int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
     const int jp = (j + 1 == ny) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
-    const float* sc = d.pd_sc;
-    const float s0 = sc[z0];
-    const float fxc_l = d.pd_fxc[u0], fxc_r = d.pd_fxc[u0 + 1];
-    const float sx_l = (fxc_l > 0.0f) ? sc[iM(k, j, im, ny, nx)]
-                                      : ((fxc_l < 0.0f) ? s0 : 1.0f);
-    const float sx_r = (fxc_r > 0.0f) ? s0
-                       : ((fxc_r < 0.0f) ? sc[iM(k, j, ip, ny, nx)] : 1.0f);
-    const float fyc_l = d.pd_fyc[v0], fyc_r = d.pd_fyc[v0 + nx];
-    const float sy_l = (fyc_l > 0.0f) ? sc[iM(k, jm, i, ny, nx)]
-                                      : ((fyc_l < 0.0f) ? s0 : 1.0f);
-    const float sy_r = (fyc_r > 0.0f) ? s0
-                       : ((fyc_r < 0.0f) ? sc[iM(k, jp, i, ny, nx)] : 1.0f);
-    const float fzc_b = d.pd_fzc[z0], fzc_t = d.pd_fzc[z0 + st];
-    const float sz_b = (fzc_b > 0.0f) ? s0
-                       : ((fzc_b < 0.0f) ? sc[z0 - st] : 1.0f);
-    const float sz_t = (fzc_t < 0.0f) ? s0
-                       : ((fzc_t > 0.0f) ? sc[z0 + st] : 1.0f);
-    const float gx = (sx_r * fxc_r + d.pd_fxl[u0 + 1]) - (sx_l * fxc_l + d.pd_fxl[u0]);
-    const float gy = (sy_r * fyc_r + d.pd_fyl[v0 + nx]) - (sy_l * fyc_l + d.pd_fyl[v0]);
-    const float gz = (sz_t * fzc_t + d.pd_fzl[z0 + st]) - (sz_b * fzc_b + d.pd_fzl[z0]);
-    const float tend = 0.0f + (((-dxi) * gx - dyi * gy) - d.rdnw[k] * gz);
+    const real* sc = d.pd_sc;
+    const real s0 = sc[z0];
+    const real fxc_l = d.pd_fxc[u0], fxc_r = d.pd_fxc[u0 + 1];
+    const real sx_l = (fxc_l > R(0.0)) ? sc[iM(k, j, im, ny, nx)]
+                                      : ((fxc_l < R(0.0)) ? s0 : R(1.0));
+    const real sx_r = (fxc_r > R(0.0)) ? s0
+                       : ((fxc_r < R(0.0)) ? sc[iM(k, j, ip, ny, nx)] : R(1.0));
+    const real fyc_l = d.pd_fyc[v0], fyc_r = d.pd_fyc[v0 + nx];
+    const real sy_l = (fyc_l > R(0.0)) ? sc[iM(k, jm, i, ny, nx)]
+                                      : ((fyc_l < R(0.0)) ? s0 : R(1.0));
+    const real sy_r = (fyc_r > R(0.0)) ? s0
+                       : ((fyc_r < R(0.0)) ? sc[iM(k, jp, i, ny, nx)] : R(1.0));
+    const real fzc_b = d.pd_fzc[z0], fzc_t = d.pd_fzc[z0 + st];
+    const real sz_b = (fzc_b > R(0.0)) ? s0
+                       : ((fzc_b < R(0.0)) ? sc[z0 - st] : R(1.0));
+    const real sz_t = (fzc_t < R(0.0)) ? s0
+                       : ((fzc_t > R(0.0)) ? sc[z0 + st] : R(1.0));
+    const real gx = (sx_r * fxc_r + d.pd_fxl[u0 + 1]) - (sx_l * fxc_l + d.pd_fxl[u0]);
+    const real gy = (sy_r * fyc_r + d.pd_fyl[v0 + nx]) - (sy_l * fyc_l + d.pd_fyl[v0]);
+    const real gz = (sz_t * fzc_t + d.pd_fzl[z0 + st]) - (sz_b * fzc_b + d.pd_fzl[z0]);
+    const real tend = R(0.0) + (((-dxi) * gx - dyi * gy) - d.rdnw[k] * gz);
     qdst[z0] = scalar_update(d, k, j * nx + i, q0[z0], tend, mup0, dt, 1);
 }
 
@@ -572,7 +574,7 @@ __global__ void k_pd_apply(Dom d, const float* __restrict__ q0,
 // Reductions
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ unsigned absbits(float x)
+__device__ __forceinline__ unsigned absbits(real x)
 {
     return __float_as_uint(x) & 0x7fffffffu;
 }
@@ -598,7 +600,7 @@ __global__ void k_health(Dom d, unsigned* out)
         if (t < nt) {
             mt = max(mt, absbits(d.thp[t]));
             int k = t / st, c = t - k * st;
-            float dz = ((basev(d.phb, k + 1, c, st, d.base3d) + d.php[t + st])
+            real dz = ((basev(d.phb, k + 1, c, st, d.base3d) + d.php[t + st])
                         - (basev(d.phb, k, c, st, d.base3d) + d.php[t])) / G_F;
             mc = max(mc, absbits(d.w[t + st] / dz));
         }
@@ -750,7 +752,7 @@ int32_t stage_prepare(const Dom& d, const Sc& s, int do_eos, hipStream_t st)
     return launched(run_tend_cols(d, s, st), "k_tend_cols", KID_TEND_COLS, st);
 }
 
-int32_t substep(const Dom& d, const Sc& s, float* pout, int write_al,
+int32_t substep(const Dom& d, const Sc& s, real* pout, int write_al,
                 int write_ww, hipStream_t st)
 {
     int32_t rc = launched(run_uv(d, s, st), "k_uv", KID_UV, st);
@@ -762,19 +764,19 @@ int32_t substep(const Dom& d, const Sc& s, float* pout, int write_al,
 // qv/qc/qr one RK stage (reference moist.py:434-698): the unlimited flux
 // form, or the positive-definite limiter on the final stage when
 // moist_adv_opt = 1.  cur: stage-state scalars; q0: time t; dst: new.
-int32_t scalar_stage(const Dom& dd, const arw_params* prm, float* const* cur,
-                     float* const* q0, float* const* dst, const float* mup0,
+int32_t scalar_stage(const Dom& dd, const arw_params* prm, real* const* cur,
+                     real* const* q0, real* const* dst, const real* mup0,
                      double dt_stage, bool final, hipStream_t st)
 {
-    const float dt = (float)dt_stage;
-    const float rdx = (float)(1.0 / prm->dx), rdy = (float)(1.0 / prm->dy);
+    const real dt = (real)dt_stage;
+    const real rdx = (real)(1.0 / prm->dx), rdy = (real)(1.0 / prm->dy);
     const bool pd = final && prm->moist_adv_opt == 1;
     const dim3 cells((dd.nx + 63) / 64, (dd.ny + 3) / 4, dd.nz);
     for (int q = 0; q < 3; ++q) {
         if (pd) {
             hipLaunchKernelGGL(k_pd_fluxes, union_grid(dd, dd.nz + 1),
                                dim3(64, 4), 0, st, dd, cur[q], q0[q],
-                               (float)prm->dx, (float)prm->dy, dt);
+                               (real)prm->dx, (real)prm->dy, dt);
             hipLaunchKernelGGL(k_pd_scale, cells, dim3(64, 4), 0, st, dd,
                                q0[q], mup0, rdx, rdy, dt);
             hipLaunchKernelGGL(k_pd_apply, cells, dim3(64, 4), 0, st, dd,
@@ -792,13 +794,17 @@ int32_t scalar_stage(const Dom& dd, const arw_params* prm, float* const* cur,
 
 }  // namespace
 
+}  // namespace ARW_NS
+
+using namespace ARW_NS;
+
 extern "C" {
 
-int32_t arw_abi_version(void) { return ARW_ABI_VERSION; }
+int32_t ARW_API(abi_version)(void) { return ARW_ABI_VERSION; }
 
-const char* arw_last_error(void) { return g_err; }
+const char* ARW_API(last_error)(void) { return g_err; }
 
-int32_t arw_calc_p_alpha(const arw_domain* d, void* stream)
+int32_t ARW_API(calc_p_alpha)(const Dom* d, void* stream)
 {
     int32_t rc = check_domain(d);
     if (rc) return rc;
@@ -807,8 +813,8 @@ int32_t arw_calc_p_alpha(const arw_domain* d, void* stream)
     return launched("k_eos");
 }
 
-int32_t arw_stage_fluxes(const arw_domain* d, const arw_params* prm,
-                         float* ru, float* rv, void* stream)
+int32_t ARW_API(stage_fluxes)(const Dom* d, const arw_params* prm,
+                         real* ru, real* rv, void* stream)
 {
     int32_t rc = check_domain(d);
     if (rc) return rc;
@@ -821,10 +827,10 @@ int32_t arw_stage_fluxes(const arw_domain* d, const arw_params* prm,
     return launched("k_faces");
 }
 
-static int32_t flux_div(int which, const float* q, const float* ru,
-                        const float* rv, const float* rw, float* tend,
-                        const float* rdz, const float* fnm, const float* fnp,
-                        float dx_inv, float dy_inv, int32_t nz, int32_t ny,
+static int32_t flux_div(int which, const real* q, const real* ru,
+                        const real* rv, const real* rw, real* tend,
+                        const real* rdz, const real* fnm, const real* fnp,
+                        real dx_inv, real dy_inv, int32_t nz, int32_t ny,
                         int32_t nx, void* stream)
 {
     if (!q || !ru || !rv || !rw || !tend || !rdz || !fnm || !fnp)
@@ -837,47 +843,47 @@ static int32_t flux_div(int which, const float* q, const float* ru,
     return launched("k_flux_div");
 }
 
-int32_t arw_flux_div_scalar(const float* q, const float* ru, const float* rv,
-                            const float* rw, float* tend, const float* rdnw,
-                            const float* fnm, const float* fnp, float dx_inv,
-                            float dy_inv, int32_t nz, int32_t ny, int32_t nx,
+int32_t ARW_API(flux_div_scalar)(const real* q, const real* ru, const real* rv,
+                            const real* rw, real* tend, const real* rdnw,
+                            const real* fnm, const real* fnp, real dx_inv,
+                            real dy_inv, int32_t nz, int32_t ny, int32_t nx,
                             void* stream)
 {
     return flux_div(0, q, ru, rv, rw, tend, rdnw, fnm, fnp, dx_inv, dy_inv,
                     nz, ny, nx, stream);
 }
 
-int32_t arw_flux_div_u(const float* u, const float* ru, const float* rv,
-                       const float* rw, float* tend, const float* rdnw,
-                       const float* fnm, const float* fnp, float dx_inv,
-                       float dy_inv, int32_t nz, int32_t ny, int32_t nx,
+int32_t ARW_API(flux_div_u)(const real* u, const real* ru, const real* rv,
+                       const real* rw, real* tend, const real* rdnw,
+                       const real* fnm, const real* fnp, real dx_inv,
+                       real dy_inv, int32_t nz, int32_t ny, int32_t nx,
                        void* stream)
 {
     return flux_div(1, u, ru, rv, rw, tend, rdnw, fnm, fnp, dx_inv, dy_inv,
                     nz, ny, nx, stream);
 }
 
-int32_t arw_flux_div_v(const float* v, const float* ru, const float* rv,
-                       const float* rw, float* tend, const float* rdnw,
-                       const float* fnm, const float* fnp, float dx_inv,
-                       float dy_inv, int32_t nz, int32_t ny, int32_t nx,
+int32_t ARW_API(flux_div_v)(const real* v, const real* ru, const real* rv,
+                       const real* rw, real* tend, const real* rdnw,
+                       const real* fnm, const real* fnp, real dx_inv,
+                       real dy_inv, int32_t nz, int32_t ny, int32_t nx,
                        void* stream)
 {
     return flux_div(2, v, ru, rv, rw, tend, rdnw, fnm, fnp, dx_inv, dy_inv,
                     nz, ny, nx, stream);
 }
 
-int32_t arw_flux_div_w(const float* w, const float* ru, const float* rv,
-                       const float* rw, float* tend, const float* rdn,
-                       const float* fnm, const float* fnp, float dx_inv,
-                       float dy_inv, int32_t nz, int32_t ny, int32_t nx,
+int32_t ARW_API(flux_div_w)(const real* w, const real* ru, const real* rv,
+                       const real* rw, real* tend, const real* rdn,
+                       const real* fnm, const real* fnp, real dx_inv,
+                       real dy_inv, int32_t nz, int32_t ny, int32_t nx,
                        void* stream)
 {
     return flux_div(3, w, ru, rv, rw, tend, rdn, fnm, fnp, dx_inv, dy_inv,
                     nz, ny, nx, stream);
 }
 
-int32_t arw_stage_prepare(const arw_domain* d, const arw_params* prm,
+int32_t ARW_API(stage_prepare)(const Dom* d, const arw_params* prm,
                           double dtau, int32_t do_eos, void* stream)
 {
     int32_t rc = check_domain(d);
@@ -887,14 +893,14 @@ int32_t arw_stage_prepare(const arw_domain* d, const arw_params* prm,
     hipStream_t st = (hipStream_t)stream;
     if ((rc = stage_prepare(*d, s, do_eos, st))) return rc;
     // the reference initialises p_pp_old = p_pp (dycore.cu:1083-1084)
-    size_t bytes = (size_t)d->nz * d->ny * d->nx * sizeof(float);
+    size_t bytes = (size_t)d->nz * d->ny * d->nx * sizeof(real);
     if (hipMemcpyAsync(d->p_pp_old, d->p_pp, bytes, hipMemcpyDeviceToDevice,
                        st) != hipSuccess)
         return fail(ARW_E_LAUNCH, "p_pp_old copy failed");
     return ARW_OK;
 }
 
-int32_t arw_calc_coefs(const arw_domain* d, const arw_params* prm,
+int32_t ARW_API(calc_coefs)(const Dom* d, const arw_params* prm,
                        double dtau, void* stream)
 {
     int32_t rc = check_domain(d);
@@ -908,7 +914,7 @@ int32_t arw_calc_coefs(const arw_domain* d, const arw_params* prm,
     return launched(run_omega(*d, s, 0, st), "k_omega", -1, st);
 }
 
-int32_t arw_acoustic_substep(const arw_domain* d, const arw_params* prm,
+int32_t ARW_API(acoustic_substep)(const Dom* d, const arw_params* prm,
                              double dtau, int32_t first, void* stream)
 {
     int32_t rc = check_domain(d);
@@ -919,14 +925,14 @@ int32_t arw_acoustic_substep(const arw_domain* d, const arw_params* prm,
     hipStream_t st = (hipStream_t)stream;
     if ((rc = launched(run_uv(*d, s, st), "k_uv", -1, st))) return rc;
     // the reference keeps p_pp_old = p'' before the column update
-    size_t bytes = (size_t)d->nz * d->ny * d->nx * sizeof(float);
+    size_t bytes = (size_t)d->nz * d->ny * d->nx * sizeof(real);
     if (hipMemcpyAsync(d->p_pp_old, d->p_pp, bytes, hipMemcpyDeviceToDevice,
                        st) != hipSuccess)
         return fail(ARW_E_LAUNCH, "p_pp_old copy failed");
     return launched(run_column(*d, s, d->p_pp, 1, 1, st), "k_column", -1, st);
 }
 
-int32_t arw_small_step_finish(const arw_domain* d, void* stream)
+int32_t ARW_API(small_step_finish)(const Dom* d, void* stream)
 {
     int32_t rc = check_domain(d);
     if (rc) return rc;
@@ -934,13 +940,13 @@ int32_t arw_small_step_finish(const arw_domain* d, void* stream)
     FinishDst o = {d->u, d->v, d->w, d->thp, d->php, d->mup_next};
     if ((rc = launched(run_finish(*d, o, st), "k_finish", -1, st))) return rc;
     if (hipMemcpyAsync(d->mup, d->mup_next,
-                       (size_t)d->ny * d->nx * sizeof(float),
+                       (size_t)d->ny * d->nx * sizeof(real),
                        hipMemcpyDeviceToDevice, st) != hipSuccess)
         return fail(ARW_E_LAUNCH, "mup copy failed");
     return ARW_OK;
 }
 
-int32_t arw_set_w_surface(const arw_domain* d, const arw_params* prm,
+int32_t ARW_API(set_w_surface)(const Dom* d, const arw_params* prm,
                           void* stream)
 {
     int32_t rc = check_domain(d);
@@ -960,40 +966,42 @@ int32_t arw_set_w_surface(const arw_domain* d, const arw_params* prm,
 // mup_next -> A because the finish reads neighbouring mu' while it writes.
 // After the step the *0 arrays hold the stage-2 estimate (they are
 // rebuilt every step, like the reference's time-t copies).
-int32_t arw_step(const arw_domain* d, const arw_params* prm, void* stream)
+int32_t ARW_API(step)(const Dom* d, const arw_params* prm, void* stream)
 {
     int32_t rc = check_domain(d);
     if (rc) return rc;
     if ((rc = check_params(prm, d))) return rc;
+    if (!ARW_F64 && prm->precision_form && !d->tri64)
+        return fail(ARW_E_ARG, "precision form needs the FP64 tridiagonal scratch tri64");
     hipStream_t st = (hipStream_t)stream;
     prof_mark(KID_COUNT, st);                    // opening event
     const int ns = prm->time_step_sound;
     const int nsub[3] = {1, ns / 2 > 1 ? ns / 2 : 1, ns};
     const double dtaus[3] = {prm->dt / 3.0, prm->dt / ns, prm->dt / ns};
-    float* A[6] = {d->u, d->v, d->w, d->thp, d->php, d->mup};
-    float* B[6] = {d->u0, d->v0, d->w0, d->thp0, d->php0, d->mup0};
-    float* mup_c = d->mup_next;
+    real* A[6] = {d->u, d->v, d->w, d->thp, d->php, d->mup};
+    real* B[6] = {d->u0, d->v0, d->w0, d->thp0, d->php0, d->mup0};
+    real* mup_c = d->mup_next;
     const bool moist = d->qv != nullptr;
-    float* QA[3] = {d->qv, d->qc, d->qr};
-    float* QB[3] = {d->qv0, d->qc0, d->qr0};
-    float* QC[3] = {d->qsv, d->qsc, d->qsr};
+    real* QA[3] = {d->qv, d->qc, d->qr};
+    real* QB[3] = {d->qv0, d->qc0, d->qr0};
+    real* QC[3] = {d->qsv, d->qsc, d->qsr};
     for (int stage = 0; stage < 3; ++stage) {
         Dom dd = *d;
         // time-t fields: always A
         dd.u0 = A[0]; dd.v0 = A[1]; dd.w0 = A[2]; dd.thp0 = A[3];
         dd.php0 = A[4]; dd.mup0 = A[5];
         // stage reference t*
-        float* const* T = (stage == 0) ? A : B;
+        real* const* T = (stage == 0) ? A : B;
         dd.u = T[0]; dd.v = T[1]; dd.w = T[2]; dd.thp = T[3]; dd.php = T[4];
         dd.mup = (stage == 0) ? A[5] : (stage == 1 ? B[5] : mup_c);
         FinishDst o;
-        float* const* D = (stage == 2) ? A : B;
+        real* const* D = (stage == 2) ? A : B;
         o.u = D[0]; o.v = D[1]; o.w = D[2]; o.thp = D[3]; o.php = D[4];
         o.mup = (stage == 0) ? B[5] : (stage == 1 ? mup_c : A[5]);
 
         if (moist) {
             // scalars of the stage state: time t, then the stage buffers
-            float* const* Q = (stage == 0) ? QA : (stage == 1 ? QB : QC);
+            real* const* Q = (stage == 0) ? QA : (stage == 1 ? QB : QC);
             dd.qv = Q[0]; dd.qc = Q[1]; dd.qr = Q[2];
         }
         Sc s = make_sc(prm, dtaus[stage], 1);
@@ -1014,7 +1022,7 @@ int32_t arw_step(const arw_domain* d, const arw_params* prm, void* stream)
                 if ((rc = launched(hipGetLastError(), "k_sumflux", KID_SCALARS, st)))
                     return rc;
             }
-            float* t = ds.p_pp;
+            real* t = ds.p_pp;
             ds.p_pp = ds.p_pp_old;
             ds.p_pp_old = t;
         }
@@ -1023,11 +1031,11 @@ int32_t arw_step(const arw_domain* d, const arw_params* prm, void* stream)
             // before the finish, which on the last stage overwrites the
             // time-t mass the update divides by
             hipLaunchKernelGGL(k_sumflux_finish, union_grid(*d, d->nz + 1),
-                               dim3(64, 4), 0, st, dd, (float)nsub[stage]);
+                               dim3(64, 4), 0, st, dd, (real)nsub[stage]);
             if ((rc = launched(hipGetLastError(), "k_sumflux_finish", KID_SCALARS, st)))
                 return rc;
-            float* const* Q = (stage == 0) ? QA : (stage == 1 ? QB : QC);
-            float* const* Dst = (stage == 0) ? QB : (stage == 1 ? QC : QA);
+            real* const* Q = (stage == 0) ? QA : (stage == 1 ? QB : QC);
+            real* const* Dst = (stage == 0) ? QB : (stage == 1 ? QC : QA);
             if ((rc = scalar_stage(dd, prm, Q, QA, Dst, A[5],
                                    nsub[stage] * dtaus[stage], stage == 2, st)))
                 return rc;
@@ -1051,7 +1059,7 @@ int32_t arw_step(const arw_domain* d, const arw_params* prm, void* stream)
                     st);
 }
 
-int32_t arw_profile_begin(int32_t max_events)
+int32_t ARW_API(profile_begin)(int32_t max_events)
 {
     if (g_prof.active) return fail(ARW_E_ARG, "profiling already active");
     if (max_events < 2) return fail(ARW_E_ARG, "max_events must be >= 2");
@@ -1073,7 +1081,7 @@ int32_t arw_profile_begin(int32_t max_events)
     return ARW_OK;
 }
 
-int32_t arw_profile_end(double* ms, int32_t* launches, int32_t n_kernels)
+int32_t ARW_API(profile_end)(double* ms, int32_t* launches, int32_t n_kernels)
 {
     if (!g_prof.active) return fail(ARW_E_ARG, "profiling not active");
     int32_t rc = ARW_OK;
@@ -1100,14 +1108,14 @@ int32_t arw_profile_end(double* ms, int32_t* launches, int32_t n_kernels)
     return rc;
 }
 
-const char* arw_kernel_name(int32_t id)
+const char* ARW_API(kernel_name)(int32_t id)
 {
     return (id >= 0 && id < KID_COUNT) ? kKernelNames[id] : "";
 }
 
-int32_t arw_kernel_count(void) { return KID_COUNT; }
+int32_t ARW_API(kernel_count)(void) { return KID_COUNT; }
 
-int32_t arw_health(const arw_domain* d, float* out, void* stream)
+int32_t ARW_API(health)(const Dom* d, real* out, void* stream)
 {
     int32_t rc = check_domain(d);
     if (rc) return rc;
@@ -1123,7 +1131,7 @@ int32_t arw_health(const arw_domain* d, float* out, void* stream)
     return launched("k_health");
 }
 
-int32_t arw_domain_mass(const arw_domain* d, double* out, void* stream)
+int32_t ARW_API(domain_mass)(const Dom* d, double* out, void* stream)
 {
     int32_t rc = check_domain(d);
     if (rc) return rc;

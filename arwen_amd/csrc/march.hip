@@ -12,7 +12,9 @@
 
 #include "march.h"
 
-#define R __restrict__
+namespace ARW_NS {
+
+#define RS __restrict__
 
 namespace {
 
@@ -22,7 +24,7 @@ template <int V>
 using Kind = std::integral_constant<int, V>;
 enum { LV_FIRST = 0, LV_SECOND = 1, LV_THIRD = 2, LV_MID = 3, LV_LAST = 4 };
 
-__device__ __forceinline__ float mu_at(const float* R mub, const float* R mup,
+__device__ __forceinline__ real mu_at(const real* RS mub, const real* RS mup,
                                        int c)
 {
     return mub[c] + mup[c];
@@ -31,48 +33,48 @@ __device__ __forceinline__ float mu_at(const float* R mub, const float* R mup,
 // Vertical face flux of a half-level field at w-level kf from the window
 // q(kf-2), q(kf-1), q(kf), q(kf+1) (reference advection.cu:189-201).
 template <class CP>
-__device__ __forceinline__ float zf(float qm2, float qm1, float q0, float qp1,
-                                    float vel, int kf, int nz, CP fnm, CP fnp)
+__device__ __forceinline__ real zf(real qm2, real qm1, real q0, real qp1,
+                                    real vel, int kf, int nz, CP fnm, CP fnp)
 {
     // both forms are evaluated and one is selected, so a level loop built
     // on zf stays one basic block (fnm/fnp have nz entries)
     const int kc = kf < nz ? kf : nz - 1;
-    const float lin = vel * (fnm[kc] * q0 + fnp[kc] * qm1);
-    float cub = flux3(qm2, qm1, q0, qp1, vel);
+    const real lin = vel * (fnm[kc] * q0 + fnp[kc] * qm1);
+    real cub = flux3(qm2, qm1, q0, qp1, vel);
     keep(cub);
-    const float r = (kf == 1 || kf == nz - 1) ? lin : cub;
-    return (kf == 0 || kf == nz) ? 0.0f : r;
+    const real r = (kf == 1 || kf == nz - 1) ? lin : cub;
+    return (kf == 0 || kf == nz) ? R(0.0) : r;
 }
 
-__device__ __forceinline__ float div3(float dfx, float dfy, float dfz,
-                                      float rdx, float rdy, float rdz)
+__device__ __forceinline__ real div3(real dfx, real dfy, real dfz,
+                                      real rdx, real rdy, real rdz)
 {
-    float t = -dfx * rdx;
+    real t = -dfx * rdx;
     t = t - dfy * rdy;
     return t - dfz * rdz;
 }
 
 // Large-step PGF on a face (reference dycore.cu:98-157) from values at
 // level k of the two columns A (right/north) and B.
-__device__ __forceinline__ float pgf_val(
-    float rd, float hrd, float c1, float c2, float rdnwk, float muf, float dmu,
-    float phA0, float phA1, float phB0, float phB1, float altA, float altB,
-    float ppA, float ppB, float alA, float alB, float pbA, float pbB,
-    int base3d, float phhA, float phhB, float dpnA0, float dpnA1, float dpnB0,
-    float dpnB1)
+__device__ __forceinline__ real pgf_val(
+    real rd, real hrd, real c1, real c2, real rdnwk, real muf, real dmu,
+    real phA0, real phA1, real phB0, real phB1, real altA, real altB,
+    real ppA, real ppB, real alA, real alB, real pbA, real pbB,
+    int base3d, real phhA, real phhB, real dpnA0, real dpnA1, real dpnB0,
+    real dpnB1)
 {
-    float layer = c1 * muf + c2;
-    float bracket = (phA1 - phB1) + (phA0 - phB0);
+    real layer = c1 * muf + c2;
+    real bracket = (phA1 - phB1) + (phA0 - phB0);
     bracket = bracket + (altA + altB) * (ppA - ppB);
     if (base3d)
         bracket = bracket + (alA + alB) * (pbA - pbB);
     else
-        bracket = bracket + (alA + alB) * 0.0f;
-    float left = (hrd * layer) * bracket;
-    float dphp = phhA - phhB;
-    float hi = 0.5f * (dpnA1 + dpnB1);
-    float lo = 0.5f * (dpnA0 + dpnB0);
-    float vertical = rdnwk * (hi - lo) - c1 * dmu;
+        bracket = bracket + (alA + alB) * R(0.0);
+    real left = (hrd * layer) * bracket;
+    real dphp = phhA - phhB;
+    real hi = R(0.5) * (dpnA1 + dpnB1);
+    real lo = R(0.5) * (dpnA0 + dpnB0);
+    real vertical = rdnwk * (hi - lo) - c1 * dmu;
     return left + (rd * dphp) * vertical;
 }
 
@@ -87,7 +89,7 @@ __device__ __forceinline__ float pgf_val(
 // float32 expression, so one sum serves both.  Pass 1 parks each level's
 // divergence in ww; pass 2 turns it into Omega.  EOS: the stage-start
 // calc_p_alpha (diagnostics.cu:37-89) when the stage state is new.
-template <bool EOS, bool B3>
+template <bool EOS, bool B3, bool PREC>
 __global__ __launch_bounds__(256) void k_omega(Dom d, Sc s)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
@@ -97,30 +99,30 @@ __global__ __launch_bounds__(256) void k_omega(Dom d, Sc s)
     const int j = c / nx, i = c - j * nx;
     const int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
     const int jp = (j + 1 == ny) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
-    kfloat* c1h = kview(d.c1h);
-    kfloat* c2h = kview(d.c2h);
-    kfloat* dnw = kview(d.dnw);
-    kfloat* rdnw = kview(d.rdnw);
-    unsigned bc = 4u * c, bu = 4u * (j * (nx + 1) + i), bu1 = bu + 4u;
-    unsigned bvn = 4u * (c + nx);
-    const float ms = d.mub2d[c] + d.mup[c];
+    kreal* c1h = kview(d.c1h);
+    kreal* c2h = kview(d.c2h);
+    kreal* dnw = kview(d.dnw);
+    kreal* rdnw = kview(d.rdnw);
+    unsigned bc = RB * c, bu = RB * (j * (nx + 1) + i), bu1 = bu + RB;
+    unsigned bvn = RB * (c + nx);
+    const real ms = d.mub2d[c] + d.mup[c];
     if (EOS) {
-        kfloat* thb1 = kview(d.thb);
-        kfloat* phb1 = kview(d.phb);
-        kfloat* alb1 = kview(d.alb);
-        float ph_lo = gld(d.php, 0, bc);
-        float gb_lo = B3 ? gld(d.phb, 0, bc) : phb1[0];
+        kreal* thb1 = kview(d.thb);
+        kreal* phb1 = kview(d.phb);
+        kreal* alb1 = kview(d.alb);
+        real ph_lo = gld(d.php, 0, bc);
+        real gb_lo = B3 ? gld(d.phb, 0, bc) : phb1[0];
 #pragma unroll 2
         for (int k = 0; k < nz; ++k) {
             colb_opaque(bc);
             const unsigned h = k * st;
-            const float ph_hi = gld(d.php, h + st, bc);
-            const float gb_hi = B3 ? gld(d.phb, h + st, bc) : phb1[k + 1];
-            float th = (B3 ? gld(d.thb, h, bc) : thb1[k]) + gld(d.thp, h, bc);
-            if (d.qv) th = th * (1.0f + RVOVRD_F * gld(d.qv, h, bc));
-            const float albk = B3 ? gld(d.alb, h, bc) : alb1[k];
-            float dph = (gb_hi + ph_hi) - (gb_lo + ph_lo);
-            float alt = (-dph * rdnw[k]) / (c1h[k] * ms + c2h[k]);
+            const real ph_hi = gld(d.php, h + st, bc);
+            const real gb_hi = B3 ? gld(d.phb, h + st, bc) : phb1[k + 1];
+            real th = (B3 ? gld(d.thb, h, bc) : thb1[k]) + gld(d.thp, h, bc);
+            if (d.qv) th = th * (R(1.0) + RVOVRD_F * gld(d.qv, h, bc));
+            const real albk = B3 ? gld(d.alb, h, bc) : alb1[k];
+            real dph = (gb_hi + ph_hi) - (gb_lo + ph_lo);
+            real alt = (-dph * rdnw[k]) / (c1h[k] * ms + c2h[k]);
             gst(d.alt, h, bc, alt);
             gst(d.al, h, bc, alt - albk);
             gst(d.p, h, bc, P0_F * pow_gamma((RD_F * th) / (P0_F * alt)));
@@ -128,39 +130,55 @@ __global__ __launch_bounds__(256) void k_omega(Dom d, Sc s)
             gb_lo = gb_hi;
         }
     }
-    const float mw = d.mub2d[j * nx + im] + d.mup[j * nx + im];
-    const float me = d.mub2d[j * nx + ip] + d.mup[j * nx + ip];
-    const float mS = d.mub2d[jm * nx + i] + d.mup[jm * nx + i];
-    const float mN = d.mub2d[jp * nx + i] + d.mup[jp * nx + i];
-    const float muxw = 0.5f * (ms + mw), muxe = 0.5f * (me + ms);
-    const float muys = 0.5f * (ms + mS), muyn = 0.5f * (mN + ms);
-    float dmdt = 0.0f, rf = 0.0f;
+    const real mw = d.mub2d[j * nx + im] + d.mup[j * nx + im];
+    const real me = d.mub2d[j * nx + ip] + d.mup[j * nx + ip];
+    const real mS = d.mub2d[jm * nx + i] + d.mup[jm * nx + i];
+    const real mN = d.mub2d[jp * nx + i] + d.mup[jp * nx + i];
+    const real muxw = R(0.5) * (ms + mw), muxe = R(0.5) * (me + ms);
+    const real muys = R(0.5) * (ms + mS), muyn = R(0.5) * (mN + ms);
+    real dmdt = R(0.0), rf = R(0.0);
+    Kahan ksum;
 #pragma unroll 4
     for (int k = 0; k < nz; ++k) {
         colb_opaque(bc); colb_opaque(bu); colb_opaque(bu1); colb_opaque(bvn);
-        const float c1 = c1h[k], c2 = c2h[k];
-        float ruw = (c1 * muxw + c2) * gld(d.u, k * stu, bu);
-        float rue = (c1 * muxe + c2) * gld(d.u, k * stu, bu1);
-        float rvs = (c1 * muys + c2) * gld(d.v, k * stv, bc);
-        float rvn = (c1 * muyn + c2) * gld(d.v, k * stv, bvn);
-        float divv = dnw[k] * (s.rdx * (rue - ruw) + s.rdy * (rvn - rvs));
-        dmdt = (k == 0) ? divv : dmdt + divv;
-        rf = rf + divv;
+        const real c1 = c1h[k], c2 = c2h[k];
+        real ruw = (c1 * muxw + c2) * gld(d.u, k * stu, bu);
+        real rue = (c1 * muxe + c2) * gld(d.u, k * stu, bu1);
+        real rvs = (c1 * muys + c2) * gld(d.v, k * stv, bc);
+        real rvn = (c1 * muyn + c2) * gld(d.v, k * stv, bvn);
+        real divv = dnw[k] * (s.rdx * (rue - ruw) + s.rdy * (rvn - rvs));
+        if (PREC) {
+            ksum.add(divv);
+        } else {
+            dmdt = (k == 0) ? divv : dmdt + divv;
+            rf = rf + divv;
+        }
         gst(d.ww, (k + 1) * st, bc, divv);
     }
-    gst(d.ww, 0, bc, 0.0f);
-    float run = 0.0f;
+    if (PREC) {
+        // precision form: the compensated column sum serves both
+        dmdt = ksum.s;
+        rf = ksum.s;
+    }
+    gst(d.ww, 0, bc, R(0.0));
+    real run = R(0.0);
+    Kahan krun;
 #pragma unroll 4
     for (int k = 0; k < nz - 1; ++k) {
         colb_opaque(bc);
         const unsigned h = (k + 1) * st;
-        float inc = (c1h[k] * dnw[k]) * dmdt + gld(d.ww, h, bc);
-        run = run + inc;
+        real inc = (c1h[k] * dnw[k]) * dmdt + gld(d.ww, h, bc);
+        if (PREC) {
+            krun.add(inc);
+            run = krun.s;
+        } else {
+            run = run + inc;
+        }
         gst(d.ww, h, bc, -run);
     }
-    gst(d.ww, nz * st, bc, 0.0f);
+    gst(d.ww, nz * st, bc, R(0.0));
     d.dmdt_rf[c] = rf;
-    d.rmu_t[c] = 0.0f;
+    d.rmu_t[c] = R(0.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -210,9 +228,9 @@ static inline dim3 rowseg_grid(int nx, int ny)
 // Vertical window of one mass column for the large-step PGF: p - pb at
 // k-1, k, k+1; php, phb and dpn at w levels k, k+1.
 struct PgfWin {
-    float pp_m = 0.0f, pp_0 = 0.0f, pp_p = 0.0f;
-    float ph_0 = 0.0f, ph_p = 0.0f, gb_0 = 0.0f, gb_p = 0.0f;
-    float dpn_0 = 0.0f, dpn_p = 0.0f;
+    real pp_m = R(0.0), pp_0 = R(0.0), pp_p = R(0.0);
+    real ph_0 = R(0.0), ph_p = R(0.0), gb_0 = R(0.0), gb_p = R(0.0);
+    real dpn_0 = R(0.0), dpn_p = R(0.0);
 };
 
 // Slow tendencies on the u face (j, i) and v face (j, i): 5th/3rd-order
@@ -229,13 +247,13 @@ __global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
     const RowSeg rs = row_seg(nx, ny);
     if (!rs.live) return;
     const int i = rs.i, j = rs.j;
-    kfloat* c1h = kview(d.c1h);
-    kfloat* c2h = kview(d.c2h);
-    kfloat* fnm = kview(d.fnm);
-    kfloat* fnp = kview(d.fnp);
-    kfloat* rdnw = kview(d.rdnw);
-    kfloat* rdzf_h = kview(d.rdzf_h);
-    kfloat* rdzc_h = kview(d.rdzc_h);
+    kreal* c1h = kview(d.c1h);
+    kreal* c2h = kview(d.c2h);
+    kreal* fnm = kview(d.fnm);
+    kreal* fnp = kview(d.fnp);
+    kreal* rdnw = kview(d.rdnw);
+    kreal* rdzf_h = kview(d.rdzf_h);
+    kreal* rdzc_h = kview(d.rdzc_h);
     // wave-uniform rows
     int yo[7];
 #pragma unroll
@@ -245,9 +263,9 @@ __global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
     // per-lane column byte offsets
     unsigned cx[7];
 #pragma unroll
-    for (int o = 0; o < 7; ++o) cx[o] = 4u * pmod(i + o - 3, nx);
-    unsigned ci1 = 4u * (i + 1);                 // unwrapped east face
-    unsigned cub = 4u * ((i == 0) ? nx : i);     // u_b column
+    for (int o = 0; o < 7; ++o) cx[o] = RB * pmod(i + o - 3, nx);
+    unsigned ci1 = RB * (i + 1);                 // unwrapped east face
+    unsigned cub = RB * ((i == 0) ? nx : i);     // u_b column
     auto opaque_all = [&]() {
 #pragma unroll
         for (int o = 0; o < 7; ++o) colb_opaque(cx[o]);
@@ -257,37 +275,37 @@ __global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
     unsigned& cW = cx[2];
     // mass columns: A = (j, i), W = (j, i-1), S = (j-1, i)
     const int rA = j * nx, rS = jm * nx;
-    const float* R mub = d.mub2d;
-    const float* R mup = d.mup;
-    const float* R mup0 = d.mup0;
+    const real* RS mub = d.mub2d;
+    const real* RS mup = d.mup;
+    const real* RS mup0 = d.mup0;
     auto mu2 = [&](int row, unsigned cb) {
         return gld(mub, row, cb) + gld(mup, row, cb);
     };
-    const float mA = mu2(rA, cC), mW = mu2(rA, cW), mS = mu2(rS, cC);
-    const float mE = mu2(rA, cx[4]), mN = mu2(jp * nx, cC);
+    const real mA = mu2(rA, cC), mW = mu2(rA, cW), mS = mu2(rS, cC);
+    const real mE = mu2(rA, cx[4]), mN = mu2(jp * nx, cC);
     // u-face masses
-    const float mux_a = 0.5f * (mW + mu2(rA, cx[1]));
-    const float mux_c = 0.5f * (mA + mW);
-    const float mux_d = 0.5f * (mE + mA);
-    const float muy_jW = 0.5f * (mW + mu2(rS, cW));
-    const float muy_jA = 0.5f * (mA + mS);
-    const float muy_pW = 0.5f * (mu2(jp * nx, cW) + mW);
-    const float muy_pA = 0.5f * (mN + mA);
+    const real mux_a = R(0.5) * (mW + mu2(rA, cx[1]));
+    const real mux_c = R(0.5) * (mA + mW);
+    const real mux_d = R(0.5) * (mE + mA);
+    const real muy_jW = R(0.5) * (mW + mu2(rS, cW));
+    const real muy_jA = R(0.5) * (mA + mS);
+    const real muy_pW = R(0.5) * (mu2(jp * nx, cW) + mW);
+    const real muy_pA = R(0.5) * (mN + mA);
     // v-face masses
-    const float muy_a = 0.5f * (mS + mu2(yo[1] * nx, cC));
-    const float muy_c = 0.5f * (mA + mS);
-    const float muy_d = 0.5f * (mN + mA);
-    const float mux_jS = 0.5f * (mS + mu2(rS, cW));
-    const float mux_jA = 0.5f * (mA + mW);
-    const float mux_pS = 0.5f * (mu2(rS, cx[4]) + mS);
-    const float mux_pA = 0.5f * (mE + mA);
+    const real muy_a = R(0.5) * (mS + mu2(yo[1] * nx, cC));
+    const real muy_c = R(0.5) * (mA + mS);
+    const real muy_d = R(0.5) * (mN + mA);
+    const real mux_jS = R(0.5) * (mS + mu2(rS, cW));
+    const real mux_jA = R(0.5) * (mA + mW);
+    const real mux_pS = R(0.5) * (mu2(rS, cx[4]) + mS);
+    const real mux_pA = R(0.5) * (mE + mA);
     // PGF and init masses
-    const float dmu_u = 0.5f * (gld(mup, rA, cC) + gld(mup, rA, cW));
-    const float dmu_v = 0.5f * (gld(mup, rA, cC) + gld(mup, rS, cC));
-    const float mtA = gld(mub, rA, cC) + gld(mup0, rA, cC);
-    const float mtf_u = 0.5f * (mtA + (gld(mub, rA, cW) + gld(mup0, rA, cW)));
-    const float mtf_v = 0.5f * (mtA + (gld(mub, rS, cC) + gld(mup0, rS, cC)));
-    const float msf_u = 0.5f * (mA + mW), msf_v = 0.5f * (mA + mS);
+    const real dmu_u = R(0.5) * (gld(mup, rA, cC) + gld(mup, rA, cW));
+    const real dmu_v = R(0.5) * (gld(mup, rA, cC) + gld(mup, rS, cC));
+    const real mtA = gld(mub, rA, cC) + gld(mup0, rA, cC);
+    const real mtf_u = R(0.5) * (mtA + (gld(mub, rA, cW) + gld(mup0, rA, cW)));
+    const real mtf_v = R(0.5) * (mtA + (gld(mub, rS, cC) + gld(mup0, rS, cC)));
+    const real msf_u = R(0.5) * (mA + mW), msf_v = R(0.5) * (mA + mS);
 
     // PGF windows of A, W, S
     auto pp_at = [&](int k, int row, unsigned cb) {
@@ -295,95 +313,95 @@ __global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
         return gld(d.p, l, cb) - (B3 ? gld(d.pb, l, cb) : kview(d.pb)[k]);
     };
     auto win_init = [&](PgfWin& X, int row, unsigned cb) {
-        float p0 = pp_at(0, row, cb), p1 = pp_at(1, row, cb);
-        float p2 = pp_at(2, row, cb);
+        real p0 = pp_at(0, row, cb), p1 = pp_at(1, row, cb);
+        real p2 = pp_at(2, row, cb);
         X.pp_0 = p0;
         X.pp_p = p1;
         X.ph_0 = gld(d.php, row, cb);
-        X.gb_0 = B3 ? gld(d.phb, row, cb) : 0.0f;
+        X.gb_0 = B3 ? gld(d.phb, row, cb) : R(0.0);
         X.dpn_0 = (d.cf1 * p0 + d.cf2 * p1) + d.cf3 * p2;
     };
     auto win_load = [&](PgfWin& X, int k, int row, unsigned cb, bool top) {
         X.ph_p = gld(d.php, (k + 1) * st + row, cb);
-        X.gb_p = B3 ? gld(d.phb, (k + 1) * st + row, cb) : 0.0f;
+        X.gb_p = B3 ? gld(d.phb, (k + 1) * st + row, cb) : R(0.0);
         const int kp = top ? nz - 1 : k + 1;
-        const float lid = s.top_lid ? d.cfn * X.pp_0 + d.cfn1 * X.pp_m : 0.0f;
-        const float mid = fnm[kp] * X.pp_p + fnp[kp] * X.pp_0;
+        const real lid = s.top_lid ? d.cfn * X.pp_0 + d.cfn1 * X.pp_m : R(0.0);
+        const real mid = fnm[kp] * X.pp_p + fnp[kp] * X.pp_0;
         X.dpn_p = top ? lid : mid;
     };
     auto win_shift = [&](PgfWin& X, int k, int row, unsigned cb) {
         X.pp_m = X.pp_0;
         X.pp_0 = X.pp_p;
-        const float np = pp_at(k + 2 < nz ? k + 2 : nz - 1, row, cb);
-        X.pp_p = (k + 2 < nz) ? np : 0.0f;
+        const real np = pp_at(k + 2 < nz ? k + 2 : nz - 1, row, cb);
+        X.pp_p = (k + 2 < nz) ? np : R(0.0);
         X.ph_0 = X.ph_p;
         X.gb_0 = X.gb_p;
         X.dpn_0 = X.dpn_p;
     };
     auto phh = [&](const PgfWin& X) {
-        if (B3) return 0.5f * ((X.gb_0 + X.gb_p) + (X.ph_0 + X.ph_p));
-        return 0.5f * (X.ph_0 + X.ph_p);
+        if (B3) return R(0.5) * ((X.gb_0 + X.gb_p) + (X.ph_0 + X.ph_p));
+        return R(0.5) * (X.ph_0 + X.ph_p);
     };
     PgfWin A, W, S;
     win_init(A, rA, cC);
     win_init(W, rA, cW);
     win_init(S, rS, cC);
 
-    float uz[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    float vz[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    real uz[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
+    real vz[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
     for (int kk = 0; kk < 3 && kk < nz; ++kk) {
         uz[2 + kk] = gld(d.u, kk * stu + j * pu, cC);
         vz[2 + kk] = gld(d.v, kk * stv + j * nx, cC);
     }
-    float wA0 = gld(d.ww, rA, cC), wW0 = gld(d.ww, rA, cW);
-    float wS0 = gld(d.ww, rS, cC);
+    real wA0 = gld(d.ww, rA, cC), wW0 = gld(d.ww, rA, cW);
+    real wS0 = gld(d.ww, rS, cC);
 
     for (int k = 0; k < nz; ++k) {
         opaque_all();
         const bool top = k + 1 == nz;
-        const float c1 = c1h[k], c2 = c2h[k];
+        const real c1 = c1h[k], c2 = c2h[k];
         const int lm = k * st;
         win_load(A, k, rA, cC, top);
         win_load(W, k, rA, cW, top);
         win_load(S, k, rS, cC, top);
-        const float wA1 = gld(d.ww, lm + st + rA, cC);
-        const float wW1 = gld(d.ww, lm + st + rA, cW);
-        const float wS1 = gld(d.ww, lm + st + rS, cC);
-        const float altA = gld(d.alt, lm + rA, cC), alA = gld(d.al, lm + rA, cC);
-        const float altW = gld(d.alt, lm + rA, cW), alW = gld(d.al, lm + rA, cW);
-        const float altS = gld(d.alt, lm + rS, cC), alS = gld(d.al, lm + rS, cC);
-        const float pbA = B3 ? gld(d.pb, lm + rA, cC) : 0.0f;
-        const float pbW = B3 ? gld(d.pb, lm + rA, cW) : 0.0f;
-        const float pbS = B3 ? gld(d.pb, lm + rS, cC) : 0.0f;
+        const real wA1 = gld(d.ww, lm + st + rA, cC);
+        const real wW1 = gld(d.ww, lm + st + rA, cW);
+        const real wS1 = gld(d.ww, lm + st + rS, cC);
+        const real altA = gld(d.alt, lm + rA, cC), alA = gld(d.al, lm + rA, cC);
+        const real altW = gld(d.alt, lm + rA, cW), alW = gld(d.al, lm + rA, cW);
+        const real altS = gld(d.alt, lm + rS, cC), alS = gld(d.al, lm + rS, cC);
+        const real pbA = B3 ? gld(d.pb, lm + rA, cC) : R(0.0);
+        const real pbW = B3 ? gld(d.pb, lm + rA, cW) : R(0.0);
+        const real pbS = B3 ? gld(d.pb, lm + rS, cC) : R(0.0);
         const int lu = k * stu, lv = k * stv;
         // ---- u face ----
         {
-            float x[7], y[7];
+            real x[7], y[7];
 #pragma unroll
             for (int o = 0; o < 7; ++o) x[o] = gld(d.u, lu + j * pu, cx[o]);
 #pragma unroll
             for (int o = 0; o < 7; ++o)
                 y[o] = (o == 3) ? x[3] : gld(d.u, lu + yo[o] * pu, cC);
-            const float u_b = gld(d.u, lu + j * pu, cub);
-            const float u_d = gld(d.u, lu + j * pu, ci1);
-            float ru_a = (c1 * mux_a + c2) * x[2];
-            float ru_b = (c1 * mux_c + c2) * u_b;
-            float ru_c = (c1 * mux_c + c2) * x[3];
-            float ru_d = (c1 * mux_d + c2) * u_d;
-            float vl = 0.5f * (ru_a + ru_b), vr = 0.5f * (ru_c + ru_d);
-            float fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], vl);
-            float fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], vr);
-            float rv_jW = (c1 * muy_jW + c2) * gld(d.v, lv + j * nx, cW);
-            float rv_jA = (c1 * muy_jA + c2) * gld(d.v, lv + j * nx, cC);
-            float rv_pW = (c1 * muy_pW + c2) * gld(d.v, lv + (j + 1) * nx, cW);
-            float rv_pA = (c1 * muy_pA + c2) * gld(d.v, lv + (j + 1) * nx, cC);
-            float v0 = 0.5f * (rv_jW + rv_jA), v1 = 0.5f * (rv_pW + rv_pA);
-            float fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], v0);
-            float fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], v1);
-            float wz0 = 0.5f * (wW0 + wA0), wz1 = 0.5f * (wW1 + wA1);
-            float fz0 = zf(uz[0], uz[1], uz[2], uz[3], wz0, k, nz, fnm, fnp);
-            float fz1 = zf(uz[1], uz[2], uz[3], uz[4], wz1, k + 1, nz, fnm, fnp);
-            float t = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy,
+            const real u_b = gld(d.u, lu + j * pu, cub);
+            const real u_d = gld(d.u, lu + j * pu, ci1);
+            real ru_a = (c1 * mux_a + c2) * x[2];
+            real ru_b = (c1 * mux_c + c2) * u_b;
+            real ru_c = (c1 * mux_c + c2) * x[3];
+            real ru_d = (c1 * mux_d + c2) * u_d;
+            real vl = R(0.5) * (ru_a + ru_b), vr = R(0.5) * (ru_c + ru_d);
+            real fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], vl);
+            real fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], vr);
+            real rv_jW = (c1 * muy_jW + c2) * gld(d.v, lv + j * nx, cW);
+            real rv_jA = (c1 * muy_jA + c2) * gld(d.v, lv + j * nx, cC);
+            real rv_pW = (c1 * muy_pW + c2) * gld(d.v, lv + (j + 1) * nx, cW);
+            real rv_pA = (c1 * muy_pA + c2) * gld(d.v, lv + (j + 1) * nx, cC);
+            real v0 = R(0.5) * (rv_jW + rv_jA), v1 = R(0.5) * (rv_pW + rv_pA);
+            real fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], v0);
+            real fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], v1);
+            real wz0 = R(0.5) * (wW0 + wA0), wz1 = R(0.5) * (wW1 + wA1);
+            real fz0 = zf(uz[0], uz[1], uz[2], uz[3], wz0, k, nz, fnm, fnp);
+            real fz1 = zf(uz[1], uz[2], uz[3], uz[4], wz1, k + 1, nz, fnm, fnp);
+            real t = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy,
                            rdnw[k]);
             t = t - pgf_val(s.rdx, s.hrdx, c1, c2, rdnw[k], mux_c, dmu_u,
                             A.ph_0, A.ph_p, W.ph_0, W.ph_p, altA, altW,
@@ -391,13 +409,13 @@ __global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
                             phh(A), phh(W), A.dpn_0, A.dpn_p, W.dpn_0,
                             W.dpn_p);
             if (DIFF) {
-                float fc = x[3];
-                float lap = s.kh * (s.dxi2 * ((x[4] - 2.0f * fc) + x[2])
-                                    + s.dyi2 * ((y[4] - 2.0f * fc) + y[2]));
-                const float gup = (uz[3] - fc) * rdzf_h[k < nz - 1 ? k : nz - 2];
-                const float gdn = (fc - uz[1]) * rdzf_h[k > 0 ? k - 1 : 0];
-                float fup = (k < nz - 1) ? gup : 0.0f;
-                float fdn = (k > 0) ? gdn : 0.0f;
+                real fc = x[3];
+                real lap = s.kh * (s.dxi2 * ((x[4] - R(2.0) * fc) + x[2])
+                                    + s.dyi2 * ((y[4] - R(2.0) * fc) + y[2]));
+                const real gup = (uz[3] - fc) * rdzf_h[k < nz - 1 ? k : nz - 2];
+                const real gdn = (fc - uz[1]) * rdzf_h[k > 0 ? k - 1 : 0];
+                real fup = (k < nz - 1) ? gup : R(0.0);
+                real fdn = (k > 0) ? gdn : R(0.0);
                 lap = lap + (s.kv * (fup - fdn)) * rdzc_h[k];
                 t = t + (c1 * msf_u + c2) * lap;
             }
@@ -406,37 +424,37 @@ __global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
                 (c1 * mtf_u + c2) * gld(d.u0, lu + j * pu, cC)
                     - (c1 * msf_u + c2) * x[3]);
             uz[0] = uz[1]; uz[1] = uz[2]; uz[2] = uz[3]; uz[3] = uz[4];
-            const float un = gld(d.u, (k + 3 < nz ? k + 3 : nz - 1) * stu + j * pu, cC);
-            uz[4] = (k + 3 < nz) ? un : 0.0f;
+            const real un = gld(d.u, (k + 3 < nz ? k + 3 : nz - 1) * stu + j * pu, cC);
+            uz[4] = (k + 3 < nz) ? un : R(0.0);
         }
         // ---- v face ----
         {
-            float x[7], y[7];
+            real x[7], y[7];
 #pragma unroll
             for (int o = 0; o < 7; ++o) y[o] = gld(d.v, lv + yo[o] * nx, cC);
 #pragma unroll
             for (int o = 0; o < 7; ++o)
                 x[o] = (o == 3) ? y[3] : gld(d.v, lv + j * nx, cx[o]);
-            float ru_jS = (c1 * mux_jS + c2) * gld(d.u, lu + jm * pu, cC);
-            float ru_jA = (c1 * mux_jA + c2) * gld(d.u, lu + j * pu, cC);
-            float ru_pS = (c1 * mux_pS + c2) * gld(d.u, lu + jm * pu, ci1);
-            float ru_pA = (c1 * mux_pA + c2) * gld(d.u, lu + j * pu, ci1);
-            float u0c = 0.5f * (ru_jS + ru_jA), u1c = 0.5f * (ru_pS + ru_pA);
-            float fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], u0c);
-            float fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], u1c);
-            const float v_b = gld(d.v, lv + jb * nx, cC);
-            const float v_d = gld(d.v, lv + (j + 1) * nx, cC);
-            float rv_a = (c1 * muy_a + c2) * y[2];
-            float rv_b = (c1 * muy_c + c2) * v_b;
-            float rv_c = (c1 * muy_c + c2) * y[3];
-            float rv_d = (c1 * muy_d + c2) * v_d;
-            float vl = 0.5f * (rv_a + rv_b), vr = 0.5f * (rv_c + rv_d);
-            float fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], vl);
-            float fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], vr);
-            float wz0 = 0.5f * (wS0 + wA0), wz1 = 0.5f * (wS1 + wA1);
-            float fz0 = zf(vz[0], vz[1], vz[2], vz[3], wz0, k, nz, fnm, fnp);
-            float fz1 = zf(vz[1], vz[2], vz[3], vz[4], wz1, k + 1, nz, fnm, fnp);
-            float t = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy,
+            real ru_jS = (c1 * mux_jS + c2) * gld(d.u, lu + jm * pu, cC);
+            real ru_jA = (c1 * mux_jA + c2) * gld(d.u, lu + j * pu, cC);
+            real ru_pS = (c1 * mux_pS + c2) * gld(d.u, lu + jm * pu, ci1);
+            real ru_pA = (c1 * mux_pA + c2) * gld(d.u, lu + j * pu, ci1);
+            real u0c = R(0.5) * (ru_jS + ru_jA), u1c = R(0.5) * (ru_pS + ru_pA);
+            real fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], u0c);
+            real fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], u1c);
+            const real v_b = gld(d.v, lv + jb * nx, cC);
+            const real v_d = gld(d.v, lv + (j + 1) * nx, cC);
+            real rv_a = (c1 * muy_a + c2) * y[2];
+            real rv_b = (c1 * muy_c + c2) * v_b;
+            real rv_c = (c1 * muy_c + c2) * y[3];
+            real rv_d = (c1 * muy_d + c2) * v_d;
+            real vl = R(0.5) * (rv_a + rv_b), vr = R(0.5) * (rv_c + rv_d);
+            real fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], vl);
+            real fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], vr);
+            real wz0 = R(0.5) * (wS0 + wA0), wz1 = R(0.5) * (wS1 + wA1);
+            real fz0 = zf(vz[0], vz[1], vz[2], vz[3], wz0, k, nz, fnm, fnp);
+            real fz1 = zf(vz[1], vz[2], vz[3], vz[4], wz1, k + 1, nz, fnm, fnp);
+            real t = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy,
                            rdnw[k]);
             t = t - pgf_val(s.rdy, s.hrdy, c1, c2, rdnw[k], muy_c, dmu_v,
                             A.ph_0, A.ph_p, S.ph_0, S.ph_p, altA, altS,
@@ -444,13 +462,13 @@ __global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
                             phh(A), phh(S), A.dpn_0, A.dpn_p, S.dpn_0,
                             S.dpn_p);
             if (DIFF) {
-                float fc = y[3];
-                float lap = s.kh * (s.dxi2 * ((x[4] - 2.0f * fc) + x[2])
-                                    + s.dyi2 * ((y[4] - 2.0f * fc) + y[2]));
-                const float gup = (vz[3] - fc) * rdzf_h[k < nz - 1 ? k : nz - 2];
-                const float gdn = (fc - vz[1]) * rdzf_h[k > 0 ? k - 1 : 0];
-                float fup = (k < nz - 1) ? gup : 0.0f;
-                float fdn = (k > 0) ? gdn : 0.0f;
+                real fc = y[3];
+                real lap = s.kh * (s.dxi2 * ((x[4] - R(2.0) * fc) + x[2])
+                                    + s.dyi2 * ((y[4] - R(2.0) * fc) + y[2]));
+                const real gup = (vz[3] - fc) * rdzf_h[k < nz - 1 ? k : nz - 2];
+                const real gdn = (fc - vz[1]) * rdzf_h[k > 0 ? k - 1 : 0];
+                real fup = (k < nz - 1) ? gup : R(0.0);
+                real fdn = (k > 0) ? gdn : R(0.0);
                 lap = lap + (s.kv * (fup - fdn)) * rdzc_h[k];
                 t = t + (c1 * msf_v + c2) * lap;
             }
@@ -459,8 +477,8 @@ __global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
                 (c1 * mtf_v + c2) * gld(d.v0, lv + j * nx, cC)
                     - (c1 * msf_v + c2) * y[3]);
             vz[0] = vz[1]; vz[1] = vz[2]; vz[2] = vz[3]; vz[3] = vz[4];
-            const float vn = gld(d.v, (k + 3 < nz ? k + 3 : nz - 1) * stv + j * nx, cC);
-            vz[4] = (k + 3 < nz) ? vn : 0.0f;
+            const real vn = gld(d.v, (k + 3 < nz ? k + 3 : nz - 1) * stv + j * nx, cC);
+            vz[4] = (k + 3 < nz) ? vn : R(0.0);
         }
         wA0 = wA1; wW0 = wW1; wS0 = wS1;
         win_shift(A, k, rA, cC);
@@ -478,17 +496,17 @@ __global__ __launch_bounds__(256) void k_tend_faces_edge(Dom d)
     const int nz = d.nz, ny = d.ny, nx = d.nx;
     const int pu = nx + 1, stu = ny * pu, stv = (ny + 1) * nx;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const float* R mub = d.mub2d;
-    const float* R mup = d.mup;
-    const float* R mup0 = d.mup0;
+    const real* RS mub = d.mub2d;
+    const real* RS mup = d.mup;
+    const real* RS mup0 = d.mup0;
     if (t < nz * ny) {
         const int k = t / ny, j = t - k * ny;
         const int cA = j * nx, cW = j * nx + nx - 1;
-        const float mA = mub[cA] + mup[cA], mW = mub[cW] + mup[cW];
-        const float mtA = mub[cA] + mup0[cA];
-        const float mtf = 0.5f * (mtA + (mub[cW] + mup0[cW]));
-        const float msf = 0.5f * (mA + mW);
-        const float c1 = d.c1h[k], c2 = d.c2h[k];
+        const real mA = mub[cA] + mup[cA], mW = mub[cW] + mup[cW];
+        const real mtA = mub[cA] + mup0[cA];
+        const real mtf = R(0.5) * (mtA + (mub[cW] + mup0[cW]));
+        const real msf = R(0.5) * (mA + mW);
+        const real c1 = d.c1h[k], c2 = d.c2h[k];
         const int f0 = k * stu + j * pu, fd = f0 + nx;
         d.ru_t[fd] = d.ru_t[f0];
         d.u_pp[fd] = (c1 * mtf + c2) * d.u0[fd] - (c1 * msf + c2) * d.u[fd];
@@ -496,11 +514,11 @@ __global__ __launch_bounds__(256) void k_tend_faces_edge(Dom d)
         const int r = t - nz * ny;
         const int k = r / nx, i = r - k * nx;
         const int cA = i, cS = (ny - 1) * nx + i;
-        const float mA = mub[cA] + mup[cA], mS = mub[cS] + mup[cS];
-        const float mtA = mub[cA] + mup0[cA];
-        const float mtf = 0.5f * (mtA + (mub[cS] + mup0[cS]));
-        const float msf = 0.5f * (mA + mS);
-        const float c1 = d.c1h[k], c2 = d.c2h[k];
+        const real mA = mub[cA] + mup[cA], mS = mub[cS] + mup[cS];
+        const real mtA = mub[cA] + mup0[cA];
+        const real mtf = R(0.5) * (mtA + (mub[cS] + mup0[cS]));
+        const real msf = R(0.5) * (mA + mS);
+        const real c1 = d.c1h[k], c2 = d.c2h[k];
         const int f0 = k * stv + i, fd = f0 + ny * nx;
         d.rv_t[fd] = d.rv_t[f0];
         d.v_pp[fd] = (c1 * mtf + c2) * d.v0[fd] - (c1 * msf + c2) * d.v[fd];
@@ -526,49 +544,49 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
     const RowSeg rs = row_seg(nx, ny);
     if (!rs.live) return;
     const int i = rs.i, j = rs.j;
-    kfloat* c1h = kview(d.c1h);
-    kfloat* c2h = kview(d.c2h);
-    kfloat* c1f = kview(d.c1f);
-    kfloat* c2f = kview(d.c2f);
-    kfloat* fnm = kview(d.fnm);
-    kfloat* fnp = kview(d.fnp);
-    kfloat* rdn = kview(d.rdn);
-    kfloat* rdnw = kview(d.rdnw);
-    kfloat* thb1 = kview(d.thb);
-    kfloat* phb1 = kview(d.phb);
-    kfloat* pb1 = kview(d.pb);
-    kfloat* rdzf_h = kview(d.rdzf_h);
-    kfloat* rdzc_h = kview(d.rdzc_h);
-    kfloat* rdzf_w = kview(d.rdzf_w);
-    kfloat* rdzc_w = kview(d.rdzc_w);
+    kreal* c1h = kview(d.c1h);
+    kreal* c2h = kview(d.c2h);
+    kreal* c1f = kview(d.c1f);
+    kreal* c2f = kview(d.c2f);
+    kreal* fnm = kview(d.fnm);
+    kreal* fnp = kview(d.fnp);
+    kreal* rdn = kview(d.rdn);
+    kreal* rdnw = kview(d.rdnw);
+    kreal* thb1 = kview(d.thb);
+    kreal* phb1 = kview(d.phb);
+    kreal* pb1 = kview(d.pb);
+    kreal* rdzf_h = kview(d.rdzf_h);
+    kreal* rdzc_h = kview(d.rdzc_h);
+    kreal* rdzf_w = kview(d.rdzf_w);
+    kreal* rdzc_w = kview(d.rdzc_w);
     int yo[7];
 #pragma unroll
     for (int o = 0; o < 7; ++o) yo[o] = pmod(j + o - 3, ny) * nx;
     const int rA = j * nx, rS = yo[2], rN = yo[4];
     unsigned cx[7];
 #pragma unroll
-    for (int o = 0; o < 7; ++o) cx[o] = 4u * pmod(i + o - 3, nx);
-    unsigned ci1 = 4u * (i + 1);
+    for (int o = 0; o < 7; ++o) cx[o] = RB * pmod(i + o - 3, nx);
+    unsigned ci1 = RB * (i + 1);
     auto opaque_all = [&]() {
 #pragma unroll
         for (int o = 0; o < 7; ++o) colb_opaque(cx[o]);
         colb_opaque(ci1);
     };
     unsigned& cC = cx[3];
-    const float* R mub = d.mub2d;
-    const float* R mup = d.mup;
+    const real* RS mub = d.mub2d;
+    const real* RS mup = d.mup;
     auto mu2 = [&](int row, unsigned cb) {
         return gld(mub, row, cb) + gld(mup, row, cb);
     };
-    const float mA = mu2(rA, cC), mW = mu2(rA, cx[2]);
-    const float mE = mu2(rA, cx[4]), mS = mu2(rS, cC), mN = mu2(rN, cC);
-    const float mux_i = 0.5f * (mA + mW), mux_ip = 0.5f * (mE + mA);
-    const float muy_j = 0.5f * (mA + mS), muy_jp = 0.5f * (mN + mA);
-    const float mupc = gld(mup, rA, cC);
-    const float mt = gld(mub, rA, cC) + gld(d.mup0, rA, cC);
-    const float mupp = gld(d.mup0, rA, cC) - mupc;
+    const real mA = mu2(rA, cC), mW = mu2(rA, cx[2]);
+    const real mE = mu2(rA, cx[4]), mS = mu2(rS, cC), mN = mu2(rN, cC);
+    const real mux_i = R(0.5) * (mA + mW), mux_ip = R(0.5) * (mE + mA);
+    const real muy_j = R(0.5) * (mA + mS), muy_jp = R(0.5) * (mN + mA);
+    const real mupc = gld(mup, rA, cC);
+    const real mt = gld(mub, rA, cC) + gld(d.mup0, rA, cC);
+    const real mupp = gld(d.mup0, rA, cC) - mupc;
     gst(d.mu_pp, rA, cC, mupp);
-    const float qx = s.qrdx, qy = s.qrdy;
+    const real qx = s.qrdx, qy = s.qrdy;
 
     auto thb_at = [&](int k, int row, unsigned cb) {
         return B3 ? gld(d.thb, k * st + row, cb) : thb1[k];
@@ -584,24 +602,24 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
     };
 
     // windows
-    float thz[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};   // theta k-2..k+2
-    float wz[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};    // w k-2..k+2
+    real thz[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};   // theta k-2..k+2
+    real wz[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};    // w k-2..k+2
     for (int kk = 0; kk < 3; ++kk) {
         if (kk < nz) thz[2 + kk] = th_at(kk, rA, cC);
         wz[2 + kk] = gld(d.w, kk * st + rA, cC);
     }
-    float ww_m = 0.0f, ww_0 = gld(d.ww, rA, cC), ww_p = gld(d.ww, st + rA, cC);
-    float ph_m = 0.0f, ph_0 = ph_at(0, rA, cC), ph_p = ph_at(1, rA, cC);
-    float pp_m = 0.0f;
-    float ru_i_m = 0.0f, ru_ip_m = 0.0f, rv_j_m = 0.0f, rv_jp_m = 0.0f;
-    float u_i_m = 0.0f, u_i_mm = 0.0f, u_ip_m = 0.0f, u_ip_mm = 0.0f;
-    float v_j_m = 0.0f, v_j_mm = 0.0f, v_jp_m = 0.0f, v_jp_mm = 0.0f;
-    float thp_m = 0.0f, thp_0 = gld(d.thp, rA, cC);
-    float thp_p = gld(d.thp, st + rA, cC);
-    float phpp_0 = gld(d.php0, rA, cC) - gld(d.php, rA, cC);
+    real ww_m = R(0.0), ww_0 = gld(d.ww, rA, cC), ww_p = gld(d.ww, st + rA, cC);
+    real ph_m = R(0.0), ph_0 = ph_at(0, rA, cC), ph_p = ph_at(1, rA, cC);
+    real pp_m = R(0.0);
+    real ru_i_m = R(0.0), ru_ip_m = R(0.0), rv_j_m = R(0.0), rv_jp_m = R(0.0);
+    real u_i_m = R(0.0), u_i_mm = R(0.0), u_ip_m = R(0.0), u_ip_mm = R(0.0);
+    real v_j_m = R(0.0), v_j_mm = R(0.0), v_jp_m = R(0.0), v_jp_mm = R(0.0);
+    real thp_m = R(0.0), thp_0 = gld(d.thp, rA, cC);
+    real thp_p = gld(d.thp, st + rA, cC);
+    real phpp_0 = gld(d.php0, rA, cC) - gld(d.php, rA, cC);
     // moist: total water (qv + qc) + qr at the level below (buoyancy)
-    float qt_m = 0.0f, qt_mm = 0.0f;
-    const float mubc = MOIST ? gld(mub, rA, cC) : 0.0f;
+    real qt_m = R(0.0), qt_mm = R(0.0);
+    const real mubc = MOIST ? gld(mub, rA, cC) : R(0.0);
 
     // kinds: 0 = surface level (w level 0 + mass level 0), 1 = interior,
     // 2 = lid (w level nz only)
@@ -613,9 +631,9 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
         opaque_all();
         const int lm = k * st;
         const int km = k < nz ? k : nz - 1;          // clamped mass level
-        float pp_0 = 0.0f;
-        float p_k = 0.0f;
-        float qt_0 = 0.0f;
+        real pp_0 = R(0.0);
+        real p_k = R(0.0);
+        real qt_0 = R(0.0);
         if (mass) {
             p_k = gld(d.p, lm + rA, cC);
             pp_0 = p_k - (B3 ? gld(d.pb, lm + rA, cC) : pb1[k]);
@@ -623,15 +641,15 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
                 qt_0 = (gld(d.qv, lm + rA, cC) + gld(d.qc, lm + rA, cC))
                        + gld(d.qr, lm + rA, cC);
         }
-        float ru_i = 0.0f, ru_ip = 0.0f, rv_j = 0.0f, rv_jp = 0.0f;
-        float u_i = 0.0f, u_ip = 0.0f, v_j = 0.0f, v_jp = 0.0f;
+        real ru_i = R(0.0), ru_ip = R(0.0), rv_j = R(0.0), rv_jp = R(0.0);
+        real u_i = R(0.0), u_ip = R(0.0), v_j = R(0.0), v_jp = R(0.0);
         if (mass) {
             const int lu = k * stu, lv = k * stv;
             u_i = gld(d.u, lu + j * pu, cC);
-            const float u_i1 = gld(d.u, lu + j * pu, ci1);
+            const real u_i1 = gld(d.u, lu + j * pu, ci1);
             u_ip = gld(d.u, lu + j * pu, cx[4]);
             v_j = gld(d.v, lv + rA, cC);
-            const float v_j1 = gld(d.v, lv + (j + 1) * nx, cC);
+            const real v_j1 = gld(d.v, lv + (j + 1) * nx, cC);
             v_jp = gld(d.v, lv + rN, cC);
             ru_i = (c1h[k] * mux_i + c2h[k]) * u_i;
             ru_ip = (c1h[k] * mux_ip + c2h[k]) * u_i1;
@@ -640,41 +658,41 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
         }
 
         // --- rw_t: advection + buoyancy + diffusion ---
-        float t = 0.0f;
-        float wx2 = 0.0f, wx4 = 0.0f, wy2 = 0.0f, wy4 = 0.0f;
+        real t = R(0.0);
+        real wx2 = R(0.0), wx4 = R(0.0), wy2 = R(0.0), wy4 = R(0.0);
         if (interior) {
-            float x[7], y[7];
+            real x[7], y[7];
 #pragma unroll
             for (int o = 0; o < 7; ++o) {
                 x[o] = (o == 3) ? wz[2] : gld(d.w, lm + rA, cx[o]);
                 y[o] = (o == 3) ? wz[2] : gld(d.w, lm + yo[o], cC);
             }
             wx2 = x[2]; wx4 = x[4]; wy2 = y[2]; wy4 = y[4];
-            float u0v = fnm[k] * ru_i + fnp[k] * ru_i_m;
-            float u1v = fnm[k] * ru_ip + fnp[k] * ru_ip_m;
-            float v0v = fnm[k] * rv_j + fnp[k] * rv_j_m;
-            float v1v = fnm[k] * rv_jp + fnp[k] * rv_jp_m;
-            float fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], u0v);
-            float fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], u1v);
-            float fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], v0v);
-            float fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], v1v);
-            float fz[2];
-            const float wwv[3] = {ww_m, ww_0, ww_p};
+            real u0v = fnm[k] * ru_i + fnp[k] * ru_i_m;
+            real u1v = fnm[k] * ru_ip + fnp[k] * ru_ip_m;
+            real v0v = fnm[k] * rv_j + fnp[k] * rv_j_m;
+            real v1v = fnm[k] * rv_jp + fnp[k] * rv_jp_m;
+            real fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], u0v);
+            real fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], u1v);
+            real fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], v0v);
+            real fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], v1v);
+            real fz[2];
+            const real wwv[3] = {ww_m, ww_0, ww_p};
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 const int m = k - 1 + q;
-                const float vel = 0.5f * (wwv[q] + wwv[q + 1]);
-                const float lin = 0.5f * vel * (wz[q + 2] + wz[q + 1]);
-                float cub = flux3(wz[q], wz[q + 1], wz[q + 2], wz[q + 3], vel);
+                const real vel = R(0.5) * (wwv[q] + wwv[q + 1]);
+                const real lin = R(0.5) * vel * (wz[q + 2] + wz[q + 1]);
+                real cub = flux3(wz[q], wz[q + 1], wz[q + 2], wz[q + 3], vel);
                 keep(cub);
                 fz[q] = (m == 0 || m == nz - 1) ? lin : cub;
             }
             t = div3(fx1 - fx0, fy1 - fy0, fz[1] - fz[0], s.rdx, s.rdy, rdn[k]);
-            float term;
+            real term;
             if (MOIST) {
                 // moist pg_buoy_w (reference dycore.cu:279-289)
-                const float cqw = 0.5f * (qt_0 + qt_m);
-                const float cq1 = 1.0f / (1.0f + cqw);
+                const real cqw = R(0.5) * (qt_0 + qt_m);
+                const real cq1 = R(1.0) / (R(1.0) + cqw);
                 term = (cq1 * rdn[k]) * (pp_0 - pp_m) - c1f[k] * mupc;
                 term = term - (cqw * cq1) * (c1f[k] * mubc + c2f[k]);
             } else {
@@ -685,26 +703,26 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
             if (MOIST) {
                 // lid: the top two layers (reference dycore.cu:258-273);
                 // qt_m holds layer nz-1, qt_mm layer nz-2
-                const float cqw = 0.5f * (qt_m + qt_mm);
-                const float cq1 = 1.0f / (1.0f + cqw);
-                float pressure = ((cq1 * 2.0f) * rdnw[nz - 1]) * (0.0f - pp_m);
-                float term = pressure - c1f[nz] * mupc;
+                const real cqw = R(0.5) * (qt_m + qt_mm);
+                const real cq1 = R(1.0) / (R(1.0) + cqw);
+                real pressure = ((cq1 * R(2.0)) * rdnw[nz - 1]) * (R(0.0) - pp_m);
+                real term = pressure - c1f[nz] * mupc;
                 term = term - (cqw * cq1) * (c1f[nz] * mubc + c2f[nz]);
                 t = t + G_F * term;
             } else {
-                float pressure = ((1.0f * 2.0f) * rdnw[nz - 1]) * (0.0f - pp_m);
-                float term = pressure - c1f[nz] * mupc;
+                real pressure = ((R(1.0) * R(2.0)) * rdnw[nz - 1]) * (R(0.0) - pp_m);
+                real term = pressure - c1f[nz] * mupc;
                 t = t + G_F * term;
             }
         }
         if (DIFF) {
-            float lap = 0.0f;
+            real lap = R(0.0);
             if (interior) {
-                float fc = wz[2];
-                lap = s.kh * (s.dxi2 * ((wx4 - 2.0f * fc) + wx2)
-                              + s.dyi2 * ((wy4 - 2.0f * fc) + wy2));
-                float fup = (wz[3] - fc) * rdzf_w[k];
-                float fdn = (fc - wz[1]) * rdzf_w[k - 1];
+                real fc = wz[2];
+                lap = s.kh * (s.dxi2 * ((wx4 - R(2.0) * fc) + wx2)
+                              + s.dyi2 * ((wy4 - R(2.0) * fc) + wy2));
+                real fup = (wz[3] - fc) * rdzf_w[k];
+                real fdn = (fc - wz[1]) * rdzf_w[k - 1];
                 lap = lap + (s.kv * (fup - fdn)) * rdzc_w[k];
             }
             t = t + (c1f[k] * mA + c2f[k]) * lap;
@@ -712,18 +730,18 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
         gst(d.rw_t, lm + rA, cC, t);
 
         // --- rph_t: WRF rhs_ph ---
-        float r = 0.0f;
+        real r = R(0.0);
         if (K != 0) {
             if (!top) {
-                float wd_lo = ((0.5f * (ww_0 + ww_m)) * rdnw[k - 1]) * (ph_0 - ph_m);
-                float wd_hi = ((0.5f * (ww_p + ww_0)) * rdnw[k]) * (ph_p - ph_0);
-                float omega = fnm[k] * wd_hi + fnp[k] * wd_lo;
+                real wd_lo = ((R(0.5) * (ww_0 + ww_m)) * rdnw[k - 1]) * (ph_0 - ph_m);
+                real wd_hi = ((R(0.5) * (ww_p + ww_0)) * rdnw[k]) * (ph_p - ph_0);
+                real omega = fnm[k] * wd_hi + fnp[k] * wd_lo;
                 r = r - omega;
             }
-            float mass_k = c1f[k] * mA + c2f[k];
+            real mass_k = c1f[k] * mA + c2f[k];
             r = r + mass_k * (G_F * wz[2]);
-            float qxk = top ? 2.0f * qx : qx, qyk = top ? 2.0f * qy : qy;
-            float wi, wip, wj, wjp;
+            real qxk = top ? R(2.0) * qx : qx, qyk = top ? R(2.0) * qy : qy;
+            real wi, wip, wj, wjp;
             if (top) {
                 wi = d.cfn * u_i_m + d.cfn1 * u_i_mm;
                 wip = d.cfn * u_ip_m + d.cfn1 * u_ip_mm;
@@ -735,16 +753,16 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
                 wj = v_j + v_j_m;
                 wjp = v_jp + v_jp_m;
             }
-            float fcx_i = (c1f[k] * mux_i + c2f[k]) * wi;
-            float fcx_ip = (c1f[k] * mux_ip + c2f[k]) * wip;
-            float fcy_j = (c1f[k] * muy_j + c2f[k]) * wj;
-            float fcy_jp = (c1f[k] * muy_jp + c2f[k]) * wjp;
-            float p0 = ph_0;
-            float fx = fcx_i * (p0 - ph_at(k, rA, cx[2]));
-            float fxp = fcx_ip * (ph_at(k, rA, cx[4]) - p0);
+            real fcx_i = (c1f[k] * mux_i + c2f[k]) * wi;
+            real fcx_ip = (c1f[k] * mux_ip + c2f[k]) * wip;
+            real fcy_j = (c1f[k] * muy_j + c2f[k]) * wj;
+            real fcy_jp = (c1f[k] * muy_jp + c2f[k]) * wjp;
+            real p0 = ph_0;
+            real fx = fcx_i * (p0 - ph_at(k, rA, cx[2]));
+            real fxp = fcx_ip * (ph_at(k, rA, cx[4]) - p0);
             r = r - qxk * (fxp + fx);
-            float fy = fcy_j * (p0 - ph_at(k, rS, cC));
-            float fyp = fcy_jp * (ph_at(k, rN, cC) - p0);
+            real fy = fcy_j * (p0 - ph_at(k, rS, cC));
+            real fyp = fcy_jp * (ph_at(k, rN, cC) - p0);
             r = r - qyk * (fyp + fy);
         }
         gst(d.rph_t, lm + rA, cC, r);
@@ -756,9 +774,9 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
         gst(d.ph_pp, lm + rA, cC, phpp_0);
 
         // --- mass level k: rth_t and the theta''/p''/alpha'' init ---
-        float phpp_p = 0.0f;
+        real phpp_p = R(0.0);
         if (mass) {
-            float qxv[7], qyv[7], tpx[7], tpy[7];
+            real qxv[7], qyv[7], tpx[7], tpy[7];
 #pragma unroll
             for (int o = 0; o < 7; ++o) {
                 if (o == 3) continue;
@@ -769,57 +787,57 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
             }
             qxv[3] = thz[2];
             qyv[3] = thz[2];
-            float fx0 = flux5(qxv[0], qxv[1], qxv[2], qxv[3], qxv[4], qxv[5], ru_i);
-            float fx1 = flux5(qxv[1], qxv[2], qxv[3], qxv[4], qxv[5], qxv[6], ru_ip);
-            float fy0 = flux5(qyv[0], qyv[1], qyv[2], qyv[3], qyv[4], qyv[5], rv_j);
-            float fy1 = flux5(qyv[1], qyv[2], qyv[3], qyv[4], qyv[5], qyv[6], rv_jp);
-            float fz0 = zf(thz[0], thz[1], thz[2], thz[3], ww_0, k, nz, fnm, fnp);
-            float fz1 = zf(thz[1], thz[2], thz[3], thz[4], ww_p, k + 1, nz, fnm, fnp);
-            float t2 = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy, rdnw[k]);
+            real fx0 = flux5(qxv[0], qxv[1], qxv[2], qxv[3], qxv[4], qxv[5], ru_i);
+            real fx1 = flux5(qxv[1], qxv[2], qxv[3], qxv[4], qxv[5], qxv[6], ru_ip);
+            real fy0 = flux5(qyv[0], qyv[1], qyv[2], qyv[3], qyv[4], qyv[5], rv_j);
+            real fy1 = flux5(qyv[1], qyv[2], qyv[3], qyv[4], qyv[5], qyv[6], rv_jp);
+            real fz0 = zf(thz[0], thz[1], thz[2], thz[3], ww_0, k, nz, fnm, fnp);
+            real fz1 = zf(thz[1], thz[2], thz[3], thz[4], ww_p, k + 1, nz, fnm, fnp);
+            real t2 = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy, rdnw[k]);
             if (DIFF) {
-                float fc = thp_0;
-                float lap = s.kh * (s.dxi2 * ((tpx[4] - 2.0f * fc) + tpx[2])
-                                    + s.dyi2 * ((tpy[4] - 2.0f * fc) + tpy[2]));
-                const float gup = (thp_p - fc) * rdzf_h[k < nz - 1 ? k : nz - 2];
-                const float gdn = (fc - thp_m) * rdzf_h[k > 0 ? k - 1 : 0];
-                float fup = (k < nz - 1) ? gup : 0.0f;
-                float fdn = (k > 0) ? gdn : 0.0f;
+                real fc = thp_0;
+                real lap = s.kh * (s.dxi2 * ((tpx[4] - R(2.0) * fc) + tpx[2])
+                                    + s.dyi2 * ((tpy[4] - R(2.0) * fc) + tpy[2]));
+                const real gup = (thp_p - fc) * rdzf_h[k < nz - 1 ? k : nz - 2];
+                const real gdn = (fc - thp_m) * rdzf_h[k > 0 ? k - 1 : 0];
+                real fup = (k < nz - 1) ? gup : R(0.0);
+                real fdn = (k > 0) ? gdn : R(0.0);
                 lap = lap + (s.kv * (fup - fdn)) * rdzc_h[k];
                 t2 = t2 + (c1h[k] * mA + c2h[k]) * lap;
             }
             gst(d.rth_t, lm + rA, cC, t2);
 
             phpp_p = gld(d.php0, lm + st + rA, cC) - gld(d.php, lm + st + rA, cC);
-            float cht = c1h[k] * mt + c2h[k];
-            float chs = c1h[k] * mA + c2h[k];
-            float thbk = thb_at(k, rA, cC);
-            float tht = thbk + gld(d.thp0, lm + rA, cC);
-            float ths = thbk + thp_0;
-            float thpp = cht * tht - chs * ths;
+            real cht = c1h[k] * mt + c2h[k];
+            real chs = c1h[k] * mA + c2h[k];
+            real thbk = thb_at(k, rA, cC);
+            real tht = thbk + gld(d.thp0, lm + rA, cC);
+            real ths = thbk + thp_0;
+            real thpp = cht * tht - chs * ths;
             gst(d.th_pp, lm + rA, cC, thpp);
-            float altk = gld(d.alt, lm + rA, cC);
-            float c2a = (GAMMA_F * p_k) / altk;
-            float al = -((altk * (c1h[k] * mupp)) + rdnw[k] * (phpp_p - phpp_0)) / cht;
+            real altk = gld(d.alt, lm + rA, cC);
+            real c2a = (GAMMA_F * p_k) / altk;
+            real al = -((altk * (c1h[k] * mupp)) + rdnw[k] * (phpp_p - phpp_0)) / cht;
             if (B3) gst(d.al_pp, lm + rA, cC, al);
-            float p5 = (altk * (thpp - (c1h[k] * mupp) * ths)) / (cht * ths);
+            real p5 = (altk * (thpp - (c1h[k] * mupp) * ths)) / (cht * ths);
             gst(d.p_pp, lm + rA, cC, c2a * (p5 - al));
         }
         (void)km;
 
         // shift windows (refills are clamped loads and selects)
         phpp_0 = phpp_p;
-        const float th_n = th_at(k + 3 < nz ? k + 3 : nz - 1, rA, cC);
+        const real th_n = th_at(k + 3 < nz ? k + 3 : nz - 1, rA, cC);
         thz[0] = thz[1]; thz[1] = thz[2]; thz[2] = thz[3]; thz[3] = thz[4];
-        thz[4] = (k + 3 < nz) ? th_n : 0.0f;
-        const float w_n = gld(d.w, (k + 3 <= nz ? k + 3 : nz) * st + rA, cC);
+        thz[4] = (k + 3 < nz) ? th_n : R(0.0);
+        const real w_n = gld(d.w, (k + 3 <= nz ? k + 3 : nz) * st + rA, cC);
         wz[0] = wz[1]; wz[1] = wz[2]; wz[2] = wz[3]; wz[3] = wz[4];
-        wz[4] = (k + 3 <= nz) ? w_n : 0.0f;
-        const float ww_n = gld(d.ww, (k + 2 <= nz ? k + 2 : nz) * st + rA, cC);
+        wz[4] = (k + 3 <= nz) ? w_n : R(0.0);
+        const real ww_n = gld(d.ww, (k + 2 <= nz ? k + 2 : nz) * st + rA, cC);
         ww_m = ww_0; ww_0 = ww_p;
-        ww_p = (k + 2 <= nz) ? ww_n : 0.0f;
-        const float ph_n = ph_at(k + 2 <= nz ? k + 2 : nz, rA, cC);
+        ww_p = (k + 2 <= nz) ? ww_n : R(0.0);
+        const real ph_n = ph_at(k + 2 <= nz ? k + 2 : nz, rA, cC);
         ph_m = ph_0; ph_0 = ph_p;
-        ph_p = (k + 2 <= nz) ? ph_n : 0.0f;
+        ph_p = (k + 2 <= nz) ? ph_n : R(0.0);
         pp_m = pp_0;
         if (MOIST) {
             qt_mm = qt_m;
@@ -828,9 +846,9 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
         ru_i_m = ru_i; ru_ip_m = ru_ip; rv_j_m = rv_j; rv_jp_m = rv_jp;
         u_i_mm = u_i_m; u_i_m = u_i; u_ip_mm = u_ip_m; u_ip_m = u_ip;
         v_j_mm = v_j_m; v_j_m = v_j; v_jp_mm = v_jp_m; v_jp_m = v_jp;
-        const float thp_n = gld(d.thp, (k + 2 < nz ? k + 2 : nz - 1) * st + rA, cC);
+        const real thp_n = gld(d.thp, (k + 2 < nz ? k + 2 : nz - 1) * st + rA, cC);
         thp_m = thp_0; thp_0 = thp_p;
-        thp_p = (k + 2 < nz) ? thp_n : 0.0f;
+        thp_p = (k + 2 < nz) ? thp_n : R(0.0);
     };
     level(0, Kind<0>{});
     for (int k = 1; k < nz; ++k) level(k, Kind<1>{});
@@ -844,14 +862,14 @@ __global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
 // Damped-pressure window of one column for the acoustic PGF.
 // Loads of one u''/v'' level, issued one level ahead.
 struct UvIn {
-    float pA = 0.0f, oA = 0.0f, pW = 0.0f, oW = 0.0f, pS = 0.0f, oS = 0.0f;
-    float phA = 0.0f, phW = 0.0f, phS = 0.0f;   // ph'' at w level k+1
-    float hpA = 0.0f, hpW = 0.0f, hpS = 0.0f;   // php at w level k+1
-    float gbA = 0.0f, gbW = 0.0f, gbS = 0.0f;   // phb at w level k+1 (terrain)
-    float altA = 0.0f, altW = 0.0f, altS = 0.0f;
-    float aA = 0.0f, aW = 0.0f, aS = 0.0f;      // al'' (terrain)
-    float bA = 0.0f, bW = 0.0f, bS = 0.0f;      // pb (terrain)
-    float u = 0.0f, ru = 0.0f, v = 0.0f, rv = 0.0f;
+    real pA = R(0.0), oA = R(0.0), pW = R(0.0), oW = R(0.0), pS = R(0.0), oS = R(0.0);
+    real phA = R(0.0), phW = R(0.0), phS = R(0.0);   // ph'' at w level k+1
+    real hpA = R(0.0), hpW = R(0.0), hpS = R(0.0);   // php at w level k+1
+    real gbA = R(0.0), gbW = R(0.0), gbS = R(0.0);   // phb at w level k+1 (terrain)
+    real altA = R(0.0), altW = R(0.0), altS = R(0.0);
+    real aA = R(0.0), aW = R(0.0), aS = R(0.0);      // al'' (terrain)
+    real bA = R(0.0), bW = R(0.0), bS = R(0.0);      // pb (terrain)
+    real u = R(0.0), ru = R(0.0), v = R(0.0), rv = R(0.0);
 };
 
 // u'' at face (j, iu) and/or v'' at face (jv, i) through all levels
@@ -867,57 +885,57 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
     const int stu = ny * (nx + 1), stv = (ny + 1) * nx;
-    const float smdiv = s.smdiv, dtau = s.dtau;
-    kfloat* c1h = kview(d.c1h);
-    kfloat* c2h = kview(d.c2h);
-    kfloat* fnm = kview(d.fnm);
-    kfloat* fnp = kview(d.fnp);
-    kfloat* rdnw = kview(d.rdnw);
+    const real smdiv = s.smdiv, dtau = s.dtau;
+    kreal* c1h = kview(d.c1h);
+    kreal* c2h = kview(d.c2h);
+    kreal* fnm = kview(d.fnm);
+    kreal* fnp = kview(d.fnp);
+    kreal* rdnw = kview(d.rdnw);
     auto opaque_all = [&]() {
         colb_opaque(bA); colb_opaque(bW); colb_opaque(bS);
         colb_opaque(bu); colb_opaque(bv);
     };
-    auto pd = [&](float a, float o) {
-        float b = USEP ? o : a;
+    auto pd = [&](real a, real o) {
+        real b = USEP ? o : a;
         return a + smdiv * (a - b);
     };
-    const float mA = gld(d.mub2d, rA, bA) + gld(d.mup, rA, bA);
-    const float muf_u = DU ? 0.5f * ((gld(d.mub2d, rA, bW) + gld(d.mup, rA, bW)) + mA)
-                           : 0.0f;
-    const float muf_v = DV ? 0.5f * ((gld(d.mub2d, rS, bS) + gld(d.mup, rS, bS)) + mA)
-                           : 0.0f;
-    const float mppA = gld(d.mu_pp, rA, bA);
-    const float dmu_u = DU ? gld(d.mu_pp, rA, bW) + mppA : 0.0f;
-    const float dmu_v = DV ? gld(d.mu_pp, rS, bS) + mppA : 0.0f;
+    const real mA = gld(d.mub2d, rA, bA) + gld(d.mup, rA, bA);
+    const real muf_u = DU ? R(0.5) * ((gld(d.mub2d, rA, bW) + gld(d.mup, rA, bW)) + mA)
+                           : R(0.0);
+    const real muf_v = DV ? R(0.5) * ((gld(d.mub2d, rS, bS) + gld(d.mup, rS, bS)) + mA)
+                           : R(0.0);
+    const real mppA = gld(d.mu_pp, rA, bA);
+    const real dmu_u = DU ? gld(d.mu_pp, rA, bW) + mppA : R(0.0);
+    const real dmu_v = DV ? gld(d.mu_pp, rS, bS) + mppA : R(0.0);
 
     // windows S(k-2), S(k-1), S(k), S(k+1) of the damped pressure
     opaque_all();
-    float pa[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pw[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float ps[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    real pa[4] = {R(0.0), R(0.0), R(0.0), R(0.0)}, pw[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
+    real ps[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
     pa[2] = pd(gld(d.p_pp, 0 + rA, bA), gld(d.p_pp_old, 0 + rA, bA));
     pa[3] = pd(gld(d.p_pp, st + rA, bA), gld(d.p_pp_old, st + rA, bA));
-    const float pa2 = pd(gld(d.p_pp, 2 * st + rA, bA), gld(d.p_pp_old, 2 * st + rA, bA));
-    float dnu_0 = 0.0f, dnv_0 = 0.0f;
+    const real pa2 = pd(gld(d.p_pp, 2 * st + rA, bA), gld(d.p_pp_old, 2 * st + rA, bA));
+    real dnu_0 = R(0.0), dnv_0 = R(0.0);
     if (DU) {
         pw[2] = pd(gld(d.p_pp, 0 + rA, bW), gld(d.p_pp_old, 0 + rA, bW));
         pw[3] = pd(gld(d.p_pp, st + rA, bW), gld(d.p_pp_old, st + rA, bW));
-        const float pw2 = pd(gld(d.p_pp, 2 * st + rA, bW), gld(d.p_pp_old, 2 * st + rA, bW));
-        float S0 = pa[2] + pw[2], S1 = pa[3] + pw[3], S2 = pa2 + pw2;
-        dnu_0 = 0.5f * ((d.cf1 * S0 + d.cf2 * S1) + d.cf3 * S2);
+        const real pw2 = pd(gld(d.p_pp, 2 * st + rA, bW), gld(d.p_pp_old, 2 * st + rA, bW));
+        real S0 = pa[2] + pw[2], S1 = pa[3] + pw[3], S2 = pa2 + pw2;
+        dnu_0 = R(0.5) * ((d.cf1 * S0 + d.cf2 * S1) + d.cf3 * S2);
     }
     if (DV) {
         ps[2] = pd(gld(d.p_pp, 0 + rS, bS), gld(d.p_pp_old, 0 + rS, bS));
         ps[3] = pd(gld(d.p_pp, st + rS, bS), gld(d.p_pp_old, st + rS, bS));
-        const float ps2 = pd(gld(d.p_pp, 2 * st + rS, bS), gld(d.p_pp_old, 2 * st + rS, bS));
-        float S0 = pa[2] + ps[2], S1 = pa[3] + ps[3], S2 = pa2 + ps2;
-        dnv_0 = 0.5f * ((d.cf1 * S0 + d.cf2 * S1) + d.cf3 * S2);
+        const real ps2 = pd(gld(d.p_pp, 2 * st + rS, bS), gld(d.p_pp_old, 2 * st + rS, bS));
+        real S0 = pa[2] + ps[2], S1 = pa[3] + ps[3], S2 = pa2 + ps2;
+        dnv_0 = R(0.5) * ((d.cf1 * S0 + d.cf2 * S1) + d.cf3 * S2);
     }
-    float phA0 = gld(d.ph_pp, 0 + rA, bA), hpA0 = gld(d.php, 0 + rA, bA);
-    float phW0 = DU ? gld(d.ph_pp, 0 + rA, bW) : 0.0f, hpW0 = DU ? gld(d.php, 0 + rA, bW) : 0.0f;
-    float phS0 = DV ? gld(d.ph_pp, 0 + rS, bS) : 0.0f, hpS0 = DV ? gld(d.php, 0 + rS, bS) : 0.0f;
-    float gbA0 = B3 ? gld(d.phb, 0 + rA, bA) : 0.0f;
-    float gbW0 = (B3 && DU) ? gld(d.phb, 0 + rA, bW) : 0.0f;
-    float gbS0 = (B3 && DV) ? gld(d.phb, 0 + rS, bS) : 0.0f;
+    real phA0 = gld(d.ph_pp, 0 + rA, bA), hpA0 = gld(d.php, 0 + rA, bA);
+    real phW0 = DU ? gld(d.ph_pp, 0 + rA, bW) : R(0.0), hpW0 = DU ? gld(d.php, 0 + rA, bW) : R(0.0);
+    real phS0 = DV ? gld(d.ph_pp, 0 + rS, bS) : R(0.0), hpS0 = DV ? gld(d.php, 0 + rS, bS) : R(0.0);
+    real gbA0 = B3 ? gld(d.phb, 0 + rA, bA) : R(0.0);
+    real gbW0 = (B3 && DU) ? gld(d.phb, 0 + rA, bW) : R(0.0);
+    real gbS0 = (B3 && DV) ? gld(d.phb, 0 + rS, bS) : R(0.0);
 
     auto load = [&](UvIn& L, int k) {
         const unsigned lp = (k + 2 < nz ? k + 2 : nz - 1) * st;
@@ -969,55 +987,55 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
         opaque_all();
         const UvIn L = nxt;
         if (!LAST) load(nxt, k + 1);
-        const float c1 = c1h[k], c2 = c2h[k];
-        const float phA1 = L.phA, hpA1 = L.hpA, gbA1 = L.gbA;
-        const float altA = L.altA;
-        auto dn_top = [&](float Sk, float Sm1, float Sm2) {
-            return s.top_lid ? 0.5f * ((d.cf1 * Sk + d.cf2 * Sm1) + d.cf3 * Sm2)
-                             : 0.0f;
+        const real c1 = c1h[k], c2 = c2h[k];
+        const real phA1 = L.phA, hpA1 = L.hpA, gbA1 = L.gbA;
+        const real altA = L.altA;
+        auto dn_top = [&](real Sk, real Sm1, real Sm2) {
+            return s.top_lid ? R(0.5) * ((d.cf1 * Sk + d.cf2 * Sm1) + d.cf3 * Sm2)
+                             : R(0.0);
         };
         if (DU) {
-            float Sk = pa[2] + pw[2];
-            float dn_up;
+            real Sk = pa[2] + pw[2];
+            real dn_up;
             if (LAST) {
                 dn_up = dn_top(Sk, pa[1] + pw[1], pa[0] + pw[0]);
             } else {
-                float Sp = pa[3] + pw[3];
-                dn_up = 0.5f * (fnm[k + 1] * Sp + fnp[k + 1] * Sk);
+                real Sp = pa[3] + pw[3];
+                dn_up = R(0.5) * (fnm[k + 1] * Sp + fnp[k + 1] * Sk);
             }
-            float dph = (phA1 - L.phW) + (phA0 - phW0);
-            float dpe = pa[2] - pw[2];
-            float brk = dph + (altA + L.altW) * dpe;
+            real dph = (phA1 - L.phW) + (phA0 - phW0);
+            real dpe = pa[2] - pw[2];
+            real brk = dph + (altA + L.altW) * dpe;
             if (B3) brk = brk + (L.aA + L.aW) * (L.bA - L.bW);
-            float dpxy = ((0.5f * s.rdx) * (c1 * muf_u + c2)) * brk;
-            float phsA = hpA0 + hpA1, phsB = hpW0 + L.hpW;
-            float dphp = 0.5f * (phsA - phsB);
-            if (B3) dphp = dphp + 0.5f * ((gbA0 + gbA1) - (gbW0 + L.gbW));
+            real dpxy = ((R(0.5) * s.rdx) * (c1 * muf_u + c2)) * brk;
+            real phsA = hpA0 + hpA1, phsB = hpW0 + L.hpW;
+            real dphp = R(0.5) * (phsA - phsB);
+            if (B3) dphp = dphp + R(0.5) * ((gbA0 + gbA1) - (gbW0 + L.gbW));
             dpxy = dpxy + (s.rdx * dphp) * (rdnw[k] * (dn_up - dnu_0)
-                                            - (0.5f * c1) * dmu_u);
+                                            - (R(0.5) * c1) * dmu_u);
             gst(d.u_pp, k * stu + rU, bu, L.u + dtau * (L.ru - dpxy));
             dnu_0 = dn_up;
             phW0 = L.phW; hpW0 = L.hpW; gbW0 = L.gbW;
         }
         if (DV) {
-            float Sk = pa[2] + ps[2];
-            float dn_up;
+            real Sk = pa[2] + ps[2];
+            real dn_up;
             if (LAST) {
                 dn_up = dn_top(Sk, pa[1] + ps[1], pa[0] + ps[0]);
             } else {
-                float Sp = pa[3] + ps[3];
-                dn_up = 0.5f * (fnm[k + 1] * Sp + fnp[k + 1] * Sk);
+                real Sp = pa[3] + ps[3];
+                dn_up = R(0.5) * (fnm[k + 1] * Sp + fnp[k + 1] * Sk);
             }
-            float dph = (phA1 - L.phS) + (phA0 - phS0);
-            float dpe = pa[2] - ps[2];
-            float brk = dph + (altA + L.altS) * dpe;
+            real dph = (phA1 - L.phS) + (phA0 - phS0);
+            real dpe = pa[2] - ps[2];
+            real brk = dph + (altA + L.altS) * dpe;
             if (B3) brk = brk + (L.aA + L.aS) * (L.bA - L.bS);
-            float dpxy = ((0.5f * s.rdy) * (c1 * muf_v + c2)) * brk;
-            float phsA = hpA0 + hpA1, phsB = hpS0 + L.hpS;
-            float dphp = 0.5f * (phsA - phsB);
-            if (B3) dphp = dphp + 0.5f * ((gbA0 + gbA1) - (gbS0 + L.gbS));
+            real dpxy = ((R(0.5) * s.rdy) * (c1 * muf_v + c2)) * brk;
+            real phsA = hpA0 + hpA1, phsB = hpS0 + L.hpS;
+            real dphp = R(0.5) * (phsA - phsB);
+            if (B3) dphp = dphp + R(0.5) * ((gbA0 + gbA1) - (gbS0 + L.gbS));
             dpxy = dpxy + (s.rdy * dphp) * (rdnw[k] * (dn_up - dnv_0)
-                                            - (0.5f * c1) * dmu_v);
+                                            - (R(0.5) * c1) * dmu_v);
             gst(d.v_pp, k * stv + rA, bv, L.v + dtau * (L.rv - dpxy));
             dnv_0 = dn_up;
             phS0 = L.phS; hpS0 = L.hpS; gbS0 = L.gbS;
@@ -1026,14 +1044,14 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
         // advance the damped-pressure windows to k+1 (level k+2 enters)
         const bool in = k + 2 < nz;
         pa[0] = pa[1]; pa[1] = pa[2]; pa[2] = pa[3];
-        pa[3] = in ? pd(L.pA, L.oA) : 0.0f;
+        pa[3] = in ? pd(L.pA, L.oA) : R(0.0);
         if (DU) {
             pw[0] = pw[1]; pw[1] = pw[2]; pw[2] = pw[3];
-            pw[3] = in ? pd(L.pW, L.oW) : 0.0f;
+            pw[3] = in ? pd(L.pW, L.oW) : R(0.0);
         }
         if (DV) {
             ps[0] = ps[1]; ps[1] = ps[2]; ps[2] = ps[3];
-            ps[3] = in ? pd(L.pS, L.oS) : 0.0f;
+            ps[3] = in ? pd(L.pS, L.oS) : R(0.0);
         }
     };
     for (int k = 0; k < nz - 1; ++k) level(k, std::false_type{});
@@ -1054,13 +1072,13 @@ __global__ __launch_bounds__(256) void k_uv(Dom d, Sc s, int ne)
         if (t < ny) {
             const int j = t;
             uv_march<B3, USEP, true, false>(
-                d, s, 0, 0, 0, 4u * (j * nx), 4u * (j * nx + nx - 1), 0u,
-                4u * (j * pu + nx), 0u);
+                d, s, 0, 0, 0, RB * (j * nx), RB * (j * nx + nx - 1), 0u,
+                RB * (j * pu + nx), 0u);
         } else if (t < ny + nx) {
             const int i = t - ny;
             uv_march<B3, USEP, false, true>(
-                d, s, 0, 0, 0, 4u * i, 0u, 4u * ((ny - 1) * nx + i), 0u,
-                4u * (ny * nx + i));
+                d, s, 0, 0, 0, RB * i, 0u, RB * ((ny - 1) * nx + i), 0u,
+                RB * (ny * nx + i));
         }
         return;
     }
@@ -1068,7 +1086,7 @@ __global__ __launch_bounds__(256) void k_uv(Dom d, Sc s, int ne)
     if (!rs.live) return;
     const int i = rs.i, j = rs.j;
     const int jm = (j == 0) ? ny - 1 : j - 1;
-    const unsigned bc = 4u * i, bw = 4u * ((i == 0) ? nx - 1 : i - 1);
+    const unsigned bc = RB * i, bw = RB * ((i == 0) ? nx - 1 : i - 1);
     uv_march<B3, USEP, true, true>(d, s, j * nx, jm * nx, j * pu, bc, bw, bc,
                                    bc, bc);
 }
@@ -1090,30 +1108,31 @@ __global__ __launch_bounds__(256) void k_uv(Dom d, Sc s, int ne)
 // the iteration that consumes them, so the loads of level m+1 stay in
 // flight while level m computes).
 struct ColUp {
-    float ui = 0.0f, uip = 0.0f;   // u'' at faces i, i+1
-    float vj = 0.0f, vjp = 0.0f;   // v'' at faces j, j+1
-    float te = 0.0f, tw = 0.0f, tn = 0.0f, ts = 0.0f;  // thp, 4 neighbours
-    float be = 0.0f, bw = 0.0f, bn = 0.0f, bs = 0.0f;  // thb there (terrain)
-    float thu = 0.0f, bhu = 0.0f;  // thp, thb at (m+1, c)
-    float thpp = 0.0f, rth = 0.0f; // theta'' and its tendency
-    float phu = 0.0f, ghu = 0.0f;  // php, phb at w level m+1
-    float alt = 0.0f, p = 0.0f;    // stage reference alt, p
-    float phpp = 0.0f, wpp = 0.0f, rph = 0.0f;  // at w level m
-    float rw = 0.0f;               // w tendency at w level m-1
+    real ui = R(0.0), uip = R(0.0);   // u'' at faces i, i+1
+    real vj = R(0.0), vjp = R(0.0);   // v'' at faces j, j+1
+    real te = R(0.0), tw = R(0.0), tn = R(0.0), ts = R(0.0);  // thp, 4 neighbours
+    real be = R(0.0), bw = R(0.0), bn = R(0.0), bs = R(0.0);  // thb there (terrain)
+    real thu = R(0.0), bhu = R(0.0);  // thp, thb at (m+1, c)
+    real thpp = R(0.0), rth = R(0.0); // theta'' and its tendency
+    real phu = R(0.0), ghu = R(0.0);  // php, phb at w level m+1
+    real alt = R(0.0), p = R(0.0);    // stage reference alt, p
+    real phpp = R(0.0), wpp = R(0.0), rph = R(0.0);  // at w level m
+    real rw = R(0.0);               // w tendency at w level m-1
 };
 
 // Inputs of downward w level k (and mass level k when k < nz).
 struct ColDn {
-    float wpp = 0.0f, gam = 0.0f, w = 0.0f, php = 0.0f, phb = 0.0f;
-    float phpp = 0.0f, thp = 0.0f, thb = 0.0f, alt = 0.0f, p = 0.0f;
-    float thpp = 0.0f;
+    real wpp = R(0.0), gam = R(0.0), w = R(0.0), php = R(0.0), phb = R(0.0);
+    real phpp = R(0.0), thp = R(0.0), thb = R(0.0), alt = R(0.0), p = R(0.0);
+    real thpp = R(0.0);
+    double wpp64 = 0.0, gam64 = 0.0;      // precision form
 };
 
 
 // B3: terrain (3-D) base state; DAMP: damp_opt = 3 w damper; AL: store
 // alpha''; WW: store Omega'' (standalone substep API).  nz >= 3.
-template <bool B3, bool DAMP, bool AL, bool WW>
-__global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout)
+template <bool B3, bool DAMP, bool AL, bool WW, bool PREC>
+__global__ __launch_bounds__(256) void k_column(Dom d, Sc s, real* RS pout)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
     const int stu = ny * (nx + 1), stv = (ny + 1) * nx;
@@ -1122,66 +1141,74 @@ __global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout)
     const int j = c / nx, i = c - j * nx;
     const int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
     const int jp = (j + 1 == ny) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
-    const float eps = s.epssm, dtau = s.dtau;
-    const float one_p = 1.0f + eps, one_m = 1.0f - eps;
-    kfloat* c1h = kview(d.c1h);
-    kfloat* c2h = kview(d.c2h);
-    kfloat* c1f = kview(d.c1f);
-    kfloat* c2f = kview(d.c2f);
-    kfloat* dnw = kview(d.dnw);
-    kfloat* rdnw = kview(d.rdnw);
-    kfloat* rdn = kview(d.rdn);
-    kfloat* fnm = kview(d.fnm);
-    kfloat* fnp = kview(d.fnp);
+    const real eps = s.epssm, dtau = s.dtau;
+    const real one_p = R(1.0) + eps, one_m = R(1.0) - eps;
+    kreal* c1h = kview(d.c1h);
+    kreal* c2h = kview(d.c2h);
+    kreal* c1f = kview(d.c1f);
+    kreal* c2f = kview(d.c2f);
+    kreal* dnw = kview(d.dnw);
+    kreal* rdnw = kview(d.rdnw);
+    kreal* rdn = kview(d.rdn);
+    kreal* fnm = kview(d.fnm);
+    kreal* fnp = kview(d.fnp);
     // per-thread column byte offsets
-    unsigned bc = 4u * c;
-    unsigned bE = 4u * (j * nx + ip), bW = 4u * (j * nx + im);
-    unsigned bN = 4u * (jp * nx + i), bS = 4u * (jm * nx + i);
-    unsigned bu = 4u * (j * (nx + 1) + i), bu1 = bu + 4u;
-    unsigned bvn = 4u * (c + nx);
+    unsigned bc = RB * c;
+    unsigned bE = RB * (j * nx + ip), bW = RB * (j * nx + im);
+    unsigned bN = RB * (jp * nx + i), bS = RB * (jm * nx + i);
+    unsigned bu = RB * (j * (nx + 1) + i), bu1 = bu + RB;
+    unsigned bvn = RB * (c + nx);
     auto opaque_all = [&]() {
         colb_opaque(bc); colb_opaque(bE); colb_opaque(bW);
         colb_opaque(bN); colb_opaque(bS); colb_opaque(bu);
         colb_opaque(bu1); colb_opaque(bvn);
     };
 
-    const float mut = d.mub2d[c] + d.mup[c];
-    const float mupp_old = d.mu_pp[c];
-    float dmdt_pp = 0.0f;
+    const real mut = d.mub2d[c] + d.mup[c];
+    const real mupp_old = d.mu_pp[c];
+    real dmdt_pp = R(0.0);
+    Kahan kdm;
 #pragma unroll 5
     for (int k = 0; k < nz; ++k) {
         opaque_all();
-        const float a = gld(d.u_pp, k * stu, bu1), b = gld(d.u_pp, k * stu, bu);
-        const float e = gld(d.v_pp, k * stv, bvn), f = gld(d.v_pp, k * stv, bc);
-        dmdt_pp = dmdt_pp + dnw[k] * (s.rdx * (a - b) + s.rdy * (e - f));
+        const real a = gld(d.u_pp, k * stu, bu1), b = gld(d.u_pp, k * stu, bu);
+        const real e = gld(d.v_pp, k * stv, bvn), f = gld(d.v_pp, k * stv, bc);
+        const real x = dnw[k] * (s.rdx * (a - b) + s.rdy * (e - f));
+        if (PREC) kdm.add(x);
+        else dmdt_pp = dmdt_pp + x;
     }
-    const float rmu = d.rmu_t[c];
-    const float mupp = mupp_old + dtau * ((dmdt_pp + d.dmdt_rf[c]) + rmu);
+    if (PREC) dmdt_pp = kdm.s;
+    // the tridiagonal's type: FP64 in the precision form
+    typedef typename std::conditional<PREC, double, real>::type tri;
+    const int stt = (nz + 1) * st;             // tri64: gamma, then w''
+
+    const real rmu = d.rmu_t[c];
+    const real mupp = mupp_old + dtau * ((dmdt_pp + d.dmdt_rf[c]) + rmu);
     d.mu_pp[c] = mupp;
-    const float muts = mut + mupp;
-    const float muave = 0.5f * (one_p * mupp + one_m * mupp_old);
-    float cof = 0.5f * dtau * G_F * one_p;
+    const real muts = mut + mupp;
+    const real muave = R(0.5) * (one_p * mupp + one_m * mupp_old);
+    real cof = R(0.5) * dtau * G_F * one_p;
     cof = cof * cof;
     auto cfm_at = [&](int kk) { return c1f[kk] * mut + c2f[kk]; };
-    kfloat* thb1 = kview(d.thb);     // 1-D base profiles (flat base)
-    kfloat* phb1 = kview(d.phb);
-    auto thbv = [&](float loaded, int kk) { return B3 ? loaded : thb1[kk]; };
-    auto phbv = [&](float loaded, int kk) { return B3 ? loaded : phb1[kk]; };
+    kreal* thb1 = kview(d.thb);     // 1-D base profiles (flat base)
+    kreal* phb1 = kview(d.phb);
+    auto thbv = [&](real loaded, int kk) { return B3 ? loaded : thb1[kk]; };
+    auto phbv = [&](real loaded, int kk) { return B3 ? loaded : phb1[kk]; };
 
     // kinematic lower boundary on w'' from the updated u''/v''
-    float w0;
+    real w0;
     {
         opaque_all();
-        auto lw = [&](const float* a, unsigned colb, int sl) {
+        auto lw = [&](const real* a, unsigned colb, int sl) {
             return (d.cf1 * gld(a, 0, colb) + d.cf2 * gld(a, sl, colb))
                    + d.cf3 * gld(a, 2 * sl, colb);
         };
-        float ue = lw(d.u_pp, bu1, stu), uw = lw(d.u_pp, bu, stu);
-        float vn = lw(d.v_pp, bvn, stv), vs = lw(d.v_pp, bc, stv);
-        const float* R ht = d.ht;
-        float h = ht[c];
-        w0 = (0.5f * s.rdy) * ((ht[jp * nx + i] - h) * vn + (h - ht[jm * nx + i]) * vs)
-             + (0.5f * s.rdx) * ((ht[j * nx + ip] - h) * ue + (h - ht[j * nx + im]) * uw);
+        real ue = lw(d.u_pp, bu1, stu), uw = lw(d.u_pp, bu, stu);
+        real vn = lw(d.v_pp, bvn, stv), vs = lw(d.v_pp, bc, stv);
+        const real* RS ht = d.ht;
+        real h = ht[c];
+        w0 = (R(0.5) * s.rdy) * ((ht[jp * nx + i] - h) * vn + (h - ht[jm * nx + i]) * vs)
+             + (R(0.5) * s.rdx) * ((ht[j * nx + ip] - h) * ue + (h - ht[j * nx + im]) * uw);
         gst(d.w_pp, 0, bc, w0);
     }
 
@@ -1221,51 +1248,59 @@ __global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout)
     };
 
     // upward sweep
-    if (WW) gst(d.ww_pp, 0, bc, 0.0f);
-    float ww_k = 0.0f, wwk = 0.0f, wdtn_k = 0.0f;
-    float th_m = (B3 ? gld(d.thb, 0, bc) : thb1[0]) + gld(d.thp, 0, bc);
-    float ph_m = (B3 ? gld(d.phb, 0, bc) : phb1[0]) + gld(d.php, 0, bc);
-    float wd_prev = 0.0f;
-    float rhs_a = 0.0f, rhs_b = 0.0f;
-    float pho_a = 0.0f, pho_b = gld(d.ph_pp, 0, bc);
-    float t2_a = 0.0f, t2_b = 0.0f;
-    float c2a_a = 0.0f, c2a_b = 0.0f;
-    float alt_a = 0.0f, alt_b = 0.0f;
-    float chm_a = 0.0f, chm_b = 0.0f;
-    float wold_b = 0.0f;
-    float wf_prev = w0;
-    float gam_prev = 0.0f;
-    const float ghalf_eps = (0.5f * G_F) * one_m;
-    const float c_force = (0.5f * dtau) * G_F;
-    const float c_buoy = dtau * G_F;
+    if (WW) gst(d.ww_pp, 0, bc, R(0.0));
+    real ww_k = R(0.0), wwk = R(0.0), wdtn_k = R(0.0);
+    Kahan kww;
+    real th_m = (B3 ? gld(d.thb, 0, bc) : thb1[0]) + gld(d.thp, 0, bc);
+    real ph_m = (B3 ? gld(d.phb, 0, bc) : phb1[0]) + gld(d.php, 0, bc);
+    real wd_prev = R(0.0);
+    real rhs_a = R(0.0), rhs_b = R(0.0);
+    real pho_a = R(0.0), pho_b = gld(d.ph_pp, 0, bc);
+    real t2_a = R(0.0), t2_b = R(0.0);
+    real c2a_a = R(0.0), c2a_b = R(0.0);
+    real alt_a = R(0.0), alt_b = R(0.0);
+    real chm_a = R(0.0), chm_b = R(0.0);
+    real wold_b = R(0.0);
+    tri wf_prev = (tri)w0;
+    tri gam_prev = (tri)R(0.0);
+    const real ghalf_eps = (R(0.5) * G_F) * one_m;
+    const real c_force = (R(0.5) * dtau) * G_F;
+    const real c_buoy = dtau * G_F;
 
-    auto force_row = [&](int r, float rwr, float rhs_dn, float rhs_c,
-                         float rhs_up, float ph_dn, float ph_c, float ph_up,
-                         float wold, float t2dn, float t2c, float c2dn,
-                         float c2c, float altdn, float altc, float chmdn,
-                         float chmc) {
-        float dph_up = one_p * (rhs_up - rhs_c) + one_m * (ph_up - ph_c);
-        float dph_dn = one_p * (rhs_c - rhs_dn) + one_m * (ph_c - ph_dn);
-        float t1 = dtau * rwr
+    auto force_row = [&](int r, real rwr, real rhs_dn, real rhs_c,
+                         real rhs_up, real ph_dn, real ph_c, real ph_up,
+                         real wold, real t2dn, real t2c, real c2dn,
+                         real c2c, real altdn, real altc, real chmdn,
+                         real chmc) {
+        real dph_up = one_p * (rhs_up - rhs_c) + one_m * (ph_up - ph_c);
+        real dph_dn = one_p * (rhs_c - rhs_dn) + one_m * (ph_c - ph_dn);
+        real t1 = dtau * rwr
                    + (c_force * rdn[r])
                          * ((c2c * rdnw[r]) / chmc * dph_up
                             - (c2dn * rdnw[r - 1]) / chmdn * dph_dn);
-        float t3 = c_buoy * (rdn[r] * ((c2c * altc) * t2c - (c2dn * altdn) * t2dn)
+        real t3 = c_buoy * (rdn[r] * ((c2c * altc) * t2c - (c2dn * altdn) * t2dn)
                              - c1f[r] * muave);
         return wold + (t1 + t3);
     };
     // LU factors of row r (1 <= r <= nz-1) from the windowed c2a/chm
-    auto lu_row = [&](int r, float c2dn, float c2c, float chmdn, float chmc,
-                      float& alpha_r, bool r_ge2) {
-        float a_r = 0.0f;
+    auto lu_row = [&](int r, real c2dn, real c2c, real chmdn, real chmc,
+                      tri& alpha_r, bool r_ge2) {
+        const tri tcof = cof, rdn_r = rdn[r], rdnw_r = rdnw[r];
+        const tri rdnw_m = rdnw[r - 1];
+        tri a_r = (tri)R(0.0);
         if (r_ge2)
-            a_r = -cof * rdn[r] * rdnw[r - 1] * c2dn / (chmdn * cfm_at(r - 1));
-        float b = 1.0f + cof * rdn[r] * (rdnw[r] * c2c / (chmc * cfm_at(r))
-                                         + rdnw[r - 1] * c2dn / (chmdn * cfm_at(r)));
-        float cc = -cof * rdn[r] * rdnw[r] * c2c / (chmc * cfm_at(r + 1));
-        alpha_r = 1.0f / (b - a_r * gam_prev);
+            a_r = -tcof * rdn_r * rdnw_m * (tri)c2dn
+                  / ((tri)chmdn * (tri)cfm_at(r - 1));
+        tri b = (tri)R(1.0) + tcof * rdn_r
+                                  * (rdnw_r * (tri)c2c / ((tri)chmc * (tri)cfm_at(r))
+                                     + rdnw_m * (tri)c2dn
+                                           / ((tri)chmdn * (tri)cfm_at(r)));
+        tri cc = -tcof * rdn_r * rdnw_r * (tri)c2c
+                 / ((tri)chmc * (tri)cfm_at(r + 1));
+        alpha_r = (tri)R(1.0) / (b - a_r * gam_prev);
         gam_prev = cc * alpha_r;
-        gst(d.gam, r * st, bc, gam_prev);
+        if (PREC) d.tri64[r * st + c] = (double)gam_prev;
+        else gst(d.gam, r * st, bc, (real)gam_prev);
         return a_r;
     };
 
@@ -1283,53 +1318,60 @@ __global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout)
         const ColUp L = nxt;
         load_up(nxt, m + 1);
         const unsigned ms = m * st;
-        float dvp = s.rdx * (L.uip - L.ui) + s.rdy * (L.vjp - L.vj);
-        float ww_kp1;
+        real dvp = s.rdx * (L.uip - L.ui) + s.rdy * (L.vjp - L.vj);
+        real ww_kp1;
         if (has_up) {
-            wwk = wwk - dnw[m] * (c1h[m] * (dmdt_pp + rmu) + dvp);
+            const real inc = dnw[m] * (c1h[m] * (dmdt_pp + rmu) + dvp);
+            if (PREC) {
+                kww.add(-inc);
+                wwk = kww.s;
+            } else {
+                wwk = wwk - inc;
+            }
             ww_kp1 = wwk;
         } else {
-            ww_kp1 = 0.0f;
+            ww_kp1 = R(0.0);
         }
         if (WW) gst(d.ww_pp, ms + st, bc, ww_kp1);
-        float th_up = has_up ? thbv(L.bhu, m + 1) + L.thu : 0.0f;
-        float wdtn_kp1 = has_up
-            ? ww_kp1 * (fnm[m + 1] * th_up + fnp[m + 1] * th_m) : 0.0f;
-        float t_c = th_m;
-        float hx = (0.5f * s.rdx) * (L.uip * (thbv(L.be, m) + L.te + t_c)
+        real th_up = has_up ? thbv(L.bhu, m + 1) + L.thu : R(0.0);
+        real wdtn_kp1 = has_up
+            ? ww_kp1 * (fnm[m + 1] * th_up + fnp[m + 1] * th_m) : R(0.0);
+        real t_c = th_m;
+        real hx = (R(0.5) * s.rdx) * (L.uip * (thbv(L.be, m) + L.te + t_c)
                                      - L.ui * (t_c + (thbv(L.bw, m) + L.tw)));
-        float hy = (0.5f * s.rdy) * (L.vjp * (thbv(L.bn, m) + L.tn + t_c)
+        real hy = (R(0.5) * s.rdy) * (L.vjp * (thbv(L.bn, m) + L.tn + t_c)
                                      - L.vj * (t_c + (thbv(L.bs, m) + L.ts)));
-        float tho = L.thpp;
-        float thn = tho + dtau * (L.rth - ((hx + hy) + rdnw[m] * (wdtn_kp1 - wdtn_k)));
+        real tho = L.thpp;
+        real thn = tho + dtau * (L.rth - ((hx + hy) + rdnw[m] * (wdtn_kp1 - wdtn_k)));
         gst(d.th_pp, ms, bc, thn);
-        float chs = c1h[m] * muts + c2h[m];
-        float t2 = (0.5f * (one_p * thn + one_m * tho)) / (chs * t_c);
-        float ph_up = phbv(L.ghu, m + 1) + L.phu;
-        float wd = ((0.5f * (ww_kp1 + ww_k)) * rdnw[m]) * (ph_up - ph_m);
-        float altm = L.alt;
-        float c2a = GAMMA_F * L.p / altm;
-        float chm = c1h[m] * mut + c2h[m];
-        float rhs_c = 0.0f, pho_c = pho_b, wold_c = 0.0f;
+        real chs = c1h[m] * muts + c2h[m];
+        real t2 = (R(0.5) * (one_p * thn + one_m * tho)) / (chs * t_c);
+        real ph_up = phbv(L.ghu, m + 1) + L.phu;
+        real wd = ((R(0.5) * (ww_kp1 + ww_k)) * rdnw[m]) * (ph_up - ph_m);
+        real altm = L.alt;
+        real c2a = GAMMA_F * L.p / altm;
+        real chm = c1h[m] * mut + c2h[m];
+        real rhs_c = R(0.0), pho_c = pho_b, wold_c = R(0.0);
         if (has_rhs) {
             pho_c = L.phpp;
             wold_c = L.wpp;
-            float r = dtau * (L.rph + ghalf_eps * wold_c);
+            real r = dtau * (L.rph + ghalf_eps * wold_c);
             r = r - dtau * (fnm[m] * wd + fnp[m] * wd_prev);
             rhs_c = pho_c + r / cfm_at(m);
             gst(d.ph_pp, ms, bc, rhs_c);
         }
         if (has_row) {
             const int r = m - 1;
-            float alpha_r;
+            tri alpha_r;
             const bool r_ge2 = K == LV_MID ? true
                                : K == LV_THIRD ? false : r >= 2;
-            float a_r = lu_row(r, c2a_a, c2a_b, chm_a, chm_b, alpha_r, r_ge2);
-            float wr = force_row(r, L.rw, rhs_a, rhs_b, rhs_c, pho_a, pho_b,
+            tri a_r = lu_row(r, c2a_a, c2a_b, chm_a, chm_b, alpha_r, r_ge2);
+            real wr = force_row(r, L.rw, rhs_a, rhs_b, rhs_c, pho_a, pho_b,
                                  pho_c, wold_b, t2_a, t2_b, c2a_a, c2a_b,
                                  alt_a, alt_b, chm_a, chm_b);
-            float wf = (wr - a_r * wf_prev) * alpha_r;
-            gst(d.w_pp, r * st, bc, wf);
+            tri wf = ((tri)wr - a_r * wf_prev) * alpha_r;
+            if (PREC) d.tri64[stt + r * st + c] = (double)wf;
+            else gst(d.w_pp, r * st, bc, (real)wf);
             wf_prev = wf;
         }
         rhs_a = rhs_b; rhs_b = rhs_c;
@@ -1353,52 +1395,62 @@ __global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout)
     {
         // nxt holds the w-level nz loads (phi'', w'', rph_t) and rw_t at nz-1
         const unsigned f = nz * st;
-        const float pho_c = nxt.phpp;
-        const float wold_c = nxt.wpp;
-        float r = dtau * (nxt.rph + ghalf_eps * wold_c);
-        float rhs_c = pho_c + r / cfm_at(nz);
-        if (s.top_lid) rhs_c = 0.0f;
+        const real pho_c = nxt.phpp;
+        const real wold_c = nxt.wpp;
+        real r = dtau * (nxt.rph + ghalf_eps * wold_c);
+        real rhs_c = pho_c + r / cfm_at(nz);
+        if (s.top_lid) rhs_c = R(0.0);
         gst(d.ph_pp, f, bc, rhs_c);
         const int rr = nz - 1;
-        float alpha_r;
-        float a_r = lu_row(rr, c2a_a, c2a_b, chm_a, chm_b, alpha_r, rr >= 2);
-        float wr = force_row(rr, nxt.rw, rhs_a, rhs_b, rhs_c, pho_a, pho_b,
+        tri alpha_r;
+        tri a_r = lu_row(rr, c2a_a, c2a_b, chm_a, chm_b, alpha_r, rr >= 2);
+        real wr = force_row(rr, nxt.rw, rhs_a, rhs_b, rhs_c, pho_a, pho_b,
                              pho_c, wold_b, t2_a, t2_b, c2a_a, c2a_b, alt_a,
                              alt_b, chm_a, chm_b);
-        float wfr = (wr - a_r * wf_prev) * alpha_r;
-        gst(d.w_pp, rr * st, bc, wfr);
-        float a_top = 0.0f;
-        float wtop;
+        tri wfr = ((tri)wr - a_r * wf_prev) * alpha_r;
+        if (PREC) d.tri64[stt + rr * st + c] = (double)wfr;
+        else gst(d.w_pp, rr * st, bc, (real)wfr);
+        tri a_top = (tri)R(0.0);
+        real wtop;
         if (s.top_lid) {
-            wtop = 0.0f;
+            wtop = R(0.0);
         } else {
-            a_top = -2.0f * cof * rdnw[nz - 1] * rdnw[nz - 1] * c2a_b
-                    / (chm_b * cfm_at(nz - 1));
-            float dph_dn = one_p * (rhs_c - rhs_b) + one_m * (pho_c - pho_b);
-            float bterm = dtau * G_F * c2a_b * rdnw[nz - 1] * rdnw[nz - 1]
+            a_top = -(tri)R(2.0) * (tri)cof * (tri)rdnw[nz - 1]
+                    * (tri)rdnw[nz - 1] * (tri)c2a_b
+                    / ((tri)chm_b * (tri)cfm_at(nz - 1));
+            real dph_dn = one_p * (rhs_c - rhs_b) + one_m * (pho_c - pho_b);
+            real bterm = dtau * G_F * c2a_b * rdnw[nz - 1] * rdnw[nz - 1]
                           / chm_b * dph_dn;
-            float cterm = dtau * G_F * (2.0f * rdnw[nz - 1] * c2a_b * alt_b * t2_b
+            real cterm = dtau * G_F * (R(2.0) * rdnw[nz - 1] * c2a_b * alt_b * t2_b
                                         + c1f[nz] * muave);
             wtop = wold_c + ((dtau * d.rw_t[f + c] - bterm) - cterm);
         }
-        float b_top = 1.0f + 2.0f * cof * rdnw[nz - 1] * rdnw[nz - 1] * c2a_b
-                             / (chm_b * cfm_at(nz));
-        float alpha_top = 1.0f / (b_top - a_top * gam_prev);
-        gst(d.w_pp, f, bc, (wtop - a_top * wfr) * alpha_top);
+        tri b_top = (tri)R(1.0) + (tri)R(2.0) * (tri)cof * (tri)rdnw[nz - 1]
+                                    * (tri)rdnw[nz - 1] * (tri)c2a_b
+                                    / ((tri)chm_b * (tri)cfm_at(nz));
+        tri alpha_top = (tri)R(1.0) / (b_top - a_top * gam_prev);
+        const tri wtf = ((tri)wtop - a_top * wfr) * alpha_top;
+        if (PREC) d.tri64[stt + f + c] = (double)wtf;
+        else gst(d.w_pp, f, bc, (real)wtf);
     }
 
     // downward sweep: back substitution, damper, phi'' update, EOS
-    float htop = 0.0f, hbot = 0.0f;
+    real htop = R(0.0), hbot = R(0.0);
     if (DAMP) {
         htop = ((B3 ? d.phb[nz * st + c] : phb1[nz]) + d.php[nz * st + c]) / G_F;
         hbot = htop - s.zdamp;
     }
-    const float c_ph = ((0.5f * dtau) * G_F) * one_p;
+    const real c_ph = ((R(0.5) * dtau) * G_F) * one_p;
     auto load_dn = [&](ColDn& L, int k) {
         const unsigned ks = k * st;
         const unsigned km = (k < nz ? k : nz - 1) * st;
-        L.wpp = gld(d.w_pp, ks, bc);
-        L.gam = gld(d.gam, ks, bc);
+        if (PREC) {
+            L.wpp64 = d.tri64[stt + ks + c];
+            L.gam64 = d.tri64[ks + c];
+        } else {
+            L.wpp = gld(d.w_pp, ks, bc);
+            L.gam = gld(d.gam, ks, bc);
+        }
         L.phpp = gld(d.ph_pp, ks, bc);
         if (DAMP) {
             L.w = gld(d.w, ks, bc);
@@ -1411,7 +1463,8 @@ __global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout)
         L.p = gld(d.p, km, bc);
         L.thpp = gld(d.th_pp, km, bc);
     };
-    float wb_next = 0.0f, phn_up = 0.0f;
+    real wb_next = R(0.0), phn_up = R(0.0);
+    double wb_next64 = 0.0;
     ColDn dn;
     load_dn(dn, nz);
     // kinds: LV_FIRST = top w level (no mass level, no gamma), LV_MID,
@@ -1422,34 +1475,43 @@ __global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout)
         const ColDn L = dn;
         load_dn(dn, k >= 1 ? k - 1 : 0);
         const unsigned f = k * st;
-        float phn;
+        real phn;
         if (K != LV_LAST) {
-            float wb = L.wpp;
-            if (K != LV_FIRST) wb = wb - L.gam * wb_next;
-            wb_next = wb;
-            float wfin = wb;
+            real wfin;
+            if (PREC) {
+                double wb = L.wpp64;
+                if (K != LV_FIRST) wb = wb - L.gam64 * wb_next64;
+                wb_next64 = wb;
+                wfin = (real)wb;
+            } else {
+                real wb = L.wpp;
+                if (K != LV_FIRST) wb = wb - L.gam * wb_next;
+                wb_next = wb;
+                wfin = wb;
+            }
+            const real wb = wfin;
             if (DAMP) {
-                float hk = (phbv(L.phb, k) + L.php) / G_F;
+                real hk = (phbv(L.phb, k) + L.php) / G_F;
                 if (hk >= hbot) {
-                    float arg = (1.5707963f * (hk - hbot)) / s.zdamp;
-                    float sn = (float)sin((double)arg);
-                    float dw = (s.dampmag * sn) * sn;
-                    wfin = (wb - (dw * cfm_at(k)) * L.w) / (1.0f + dw);
+                    real arg = (R(1.5707963) * (hk - hbot)) / s.zdamp;
+                    real sn = (real)sin((double)arg);
+                    real dw = (s.dampmag * sn) * sn;
+                    wfin = (wb - (dw * cfm_at(k)) * L.w) / (R(1.0) + dw);
                 }
             }
             gst(d.w_pp, f, bc, wfin);
-            float cfs = c1f[k] * muts + c2f[k];
+            real cfs = c1f[k] * muts + c2f[k];
             phn = L.phpp + (c_ph * wfin) / cfs;
             gst(d.ph_pp, f, bc, phn);
         } else {
             phn = L.phpp;
         }
         if (K != LV_FIRST) {
-            float chm2 = c1h[k] * muts + c2h[k];
-            float th_ref = thbv(L.thb, k) + L.thp;
-            float altk = L.alt;
-            float c2a = GAMMA_F * L.p / altk;
-            float al = -((altk * (c1h[k] * mupp)) + rdnw[k] * (phn_up - phn)) / chm2;
+            real chm2 = c1h[k] * muts + c2h[k];
+            real th_ref = thbv(L.thb, k) + L.thp;
+            real altk = L.alt;
+            real c2a = GAMMA_F * L.p / altk;
+            real al = -((altk * (c1h[k] * mupp)) + rdnw[k] * (phn_up - phn)) / chm2;
             if (AL) gst(d.al_pp, f, bc, al);
             gst(pout, f, bc, c2a * ((altk * (L.thpp - (c1h[k] * mupp) * th_ref))
                                     / (chm2 * th_ref) - al));
@@ -1468,29 +1530,29 @@ __global__ __launch_bounds__(256) void k_column(Dom d, Sc s, float* R pout)
 // Coupled-to-uncoupled fold of one face (reference dycore.cu:1087-1127):
 // new = (C(mu_t*) x_t* + x'') / C(mu_t* + mu'').
 template <int STRIDE_KIND>
-__device__ __forceinline__ void finish_face(const Dom& d, float* R out,
-                                            const float* R x,
-                                            const float* R xpp, int f,
-                                            float msf, float mnf)
+__device__ __forceinline__ void finish_face(const Dom& d, real* RS out,
+                                            const real* RS x,
+                                            const real* RS xpp, int f,
+                                            real msf, real mnf)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx;
     const int sf = STRIDE_KIND == 0 ? ny * (nx + 1) : (ny + 1) * nx;
-    kfloat* c1h = kview(d.c1h);
-    kfloat* c2h = kview(d.c2h);
-    unsigned bf = 4u * f;
+    kreal* c1h = kview(d.c1h);
+    kreal* c2h = kview(d.c2h);
+    unsigned bf = RB * f;
     for (int k = 0; k < nz; ++k) {
         colb_opaque(bf);
-        const float c1 = c1h[k], c2 = c2h[k];
+        const real c1 = c1h[k], c2 = c2h[k];
         gst(out, k * sf, bf, ((c1 * msf + c2) * gld(x, k * sf, bf)
                               + gld(xpp, k * sf, bf)) / (c1 * mnf + c2));
     }
 }
 
 struct FinIn {
-    float u = 0.0f, upp = 0.0f, v = 0.0f, vpp = 0.0f;
-    float w = 0.0f, wpp = 0.0f, php = 0.0f, phpp = 0.0f;
-    float thp = 0.0f, thpp = 0.0f;
-    float thb = 0.0f, phb = 0.0f, alb = 0.0f;   // terrain base
+    real u = R(0.0), upp = R(0.0), v = R(0.0), vpp = R(0.0);
+    real w = R(0.0), wpp = R(0.0), php = R(0.0), phpp = R(0.0);
+    real thp = R(0.0), thpp = R(0.0);
+    real thb = R(0.0), phb = R(0.0), alb = R(0.0);   // terrain base
 };
 
 // Small-step finish (reference dycore.cu:1087-1198) of the mass column
@@ -1506,26 +1568,26 @@ __global__ __launch_bounds__(256) void k_finish(Dom d, FinishDst o)
     const int j = c / nx, i = c - j * nx;
     const int im = (i == 0) ? nx - 1 : i - 1;
     const int jm = (j == 0) ? ny - 1 : j - 1;
-    kfloat* c1h = kview(d.c1h);
-    kfloat* c2h = kview(d.c2h);
-    kfloat* c1f = kview(d.c1f);
-    kfloat* c2f = kview(d.c2f);
-    kfloat* rdnw = kview(d.rdnw);
-    kfloat* thb1 = kview(d.thb);
-    kfloat* phb1 = kview(d.phb);
-    kfloat* alb1 = kview(d.alb);
-    const float* R mub = d.mub2d;
-    const float* R mup = d.mup;
-    const float* R mu_pp = d.mu_pp;
-    const float msA = mub[c] + mup[c], mnA = msA + mu_pp[c];
+    kreal* c1h = kview(d.c1h);
+    kreal* c2h = kview(d.c2h);
+    kreal* c1f = kview(d.c1f);
+    kreal* c2f = kview(d.c2f);
+    kreal* rdnw = kview(d.rdnw);
+    kreal* thb1 = kview(d.thb);
+    kreal* phb1 = kview(d.phb);
+    kreal* alb1 = kview(d.alb);
+    const real* RS mub = d.mub2d;
+    const real* RS mup = d.mup;
+    const real* RS mu_pp = d.mu_pp;
+    const real msA = mub[c] + mup[c], mnA = msA + mu_pp[c];
     const int cW = j * nx + im, cS = jm * nx + i;
-    const float msW = mub[cW] + mup[cW], mnW = msW + mu_pp[cW];
-    const float msS = mub[cS] + mup[cS], mnS = msS + mu_pp[cS];
-    const float msf_u = 0.5f * (msA + msW), mnf_u = 0.5f * (mnA + mnW);
-    const float msf_v = 0.5f * (msA + msS), mnf_v = 0.5f * (mnA + mnS);
-    const float mup_new = mup[c] + mu_pp[c];
-    const float mu_eos = mub[c] + mup_new;
-    unsigned bc = 4u * c, bu = 4u * (j * (nx + 1) + i);
+    const real msW = mub[cW] + mup[cW], mnW = msW + mu_pp[cW];
+    const real msS = mub[cS] + mup[cS], mnS = msS + mu_pp[cS];
+    const real msf_u = R(0.5) * (msA + msW), mnf_u = R(0.5) * (mnA + mnW);
+    const real msf_v = R(0.5) * (msA + msS), mnf_v = R(0.5) * (mnA + mnS);
+    const real mup_new = mup[c] + mu_pp[c];
+    const real mu_eos = mub[c] + mup_new;
+    unsigned bc = RB * c, bu = RB * (j * (nx + 1) + i);
 
     auto load = [&](FinIn& L, int k) {
         const int km = k < nz ? k : nz - 1;
@@ -1545,8 +1607,8 @@ __global__ __launch_bounds__(256) void k_finish(Dom d, FinishDst o)
             L.alb = gld(d.alb, km * st, bc);
         }
     };
-    float php_lo = 0.0f, thp_lo = 0.0f, thb_lo = 0.0f, phb_lo = 0.0f;
-    float alb_lo = 0.0f;
+    real php_lo = R(0.0), thp_lo = R(0.0), thb_lo = R(0.0), phb_lo = R(0.0);
+    real alb_lo = R(0.0);
     FinIn nxt;
     load(nxt, 0);
     // kinds: 0 = surface (no EOS below), 1 = interior, 2 = top w level
@@ -1558,7 +1620,7 @@ __global__ __launch_bounds__(256) void k_finish(Dom d, FinishDst o)
         if (K != 2) load(nxt, k + 1);
         const unsigned f = k * st;
         if (K != 2) {
-            const float c1 = c1h[k], c2 = c2h[k];
+            const real c1 = c1h[k], c2 = c2h[k];
             gst(o.u, k * stu, bu, ((c1 * msf_u + c2) * L.u + L.upp)
                                       / (c1 * mnf_u + c2));
             gst(o.v, k * stv, bc, ((c1 * msf_v + c2) * L.v + L.vpp)
@@ -1566,22 +1628,22 @@ __global__ __launch_bounds__(256) void k_finish(Dom d, FinishDst o)
         }
         gst(o.w, f, bc, ((c1f[k] * msA + c2f[k]) * L.w + L.wpp)
                             / (c1f[k] * mnA + c2f[k]));
-        const float php_new = L.php + L.phpp;
+        const real php_new = L.php + L.phpp;
         gst(o.php, f, bc, php_new);
-        const float phb_k = B3 ? L.phb : phb1[k];
+        const real phb_k = B3 ? L.phb : phb1[k];
         if (EOS && K != 0) {
             const unsigned h = f - st;
-            float th = thb_lo + thp_lo;
-            float dph = (phb_k + php_new) - (phb_lo + php_lo);
-            float alt = (-dph * rdnw[k - 1]) / (c1h[k - 1] * mu_eos + c2h[k - 1]);
+            real th = thb_lo + thp_lo;
+            real dph = (phb_k + php_new) - (phb_lo + php_lo);
+            real alt = (-dph * rdnw[k - 1]) / (c1h[k - 1] * mu_eos + c2h[k - 1]);
             gst(d.alt, h, bc, alt);
             gst(d.al, h, bc, alt - alb_lo);
             gst(d.p, h, bc, P0_F * pow_gamma((RD_F * th) / (P0_F * alt)));
         }
         if (K != 2) {
-            const float thbk = B3 ? L.thb : thb1[k];
-            float num = (c1h[k] * msA + c2h[k]) * (thbk + L.thp) + L.thpp;
-            float thn = num / (c1h[k] * mnA + c2h[k]) - thbk;
+            const real thbk = B3 ? L.thb : thb1[k];
+            real num = (c1h[k] * msA + c2h[k]) * (thbk + L.thp) + L.thpp;
+            real thn = num / (c1h[k] * mnA + c2h[k]) - thbk;
             gst(o.thp, f, bc, thn);
             thp_lo = thn;
             thb_lo = thbk;
@@ -1601,21 +1663,21 @@ __global__ __launch_bounds__(256) void k_finish_edge(Dom d, FinishDst o)
 {
     const int ny = d.ny, nx = d.nx;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const float* R mub = d.mub2d;
-    const float* R mup = d.mup;
-    const float* R mu_pp = d.mu_pp;
+    const real* RS mub = d.mub2d;
+    const real* RS mup = d.mup;
+    const real* RS mu_pp = d.mu_pp;
     if (t < ny) {
         const int j = t, cA = j * nx, cB = j * nx + nx - 1;
-        const float msA = mub[cA] + mup[cA], mnA = msA + mu_pp[cA];
-        const float msB = mub[cB] + mup[cB], mnB = msB + mu_pp[cB];
+        const real msA = mub[cA] + mup[cA], mnA = msA + mu_pp[cA];
+        const real msB = mub[cB] + mup[cB], mnB = msB + mu_pp[cB];
         finish_face<0>(d, o.u, d.u, d.u_pp, j * (nx + 1) + nx,
-                       0.5f * (msA + msB), 0.5f * (mnA + mnB));
+                       R(0.5) * (msA + msB), R(0.5) * (mnA + mnB));
     } else if (t < ny + nx) {
         const int i = t - ny, cA = i, cB = (ny - 1) * nx + i;
-        const float msA = mub[cA] + mup[cA], mnA = msA + mu_pp[cA];
-        const float msB = mub[cB] + mup[cB], mnB = msB + mu_pp[cB];
+        const real msA = mub[cA] + mup[cA], mnA = msA + mu_pp[cA];
+        const real msB = mub[cB] + mup[cB], mnB = msB + mu_pp[cB];
         finish_face<1>(d, o.v, d.v, d.v_pp, ny * nx + i,
-                       0.5f * (msA + msB), 0.5f * (mnA + mnB));
+                       R(0.5) * (msA + msB), R(0.5) * (mnA + mnB));
     }
 }
 
@@ -1638,15 +1700,22 @@ static inline dim3 masses(const Dom& d)
     return dim3((d.nx + 63) / 64, (d.ny + 3) / 4);
 }
 
-hipError_t run_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st)
+template <bool PREC>
+static void launch_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st)
 {
     const bool b3 = d.base3d != 0;
     if (do_eos && b3)
-        hipLaunchKernelGGL((k_omega<true, true>), cols(d), dim3(256), 0, st, d, s);
+        hipLaunchKernelGGL((k_omega<true, true, PREC>), cols(d), dim3(256), 0, st, d, s);
     else if (do_eos)
-        hipLaunchKernelGGL((k_omega<true, false>), cols(d), dim3(256), 0, st, d, s);
+        hipLaunchKernelGGL((k_omega<true, false, PREC>), cols(d), dim3(256), 0, st, d, s);
     else
-        hipLaunchKernelGGL((k_omega<false, false>), cols(d), dim3(256), 0, st, d, s);
+        hipLaunchKernelGGL((k_omega<false, false, PREC>), cols(d), dim3(256), 0, st, d, s);
+}
+
+hipError_t run_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st)
+{
+    if (s.prec) launch_omega<true>(d, s, do_eos, st);
+    else launch_omega<false>(d, s, do_eos, st);
     return hipGetLastError();
 }
 
@@ -1707,15 +1776,19 @@ hipError_t run_uv(const Dom& d, const Sc& s, hipStream_t st)
 }
 
 template <bool B3, bool DAMP, bool AL, bool WW>
-static void launch_column(const Dom& d, const Sc& s, float* pout,
+static void launch_column(const Dom& d, const Sc& s, real* pout,
                           hipStream_t st)
 {
-    hipLaunchKernelGGL((k_column<B3, DAMP, AL, WW>), cols(d), dim3(256), 0, st,
-                       d, s, pout);
+    if (s.prec)
+        hipLaunchKernelGGL((k_column<B3, DAMP, AL, WW, true>), cols(d),
+                           dim3(256), 0, st, d, s, pout);
+    else
+        hipLaunchKernelGGL((k_column<B3, DAMP, AL, WW, false>), cols(d),
+                           dim3(256), 0, st, d, s, pout);
 }
 
 template <bool B3, bool DAMP>
-static void launch_column_bd(const Dom& d, const Sc& s, float* pout, int al,
+static void launch_column_bd(const Dom& d, const Sc& s, real* pout, int al,
                              int ww, hipStream_t st)
 {
     if (al && ww) launch_column<B3, DAMP, true, true>(d, s, pout, st);
@@ -1724,7 +1797,7 @@ static void launch_column_bd(const Dom& d, const Sc& s, float* pout, int al,
     else launch_column<B3, DAMP, false, false>(d, s, pout, st);
 }
 
-hipError_t run_column(const Dom& d, const Sc& s, float* pout, int write_al,
+hipError_t run_column(const Dom& d, const Sc& s, real* pout, int write_al,
                       int write_ww, hipStream_t st)
 {
     const bool b3 = d.base3d != 0, damp = s.damp != 0;
@@ -1751,3 +1824,5 @@ hipError_t run_finish(const Dom& d, const FinishDst& o, hipStream_t st)
                        dim3(256), 0, st, d, o);
     return hipGetLastError();
 }
+
+}  // namespace ARW_NS

@@ -22,6 +22,13 @@ def _stream() -> int:
     return torch.cuda.current_stream().cuda_stream
 
 
+def _fn(state: DomainState, name: str):
+    """The entry point of the state's precision: arw_* (float32) or the
+    FP64 build arw64_* of the same discretisation."""
+    prefix = "arw64_" if state.dtype == torch.float64 else "arw_"
+    return prefix + name, getattr(_lib.load(), prefix + name)
+
+
 def params(cfg: RunConfig) -> _lib.ArwParams:
     p = _lib.ArwParams()
     p.dx, p.dy, p.dt = float(cfg.dx), float(cfg.dy), float(cfg.dt)
@@ -35,6 +42,7 @@ def params(cfg: RunConfig) -> _lib.ArwParams:
     p.zdamp, p.dampcoef = float(cfg.zdamp), float(cfg.dampcoef)
     p.top_lid = int(bool(cfg.top_lid))
     p.moist_adv_opt = int(cfg.moist_adv_opt)
+    p.precision_form = int(bool(cfg.precision_form))
     return p
 
 
@@ -73,10 +81,11 @@ def step(state: DomainState, cfg: RunConfig, *, acoustic: bool = True,
             "cfg.moist must match the state: a moist step needs a state "
             "allocated with the qv/qc/qr scalars, and a dry one none")
     _check_state(state)
-    lib = _lib.load()
-    d = state.domain_struct(diffusion=_uses_diffusion(cfg))
+    d = state.domain_struct(diffusion=_uses_diffusion(cfg),
+                            precision_form=bool(cfg.precision_form))
     p = params(cfg)
-    _lib.check(lib.arw_step(d, p, _stream()), "arw_step")
+    name, fn = _fn(state, "step")
+    _lib.check(fn(d, p, _stream()), name)
     state.elapsed_seconds += cfg.dt
 
 
@@ -92,8 +101,8 @@ def update_diagnostics(state: DomainState, hypsometric_opt: int = 1) -> None:
     if hypsometric_opt != 1:
         raise NotImplementedError("only hypsometric_opt=1 is ported")
     _check_state(state)
-    _lib.check(_lib.load().arw_calc_p_alpha(state.domain_struct(), _stream()),
-               "arw_calc_p_alpha")
+    name, fn = _fn(state, "calc_p_alpha")
+    _lib.check(fn(state.domain_struct(), _stream()), name)
 
 
 def stage_fluxes(state: DomainState, cfg: RunConfig):
@@ -103,18 +112,17 @@ def stage_fluxes(state: DomainState, cfg: RunConfig):
     ru = state.scratch((nz, ny, nx + 1), "rk_ru")
     rv = state.scratch((nz, ny + 1, nx), "rk_rv")
     d = state.domain_struct()
-    _lib.check(_lib.load().arw_stage_fluxes(d, params(cfg), ru.data_ptr(),
-                                            rv.data_ptr(), _stream()),
-               "arw_stage_fluxes")
+    name, fn = _fn(state, "stage_fluxes")
+    _lib.check(fn(d, params(cfg), ru.data_ptr(), rv.data_ptr(), _stream()),
+               name)
     return ru, rv, state.scratch((nz + 1, ny, nx), "arw_ww")
 
 
 def set_w_surface(state: DomainState, cfg: RunConfig) -> None:
     """w(sfc) = u . grad(ht) (reference dycore.py:1999-2026)."""
     _check_state(state)
-    _lib.check(_lib.load().arw_set_w_surface(state.domain_struct(),
-                                             params(cfg), _stream()),
-               "arw_set_w_surface")
+    name, fn = _fn(state, "set_w_surface")
+    _lib.check(fn(state.domain_struct(), params(cfg), _stream()), name)
 
 
 def prepare_acoustic_coefficients(state: DomainState, cfg: RunConfig,
@@ -122,9 +130,9 @@ def prepare_acoustic_coefficients(state: DomainState, cfg: RunConfig,
     """Implicit-w coefficients for the current reference state (reference
     acoustic.py:299-327)."""
     _check_state(state)
-    _lib.check(_lib.load().arw_calc_coefs(state.domain_struct(), params(cfg),
-                                          float(dtau), _stream()),
-               "arw_calc_coefs")
+    name, fn = _fn(state, "calc_coefs")
+    _lib.check(fn(state.domain_struct(), params(cfg), float(dtau), _stream()),
+               name)
 
 
 def acoustic_substep(state: DomainState, cfg: RunConfig, dtau: float,
@@ -132,9 +140,9 @@ def acoustic_substep(state: DomainState, cfg: RunConfig, dtau: float,
     """One acoustic substep with the prepared coefficients (reference
     acoustic.py:482-567)."""
     _check_state(state)
-    _lib.check(_lib.load().arw_acoustic_substep(
-        state.domain_struct(), params(cfg), float(dtau), int(bool(first)),
-        _stream()), "arw_acoustic_substep")
+    name, fn = _fn(state, "acoustic_substep")
+    _lib.check(fn(state.domain_struct(), params(cfg), float(dtau),
+                  int(bool(first)), _stream()), name)
 
 
 def run_acoustic_only(state: DomainState, cfg: RunConfig, dtau: float,
@@ -153,9 +161,8 @@ def domain_mass_measure(state: DomainState) -> float:
     identity map factors)."""
     _check_state(state)
     out = torch.zeros(1, dtype=torch.float64, device=state.device)
-    _lib.check(_lib.load().arw_domain_mass(state.domain_struct(),
-                                           out.data_ptr(), _stream()),
-               "arw_domain_mass")
+    name, fn = _fn(state, "domain_mass")
+    _lib.check(fn(state.domain_struct(), out.data_ptr(), _stream()), name)
     return float(out.item())
 
 
@@ -166,9 +173,9 @@ def stability_report(state: DomainState, cfg: RunConfig | None = None) -> dict:
     if state.u.numel() == 0 or state.w.numel() == 0:
         raise ValueError("zero-size array to reduction operation maximum "
                          "which has no identity")
-    out = torch.zeros(4, dtype=torch.float32, device=state.device)
-    _lib.check(_lib.load().arw_health(state.domain_struct(), out.data_ptr(),
-                                      _stream()), "arw_health")
+    out = torch.zeros(4, dtype=state.dtype, device=state.device)
+    name, fn = _fn(state, "health")
+    _lib.check(fn(state.domain_struct(), out.data_ptr(), _stream()), name)
     u_max, w_max, th_max, wcfl = (float(x) for x in out.cpu().tolist())
     nan = not (math.isfinite(u_max) and math.isfinite(w_max)
                and math.isfinite(th_max))

@@ -13,11 +13,11 @@ import torch
 
 from . import constants as c
 from ._lib import (DIFF_ARRAYS, DOMAIN_ARRAYS, MOIST_ARRAYS, MOIST_SCRATCH,
-                   SCRATCH_ARRAYS, ArwDomain)
+                   SCRATCH_ARRAYS, ArwDomain, ArwDomain64)
 from .config import RunConfig
 from .grid import BaseState, VerticalCoord, surface_extrapolation
 
-DTYPE = torch.float32
+DTYPE = torch.float32          # the step's precision; FP64 is the reference build
 
 _COORD = ("dnw", "rdnw", "dn", "rdn", "fnp", "fnm", "znu", "znw",
           "c1h", "c2h", "c1f", "c2f", "c3h", "c4h", "c3f", "c4f")
@@ -40,11 +40,15 @@ class DomainState:
     """All prognostic, diagnostic, tendency, acoustic, base and coordinate
     arrays of one domain (reference state.py:359-695), on ``device``."""
 
-    def __init__(self, cfg: RunConfig, device="cuda"):
+    def __init__(self, cfg: RunConfig, device="cuda", dtype=DTYPE):
         nz, ny, nx = cfg.nz, cfg.ny, cfg.nx
         self.cfg = cfg
         self.device = torch.device(device)
-        z = lambda *sh: torch.zeros(sh, dtype=DTYPE, device=self.device)
+        if dtype not in (torch.float32, torch.float64):
+            raise ValueError("the step runs in float32 (or float64, the "
+                             f"reference build), not {dtype}")
+        self.dtype = dtype
+        z = lambda *sh: torch.zeros(sh, dtype=dtype, device=self.device)
         self.u, self.v, self.w = z(nz, ny, nx + 1), z(nz, ny + 1, nx), \
             z(nz + 1, ny, nx)
         self.thp, self.php, self.mup = z(nz, ny, nx), z(nz + 1, ny, nx), \
@@ -87,9 +91,9 @@ class DomainState:
         for name in _COORD:
             setattr(self, name, z(nz + 1 if name.endswith("f")
                                   or name == "znw" else nz))
-        self.msft = torch.ones((ny, nx), dtype=DTYPE, device=self.device)
-        self.msfu = torch.ones((ny, nx + 1), dtype=DTYPE, device=self.device)
-        self.msfv = torch.ones((ny + 1, nx), dtype=DTYPE, device=self.device)
+        self.msft = torch.ones((ny, nx), dtype=dtype, device=self.device)
+        self.msfu = torch.ones((ny, nx + 1), dtype=dtype, device=self.device)
+        self.msfv = torch.ones((ny + 1, nx), dtype=dtype, device=self.device)
         self.f, self.e = z(ny, nx), z(ny, nx)
         self.has_msf = False
         self.rotational = False
@@ -104,7 +108,7 @@ class DomainState:
         """Copy the FP64 host coordinate/base arrays to FP32 device arrays
         (reference state.py:722-771)."""
         put = lambda t, a: t.copy_(torch.as_tensor(
-            np.asarray(a, dtype=np.float32)))
+            np.asarray(a, dtype=self.np_dtype)))
         for name in _COORD:
             put(getattr(self, name), getattr(coord, name))
         for name in ("thb", "pb", "alb", "phb"):
@@ -116,9 +120,9 @@ class DomainState:
                     f"allocated for {dev.ndim}-D profiles: cfg.terrain_opt "
                     "must match the terrain the base state was built with")
             put(dev, host)
-        self.p_top = float(np.float32(base.p_top))
+        self.p_top = float(self.np_dtype.type(base.p_top))
         if np.ndim(base.mub) == 0:
-            self.mub = float(np.float32(base.mub))
+            self.mub = float(self.np_dtype.type(base.mub))
             self.mub2d.fill_(self.mub)
         else:
             self.mub = None
@@ -129,26 +133,33 @@ class DomainState:
             self.ht.zero_()
         self._phb_host = np.array(base.phb, dtype=np.float64, copy=True)
         for name, val in surface_extrapolation(coord).items():
-            setattr(self, name, float(np.float32(val)))
+            setattr(self, name, float(self.np_dtype.type(val)))
+
+    @property
+    def np_dtype(self) -> np.dtype:
+        return np.dtype(np.float64 if self.dtype == torch.float64
+                        else np.float32)
 
     @classmethod
-    def from_host(cls, case, device="cuda") -> "DomainState":
-        """Upload a host initial condition (``arwen_amd.cases.HostCase``)."""
-        s = cls(case.cfg, device=device)
+    def from_host(cls, case, device="cuda", dtype=DTYPE) -> "DomainState":
+        """Upload a host initial condition (``arwen_amd.cases.HostCase``),
+        rounded once to the state's precision."""
+        s = cls(case.cfg, device=device, dtype=dtype)
         s.load_base(case.coord, case.base)
         for name, arr in case.fields.items():
             getattr(s, name).copy_(torch.as_tensor(
-                np.asarray(arr, dtype=np.float32)))
+                np.asarray(arr, dtype=s.np_dtype)))
         return s
 
     # -- reference helpers ----------------------------------------------------
 
-    def scratch(self, shape, slot: str) -> torch.Tensor:
+    def scratch(self, shape, slot: str, dtype=None) -> torch.Tensor:
         """Persistent named scratch buffer (reference state.py:805-830)."""
         shape = tuple(shape)
         buf = self._scratch.get(slot)
         if buf is None:
-            buf = torch.zeros(shape, dtype=DTYPE, device=self.device)
+            buf = torch.zeros(shape, dtype=dtype or self.dtype,
+                              device=self.device)
             self._scratch[slot] = buf
         elif tuple(buf.shape) != shape:
             raise ValueError(f"scratch slot {slot!r} has shape "
@@ -190,20 +201,22 @@ class DomainState:
         rdzf_h = 1.0 / np.diff(zh)
         rdzc_h = 1.0 / np.diff(zf)
         self._diff_rdz = [
-            torch.as_tensor(np.asarray(a, dtype=np.float32)).to(self.device)
+            torch.as_tensor(np.asarray(a, dtype=self.np_dtype)).to(self.device)
             for a in (rdzf_h, rdzc_h, rdzf_w, rdzc_w)]
         return self._diff_rdz
 
-    def domain_struct(self, diffusion: bool = False) -> ArwDomain:
+    def domain_struct(self, diffusion: bool = False,
+                      precision_form: bool = False) -> ArwDomain:
         """Raw-pointer view of the state for the C-ABI."""
         nz, ny, nx = self.shape
-        d = ArwDomain()
+        d = ArwDomain64() if self.dtype == torch.float64 else ArwDomain()
         d.nz, d.ny, d.nx = nz, ny, nx
         d.base3d = int(self.thb.dim() == 3)
         for name in DOMAIN_ARRAYS:
             t = getattr(self, name)
-            if not t.is_contiguous() or t.dtype != DTYPE:
-                raise ValueError(f"state.{name} must be contiguous float32")
+            if not t.is_contiguous() or t.dtype != self.dtype:
+                raise ValueError(f"state.{name} must be contiguous "
+                                 f"{self.dtype}")
             setattr(d, name, t.data_ptr())
         for name in ("cf1", "cf2", "cf3", "cfn", "cfn1"):
             setattr(d, name, float(getattr(self, name)))
@@ -221,8 +234,9 @@ class DomainState:
             for name in MOIST_ARRAYS:
                 t = (getattr(self, name) if not name.startswith("qs")
                      else self.scratch(m3, "arw_" + name))
-                if not t.is_contiguous() or t.dtype != DTYPE:
-                    raise ValueError(f"state.{name} must be contiguous float32")
+                if not t.is_contiguous() or t.dtype != self.dtype:
+                    raise ValueError(f"state.{name} must be contiguous "
+                                     f"{self.dtype}")
                 setattr(d, name, t.data_ptr())
             shapes = {"ru_m": u3, "rv_m": v3, "ww_m": w3, "pd_fxl": u3,
                       "pd_fxc": u3, "pd_fyl": v3, "pd_fyc": v3, "pd_fzl": w3,
@@ -230,6 +244,9 @@ class DomainState:
             for name in MOIST_SCRATCH:
                 setattr(d, name, self.scratch(shapes[name],
                                               "arw_" + name).data_ptr())
+        if precision_form and self.dtype == torch.float32:
+            d.tri64 = self.scratch((2, nz + 1, ny, nx), "arw_tri64",
+                                   dtype=torch.float64).data_ptr()
         return d
 
     def numpy_fields(self, names=("u", "v", "w", "thp", "php", "mup")):

@@ -72,7 +72,39 @@ typedef struct arw_domain {
      * like v, fzl/fzc like w) and cell scales (nz,ny,nx) */
     float *ru_m, *rv_m, *ww_m;
     float *pd_fxl, *pd_fxc, *pd_fyl, *pd_fyc, *pd_fzl, *pd_fzc, *pd_sc;
+    /* precision form only (else NULL): FP64 tridiagonal intermediates,
+     * the LU factor gamma then the forward-eliminated w'', each
+     * (nz+1,ny,nx) */
+    double *tri64;
 } arw_domain;
+
+/*
+ * The same domain in double precision: identical fields and order, FP64
+ * arrays.  The arw64_* entry points run the identical discretisation in
+ * FP64 -- the reference the float32 step's per-field error is stated
+ * against, and the precision of the adjoint's dot-product test.  No
+ * reference counterpart: ArWen's verification mirrors are CPU-only
+ * (gpuwm/verify/npref.py).
+ */
+typedef struct arw64_domain {
+    int32_t nz, ny, nx;
+    int32_t base3d;
+    double *u, *v, *w, *thp, *php, *mup;
+    double *p, *al, *alt;
+    double *u0, *v0, *w0, *thp0, *php0, *mup0;
+    double *ru_t, *rv_t, *rw_t, *rth_t, *rph_t, *rmu_t;
+    double *u_pp, *v_pp, *w_pp, *th_pp, *ph_pp, *mu_pp;
+    double *p_pp, *p_pp_old, *ww_pp, *al_pp;
+    const double *thb, *pb, *alb, *phb, *mub2d, *ht;
+    const double *c1h, *c2h, *c1f, *c2f, *dnw, *rdnw, *dn, *rdn, *fnm, *fnp;
+    double cf1, cf2, cf3, cfn, cfn1;
+    double *ww, *gam, *dmdt_rf, *mup_next;
+    const double *rdzf_h, *rdzc_h, *rdzf_w, *rdzc_w;
+    double *qv, *qc, *qr, *qv0, *qc0, *qr0, *qsv, *qsc, *qsr;
+    double *ru_m, *rv_m, *ww_m;
+    double *pd_fxl, *pd_fxc, *pd_fyl, *pd_fyc, *pd_fzl, *pd_fzc, *pd_sc;
+    double *tri64;
+} arw64_domain;
 
 /* Step configuration (the RunConfig fields the step reads,
  * gpuwm/config.py:28-190).  Scalars that the reference derives in
@@ -88,6 +120,11 @@ typedef struct arw_params {
     double zdamp, dampcoef;
     int32_t top_lid;
     int32_t moist_adv_opt;     /* 1: PD limiter on the final RK stage */
+    /* 1: precision form of the float32 step -- compensated (Kahan) column
+     * sums of the mass divergence and Omega, the implicit w/phi
+     * tridiagonal factored and solved in FP64 (needs d->tri64).  No
+     * reference counterpart; ignored by the FP64 build. */
+    int32_t precision_form;
 } arw_params;
 
 int32_t arw_abi_version(void);
@@ -191,6 +228,34 @@ int32_t arw_profile_begin(int32_t max_events);
 int32_t arw_profile_end(double *ms, int32_t *launches, int32_t n_kernels);
 const char *arw_kernel_name(int32_t id);
 int32_t arw_kernel_count(void);
+
+/* FP64 build of the same entry points (see arw64_domain). */
+int32_t arw64_abi_version(void);
+const char *arw64_last_error(void);
+int32_t arw64_step(const arw64_domain *d, const arw_params *prm,
+                   void *stream);
+int32_t arw64_calc_p_alpha(const arw64_domain *d, void *stream);
+int32_t arw64_stage_fluxes(const arw64_domain *d, const arw_params *prm,
+                           double *ru, double *rv, void *stream);
+int32_t arw64_stage_prepare(const arw64_domain *d, const arw_params *prm,
+                            double dtau, int32_t do_eos, void *stream);
+int32_t arw64_acoustic_substep(const arw64_domain *d,
+                               const arw_params *prm, double dtau,
+                               int32_t first, void *stream);
+int32_t arw64_calc_coefs(const arw64_domain *d, const arw_params *prm,
+                         double dtau, void *stream);
+int32_t arw64_small_step_finish(const arw64_domain *d, void *stream);
+int32_t arw64_set_w_surface(const arw64_domain *d, const arw_params *prm,
+                            void *stream);
+int32_t arw64_health(const arw64_domain *d, double *out, void *stream);
+int32_t arw64_domain_mass(const arw64_domain *d, double *out,
+                          void *stream);
+int32_t arw64_flux_div_scalar(const double *q, const double *ru,
+                              const double *rv, const double *rw,
+                              double *tend, const double *rdnw,
+                              const double *fnm, const double *fnp,
+                              double dx_inv, double dy_inv, int32_t nz,
+                              int32_t ny, int32_t nx, void *stream);
 
 #ifdef __cplusplus
 }

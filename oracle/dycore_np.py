@@ -129,6 +129,22 @@ def calc_p_alpha(s) -> None:
     s.p[...] = _c(P0, s) * _pow_gamma((_c(RD, s) * th) / (_c(P0, s) * alt))
 
 
+class _Kahan:
+    """Compensated running sum (the precision form's column sums): s
+    carries the sum, e the low-order part lost by the last addition."""
+
+    def __init__(self, like):
+        self.s = np.zeros_like(like)
+        self.e = np.zeros_like(like)
+
+    def add(self, x):
+        y = x - self.e
+        t = self.s + y
+        self.e = (t - self.s) - y
+        self.s = t
+        return t
+
+
 def _pow_gamma(x):
     """x**GAMMA.  For float32 the power is evaluated in float64 and rounded
     once (the HIP kernels do the same), so both sides agree to the bit."""
@@ -155,7 +171,7 @@ def mu_at_v_faces(mu):
     return np.concatenate([muy, muy[:1, :]], axis=0).astype(mu.dtype)
 
 
-def stage_fluxes(s, dx, dy):
+def stage_fluxes(s, dx, dy, prec=False):
     """Coupled horizontal mass fluxes ru, rv and Omega ww at the stage
     reference state (reference dycore.py:162-206, 134-159, 102-131)."""
     mu = s.mub2d + s.mup
@@ -163,24 +179,32 @@ def stage_fluxes(s, dx, dy):
     c1h, c2h = s.c1h[:, None, None], s.c2h[:, None, None]
     ru = (c1h * mux[None] + c2h) * s.u
     rv = (c1h * muy[None] + c2h) * s.v
-    return ru, rv, omega_ref(s, ru, rv, dx, dy)
+    return ru, rv, omega_ref(s, ru, rv, dx, dy, prec)
 
 
-def omega_ref(s, ru, rv, dx, dy):
-    """WRF calc_ww_cp (reference dycore.py:102-131)."""
+def omega_ref(s, ru, rv, dx, dy, prec=False):
+    """WRF calc_ww_cp (reference dycore.py:102-131).  ``prec``: the
+    precision form's compensated column sums."""
     nz = s.thp.shape[0]
     rdx, rdy = _c(1.0 / dx, s), _c(1.0 / dy, s)
     dnw = s.dnw[:, None, None]
     divv = dnw * (rdx * (ru[:, :, 1:] - ru[:, :, :-1])
                   + rdy * (rv[:, 1:, :] - rv[:, :-1, :]))
-    dmdt = divv[0].copy()
-    for k in range(1, nz):
-        dmdt = dmdt + divv[k]
+    if prec:
+        ks = _Kahan(divv[0])
+        for k in range(nz):
+            ks.add(divv[k])
+        dmdt = ks.s
+    else:
+        dmdt = divv[0].copy()
+        for k in range(1, nz):
+            dmdt = dmdt + divv[k]
     ww = np.zeros((nz + 1,) + divv.shape[1:], dtype=s.dtype)
     inc = (s.c1h * s.dnw)[:nz - 1, None, None] * dmdt[None] + divv[:nz - 1]
     run = np.zeros_like(dmdt)
+    kr = _Kahan(dmdt)
     for k in range(nz - 1):
-        run = run + inc[k]
+        run = kr.add(inc[k]) if prec else run + inc[k]
         ww[k + 1] = -run
     return ww
 
@@ -578,36 +602,48 @@ def small_step_init(s) -> None:
     s.p_pp_old[...] = s.p_pp
 
 
-def calc_coefs(s, dtau, epssm, top_lid=True):
+def calc_coefs(s, dtau, epssm, top_lid=True, prec=False):
     """c2a and the LU factors a/alpha/gam of the implicit w system
-    (reference acoustic.cu:675-759, WRF calc_coef_w).  Returns a dict."""
+    (reference acoustic.cu:675-759, WRF calc_coef_w).  Returns a dict.
+    ``prec``: the precision form -- inputs formed in the state's precision
+    as usual, the factors evaluated in float64 (same expression order)."""
     f = s.dtype.type
     nz = s.thp.shape[0]
     mut = s.mub2d + s.mup
     cof = f(0.5) * f(dtau) * f(G) * (f(1.0) + f(epssm))
     cof = cof * cof
-    c2a = f(GAMMA) * s.p / s.alt
-    shp = (nz + 1,) + mut.shape
-    a = np.zeros(shp, s.dtype)
+    c2a_s = f(GAMMA) * s.p / s.alt
     chm = [s.c1h[k] * mut + s.c2h[k] for k in range(nz)]
     cfm = [s.c1f[k] * mut + s.c2f[k] for k in range(nz + 1)]
+    rdn, rdnw = s.rdn, s.rdnw
+    if prec:
+        f = np.float64
+        cof = f(cof)
+        c2a = c2a_s.astype(f)
+        chm = [x.astype(f) for x in chm]
+        cfm = [x.astype(f) for x in cfm]
+        rdn, rdnw = rdn.astype(f), rdnw.astype(f)
+    else:
+        c2a = c2a_s
+    shp = (nz + 1,) + mut.shape
+    a = np.zeros(shp, c2a.dtype)
     for k in range(2, nz):
-        a[k] = (-cof * s.rdn[k] * s.rdnw[k - 1] * c2a[k - 1]
+        a[k] = (-cof * rdn[k] * rdnw[k - 1] * c2a[k - 1]
                 / (chm[k - 1] * cfm[k - 1]))
     if not top_lid:
-        a[nz] = (f(-2.0) * cof * s.rdnw[nz - 1] * s.rdnw[nz - 1]
+        a[nz] = (f(-2.0) * cof * rdnw[nz - 1] * rdnw[nz - 1]
                  * c2a[nz - 1] / (chm[nz - 1] * cfm[nz - 1]))
-    b = np.zeros(shp, s.dtype)
-    cc = np.zeros(shp, s.dtype)
+    b = np.zeros(shp, c2a.dtype)
+    cc = np.zeros(shp, c2a.dtype)
     for k in range(1, nz):
-        b[k] = f(1.0) + cof * s.rdn[k] * (
-            s.rdnw[k] * c2a[k] / (chm[k] * cfm[k])
-            + s.rdnw[k - 1] * c2a[k - 1] / (chm[k - 1] * cfm[k]))
-        cc[k] = -cof * s.rdn[k] * s.rdnw[k] * c2a[k] / (chm[k] * cfm[k + 1])
-    b[nz] = f(1.0) + f(2.0) * cof * s.rdnw[nz - 1] * s.rdnw[nz - 1] \
+        b[k] = f(1.0) + cof * rdn[k] * (
+            rdnw[k] * c2a[k] / (chm[k] * cfm[k])
+            + rdnw[k - 1] * c2a[k - 1] / (chm[k - 1] * cfm[k]))
+        cc[k] = -cof * rdn[k] * rdnw[k] * c2a[k] / (chm[k] * cfm[k + 1])
+    b[nz] = f(1.0) + f(2.0) * cof * rdnw[nz - 1] * rdnw[nz - 1] \
         * c2a[nz - 1] / (chm[nz - 1] * cfm[nz])
     alpha, gam = lu_factors(a, b, cc)
-    return {"c2a": c2a, "a": a, "alpha": alpha, "gam": gam}
+    return {"c2a": c2a_s, "a": a, "alpha": alpha, "gam": gam}
 
 
 def lu_factors(a, b, c):
@@ -694,7 +730,7 @@ def advance_uv(s, dtau, smdiv, dx, dy, top_lid=True):
         pp += dtau * (rt - dpxy)
 
 
-def advance_mu_th(s, dtau, dx, dy):
+def advance_mu_th(s, dtau, dx, dy, prec=False):
     """mu'', Omega'' and (mu theta)'' forward step plus history saves
     (reference acoustic.cu:329-456)."""
     f = s.dtype.type
@@ -721,16 +757,23 @@ def advance_mu_th(s, dtau, dx, dy):
     dnw = s.dnw[:, None, None]
     dmdt_pp = np.zeros_like(mut)
     dmdt_rf = np.zeros_like(mut)
+    kp, kr = _Kahan(mut), _Kahan(mut)
     for k in range(nz):
-        dmdt_pp = dmdt_pp + dnw[k] * dvp[k]
-        dmdt_rf = dmdt_rf + dnw[k] * dvr[k]
+        if prec:
+            dmdt_pp = kp.add(dnw[k] * dvp[k])
+            dmdt_rf = kr.add(dnw[k] * dvr[k])
+        else:
+            dmdt_pp = dmdt_pp + dnw[k] * dvp[k]
+            dmdt_rf = dmdt_rf + dnw[k] * dvr[k]
     rmu = s.rmu_t
     s.mu_pp[...] = s.mu_pp + dtau * ((dmdt_pp + dmdt_rf) + rmu)
     ww = np.zeros_like(s.ww_pp)
     wwk = np.zeros_like(mut)
+    kw = _Kahan(mut)
     for kk in range(1, nz):
         k = kk - 1
-        wwk = wwk - dnw[k] * (c1h[k] * (dmdt_pp + rmu) + dvp[k])
+        inc = dnw[k] * (c1h[k] * (dmdt_pp + rmu) + dvp[k])
+        wwk = kw.add(-inc) if prec else wwk - inc
         ww[kk] = wwk
     s.ww_pp[...] = ww
     th = _b3(s.thb) + s.thp
@@ -814,7 +857,10 @@ def advance_w_phi(s, coefs, dtau, epssm, dx, dy, top_lid=True,
         cterm = dtau * g * (f(2.0) * s.rdnw[nz - 1] * c2a[nz - 1]
                             * s.alt[nz - 1] * t2[nz - 1] + s.c1f[nz] * muave)
         w[nz] = w[nz] + ((dtau * s.rw_t[nz] - bterm) - cterm)
-    thomas_solve(w, a, alpha, gam)
+    if a.dtype != w.dtype:          # precision form: FP64 tridiagonal
+        w = thomas_solve(w.astype(a.dtype), a, alpha, gam).astype(w.dtype)
+    else:
+        thomas_solve(w, a, alpha, gam)
     if dampmag > 0.0:
         phf = (s.phb if s.base3d else s.phb[:, None, None]) + s.php
         htop = phf[nz] / g
@@ -853,7 +899,9 @@ def acoustic_substep(s, coefs, dtau, cfg_like, first):
     smdiv = 0.0 if first else c.smdiv
     d32 = float(s.dtype.type(dtau))
     advance_uv(s, d32, smdiv, c.dx, c.dy, c.top_lid)
-    advance_mu_th(s, d32, c.dx, c.dy)
+    advance_mu_th(s, d32, c.dx, c.dy,
+                  prec=bool(getattr(c, "precision_form", False))
+                  and s.dtype == np.float32)
     dampmag = float(s.dtype.type(dtau * c.dampcoef)) \
         if c.damp_opt == 3 else 0.0
     advance_w_phi(s, coefs, d32, c.epssm, c.dx, c.dy, c.top_lid,
@@ -956,13 +1004,15 @@ def step(s, cfg) -> None:
         for name in ("ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t"):
             getattr(s, name)[...] = 0.0
         calc_p_alpha(s)
-        ru, rv, ww = stage_fluxes(s, cfg.dx, cfg.dy)
+        prec = bool(getattr(cfg, "precision_form", False)) \
+            and s.dtype == np.float32
+        ru, rv, ww = stage_fluxes(s, cfg.dx, cfg.dy, prec)
         add_slow_tendencies(s, ru, rv, ww, cfg.dx, cfg.dy, cfg.top_lid)
         if cfg.km_opt == 1:
             add_diffusion_tendencies(s, cfg.khdif, cfg.kvdif, cfg.dx, cfg.dy)
         small_step_init(s)
         dtau32 = float(f(dtau))
-        coefs = calc_coefs(s, dtau, cfg.epssm, cfg.top_lid)
+        coefs = calc_coefs(s, dtau, cfg.epssm, cfg.top_lid, prec)
         if moist:
             ru_m = np.zeros_like(s.u)
             rv_m = np.zeros_like(s.v)

@@ -16,7 +16,7 @@ HEADER = ROOT / "include" / "arwen_dycore.h"
 def _declared():
     text = HEADER.read_text()
     return sorted(set(re.findall(
-        r"^(?:int32_t|const char \*|void|double)\s*\*?\s*(arw_\w+)\(",
+        r"^(?:int32_t|const char \*|void|double)\s*\*?\s*(arw(?:64)?_\w+)\(",
         text, flags=re.M)))
 
 
@@ -71,6 +71,7 @@ def _struct_fields(name):
 
 
 @pytest.mark.parametrize("cname,pyt", [("arw_domain", _lib.ArwDomain),
+                                       ("arw64_domain", _lib.ArwDomain64),
                                        ("arw_params", _lib.ArwParams)])
 def test_ctypes_layout_matches_header(cname, pyt):
     kinds = {"ptr": ctypes.c_void_p, "int32_t": ctypes.c_int32,

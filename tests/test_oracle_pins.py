@@ -118,8 +118,8 @@ MISTAKES = {
     "large-step PGF dropped": lambda mp: mp.setattr(
         O, "slow_pgf", lambda *a, **k: None),
     "theta'' update dropped": lambda mp: mp.setattr(
-        O, "advance_mu_th", lambda s, dtau, dx, dy: _ORIG["advance_mu_th"](
-            s, 0.0, dx, dy)),
+        O, "advance_mu_th", lambda s, dtau, dx, dy, prec=False:
+        _ORIG["advance_mu_th"](s, 0.0, dx, dy, prec)),
 }
 _ORIG = {n: getattr(O, n) for n in ("omega_ref", "advance_uv",
                                     "advance_w_phi", "advance_mu_th")}
