@@ -64,6 +64,17 @@ class ArwParams(ctypes.Structure):
                 ("precision_form", _I)]
 
 
+class ArwTracerCkpt(ctypes.Structure):
+    """arw_tracer_ckpt: per-stage sumflux averages, post-acoustic mu' and
+    stage lengths of one step, plus the time-t mu'."""
+    _fields_ = [("ru_m", _P * 3), ("rv_m", _P * 3), ("ww_m", _P * 3),
+                ("mup_new", _P * 3), ("mup_t", _P), ("dt", _D * 3)]
+
+
+#: arw64_tracer_ckpt has the same layout (pointers and doubles)
+ArwTracerCkpt64 = ArwTracerCkpt
+
+
 class ArwenError(RuntimeError):
     """A library entry point returned a nonzero status."""
 
@@ -100,6 +111,11 @@ def load():
         "arw_profile_end": ([_P, _P, _I], _I),
         "arw_kernel_name": ([_I], ctypes.c_char_p),
         "arw_kernel_count": ([], _I),
+        "arw_step_ckpt": ([dp, pp, ctypes.POINTER(ArwTracerCkpt), _P], _I),
+        "arw_tracer_step": ([dp, pp, ctypes.POINTER(ArwTracerCkpt), _P, _P,
+                             _P, _I, _P], _I),
+        "arw_tracer_adjoint": ([dp, pp, ctypes.POINTER(ArwTracerCkpt), _P,
+                                _P, _P, _P], _I),
     }
     flux = [_P, _P, _P, _P, _P, _P, _P, _P, _F, _F, _I, _I, _I, _P]
     for name in ("arw_flux_div_scalar", "arw_flux_div_u", "arw_flux_div_v",
@@ -141,7 +157,8 @@ FP64_ENTRY_POINTS = (
     "arw64_calc_p_alpha", "arw64_stage_fluxes", "arw64_stage_prepare",
     "arw64_acoustic_substep", "arw64_calc_coefs", "arw64_small_step_finish",
     "arw64_set_w_surface", "arw64_health", "arw64_domain_mass",
-    "arw64_flux_div_scalar")
+    "arw64_flux_div_scalar", "arw64_step_ckpt", "arw64_tracer_step",
+    "arw64_tracer_adjoint")
 
 
 def exported_symbols() -> list[str]:
@@ -152,7 +169,8 @@ def exported_symbols() -> list[str]:
             "arw_set_w_surface", "arw_health", "arw_domain_mass",
             "arw_flux_div_scalar", "arw_flux_div_u", "arw_flux_div_v",
             "arw_flux_div_w", "arw_profile_begin", "arw_profile_end",
-            "arw_kernel_name", "arw_kernel_count"]
+            "arw_kernel_name", "arw_kernel_count", "arw_step_ckpt",
+            "arw_tracer_step", "arw_tracer_adjoint"]
 
 
 class KernelTimer:

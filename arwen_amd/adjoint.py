@@ -1,0 +1,147 @@
+"""Scalar-transport adjoint with per-RK-stage checkpoints.
+
+The dycore step moves a passive scalar (qv and friends; WRF
+rk_scalar_tend + rk_update_scalar, reference moist.py:385-398 and
+:434-698) with the substep-averaged mass fluxes of each RK stage.  For
+given fluxes and column masses that transport is linear in the scalar, so
+one step is q_{n+1} = J q_n with J fixed by the step's dynamics.
+
+``step_with_checkpoints`` runs the ordinary step and records what J needs:
+per stage the averaged fluxes ru_m, rv_m, ww_m, the post-acoustic mu' and
+the stage length, plus the time-t mu'.  ``TracerStep`` is the
+``torch.autograd.Function`` over J: forward launches the HIP transport
+(``arw_tracer_step``, bit-identical to the update inside the step),
+backward the hand-written transpose (``arw_tracer_adjoint``).  The
+reference has no adjoint; these entry points are new.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from .config import RunConfig, validate_for_step
+from .dycore import _check_state, _fn, _stream, _uses_diffusion, params
+from .state import DomainState
+
+
+class TracerCheckpoints:
+    """Device buffers of one step's stage checkpoints (about 9 fields of
+    the state's size) and the ctypes view the library fills."""
+
+    def __init__(self, state: DomainState):
+        nz, ny, nx = state.shape
+        dev, dt = state.device, state.dtype
+
+        def z(*shape):
+            return torch.zeros(shape, dtype=dt, device=dev)
+
+        self.ru_m = [z(nz, ny, nx + 1) for _ in range(3)]
+        self.rv_m = [z(nz, ny + 1, nx) for _ in range(3)]
+        self.ww_m = [z(nz + 1, ny, nx) for _ in range(3)]
+        self.mup_new = [z(ny, nx) for _ in range(3)]
+        self.mup_t = z(ny, nx)
+        self.shape = (nz, ny, nx)
+        self.dtype = dt
+        c = _lib.ArwTracerCkpt()
+        for s in range(3):
+            c.ru_m[s] = self.ru_m[s].data_ptr()
+            c.rv_m[s] = self.rv_m[s].data_ptr()
+            c.ww_m[s] = self.ww_m[s].data_ptr()
+            c.mup_new[s] = self.mup_new[s].data_ptr()
+        c.mup_t = self.mup_t.data_ptr()
+        self.c = c
+
+    @property
+    def dt(self) -> tuple[float, float, float]:
+        """Stage lengths nsub * dtau (dt/3, dt/2, dt for even ns)."""
+        return tuple(self.c.dt)
+
+
+def step_with_checkpoints(state: DomainState, cfg: RunConfig,
+                          ckpt: TracerCheckpoints | None = None
+                          ) -> TracerCheckpoints:
+    """``dycore.step(state, cfg)`` that also records the stage checkpoints
+    of the scalar transport.  The state advances exactly as in ``step``."""
+    validate_for_step(cfg, acoustic=True)
+    if bool(cfg.moist) != (state.qv is not None):
+        raise ValueError("cfg.moist must match the state")
+    _check_state(state)
+    if ckpt is None:
+        ckpt = TracerCheckpoints(state)
+    if ckpt.shape != state.shape or ckpt.dtype != state.dtype:
+        raise ValueError("checkpoint buffers do not match the state")
+    d = state.domain_struct(diffusion=_uses_diffusion(cfg),
+                            precision_form=bool(cfg.precision_form))
+    name, fn = _fn(state, "step_ckpt")
+    _lib.check(fn(d, params(cfg), ctypes.byref(ckpt.c), _stream()), name)
+    state.elapsed_seconds += cfg.dt
+    return ckpt
+
+
+def _tracer_call(state, cfg, ckpt, kind, src, dst, clamp=False):
+    n = src.numel()
+    work = torch.empty((3 if kind == "tracer_adjoint" else 2) * n,
+                       dtype=state.dtype, device=state.device)
+    name, fn = _fn(state, kind)
+    d, p = state.domain_struct(), params(cfg)
+    if kind == "tracer_step":
+        rc = fn(d, p, ctypes.byref(ckpt.c), src.data_ptr(), dst.data_ptr(),
+                work.data_ptr(), int(bool(clamp)), _stream())
+    else:
+        rc = fn(d, p, ctypes.byref(ckpt.c), src.data_ptr(), dst.data_ptr(),
+                work.data_ptr(), _stream())
+    _lib.check(rc, name)
+    return dst
+
+
+def _as_field(x: torch.Tensor, state: DomainState) -> torch.Tensor:
+    if tuple(x.shape) != state.shape:
+        raise ValueError(f"scalar field shape {tuple(x.shape)} != "
+                         f"{state.shape}")
+    if x.dtype != state.dtype or x.device != state.device:
+        raise ValueError("scalar field must match the state's dtype/device")
+    return x.contiguous()
+
+
+def tracer_step(q: torch.Tensor, state: DomainState, cfg: RunConfig,
+                ckpt: TracerCheckpoints, clamp: bool = False) -> torch.Tensor:
+    """J q: one step of the unlimited scalar transport recorded in
+    ``ckpt`` (the final-stage clamp at zero when ``clamp``)."""
+    _check_state(state)
+    q = _as_field(q, state)
+    return _tracer_call(state, cfg, ckpt, "tracer_step", q,
+                        torch.empty_like(q), clamp)
+
+
+def tracer_adjoint(lam: torch.Tensor, state: DomainState, cfg: RunConfig,
+                   ckpt: TracerCheckpoints) -> torch.Tensor:
+    """J^T lam (the clamp, a no-op on positive fields, is not
+    differentiated)."""
+    _check_state(state)
+    lam = _as_field(lam, state)
+    return _tracer_call(state, cfg, ckpt, "tracer_adjoint", lam,
+                        torch.empty_like(lam))
+
+
+class TracerStep(torch.autograd.Function):
+    """q -> J q with the hand-written adjoint as its backward."""
+
+    @staticmethod
+    def forward(ctx, q, state, cfg, ckpt):
+        ctx.state, ctx.cfg, ctx.ckpt = state, cfg, ckpt
+        return tracer_step(q.detach(), state, cfg, ckpt)
+
+    @staticmethod
+    def backward(ctx, grad):
+        return (tracer_adjoint(grad, ctx.state, ctx.cfg, ctx.ckpt), None,
+                None, None)
+
+
+def transport(q: torch.Tensor, state: DomainState, cfg: RunConfig,
+              ckpts: list[TracerCheckpoints]) -> torch.Tensor:
+    """Differentiable multi-step transport: J_n ... J_1 q."""
+    for c in ckpts:
+        q = TracerStep.apply(q, state, cfg, c)
+    return q

@@ -22,12 +22,14 @@
 #if ARW_F64
 typedef double real;
 typedef arw64_domain Dom;
+typedef arw64_tracer_ckpt Ckpt;
 #define R(x) x
 #define ARW_NS prec64
 #define ARW_API(name) arw64_##name
 #else
 typedef float real;
 typedef arw_domain Dom;
+typedef arw_tracer_ckpt Ckpt;
 #define R(x) x##f
 #define ARW_NS prec32
 #define ARW_API(name) arw_##name

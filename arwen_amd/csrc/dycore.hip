@@ -571,6 +571,146 @@ __global__ void k_pd_apply(Dom d, const real* __restrict__ q0,
 }
 
 // ---------------------------------------------------------------------------
+// Scalar-transport tangent and adjoint (stage checkpoints)
+// ---------------------------------------------------------------------------
+
+// Post-acoustic mu' of the stage into a checkpoint slot.
+__global__ void k_mup_new(Dom d, real* __restrict__ out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < d.ny * d.nx) out[c] = mup_new_at(d, c);
+}
+
+// One stage of the unlimited transport with explicit checkpointed
+// fluxes and masses: the arithmetic of k_scalar_update.
+__global__ void k_tracer_stage(Dom d, const real* __restrict__ qcur,
+                               const real* __restrict__ q0,
+                               real* __restrict__ qdst,
+                               const real* __restrict__ ru,
+                               const real* __restrict__ rv,
+                               const real* __restrict__ ww,
+                               const real* __restrict__ mup_t,
+                               const real* __restrict__ mup_new, real dt,
+                               int clamp, real rdx, real rdy)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    Arr3 Q{qcur, ny, nx}, RU{ru, ny, nx + 1}, RV{rv, ny + 1, nx},
+        RW{ww, ny, nx};
+    const real tend = R(0.0) + adv_scalar(Q, RU, RV, RW, k, j, i, nz, ny, nx,
+                                         d.rdnw, d.fnm, d.fnp, rdx, rdy);
+    const int x = iM(k, j, i, ny, nx), c = j * nx + i;
+    const real mu0 = d.mub2d[c] + mup_t[c];
+    const real mu = d.mub2d[c] + mup_new[c];
+    const real chm0 = d.c1h[k] * mu0 + d.c2h[k];
+    const real chm = d.c1h[k] * mu + d.c2h[k];
+    real v = chm0 * q0[x];
+    const real t = dt * tend;
+    v = v + t;
+    v = v / chm;
+    if (clamp && !(v > R(0.0)) && v == v) v = R(0.0);
+    qdst[x] = v;
+}
+
+// Adjoint stage, part 1: m = g / C(mu_new) (the adjoint of the division)
+// and gq (+)= C(mu_t) m (the adjoint of the time-t term).
+__global__ void k_tracer_adj_scale(Dom d, const real* __restrict__ g,
+                                   real* __restrict__ m, real* __restrict__ gq,
+                                   const real* __restrict__ mup_t,
+                                   const real* __restrict__ mup_new, int first)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    const int x = iM(k, j, i, ny, nx), c = j * nx + i;
+    const real chm0 = d.c1h[k] * (d.mub2d[c] + mup_t[c]) + d.c2h[k];
+    const real chm = d.c1h[k] * (d.mub2d[c] + mup_new[c]) + d.c2h[k];
+    const real mm = g[x] / chm;
+    m[x] = mm;
+    gq[x] = (first ? R(0.0) : gq[x]) + chm0 * mm;
+}
+
+// flux5 / flux3 coefficient of the o-th stencil cell (flux5: qm3..qp2,
+// flux3: qm2..qp1), for the transposed gathers.
+__device__ __forceinline__ real w5(int o, real v)
+{
+    const real a = fabs(v);
+    switch (o) {
+    case 0: return (v + a) / R(60.0);
+    case 1: return (-R(8.0) * v - R(5.0) * a) / R(60.0);
+    case 2: return (R(37.0) * v + R(10.0) * a) / R(60.0);
+    case 3: return (R(37.0) * v - R(10.0) * a) / R(60.0);
+    case 4: return (-R(8.0) * v + R(5.0) * a) / R(60.0);
+    default: return (v - a) / R(60.0);
+    }
+}
+
+__device__ __forceinline__ real w3(int o, real v)
+{
+    const real a = fabs(v);
+    switch (o) {
+    case 0: return (-v + a) / R(12.0);
+    case 1: return (R(7.0) * v - R(3.0) * a) / R(12.0);
+    case 2: return (R(7.0) * v + R(3.0) * a) / R(12.0);
+    default: return (-v - a) / R(12.0);
+    }
+}
+
+// Adjoint stage, part 2: gout = dt * D^T m, D the flux-form divergence
+// of flux_div_scalar.  Face f (between cells f-1 and f) carries the
+// adjoint Lambda_f = rdx (m[f] - m[f-1]); cell c gathers it with the
+// weight it had in face f's stencil.
+__global__ void k_tracer_adj_div(Dom d, const real* __restrict__ m,
+                                 real* __restrict__ gout,
+                                 const real* __restrict__ ru,
+                                 const real* __restrict__ rv,
+                                 const real* __restrict__ ww, real dt,
+                                 real rdx, real rdy)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    auto M = [&](int kk, int jj, int ii) { return m[iM(kk, jj, ii, ny, nx)]; };
+    real acc = R(0.0);
+    for (int o = 0; o < 6; ++o) {
+        // x face f reads cell f + o - 3 == i
+        const int f = pmod(i + 3 - o, nx);
+        const real lx = rdx * (M(k, j, f) - M(k, j, pmod(f - 1, nx)));
+        acc += w5(o, ru[iU(k, j, f, ny, nx)]) * lx;
+        const int g = pmod(j + 3 - o, ny);
+        const real ly = rdy * (M(k, g, i) - M(k, pmod(g - 1, ny), i));
+        acc += w5(o, rv[iV(k, g, i, ny, nx)]) * ly;
+    }
+    // eta faces kf = 1..nz-1 touching cell k
+    for (int kf = k - 1; kf <= k + 2; ++kf) {
+        if (kf < 1 || kf > nz - 1) continue;
+        const real lz = d.rdnw[kf] * M(kf, j, i) - d.rdnw[kf - 1] * M(kf - 1, j, i);
+        const real vel = ww[iM(kf, j, i, ny, nx)];
+        if (kf == 1 || kf == nz - 1) {
+            if (k == kf) acc += vel * d.fnm[kf] * lz;
+            else if (k == kf - 1) acc += vel * d.fnp[kf] * lz;
+        } else {
+            acc += w3(k - kf + 2, vel) * lz;   // face kf reads cell kf+o-2
+        }
+    }
+    gout[iM(k, j, i, ny, nx)] = dt * acc;
+}
+
+__global__ void k_add3(real* __restrict__ out, const real* __restrict__ a,
+                       const real* __restrict__ b, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = a[t] + b[t];
+}
+
+// ---------------------------------------------------------------------------
 // Reductions
 // ---------------------------------------------------------------------------
 
@@ -966,13 +1106,22 @@ int32_t ARW_API(set_w_surface)(const Dom* d, const arw_params* prm,
 // mup_next -> A because the finish reads neighbouring mu' while it writes.
 // After the step the *0 arrays hold the stage-2 estimate (they are
 // rebuilt every step, like the reference's time-t copies).
-int32_t ARW_API(step)(const Dom* d, const arw_params* prm, void* stream)
+// ck (optional) receives the stage checkpoints of the scalar transport:
+// the sumflux averages accumulate straight into its per-stage buffers.
+static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
+                         void* stream)
 {
     int32_t rc = check_domain(d);
     if (rc) return rc;
     if ((rc = check_params(prm, d))) return rc;
     if (!ARW_F64 && prm->precision_form && !d->tri64)
         return fail(ARW_E_ARG, "precision form needs the FP64 tridiagonal scratch tri64");
+    if (ck) {
+        if (!ck->mup_t) return fail(ARW_E_ARG, "checkpoint: null mup_t");
+        for (int s = 0; s < 3; ++s)
+            if (!ck->ru_m[s] || !ck->rv_m[s] || !ck->ww_m[s] || !ck->mup_new[s])
+                return fail(ARW_E_ARG, "checkpoint: null stage buffer");
+    }
     hipStream_t st = (hipStream_t)stream;
     prof_mark(KID_COUNT, st);                    // opening event
     const int ns = prm->time_step_sound;
@@ -985,6 +1134,10 @@ int32_t ARW_API(step)(const Dom* d, const arw_params* prm, void* stream)
     real* QA[3] = {d->qv, d->qc, d->qr};
     real* QB[3] = {d->qv0, d->qc0, d->qr0};
     real* QC[3] = {d->qsv, d->qsc, d->qsr};
+    const bool sum = moist || ck;                // substep flux averages
+    if (ck && hipMemcpyAsync(ck->mup_t, A[5], (size_t)d->ny * d->nx * RB,
+                             hipMemcpyDeviceToDevice, st) != hipSuccess)
+        return fail(ARW_E_LAUNCH, "checkpoint mu' copy failed");
     for (int stage = 0; stage < 3; ++stage) {
         Dom dd = *d;
         // time-t fields: always A
@@ -1004,6 +1157,11 @@ int32_t ARW_API(step)(const Dom* d, const arw_params* prm, void* stream)
             real* const* Q = (stage == 0) ? QA : (stage == 1 ? QB : QC);
             dd.qv = Q[0]; dd.qc = Q[1]; dd.qr = Q[2];
         }
+        if (ck) {
+            dd.ru_m = ck->ru_m[stage]; dd.rv_m = ck->rv_m[stage];
+            dd.ww_m = ck->ww_m[stage];
+            ck->dt[stage] = nsub[stage] * dtaus[stage];
+        }
         Sc s = make_sc(prm, dtaus[stage], 1);
         if ((rc = stage_prepare(dd, s, stage == 0, st))) return rc;
         Dom ds = dd;
@@ -1013,10 +1171,10 @@ int32_t ARW_API(step)(const Dom* d, const arw_params* prm, void* stream)
             const bool last = (n + 1 == nsub[stage]);
             // new p'' goes to the buffer holding the dead p''(n-1); the two
             // names swap afterwards
-            if ((rc = substep(ds, sn, ds.p_pp_old, ds.base3d || last, moist,
+            if ((rc = substep(ds, sn, ds.p_pp_old, ds.base3d || last, sum,
                               st)))
                 return rc;
-            if (moist) {
+            if (sum) {
                 hipLaunchKernelGGL(k_sumflux, union_grid(*d, d->nz + 1),
                                    dim3(64, 4), 0, st, ds, n == 0);
                 if ((rc = launched(hipGetLastError(), "k_sumflux", KID_SCALARS, st)))
@@ -1026,7 +1184,7 @@ int32_t ARW_API(step)(const Dom* d, const arw_params* prm, void* stream)
             ds.p_pp = ds.p_pp_old;
             ds.p_pp_old = t;
         }
-        if (moist) {
+        if (sum) {
             // WRF sumflux finish, then the scalar stage update -- both
             // before the finish, which on the last stage overwrites the
             // time-t mass the update divides by
@@ -1034,6 +1192,14 @@ int32_t ARW_API(step)(const Dom* d, const arw_params* prm, void* stream)
                                dim3(64, 4), 0, st, dd, (real)nsub[stage]);
             if ((rc = launched(hipGetLastError(), "k_sumflux_finish", KID_SCALARS, st)))
                 return rc;
+        }
+        if (ck) {
+            hipLaunchKernelGGL(k_mup_new, col_grid(*d), dim3(256), 0, st, dd,
+                               ck->mup_new[stage]);
+            if ((rc = launched(hipGetLastError(), "k_mup_new", KID_SCALARS, st)))
+                return rc;
+        }
+        if (moist) {
             real* const* Q = (stage == 0) ? QA : (stage == 1 ? QB : QC);
             real* const* Dst = (stage == 0) ? QB : (stage == 1 ? QC : QA);
             if ((rc = scalar_stage(dd, prm, Q, QA, Dst, A[5],
@@ -1057,6 +1223,89 @@ int32_t ARW_API(step)(const Dom* d, const arw_params* prm, void* stream)
                        s);
     return launched(hipGetLastError(), "k_set_w_surface", KID_SET_W_SURFACE,
                     st);
+}
+
+int32_t ARW_API(step)(const Dom* d, const arw_params* prm, void* stream)
+{
+    return step_impl(d, prm, nullptr, stream);
+}
+
+int32_t ARW_API(step_ckpt)(const Dom* d, const arw_params* prm, Ckpt* ck,
+                           void* stream)
+{
+    if (!ck) return fail(ARW_E_ARG, "null checkpoint");
+    return step_impl(d, prm, ck, stream);
+}
+
+static int32_t check_tracer(const Dom* d, const arw_params* prm,
+                            const Ckpt* ck, const void* a, const void* b,
+                            const void* work)
+{
+    int32_t rc = check_domain(d);
+    if (rc) return rc;
+    if (!prm || !(prm->dx > 0.0) || !(prm->dy > 0.0))
+        return fail(ARW_E_ARG, "params: dx, dy must be positive");
+    if (!ck || !ck->mup_t) return fail(ARW_E_ARG, "null checkpoint");
+    for (int s = 0; s < 3; ++s)
+        if (!ck->ru_m[s] || !ck->rv_m[s] || !ck->ww_m[s] || !ck->mup_new[s])
+            return fail(ARW_E_ARG, "checkpoint: null stage buffer");
+    if (!a || !b || !work) return fail(ARW_E_ARG, "null scalar or work array");
+    return ARW_OK;
+}
+
+int32_t ARW_API(tracer_step)(const Dom* d, const arw_params* prm,
+                             const Ckpt* ck, const real* q_in, real* q_out,
+                             real* work, int32_t clamp, void* stream)
+{
+    int32_t rc = check_tracer(d, prm, ck, q_in, q_out, work);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = (size_t)d->nz * d->ny * d->nx;
+    const real rdx = (real)(1.0 / prm->dx), rdy = (real)(1.0 / prm->dy);
+    const dim3 cells((d->nx + 63) / 64, (d->ny + 3) / 4, d->nz);
+    const real* cur[3] = {q_in, work, work + n};
+    real* dst[3] = {work, work + n, q_out};
+    for (int s = 0; s < 3; ++s) {
+        hipLaunchKernelGGL(k_tracer_stage, cells, dim3(64, 4), 0, st, *d,
+                           cur[s], q_in, dst[s], ck->ru_m[s], ck->rv_m[s],
+                           ck->ww_m[s], ck->mup_t, ck->mup_new[s],
+                           (real)ck->dt[s], (clamp && s == 2) ? 1 : 0, rdx,
+                           rdy);
+        if ((rc = launched("k_tracer_stage"))) return rc;
+    }
+    return ARW_OK;
+}
+
+// lam_out = J^T lam_in for J = d tracer_step / d q_in (no active clamp).
+// Stage s maps g -> m = g / C(mu_s), gq += C(mu_t) m, g = dt_s D_s^T m,
+// from the last stage back; lam_out = gq + g.
+int32_t ARW_API(tracer_adjoint)(const Dom* d, const arw_params* prm,
+                                const Ckpt* ck, const real* lam_in,
+                                real* lam_out, real* work, void* stream)
+{
+    int32_t rc = check_tracer(d, prm, ck, lam_in, lam_out, work);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = (size_t)d->nz * d->ny * d->nx;
+    const real rdx = (real)(1.0 / prm->dx), rdy = (real)(1.0 / prm->dy);
+    const dim3 cells((d->nx + 63) / 64, (d->ny + 3) / 4, d->nz);
+    real* m = work;
+    real* gq = work + n;
+    real* g = work + 2 * n;
+    for (int s = 2; s >= 0; --s) {
+        hipLaunchKernelGGL(k_tracer_adj_scale, cells, dim3(64, 4), 0, st, *d,
+                           s == 2 ? lam_in : g, m, gq, ck->mup_t,
+                           ck->mup_new[s], s == 2 ? 1 : 0);
+        if ((rc = launched("k_tracer_adj_scale"))) return rc;
+        hipLaunchKernelGGL(k_tracer_adj_div, cells, dim3(64, 4), 0, st, *d, m,
+                           g, ck->ru_m[s], ck->rv_m[s], ck->ww_m[s],
+                           (real)ck->dt[s], rdx, rdy);
+        if ((rc = launched("k_tracer_adj_div"))) return rc;
+    }
+    const int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_add3, dim3(blocks), dim3(256), 0, st, lam_out, gq, g,
+                       (int)n);
+    return launched("k_add3");
 }
 
 int32_t ARW_API(profile_begin)(int32_t max_events)

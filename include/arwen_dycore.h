@@ -139,6 +139,44 @@ const char *arw_last_error(void);
  * like the reference's closing update_diagnostics (dycore.py:2482). */
 int32_t arw_step(const arw_domain *d, const arw_params *prm, void *stream);
 
+/*
+ * Stage checkpoints of one step's scalar transport: per RK stage the
+ * substep-averaged mass fluxes (WRF sumflux: ru_m (nz,ny,nx+1), rv_m
+ * (nz,ny+1,nx), ww_m (nz+1,ny,nx)), the post-acoustic mu' (ny,nx) and
+ * the stage length; mup_t is the time-t mu'.  Filled by arw_step_ckpt,
+ * read by arw_tracer_step / arw_tracer_adjoint.  No reference
+ * counterpart: ArWen has no adjoint (the checkpoints follow WRF's
+ * rk_scalar_tend inputs, reference moist.py:434-698).
+ */
+typedef struct arw_tracer_ckpt {
+    float *ru_m[3], *rv_m[3], *ww_m[3], *mup_new[3];
+    float *mup_t;
+    double dt[3];
+} arw_tracer_ckpt;
+
+/* arw_step that also records the stage checkpoints (the sumflux
+ * averages are accumulated even for a dry domain, which then needs the
+ * ru_m/rv_m/ww_m scratch). */
+int32_t arw_step_ckpt(const arw_domain *d, const arw_params *prm,
+                      arw_tracer_ckpt *ck, void *stream);
+
+/* One RK3 step of unlimited flux-form transport of a scalar (reference
+ * moist.py:385-398 per stage, moist_adv_opt = 0) with the checkpointed
+ * fluxes and masses: q_out from q_in; work holds 2*nz*ny*nx; the final
+ * stage clamps at zero when clamp != 0.  Bit-identical to the scalar
+ * update inside arw_step. */
+int32_t arw_tracer_step(const arw_domain *d, const arw_params *prm,
+                        const arw_tracer_ckpt *ck, const float *q_in,
+                        float *q_out, float *work, int32_t clamp,
+                        void *stream);
+
+/* Hand-written adjoint (transpose) of arw_tracer_step without an active
+ * clamp: lam_out = J^T lam_in; work holds 3*nz*ny*nx.  No reference
+ * counterpart. */
+int32_t arw_tracer_adjoint(const arw_domain *d, const arw_params *prm,
+                           const arw_tracer_ckpt *ck, const float *lam_in,
+                           float *lam_out, float *work, void *stream);
+
 /* EOS p, al, alt from thp, php, mup (replaces diagnostics.cu:37-89
  * calc_p_alpha, launched by diagnostics.py:14-57; hypsometric_opt = 1). */
 int32_t arw_calc_p_alpha(const arw_domain *d, void *stream);
@@ -256,6 +294,22 @@ int32_t arw64_flux_div_scalar(const double *q, const double *ru,
                               const double *fnm, const double *fnp,
                               double dx_inv, double dy_inv, int32_t nz,
                               int32_t ny, int32_t nx, void *stream);
+
+typedef struct arw64_tracer_ckpt {
+    double *ru_m[3], *rv_m[3], *ww_m[3], *mup_new[3];
+    double *mup_t;
+    double dt[3];
+} arw64_tracer_ckpt;
+int32_t arw64_step_ckpt(const arw64_domain *d, const arw_params *prm,
+                        arw64_tracer_ckpt *ck, void *stream);
+int32_t arw64_tracer_step(const arw64_domain *d, const arw_params *prm,
+                          const arw64_tracer_ckpt *ck, const double *q_in,
+                          double *q_out, double *work, int32_t clamp,
+                          void *stream);
+int32_t arw64_tracer_adjoint(const arw64_domain *d, const arw_params *prm,
+                             const arw64_tracer_ckpt *ck,
+                             const double *lam_in, double *lam_out,
+                             double *work, void *stream);
 
 #ifdef __cplusplus
 }

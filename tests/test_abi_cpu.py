@@ -81,3 +81,33 @@ def test_ctypes_layout_matches_header(cname, pyt):
     assert [n for n, _ in have] == [n for n, _ in want]
     assert all(ctypes.sizeof(a) == ctypes.sizeof(b)
                for (_, a), (_, b) in zip(have, want))
+
+
+def _c_layout(struct, fields, tmp_path):
+    """sizeof and offsetof from the C compiler for the header's struct."""
+    import subprocess
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>',
+             '#include "arwen_dycore.h"', 'int main(void) {',
+             f'  printf("%zu\\n", sizeof({struct}));']
+    lines += [f'  printf("%zu\\n", offsetof({struct}, {f}));' for f in fields]
+    lines += ['  return 0;', '}']
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", str(ROOT / "include"), str(src),
+                    "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True,
+                         text=True).stdout.split()
+    return [int(v) for v in out]
+
+
+@pytest.mark.parametrize("cname,pyt", [
+    ("arw_tracer_ckpt", _lib.ArwTracerCkpt),
+    ("arw64_tracer_ckpt", _lib.ArwTracerCkpt64),
+    ("arw_domain", _lib.ArwDomain), ("arw64_domain", _lib.ArwDomain64),
+    ("arw_params", _lib.ArwParams)])
+def test_ctypes_offsets_match_the_c_compiler(cname, pyt, tmp_path):
+    names = [n for n, _ in pyt._fields_]
+    got = _c_layout(cname, names, tmp_path)
+    assert got[0] == ctypes.sizeof(pyt)
+    assert got[1:] == [getattr(pyt, n).offset for n in names]
