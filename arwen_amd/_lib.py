@@ -116,6 +116,12 @@ def load():
                              _P, _I, _P], _I),
         "arw_tracer_adjoint": ([dp, pp, ctypes.POINTER(ArwTracerCkpt), _P,
                                 _P, _P, _P], _I),
+        "arw_ensemble_create": ([dp, _I, pp, _P, _I,
+                                 ctypes.POINTER(ctypes.c_void_p)], _I),
+        "arw_ensemble_step": ([_P, _I, _P], _I),
+        "arw_ensemble_capture": ([_P, _I], _I),
+        "arw_ensemble_launch": ([_P, _P], _I),
+        "arw_ensemble_destroy": ([_P], _I),
     }
     flux = [_P, _P, _P, _P, _P, _P, _P, _P, _F, _F, _I, _I, _I, _P]
     for name in ("arw_flux_div_scalar", "arw_flux_div_u", "arw_flux_div_v",
@@ -158,7 +164,9 @@ FP64_ENTRY_POINTS = (
     "arw64_acoustic_substep", "arw64_calc_coefs", "arw64_small_step_finish",
     "arw64_set_w_surface", "arw64_health", "arw64_domain_mass",
     "arw64_flux_div_scalar", "arw64_step_ckpt", "arw64_tracer_step",
-    "arw64_tracer_adjoint")
+    "arw64_tracer_adjoint", "arw64_ensemble_create", "arw64_ensemble_step",
+    "arw64_ensemble_capture", "arw64_ensemble_launch",
+    "arw64_ensemble_destroy")
 
 
 def exported_symbols() -> list[str]:
@@ -170,7 +178,9 @@ def exported_symbols() -> list[str]:
             "arw_flux_div_scalar", "arw_flux_div_u", "arw_flux_div_v",
             "arw_flux_div_w", "arw_profile_begin", "arw_profile_end",
             "arw_kernel_name", "arw_kernel_count", "arw_step_ckpt",
-            "arw_tracer_step", "arw_tracer_adjoint"]
+            "arw_tracer_step", "arw_tracer_adjoint", "arw_ensemble_create",
+            "arw_ensemble_step", "arw_ensemble_capture",
+            "arw_ensemble_launch", "arw_ensemble_destroy"]
 
 
 class KernelTimer:

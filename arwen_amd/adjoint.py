@@ -100,7 +100,7 @@ def _as_field(x: torch.Tensor, state: DomainState) -> torch.Tensor:
     if tuple(x.shape) != state.shape:
         raise ValueError(f"scalar field shape {tuple(x.shape)} != "
                          f"{state.shape}")
-    if x.dtype != state.dtype or x.device != state.device:
+    if x.dtype != state.dtype or x.device != state.u.device:
         raise ValueError("scalar field must match the state's dtype/device")
     return x.contiguous()
 

@@ -228,6 +228,46 @@ def warm_bubbles(nx=1000, ny=1000, nz=50, dx=1000.0, dt=6.0,
     return case
 
 
+def ensemble_member(nx, ny, nz, seed, dx=3000.0, dt=18.0,
+                    time_step_sound=4, ztop=20000.0, n_bv=0.01, u0=10.0,
+                    scale=30000.0) -> HostCase:
+    """One ensemble member: flat stratified dry state (theta = 300
+    exp(N^2 z/g)), u = u0 + u', v = v', hydrostatically balanced theta',
+    seeded smooth noise (sigma 1 m/s, 1 m/s, 0.5 K at ``scale``), rigid
+    lid with the w damper over the top 5 km."""
+    cfg = RunConfig(nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, ztop=ztop, dt=dt,
+                    time_step_sound=time_step_sound, damp_opt=3,
+                    zdamp=5000.0, dampcoef=0.2, case="ensemble_member")
+    coord = make_vertical_coord(nz)
+    base = make_base_state(coord, stratified_sounding(n_bv), cfg.p_surf,
+                           ztop)
+    rng = np.random.default_rng(seed)
+    th = smooth_noise(rng, (nz, ny, nx), dx, dx, scale, 0.5, 3)
+    case = init_theta_perturbation(cfg, coord, base, lambda x, z: th)
+    case.fields["u"][..., :nx] = u0 + smooth_noise(
+        rng, (nz, ny, nx), dx, dx, scale, 1.0, 3)
+    case.fields["v"][:, :ny, :] = smooth_noise(
+        rng, (nz, ny, nx), dx, dx, scale, 1.0, 3)
+    set_periodic_duplicates(case.fields)
+    return case
+
+
+def ensemble_shapes(n=32, seed=20260731, nxy=(96, 256), nzr=(40, 60)):
+    """Member extents (nx, ny, nz): nx, ny ~ U{96..256}, nz ~ U{40..60}
+    from ``seed``."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.integers(nxy[0], nxy[1] + 1)),
+             int(rng.integers(nxy[0], nxy[1] + 1)),
+             int(rng.integers(nzr[0], nzr[1] + 1))) for _ in range(n)]
+
+
+def ensemble_members(n=32, seed=20260731, **kw) -> list:
+    """The uneven 32-member ensemble: shapes from ensemble_shapes, member
+    i seeded with seed + 1 + i."""
+    return [ensemble_member(nx, ny, nz, seed + 1 + i, **kw)
+            for i, (nx, ny, nz) in enumerate(ensemble_shapes(n, seed))]
+
+
 def init_moist_balanced(cfg: RunConfig, coord: VerticalCoord,
                         base: BaseState, qv3: np.ndarray,
                         thp: np.ndarray | None = None) -> HostCase:

@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include <new>
+#include <vector>
 
 namespace ARW_NS {
 
@@ -297,10 +298,14 @@ __global__ void k_eos(Dom d)
     for (int k = 0; k < d.nz; ++k) eos_level(d, k, c, st, mu);
 }
 
-__global__ void k_set_w_surface(Dom d, Sc s)
+template <class SRC>
+__global__ void k_set_w_surface(typename SRC::Arg a, Sc s)
 {
+    int m, g;
+    if (!SRC::locate(a, m, g)) return;
+    const Dom& d = SRC::dom(a, m);
     const int ny = d.ny, nx = d.nx;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = g * blockDim.x + threadIdx.x;
     if (c >= ny * nx) return;
     int j = c / nx, i = c - j * nx;
     int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
@@ -932,11 +937,98 @@ int32_t scalar_stage(const Dom& dd, const arw_params* prm, real* const* cur,
     return ARW_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Ensemble launch sequences
+// ---------------------------------------------------------------------------
+
+// Member views: per stage s the stage view (role 3s) and the substep views
+// with p''/p''_old in place (3s+1) and swapped (3s+2), then the time-level
+// view for the lower boundary (ENS_BASE).
+enum { ENS_BASE = 9, ENS_ROLES = 10 };
+
+struct EnsGroup {
+    BatchSet sets[ENS_ROLES];
+};
+
+// The views arw_step's stage loop forms for one member (same rotation).
+void member_views(const Dom& d, Dom* role, FinishDst* fin)
+{
+    real* A[6] = {d.u, d.v, d.w, d.thp, d.php, d.mup};
+    real* B[6] = {d.u0, d.v0, d.w0, d.thp0, d.php0, d.mup0};
+    real* mup_c = d.mup_next;
+    for (int stage = 0; stage < 3; ++stage) {
+        Dom dd = d;
+        dd.u0 = A[0]; dd.v0 = A[1]; dd.w0 = A[2]; dd.thp0 = A[3];
+        dd.php0 = A[4]; dd.mup0 = A[5];
+        real* const* T = (stage == 0) ? A : B;
+        dd.u = T[0]; dd.v = T[1]; dd.w = T[2]; dd.thp = T[3]; dd.php = T[4];
+        dd.mup = (stage == 0) ? A[5] : (stage == 1 ? B[5] : mup_c);
+        real* const* D = (stage == 2) ? A : B;
+        FinishDst& o = fin[stage];
+        o.u = D[0]; o.v = D[1]; o.w = D[2]; o.thp = D[3]; o.php = D[4];
+        o.mup = (stage == 0) ? B[5] : (stage == 1 ? mup_c : A[5]);
+        role[3 * stage] = dd;
+        role[3 * stage + 1] = dd;
+        Dom sw = dd;
+        sw.p_pp = dd.p_pp_old;
+        sw.p_pp_old = dd.p_pp;
+        role[3 * stage + 2] = sw;
+    }
+    role[ENS_BASE] = d;
+}
+
+// One step of one group: the launch sequence of arw_step, each launch
+// covering every member of the group.
+int32_t ens_issue(const EnsGroup& G, const arw_params* prm, bool b3,
+                  hipStream_t st)
+{
+    const int ns = prm->time_step_sound;
+    const int nsub[3] = {1, ns / 2 > 1 ? ns / 2 : 1, ns};
+    const double dtaus[3] = {prm->dt / 3.0, prm->dt / ns, prm->dt / ns};
+    int32_t rc;
+    for (int stage = 0; stage < 3; ++stage) {
+        const BatchSet& DD = G.sets[3 * stage];
+        Sc s = make_sc(prm, dtaus[stage], 1);
+        if ((rc = launched(run_omega_b(DD, s, stage == 0, b3, st), "k_omega (ensemble)", -1, st)))
+            return rc;
+        if ((rc = launched(run_tend_faces_b(DD, s, b3, st), "k_tend_faces (ensemble)", -1, st)))
+            return rc;
+        if ((rc = launched(run_tend_cols_b(DD, s, b3, st), "k_tend_cols (ensemble)", -1, st)))
+            return rc;
+        for (int n = 0; n < nsub[stage]; ++n) {
+            Sc sn = make_sc(prm, dtaus[stage], n == 0);
+            sn.use_pold = (n > 0);
+            const bool last = (n + 1 == nsub[stage]);
+            const BatchSet& DS = G.sets[3 * stage + 1 + (n & 1)];
+            if ((rc = launched(run_uv_b(DS, sn, b3, st), "k_uv (ensemble)", -1, st)))
+                return rc;
+            if ((rc = launched(run_column_b(DS, sn, b3, b3 || last, st), "k_column (ensemble)", -1, st)))
+                return rc;
+        }
+        if ((rc = launched(run_finish_b(DD, b3, st), "k_finish (ensemble)", -1, st)))
+            return rc;
+    }
+    Sc s = make_sc(prm, prm->dt, 1);
+    const Batch& bc = G.sets[ENS_BASE].k[BK_COLS];
+    hipLaunchKernelGGL(k_set_w_surface<Many>, dim3(xcd_grid((unsigned)bc.total)),
+                       dim3(256), 0, st, bc, s);
+    return launched(hipGetLastError(), "k_set_w_surface (ensemble)", -1, st);
+}
+
 }  // namespace
 
 }  // namespace ARW_NS
 
 using namespace ARW_NS;
+
+struct ARW_API(ensemble) {
+    int n = 0;
+    arw_params prm{};
+    bool b3 = false;
+    void* dev = nullptr;                  // views and work tables
+    std::vector<EnsGroup> groups;
+    hipGraphExec_t exec = nullptr;
+};
 
 extern "C" {
 
@@ -1093,7 +1185,7 @@ int32_t ARW_API(set_w_surface)(const Dom* d, const arw_params* prm,
     if (rc) return rc;
     if (!prm) return fail(ARW_E_ARG, "null params");
     Sc s = make_sc(prm, prm->dt, 1);
-    hipLaunchKernelGGL(k_set_w_surface, col_grid(*d), dim3(256), 0,
+    hipLaunchKernelGGL(k_set_w_surface<One>, col_grid(*d), dim3(256), 0,
                        (hipStream_t)stream, *d, s);
     return launched("k_set_w_surface");
 }
@@ -1219,7 +1311,7 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
         }
     }
     Sc s = make_sc(prm, prm->dt, 1);
-    hipLaunchKernelGGL(k_set_w_surface, col_grid(*d), dim3(256), 0, st, *d,
+    hipLaunchKernelGGL(k_set_w_surface<One>, col_grid(*d), dim3(256), 0, st, *d,
                        s);
     return launched(hipGetLastError(), "k_set_w_surface", KID_SET_W_SURFACE,
                     st);
@@ -1306,6 +1398,222 @@ int32_t ARW_API(tracer_adjoint)(const Dom* d, const arw_params* prm,
     hipLaunchKernelGGL(k_add3, dim3(blocks), dim3(256), 0, st, lam_out, gq, g,
                        (int)n);
     return launched("k_add3");
+}
+
+int32_t ARW_API(ensemble_create)(const Dom* members, int32_t n,
+                                 const arw_params* prm, const int32_t* order,
+                                 int32_t groups, ARW_API(ensemble)** out)
+{
+    if (!out) return fail(ARW_E_ARG, "null output handle");
+    *out = nullptr;
+    if (!members || n < 1 || n > 65536)
+        return fail(ARW_E_ARG, "need 1 <= n <= 65536 members");
+    if (groups < 1 || groups > n)
+        return fail(ARW_E_ARG, "need 1 <= groups <= n");
+    int32_t rc;
+    for (int m = 0; m < n; ++m) {
+        if ((rc = check_domain(&members[m]))) return rc;
+        if ((rc = check_params(prm, &members[m]))) return rc;
+        if (members[m].qv)
+            return fail(ARW_E_ARG, "ensemble members must be dry");
+        if ((members[m].base3d != 0) != (members[0].base3d != 0))
+            return fail(ARW_E_ARG, "ensemble members must share base3d");
+        if (!ARW_F64 && prm->precision_form && !members[m].tri64)
+            return fail(ARW_E_ARG, "precision form needs tri64 on every member");
+    }
+    std::vector<int> ord(n);
+    std::vector<char> seen(n, 0);
+    for (int i = 0; i < n; ++i) {
+        ord[i] = order ? order[i] : i;
+        if (ord[i] < 0 || ord[i] >= n || seen[ord[i]])
+            return fail(ARW_E_ARG, "order is not a permutation of 0..n-1");
+        seen[ord[i]] = 1;
+    }
+    // contiguous groups of about equal work (mass points)
+    double total_work = 0.0;
+    for (int m = 0; m < n; ++m)
+        total_work += (double)members[m].nz * members[m].ny * members[m].nx;
+    std::vector<std::vector<int>> gm(groups);
+    {
+        double acc = 0.0;
+        int g = 0;
+        for (int i = 0; i < n; ++i) {
+            // next group once this one holds its share of the work, or
+            // when the members left are just enough for one per later group
+            if (!gm[g].empty() && g + 1 < groups
+                && (acc >= total_work * (g + 1) / groups
+                    || n - i <= groups - 1 - g))
+                ++g;
+            const int m = ord[i];
+            gm[g].push_back(m);
+            acc += (double)members[m].nz * members[m].ny * members[m].nx;
+        }
+    }
+    // host image of every group's views and tables, then one device copy
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    struct Off { size_t doms, fin, start[BK_COUNT], owner[BK_COUNT]; int total[BK_COUNT]; };
+    std::vector<Off> off(groups);
+    size_t bytes = 0;
+    for (int g = 0; g < groups; ++g) {
+        const size_t ng = gm[g].size();
+        off[g].doms = bytes; bytes = align(bytes + ENS_ROLES * ng * sizeof(Dom));
+        off[g].fin = bytes; bytes = align(bytes + 3 * ng * sizeof(FinishDst));
+        for (int k = 0; k < BK_COUNT; ++k) {
+            long long t = 0;
+            for (int m : gm[g]) t += batch_blocks(members[m], k);
+            if (t >= (1LL << 30)) return fail(ARW_E_ARG, "ensemble too large for one launch");
+            off[g].total[k] = (int)t;
+            off[g].start[k] = bytes; bytes = align(bytes + (ng + 1) * sizeof(int));
+            off[g].owner[k] = bytes; bytes = align(bytes + (size_t)t * sizeof(int));
+        }
+    }
+    std::vector<unsigned char> img(bytes, 0);
+    for (int g = 0; g < groups; ++g) {
+        const size_t ng = gm[g].size();
+        Dom* doms = (Dom*)(img.data() + off[g].doms);
+        FinishDst* fin = (FinishDst*)(img.data() + off[g].fin);
+        for (size_t i = 0; i < ng; ++i) {
+            Dom role[ENS_ROLES];
+            FinishDst f[3];
+            member_views(members[gm[g][i]], role, f);
+            for (int r = 0; r < ENS_ROLES; ++r) doms[r * ng + i] = role[r];
+            for (int s3 = 0; s3 < 3; ++s3) fin[s3 * ng + i] = f[s3];
+        }
+        for (int k = 0; k < BK_COUNT; ++k) {
+            int* start = (int*)(img.data() + off[g].start[k]);
+            int* owner = (int*)(img.data() + off[g].owner[k]);
+            int t = 0;
+            for (size_t i = 0; i < ng; ++i) {
+                start[i] = t;
+                const int nb = batch_blocks(members[gm[g][i]], k);
+                for (int b = 0; b < nb; ++b) owner[t + b] = (int)i;
+                t += nb;
+            }
+            start[ng] = t;
+        }
+    }
+    ARW_API(ensemble)* e = new (std::nothrow) ARW_API(ensemble)();
+    if (!e) return fail(ARW_E_ARG, "out of host memory");
+    if (hipMalloc(&e->dev, bytes) != hipSuccess) {
+        delete e;
+        return fail(ARW_E_LAUNCH, "hipMalloc of the ensemble tables failed");
+    }
+    if (hipMemcpy(e->dev, img.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(e->dev);
+        delete e;
+        return fail(ARW_E_LAUNCH, "copy of the ensemble tables failed");
+    }
+    unsigned char* dev = (unsigned char*)e->dev;
+    e->n = n;
+    e->prm = *prm;
+    e->b3 = members[0].base3d != 0;
+    e->groups.resize(groups);
+    for (int g = 0; g < groups; ++g) {
+        const size_t ng = gm[g].size();
+        for (int r = 0; r < ENS_ROLES; ++r) {
+            const int stage = r < ENS_BASE ? r / 3 : -1;
+            for (int k = 0; k < BK_COUNT; ++k) {
+                Batch& b = e->groups[g].sets[r].k[k];
+                b.doms = (const Dom*)(dev + off[g].doms) + r * ng;
+                b.fin = (stage >= 0 && r % 3 == 0)
+                            ? (const FinishDst*)(dev + off[g].fin) + stage * ng
+                            : nullptr;
+                b.owner = (const int*)(dev + off[g].owner[k]);
+                b.start = (const int*)(dev + off[g].start[k]);
+                b.total = off[g].total[k];
+            }
+        }
+    }
+    *out = e;
+    return ARW_OK;
+}
+
+int32_t ARW_API(ensemble_step)(ARW_API(ensemble)* e, int32_t nsteps,
+                               void* stream)
+{
+    if (!e) return fail(ARW_E_ARG, "null ensemble");
+    if (nsteps < 0) return fail(ARW_E_ARG, "nsteps must be >= 0");
+    hipStream_t st = (hipStream_t)stream;
+    for (int t = 0; t < nsteps; ++t)
+        for (const EnsGroup& G : e->groups) {
+            int32_t rc = ens_issue(G, &e->prm, e->b3, st);
+            if (rc) return rc;
+        }
+    return ARW_OK;
+}
+
+int32_t ARW_API(ensemble_capture)(ARW_API(ensemble)* e, int32_t nsteps)
+{
+    if (!e) return fail(ARW_E_ARG, "null ensemble");
+    if (nsteps < 1) return fail(ARW_E_ARG, "nsteps must be >= 1");
+    if (e->exec) {
+        hipGraphExecDestroy(e->exec);
+        e->exec = nullptr;
+    }
+    const int ng = (int)e->groups.size();
+    std::vector<hipStream_t> cs(ng, nullptr);
+    std::vector<hipEvent_t> ev(ng, nullptr);
+    int32_t rc = ARW_OK;
+    for (int g = 0; g < ng && rc == ARW_OK; ++g) {
+        if (hipStreamCreateWithFlags(&cs[g], hipStreamNonBlocking) != hipSuccess
+            || hipEventCreateWithFlags(&ev[g], hipEventDisableTiming) != hipSuccess)
+            rc = fail(ARW_E_LAUNCH, "stream/event creation failed");
+    }
+    hipGraph_t graph = nullptr;
+    bool capturing = false;
+    if (rc == ARW_OK) {
+        if (hipStreamBeginCapture(cs[0], hipStreamCaptureModeThreadLocal) != hipSuccess)
+            rc = fail(ARW_E_LAUNCH, "hipStreamBeginCapture failed");
+        else
+            capturing = true;
+    }
+    // fork: every group's stream joins the capture after cs[0]'s start
+    if (rc == ARW_OK && ng > 1) {
+        if (hipEventRecord(ev[0], cs[0]) != hipSuccess) rc = fail(ARW_E_LAUNCH, "fork record failed");
+        for (int g = 1; g < ng && rc == ARW_OK; ++g)
+            if (hipStreamWaitEvent(cs[g], ev[0], 0) != hipSuccess)
+                rc = fail(ARW_E_LAUNCH, "fork wait failed");
+    }
+    for (int g = 0; g < ng && rc == ARW_OK; ++g)
+        for (int t = 0; t < nsteps && rc == ARW_OK; ++t)
+            rc = ens_issue(e->groups[g], &e->prm, e->b3, cs[g]);
+    // join
+    for (int g = 1; g < ng && rc == ARW_OK; ++g)
+        if (hipEventRecord(ev[g], cs[g]) != hipSuccess
+            || hipStreamWaitEvent(cs[0], ev[g], 0) != hipSuccess)
+            rc = fail(ARW_E_LAUNCH, "join failed");
+    if (capturing) {
+        hipError_t er = hipStreamEndCapture(cs[0], &graph);
+        if (rc == ARW_OK && er != hipSuccess) rc = fail(ARW_E_LAUNCH, "hipStreamEndCapture failed");
+    }
+    if (rc == ARW_OK && hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        e->exec = nullptr;
+        rc = fail(ARW_E_LAUNCH, "hipGraphInstantiate failed");
+    }
+    if (graph) hipGraphDestroy(graph);
+    for (int g = 0; g < ng; ++g) {
+        if (ev[g]) hipEventDestroy(ev[g]);
+        if (cs[g]) hipStreamDestroy(cs[g]);
+    }
+    return rc;
+}
+
+int32_t ARW_API(ensemble_launch)(ARW_API(ensemble)* e, void* stream)
+{
+    if (!e) return fail(ARW_E_ARG, "null ensemble");
+    if (!e->exec) return fail(ARW_E_ARG, "no captured graph (call ensemble_capture)");
+    if (hipGraphLaunch(e->exec, (hipStream_t)stream) != hipSuccess)
+        return fail(ARW_E_LAUNCH, "hipGraphLaunch failed");
+    return ARW_OK;
+}
+
+int32_t ARW_API(ensemble_destroy)(ARW_API(ensemble)* e)
+{
+    if (!e) return ARW_OK;
+    if (e->exec) hipGraphExecDestroy(e->exec);
+    if (e->dev) hipFree(e->dev);
+    delete e;
+    return ARW_OK;
 }
 
 int32_t ARW_API(profile_begin)(int32_t max_events)

@@ -11,6 +11,90 @@ struct FinishDst {
     real *u, *v, *w, *thp, *php, *mup;
 };
 
+// One launch over every member of an ensemble (see Many below); doms (and
+// fin for the finish) hold the members' views for this launch, owner and
+// start the workgroup tables of its launch shape.  Device memory.
+struct Batch {
+    const Dom* doms;
+    const FinishDst* fin;
+    const int* owner;
+    const int* start;
+    int total;
+};
+
+// Workgroups one member needs in each batched launch shape.
+enum BatchKind { BK_COLS, BK_ROWS, BK_UV, BK_TFE, BK_FE, BK_COUNT };
+int batch_blocks(const Dom& d, int kind);
+
+// The members' views for one launch role, with the per-shape tables.
+struct BatchSet {
+    Batch k[BK_COUNT];
+};
+
+// ---------------------------------------------------------------------------
+// Where a kernel finds its domain.  One: the kernel argument, workgroup
+// blockIdx.x.  Many: a member of an ensemble launch (Batch) -- logical
+// workgroup g of the launch (XCD-grouped) belongs to member owner[g] and
+// is that member's workgroup g - start[owner[g]].  The member views are
+// read through the constant address space, like kernel arguments: scalar
+// loads the compiler may repeat anywhere.
+// ---------------------------------------------------------------------------
+typedef const __attribute__((address_space(4))) int kint;
+typedef const __attribute__((address_space(4))) Dom kdom;
+typedef const __attribute__((address_space(4))) FinishDst kfin;
+
+struct One {
+    static constexpr bool many = false;
+    typedef Dom Arg;
+    static __device__ __forceinline__ bool locate(const Dom&, int& m, int& g)
+    {
+        m = 0;
+        g = (int)blockIdx.x;
+        return true;
+    }
+    static __device__ __forceinline__ const Dom& dom(const Dom& a, int)
+    {
+        return a;
+    }
+};
+
+struct Many {
+    static constexpr bool many = true;
+    typedef Batch Arg;
+    static __device__ __forceinline__ bool locate(const Batch& b, int& m,
+                                                  int& g)
+    {
+        const int x = xcd_block();
+        if (x >= b.total) return false;
+        m = ((kint*)b.owner)[x];
+        g = x - ((kint*)b.start)[m];
+        return true;
+    }
+    static __device__ __forceinline__ const Dom& dom(const Batch& b, int m)
+    {
+        return *(const Dom*)((kdom*)b.doms + m);
+    }
+    static __device__ __forceinline__ const FinishDst& fin(const Batch& b,
+                                                           int m)
+    {
+        return *(const FinishDst*)((kfin*)b.fin + m);
+    }
+};
+
+template <class SRC>
+__device__ __forceinline__ const FinishDst& fin_of(const Dom&, int,
+                                                   const FinishDst& o)
+{
+    return o;
+}
+
+template <class SRC>
+__device__ __forceinline__ const FinishDst& fin_of(const Batch& b, int m,
+                                                   const FinishDst&)
+{
+    return Many::fin(b, m);
+}
+
 hipError_t run_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st);
 hipError_t run_tend_faces(const Dom& d, const Sc& s, hipStream_t st);
 hipError_t run_tend_cols(const Dom& d, const Sc& s, hipStream_t st);
@@ -18,5 +102,17 @@ hipError_t run_uv(const Dom& d, const Sc& s, hipStream_t st);
 hipError_t run_column(const Dom& d, const Sc& s, real* pout, int write_al,
                       int write_ww, hipStream_t st);
 hipError_t run_finish(const Dom& d, const FinishDst& o, hipStream_t st);
+
+// Batched forms (members share base3d and every flag in Sc).
+hipError_t run_omega_b(const BatchSet& b, const Sc& s, int do_eos, bool b3,
+                       hipStream_t st);
+hipError_t run_tend_faces_b(const BatchSet& b, const Sc& s, bool b3,
+                            hipStream_t st);
+hipError_t run_tend_cols_b(const BatchSet& b, const Sc& s, bool b3,
+                           hipStream_t st);
+hipError_t run_uv_b(const BatchSet& b, const Sc& s, bool b3, hipStream_t st);
+hipError_t run_column_b(const BatchSet& b, const Sc& s, bool b3, int write_al,
+                        hipStream_t st);
+hipError_t run_finish_b(const BatchSet& b, bool b3, hipStream_t st);
 
 }  // namespace ARW_NS

@@ -82,6 +82,7 @@ __device__ __forceinline__ real pgf_val(
 // Omega and the stage EOS
 // ---------------------------------------------------------------------------
 
+
 // WRF calc_ww_cp (reference dycore.py:102-131): the column sum of the
 // coupled-flux divergence, then Omega by the upward recurrence.  The sum
 // also serves the acoustic substeps as the reference-momentum mass
@@ -89,12 +90,15 @@ __device__ __forceinline__ real pgf_val(
 // float32 expression, so one sum serves both.  Pass 1 parks each level's
 // divergence in ww; pass 2 turns it into Omega.  EOS: the stage-start
 // calc_p_alpha (diagnostics.cu:37-89) when the stage state is new.
-template <bool EOS, bool B3, bool PREC>
-__global__ __launch_bounds__(256) void k_omega(Dom d, Sc s)
+template <class SRC, bool EOS, bool B3, bool PREC>
+__global__ __launch_bounds__(256) void k_omega(typename SRC::Arg a, Sc s)
 {
+    int m, g;
+    if (!SRC::locate(a, m, g)) return;
+    const Dom& d = SRC::dom(a, m);
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
     const int stu = ny * (nx + 1), stv = (ny + 1) * nx;
-    const int c = xcd_block() * blockDim.x + threadIdx.x;
+    const int c = (SRC::many ? g : xcd_block()) * blockDim.x + threadIdx.x;
     if (c >= st) return;
     const int j = c / nx, i = c - j * nx;
     const int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
@@ -220,6 +224,18 @@ __device__ __forceinline__ RowSeg row_seg(int nx, int ny)
     return r;
 }
 
+// Row segment of a member's logical workgroup L (batched launches).
+__device__ __forceinline__ RowSeg row_seg_at(int nx, int ny, int L)
+{
+    const int nbx = (nx + 63) / 64;
+    const int by = L / nbx, bx = L - by * nbx;
+    RowSeg r;
+    r.i = bx * 64 + (int)threadIdx.x;
+    r.j = __builtin_amdgcn_readfirstlane(by * 4 + (int)threadIdx.y);
+    r.live = r.i < nx && r.j < ny;
+    return r;
+}
+
 static inline dim3 rowseg_grid(int nx, int ny)
 {
     return dim3(xcd_grid((unsigned)((nx + 63) / 64) * ((ny + 3) / 4)));
@@ -238,13 +254,16 @@ struct PgfWin {
 // (dycore.cu:98-157), constant-K diffusion (diffusion.cu:22-51), and the
 // u''/v'' small-step initialisation (dycore.cu:947-1010).  The periodic
 // duplicate faces are finished by k_tend_faces_edge.
-template <bool B3, bool DIFF>
-__global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
+template <class SRC, bool B3, bool DIFF>
+__global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
 {
+    int m, g;
+    if (!SRC::locate(a, m, g)) return;
+    const Dom& d = SRC::dom(a, m);
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
     const int pu = nx + 1;                       // u row pitch
     const int stu = ny * pu, stv = (ny + 1) * nx;
-    const RowSeg rs = row_seg(nx, ny);
+    const RowSeg rs = SRC::many ? row_seg_at(nx, ny, g) : row_seg(nx, ny);
     if (!rs.live) return;
     const int i = rs.i, j = rs.j;
     kreal* c1h = kview(d.c1h);
@@ -491,11 +510,15 @@ __global__ __launch_bounds__(256) void k_tend_faces(Dom d, Sc s)
 // equals face 0's (same stencil inputs), the face-ny v tendency equals
 // row 0's; u''/v'' init uses the duplicate's own u/u0 (v/v0).  One thread
 // per (k, j) duplicate u face and per (k, i) duplicate v face.
-__global__ __launch_bounds__(256) void k_tend_faces_edge(Dom d)
+template <class SRC>
+__global__ __launch_bounds__(256) void k_tend_faces_edge(typename SRC::Arg a)
 {
+    int m, g;
+    if (!SRC::locate(a, m, g)) return;
+    const Dom& d = SRC::dom(a, m);
     const int nz = d.nz, ny = d.ny, nx = d.nx;
     const int pu = nx + 1, stu = ny * pu, stv = (ny + 1) * nx;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = g * blockDim.x + threadIdx.x;
     const real* RS mub = d.mub2d;
     const real* RS mup = d.mup;
     const real* RS mup0 = d.mup0;
@@ -536,12 +559,15 @@ __global__ __launch_bounds__(256) void k_tend_faces_edge(Dom d)
 // w''/phi''/theta''/p''/alpha'' small-step initialisation
 // (dycore.cu:1012-1085).  Row-segment waves; level kinds peel the surface
 // (k = 0) and the lid (k = nz) so the interior level body is one block.
-template <bool B3, bool DIFF, bool MOIST>
-__global__ __launch_bounds__(256) void k_tend_cols(Dom d, Sc s)
+template <class SRC, bool B3, bool DIFF, bool MOIST>
+__global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
 {
+    int m, g;
+    if (!SRC::locate(a, m, g)) return;
+    const Dom& d = SRC::dom(a, m);
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
     const int pu = nx + 1, stu = ny * pu, stv = (ny + 1) * nx;
-    const RowSeg rs = row_seg(nx, ny);
+    const RowSeg rs = SRC::many ? row_seg_at(nx, ny, g) : row_seg(nx, ny);
     if (!rs.live) return;
     const int i = rs.i, j = rs.j;
     kreal* c1h = kview(d.c1h);
@@ -1063,12 +1089,18 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
 // own u''/ru_t and the PGF of face 0), dispatched first so they overlap the
 // main work; the rest are row-segment waves in which the thread of mass
 // column (j, i) advances u'' on face (j, i) and v'' on face (j, i).
-template <bool B3, bool USEP>
-__global__ __launch_bounds__(256) void k_uv(Dom d, Sc s, int ne)
+// Batched (Many): each member's logical range holds its edge workgroups
+// first; ne_arg is unused.
+template <class SRC, bool B3, bool USEP>
+__global__ __launch_bounds__(256) void k_uv(typename SRC::Arg a, Sc s, int ne_arg)
 {
+    int m, g;
+    if (!SRC::locate(a, m, g)) return;
+    const Dom& d = SRC::dom(a, m);
     const int ny = d.ny, nx = d.nx, pu = nx + 1;
-    if ((int)blockIdx.x < ne) {
-        const int t = blockIdx.x * 256 + threadIdx.y * 64 + threadIdx.x;
+    const int ne = SRC::many ? (ny + nx + 255) / 256 : ne_arg;
+    if (g < ne) {
+        const int t = g * 256 + threadIdx.y * 64 + threadIdx.x;
         if (t < ny) {
             const int j = t;
             uv_march<B3, USEP, true, false>(
@@ -1082,7 +1114,8 @@ __global__ __launch_bounds__(256) void k_uv(Dom d, Sc s, int ne)
         }
         return;
     }
-    const RowSeg rs = row_seg_from(nx, ny, ne);
+    const RowSeg rs = SRC::many ? row_seg_at(nx, ny, g - ne)
+                              : row_seg_from(nx, ny, ne);
     if (!rs.live) return;
     const int i = rs.i, j = rs.j;
     const int jm = (j == 0) ? ny - 1 : j - 1;
@@ -1131,12 +1164,18 @@ struct ColDn {
 
 // B3: terrain (3-D) base state; DAMP: damp_opt = 3 w damper; AL: store
 // alpha''; WW: store Omega'' (standalone substep API).  nz >= 3.
-template <bool B3, bool DAMP, bool AL, bool WW, bool PREC>
-__global__ __launch_bounds__(256) void k_column(Dom d, Sc s, real* RS pout)
+// Batched (Many): the new p'' goes to each member's p_pp_old.
+template <class SRC, bool B3, bool DAMP, bool AL, bool WW, bool PREC>
+__global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
+                                                real* RS pout_arg)
 {
+    int m, g;
+    if (!SRC::locate(a, m, g)) return;
+    const Dom& d = SRC::dom(a, m);
+    real* RS pout = SRC::many ? d.p_pp_old : pout_arg;
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
     const int stu = ny * (nx + 1), stv = (ny + 1) * nx;
-    const int c = xcd_block() * blockDim.x + threadIdx.x;
+    const int c = (SRC::many ? g : xcd_block()) * blockDim.x + threadIdx.x;
     if (c >= st) return;
     const int j = c / nx, i = c - j * nx;
     const int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
@@ -1558,12 +1597,17 @@ struct FinIn {
 // Small-step finish (reference dycore.cu:1087-1198) of the mass column
 // (j, i), its u face (j, i) and v face (j, i), then the EOS of the new
 // state (diagnostics.cu:37-89).
-template <bool B3, bool EOS>
-__global__ __launch_bounds__(256) void k_finish(Dom d, FinishDst o)
+template <class SRC, bool B3, bool EOS>
+__global__ __launch_bounds__(256) void k_finish(typename SRC::Arg a,
+                                                FinishDst o_arg)
 {
+    int m, g;
+    if (!SRC::locate(a, m, g)) return;
+    const Dom& d = SRC::dom(a, m);
+    const FinishDst& o = fin_of<SRC>(a, m, o_arg);
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
     const int stu = ny * (nx + 1), stv = (ny + 1) * nx;
-    const int c = xcd_block() * blockDim.x + threadIdx.x;
+    const int c = (SRC::many ? g : xcd_block()) * blockDim.x + threadIdx.x;
     if (c >= st) return;
     const int j = c / nx, i = c - j * nx;
     const int im = (i == 0) ? nx - 1 : i - 1;
@@ -1659,10 +1703,16 @@ __global__ __launch_bounds__(256) void k_finish(Dom d, FinishDst o)
 }
 
 // Periodic duplicate faces of the finish: u at i = nx, v at j = ny.
-__global__ __launch_bounds__(256) void k_finish_edge(Dom d, FinishDst o)
+template <class SRC>
+__global__ __launch_bounds__(256) void k_finish_edge(typename SRC::Arg a,
+                                                     FinishDst o_arg)
 {
+    int m, g;
+    if (!SRC::locate(a, m, g)) return;
+    const Dom& d = SRC::dom(a, m);
+    const FinishDst& o = fin_of<SRC>(a, m, o_arg);
     const int ny = d.ny, nx = d.nx;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = g * blockDim.x + threadIdx.x;
     const real* RS mub = d.mub2d;
     const real* RS mup = d.mup;
     const real* RS mu_pp = d.mu_pp;
@@ -1705,11 +1755,11 @@ static void launch_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st)
 {
     const bool b3 = d.base3d != 0;
     if (do_eos && b3)
-        hipLaunchKernelGGL((k_omega<true, true, PREC>), cols(d), dim3(256), 0, st, d, s);
+        hipLaunchKernelGGL((k_omega<One, true, true, PREC>), cols(d), dim3(256), 0, st, d, s);
     else if (do_eos)
-        hipLaunchKernelGGL((k_omega<true, false, PREC>), cols(d), dim3(256), 0, st, d, s);
+        hipLaunchKernelGGL((k_omega<One, true, false, PREC>), cols(d), dim3(256), 0, st, d, s);
     else
-        hipLaunchKernelGGL((k_omega<false, false, PREC>), cols(d), dim3(256), 0, st, d, s);
+        hipLaunchKernelGGL((k_omega<One, false, false, PREC>), cols(d), dim3(256), 0, st, d, s);
 }
 
 hipError_t run_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st)
@@ -1723,15 +1773,15 @@ hipError_t run_tend_faces(const Dom& d, const Sc& s, hipStream_t st)
 {
     const dim3 g = rowseg_grid(d.nx, d.ny), b(64, 4);
     if (d.base3d && s.diff)
-        hipLaunchKernelGGL((k_tend_faces<true, true>), g, b, 0, st, d, s);
+        hipLaunchKernelGGL((k_tend_faces<One, true, true>), g, b, 0, st, d, s);
     else if (d.base3d)
-        hipLaunchKernelGGL((k_tend_faces<true, false>), g, b, 0, st, d, s);
+        hipLaunchKernelGGL((k_tend_faces<One, true, false>), g, b, 0, st, d, s);
     else if (s.diff)
-        hipLaunchKernelGGL((k_tend_faces<false, true>), g, b, 0, st, d, s);
+        hipLaunchKernelGGL((k_tend_faces<One, false, true>), g, b, 0, st, d, s);
     else
-        hipLaunchKernelGGL((k_tend_faces<false, false>), g, b, 0, st, d, s);
+        hipLaunchKernelGGL((k_tend_faces<One, false, false>), g, b, 0, st, d, s);
     const int n = d.nz * d.ny + d.nz * d.nx;
-    hipLaunchKernelGGL(k_tend_faces_edge, dim3((n + 255) / 256), dim3(256), 0,
+    hipLaunchKernelGGL(k_tend_faces_edge<One>, dim3((n + 255) / 256), dim3(256), 0,
                        st, d);
     return hipGetLastError();
 }
@@ -1741,13 +1791,13 @@ static void launch_tend_cols(const Dom& d, const Sc& s, hipStream_t st)
 {
     const dim3 g = rowseg_grid(d.nx, d.ny), b(64, 4);
     if (d.base3d && s.diff)
-        hipLaunchKernelGGL((k_tend_cols<true, true, MOIST>), g, b, 0, st, d, s);
+        hipLaunchKernelGGL((k_tend_cols<One, true, true, MOIST>), g, b, 0, st, d, s);
     else if (d.base3d)
-        hipLaunchKernelGGL((k_tend_cols<true, false, MOIST>), g, b, 0, st, d, s);
+        hipLaunchKernelGGL((k_tend_cols<One, true, false, MOIST>), g, b, 0, st, d, s);
     else if (s.diff)
-        hipLaunchKernelGGL((k_tend_cols<false, true, MOIST>), g, b, 0, st, d, s);
+        hipLaunchKernelGGL((k_tend_cols<One, false, true, MOIST>), g, b, 0, st, d, s);
     else
-        hipLaunchKernelGGL((k_tend_cols<false, false, MOIST>), g, b, 0, st, d, s);
+        hipLaunchKernelGGL((k_tend_cols<One, false, false, MOIST>), g, b, 0, st, d, s);
 }
 
 hipError_t run_tend_cols(const Dom& d, const Sc& s, hipStream_t st)
@@ -1762,7 +1812,7 @@ static void launch_uv(const Dom& d, const Sc& s, hipStream_t st)
 {
     const int ne = (int)xcd_grid((unsigned)((d.ny + d.nx + 255) / 256));
     const dim3 g(rowseg_grid(d.nx, d.ny).x + ne);
-    hipLaunchKernelGGL((k_uv<B3, USEP>), g, dim3(64, 4), 0, st, d, s, ne);
+    hipLaunchKernelGGL((k_uv<One, B3, USEP>), g, dim3(64, 4), 0, st, d, s, ne);
 }
 
 hipError_t run_uv(const Dom& d, const Sc& s, hipStream_t st)
@@ -1780,10 +1830,10 @@ static void launch_column(const Dom& d, const Sc& s, real* pout,
                           hipStream_t st)
 {
     if (s.prec)
-        hipLaunchKernelGGL((k_column<B3, DAMP, AL, WW, true>), cols(d),
+        hipLaunchKernelGGL((k_column<One, B3, DAMP, AL, WW, true>), cols(d),
                            dim3(256), 0, st, d, s, pout);
     else
-        hipLaunchKernelGGL((k_column<B3, DAMP, AL, WW, false>), cols(d),
+        hipLaunchKernelGGL((k_column<One, B3, DAMP, AL, WW, false>), cols(d),
                            dim3(256), 0, st, d, s, pout);
 }
 
@@ -1813,15 +1863,150 @@ hipError_t run_finish(const Dom& d, const FinishDst& o, hipStream_t st)
     // a moist state's EOS waits for the scalar update of the stage
     const bool eos = d.qv == nullptr;
     if (d.base3d && eos)
-        hipLaunchKernelGGL((k_finish<true, true>), cols(d), dim3(256), 0, st, d, o);
+        hipLaunchKernelGGL((k_finish<One, true, true>), cols(d), dim3(256), 0, st, d, o);
     else if (d.base3d)
-        hipLaunchKernelGGL((k_finish<true, false>), cols(d), dim3(256), 0, st, d, o);
+        hipLaunchKernelGGL((k_finish<One, true, false>), cols(d), dim3(256), 0, st, d, o);
     else if (eos)
-        hipLaunchKernelGGL((k_finish<false, true>), cols(d), dim3(256), 0, st, d, o);
+        hipLaunchKernelGGL((k_finish<One, false, true>), cols(d), dim3(256), 0, st, d, o);
     else
-        hipLaunchKernelGGL((k_finish<false, false>), cols(d), dim3(256), 0, st, d, o);
-    hipLaunchKernelGGL(k_finish_edge, dim3((d.ny + d.nx + 255) / 256),
+        hipLaunchKernelGGL((k_finish<One, false, false>), cols(d), dim3(256), 0, st, d, o);
+    hipLaunchKernelGGL(k_finish_edge<One>, dim3((d.ny + d.nx + 255) / 256),
                        dim3(256), 0, st, d, o);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batched launchers (Many)
+// ---------------------------------------------------------------------------
+
+int batch_blocks(const Dom& d, int kind)
+{
+    const int rows = ((d.nx + 63) / 64) * ((d.ny + 3) / 4);
+    switch (kind) {
+    case BK_COLS: return (d.ny * d.nx + 255) / 256;
+    case BK_ROWS: return rows;
+    case BK_UV: return (d.ny + d.nx + 255) / 256 + rows;
+    case BK_TFE: return (d.nz * d.ny + d.nz * d.nx + 255) / 256;
+    default: return (d.ny + d.nx + 255) / 256;
+    }
+}
+
+static inline dim3 bgrid(const Batch& b)
+{
+    return dim3(xcd_grid((unsigned)b.total));
+}
+
+template <bool PREC>
+static void launch_omega_b(const Batch& b, const Sc& s, int do_eos, bool b3,
+                           hipStream_t st)
+{
+    if (do_eos && b3)
+        hipLaunchKernelGGL((k_omega<Many, true, true, PREC>), bgrid(b), dim3(256), 0, st, b, s);
+    else if (do_eos)
+        hipLaunchKernelGGL((k_omega<Many, true, false, PREC>), bgrid(b), dim3(256), 0, st, b, s);
+    else
+        hipLaunchKernelGGL((k_omega<Many, false, false, PREC>), bgrid(b), dim3(256), 0, st, b, s);
+}
+
+hipError_t run_omega_b(const BatchSet& b, const Sc& s, int do_eos, bool b3,
+                       hipStream_t st)
+{
+    if (s.prec) launch_omega_b<true>(b.k[BK_COLS], s, do_eos, b3, st);
+    else launch_omega_b<false>(b.k[BK_COLS], s, do_eos, b3, st);
+    return hipGetLastError();
+}
+
+hipError_t run_tend_faces_b(const BatchSet& b, const Sc& s, bool b3,
+                            hipStream_t st)
+{
+    const Batch& r = b.k[BK_ROWS];
+    const dim3 g = bgrid(r), blk(64, 4);
+    if (b3 && s.diff)
+        hipLaunchKernelGGL((k_tend_faces<Many, true, true>), g, blk, 0, st, r, s);
+    else if (b3)
+        hipLaunchKernelGGL((k_tend_faces<Many, true, false>), g, blk, 0, st, r, s);
+    else if (s.diff)
+        hipLaunchKernelGGL((k_tend_faces<Many, false, true>), g, blk, 0, st, r, s);
+    else
+        hipLaunchKernelGGL((k_tend_faces<Many, false, false>), g, blk, 0, st, r, s);
+    const Batch& e = b.k[BK_TFE];
+    hipLaunchKernelGGL(k_tend_faces_edge<Many>, bgrid(e), dim3(256), 0, st, e);
+    return hipGetLastError();
+}
+
+hipError_t run_tend_cols_b(const BatchSet& b, const Sc& s, bool b3,
+                           hipStream_t st)
+{
+    const Batch& r = b.k[BK_ROWS];
+    const dim3 g = bgrid(r), blk(64, 4);
+    if (b3 && s.diff)
+        hipLaunchKernelGGL((k_tend_cols<Many, true, true, false>), g, blk, 0, st, r, s);
+    else if (b3)
+        hipLaunchKernelGGL((k_tend_cols<Many, true, false, false>), g, blk, 0, st, r, s);
+    else if (s.diff)
+        hipLaunchKernelGGL((k_tend_cols<Many, false, true, false>), g, blk, 0, st, r, s);
+    else
+        hipLaunchKernelGGL((k_tend_cols<Many, false, false, false>), g, blk, 0, st, r, s);
+    return hipGetLastError();
+}
+
+hipError_t run_uv_b(const BatchSet& b, const Sc& s, bool b3, hipStream_t st)
+{
+    const Batch& r = b.k[BK_UV];
+    const dim3 g = bgrid(r), blk(64, 4);
+    const bool usep = s.use_pold != 0;
+    if (b3 && usep)
+        hipLaunchKernelGGL((k_uv<Many, true, true>), g, blk, 0, st, r, s, 0);
+    else if (b3)
+        hipLaunchKernelGGL((k_uv<Many, true, false>), g, blk, 0, st, r, s, 0);
+    else if (usep)
+        hipLaunchKernelGGL((k_uv<Many, false, true>), g, blk, 0, st, r, s, 0);
+    else
+        hipLaunchKernelGGL((k_uv<Many, false, false>), g, blk, 0, st, r, s, 0);
+    return hipGetLastError();
+}
+
+template <bool B3, bool DAMP, bool AL>
+static void launch_column_b(const Batch& r, const Sc& s, hipStream_t st)
+{
+    if (s.prec)
+        hipLaunchKernelGGL((k_column<Many, B3, DAMP, AL, false, true>),
+                           bgrid(r), dim3(256), 0, st, r, s, nullptr);
+    else
+        hipLaunchKernelGGL((k_column<Many, B3, DAMP, AL, false, false>),
+                           bgrid(r), dim3(256), 0, st, r, s, nullptr);
+}
+
+template <bool B3, bool DAMP>
+static void launch_column_b_al(const Batch& r, const Sc& s, int al,
+                               hipStream_t st)
+{
+    if (al) launch_column_b<B3, DAMP, true>(r, s, st);
+    else launch_column_b<B3, DAMP, false>(r, s, st);
+}
+
+hipError_t run_column_b(const BatchSet& b, const Sc& s, bool b3, int write_al,
+                        hipStream_t st)
+{
+    const Batch& r = b.k[BK_COLS];
+    const bool damp = s.damp != 0;
+    if (b3 && damp) launch_column_b_al<true, true>(r, s, write_al, st);
+    else if (b3) launch_column_b_al<true, false>(r, s, write_al, st);
+    else if (damp) launch_column_b_al<false, true>(r, s, write_al, st);
+    else launch_column_b_al<false, false>(r, s, write_al, st);
+    return hipGetLastError();
+}
+
+hipError_t run_finish_b(const BatchSet& b, bool b3, hipStream_t st)
+{
+    const Batch& r = b.k[BK_COLS];
+    const FinishDst none = {};
+    if (b3)
+        hipLaunchKernelGGL((k_finish<Many, true, true>), bgrid(r), dim3(256), 0, st, r, none);
+    else
+        hipLaunchKernelGGL((k_finish<Many, false, true>), bgrid(r), dim3(256), 0, st, r, none);
+    const Batch& e = b.k[BK_FE];
+    hipLaunchKernelGGL(k_finish_edge<Many>, bgrid(e), dim3(256), 0, st, e, none);
     return hipGetLastError();
 }
 

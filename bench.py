@@ -42,7 +42,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("c3", "c2"), default="c3")
+    ap.add_argument("--workload", choices=("c3", "c2", "c5"), default="c3",
+                    help="c5: the batched 32-member ensemble against the "
+                         "same members stepped one after another")
+    ap.add_argument("--ens-steps", type=int, default=100,
+                    help="c5: steps captured in the one graph launch")
+    ap.add_argument("--ens-members", type=int, default=32)
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="c5: skip the FP64 error reference")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=160,
                     help="y rows of the CPU-baseline strip")
@@ -109,6 +116,8 @@ def main():
               f"{args.gpus}); the step is a single-GPU kernel set and the "
               "multi-GPU sharding is design-only", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c5":
+        return main_ensemble(args)
     import torch
     from arwen_amd import _lib, dycore
     from arwen_amd.state import DomainState
@@ -196,6 +205,137 @@ def main():
                                            args.cpu_steps)
     else:
         out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def _rel_err(a, b):
+    """max |a - b| / max |b| over one field (b the FP64 reference)."""
+    import torch
+    a64, b64 = a.double(), b.double()
+    den = float(b64.abs().max())
+    return float((a64 - b64).abs().max()) / den if den > 0 else 0.0
+
+
+def main_ensemble(args):
+    """C5: 32 dry members of uneven extents (nx, ny ~ U{96..256}, nz ~
+    U{40..60}, dx = 3 km, dt = 18 s, 4 substeps).  Timed: one HIP graph
+    launch of ``--ens-steps`` batched steps (layout from the autotuner)
+    against the same members stepped one after another -- eagerly through
+    dycore.step, and as one captured graph per member."""
+    import torch
+    from arwen_amd import cases, dycore
+    from arwen_amd import ensemble as E
+    from arwen_amd.state import DomainState
+
+    t_init = time.perf_counter()
+    hosts = cases.ensemble_members(args.ens_members)
+    cfg = hosts[0].cfg
+    states = [DomainState.from_host(h) for h in hosts]
+    print(f"c5: {len(states)} members built in "
+          f"{time.perf_counter() - t_init:.1f} s", file=sys.stderr, flush=True)
+    pts = sum(st.shape[0] * st.shape[1] * st.shape[2] for st in states)
+    n = args.ens_steps
+    init = E.save_state(states)
+
+    tune = E.autotune(states, cfg, steps=2, reps=2, verbose=True)
+    ens = E.Ensemble(states, cfg, order=tune["order"], groups=tune["groups"])
+    ens.capture(n)
+    # warm-up: the first replay of a fresh graph pays one-off costs
+    ens.launch()
+    torch.cuda.synchronize()
+    E.restore_state(states, init)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.launch()
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    batched = E.save_state(states)
+    ens.close()
+    print(f"c5: batched graph {t_batched * 1e3:.1f} ms for {n} steps",
+          file=sys.stderr, flush=True)
+
+    # sequential, eager: the reference's launch model
+    E.restore_state(states, init)
+    for st in states:                     # warm every member's launch path
+        dycore.step(st, cfg)
+    E.restore_state(states, init)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E.sequential_steps(states, cfg, n)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    same = all(torch.equal(getattr(st, a), sv[0][a])
+               for st, sv in zip(states, batched) for a in E.STATE_ARRAYS)
+    print(f"c5: sequential eager {t_seq * 1e3:.1f} ms; bitwise equal to "
+          f"batched: {same}", file=sys.stderr, flush=True)
+
+    # sequential, one captured graph per member (launch gaps removed)
+    E.restore_state(states, init)
+    singles = []
+    for st in states:
+        e1 = E.Ensemble([st], cfg)
+        e1.capture(n)
+        singles.append(e1)
+    for e1 in singles:
+        e1.launch()
+    torch.cuda.synchronize()
+    E.restore_state(states, init)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e1 in singles:
+        e1.launch()
+    torch.cuda.synchronize()
+    t_seq_graph = time.perf_counter() - t0
+    for e1 in singles:
+        e1.close()
+    del singles
+
+    errs = None
+    if not args.no_fp64:
+        # the same members in FP64 (batched; bit-identical to FP64
+        # sequential), then per-field max relative error of the FP32 run
+        s64 = [DomainState.from_host(h, dtype=torch.float64) for h in hosts]
+        with E.Ensemble(s64, cfg, groups=1) as e64:
+            e64.step(n)
+        torch.cuda.synchronize()
+        errs = {}
+        for a in ("u", "v", "w", "thp", "php", "mup", "p"):
+            errs[a] = max(_rel_err(sv[0][a], getattr(s64i, a))
+                          for sv, s64i in zip(batched, s64))
+        del s64
+
+    value = pts * n / t_batched / 1e6
+    out = {
+        "metric": "RK3 dycore Mpoint-steps/s, % HBM roofline; "
+                  "adjoint:forward cost at stated err",
+        "value": round(value, 3),
+        "unit": "Mpoint-steps/s",
+        "n_gpus": 1,
+        "steps": n,
+        "warmup": 1,
+        "ms_per_step": round(t_batched * 1e3 / n, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded initial states)",
+        "config": {"workload": "C5 batched ensemble: 32 dry members, nx,ny "
+                               "~U{96..256}, nz ~U{40..60}, dx=3 km, dt=18 s, "
+                               "4 substeps, one HIP graph launch",
+                   "members": [list(st.shape) for st in states],
+                   "points": pts, "parallelism": "single GPU"},
+        "ensemble": {
+            "layout": {"order": tune["order_name"], "groups": tune["groups"]},
+            "autotune_trials": tune["trials"],
+            "batched_ms": round(t_batched * 1e3, 2),
+            "sequential_eager_ms": round(t_seq * 1e3, 2),
+            "sequential_graph_ms": round(t_seq_graph * 1e3, 2),
+            "speedup_vs_sequential_eager": round(t_seq / t_batched, 3),
+            "speedup_vs_sequential_graph": round(t_seq_graph / t_batched, 3),
+            "bitwise_equal_to_sequential": bool(same),
+            "max_rel_err_vs_fp64": errs,
+        },
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -177,6 +177,35 @@ int32_t arw_tracer_adjoint(const arw_domain *d, const arw_params *prm,
                            const arw_tracer_ckpt *ck, const float *lam_in,
                            float *lam_out, float *work, void *stream);
 
+/*
+ * Ensemble: independent periodic domains stepped together.  No reference
+ * counterpart: ArWen runs ensemble members one after another
+ * (gpuwm/ensemble/engine.py:1-14).  Every pass of the step is one launch
+ * over all members of a group; a work table maps each workgroup to a
+ * member and to a workgroup of that member.  Members may differ in nx,
+ * ny and nz; they share arw_params and base3d and are dry.  Each member
+ * advances exactly as arw_step would advance it (same kernels, same
+ * arithmetic, bit-identical results).
+ *
+ * order:  member order in the work tables (a permutation of 0..n-1, or
+ *         NULL for 0..n-1); members early in the order start first.
+ * groups: split the ordered members into this many groups of about equal
+ *         work; each group is its own launch sequence, and a captured
+ *         graph runs the groups as parallel branches (1 <= groups <= n).
+ * The handle owns its device tables until arw_ensemble_destroy.
+ */
+typedef struct arw_ensemble arw_ensemble;
+int32_t arw_ensemble_create(const arw_domain *members, int32_t n,
+                            const arw_params *prm, const int32_t *order,
+                            int32_t groups, arw_ensemble **out);
+/* nsteps steps, launched on stream. */
+int32_t arw_ensemble_step(arw_ensemble *e, int32_t nsteps, void *stream);
+/* Capture nsteps steps into one HIP graph held by the handle (replacing
+ * an earlier one); arw_ensemble_launch runs it on stream. */
+int32_t arw_ensemble_capture(arw_ensemble *e, int32_t nsteps);
+int32_t arw_ensemble_launch(arw_ensemble *e, void *stream);
+int32_t arw_ensemble_destroy(arw_ensemble *e);
+
 /* EOS p, al, alt from thp, php, mup (replaces diagnostics.cu:37-89
  * calc_p_alpha, launched by diagnostics.py:14-57; hypsometric_opt = 1). */
 int32_t arw_calc_p_alpha(const arw_domain *d, void *stream);
@@ -310,6 +339,16 @@ int32_t arw64_tracer_adjoint(const arw64_domain *d, const arw_params *prm,
                              const arw64_tracer_ckpt *ck,
                              const double *lam_in, double *lam_out,
                              double *work, void *stream);
+
+typedef struct arw64_ensemble arw64_ensemble;
+int32_t arw64_ensemble_create(const arw64_domain *members, int32_t n,
+                              const arw_params *prm, const int32_t *order,
+                              int32_t groups, arw64_ensemble **out);
+int32_t arw64_ensemble_step(arw64_ensemble *e, int32_t nsteps,
+                            void *stream);
+int32_t arw64_ensemble_capture(arw64_ensemble *e, int32_t nsteps);
+int32_t arw64_ensemble_launch(arw64_ensemble *e, void *stream);
+int32_t arw64_ensemble_destroy(arw64_ensemble *e);
 
 #ifdef __cplusplus
 }
