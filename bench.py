@@ -42,9 +42,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("c3", "c2", "c5"), default="c3",
+    ap.add_argument("--workload", choices=("c3", "c2", "c5", "c3-precision"),
+                    default="c3",
                     help="c5: the batched 32-member ensemble against the "
-                         "same members stepped one after another")
+                         "same members stepped one after another; "
+                         "c3-precision: FP32 and the precision form against "
+                         "the FP64 build after --prec-steps steps")
+    ap.add_argument("--prec-steps", type=int, default=600)
     ap.add_argument("--ens-steps", type=int, default=100,
                     help="c5: steps captured in the one graph launch")
     ap.add_argument("--ens-members", type=int, default=32)
@@ -118,6 +122,8 @@ def main():
         sys.exit(2)
     if args.workload == "c5":
         return main_ensemble(args)
+    if args.workload == "c3-precision":
+        return main_precision(args)
     import torch
     from arwen_amd import _lib, dycore
     from arwen_amd.state import DomainState
@@ -214,6 +220,67 @@ def _rel_err(a, b):
     a64, b64 = a.double(), b.double()
     den = float(b64.abs().max())
     return float((a64 - b64).abs().max()) / den if den > 0 else 0.0
+
+
+def main_precision(args):
+    """C3 precision form: the warm-bubble 1000x1000x50 state advanced
+    ``--prec-steps`` steps in plain FP32, in the precision form (FP32 bulk,
+    Kahan column sums, FP64 tridiagonal) and in the FP64 build of the same
+    discretisation; per-field max and RMS error of both FP32 runs against
+    FP64, and the three step times."""
+    import dataclasses
+
+    import torch
+    from arwen_amd import dycore
+    from arwen_amd.state import DomainState
+
+    case, desc = build_case("c3")
+    n = args.prec_steps
+    fields = ("u", "v", "w", "thp", "php", "mup", "p")
+    runs = {}
+    for name, dtype, prec in (("fp64", torch.float64, False),
+                              ("fp32", torch.float32, False),
+                              ("precision_form", torch.float32, True)):
+        cfg = dataclasses.replace(case.cfg, precision_form=prec)
+        st = DomainState.from_host(case, dtype=dtype)
+        m0 = dycore.domain_mass_measure(st)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(n):
+            dycore.step(st, cfg)
+            if (k + 1) % 100 == 0:
+                torch.cuda.synchronize()
+                print(f"{name}: {k + 1}/{n} steps", file=sys.stderr,
+                      flush=True)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / n
+        m1 = dycore.domain_mass_measure(st)
+        if dycore.stability_report(st, cfg)["nan"]:
+            raise SystemExit(f"bench.py: {name} went NaN")
+        runs[name] = {"ms_per_step": round(ms, 3),
+                      "mass_rel_change": (m1 - m0) / m0,
+                      "state": {f: getattr(st, f).double().cpu()
+                                for f in fields}}
+        del st
+        torch.cuda.empty_cache()
+    ref = runs["fp64"]["state"]
+    err = {}
+    for name in ("fp32", "precision_form"):
+        e = {}
+        for f in fields:
+            a, b = runs[name]["state"][f], ref[f]
+            scale = float(b.abs().max())
+            d = a - b
+            e[f] = {"max_rel": float(d.abs().max()) / scale,
+                    "rms_rel": float(d.pow(2).mean().sqrt()) / scale}
+        err[name] = e
+    out = {"workload": desc + f", {n} steps", "steps": n,
+           "ms_per_step": {k: v["ms_per_step"] for k, v in runs.items()},
+           "mass_rel_change": {k: v["mass_rel_change"]
+                               for k, v in runs.items()},
+           "error_vs_fp64": err,
+           "scale": "max/RMS of |field - fp64| over max |fp64 field|"}
+    print(json.dumps(out), flush=True)
 
 
 def main_ensemble(args):
