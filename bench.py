@@ -42,7 +42,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("c3", "c2", "c5", "c3-precision"),
+    ap.add_argument("--workload", choices=("c3", "c2", "c5", "c3-precision",
+                                           "c4-adjoint"),
                     default="c3",
                     help="c5: the batched 32-member ensemble against the "
                          "same members stepped one after another; "
@@ -62,7 +63,8 @@ def parse():
                     help="also launch one 1 GiB device copy (for PMC "
                          "byte-counter calibration under rocprofv3)")
     ap.add_argument("--traffic-file", default=None,
-                    help="PMC summary giving HBM bytes per launch")
+                    help="PMC summary giving HBM bytes per launch (default "
+                         "for c3: the committed rocprofv3 summary)")
     return ap.parse_args()
 
 
@@ -124,6 +126,8 @@ def main():
         return main_ensemble(args)
     if args.workload == "c3-precision":
         return main_precision(args)
+    if args.workload == "c4-adjoint":
+        return main_adjoint(args)
     import torch
     from arwen_amd import _lib, dycore
     from arwen_amd.state import DomainState
@@ -167,8 +171,13 @@ def main():
     u_bytes = WORDS_U * 4 * pts
     u_gbs = u_bytes / (u_avg_ms * 1e-3) / 1e9 if n_sub else None
     traffic = None
-    t_uv = read_traffic(args.traffic_file, "k_uv")
-    t_col = read_traffic(args.traffic_file, "k_column")
+    tfile = args.traffic_file
+    if tfile is None and args.workload == "c3":
+        tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                             "profiles",
+                             "dycore_step_1000x1000x50_pmc_traffic.txt")
+    t_uv = read_traffic(tfile, "k_uv")
+    t_col = read_traffic(tfile, "k_column")
     if t_uv is not None and t_col is not None:
         traffic = t_uv + t_col
 
@@ -280,6 +289,97 @@ def main_precision(args):
                                for k, v in runs.items()},
            "error_vs_fp64": err,
            "scale": "max/RMS of |field - fp64| over max |fp64 field|"}
+    print(json.dumps(out), flush=True)
+
+
+def _time_ms(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def main_adjoint(args):
+    """C4 adjoint form: the 192x160x50 moist-tracer case (dx = 3 km, dt =
+    18 s).  Times one checkpointed step, the checkpointed scalar transport
+    J q and its hand-written adjoint J^T lam (FP32 and FP64), and states
+    the FP64 dot-product residual and the central-difference agreement of
+    the autograd gradient."""
+    import dataclasses
+
+    import torch
+    from arwen_amd import adjoint, cases, dycore
+    from arwen_amd.state import DomainState
+
+    case = cases.moist_tracer(moist_adv_opt=0)
+    cfg = case.cfg
+    reps = max(args.steps, 5)
+    out = {"workload": "C4 moist tracer 192x160x50, dx=3 km, dt=18 s, "
+                       "4 substeps, qv U(0,16) g/kg exp(-z/2.5 km)",
+           "reps": reps}
+    for name, dtype in (("fp32", torch.float32), ("fp64", torch.float64)):
+        st = DomainState.from_host(case, dtype=dtype)
+        ck = adjoint.TracerCheckpoints(st)
+        q0 = st.qv.clone()
+        saved = {a: getattr(st, a).clone() for a in
+                 ("u", "v", "w", "thp", "php", "mup", "p", "al", "alt",
+                  "qv", "qc", "qr")}
+
+        def fwd_step():
+            for a, t in saved.items():
+                getattr(st, a).copy_(t)
+            adjoint.step_with_checkpoints(st, cfg, ck)
+
+        def plain_step():
+            for a, t in saved.items():
+                getattr(st, a).copy_(t)
+            dycore.step(st, cfg)
+
+        t_copy = _time_ms(lambda: [getattr(st, a).copy_(t)
+                                   for a, t in saved.items()], reps)
+        t_step = _time_ms(plain_step, reps) - t_copy
+        t_ckpt = _time_ms(fwd_step, reps) - t_copy
+        lam = torch.randn_like(q0)
+        t_fwd = _time_ms(lambda: adjoint.tracer_step(q0, st, cfg, ck), reps)
+        t_adj = _time_ms(lambda: adjoint.tracer_adjoint(lam, st, cfg, ck),
+                         reps)
+        rec = {"step_ms": round(t_step, 4),
+               "checkpointed_step_ms": round(t_ckpt, 4),
+               "tracer_forward_ms": round(t_fwd, 4),
+               "tracer_adjoint_ms": round(t_adj, 4),
+               "adjoint_to_forward": round(t_adj / t_fwd, 3),
+               "adjoint_to_step": round(t_adj / t_step, 3)}
+        if dtype == torch.float64:
+            gen = torch.Generator(device=st.device).manual_seed(7)
+            x = torch.randn(q0.shape, dtype=dtype, device=st.device,
+                            generator=gen)
+            y = torch.randn(q0.shape, dtype=dtype, device=st.device,
+                            generator=gen)
+            lhs = float(torch.sum(adjoint.tracer_step(x, st, cfg, ck) * y))
+            rhs = float(torch.sum(x * adjoint.tracer_adjoint(y, st, cfg, ck)))
+            rec["dot_product_rel_residual"] = abs(lhs - rhs) / abs(lhs)
+            w = torch.randn(q0.shape, dtype=dtype, device=st.device,
+                            generator=gen)
+
+            def loss(qq):
+                return 0.5 * torch.sum(
+                    w * adjoint.transport(qq, st, cfg, [ck])) ** 2
+
+            qv = q0.clone().requires_grad_(True)
+            (grad,) = torch.autograd.grad(loss(qv), qv)
+            eps = 1e-4
+            with torch.no_grad():
+                fd = (float(loss(q0 + eps * x))
+                      - float(loss(q0 - eps * x))) / (2 * eps)
+            ad = float(torch.sum(grad * x))
+            rec["fd_rel_err"] = abs(fd - ad) / abs(ad)
+        out[name] = rec
+        del st, ck
+        torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
 
 

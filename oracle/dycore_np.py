@@ -685,7 +685,7 @@ def _dpn_face_levels(s, pd, ax, top_lid):
     f = s.dtype.type
     half = f(0.5)
     S = pd + np.roll(pd, 1, axis=ax)           # A + B per level
-    d = np.zeros((nz + 1,) + pd.shape[1:], s.dtype)
+    d = np.zeros((nz + 1,) + pd.shape[1:], pd.dtype)
     d[0] = half * (s.cf1 * S[0] + s.cf2 * S[1] + s.cf3 * S[2])
     for kf in range(1, nz):
         d[kf] = half * (s.fnm[kf] * S[kf] + s.fnp[kf] * S[kf - 1])
@@ -777,7 +777,7 @@ def advance_mu_th(s, dtau, dx, dy, prec=False):
         ww[kk] = wwk
     s.ww_pp[...] = ww
     th = _b3(s.thb) + s.thp
-    wdtn = np.zeros((nz + 1,) + mut.shape, s.dtype)
+    wdtn = np.zeros((nz + 1,) + mut.shape, ww.dtype)
     for k in range(1, nz):
         wdtn[k] = ww[k] * (s.fnm[k] * th[k] + s.fnp[k] * th[k - 1])
     t_ip = np.roll(th, -1, axis=2)
@@ -857,7 +857,7 @@ def advance_w_phi(s, coefs, dtau, epssm, dx, dy, top_lid=True,
         cterm = dtau * g * (f(2.0) * s.rdnw[nz - 1] * c2a[nz - 1]
                             * s.alt[nz - 1] * t2[nz - 1] + s.c1f[nz] * muave)
         w[nz] = w[nz] + ((dtau * s.rw_t[nz] - bterm) - cterm)
-    if a.dtype != w.dtype:          # precision form: FP64 tridiagonal
+    if np.result_type(a, w) != w.dtype:   # precision form: FP64 tridiagonal
         w = thomas_solve(w.astype(a.dtype), a, alpha, gam).astype(w.dtype)
     else:
         thomas_solve(w, a, alpha, gam)

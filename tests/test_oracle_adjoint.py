@@ -90,3 +90,107 @@ def test_tracer_step_adjoint_dot_product():
     fd = (A.tracer_step(x + eps * y, *args)
           - A.tracer_step(x - eps * y, *args)) / (2 * eps)
     assert np.allclose(fd, A.tracer_step(y, *args), rtol=1e-9, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# Acoustic substep adjoint, pinned by complex-step derivatives of the
+# oracle's own forward substep (exact to rounding: no cancellation)
+# ---------------------------------------------------------------------------
+
+def acoustic_case(seed=4, hill=0.0, damp=False, top_lid=True, nz=9, ny=6,
+                  nx=10):
+    """FP64 oracle state about a seeded random reference state with random
+    small-step fields and tendencies (the pattern of the reference's
+    test_acoustic.py:39-115), its coefficients and config."""
+    import dataclasses
+    from tests.helpers import random_state_case
+    case, rng = random_state_case(seed=seed, nz=nz, ny=ny, nx=nx, hill=hill)
+    cfg = dataclasses.replace(case.cfg, top_lid=top_lid,
+                              damp_opt=3 if damp else 0, zdamp=5000.0,
+                              dampcoef=0.2)
+    o = D.from_case(case, np.float64)
+    D.calc_p_alpha(o)
+    amp = {"ru_t": 0.2, "rv_t": 0.2, "rw_t": 0.2, "rth_t": 0.2,
+           "rph_t": 0.2, "rmu_t": 0.5, "u_pp": 0.2, "v_pp": 0.2,
+           "w_pp": 0.2, "th_pp": 0.2, "mu_pp": 0.5, "ph_pp": 1.5,
+           "p_pp": 1.0, "p_pp_old": 1.0, "al_pp": 3e-7}
+    for n, a in amp.items():
+        arr = getattr(o, n)
+        arr[...] = a * rng.standard_normal(arr.shape)
+    dtau = cfg.dt / cfg.time_step_sound
+    coefs = D.calc_coefs(o, dtau, cfg.epssm, cfg.top_lid)
+    return o, coefs, dtau, cfg, rng
+
+
+def _complex_jvp(o, coefs, dtau, cfg, first, dx, h=1e-30):
+    import copy
+    sc = copy.deepcopy(o)
+    for n in A.SUBSTEP_STATE + A.SUBSTEP_TEND + ("ww_pp",):
+        setattr(sc, n, getattr(sc, n).astype(np.complex128))
+    for n, d in dx.items():
+        getattr(sc, n)[...] += 1j * h * d
+    D.acoustic_substep(sc, coefs, dtau, cfg, first)
+    return {n: np.imag(getattr(sc, n)) / h
+            for n in A.SUBSTEP_STATE + ("ww_pp",)}
+
+
+def _rand_like(o, names, rng):
+    return {n: rng.standard_normal(getattr(o, n).shape) for n in names}
+
+
+@pytest.mark.parametrize("hill,damp,top_lid,first", [
+    (0.0, False, True, True), (0.0, False, True, False),
+    (300.0, True, True, False), (0.0, True, False, False)])
+def test_acoustic_substep_adjoint_dot_product(hill, damp, top_lid, first):
+    o, coefs, dtau, cfg, rng = acoustic_case(hill=hill, damp=damp,
+                                             top_lid=top_lid)
+    dx = _rand_like(o, A.SUBSTEP_STATE + A.SUBSTEP_TEND, rng)
+    y = _rand_like(o, A.SUBSTEP_STATE + ("ww_pp",), rng)
+    jdx = _complex_jvp(o, coefs, dtau, cfg, first, dx)
+    jty = A.acoustic_substep_adjoint(o, coefs, dtau, cfg, first, y)
+    lhs = sum(np.vdot(jdx[n], y[n]) for n in y)
+    rhs = sum(np.vdot(dx[n], jty[n]) for n in dx)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
+
+
+def test_acoustic_adjoint_per_field_blocks():
+    """Each input/output block of J^T on its own (a wrong block cannot
+    hide behind the others)."""
+    o, coefs, dtau, cfg, rng = acoustic_case(hill=200.0, damp=True)
+    for nin in A.SUBSTEP_STATE + A.SUBSTEP_TEND:
+        dx = _rand_like(o, (nin,), rng)
+        jdx = _complex_jvp(o, coefs, dtau, cfg, False, dx)
+        for nout in A.SUBSTEP_STATE + ("ww_pp",):
+            y = _rand_like(o, (nout,), rng)
+            jty = A.acoustic_substep_adjoint(o, coefs, dtau, cfg, False, y)
+            lhs = np.vdot(jdx[nout], y[nout])
+            rhs = np.vdot(dx[nin], jty[nin])
+            scale = max(abs(lhs), 1e-300)
+            assert abs(lhs - rhs) <= 1e-11 * scale + 1e-280, (nin, nout,
+                                                              lhs, rhs)
+
+
+def test_planted_acoustic_adjoint_mistake_fails(monkeypatch):
+    """Dropping the mass-denominator term of the phi'' update (treating
+    the substep as if it were affine) breaks the identity."""
+    o, coefs, dtau, cfg, rng = acoustic_case()
+    o.mu_pp[...] *= 1e3                     # make that term visible
+    dx = {"mu_pp": rng.standard_normal(o.mu_pp.shape)}
+    y = {"ph_pp": rng.standard_normal(o.ph_pp.shape)}
+    jdx = _complex_jvp(o, coefs, dtau, cfg, False, dx)
+    jty = A.acoustic_substep_adjoint(o, coefs, dtau, cfg, False, y)
+    good = abs(np.vdot(jdx["ph_pp"], y["ph_pp"]) - np.vdot(dx["mu_pp"],
+                                                            jty["mu_pp"]))
+    o2 = o
+    orig = A._forward_pieces
+
+    def no_mass(*a, **k):
+        fw = orig(*a, **k)
+        fw["W1"] = fw["W1"] * 0.0           # kills -K W1 c1f / cfs^2 term
+        return fw
+    monkeypatch.setattr(A, "_forward_pieces", no_mass)
+    jty2 = A.acoustic_substep_adjoint(o2, coefs, dtau, cfg, False, y)
+    bad = abs(np.vdot(jdx["ph_pp"], y["ph_pp"]) - np.vdot(dx["mu_pp"],
+                                                           jty2["mu_pp"]))
+    assert good <= 1e-11 * abs(np.vdot(jdx["ph_pp"], y["ph_pp"]))
+    assert bad > 1e3 * max(good, 1e-300)
