@@ -74,6 +74,20 @@ class ArwTracerCkpt(ctypes.Structure):
 #: arw64_tracer_ckpt has the same layout (pointers and doubles)
 ArwTracerCkpt64 = ArwTracerCkpt
 
+#: small-step state and tendencies of the acoustic substep adjoint
+ACOUSTIC_STATE = ("u_pp", "v_pp", "w_pp", "th_pp", "ph_pp", "mu_pp", "p_pp",
+                  "p_pp_old", "al_pp")
+ACOUSTIC_TEND = ("ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t")
+
+
+class ArwAcousticAdj(ctypes.Structure):
+    """arw_acoustic_adj (and arw64_acoustic_adj: same layout)."""
+    _fields_ = ([("l_" + n, _P) for n in ACOUSTIC_STATE + ("ww_pp",)]
+                + [("g_" + n, _P) for n in ACOUSTIC_STATE + ACOUSTIC_TEND])
+
+
+ArwAcousticAdj64 = ArwAcousticAdj
+
 
 class ArwenError(RuntimeError):
     """A library entry point returned a nonzero status."""
@@ -116,6 +130,11 @@ def load():
                              _P, _I, _P], _I),
         "arw_tracer_adjoint": ([dp, pp, ctypes.POINTER(ArwTracerCkpt), _P,
                                 _P, _P, _P], _I),
+        "arw_acoustic_substep_adjoint": ([dp, pp, _D, _I,
+                                          ctypes.POINTER(ArwAcousticAdj), _P,
+                                          _P], _I),
+        "arw_acoustic_adjoint_work": ([_I, _I, _I,
+                                       ctypes.POINTER(ctypes.c_int64)], _I),
         "arw_ensemble_create": ([dp, _I, pp, _P, _I,
                                  ctypes.POINTER(ctypes.c_void_p)], _I),
         "arw_ensemble_step": ([_P, _I, _P], _I),
@@ -164,7 +183,9 @@ FP64_ENTRY_POINTS = (
     "arw64_acoustic_substep", "arw64_calc_coefs", "arw64_small_step_finish",
     "arw64_set_w_surface", "arw64_health", "arw64_domain_mass",
     "arw64_flux_div_scalar", "arw64_step_ckpt", "arw64_tracer_step",
-    "arw64_tracer_adjoint", "arw64_ensemble_create", "arw64_ensemble_step",
+    "arw64_tracer_adjoint", "arw64_acoustic_substep_adjoint",
+    "arw64_acoustic_adjoint_work", "arw64_ensemble_create",
+    "arw64_ensemble_step",
     "arw64_ensemble_capture", "arw64_ensemble_launch",
     "arw64_ensemble_destroy")
 
@@ -178,7 +199,9 @@ def exported_symbols() -> list[str]:
             "arw_flux_div_scalar", "arw_flux_div_u", "arw_flux_div_v",
             "arw_flux_div_w", "arw_profile_begin", "arw_profile_end",
             "arw_kernel_name", "arw_kernel_count", "arw_step_ckpt",
-            "arw_tracer_step", "arw_tracer_adjoint", "arw_ensemble_create",
+            "arw_tracer_step", "arw_tracer_adjoint",
+            "arw_acoustic_substep_adjoint", "arw_acoustic_adjoint_work",
+            "arw_ensemble_create",
             "arw_ensemble_step", "arw_ensemble_capture",
             "arw_ensemble_launch", "arw_ensemble_destroy"]
 

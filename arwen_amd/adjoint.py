@@ -1,4 +1,6 @@
-"""Scalar-transport adjoint with per-RK-stage checkpoints.
+"""Adjoints of the step's transport and acoustic kernels.
+
+Scalar transport, with per-RK-stage checkpoints:
 
 The dycore step moves a passive scalar (qv and friends; WRF
 rk_scalar_tend + rk_update_scalar, reference moist.py:385-398 and
@@ -11,8 +13,14 @@ per stage the averaged fluxes ru_m, rv_m, ww_m, the post-acoustic mu' and
 the stage length, plus the time-t mu'.  ``TracerStep`` is the
 ``torch.autograd.Function`` over J: forward launches the HIP transport
 (``arw_tracer_step``, bit-identical to the update inside the step),
-backward the hand-written transpose (``arw_tracer_adjoint``).  The
-reference has no adjoint; these entry points are new.
+backward the hand-written transpose (``arw_tracer_adjoint``).
+
+Acoustic substep: ``acoustic_substep_adjoint`` applies the transpose of
+one acoustic substep's Jacobian (``arw_acoustic_substep_adjoint``) at the
+state's current small-step fields; ``AcousticSubstep`` is the
+``torch.autograd.Function`` over the substep (inputs: the nine small-step
+fields and six slow tendencies; outputs: the nine fields and Omega'').
+The reference has no adjoint; these entry points are new.
 """
 from __future__ import annotations
 
@@ -145,3 +153,93 @@ def transport(q: torch.Tensor, state: DomainState, cfg: RunConfig,
     for c in ckpts:
         q = TracerStep.apply(q, state, cfg, c)
     return q
+
+
+# ---------------------------------------------------------------------------
+# Acoustic substep
+# ---------------------------------------------------------------------------
+
+ACOUSTIC_STATE = _lib.ACOUSTIC_STATE
+ACOUSTIC_TEND = _lib.ACOUSTIC_TEND
+
+
+def acoustic_substep_adjoint(state: DomainState, cfg: RunConfig,
+                             dtau: float, first: bool,
+                             lam: dict) -> dict:
+    """J^T lam for one acoustic substep taken from the state's current
+    small-step fields (coefficients prepared as for the forward substep,
+    ``dycore.prepare_acoustic_coefficients``).  ``lam`` maps the outputs
+    (ACOUSTIC_STATE, optionally ``ww_pp``) to adjoints (missing = 0);
+    returns the adjoints of ACOUSTIC_STATE + ACOUSTIC_TEND.  The state is
+    not modified."""
+    _check_state(state)
+    lib = _lib.load()
+    pre = "arw64_" if state.dtype == torch.float64 else "arw_"
+    nz, ny, nx = state.shape
+    n = ctypes.c_int64()
+    _lib.check(getattr(lib, pre + "acoustic_adjoint_work")(
+        nz, ny, nx, ctypes.byref(n)), pre + "acoustic_adjoint_work")
+    work = torch.empty(n.value, dtype=state.dtype, device=state.device)
+    keep = []
+    a = _lib.ArwAcousticAdj()
+    for name in ACOUSTIC_STATE + ("ww_pp",):
+        t = lam.get(name)
+        if t is None:
+            if name == "ww_pp":
+                setattr(a, "l_ww_pp", None)
+                continue
+            t = torch.zeros_like(getattr(state, name))
+        t = _as_like(t, getattr(state, name), name)
+        keep.append(t)
+        setattr(a, "l_" + name, t.data_ptr())
+    out = {}
+    for name in ACOUSTIC_STATE + ACOUSTIC_TEND:
+        out[name] = torch.empty_like(getattr(state, name))
+        setattr(a, "g_" + name, out[name].data_ptr())
+    fn = getattr(lib, pre + "acoustic_substep_adjoint")
+    _lib.check(fn(state.domain_struct(), params(cfg), float(dtau),
+                  int(bool(first)), ctypes.byref(a), work.data_ptr(),
+                  _stream()), pre + "acoustic_substep_adjoint")
+    return out
+
+
+def _as_like(t, ref, name):
+    if tuple(t.shape) != tuple(ref.shape) or t.dtype != ref.dtype \
+            or t.device != ref.device:
+        raise ValueError(f"adjoint of {name}: expected {tuple(ref.shape)} "
+                         f"{ref.dtype} on {ref.device}")
+    return t.contiguous()
+
+
+class AcousticSubstep(torch.autograd.Function):
+    """(X, tendencies) -> (X_new, Omega'') for one acoustic substep, with
+    the hand-written adjoint as its backward.  ``apply(state, cfg, dtau,
+    first, *inputs)``: ``inputs`` are the 9 ACOUSTIC_STATE fields then the
+    6 ACOUSTIC_TEND fields; the call writes them into the state (the
+    substep runs in the state's arrays) and returns new tensors.  The
+    stage coefficients must be prepared (prepare_acoustic_coefficients)."""
+
+    @staticmethod
+    def forward(ctx, state, cfg, dtau, first, *inputs):
+        from .dycore import acoustic_substep
+        names = ACOUSTIC_STATE + ACOUSTIC_TEND
+        if len(inputs) != len(names):
+            raise ValueError(f"expected {len(names)} input tensors")
+        for name, x in zip(names, inputs):
+            getattr(state, name).copy_(x)
+        acoustic_substep(state, cfg, dtau, first)
+        ctx.state, ctx.cfg, ctx.dtau, ctx.first = state, cfg, dtau, first
+        ctx.save_for_backward(*[x.detach() for x in inputs])
+        return tuple(getattr(state, n).clone()
+                     for n in ACOUSTIC_STATE + ("ww_pp",))
+
+    @staticmethod
+    def backward(ctx, *grads):
+        st = ctx.state
+        names = ACOUSTIC_STATE + ACOUSTIC_TEND
+        for name, x in zip(names, ctx.saved_tensors):
+            getattr(st, name).copy_(x)
+        lam = {n: g for n, g in zip(ACOUSTIC_STATE + ("ww_pp",), grads)
+               if g is not None}
+        out = acoustic_substep_adjoint(st, ctx.cfg, ctx.dtau, ctx.first, lam)
+        return (None, None, None, None) + tuple(out[n] for n in names)

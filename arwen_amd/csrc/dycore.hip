@@ -1017,6 +1017,13 @@ int32_t ens_issue(const EnsGroup& G, const arw_params* prm, bool b3,
 
 }  // namespace
 
+int32_t lib_fail(int32_t code, const char* msg) { return fail(code, msg); }
+int32_t lib_launched(hipError_t e, const char* what)
+{
+    return launched(e, what, -1, nullptr);
+}
+int32_t lib_check_domain(const Dom* d) { return check_domain(d); }
+
 }  // namespace ARW_NS
 
 using namespace ARW_NS;

@@ -95,6 +95,12 @@ __device__ __forceinline__ const FinishDst& fin_of(const Batch& b, int m,
     return Many::fin(b, m);
 }
 
+// Error reporting and argument checks shared by the library's sources
+// (defined in dycore.hip).
+int32_t lib_fail(int32_t code, const char* msg);
+int32_t lib_launched(hipError_t e, const char* what);
+int32_t lib_check_domain(const Dom* d);
+
 hipError_t run_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st);
 hipError_t run_tend_faces(const Dom& d, const Sc& s, hipStream_t st);
 hipError_t run_tend_cols(const Dom& d, const Sc& s, hipStream_t st);

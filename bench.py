@@ -377,6 +377,31 @@ def main_adjoint(args):
                       - float(loss(q0 - eps * x))) / (2 * eps)
             ad = float(torch.sum(grad * x))
             rec["fd_rel_err"] = abs(fd - ad) / abs(ad)
+        # one acoustic substep (stage-3 shape, state after a step) and its
+        # adjoint at the same small-step state
+        dtau = cfg.dt / cfg.time_step_sound
+        dycore.prepare_acoustic_coefficients(st, cfg, dtau)
+        xs = {n: getattr(st, n).clone() for n in adjoint.ACOUSTIC_STATE}
+
+        def restore():
+            for n, t in xs.items():
+                getattr(st, n).copy_(t)
+
+        def sub_fwd():
+            restore()
+            dycore.acoustic_substep(st, cfg, dtau, False)
+
+        lam = {n: torch.randn_like(getattr(st, n))
+               for n in adjoint.ACOUSTIC_STATE}
+        t_rest = _time_ms(restore, reps)
+        t_sub = _time_ms(sub_fwd, reps) - t_rest
+        restore()
+        t_sub_adj = _time_ms(lambda: adjoint.acoustic_substep_adjoint(
+            st, cfg, dtau, False, lam), reps)
+        rec.update({"acoustic_substep_ms": round(t_sub, 4),
+                    "acoustic_adjoint_ms": round(t_sub_adj, 4),
+                    "acoustic_adjoint_to_forward": round(t_sub_adj / t_sub,
+                                                         3)})
         out[name] = rec
         del st, ck
         torch.cuda.empty_cache()

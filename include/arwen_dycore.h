@@ -206,6 +206,34 @@ int32_t arw_ensemble_capture(arw_ensemble *e, int32_t nsteps);
 int32_t arw_ensemble_launch(arw_ensemble *e, void *stream);
 int32_t arw_ensemble_destroy(arw_ensemble *e);
 
+/*
+ * Adjoint of one acoustic substep (arw_acoustic_substep; reference
+ * acoustic.cu:160-982 advance_uv -> advance_mu_th -> advance_w_phi, which
+ * has no adjoint in the reference).  The substep's input X is the
+ * domain's small-step state (u_pp, v_pp, w_pp, th_pp, ph_pp, mu_pp, p_pp,
+ * p_pp_old, al_pp) with the stage coefficients prepared as for the
+ * forward substep (arw_calc_coefs); the domain is not modified.  l_*
+ * are the adjoints of the substep's outputs (l_ww_pp may be NULL), g_*
+ * receive J^T l for the inputs and the slow tendencies (same shapes as the
+ * arw_domain fields of the same names).  The Jacobian is taken at X (the
+ * new column mass enters three denominators).  work holds
+ * arw_acoustic_adjoint_work(...) elements.
+ */
+typedef struct arw_acoustic_adj {
+    const float *l_u_pp, *l_v_pp, *l_w_pp, *l_th_pp, *l_ph_pp, *l_mu_pp,
+        *l_p_pp, *l_p_pp_old, *l_al_pp, *l_ww_pp;
+    float *g_u_pp, *g_v_pp, *g_w_pp, *g_th_pp, *g_ph_pp, *g_mu_pp, *g_p_pp,
+        *g_p_pp_old, *g_al_pp;
+    float *g_ru_t, *g_rv_t, *g_rw_t, *g_rth_t, *g_rph_t, *g_rmu_t;
+} arw_acoustic_adj;
+int32_t arw_acoustic_substep_adjoint(const arw_domain *d,
+                                     const arw_params *prm, double dtau,
+                                     int32_t first,
+                                     const arw_acoustic_adj *adj,
+                                     float *work, void *stream);
+int32_t arw_acoustic_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
+                                  int64_t *elements);
+
 /* EOS p, al, alt from thp, php, mup (replaces diagnostics.cu:37-89
  * calc_p_alpha, launched by diagnostics.py:14-57; hypsometric_opt = 1). */
 int32_t arw_calc_p_alpha(const arw_domain *d, void *stream);
@@ -339,6 +367,21 @@ int32_t arw64_tracer_adjoint(const arw64_domain *d, const arw_params *prm,
                              const arw64_tracer_ckpt *ck,
                              const double *lam_in, double *lam_out,
                              double *work, void *stream);
+
+typedef struct arw64_acoustic_adj {
+    const double *l_u_pp, *l_v_pp, *l_w_pp, *l_th_pp, *l_ph_pp, *l_mu_pp,
+        *l_p_pp, *l_p_pp_old, *l_al_pp, *l_ww_pp;
+    double *g_u_pp, *g_v_pp, *g_w_pp, *g_th_pp, *g_ph_pp, *g_mu_pp,
+        *g_p_pp, *g_p_pp_old, *g_al_pp;
+    double *g_ru_t, *g_rv_t, *g_rw_t, *g_rth_t, *g_rph_t, *g_rmu_t;
+} arw64_acoustic_adj;
+int32_t arw64_acoustic_substep_adjoint(const arw64_domain *d,
+                                       const arw_params *prm, double dtau,
+                                       int32_t first,
+                                       const arw64_acoustic_adj *adj,
+                                       double *work, void *stream);
+int32_t arw64_acoustic_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
+                                    int64_t *elements);
 
 typedef struct arw64_ensemble arw64_ensemble;
 int32_t arw64_ensemble_create(const arw64_domain *members, int32_t n,

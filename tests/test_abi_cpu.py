@@ -104,6 +104,8 @@ def _c_layout(struct, fields, tmp_path):
 @pytest.mark.parametrize("cname,pyt", [
     ("arw_tracer_ckpt", _lib.ArwTracerCkpt),
     ("arw64_tracer_ckpt", _lib.ArwTracerCkpt64),
+    ("arw_acoustic_adj", _lib.ArwAcousticAdj),
+    ("arw64_acoustic_adj", _lib.ArwAcousticAdj64),
     ("arw_domain", _lib.ArwDomain), ("arw64_domain", _lib.ArwDomain64),
     ("arw_params", _lib.ArwParams)])
 def test_ctypes_offsets_match_the_c_compiler(cname, pyt, tmp_path):
