@@ -54,82 +54,45 @@ __device__ __forceinline__ real th_ref(const Dom& d, int k, int c)
     return (d.base3d ? d.thb[k * d.ny * d.nx + c] : d.thb[k]) + d.thp[k * d.ny * d.nx + c];
 }
 
-__device__ __forceinline__ real phi_ref(const Dom& d, int k, int c)
+__device__ __forceinline__ real phi_full(const Dom& d, int k, int c)
 {
     return (d.base3d ? d.phb[k * d.ny * d.nx + c] : d.phb[k]) + d.php[k * d.ny * d.nx + c];
 }
 
-// Mass-column reverse of advance_w_phi + diagnose_p and of the vertical
-// parts of advance_mu_th.
-__global__ void k_adj_col(Dom d, Sc s, Adj a, Fw f)
+// Column arrays of the mass-column pass.  Every pass below streams the
+// column once in one direction, carrying neighbour-level values in
+// registers, so no level reads what another level's iteration wrote; the
+// restrict qualifiers then let the loads of unrolled iterations issue
+// together.
+struct ColIO {
+    const real* __restrict__ lth; const real* __restrict__ lp;
+    const real* __restrict__ lal; const real* __restrict__ lph;
+    const real* __restrict__ lw; const real* __restrict__ lmu;
+    const real* __restrict__ lww;
+    const real* __restrict__ T1; const real* __restrict__ AL1;
+    const real* __restrict__ W1; const real* __restrict__ M1;
+};
+
+__device__ __forceinline__ void adj_col_body(
+    const Dom& d, const Sc& s, const int c, const ColIO io,
+    real* __restrict__ GG, real* __restrict__ LRHS, real* __restrict__ LW,
+    real* __restrict__ SA, real* __restrict__ SG, real* __restrict__ LT2,
+    real* __restrict__ GPH, real* __restrict__ LWW, real* __restrict__ LDVP,
+    real* __restrict__ L0, real* __restrict__ gw, real* __restrict__ grw,
+    real* __restrict__ gph, real* __restrict__ grph, real* __restrict__ gth,
+    real* __restrict__ grth, real* __restrict__ gmu, real* __restrict__ grmu)
 {
-    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= st) return;
-    const real half = R(0.5), one = R(1.0), g = G_F;
+    const int nz = d.nz, st = d.ny * d.nx;
+    const real half = R(0.5), one = R(1.0), g = G_F, zero = R(0.0);
     const real dt = s.dtau, eps = s.epssm;
     const real mut = d.mub2d[c] + d.mup[c];
-    const real M1 = f.M1[c], M0 = d.mu_pp[c];
-    real* LPH1 = f.s[W_LPH1];
-    real* LW = f.s[W_LW];
-    real* SA = f.s[W_SA];
-    real* SG = f.s[W_SG];
-    real* LRHS = f.s[W_LRHS];
-    real* LT2 = f.s[W_LT2];
-    real* LWD = f.s[W_LWD];
-    real* LWW = f.s[W_LWW];
-    real* GG = f.s[W_GG];
-    real* LDVP = f.s[W_LDVP];
+    const real M1 = io.M1[c];
+    const real K = half * dt * g * (one + eps);
+    real cof = half * dt * g * (one + eps);
+    cof = cof * cof;
     auto c2a_at = [&](int k) {
         return (GAMMA_F * d.p[k * st + c]) / d.alt[k * st + c];
     };
-    real lM1 = a.lmu[c];
-
-    // phi''-adjoint column, EOS reverse; GG holds lambda(theta''_new)
-    for (int k = 0; k <= nz; ++k) LPH1[k * st + c] = a.lph[k * st + c];
-    for (int k = 0; k < nz; ++k) {
-        const int x = k * st + c;
-        const real thr = th_ref(d, k, c), alt = d.alt[x], c2a = c2a_at(k);
-        const real chmp = d.c1h[k] * (mut + M1) + d.c2h[k];
-        const real q = alt / (chmp * thr);
-        const real X = f.T1[x] - d.c1h[k] * M1 * thr;
-        const real lPP = a.lp[x];
-        GG[x] = a.lth[x] + c2a * q * lPP;
-        lM1 += c2a * (-q * d.c1h[k] * thr - q * X * d.c1h[k] / chmp) * lPP;
-        const real lAL = a.lal[x] - c2a * lPP;
-        lM1 += (-(alt * d.c1h[k]) / chmp - f.AL1[x] * d.c1h[k] / chmp) * lAL;
-        LPH1[(k + 1) * st + c] += -d.rdnw[k] / chmp * lAL;
-        LPH1[x] += d.rdnw[k] / chmp * lAL;
-    }
-    // phi'' update: Ph1 = rhs + K W1 / cfs
-    const real K = half * dt * g * (one + eps);
-    LRHS[c] = R(0.0);
-    LW[c] = a.lw[c];
-    for (int k = 1; k <= nz; ++k) {
-        const int x = k * st + c;
-        const real cfs = d.c1f[k] * (mut + M1) + d.c2f[k];
-        LRHS[x] = LPH1[x];
-        LW[x] = a.lw[x] + K / cfs * LPH1[x];
-        lM1 += -K * f.W1[x] * d.c1f[k] / (cfs * cfs) * LPH1[x];
-    }
-    real gph0 = LPH1[c];                       // phi''(0) passes through
-    // damper
-    if (s.damp) {
-        const real htop = phi_ref(d, nz, c) / g;
-        const real hbot = htop - s.zdamp;
-        for (int k = 1; k <= nz; ++k) {
-            const real hk = phi_ref(d, k, c) / g;
-            if (hk >= hbot) {
-                const real arg = R(1.5707963) * (hk - hbot) / s.zdamp;
-                const real sn = (real)sin((double)arg);
-                const real dw = s.dampmag * sn * sn;
-                LW[k * st + c] = LW[k * st + c] / (one + dw);
-            }
-        }
-    }
-    // Thomas factors of the implicit w system (calc_coef_w)
-    real cof = half * dt * g * (one + eps);
-    cof = cof * cof;
     auto chm_at = [&](int k) { return d.c1h[k] * mut + d.c2h[k]; };
     auto cfm_at = [&](int k) { return d.c1f[k] * mut + d.c2f[k]; };
     auto a_at = [&](int k) -> real {
@@ -139,291 +102,386 @@ __global__ void k_adj_col(Dom d, Sc s, Adj a, Fw f)
         if (k == nz && !s.top_lid)
             return R(-2.0) * cof * d.rdnw[nz - 1] * d.rdnw[nz - 1]
                    * c2a_at(nz - 1) / (chm_at(nz - 1) * cfm_at(nz - 1));
-        return R(0.0);
+        return zero;
     };
-    {
-        real gprev = R(0.0);
-        for (int k = 1; k < nz; ++k) {
-            const real b = one + cof * d.rdn[k] * (
-                d.rdnw[k] * c2a_at(k) / (chm_at(k) * cfm_at(k))
-                + d.rdnw[k - 1] * c2a_at(k - 1) / (chm_at(k - 1) * cfm_at(k)));
-            const real cc = -cof * d.rdn[k] * d.rdnw[k] * c2a_at(k)
-                            / (chm_at(k) * cfm_at(k + 1));
-            const real al = one / (b - a_at(k) * gprev);
-            SA[k * st + c] = al;
-            gprev = cc * al;
-            SG[k * st + c] = gprev;
+    // damper factor 1/(1+dw) of level k (1 where undamped)
+    real hbot = zero;
+    if (s.damp) hbot = phi_full(d, nz, c) / g - s.zdamp;
+    auto undamp = [&](int k) -> real {
+        if (!s.damp) return one;
+        const real hk = phi_full(d, k, c) / g;
+        if (!(hk >= hbot)) return one;
+        const real arg = R(1.5707963) * (hk - hbot) / s.zdamp;
+        const real sn = (real)sin((double)arg);
+        return one / (one + s.dampmag * sn * sn);
+    };
+    real lM1 = io.lmu[c];
+
+    // A (up): EOS reverse, phi'' update reverse, damper, Thomas factors and
+    // the forward sweep of the transposed solve
+    real carry_ph = zero, gph0 = zero, vprev = zero, gprev = zero;
+    auto w_in = [&](int k, real ph1) {          // lambda(w) entering the solve
+        const real cfs = d.c1f[k] * (mut + M1) + d.c2f[k];
+        lM1 += -K * io.W1[k * st + c] * d.c1f[k] / (cfs * cfs) * ph1;
+        real lw = io.lw[k * st + c] + K / cfs * ph1;
+        const real u = undamp(k);
+        return s.damp ? lw * u : lw;
+    };
+#pragma unroll 4
+    for (int k = 0; k < nz; ++k) {
+        const int x = k * st + c;
+        const real thr = th_ref(d, k, c), alt = d.alt[x], c2a = c2a_at(k);
+        const real chmp = d.c1h[k] * (mut + M1) + d.c2h[k];
+        const real q = alt / (chmp * thr);
+        const real X = io.T1[x] - d.c1h[k] * M1 * thr;
+        const real lPP = io.lp[x];
+        GG[x] = io.lth[x] + c2a * q * lPP;
+        lM1 += c2a * (-q * d.c1h[k] * thr - q * X * d.c1h[k] / chmp) * lPP;
+        const real lAL = io.lal[x] - c2a * lPP;
+        lM1 += (-(alt * d.c1h[k]) / chmp - io.AL1[x] * d.c1h[k] / chmp) * lAL;
+        const real ph1 = io.lph[x] + carry_ph + d.rdnw[k] / chmp * lAL;
+        carry_ph = -d.rdnw[k] / chmp * lAL;
+        if (k == 0) {
+            gph0 = ph1;
+            LW[c] = io.lw[c];
+            continue;
         }
+        LRHS[x] = ph1;
+        const real lw = w_in(k, ph1);
+        // Thomas factors of row k (calc_coef_w) and the transposed sweep
+        const real b = one + cof * d.rdn[k] * (
+            d.rdnw[k] * c2a / (chm_at(k) * cfm_at(k))
+            + d.rdnw[k - 1] * c2a_at(k - 1) / (chm_at(k - 1) * cfm_at(k)));
+        const real cc = -cof * d.rdn[k] * d.rdnw[k] * c2a / (chm_at(k) * cfm_at(k + 1));
+        const real ak = a_at(k);
+        const real al = one / (b - ak * gprev);
+        const real v = lw - gprev * vprev;
+        gprev = cc * al;
+        vprev = v;
+        LW[x] = v;
+        SA[x] = al;
+        SG[x] = ak;
+    }
+    {
+        const int x = nz * st + c;
+        const real ph1 = io.lph[x] + carry_ph;
+        LRHS[x] = ph1;
+        const real lw = w_in(nz, ph1);
         const real bn = one + R(2.0) * cof * d.rdnw[nz - 1] * d.rdnw[nz - 1]
                         * c2a_at(nz - 1) / (chm_at(nz - 1) * cfm_at(nz));
-        SA[nz * st + c] = one / (bn - a_at(nz) * gprev);
+        const real an = a_at(nz);
+        LW[x] = lw - gprev * vprev;
+        SA[x] = one / (bn - an * gprev);
+        SG[x] = an;
     }
-    for (int k = 1; k < nz; ++k)
-        LW[(k + 1) * st + c] += -SG[k * st + c] * LW[k * st + c];
+    // B (down): backward sweep of the transposed solve
+    real carry = zero;
+#pragma unroll 4
     for (int k = nz; k >= 1; --k) {
-        const real v = SA[k * st + c] * LW[k * st + c];
-        LW[k * st + c] = v;
-        LW[(k - 1) * st + c] += -a_at(k) * v;
-    }
-    // w accumulation reverse
-    for (int k = 0; k < nz; ++k) LT2[k * st + c] = R(0.0);
-    real lmuave = R(0.0);
-    a.gw[c] = R(0.0);
-    for (int k = 1; k <= nz; ++k) a.gw[k * st + c] = R(0.0);
-    for (int k = 0; k <= nz; ++k) a.gph[k * st + c] = R(0.0);
-    for (int k = 1; k < nz; ++k) {
         const int x = k * st + c;
-        const real la = LW[x];
-        a.gw[x] += la;
-        a.grw[x] = dt * la;
-        const real cup = half * dt * g * d.rdn[k] * c2a_at(k) * d.rdnw[k] / chm_at(k);
-        const real cdn = half * dt * g * d.rdn[k] * c2a_at(k - 1) * d.rdnw[k - 1]
-                         / chm_at(k - 1);
-        const real lup = cup * la, ldn = -cdn * la;
-        LRHS[(k + 1) * st + c] += (one + eps) * lup;
-        LRHS[x] -= (one + eps) * lup;
-        a.gph[(k + 1) * st + c] += (one - eps) * lup;
-        a.gph[x] -= (one - eps) * lup;
-        LRHS[x] += (one + eps) * ldn;
-        LRHS[(k - 1) * st + c] -= (one + eps) * ldn;
-        a.gph[x] += (one - eps) * ldn;
-        a.gph[(k - 1) * st + c] -= (one - eps) * ldn;
-        LT2[x] += dt * g * d.rdn[k] * c2a_at(k) * d.alt[x] * la;
-        LT2[(k - 1) * st + c] -= dt * g * d.rdn[k] * c2a_at(k - 1)
-                                 * d.alt[(k - 1) * st + c] * la;
-        lmuave += -dt * g * d.c1f[k] * la;
+        const real v = SA[x] * (LW[x] + carry);
+        LW[x] = v;
+        carry = -SG[x] * v;
     }
-    a.grw[c] = R(0.0);
-    if (s.top_lid) {
-        a.grw[nz * st + c] = R(0.0);
-    } else {
-        const int x = nz * st + c;
-        const real la = LW[x];
-        a.gw[x] += la;
-        a.grw[x] = dt * la;
-        const real cb = dt * g * c2a_at(nz - 1) * d.rdnw[nz - 1] * d.rdnw[nz - 1]
-                        / chm_at(nz - 1);
-        const real ldn = -cb * la;
-        LRHS[x] += (one + eps) * ldn;
-        LRHS[(nz - 1) * st + c] -= (one + eps) * ldn;
-        a.gph[x] += (one - eps) * ldn;
-        a.gph[(nz - 1) * st + c] -= (one - eps) * ldn;
-        LT2[(nz - 1) * st + c] -= dt * g * R(2.0) * d.rdnw[nz - 1] * c2a_at(nz - 1)
-                                  * d.alt[(nz - 1) * st + c] * la;
-        lmuave += -dt * g * d.c1f[nz] * la;
+    L0[c] = LW[c] + carry;
+
+    // C (up): reverse of the w accumulation; la(k) = lambda of row k
+    auto la_at = [&](int k) -> real {
+        if (k >= 1 && k < nz) return LW[k * st + c];
+        if (k == nz && !s.top_lid) return LW[nz * st + c];
+        return zero;
+    };
+    auto lup_at = [&](int k, real la) -> real {
+        if (k < 1 || k >= nz) return zero;
+        return half * dt * g * d.rdn[k] * c2a_at(k) * d.rdnw[k] / chm_at(k) * la;
+    };
+    auto ldn_at = [&](int k, real la) -> real {
+        if (k >= 1 && k < nz)
+            return -(half * dt * g * d.rdn[k] * c2a_at(k - 1) * d.rdnw[k - 1]
+                     / chm_at(k - 1)) * la;
+        if (k == nz && !s.top_lid)
+            return -(dt * g * c2a_at(nz - 1) * d.rdnw[nz - 1] * d.rdnw[nz - 1]
+                     / chm_at(nz - 1)) * la;
+        return zero;
+    };
+    real lmuave = zero;
+    real la_k = zero, lup_km = zero;            // level 0: no row
+    real la_kp = la_at(1);
+    real ldn_k = zero;
+#pragma unroll 4
+    for (int k = 0; k <= nz; ++k) {
+        const int x = k * st + c;
+        const real la_next = (k + 2 <= nz) ? la_at(k + 2) : zero;
+        const real lup_k = lup_at(k, la_k);
+        const real ldn_kp = ldn_at(k + 1, la_kp);
+        const real T = (lup_km - lup_k) + (ldn_k - ldn_kp);
+        GPH[x] = (one - eps) * T;
+        if (k >= 1) LRHS[x] = LRHS[x] + (one + eps) * T;
+        const bool row = (k >= 1 && k < nz) || (k == nz && !s.top_lid);
+        gw[x] = row ? la_k : zero;
+        grw[x] = row ? dt * la_k : zero;
+        if (k < nz) {
+            real t2 = zero;
+            if (k >= 1) t2 += dt * g * d.rdn[k] * c2a_at(k) * d.alt[x] * la_k;
+            if (k + 1 < nz)
+                t2 -= dt * g * d.rdn[k + 1] * c2a_at(k) * d.alt[x] * la_kp;
+            if (k == nz - 1 && !s.top_lid)
+                t2 -= dt * g * R(2.0) * d.rdnw[nz - 1] * c2a_at(nz - 1)
+                      * d.alt[x] * la_kp;
+            LT2[x] = t2;
+        }
+        if (row) lmuave += -dt * g * d.c1f[k] * la_k;
+        lup_km = lup_k;
+        ldn_k = ldn_kp;
+        la_k = la_kp;
+        la_kp = la_next;
     }
-    f.s[W_L0][c] = LW[c];                        // lower-boundary w''
-    // t2 and muave
+
+    // D (up): rhs reverse -> phi'', rph_t, w'', and Omega'' via wd
+    if (s.top_lid) LRHS[nz * st + c] = zero;
+    real lwd_prev = zero;
+    auto lr2_at = [&](int k) -> real {
+        return (k >= 1 && k <= nz) ? LRHS[k * st + c] / cfm_at(k) : zero;
+    };
+    real lr2_k = zero;                          // level 0
+    real lr2_kp = lr2_at(1);
+#pragma unroll 4
+    for (int k = 0; k <= nz; ++k) {
+        const int x = k * st + c;
+        const real lr2_next = (k + 2 <= nz) ? lr2_at(k + 2) : zero;
+        if (k == 0) {
+            gph[x] = GPH[x] + gph0;
+            grph[x] = zero;
+        } else {
+            gph[x] = GPH[x] + LRHS[x];
+            grph[x] = dt * lr2_k;
+            gw[x] = gw[x] + dt * half * g * (one - eps) * lr2_k;
+        }
+        // lambda(wd[k]) (k < nz): -dt (fnm[k] lr2[k] + fnp[k+1] lr2[k+1])
+        real lwd = zero;
+        if (k < nz) {
+            if (k >= 1) lwd += -dt * d.fnm[k] * lr2_k;
+            if (k + 1 <= nz - 1) lwd += -dt * d.fnp[k + 1] * lr2_kp;
+        }
+        // Omega''(k) gets cwd[k] lwd[k] + cwd[k-1] lwd[k-1]
+        const real cwd = (k < nz) ? half * d.rdnw[k] * (phi_full(d, k + 1, c)
+                                                        - phi_full(d, k, c))
+                                  : zero;
+        real ww = io.lww ? io.lww[x] : zero;
+        ww += cwd * lwd + lwd_prev;
+        LWW[x] = ww;
+        lwd_prev = (k < nz) ? cwd * lwd : zero;
+        lr2_k = lr2_kp;
+        lr2_kp = lr2_next;
+    }
+
+    // E (up): theta'' adjoint (t2 term, then the theta'' update), the
+    // vertical theta flux into Omega''
     real lM0old = half * (one - eps) * lmuave;
     lM1 += half * (one + eps) * lmuave;
+    real gg_prev = zero;
+#pragma unroll 4
     for (int k = 0; k < nz; ++k) {
         const int x = k * st + c;
         const real thr = th_ref(d, k, c);
         const real chs = d.c1h[k] * (mut + M1) + d.c2h[k];
-        const real t2 = half * ((one + eps) * f.T1[x] + (one - eps) * d.th_pp[x])
+        const real t2 = half * ((one + eps) * io.T1[x] + (one - eps) * d.th_pp[x])
                         / (chs * thr);
-        GG[x] += half * (one + eps) / (chs * thr) * LT2[x];
-        a.gth[x] = half * (one - eps) / (chs * thr) * LT2[x];
-        lM1 += -t2 * d.c1h[k] / chs * LT2[x];
-    }
-    // rhs reverse
-    if (s.top_lid) LRHS[nz * st + c] = R(0.0);
-    for (int k = 0; k < nz; ++k) LWD[k * st + c] = R(0.0);
-    a.grph[c] = R(0.0);
-    for (int k = 1; k <= nz; ++k) {
-        const int x = k * st + c;
-        a.gph[x] += LRHS[x];
-        const real lr2 = LRHS[x] / cfm_at(k);
-        a.grph[x] = dt * lr2;
-        a.gw[x] += dt * half * g * (one - eps) * lr2;
-        if (k < nz) {
-            LWD[x] += -dt * d.fnm[k] * lr2;
-            LWD[(k - 1) * st + c] += -dt * d.fnp[k] * lr2;
+        const real lt2 = LT2[x];
+        const real lT1 = GG[x] + half * (one + eps) / (chs * thr) * lt2;
+        gth[x] = half * (one - eps) / (chs * thr) * lt2 + lT1;
+        grth[x] = dt * lT1;
+        lM1 += -t2 * d.c1h[k] / chs * lt2;
+        const real gg = -dt * lT1;
+        GG[x] = gg;
+        if (k >= 1) {
+            const real lwdtn = d.rdnw[k - 1] * gg_prev - d.rdnw[k] * gg;
+            LWW[x] = LWW[x] + (d.fnm[k] * thr + d.fnp[k] * th_ref(d, k - 1, c)) * lwdtn;
         }
+        gg_prev = gg;
     }
-    a.gph[c] += gph0;
-    for (int k = 0; k <= nz; ++k)
-        LWW[k * st + c] = a.lww ? a.lww[k * st + c] : R(0.0);
-    for (int k = 0; k < nz; ++k) {
-        const real cwd = half * d.rdnw[k] * (phi_ref(d, k + 1, c) - phi_ref(d, k, c));
-        LWW[(k + 1) * st + c] += cwd * LWD[k * st + c];
-        LWW[k * st + c] += cwd * LWD[k * st + c];
-    }
-    // advance_mu_th reverse (vertical parts); GG becomes -dt lambda(theta'')
-    for (int k = 0; k < nz; ++k) {
-        const int x = k * st + c;
-        const real lT1 = GG[x];
-        a.gth[x] += lT1;
-        a.grth[x] = dt * lT1;
-        GG[x] = -dt * lT1;
-    }
-    for (int k = 1; k < nz; ++k) {
-        // lambda(wdtn[k]) = rdnw[k-1] gg[k-1] - rdnw[k] gg[k]
-        const real lwdtn = d.rdnw[k - 1] * GG[(k - 1) * st + c]
-                           - d.rdnw[k] * GG[k * st + c];
-        LWW[k * st + c] += (d.fnm[k] * th_ref(d, k, c)
-                            + d.fnp[k] * th_ref(d, k - 1, c)) * lwdtn;
-    }
-    real acc = R(0.0), ldmdt = R(0.0), grmu = R(0.0);
+
+    // F (down): Omega'' suffix sums -> mu'' and the divergence adjoint
+    real acc = zero, ldmdt = zero, gr = zero;
+#pragma unroll 4
     for (int k = nz - 1; k >= 0; --k) {
-        // linc[k] = -sum_{kk=k+1}^{nz-1} lww[kk]
-        real linc = R(0.0);
+        real linc = zero;
         if (k < nz - 1) {
             acc += LWW[(k + 1) * st + c];
             linc = -acc;
         }
         LDVP[k * st + c] = d.dnw[k] * linc;
         ldmdt += d.dnw[k] * d.c1h[k] * linc;
-        grmu += d.dnw[k] * d.c1h[k] * linc;
+        gr += d.dnw[k] * d.c1h[k] * linc;
     }
     ldmdt += dt * lM1;
-    grmu += dt * lM1;
-    a.grmu[c] = grmu;
-    a.gmu[c] = lM1 + lM0old;
-    for (int k = 0; k < nz; ++k) LDVP[k * st + c] += d.dnw[k] * ldmdt;
-    (void)M0;
+    gr += dt * lM1;
+    grmu[c] = gr;
+    gmu[c] = lM1 + lM0old;
+#pragma unroll 4
+    for (int k = 0; k < nz; ++k) {
+        const int x = k * st + c;
+        LDVP[x] = LDVP[x] + d.dnw[k] * ldmdt;
+    }
 }
 
-// lambda of u''_new at u face (j, fc) (fc = 0..nx) and of v''_new at v
-// face (g, i): the output adjoint plus what the mass cells on both sides
-// took from it (divergence, theta flux, lower boundary).
-__device__ __forceinline__ real lam_u1(const Dom& d, const Adj& a,
-                                       const Fw& f, int k, int j, int fc,
-                                       const real* cf, real rdx)
+// Mass-column reverse of advance_w_phi + diagnose_p and of the vertical
+// parts of advance_mu_th.
+__global__ __launch_bounds__(64) void k_adj_col(Dom d, Sc s, Adj a, Fw f)
 {
-    const int ny = d.ny, nx = d.nx, st = ny * nx;
-    const real half = R(0.5);
-    const real* GG = f.s[W_GG];
-    const real* LDVP = f.s[W_LDVP];
-    const real* L0 = f.s[W_L0];
-    real v = a.lu[iU(k, j, fc, ny, nx)];
-    const int ce = fc - 1;                      // cell east of which fc is (fc >= 1)
-    const int cw = fc;                          // cell whose west face fc is (fc < nx)
-    if (fc >= 1) {
-        const int c = j * nx + ce;
-        const int cp = j * nx + (fc == nx ? 0 : fc);
-        v += half * rdx * (th_ref(d, k, cp) + th_ref(d, k, c)) * GG[k * st + c];
-        v += rdx * LDVP[k * st + c];
-        if (k < 3) v += cf[k] * half * rdx * (d.ht[cp] - d.ht[c]) * L0[c];
-    }
-    if (fc < nx) {
-        const int c = j * nx + cw;
-        const int cm = j * nx + (fc == 0 ? nx - 1 : fc - 1);
-        v -= half * rdx * (th_ref(d, k, c) + th_ref(d, k, cm)) * GG[k * st + c];
-        v -= rdx * LDVP[k * st + c];
-        if (k < 3) v += cf[k] * half * rdx * (d.ht[c] - d.ht[cm]) * L0[c];
-    }
-    return v;
-}
-
-__device__ __forceinline__ real lam_v1(const Dom& d, const Adj& a,
-                                       const Fw& f, int k, int g, int i,
-                                       const real* cf, real rdy)
-{
-    const int ny = d.ny, nx = d.nx, st = ny * nx;
-    const real half = R(0.5);
-    const real* GG = f.s[W_GG];
-    const real* LDVP = f.s[W_LDVP];
-    const real* L0 = f.s[W_L0];
-    real v = a.lv[iV(k, g, i, ny, nx)];
-    if (g >= 1) {
-        const int c = (g - 1) * nx + i;
-        const int cp = (g == ny ? 0 : g) * nx + i;
-        v += half * rdy * (th_ref(d, k, cp) + th_ref(d, k, c)) * GG[k * st + c];
-        v += rdy * LDVP[k * st + c];
-        if (k < 3) v += cf[k] * half * rdy * (d.ht[cp] - d.ht[c]) * L0[c];
-    }
-    if (g < ny) {
-        const int c = g * nx + i;
-        const int cm = (g == 0 ? ny - 1 : g - 1) * nx + i;
-        v -= half * rdy * (th_ref(d, k, c) + th_ref(d, k, cm)) * GG[k * st + c];
-        v -= rdy * LDVP[k * st + c];
-        if (k < 3) v += cf[k] * half * rdy * (d.ht[c] - d.ht[cm]) * L0[c];
-    }
-    return v;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.ny * d.nx) return;
+    ColIO io;
+    io.lth = a.lth; io.lp = a.lp; io.lal = a.lal; io.lph = a.lph;
+    io.lw = a.lw; io.lmu = a.lmu; io.lww = a.lww;
+    io.T1 = f.T1; io.AL1 = f.AL1; io.W1 = f.W1; io.M1 = f.M1;
+    adj_col_body(d, s, c, io, f.s[W_GG], f.s[W_LRHS], f.s[W_LW], f.s[W_SA],
+                 f.s[W_SG], f.s[W_LT2], f.s[W_LWD], f.s[W_LWW], f.s[W_LDVP],
+                 f.s[W_L0], a.gw, a.grw, a.gph, a.grph, a.gth, a.grth, a.gmu,
+                 a.grmu);
 }
 
 // Face pass: u faces (j, fc), fc = 0..nx, then v faces (g, i), g = 0..ny.
-// For the non-duplicate faces also the advance_uv transpose pieces: the
-// adjoint of the bracket (LB), of the face-averaged damped p'' levels
-// folded onto the p'' levels (LS), and of the face mass sum (LM).
-__global__ void k_adj_faces(Dom d, Sc s, Adj a, Fw f)
+// lambda of u''_new (v''_new) is the output adjoint plus what the mass
+// cells on both sides took from it (divergence, theta flux, lower
+// boundary).  For the non-duplicate faces also the advance_uv transpose
+// pieces: the adjoint of the bracket (LB), of the face-averaged damped p''
+// levels folded onto the p'' levels (LS), and of the face mass sum (LM).
+struct FaceGeo {
+    int out0, ostride;       // output adjoint: index of level 0, level stride
+    int cE, cEp;             // cell whose east/north face this is, its +1 cell
+    int cW, cWm;             // cell whose west/south face this is, its -1 cell
+    bool hasE, hasW, own;
+    int cA, cB, slot;        // advance_uv cells of an own face, table slot
+};
+
+__device__ __forceinline__ void adj_face_body(
+    const Dom& d, const Sc& s, const FaceGeo& fg, const real rd,
+    const real* __restrict__ lout, const real* __restrict__ GG,
+    const real* __restrict__ LDVP, const real* __restrict__ L0,
+    real* __restrict__ gout, real* __restrict__ grt, real* __restrict__ LB,
+    real* __restrict__ LT, real* __restrict__ LS, real* __restrict__ LM,
+    const FaceGeo* dup)
 {
-    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    const int nz = d.nz, st = d.ny * d.nx;
+    const real half = R(0.5), dt = s.dtau, zero = R(0.0);
+    const real cf[3] = {d.cf1, d.cf2, d.cf3};
+    auto lam1 = [&](const FaceGeo& q, int k) -> real {
+        real v = lout[q.out0 + k * q.ostride];
+        if (q.hasE) {
+            const int c = q.cE, cp = q.cEp;
+            v += half * rd * (th_ref(d, k, cp) + th_ref(d, k, c)) * GG[k * st + c];
+            v += rd * LDVP[k * st + c];
+            if (k < 3) v += cf[k] * half * rd * (d.ht[cp] - d.ht[c]) * L0[c];
+        }
+        if (q.hasW) {
+            const int c = q.cW, cm = q.cWm;
+            v -= half * rd * (th_ref(d, k, c) + th_ref(d, k, cm)) * GG[k * st + c];
+            v -= rd * LDVP[k * st + c];
+            if (k < 3) v += cf[k] * half * rd * (d.ht[c] - d.ht[cm]) * L0[c];
+        }
+        return v;
+    };
+    const real mA = d.mub2d[fg.cA] + d.mup[fg.cA];
+    const real mB = d.mub2d[fg.cB] + d.mup[fg.cB];
+    const real muf = half * (mB + mA);
+    real lms = zero;
+#pragma unroll 4
+    for (int k = 0; k < nz; ++k) {
+        const real l1 = lam1(fg, k);
+        gout[fg.out0 + k * fg.ostride] = l1;
+        grt[fg.out0 + k * fg.ostride] = dt * l1;
+        if (!fg.own) continue;
+        const real ldup = dup ? lam1(*dup, k) : zero;
+        const real ld = -dt * l1 - dt * ldup;
+        const int x = k * st + fg.slot;
+        LB[x] = half * rd * (d.c1h[k] * muf + d.c2h[k]) * ld;
+        real dphp = half * ((d.php[k * st + fg.cA] + d.php[(k + 1) * st + fg.cA])
+                            - (d.php[k * st + fg.cB] + d.php[(k + 1) * st + fg.cB]));
+        if (d.base3d)
+            dphp = dphp + half * ((d.phb[k * st + fg.cA] + d.phb[(k + 1) * st + fg.cA])
+                                  - (d.phb[k * st + fg.cB] + d.phb[(k + 1) * st + fg.cB]));
+        const real tk = rd * dphp * ld;
+        LT[x] = tk;
+        lms += -half * d.c1h[k] * tk;
+    }
+    if (!fg.own) return;
+    LM[fg.slot] = lms;
+    // lambda(dpn) at full level kf: rdnw[kf-1] t[kf-1] - rdnw[kf] t[kf]
+    auto ldpn = [&](int kf) -> real {
+        real v = zero;
+        if (kf >= 1) v += d.rdnw[kf - 1] * LT[(kf - 1) * st + fg.slot];
+        if (kf <= nz - 1) v -= d.rdnw[kf] * LT[kf * st + fg.slot];
+        return v;
+    };
+    const real l0 = ldpn(0), ltop = s.top_lid ? ldpn(nz) : zero;
+    real dk = zero, dkp = ldpn(1);
+#pragma unroll 4
+    for (int k = 0; k < nz; ++k) {
+        const real dnext = (k + 2 <= nz - 1) ? ldpn(k + 2) : zero;
+        real v = zero;
+        if (k >= 1) v += half * d.fnm[k] * dk;
+        if (k + 1 <= nz - 1) v += half * d.fnp[k + 1] * dkp;
+        if (k < 3) v += half * cf[k] * l0;
+        if (s.top_lid && k >= nz - 3) v += half * cf[nz - 1 - k] * ltop;
+        LS[k * st + fg.slot] = v;
+        dk = dkp;
+        dkp = dnext;
+    }
+}
+
+__global__ __launch_bounds__(128) void k_adj_faces(Dom d, Sc s, Adj a, Fw f)
+{
+    const int ny = d.ny, nx = d.nx, st = ny * nx;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nu = ny * (nx + 1), nv = (ny + 1) * nx;
     if (t >= nu + nv) return;
-    const real half = R(0.5), dt = s.dtau;
-    const real cf[3] = {d.cf1, d.cf2, d.cf3};
-    const bool isu = t < nu;
-    int j, fc, g, i;
-    if (isu) { j = t / (nx + 1); fc = t - j * (nx + 1); g = 0; i = 0; }
-    else { const int r = t - nu; g = r / nx; i = r - g * nx; j = 0; fc = 0; }
-    real* LB = f.s[isu ? W_LBX : W_LBY];
-    real* LS = f.s[isu ? W_LSX : W_LSY];
-    real* LT = f.s[isu ? W_LPH1 : W_LW];         // dead column scratch reused
-    real* LM = f.s[isu ? W_LMX : W_LMY];
-    const real rd = isu ? s.rdx : s.rdy;
-    // cells A (this face's own index) and B (the one before it)
-    int cA, cB;
-    bool own;                                    // has its own dpxy
-    if (isu) {
-        own = fc < nx;
-        cA = j * nx + (fc < nx ? fc : 0);
-        cB = j * nx + (fc == 0 || fc == nx ? nx - 1 : fc - 1);
+    FaceGeo fg, dg;
+    bool has_dup = false;
+    if (t < nu) {
+        const int j = t / (nx + 1), fc = t - j * (nx + 1);
+        auto geo = [&](int f1, FaceGeo& q) {
+            q.out0 = iU(0, j, f1, ny, nx);
+            q.ostride = ny * (nx + 1);
+            q.hasE = f1 >= 1;
+            q.cE = j * nx + (f1 >= 1 ? f1 - 1 : 0);
+            q.cEp = j * nx + (f1 == nx ? 0 : f1);
+            q.hasW = f1 < nx;
+            q.cW = j * nx + (f1 < nx ? f1 : 0);
+            q.cWm = j * nx + (f1 == 0 ? nx - 1 : (f1 < nx ? f1 - 1 : 0));
+            q.own = f1 < nx;
+            q.cA = j * nx + (f1 < nx ? f1 : 0);
+            q.cB = j * nx + (f1 == 0 || f1 == nx ? nx - 1 : f1 - 1);
+            q.slot = j * nx + (f1 < nx ? f1 : 0);
+        };
+        geo(fc, fg);
+        if (fc == 0) { geo(nx, dg); has_dup = true; }
+        adj_face_body(d, s, fg, s.rdx, a.lu, f.s[W_GG], f.s[W_LDVP],
+                      f.s[W_L0], a.gu, a.gru, f.s[W_LBX], f.s[W_LPH1],
+                      f.s[W_LSX], f.s[W_LMX], has_dup ? &dg : nullptr);
     } else {
-        own = g < ny;
-        cA = (g < ny ? g : 0) * nx + i;
-        cB = (g == 0 || g == ny ? ny - 1 : g - 1) * nx + i;
+        const int r = t - nu, g = r / nx, i = r - g * nx;
+        auto geo = [&](int g1, FaceGeo& q) {
+            q.out0 = iV(0, g1, i, ny, nx);
+            q.ostride = (ny + 1) * nx;
+            q.hasE = g1 >= 1;
+            q.cE = (g1 >= 1 ? g1 - 1 : 0) * nx + i;
+            q.cEp = (g1 == ny ? 0 : g1) * nx + i;
+            q.hasW = g1 < ny;
+            q.cW = (g1 < ny ? g1 : 0) * nx + i;
+            q.cWm = (g1 == 0 ? ny - 1 : (g1 < ny ? g1 - 1 : 0)) * nx + i;
+            q.own = g1 < ny;
+            q.cA = (g1 < ny ? g1 : 0) * nx + i;
+            q.cB = (g1 == 0 || g1 == ny ? ny - 1 : g1 - 1) * nx + i;
+            q.slot = (g1 < ny ? g1 : 0) * nx + i;
+        };
+        geo(g, fg);
+        if (g == 0) { geo(ny, dg); has_dup = true; }
+        adj_face_body(d, s, fg, s.rdy, a.lv, f.s[W_GG], f.s[W_LDVP],
+                      f.s[W_L0], a.gv, a.grv, f.s[W_LBY], f.s[W_LW],
+                      f.s[W_LSY], f.s[W_LMY], has_dup ? &dg : nullptr);
     }
-    const int slot = isu ? j * nx + (own ? fc : 0) : (own ? g : 0) * nx + i;
-    const real mA = d.mub2d[cA] + d.mup[cA], mB = d.mub2d[cB] + d.mup[cB];
-    const real muf = half * (mB + mA);
-    real lms = R(0.0);
-    for (int k = 0; k < nz; ++k) {
-        real l1, ldup = R(0.0);
-        if (isu) {
-            l1 = lam_u1(d, a, f, k, j, fc, cf, s.rdx);
-            a.gu[iU(k, j, fc, ny, nx)] = l1;
-            a.gru[iU(k, j, fc, ny, nx)] = dt * l1;
-            if (fc == 0) ldup = lam_u1(d, a, f, k, j, nx, cf, s.rdx);
-        } else {
-            l1 = lam_v1(d, a, f, k, g, i, cf, s.rdy);
-            a.gv[iV(k, g, i, ny, nx)] = l1;
-            a.grv[iV(k, g, i, ny, nx)] = dt * l1;
-            if (g == 0) ldup = lam_v1(d, a, f, k, ny, i, cf, s.rdy);
-        }
-        if (!own) continue;
-        const real ld = -dt * l1 - dt * ldup;
-        const real cAk = half * rd * (d.c1h[k] * muf + d.c2h[k]);
-        LB[k * st + slot] = cAk * ld;
-        real dphp = half * ((d.php[k * st + cA] + d.php[(k + 1) * st + cA])
-                            - (d.php[k * st + cB] + d.php[(k + 1) * st + cB]));
-        if (d.base3d)
-            dphp = dphp + half * ((d.phb[k * st + cA] + d.phb[(k + 1) * st + cA])
-                                  - (d.phb[k * st + cB] + d.phb[(k + 1) * st + cB]));
-        const real tk = rd * dphp * ld;
-        LT[k * st + slot] = tk;
-        lms += -half * d.c1h[k] * tk;
-    }
-    if (!own) return;
-    LM[slot] = lms;
-    // lambda(dpn) at full levels kf = 0..nz, folded onto the levels k of S
-    auto ldpn = [&](int kf) -> real {
-        real v = R(0.0);
-        if (kf >= 1) v += d.rdnw[kf - 1] * LT[(kf - 1) * st + slot];
-        if (kf <= nz - 1) v -= d.rdnw[kf] * LT[kf * st + slot];
-        return v;
-    };
-    const real l0 = ldpn(0), ltop = s.top_lid ? ldpn(nz) : R(0.0);
-    for (int k = 0; k < nz; ++k) {
-        real v = R(0.0);
-        if (k >= 1) v += half * d.fnm[k] * ldpn(k);
-        if (k + 1 <= nz - 1) v += half * d.fnp[k + 1] * ldpn(k + 1);
-        if (k < 3) v += half * cf[k] * l0;
-        if (s.top_lid && k >= nz - 3) v += half * cf[nz - 1 - k] * ltop;
-        LS[k * st + slot] = v;
-    }
+    (void)st;
 }
 
 // Cell pass: gather the face pieces of the advance_uv transpose.
@@ -535,7 +593,7 @@ int32_t ARW_API(acoustic_substep_adjoint)(const Dom* d, const arw_params* prm,
     a.grth = adj->g_rth_t; a.grph = adj->g_rph_t; a.grmu = adj->g_rmu_t;
     const int cols = ny * nx;
     // the damper and dt follow the forward launcher's Sc
-    hipLaunchKernelGGL(k_adj_col, dim3((cols + 127) / 128), dim3(128), 0, st,
+    hipLaunchKernelGGL(k_adj_col, dim3((cols + 63) / 64), dim3(64), 0, st,
                        *d, s, a, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_col"))) return rc;
     const int nf = ny * (nx + 1) + (ny + 1) * nx;
