@@ -1578,9 +1578,11 @@ __device__ __forceinline__ void finish_face(const Dom& d, real* RS out,
     const int sf = STRIDE_KIND == 0 ? ny * (nx + 1) : (ny + 1) * nx;
     kreal* c1h = kview(d.c1h);
     kreal* c2h = kview(d.c2h);
-    unsigned bf = RB * f;
+    const unsigned bf = RB * f;
+    // a lone column per thread: unrolled, with no per-level opaque offset,
+    // so several levels' loads are in flight at once
+#pragma unroll 8
     for (int k = 0; k < nz; ++k) {
-        colb_opaque(bf);
         const real c1 = c1h[k], c2 = c2h[k];
         gst(out, k * sf, bf, ((c1 * msf + c2) * gld(x, k * sf, bf)
                               + gld(xpp, k * sf, bf)) / (c1 * mnf + c2));

@@ -96,23 +96,29 @@ def read_traffic(path, kernel):
     return None
 
 
-def cpu_baseline(workload: str, rows: int, steps: int):
-    """Float32 NumPy oracle on a (nz, rows, nx) strip of the workload."""
+def cpu_baseline(workload: str, rows: int, steps: int, dtype_name="float32"):
+    """NumPy oracle (float32, or float64) on a (nz, rows, nx) strip of the
+    workload."""
     import numpy as np
     from oracle import dycore_np as O
     case, _ = build_case(workload, ny_override=rows)
-    s = O.from_case(case, np.float32)
+    s = O.from_case(case, np.dtype(dtype_name).type)
     cfg = case.cfg
     t0 = time.perf_counter()
     for _ in range(steps):
         O.step(s, cfg)
     dt = time.perf_counter() - t0
     pts = cfg.nx * cfg.ny * cfg.nz
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count()
     return {"value": pts * steps / dt / 1e6, "unit": "Mpoint-steps/s",
             "cores": 1, "kind": "port",
-            "sample": f"float32 NumPy oracle, {steps} steps on a "
+            "sample": f"{dtype_name} NumPy oracle, {steps} steps on a "
                       f"{cfg.nx}x{cfg.ny}x{cfg.nz} strip of the same "
-                      f"workload ({dt:.1f} s, single-threaded ufuncs)"}
+                      f"workload ({dt:.1f} s, single-threaded ufuncs; "
+                      f"{ncpu} host cores visible)"}
 
 
 def main():
@@ -218,6 +224,8 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_rows,
                                            args.cpu_steps)
+        out["cpu_baseline_fp64"] = cpu_baseline(
+            args.workload, max(args.cpu_rows // 2, 8), 1, "float64")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
