@@ -10,14 +10,15 @@
 // adjoint first re-runs the forward substep on copies of X (the same
 // kernels as arw_acoustic_substep) and then applies J^T at X.
 //
-// Three passes, each a plain thread-per-column march (this is not the
-// throughput path):
-//   k_adj_col    mass columns: the vertical reverse of the w/phi solve,
-//                the EOS, the Thomas solve, the theta'' and mu''/Omega''
-//                updates; leaves per-cell adjoints for the horizontal parts
-//   k_adj_faces  u and v faces: adjoint of u''/v'' (gathered from the two
-//                cells of each face) and the per-face pieces of the
-//                advance_uv transpose
+// Passes:
+//   k_adj_p1/s1/p2/p3/s2  mass columns: the vertical reverse of the w/phi
+//                solve, the EOS, the Thomas solve, the theta'' and
+//                mu''/Omega'' updates, as level-parallel kernels around two
+//                column-serial ones; leaves per-cell adjoints for the
+//                horizontal parts
+//   k_adj_f1/f2  u and v faces, per level: adjoint of u''/v'' (gathered
+//                from the two cells of each face) and the per-face pieces
+//                of the advance_uv transpose
 //   k_adj_cells  mass columns: gather of the face pieces into p'', p''_old,
 //                phi'', mu'', alpha''
 #include "common.h"
@@ -34,7 +35,8 @@ enum {
     W_U, W_V, W_W, W_TH, W_PH, W_MU, W_P, W_PO, W_AL, W_WW,   // forward copies
     W_LPH1, W_LW, W_SA, W_SG, W_LRHS, W_LT2, W_LWD, W_LWW,   // column scratch
     W_GG, W_LDVP, W_L0,                                      // cell -> faces
-    W_LBX, W_LSX, W_LBY, W_LSY, W_LMX, W_LMY,                // faces -> cells
+    W_MC1, W_MC2, W_MC3, W_GGF, W_G0,                        // level pieces
+    W_LBX, W_LSX, W_LBY, W_LSY,                              // faces -> cells
     W_COUNT
 };
 
@@ -59,253 +61,288 @@ __device__ __forceinline__ real phi_full(const Dom& d, int k, int c)
     return (d.base3d ? d.phb[k * d.ny * d.nx + c] : d.phb[k]) + d.php[k * d.ny * d.nx + c];
 }
 
-// Column arrays of the mass-column pass.  Every pass below streams the
-// column once in one direction, carrying neighbour-level values in
-// registers, so no level reads what another level's iteration wrote; the
-// restrict qualifiers then let the loads of unrolled iterations issue
-// together.
-struct ColIO {
-    const real* __restrict__ lth; const real* __restrict__ lp;
-    const real* __restrict__ lal; const real* __restrict__ lph;
-    const real* __restrict__ lw; const real* __restrict__ lmu;
-    const real* __restrict__ lww;
-    const real* __restrict__ T1; const real* __restrict__ AL1;
-    const real* __restrict__ W1; const real* __restrict__ M1;
-};
-
-__device__ __forceinline__ void adj_col_body(
-    const Dom& d, const Sc& s, const int c, const ColIO io,
-    real* __restrict__ GG, real* __restrict__ LRHS, real* __restrict__ LW,
-    real* __restrict__ SA, real* __restrict__ SG, real* __restrict__ LT2,
-    real* __restrict__ GPH, real* __restrict__ LWW, real* __restrict__ LDVP,
-    real* __restrict__ L0, real* __restrict__ gw, real* __restrict__ grw,
-    real* __restrict__ gph, real* __restrict__ grph, real* __restrict__ gth,
-    real* __restrict__ grth, real* __restrict__ gmu, real* __restrict__ grmu)
-{
-    const int nz = d.nz, st = d.ny * d.nx;
-    const real half = R(0.5), one = R(1.0), g = G_F, zero = R(0.0);
-    const real dt = s.dtau, eps = s.epssm;
-    const real mut = d.mub2d[c] + d.mup[c];
-    const real M1 = io.M1[c];
-    const real K = half * dt * g * (one + eps);
-    real cof = half * dt * g * (one + eps);
-    cof = cof * cof;
-    auto c2a_at = [&](int k) {
+// The mass-column reverse of advance_w_phi + diagnose_p and of the
+// vertical parts of advance_mu_th, split by dependence: level-parallel
+// kernels (one thread per mass point and level, recomputing the few
+// neighbour-level terms they need) around two lean column-serial kernels
+// (the transposed Thomas solve; the column sums and the Omega'' suffix
+// sums).  Per-level pieces of column sums go to MC1..MC3.
+struct ColCtx {
+    const Dom& d;
+    const Sc& s;
+    int c, st, nz;
+    real mut, M1, K, cof;
+    __device__ ColCtx(const Dom& dd, const Sc& ss, int cc, real m1)
+        : d(dd), s(ss), c(cc), st(dd.ny * dd.nx), nz(dd.nz)
+    {
+        mut = d.mub2d[c] + d.mup[c];
+        M1 = m1;
+        K = R(0.5) * s.dtau * G_F * (R(1.0) + s.epssm);
+        cof = K * K;
+    }
+    __device__ real c2a(int k) const
+    {
         return (GAMMA_F * d.p[k * st + c]) / d.alt[k * st + c];
-    };
-    auto chm_at = [&](int k) { return d.c1h[k] * mut + d.c2h[k]; };
-    auto cfm_at = [&](int k) { return d.c1f[k] * mut + d.c2f[k]; };
-    auto a_at = [&](int k) -> real {
+    }
+    __device__ real chm(int k) const { return d.c1h[k] * mut + d.c2h[k]; }
+    __device__ real cfm(int k) const { return d.c1f[k] * mut + d.c2f[k]; }
+    __device__ real chmp(int k) const { return d.c1h[k] * (mut + M1) + d.c2h[k]; }
+    __device__ real cfs(int k) const { return d.c1f[k] * (mut + M1) + d.c2f[k]; }
+    __device__ real a(int k) const
+    {
         if (k >= 2 && k < nz)
-            return -cof * d.rdn[k] * d.rdnw[k - 1] * c2a_at(k - 1)
-                   / (chm_at(k - 1) * cfm_at(k - 1));
+            return -cof * d.rdn[k] * d.rdnw[k - 1] * c2a(k - 1)
+                   / (chm(k - 1) * cfm(k - 1));
         if (k == nz && !s.top_lid)
-            return R(-2.0) * cof * d.rdnw[nz - 1] * d.rdnw[nz - 1]
-                   * c2a_at(nz - 1) / (chm_at(nz - 1) * cfm_at(nz - 1));
-        return zero;
-    };
-    // damper factor 1/(1+dw) of level k (1 where undamped)
-    real hbot = zero;
-    if (s.damp) hbot = phi_full(d, nz, c) / g - s.zdamp;
-    auto undamp = [&](int k) -> real {
-        if (!s.damp) return one;
+            return R(-2.0) * cof * d.rdnw[nz - 1] * d.rdnw[nz - 1] * c2a(nz - 1)
+                   / (chm(nz - 1) * cfm(nz - 1));
+        return R(0.0);
+    }
+    // 1/(1+dw) of the damp_opt = 3 damper at w level k (1 where undamped)
+    __device__ real undamp(int k) const
+    {
+        if (!s.damp) return R(1.0);
+        const real g = G_F;
+        const real hbot = phi_full(d, nz, c) / g - s.zdamp;
         const real hk = phi_full(d, k, c) / g;
-        if (!(hk >= hbot)) return one;
+        if (!(hk >= hbot)) return R(1.0);
         const real arg = R(1.5707963) * (hk - hbot) / s.zdamp;
         const real sn = (real)sin((double)arg);
-        return one / (one + s.dampmag * sn * sn);
-    };
-    real lM1 = io.lmu[c];
-
-    // A (up): EOS reverse, phi'' update reverse, damper, Thomas factors and
-    // the forward sweep of the transposed solve
-    real carry_ph = zero, gph0 = zero, vprev = zero, gprev = zero;
-    auto w_in = [&](int k, real ph1) {          // lambda(w) entering the solve
-        const real cfs = d.c1f[k] * (mut + M1) + d.c2f[k];
-        lM1 += -K * io.W1[k * st + c] * d.c1f[k] / (cfs * cfs) * ph1;
-        real lw = io.lw[k * st + c] + K / cfs * ph1;
-        const real u = undamp(k);
-        return s.damp ? lw * u : lw;
-    };
-#pragma unroll 4
-    for (int k = 0; k < nz; ++k) {
+        return R(1.0) / (R(1.0) + s.dampmag * sn * sn);
+    }
+    // lambda(alpha'') of level k from the EOS reverse
+    __device__ real lAL(const Adj& A, int k) const
+    {
         const int x = k * st + c;
-        const real thr = th_ref(d, k, c), alt = d.alt[x], c2a = c2a_at(k);
-        const real chmp = d.c1h[k] * (mut + M1) + d.c2h[k];
+        return A.lal[x] - c2a(k) * A.lp[x];
+    }
+    // lambda of w row k (rows 1..nz-1, and nz with an open lid)
+    __device__ bool row(int k) const
+    {
+        return (k >= 1 && k < nz) || (k == nz && !s.top_lid);
+    }
+    __device__ real lup(int k, real la) const
+    {
+        if (k < 1 || k >= nz) return R(0.0);
+        return R(0.5) * s.dtau * G_F * d.rdn[k] * c2a(k) * d.rdnw[k] / chm(k) * la;
+    }
+    __device__ real ldn(int k, real la) const
+    {
+        if (k >= 1 && k < nz)
+            return -(R(0.5) * s.dtau * G_F * d.rdn[k] * c2a(k - 1) * d.rdnw[k - 1]
+                     / chm(k - 1)) * la;
+        if (k == nz && !s.top_lid)
+            return -(s.dtau * G_F * c2a(nz - 1) * d.rdnw[nz - 1] * d.rdnw[nz - 1]
+                     / chm(nz - 1)) * la;
+        return R(0.0);
+    }
+};
+
+__device__ __forceinline__ bool col_level(const Dom& d, int& c, int& k)
+{
+    c = blockIdx.x * blockDim.x + threadIdx.x;
+    k = blockIdx.y;
+    return c < d.ny * d.nx;
+}
+
+// P1: EOS reverse, phi'' update reverse, damper; w-solve right-hand side.
+__global__ __launch_bounds__(256) void k_adj_p1(Dom d, Sc s, Adj a, Fw f)
+{
+    int c, k;
+    if (!col_level(d, c, k)) return;
+    const ColCtx C(d, s, c, f.M1[c]);
+    const int st = C.st, nz = d.nz, x = k * st + c;
+    real* LRHS = f.s[W_LRHS];
+    real* LWIN = f.s[W_LPH1];
+    real mcon = R(0.0);
+    real ph1 = a.lph[x];
+    if (k < nz) {
+        const real thr = th_ref(d, k, c), alt = d.alt[x], c2a = C.c2a(k);
+        const real chmp = C.chmp(k);
         const real q = alt / (chmp * thr);
-        const real X = io.T1[x] - d.c1h[k] * M1 * thr;
-        const real lPP = io.lp[x];
-        GG[x] = io.lth[x] + c2a * q * lPP;
-        lM1 += c2a * (-q * d.c1h[k] * thr - q * X * d.c1h[k] / chmp) * lPP;
-        const real lAL = io.lal[x] - c2a * lPP;
-        lM1 += (-(alt * d.c1h[k]) / chmp - io.AL1[x] * d.c1h[k] / chmp) * lAL;
-        const real ph1 = io.lph[x] + carry_ph + d.rdnw[k] / chmp * lAL;
-        carry_ph = -d.rdnw[k] / chmp * lAL;
-        if (k == 0) {
-            gph0 = ph1;
-            LW[c] = io.lw[c];
-            continue;
-        }
+        const real X = f.T1[x] - d.c1h[k] * C.M1 * thr;
+        const real lPP = a.lp[x];
+        f.s[W_GG][x] = a.lth[x] + c2a * q * lPP;
+        mcon += c2a * (-q * d.c1h[k] * thr - q * X * d.c1h[k] / chmp) * lPP;
+        const real lAL = a.lal[x] - c2a * lPP;
+        mcon += (-(alt * d.c1h[k]) / chmp - f.AL1[x] * d.c1h[k] / chmp) * lAL;
+        ph1 += d.rdnw[k] / chmp * lAL;
+    }
+    if (k >= 1) ph1 += -d.rdnw[k - 1] / C.chmp(k - 1) * C.lAL(a, k - 1);
+    if (k == 0) {
+        f.s[W_G0][c] = ph1;                    // phi''(0) passes through
+        LWIN[c] = a.lw[c];
+    } else {
         LRHS[x] = ph1;
-        const real lw = w_in(k, ph1);
-        // Thomas factors of row k (calc_coef_w) and the transposed sweep
-        const real b = one + cof * d.rdn[k] * (
-            d.rdnw[k] * c2a / (chm_at(k) * cfm_at(k))
-            + d.rdnw[k - 1] * c2a_at(k - 1) / (chm_at(k - 1) * cfm_at(k)));
-        const real cc = -cof * d.rdn[k] * d.rdnw[k] * c2a / (chm_at(k) * cfm_at(k + 1));
-        const real ak = a_at(k);
+        const real cfs = C.cfs(k);
+        mcon += -C.K * f.W1[x] * d.c1f[k] / (cfs * cfs) * ph1;
+        const real lw = a.lw[x] + C.K / cfs * ph1;
+        LWIN[x] = s.damp ? lw * C.undamp(k) : lw;
+    }
+    f.s[W_MC1][x] = mcon;
+}
+
+// S1: the transposed Thomas solve of one column.
+__global__ __launch_bounds__(64) void k_adj_s1(Dom d, Sc s, Fw f)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.ny * d.nx) return;
+    const ColCtx C(d, s, c, f.M1[c]);
+    const int st = C.st, nz = d.nz;
+    const real one = R(1.0);
+    const real* __restrict__ LWIN = f.s[W_LPH1];
+    real* __restrict__ LW = f.s[W_LW];
+    real* __restrict__ SA = f.s[W_SA];
+    real* __restrict__ SG = f.s[W_SG];
+    real gprev = R(0.0), vprev = R(0.0);
+    real c2a_k = C.c2a(0), c2a_m;
+    for (int k = 1; k < nz; ++k) {
+        c2a_m = c2a_k;
+        c2a_k = C.c2a(k);
+        const real b = one + C.cof * d.rdn[k] * (
+            d.rdnw[k] * c2a_k / (C.chm(k) * C.cfm(k))
+            + d.rdnw[k - 1] * c2a_m / (C.chm(k - 1) * C.cfm(k)));
+        const real cc = -C.cof * d.rdn[k] * d.rdnw[k] * c2a_k
+                        / (C.chm(k) * C.cfm(k + 1));
+        const real ak = C.a(k);
         const real al = one / (b - ak * gprev);
-        const real v = lw - gprev * vprev;
+        const real v = LWIN[k * st + c] - gprev * vprev;
         gprev = cc * al;
         vprev = v;
-        LW[x] = v;
-        SA[x] = al;
-        SG[x] = ak;
+        LW[k * st + c] = v;
+        SA[k * st + c] = al;
+        SG[k * st + c] = ak;
     }
     {
-        const int x = nz * st + c;
-        const real ph1 = io.lph[x] + carry_ph;
-        LRHS[x] = ph1;
-        const real lw = w_in(nz, ph1);
-        const real bn = one + R(2.0) * cof * d.rdnw[nz - 1] * d.rdnw[nz - 1]
-                        * c2a_at(nz - 1) / (chm_at(nz - 1) * cfm_at(nz));
-        const real an = a_at(nz);
-        LW[x] = lw - gprev * vprev;
-        SA[x] = one / (bn - an * gprev);
-        SG[x] = an;
+        const real bn = one + R(2.0) * C.cof * d.rdnw[nz - 1] * d.rdnw[nz - 1]
+                        * c2a_k / (C.chm(nz - 1) * C.cfm(nz));
+        const real an = C.a(nz);
+        LW[nz * st + c] = LWIN[nz * st + c] - gprev * vprev;
+        SA[nz * st + c] = one / (bn - an * gprev);
+        SG[nz * st + c] = an;
     }
-    // B (down): backward sweep of the transposed solve
-    real carry = zero;
-#pragma unroll 4
+    real carry = R(0.0);
     for (int k = nz; k >= 1; --k) {
         const int x = k * st + c;
         const real v = SA[x] * (LW[x] + carry);
         LW[x] = v;
         carry = -SG[x] * v;
     }
-    L0[c] = LW[c] + carry;
+    f.s[W_L0][c] = LWIN[c] + carry;
+}
 
-    // C (up): reverse of the w accumulation; la(k) = lambda of row k
-    auto la_at = [&](int k) -> real {
-        if (k >= 1 && k < nz) return LW[k * st + c];
-        if (k == nz && !s.top_lid) return LW[nz * st + c];
-        return zero;
-    };
-    auto lup_at = [&](int k, real la) -> real {
-        if (k < 1 || k >= nz) return zero;
-        return half * dt * g * d.rdn[k] * c2a_at(k) * d.rdnw[k] / chm_at(k) * la;
-    };
-    auto ldn_at = [&](int k, real la) -> real {
-        if (k >= 1 && k < nz)
-            return -(half * dt * g * d.rdn[k] * c2a_at(k - 1) * d.rdnw[k - 1]
-                     / chm_at(k - 1)) * la;
-        if (k == nz && !s.top_lid)
-            return -(dt * g * c2a_at(nz - 1) * d.rdnw[nz - 1] * d.rdnw[nz - 1]
-                     / chm_at(nz - 1)) * la;
-        return zero;
-    };
-    real lmuave = zero;
-    real la_k = zero, lup_km = zero;            // level 0: no row
-    real la_kp = la_at(1);
-    real ldn_k = zero;
-#pragma unroll 4
-    for (int k = 0; k <= nz; ++k) {
-        const int x = k * st + c;
-        const real la_next = (k + 2 <= nz) ? la_at(k + 2) : zero;
-        const real lup_k = lup_at(k, la_k);
-        const real ldn_kp = ldn_at(k + 1, la_kp);
-        const real T = (lup_km - lup_k) + (ldn_k - ldn_kp);
-        GPH[x] = (one - eps) * T;
-        if (k >= 1) LRHS[x] = LRHS[x] + (one + eps) * T;
-        const bool row = (k >= 1 && k < nz) || (k == nz && !s.top_lid);
-        gw[x] = row ? la_k : zero;
-        grw[x] = row ? dt * la_k : zero;
-        if (k < nz) {
-            real t2 = zero;
-            if (k >= 1) t2 += dt * g * d.rdn[k] * c2a_at(k) * d.alt[x] * la_k;
-            if (k + 1 < nz)
-                t2 -= dt * g * d.rdn[k + 1] * c2a_at(k) * d.alt[x] * la_kp;
-            if (k == nz - 1 && !s.top_lid)
-                t2 -= dt * g * R(2.0) * d.rdnw[nz - 1] * c2a_at(nz - 1)
-                      * d.alt[x] * la_kp;
-            LT2[x] = t2;
-        }
-        if (row) lmuave += -dt * g * d.c1f[k] * la_k;
-        lup_km = lup_k;
-        ldn_k = ldn_kp;
-        la_k = la_kp;
-        la_kp = la_next;
+// P2: reverse of the w accumulation.
+__global__ __launch_bounds__(256) void k_adj_p2(Dom d, Sc s, Adj a, Fw f)
+{
+    int c, k;
+    if (!col_level(d, c, k)) return;
+    const ColCtx C(d, s, c, f.M1[c]);
+    const int st = C.st, nz = d.nz, x = k * st + c;
+    const real one = R(1.0), dt = s.dtau, g = G_F, eps = s.epssm;
+    const real* LW = f.s[W_LW];
+    auto la = [&](int kk) { return C.row(kk) ? LW[kk * st + c] : R(0.0); };
+    const real la_k = la(k), la_kp = la(k + 1);
+    const real T = (C.lup(k - 1, la(k - 1)) - C.lup(k, la_k))
+                   + (C.ldn(k, la_k) - C.ldn(k + 1, la_kp));
+    f.s[W_LWD][x] = (one - eps) * T;                      // GPH
+    if (k >= 1) f.s[W_LRHS][x] = f.s[W_LRHS][x] + (one + eps) * T;
+    const bool row = C.row(k);
+    a.gw[x] = row ? la_k : R(0.0);
+    a.grw[x] = row ? dt * la_k : R(0.0);
+    f.s[W_MC2][x] = row ? -dt * g * d.c1f[k] * la_k : R(0.0);
+    if (k < nz) {
+        real t2 = R(0.0);
+        if (k >= 1) t2 += dt * g * d.rdn[k] * C.c2a(k) * d.alt[x] * la_k;
+        if (k + 1 < nz)
+            t2 -= dt * g * d.rdn[k + 1] * C.c2a(k) * d.alt[x] * la_kp;
+        if (k == nz - 1 && !s.top_lid)
+            t2 -= dt * g * R(2.0) * d.rdnw[nz - 1] * C.c2a(nz - 1) * d.alt[x] * la_kp;
+        f.s[W_LT2][x] = t2;
     }
+}
 
-    // D (up): rhs reverse -> phi'', rph_t, w'', and Omega'' via wd
-    if (s.top_lid) LRHS[nz * st + c] = zero;
-    real lwd_prev = zero;
-    auto lr2_at = [&](int k) -> real {
-        return (k >= 1 && k <= nz) ? LRHS[k * st + c] / cfm_at(k) : zero;
+// P3: rhs reverse (phi'', rph_t, w''), theta'' adjoint, Omega'' adjoint.
+__global__ __launch_bounds__(256) void k_adj_p3(Dom d, Sc s, Adj a, Fw f)
+{
+    int c, k;
+    if (!col_level(d, c, k)) return;
+    const ColCtx C(d, s, c, f.M1[c]);
+    const int st = C.st, nz = d.nz, x = k * st + c;
+    const real half = R(0.5), one = R(1.0), dt = s.dtau, g = G_F, eps = s.epssm;
+    const real* LRHS = f.s[W_LRHS];
+    const real* LT2 = f.s[W_LT2];
+    auto lr2 = [&](int kk) -> real {
+        if (kk < 1 || kk > nz || (kk == nz && s.top_lid)) return R(0.0);
+        return LRHS[kk * st + c] / C.cfm(kk);
     };
-    real lr2_k = zero;                          // level 0
-    real lr2_kp = lr2_at(1);
-#pragma unroll 4
-    for (int k = 0; k <= nz; ++k) {
-        const int x = k * st + c;
-        const real lr2_next = (k + 2 <= nz) ? lr2_at(k + 2) : zero;
-        if (k == 0) {
-            gph[x] = GPH[x] + gph0;
-            grph[x] = zero;
-        } else {
-            gph[x] = GPH[x] + LRHS[x];
-            grph[x] = dt * lr2_k;
-            gw[x] = gw[x] + dt * half * g * (one - eps) * lr2_k;
-        }
-        // lambda(wd[k]) (k < nz): -dt (fnm[k] lr2[k] + fnp[k+1] lr2[k+1])
-        real lwd = zero;
-        if (k < nz) {
-            if (k >= 1) lwd += -dt * d.fnm[k] * lr2_k;
-            if (k + 1 <= nz - 1) lwd += -dt * d.fnp[k + 1] * lr2_kp;
-        }
-        // Omega''(k) gets cwd[k] lwd[k] + cwd[k-1] lwd[k-1]
-        const real cwd = (k < nz) ? half * d.rdnw[k] * (phi_full(d, k + 1, c)
-                                                        - phi_full(d, k, c))
-                                  : zero;
-        real ww = io.lww ? io.lww[x] : zero;
-        ww += cwd * lwd + lwd_prev;
-        LWW[x] = ww;
-        lwd_prev = (k < nz) ? cwd * lwd : zero;
-        lr2_k = lr2_kp;
-        lr2_kp = lr2_next;
-    }
-
-    // E (up): theta'' adjoint (t2 term, then the theta'' update), the
-    // vertical theta flux into Omega''
-    real lM0old = half * (one - eps) * lmuave;
-    lM1 += half * (one + eps) * lmuave;
-    real gg_prev = zero;
-#pragma unroll 4
-    for (int k = 0; k < nz; ++k) {
-        const int x = k * st + c;
+    auto lrhs = [&](int kk) -> real {
+        if (kk < 1 || kk > nz || (kk == nz && s.top_lid)) return R(0.0);
+        return LRHS[kk * st + c];
+    };
+    auto lwd = [&](int kk) -> real {            // lambda(wd[kk]), kk < nz
+        real v = R(0.0);
+        if (kk >= 1) v += -dt * d.fnm[kk] * lr2(kk);
+        if (kk + 1 <= nz - 1) v += -dt * d.fnp[kk + 1] * lr2(kk + 1);
+        return v;
+    };
+    auto cwd = [&](int kk) {
+        return half * d.rdnw[kk] * (phi_full(d, kk + 1, c) - phi_full(d, kk, c));
+    };
+    auto lT1 = [&](int kk) {
+        const real chs = d.c1h[kk] * (C.mut + C.M1) + d.c2h[kk];
+        return f.s[W_GG][kk * st + c]
+               + half * (one + eps) / (chs * th_ref(d, kk, c)) * LT2[kk * st + c];
+    };
+    a.gph[x] = f.s[W_LWD][x] + (k == 0 ? f.s[W_G0][c] : lrhs(k));
+    a.grph[x] = k >= 1 ? dt * lr2(k) : R(0.0);
+    if (k >= 1) a.gw[x] = a.gw[x] + dt * half * g * (one - eps) * lr2(k);
+    real ww = a.lww ? a.lww[x] : R(0.0);
+    if (k < nz) ww += cwd(k) * lwd(k);
+    if (k >= 1) ww += cwd(k - 1) * lwd(k - 1);
+    if (k < nz) {
         const real thr = th_ref(d, k, c);
-        const real chs = d.c1h[k] * (mut + M1) + d.c2h[k];
-        const real t2 = half * ((one + eps) * io.T1[x] + (one - eps) * d.th_pp[x])
+        const real chs = d.c1h[k] * (C.mut + C.M1) + d.c2h[k];
+        const real t2 = half * ((one + eps) * f.T1[x] + (one - eps) * d.th_pp[x])
                         / (chs * thr);
         const real lt2 = LT2[x];
-        const real lT1 = GG[x] + half * (one + eps) / (chs * thr) * lt2;
-        gth[x] = half * (one - eps) / (chs * thr) * lt2 + lT1;
-        grth[x] = dt * lT1;
-        lM1 += -t2 * d.c1h[k] / chs * lt2;
-        const real gg = -dt * lT1;
-        GG[x] = gg;
+        const real l1 = lT1(k);
+        a.gth[x] = half * (one - eps) / (chs * thr) * lt2 + l1;
+        a.grth[x] = dt * l1;
+        f.s[W_GGF][x] = -dt * l1;
+        f.s[W_MC3][x] = -t2 * d.c1h[k] / chs * lt2;
         if (k >= 1) {
-            const real lwdtn = d.rdnw[k - 1] * gg_prev - d.rdnw[k] * gg;
-            LWW[x] = LWW[x] + (d.fnm[k] * thr + d.fnp[k] * th_ref(d, k - 1, c)) * lwdtn;
+            const real lwdtn = d.rdnw[k - 1] * (-dt * lT1(k - 1))
+                               - d.rdnw[k] * (-dt * l1);
+            ww += (d.fnm[k] * thr + d.fnp[k] * th_ref(d, k - 1, c)) * lwdtn;
         }
-        gg_prev = gg;
     }
+    f.s[W_LWW][x] = ww;
+}
 
-    // F (down): Omega'' suffix sums -> mu'' and the divergence adjoint
-    real acc = zero, ldmdt = zero, gr = zero;
-#pragma unroll 4
+// S2: column sums of the mu'' pieces, Omega'' suffix sums, and the
+// divergence adjoint.
+__global__ __launch_bounds__(64) void k_adj_s2(Dom d, Sc s, Adj a, Fw f)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.ny * d.nx) return;
+    const int st = d.ny * d.nx, nz = d.nz;
+    const real half = R(0.5), one = R(1.0), dt = s.dtau, eps = s.epssm;
+    const real* __restrict__ MC1 = f.s[W_MC1];
+    const real* __restrict__ MC2 = f.s[W_MC2];
+    const real* __restrict__ MC3 = f.s[W_MC3];
+    const real* __restrict__ LWW = f.s[W_LWW];
+    real* __restrict__ LDVP = f.s[W_LDVP];
+    real lM1 = a.lmu[c], lmuave = R(0.0);
+    for (int k = 0; k <= nz; ++k) {
+        lM1 += MC1[k * st + c];
+        lmuave += MC2[k * st + c];
+        if (k < nz) lM1 += MC3[k * st + c];
+    }
+    const real lM0old = half * (one - eps) * lmuave;
+    lM1 += half * (one + eps) * lmuave;
+    real acc = R(0.0), ldmdt = R(0.0), gr = R(0.0);
     for (int k = nz - 1; k >= 0; --k) {
-        real linc = zero;
+        real linc = R(0.0);
         if (k < nz - 1) {
             acc += LWW[(k + 1) * st + c];
             linc = -acc;
@@ -316,29 +353,12 @@ __device__ __forceinline__ void adj_col_body(
     }
     ldmdt += dt * lM1;
     gr += dt * lM1;
-    grmu[c] = gr;
-    gmu[c] = lM1 + lM0old;
-#pragma unroll 4
+    a.grmu[c] = gr;
+    a.gmu[c] = lM1 + lM0old;
     for (int k = 0; k < nz; ++k) {
         const int x = k * st + c;
         LDVP[x] = LDVP[x] + d.dnw[k] * ldmdt;
     }
-}
-
-// Mass-column reverse of advance_w_phi + diagnose_p and of the vertical
-// parts of advance_mu_th.
-__global__ __launch_bounds__(64) void k_adj_col(Dom d, Sc s, Adj a, Fw f)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= d.ny * d.nx) return;
-    ColIO io;
-    io.lth = a.lth; io.lp = a.lp; io.lal = a.lal; io.lph = a.lph;
-    io.lw = a.lw; io.lmu = a.lmu; io.lww = a.lww;
-    io.T1 = f.T1; io.AL1 = f.AL1; io.W1 = f.W1; io.M1 = f.M1;
-    adj_col_body(d, s, c, io, f.s[W_GG], f.s[W_LRHS], f.s[W_LW], f.s[W_SA],
-                 f.s[W_SG], f.s[W_LT2], f.s[W_LWD], f.s[W_LWW], f.s[W_LDVP],
-                 f.s[W_L0], a.gw, a.grw, a.gph, a.grph, a.gth, a.grth, a.gmu,
-                 a.grmu);
 }
 
 // Face pass: u faces (j, fc), fc = 0..nx, then v faces (g, i), g = 0..ny.
@@ -355,90 +375,18 @@ struct FaceGeo {
     int cA, cB, slot;        // advance_uv cells of an own face, table slot
 };
 
-__device__ __forceinline__ void adj_face_body(
-    const Dom& d, const Sc& s, const FaceGeo& fg, const real rd,
-    const real* __restrict__ lout, const real* __restrict__ GG,
-    const real* __restrict__ LDVP, const real* __restrict__ L0,
-    real* __restrict__ gout, real* __restrict__ grt, real* __restrict__ LB,
-    real* __restrict__ LT, real* __restrict__ LS, real* __restrict__ LM,
-    const FaceGeo* dup)
+// Geometry of the face of thread t (u faces first, then v faces); dg is
+// the duplicate face (nx, or ny) whose adjoint face 0 folds in.
+__device__ __forceinline__ bool face_geo(const Dom& d, int t, FaceGeo& fg,
+                                         FaceGeo& dg, bool& has_dup,
+                                         bool& isu)
 {
-    const int nz = d.nz, st = d.ny * d.nx;
-    const real half = R(0.5), dt = s.dtau, zero = R(0.0);
-    const real cf[3] = {d.cf1, d.cf2, d.cf3};
-    auto lam1 = [&](const FaceGeo& q, int k) -> real {
-        real v = lout[q.out0 + k * q.ostride];
-        if (q.hasE) {
-            const int c = q.cE, cp = q.cEp;
-            v += half * rd * (th_ref(d, k, cp) + th_ref(d, k, c)) * GG[k * st + c];
-            v += rd * LDVP[k * st + c];
-            if (k < 3) v += cf[k] * half * rd * (d.ht[cp] - d.ht[c]) * L0[c];
-        }
-        if (q.hasW) {
-            const int c = q.cW, cm = q.cWm;
-            v -= half * rd * (th_ref(d, k, c) + th_ref(d, k, cm)) * GG[k * st + c];
-            v -= rd * LDVP[k * st + c];
-            if (k < 3) v += cf[k] * half * rd * (d.ht[c] - d.ht[cm]) * L0[c];
-        }
-        return v;
-    };
-    const real mA = d.mub2d[fg.cA] + d.mup[fg.cA];
-    const real mB = d.mub2d[fg.cB] + d.mup[fg.cB];
-    const real muf = half * (mB + mA);
-    real lms = zero;
-#pragma unroll 4
-    for (int k = 0; k < nz; ++k) {
-        const real l1 = lam1(fg, k);
-        gout[fg.out0 + k * fg.ostride] = l1;
-        grt[fg.out0 + k * fg.ostride] = dt * l1;
-        if (!fg.own) continue;
-        const real ldup = dup ? lam1(*dup, k) : zero;
-        const real ld = -dt * l1 - dt * ldup;
-        const int x = k * st + fg.slot;
-        LB[x] = half * rd * (d.c1h[k] * muf + d.c2h[k]) * ld;
-        real dphp = half * ((d.php[k * st + fg.cA] + d.php[(k + 1) * st + fg.cA])
-                            - (d.php[k * st + fg.cB] + d.php[(k + 1) * st + fg.cB]));
-        if (d.base3d)
-            dphp = dphp + half * ((d.phb[k * st + fg.cA] + d.phb[(k + 1) * st + fg.cA])
-                                  - (d.phb[k * st + fg.cB] + d.phb[(k + 1) * st + fg.cB]));
-        const real tk = rd * dphp * ld;
-        LT[x] = tk;
-        lms += -half * d.c1h[k] * tk;
-    }
-    if (!fg.own) return;
-    LM[fg.slot] = lms;
-    // lambda(dpn) at full level kf: rdnw[kf-1] t[kf-1] - rdnw[kf] t[kf]
-    auto ldpn = [&](int kf) -> real {
-        real v = zero;
-        if (kf >= 1) v += d.rdnw[kf - 1] * LT[(kf - 1) * st + fg.slot];
-        if (kf <= nz - 1) v -= d.rdnw[kf] * LT[kf * st + fg.slot];
-        return v;
-    };
-    const real l0 = ldpn(0), ltop = s.top_lid ? ldpn(nz) : zero;
-    real dk = zero, dkp = ldpn(1);
-#pragma unroll 4
-    for (int k = 0; k < nz; ++k) {
-        const real dnext = (k + 2 <= nz - 1) ? ldpn(k + 2) : zero;
-        real v = zero;
-        if (k >= 1) v += half * d.fnm[k] * dk;
-        if (k + 1 <= nz - 1) v += half * d.fnp[k + 1] * dkp;
-        if (k < 3) v += half * cf[k] * l0;
-        if (s.top_lid && k >= nz - 3) v += half * cf[nz - 1 - k] * ltop;
-        LS[k * st + fg.slot] = v;
-        dk = dkp;
-        dkp = dnext;
-    }
-}
-
-__global__ __launch_bounds__(128) void k_adj_faces(Dom d, Sc s, Adj a, Fw f)
-{
-    const int ny = d.ny, nx = d.nx, st = ny * nx;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ny = d.ny, nx = d.nx;
     const int nu = ny * (nx + 1), nv = (ny + 1) * nx;
-    if (t >= nu + nv) return;
-    FaceGeo fg, dg;
-    bool has_dup = false;
-    if (t < nu) {
+    if (t >= nu + nv) return false;
+    has_dup = false;
+    isu = t < nu;
+    if (isu) {
         const int j = t / (nx + 1), fc = t - j * (nx + 1);
         auto geo = [&](int f1, FaceGeo& q) {
             q.out0 = iU(0, j, f1, ny, nx);
@@ -456,9 +404,6 @@ __global__ __launch_bounds__(128) void k_adj_faces(Dom d, Sc s, Adj a, Fw f)
         };
         geo(fc, fg);
         if (fc == 0) { geo(nx, dg); has_dup = true; }
-        adj_face_body(d, s, fg, s.rdx, a.lu, f.s[W_GG], f.s[W_LDVP],
-                      f.s[W_L0], a.gu, a.gru, f.s[W_LBX], f.s[W_LPH1],
-                      f.s[W_LSX], f.s[W_LMX], has_dup ? &dg : nullptr);
     } else {
         const int r = t - nu, g = r / nx, i = r - g * nx;
         auto geo = [&](int g1, FaceGeo& q) {
@@ -477,11 +422,85 @@ __global__ __launch_bounds__(128) void k_adj_faces(Dom d, Sc s, Adj a, Fw f)
         };
         geo(g, fg);
         if (g == 0) { geo(ny, dg); has_dup = true; }
-        adj_face_body(d, s, fg, s.rdy, a.lv, f.s[W_GG], f.s[W_LDVP],
-                      f.s[W_L0], a.gv, a.grv, f.s[W_LBY], f.s[W_LW],
-                      f.s[W_LSY], f.s[W_LMY], has_dup ? &dg : nullptr);
     }
-    (void)st;
+    return true;
+}
+
+// F1 (face, level): lambda of u''_new / v''_new -> the u''/v'' and ru_t/rv_t
+// adjoints, and for own faces the bracket (LB) and dpn-level (LT) pieces.
+__global__ __launch_bounds__(256) void k_adj_f1(Dom d, Sc s, Adj a, Fw f)
+{
+    FaceGeo fg, dg;
+    bool has_dup, isu;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (!face_geo(d, t, fg, dg, has_dup, isu)) return;
+    const int st = d.ny * d.nx;
+    const real half = R(0.5), dt = s.dtau;
+    const real rd = isu ? s.rdx : s.rdy;
+    const real* lout = isu ? a.lu : a.lv;
+    const real* GG = f.s[W_GGF];
+    const real* LDVP = f.s[W_LDVP];
+    const real* L0 = f.s[W_L0];
+    const real cf[3] = {d.cf1, d.cf2, d.cf3};
+    auto lam1 = [&](const FaceGeo& q) -> real {
+        real v = lout[q.out0 + k * q.ostride];
+        if (q.hasE) {
+            const int c = q.cE, cp = q.cEp;
+            v += half * rd * (th_ref(d, k, cp) + th_ref(d, k, c)) * GG[k * st + c];
+            v += rd * LDVP[k * st + c];
+            if (k < 3) v += cf[k] * half * rd * (d.ht[cp] - d.ht[c]) * L0[c];
+        }
+        if (q.hasW) {
+            const int c = q.cW, cm = q.cWm;
+            v -= half * rd * (th_ref(d, k, c) + th_ref(d, k, cm)) * GG[k * st + c];
+            v -= rd * LDVP[k * st + c];
+            if (k < 3) v += cf[k] * half * rd * (d.ht[c] - d.ht[cm]) * L0[c];
+        }
+        return v;
+    };
+    const real l1 = lam1(fg);
+    (isu ? a.gu : a.gv)[fg.out0 + k * fg.ostride] = l1;
+    (isu ? a.gru : a.grv)[fg.out0 + k * fg.ostride] = dt * l1;
+    if (!fg.own) return;
+    const real ldup = has_dup ? lam1(dg) : R(0.0);
+    const real ld = -dt * l1 - dt * ldup;
+    const real mA = d.mub2d[fg.cA] + d.mup[fg.cA];
+    const real mB = d.mub2d[fg.cB] + d.mup[fg.cB];
+    const real muf = half * (mB + mA);
+    const int x = k * st + fg.slot;
+    f.s[isu ? W_LBX : W_LBY][x] = half * rd * (d.c1h[k] * muf + d.c2h[k]) * ld;
+    real dphp = half * ((d.php[k * st + fg.cA] + d.php[(k + 1) * st + fg.cA])
+                        - (d.php[k * st + fg.cB] + d.php[(k + 1) * st + fg.cB]));
+    if (d.base3d)
+        dphp = dphp + half * ((d.phb[k * st + fg.cA] + d.phb[(k + 1) * st + fg.cA])
+                              - (d.phb[k * st + fg.cB] + d.phb[(k + 1) * st + fg.cB]));
+    f.s[isu ? W_LPH1 : W_LW][x] = rd * dphp * ld;
+}
+
+// F2 (own face, level): the damped-p'' level adjoints folded onto the p''
+// levels (LS), from LT at the neighbouring levels.
+__global__ __launch_bounds__(256) void k_adj_f2(Dom d, Sc s, Fw f)
+{
+    FaceGeo fg, dg;
+    bool has_dup, isu;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (!face_geo(d, t, fg, dg, has_dup, isu) || !fg.own) return;
+    const int st = d.ny * d.nx, nz = d.nz;
+    const real half = R(0.5);
+    const real cf[3] = {d.cf1, d.cf2, d.cf3};
+    const real* LT = f.s[isu ? W_LPH1 : W_LW];
+    auto ldpn = [&](int kf) -> real {
+        real v = R(0.0);
+        if (kf >= 1) v += d.rdnw[kf - 1] * LT[(kf - 1) * st + fg.slot];
+        if (kf <= nz - 1) v -= d.rdnw[kf] * LT[kf * st + fg.slot];
+        return v;
+    };
+    real v = R(0.0);
+    if (k >= 1) v += half * d.fnm[k] * ldpn(k);
+    if (k + 1 <= nz - 1) v += half * d.fnp[k + 1] * ldpn(k + 1);
+    if (k < 3) v += half * cf[k] * ldpn(0);
+    if (s.top_lid && k >= nz - 3) v += half * cf[nz - 1 - k] * ldpn(nz);
+    f.s[isu ? W_LSX : W_LSY][k * st + fg.slot] = v;
 }
 
 // Cell pass: gather the face pieces of the advance_uv transpose.
@@ -500,9 +519,14 @@ __global__ void k_adj_cells(Dom d, Sc s, Adj a, Fw f)
     const real* LBY = f.s[W_LBY];
     const real* LSY = f.s[W_LSY];
     const real one = R(1.0), sm = s.smdiv;
-    a.gmu[c] += f.s[W_LMX][c] + f.s[W_LMX][ip] + f.s[W_LMY][c] + f.s[W_LMY][jp];
+    const real* LTX = f.s[W_LPH1];
+    const real* LTY = f.s[W_LW];
+    real lms = R(0.0);
     for (int k = 0; k < nz; ++k) {
         const int x = k * st;
+        // the face mass sums: -1/2 c1h t of the faces the cell touches
+        lms += -R(0.5) * d.c1h[k] * ((LTX[x + c] + LTX[x + ip])
+                                     + (LTY[x + c] + LTY[x + jp]));
         auto alt = [&](int cc) { return d.alt[x + cc]; };
         // face c is the cell's own (A side), face ip/jp has it as B side
         real lpd = (alt(c) + alt(im)) * LBX[x + c] - (alt(ip) + alt(c)) * LBX[x + ip]
@@ -521,6 +545,7 @@ __global__ void k_adj_cells(Dom d, Sc s, Adj a, Fw f)
         a.gp[x + c] = (one + sm) * lpd + a.lpo[x + c];
         a.gpo[x + c] = -sm * lpd;
     }
+    a.gmu[c] += lms;
 }
 
 }  // namespace
@@ -593,13 +618,23 @@ int32_t ARW_API(acoustic_substep_adjoint)(const Dom* d, const arw_params* prm,
     a.grth = adj->g_rth_t; a.grph = adj->g_rph_t; a.grmu = adj->g_rmu_t;
     const int cols = ny * nx;
     // the damper and dt follow the forward launcher's Sc
-    hipLaunchKernelGGL(k_adj_col, dim3((cols + 63) / 64), dim3(64), 0, st,
-                       *d, s, a, fw);
-    if ((rc = lib_launched(hipGetLastError(), "k_adj_col"))) return rc;
+    const dim3 lev((cols + 255) / 256, nz + 1), colg((cols + 63) / 64);
+    hipLaunchKernelGGL(k_adj_p1, lev, dim3(256), 0, st, *d, s, a, fw);
+    if ((rc = lib_launched(hipGetLastError(), "k_adj_p1"))) return rc;
+    hipLaunchKernelGGL(k_adj_s1, colg, dim3(64), 0, st, *d, s, fw);
+    if ((rc = lib_launched(hipGetLastError(), "k_adj_s1"))) return rc;
+    hipLaunchKernelGGL(k_adj_p2, lev, dim3(256), 0, st, *d, s, a, fw);
+    if ((rc = lib_launched(hipGetLastError(), "k_adj_p2"))) return rc;
+    hipLaunchKernelGGL(k_adj_p3, lev, dim3(256), 0, st, *d, s, a, fw);
+    if ((rc = lib_launched(hipGetLastError(), "k_adj_p3"))) return rc;
+    hipLaunchKernelGGL(k_adj_s2, colg, dim3(64), 0, st, *d, s, a, fw);
+    if ((rc = lib_launched(hipGetLastError(), "k_adj_s2"))) return rc;
     const int nf = ny * (nx + 1) + (ny + 1) * nx;
-    hipLaunchKernelGGL(k_adj_faces, dim3((nf + 127) / 128), dim3(128), 0, st,
-                       *d, s, a, fw);
-    if ((rc = lib_launched(hipGetLastError(), "k_adj_faces"))) return rc;
+    const dim3 flev((nf + 255) / 256, nz);
+    hipLaunchKernelGGL(k_adj_f1, flev, dim3(256), 0, st, *d, s, a, fw);
+    if ((rc = lib_launched(hipGetLastError(), "k_adj_f1"))) return rc;
+    hipLaunchKernelGGL(k_adj_f2, flev, dim3(256), 0, st, *d, s, fw);
+    if ((rc = lib_launched(hipGetLastError(), "k_adj_f2"))) return rc;
     hipLaunchKernelGGL(k_adj_cells, dim3((cols + 127) / 128), dim3(128), 0, st,
                        *d, s, a, fw);
     return lib_launched(hipGetLastError(), "k_adj_cells");
