@@ -107,6 +107,25 @@ __device__ __forceinline__ int iV(int k, int j, int i, int ny, int nx)
     return (k * (ny + 1) + j) * nx + i;
 }
 
+// x / c for the flux denominators c = 60, 12, correctly rounded.  In the
+// FP32 build the quotient is the double product x * (1/c) rounded once to
+// float (three instructions instead of a full division): the exact
+// quotient's binary tail beyond float precision is a repeating block
+// (1/15 = 0.(0001), 1/3 = 0.(01) in binary), so unless the quotient is
+// exact it never lies within the double product's error of a float
+// rounding midpoint, and the result is the correctly rounded float
+// quotient.  Checked against true division for every float input.
+__device__ __forceinline__ real div_const(real x, double inv, real c)
+{
+#if ARW_F64
+    (void)inv;
+    return x / c;
+#else
+    (void)c;
+    return (real)((double)x * inv);
+#endif
+}
+
 // WRF flux5 (upwind-biased 5th order); face between qm1 and q0.
 __device__ __forceinline__ real flux5(real qm3, real qm2, real qm1,
                                        real q0, real qp1, real qp2,
@@ -114,7 +133,7 @@ __device__ __forceinline__ real flux5(real qm3, real qm2, real qm1,
 {
     real a = R(37.0) * (q0 + qm1) - R(8.0) * (qp1 + qm2) + (qp2 + qm3);
     real b = R(10.0) * (q0 - qm1) - R(5.0) * (qp1 - qm2) + (qp2 - qm3);
-    return (vel * a - fabs(vel) * b) / R(60.0);
+    return div_const(vel * a - fabs(vel) * b, 1.0 / 60.0, R(60.0));
 }
 
 // Omega-signed vertical flux3 (eta decreases upward).
@@ -123,7 +142,7 @@ __device__ __forceinline__ real flux3(real qm2, real qm1, real q0,
 {
     real a = R(7.0) * (q0 + qm1) - (qp1 + qm2);
     real b = R(3.0) * (q0 - qm1) - (qp1 - qm2);
-    return (vel * a + fabs(vel) * b) / R(12.0);
+    return div_const(vel * a + fabs(vel) * b, 1.0 / 12.0, R(12.0));
 }
 
 // x**GAMMA evaluated in double and rounded once.
