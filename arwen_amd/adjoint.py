@@ -90,8 +90,7 @@ def step_with_checkpoints(state: DomainState, cfg: RunConfig,
 
 def _tracer_call(state, cfg, ckpt, kind, src, dst, clamp=False):
     n = src.numel()
-    work = torch.empty((3 if kind == "tracer_adjoint" else 2) * n,
-                       dtype=state.dtype, device=state.device)
+    work = state.scratch((3 * n,), "arw_tracer_work")   # J uses 2n, J^T 3n
     name, fn = _fn(state, kind)
     d, p = state.domain_struct(), params(cfg)
     if kind == "tracer_step":
@@ -179,7 +178,8 @@ def acoustic_substep_adjoint(state: DomainState, cfg: RunConfig,
     n = ctypes.c_int64()
     _lib.check(getattr(lib, pre + "acoustic_adjoint_work")(
         nz, ny, nx, ctypes.byref(n)), pre + "acoustic_adjoint_work")
-    work = torch.empty(n.value, dtype=state.dtype, device=state.device)
+    # the work area (about 30 fields) persists in the state's scratch pool
+    work = state.scratch((n.value,), "arw_acoustic_adjoint_work")
     keep = []
     a = _lib.ArwAcousticAdj()
     for name in ACOUSTIC_STATE + ("ww_pp",):
