@@ -138,7 +138,7 @@ def load():
         "arw_ensemble_create": ([dp, _I, pp, _P, _I,
                                  ctypes.POINTER(ctypes.c_void_p)], _I),
         "arw_ensemble_step": ([_P, _I, _P], _I),
-        "arw_ensemble_capture": ([_P, _I], _I),
+        "arw_ensemble_capture": ([_P, _I, _I], _I),
         "arw_ensemble_launch": ([_P, _P], _I),
         "arw_ensemble_destroy": ([_P], _I),
     }

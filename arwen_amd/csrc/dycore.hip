@@ -1549,10 +1549,13 @@ int32_t ARW_API(ensemble_step)(ARW_API(ensemble)* e, int32_t nsteps,
     return ARW_OK;
 }
 
-int32_t ARW_API(ensemble_capture)(ARW_API(ensemble)* e, int32_t nsteps)
+int32_t ARW_API(ensemble_capture)(ARW_API(ensemble)* e, int32_t nsteps,
+                                  int32_t join_every)
 {
     if (!e) return fail(ARW_E_ARG, "null ensemble");
     if (nsteps < 1) return fail(ARW_E_ARG, "nsteps must be >= 1");
+    if (join_every < 0) return fail(ARW_E_ARG, "join_every must be >= 0");
+    const int chunk = (join_every == 0 || join_every > nsteps) ? nsteps : join_every;
     if (e->exec) {
         hipGraphExecDestroy(e->exec);
         e->exec = nullptr;
@@ -1574,21 +1577,25 @@ int32_t ARW_API(ensemble_capture)(ARW_API(ensemble)* e, int32_t nsteps)
         else
             capturing = true;
     }
-    // fork: every group's stream joins the capture after cs[0]'s start
-    if (rc == ARW_OK && ng > 1) {
-        if (hipEventRecord(ev[0], cs[0]) != hipSuccess) rc = fail(ARW_E_LAUNCH, "fork record failed");
+    for (int t0 = 0; t0 < nsteps && rc == ARW_OK; t0 += chunk) {
+        const int nt = (nsteps - t0 < chunk) ? nsteps - t0 : chunk;
+        // fork: every group's stream continues after cs[0]'s work so far
+        if (ng > 1) {
+            if (hipEventRecord(ev[0], cs[0]) != hipSuccess)
+                rc = fail(ARW_E_LAUNCH, "fork record failed");
+            for (int g = 1; g < ng && rc == ARW_OK; ++g)
+                if (hipStreamWaitEvent(cs[g], ev[0], 0) != hipSuccess)
+                    rc = fail(ARW_E_LAUNCH, "fork wait failed");
+        }
+        for (int g = 0; g < ng && rc == ARW_OK; ++g)
+            for (int t = 0; t < nt && rc == ARW_OK; ++t)
+                rc = ens_issue(e->groups[g], &e->prm, e->b3, cs[g]);
+        // join
         for (int g = 1; g < ng && rc == ARW_OK; ++g)
-            if (hipStreamWaitEvent(cs[g], ev[0], 0) != hipSuccess)
-                rc = fail(ARW_E_LAUNCH, "fork wait failed");
+            if (hipEventRecord(ev[g], cs[g]) != hipSuccess
+                || hipStreamWaitEvent(cs[0], ev[g], 0) != hipSuccess)
+                rc = fail(ARW_E_LAUNCH, "join failed");
     }
-    for (int g = 0; g < ng && rc == ARW_OK; ++g)
-        for (int t = 0; t < nsteps && rc == ARW_OK; ++t)
-            rc = ens_issue(e->groups[g], &e->prm, e->b3, cs[g]);
-    // join
-    for (int g = 1; g < ng && rc == ARW_OK; ++g)
-        if (hipEventRecord(ev[g], cs[g]) != hipSuccess
-            || hipStreamWaitEvent(cs[0], ev[g], 0) != hipSuccess)
-            rc = fail(ARW_E_LAUNCH, "join failed");
     if (capturing) {
         hipError_t er = hipStreamEndCapture(cs[0], &graph);
         if (rc == ARW_OK && er != hipSuccess) rc = fail(ARW_E_LAUNCH, "hipStreamEndCapture failed");

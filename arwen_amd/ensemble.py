@@ -19,6 +19,7 @@ and restored around the trials).
 from __future__ import annotations
 
 import ctypes
+import sys
 import time
 
 import torch
@@ -47,7 +48,8 @@ class Ensemble:
     parameters (members may differ in nx, ny, nz)."""
 
     def __init__(self, states: list[DomainState], cfg: RunConfig,
-                 order: list[int] | None = None, groups: int = 1):
+                 order: list[int] | None = None, groups: int = 1,
+                 join_every: int = 0):
         if not states:
             raise ValueError("an ensemble needs at least one member")
         validate_for_step(cfg, acoustic=True)
@@ -66,6 +68,7 @@ class Ensemble:
         self.cfg = cfg
         self.order = list(order) if order is not None else None
         self.groups = int(groups)
+        self.join_every = int(join_every)
         n = len(states)
         Dom = _lib.ArwDomain64 if dt0 == torch.float64 else _lib.ArwDomain
         arr = (Dom * n)()
@@ -101,10 +104,14 @@ class Ensemble:
         for st in self.states:
             st.elapsed_seconds += n * self.cfg.dt
 
-    def capture(self, nsteps: int) -> None:
-        """Record nsteps steps as one HIP graph (nothing runs yet)."""
+    def capture(self, nsteps: int, join_every: int | None = None) -> None:
+        """Record nsteps steps as one HIP graph (nothing runs yet).  With
+        several groups the parallel branches rejoin every ``join_every``
+        steps (default: the layout's own, else only at the end)."""
+        if join_every is None:
+            join_every = self.join_every
         name, fn = self._fn("ensemble_capture")
-        _lib.check(fn(self._h, int(nsteps)), name)
+        _lib.check(fn(self._h, int(nsteps), int(join_every)), name)
         self.graph_steps = int(nsteps)
 
     def launch(self) -> None:
@@ -169,15 +176,33 @@ def member_orders(states) -> dict[str, list[int]]:
 _TUNED: dict = {}
 
 
-def autotune(states: list[DomainState], cfg: RunConfig, *, steps: int = 2,
-             reps: int = 3, groups=(1, 2, 4), orders=None,
-             verbose: bool = False) -> dict:
-    """Time every (order, groups) layout on a graph of ``steps`` steps and
-    return the fastest: {"order", "groups", "ms_per_step", "trials"}.
-    The members' state is restored afterwards; results are cached per
-    set of member shapes and step parameters."""
+def _time_layout(states, cfg, saved, order, groups, join_every, steps, reps):
+    with Ensemble(states, cfg, order=order, groups=groups,
+                  join_every=join_every) as ens:
+        ens.capture(steps)
+        ens.launch()                               # warm-up replay
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ens.launch()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / (reps * steps)
+    restore_state(states, saved)
+    return ms
+
+
+def autotune(states: list[DomainState], cfg: RunConfig, *, steps: int = 10,
+             reps: int = 2, groups=(1, 2, 4, 8), joins=(1, 5, 0),
+             orders=None, verbose: bool = False) -> dict:
+    """Pick the launch layout by timing it on the ensemble itself: first
+    the member order of the work tables (one group), then the number of
+    groups that run as parallel graph branches and how often the branches
+    rejoin.  Returns {"order", "order_name", "groups", "join_every",
+    "ms_per_step", "trials"}; the members' state is restored afterwards
+    and the result is cached per member shapes and step parameters."""
     key = (tuple(st.shape for st in states), states[0].dtype,
-           tuple(getattr(cfg, f) for f in SHARED), tuple(groups))
+           tuple(getattr(cfg, f) for f in SHARED), tuple(groups),
+           tuple(joins), steps)
     if key in _TUNED:
         return _TUNED[key]
     saved = save_state(states)
@@ -185,27 +210,26 @@ def autotune(states: list[DomainState], cfg: RunConfig, *, steps: int = 2,
     if orders is not None:
         cands = {k: v for k, v in cands.items() if k in orders}
     trials = []
-    for oname, order in cands.items():
-        for g in groups:
-            if g > len(states):
-                continue
-            with Ensemble(states, cfg, order=order, groups=g) as ens:
-                ens.capture(steps)
-                ens.launch()                       # warm-up replay
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                for _ in range(reps):
-                    ens.launch()
-                torch.cuda.synchronize()
-                ms = (time.perf_counter() - t0) * 1e3 / (reps * steps)
-            restore_state(states, saved)
-            trials.append({"order": oname, "groups": g, "ms_per_step": ms})
-            if verbose:
-                print(f"autotune order={oname} groups={g}: {ms:.3f} ms/step",
-                      flush=True)
+
+    def run(oname, g, j):
+        ms = _time_layout(states, cfg, saved, cands[oname], g, j, steps,
+                          reps)
+        trials.append({"order": oname, "groups": g, "join_every": j,
+                       "ms_per_step": ms})
+        if verbose:
+            print(f"autotune order={oname} groups={g} join_every={j}: "
+                  f"{ms:.3f} ms/step", file=sys.stderr, flush=True)
+        return ms
+
+    best_order = min(cands, key=lambda o: run(o, 1, 0))
+    for g in groups:
+        if g == 1 or g > len(states):
+            continue
+        for j in joins:
+            run(best_order, g, j)
     best = min(trials, key=lambda t: t["ms_per_step"])
     out = {"order": cands[best["order"]], "order_name": best["order"],
-           "groups": best["groups"], "ms_per_step": best["ms_per_step"],
-           "trials": trials}
+           "groups": best["groups"], "join_every": best["join_every"],
+           "ms_per_step": best["ms_per_step"], "trials": trials}
     _TUNED[key] = out
     return out

@@ -201,8 +201,11 @@ int32_t arw_ensemble_create(const arw_domain *members, int32_t n,
 /* nsteps steps, launched on stream. */
 int32_t arw_ensemble_step(arw_ensemble *e, int32_t nsteps, void *stream);
 /* Capture nsteps steps into one HIP graph held by the handle (replacing
- * an earlier one); arw_ensemble_launch runs it on stream. */
-int32_t arw_ensemble_capture(arw_ensemble *e, int32_t nsteps);
+ * an earlier one); arw_ensemble_launch runs it on stream.  With several
+ * groups the branches rejoin every join_every steps (0: only at the end),
+ * part of the launch schedule the autotuner picks. */
+int32_t arw_ensemble_capture(arw_ensemble *e, int32_t nsteps,
+                             int32_t join_every);
 int32_t arw_ensemble_launch(arw_ensemble *e, void *stream);
 int32_t arw_ensemble_destroy(arw_ensemble *e);
 
@@ -389,7 +392,8 @@ int32_t arw64_ensemble_create(const arw64_domain *members, int32_t n,
                               int32_t groups, arw64_ensemble **out);
 int32_t arw64_ensemble_step(arw64_ensemble *e, int32_t nsteps,
                             void *stream);
-int32_t arw64_ensemble_capture(arw64_ensemble *e, int32_t nsteps);
+int32_t arw64_ensemble_capture(arw64_ensemble *e, int32_t nsteps,
+                               int32_t join_every);
 int32_t arw64_ensemble_launch(arw64_ensemble *e, void *stream);
 int32_t arw64_ensemble_destroy(arw64_ensemble *e);
 

@@ -54,15 +54,18 @@ def test_eager_ensemble_matches_member_steps_bitwise(terrain):
     _assert_same(seq, ens_states)
 
 
-@pytest.mark.parametrize("groups,order", [(1, None), (2, "largest_first"),
-                                          (3, "smallest_first")])
-def test_graph_replays_match_member_steps_bitwise(groups, order):
+@pytest.mark.parametrize("groups,order,join", [(1, None, 0),
+                                               (2, "largest_first", 0),
+                                               (3, "smallest_first", 1)])
+def test_graph_replays_match_member_steps_bitwise(groups, order, join):
     from arwen_amd import dycore
     from arwen_amd.ensemble import Ensemble, member_orders
     cfg, seq, ens_states = _pair()
-    print(f"ensemble graph groups={groups} order={order}", flush=True)
+    print(f"ensemble graph groups={groups} order={order} join={join}",
+          flush=True)
     ordl = member_orders(ens_states)[order] if order else None
-    with Ensemble(ens_states, cfg, order=ordl, groups=groups) as ens:
+    with Ensemble(ens_states, cfg, order=ordl, groups=groups,
+                  join_every=join) as ens:
         ens.capture(2)
         for rep in range(3):
             ens.launch()
@@ -100,7 +103,8 @@ def test_autotune_leaves_state_and_picks_a_layout():
         for n, t in sv.items():
             assert torch.equal(getattr(st, n), t), n
     assert res["groups"] in (1, 2) and sorted(res["order"]) == list(range(5))
-    assert len(res["trials"]) == 6
+    assert res["join_every"] in (0, 1, 5)
+    assert len(res["trials"]) == 6          # 3 orders, then 3 joins x 2 groups
 
 
 def test_ensemble_rejects_bad_members():
