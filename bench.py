@@ -244,7 +244,8 @@ def main_precision(args):
     ``--prec-steps`` steps in plain FP32, in the precision form (FP32 bulk,
     Kahan column sums, FP64 tridiagonal) and in the FP64 build of the same
     discretisation; per-field max and RMS error of both FP32 runs against
-    FP64, and the three step times."""
+    FP64 after 1, 10 and the last step, and the three step times.  The
+    FP64 snapshots stay in HBM."""
     import dataclasses
 
     import torch
@@ -253,8 +254,10 @@ def main_precision(args):
 
     case, desc = build_case("c3")
     n = args.prec_steps
+    marks = sorted({1, min(10, n), n})
     fields = ("u", "v", "w", "thp", "php", "mup", "p")
-    runs = {}
+    ref = {}                                    # step -> field -> FP64 tensor
+    runs, err = {}, {}
     for name, dtype, prec in (("fp64", torch.float64, False),
                               ("fp32", torch.float32, False),
                               ("precision_form", torch.float32, True)):
@@ -262,35 +265,40 @@ def main_precision(args):
         st = DomainState.from_host(case, dtype=dtype)
         m0 = dycore.domain_mass_measure(st)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
+        t_step = 0.0
+        e_run = {}
         for k in range(n):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
             dycore.step(st, cfg)
+            torch.cuda.synchronize()
+            t_step += time.perf_counter() - t0
+            if k + 1 in marks:
+                if name == "fp64":
+                    ref[k + 1] = {f: getattr(st, f).clone() for f in fields}
+                else:
+                    e = {}
+                    for f in fields:
+                        b = ref[k + 1][f]
+                        dlt = getattr(st, f).double() - b
+                        scale = float(b.abs().max())
+                        e[f] = {"max_rel": float(dlt.abs().max()) / scale,
+                                "rms_rel": float(dlt.pow(2).mean().sqrt())
+                                / scale}
+                        del dlt
+                    e_run[f"step_{k + 1}"] = e
             if (k + 1) % 100 == 0:
-                torch.cuda.synchronize()
                 print(f"{name}: {k + 1}/{n} steps", file=sys.stderr,
                       flush=True)
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) * 1e3 / n
         m1 = dycore.domain_mass_measure(st)
         if dycore.stability_report(st, cfg)["nan"]:
             raise SystemExit(f"bench.py: {name} went NaN")
-        runs[name] = {"ms_per_step": round(ms, 3),
-                      "mass_rel_change": (m1 - m0) / m0,
-                      "state": {f: getattr(st, f).double().cpu()
-                                for f in fields}}
+        runs[name] = {"ms_per_step": round(t_step * 1e3 / n, 3),
+                      "mass_rel_change": (m1 - m0) / m0}
+        if name != "fp64":
+            err[name] = e_run
         del st
         torch.cuda.empty_cache()
-    ref = runs["fp64"]["state"]
-    err = {}
-    for name in ("fp32", "precision_form"):
-        e = {}
-        for f in fields:
-            a, b = runs[name]["state"][f], ref[f]
-            scale = float(b.abs().max())
-            d = a - b
-            e[f] = {"max_rel": float(d.abs().max()) / scale,
-                    "rms_rel": float(d.pow(2).mean().sqrt()) / scale}
-        err[name] = e
     out = {"workload": desc + f", {n} steps", "steps": n,
            "ms_per_step": {k: v["ms_per_step"] for k, v in runs.items()},
            "mass_rel_change": {k: v["mass_rel_change"]
