@@ -146,13 +146,16 @@ def load():
     for name in ("arw_flux_div_scalar", "arw_flux_div_u", "arw_flux_div_v",
                  "arw_flux_div_w"):
         sigs[name] = (flux, _I)
+    flux_adj = [_P] * 12 + [_F, _F, _I, _I, _I, _P]
+    for name in FLUX_DIV_ADJOINTS:
+        sigs["arw_" + name] = (flux_adj, _I)
     # the FP64 build of the same entry points
     d64 = ctypes.POINTER(ArwDomain64)
     for name in FP64_ENTRY_POINTS:
         base = "arw_" + name[len("arw64_"):]
         args, res = sigs[base]
         args = [d64 if a is dp else a for a in args]
-        if name == "arw64_flux_div_scalar":
+        if name.startswith("arw64_flux_div_"):
             args = [_D if a is _F else a for a in args]
         sigs[name] = (args, res)
     for name, (args, res) in sigs.items():
@@ -176,18 +179,23 @@ def check(rc: int, what: str) -> None:
                          f"{msg.decode() if msg else ''}")
 
 
+#: adjoints of the advection kernels (arw_ / arw64_ prefixed)
+FLUX_DIV_ADJOINTS = ("flux_div_scalar_adjoint", "flux_div_u_adjoint",
+                     "flux_div_v_adjoint", "flux_div_w_adjoint")
+
 #: FP64 entry points (same discretisation in double precision)
 FP64_ENTRY_POINTS = (
     "arw64_abi_version", "arw64_last_error", "arw64_step",
     "arw64_calc_p_alpha", "arw64_stage_fluxes", "arw64_stage_prepare",
     "arw64_acoustic_substep", "arw64_calc_coefs", "arw64_small_step_finish",
     "arw64_set_w_surface", "arw64_health", "arw64_domain_mass",
-    "arw64_flux_div_scalar", "arw64_step_ckpt", "arw64_tracer_step",
+    "arw64_flux_div_scalar", "arw64_flux_div_u", "arw64_flux_div_v",
+    "arw64_flux_div_w", "arw64_step_ckpt", "arw64_tracer_step",
     "arw64_tracer_adjoint", "arw64_acoustic_substep_adjoint",
     "arw64_acoustic_adjoint_work", "arw64_ensemble_create",
     "arw64_ensemble_step",
     "arw64_ensemble_capture", "arw64_ensemble_launch",
-    "arw64_ensemble_destroy")
+    "arw64_ensemble_destroy") + tuple("arw64_" + n for n in FLUX_DIV_ADJOINTS)
 
 
 def exported_symbols() -> list[str]:
@@ -203,7 +211,8 @@ def exported_symbols() -> list[str]:
             "arw_acoustic_substep_adjoint", "arw_acoustic_adjoint_work",
             "arw_ensemble_create",
             "arw_ensemble_step", "arw_ensemble_capture",
-            "arw_ensemble_launch", "arw_ensemble_destroy"]
+            "arw_ensemble_launch", "arw_ensemble_destroy"] + [
+            "arw_" + n for n in FLUX_DIV_ADJOINTS]
 
 
 class KernelTimer:

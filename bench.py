@@ -329,6 +329,7 @@ def main_adjoint(args):
 
     import torch
     from arwen_amd import adjoint, cases, dycore
+    from arwen_amd import advection as A_
     from arwen_amd.state import DomainState
 
     case = cases.moist_tracer(moist_adv_opt=0)
@@ -418,6 +419,20 @@ def main_adjoint(args):
                     "acoustic_adjoint_ms": round(t_sub_adj, 4),
                     "acoustic_adjoint_to_forward": round(t_sub_adj / t_sub,
                                                          3)})
+        # the four advection kernels at the stage mass fluxes of the
+        # state: forward tendency against the adjoint (field and all three
+        # mass-flux gradients), both writing fresh zeroed outputs
+        ru, rv, ww = dycore.stage_fluxes(st, cfg)
+        fields = {"scalar": st.thp, "u": st.u, "v": st.v, "w": st.w}
+        for op, q in fields.items():
+            lam_q = torch.randn_like(q)
+            t_f = _time_ms(lambda: A_.flux_div(op, q, ru, rv, ww, st, cfg.dx,
+                                               cfg.dy), reps)
+            t_a = _time_ms(lambda: A_.flux_div_adjoint(
+                op, lam_q, q, ru, rv, ww, st, cfg.dx, cfg.dy), reps)
+            rec[f"advect_{op}_forward_ms"] = round(t_f, 4)
+            rec[f"advect_{op}_adjoint_ms"] = round(t_a, 4)
+            rec[f"advect_{op}_adjoint_to_forward"] = round(t_a / t_f, 3)
         out[name] = rec
         del st, ck
         torch.cuda.empty_cache()

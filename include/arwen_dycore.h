@@ -272,6 +272,44 @@ int32_t arw_flux_div_w(const float *w, const float *ru, const float *rv,
                        float dy_inv, int32_t nz, int32_t ny, int32_t nx,
                        void *stream);
 
+/* Adjoints of the four advection kernels above (no counterpart in the
+ * reference, which has no adjoint): for the output adjoint lam (shaped like
+ * that kernel's tend) they ADD the transposed Jacobian of tend with respect
+ * to the advected field, ru, rv and rw, at the given (field, ru, rv, rw),
+ * into g_q, g_ru, g_rv, g_rw (any of which may be NULL to skip it).  The
+ * duplicate column / row of ru, rv and of the u / v fields are separate
+ * inputs, as the forward reads them.  |vel| is differentiated as
+ * sign(vel), sign(0) = 0.  Outputs must not alias inputs. */
+int32_t arw_flux_div_scalar_adjoint(const float *q, const float *ru,
+                                    const float *rv, const float *rw,
+                                    const float *lam, float *g_q, float *g_ru,
+                                    float *g_rv, float *g_rw,
+                                    const float *rdz, const float *fnm,
+                                    const float *fnp, float dx_inv,
+                                    float dy_inv, int32_t nz, int32_t ny,
+                                    int32_t nx, void *stream);
+int32_t arw_flux_div_u_adjoint(const float *q, const float *ru,
+                               const float *rv, const float *rw,
+                               const float *lam, float *g_q, float *g_ru,
+                               float *g_rv, float *g_rw, const float *rdz,
+                               const float *fnm, const float *fnp,
+                               float dx_inv, float dy_inv, int32_t nz,
+                               int32_t ny, int32_t nx, void *stream);
+int32_t arw_flux_div_v_adjoint(const float *q, const float *ru,
+                               const float *rv, const float *rw,
+                               const float *lam, float *g_q, float *g_ru,
+                               float *g_rv, float *g_rw, const float *rdz,
+                               const float *fnm, const float *fnp,
+                               float dx_inv, float dy_inv, int32_t nz,
+                               int32_t ny, int32_t nx, void *stream);
+int32_t arw_flux_div_w_adjoint(const float *q, const float *ru,
+                               const float *rv, const float *rw,
+                               const float *lam, float *g_q, float *g_ru,
+                               float *g_rv, float *g_rw, const float *rdz,
+                               const float *fnm, const float *fnp,
+                               float dx_inv, float dy_inv, int32_t nz,
+                               int32_t ny, int32_t nx, void *stream);
+
 /* Stage preparation: EOS (when do_eos), Omega, the slow tendencies
  * ru_t..rph_t (advection + PGF + buoyancy + rhs_ph + constant-K
  * diffusion; replaces dycore.py:502-569 _add_slow_tendencies and
@@ -354,6 +392,55 @@ int32_t arw64_flux_div_scalar(const double *q, const double *ru,
                               const double *fnm, const double *fnp,
                               double dx_inv, double dy_inv, int32_t nz,
                               int32_t ny, int32_t nx, void *stream);
+
+int32_t arw64_flux_div_u(const double *u, const double *ru,
+                         const double *rv, const double *rw, double *tend,
+                         const double *rdnw, const double *fnm,
+                         const double *fnp, double dx_inv, double dy_inv,
+                         int32_t nz, int32_t ny, int32_t nx, void *stream);
+int32_t arw64_flux_div_v(const double *v, const double *ru,
+                         const double *rv, const double *rw, double *tend,
+                         const double *rdnw, const double *fnm,
+                         const double *fnp, double dx_inv, double dy_inv,
+                         int32_t nz, int32_t ny, int32_t nx, void *stream);
+int32_t arw64_flux_div_w(const double *w, const double *ru,
+                         const double *rv, const double *rw, double *tend,
+                         const double *rdn, const double *fnm,
+                         const double *fnp, double dx_inv, double dy_inv,
+                         int32_t nz, int32_t ny, int32_t nx, void *stream);
+int32_t arw64_flux_div_scalar_adjoint(const double *q, const double *ru,
+                                      const double *rv, const double *rw,
+                                      const double *lam, double *g_q,
+                                      double *g_ru, double *g_rv,
+                                      double *g_rw, const double *rdz,
+                                      const double *fnm, const double *fnp,
+                                      double dx_inv, double dy_inv,
+                                      int32_t nz, int32_t ny, int32_t nx,
+                                      void *stream);
+int32_t arw64_flux_div_u_adjoint(const double *q, const double *ru,
+                                 const double *rv, const double *rw,
+                                 const double *lam, double *g_q, double *g_ru,
+                                 double *g_rv, double *g_rw,
+                                 const double *rdz, const double *fnm,
+                                 const double *fnp, double dx_inv,
+                                 double dy_inv, int32_t nz, int32_t ny,
+                                 int32_t nx, void *stream);
+int32_t arw64_flux_div_v_adjoint(const double *q, const double *ru,
+                                 const double *rv, const double *rw,
+                                 const double *lam, double *g_q, double *g_ru,
+                                 double *g_rv, double *g_rw,
+                                 const double *rdz, const double *fnm,
+                                 const double *fnp, double dx_inv,
+                                 double dy_inv, int32_t nz, int32_t ny,
+                                 int32_t nx, void *stream);
+int32_t arw64_flux_div_w_adjoint(const double *q, const double *ru,
+                                 const double *rv, const double *rw,
+                                 const double *lam, double *g_q, double *g_ru,
+                                 double *g_rv, double *g_rw,
+                                 const double *rdz, const double *fnm,
+                                 const double *fnp, double dx_inv,
+                                 double dy_inv, int32_t nz, int32_t ny,
+                                 int32_t nx, void *stream);
 
 typedef struct arw64_tracer_ckpt {
     double *ru_m[3], *rv_m[3], *ww_m[3], *mup_new[3];

@@ -398,3 +398,224 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
     out["p_pp"] += (one + smdiv) * lpd
     out["p_pp_old"] += -smdiv * lpd
     return out
+
+
+# ---------------------------------------------------------------------------
+# Flux-form advection operators: brute-force Jacobians (tiny grids)
+# ---------------------------------------------------------------------------
+#
+# The operators D(q; ru, rv, rw) of dycore_np (flux_div_scalar/u/v/w) are
+# bilinear in the advected field and the mass fluxes except for |vel| in the
+# upwind term.  Away from vel = 0 that is sign(vel) vel, analytic, so the
+# complex-step derivative of D with respect to every input element (one
+# forward call per element) builds the exact Jacobian on a tiny grid; its
+# transpose is the reference for the hand-written device adjoints.
+
+def _flux5_cs(qm3, qm2, qm1, q0, qp1, qp2, vel):
+    sv = np.sign(np.real(vel))
+    return (vel * (37.0 * (q0 + qm1) - 8.0 * (qp1 + qm2) + (qp2 + qm3))
+            - sv * vel * (10.0 * (q0 - qm1) - 5.0 * (qp1 - qm2)
+                          + (qp2 - qm3))) / 60.0
+
+
+def _flux3_cs(qm2, qm1, q0, qp1, vel):
+    sv = np.sign(np.real(vel))
+    return (vel * (7.0 * (q0 + qm1) - (qp1 + qm2))
+            + sv * vel * (3.0 * (q0 - qm1) - (qp1 - qm2))) / 12.0
+
+
+def flux_div_jacobians(op, q, ru, rv, rw, rz, fnm, fnp, dx_inv, dy_inv,
+                       h=1e-30):
+    """{"q": J_q, "ru": J_ru, "rv": J_rv, "rw": J_rw}, each (n_tend, n_in),
+    of D = dycore_np.flux_div_<op> at (q, ru, rv, rw), by complex steps."""
+    fn = {"scalar": D.flux_div_scalar, "u": D.flux_div_u,
+          "v": D.flux_div_v, "w": D.flux_div_w}[op]
+    args = {"q": q, "ru": ru, "rv": rv, "rw": rw}
+    saved = (D.flux5, D.flux3)
+    D.flux5, D.flux3 = _flux5_cs, _flux3_cs
+    try:
+        base = fn(q, ru, rv, rw, rz, fnm, fnp, dx_inv, dy_inv)
+        out = {}
+        for name, arr in args.items():
+            cols = []
+            for idx in range(arr.size):
+                x = {k: v.astype(np.complex128) for k, v in args.items()}
+                x[name].reshape(-1)[idx] += 1j * h
+                t = fn(x["q"], x["ru"], x["rv"], x["rw"], rz, fnm, fnp,
+                       dx_inv, dy_inv)
+                cols.append(np.imag(t).reshape(-1) / h)
+            out[name] = np.stack(cols, axis=1)
+    finally:
+        D.flux5, D.flux3 = saved
+    out["tend"] = base
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Hand adjoints of the advection operators (field and mass fluxes)
+# ---------------------------------------------------------------------------
+#
+# Every operator is a sum over face fluxes F_f = sum_o w_o(vel_f) q[s_f + o]
+# with w_o(v) = C_o v + S_o |v| (flux5 / flux3) or a 2nd-order average, and
+# vel_f a fixed linear map of the mass fluxes.  With Lambda_f the adjoint of
+# F_f (the difference of the adjacent cells' output adjoints, scaled), the
+# transpose is
+#   dL/dq[c]    += sum over faces reading c of w_o(vel_f) Lambda_f,
+#   dL/dvel_f    = Lambda_f sum_o (C_o + S_o sign(vel_f)) q[s_f + o],
+# and dL/dru etc. gather dL/dvel_f through the transpose of the velocity
+# averages.  sign(0) = 0 (a subgradient of |v|).
+
+C5 = np.array([1.0, -8.0, 37.0, 37.0, -8.0, 1.0]) / 60.0
+S5 = np.array([1.0, -5.0, 10.0, -10.0, 5.0, -1.0]) / 60.0
+C3 = np.array([-1.0, 7.0, 7.0, -1.0]) / 12.0
+S3 = np.array([1.0, -3.0, 3.0, -1.0]) / 12.0
+
+
+def _h_adjoint(q, vel, lam, ax, rd, dup):
+    """One horizontal direction of flux5 transport along axis ax.  Cells
+    c = 0..n-1 (periodic); face f lies between cells f-1 and f.  dup: faces
+    0..n with face n a separate copy of face 0 (its own velocity, its flux
+    entering only cell n-1); else n periodic faces.  Returns (dL/dq,
+    dL/dvel)."""
+    n = q.shape[ax]
+    if dup:
+        z = np.zeros_like(np.take(lam, [0], axis=ax))
+        L = rd * (np.concatenate([lam, z], axis=ax)
+                  - np.concatenate([z, lam], axis=ax))
+    else:
+        L = rd * (lam - np.roll(lam, 1, axis=ax))
+    a, sv = np.abs(vel), np.sign(vel)
+    gq = np.zeros_like(q)
+    dv = np.zeros_like(vel)
+    for o in range(6):                       # face f reads cell f + o - 3
+        c = (C5[o] * vel + S5[o] * a) * L
+        if dup:
+            c = np.take(c, range(n), axis=ax) + np.where(
+                np.arange(n).reshape([-1 if d == (ax % q.ndim) else 1
+                                      for d in range(q.ndim)]) == 0,
+                np.take(c, [n], axis=ax), 0.0)
+        gq += np.roll(c, o - 3, axis=ax)
+        qs = np.roll(q, 3 - o, axis=ax)
+        if dup:
+            qs = np.concatenate([qs, np.take(qs, [0], axis=ax)], axis=ax)
+        dv += (C5[o] + S5[o] * sv) * qs
+    return gq, dv * L
+
+
+def _zhalf_adjoint(q, vel, lam, rdz, fnm, fnp):
+    """Vertical transport of a half-level field (faces kf = 1..nz-1, 2nd
+    order one face in from each boundary, flux3 inside)."""
+    nz = q.shape[0]
+    gq = np.zeros_like(q)
+    dv = np.zeros_like(vel)
+    for kf in range(1, nz):
+        L = rdz[kf] * lam[kf] - rdz[kf - 1] * lam[kf - 1]
+        v = vel[kf]
+        if kf == 1 or kf == nz - 1:
+            gq[kf] += v * fnm[kf] * L
+            gq[kf - 1] += v * fnp[kf] * L
+            dv[kf] = L * (fnm[kf] * q[kf] + fnp[kf] * q[kf - 1])
+        else:
+            for o in range(4):               # face kf reads cell kf + o - 2
+                gq[kf + o - 2] += (C3[o] * v + S3[o] * np.abs(v)) * L
+                dv[kf] += L * (C3[o] + S3[o] * np.sign(v)) * q[kf + o - 2]
+    return gq, dv
+
+
+def _zfull_adjoint(w, vel, lam, rdn):
+    """Vertical transport of w: faces at mass levels m = 0..nz-1 between
+    w[m] and w[m+1] (2nd order at m = 0 and nz-1, flux3 inside); the
+    output rows 0 and nz are not part of the operator."""
+    nz = w.shape[0] - 1
+    gw = np.zeros_like(w)
+    dv = np.zeros_like(vel)
+    for m in range(nz):
+        L = -(rdn[m] * lam[m]) if m >= 1 else np.zeros_like(lam[0])
+        if m + 1 <= nz - 1:
+            L = L + rdn[m + 1] * lam[m + 1]
+        v = vel[m]
+        if m == 0 or m == nz - 1:
+            gw[m] += 0.5 * v * L
+            gw[m + 1] += 0.5 * v * L
+            dv[m] = L * 0.5 * (w[m + 1] + w[m])
+        else:
+            for o in range(4):               # face m reads w[m + o - 1]
+                gw[m + o - 1] += (C3[o] * v + S3[o] * np.abs(v)) * L
+                dv[m] += L * (C3[o] + S3[o] * np.sign(v)) * w[m + o - 1]
+    return gw, dv
+
+
+def flux_div_adjoint(op, lam, q, ru, rv, rw, rdz, fnm, fnp, dx_inv,
+                     dy_inv):
+    """Transpose of the Jacobian of dycore_np.flux_div_<op> at (q, ru, rv,
+    rw) applied to lam (an array shaped like the operator's output):
+    {"q", "ru", "rv", "rw"} gradients (reference advection.cu:206-1051)."""
+    g = {"ru": np.zeros_like(ru), "rv": np.zeros_like(rv),
+         "rw": np.zeros_like(rw)}
+    half = 0.5
+    if op == "scalar":
+        gx, g["ru"] = _h_adjoint(q, ru, lam, -1, dx_inv, True)
+        gy, g["rv"] = _h_adjoint(q, rv, lam, -2, dy_inv, True)
+        gz, g["rw"] = _zhalf_adjoint(q, rw, lam, rdz, fnm, fnp)
+        g["q"] = gx + gy + gz
+        return g
+    if op == "u":
+        nx = q.shape[-1] - 1
+        le = lam[..., :nx].copy()
+        le[..., 0] += lam[..., nx]
+        uc = q[..., :nx]
+        velx = half * (ru[..., :nx] + ru[..., 1:])      # mass centre m
+        gx, dvp = _h_adjoint(uc, np.roll(velx, 1, axis=-1), le, -1,
+                             dx_inv, False)
+        dvx = np.roll(dvp, -1, axis=-1)
+        g["ru"][..., :nx] += half * dvx
+        g["ru"][..., 1:] += half * dvx
+        vely = half * (np.roll(rv, 1, axis=-1) + rv)
+        gy, dvy = _h_adjoint(uc, vely, le, -2, dy_inv, True)
+        g["rv"] += half * (dvy + np.roll(dvy, -1, axis=-1))
+        velz = half * (np.roll(rw, 1, axis=-1) + rw)
+        gz, dvz = _zhalf_adjoint(uc, velz, le, rdz, fnm, fnp)
+        g["rw"] += half * (dvz + np.roll(dvz, -1, axis=-1))
+        g["q"] = np.zeros_like(q)
+        g["q"][..., :nx] = gx + gy + gz
+        return g
+    if op == "v":
+        ny = q.shape[-2] - 1
+        le = lam[:, :ny, :].copy()
+        le[:, 0, :] += lam[:, ny, :]
+        vc = q[:, :ny, :]
+        velx = half * (np.roll(ru, 1, axis=-2) + ru)
+        gx, dvx = _h_adjoint(vc, velx, le, -1, dx_inv, True)
+        g["ru"] += half * (dvx + np.roll(dvx, -1, axis=-2))
+        vely = half * (rv[:, :ny, :] + rv[:, 1:, :])
+        gy, dvp = _h_adjoint(vc, np.roll(vely, 1, axis=-2), le, -2,
+                             dy_inv, False)
+        dvy = np.roll(dvp, -1, axis=-2)
+        g["rv"][:, :ny] += half * dvy
+        g["rv"][:, 1:] += half * dvy
+        velz = half * (np.roll(rw, 1, axis=-2) + rw)
+        gz, dvz = _zhalf_adjoint(vc, velz, le, rdz, fnm, fnp)
+        g["rw"] += half * (dvz + np.roll(dvz, -1, axis=-2))
+        g["q"] = np.zeros_like(q)
+        g["q"][:, :ny] = gx + gy + gz
+        return g
+    if op == "w":
+        nz = q.shape[0] - 1
+        gq = np.zeros_like(q)
+        for k in range(1, nz):
+            velx = fnm[k] * ru[k] + fnp[k] * ru[k - 1]
+            gx, dvx = _h_adjoint(q[k], velx, lam[k], -1, dx_inv, True)
+            vely = fnm[k] * rv[k] + fnp[k] * rv[k - 1]
+            gy, dvy = _h_adjoint(q[k], vely, lam[k], -2, dy_inv, True)
+            gq[k] += gx + gy
+            g["ru"][k] += fnm[k] * dvx
+            g["ru"][k - 1] += fnp[k] * dvx
+            g["rv"][k] += fnm[k] * dvy
+            g["rv"][k - 1] += fnp[k] * dvy
+        velz = half * (rw[:-1] + rw[1:])
+        gz, dvz = _zfull_adjoint(q, velz, lam, rdz)
+        g["rw"][:-1] += half * dvz
+        g["rw"][1:] += half * dvz
+        g["q"] = gq + gz
+        return g
+    raise ValueError(op)

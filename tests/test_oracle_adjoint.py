@@ -194,3 +194,94 @@ def test_planted_acoustic_adjoint_mistake_fails(monkeypatch):
                                                            jty2["mu_pp"]))
     assert good <= 1e-11 * abs(np.vdot(jdx["ph_pp"], y["ph_pp"]))
     assert bad > 1e3 * max(good, 1e-300)
+
+
+# ---------------------------------------------------------------------------
+# Brute-force Jacobians of the advection operators
+# ---------------------------------------------------------------------------
+
+def advection_case(op, seed=7, nz=5, ny=4, nx=6):
+    """Tiny periodic inputs for flux_div_<op>: field, mass fluxes with
+    consistent duplicate faces, coordinate factors (vel kept away from 0)."""
+    rng = np.random.default_rng(seed)
+    shape = {"scalar": (nz, ny, nx), "u": (nz, ny, nx + 1),
+             "v": (nz, ny + 1, nx), "w": (nz + 1, ny, nx)}[op]
+    q = rng.standard_normal(shape)
+    sgn = lambda s: np.where(rng.uniform(size=s) < 0.5, -1.0, 1.0)
+    ru = sgn((nz, ny, nx + 1)) * rng.uniform(0.5, 2.0, (nz, ny, nx + 1))
+    ru[..., -1] = ru[..., 0]
+    rv = sgn((nz, ny + 1, nx)) * rng.uniform(0.5, 2.0, (nz, ny + 1, nx))
+    rv[:, -1] = rv[:, 0]
+    rw = sgn((nz + 1, ny, nx)) * rng.uniform(0.5, 2.0, (nz + 1, ny, nx))
+    rw[0] = 0.0
+    rw[-1] = 0.0
+    if op == "u":
+        q[..., -1] = q[..., 0]
+    if op == "v":
+        q[:, -1] = q[:, 0]
+    rz = -1.0 / rng.uniform(0.05, 0.2, nz + (1 if op == "w" else 0))
+    fnm = np.r_[0.0, rng.uniform(0.4, 0.6, nz)]
+    fnp = np.r_[0.0, 1.0 - fnm[1:]]
+    return q, ru, rv, rw, rz, fnm, fnp, 0.37, 0.29
+
+
+@pytest.mark.parametrize("op", ["scalar", "u", "v", "w"])
+def test_complex_step_jacobian_matches_differences(op):
+    args = advection_case(op)
+    J = A.flux_div_jacobians(op, *args)
+    rng = np.random.default_rng(1)
+    fn = {"scalar": D.flux_div_scalar, "u": D.flux_div_u,
+          "v": D.flux_div_v, "w": D.flux_div_w}[op]
+    q, ru, rv, rw = args[:4]
+    for name, x in (("q", q), ("ru", ru), ("rv", rv), ("rw", rw)):
+        d = rng.standard_normal(x.shape) * 1e-6
+        xs = {"q": q, "ru": ru, "rv": rv, "rw": rw}
+        xp = dict(xs); xp[name] = x + d
+        xm = dict(xs); xm[name] = x - d
+        fd = (fn(xp["q"], xp["ru"], xp["rv"], xp["rw"], *args[4:])
+              - fn(xm["q"], xm["ru"], xm["rv"], xm["rw"], *args[4:])) / 2
+        jd = (J[name] @ d.reshape(-1)).reshape(fd.shape)
+        assert np.allclose(fd, jd, rtol=1e-6, atol=1e-12), name
+
+
+def test_scalar_jacobian_transpose_is_the_hand_adjoint():
+    q, ru, rv, rw, rz, fnm, fnp, dxi, dyi = advection_case("scalar")
+    J = A.flux_div_jacobians("scalar", q, ru, rv, rw, rz, fnm, fnp, dxi, dyi)
+    lam = np.random.default_rng(2).standard_normal(q.shape)
+    ref = A.flux_div_scalar_adjoint(lam, ru, rv, rw, rz, fnm, fnp, dxi, dyi)
+    got = (J["q"].T @ lam.reshape(-1)).reshape(q.shape)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("op", ["scalar", "u", "v", "w"])
+@pytest.mark.parametrize("shape", [(5, 4, 6), (6, 3, 7), (4, 1, 5)])
+def test_advection_adjoint_is_jacobian_transpose(op, shape):
+    """The hand adjoint of every advection operator equals the transpose
+    of its brute-force Jacobian, for the field and all three mass fluxes,
+    with independent (inconsistent) duplicate faces."""
+    nz, ny, nx = shape
+    args = list(advection_case(op, seed=11, nz=nz, ny=ny, nx=nx))
+    rng = np.random.default_rng(3)
+    for i in (1, 2):          # break the duplicate-face consistency
+        args[i] = args[i] * rng.uniform(0.5, 1.5, args[i].shape)
+    if op in ("u", "v"):
+        args[0] = args[0] + rng.standard_normal(args[0].shape)
+    J = A.flux_div_jacobians(op, *args)
+    lam = rng.standard_normal(J["tend"].shape)
+    got = A.flux_div_adjoint(op, lam, *args)
+    for name in ("q", "ru", "rv", "rw"):
+        ref = (J[name].T @ lam.reshape(-1)).reshape(got[name].shape)
+        scale = np.abs(ref).max()
+        assert np.abs(got[name] - ref).max() <= 1e-12 * scale, name
+
+
+def test_planted_advection_adjoint_mistake_fails(monkeypatch):
+    """A wrong upwind coefficient in the adjoint (S5 sign) is caught by the
+    Jacobian comparison."""
+    args = advection_case("u", seed=5)
+    J = A.flux_div_jacobians("u", *args)
+    lam = np.random.default_rng(4).standard_normal(J["tend"].shape)
+    monkeypatch.setattr(A, "S5", -A.S5)
+    got = A.flux_div_adjoint("u", lam, *args)
+    ref = (J["ru"].T @ lam.reshape(-1)).reshape(got["ru"].shape)
+    assert np.abs(got["ru"] - ref).max() > 1e-3 * np.abs(ref).max()
