@@ -50,11 +50,11 @@ def _coef(x, like: torch.Tensor, n: int, name: str) -> torch.Tensor:
 
 
 def _coords(op, coord, q, nz):
+    """rdnw (rdn for w), fnm, fnp as device vectors of the field's dtype;
+    the kernels read entries 0..nz-1 of each."""
     rdz = coord.rdn if op == "w" else coord.rdnw
-    return (_coef(rdz, q, nz + (1 if op == "w" else 0), "rdn" if op == "w"
-                  else "rdnw"),
-            _coef(coord.fnm, q, nz + 1, "fnm"),
-            _coef(coord.fnp, q, nz + 1, "fnp"))
+    return (_coef(rdz, q, nz, "rdn" if op == "w" else "rdnw"),
+            _coef(coord.fnm, q, nz, "fnm"), _coef(coord.fnp, q, nz, "fnp"))
 
 
 def _check(op, q, ru, rv, rw, extra):

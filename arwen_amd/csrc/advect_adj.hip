@@ -13,7 +13,10 @@
 //   dL/dru ... = the transpose of the velocity averages applied to dL/dvel.
 // sign(0) = 0 (a subgradient of |v|).  One thread per point of the union of
 // the staggers gathers all four gradients at its indices, so there are no
-// atomics and every output is written by exactly one thread.
+// atomics and every output is written by exactly one thread; what the
+// gathers at one cell share (the output adjoint of its neighbours, the
+// velocities of the faces that read it, the field stencil) is loaded once
+// per thread.
 //
 // Faces along a horizontal line of n cells (face f between cells f-1 and
 // f): "dup" lines have faces 0..n, face n being a separate copy of face 0
@@ -210,89 +213,141 @@ struct OpW {
 };
 
 // ---------------------------------------------------------------------------
-// Horizontal directions.  The x line of (k, j) and the y line of (k, i)
-// are written once each; DUP selects the face layout.
+// Horizontal directions.  A line is the x line of (k, j) or the y line of
+// (k, i) of one operator: its cells' folded output adjoint and field, and
+// the velocity of each face.  "Windows" hold what the gathers at one cell
+// need, loaded once: the output adjoint of cells c0-3..c0+3 and the
+// velocities of the six faces c0-2..c0+3 that read cell c0.
 // ---------------------------------------------------------------------------
 
-template <class OP>
-__device__ real lam_x(const Args& a, int k, int j, int f)
+// c in [-3, n + 3) to [0, n): one correction when n >= 4
+__device__ __forceinline__ int wrapc(int c, int n)
 {
-    const int n = a.nx;
-    if (OP::dupx) {
-        real r = (f < n) ? OP::lam(a, k, j, f) : R(0.0);
-        if (f > 0) r -= OP::lam(a, k, j, f - 1);
-        return a.rdx * r;
-    }
-    return a.rdx * (OP::lam(a, k, j, f) - OP::lam(a, k, j, f == 0 ? n - 1 : f - 1));
+    if (n >= 4) return c < 0 ? c + n : (c >= n ? c - n : c);
+    return pmod(c, n);
 }
 
 template <class OP>
-__device__ real lam_y(const Args& a, int k, int g, int i)
-{
-    const int n = a.ny;
-    if (OP::dupy) {
-        real r = (g < n) ? OP::lam(a, k, g, i) : R(0.0);
-        if (g > 0) r -= OP::lam(a, k, g - 1, i);
-        return a.rdy * r;
-    }
-    return a.rdy * (OP::lam(a, k, g, i) - OP::lam(a, k, g == 0 ? n - 1 : g - 1, i));
-}
+struct XLine {
+    const Args& a;
+    int k, j;
+    static constexpr bool dup = OP::dupx;
+    __device__ int n() const { return a.nx; }
+    __device__ real rd() const { return a.rdx; }
+    __device__ real lam(int c) const { return OP::lam(a, k, j, c); }
+    __device__ real q(int c) const { return OP::q(a, k, j, c); }
+    __device__ real vel(int f) const { return OP::velx(a, k, j, f); }
+};
 
 template <class OP>
-__device__ real field_x(const Args& a, int k, int j, int i)
+struct YLine {
+    const Args& a;
+    int k, i;
+    static constexpr bool dup = OP::dupy;
+    __device__ int n() const { return a.ny; }
+    __device__ real rd() const { return a.rdy; }
+    __device__ real lam(int c) const { return OP::lam(a, k, c, i); }
+    __device__ real q(int c) const { return OP::q(a, k, c, i); }
+    __device__ real vel(int g) const { return OP::vely(a, k, g, i); }
+};
+
+// Face c0-2+t separates cells c0-3+t (left) and c0-2+t (right).  On a dup
+// line the periodic face 0 is two faces: face 0 (velocity vel(0), flux into
+// the right cell) and face n (vel(n), flux out of the left cell), hence a
+// velocity per side.
+struct Win {
+    real lw[7];           // output adjoint of cells c0-3 .. c0+3
+    real vr[6], vl[6];    // velocity of face c0-2+t seen from right / left
+};
+
+template <class L>
+__device__ __forceinline__ void load_win(const L& l, int c0, Win& w)
 {
-    real acc = R(0.0);
+    const int n = l.n();
 #pragma unroll
-    for (int o = 0; o < 6; ++o) {            // face f reads cell f + o - 3
-        const int f = pmod(i + 3 - o, a.nx);
-        real v = OP::velx(a, k, j, f);
-        acc += (c5(o) * v + s5(o) * fabs(v)) * lam_x<OP>(a, k, j, f);
-        if (OP::dupx && f == 0) {
-            v = OP::velx(a, k, j, a.nx);
-            acc += (c5(o) * v + s5(o) * fabs(v)) * lam_x<OP>(a, k, j, a.nx);
-        }
+    for (int p = 0; p < 7; ++p) w.lw[p] = l.lam(wrapc(c0 - 3 + p, n));
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        const int f = wrapc(c0 - 2 + t, n);
+        w.vr[t] = l.vel(f);
+        w.vl[t] = (L::dup && f == 0) ? l.vel(n) : w.vr[t];
     }
-    return acc;
 }
 
-template <class OP>
-__device__ real field_y(const Args& a, int k, int j, int i)
+// dL/dq at the window's cell: face c0-2+t reads it as stencil cell
+// o = 5 - t, with Lambda = rd (vr lam_right - vl lam_left) split into its
+// v and |v| parts.
+template <class L>
+__device__ __forceinline__ real win_field(const L& l, const Win& w)
 {
     real acc = R(0.0);
 #pragma unroll
     for (int o = 0; o < 6; ++o) {
-        const int g = pmod(j + 3 - o, a.ny);
-        real v = OP::vely(a, k, g, i);
-        acc += (c5(o) * v + s5(o) * fabs(v)) * lam_y<OP>(a, k, g, i);
-        if (OP::dupy && g == 0) {
-            v = OP::vely(a, k, a.ny, i);
-            acc += (c5(o) * v + s5(o) * fabs(v)) * lam_y<OP>(a, k, a.ny, i);
-        }
+        const int t = 5 - o;
+        const real lr = w.lw[t + 1], ll = w.lw[t];
+        acc += c5(o) * (w.vr[t] * lr - w.vl[t] * ll)
+               + s5(o) * (fabs(w.vr[t]) * lr - fabs(w.vl[t]) * ll);
     }
-    return acc;
+    return l.rd() * acc;
 }
 
-// dL/dvel of x face f on the line (k, j)
-template <class OP>
-__device__ real dvel_x(const Args& a, int k, int j, int f)
+// dL/dvel of a face with velocity v, cell adjoints lr / ll on its sides
+// (0 where the face has no such cell) and stencil q[f-3 .. f+2].
+template <class L>
+__device__ __forceinline__ real face_dvel(const L& l, int f, real v, real lr,
+                                          real ll)
 {
-    const real s = sgn(OP::velx(a, k, j, f));
+    const int n = l.n();
+    const real s = sgn(v);
     real acc = R(0.0);
 #pragma unroll
     for (int o = 0; o < 6; ++o)
-        acc += (c5(o) + s5(o) * s) * OP::q(a, k, j, pmod(f + o - 3, a.nx));
-    return acc * lam_x<OP>(a, k, j, f);
+        acc += (c5(o) + s5(o) * s) * l.q(wrapc(f + o - 3, n));
+    return l.rd() * (lr - ll) * acc;
 }
 
-template <class OP>
-__device__ real dvel_y(const Args& a, int k, int g, int i)
+// dup line, face f in [0, n], from the window at c0 = f mod n
+template <class L>
+__device__ __forceinline__ real win_dvel_dup(const L& l, int f, int c0,
+                                             const Win& w)
 {
-    const real s = sgn(OP::vely(a, k, g, i));
-    real acc = R(0.0);
+    const int n = l.n();
+    const real v = (f < n) ? w.vr[2] : w.vl[2];
+    const real lr = (f < n) ? w.lw[3] : R(0.0);
+    const real ll = (f > 0) ? w.lw[2] : R(0.0);
+    return face_dvel(l, c0, v, lr, ll);
+}
+
+// dup line, face f in [0, n], without a window (a neighbouring line)
+template <class L>
+__device__ __forceinline__ real dvel_dup(const L& l, int f)
+{
+    const int n = l.n();
+    const real lr = (f < n) ? l.lam(f) : R(0.0);
+    const real ll = (f > 0) ? l.lam(f - 1) : R(0.0);
+    return face_dvel(l, f == n ? 0 : f, l.vel(f), lr, ll);
+}
+
+// periodic line: dL/dvel of faces c0 and c0+1 from the window at c0, with
+// one shared stencil q[c0-3 .. c0+3]
+template <class L>
+__device__ __forceinline__ void win_dvel_pair(const L& l, int c0,
+                                              const Win& w, real& d0,
+                                              real& d1)
+{
+    const int n = l.n();
+    real qw[7];
 #pragma unroll
-    for (int o = 0; o < 6; ++o)
-        acc += (c5(o) + s5(o) * s) * OP::q(a, k, pmod(g + o - 3, a.ny), i);
-    return acc * lam_y<OP>(a, k, g, i);
+    for (int p = 0; p < 7; ++p) qw[p] = l.q(wrapc(c0 - 3 + p, n));
+    const real s0 = sgn(w.vr[2]), s1 = sgn(w.vr[3]);
+    real a0 = R(0.0), a1 = R(0.0);
+#pragma unroll
+    for (int o = 0; o < 6; ++o) {
+        a0 += (c5(o) + s5(o) * s0) * qw[o];
+        a1 += (c5(o) + s5(o) * s1) * qw[o + 1];
+    }
+    d0 = l.rd() * (w.lw[3] - w.lw[2]) * a0;
+    d1 = l.rd() * (w.lw[4] - w.lw[3]) * a1;
 }
 
 // ---------------------------------------------------------------------------
@@ -384,104 +439,196 @@ __device__ real dvel_zw(const Args& a, int m, int j, int i)
 }
 
 // ---------------------------------------------------------------------------
-// Gathers of the mass-flux gradients: each ru / rv / rw point collects the
-// velocity adjoints of the faces whose velocity it enters, with the weight
-// it has there.
+// The kernels: one thread per (k, j, i) of the union of the staggers
+// (i <= nx, j <= ny, k <= nz).  A thread gathers the field gradient of its
+// cell and the mass-flux gradients of its ru / rv / rw points; each mass
+// flux collects the face velocity adjoints of the faces whose velocity it
+// enters, with its weight there.  Every gradient is added into its output.
 // ---------------------------------------------------------------------------
 
-template <class OP>
-__device__ real grad_ru(const Args& a, int k, int j, int f)
-{
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    if (OP::wlev) {                 // velx(k) = fnm[k] ru[k] + fnp[k] ru[k-1]
-        real g = R(0.0);
-        if (k >= 1) g += a.fnm[k] * dvel_x<OP>(a, k, j, f);
-        if (k + 1 <= nz - 1) g += a.fnp[k + 1] * dvel_x<OP>(a, k + 1, j, f);
-        return g;
-    }
-    if (!OP::dupx) {                // mass centres f-1 and f (faces f, f+1)
-        real g = R(0.0);
-        if (f < nx) g += dvel_x<OP>(a, k, j, f + 1 == nx ? 0 : f + 1);
-        if (f >= 1) g += dvel_x<OP>(a, k, j, f == nx ? 0 : f);
-        return R(0.5) * g;
-    }
-    if (OP::dupy && !OP::wlev)      // scalar: the face itself
-        return dvel_x<OP>(a, k, j, f);
-    // v: x faces of rows j and j+1 average this row with its neighbour
-    return R(0.5) * (dvel_x<OP>(a, k, j, f)
-                     + dvel_x<OP>(a, k, j + 1 == ny ? 0 : j + 1, f));
-}
-
-template <class OP>
-__device__ real grad_rv(const Args& a, int k, int g, int i)
-{
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    if (OP::wlev) {
-        real r = R(0.0);
-        if (k >= 1) r += a.fnm[k] * dvel_y<OP>(a, k, g, i);
-        if (k + 1 <= nz - 1) r += a.fnp[k + 1] * dvel_y<OP>(a, k + 1, g, i);
-        return r;
-    }
-    if (!OP::dupy) {
-        real r = R(0.0);
-        if (g < ny) r += dvel_y<OP>(a, k, g + 1 == ny ? 0 : g + 1, i);
-        if (g >= 1) r += dvel_y<OP>(a, k, g == ny ? 0 : g, i);
-        return R(0.5) * r;
-    }
-    if (OP::dupx)                   // scalar
-        return dvel_y<OP>(a, k, g, i);
-    // u: y faces of columns i and i+1
-    return R(0.5) * (dvel_y<OP>(a, k, g, i)
-                     + dvel_y<OP>(a, k, g, i + 1 == nx ? 0 : i + 1));
-}
-
-template <class OP>
-__device__ real grad_rw(const Args& a, int kf, int j, int i)
-{
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    if (OP::wlev) {
-        real r = R(0.0);
-        if (kf <= nz - 1) r += dvel_zw(a, kf, j, i);
-        if (kf >= 1) r += dvel_zw(a, kf - 1, j, i);
-        return R(0.5) * r;
-    }
-    if (OP::dupx && OP::dupy)       // scalar
-        return dvel_zh<OP>(a, kf, j, i);
-    if (!OP::dupx)                  // u: columns i and i+1
-        return R(0.5) * (dvel_zh<OP>(a, kf, j, i)
-                         + dvel_zh<OP>(a, kf, j, i + 1 == nx ? 0 : i + 1));
-    return R(0.5) * (dvel_zh<OP>(a, kf, j, i)        // v: rows j and j+1
-                     + dvel_zh<OP>(a, kf, j + 1 == ny ? 0 : j + 1, i));
-}
-
-// One thread per (k, j, i) of the union of the staggers (i <= nx,
-// j <= ny, k <= nz); every gradient is added into its output.
-template <class OP>
-__global__ void k_flux_div_adj(Args a)
+// scalar: faces are the mass-flux points themselves
+__global__ void k_adj_scalar(Args a)
 {
     const int nz = a.nz, ny = a.ny, nx = a.nx;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
     if (i > nx || j > ny || k > nz) return;
-    if (a.gq && i < nx && j < ny) {
-        if (OP::wlev) {
-            real g = field_zw(a, k, j, i);
-            if (k >= 1 && k <= nz - 1)
-                g += field_x<OP>(a, k, j, i) + field_y<OP>(a, k, j, i);
-            a.gq[OP::qidx(a, k, j, i)] += g;
-        } else if (k < nz) {
-            a.gq[OP::qidx(a, k, j, i)] += field_x<OP>(a, k, j, i)
-                                          + field_y<OP>(a, k, j, i)
-                                          + field_zh<OP>(a, k, j, i);
+    const int ic = (i == nx) ? 0 : i, jc = (j == ny) ? 0 : j;
+    real gq = R(0.0);
+    if (k < nz && j < ny) {
+        XLine<OpScalar> l{a, k, j};
+        Win w;
+        load_win(l, ic, w);
+        if (i < nx) gq += win_field(l, w);
+        if (a.gru) a.gru[iU(k, j, i, ny, nx)] += win_dvel_dup(l, i, ic, w);
+    }
+    if (k < nz && i < nx) {
+        YLine<OpScalar> l{a, k, i};
+        Win w;
+        load_win(l, jc, w);
+        if (j < ny) gq += win_field(l, w);
+        if (a.grv) a.grv[iV(k, j, i, ny, nx)] += win_dvel_dup(l, j, jc, w);
+    }
+    if (i < nx && j < ny) {
+        if (a.gq && k < nz)
+            a.gq[iM(k, j, i, ny, nx)] += gq + field_zh<OpScalar>(a, k, j, i);
+        if (a.grw) a.grw[iM(k, j, i, ny, nx)] += dvel_zh<OpScalar>(a, k, j, i);
+    }
+}
+
+// u: x faces at mass centres (periodic, velocity 0.5 (ru[m] + ru[m+1]));
+// y and z faces average rv / rw over the columns i-1 and i
+__global__ void k_adj_u(Args a)
+{
+    const int nz = a.nz, ny = a.ny, nx = a.nx;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    const int ic = (i == nx) ? 0 : i, jc = (j == ny) ? 0 : j;
+    const int ip = (i + 1 >= nx) ? i + 1 - nx : i + 1;
+    real gq = R(0.0);
+    if (k < nz && j < ny) {
+        XLine<OpU> l{a, k, j};
+        Win w;
+        load_win(l, ic, w);
+        if (i < nx) gq += win_field(l, w);
+        if (a.gru) {           // ru[i] enters mass centres i-1 and i
+            real d0, d1;
+            win_dvel_pair(l, ic, w, d0, d1);
+            a.gru[iU(k, j, i, ny, nx)] +=
+                R(0.5) * ((i < nx ? d1 : R(0.0)) + (i >= 1 ? d0 : R(0.0)));
         }
     }
-    if (a.gru && j < ny && k < nz)
-        a.gru[iU(k, j, i, ny, nx)] += grad_ru<OP>(a, k, j, i);
-    if (a.grv && i < nx && k < nz)
-        a.grv[iV(k, j, i, ny, nx)] += grad_rv<OP>(a, k, j, i);
-    if (a.grw && i < nx && j < ny)
-        a.grw[iM(k, j, i, ny, nx)] += grad_rw<OP>(a, k, j, i);
+    if (k < nz && i < nx) {
+        YLine<OpU> l{a, k, i};
+        Win w;
+        load_win(l, jc, w);
+        if (j < ny) gq += win_field(l, w);
+        if (a.grv)             // rv[., i] enters the y faces of u columns i, i+1
+            a.grv[iV(k, j, i, ny, nx)] +=
+                R(0.5) * (win_dvel_dup(l, j, jc, w)
+                          + dvel_dup(YLine<OpU>{a, k, ip}, j));
+    }
+    if (i < nx && j < ny) {
+        if (a.gq && k < nz)
+            a.gq[iU(k, j, i, ny, nx)] += gq + field_zh<OpU>(a, k, j, i);
+        if (a.grw)
+            a.grw[iM(k, j, i, ny, nx)] +=
+                R(0.5) * (dvel_zh<OpU>(a, k, j, i) + dvel_zh<OpU>(a, k, j, ip));
+    }
+}
+
+// v: the mirror of u
+__global__ void k_adj_v(Args a)
+{
+    const int nz = a.nz, ny = a.ny, nx = a.nx;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    const int ic = (i == nx) ? 0 : i, jc = (j == ny) ? 0 : j;
+    const int jp = (j + 1 >= ny) ? j + 1 - ny : j + 1;
+    real gq = R(0.0);
+    if (k < nz && j < ny) {
+        XLine<OpV> l{a, k, j};
+        Win w;
+        load_win(l, ic, w);
+        if (i < nx) gq += win_field(l, w);
+        if (a.gru)             // ru[j, .] enters the x faces of v rows j, j+1
+            a.gru[iU(k, j, i, ny, nx)] +=
+                R(0.5) * (win_dvel_dup(l, i, ic, w)
+                          + dvel_dup(XLine<OpV>{a, k, jp}, i));
+    }
+    if (k < nz && i < nx) {
+        YLine<OpV> l{a, k, i};
+        Win w;
+        load_win(l, jc, w);
+        if (j < ny) gq += win_field(l, w);
+        if (a.grv) {           // rv[j] enters mass centres j-1 and j
+            real d0, d1;
+            win_dvel_pair(l, jc, w, d0, d1);
+            a.grv[iV(k, j, i, ny, nx)] +=
+                R(0.5) * ((j < ny ? d1 : R(0.0)) + (j >= 1 ? d0 : R(0.0)));
+        }
+    }
+    if (i < nx && j < ny) {
+        if (a.gq && k < nz)
+            a.gq[iV(k, j, i, ny, nx)] += gq + field_zh<OpV>(a, k, j, i);
+        if (a.grw)
+            a.grw[iM(k, j, i, ny, nx)] +=
+                R(0.5) * (dvel_zh<OpV>(a, k, j, i) + dvel_zh<OpV>(a, k, jp, i));
+    }
+}
+
+// w: horizontal faces at levels 1..nz-1 with velocities fnm[k] r(k) +
+// fnp[k] r(k-1); vertical faces at mass levels
+__global__ void k_adj_w(Args a)
+{
+    const int nz = a.nz, ny = a.ny, nx = a.nx;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    const int ic = (i == nx) ? 0 : i, jc = (j == ny) ? 0 : j;
+    const bool own = (k >= 1 && k <= nz - 1);      // level k has faces
+    const bool up = (k + 1 <= nz - 1);             // so has level k+1
+    real gq = R(0.0);
+    if (j < ny && k < nz) {
+        real g = R(0.0);
+        if (own) {
+            XLine<OpW> l{a, k, j};
+            Win w;
+            load_win(l, ic, w);
+            if (i < nx) gq += win_field(l, w);
+            g = a.fnm[k] * win_dvel_dup(l, i, ic, w);
+        }
+        if (a.gru) {           // ru[k] enters levels k (fnm) and k+1 (fnp)
+            if (up) g += a.fnp[k + 1] * dvel_dup(XLine<OpW>{a, k + 1, j}, i);
+            a.gru[iU(k, j, i, ny, nx)] += g;
+        }
+    }
+    if (i < nx && k < nz) {
+        real g = R(0.0);
+        if (own) {
+            YLine<OpW> l{a, k, i};
+            Win w;
+            load_win(l, jc, w);
+            if (j < ny) gq += win_field(l, w);
+            g = a.fnm[k] * win_dvel_dup(l, j, jc, w);
+        }
+        if (a.grv) {
+            if (up) g += a.fnp[k + 1] * dvel_dup(YLine<OpW>{a, k + 1, i}, j);
+            a.grv[iV(k, j, i, ny, nx)] += g;
+        }
+    }
+    if (i < nx && j < ny) {
+        if (a.gq) a.gq[iM(k, j, i, ny, nx)] += gq + field_zw(a, k, j, i);
+        if (a.grw) {           // rw[kf] enters the faces at m = kf-1 and kf
+            real r = R(0.0);
+            if (k <= nz - 1) r += dvel_zw(a, k, j, i);
+            if (k >= 1) r += dvel_zw(a, k - 1, j, i);
+            a.grw[iM(k, j, i, ny, nx)] += R(0.5) * r;
+        }
+    }
+}
+
+void launch_adj(OpScalar, dim3 g, hipStream_t s, const Args& a)
+{
+    hipLaunchKernelGGL(k_adj_scalar, g, dim3(64, 4), 0, s, a);
+}
+void launch_adj(OpU, dim3 g, hipStream_t s, const Args& a)
+{
+    hipLaunchKernelGGL(k_adj_u, g, dim3(64, 4), 0, s, a);
+}
+void launch_adj(OpV, dim3 g, hipStream_t s, const Args& a)
+{
+    hipLaunchKernelGGL(k_adj_v, g, dim3(64, 4), 0, s, a);
+}
+void launch_adj(OpW, dim3 g, hipStream_t s, const Args& a)
+{
+    hipLaunchKernelGGL(k_adj_w, g, dim3(64, 4), 0, s, a);
 }
 
 template <class OP>
@@ -501,9 +648,8 @@ int32_t flux_div_adjoint(const real* q, const real* ru, const real* rv,
     Args a{q, ru, rv, rw, lam, rdz, fnm, fnp, g_q, g_ru, g_rv, g_rw,
            dx_inv, dy_inv, nz, ny, nx};
     dim3 grid((nx + 1 + 63) / 64, (ny + 1 + 3) / 4, nz + 1);
-    hipLaunchKernelGGL(k_flux_div_adj<OP>, grid, dim3(64, 4), 0,
-                       (hipStream_t)stream, a);
-    return lib_launched(hipGetLastError(), "k_flux_div_adj");
+    launch_adj(OP{}, grid, (hipStream_t)stream, a);
+    return lib_launched(hipGetLastError(), "k_adj (advection adjoint)");
 }
 
 }  // namespace

@@ -231,7 +231,8 @@ struct Arr3 {
     }
 };
 
-__global__ void k_flux_div(int which, const real* __restrict__ q,
+template <int WHICH>
+__global__ void k_flux_div(const real* __restrict__ q,
                            const real* __restrict__ ru,
                            const real* __restrict__ rv,
                            const real* __restrict__ rw,
@@ -245,18 +246,18 @@ __global__ void k_flux_div(int which, const real* __restrict__ q,
     int j = blockIdx.y * blockDim.y + threadIdx.y;
     int k = blockIdx.z;
     Arr3 RU{ru, ny, nx + 1}, RV{rv, ny + 1, nx}, RW{rw, ny, nx};
-    if (which == 0) {
+    if (WHICH == 0) {
         if (i >= nx || j >= ny || k >= nz) return;
         Arr3 Q{q, ny, nx};
         tend[iM(k, j, i, ny, nx)] += adv_scalar(Q, RU, RV, RW, k, j, i, nz,
                                                 ny, nx, rdz, fnm, fnp, rdx,
                                                 rdy);
-    } else if (which == 1) {
+    } else if (WHICH == 1) {
         if (i > nx || j >= ny || k >= nz) return;
         Arr3 Q{q, ny, nx + 1};
         tend[iU(k, j, i, ny, nx)] += adv_u(Q, RU, RV, RW, k, j, i, nz, ny,
                                            nx, rdz, fnm, fnp, rdx, rdy);
-    } else if (which == 2) {
+    } else if (WHICH == 2) {
         if (i >= nx || j > ny || k >= nz) return;
         Arr3 Q{q, ny + 1, nx};
         tend[iV(k, j, i, ny, nx)] += adv_v(Q, RU, RV, RW, k, j, i, nz, ny,
@@ -1075,10 +1076,27 @@ static int32_t flux_div(int which, const real* q, const real* ru,
     if (!q || !ru || !rv || !rw || !tend || !rdz || !fnm || !fnp)
         return fail(ARW_E_ARG, "null argument");
     if (nz < 1 || ny < 1 || nx < 1) return fail(ARW_E_ARG, "bad extents");
+    if ((int64_t)(nz + 1) * (ny + 1) * (nx + 1) >= ((int64_t)1 << 31))
+        return fail(ARW_E_ARG, "extents exceed 32-bit indexing");
     dim3 grid((nx + 1 + 63) / 64, (ny + 1 + 3) / 4, nz + 1);
-    hipLaunchKernelGGL(k_flux_div, grid, dim3(64, 4), 0, (hipStream_t)stream,
-                       which, q, ru, rv, rw, tend, rdz, fnm, fnp, dx_inv,
-                       dy_inv, nz, ny, nx);
+    hipStream_t st = (hipStream_t)stream;
+    switch (which) {
+    case 0:
+        hipLaunchKernelGGL(k_flux_div<0>, grid, dim3(64, 4), 0, st, q, ru, rv,
+                           rw, tend, rdz, fnm, fnp, dx_inv, dy_inv, nz, ny, nx);
+        break;
+    case 1:
+        hipLaunchKernelGGL(k_flux_div<1>, grid, dim3(64, 4), 0, st, q, ru, rv,
+                           rw, tend, rdz, fnm, fnp, dx_inv, dy_inv, nz, ny, nx);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_flux_div<2>, grid, dim3(64, 4), 0, st, q, ru, rv,
+                           rw, tend, rdz, fnm, fnp, dx_inv, dy_inv, nz, ny, nx);
+        break;
+    default:
+        hipLaunchKernelGGL(k_flux_div<3>, grid, dim3(64, 4), 0, st, q, ru, rv,
+                           rw, tend, rdz, fnm, fnp, dx_inv, dy_inv, nz, ny, nx);
+    }
     return launched("k_flux_div");
 }
 
