@@ -12,7 +12,7 @@ from pathlib import Path
 LIB_NAME = "libarwen_dycore.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -46,7 +46,7 @@ class ArwDomain(ctypes.Structure):
                 + [(n, _P) for n in DIFF_ARRAYS]
                 + [(n, _P) for n in MOIST_ARRAYS]
                 + [(n, _P) for n in MOIST_SCRATCH]
-                + [("tri64", _P)])
+                + [("tri64", _P), ("pp", _P)])
 
 
 class ArwDomain64(ctypes.Structure):

@@ -30,7 +30,8 @@ import torch
 
 from . import _lib
 from .config import RunConfig, validate_for_step
-from .dycore import _check_state, _fn, _stream, _uses_diffusion, params
+from .dycore import (_check_state, _dom, _fn, _stream, _uses_diffusion,
+                     params)
 from .state import DomainState
 
 
@@ -92,7 +93,7 @@ def _tracer_call(state, cfg, ckpt, kind, src, dst, clamp=False):
     n = src.numel()
     work = state.scratch((3 * n,), "arw_tracer_work")   # J uses 2n, J^T 3n
     name, fn = _fn(state, kind)
-    d, p = state.domain_struct(), params(cfg)
+    d, p = _dom(state, cfg), params(cfg)
     if kind == "tracer_step":
         rc = fn(d, p, ctypes.byref(ckpt.c), src.data_ptr(), dst.data_ptr(),
                 work.data_ptr(), int(bool(clamp)), _stream())
@@ -197,7 +198,7 @@ def acoustic_substep_adjoint(state: DomainState, cfg: RunConfig,
         out[name] = torch.empty_like(getattr(state, name))
         setattr(a, "g_" + name, out[name].data_ptr())
     fn = getattr(lib, pre + "acoustic_substep_adjoint")
-    _lib.check(fn(state.domain_struct(), params(cfg), float(dtau),
+    _lib.check(fn(_dom(state, cfg), params(cfg), float(dtau),
                   int(bool(first)), ctypes.byref(a), work.data_ptr(),
                   _stream()), pre + "acoustic_substep_adjoint")
     return out

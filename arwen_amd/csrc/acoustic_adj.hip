@@ -564,6 +564,7 @@ int32_t ARW_API(acoustic_substep_adjoint)(const Dom* d, const arw_params* prm,
     int32_t rc = lib_check_domain(d);
     if (rc) return rc;
     if (!prm || !adj || !work) return lib_fail(ARW_E_ARG, "null argument");
+    if ((rc = lib_check_prec(prm, d))) return rc;
     const real* lin[] = {adj->l_u_pp, adj->l_v_pp, adj->l_w_pp, adj->l_th_pp,
                          adj->l_ph_pp, adj->l_mu_pp, adj->l_p_pp,
                          adj->l_p_pp_old, adj->l_al_pp};

@@ -151,6 +151,36 @@ __device__ __forceinline__ real pow_gamma(real x)
     return (real)pow((double)x, (double)GAMMA_F);
 }
 
+// The EOS of one cell in perturbation-from-base form (the precision form).
+// The base state is discretely hydrostatic (alb = -rdnw dphb / (c1h mub +
+// c2h)) and in its own EOS (pb = P0 (RD thb / (P0 alb))^GAMMA), so with
+// mh = c1h (mub + mu') + c2h
+//   al = -(alb c1h mu' + rdnw dphp) / mh,  alt = alb + al,
+//   p' = pb expm1(GAMMA (log1p(theta'/thb) [+ log1p(RVOVRD qv)]
+//                        - log1p(al/alb))),  p = pb + p',
+// which never differences the full geopotential nor subtracts pb from a
+// full pressure.  The logarithms and the exponential run in double and p'
+// is rounded once.
+struct EosPert {
+    real al, alt, pp, p;
+};
+
+__device__ __forceinline__ EosPert eos_pert(real thp, real thb, real qv,
+                                            bool moist, real alb, real pb,
+                                            real mup, real mh, real c1hk,
+                                            real rdnwk, real dphp)
+{
+    EosPert o;
+    o.al = -((alb * (c1hk * mup)) + rdnwk * dphp) / mh;
+    o.alt = alb + o.al;
+    double x = log1p((double)thp / (double)thb);
+    if (moist) x = x + log1p((double)RVOVRD_F * (double)qv);
+    const double y = log1p((double)o.al / (double)alb);
+    o.pp = (real)((double)pb * expm1((double)GAMMA_F * (x - y)));
+    o.p = pb + o.pp;
+    return o;
+}
+
 // Base-profile value: 1-D column (base3d = 0) or per-point 3-D field.
 __device__ __forceinline__ real basev(const real* __restrict__ b, int k,
                                        int c, int st, int base3d)

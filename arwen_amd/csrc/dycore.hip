@@ -290,13 +290,33 @@ __device__ __forceinline__ void eos_level(const Dom& d, int k, int c, int st,
     d.p[k * st + c] = P0_F * pow_gamma((RD_F * th) / (P0_F * alt));
 }
 
+// The precision form's EOS of one cell (eos_pert), p' kept in d.pp.
+__device__ __forceinline__ void eos_level_pert(const Dom& d, int k, int c,
+                                               int st, real mu)
+{
+    const int x = k * st + c;
+    const EosPert e = eos_pert(
+        d.thp[x], basev(d.thb, k, c, st, d.base3d), d.qv ? d.qv[x] : R(0.0),
+        d.qv != nullptr, basev(d.alb, k, c, st, d.base3d),
+        basev(d.pb, k, c, st, d.base3d), d.mup[c], d.c1h[k] * mu + d.c2h[k],
+        d.c1h[k], d.rdnw[k], d.php[x + st] - d.php[x]);
+    d.alt[x] = e.alt;
+    d.al[x] = e.al;
+    d.p[x] = e.p;
+    d.pp[x] = e.pp;
+}
+
+template <bool PREC>
 __global__ void k_eos(Dom d)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     int st = d.ny * d.nx;
     if (c >= st) return;
     real mu = mut_at(d, c);
-    for (int k = 0; k < d.nz; ++k) eos_level(d, k, c, st, mu);
+    for (int k = 0; k < d.nz; ++k) {
+        if (PREC) eos_level_pert(d, k, c, st, mu);
+        else eos_level(d, k, c, st, mu);
+    }
 }
 
 template <class SRC>
@@ -852,9 +872,21 @@ int32_t check_domain(const Dom* d)
     return ARW_OK;
 }
 
+// The precision form's extra arrays (the float32 build only).
+int32_t check_prec(const arw_params* p, const Dom* d)
+{
+    if (ARW_F64 || !p || !p->precision_form) return ARW_OK;
+    if (!d->tri64)
+        return fail(ARW_E_ARG, "precision form needs the FP64 tridiagonal scratch tri64");
+    if (!d->pp)
+        return fail(ARW_E_ARG, "precision form needs the perturbation pressure array pp");
+    return ARW_OK;
+}
+
 int32_t check_params(const arw_params* p, const Dom* d)
 {
     if (!p) return fail(ARW_E_ARG, "null params");
+    if (int32_t rc = check_prec(p, d)) return rc;
     if (!(p->dx > 0.0) || !(p->dy > 0.0) || !(p->dt > 0.0))
         return fail(ARW_E_ARG, "dx, dy, dt must be positive");
     if (p->time_step_sound < 1 || p->time_step_sound % 2)
@@ -1006,7 +1038,7 @@ int32_t ens_issue(const EnsGroup& G, const arw_params* prm, bool b3,
             if ((rc = launched(run_column_b(DS, sn, b3, b3 || last, st), "k_column (ensemble)", -1, st)))
                 return rc;
         }
-        if ((rc = launched(run_finish_b(DD, b3, st), "k_finish (ensemble)", -1, st)))
+        if ((rc = launched(run_finish_b(DD, b3, s.prec != 0, st), "k_finish (ensemble)", -1, st)))
             return rc;
     }
     Sc s = make_sc(prm, prm->dt, 1);
@@ -1024,6 +1056,10 @@ int32_t lib_launched(hipError_t e, const char* what)
     return launched(e, what, -1, nullptr);
 }
 int32_t lib_check_domain(const Dom* d) { return check_domain(d); }
+int32_t lib_check_prec(const arw_params* p, const Dom* d)
+{
+    return check_prec(p, d);
+}
 
 }  // namespace ARW_NS
 
@@ -1048,7 +1084,7 @@ int32_t ARW_API(calc_p_alpha)(const Dom* d, void* stream)
 {
     int32_t rc = check_domain(d);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_eos, col_grid(*d), dim3(256), 0,
+    hipLaunchKernelGGL(k_eos<false>, col_grid(*d), dim3(256), 0,
                        (hipStream_t)stream, *d);
     return launched("k_eos");
 }
@@ -1059,6 +1095,7 @@ int32_t ARW_API(stage_fluxes)(const Dom* d, const arw_params* prm,
     int32_t rc = check_domain(d);
     if (rc) return rc;
     if (!prm || !ru || !rv) return fail(ARW_E_ARG, "null argument");
+    if ((rc = check_prec(prm, d))) return rc;
     Sc s = make_sc(prm, prm->dt, 1);
     hipStream_t st = (hipStream_t)stream;
     if ((rc = launched(run_omega(*d, s, 0, st), "k_omega", -1, st))) return rc;
@@ -1163,6 +1200,7 @@ int32_t ARW_API(calc_coefs)(const Dom* d, const arw_params* prm,
     int32_t rc = check_domain(d);
     if (rc) return rc;
     if (!prm) return fail(ARW_E_ARG, "null params");
+    if ((rc = check_prec(prm, d))) return rc;
     // The implicit-w factors are formed inside every substep from the
     // stage reference p/alt; what persists across substeps is the
     // reference mass divergence (and Omega), formed here.
@@ -1177,6 +1215,7 @@ int32_t ARW_API(acoustic_substep)(const Dom* d, const arw_params* prm,
     int32_t rc = check_domain(d);
     if (rc) return rc;
     if (!prm) return fail(ARW_E_ARG, "null params");
+    if ((rc = check_prec(prm, d))) return rc;
     Sc s = make_sc(prm, dtau, first);
     s.use_pold = 1;
     hipStream_t st = (hipStream_t)stream;
@@ -1195,7 +1234,7 @@ int32_t ARW_API(small_step_finish)(const Dom* d, void* stream)
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     FinishDst o = {d->u, d->v, d->w, d->thp, d->php, d->mup_next};
-    if ((rc = launched(run_finish(*d, o, st), "k_finish", -1, st))) return rc;
+    if ((rc = launched(run_finish(*d, o, false, st), "k_finish", -1, st))) return rc;
     if (hipMemcpyAsync(d->mup, d->mup_next,
                        (size_t)d->ny * d->nx * sizeof(real),
                        hipMemcpyDeviceToDevice, st) != hipSuccess)
@@ -1231,8 +1270,7 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
     int32_t rc = check_domain(d);
     if (rc) return rc;
     if ((rc = check_params(prm, d))) return rc;
-    if (!ARW_F64 && prm->precision_form && !d->tri64)
-        return fail(ARW_E_ARG, "precision form needs the FP64 tridiagonal scratch tri64");
+    const bool prec = (!ARW_F64) && prm->precision_form != 0;
     if (ck) {
         if (!ck->mup_t) return fail(ARW_E_ARG, "checkpoint: null mup_t");
         for (int s = 0; s < 3; ++s)
@@ -1323,14 +1361,17 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
                                    nsub[stage] * dtaus[stage], stage == 2, st)))
                 return rc;
         }
-        if ((rc = launched(run_finish(dd, o, st), "k_finish", KID_FINISH, st)))
+        if ((rc = launched(run_finish(dd, o, prec, st), "k_finish", KID_FINISH, st)))
             return rc;
         if (moist) {
             // moist EOS of the new stage state (needs its updated qv)
             Dom de = dd;
             de.thp = o.thp; de.php = o.php; de.mup = o.mup;
             de.qv = (stage == 0) ? QB[0] : (stage == 1 ? QC[0] : QA[0]);
-            hipLaunchKernelGGL(k_eos, col_grid(*d), dim3(256), 0, st, de);
+            if (prec)
+                hipLaunchKernelGGL(k_eos<true>, col_grid(*d), dim3(256), 0, st, de);
+            else
+                hipLaunchKernelGGL(k_eos<false>, col_grid(*d), dim3(256), 0, st, de);
             if ((rc = launched(hipGetLastError(), "k_eos", KID_SCALARS, st)))
                 return rc;
         }
@@ -1443,8 +1484,6 @@ int32_t ARW_API(ensemble_create)(const Dom* members, int32_t n,
             return fail(ARW_E_ARG, "ensemble members must be dry");
         if ((members[m].base3d != 0) != (members[0].base3d != 0))
             return fail(ARW_E_ARG, "ensemble members must share base3d");
-        if (!ARW_F64 && prm->precision_form && !members[m].tri64)
-            return fail(ARW_E_ARG, "precision form needs tri64 on every member");
     }
     std::vector<int> ord(n);
     std::vector<char> seen(n, 0);

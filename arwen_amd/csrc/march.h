@@ -100,6 +100,7 @@ __device__ __forceinline__ const FinishDst& fin_of(const Batch& b, int m,
 int32_t lib_fail(int32_t code, const char* msg);
 int32_t lib_launched(hipError_t e, const char* what);
 int32_t lib_check_domain(const Dom* d);
+int32_t lib_check_prec(const arw_params* p, const Dom* d);
 
 hipError_t run_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st);
 hipError_t run_tend_faces(const Dom& d, const Sc& s, hipStream_t st);
@@ -107,7 +108,8 @@ hipError_t run_tend_cols(const Dom& d, const Sc& s, hipStream_t st);
 hipError_t run_uv(const Dom& d, const Sc& s, hipStream_t st);
 hipError_t run_column(const Dom& d, const Sc& s, real* pout, int write_al,
                       int write_ww, hipStream_t st);
-hipError_t run_finish(const Dom& d, const FinishDst& o, hipStream_t st);
+hipError_t run_finish(const Dom& d, const FinishDst& o, bool prec,
+                      hipStream_t st);
 
 // Batched forms (members share base3d and every flag in Sc).
 hipError_t run_omega_b(const BatchSet& b, const Sc& s, int do_eos, bool b3,
@@ -119,6 +121,6 @@ hipError_t run_tend_cols_b(const BatchSet& b, const Sc& s, bool b3,
 hipError_t run_uv_b(const BatchSet& b, const Sc& s, bool b3, hipStream_t st);
 hipError_t run_column_b(const BatchSet& b, const Sc& s, bool b3, int write_al,
                         hipStream_t st);
-hipError_t run_finish_b(const BatchSet& b, bool b3, hipStream_t st);
+hipError_t run_finish_b(const BatchSet& b, bool b3, bool prec, hipStream_t st);
 
 }  // namespace ARW_NS

@@ -89,7 +89,9 @@ __device__ __forceinline__ real pgf_val(
 // divergence (acoustic.cu:396-405): dnw*div(C(mu_face) u_t*) is the same
 // float32 expression, so one sum serves both.  Pass 1 parks each level's
 // divergence in ww; pass 2 turns it into Omega.  EOS: the stage-start
-// calc_p_alpha (diagnostics.cu:37-89) when the stage state is new.
+// calc_p_alpha (diagnostics.cu:37-89) when the stage state is new; in the
+// precision form (PREC) its perturbation form, eos_pert, which also keeps
+// p' in d.pp.
 template <class SRC, bool EOS, bool B3, bool PREC>
 __global__ __launch_bounds__(256) void k_omega(typename SRC::Arg a, Sc s)
 {
@@ -114,6 +116,8 @@ __global__ __launch_bounds__(256) void k_omega(typename SRC::Arg a, Sc s)
         kreal* thb1 = kview(d.thb);
         kreal* phb1 = kview(d.phb);
         kreal* alb1 = kview(d.alb);
+        kreal* pb1 = kview(d.pb);
+        const real mupc = d.mup[c];
         real ph_lo = gld(d.php, 0, bc);
         real gb_lo = B3 ? gld(d.phb, 0, bc) : phb1[0];
 #pragma unroll 2
@@ -122,14 +126,28 @@ __global__ __launch_bounds__(256) void k_omega(typename SRC::Arg a, Sc s)
             const unsigned h = k * st;
             const real ph_hi = gld(d.php, h + st, bc);
             const real gb_hi = B3 ? gld(d.phb, h + st, bc) : phb1[k + 1];
-            real th = (B3 ? gld(d.thb, h, bc) : thb1[k]) + gld(d.thp, h, bc);
-            if (d.qv) th = th * (R(1.0) + RVOVRD_F * gld(d.qv, h, bc));
             const real albk = B3 ? gld(d.alb, h, bc) : alb1[k];
-            real dph = (gb_hi + ph_hi) - (gb_lo + ph_lo);
-            real alt = (-dph * rdnw[k]) / (c1h[k] * ms + c2h[k]);
-            gst(d.alt, h, bc, alt);
-            gst(d.al, h, bc, alt - albk);
-            gst(d.p, h, bc, P0_F * pow_gamma((RD_F * th) / (P0_F * alt)));
+            if (PREC) {
+                const real thbk = B3 ? gld(d.thb, h, bc) : thb1[k];
+                const real pbk = B3 ? gld(d.pb, h, bc) : pb1[k];
+                const real qvk = d.qv ? gld(d.qv, h, bc) : R(0.0);
+                const EosPert e = eos_pert(gld(d.thp, h, bc), thbk, qvk,
+                                           d.qv != nullptr, albk, pbk, mupc,
+                                           c1h[k] * ms + c2h[k], c1h[k],
+                                           rdnw[k], ph_hi - ph_lo);
+                gst(d.alt, h, bc, e.alt);
+                gst(d.al, h, bc, e.al);
+                gst(d.p, h, bc, e.p);
+                gst(d.pp, h, bc, e.pp);
+            } else {
+                real th = (B3 ? gld(d.thb, h, bc) : thb1[k]) + gld(d.thp, h, bc);
+                if (d.qv) th = th * (R(1.0) + RVOVRD_F * gld(d.qv, h, bc));
+                real dph = (gb_hi + ph_hi) - (gb_lo + ph_lo);
+                real alt = (-dph * rdnw[k]) / (c1h[k] * ms + c2h[k]);
+                gst(d.alt, h, bc, alt);
+                gst(d.al, h, bc, alt - albk);
+                gst(d.p, h, bc, P0_F * pow_gamma((RD_F * th) / (P0_F * alt)));
+            }
             ph_lo = ph_hi;
             gb_lo = gb_hi;
         }
@@ -253,8 +271,9 @@ struct PgfWin {
 // advection (reference advection.cu:424-794), large-step PGF
 // (dycore.cu:98-157), constant-K diffusion (diffusion.cu:22-51), and the
 // u''/v'' small-step initialisation (dycore.cu:947-1010).  The periodic
-// duplicate faces are finished by k_tend_faces_edge.
-template <class SRC, bool B3, bool DIFF>
+// duplicate faces are finished by k_tend_faces_edge.  PREC (precision
+// form): the pressure gradient takes p' from the perturbation-form EOS.
+template <class SRC, bool B3, bool DIFF, bool PREC>
 __global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
 {
     int m, g;
@@ -329,6 +348,7 @@ __global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
     // PGF windows of A, W, S
     auto pp_at = [&](int k, int row, unsigned cb) {
         const int l = k * st + row;
+        if (PREC) return gld(d.pp, l, cb);      // the EOS's own p'
         return gld(d.p, l, cb) - (B3 ? gld(d.pb, l, cb) : kview(d.pb)[k]);
     };
     auto win_init = [&](PgfWin& X, int row, unsigned cb) {
@@ -559,7 +579,13 @@ __global__ __launch_bounds__(256) void k_tend_faces_edge(typename SRC::Arg a)
 // w''/phi''/theta''/p''/alpha'' small-step initialisation
 // (dycore.cu:1012-1085).  Row-segment waves; level kinds peel the surface
 // (k = 0) and the lid (k = nz) so the interior level body is one block.
-template <class SRC, bool B3, bool DIFF, bool MOIST>
+// PREC (precision form): buoyancy takes p' from the perturbation-form EOS;
+// rhs_ph's layer thickness is dphb + dphp with dphb = -(dnw (c1h mub +
+// c2h)) alb (the base state's hydrostatic relation) and, on a flat base,
+// its horizontal differences are those of php; theta'' is formed as
+// (c1h mu'') thb + (C(mu_t) thp_t - C(mu_s) thp_s), free of the base
+// theta's cancellation.
+template <class SRC, bool B3, bool DIFF, bool MOIST, bool PREC>
 __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
 {
     int m, g;
@@ -581,6 +607,8 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
     kreal* thb1 = kview(d.thb);
     kreal* phb1 = kview(d.phb);
     kreal* pb1 = kview(d.pb);
+    kreal* alb1 = kview(d.alb);
+    kreal* dnw = kview(d.dnw);
     kreal* rdzf_h = kview(d.rdzf_h);
     kreal* rdzc_h = kview(d.rdzc_h);
     kreal* rdzf_w = kview(d.rdzf_w);
@@ -626,6 +654,18 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
     auto ph_at = [&](int k, int row, unsigned cb) {
         return phb_at(k, row, cb) + gld(d.php, k * st + row, cb);
     };
+    // the geopotential the vertical window carries: full, or php alone in
+    // the precision form (whose rhs_ph takes the base part from alb)
+    auto phw_at = [&](int k, int row, unsigned cb) {
+        if (PREC) return gld(d.php, k * st + row, cb);
+        return ph_at(k, row, cb);
+    };
+    // horizontal neighbours of rhs_ph: php alone on a flat base in the
+    // precision form (phb is horizontally uniform there)
+    auto phh_at = [&](int k, int row, unsigned cb) {
+        if (PREC && !B3) return gld(d.php, k * st + row, cb);
+        return ph_at(k, row, cb);
+    };
 
     // windows
     real thz[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};   // theta k-2..k+2
@@ -635,7 +675,7 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
         wz[2 + kk] = gld(d.w, kk * st + rA, cC);
     }
     real ww_m = R(0.0), ww_0 = gld(d.ww, rA, cC), ww_p = gld(d.ww, st + rA, cC);
-    real ph_m = R(0.0), ph_0 = ph_at(0, rA, cC), ph_p = ph_at(1, rA, cC);
+    real ph_m = R(0.0), ph_0 = phw_at(0, rA, cC), ph_p = phw_at(1, rA, cC);
     real pp_m = R(0.0);
     real ru_i_m = R(0.0), ru_ip_m = R(0.0), rv_j_m = R(0.0), rv_jp_m = R(0.0);
     real u_i_m = R(0.0), u_i_mm = R(0.0), u_ip_m = R(0.0), u_ip_mm = R(0.0);
@@ -645,7 +685,12 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
     real phpp_0 = gld(d.php0, rA, cC) - gld(d.php, rA, cC);
     // moist: total water (qv + qc) + qr at the level below (buoyancy)
     real qt_m = R(0.0), qt_mm = R(0.0);
-    const real mubc = MOIST ? gld(mub, rA, cC) : R(0.0);
+    const real mubc = (MOIST || PREC) ? gld(mub, rA, cC) : R(0.0);
+    // precision form: base layer thickness at mass level k
+    auto dphb = [&](int k) {
+        const real albk = B3 ? gld(d.alb, k * st + rA, cC) : alb1[k];
+        return -(dnw[k] * (c1h[k] * mubc + c2h[k])) * albk;
+    };
 
     // kinds: 0 = surface level (w level 0 + mass level 0), 1 = interior,
     // 2 = lid (w level nz only)
@@ -662,7 +707,10 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
         real qt_0 = R(0.0);
         if (mass) {
             p_k = gld(d.p, lm + rA, cC);
-            pp_0 = p_k - (B3 ? gld(d.pb, lm + rA, cC) : pb1[k]);
+            if (PREC)
+                pp_0 = gld(d.pp, lm + rA, cC);
+            else
+                pp_0 = p_k - (B3 ? gld(d.pb, lm + rA, cC) : pb1[k]);
             if (MOIST)
                 qt_0 = (gld(d.qv, lm + rA, cC) + gld(d.qc, lm + rA, cC))
                        + gld(d.qr, lm + rA, cC);
@@ -759,8 +807,13 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
         real r = R(0.0);
         if (K != 0) {
             if (!top) {
-                real wd_lo = ((R(0.5) * (ww_0 + ww_m)) * rdnw[k - 1]) * (ph_0 - ph_m);
-                real wd_hi = ((R(0.5) * (ww_p + ww_0)) * rdnw[k]) * (ph_p - ph_0);
+                real dph_lo = ph_0 - ph_m, dph_hi = ph_p - ph_0;
+                if (PREC) {
+                    dph_lo = dphb(k - 1) + dph_lo;
+                    dph_hi = dphb(k) + dph_hi;
+                }
+                real wd_lo = ((R(0.5) * (ww_0 + ww_m)) * rdnw[k - 1]) * dph_lo;
+                real wd_hi = ((R(0.5) * (ww_p + ww_0)) * rdnw[k]) * dph_hi;
                 real omega = fnm[k] * wd_hi + fnp[k] * wd_lo;
                 r = r - omega;
             }
@@ -783,12 +836,13 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
             real fcx_ip = (c1f[k] * mux_ip + c2f[k]) * wip;
             real fcy_j = (c1f[k] * muy_j + c2f[k]) * wj;
             real fcy_jp = (c1f[k] * muy_jp + c2f[k]) * wjp;
-            real p0 = ph_0;
-            real fx = fcx_i * (p0 - ph_at(k, rA, cx[2]));
-            real fxp = fcx_ip * (ph_at(k, rA, cx[4]) - p0);
+            // the window holds php alone in the precision form
+            real p0 = (PREC && B3) ? phb_at(k, rA, cC) + ph_0 : ph_0;
+            real fx = fcx_i * (p0 - phh_at(k, rA, cx[2]));
+            real fxp = fcx_ip * (phh_at(k, rA, cx[4]) - p0);
             r = r - qxk * (fxp + fx);
-            real fy = fcy_j * (p0 - ph_at(k, rS, cC));
-            real fyp = fcy_jp * (ph_at(k, rN, cC) - p0);
+            real fy = fcy_j * (p0 - phh_at(k, rS, cC));
+            real fyp = fcy_jp * (phh_at(k, rN, cC) - p0);
             r = r - qyk * (fyp + fy);
         }
         gst(d.rph_t, lm + rA, cC, r);
@@ -837,9 +891,14 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
             real cht = c1h[k] * mt + c2h[k];
             real chs = c1h[k] * mA + c2h[k];
             real thbk = thb_at(k, rA, cC);
-            real tht = thbk + gld(d.thp0, lm + rA, cC);
+            const real thp0k = gld(d.thp0, lm + rA, cC);
+            real tht = thbk + thp0k;
             real ths = thbk + thp_0;
-            real thpp = cht * tht - chs * ths;
+            real thpp;
+            if (PREC)
+                thpp = (c1h[k] * mupp) * thbk + (cht * thp0k - chs * thp_0);
+            else
+                thpp = cht * tht - chs * ths;
             gst(d.th_pp, lm + rA, cC, thpp);
             real altk = gld(d.alt, lm + rA, cC);
             real c2a = (GAMMA_F * p_k) / altk;
@@ -861,7 +920,7 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
         const real ww_n = gld(d.ww, (k + 2 <= nz ? k + 2 : nz) * st + rA, cC);
         ww_m = ww_0; ww_0 = ww_p;
         ww_p = (k + 2 <= nz) ? ww_n : R(0.0);
-        const real ph_n = ph_at(k + 2 <= nz ? k + 2 : nz, rA, cC);
+        const real ph_n = phw_at(k + 2 <= nz ? k + 2 : nz, rA, cC);
         ph_m = ph_0; ph_0 = ph_p;
         ph_p = (k + 2 <= nz) ? ph_n : R(0.0);
         pp_m = pp_0;
@@ -1594,12 +1653,15 @@ struct FinIn {
     real w = R(0.0), wpp = R(0.0), php = R(0.0), phpp = R(0.0);
     real thp = R(0.0), thpp = R(0.0);
     real thb = R(0.0), phb = R(0.0), alb = R(0.0);   // terrain base
+    real pb = R(0.0);                                // terrain, PREC
 };
 
 // Small-step finish (reference dycore.cu:1087-1198) of the mass column
 // (j, i), its u face (j, i) and v face (j, i), then the EOS of the new
-// state (diagnostics.cu:37-89).
-template <class SRC, bool B3, bool EOS>
+// state (diagnostics.cu:37-89).  PREC (precision form): the new theta' is
+// (C(mu_s) thp + theta'' - (c1h mu'') thb) / C(mu_n), free of the base
+// theta's cancellation, and the EOS is eos_pert.
+template <class SRC, bool B3, bool EOS, bool PREC>
 __global__ __launch_bounds__(256) void k_finish(typename SRC::Arg a,
                                                 FinishDst o_arg)
 {
@@ -1622,10 +1684,12 @@ __global__ __launch_bounds__(256) void k_finish(typename SRC::Arg a,
     kreal* thb1 = kview(d.thb);
     kreal* phb1 = kview(d.phb);
     kreal* alb1 = kview(d.alb);
+    kreal* pb1 = kview(d.pb);
     const real* RS mub = d.mub2d;
     const real* RS mup = d.mup;
     const real* RS mu_pp = d.mu_pp;
     const real msA = mub[c] + mup[c], mnA = msA + mu_pp[c];
+    const real muppA = mu_pp[c];
     const int cW = j * nx + im, cS = jm * nx + i;
     const real msW = mub[cW] + mup[cW], mnW = msW + mu_pp[cW];
     const real msS = mub[cS] + mup[cS], mnS = msS + mu_pp[cS];
@@ -1651,10 +1715,11 @@ __global__ __launch_bounds__(256) void k_finish(typename SRC::Arg a,
             L.thb = gld(d.thb, km * st, bc);
             L.phb = gld(d.phb, k * st, bc);
             L.alb = gld(d.alb, km * st, bc);
+            if (PREC && EOS) L.pb = gld(d.pb, km * st, bc);
         }
     };
     real php_lo = R(0.0), thp_lo = R(0.0), thb_lo = R(0.0), phb_lo = R(0.0);
-    real alb_lo = R(0.0);
+    real alb_lo = R(0.0), pb_lo = R(0.0);
     FinIn nxt;
     load(nxt, 0);
     // kinds: 0 = surface (no EOS below), 1 = interior, 2 = top w level
@@ -1679,21 +1744,40 @@ __global__ __launch_bounds__(256) void k_finish(typename SRC::Arg a,
         const real phb_k = B3 ? L.phb : phb1[k];
         if (EOS && K != 0) {
             const unsigned h = f - st;
-            real th = thb_lo + thp_lo;
-            real dph = (phb_k + php_new) - (phb_lo + php_lo);
-            real alt = (-dph * rdnw[k - 1]) / (c1h[k - 1] * mu_eos + c2h[k - 1]);
-            gst(d.alt, h, bc, alt);
-            gst(d.al, h, bc, alt - alb_lo);
-            gst(d.p, h, bc, P0_F * pow_gamma((RD_F * th) / (P0_F * alt)));
+            if (PREC) {
+                const EosPert e = eos_pert(
+                    thp_lo, thb_lo, R(0.0), false, alb_lo, pb_lo, mup_new,
+                    c1h[k - 1] * mu_eos + c2h[k - 1], c1h[k - 1], rdnw[k - 1],
+                    php_new - php_lo);
+                gst(d.alt, h, bc, e.alt);
+                gst(d.al, h, bc, e.al);
+                gst(d.p, h, bc, e.p);
+                gst(d.pp, h, bc, e.pp);
+            } else {
+                real th = thb_lo + thp_lo;
+                real dph = (phb_k + php_new) - (phb_lo + php_lo);
+                real alt = (-dph * rdnw[k - 1]) / (c1h[k - 1] * mu_eos + c2h[k - 1]);
+                gst(d.alt, h, bc, alt);
+                gst(d.al, h, bc, alt - alb_lo);
+                gst(d.p, h, bc, P0_F * pow_gamma((RD_F * th) / (P0_F * alt)));
+            }
         }
         if (K != 2) {
             const real thbk = B3 ? L.thb : thb1[k];
-            real num = (c1h[k] * msA + c2h[k]) * (thbk + L.thp) + L.thpp;
-            real thn = num / (c1h[k] * mnA + c2h[k]) - thbk;
+            real thn;
+            if (PREC) {
+                real num = ((c1h[k] * msA + c2h[k]) * L.thp + L.thpp)
+                           - (c1h[k] * muppA) * thbk;
+                thn = num / (c1h[k] * mnA + c2h[k]);
+            } else {
+                real num = (c1h[k] * msA + c2h[k]) * (thbk + L.thp) + L.thpp;
+                thn = num / (c1h[k] * mnA + c2h[k]) - thbk;
+            }
             gst(o.thp, f, bc, thn);
             thp_lo = thn;
             thb_lo = thbk;
             alb_lo = B3 ? L.alb : alb1[k];
+            if (PREC && EOS) pb_lo = B3 ? L.pb : pb1[k];
         }
         php_lo = php_new;
         phb_lo = phb_k;
@@ -1771,41 +1855,88 @@ hipError_t run_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st)
     return hipGetLastError();
 }
 
+// Calls f(std::true_type) for the precision form, else f(std::false_type);
+// the FP64 build has no precision form, so it instantiates only the
+// latter.
+template <class F>
+static void with_prec(bool prec, F&& f)
+{
+    if constexpr (!ARW_F64) {
+        if (prec) {
+            f(std::true_type{});
+            return;
+        }
+    }
+    (void)prec;
+    f(std::false_type{});
+}
+
+template <class SRC, bool PREC>
+static void launch_tend_faces(const typename SRC::Arg& a, dim3 g, const Sc& s,
+                              bool b3, hipStream_t st)
+{
+    const dim3 b(64, 4);
+    if (b3 && s.diff)
+        hipLaunchKernelGGL((k_tend_faces<SRC, true, true, PREC>), g, b, 0, st, a, s);
+    else if (b3)
+        hipLaunchKernelGGL((k_tend_faces<SRC, true, false, PREC>), g, b, 0, st, a, s);
+    else if (s.diff)
+        hipLaunchKernelGGL((k_tend_faces<SRC, false, true, PREC>), g, b, 0, st, a, s);
+    else
+        hipLaunchKernelGGL((k_tend_faces<SRC, false, false, PREC>), g, b, 0, st, a, s);
+}
+
+template <class SRC, bool MOIST, bool PREC>
+static void launch_tend_cols(const typename SRC::Arg& a, dim3 g, const Sc& s,
+                             bool b3, hipStream_t st)
+{
+    const dim3 b(64, 4);
+    if (b3 && s.diff)
+        hipLaunchKernelGGL((k_tend_cols<SRC, true, true, MOIST, PREC>), g, b, 0, st, a, s);
+    else if (b3)
+        hipLaunchKernelGGL((k_tend_cols<SRC, true, false, MOIST, PREC>), g, b, 0, st, a, s);
+    else if (s.diff)
+        hipLaunchKernelGGL((k_tend_cols<SRC, false, true, MOIST, PREC>), g, b, 0, st, a, s);
+    else
+        hipLaunchKernelGGL((k_tend_cols<SRC, false, false, MOIST, PREC>), g, b, 0, st, a, s);
+}
+
+template <class SRC, bool PREC>
+static void launch_finish(const typename SRC::Arg& a, dim3 g,
+                          const FinishDst& o, bool b3, bool eos,
+                          hipStream_t st)
+{
+    if (b3 && eos)
+        hipLaunchKernelGGL((k_finish<SRC, true, true, PREC>), g, dim3(256), 0, st, a, o);
+    else if (b3)
+        hipLaunchKernelGGL((k_finish<SRC, true, false, PREC>), g, dim3(256), 0, st, a, o);
+    else if (eos)
+        hipLaunchKernelGGL((k_finish<SRC, false, true, PREC>), g, dim3(256), 0, st, a, o);
+    else
+        hipLaunchKernelGGL((k_finish<SRC, false, false, PREC>), g, dim3(256), 0, st, a, o);
+}
+
 hipError_t run_tend_faces(const Dom& d, const Sc& s, hipStream_t st)
 {
-    const dim3 g = rowseg_grid(d.nx, d.ny), b(64, 4);
-    if (d.base3d && s.diff)
-        hipLaunchKernelGGL((k_tend_faces<One, true, true>), g, b, 0, st, d, s);
-    else if (d.base3d)
-        hipLaunchKernelGGL((k_tend_faces<One, true, false>), g, b, 0, st, d, s);
-    else if (s.diff)
-        hipLaunchKernelGGL((k_tend_faces<One, false, true>), g, b, 0, st, d, s);
-    else
-        hipLaunchKernelGGL((k_tend_faces<One, false, false>), g, b, 0, st, d, s);
+    const dim3 g = rowseg_grid(d.nx, d.ny);
+    with_prec(s.prec, [&](auto p) {
+        launch_tend_faces<One, decltype(p)::value>(d, g, s, d.base3d != 0, st);
+    });
     const int n = d.nz * d.ny + d.nz * d.nx;
     hipLaunchKernelGGL(k_tend_faces_edge<One>, dim3((n + 255) / 256), dim3(256), 0,
                        st, d);
     return hipGetLastError();
 }
 
-template <bool MOIST>
-static void launch_tend_cols(const Dom& d, const Sc& s, hipStream_t st)
-{
-    const dim3 g = rowseg_grid(d.nx, d.ny), b(64, 4);
-    if (d.base3d && s.diff)
-        hipLaunchKernelGGL((k_tend_cols<One, true, true, MOIST>), g, b, 0, st, d, s);
-    else if (d.base3d)
-        hipLaunchKernelGGL((k_tend_cols<One, true, false, MOIST>), g, b, 0, st, d, s);
-    else if (s.diff)
-        hipLaunchKernelGGL((k_tend_cols<One, false, true, MOIST>), g, b, 0, st, d, s);
-    else
-        hipLaunchKernelGGL((k_tend_cols<One, false, false, MOIST>), g, b, 0, st, d, s);
-}
-
 hipError_t run_tend_cols(const Dom& d, const Sc& s, hipStream_t st)
 {
-    if (d.qv) launch_tend_cols<true>(d, s, st);
-    else launch_tend_cols<false>(d, s, st);
+    const dim3 g = rowseg_grid(d.nx, d.ny);
+    const bool b3 = d.base3d != 0;
+    with_prec(s.prec, [&](auto p) {
+        constexpr bool P = decltype(p)::value;
+        if (d.qv) launch_tend_cols<One, true, P>(d, g, s, b3, st);
+        else launch_tend_cols<One, false, P>(d, g, s, b3, st);
+    });
     return hipGetLastError();
 }
 
@@ -1860,18 +1991,15 @@ hipError_t run_column(const Dom& d, const Sc& s, real* pout, int write_al,
     return hipGetLastError();
 }
 
-hipError_t run_finish(const Dom& d, const FinishDst& o, hipStream_t st)
+hipError_t run_finish(const Dom& d, const FinishDst& o, bool prec,
+                      hipStream_t st)
 {
     // a moist state's EOS waits for the scalar update of the stage
     const bool eos = d.qv == nullptr;
-    if (d.base3d && eos)
-        hipLaunchKernelGGL((k_finish<One, true, true>), cols(d), dim3(256), 0, st, d, o);
-    else if (d.base3d)
-        hipLaunchKernelGGL((k_finish<One, true, false>), cols(d), dim3(256), 0, st, d, o);
-    else if (eos)
-        hipLaunchKernelGGL((k_finish<One, false, true>), cols(d), dim3(256), 0, st, d, o);
-    else
-        hipLaunchKernelGGL((k_finish<One, false, false>), cols(d), dim3(256), 0, st, d, o);
+    with_prec(prec, [&](auto p) {
+        launch_finish<One, decltype(p)::value>(d, cols(d), o, d.base3d != 0,
+                                               eos, st);
+    });
     hipLaunchKernelGGL(k_finish_edge<One>, dim3((d.ny + d.nx + 255) / 256),
                        dim3(256), 0, st, d, o);
     return hipGetLastError();
@@ -1922,15 +2050,9 @@ hipError_t run_tend_faces_b(const BatchSet& b, const Sc& s, bool b3,
                             hipStream_t st)
 {
     const Batch& r = b.k[BK_ROWS];
-    const dim3 g = bgrid(r), blk(64, 4);
-    if (b3 && s.diff)
-        hipLaunchKernelGGL((k_tend_faces<Many, true, true>), g, blk, 0, st, r, s);
-    else if (b3)
-        hipLaunchKernelGGL((k_tend_faces<Many, true, false>), g, blk, 0, st, r, s);
-    else if (s.diff)
-        hipLaunchKernelGGL((k_tend_faces<Many, false, true>), g, blk, 0, st, r, s);
-    else
-        hipLaunchKernelGGL((k_tend_faces<Many, false, false>), g, blk, 0, st, r, s);
+    with_prec(s.prec, [&](auto p) {
+        launch_tend_faces<Many, decltype(p)::value>(r, bgrid(r), s, b3, st);
+    });
     const Batch& e = b.k[BK_TFE];
     hipLaunchKernelGGL(k_tend_faces_edge<Many>, bgrid(e), dim3(256), 0, st, e);
     return hipGetLastError();
@@ -1940,15 +2062,9 @@ hipError_t run_tend_cols_b(const BatchSet& b, const Sc& s, bool b3,
                            hipStream_t st)
 {
     const Batch& r = b.k[BK_ROWS];
-    const dim3 g = bgrid(r), blk(64, 4);
-    if (b3 && s.diff)
-        hipLaunchKernelGGL((k_tend_cols<Many, true, true, false>), g, blk, 0, st, r, s);
-    else if (b3)
-        hipLaunchKernelGGL((k_tend_cols<Many, true, false, false>), g, blk, 0, st, r, s);
-    else if (s.diff)
-        hipLaunchKernelGGL((k_tend_cols<Many, false, true, false>), g, blk, 0, st, r, s);
-    else
-        hipLaunchKernelGGL((k_tend_cols<Many, false, false, false>), g, blk, 0, st, r, s);
+    with_prec(s.prec, [&](auto p) {
+        launch_tend_cols<Many, false, decltype(p)::value>(r, bgrid(r), s, b3, st);
+    });
     return hipGetLastError();
 }
 
@@ -1999,14 +2115,13 @@ hipError_t run_column_b(const BatchSet& b, const Sc& s, bool b3, int write_al,
     return hipGetLastError();
 }
 
-hipError_t run_finish_b(const BatchSet& b, bool b3, hipStream_t st)
+hipError_t run_finish_b(const BatchSet& b, bool b3, bool prec, hipStream_t st)
 {
     const Batch& r = b.k[BK_COLS];
     const FinishDst none = {};
-    if (b3)
-        hipLaunchKernelGGL((k_finish<Many, true, true>), bgrid(r), dim3(256), 0, st, r, none);
-    else
-        hipLaunchKernelGGL((k_finish<Many, false, true>), bgrid(r), dim3(256), 0, st, r, none);
+    with_prec(prec, [&](auto p) {
+        launch_finish<Many, decltype(p)::value>(r, bgrid(r), none, b3, true, st);
+    });
     const Batch& e = b.k[BK_FE];
     hipLaunchKernelGGL(k_finish_edge<Many>, bgrid(e), dim3(256), 0, st, e, none);
     return hipGetLastError();

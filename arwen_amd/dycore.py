@@ -29,6 +29,13 @@ def _fn(state: DomainState, name: str):
     return prefix + name, getattr(_lib.load(), prefix + name)
 
 
+def _dom(state: DomainState, cfg: RunConfig):
+    """The state's C view with the arrays cfg's step options need
+    (diffusion spacings, the precision form's pp and FP64 scratch)."""
+    return state.domain_struct(diffusion=_uses_diffusion(cfg),
+                               precision_form=bool(cfg.precision_form))
+
+
 def params(cfg: RunConfig) -> _lib.ArwParams:
     p = _lib.ArwParams()
     p.dx, p.dy, p.dt = float(cfg.dx), float(cfg.dy), float(cfg.dt)
@@ -111,7 +118,7 @@ def stage_fluxes(state: DomainState, cfg: RunConfig):
     nz, ny, nx = state.shape
     ru = state.scratch((nz, ny, nx + 1), "rk_ru")
     rv = state.scratch((nz, ny + 1, nx), "rk_rv")
-    d = state.domain_struct()
+    d = _dom(state, cfg)
     name, fn = _fn(state, "stage_fluxes")
     _lib.check(fn(d, params(cfg), ru.data_ptr(), rv.data_ptr(), _stream()),
                name)
@@ -122,7 +129,7 @@ def set_w_surface(state: DomainState, cfg: RunConfig) -> None:
     """w(sfc) = u . grad(ht) (reference dycore.py:1999-2026)."""
     _check_state(state)
     name, fn = _fn(state, "set_w_surface")
-    _lib.check(fn(state.domain_struct(), params(cfg), _stream()), name)
+    _lib.check(fn(_dom(state, cfg), params(cfg), _stream()), name)
 
 
 def prepare_acoustic_coefficients(state: DomainState, cfg: RunConfig,
@@ -131,7 +138,7 @@ def prepare_acoustic_coefficients(state: DomainState, cfg: RunConfig,
     acoustic.py:299-327)."""
     _check_state(state)
     name, fn = _fn(state, "calc_coefs")
-    _lib.check(fn(state.domain_struct(), params(cfg), float(dtau), _stream()),
+    _lib.check(fn(_dom(state, cfg), params(cfg), float(dtau), _stream()),
                name)
 
 
@@ -141,7 +148,7 @@ def acoustic_substep(state: DomainState, cfg: RunConfig, dtau: float,
     acoustic.py:482-567)."""
     _check_state(state)
     name, fn = _fn(state, "acoustic_substep")
-    _lib.check(fn(state.domain_struct(), params(cfg), float(dtau),
+    _lib.check(fn(_dom(state, cfg), params(cfg), float(dtau),
                   int(bool(first)), _stream()), name)
 
 

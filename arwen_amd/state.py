@@ -247,6 +247,7 @@ class DomainState:
         if precision_form and self.dtype == torch.float32:
             d.tri64 = self.scratch((2, nz + 1, ny, nx), "arw_tri64",
                                    dtype=torch.float64).data_ptr()
+            d.pp = self.scratch((nz, ny, nx), "arw_pp").data_ptr()
         return d
 
     def numpy_fields(self, names=("u", "v", "w", "thp", "php", "mup")):

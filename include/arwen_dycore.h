@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define ARW_ABI_VERSION 3
+#define ARW_ABI_VERSION 4
 
 #define ARW_OK 0
 #define ARW_E_ARG (-1)      /* invalid argument (null pointer, bad extent) */
@@ -74,8 +74,11 @@ typedef struct arw_domain {
     float *pd_fxl, *pd_fxc, *pd_fyl, *pd_fyc, *pd_fzl, *pd_fzc, *pd_sc;
     /* precision form only (else NULL): FP64 tridiagonal intermediates,
      * the LU factor gamma then the forward-eliminated w'', each
-     * (nz+1,ny,nx) */
+     * (nz+1,ny,nx); and pp, the perturbation pressure p - pb of the last
+     * EOS (nz,ny,nx), which the perturbation-form EOS keeps for the
+     * pressure-gradient terms */
     double *tri64;
+    float *pp;
 } arw_domain;
 
 /*
@@ -104,6 +107,7 @@ typedef struct arw64_domain {
     double *ru_m, *rv_m, *ww_m;
     double *pd_fxl, *pd_fxc, *pd_fyl, *pd_fyc, *pd_fzl, *pd_fzc, *pd_sc;
     double *tri64;
+    double *pp;                /* unused by the FP64 build */
 } arw64_domain;
 
 /* Step configuration (the RunConfig fields the step reads,
@@ -120,9 +124,11 @@ typedef struct arw_params {
     double zdamp, dampcoef;
     int32_t top_lid;
     int32_t moist_adv_opt;     /* 1: PD limiter on the final RK stage */
-    /* 1: precision form of the float32 step -- compensated (Kahan) column
-     * sums of the mass divergence and Omega, the implicit w/phi
-     * tridiagonal factored and solved in FP64 (needs d->tri64).  No
+    /* 1: precision form of the float32 step -- the EOS, the pressure
+     * gradient, buoyancy and rhs_ph terms and the theta'' init / theta
+     * finish in perturbation-from-base form (needs d->pp), compensated
+     * (Kahan) column sums of the mass divergence and Omega, the implicit
+     * w/phi tridiagonal factored and solved in FP64 (needs d->tri64).  No
      * reference counterpart; ignored by the FP64 build. */
     int32_t precision_form;
 } arw_params;

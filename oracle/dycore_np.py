@@ -71,6 +71,7 @@ def make_state(nz, ny, nx, coord, base, *, dtype=np.float32, moist=False,
     s.th_pp, s.ph_pp, s.mu_pp = z(nz, ny, nx), z(nz + 1, ny, nx), z(ny, nx)
     s.p_pp, s.p_pp_old, s.al_pp = z(nz, ny, nx), z(nz, ny, nx), z(nz, ny, nx)
     s.ww_pp = z(nz + 1, ny, nx)
+    s.pp = z(nz, ny, nx)       # precision form: p' of the last EOS
     if moist:
         for q in ("qv", "qc", "qr"):
             setattr(s, q, z(nz, ny, nx))
@@ -113,9 +114,13 @@ def _c(v, s):
 # EOS diagnostics
 # ---------------------------------------------------------------------------
 
-def calc_p_alpha(s) -> None:
+def calc_p_alpha(s, prec=False) -> None:
     """Full pressure, alpha', total alpha (reference diagnostics.cu:37-89,
-    hypso=1; launcher diagnostics.py:14-57)."""
+    hypso=1; launcher diagnostics.py:14-57).  ``prec``: the precision
+    form's perturbation EOS (calc_p_alpha_pert)."""
+    if prec:
+        calc_p_alpha_pert(s)
+        return
     mu = s.mub2d + s.mup
     th = _b3(s.thb) + s.thp
     if s.qv is not None:
@@ -127,6 +132,38 @@ def calc_p_alpha(s) -> None:
     s.alt[...] = alt
     s.al[...] = alt - _b3(s.alb)
     s.p[...] = _c(P0, s) * _pow_gamma((_c(RD, s) * th) / (_c(P0, s) * alt))
+
+
+def calc_p_alpha_pert(s) -> None:
+    """The EOS of calc_p_alpha in perturbation-from-base form (precision
+    form).  The base state is discretely hydrostatic, alb = -rdnw dphb /
+    (c1h mub + c2h), and in the EOS, pb = P0 (RD thb / (P0 alb))^GAMMA
+    (grid.make_base_state), so with mh = c1h mu + c2h
+      al  = -(alb c1h mu' + rdnw dphp) / mh         (WRF calc_p_rho's al)
+      p'  = pb expm1(GAMMA (log1p(theta'_m / thb) - log1p(al / alb)))
+    never difference the full geopotential or subtract pb from a full
+    pressure; the float32 state keeps alt = alb + al and p = pb + p', and
+    s.pp holds p' for the pressure-gradient terms.  The logarithms and the
+    exponential run in float64 and p' is rounded once (the HIP kernels do
+    the same)."""
+    f64 = np.float64
+    mu = s.mub2d + s.mup
+    c1h, c2h = s.c1h[:, None, None], s.c2h[:, None, None]
+    alb = _b3(s.alb)
+    mh = c1h * mu + c2h
+    al = -((alb * (c1h * s.mup)) + s.rdnw[:, None, None]
+           * (s.php[1:] - s.php[:-1])) / mh
+    s.al[...] = al
+    s.alt[...] = alb + al
+    x = np.log1p(s.thp.astype(f64) / _b3(s.thb).astype(f64))
+    if s.qv is not None:
+        x = x + np.log1p(f64(np.float32(RVOVRD)) * s.qv.astype(f64))
+    y = np.log1p(al.astype(f64) / alb.astype(f64))
+    pb = _b3(s.pb)
+    # the exponent the state's precision carries (as _pow_gamma)
+    g = f64(np.float32(GAMMA)) if s.dtype == np.float32 else f64(GAMMA)
+    s.pp[...] = (pb.astype(f64) * np.expm1(g * (x - y))).astype(s.dtype)
+    s.p[...] = pb + s.pp
 
 
 class _Kahan:
@@ -333,8 +370,9 @@ def flux_div_w(w, ru, rv, rw, rdn, fnm, fnp, dx_inv, dy_inv):
 # Large-step pressure gradient, buoyancy, geopotential RHS
 # ---------------------------------------------------------------------------
 
-def _pp(s):
-    return s.p - _b3(s.pb)
+def _pp(s, prec=False):
+    """p' = p - pb; the precision form's EOS keeps it (s.pp)."""
+    return s.pp if prec else s.p - _b3(s.pb)
 
 
 def _dpn_levels(s, pp, cfn, cfn1, top_lid):
@@ -350,7 +388,7 @@ def _dpn_levels(s, pp, cfn, cfn1, top_lid):
     return d
 
 
-def slow_pgf(s, dx, dy, top_lid=True):
+def slow_pgf(s, dx, dy, top_lid=True, prec=False):
     """WRF horizontal_pressure_gradient on u and v faces, subtracted from
     ru_t, rv_t (reference dycore.cu:98-211; launcher dycore.py:374-394)."""
     f = s.dtype.type
@@ -358,7 +396,7 @@ def slow_pgf(s, dx, dy, top_lid=True):
     hrdx, hrdy = f(0.5 / dx), f(0.5 / dy)
     half = f(0.5)
     mut = s.mub2d + s.mup
-    pp = _pp(s)
+    pp = _pp(s, prec)
     pbd = _b3(s.pb)
     dpn = _dpn_levels(s, pp, s.cfn, s.cfn1, top_lid)
     if s.base3d:
@@ -398,12 +436,12 @@ def _q_total(s):
     return (s.qv + s.qc) + s.qr
 
 
-def slow_buoyancy(s):
+def slow_buoyancy(s, prec=False):
     """WRF pg_buoy_w: rw_t += g (rdn dp'/deta - c1f mu' [moist loading])
     at w levels 1..nz (reference dycore.cu:231-308)."""
     f = s.dtype.type
     nz = s.thp.shape[0]
-    pp = _pp(s)
+    pp = _pp(s, prec)
     one, half, two = f(1.0), f(0.5), f(2.0)
     moist = s.qv is not None
     qt = _q_total(s) if moist else None
@@ -426,10 +464,14 @@ def slow_buoyancy(s):
     s.rw_t[nz] += f(G) * term
 
 
-def slow_geopotential(s, ww, dx, dy):
+def slow_geopotential(s, ww, dx, dy, prec=False):
     """WRF rhs_ph, order-2 horizontal branch with the vertical Omega and
     g*w terms (reference dycore.cu:541-667; launcher dycore.py:443-462).
-    Writes rph_t at levels 1..nz (the top row is overwritten, not added)."""
+    Writes rph_t at levels 1..nz (the top row is overwritten, not added).
+    ``prec`` (precision form): the layer thickness of the full geopotential
+    is dphb + dphp with dphb = -(dnw (c1h mub + c2h)) alb (the base
+    state's hydrostatic relation), and on a flat base the horizontal
+    differences are those of php (phb is horizontally uniform)."""
     f = s.dtype.type
     nz = s.thp.shape[0]
     half = f(0.5)
@@ -438,8 +480,14 @@ def slow_geopotential(s, ww, dx, dy):
     ph = (s.phb + s.php) if s.base3d else (s.phb[:, None, None] + s.php)
     rdnw = s.rdnw
     # vertical Omega term at interior levels 1..nz-1
+    if prec:
+        c1h, c2h = s.c1h[:, None, None], s.c2h[:, None, None]
+        dphb = -(s.dnw[:, None, None] * (c1h * s.mub2d + c2h)) * _b3(s.alb)
+        dph = dphb + (s.php[1:] - s.php[:-1])
+    else:
+        dph = ph[1:] - ph[:-1]
     wd = ((half * (ww[1:] + ww[:-1])) * rdnw[:, None, None]) \
-        * (ph[1:] - ph[:-1])                                # levels 0..nz-1
+        * dph                                               # levels 0..nz-1
     tend = s.rph_t.copy()
     tend[nz] = 0.0
     k = slice(1, nz)
@@ -465,7 +513,7 @@ def slow_geopotential(s, ww, dx, dy):
             wy = vc[kk] + vc[kk - 1]
         fcx = mx * wx                       # face i (between i-1 and i)
         fcy = my * wy
-        p = ph[kk]
+        p = s.php[kk] if (prec and not s.base3d) else ph[kk]
         fx = fcx * (p - np.roll(p, 1, axis=1))
         fxp = np.roll(fx, -1, axis=1)
         tend[kk] = tend[kk] - qx * (fxp + fx)
@@ -543,7 +591,7 @@ def add_diffusion_tendencies(s, khdif, kvdif, dx, dy):
         tend += (c1 * muf[None] + c2) * tmp
 
 
-def add_slow_tendencies(s, ru, rv, ww, dx, dy, top_lid=True):
+def add_slow_tendencies(s, ru, rv, ww, dx, dy, top_lid=True, prec=False):
     """Advection of theta/u/v/w plus PGF, buoyancy and rhs_ph (reference
     dycore.py:502-569)."""
     f = s.dtype.type
@@ -554,18 +602,20 @@ def add_slow_tendencies(s, ru, rv, ww, dx, dy, top_lid=True):
     s.ru_t += flux_div_u(s.u, ru, rv, ww, s.rdnw, s.fnm, s.fnp, dxi, dyi)
     s.rv_t += flux_div_v(s.v, ru, rv, ww, s.rdnw, s.fnm, s.fnp, dxi, dyi)
     s.rw_t += flux_div_w(s.w, ru, rv, ww, s.rdn, s.fnm, s.fnp, dxi, dyi)
-    slow_pgf(s, dx, dy, top_lid)
-    slow_buoyancy(s)
-    slow_geopotential(s, ww, dx, dy)
+    slow_pgf(s, dx, dy, top_lid, prec)
+    slow_buoyancy(s, prec)
+    slow_geopotential(s, ww, dx, dy, prec)
 
 
 # ---------------------------------------------------------------------------
 # Small-step preparation / acoustic substeps / finish
 # ---------------------------------------------------------------------------
 
-def small_step_init(s) -> None:
+def small_step_init(s, prec=False) -> None:
     """WRF small_step_prep + calc_p_rho step 0 (reference dycore.cu:
-    947-1085)."""
+    947-1085).  ``prec`` (precision form): theta'' = C(mu_t) theta_t -
+    C(mu_s) theta_s is formed without the base theta's cancellation, as
+    (c1h mu'') thb + (C(mu_t) thp_t - C(mu_s) thp_s)."""
     f = s.dtype.type
     half = f(0.5)
     mt = s.mub2d + s.mup0
@@ -591,7 +641,10 @@ def small_step_init(s) -> None:
     thb = _b3(s.thb)
     tht = thb + s.thp0
     ths = thb + s.thp
-    thpp = cht * tht - chs * ths
+    if prec:
+        thpp = (c1h * mupp) * thb + (cht * s.thp0 - chs * s.thp)
+    else:
+        thpp = cht * tht - chs * ths
     s.th_pp[...] = thpp
     c2a = (f(GAMMA) * s.p) / s.alt
     al = -((s.alt * (c1h * mupp)) + s.rdnw[:, None, None]
@@ -918,8 +971,11 @@ def run_acoustic_only(s, cfg_like, dtau, n):
         acoustic_substep(s, coefs, dtau, cfg_like, first=(i == 0))
 
 
-def small_step_finish(s) -> None:
-    """WRF small_step_finish (reference dycore.cu:1087-1198)."""
+def small_step_finish(s, prec=False) -> None:
+    """WRF small_step_finish (reference dycore.cu:1087-1198).  ``prec``
+    (precision form): theta' = (C(mu_s) theta_s + theta'') / C(mu_n) - thb
+    is formed without the base theta's cancellation, as (C(mu_s) thp +
+    theta'' - (c1h mu'') thb) / C(mu_n)."""
     f = s.dtype.type
     half = f(0.5)
     ms = s.mub2d + s.mup
@@ -939,8 +995,12 @@ def small_step_finish(s) -> None:
     s.w[...] = ((c1f * ms + c2f) * s.w + s.w_pp) / (c1f * mn + c2f)
     s.php[...] = s.php + s.ph_pp
     thb = _b3(s.thb)
-    num = (c1h * ms + c2h) * (thb + s.thp) + s.th_pp
-    s.thp[...] = num / (c1h * mn + c2h) - thb
+    if prec:
+        num = ((c1h * ms + c2h) * s.thp + s.th_pp) - (c1h * s.mu_pp) * thb
+        s.thp[...] = num / (c1h * mn + c2h)
+    else:
+        num = (c1h * ms + c2h) * (thb + s.thp) + s.th_pp
+        s.thp[...] = num / (c1h * mn + c2h) - thb
     s.mup[...] = s.mup + s.mu_pp
 
 
@@ -1000,17 +1060,17 @@ def step(s, cfg) -> None:
     ns = cfg.time_step_sound
     stages = ((1, cfg.dt / 3.0), (max(ns // 2, 1), cfg.dt / ns),
               (ns, cfg.dt / ns))
+    prec = bool(getattr(cfg, "precision_form", False)) \
+        and s.dtype == np.float32
     for istage, (nsub, dtau) in enumerate(stages):
         for name in ("ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t"):
             getattr(s, name)[...] = 0.0
-        calc_p_alpha(s)
-        prec = bool(getattr(cfg, "precision_form", False)) \
-            and s.dtype == np.float32
+        calc_p_alpha(s, prec)
         ru, rv, ww = stage_fluxes(s, cfg.dx, cfg.dy, prec)
-        add_slow_tendencies(s, ru, rv, ww, cfg.dx, cfg.dy, cfg.top_lid)
+        add_slow_tendencies(s, ru, rv, ww, cfg.dx, cfg.dy, cfg.top_lid, prec)
         if cfg.km_opt == 1:
             add_diffusion_tendencies(s, cfg.khdif, cfg.kvdif, cfg.dx, cfg.dy)
-        small_step_init(s)
+        small_step_init(s, prec)
         dtau32 = float(f(dtau))
         coefs = calc_coefs(s, dtau, cfg.epssm, cfg.top_lid, prec)
         if moist:
@@ -1023,7 +1083,7 @@ def step(s, cfg) -> None:
                 ru_m += s.u_pp
                 rv_m += s.v_pp
                 ww_m += s.ww_pp
-        small_step_finish(s)
+        small_step_finish(s, prec)
         if moist:
             ru_m, rv_m, ww_m = sumflux_finish(ru_m, rv_m, ww_m, ru, rv, ww,
                                               nsub)
@@ -1031,7 +1091,7 @@ def step(s, cfg) -> None:
                                   istage == 2, cfg.dx, cfg.dy,
                                   getattr(cfg, "moist_adv_opt", 1))
     set_w_surface(s, cfg.dx, cfg.dy)
-    calc_p_alpha(s)
+    calc_p_alpha(s, prec)
     s.elapsed_seconds += cfg.dt
 
 

@@ -120,3 +120,23 @@ def test_ensemble_rejects_bad_members():
     from arwen_amd import _lib
     with pytest.raises(_lib.ArwenError):
         Ensemble(states, cfg, groups=9)
+
+
+@pytest.mark.parametrize("terrain", [False, True])
+def test_precision_form_ensemble_matches_member_steps_bitwise(terrain):
+    """The precision form (perturbation EOS, p' array, FP64 tridiagonal)
+    through the batched launches equals each member's own precision-form
+    step."""
+    from arwen_amd import dycore
+    from arwen_amd.ensemble import Ensemble
+    cfg, seq, ens_states = _pair(terrain=terrain)
+    cfg = cfg.with_(precision_form=True)
+    for st in seq + ens_states:
+        st.cfg = st.cfg.with_(precision_form=True)
+    print(f"ensemble precision form terrain={terrain}", flush=True)
+    for st in seq:
+        dycore.run_steps(st, st.cfg, 2)
+    with Ensemble(ens_states, cfg, groups=2) as ens:
+        ens.step(2)
+    torch.cuda.synchronize()
+    _assert_same(seq, ens_states)

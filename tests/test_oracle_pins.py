@@ -112,9 +112,9 @@ MISTAKES = {
         O, "advance_uv", lambda s, dtau, smdiv, *a: _ORIG["advance_uv"](
             s, dtau, 0.0, *a)),
     "buoyancy dropped": lambda mp: mp.setattr(O, "slow_buoyancy",
-                                              lambda s: None),
-    "rhs_ph dropped": lambda mp: mp.setattr(O, "slow_geopotential",
-                                            lambda s, ww, dx, dy: None),
+                                              lambda s, prec=False: None),
+    "rhs_ph dropped": lambda mp: mp.setattr(
+        O, "slow_geopotential", lambda s, ww, dx, dy, prec=False: None),
     "large-step PGF dropped": lambda mp: mp.setattr(
         O, "slow_pgf", lambda *a, **k: None),
     "theta'' update dropped": lambda mp: mp.setattr(
