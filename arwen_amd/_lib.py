@@ -88,6 +88,19 @@ class ArwAcousticAdj(ctypes.Structure):
 
 ArwAcousticAdj64 = ArwAcousticAdj
 
+#: stage state and slow tendencies of the tendency-pass adjoint
+STAGE_STATE = ("u", "v", "w", "thp", "php", "mup")
+STAGE_TEND = ("ru_t", "rv_t", "rw_t", "rth_t", "rph_t")
+
+
+class ArwStageAdj(ctypes.Structure):
+    """arw_stage_adj (and arw64_stage_adj: same layout)."""
+    _fields_ = ([("l_" + n, _P) for n in STAGE_TEND]
+                + [("g_" + n, _P) for n in STAGE_STATE])
+
+
+ArwStageAdj64 = ArwStageAdj
+
 
 class ArwenError(RuntimeError):
     """A library entry point returned a nonzero status."""
@@ -135,6 +148,10 @@ def load():
                                           _P], _I),
         "arw_acoustic_adjoint_work": ([_I, _I, _I,
                                        ctypes.POINTER(ctypes.c_int64)], _I),
+        "arw_stage_tendencies_adjoint": ([dp, pp, ctypes.POINTER(ArwStageAdj),
+                                          _P, _P], _I),
+        "arw_stage_adjoint_work": ([_I, _I, _I,
+                                    ctypes.POINTER(ctypes.c_int64)], _I),
         "arw_ensemble_create": ([dp, _I, pp, _P, _I,
                                  ctypes.POINTER(ctypes.c_void_p)], _I),
         "arw_ensemble_step": ([_P, _I, _P], _I),
@@ -195,7 +212,8 @@ FP64_ENTRY_POINTS = (
     "arw64_acoustic_adjoint_work", "arw64_ensemble_create",
     "arw64_ensemble_step",
     "arw64_ensemble_capture", "arw64_ensemble_launch",
-    "arw64_ensemble_destroy") + tuple("arw64_" + n for n in FLUX_DIV_ADJOINTS)
+    "arw64_ensemble_destroy", "arw64_stage_tendencies_adjoint",
+    "arw64_stage_adjoint_work") + tuple("arw64_" + n for n in FLUX_DIV_ADJOINTS)
 
 
 def exported_symbols() -> list[str]:
@@ -211,7 +229,8 @@ def exported_symbols() -> list[str]:
             "arw_acoustic_substep_adjoint", "arw_acoustic_adjoint_work",
             "arw_ensemble_create",
             "arw_ensemble_step", "arw_ensemble_capture",
-            "arw_ensemble_launch", "arw_ensemble_destroy"] + [
+            "arw_ensemble_launch", "arw_ensemble_destroy",
+            "arw_stage_tendencies_adjoint", "arw_stage_adjoint_work"] + [
             "arw_" + n for n in FLUX_DIV_ADJOINTS]
 
 

@@ -244,3 +244,83 @@ class AcousticSubstep(torch.autograd.Function):
                if g is not None}
         out = acoustic_substep_adjoint(st, ctx.cfg, ctx.dtau, ctx.first, lam)
         return (None, None, None, None) + tuple(out[n] for n in names)
+
+
+# ---------------------------------------------------------------------------
+# Large-step tendency pass
+# ---------------------------------------------------------------------------
+
+STAGE_STATE = _lib.STAGE_STATE
+STAGE_TEND = _lib.STAGE_TEND
+
+
+def stage_tendencies(state: DomainState, cfg: RunConfig) -> dict:
+    """The slow tendencies of one RK stage at the state (EOS, Omega,
+    advection, PGF, buoyancy, rhs_ph; reference dycore.py:2378-2389) as
+    new tensors.  Runs the stage preparation in the state (its
+    diagnostics, Omega and small-step fields are overwritten)."""
+    from .dycore import stage_prepare
+    stage_prepare(state, cfg, cfg.dt / cfg.time_step_sound, True)
+    return {n: getattr(state, n).clone() for n in STAGE_TEND}
+
+
+def stage_tendencies_adjoint(state: DomainState, cfg: RunConfig,
+                             lam: dict) -> dict:
+    """J^T lam for the tendency pass at the state's stage fields: ``lam``
+    maps STAGE_TEND names to adjoints (missing = 0); returns the
+    gradients of STAGE_STATE.  Dry states without constant-K diffusion,
+    plain form.  The state is not modified."""
+    _check_state(state)
+    lib = _lib.load()
+    pre = "arw64_" if state.dtype == torch.float64 else "arw_"
+    nz, ny, nx = state.shape
+    n = ctypes.c_int64()
+    _lib.check(getattr(lib, pre + "stage_adjoint_work")(
+        nz, ny, nx, ctypes.byref(n)), pre + "stage_adjoint_work")
+    work = state.scratch((n.value,), "arw_stage_adjoint_work")
+    a = _lib.ArwStageAdj()
+    keep = []
+    for name in STAGE_TEND:
+        t = lam.get(name)
+        if t is None:
+            t = torch.zeros_like(getattr(state, name))
+        t = _as_like(t, getattr(state, name), name)
+        keep.append(t)
+        setattr(a, "l_" + name, t.data_ptr())
+    out = {}
+    for name in STAGE_STATE:
+        out[name] = torch.empty_like(getattr(state, name))
+        setattr(a, "g_" + name, out[name].data_ptr())
+    fn = getattr(lib, pre + "stage_tendencies_adjoint")
+    _lib.check(fn(state.domain_struct(), params(cfg), ctypes.byref(a),
+                  work.data_ptr(), _stream()),
+               pre + "stage_tendencies_adjoint")
+    return out
+
+
+class StageTendencies(torch.autograd.Function):
+    """(u, v, w, thp, php, mu') -> (ru_t, rv_t, rw_t, rth_t, rph_t) of
+    one RK stage, with the hand-written adjoint as its backward.
+    ``apply(state, cfg, *fields)`` writes the fields into the state and
+    runs the stage preparation there."""
+
+    @staticmethod
+    def forward(ctx, state, cfg, *fields):
+        if len(fields) != len(STAGE_STATE):
+            raise ValueError(f"expected {len(STAGE_STATE)} fields")
+        for name, x in zip(STAGE_STATE, fields):
+            getattr(state, name).copy_(x)
+        ctx.state, ctx.cfg = state, cfg
+        ctx.save_for_backward(*[x.detach() for x in fields])
+        t = stage_tendencies(state, cfg)
+        return tuple(t[n] for n in STAGE_TEND)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        st = ctx.state
+        for name, x in zip(STAGE_STATE, ctx.saved_tensors):
+            getattr(st, name).copy_(x)
+        lam = {n: (g if g is not None else torch.zeros_like(getattr(st, n)))
+               for n, g in zip(STAGE_TEND, grads)}
+        g = stage_tendencies_adjoint(st, ctx.cfg, lam)
+        return (None, None) + tuple(g[n] for n in STAGE_STATE)

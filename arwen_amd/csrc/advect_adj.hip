@@ -654,6 +654,33 @@ int32_t flux_div_adjoint(const real* q, const real* ru, const real* rv,
 
 }  // namespace
 
+int32_t lib_flux_div_adjoint(int op, const real* q, const real* ru,
+                             const real* rv, const real* rw, const real* lam,
+                             real* g_q, real* g_ru, real* g_rv, real* g_rw,
+                             const real* rdz, const real* fnm,
+                             const real* fnp, real dx_inv, real dy_inv,
+                             int nz, int ny, int nx, hipStream_t st)
+{
+    switch (op) {
+    case 0:
+        return flux_div_adjoint<OpScalar>(q, ru, rv, rw, lam, g_q, g_ru, g_rv,
+                                          g_rw, rdz, fnm, fnp, dx_inv, dy_inv,
+                                          nz, ny, nx, st);
+    case 1:
+        return flux_div_adjoint<OpU>(q, ru, rv, rw, lam, g_q, g_ru, g_rv, g_rw,
+                                     rdz, fnm, fnp, dx_inv, dy_inv, nz, ny,
+                                     nx, st);
+    case 2:
+        return flux_div_adjoint<OpV>(q, ru, rv, rw, lam, g_q, g_ru, g_rv, g_rw,
+                                     rdz, fnm, fnp, dx_inv, dy_inv, nz, ny,
+                                     nx, st);
+    default:
+        return flux_div_adjoint<OpW>(q, ru, rv, rw, lam, g_q, g_ru, g_rv, g_rw,
+                                     rdz, fnm, fnp, dx_inv, dy_inv, nz, ny,
+                                     nx, st);
+    }
+}
+
 }  // namespace ARW_NS
 
 using namespace ARW_NS;

@@ -356,6 +356,13 @@ __global__ void k_faces(Dom d, real* ru, real* rv)
     if (i < d.nx) rv[iV(k, j, i, d.ny, d.nx)] = rv_at(d, k, j, i);
 }
 
+hipError_t run_faces(const Dom& d, real* ru, real* rv, hipStream_t st)
+{
+    dim3 grid((d.nx + 1 + 63) / 64, (d.ny + 1 + 3) / 4, d.nz);
+    hipLaunchKernelGGL(k_faces, grid, dim3(64, 4), 0, st, d, ru, rv);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Moist scalar transport (qv/qc/qr), periodic, msf = 1
 // ---------------------------------------------------------------------------

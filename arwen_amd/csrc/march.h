@@ -103,6 +103,16 @@ int32_t lib_check_domain(const Dom* d);
 int32_t lib_check_prec(const arw_params* p, const Dom* d);
 
 hipError_t run_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st);
+// Coupled mass fluxes ru, rv of the state (dycore.hip, k_faces).
+hipError_t run_faces(const Dom& d, real* ru, real* rv, hipStream_t st);
+// The advection adjoints of advect_adj.hip (op 0 scalar, 1 u, 2 v, 3 w),
+// adding into g_q, g_ru, g_rv, g_rw.
+int32_t lib_flux_div_adjoint(int op, const real* q, const real* ru,
+                             const real* rv, const real* rw, const real* lam,
+                             real* g_q, real* g_ru, real* g_rv, real* g_rw,
+                             const real* rdz, const real* fnm,
+                             const real* fnp, real dx_inv, real dy_inv,
+                             int nz, int ny, int nx, hipStream_t st);
 hipError_t run_tend_faces(const Dom& d, const Sc& s, hipStream_t st);
 hipError_t run_tend_cols(const Dom& d, const Sc& s, hipStream_t st);
 hipError_t run_uv(const Dom& d, const Sc& s, hipStream_t st);

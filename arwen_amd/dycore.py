@@ -125,6 +125,17 @@ def stage_fluxes(state: DomainState, cfg: RunConfig):
     return ru, rv, state.scratch((nz + 1, ny, nx), "arw_ww")
 
 
+def stage_prepare(state: DomainState, cfg: RunConfig, dtau: float,
+                  do_eos: bool = True) -> None:
+    """One RK stage's preparation at the current state: EOS (when do_eos),
+    Omega, the slow tendencies ru_t..rph_t and the small-step
+    initialisation (reference dycore.py:2378-2389, :1502-1532)."""
+    _check_state(state)
+    name, fn = _fn(state, "stage_prepare")
+    _lib.check(fn(_dom(state, cfg), params(cfg), float(dtau),
+                  int(bool(do_eos)), _stream()), name)
+
+
 def set_w_surface(state: DomainState, cfg: RunConfig) -> None:
     """w(sfc) = u . grad(ht) (reference dycore.py:1999-2026)."""
     _check_state(state)

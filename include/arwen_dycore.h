@@ -278,6 +278,27 @@ int32_t arw_flux_div_w(const float *w, const float *ru, const float *rv,
                        float dy_inv, int32_t nz, int32_t ny, int32_t nx,
                        void *stream);
 
+/* Adjoint of the large-step tendency pass of one RK stage (no
+ * counterpart in the reference): the stage-start EOS, the coupled mass
+ * fluxes and Omega, the advection of theta/u/v/w, the large-step PGF,
+ * buoyancy and rhs_ph (diagnostics.cu:37-89, dycore.py:102-206,
+ * advection.cu:206-1051, dycore.cu:98-308, :541-667) for a dry state
+ * without constant-K diffusion, plain form.  For the adjoints l_* of
+ * ru_t, rv_t, rw_t, rth_t, rph_t it WRITES the gradients g_* of the stage
+ * state u, v, w, thp, php, mup (the transposed Jacobian at the state d).
+ * The state is not modified: its EOS, Omega and fluxes are recomputed
+ * into work (arw_stage_adjoint_work elements). */
+typedef struct arw_stage_adj {
+    const float *l_ru_t, *l_rv_t, *l_rw_t, *l_rth_t, *l_rph_t;
+    float *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+} arw_stage_adj;
+int32_t arw_stage_tendencies_adjoint(const arw_domain *d,
+                                     const arw_params *prm,
+                                     const arw_stage_adj *adj, float *work,
+                                     void *stream);
+int32_t arw_stage_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
+                               int64_t *elements);
+
 /* Adjoints of the four advection kernels above (no counterpart in the
  * reference, which has no adjoint): for the output adjoint lam (shaped like
  * that kernel's tend) they ADD the transposed Jacobian of tend with respect
@@ -478,6 +499,17 @@ int32_t arw64_acoustic_substep_adjoint(const arw64_domain *d,
                                        double *work, void *stream);
 int32_t arw64_acoustic_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
                                     int64_t *elements);
+
+typedef struct arw64_stage_adj {
+    const double *l_ru_t, *l_rv_t, *l_rw_t, *l_rth_t, *l_rph_t;
+    double *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+} arw64_stage_adj;
+int32_t arw64_stage_tendencies_adjoint(const arw64_domain *d,
+                                       const arw_params *prm,
+                                       const arw64_stage_adj *adj,
+                                       double *work, void *stream);
+int32_t arw64_stage_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
+                                 int64_t *elements);
 
 typedef struct arw64_ensemble arw64_ensemble;
 int32_t arw64_ensemble_create(const arw64_domain *members, int32_t n,

@@ -619,3 +619,309 @@ def flux_div_adjoint(op, lam, q, ru, rv, rw, rdz, fnm, fnp, dx_inv,
         g["q"] = gq + gz
         return g
     raise ValueError(op)
+
+
+# ---------------------------------------------------------------------------
+# The large-step tendency pass: EOS -> coupled fluxes and Omega ->
+# advection + PGF + buoyancy + rhs_ph  (dry, periodic, msf = 1, no
+# diffusion), and its hand-written transpose
+# ---------------------------------------------------------------------------
+
+STAGE_IN = ("u", "v", "w", "thp", "php", "mup")
+STAGE_OUT = ("ru_t", "rv_t", "rw_t", "rth_t", "rph_t")
+
+
+def stage_tendencies(s, dx, dy, top_lid=True):
+    """The slow tendencies of one RK stage at the state s (reference
+    dycore.py:2378-2389 with km_opt = 0: calc_p_alpha, stage_fluxes,
+    _add_slow_tendencies).  Mutates s's diagnostics and tendencies;
+    returns {name: tendency} and the stage fluxes (ru, rv, ww)."""
+    for name in STAGE_OUT + ("rmu_t",):
+        getattr(s, name)[...] = 0.0
+    D.calc_p_alpha(s)
+    ru, rv, ww = D.stage_fluxes(s, dx, dy)
+    D.add_slow_tendencies(s, ru, rv, ww, dx, dy, top_lid)
+    return {n: getattr(s, n).copy() for n in STAGE_OUT}, (ru, rv, ww)
+
+
+def stage_tendencies_cs(s, dx, dy, top_lid=True):
+    """stage_tendencies with the complex-step-safe flux5/flux3 (|v| as
+    sign(Re v) v), for complex states."""
+    saved = (D.flux5, D.flux3)
+    D.flux5, D.flux3 = _flux5_cs, _flux3_cs
+    try:
+        return stage_tendencies(s, dx, dy, top_lid)[0]
+    finally:
+        D.flux5, D.flux3 = saved
+
+
+def _face_fold(x, ax):
+    """A face array (n+1 along ax) folded onto the n periodic faces: the
+    duplicate face n adds into face 0."""
+    n = x.shape[ax] - 1
+    out = np.take(x, range(n), axis=ax).copy()
+    idx = [slice(None)] * x.ndim
+    idx[ax] = 0
+    src = [slice(None)] * x.ndim
+    src[ax] = n
+    out[tuple(idx)] += x[tuple(src)]
+    return out
+
+
+def eos_adjoint(s, g_p, g_al, g_alt):
+    """(thp, php, mup) gradients of calc_p_alpha (dry, hypso = 1) given
+    those of p, al, alt (reference diagnostics.cu:37-89)."""
+    mu = s.mub2d + s.mup
+    c1h, c2h = s.c1h[:, None, None], s.c2h[:, None, None]
+    th = D._b3(s.thb) + s.thp
+    mh = c1h * mu + c2h
+    ga = g_alt + g_al - g_p * (GAMMA * s.p / s.alt)
+    g_thp = g_p * (GAMMA * s.p / th)
+    g_dph = ga * (-s.rdnw[:, None, None] / mh)
+    g_php = np.zeros_like(s.php)
+    g_php[1:] += g_dph
+    g_php[:-1] -= g_dph
+    g_mup = np.sum(ga * (-s.alt * c1h / mh), axis=0)
+    return g_thp, g_php, g_mup
+
+
+GAMMA = D.GAMMA
+
+
+def coupling_adjoint(s, g_ru, g_rv):
+    """(u, v, mup) gradients of ru = (c1h mu_u + c2h) u, rv likewise
+    (reference dycore.py:162-206, state.py:339-356)."""
+    mu = s.mub2d + s.mup
+    mux, muy = D.mu_at_u_faces(mu), D.mu_at_v_faces(mu)
+    c1h, c2h = s.c1h[:, None, None], s.c2h[:, None, None]
+    g_u = (c1h * mux[None] + c2h) * g_ru
+    g_v = (c1h * muy[None] + c2h) * g_rv
+    gmx = _face_fold(np.sum(c1h * s.u * g_ru, axis=0), -1)   # (ny, nx)
+    gmy = _face_fold(np.sum(c1h * s.v * g_rv, axis=0), -2)
+    g_mu = 0.5 * (gmx + np.roll(gmx, -1, axis=1)) \
+        + 0.5 * (gmy + np.roll(gmy, -1, axis=0))
+    return g_u, g_v, g_mu
+
+
+def omega_adjoint(s, g_ww, dx, dy):
+    """(ru, rv) gradients of omega_ref's Omega (WRF calc_ww_cp, reference
+    dycore.py:102-131)."""
+    nz = s.thp.shape[0]
+    rdx, rdy = 1.0 / dx, 1.0 / dy
+    dnw = s.dnw[:, None, None]
+    # ww[k+1] = -sum_{m<=k} inc[m], k = 0..nz-2
+    g_inc = np.zeros((nz,) + g_ww.shape[1:], dtype=g_ww.dtype)
+    run = np.zeros_like(g_ww[0])
+    for k in range(nz - 2, -1, -1):
+        run = run - g_ww[k + 1]
+        g_inc[k] = run
+    g_dmdt = np.sum((s.c1h * s.dnw)[:nz - 1, None, None] * g_inc[:nz - 1],
+                    axis=0)
+    g_divv = g_inc.copy()
+    g_divv[nz - 1] = 0.0
+    g_divv = g_divv + g_dmdt[None]
+    gx = dnw * rdx * g_divv            # d divv / d ru[f] = +-dnw rdx
+    gy = dnw * rdy * g_divv
+    g_ru = np.zeros((nz,) + (g_ww.shape[1], g_ww.shape[2] + 1),
+                    dtype=g_ww.dtype)
+    g_ru[..., 1:] += gx
+    g_ru[..., :-1] -= gx
+    g_rv = np.zeros((nz, g_ww.shape[1] + 1, g_ww.shape[2]), dtype=g_ww.dtype)
+    g_rv[:, 1:] += gy
+    g_rv[:, :-1] -= gy
+    return g_ru, g_rv
+
+
+def pgf_adjoint(s, lam_u, lam_v, dx, dy, top_lid=True):
+    """Gradients (p, al, alt, php, mup) of slow_pgf's subtraction from
+    ru_t, rv_t (reference dycore.cu:98-211) for output adjoints lam_u,
+    lam_v (shaped like ru_t, rv_t)."""
+    nz = s.thp.shape[0]
+    rdx, rdy = 1.0 / dx, 1.0 / dy
+    hrdx, hrdy = 0.5 / dx, 0.5 / dy
+    mut = s.mub2d + s.mup
+    pp = s.p - D._b3(s.pb)
+    pbd = D._b3(s.pb)
+    dpn = D._dpn_levels(s, pp, s.cfn, s.cfn1, top_lid)
+    if s.base3d:
+        phh = 0.5 * ((s.phb[:-1] + s.phb[1:]) + (s.php[:-1] + s.php[1:]))
+    else:
+        phh = 0.5 * (s.php[:-1] + s.php[1:])
+    c1h, c2h = s.c1h[:, None, None], s.c2h[:, None, None]
+    rdnw = s.rdnw[:, None, None]
+    g = {"pp": np.zeros_like(pp), "al": np.zeros_like(s.al),
+         "alt": np.zeros_like(s.alt), "php": np.zeros_like(s.php),
+         "mup": np.zeros_like(s.mup), "dpn": np.zeros_like(dpn)}
+
+    def face(lam, ax, rd, hrd):
+        lt = -_face_fold(lam, ax)              # ru_t -= tu
+        A = lambda a: a
+        B = lambda a: np.roll(a, 1, axis=ax)
+        BT = lambda a: np.roll(a, -1, axis=ax)  # transpose of B
+        muf = 0.5 * (A(mut) + B(mut))
+        dmu = 0.5 * (A(s.mup) + B(s.mup))
+        layer = c1h * muf[None] + c2h
+        bracket = (A(s.php[1:]) - B(s.php[1:])) + (A(s.php[:-1])
+                                                   - B(s.php[:-1]))
+        bracket = bracket + (A(s.alt) + B(s.alt)) * (A(pp) - B(pp))
+        if s.base3d:
+            bracket = bracket + (A(s.al) + B(s.al)) * (A(pbd) - B(pbd))
+        dphp = A(phh) - B(phh)
+        hi = 0.5 * (A(dpn[1:]) + B(dpn[1:]))
+        lo = 0.5 * (A(dpn[:-1]) + B(dpn[:-1]))
+        vertical = rdnw * (hi - lo) - c1h * dmu[None]
+        g_br = lt * hrd * layer
+        g_muf = np.sum(c1h * lt * hrd * bracket, axis=0)
+        g_dphp = lt * rd * vertical
+        g_vert = lt * rd * dphp
+        both = lambda x: x + BT(x)
+        anti = lambda x: x - BT(x)
+        g["mup"] += 0.5 * both(g_muf)
+        g["php"][1:] += anti(g_br)
+        g["php"][:-1] += anti(g_br)
+        g["alt"] += both(g_br * (A(pp) - B(pp)))
+        g["pp"] += anti(g_br * (A(s.alt) + B(s.alt)))
+        if s.base3d:
+            g["al"] += both(g_br * (A(pbd) - B(pbd)))
+        g["php"][:-1] += 0.5 * anti(g_dphp)
+        g["php"][1:] += 0.5 * anti(g_dphp)
+        g_hi = g_vert * rdnw
+        g["dpn"][1:] += 0.5 * both(g_hi)
+        g["dpn"][:-1] -= 0.5 * both(g_hi)
+        g["mup"] += 0.5 * both(np.sum(-c1h * g_vert, axis=0))
+
+    face(lam_u, -1, rdx, hrdx)
+    face(lam_v, -2, rdy, hrdy)
+    gd = g["dpn"]
+    g["pp"][0] += s.cf1 * gd[0]
+    g["pp"][1] += s.cf2 * gd[0]
+    g["pp"][2] += s.cf3 * gd[0]
+    for kf in range(1, nz):
+        g["pp"][kf] += s.fnm[kf] * gd[kf]
+        g["pp"][kf - 1] += s.fnp[kf] * gd[kf]
+    if top_lid:
+        g["pp"][nz - 1] += s.cfn * gd[nz]
+        g["pp"][nz - 2] += s.cfn1 * gd[nz]
+    return g
+
+
+def buoyancy_adjoint(s, lam_w):
+    """(p, mup) gradients of slow_buoyancy (dry; reference dycore.cu:
+    231-308)."""
+    nz = s.thp.shape[0]
+    g_pp = np.zeros_like(s.p)
+    lw = lam_w[1:nz]
+    g_pp[1:] += D.G * s.rdn[1:, None, None] * lw
+    g_pp[:-1] -= D.G * s.rdn[1:, None, None] * lw
+    g_mup = -D.G * np.sum(s.c1f[1:nz, None, None] * lw, axis=0)
+    g_pp[nz - 1] -= D.G * 2.0 * s.rdnw[nz - 1] * lam_w[nz]
+    g_mup = g_mup - D.G * s.c1f[nz] * lam_w[nz]
+    return g_pp, g_mup
+
+
+def geopotential_adjoint(s, ww, lam, dx, dy):
+    """(ww, w, u, v, php, mup) gradients of slow_geopotential (reference
+    dycore.cu:541-667), lam shaped like rph_t (rows 1..nz used)."""
+    nz = s.thp.shape[0]
+    qrx, qry = 0.25 / dx, 0.25 / dy
+    mut = s.mub2d + s.mup
+    ph = (s.phb + s.php) if s.base3d else (s.phb[:, None, None] + s.php)
+    rdnw = s.rdnw[:, None, None]
+    g = {"ww": np.zeros_like(ww), "w": np.zeros_like(s.w),
+         "u": np.zeros_like(s.u), "v": np.zeros_like(s.v),
+         "php": np.zeros_like(s.php), "mup": np.zeros_like(s.mup)}
+    g_wd = np.zeros_like(s.thp)
+    for k in range(1, nz):
+        g_wd[k] -= s.fnm[k] * lam[k]
+        g_wd[k - 1] -= s.fnp[k] * lam[k]
+    dph = ph[1:] - ph[:-1]
+    wavg = 0.5 * (ww[1:] + ww[:-1])
+    gwa = g_wd * rdnw * dph
+    g["ww"][1:] += 0.5 * gwa
+    g["ww"][:-1] += 0.5 * gwa
+    gd = g_wd * wavg * rdnw
+    g["php"][1:] += gd
+    g["php"][:-1] -= gd
+    G = D.G
+    for k in range(1, nz + 1):
+        g["mup"] += s.c1f[k] * G * s.w[k] * lam[k]
+        g["w"][k] += (s.c1f[k] * mut + s.c2f[k]) * G * lam[k]
+    avgx = 0.5 * (mut + np.roll(mut, 1, axis=1))
+    avgy = 0.5 * (mut + np.roll(mut, 1, axis=0))
+    uc, vc = s.u[..., :-1], s.v[:, :-1, :]
+    for kk in range(1, nz + 1):
+        top = kk == nz
+        qx = 2.0 * qrx if top else qrx
+        qy = 2.0 * qry if top else qry
+        lk = lam[kk]
+        for ax, q, avg, xc, gname in ((1, qx, avgx, uc, "u"),
+                                      (0, qy, avgy, vc, "v")):
+            m = s.c1f[kk] * avg + s.c2f[kk]
+            if top:
+                wv = s.cfn * xc[kk - 1] + s.cfn1 * xc[kk - 2]
+            else:
+                wv = xc[kk] + xc[kk - 1]
+            p = ph[kk]
+            dp = p - np.roll(p, 1, axis=ax)
+            g_f = -q * (lk + np.roll(lk, 1, axis=ax))   # f[i] in tend i, i-1
+            g_dp = g_f * m * wv
+            g["php"][kk] += g_dp - np.roll(g_dp, -1, axis=ax)
+            g_wv = g_f * m * dp
+            gx = g[gname]
+            sl = (slice(None), slice(0, -1)) if gname == "u" else \
+                (slice(0, -1), slice(None))
+            if top:
+                gx[(kk - 1,) + sl] += s.cfn * g_wv
+                gx[(kk - 2,) + sl] += s.cfn1 * g_wv
+            else:
+                gx[(kk,) + sl] += g_wv
+                gx[(kk - 1,) + sl] += g_wv
+            g_avg = s.c1f[kk] * g_f * dp * wv
+            g["mup"] += 0.5 * (g_avg + np.roll(g_avg, -1, axis=ax))
+    return g
+
+
+def stage_tendencies_adjoint(s, lam, dx, dy, top_lid=True):
+    """Transpose of stage_tendencies' Jacobian at the state s applied to
+    lam = {STAGE_OUT name: adjoint}: returns {STAGE_IN name: gradient}.
+    s must hold the stage state (its diagnostics are recomputed)."""
+    work = D.OState(**{k: (v.copy() if isinstance(v, np.ndarray) else v)
+                       for k, v in vars(s).items()})
+    _, (ru, rv, ww) = stage_tendencies(work, dx, dy, top_lid)
+    z = lambda n: lam.get(n, np.zeros_like(getattr(s, n)))
+    l_ru, l_rv, l_rw, l_rth, l_rph = (z(n) for n in STAGE_OUT)
+    dxi, dyi = 1.0 / dx, 1.0 / dy
+    g = {n: np.zeros_like(getattr(s, n)) for n in STAGE_IN}
+    g_ru, g_rv, g_ww = np.zeros_like(ru), np.zeros_like(rv), np.zeros_like(ww)
+    theta = D._b3(work.thb) + work.thp
+    for op, q, l, rdz, name in (("scalar", theta, l_rth, work.rdnw, "thp"),
+                                ("u", work.u, l_ru, work.rdnw, "u"),
+                                ("v", work.v, l_rv, work.rdnw, "v"),
+                                ("w", work.w, l_rw, work.rdn, "w")):
+        a = flux_div_adjoint(op, l, q, ru, rv, ww, rdz, work.fnm, work.fnp,
+                             dxi, dyi)
+        g[name] += a["q"]
+        g_ru += a["ru"]
+        g_rv += a["rv"]
+        g_ww += a["rw"]
+    gp = pgf_adjoint(work, l_ru, l_rv, dx, dy, top_lid)
+    g_pp, g_mup_b = buoyancy_adjoint(work, l_rw)
+    gg = geopotential_adjoint(work, ww, l_rph, dx, dy)
+    g_ww += gg["ww"]
+    for n in ("w", "u", "v", "php", "mup"):
+        g[n] += gg[n]
+    g["php"] += gp["php"]
+    g["mup"] += gp["mup"] + g_mup_b
+    g_p = gp["pp"] + g_pp
+    a_ru, a_rv = omega_adjoint(work, g_ww, dx, dy)
+    g_ru += a_ru
+    g_rv += a_rv
+    gu, gv, gmu = coupling_adjoint(work, g_ru, g_rv)
+    g["u"] += gu
+    g["v"] += gv
+    g["mup"] += gmu
+    gt, gph, gm = eos_adjoint(work, g_p, gp["al"], gp["alt"])
+    g["thp"] += gt
+    g["php"] += gph
+    g["mup"] += gm
+    return g

@@ -72,7 +72,9 @@ def _struct_fields(name):
 
 @pytest.mark.parametrize("cname,pyt", [("arw_domain", _lib.ArwDomain),
                                        ("arw64_domain", _lib.ArwDomain64),
-                                       ("arw_params", _lib.ArwParams)])
+                                       ("arw_params", _lib.ArwParams),
+                                       ("arw_stage_adj", _lib.ArwStageAdj),
+                                       ("arw64_stage_adj", _lib.ArwStageAdj64)])
 def test_ctypes_layout_matches_header(cname, pyt):
     kinds = {"ptr": ctypes.c_void_p, "int32_t": ctypes.c_int32,
              "float": ctypes.c_float, "double": ctypes.c_double}

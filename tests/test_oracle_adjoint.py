@@ -285,3 +285,56 @@ def test_planted_advection_adjoint_mistake_fails(monkeypatch):
     got = A.flux_div_adjoint("u", lam, *args)
     ref = (J["ru"].T @ lam.reshape(-1)).reshape(got["ru"].shape)
     assert np.abs(got["ru"] - ref).max() > 1e-3 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------------------------
+# The large-step tendency pass: hand adjoint vs complex-step JVPs
+# ---------------------------------------------------------------------------
+
+def _stage_state(name, seed, dtype=np.float64):
+    from tests import pins
+    case = pins.CASES[name]()
+    s = D.from_case(case, dtype)
+    rng = np.random.default_rng(seed)
+    s.mup[...] += rng.standard_normal(s.mup.shape) * 20.0
+    s.php[1:] += rng.standard_normal(s.php[1:].shape) * 30.0
+    s.w[1:-1] += rng.standard_normal(s.w[1:-1].shape) * 0.3
+    s.v[...] += rng.standard_normal(s.v.shape) * 0.5
+    return case, s
+
+
+def _cplx(s):
+    c = D.OState(**{k: (v.astype(np.complex128) if isinstance(v, np.ndarray)
+                        and v.dtype.kind == "f" else v)
+                    for k, v in vars(s).items()})
+    c.dtype = np.dtype(np.complex128)
+    return c
+
+
+@pytest.mark.parametrize("name", ["dry_flat", "terrain"])
+@pytest.mark.parametrize("top_lid", [True, False])
+def test_stage_tendency_adjoint_dot_product_vs_complex_step(name, top_lid):
+    """<J v, lam> from complex-step JVPs of the oracle's own forward stage
+    pass equals <v, J^T lam> from the hand adjoint, for random directions
+    in all six stage inputs at once and one input at a time."""
+    case, s = _stage_state(name, seed=13)
+    dx, dy = case.cfg.dx, case.cfg.dy
+    rng = np.random.default_rng(17)
+    lam = {n: rng.standard_normal(getattr(s, n).shape)
+           for n in A.STAGE_OUT}
+    g = A.stage_tendencies_adjoint(s, lam, dx, dy, top_lid)
+    h = 1e-20
+    scale = {"u": 1.0, "v": 1.0, "w": 0.1, "thp": 0.5, "php": 20.0,
+             "mup": 20.0}
+    for which in (A.STAGE_IN,) + tuple((n,) for n in A.STAGE_IN):
+        c = _cplx(s)
+        dirs = {}
+        for n in which:
+            d = rng.standard_normal(getattr(s, n).shape) * scale[n]
+            dirs[n] = d
+            getattr(c, n)[...] = getattr(c, n) + 1j * h * d
+        t = A.stage_tendencies_cs(c, dx, dy, top_lid)
+        jv = sum(float(np.sum(np.imag(t[n]) / h * lam[n]))
+                 for n in A.STAGE_OUT)
+        vjt = sum(float(np.sum(dirs[n] * g[n])) for n in which)
+        assert abs(jv - vjt) <= 1e-10 * max(abs(jv), 1e-30), (which, jv, vjt)
