@@ -419,6 +419,25 @@ def main_adjoint(args):
                     "acoustic_adjoint_ms": round(t_sub_adj, 4),
                     "acoustic_adjoint_to_forward": round(t_sub_adj / t_sub,
                                                          3)})
+        # the large-step tendency pass (a dry copy of the state: the
+        # pass's adjoint covers dry states) against its adjoint
+        cfg_dry = dataclasses.replace(cfg, moist=False)
+        dry_case = dataclasses.replace(
+            case, cfg=cfg_dry, fields={k: v for k, v in case.fields.items()
+                                       if k not in ("qv", "qc", "qr")})
+        sd = DomainState.from_host(dry_case, dtype=dtype)
+        for n in adjoint.STAGE_STATE:
+            getattr(sd, n).copy_(getattr(st, n))
+        lam_t = {n: torch.randn_like(getattr(sd, n))
+                 for n in adjoint.STAGE_TEND}
+        t_sp = _time_ms(lambda: dycore.stage_prepare(
+            sd, cfg_dry, cfg.dt / cfg.time_step_sound, True), reps)
+        t_sa = _time_ms(lambda: adjoint.stage_tendencies_adjoint(
+            sd, cfg_dry, lam_t), reps)
+        rec.update({"stage_pass_ms": round(t_sp, 4),
+                    "stage_pass_adjoint_ms": round(t_sa, 4),
+                    "stage_pass_adjoint_to_forward": round(t_sa / t_sp, 3)})
+        del sd
         # the four advection kernels at the stage mass fluxes of the
         # state: forward tendency against the adjoint (field and all three
         # mass-flux gradients), both writing fresh zeroed outputs
