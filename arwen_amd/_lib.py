@@ -101,6 +101,29 @@ class ArwStageAdj(ctypes.Structure):
 
 ArwStageAdj64 = ArwStageAdj
 
+#: outputs of the small-step initialisation, inputs of the finish
+INIT_OUT = ("u_pp", "v_pp", "w_pp", "th_pp", "ph_pp", "mu_pp", "al_pp",
+            "p_pp", "p_pp_old")
+FINISH_PP = ("u_pp", "v_pp", "w_pp", "th_pp", "ph_pp", "mu_pp")
+
+
+class ArwInitAdj(ctypes.Structure):
+    """arw_init_adj (and arw64_init_adj: same layout)."""
+    _fields_ = ([("l_" + n, _P) for n in INIT_OUT]
+                + [("g_" + n + "0", _P) for n in STAGE_STATE]
+                + [("g_" + n, _P) for n in STAGE_STATE])
+
+
+class ArwFinishAdj(ctypes.Structure):
+    """arw_finish_adj (and arw64_finish_adj: same layout)."""
+    _fields_ = ([("l_" + n, _P) for n in STAGE_STATE]
+                + [("g_" + n, _P) for n in STAGE_STATE]
+                + [("g_" + n, _P) for n in FINISH_PP])
+
+
+ArwInitAdj64 = ArwInitAdj
+ArwFinishAdj64 = ArwFinishAdj
+
 
 class ArwenError(RuntimeError):
     """A library entry point returned a nonzero status."""
@@ -152,6 +175,12 @@ def load():
                                           _P, _P], _I),
         "arw_stage_adjoint_work": ([_I, _I, _I,
                                     ctypes.POINTER(ctypes.c_int64)], _I),
+        "arw_small_step_init_adjoint": ([dp, ctypes.POINTER(ArwInitAdj), _P,
+                                         _P], _I),
+        "arw_small_step_finish_adjoint": ([dp, ctypes.POINTER(ArwFinishAdj),
+                                           _P, _P], _I),
+        "arw_small_step_adjoint_work": ([_I, _I, _I,
+                                         ctypes.POINTER(ctypes.c_int64)], _I),
         "arw_ensemble_create": ([dp, _I, pp, _P, _I,
                                  ctypes.POINTER(ctypes.c_void_p)], _I),
         "arw_ensemble_step": ([_P, _I, _P], _I),
@@ -213,7 +242,8 @@ FP64_ENTRY_POINTS = (
     "arw64_ensemble_step",
     "arw64_ensemble_capture", "arw64_ensemble_launch",
     "arw64_ensemble_destroy", "arw64_stage_tendencies_adjoint",
-    "arw64_stage_adjoint_work") + tuple("arw64_" + n for n in FLUX_DIV_ADJOINTS)
+    "arw64_stage_adjoint_work", "arw64_small_step_init_adjoint",
+    "arw64_small_step_finish_adjoint", "arw64_small_step_adjoint_work") + tuple("arw64_" + n for n in FLUX_DIV_ADJOINTS)
 
 
 def exported_symbols() -> list[str]:
@@ -230,7 +260,9 @@ def exported_symbols() -> list[str]:
             "arw_ensemble_create",
             "arw_ensemble_step", "arw_ensemble_capture",
             "arw_ensemble_launch", "arw_ensemble_destroy",
-            "arw_stage_tendencies_adjoint", "arw_stage_adjoint_work"] + [
+            "arw_stage_tendencies_adjoint", "arw_stage_adjoint_work",
+            "arw_small_step_init_adjoint", "arw_small_step_finish_adjoint",
+            "arw_small_step_adjoint_work"] + [
             "arw_" + n for n in FLUX_DIV_ADJOINTS]
 
 

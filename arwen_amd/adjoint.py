@@ -324,3 +324,75 @@ class StageTendencies(torch.autograd.Function):
                for n, g in zip(STAGE_TEND, grads)}
         g = stage_tendencies_adjoint(st, ctx.cfg, lam)
         return (None, None) + tuple(g[n] for n in STAGE_STATE)
+
+
+# ---------------------------------------------------------------------------
+# Small-step initialisation and finish
+# ---------------------------------------------------------------------------
+
+INIT_OUT = _lib.INIT_OUT
+FINISH_PP = _lib.FINISH_PP
+
+
+def _small_step_work(state):
+    lib = _lib.load()
+    pre = "arw64_" if state.dtype == torch.float64 else "arw_"
+    nz, ny, nx = state.shape
+    n = ctypes.c_int64()
+    _lib.check(getattr(lib, pre + "small_step_adjoint_work")(
+        nz, ny, nx, ctypes.byref(n)), pre + "small_step_adjoint_work")
+    return lib, pre, state.scratch((n.value,), "arw_small_step_adjoint_work")
+
+
+def small_step_init_adjoint(state: DomainState, lam: dict) -> dict:
+    """J^T lam of the small-step initialisation (reference dycore.cu:
+    947-1085) at the state's time-t fields (u0..mup0) and stage fields
+    (u..mup): ``lam`` maps INIT_OUT names to adjoints (missing = 0);
+    returns gradients named u0..mup0 and u..mup (the latter including the
+    stage EOS's share).  Dry, plain form; the state is not modified."""
+    _check_state(state)
+    lib, pre, work = _small_step_work(state)
+    a = _lib.ArwInitAdj()
+    keep = []
+    for name in INIT_OUT:
+        t = lam.get(name)
+        if t is None:
+            t = torch.zeros_like(getattr(state, name))
+        t = _as_like(t, getattr(state, name), name)
+        keep.append(t)
+        setattr(a, "l_" + name, t.data_ptr())
+    out = {}
+    for name in STAGE_STATE:
+        for n in (name + "0", name):
+            out[n] = torch.empty_like(getattr(state, name))
+            setattr(a, "g_" + n, out[n].data_ptr())
+    _lib.check(getattr(lib, pre + "small_step_init_adjoint")(
+        state.domain_struct(), ctypes.byref(a), work.data_ptr(), _stream()),
+        pre + "small_step_init_adjoint")
+    return out
+
+
+def small_step_finish_adjoint(state: DomainState, lam: dict) -> dict:
+    """J^T lam of the small-step finish (reference dycore.cu:1087-1198) at
+    the state's stage fields and small-step fields: ``lam`` maps the new
+    stage fields (STAGE_STATE names) to adjoints; returns gradients of
+    STAGE_STATE and FINISH_PP."""
+    _check_state(state)
+    lib, pre, work = _small_step_work(state)
+    a = _lib.ArwFinishAdj()
+    keep = []
+    for name in STAGE_STATE:
+        t = lam.get(name)
+        if t is None:
+            t = torch.zeros_like(getattr(state, name))
+        t = _as_like(t, getattr(state, name), name)
+        keep.append(t)
+        setattr(a, "l_" + name, t.data_ptr())
+    out = {}
+    for name in STAGE_STATE + FINISH_PP:
+        out[name] = torch.empty_like(getattr(state, name))
+        setattr(a, "g_" + name, out[name].data_ptr())
+    _lib.check(getattr(lib, pre + "small_step_finish_adjoint")(
+        state.domain_struct(), ctypes.byref(a), work.data_ptr(), _stream()),
+        pre + "small_step_finish_adjoint")
+    return out

@@ -356,6 +356,13 @@ __global__ void k_faces(Dom d, real* ru, real* rv)
     if (i < d.nx) rv[iV(k, j, i, d.ny, d.nx)] = rv_at(d, k, j, i);
 }
 
+hipError_t run_eos(const Dom& d, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_eos<false>, dim3((d.ny * d.nx + 255) / 256),
+                       dim3(256), 0, st, d);
+    return hipGetLastError();
+}
+
 hipError_t run_faces(const Dom& d, real* ru, real* rv, hipStream_t st)
 {
     dim3 grid((d.nx + 1 + 63) / 64, (d.ny + 1 + 3) / 4, d.nz);

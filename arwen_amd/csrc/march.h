@@ -103,6 +103,8 @@ int32_t lib_check_domain(const Dom* d);
 int32_t lib_check_prec(const arw_params* p, const Dom* d);
 
 hipError_t run_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st);
+// The plain EOS of the state into d.p, d.al, d.alt (dycore.hip, k_eos).
+hipError_t run_eos(const Dom& d, hipStream_t st);
 // Coupled mass fluxes ru, rv of the state (dycore.hip, k_faces).
 hipError_t run_faces(const Dom& d, real* ru, real* rv, hipStream_t st);
 // The advection adjoints of advect_adj.hip (op 0 scalar, 1 u, 2 v, 3 w),

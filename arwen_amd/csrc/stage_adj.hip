@@ -21,6 +21,10 @@
 //              (and the mass, parked per face)
 //   k_sa_eos   per mass point: the coupling's mass share and the EOS
 //   k_sa_php, k_sa_mu  the EOS's phi' share; the column sum into mu'
+//
+// Also the transposes of the small-step initialisation (through the stage
+// EOS) and of the small-step finish, per point with a column pass for the
+// column masses.
 #include "common.h"
 #include "march.h"
 
@@ -484,11 +488,402 @@ __global__ void k_sa_mu(SA a)
     a.gmup[c] = s;
 }
 
+// ---------------------------------------------------------------------------
+// Small-step initialisation (dycore.cu:947-1085) and finish (:1087-1198)
+// ---------------------------------------------------------------------------
+
+enum {
+    I_P, I_AL, I_ALT, I_GP, I_GALT, I_GDPH, I_GDPH2, I_M3, I_S3, I_U3,
+    I_U3B, I_MTW, I_MSW, I_PU0, I_PU, I_PV0, I_PV, I_COUNT
+};
+enum { F_S3, F_N3, F_SW, F_NW, F_PSU, F_PNU, F_PSV, F_PNV, F_COUNT };
+
+struct IA {
+    Dom d;                   // p / al / alt point at the recomputed EOS
+    const real *lu, *lv, *lw, *lth, *lph, *lmu, *lal, *lp, *lpo;
+    real *gu0, *gv0, *gw0, *gthp0, *gphp0, *gmup0;
+    real *gu, *gv, *gw, *gthp, *gphp, *gmup;
+    real* w[I_COUNT];
+};
+
+__device__ __forceinline__ real mu0_c(const Dom& d, int c)
+{
+    return d.mub2d[c] + d.mup0[c];
+}
+
+// per point of the union grid: the mass, w-level and face parts of the
+// transposed initialisation
+__global__ void k_ia_pts(IA a)
+{
+    const Dom& d = a.d;
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    if (i < nx && j < ny) {
+        const int c = j * nx + i;
+        const real mt = mu0_c(d, c), ms = mut_c(d, c);
+        if (k < nz) {
+            const int t = k * st + c;
+            const real c1 = d.c1h[k], c2 = d.c2h[k];
+            const real thb = bv(d.thb, k, c, st, d.base3d);
+            const real tht = thb + d.thp0[t], ths = thb + d.thp[t];
+            const real cht = c1 * mt + c2, chs = c1 * ms + c2;
+            const real mupp = d.mup0[c] - d.mup[c];
+            const real dpp = (d.php0[t + st] - d.php[t + st])
+                             - (d.php0[t] - d.php[t]);
+            const real thpp = cht * tht - chs * ths;
+            const real p = a.w[I_P][t], alt = a.w[I_ALT][t];
+            const real c2a = GAMMA_F * p / alt;
+            const real al = -((alt * (c1 * mupp)) + d.rdnw[k] * dpp) / cht;
+            const real N = thpp - (c1 * mupp) * ths;
+            const real p5 = (alt * N) / (cht * ths);
+            const real Lp = a.lp[t] + a.lpo[t];
+            const real g_c2a = Lp * (p5 - al), g_p5 = Lp * c2a;
+            const real g_al = a.lal[t] - Lp * c2a;
+            real g_alt = g_p5 * N / (cht * ths);
+            const real g_N = g_p5 * alt / (cht * ths);
+            real g_cht = -g_p5 * p5 / cht;
+            real g_ths = -g_p5 * p5 / ths - g_N * (c1 * mupp);
+            const real g_thpp = g_N + a.lth[t];
+            real g_mupp = -g_N * c1 * ths;
+            g_alt = g_alt - g_al * c1 * mupp / cht;
+            g_mupp = g_mupp - g_al * alt * c1 / cht;
+            const real g_dph = -g_al * d.rdnw[k] / cht;
+            g_cht = g_cht - g_al * al / cht;
+            const real g_p = g_c2a * GAMMA_F / alt;
+            g_alt = g_alt - g_c2a * c2a / alt;
+            g_cht = g_cht + g_thpp * tht;
+            const real g_chs = -g_thpp * ths;
+            g_ths = g_ths - g_thpp * chs;
+            a.gthp0[t] = g_thpp * cht;
+            a.gthp[t] = g_ths;
+            a.w[I_GP][t] = g_p;
+            a.w[I_GALT][t] = g_alt;
+            a.w[I_GDPH][t] = g_dph;
+            a.w[I_M3][t] = c1 * g_cht;
+            a.w[I_S3][t] = c1 * g_chs;
+            a.w[I_U3][t] = g_mupp;
+        }
+        const int t = k * st + c;              // w level k
+        const real lw = a.lw[t];
+        a.gw0[t] = (d.c1f[k] * mt + d.c2f[k]) * lw;
+        a.gw[t] = -((d.c1f[k] * ms + d.c2f[k]) * lw);
+        a.w[I_MTW][t] = d.c1f[k] * d.w0[t] * lw;
+        a.w[I_MSW][t] = -(d.c1f[k] * d.w[t] * lw);
+    }
+    if (k < nz && j < ny) {                    // u face (k, j, i)
+        const int x = iU(k, j, i, ny, nx);
+        const int ca = j * nx + (i >= nx ? i - nx : i);
+        const int cb = j * nx + (i == 0 ? nx - 1 : i - 1);
+        const real mtf = R(0.5) * (mu0_c(d, ca) + mu0_c(d, cb));
+        const real msf = R(0.5) * (mut_c(d, ca) + mut_c(d, cb));
+        const real c1 = d.c1h[k], c2 = d.c2h[k], L = a.lu[x];
+        a.gu0[x] = (c1 * mtf + c2) * L;
+        a.gu[x] = -((c1 * msf + c2) * L);
+        a.w[I_PU0][x] = c1 * d.u0[x] * L;
+        a.w[I_PU][x] = -(c1 * d.u[x] * L);
+    }
+    if (k < nz && i < nx) {                    // v face (k, j, i)
+        const int x = iV(k, j, i, ny, nx);
+        const int ca = (j >= ny ? j - ny : j) * nx + i;
+        const int cb = (j == 0 ? ny - 1 : j - 1) * nx + i;
+        const real mtf = R(0.5) * (mu0_c(d, ca) + mu0_c(d, cb));
+        const real msf = R(0.5) * (mut_c(d, ca) + mut_c(d, cb));
+        const real c1 = d.c1h[k], c2 = d.c2h[k], L = a.lv[x];
+        a.gv0[x] = (c1 * mtf + c2) * L;
+        a.gv[x] = -((c1 * msf + c2) * L);
+        a.w[I_PV0][x] = c1 * d.v0[x] * L;
+        a.w[I_PV][x] = -(c1 * d.v[x] * L);
+    }
+}
+
+// the stage EOS's transpose (p and alt gradients of the initialisation)
+__global__ void k_ia_eos(IA a)
+{
+    const Dom& d = a.d;
+    const int nz = d.nz, st = d.ny * d.nx;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nz * st) return;
+    const int k = t / st, c = t - k * st;
+    const real p = a.w[I_P][t], alt = a.w[I_ALT][t];
+    const real th = bv(d.thb, k, c, st, d.base3d) + d.thp[t];
+    const real mh = d.c1h[k] * mut_c(d, c) + d.c2h[k];
+    const real gp = a.w[I_GP][t];
+    const real ga = a.w[I_GALT][t] - gp * (GAMMA_F * p / alt);
+    a.gthp[t] += gp * (GAMMA_F * p / th);
+    a.w[I_GDPH2][t] = ga * (-d.rdnw[k] / mh);
+    a.w[I_U3B][t] = ga * (-alt * d.c1h[k] / mh);
+}
+
+// w levels: ph'' = php0 - php through al'', and the EOS's phi' layer
+__global__ void k_ia_php(IA a)
+{
+    const Dom& d = a.d;
+    const int nz = d.nz, st = d.ny * d.nx;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (nz + 1) * st) return;
+    const int k = t / st;
+    real g = a.lph[t], e = R(0.0);
+    if (k >= 1) {
+        g += a.w[I_GDPH][t - st];
+        e += a.w[I_GDPH2][t - st];
+    }
+    if (k <= nz - 1) {
+        g -= a.w[I_GDPH][t];
+        e -= a.w[I_GDPH2][t];
+    }
+    a.gphp0[t] = g;
+    a.gphp[t] = -g + e;
+}
+
+// the face products of a column folded onto the periodic faces c, c+1
+__device__ __forceinline__ real face_pair_u(const real* P, int k, int j,
+                                            int i, int ny, int nx)
+{
+    auto f = [&](int ff) {
+        real v = P[iU(k, j, ff, ny, nx)];
+        if (ff == 0) v += P[iU(k, j, nx, ny, nx)];
+        return v;
+    };
+    return R(0.5) * (f(i) + f(i + 1 == nx ? 0 : i + 1));
+}
+
+__device__ __forceinline__ real face_pair_v(const real* P, int k, int j,
+                                            int i, int ny, int nx)
+{
+    auto f = [&](int gg) {
+        real v = P[iV(k, gg, i, ny, nx)];
+        if (gg == 0) v += P[iV(k, ny, i, ny, nx)];
+        return v;
+    };
+    return R(0.5) * (f(j) + f(j + 1 == ny ? 0 : j + 1));
+}
+
+// per column: the time-t and stage mass gradients
+__global__ void k_ia_cols(IA a)
+{
+    const Dom& d = a.d;
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= st) return;
+    const int j = c / nx, i = c - j * nx;
+    real gmt = R(0.0), gms = R(0.0), gmupp = a.lmu[c], geos = R(0.0);
+    for (int k = 0; k < nz; ++k) {
+        const int t = k * st + c;
+        gmt += a.w[I_M3][t] + face_pair_u(a.w[I_PU0], k, j, i, ny, nx)
+               + face_pair_v(a.w[I_PV0], k, j, i, ny, nx);
+        gms += a.w[I_S3][t] + face_pair_u(a.w[I_PU], k, j, i, ny, nx)
+               + face_pair_v(a.w[I_PV], k, j, i, ny, nx);
+        gmupp += a.w[I_U3][t];
+        geos += a.w[I_U3B][t];
+    }
+    for (int k = 0; k <= nz; ++k) {
+        gmt += a.w[I_MTW][k * st + c];
+        gms += a.w[I_MSW][k * st + c];
+    }
+    a.gmup0[c] = gmupp + gmt;
+    a.gmup[c] = (gms - gmupp) + geos;
+}
+
+struct FA {
+    Dom d;
+    const real *lu, *lv, *lw, *lthp, *lphp, *lmup;
+    real *gu, *gv, *gw, *gthp, *gphp, *gmup;
+    real *gupp, *gvpp, *gwpp, *gthpp, *gphpp, *gmupp;
+    real* w[F_COUNT];
+};
+
+__global__ void k_fa_pts(FA a)
+{
+    const Dom& d = a.d;
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    auto mn_c = [&](int c) { return mut_c(d, c) + d.mu_pp[c]; };
+    if (k < nz && j < ny) {                    // u face (k, j, i)
+        const int x = iU(k, j, i, ny, nx);
+        const int ca = j * nx + (i >= nx ? i - nx : i);
+        const int cb = j * nx + (i == 0 ? nx - 1 : i - 1);
+        const real msf = R(0.5) * (mut_c(d, ca) + mut_c(d, cb));
+        const real mnf = R(0.5) * (mn_c(ca) + mn_c(cb));
+        const real c1 = d.c1h[k], c2 = d.c2h[k];
+        const real den = c1 * mnf + c2;
+        const real nw = ((c1 * msf + c2) * d.u[x] + d.u_pp[x]) / den;
+        const real gn = a.lu[x] / den;
+        a.gu[x] = (c1 * msf + c2) * gn;
+        a.gupp[x] = gn;
+        a.w[F_PSU][x] = c1 * d.u[x] * gn;
+        a.w[F_PNU][x] = -(c1 * gn * nw);
+    }
+    if (k < nz && i < nx) {                    // v face (k, j, i)
+        const int x = iV(k, j, i, ny, nx);
+        const int ca = (j >= ny ? j - ny : j) * nx + i;
+        const int cb = (j == 0 ? ny - 1 : j - 1) * nx + i;
+        const real msf = R(0.5) * (mut_c(d, ca) + mut_c(d, cb));
+        const real mnf = R(0.5) * (mn_c(ca) + mn_c(cb));
+        const real c1 = d.c1h[k], c2 = d.c2h[k];
+        const real den = c1 * mnf + c2;
+        const real nw = ((c1 * msf + c2) * d.v[x] + d.v_pp[x]) / den;
+        const real gn = a.lv[x] / den;
+        a.gv[x] = (c1 * msf + c2) * gn;
+        a.gvpp[x] = gn;
+        a.w[F_PSV][x] = c1 * d.v[x] * gn;
+        a.w[F_PNV][x] = -(c1 * gn * nw);
+    }
+    if (i < nx && j < ny) {
+        const int c = j * nx + i;
+        const real ms = mut_c(d, c), mn = ms + d.mu_pp[c];
+        const int t = k * st + c;               // w level k
+        {
+            const real c1 = d.c1f[k], c2 = d.c2f[k];
+            const real den = c1 * mn + c2;
+            const real nw = ((c1 * ms + c2) * d.w[t] + d.w_pp[t]) / den;
+            const real gn = a.lw[t] / den;
+            a.gw[t] = (c1 * ms + c2) * gn;
+            a.gwpp[t] = gn;
+            a.w[F_SW][t] = c1 * d.w[t] * gn;
+            a.w[F_NW][t] = -(c1 * gn * nw);
+            a.gphp[t] = a.lphp[t];
+            a.gphpp[t] = a.lphp[t];
+        }
+        if (k < nz) {                           // mass level k
+            const real c1 = d.c1h[k], c2 = d.c2h[k];
+            const real thb = bv(d.thb, k, c, st, d.base3d);
+            const real den = c1 * mn + c2;
+            const real th = ((c1 * ms + c2) * (thb + d.thp[t]) + d.th_pp[t]) / den;
+            const real gn = a.lthp[t] / den;
+            a.gthp[t] = (c1 * ms + c2) * gn;
+            a.gthpp[t] = gn;
+            a.w[F_S3][t] = c1 * (thb + d.thp[t]) * gn;
+            a.w[F_N3][t] = -(c1 * gn * th);
+        }
+    }
+}
+
+__global__ void k_fa_cols(FA a)
+{
+    const Dom& d = a.d;
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= st) return;
+    const int j = c / nx, i = c - j * nx;
+    real gms = R(0.0), gmn = R(0.0);
+    for (int k = 0; k < nz; ++k) {
+        gms += a.w[F_S3][k * st + c] + face_pair_u(a.w[F_PSU], k, j, i, ny, nx)
+               + face_pair_v(a.w[F_PSV], k, j, i, ny, nx);
+        gmn += a.w[F_N3][k * st + c] + face_pair_u(a.w[F_PNU], k, j, i, ny, nx)
+               + face_pair_v(a.w[F_PNV], k, j, i, ny, nx);
+    }
+    for (int k = 0; k <= nz; ++k) {
+        gms += a.w[F_SW][k * st + c];
+        gmn += a.w[F_NW][k * st + c];
+    }
+    const real L = a.lmup[c];
+    a.gmup[c] = (L + gms) + gmn;
+    a.gmupp[c] = L + gmn;
+}
+
 }  // namespace
 
 }  // namespace ARW_NS
 
 using namespace ARW_NS;
+
+extern "C" int32_t ARW_API(small_step_adjoint_work)(int32_t nz, int32_t ny,
+                                                    int32_t nx,
+                                                    int64_t* elements)
+{
+    if (!elements) return lib_fail(ARW_E_ARG, "null argument");
+    if (nz < 3 || ny < 1 || nx < 1) return lib_fail(ARW_E_ARG, "bad extents");
+    *elements = (int64_t)I_COUNT * (nz + 1) * (ny + 1) * (nx + 1);
+    return 0;
+}
+
+extern "C" int32_t ARW_API(small_step_init_adjoint)(
+    const Dom* d, const ARW_API(init_adj)* adj, real* work, void* stream)
+{
+    int32_t rc = lib_check_domain(d);
+    if (rc) return rc;
+    if (!adj || !work) return lib_fail(ARW_E_ARG, "null argument");
+    const real* lin[] = {adj->l_u_pp, adj->l_v_pp, adj->l_w_pp, adj->l_th_pp,
+                         adj->l_ph_pp, adj->l_mu_pp, adj->l_al_pp,
+                         adj->l_p_pp, adj->l_p_pp_old};
+    real* gout[] = {adj->g_u0, adj->g_v0, adj->g_w0, adj->g_thp0,
+                    adj->g_php0, adj->g_mup0, adj->g_u, adj->g_v, adj->g_w,
+                    adj->g_thp, adj->g_php, adj->g_mup};
+    for (const real* p : lin)
+        if (!p) return lib_fail(ARW_E_ARG, "null output adjoint");
+    for (real* p : gout)
+        if (!p) return lib_fail(ARW_E_ARG, "null input adjoint");
+    if (d->qv) return lib_fail(ARW_E_UNSUPPORTED, "init adjoint: dry states only");
+    const int nz = d->nz, ny = d->ny, nx = d->nx, st = ny * nx;
+    if (nz < 3) return lib_fail(ARW_E_ARG, "init adjoint needs nz >= 3");
+    hipStream_t sm = (hipStream_t)stream;
+    const size_t F = (size_t)(nz + 1) * (ny + 1) * (nx + 1);
+    IA a;
+    a.d = *d;
+    for (int w = 0; w < I_COUNT; ++w) a.w[w] = work + (size_t)w * F;
+    a.lu = adj->l_u_pp; a.lv = adj->l_v_pp; a.lw = adj->l_w_pp;
+    a.lth = adj->l_th_pp; a.lph = adj->l_ph_pp; a.lmu = adj->l_mu_pp;
+    a.lal = adj->l_al_pp; a.lp = adj->l_p_pp; a.lpo = adj->l_p_pp_old;
+    a.gu0 = adj->g_u0; a.gv0 = adj->g_v0; a.gw0 = adj->g_w0;
+    a.gthp0 = adj->g_thp0; a.gphp0 = adj->g_php0; a.gmup0 = adj->g_mup0;
+    a.gu = adj->g_u; a.gv = adj->g_v; a.gw = adj->g_w; a.gthp = adj->g_thp;
+    a.gphp = adj->g_php; a.gmup = adj->g_mup;
+    a.d.p = a.w[I_P]; a.d.al = a.w[I_AL]; a.d.alt = a.w[I_ALT];
+    if ((rc = lib_launched(run_eos(a.d, sm), "k_eos (init adjoint)"))) return rc;
+    const dim3 blk(64, 4);
+    const dim3 gu((nx + 1 + 63) / 64, (ny + 1 + 3) / 4, nz + 1);
+    hipLaunchKernelGGL(k_ia_pts, gu, blk, 0, sm, a);
+    if ((rc = lib_launched(hipGetLastError(), "k_ia_pts"))) return rc;
+    const int nm = nz * st, nwl = (nz + 1) * st;
+    hipLaunchKernelGGL(k_ia_eos, dim3((nm + 255) / 256), dim3(256), 0, sm, a);
+    if ((rc = lib_launched(hipGetLastError(), "k_ia_eos"))) return rc;
+    hipLaunchKernelGGL(k_ia_php, dim3((nwl + 255) / 256), dim3(256), 0, sm, a);
+    if ((rc = lib_launched(hipGetLastError(), "k_ia_php"))) return rc;
+    hipLaunchKernelGGL(k_ia_cols, dim3((st + 255) / 256), dim3(256), 0, sm, a);
+    return lib_launched(hipGetLastError(), "k_ia_cols");
+}
+
+extern "C" int32_t ARW_API(small_step_finish_adjoint)(
+    const Dom* d, const ARW_API(finish_adj)* adj, real* work, void* stream)
+{
+    int32_t rc = lib_check_domain(d);
+    if (rc) return rc;
+    if (!adj || !work) return lib_fail(ARW_E_ARG, "null argument");
+    const real* lin[] = {adj->l_u, adj->l_v, adj->l_w, adj->l_thp, adj->l_php,
+                         adj->l_mup};
+    real* gout[] = {adj->g_u, adj->g_v, adj->g_w, adj->g_thp, adj->g_php,
+                    adj->g_mup, adj->g_u_pp, adj->g_v_pp, adj->g_w_pp,
+                    adj->g_th_pp, adj->g_ph_pp, adj->g_mu_pp};
+    for (const real* p : lin)
+        if (!p) return lib_fail(ARW_E_ARG, "null output adjoint");
+    for (real* p : gout)
+        if (!p) return lib_fail(ARW_E_ARG, "null input adjoint");
+    const int nz = d->nz, ny = d->ny, nx = d->nx, st = ny * nx;
+    if (nz < 3) return lib_fail(ARW_E_ARG, "finish adjoint needs nz >= 3");
+    hipStream_t sm = (hipStream_t)stream;
+    const size_t F = (size_t)(nz + 1) * (ny + 1) * (nx + 1);
+    FA a;
+    a.d = *d;
+    for (int w = 0; w < F_COUNT; ++w) a.w[w] = work + (size_t)w * F;
+    a.lu = adj->l_u; a.lv = adj->l_v; a.lw = adj->l_w; a.lthp = adj->l_thp;
+    a.lphp = adj->l_php; a.lmup = adj->l_mup;
+    a.gu = adj->g_u; a.gv = adj->g_v; a.gw = adj->g_w; a.gthp = adj->g_thp;
+    a.gphp = adj->g_php; a.gmup = adj->g_mup;
+    a.gupp = adj->g_u_pp; a.gvpp = adj->g_v_pp; a.gwpp = adj->g_w_pp;
+    a.gthpp = adj->g_th_pp; a.gphpp = adj->g_ph_pp; a.gmupp = adj->g_mu_pp;
+    const dim3 blk(64, 4);
+    const dim3 gu((nx + 1 + 63) / 64, (ny + 1 + 3) / 4, nz + 1);
+    hipLaunchKernelGGL(k_fa_pts, gu, blk, 0, sm, a);
+    if ((rc = lib_launched(hipGetLastError(), "k_fa_pts"))) return rc;
+    hipLaunchKernelGGL(k_fa_cols, dim3((st + 255) / 256), dim3(256), 0, sm, a);
+    return lib_launched(hipGetLastError(), "k_fa_cols");
+}
 
 extern "C" int32_t ARW_API(stage_adjoint_work)(int32_t nz, int32_t ny,
                                                int32_t nx, int64_t* elements)

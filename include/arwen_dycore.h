@@ -299,6 +299,34 @@ int32_t arw_stage_tendencies_adjoint(const arw_domain *d,
 int32_t arw_stage_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
                                int64_t *elements);
 
+/* Adjoints of the small-step initialisation and finish (no counterpart in
+ * the reference; dycore.cu:947-1085 and :1087-1198, plain form, dry).
+ * The initialisation's transpose takes the adjoints of u_pp..p_pp_old and
+ * WRITES the gradients of the time-t fields (g_*0) and of the stage fields
+ * (g_*, including through the stage EOS's p and alt, recomputed into
+ * work).  The finish's transpose takes the adjoints of the new stage
+ * fields and WRITES the gradients of the stage fields and of the
+ * small-step fields.  work: arw_small_step_adjoint_work elements. */
+typedef struct arw_init_adj {
+    const float *l_u_pp, *l_v_pp, *l_w_pp, *l_th_pp, *l_ph_pp, *l_mu_pp,
+        *l_al_pp, *l_p_pp, *l_p_pp_old;
+    float *g_u0, *g_v0, *g_w0, *g_thp0, *g_php0, *g_mup0;
+    float *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+} arw_init_adj;
+typedef struct arw_finish_adj {
+    const float *l_u, *l_v, *l_w, *l_thp, *l_php, *l_mup;
+    float *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+    float *g_u_pp, *g_v_pp, *g_w_pp, *g_th_pp, *g_ph_pp, *g_mu_pp;
+} arw_finish_adj;
+int32_t arw_small_step_init_adjoint(const arw_domain *d,
+                                    const arw_init_adj *adj, float *work,
+                                    void *stream);
+int32_t arw_small_step_finish_adjoint(const arw_domain *d,
+                                      const arw_finish_adj *adj, float *work,
+                                      void *stream);
+int32_t arw_small_step_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
+                                    int64_t *elements);
+
 /* Adjoints of the four advection kernels above (no counterpart in the
  * reference, which has no adjoint): for the output adjoint lam (shaped like
  * that kernel's tend) they ADD the transposed Jacobian of tend with respect
@@ -510,6 +538,26 @@ int32_t arw64_stage_tendencies_adjoint(const arw64_domain *d,
                                        double *work, void *stream);
 int32_t arw64_stage_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
                                  int64_t *elements);
+
+typedef struct arw64_init_adj {
+    const double *l_u_pp, *l_v_pp, *l_w_pp, *l_th_pp, *l_ph_pp, *l_mu_pp,
+        *l_al_pp, *l_p_pp, *l_p_pp_old;
+    double *g_u0, *g_v0, *g_w0, *g_thp0, *g_php0, *g_mup0;
+    double *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+} arw64_init_adj;
+typedef struct arw64_finish_adj {
+    const double *l_u, *l_v, *l_w, *l_thp, *l_php, *l_mup;
+    double *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+    double *g_u_pp, *g_v_pp, *g_w_pp, *g_th_pp, *g_ph_pp, *g_mu_pp;
+} arw64_finish_adj;
+int32_t arw64_small_step_init_adjoint(const arw64_domain *d,
+                                      const arw64_init_adj *adj,
+                                      double *work, void *stream);
+int32_t arw64_small_step_finish_adjoint(const arw64_domain *d,
+                                        const arw64_finish_adj *adj,
+                                        double *work, void *stream);
+int32_t arw64_small_step_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
+                                      int64_t *elements);
 
 typedef struct arw64_ensemble arw64_ensemble;
 int32_t arw64_ensemble_create(const arw64_domain *members, int32_t n,

@@ -925,3 +925,169 @@ def stage_tendencies_adjoint(s, lam, dx, dy, top_lid=True):
     g["php"] += gph
     g["mup"] += gm
     return g
+
+
+# ---------------------------------------------------------------------------
+# Small-step initialisation and finish (plain form)
+# ---------------------------------------------------------------------------
+
+INIT_OUT = ("u_pp", "v_pp", "w_pp", "th_pp", "ph_pp", "mu_pp", "al_pp",
+            "p_pp", "p_pp_old")
+FINISH_IN = ("u_pp", "v_pp", "w_pp", "th_pp", "ph_pp", "mu_pp")
+
+
+def _faces_to_mass(g, ax):
+    """Transpose of a periodic face average with a duplicate face: g per
+    face (n+1 along ax) of 0.5 (m[f] + m[f-1]) -> per mass column."""
+    gf = _face_fold(g, ax)
+    return 0.5 * (gf + np.roll(gf, -1, axis=ax))
+
+
+def init_forward(s):
+    """small_step_init at the state s (time-t fields in u0..mup0, stage
+    fields in u..mup, the stage EOS recomputed first); returns copies of
+    INIT_OUT."""
+    D.calc_p_alpha(s)
+    D.small_step_init(s)
+    return {n: getattr(s, n).copy() for n in INIT_OUT}
+
+
+def init_adjoint(s, lam):
+    """Transpose of init_forward: gradients of the time-t fields
+    (u0..mup0) and of the stage fields (u..mup, through the stage EOS's p
+    and alt as well) for lam over INIT_OUT (reference dycore.cu:947-1085)."""
+    w = D.OState(**{k: (v.copy() if isinstance(v, np.ndarray) else v)
+                    for k, v in vars(s).items()})
+    D.calc_p_alpha(w)
+    z = lambda n: lam.get(n, np.zeros_like(getattr(s, n)))
+    Lu, Lv, Lw, Lth, Lph, Lmu, Lal = (z(n) for n in INIT_OUT[:7])
+    Lp = z("p_pp") + z("p_pp_old")
+    c1h, c2h = w.c1h[:, None, None], w.c2h[:, None, None]
+    c1f, c2f = w.c1f[:, None, None], w.c2f[:, None, None]
+    rdnw = w.rdnw[:, None, None]
+    mt = w.mub2d + w.mup0
+    ms = w.mub2d + w.mup
+    thb = D._b3(w.thb)
+    tht, ths = thb + w.thp0, thb + w.thp
+    cht, chs = c1h * mt + c2h, c1h * ms + c2h
+    mupp = w.mup0 - w.mup
+    phpp = w.php0 - w.php
+    thpp = cht * tht - chs * ths
+    c2a = GAMMA * w.p / w.alt
+    al = -((w.alt * (c1h * mupp)) + rdnw * (phpp[1:] - phpp[:-1])) / cht
+    N = thpp - (c1h * mupp) * ths
+    p5 = (w.alt * N) / (cht * ths)
+    g = {n: np.zeros_like(getattr(s, n)) for n in STAGE_IN}
+    g.update({n + "0": np.zeros_like(getattr(s, n)) for n in STAGE_IN})
+    # p'' = c2a (p5 - al)
+    g_c2a = Lp * (p5 - al)
+    g_p5 = Lp * c2a
+    g_al = Lal - Lp * c2a
+    g_alt = g_p5 * N / (cht * ths)
+    g_N = g_p5 * w.alt / (cht * ths)
+    g_cht = -g_p5 * p5 / cht
+    g_ths = -g_p5 * p5 / ths - g_N * (c1h * mupp)
+    g_thpp = g_N + Lth
+    g_mupp3 = -g_N * c1h * ths
+    # al'' = -(alt c1h mu'' + rdnw dph'') / cht
+    g_alt = g_alt - g_al * c1h * mupp / cht
+    g_mupp3 = g_mupp3 - g_al * w.alt * c1h / cht
+    g_dph = -g_al * rdnw / cht
+    g_phpp = Lph.copy()
+    g_phpp[1:] += g_dph
+    g_phpp[:-1] -= g_dph
+    g_cht = g_cht - g_al * al / cht
+    # c2a = gamma p / alt
+    g_p = g_c2a * GAMMA / w.alt
+    g_alt = g_alt - g_c2a * c2a / w.alt
+    # theta'' = cht tht - chs ths
+    g_cht = g_cht + g_thpp * tht
+    g["thp0"] += g_thpp * cht
+    g_chs = -g_thpp * ths
+    g_ths = g_ths - g_thpp * chs
+    g["thp"] += g_ths
+    g_mt = np.sum(c1h * g_cht, axis=0)
+    g_ms = np.sum(c1h * g_chs, axis=0)
+    # ph'' and mu''
+    g["php0"] += g_phpp
+    g["php"] -= g_phpp
+    g_mupp = Lmu + np.sum(g_mupp3, axis=0)
+    g["mup0"] += g_mupp
+    g["mup"] -= g_mupp
+    # w''
+    g["w0"] += (c1f * mt + c2f) * Lw
+    g["w"] -= (c1f * ms + c2f) * Lw
+    g_mt = g_mt + np.sum(c1f * w.w0 * Lw, axis=0)
+    g_ms = g_ms - np.sum(c1f * w.w * Lw, axis=0)
+    # u'', v''
+    for L, n, ax, fac in ((Lu, "u", -1, D.mu_at_u_faces),
+                          (Lv, "v", -2, D.mu_at_v_faces)):
+        mtf, msf = fac(mt), fac(ms)
+        g[n + "0"] += (c1h * mtf[None] + c2h) * L
+        g[n] -= (c1h * msf[None] + c2h) * L
+        g_mt = g_mt + _faces_to_mass(np.sum(c1h * getattr(w, n + "0") * L,
+                                            axis=0), ax)
+        g_ms = g_ms - _faces_to_mass(np.sum(c1h * getattr(w, n) * L, axis=0),
+                                     ax)
+    g["mup0"] += g_mt
+    g["mup"] += g_ms
+    gt, gph, gm = eos_adjoint(w, g_p, np.zeros_like(w.al), g_alt)
+    g["thp"] += gt
+    g["php"] += gph
+    g["mup"] += gm
+    return g
+
+
+def finish_forward(s):
+    """small_step_finish at s: the new stage fields from the stage fields
+    (u..mup) and the small-step fields; returns copies of STAGE_IN."""
+    D.small_step_finish(s)
+    return {n: getattr(s, n).copy() for n in STAGE_IN}
+
+
+def finish_adjoint(s, lam):
+    """Transpose of finish_forward for lam over STAGE_IN (the new stage
+    fields): gradients of the stage fields and of FINISH_IN (reference
+    dycore.cu:1087-1198)."""
+    z = lambda n: lam.get(n, np.zeros_like(getattr(s, n)))
+    c1h, c2h = s.c1h[:, None, None], s.c2h[:, None, None]
+    c1f, c2f = s.c1f[:, None, None], s.c2f[:, None, None]
+    ms = s.mub2d + s.mup
+    mn = ms + s.mu_pp
+    g = {n: np.zeros_like(getattr(s, n)) for n in STAGE_IN + FINISH_IN}
+    g_ms = np.zeros_like(ms)
+    g_mn = np.zeros_like(ms)
+    for n, pn, ax, fac in (("u", "u_pp", -1, D.mu_at_u_faces),
+                           ("v", "v_pp", -2, D.mu_at_v_faces)):
+        L = z(n)
+        msf, mnf = fac(ms), fac(mn)
+        x, xpp = getattr(s, n), getattr(s, pn)
+        den = c1h * mnf[None] + c2h
+        new = ((c1h * msf[None] + c2h) * x + xpp) / den
+        g_num = L / den
+        g[n] += (c1h * msf[None] + c2h) * g_num
+        g[pn] += g_num
+        g_ms = g_ms + _faces_to_mass(np.sum(c1h * x * g_num, axis=0), ax)
+        g_mn = g_mn + _faces_to_mass(np.sum(-c1h * g_num * new, axis=0), ax)
+    L = z("w")
+    den = c1f * mn + c2f
+    new = ((c1f * ms + c2f) * s.w + s.w_pp) / den
+    g["w"] += (c1f * ms + c2f) * L / den
+    g["w_pp"] += L / den
+    g_ms = g_ms + np.sum(c1f * s.w * L / den, axis=0)
+    g_mn = g_mn - np.sum(c1f * L / den * new, axis=0)
+    L = z("php")
+    g["php"] += L
+    g["ph_pp"] += L
+    L = z("thp")
+    thb = D._b3(s.thb)
+    den = c1h * mn + c2h
+    th_new = ((c1h * ms + c2h) * (thb + s.thp) + s.th_pp) / den
+    g["thp"] += (c1h * ms + c2h) * L / den
+    g["th_pp"] += L / den
+    g_ms = g_ms + np.sum(c1h * (thb + s.thp) * L / den, axis=0)
+    g_mn = g_mn - np.sum(c1h * L / den * th_new, axis=0)
+    L = z("mup")
+    g["mup"] += L + g_ms + g_mn
+    g["mu_pp"] += L + g_mn
+    return g

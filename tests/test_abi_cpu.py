@@ -74,7 +74,11 @@ def _struct_fields(name):
                                        ("arw64_domain", _lib.ArwDomain64),
                                        ("arw_params", _lib.ArwParams),
                                        ("arw_stage_adj", _lib.ArwStageAdj),
-                                       ("arw64_stage_adj", _lib.ArwStageAdj64)])
+                                       ("arw64_stage_adj", _lib.ArwStageAdj64),
+                                       ("arw_init_adj", _lib.ArwInitAdj),
+                                       ("arw64_init_adj", _lib.ArwInitAdj64),
+                                       ("arw_finish_adj", _lib.ArwFinishAdj),
+                                       ("arw64_finish_adj", _lib.ArwFinishAdj64)])
 def test_ctypes_layout_matches_header(cname, pyt):
     kinds = {"ptr": ctypes.c_void_p, "int32_t": ctypes.c_int32,
              "float": ctypes.c_float, "double": ctypes.c_double}

@@ -10,7 +10,10 @@ arw64_*) and its autograd wrapper ``adjoint.StageTendencies``:
   the differencing error);
 - FP32 agrees with FP64 to 1e-4;
 - autograd through StageTendencies gives the same gradient as the direct
-  call, and the state is left unmodified by the adjoint.
+  call, and the state is left unmodified by the adjoint;
+- the small-step initialisation and finish adjoints
+  (arw64_small_step_init_adjoint / _finish_adjoint) equal the oracle's
+  hand adjoints (checked against complex-step JVPs) in FP64.
 """
 import numpy as np
 import pytest
@@ -145,3 +148,44 @@ def test_stage_adjoint_rejects_unsupported():
         adjoint.stage_tendencies_adjoint(g, case.cfg.with_(khdif=75.0,
                                                            kvdif=75.0,
                                                            km_opt=1), lam)
+
+
+def _with_small_step_fields(o, g, seed):
+    """Time-t fields near the stage fields and small-step fields, the same
+    on the oracle state and the device state."""
+    rng = np.random.default_rng(seed)
+    for n in A.STAGE_IN:
+        x = getattr(o, n)
+        getattr(o, n + "0")[...] = x + rng.standard_normal(x.shape) * (
+            0.01 * (np.abs(x).max() + 1.0))
+        getattr(g, n + "0").copy_(torch.as_tensor(getattr(o, n + "0")))
+    for n in A.FINISH_IN:
+        getattr(o, n)[...] = rng.standard_normal(getattr(o, n).shape) * 0.1
+    o.mu_pp[...] *= 50.0
+    for n in A.FINISH_IN:
+        getattr(g, n).copy_(torch.as_tensor(getattr(o, n)))
+
+
+@pytest.mark.parametrize("name", ["dry_flat", "terrain"])
+def test_init_and_finish_adjoints_match_oracle(name):
+    from arwen_amd import adjoint
+    case, o, g = _states(name, seed=21)
+    _with_small_step_fields(o, g, 23)
+    print(f"init/finish adjoints vs oracle {name}", flush=True)
+    gen = torch.Generator(device=g.u.device).manual_seed(25)
+    lam_i = {n: torch.randn(getattr(g, n).shape, dtype=g.dtype,
+                            device=g.u.device, generator=gen)
+             for n in A.INIT_OUT}
+    got = adjoint.small_step_init_adjoint(g, lam_i)
+    ref = A.init_adjoint(o, {n: t.cpu().numpy() for n, t in lam_i.items()})
+    for n, r in ref.items():
+        err = np.abs(got[n].cpu().numpy() - r).max()
+        assert err <= ORACLE_TOL * np.abs(r).max(), ("init", n, err)
+    lam_f = {n: torch.randn(getattr(g, n).shape, dtype=g.dtype,
+                            device=g.u.device, generator=gen)
+             for n in A.STAGE_IN}
+    got = adjoint.small_step_finish_adjoint(g, lam_f)
+    ref = A.finish_adjoint(o, {n: t.cpu().numpy() for n, t in lam_f.items()})
+    for n, r in ref.items():
+        err = np.abs(got[n].cpu().numpy() - r).max()
+        assert err <= ORACLE_TOL * np.abs(r).max(), ("finish", n, err)

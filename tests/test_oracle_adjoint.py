@@ -338,3 +338,42 @@ def test_stage_tendency_adjoint_dot_product_vs_complex_step(name, top_lid):
                  for n in A.STAGE_OUT)
         vjt = sum(float(np.sum(dirs[n] * g[n])) for n in which)
         assert abs(jv - vjt) <= 1e-10 * max(abs(jv), 1e-30), (which, jv, vjt)
+
+
+@pytest.mark.parametrize("name", ["dry_flat", "terrain"])
+def test_init_and_finish_adjoints_vs_complex_step(name):
+    """Hand adjoints of the small-step initialisation (through the stage
+    EOS) and of the small-step finish against complex-step JVPs of the
+    oracle's own forwards, for all inputs at once and one at a time."""
+    case, s = _stage_state(name, seed=23)
+    rng = np.random.default_rng(29)
+    for n in A.STAGE_IN:
+        x = getattr(s, n)
+        getattr(s, n + "0")[...] = x + rng.standard_normal(x.shape) * (
+            0.01 * (np.abs(x).max() + 1.0))
+    for n in A.FINISH_IN:
+        getattr(s, n)[...] = rng.standard_normal(getattr(s, n).shape) * 0.1
+    s.mu_pp[...] *= 50.0
+    h = 1e-20
+    scale = {"u": 1.0, "v": 1.0, "w": 0.1, "thp": 0.5, "php": 20.0,
+             "mup": 20.0}
+    cases = (("init", A.init_forward, A.init_adjoint, A.INIT_OUT,
+              A.STAGE_IN + tuple(n + "0" for n in A.STAGE_IN)),
+             ("finish", A.finish_forward, A.finish_adjoint, A.STAGE_IN,
+              A.STAGE_IN + A.FINISH_IN))
+    for label, fwd, adj, outs, ins in cases:
+        lam = {n: rng.standard_normal(getattr(s, n).shape) for n in outs}
+        g = adj(s, lam)
+        for which in (ins,) + tuple((n,) for n in ins):
+            c = _cplx(s)
+            dirs = {}
+            for n in which:
+                d = rng.standard_normal(getattr(s, n).shape) * \
+                    scale.get(n.rstrip("0"), 0.1)
+                dirs[n] = d
+                getattr(c, n)[...] = getattr(c, n) + 1j * h * d
+            t = fwd(c)
+            jv = sum(float(np.sum(np.imag(t[n]) / h * lam[n])) for n in outs)
+            vjt = sum(float(np.sum(dirs[n] * g[n])) for n in which)
+            assert abs(jv - vjt) <= 1e-10 * max(abs(jv), 1e-30), \
+                (label, which, jv, vjt)
