@@ -146,11 +146,15 @@ def _forward_pieces(s, coefs, dtau, c, first):
             "Ph1": s1.ph_pp, "AL1": s1.al_pp, "PP1": s1.p_pp}
 
 
-def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
+def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam, ref=False):
     """J^T lam for one acoustic substep at the state s (its small-step
     fields are the substep's input X).  ``lam`` maps output names
     (SUBSTEP_STATE, optionally ww_pp) to adjoints; returns the adjoints of
-    the inputs (SUBSTEP_STATE) and of the tendencies (SUBSTEP_TEND)."""
+    the inputs (SUBSTEP_STATE) and of the tendencies (SUBSTEP_TEND).
+    ``ref``: also the adjoints of the stage reference state the substep
+    reads (out["ref"], keyed u, v, w, thp, php, mup), through the stage EOS
+    (p, alt) and the implicit-w coefficients (coefs must be calc_coefs of
+    s, plain form)."""
     fw = _forward_pieces(s, coefs, dtau, c, first)
     f = s.dtype.type
     half, one = f(0.5), f(1.0)
@@ -177,6 +181,10 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
     lM0old, lT0old = z(M0), z(T0)
     lPh0 = out["ph_pp"]
     out["p_pp"] += L["p_pp_old"]                # output p_pp_old = input p_pp
+    if ref:                       # reference-state adjoints, accumulated
+        R = {"mut": z(mut), "alt": z(s.alt), "c2a": z(s.alt), "thr": z(thr),
+             "ph": z(s.w_pp), "w": z(s.w), "u": z(s.u), "v": z(s.v),
+             "chm": z(s.alt), "cfm": z(s.w_pp)}
 
     # ---- diagnose_p ------------------------------------------------------
     chmp = c1h * (mut + M1) + c2h
@@ -187,6 +195,13 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
     lM1 += np.sum(c2a * (-q * c1h * thr - q * X * c1h / chmp) * lPP1, axis=0)
     lAL = L["al_pp"] - c2a * lPP1
     lM1 += np.sum((-(s.alt * c1h) / chmp - AL1 * c1h / chmp) * lAL, axis=0)
+    if ref:
+        R["c2a"] += lPP1 * (q * X - AL1)
+        gq = lPP1 * c2a * X
+        R["alt"] += gq / (chmp * thr) - lAL * c1h * M1 / chmp
+        gchmp = -gq * q / chmp - lAL * AL1 / chmp
+        R["thr"] += -gq * q / thr + lPP1 * c2a * q * (-c1h * M1)
+        R["mut"] += np.sum(c1h * gchmp, axis=0)
     lPh1 = L["ph_pp"]
     lPh1[1:] += -rdnw / chmp * lAL
     lPh1[:-1] += rdnw / chmp * lAL
@@ -199,10 +214,14 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
     lW = np.array(L["w_pp"], copy=True)
     lW[1:] += K / cfs[1:] * lPh1[1:]
     lM1 += np.sum(-K * W1[1:] * c1f[1:] / cfs[1:] ** 2 * lPh1[1:], axis=0)
+    if ref:
+        R["mut"] += np.sum(-K * W1[1:] * c1f[1:] / cfs[1:] ** 2 * lPh1[1:],
+                           axis=0)
     lPh0[0] += lPh1[0]
 
     # ---- damper (damp_opt = 3): w = (w - dw cfm w_ref) / (1 + dw) -------
     cfm = c1f * mut + c2f
+    wsol = np.array(W1, copy=True)               # the solve's output
     if fw["dampmag"] > 0.0:
         phf = (s.phb if s.base3d else s.phb[:, None, None]) + s.php
         htop = phf[nz] / g
@@ -212,7 +231,19 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
             arg = f(1.5707963) * (hk - hbot) / f(c.zdamp)
             sn = np.sin(arg.astype(np.float64)).astype(s.dtype)
             dw = f(fw["dampmag"]) * sn * sn
-            lW[k] = np.where(hk >= hbot, lW[k] / (one + dw), lW[k])
+            on = hk >= hbot
+            if ref:
+                Cw = cfm[k] * s.w[k]
+                wsol[k] = np.where(on, W1[k] * (one + dw) + dw * Cw, W1[k])
+                lk = np.where(on, lW[k], 0.0)
+                R["w"][k] += -lk * dw * cfm[k] / (one + dw)
+                R["cfm"][k] += -lk * dw * s.w[k] / (one + dw)
+                gdw = -lk * (Cw + wsol[k]) / (one + dw) ** 2
+                garg = gdw * f(fw["dampmag"]) * 2.0 * sn * np.cos(arg)
+                ghk = garg * f(1.5707963) / f(c.zdamp)
+                R["ph"][k] += ghk / g
+                R["ph"][nz] += -ghk / g
+            lW[k] = np.where(on, lW[k] / (one + dw), lW[k])
 
     # ---- Thomas solve (rows 1..nz, w[0] coupled through a[1]) ------------
     n = nz
@@ -222,6 +253,58 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
         lW[k] = alpha[k] * lW[k]
         lW[k - 1] += -a[k] * lW[k]
     lacc = lW                                   # adjoint of the pre-solve w
+    if ref:
+        # A x = r with rows k = 1..nz: d(x) = -A^-1 dA x, so the entries'
+        # adjoints are -mu x (mu = lacc); then a, b, c -> c2a, chm, cfm
+        cof = K * K
+        chm_ = c1h * mut + c2h
+        rdn, rdnw1 = s.rdn, s.rdnw
+        mu_, x_ = lacc, wsol
+        for k in range(1, nz + 1):
+            if (2 <= k <= nz - 1) or (k == nz and not c.top_lid):
+                ga = -mu_[k] * x_[k - 1]
+                if k < nz:
+                    av = -cof * rdn[k] * rdnw1[k - 1] * c2a[k - 1] \
+                        / (chm_[k - 1] * cfm[k - 1])
+                else:
+                    av = f(-2.0) * cof * rdnw1[nz - 1] * rdnw1[nz - 1] \
+                        * c2a[nz - 1] / (chm_[nz - 1] * cfm[nz - 1])
+                R["c2a"][k - 1] += ga * av / c2a[k - 1]
+                R["chm"][k - 1] += -ga * av / chm_[k - 1]
+                R["cfm"][k - 1] += -ga * av / cfm[k - 1]
+            gb = -mu_[k] * x_[k]
+            if k <= nz - 1:
+                t1 = cof * rdn[k] * rdnw1[k] * c2a[k] / (chm_[k] * cfm[k])
+                t2b = cof * rdn[k] * rdnw1[k - 1] * c2a[k - 1] \
+                    / (chm_[k - 1] * cfm[k])
+                R["c2a"][k] += gb * t1 / c2a[k]
+                R["chm"][k] += -gb * t1 / chm_[k]
+                R["cfm"][k] += -gb * (t1 + t2b) / cfm[k]
+                R["c2a"][k - 1] += gb * t2b / c2a[k - 1]
+                R["chm"][k - 1] += -gb * t2b / chm_[k - 1]
+                gc = -mu_[k] * x_[k + 1]
+                cv = -cof * rdn[k] * rdnw1[k] * c2a[k] / (chm_[k] * cfm[k + 1])
+                R["c2a"][k] += gc * cv / c2a[k]
+                R["chm"][k] += -gc * cv / chm_[k]
+                R["cfm"][k + 1] += -gc * cv / cfm[k + 1]
+            else:
+                t1 = f(2.0) * cof * rdnw1[nz - 1] * rdnw1[nz - 1] \
+                    * c2a[nz - 1] / (chm_[nz - 1] * cfm[nz])
+                R["c2a"][nz - 1] += gb * t1 / c2a[nz - 1]
+                R["chm"][nz - 1] += -gb * t1 / chm_[nz - 1]
+                R["cfm"][nz] += -gb * t1 / cfm[nz]
+        # the phi'' right-hand side of the forward (for dph_up / dph_dn)
+        ph_ = (s.phb if s.base3d else s.phb[:, None, None]) + s.php
+        wd_ = half * (fw["WW1"][1:] + fw["WW1"][:-1]) * rdnw \
+            * (ph_[1:] - ph_[:-1])
+        r_ = np.zeros_like(s.w_pp)
+        r_[1:] = dt * (s.rph_t[1:] + half * g * (one - eps) * s.w_pp[1:])
+        for k in range(1, nz):
+            r_[k] = r_[k] - dt * (s.fnm[k] * wd_[k] + s.fnp[k] * wd_[k - 1])
+        rhs_ = np.zeros_like(s.w_pp)
+        rhs_[1:] = s.ph_pp[1:] + r_[1:] / cfm[1:]
+        if c.top_lid:
+            rhs_[nz] = 0.0
 
     # ---- w accumulation ---------------------------------------------------
     chm = c1h * mut + c2h
@@ -247,6 +330,22 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
         lt2[k] += dt * g * s_rdn[k] * c2a[k] * s.alt[k] * la
         lt2[k - 1] -= dt * g * s_rdn[k] * c2a[k - 1] * s.alt[k - 1] * la
         lmuave += -dt * g * s.c1f[k] * la
+        if ref:
+            P0 = s.ph_pp
+            dup = (one + eps) * (rhs_[k + 1] - rhs_[k]) \
+                + (one - eps) * (P0[k + 1] - P0[k])
+            ddn = (one + eps) * (rhs_[k] - rhs_[k - 1]) \
+                + (one - eps) * (P0[k] - P0[k - 1])
+            R["c2a"][k] += la * dup * cup / c2a[k]
+            R["chm"][k] += -la * dup * cup / chm[k]
+            R["c2a"][k - 1] += -la * ddn * cdn / c2a[k - 1]
+            R["chm"][k - 1] += la * ddn * cdn / chm[k - 1]
+            t2v = half * ((one + eps) * fw["T1"] + (one - eps) * T0) \
+                / ((c1h * (mut + M1) + c2h) * thr)
+            R["c2a"][k] += la * dt * g * s_rdn[k] * s.alt[k] * t2v[k]
+            R["alt"][k] += la * dt * g * s_rdn[k] * c2a[k] * t2v[k]
+            R["c2a"][k - 1] -= la * dt * g * s_rdn[k] * s.alt[k - 1] * t2v[k - 1]
+            R["alt"][k - 1] -= la * dt * g * s_rdn[k] * c2a[k - 1] * t2v[k - 1]
     if not c.top_lid:
         la = lacc[nz]
         lW0[nz] += la
@@ -262,6 +361,19 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
         lt2[nz - 1] -= dt * g * f(2.0) * s.rdnw[nz - 1] * c2a[nz - 1] \
             * s.alt[nz - 1] * la
         lmuave += -dt * g * s.c1f[nz] * la
+        if ref:
+            P0 = s.ph_pp
+            ddn = (one + eps) * (rhs_[nz] - rhs_[nz - 1]) \
+                + (one - eps) * (P0[nz] - P0[nz - 1])
+            bt = cb * ddn
+            R["c2a"][nz - 1] += -la * bt / c2a[nz - 1]
+            R["chm"][nz - 1] += la * bt / chm[nz - 1]
+            t2v = half * ((one + eps) * fw["T1"] + (one - eps) * T0) \
+                / ((c1h * (mut + M1) + c2h) * thr)
+            R["c2a"][nz - 1] += -la * dt * g * f(2.0) * s.rdnw[nz - 1] \
+                * s.alt[nz - 1] * t2v[nz - 1]
+            R["alt"][nz - 1] += -la * dt * g * f(2.0) * s.rdnw[nz - 1] \
+                * c2a[nz - 1] * t2v[nz - 1]
     lrhs[0] = 0.0                                # rhs[0] is the constant 0
 
     # ---- lower boundary w[0] from u'', v'' ---------------------------------
@@ -283,6 +395,9 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
     lT1 += half * (one + eps) / (chs * thr) * lt2
     lT0old += half * (one - eps) / (chs * thr) * lt2
     lM1 += np.sum(-t2 * c1h / chs * lt2, axis=0)
+    if ref:
+        R["mut"] += np.sum(-t2 * c1h / chs * lt2, axis=0)
+        R["thr"] += -t2 / thr * lt2
     lM1 += half * (one + eps) * lmuave
     lM0old += half * (one - eps) * lmuave
 
@@ -301,6 +416,11 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
     cwd = half * rdnw * (ph[1:] - ph[:-1])
     lWW1[1:] += cwd * lwd
     lWW1[:-1] += cwd * lwd
+    if ref:
+        R["cfm"][1:] += -lrhs[1:] * r_[1:] / cfm[1:] ** 2
+        gdp = lwd * half * (fw["WW1"][1:] + fw["WW1"][:-1]) * rdnw
+        R["ph"][1:] += gdp
+        R["ph"][:-1] -= gdp
 
     # ---- advance_mu_th ------------------------------------------------------
     lT0 = out["th_pp"]
@@ -317,6 +437,19 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
     lU1[..., :-1] -= half * rdx * (th + t_im) * gg
     lV1[:, 1:, :] += half * rdy * (t_jp + th) * gg
     lV1[:, :-1, :] -= half * rdy * (th + t_jm) * gg
+    if ref:
+        WW1_ = fw["WW1"]
+        for k in range(1, nz):
+            R["thr"][k] += lwdtn[k] * WW1_[k] * s.fnm[k]
+            R["thr"][k - 1] += lwdtn[k] * WW1_[k] * s.fnp[k]
+        ue_, uw_ = U1[..., 1:], U1[..., :-1]
+        vn_, vs_ = V1[:, 1:, :], V1[:, :-1, :]
+        R["thr"] += half * rdx * (ue_ - uw_) * gg \
+            + half * rdy * (vn_ - vs_) * gg
+        R["thr"] += np.roll(half * rdx * ue_ * gg, 1, axis=2) \
+            - np.roll(half * rdx * uw_ * gg, -1, axis=2) \
+            + np.roll(half * rdy * vn_ * gg, 1, axis=1) \
+            - np.roll(half * rdy * vs_ * gg, -1, axis=1)
     lww = np.array(lWW1, copy=True)
     for k in range(1, nz):
         lww[k] += (s.fnm[k] * th[k] + s.fnp[k] * th[k - 1]) * lwdtn[k]
@@ -337,6 +470,23 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
     lM0 = out["mu_pp"]
     lM0 += lM1
     ldmdt += dt * lM1
+    if ref:                       # dmdt_rf = sum_k dnw dvr (reference u, v)
+        grf = dt * lM1
+        for ax, rdd, xn in ((2, rdx, "u"), (1, rdy, "v")):
+            x = getattr(s, xn)
+            hi_ = (slice(None), slice(None), slice(1, None)) if ax == 2 \
+                else (slice(None), slice(1, None), slice(None))
+            lo_ = (slice(None), slice(None), slice(0, -1)) if ax == 2 \
+                else (slice(None), slice(0, -1), slice(None))
+            mhi = half * (mut + np.roll(mut, -1, axis=ax - 1))
+            mlo = half * (np.roll(mut, 1, axis=ax - 1) + mut)
+            gdv = dnw[:, None, None] * grf[None]
+            R[xn][hi_] += rdd * (c1h * mhi + c2h) * gdv
+            R[xn][lo_] -= rdd * (c1h * mlo + c2h) * gdv
+            gmh = np.sum(rdd * c1h * x[hi_] * gdv, axis=0)
+            gml = -np.sum(rdd * c1h * x[lo_] * gdv, axis=0)
+            R["mut"] += half * (gmh + np.roll(gmh, 1, axis=ax - 1)) \
+                + half * (gml + np.roll(gml, -1, axis=ax - 1))
     out["rmu_t"] += dt * lM1
     for k in range(nz):
         ldvp[k] += dnw[k] * ldmdt
@@ -376,6 +526,24 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
         ldpn[:-1] -= rdnw * t
         lms = np.sum(-half * c1h * t, axis=0)
         lM0 += lms + _bt(lms, ax)
+        if ref:
+            pd_ = D._pdmp(s.p_pp, s.p_pp_old, smdiv)
+            dph_ = (s.ph_pp[1:] - B(s.ph_pp[1:])) + (s.ph_pp[:-1]
+                                                     - B(s.ph_pp[:-1]))
+            dpe_ = pd_ - B(pd_)
+            brk_ = dph_ + (s.alt + B(s.alt)) * dpe_
+            if s.base3d:
+                brk_ = brk_ + (s.al_pp + B(s.al_pp)) * (s.pb - B(s.pb))
+            gmf = np.sum(half * rd * c1h * brk_ * ld, axis=0)
+            R["mut"] += half * (gmf + _bt(gmf, ax))
+            xa = lbrk * dpe_
+            R["alt"] += xa + _bt(xa, ax)
+            dpn_ = D._dpn_face_levels(s, pd_, ax, c.top_lid)
+            gdphp = rd * (rdnw * (dpn_[1:] - dpn_[:-1])
+                          - half * c1h * (B(s.mu_pp) + s.mu_pp)) * ld
+            gphs = half * (gdphp - _bt(gdphp, ax))
+            R["ph"][:-1] += gphs
+            R["ph"][1:] += gphs
         lPh0[1:] += lbrk - _bt(lbrk, ax)
         lPh0[:-1] += lbrk - _bt(lbrk, ax)
         ldpe = (s.alt + B(s.alt)) * lbrk
@@ -397,6 +565,16 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam):
         lpd += lS + _bt(lS, ax)
     out["p_pp"] += (one + smdiv) * lpd
     out["p_pp_old"] += -smdiv * lpd
+    if ref:
+        # chm, cfm -> mut; c2a -> p, alt; the stage EOS -> thp, php, mup
+        R["mut"] += np.sum(c1h * R["chm"], axis=0) \
+            + np.sum(c1f * R["cfm"], axis=0)
+        gp = R["c2a"] * GAMMA / s.alt
+        galt = R["alt"] - R["c2a"] * c2a / s.alt
+        gt, gph, gm = eos_adjoint(s, gp, np.zeros_like(s.al), galt)
+        out["ref"] = {"u": R["u"], "v": R["v"], "w": R["w"],
+                      "thp": R["thr"] + gt, "php": R["ph"] + gph,
+                      "mup": R["mut"] + gm}
     return out
 
 
@@ -1091,3 +1269,103 @@ def finish_adjoint(s, lam):
     g["mup"] += L + g_ms + g_mn
     g["mu_pp"] += L + g_mn
     return g
+
+
+# ---------------------------------------------------------------------------
+# The whole RK3 step (dry, plain form, km_opt = 0): the pieces above
+# chained in reverse over the stages from per-stage checkpoints
+# ---------------------------------------------------------------------------
+
+def set_w_surface_adjoint(s, lam_w0, dx, dy):
+    """(u, v) gradients of set_w_surface's w(sfc) = u . grad(ht)
+    (reference dycore.py:1999-2026) for the adjoint lam_w0 of w[0]."""
+    ht = s.ht
+    dyn = np.roll(ht, -1, 0) - ht
+    dys = ht - np.roll(ht, 1, 0)
+    dxe = np.roll(ht, -1, 1) - ht
+    dxw = ht - np.roll(ht, 1, 1)
+    gvc_n = (0.5 / dy) * dyn * lam_w0           # vc[1:, :]
+    gvc_s = (0.5 / dy) * dys * lam_w0           # vc[:-1, :]
+    guc_e = (0.5 / dx) * dxe * lam_w0           # uc[:, 1:]
+    guc_w = (0.5 / dx) * dxw * lam_w0           # uc[:, :-1]
+    g_u, g_v = np.zeros_like(s.u), np.zeros_like(s.v)
+    for lev, cf in enumerate((s.cf1, s.cf2, s.cf3)):
+        g_u[lev, :, 1:] += cf * guc_e
+        g_u[lev, :, :-1] += cf * guc_w
+        g_v[lev, 1:, :] += cf * gvc_n
+        g_v[lev, :-1, :] += cf * gvc_s
+    return g_u, g_v
+
+
+def _copy_state(s):
+    return D.OState(**{k: (v.copy() if isinstance(v, np.ndarray) else v)
+                       for k, v in vars(s).items()})
+
+
+def step_adjoint(s, cfg, lam):
+    """J^T lam for one dycore step (D.step) at the state s: ``lam`` maps
+    the new state's STAGE_IN fields to adjoints; returns the gradients of
+    the step's input STAGE_IN fields.  Dry, plain form, no diffusion.
+    The forward is re-run on a copy, checkpointing each stage's state and
+    each substep's input; s is not modified."""
+    diffusion = cfg.km_opt == 1 and (cfg.khdif > 0.0 or cfg.kvdif > 0.0)
+    if s.qv is not None or diffusion or getattr(cfg, "precision_form",
+                                                 False):
+        raise NotImplementedError("step adjoint: dry, plain, no diffusion")
+    ns = cfg.time_step_sound
+    stages = ((1, cfg.dt / 3.0), (max(ns // 2, 1), cfg.dt / ns),
+              (ns, cfg.dt / ns))
+    w = _copy_state(s)
+    for n in D.PROGNOSTICS:
+        getattr(w, n + "0")[...] = getattr(w, n)
+    ck = []                                   # per stage: state, P_i list
+    for istage, (nsub, dtau) in enumerate(stages):
+        for n in STAGE_OUT + ("rmu_t",):
+            getattr(w, n)[...] = 0.0
+        D.calc_p_alpha(w)
+        ru, rv, ww = D.stage_fluxes(w, cfg.dx, cfg.dy)
+        D.add_slow_tendencies(w, ru, rv, ww, cfg.dx, cfg.dy, cfg.top_lid)
+        D.small_step_init(w)
+        coefs = D.calc_coefs(w, dtau, cfg.epssm, cfg.top_lid)
+        subs = []
+        for i in range(nsub):
+            subs.append(_copy_state(w))       # substep i's input
+            D.acoustic_substep(w, coefs, dtau, cfg, first=(i == 0))
+        ck.append((_copy_state(w), subs, coefs, dtau, nsub))
+        D.small_step_finish(w)
+    # reverse
+    g0 = {n: np.zeros_like(getattr(s, n)) for n in STAGE_IN}   # time-t
+    lx = {n: np.array(lam.get(n, np.zeros_like(getattr(s, n))), copy=True)
+          for n in STAGE_IN}
+    gu, gv = set_w_surface_adjoint(s, lx["w"][0], cfg.dx, cfg.dy)
+    lx["u"] += gu
+    lx["v"] += gv
+    lx["w"][0] = 0.0
+    for istage in range(2, -1, -1):
+        after_subs, subs, coefs, dtau, nsub = ck[istage]
+        # the stage state Xs is the finish's input: after_subs holds it
+        fin = finish_adjoint(after_subs, lx)
+        gs = {n: fin[n] for n in STAGE_IN}
+        lp = {n: fin[n] for n in FINISH_IN}
+        lt = {n: np.zeros_like(getattr(s, n)) for n in SUBSTEP_TEND}
+        for i in range(nsub - 1, -1, -1):
+            st = subs[i]
+            out = acoustic_substep_adjoint(st, coefs, dtau, cfg, i == 0, lp,
+                                           ref=True)
+            lp = {n: out[n] for n in SUBSTEP_STATE}
+            for n in SUBSTEP_TEND:
+                lt[n] += out[n]
+            for n in STAGE_IN:
+                gs[n] += out["ref"][n]
+        gi = init_adjoint(subs[0], lp)
+        for n in STAGE_IN:
+            g0[n] += gi[n + "0"]
+            gs[n] += gi[n]
+        gt = stage_tendencies_adjoint(subs[0], {n: lt[n] for n in STAGE_OUT},
+                                      cfg.dx, cfg.dy, cfg.top_lid)
+        for n in STAGE_IN:
+            gs[n] += gt[n]
+        lx = gs                                # the previous stage's output
+    for n in STAGE_IN:
+        g0[n] += lx[n]
+    return g0

@@ -921,10 +921,13 @@ def advance_w_phi(s, coefs, dtau, epssm, dx, dy, top_lid=True,
         for k in range(1, nz + 1):
             hk = phf[k] / g
             arg = f(1.5707963) * (hk - hbot) / f(zdamp)
-            sn = np.sin(arg.astype(np.float64)).astype(s.dtype)
+            if np.iscomplexobj(arg):        # complex-step derivatives
+                sn = np.sin(arg)
+            else:
+                sn = np.sin(arg.astype(np.float64)).astype(s.dtype)
             dw = f(dampmag) * sn * sn
             new = (w[k] - dw * cfm[k] * s.w[k]) / (one + dw)
-            w[k] = np.where(hk >= hbot, new, w[k])
+            w[k] = np.where(np.real(hk) >= np.real(hbot), new, w[k])
     s.w_pp[...] = w
     cfs = s.c1f[:, None, None] * muts + s.c2f[:, None, None]
     s.ph_pp[1:] = rhs[1:] + half * dtau * g * (one + eps) * w[1:] / cfs[1:]
@@ -950,12 +953,12 @@ def acoustic_substep(s, coefs, dtau, cfg_like, first):
     first (acoustic.py:424) exactly like the reference launcher."""
     c = cfg_like
     smdiv = 0.0 if first else c.smdiv
-    d32 = float(s.dtype.type(dtau))
+    d32 = float(np.real(s.dtype.type(dtau)))
     advance_uv(s, d32, smdiv, c.dx, c.dy, c.top_lid)
     advance_mu_th(s, d32, c.dx, c.dy,
                   prec=bool(getattr(c, "precision_form", False))
                   and s.dtype == np.float32)
-    dampmag = float(s.dtype.type(dtau * c.dampcoef)) \
+    dampmag = float(np.real(s.dtype.type(dtau * c.dampcoef))) \
         if c.damp_opt == 3 else 0.0
     advance_w_phi(s, coefs, d32, c.epssm, c.dx, c.dy, c.top_lid,
                   dampmag, c.zdamp)
@@ -1071,7 +1074,6 @@ def step(s, cfg) -> None:
         if cfg.km_opt == 1:
             add_diffusion_tendencies(s, cfg.khdif, cfg.kvdif, cfg.dx, cfg.dy)
         small_step_init(s, prec)
-        dtau32 = float(f(dtau))
         coefs = calc_coefs(s, dtau, cfg.epssm, cfg.top_lid, prec)
         if moist:
             ru_m = np.zeros_like(s.u)
