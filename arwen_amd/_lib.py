@@ -12,7 +12,7 @@ from pathlib import Path
 LIB_NAME = "libarwen_dycore.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -124,6 +124,17 @@ class ArwFinishAdj(ctypes.Structure):
 ArwInitAdj64 = ArwInitAdj
 ArwFinishAdj64 = ArwFinishAdj
 
+#: accumulators of the substep adjoint's stage-reference share
+ACOUSTIC_REF = ("mut", "c2a", "alt", "thr", "ph", "w", "u", "v")
+
+
+class ArwAcousticRef(ctypes.Structure):
+    """arw_acoustic_ref (and arw64_acoustic_ref: same layout)."""
+    _fields_ = [(n, _P) for n in ACOUSTIC_REF]
+
+
+ArwAcousticRef64 = ArwAcousticRef
+
 
 class ArwenError(RuntimeError):
     """A library entry point returned a nonzero status."""
@@ -169,6 +180,14 @@ def load():
         "arw_acoustic_substep_adjoint": ([dp, pp, _D, _I,
                                           ctypes.POINTER(ArwAcousticAdj), _P,
                                           _P], _I),
+        "arw_acoustic_substep_adjoint_ref": ([dp, pp, _D, _I,
+                                              ctypes.POINTER(ArwAcousticAdj),
+                                              ctypes.POINTER(ArwAcousticRef),
+                                              _P, _P], _I),
+        "arw_acoustic_reference_gradients": ([dp,
+                                              ctypes.POINTER(ArwAcousticRef),
+                                              _P, _P, _P, _P, _P], _I),
+        "arw_set_w_surface_adjoint": ([dp, pp, _P, _P, _P, _P], _I),
         "arw_acoustic_adjoint_work": ([_I, _I, _I,
                                        ctypes.POINTER(ctypes.c_int64)], _I),
         "arw_stage_tendencies_adjoint": ([dp, pp, ctypes.POINTER(ArwStageAdj),
@@ -243,7 +262,11 @@ FP64_ENTRY_POINTS = (
     "arw64_ensemble_capture", "arw64_ensemble_launch",
     "arw64_ensemble_destroy", "arw64_stage_tendencies_adjoint",
     "arw64_stage_adjoint_work", "arw64_small_step_init_adjoint",
-    "arw64_small_step_finish_adjoint", "arw64_small_step_adjoint_work") + tuple("arw64_" + n for n in FLUX_DIV_ADJOINTS)
+    "arw64_small_step_finish_adjoint", "arw64_small_step_adjoint_work",
+    "arw64_acoustic_substep_adjoint_ref",
+    "arw64_acoustic_reference_gradients",
+    "arw64_set_w_surface_adjoint") + tuple(
+        "arw64_" + n for n in FLUX_DIV_ADJOINTS)
 
 
 def exported_symbols() -> list[str]:
@@ -262,7 +285,10 @@ def exported_symbols() -> list[str]:
             "arw_ensemble_launch", "arw_ensemble_destroy",
             "arw_stage_tendencies_adjoint", "arw_stage_adjoint_work",
             "arw_small_step_init_adjoint", "arw_small_step_finish_adjoint",
-            "arw_small_step_adjoint_work"] + [
+            "arw_small_step_adjoint_work",
+            "arw_acoustic_substep_adjoint_ref",
+            "arw_acoustic_reference_gradients",
+            "arw_set_w_surface_adjoint"] + [
             "arw_" + n for n in FLUX_DIV_ADJOINTS]
 
 

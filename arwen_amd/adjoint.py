@@ -165,13 +165,16 @@ ACOUSTIC_TEND = _lib.ACOUSTIC_TEND
 
 def acoustic_substep_adjoint(state: DomainState, cfg: RunConfig,
                              dtau: float, first: bool,
-                             lam: dict) -> dict:
+                             lam: dict, ref: dict | None = None) -> dict:
     """J^T lam for one acoustic substep taken from the state's current
     small-step fields (coefficients prepared as for the forward substep,
     ``dycore.prepare_acoustic_coefficients``).  ``lam`` maps the outputs
     (ACOUSTIC_STATE, optionally ``ww_pp``) to adjoints (missing = 0);
-    returns the adjoints of ACOUSTIC_STATE + ACOUSTIC_TEND.  The state is
-    not modified."""
+    returns the adjoints of ACOUSTIC_STATE + ACOUSTIC_TEND.  ``ref``
+    (from ``reference_accumulators``): also add the share of the stage
+    reference state to those accumulators (dry states; map them with
+    ``reference_gradients`` once the stage's substeps are done).  The
+    state is not modified."""
     _check_state(state)
     lib = _lib.load()
     pre = "arw64_" if state.dtype == torch.float64 else "arw_"
@@ -197,11 +200,95 @@ def acoustic_substep_adjoint(state: DomainState, cfg: RunConfig,
     for name in ACOUSTIC_STATE + ACOUSTIC_TEND:
         out[name] = torch.empty_like(getattr(state, name))
         setattr(a, "g_" + name, out[name].data_ptr())
-    fn = getattr(lib, pre + "acoustic_substep_adjoint")
+    if ref is None:
+        fn = getattr(lib, pre + "acoustic_substep_adjoint")
+        _lib.check(fn(_dom(state, cfg), params(cfg), float(dtau),
+                      int(bool(first)), ctypes.byref(a), work.data_ptr(),
+                      _stream()), pre + "acoustic_substep_adjoint")
+        return out
+    r = _ref_struct(state, ref)
+    fn = getattr(lib, pre + "acoustic_substep_adjoint_ref")
     _lib.check(fn(_dom(state, cfg), params(cfg), float(dtau),
-                  int(bool(first)), ctypes.byref(a), work.data_ptr(),
-                  _stream()), pre + "acoustic_substep_adjoint")
+                  int(bool(first)), ctypes.byref(a), ctypes.byref(r),
+                  work.data_ptr(), _stream()),
+               pre + "acoustic_substep_adjoint_ref")
     return out
+
+
+ACOUSTIC_REF = _lib.ACOUSTIC_REF
+
+
+def _ref_shapes(state: DomainState) -> dict:
+    nz, ny, nx = state.shape
+    return {"mut": (ny, nx), "c2a": (nz, ny, nx), "alt": (nz, ny, nx),
+            "thr": (nz, ny, nx), "ph": (nz + 1, ny, nx),
+            "w": (nz + 1, ny, nx), "u": (nz, ny, nx + 1),
+            "v": (nz, ny + 1, nx)}
+
+
+def reference_accumulators(state: DomainState) -> dict:
+    """Zeroed accumulators of the substep adjoint's stage-reference
+    share: mut (ny,nx); c2a, alt, thr (mass points); ph, w (w levels);
+    u, v (faces)."""
+    return {n: torch.zeros(shp, dtype=state.dtype, device=state.u.device)
+            for n, shp in _ref_shapes(state).items()}
+
+
+def _ref_struct(state, ref):
+    want = _ref_shapes(state)
+    r = _lib.ArwAcousticRef()
+    for n in ACOUSTIC_REF:
+        t = ref[n]
+        if tuple(t.shape) != want[n] or t.dtype != state.dtype \
+                or t.device != state.u.device or not t.is_contiguous():
+            raise ValueError(f"reference accumulator {n}: expected a "
+                             f"contiguous {want[n]} {state.dtype} tensor on "
+                             f"{state.u.device}")
+        setattr(r, n, t.data_ptr())
+    return r
+
+
+def reference_gradients(state: DomainState, ref: dict) -> dict:
+    """The stage-state gradients (STAGE_STATE names) of the substep
+    adjoints accumulated in ``ref`` over one stage: c2a and alpha_t mapped
+    through the stage EOS (the state's p and alt must hold it, as after
+    ``dycore.stage_prepare``).  u, v, w are the accumulators themselves."""
+    _check_state(state)
+    lib = _lib.load()
+    pre = "arw64_" if state.dtype == torch.float64 else "arw_"
+    r = _ref_struct(state, ref)
+    nz, ny, nx = state.shape
+    work = state.scratch((2 * (nz + 1) * (ny + 1) * (nx + 1),),
+                         "arw_reference_gradients_work")
+    out = {"u": ref["u"], "v": ref["v"], "w": ref["w"],
+           "thp": torch.empty_like(state.thp),
+           "php": torch.empty_like(state.php),
+           "mup": torch.empty_like(state.mup)}
+    _lib.check(getattr(lib, pre + "acoustic_reference_gradients")(
+        state.domain_struct(), ctypes.byref(r), out["thp"].data_ptr(),
+        out["php"].data_ptr(), out["mup"].data_ptr(), work.data_ptr(),
+        _stream()), pre + "acoustic_reference_gradients")
+    return out
+
+
+def set_w_surface_adjoint(state: DomainState, cfg: RunConfig,
+                          lam_w0: torch.Tensor, g_u: torch.Tensor,
+                          g_v: torch.Tensor) -> None:
+    """Add the (u, v) gradients of the kinematic lower boundary w(sfc) =
+    u . grad(ht) (reference dycore.py:1999-2026) for the adjoint ``lam_w0``
+    (ny, nx) of w(sfc) to ``g_u``, ``g_v`` in place."""
+    _check_state(state)
+    lib = _lib.load()
+    pre = "arw64_" if state.dtype == torch.float64 else "arw_"
+    l0 = _as_like(lam_w0, state.mup, "w(sfc)")
+    for t, ref_t, n in ((g_u, state.u, "u"), (g_v, state.v, "v")):
+        if tuple(t.shape) != tuple(ref_t.shape) or t.dtype != ref_t.dtype \
+                or t.device != ref_t.device or not t.is_contiguous():
+            raise ValueError(f"gradient of {n}: expected a contiguous "
+                             f"{tuple(ref_t.shape)} {ref_t.dtype} tensor")
+    _lib.check(getattr(lib, pre + "set_w_surface_adjoint")(
+        _dom(state, cfg), params(cfg), l0.data_ptr(), g_u.data_ptr(),
+        g_v.data_ptr(), _stream()), pre + "set_w_surface_adjoint")
 
 
 def _as_like(t, ref, name):
@@ -396,3 +483,121 @@ def small_step_finish_adjoint(state: DomainState, lam: dict) -> dict:
         state.domain_struct(), ctypes.byref(a), work.data_ptr(), _stream()),
         pre + "small_step_finish_adjoint")
     return out
+
+
+# ---------------------------------------------------------------------------
+# One whole step
+# ---------------------------------------------------------------------------
+
+def _step_stages(cfg: RunConfig):
+    """(substeps, dtau) of the three RK stages (reference dycore.py:
+    2290-2300)."""
+    ns = cfg.time_step_sound
+    return ((1, cfg.dt / 3.0), (max(ns // 2, 1), cfg.dt / ns),
+            (ns, cfg.dt / ns))
+
+
+def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict) -> dict:
+    """J^T lam of one dycore step (``dycore.step``; reference dycore.py:
+    2161-2503) at the state's prognostic fields: ``lam`` maps the new
+    state's STAGE_STATE fields to adjoints (missing = 0); returns the
+    gradients of the step's input fields (STAGE_STATE names).  Dry states
+    without constant-K diffusion, plain form.
+
+    The forward is re-run stage by stage on the device with checkpoints
+    (each stage's state, each substep's nine small-step fields, the
+    finish's input), then reversed: per stage the finish adjoint, the
+    substep adjoints in reverse order (adding the stage-reference share),
+    the small-step initialisation adjoint (to the time-t and stage fields)
+    and the tendency-pass adjoint; the stages chain through their states
+    and the kinematic lower boundary of the step's end comes first.  The
+    state's prognostic fields are left as they were (their EOS
+    recomputed); its other fields are overwritten."""
+    from . import dycore as dyn
+    _check_state(state)
+    if state.qv is not None or _uses_diffusion(cfg) \
+            or getattr(cfg, "precision_form", False):
+        raise _lib.ArwenError("step adjoint: dry states, no constant-K "
+                              "diffusion, plain form")
+    names = STAGE_STATE
+    x_in = {n: getattr(state, n).clone() for n in names}
+    for n in names:
+        getattr(state, n + "0").copy_(getattr(state, n))
+    ck = []
+    for nsub, dtau in _step_stages(cfg):
+        xs = {n: getattr(state, n).clone() for n in names}
+        dyn.stage_prepare(state, cfg, dtau, True)
+        subs = []
+        for i in range(nsub):
+            subs.append({n: getattr(state, n).clone() for n in ACOUSTIC_STATE})
+            dyn.acoustic_substep(state, cfg, dtau, first=(i == 0))
+        after = {n: getattr(state, n).clone() for n in FINISH_PP}
+        ck.append((xs, subs, after, dtau))
+        dyn.small_step_finish(state)
+    lx = {}
+    for n in names:
+        t = lam.get(n)
+        lx[n] = (torch.zeros_like(getattr(state, n)) if t is None
+                 else _as_like(t, getattr(state, n), n).clone())
+    set_w_surface_adjoint(state, cfg, lx["w"][0], lx["u"], lx["v"])
+    lx["w"][0].zero_()
+    g0 = {n: torch.zeros_like(getattr(state, n)) for n in names}
+    for xs, subs, after, dtau in reversed(ck):
+        for n in names:
+            getattr(state, n).copy_(xs[n])
+        dyn.stage_prepare(state, cfg, dtau, True)       # EOS and tendencies
+        for n in FINISH_PP:
+            getattr(state, n).copy_(after[n])
+        fin = small_step_finish_adjoint(state, lx)
+        gs = {n: fin[n] for n in names}
+        lp = {n: fin[n] for n in FINISH_PP}
+        lt = {n: torch.zeros_like(getattr(state, n)) for n in STAGE_TEND}
+        acc = reference_accumulators(state)
+        for i in range(len(subs) - 1, -1, -1):
+            for n in ACOUSTIC_STATE:
+                getattr(state, n).copy_(subs[i][n])
+            out = acoustic_substep_adjoint(state, cfg, dtau, i == 0, lp, acc)
+            lp = {n: out[n] for n in ACOUSTIC_STATE}
+            for n in STAGE_TEND:
+                lt[n] += out[n]
+        ref = reference_gradients(state, acc)
+        gi = small_step_init_adjoint(state, lp)
+        gt = stage_tendencies_adjoint(state, cfg, lt)
+        for n in names:
+            g0[n] += gi[n + "0"]
+            gs[n] = gs[n] + ref[n] + gi[n] + gt[n]
+        lx = gs
+    for n in names:
+        g0[n] += lx[n]
+        getattr(state, n).copy_(x_in[n])
+    dyn.update_diagnostics(state)
+    return g0
+
+
+class DycoreStep(torch.autograd.Function):
+    """(u, v, w, thp, php, mu') -> the same fields one dycore step later
+    (the device step, ``dycore.step``), with ``step_adjoint`` as its
+    backward.  ``apply(state, cfg, *fields)`` writes the fields into the
+    state and steps it there; the backward re-runs the forward from the
+    saved inputs."""
+
+    @staticmethod
+    def forward(ctx, state, cfg, *fields):
+        from .dycore import step
+        if len(fields) != len(STAGE_STATE):
+            raise ValueError(f"expected {len(STAGE_STATE)} fields")
+        for name, x in zip(STAGE_STATE, fields):
+            getattr(state, name).copy_(x)
+        ctx.state, ctx.cfg = state, cfg
+        ctx.save_for_backward(*[x.detach() for x in fields])
+        step(state, cfg)
+        return tuple(getattr(state, n).clone() for n in STAGE_STATE)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        st = ctx.state
+        for name, x in zip(STAGE_STATE, ctx.saved_tensors):
+            getattr(st, name).copy_(x)
+        lam = {n: g for n, g in zip(STAGE_STATE, grads) if g is not None}
+        g = step_adjoint(st, ctx.cfg, lam)
+        return (None, None) + tuple(g[n] for n in STAGE_STATE)

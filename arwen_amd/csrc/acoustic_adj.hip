@@ -37,6 +37,8 @@ enum {
     W_GG, W_LDVP, W_L0,                                      // cell -> faces
     W_MC1, W_MC2, W_MC3, W_GGF, W_G0,                        // level pieces
     W_LBX, W_LSX, W_LBY, W_LSY,                              // faces -> cells
+    W_LM1, W_LDX, W_LDY,                                     // lambda(mu''), ld
+    W_RM, W_RPH, W_RGX, W_RGY, W_RMX, W_RMY, W_RAX, W_RAY,   // reference pieces
     W_COUNT
 };
 
@@ -54,6 +56,11 @@ struct Fw {                 // forward recompute and scratch
 __device__ __forceinline__ real th_ref(const Dom& d, int k, int c)
 {
     return (d.base3d ? d.thb[k * d.ny * d.nx + c] : d.thb[k]) + d.thp[k * d.ny * d.nx + c];
+}
+
+__device__ __forceinline__ real mut_of(const Dom& d, int c)
+{
+    return d.mub2d[c] + d.mup[c];
 }
 
 __device__ __forceinline__ real phi_full(const Dom& d, int k, int c)
@@ -351,6 +358,7 @@ __global__ __launch_bounds__(64) void k_adj_s2(Dom d, Sc s, Adj a, Fw f)
         ldmdt += d.dnw[k] * d.c1h[k] * linc;
         gr += d.dnw[k] * d.c1h[k] * linc;
     }
+    f.s[W_LM1][c] = lM1;
     ldmdt += dt * lM1;
     gr += dt * lM1;
     a.grmu[c] = gr;
@@ -464,6 +472,7 @@ __global__ __launch_bounds__(256) void k_adj_f1(Dom d, Sc s, Adj a, Fw f)
     if (!fg.own) return;
     const real ldup = has_dup ? lam1(dg) : R(0.0);
     const real ld = -dt * l1 - dt * ldup;
+    f.s[isu ? W_LDX : W_LDY][k * st + fg.slot] = ld;
     const real mA = d.mub2d[fg.cA] + d.mup[fg.cA];
     const real mB = d.mub2d[fg.cB] + d.mup[fg.cB];
     const real muf = half * (mB + mA);
@@ -548,18 +557,481 @@ __global__ void k_adj_cells(Dom d, Sc s, Adj a, Fw f)
     a.gmu[c] += lms;
 }
 
+// ---------------------------------------------------------------------------
+// The substep's dependence on the stage reference state
+// ---------------------------------------------------------------------------
+//
+// The stage fields enter the substep as coefficients: c2a = gamma p /
+// alpha_t and alpha_t of the stage EOS, the column mass mu_t = mub + mu'
+// (C(mu_t) of the implicit w system, of the phi'' update and of the
+// linearised EOS, the face masses of advance_uv), theta = thb + theta' (the
+// theta'' fluxes, the EOS), the full phi (the Omega'' term of the phi''
+// right-hand side, the PGF's phi' difference, the damper's heights), w (the
+// damper's reference) and u, v (dmdt_rf).  The kernels below add J^T lam
+// with respect to those to per-stage accumulators, which the caller clears
+// at the stage start; arw_acoustic_reference_gradients then maps c2a and
+// alpha_t through the stage EOS once per stage.  The same restated forward
+// terms as the mass-column kernels above, one thread per point it writes.
+
+struct RefAcc {
+    real *mut, *c2a, *alt, *thr, *ph, *w, *u, *v;
+};
+
+// the damper at w level k: inside the layer?, dw, its sine argument
+__device__ __forceinline__ bool damp_at(const Dom& d, const Sc& s, int k,
+                                        int c, real& dw, real& arg, real& sn)
+{
+    const real g = G_F;
+    const real hbot = phi_full(d, d.nz, c) / g - s.zdamp;
+    const real hk = phi_full(d, k, c) / g;
+    arg = R(1.5707963) * (hk - hbot) / s.zdamp;
+    sn = (real)sin((double)arg);
+    dw = s.dampmag * sn * sn;
+    return s.damp && k >= 1 && (hk >= hbot);
+}
+
+// (column, level 0..nz): the EOS, phi'' update, damper, Thomas solve,
+// w accumulation, rhs, theta'' flux and t2 terms.  Level pieces of the
+// column-mass gradient go to RM, the damper's top-height pieces to RPH.
+__global__ __launch_bounds__(256) void k_ref_lev(Dom d, Sc s, Adj a, Fw f,
+                                                 RefAcc r)
+{
+    int c, k;
+    if (!col_level(d, c, k)) return;
+    const ColCtx C(d, s, c, f.M1[c]);
+    const int st = C.st, nz = d.nz, ny = d.ny, nx = d.nx, x = k * st + c;
+    const real half = R(0.5), one = R(1.0), dt = s.dtau, g = G_F,
+               eps = s.epssm, cof = C.cof;
+    const real* LW = f.s[W_LW];
+    const real* LRHS = f.s[W_LRHS];
+    const real* LT2 = f.s[W_LT2];
+    const real* GGF = f.s[W_GGF];
+    const real* WW1 = f.s[W_WW];
+    auto mu = [&](int kk) { return LW[kk * st + c]; };   // lambda, rows 1..nz
+    auto wsol = [&](int kk) -> real {                    // pre-damper solve
+        const real w1 = f.W1[kk * st + c];
+        real dw, arg, sn;
+        if (!damp_at(d, s, kk, c, dw, arg, sn)) return w1;
+        return w1 * (one + dw) + dw * (C.cfm(kk) * d.w[kk * st + c]);
+    };
+    auto wd = [&](int kk) {                              // kk in 0..nz-1
+        return half * (WW1[(kk + 1) * st + c] + WW1[kk * st + c]) * d.rdnw[kk]
+               * (phi_full(d, kk + 1, c) - phi_full(d, kk, c));
+    };
+    auto rr = [&](int kk) {                              // kk in 1..nz
+        real v = dt * (d.rph_t[kk * st + c]
+                       + half * g * (one - eps) * d.w_pp[kk * st + c]);
+        if (kk < nz) v = v - dt * (d.fnm[kk] * wd(kk) + d.fnp[kk] * wd(kk - 1));
+        return v;
+    };
+    auto rhs = [&](int kk) -> real {                     // predicted phi''
+        if (kk < 1 || kk > nz || (kk == nz && s.top_lid)) return R(0.0);
+        return d.ph_pp[kk * st + c] + rr(kk) / C.cfm(kk);
+    };
+    auto P0 = [&](int kk) { return d.ph_pp[kk * st + c]; };
+    auto dup = [&](int rw) {
+        return (one + eps) * (rhs(rw + 1) - rhs(rw))
+               + (one - eps) * (P0(rw + 1) - P0(rw));
+    };
+    auto ddn = [&](int rw) {
+        return (one + eps) * (rhs(rw) - rhs(rw - 1))
+               + (one - eps) * (P0(rw) - P0(rw - 1));
+    };
+    auto lr2 = [&](int kk) -> real {
+        if (kk < 1 || kk > nz || (kk == nz && s.top_lid)) return R(0.0);
+        return LRHS[kk * st + c] / C.cfm(kk);
+    };
+    auto lwd = [&](int kk) -> real {
+        real v = R(0.0);
+        if (kk >= 1) v += -dt * d.fnm[kk] * lr2(kk);
+        if (kk + 1 <= nz - 1) v += -dt * d.fnp[kk + 1] * lr2(kk + 1);
+        return v;
+    };
+    auto gg = [&](int kk, int cc) { return GGF[kk * st + cc]; };
+
+    real gc2a = R(0.0), galt = R(0.0), gthr = R(0.0), gchm = R(0.0);
+    real gcfm = R(0.0), gph = R(0.0), gw = R(0.0), mpc = R(0.0), top = R(0.0);
+    const int r1 = k + 1;
+    const bool arow1 = (r1 >= 2 && r1 <= nz - 1) || (r1 == nz && !s.top_lid);
+    if (k < nz) {
+        const real thr = th_ref(d, k, c), alt = d.alt[x], c2a = C.c2a(k);
+        const real chm = C.chm(k), chmp = C.chmp(k);
+        // linearised EOS
+        const real q = alt / (chmp * thr);
+        const real X = f.T1[x] - d.c1h[k] * C.M1 * thr;
+        const real lPP = a.lp[x];
+        const real lAL = a.lal[x] - c2a * lPP;
+        gc2a += lPP * (q * X - f.AL1[x]);
+        const real gq = lPP * c2a * X;
+        galt += gq / (chmp * thr) - lAL * d.c1h[k] * C.M1 / chmp;
+        const real gchmp = -gq * q / chmp - lAL * f.AL1[x] / chmp;
+        gthr += -gq * q / thr + lPP * c2a * q * (-d.c1h[k] * C.M1);
+        mpc += d.c1h[k] * gchmp;
+        // Thomas entries that hold level k's c2a / C(mu_t)
+        if (arow1) {
+            const real ga = -mu(r1) * wsol(k), av = C.a(r1);
+            gc2a += ga * av / c2a;
+            gchm += -ga * av / chm;
+        }
+        if (k >= 1) {
+            const real gb = -mu(k) * wsol(k);
+            const real t1 = cof * d.rdn[k] * d.rdnw[k] * c2a / (chm * C.cfm(k));
+            gc2a += gb * t1 / c2a;
+            gchm += -gb * t1 / chm;
+            const real gcv = -mu(k) * wsol(k + 1);
+            const real cv = -cof * d.rdn[k] * d.rdnw[k] * c2a / (chm * C.cfm(k + 1));
+            gc2a += gcv * cv / c2a;
+            gchm += -gcv * cv / chm;
+        }
+        if (r1 <= nz - 1) {
+            const real gb = -mu(r1) * wsol(r1);
+            const real t2b = cof * d.rdn[r1] * d.rdnw[k] * c2a / (chm * C.cfm(r1));
+            gc2a += gb * t2b / c2a;
+            gchm += -gb * t2b / chm;
+        }
+        if (k == nz - 1) {
+            const real gb = -mu(nz) * wsol(nz);
+            const real t1n = R(2.0) * cof * d.rdnw[nz - 1] * d.rdnw[nz - 1] * c2a
+                             / (chm * C.cfm(nz));
+            gc2a += gb * t1n / c2a;
+            gchm += -gb * t1n / chm;
+        }
+        // w accumulation: the PGF and theta'' terms of rows k and k+1
+        const real tv = half * ((one + eps) * f.T1[x] + (one - eps) * d.th_pp[x])
+                        / ((d.c1h[k] * (C.mut + C.M1) + d.c2h[k]) * thr);
+        if (k >= 1) {
+            const real la = mu(k), du = dup(k);
+            const real cup = half * dt * g * d.rdn[k] * c2a * d.rdnw[k] / chm;
+            gc2a += la * du * cup / c2a;
+            gchm += -la * du * cup / chm;
+            gc2a += la * dt * g * d.rdn[k] * alt * tv;
+            galt += la * dt * g * d.rdn[k] * c2a * tv;
+        }
+        if (r1 <= nz - 1) {
+            const real la = mu(r1), dd = ddn(r1);
+            const real cdn = half * dt * g * d.rdn[r1] * c2a * d.rdnw[k] / chm;
+            gc2a += -la * dd * cdn / c2a;
+            gchm += la * dd * cdn / chm;
+            gc2a -= la * dt * g * d.rdn[r1] * alt * tv;
+            galt -= la * dt * g * d.rdn[r1] * c2a * tv;
+        }
+        if (k == nz - 1 && !s.top_lid) {
+            const real la = mu(nz);
+            const real cb = dt * g * c2a * d.rdnw[nz - 1] * d.rdnw[nz - 1] / chm;
+            const real bt = cb * ddn(nz);
+            gc2a += -la * bt / c2a;
+            gchm += la * bt / chm;
+            gc2a += -la * dt * g * R(2.0) * d.rdnw[nz - 1] * alt * tv;
+            galt += -la * dt * g * R(2.0) * d.rdnw[nz - 1] * c2a * tv;
+        }
+        // t2 = theta''-average / (C(mu_t + mu'') theta)
+        const real chs = d.c1h[k] * (C.mut + C.M1) + d.c2h[k];
+        mpc += -tv * d.c1h[k] / chs * LT2[x];
+        gthr += -tv / thr * LT2[x];
+        // theta of the theta'' fluxes: vertical (Omega'') and horizontal
+        if (k >= 1)
+            gthr += (d.rdnw[k - 1] * gg(k - 1, c) - d.rdnw[k] * gg(k, c))
+                    * WW1[x] * d.fnm[k];
+        if (r1 <= nz - 1)
+            gthr += (d.rdnw[k] * gg(k, c) - d.rdnw[r1] * gg(r1, c))
+                    * WW1[r1 * st + c] * d.fnp[r1];
+        const int j = c / nx, i = c - j * nx;
+        const int cim = j * nx + (i == 0 ? nx - 1 : i - 1);
+        const int cip = j * nx + (i + 1 == nx ? 0 : i + 1);
+        const int cjm = (j == 0 ? ny - 1 : j - 1) * nx + i;
+        const int cjp = (j + 1 == ny ? 0 : j + 1) * nx + i;
+        const real* U1 = f.s[W_U];
+        const real* V1 = f.s[W_V];
+        const real g0 = gg(k, c);
+        gthr += half * s.rdx * (U1[iU(k, j, i + 1, ny, nx)] - U1[iU(k, j, i, ny, nx)]) * g0
+                + half * s.rdy * (V1[iV(k, j + 1, i, ny, nx)] - V1[iV(k, j, i, ny, nx)]) * g0;
+        gthr += half * s.rdx * U1[iU(k, j, i == 0 ? nx : i, ny, nx)] * gg(k, cim)
+                - half * s.rdx * U1[iU(k, j, i + 1 == nx ? 0 : i + 1, ny, nx)] * gg(k, cip)
+                + half * s.rdy * V1[iV(k, j == 0 ? ny : j, i, ny, nx)] * gg(k, cjm)
+                - half * s.rdy * V1[iV(k, j + 1 == ny ? 0 : j + 1, i, ny, nx)] * gg(k, cjp);
+        r.c2a[x] += gc2a;
+        r.alt[x] += galt;
+        r.thr[x] += gthr;
+        mpc += d.c1h[k] * gchm;
+    }
+    // w level k: C(mu_t) of the Thomas rows, the rhs, the phi'' update and
+    // the damper; the Omega'' term's phi difference
+    const real cfmk = C.cfm(k);
+    if (arow1) {
+        const real ga = -mu(r1) * wsol(k);
+        gcfm += -ga * C.a(r1) / cfmk;
+    }
+    if (k >= 1 && k <= nz - 1) {
+        const real gb = -mu(k) * wsol(k);
+        const real t1 = cof * d.rdn[k] * d.rdnw[k] * C.c2a(k) / (C.chm(k) * cfmk);
+        const real t2b = cof * d.rdn[k] * d.rdnw[k - 1] * C.c2a(k - 1)
+                         / (C.chm(k - 1) * cfmk);
+        gcfm += -gb * (t1 + t2b) / cfmk;
+    }
+    if (k >= 2) {
+        const int rm = k - 1;
+        const real gcv = -mu(rm) * wsol(k);
+        const real cv = -cof * d.rdn[rm] * d.rdnw[rm] * C.c2a(rm) / (C.chm(rm) * cfmk);
+        gcfm += -gcv * cv / cfmk;
+    }
+    if (k == nz) {
+        const real gb = -mu(nz) * wsol(nz);
+        const real t1n = R(2.0) * cof * d.rdnw[nz - 1] * d.rdnw[nz - 1] * C.c2a(nz - 1)
+                         / (C.chm(nz - 1) * cfmk);
+        gcfm += -gb * t1n / cfmk;
+    }
+    if (k >= 1) {
+        const real lr = (k == nz && s.top_lid) ? R(0.0) : LRHS[x];
+        gcfm += -lr * rr(k) / (cfmk * cfmk);
+        real ph1 = a.lph[x];
+        if (k < nz) ph1 += d.rdnw[k] / C.chmp(k) * C.lAL(a, k);
+        ph1 += -d.rdnw[k - 1] / C.chmp(k - 1) * C.lAL(a, k - 1);
+        const real cfs = C.cfs(k);
+        mpc += -C.K * f.W1[x] * d.c1f[k] / (cfs * cfs) * ph1;
+        real dw, arg, sn;
+        if (damp_at(d, s, k, c, dw, arg, sn)) {
+            const real lk = a.lw[x] + C.K / cfs * ph1;
+            const real wk = d.w[x], cw = cfmk * wk;
+            const real ws = f.W1[x] * (one + dw) + dw * cw;
+            gw += -lk * dw * cfmk / (one + dw);
+            gcfm += -lk * dw * wk / (one + dw);
+            const real gdw = -lk * (cw + ws) / ((one + dw) * (one + dw));
+            const real garg = gdw * s.dampmag * R(2.0) * sn * (real)cos((double)arg);
+            const real ghk = garg * R(1.5707963) / s.zdamp;
+            gph += ghk / g;
+            top = -ghk / g;
+        }
+    }
+    auto gdp = [&](int kk) {
+        return lwd(kk) * half * (WW1[(kk + 1) * st + c] + WW1[kk * st + c]) * d.rdnw[kk];
+    };
+    if (k >= 1) gph += gdp(k - 1);
+    if (k <= nz - 1) gph -= gdp(k);
+    mpc += d.c1f[k] * gcfm;
+    r.ph[x] += gph;
+    r.w[x] += gw;
+    f.s[W_RM][x] = mpc;
+    f.s[W_RPH][x] = top;
+}
+
+// per column: the level pieces of the column-mass gradient, the damper's
+// top height, and dmdt_rf's mass (the stage u, v face column sums)
+__global__ __launch_bounds__(64) void k_ref_col(Dom d, Sc s, Fw f, RefAcc r)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    if (c >= st) return;
+    const int j = c / nx, i = c - j * nx;
+    const real half = R(0.5), dt = s.dtau;
+    real m = R(0.0), top = R(0.0);
+    for (int k = 0; k <= nz; ++k) {
+        m += f.s[W_RM][k * st + c];
+        top += f.s[W_RPH][k * st + c];
+    }
+    r.ph[nz * st + c] += top;
+    const real* LM1 = f.s[W_LM1];
+    auto su = [&](int ff) {
+        real v = R(0.0);
+        for (int k = 0; k < nz; ++k)
+            v += d.c1h[k] * d.u[iU(k, j, ff, ny, nx)] * d.dnw[k];
+        return v;
+    };
+    auto sv = [&](int gg) {
+        real v = R(0.0);
+        for (int k = 0; k < nz; ++k)
+            v += d.c1h[k] * d.v[iV(k, gg, i, ny, nx)] * d.dnw[k];
+        return v;
+    };
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i + 1 == nx ? 0 : i + 1;
+    const int jm = j == 0 ? ny - 1 : j - 1, jp = j + 1 == ny ? 0 : j + 1;
+    const real gc = dt * LM1[c];
+    const real gmh_c = s.rdx * gc * su(i + 1);
+    const real gmh_m = s.rdx * (dt * LM1[j * nx + im]) * su(i == 0 ? nx : i);
+    const real gml_c = -s.rdx * gc * su(i);
+    const real gml_p = -s.rdx * (dt * LM1[j * nx + ip]) * su(ip);
+    m += half * (gmh_c + gmh_m) + half * (gml_c + gml_p);
+    const real gnh_c = s.rdy * gc * sv(j + 1);
+    const real gnh_m = s.rdy * (dt * LM1[jm * nx + i]) * sv(j == 0 ? ny : j);
+    const real gnl_c = -s.rdy * gc * sv(j);
+    const real gnl_p = -s.rdy * (dt * LM1[jp * nx + i]) * sv(jp);
+    m += half * (gnh_c + gnh_m) + half * (gnl_c + gnl_p);
+    r.mut[c] += m;
+}
+
+// (face, level): dmdt_rf's u / v gradients on every face, and for own faces
+// the advance_uv pieces: the column-mass weight (RM*), the alpha_t term
+// (RA*) and the phi' difference (RG*)
+__global__ __launch_bounds__(256) void k_ref_face(Dom d, Sc s, Fw f, RefAcc r)
+{
+    FaceGeo fg, dg;
+    bool has_dup, isu;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (!face_geo(d, t, fg, dg, has_dup, isu)) return;
+    const int st = d.ny * d.nx, nz = d.nz;
+    const real half = R(0.5), dt = s.dtau;
+    const real rd = isu ? s.rdx : s.rdy;
+    const real* LM1 = f.s[W_LM1];
+    const real c1 = d.c1h[k], c2 = d.c2h[k];
+    real v = R(0.0);
+    if (fg.hasE)
+        v += rd * (c1 * (half * (mut_of(d, fg.cE) + mut_of(d, fg.cEp))) + c2)
+             * (d.dnw[k] * (dt * LM1[fg.cE]));
+    if (fg.hasW)
+        v -= rd * (c1 * (half * (mut_of(d, fg.cWm) + mut_of(d, fg.cW))) + c2)
+             * (d.dnw[k] * (dt * LM1[fg.cW]));
+    (isu ? r.u : r.v)[fg.out0 + k * fg.ostride] += v;
+    if (!fg.own) return;
+    const int A = fg.cA, B = fg.cB, xs = k * st + fg.slot;
+    const real ld = f.s[isu ? W_LDX : W_LDY][xs];
+    auto pd = [&](int kk, int cc) {
+        const int q = kk * st + cc;
+        return d.p_pp[q] + s.smdiv * (d.p_pp[q] - d.p_pp_old[q]);
+    };
+    const int h = k * st, hp = (k + 1) * st;
+    const real dph = (d.ph_pp[hp + A] - d.ph_pp[hp + B])
+                     + (d.ph_pp[h + A] - d.ph_pp[h + B]);
+    const real dpe = pd(k, A) - pd(k, B);
+    real brk = dph + (d.alt[h + A] + d.alt[h + B]) * dpe;
+    if (d.base3d)
+        brk = brk + (d.al_pp[h + A] + d.al_pp[h + B]) * (d.pb[h + A] - d.pb[h + B]);
+    f.s[isu ? W_RMX : W_RMY][xs] = half * rd * c1 * brk * ld;
+    f.s[isu ? W_RAX : W_RAY][xs] = f.s[isu ? W_LBX : W_LBY][xs] * dpe;
+    auto S = [&](int kk) { return pd(kk, A) + pd(kk, B); };
+    auto dpn = [&](int kf) -> real {
+        if (kf == 0) return half * (d.cf1 * S(0) + d.cf2 * S(1) + d.cf3 * S(2));
+        if (kf < nz) return half * (d.fnm[kf] * S(kf) + d.fnp[kf] * S(kf - 1));
+        if (s.top_lid)
+            return half * (d.cf1 * S(nz - 1) + d.cf2 * S(nz - 2) + d.cf3 * S(nz - 3));
+        return R(0.0);
+    };
+    f.s[isu ? W_RGX : W_RGY][xs] =
+        rd * (d.rdnw[k] * (dpn(k + 1) - dpn(k)) - half * c1 * (d.mu_pp[B] + d.mu_pp[A]))
+        * ld;
+}
+
+// per column: gather the own-face pieces onto the cells
+__global__ void k_ref_cells(Dom d, Fw f, RefAcc r)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= st) return;
+    const int j = c / nx, i = c - j * nx;
+    const int ip = j * nx + (i + 1 == nx ? 0 : i + 1);
+    const int jp = (j + 1 == ny ? 0 : j + 1) * nx + i;
+    const real half = R(0.5);
+    const real *RAX = f.s[W_RAX], *RAY = f.s[W_RAY];
+    const real *RGX = f.s[W_RGX], *RGY = f.s[W_RGY];
+    const real *RMX = f.s[W_RMX], *RMY = f.s[W_RMY];
+    real mx = R(0.0), mxp = R(0.0), my = R(0.0), myp = R(0.0);
+    for (int k = 0; k < nz; ++k) {
+        const int x = k * st;
+        r.alt[x + c] += (RAX[x + c] + RAX[x + ip]) + (RAY[x + c] + RAY[x + jp]);
+        const real gx = half * (RGX[x + c] - RGX[x + ip]);
+        const real gy = half * (RGY[x + c] - RGY[x + jp]);
+        r.ph[x + c] += gx + gy;
+        r.ph[x + st + c] += gx + gy;
+        mx += RMX[x + c];
+        mxp += RMX[x + ip];
+        my += RMY[x + c];
+        myp += RMY[x + jp];
+    }
+    r.mut[c] += half * (mx + mxp) + half * (my + myp);
+}
+
+// ---- c2a and alpha_t through the stage EOS (dry, hypsometric_opt = 1) ----
+enum { G_DPH, G_MU3, G_COUNT };
+
+__global__ void k_refg_eos(Dom d, RefAcc r, real* gthp, real* w)
+{
+    const int nz = d.nz, st = d.ny * d.nx;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nz * st) return;
+    const int k = t / st, c = t - k * st;
+    const size_t F = (size_t)(nz + 1) * (d.ny + 1) * (d.nx + 1);
+    const real p = d.p[t], alt = d.alt[t];
+    const real c2a = (GAMMA_F * p) / alt;
+    const real gp = r.c2a[t] * GAMMA_F / alt;
+    const real galt = r.alt[t] - r.c2a[t] * c2a / alt;
+    const real th = th_ref(d, k, c);
+    const real mh = d.c1h[k] * (d.mub2d[c] + d.mup[c]) + d.c2h[k];
+    const real ga = galt - gp * (GAMMA_F * p / alt);
+    gthp[t] = r.thr[t] + gp * (GAMMA_F * p / th);
+    w[G_DPH * F + t] = ga * (-d.rdnw[k] / mh);
+    w[G_MU3 * F + t] = ga * (-alt * d.c1h[k] / mh);
+}
+
+__global__ void k_refg_php(Dom d, RefAcc r, real* gphp, const real* w)
+{
+    const int nz = d.nz, st = d.ny * d.nx;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (nz + 1) * st) return;
+    const int k = t / st;
+    real g = r.ph[t];
+    if (k >= 1) g += w[t - st];
+    if (k <= nz - 1) g -= w[t];
+    gphp[t] = g;
+}
+
+__global__ void k_refg_mu(Dom d, RefAcc r, real* gmup, const real* w)
+{
+    const int nz = d.nz, st = d.ny * d.nx;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= st) return;
+    const size_t F = (size_t)(nz + 1) * (d.ny + 1) * (d.nx + 1);
+    real s = R(0.0);
+    for (int k = 0; k < nz; ++k) s += w[G_MU3 * F + (size_t)k * st + c];
+    gmup[c] = r.mut[c] + s;
+}
+
+// ---- the kinematic lower boundary w(sfc) = u . grad(ht) ----------------
+// (u, v) gradients for lam_w0, added to gu / gv at levels 0..2 (the
+// extrapolation weights cf1..cf3), one thread per face of those levels
+__global__ void k_wsfc_adj(Dom d, real hdx, real hdy, const real* l0,
+                           real* gu, real* gv)
+{
+    const int ny = d.ny, nx = d.nx;
+    const int nu = ny * (nx + 1), nv = (ny + 1) * nx;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (t >= nu + nv) return;
+    const real cf = k == 0 ? d.cf1 : (k == 1 ? d.cf2 : d.cf3);
+    const real* ht = d.ht;
+    if (t < nu) {
+        const int j = t / (nx + 1), fc = t - j * (nx + 1);
+        real v = gu[iU(k, j, fc, ny, nx)];
+        if (fc >= 1) {                  // east face of cell fc-1
+            const int c = j * nx + fc - 1, ce = j * nx + (fc == nx ? 0 : fc);
+            v += cf * (hdx * (ht[ce] - ht[c]) * l0[c]);
+        }
+        if (fc <= nx - 1) {             // west face of cell fc
+            const int c = j * nx + fc, cw = j * nx + (fc == 0 ? nx - 1 : fc - 1);
+            v += cf * (hdx * (ht[c] - ht[cw]) * l0[c]);
+        }
+        gu[iU(k, j, fc, ny, nx)] = v;
+    } else {
+        const int q = t - nu, g = q / nx, i = q - g * nx;
+        real v = gv[iV(k, g, i, ny, nx)];
+        if (g >= 1) {
+            const int c = (g - 1) * nx + i, cn = (g == ny ? 0 : g) * nx + i;
+            v += cf * (hdy * (ht[cn] - ht[c]) * l0[c]);
+        }
+        if (g <= ny - 1) {
+            const int c = g * nx + i, cs = (g == 0 ? ny - 1 : g - 1) * nx + i;
+            v += cf * (hdy * (ht[c] - ht[cs]) * l0[c]);
+        }
+        gv[iV(k, g, i, ny, nx)] = v;
+    }
+}
+
 }  // namespace
 
 }  // namespace ARW_NS
 
 using namespace ARW_NS;
 
-extern "C" {
-
-int32_t ARW_API(acoustic_substep_adjoint)(const Dom* d, const arw_params* prm,
-                                          double dtau, int32_t first,
-                                          const ARW_API(acoustic_adj)* adj,
-                                          real* work, void* stream)
+static int32_t substep_adjoint(const Dom* d, const arw_params* prm,
+                               double dtau, int32_t first,
+                               const ARW_API(acoustic_adj)* adj,
+                               const RefAcc* ref, real* work, void* stream)
 {
     int32_t rc = lib_check_domain(d);
     if (rc) return rc;
@@ -628,17 +1100,108 @@ int32_t ARW_API(acoustic_substep_adjoint)(const Dom* d, const arw_params* prm,
     if ((rc = lib_launched(hipGetLastError(), "k_adj_p2"))) return rc;
     hipLaunchKernelGGL(k_adj_p3, lev, dim3(256), 0, st, *d, s, a, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_p3"))) return rc;
+    if (ref) {       // reads the solve's lambda before the face pass reuses it
+        hipLaunchKernelGGL(k_ref_lev, lev, dim3(256), 0, st, *d, s, a, fw, *ref);
+        if ((rc = lib_launched(hipGetLastError(), "k_ref_lev"))) return rc;
+    }
     hipLaunchKernelGGL(k_adj_s2, colg, dim3(64), 0, st, *d, s, a, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_s2"))) return rc;
+    if (ref) {
+        hipLaunchKernelGGL(k_ref_col, colg, dim3(64), 0, st, *d, s, fw, *ref);
+        if ((rc = lib_launched(hipGetLastError(), "k_ref_col"))) return rc;
+    }
     const int nf = ny * (nx + 1) + (ny + 1) * nx;
     const dim3 flev((nf + 255) / 256, nz);
     hipLaunchKernelGGL(k_adj_f1, flev, dim3(256), 0, st, *d, s, a, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_f1"))) return rc;
+    if (ref) {
+        hipLaunchKernelGGL(k_ref_face, flev, dim3(256), 0, st, *d, s, fw, *ref);
+        if ((rc = lib_launched(hipGetLastError(), "k_ref_face"))) return rc;
+    }
     hipLaunchKernelGGL(k_adj_f2, flev, dim3(256), 0, st, *d, s, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_f2"))) return rc;
     hipLaunchKernelGGL(k_adj_cells, dim3((cols + 127) / 128), dim3(128), 0, st,
                        *d, s, a, fw);
-    return lib_launched(hipGetLastError(), "k_adj_cells");
+    if ((rc = lib_launched(hipGetLastError(), "k_adj_cells"))) return rc;
+    if (!ref) return ARW_OK;
+    hipLaunchKernelGGL(k_ref_cells, dim3((cols + 127) / 128), dim3(128), 0, st,
+                       *d, fw, *ref);
+    return lib_launched(hipGetLastError(), "k_ref_cells");
+}
+
+static bool ref_ok(const ARW_API(acoustic_ref)* acc)
+{
+    return acc && acc->mut && acc->c2a && acc->alt && acc->thr && acc->ph
+           && acc->w && acc->u && acc->v;
+}
+
+static RefAcc ref_of(const ARW_API(acoustic_ref)* acc)
+{
+    RefAcc r;
+    r.mut = acc->mut; r.c2a = acc->c2a; r.alt = acc->alt; r.thr = acc->thr;
+    r.ph = acc->ph; r.w = acc->w; r.u = acc->u; r.v = acc->v;
+    return r;
+}
+
+extern "C" {
+
+int32_t ARW_API(acoustic_substep_adjoint)(const Dom* d, const arw_params* prm,
+                                          double dtau, int32_t first,
+                                          const ARW_API(acoustic_adj)* adj,
+                                          real* work, void* stream)
+{
+    return substep_adjoint(d, prm, dtau, first, adj, nullptr, work, stream);
+}
+
+int32_t ARW_API(acoustic_substep_adjoint_ref)(
+    const Dom* d, const arw_params* prm, double dtau, int32_t first,
+    const ARW_API(acoustic_adj)* adj, const ARW_API(acoustic_ref)* acc,
+    real* work, void* stream)
+{
+    if (!ref_ok(acc)) return lib_fail(ARW_E_ARG, "null reference accumulator");
+    if (d && d->qv)
+        return lib_fail(ARW_E_UNSUPPORTED, "reference adjoint: dry states only");
+    const RefAcc r = ref_of(acc);
+    return substep_adjoint(d, prm, dtau, first, adj, &r, work, stream);
+}
+
+int32_t ARW_API(acoustic_reference_gradients)(
+    const Dom* d, const ARW_API(acoustic_ref)* acc, real* g_thp, real* g_php,
+    real* g_mup, real* work, void* stream)
+{
+    int32_t rc = lib_check_domain(d);
+    if (rc) return rc;
+    if (!ref_ok(acc) || !g_thp || !g_php || !g_mup || !work)
+        return lib_fail(ARW_E_ARG, "null argument");
+    if (d->qv) return lib_fail(ARW_E_UNSUPPORTED, "reference adjoint: dry states only");
+    const RefAcc r = ref_of(acc);
+    hipStream_t st = (hipStream_t)stream;
+    const int nz = d->nz, cols = d->ny * d->nx;
+    const int nm = nz * cols, nw = (nz + 1) * cols;
+    hipLaunchKernelGGL(k_refg_eos, dim3((nm + 255) / 256), dim3(256), 0, st,
+                       *d, r, g_thp, work);
+    if ((rc = lib_launched(hipGetLastError(), "k_refg_eos"))) return rc;
+    hipLaunchKernelGGL(k_refg_php, dim3((nw + 255) / 256), dim3(256), 0, st,
+                       *d, r, g_php, work);
+    if ((rc = lib_launched(hipGetLastError(), "k_refg_php"))) return rc;
+    hipLaunchKernelGGL(k_refg_mu, dim3((cols + 255) / 256), dim3(256), 0, st,
+                       *d, r, g_mup, work);
+    return lib_launched(hipGetLastError(), "k_refg_mu");
+}
+
+int32_t ARW_API(set_w_surface_adjoint)(const Dom* d, const arw_params* prm,
+                                       const real* lam_w0, real* g_u,
+                                       real* g_v, void* stream)
+{
+    int32_t rc = lib_check_domain(d);
+    if (rc) return rc;
+    if (!prm || !lam_w0 || !g_u || !g_v) return lib_fail(ARW_E_ARG, "null argument");
+    if (d->nz < 3) return lib_fail(ARW_E_ARG, "set_w_surface adjoint needs nz >= 3");
+    const int nf = d->ny * (d->nx + 1) + (d->ny + 1) * d->nx;
+    hipLaunchKernelGGL(k_wsfc_adj, dim3((nf + 255) / 256, 3), dim3(256), 0,
+                       (hipStream_t)stream, *d, (real)(0.5 / prm->dx),
+                       (real)(0.5 / prm->dy), lam_w0, g_u, g_v);
+    return lib_launched(hipGetLastError(), "k_wsfc_adj");
 }
 
 int32_t ARW_API(acoustic_adjoint_work)(int32_t nz, int32_t ny, int32_t nx,

@@ -136,6 +136,14 @@ def stage_prepare(state: DomainState, cfg: RunConfig, dtau: float,
                   int(bool(do_eos)), _stream()), name)
 
 
+def small_step_finish(state: DomainState) -> None:
+    """Recover the stage's new u, v, w, theta', phi', mu' from the
+    small-step fields (reference dycore.cu:1087-1198)."""
+    _check_state(state)
+    name, fn = _fn(state, "small_step_finish")
+    _lib.check(fn(state.domain_struct(), _stream()), name)
+
+
 def set_w_surface(state: DomainState, cfg: RunConfig) -> None:
     """w(sfc) = u . grad(ht) (reference dycore.py:1999-2026)."""
     _check_state(state)

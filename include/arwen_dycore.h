@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define ARW_ABI_VERSION 4
+#define ARW_ABI_VERSION 5
 
 #define ARW_OK 0
 #define ARW_E_ARG (-1)      /* invalid argument (null pointer, bad extent) */
@@ -242,6 +242,41 @@ int32_t arw_acoustic_substep_adjoint(const arw_domain *d,
                                      float *work, void *stream);
 int32_t arw_acoustic_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
                                   int64_t *elements);
+
+/* The substep adjoint with the stage reference state's share (no
+ * counterpart in the reference, which has no adjoint; with the
+ * small-step init/finish and tendency-pass adjoints it composes the
+ * adjoint of a whole dycore.py:2161-2503 step).  The stage fields enter a
+ * substep through c2a = gamma p / alpha_t and alpha_t of the stage EOS,
+ * the column mass mu_t, theta, the full phi, w (the damper) and u, v
+ * (dmdt_rf): the call ADDS J^T lam with respect to those to the
+ * accumulators of acc (mut: (ny,nx); c2a, alt, thr: mass points; ph, w:
+ * w levels; u, v: faces), which the caller clears at the stage start and
+ * keeps over the stage's substeps.  Otherwise as
+ * arw_acoustic_substep_adjoint (same work size).  Dry states only.
+ * arw_acoustic_reference_gradients maps the accumulated c2a / alt through
+ * the stage EOS (d->p, d->alt must hold it): g_thp, g_php, g_mup are
+ * written, the u, v, w gradients are acc->u, v, w themselves; work: 2
+ * fields of (nz+1)(ny+1)(nx+1), e.g. the substep adjoint's work.
+ * arw_set_w_surface_adjoint ADDS the (u, v) gradients of the kinematic
+ * lower boundary w(sfc) = u . grad(ht) (replaces nothing: the transpose of
+ * dycore.py:1999-2026) for the adjoint lam_w0 (ny,nx) of w(sfc). */
+typedef struct arw_acoustic_ref {
+    float *mut, *c2a, *alt, *thr, *ph, *w, *u, *v;
+} arw_acoustic_ref;
+int32_t arw_acoustic_substep_adjoint_ref(const arw_domain *d,
+                                         const arw_params *prm, double dtau,
+                                         int32_t first,
+                                         const arw_acoustic_adj *adj,
+                                         const arw_acoustic_ref *acc,
+                                         float *work, void *stream);
+int32_t arw_acoustic_reference_gradients(const arw_domain *d,
+                                         const arw_acoustic_ref *acc,
+                                         float *g_thp, float *g_php, float *g_mup,
+                                         float *work, void *stream);
+int32_t arw_set_w_surface_adjoint(const arw_domain *d, const arw_params *prm,
+                                  const float *lam_w0, float *g_u, float *g_v,
+                                  void *stream);
 
 /* EOS p, al, alt from thp, php, mup (replaces diagnostics.cu:37-89
  * calc_p_alpha, launched by diagnostics.py:14-57; hypsometric_opt = 1). */
@@ -527,6 +562,23 @@ int32_t arw64_acoustic_substep_adjoint(const arw64_domain *d,
                                        double *work, void *stream);
 int32_t arw64_acoustic_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
                                     int64_t *elements);
+
+typedef struct arw64_acoustic_ref {
+    double *mut, *c2a, *alt, *thr, *ph, *w, *u, *v;
+} arw64_acoustic_ref;
+int32_t arw64_acoustic_substep_adjoint_ref(const arw64_domain *d,
+                                         const arw_params *prm, double dtau,
+                                         int32_t first,
+                                         const arw64_acoustic_adj *adj,
+                                         const arw64_acoustic_ref *acc,
+                                         double *work, void *stream);
+int32_t arw64_acoustic_reference_gradients(const arw64_domain *d,
+                                         const arw64_acoustic_ref *acc,
+                                         double *g_thp, double *g_php, double *g_mup,
+                                         double *work, void *stream);
+int32_t arw64_set_w_surface_adjoint(const arw64_domain *d, const arw_params *prm,
+                                  const double *lam_w0, double *g_u, double *g_v,
+                                  void *stream);
 
 typedef struct arw64_stage_adj {
     const double *l_ru_t, *l_rv_t, *l_rw_t, *l_rth_t, *l_rph_t;

@@ -1,0 +1,220 @@
+"""GPU checks of the whole-step adjoint (adjoint.step_adjoint,
+adjoint.DycoreStep) and of its last two pieces, the substep adjoint's
+stage-reference share (arw_acoustic_substep_adjoint_ref with
+arw_acoustic_reference_gradients) and the kinematic lower boundary's
+adjoint (arw_set_w_surface_adjoint):
+
+- FP64: each equals the oracle's hand adjoint (oracle/adjoint_np.py:
+  acoustic_substep_adjoint(ref=True), set_w_surface_adjoint, step_adjoint,
+  themselves checked against complex-step JVPs of the oracle's forward) to
+  1e-10 of the largest gradient, flat and terrain, rigid and open top,
+  with and without the upper damper;
+- FP64: <J v, lam> from central differences of the device's own fused
+  step (arw_step) agrees with <v, J^T lam> (the step is nonlinear, so to
+  the differencing error);
+- autograd through DycoreStep gives the direct call's gradients and leaves
+  the inputs untouched; FP32 agrees with FP64 to FP32 accuracy.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import adjoint_np as A
+from oracle import dycore_np as O
+from tests.test_gpu_acoustic_adjoint import NAMES_OUT, _device_pair, _rel
+
+pytestmark = pytest.mark.gpu
+
+ORACLE_TOL = 1e-10
+
+
+@pytest.mark.parametrize("hill,damp,top_lid,first", [
+    (0.0, False, True, True), (0.0, False, True, False),
+    (300.0, True, True, False), (0.0, True, False, False),
+    (300.0, False, False, True)])
+def test_substep_reference_share_matches_oracle(hill, damp, top_lid, first):
+    from arwen_amd import adjoint
+    o, coefs, dtau, cfg, rng, g = _device_pair(hill=hill, damp=damp,
+                                               top_lid=top_lid)
+    print(f"substep reference adjoint hill={hill} damp={damp} "
+          f"lid={top_lid} first={first}", flush=True)
+    lam = {n: rng.standard_normal(getattr(o, n).shape) for n in NAMES_OUT}
+    ref = A.acoustic_substep_adjoint(o, coefs, dtau, cfg, first, lam,
+                                     ref=True)
+    tl = {n: torch.as_tensor(v, device=g.u.device) for n, v in lam.items()}
+    acc = adjoint.reference_accumulators(g)
+    got = adjoint.acoustic_substep_adjoint(g, cfg, dtau, first, tl, acc)
+    plain = adjoint.acoustic_substep_adjoint(g, cfg, dtau, first, tl)
+    for n in A.SUBSTEP_STATE + A.SUBSTEP_TEND:
+        assert torch.equal(got[n], plain[n]), n
+    rg = adjoint.reference_gradients(g, acc)
+    bad = {n: _rel(rg[n], ref["ref"][n]) for n in A.STAGE_IN}
+    assert max(bad.values()) < ORACLE_TOL, bad
+
+
+def test_reference_accumulators_add_over_substeps():
+    from arwen_amd import adjoint
+    o, coefs, dtau, cfg, rng, g = _device_pair(hill=200.0, damp=True)
+    print("substep reference accumulation", flush=True)
+    dev = g.u.device
+    lams = [{n: torch.as_tensor(rng.standard_normal(getattr(o, n).shape),
+                                device=dev) for n in NAMES_OUT}
+            for _ in range(2)]
+    both = adjoint.reference_accumulators(g)
+    single = []
+    for lam in lams:
+        adjoint.acoustic_substep_adjoint(g, cfg, dtau, False, lam, both)
+        one = adjoint.reference_accumulators(g)
+        adjoint.acoustic_substep_adjoint(g, cfg, dtau, False, lam, one)
+        single.append(one)
+    for n in adjoint.ACOUSTIC_REF:
+        s = single[0][n] + single[1][n]
+        scale = float(s.abs().max()) or 1.0
+        assert float((both[n] - s).abs().max()) <= 1e-13 * scale, n
+
+
+def test_set_w_surface_adjoint_matches_oracle():
+    from arwen_amd import adjoint
+    o, coefs, dtau, cfg, rng, g = _device_pair(hill=300.0)
+    print("set_w_surface adjoint", flush=True)
+    l0 = rng.standard_normal(o.mup.shape)
+    gu0 = rng.standard_normal(o.u.shape)
+    gv0 = rng.standard_normal(o.v.shape)
+    eu, ev = A.set_w_surface_adjoint(o, l0, cfg.dx, cfg.dy)
+    dev = g.u.device
+    gu = torch.as_tensor(gu0.copy(), device=dev)
+    gv = torch.as_tensor(gv0.copy(), device=dev)
+    adjoint.set_w_surface_adjoint(g, cfg, torch.as_tensor(l0, device=dev),
+                                  gu, gv)
+    assert _rel(gu, gu0 + eu) < 1e-13
+    assert _rel(gv, gv0 + ev) < 1e-13
+    assert np.abs(eu).max() > 0.0
+
+
+def _step_states(name, dtype=torch.float64, seed=31):
+    """A disturbed oracle FP64 state (as the oracle's whole-step test) and a
+    device state of dtype holding the same fields."""
+    from arwen_amd import dycore
+    from arwen_amd.state import DomainState
+    from tests import pins
+    case = pins.CASES[name]()
+    cfg = case.cfg
+    s = O.from_case(case, np.float64)
+    rng = np.random.default_rng(seed)
+    s.u[...] += rng.standard_normal(s.u.shape) * 0.5
+    s.v[...] += rng.standard_normal(s.v.shape) * 0.5
+    s.w[1:-1] += rng.standard_normal(s.w[1:-1].shape) * 0.3
+    s.mup[...] += rng.standard_normal(s.mup.shape) * 20.0
+    s.php[1:] += rng.standard_normal(s.php[1:].shape) * 30.0
+    O.set_w_surface(s, cfg.dx, cfg.dy)
+    if dtype == torch.float32:
+        for n in A.STAGE_IN:
+            x = getattr(s, n)
+            x[...] = x.astype(np.float32).astype(np.float64)
+    O.calc_p_alpha(s)
+    g = DomainState.from_host(case, dtype=dtype)
+    for n in A.STAGE_IN:
+        getattr(g, n).copy_(torch.as_tensor(getattr(s, n)))
+    dycore.update_diagnostics(g)
+    return case, s, g, rng
+
+
+@pytest.mark.parametrize("name", ["dry_flat", "terrain"])
+def test_step_adjoint_matches_oracle(name):
+    from arwen_amd import adjoint
+    case, s, g, rng = _step_states(name)
+    print(f"step adjoint vs oracle {name}", flush=True)
+    lam = {n: rng.standard_normal(getattr(s, n).shape) for n in A.STAGE_IN}
+    ref = A.step_adjoint(s, case.cfg, lam)
+    before = {n: getattr(g, n).clone() for n in A.STAGE_IN}
+    got = adjoint.step_adjoint(
+        g, case.cfg, {n: torch.as_tensor(v, device=g.u.device)
+                      for n, v in lam.items()})
+    for n in A.STAGE_IN:
+        assert torch.equal(getattr(g, n), before[n]), n
+    bad = {n: _rel(got[n], ref[n]) for n in A.STAGE_IN}
+    assert max(bad.values()) < ORACLE_TOL, bad
+
+
+@pytest.mark.parametrize("name", ["dry_flat", "terrain"])
+def test_step_adjoint_dot_product_vs_device_step(name):
+    from arwen_amd import adjoint, dycore
+    case, s, g, rng = _step_states(name, seed=37)
+    cfg = case.cfg
+    print(f"step adjoint FD dot product {name}", flush=True)
+    dev = g.u.device
+    base = {n: getattr(g, n).clone() for n in A.STAGE_IN}
+    lam = {n: torch.as_tensor(rng.standard_normal(getattr(s, n).shape),
+                              device=dev) for n in A.STAGE_IN}
+    gt = adjoint.step_adjoint(g, cfg, lam)
+    scale = {"u": 1.0, "v": 1.0, "w": 0.1, "thp": 0.5, "php": 20.0,
+             "mup": 20.0}
+    v = {n: torch.as_tensor(rng.standard_normal(getattr(s, n).shape)
+                            * scale[n], device=dev) for n in A.STAGE_IN}
+    h = 1e-6
+
+    def run(sign):
+        for n in A.STAGE_IN:
+            getattr(g, n).copy_(base[n] + sign * h * v[n])
+        dycore.update_diagnostics(g)
+        dycore.step(g, cfg)
+        return {n: getattr(g, n).clone() for n in A.STAGE_IN}
+
+    fp, fm = run(1.0), run(-1.0)
+    jv = sum(float(torch.sum((fp[n] - fm[n]) / (2 * h) * lam[n]))
+             for n in A.STAGE_IN)
+    vjt = sum(float(torch.sum(v[n] * gt[n])) for n in A.STAGE_IN)
+    assert abs(jv - vjt) <= 1e-6 * abs(jv), (jv, vjt)
+
+
+def test_dycore_step_autograd():
+    from arwen_amd import adjoint
+    case, s, g, rng = _step_states("terrain", seed=41)
+    cfg = case.cfg
+    print("dycore step autograd", flush=True)
+    dev = g.u.device
+    x0 = [getattr(g, n).clone() for n in A.STAGE_IN]
+    lam = {n: torch.as_tensor(rng.standard_normal(getattr(s, n).shape),
+                              device=dev) for n in A.STAGE_IN}
+    direct = adjoint.step_adjoint(g, cfg, lam)
+    xs = [x.clone().requires_grad_() for x in x0]
+    outs = adjoint.DycoreStep.apply(g, cfg, *xs)
+    loss = sum(torch.sum(o * lam[n]) for o, n in zip(outs, A.STAGE_IN))
+    grads = torch.autograd.grad(loss, xs)
+    for x, x_ in zip(xs, x0):
+        assert torch.equal(x.detach(), x_)
+    for n, gr in zip(A.STAGE_IN, grads):
+        assert torch.allclose(gr, direct[n], rtol=1e-12, atol=0.0), n
+
+
+def test_step_adjoint_fp32_close_to_fp64():
+    from arwen_amd import adjoint
+    case, s, g64, rng = _step_states("terrain", seed=43,
+                                     dtype=torch.float32)
+    _, _, g32, _ = _step_states("terrain", seed=43, dtype=torch.float32)
+    from arwen_amd.state import DomainState
+    g64 = DomainState.from_host(case, dtype=torch.float64)
+    for n in A.STAGE_IN:
+        getattr(g64, n).copy_(getattr(g32, n).double())
+    from arwen_amd import dycore
+    dycore.update_diagnostics(g64)
+    print("step adjoint fp32 vs fp64", flush=True)
+    lam = {n: torch.as_tensor(rng.standard_normal(getattr(s, n).shape),
+                              device=g64.u.device) for n in A.STAGE_IN}
+    a64 = adjoint.step_adjoint(g64, case.cfg, lam)
+    a32 = adjoint.step_adjoint(g32, case.cfg,
+                               {n: t.float() for n, t in lam.items()})
+    bad = {n: _rel(a32[n].double(), a64[n].cpu().numpy())
+           for n in A.STAGE_IN}
+    print("fp32 vs fp64 relative errors", bad, flush=True)
+    assert max(bad.values()) < 1e-3, bad
+
+
+def test_step_adjoint_rejects_unsupported():
+    from arwen_amd import adjoint
+    from arwen_amd._lib import ArwenError
+    case, s, g, rng = _step_states("dry_flat")
+    lam = {"u": torch.ones_like(g.u)}
+    with pytest.raises(ArwenError, match="dry states"):
+        adjoint.step_adjoint(g, case.cfg.with_(khdif=75.0, kvdif=75.0,
+                                               km_opt=1), lam)
