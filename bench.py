@@ -437,6 +437,29 @@ def main_adjoint(args):
         rec.update({"stage_pass_ms": round(t_sp, 4),
                     "stage_pass_adjoint_ms": round(t_sa, 4),
                     "stage_pass_adjoint_to_forward": round(t_sa / t_sp, 3)})
+        # one whole dry step against its adjoint (the forward re-run with
+        # checkpoints, then the reverse sweep over the three stages)
+        saved_d = {n: getattr(sd, n).clone() for n in adjoint.STAGE_STATE}
+
+        def restore_d():
+            for n, t in saved_d.items():
+                getattr(sd, n).copy_(t)
+
+        def dry_step():
+            restore_d()
+            dycore.step(sd, cfg_dry)
+
+        t_rd = _time_ms(restore_d, reps)
+        t_ds = _time_ms(dry_step, reps) - t_rd
+        restore_d()
+        dycore.update_diagnostics(sd)
+        lam_s = {n: torch.randn_like(getattr(sd, n))
+                 for n in adjoint.STAGE_STATE}
+        t_da = _time_ms(lambda: adjoint.step_adjoint(sd, cfg_dry, lam_s),
+                        reps)
+        rec.update({"dry_step_ms": round(t_ds, 4),
+                    "step_adjoint_ms": round(t_da, 4),
+                    "step_adjoint_to_forward": round(t_da / t_ds, 3)})
         del sd
         # the four advection kernels at the stage mass fluxes of the
         # state: forward tendency against the adjoint (field and all three
