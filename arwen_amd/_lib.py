@@ -125,7 +125,9 @@ ArwInitAdj64 = ArwInitAdj
 ArwFinishAdj64 = ArwFinishAdj
 
 #: accumulators of the substep adjoint's stage-reference share
-ACOUSTIC_REF = ("mut", "c2a", "alt", "thr", "ph", "w", "u", "v")
+ACOUSTIC_REF = ("mut", "c2a", "alt", "thr", "ph", "w", "u", "v", "grf")
+#: outputs of a substep the reference-share adjoint can take as given
+ACOUSTIC_FWD = ("u_pp", "v_pp", "w_pp", "th_pp", "mu_pp", "al_pp", "ww_pp")
 
 
 class ArwAcousticRef(ctypes.Structure):
@@ -134,6 +136,14 @@ class ArwAcousticRef(ctypes.Structure):
 
 
 ArwAcousticRef64 = ArwAcousticRef
+
+
+class ArwAcousticFwd(ctypes.Structure):
+    """arw_acoustic_fwd (and arw64_acoustic_fwd: same layout)."""
+    _fields_ = [(n, _P) for n in ACOUSTIC_FWD]
+
+
+ArwAcousticFwd64 = ArwAcousticFwd
 
 
 class ArwenError(RuntimeError):
@@ -183,8 +193,9 @@ def load():
         "arw_acoustic_substep_adjoint_ref": ([dp, pp, _D, _I,
                                               ctypes.POINTER(ArwAcousticAdj),
                                               ctypes.POINTER(ArwAcousticRef),
+                                              ctypes.POINTER(ArwAcousticFwd),
                                               _P, _P], _I),
-        "arw_acoustic_reference_gradients": ([dp,
+        "arw_acoustic_reference_gradients": ([dp, pp,
                                               ctypes.POINTER(ArwAcousticRef),
                                               _P, _P, _P, _P, _P], _I),
         "arw_set_w_surface_adjoint": ([dp, pp, _P, _P, _P, _P], _I),

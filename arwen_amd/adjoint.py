@@ -163,9 +163,27 @@ ACOUSTIC_STATE = _lib.ACOUSTIC_STATE
 ACOUSTIC_TEND = _lib.ACOUSTIC_TEND
 
 
+def _dom_views(state: DomainState, cfg: RunConfig, views: dict | None):
+    """The state's C view with some arrays replaced by other tensors of the
+    same shape (checkpoints the call reads in place of the state's)."""
+    d = _dom(state, cfg)
+    nz, ny, nx = state.shape
+    scratch = {"ww": (nz + 1, ny, nx), "dmdt_rf": (ny, nx)}
+    for n, t in (views or {}).items():
+        want = scratch[n] if n in scratch else tuple(getattr(state, n).shape)
+        if tuple(t.shape) != want or t.dtype != state.dtype \
+                or t.device != state.u.device or not t.is_contiguous():
+            raise ValueError(f"view of {n}: expected a contiguous {want} "
+                             f"{state.dtype} tensor on {state.u.device}")
+        setattr(d, n, t.data_ptr())
+    return d
+
+
 def acoustic_substep_adjoint(state: DomainState, cfg: RunConfig,
                              dtau: float, first: bool,
-                             lam: dict, ref: dict | None = None) -> dict:
+                             lam: dict, ref: dict | None = None,
+                             fwd: dict | None = None,
+                             views: dict | None = None) -> dict:
     """J^T lam for one acoustic substep taken from the state's current
     small-step fields (coefficients prepared as for the forward substep,
     ``dycore.prepare_acoustic_coefficients``).  ``lam`` maps the outputs
@@ -173,8 +191,12 @@ def acoustic_substep_adjoint(state: DomainState, cfg: RunConfig,
     returns the adjoints of ACOUSTIC_STATE + ACOUSTIC_TEND.  ``ref``
     (from ``reference_accumulators``): also add the share of the stage
     reference state to those accumulators (dry states; map them with
-    ``reference_gradients`` once the stage's substeps are done).  The
-    state is not modified."""
+    ``reference_gradients`` once the stage's substeps are done); with
+    ``ref``, ``fwd`` may give the substep's outputs (ACOUSTIC_FWD names:
+    the small-step fields and Omega'' after the forward substep), which
+    are then not recomputed.  ``views``: tensors read in place of the
+    state's arrays of those names (e.g. a checkpointed substep input).
+    The state is not modified."""
     _check_state(state)
     lib = _lib.load()
     pre = "arw64_" if state.dtype == torch.float64 else "arw_"
@@ -200,18 +222,32 @@ def acoustic_substep_adjoint(state: DomainState, cfg: RunConfig,
     for name in ACOUSTIC_STATE + ACOUSTIC_TEND:
         out[name] = torch.empty_like(getattr(state, name))
         setattr(a, "g_" + name, out[name].data_ptr())
+    d = _dom_views(state, cfg, views)
     if ref is None:
+        if fwd is not None:
+            raise ValueError("fwd needs ref")
         fn = getattr(lib, pre + "acoustic_substep_adjoint")
-        _lib.check(fn(_dom(state, cfg), params(cfg), float(dtau),
-                      int(bool(first)), ctypes.byref(a), work.data_ptr(),
-                      _stream()), pre + "acoustic_substep_adjoint")
+        _lib.check(fn(d, params(cfg), float(dtau), int(bool(first)),
+                      ctypes.byref(a), work.data_ptr(), _stream()),
+                   pre + "acoustic_substep_adjoint")
         return out
     r = _ref_struct(state, ref)
+    fp = None
+    if fwd is not None:
+        fs = _lib.ArwAcousticFwd()
+        for n in _lib.ACOUSTIC_FWD:
+            t = fwd[n]
+            like = getattr(state, n)
+            if tuple(t.shape) != tuple(like.shape) or t.dtype != like.dtype \
+                    or t.device != like.device or not t.is_contiguous():
+                raise ValueError(f"substep output {n}: expected a contiguous "
+                                 f"{tuple(like.shape)} {like.dtype} tensor")
+            setattr(fs, n, t.data_ptr())
+        fp = ctypes.byref(fs)
     fn = getattr(lib, pre + "acoustic_substep_adjoint_ref")
-    _lib.check(fn(_dom(state, cfg), params(cfg), float(dtau),
-                  int(bool(first)), ctypes.byref(a), ctypes.byref(r),
-                  work.data_ptr(), _stream()),
-               pre + "acoustic_substep_adjoint_ref")
+    _lib.check(fn(d, params(cfg), float(dtau), int(bool(first)),
+                  ctypes.byref(a), ctypes.byref(r), fp, work.data_ptr(),
+                  _stream()), pre + "acoustic_substep_adjoint_ref")
     return out
 
 
@@ -223,13 +259,13 @@ def _ref_shapes(state: DomainState) -> dict:
     return {"mut": (ny, nx), "c2a": (nz, ny, nx), "alt": (nz, ny, nx),
             "thr": (nz, ny, nx), "ph": (nz + 1, ny, nx),
             "w": (nz + 1, ny, nx), "u": (nz, ny, nx + 1),
-            "v": (nz, ny + 1, nx)}
+            "v": (nz, ny + 1, nx), "grf": (ny, nx)}
 
 
 def reference_accumulators(state: DomainState) -> dict:
     """Zeroed accumulators of the substep adjoint's stage-reference
-    share: mut (ny,nx); c2a, alt, thr (mass points); ph, w (w levels);
-    u, v (faces)."""
+    share: mut, grf (ny,nx); c2a, alt, thr (mass points); ph, w (w
+    levels); u, v (faces)."""
     return {n: torch.zeros(shp, dtype=state.dtype, device=state.u.device)
             for n, shp in _ref_shapes(state).items()}
 
@@ -248,11 +284,14 @@ def _ref_struct(state, ref):
     return r
 
 
-def reference_gradients(state: DomainState, ref: dict) -> dict:
+def reference_gradients(state: DomainState, cfg: RunConfig, ref: dict,
+                        views: dict | None = None) -> dict:
     """The stage-state gradients (STAGE_STATE names) of the substep
-    adjoints accumulated in ``ref`` over one stage: c2a and alpha_t mapped
-    through the stage EOS (the state's p and alt must hold it, as after
-    ``dycore.stage_prepare``).  u, v, w are the accumulators themselves."""
+    adjoints accumulated in ``ref`` over one stage: dmdt_rf's share added
+    to ref's u, v, then c2a and alpha_t mapped through the stage EOS (the
+    state's p and alt, or those of ``views``, must hold it, as after
+    ``dycore.stage_prepare``).  u, v, w are the accumulators
+    themselves."""
     _check_state(state)
     lib = _lib.load()
     pre = "arw64_" if state.dtype == torch.float64 else "arw_"
@@ -265,7 +304,8 @@ def reference_gradients(state: DomainState, ref: dict) -> dict:
            "php": torch.empty_like(state.php),
            "mup": torch.empty_like(state.mup)}
     _lib.check(getattr(lib, pre + "acoustic_reference_gradients")(
-        state.domain_struct(), ctypes.byref(r), out["thp"].data_ptr(),
+        _dom_views(state, cfg, views), params(cfg), ctypes.byref(r),
+        out["thp"].data_ptr(),
         out["php"].data_ptr(), out["mup"].data_ptr(), work.data_ptr(),
         _stream()), pre + "acoustic_reference_gradients")
     return out
@@ -459,11 +499,13 @@ def small_step_init_adjoint(state: DomainState, lam: dict) -> dict:
     return out
 
 
-def small_step_finish_adjoint(state: DomainState, lam: dict) -> dict:
+def small_step_finish_adjoint(state: DomainState, lam: dict,
+                              views: dict | None = None) -> dict:
     """J^T lam of the small-step finish (reference dycore.cu:1087-1198) at
-    the state's stage fields and small-step fields: ``lam`` maps the new
-    stage fields (STAGE_STATE names) to adjoints; returns gradients of
-    STAGE_STATE and FINISH_PP."""
+    the state's stage fields and small-step fields (or the tensors of
+    ``views`` in their place): ``lam`` maps the new stage fields
+    (STAGE_STATE names) to adjoints; returns gradients of STAGE_STATE and
+    FINISH_PP."""
     _check_state(state)
     lib, pre, work = _small_step_work(state)
     a = _lib.ArwFinishAdj()
@@ -479,8 +521,12 @@ def small_step_finish_adjoint(state: DomainState, lam: dict) -> dict:
     for name in STAGE_STATE + FINISH_PP:
         out[name] = torch.empty_like(getattr(state, name))
         setattr(a, "g_" + name, out[name].data_ptr())
+    d = state.domain_struct()
+    for n, t in (views or {}).items():
+        _as_like(t, getattr(state, n), n)
+        setattr(d, n, t.data_ptr())
     _lib.check(getattr(lib, pre + "small_step_finish_adjoint")(
-        state.domain_struct(), ctypes.byref(a), work.data_ptr(), _stream()),
+        d, ctypes.byref(a), work.data_ptr(), _stream()),
         pre + "small_step_finish_adjoint")
     return out
 
@@ -497,6 +543,10 @@ def _step_stages(cfg: RunConfig):
             (ns, cfg.dt / ns))
 
 
+#: what a stage's preparation leaves that its substeps read
+_PREP = ("p", "al", "alt", "ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t")
+
+
 def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict) -> dict:
     """J^T lam of one dycore step (``dycore.step``; reference dycore.py:
     2161-2503) at the state's prognostic fields: ``lam`` maps the new
@@ -505,14 +555,15 @@ def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict) -> dict:
     without constant-K diffusion, plain form.
 
     The forward is re-run stage by stage on the device with checkpoints
-    (each stage's state, each substep's nine small-step fields, the
-    finish's input), then reversed: per stage the finish adjoint, the
-    substep adjoints in reverse order (adding the stage-reference share),
-    the small-step initialisation adjoint (to the time-t and stage fields)
-    and the tendency-pass adjoint; the stages chain through their states
-    and the kinematic lower boundary of the step's end comes first.  The
-    state's prognostic fields are left as they were (their EOS
-    recomputed); its other fields are overwritten."""
+    (each stage's state and what its preparation leaves -- EOS, slow
+    tendencies, Omega, dmdt_rf -- and every substep's small-step fields and
+    Omega''), then reversed: per stage the finish adjoint, the substep
+    adjoints in reverse order (given each substep's outputs, adding the
+    stage-reference share), the small-step initialisation adjoint (to the
+    time-t and stage fields) and the tendency-pass adjoint; the stages
+    chain through their states and the kinematic lower boundary of the
+    step's end comes first.  The state's prognostic fields are left as
+    they were (their EOS recomputed); its other fields are overwritten."""
     from . import dycore as dyn
     _check_state(state)
     if state.qv is not None or _uses_diffusion(cfg) \
@@ -520,19 +571,26 @@ def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict) -> dict:
         raise _lib.ArwenError("step adjoint: dry states, no constant-K "
                               "diffusion, plain form")
     names = STAGE_STATE
+    nz, ny, nx = state.shape
     x_in = {n: getattr(state, n).clone() for n in names}
     for n in names:
         getattr(state, n + "0").copy_(getattr(state, n))
+    ww = state.scratch((nz + 1, ny, nx), "arw_ww")
+    dmdt = state.scratch((ny, nx), "arw_dmdt_rf")
     ck = []
     for nsub, dtau in _step_stages(cfg):
         xs = {n: getattr(state, n).clone() for n in names}
         dyn.stage_prepare(state, cfg, dtau, True)
-        subs = []
+        prep = {n: getattr(state, n).clone() for n in _PREP}
+        prep["ww"] = ww.clone()
+        prep["dmdt_rf"] = dmdt.clone()
+        subs = [{n: getattr(state, n).clone() for n in ACOUSTIC_STATE}]
         for i in range(nsub):
-            subs.append({n: getattr(state, n).clone() for n in ACOUSTIC_STATE})
             dyn.acoustic_substep(state, cfg, dtau, first=(i == 0))
-        after = {n: getattr(state, n).clone() for n in FINISH_PP}
-        ck.append((xs, subs, after, dtau))
+            x = {n: getattr(state, n).clone() for n in ACOUSTIC_STATE}
+            x["ww_pp"] = state.ww_pp.clone()
+            subs.append(x)
+        ck.append((xs, prep, subs, dtau))
         dyn.small_step_finish(state)
     lx = {}
     for n in names:
@@ -542,25 +600,27 @@ def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict) -> dict:
     set_w_surface_adjoint(state, cfg, lx["w"][0], lx["u"], lx["v"])
     lx["w"][0].zero_()
     g0 = {n: torch.zeros_like(getattr(state, n)) for n in names}
-    for xs, subs, after, dtau in reversed(ck):
+    for xs, prep, subs, dtau in reversed(ck):
         for n in names:
             getattr(state, n).copy_(xs[n])
-        dyn.stage_prepare(state, cfg, dtau, True)       # EOS and tendencies
-        for n in FINISH_PP:
-            getattr(state, n).copy_(after[n])
-        fin = small_step_finish_adjoint(state, lx)
+        nsub = len(subs) - 1
+        fin = small_step_finish_adjoint(
+            state, lx, views={n: subs[nsub][n] for n in FINISH_PP})
         gs = {n: fin[n] for n in names}
         lp = {n: fin[n] for n in FINISH_PP}
         lt = {n: torch.zeros_like(getattr(state, n)) for n in STAGE_TEND}
         acc = reference_accumulators(state)
-        for i in range(len(subs) - 1, -1, -1):
-            for n in ACOUSTIC_STATE:
-                getattr(state, n).copy_(subs[i][n])
-            out = acoustic_substep_adjoint(state, cfg, dtau, i == 0, lp, acc)
+        for i in range(nsub - 1, -1, -1):
+            views = dict(prep)
+            views.update(subs[i])
+            views.pop("ww_pp", None)
+            fwd = {n: subs[i + 1][n] for n in _lib.ACOUSTIC_FWD}
+            out = acoustic_substep_adjoint(state, cfg, dtau, i == 0, lp, acc,
+                                           fwd=fwd, views=views)
             lp = {n: out[n] for n in ACOUSTIC_STATE}
             for n in STAGE_TEND:
                 lt[n] += out[n]
-        ref = reference_gradients(state, acc)
+        ref = reference_gradients(state, cfg, acc, views=prep)
         gi = small_step_init_adjoint(state, lp)
         gt = stage_tendencies_adjoint(state, cfg, lt)
         for n in names:

@@ -39,6 +39,7 @@ enum {
     W_LBX, W_LSX, W_LBY, W_LSY,                              // faces -> cells
     W_LM1, W_LDX, W_LDY,                                     // lambda(mu''), ld
     W_RM, W_RPH, W_RGX, W_RGY, W_RMX, W_RMY, W_RAX, W_RAY,   // reference pieces
+    W_RHS, W_PRH, W_WS,
     W_COUNT
 };
 
@@ -48,8 +49,8 @@ struct Adj {
     real *gru, *grv, *grw, *grth, *grph, *grmu;
 };
 
-struct Fw {                 // forward recompute and scratch
-    const real *M1, *T1, *W1, *AL1;
+struct Fw {                 // the substep's outputs, and scratch
+    const real *M1, *T1, *W1, *AL1, *U1, *V1, *WW1;
     real* s[W_COUNT];
 };
 
@@ -574,7 +575,7 @@ __global__ void k_adj_cells(Dom d, Sc s, Adj a, Fw f)
 // terms as the mass-column kernels above, one thread per point it writes.
 
 struct RefAcc {
-    real *mut, *c2a, *alt, *thr, *ph, *w, *u, *v;
+    real *mut, *c2a, *alt, *thr, *ph, *w, *u, *v, *grf;
 };
 
 // the damper at w level k: inside the layer?, dw, its sine argument
@@ -588,6 +589,36 @@ __device__ __forceinline__ bool damp_at(const Dom& d, const Sc& s, int k,
     sn = (real)sin((double)arg);
     dw = s.dampmag * sn * sn;
     return s.damp && k >= 1 && (hk >= hbot);
+}
+
+// (column, w level): r_ (the phi'' rhs before its C(mu_t) division), the
+// predicted phi'' the w accumulation differences, and the implicit solve's
+// output before the damper -- each read by k_ref_lev at up to four levels
+__global__ __launch_bounds__(256) void k_ref_pre(Dom d, Sc s, Fw f)
+{
+    int c, k;
+    if (!col_level(d, c, k)) return;
+    const ColCtx C(d, s, c, f.M1[c]);
+    const int st = C.st, nz = d.nz, x = k * st + c;
+    const real half = R(0.5), one = R(1.0), dt = s.dtau, g = G_F, eps = s.epssm;
+    const real* WW1 = f.WW1;
+    auto wd = [&](int kk) {                              // kk in 0..nz-1
+        return half * (WW1[(kk + 1) * st + c] + WW1[kk * st + c]) * d.rdnw[kk]
+               * (phi_full(d, kk + 1, c) - phi_full(d, kk, c));
+    };
+    real r = R(0.0), rhs = R(0.0);
+    if (k >= 1) {
+        r = dt * (d.rph_t[x] + half * g * (one - eps) * d.w_pp[x]);
+        if (k < nz) r = r - dt * (d.fnm[k] * wd(k) + d.fnp[k] * wd(k - 1));
+        if (!(k == nz && s.top_lid)) rhs = d.ph_pp[x] + r / C.cfm(k);
+    }
+    f.s[W_RHS][x] = r;
+    f.s[W_PRH][x] = rhs;
+    real ws = f.W1[x];
+    real dw, arg, sn;
+    if (damp_at(d, s, k, c, dw, arg, sn))
+        ws = ws * (one + dw) + dw * (C.cfm(k) * d.w[x]);
+    f.s[W_WS][x] = ws;
 }
 
 // (column, level 0..nz): the EOS, phi'' update, damper, Thomas solve,
@@ -606,27 +637,14 @@ __global__ __launch_bounds__(256) void k_ref_lev(Dom d, Sc s, Adj a, Fw f,
     const real* LRHS = f.s[W_LRHS];
     const real* LT2 = f.s[W_LT2];
     const real* GGF = f.s[W_GGF];
-    const real* WW1 = f.s[W_WW];
+    const real* WW1 = f.WW1;
+    const real* RR = f.s[W_RHS];
+    const real* PRH = f.s[W_PRH];
+    const real* WS = f.s[W_WS];
     auto mu = [&](int kk) { return LW[kk * st + c]; };   // lambda, rows 1..nz
-    auto wsol = [&](int kk) -> real {                    // pre-damper solve
-        const real w1 = f.W1[kk * st + c];
-        real dw, arg, sn;
-        if (!damp_at(d, s, kk, c, dw, arg, sn)) return w1;
-        return w1 * (one + dw) + dw * (C.cfm(kk) * d.w[kk * st + c]);
-    };
-    auto wd = [&](int kk) {                              // kk in 0..nz-1
-        return half * (WW1[(kk + 1) * st + c] + WW1[kk * st + c]) * d.rdnw[kk]
-               * (phi_full(d, kk + 1, c) - phi_full(d, kk, c));
-    };
-    auto rr = [&](int kk) {                              // kk in 1..nz
-        real v = dt * (d.rph_t[kk * st + c]
-                       + half * g * (one - eps) * d.w_pp[kk * st + c]);
-        if (kk < nz) v = v - dt * (d.fnm[kk] * wd(kk) + d.fnp[kk] * wd(kk - 1));
-        return v;
-    };
+    auto wsol = [&](int kk) { return WS[kk * st + c]; }; // pre-damper solve
     auto rhs = [&](int kk) -> real {                     // predicted phi''
-        if (kk < 1 || kk > nz || (kk == nz && s.top_lid)) return R(0.0);
-        return d.ph_pp[kk * st + c] + rr(kk) / C.cfm(kk);
+        return (kk < 1 || kk > nz) ? R(0.0) : PRH[kk * st + c];
     };
     auto P0 = [&](int kk) { return d.ph_pp[kk * st + c]; };
     auto dup = [&](int rw) {
@@ -740,8 +758,8 @@ __global__ __launch_bounds__(256) void k_ref_lev(Dom d, Sc s, Adj a, Fw f,
         const int cip = j * nx + (i + 1 == nx ? 0 : i + 1);
         const int cjm = (j == 0 ? ny - 1 : j - 1) * nx + i;
         const int cjp = (j + 1 == ny ? 0 : j + 1) * nx + i;
-        const real* U1 = f.s[W_U];
-        const real* V1 = f.s[W_V];
+        const real* U1 = f.U1;
+        const real* V1 = f.V1;
         const real g0 = gg(k, c);
         gthr += half * s.rdx * (U1[iU(k, j, i + 1, ny, nx)] - U1[iU(k, j, i, ny, nx)]) * g0
                 + half * s.rdy * (V1[iV(k, j + 1, i, ny, nx)] - V1[iV(k, j, i, ny, nx)]) * g0;
@@ -782,7 +800,7 @@ __global__ __launch_bounds__(256) void k_ref_lev(Dom d, Sc s, Adj a, Fw f,
     }
     if (k >= 1) {
         const real lr = (k == nz && s.top_lid) ? R(0.0) : LRHS[x];
-        gcfm += -lr * rr(k) / (cfmk * cfmk);
+        gcfm += -lr * RR[x] / (cfmk * cfmk);
         real ph1 = a.lph[x];
         if (k < nz) ph1 += d.rdnw[k] / C.chmp(k) * C.lAL(a, k);
         ph1 += -d.rdnw[k - 1] / C.chmp(k - 1) * C.lAL(a, k - 1);
@@ -792,7 +810,7 @@ __global__ __launch_bounds__(256) void k_ref_lev(Dom d, Sc s, Adj a, Fw f,
         if (damp_at(d, s, k, c, dw, arg, sn)) {
             const real lk = a.lw[x] + C.K / cfs * ph1;
             const real wk = d.w[x], cw = cfmk * wk;
-            const real ws = f.W1[x] * (one + dw) + dw * cw;
+            const real ws = wsol(k);
             gw += -lk * dw * cfmk / (one + dw);
             gcfm += -lk * dw * wk / (one + dw);
             const real gdw = -lk * (cw + ws) / ((one + dw) * (one + dw));
@@ -814,73 +832,18 @@ __global__ __launch_bounds__(256) void k_ref_lev(Dom d, Sc s, Adj a, Fw f,
     f.s[W_RPH][x] = top;
 }
 
-// per column: the level pieces of the column-mass gradient, the damper's
-// top height, and dmdt_rf's mass (the stage u, v face column sums)
-__global__ __launch_bounds__(64) void k_ref_col(Dom d, Sc s, Fw f, RefAcc r)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
-    if (c >= st) return;
-    const int j = c / nx, i = c - j * nx;
-    const real half = R(0.5), dt = s.dtau;
-    real m = R(0.0), top = R(0.0);
-    for (int k = 0; k <= nz; ++k) {
-        m += f.s[W_RM][k * st + c];
-        top += f.s[W_RPH][k * st + c];
-    }
-    r.ph[nz * st + c] += top;
-    const real* LM1 = f.s[W_LM1];
-    auto su = [&](int ff) {
-        real v = R(0.0);
-        for (int k = 0; k < nz; ++k)
-            v += d.c1h[k] * d.u[iU(k, j, ff, ny, nx)] * d.dnw[k];
-        return v;
-    };
-    auto sv = [&](int gg) {
-        real v = R(0.0);
-        for (int k = 0; k < nz; ++k)
-            v += d.c1h[k] * d.v[iV(k, gg, i, ny, nx)] * d.dnw[k];
-        return v;
-    };
-    const int im = i == 0 ? nx - 1 : i - 1, ip = i + 1 == nx ? 0 : i + 1;
-    const int jm = j == 0 ? ny - 1 : j - 1, jp = j + 1 == ny ? 0 : j + 1;
-    const real gc = dt * LM1[c];
-    const real gmh_c = s.rdx * gc * su(i + 1);
-    const real gmh_m = s.rdx * (dt * LM1[j * nx + im]) * su(i == 0 ? nx : i);
-    const real gml_c = -s.rdx * gc * su(i);
-    const real gml_p = -s.rdx * (dt * LM1[j * nx + ip]) * su(ip);
-    m += half * (gmh_c + gmh_m) + half * (gml_c + gml_p);
-    const real gnh_c = s.rdy * gc * sv(j + 1);
-    const real gnh_m = s.rdy * (dt * LM1[jm * nx + i]) * sv(j == 0 ? ny : j);
-    const real gnl_c = -s.rdy * gc * sv(j);
-    const real gnl_p = -s.rdy * (dt * LM1[jp * nx + i]) * sv(jp);
-    m += half * (gnh_c + gnh_m) + half * (gnl_c + gnl_p);
-    r.mut[c] += m;
-}
-
-// (face, level): dmdt_rf's u / v gradients on every face, and for own faces
-// the advance_uv pieces: the column-mass weight (RM*), the alpha_t term
-// (RA*) and the phi' difference (RG*)
-__global__ __launch_bounds__(256) void k_ref_face(Dom d, Sc s, Fw f, RefAcc r)
+// (own face, level): the advance_uv pieces -- the column-mass weight
+// (RM*), the alpha_t term (RA*) and the phi' difference (RG*)
+__global__ __launch_bounds__(256) void k_ref_face(Dom d, Sc s, Fw f)
 {
     FaceGeo fg, dg;
     bool has_dup, isu;
     const int t = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
-    if (!face_geo(d, t, fg, dg, has_dup, isu)) return;
+    if (!face_geo(d, t, fg, dg, has_dup, isu) || !fg.own) return;
     const int st = d.ny * d.nx, nz = d.nz;
-    const real half = R(0.5), dt = s.dtau;
+    const real half = R(0.5);
     const real rd = isu ? s.rdx : s.rdy;
-    const real* LM1 = f.s[W_LM1];
-    const real c1 = d.c1h[k], c2 = d.c2h[k];
-    real v = R(0.0);
-    if (fg.hasE)
-        v += rd * (c1 * (half * (mut_of(d, fg.cE) + mut_of(d, fg.cEp))) + c2)
-             * (d.dnw[k] * (dt * LM1[fg.cE]));
-    if (fg.hasW)
-        v -= rd * (c1 * (half * (mut_of(d, fg.cWm) + mut_of(d, fg.cW))) + c2)
-             * (d.dnw[k] * (dt * LM1[fg.cW]));
-    (isu ? r.u : r.v)[fg.out0 + k * fg.ostride] += v;
-    if (!fg.own) return;
+    const real c1 = d.c1h[k];
     const int A = fg.cA, B = fg.cB, xs = k * st + fg.slot;
     const real ld = f.s[isu ? W_LDX : W_LDY][xs];
     auto pd = [&](int kk, int cc) {
@@ -909,33 +872,51 @@ __global__ __launch_bounds__(256) void k_ref_face(Dom d, Sc s, Fw f, RefAcc r)
         * ld;
 }
 
-// per column: gather the own-face pieces onto the cells
-__global__ void k_ref_cells(Dom d, Fw f, RefAcc r)
+// (column, w level): gather the own-face pieces onto the cells; the
+// column-mass pieces join RM
+__global__ __launch_bounds__(256) void k_ref_cells(Dom d, Fw f, RefAcc r)
 {
-    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= st) return;
+    int c, k;
+    if (!col_level(d, c, k)) return;
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx, x = k * st + c;
     const int j = c / nx, i = c - j * nx;
     const int ip = j * nx + (i + 1 == nx ? 0 : i + 1);
     const int jp = (j + 1 == ny ? 0 : j + 1) * nx + i;
     const real half = R(0.5);
-    const real *RAX = f.s[W_RAX], *RAY = f.s[W_RAY];
     const real *RGX = f.s[W_RGX], *RGY = f.s[W_RGY];
-    const real *RMX = f.s[W_RMX], *RMY = f.s[W_RMY];
-    real mx = R(0.0), mxp = R(0.0), my = R(0.0), myp = R(0.0);
-    for (int k = 0; k < nz; ++k) {
-        const int x = k * st;
-        r.alt[x + c] += (RAX[x + c] + RAX[x + ip]) + (RAY[x + c] + RAY[x + jp]);
-        const real gx = half * (RGX[x + c] - RGX[x + ip]);
-        const real gy = half * (RGY[x + c] - RGY[x + jp]);
-        r.ph[x + c] += gx + gy;
-        r.ph[x + st + c] += gx + gy;
-        mx += RMX[x + c];
-        mxp += RMX[x + ip];
-        my += RMY[x + c];
-        myp += RMY[x + jp];
+    auto gphs = [&](int kk) {
+        const int y = kk * st;
+        return half * (RGX[y + c] - RGX[y + ip]) + half * (RGY[y + c] - RGY[y + jp]);
+    };
+    real gph = R(0.0);
+    if (k < nz) {
+        const real *RAX = f.s[W_RAX], *RAY = f.s[W_RAY];
+        const real *RMX = f.s[W_RMX], *RMY = f.s[W_RMY];
+        r.alt[x] += (RAX[x] + RAX[k * st + ip]) + (RAY[x] + RAY[k * st + jp]);
+        f.s[W_RM][x] += half * (RMX[x] + RMX[k * st + ip])
+                        + half * (RMY[x] + RMY[k * st + jp]);
+        gph += gphs(k);
     }
-    r.mut[c] += half * (mx + mxp) + half * (my + myp);
+    if (k >= 1) gph += gphs(k - 1);
+    r.ph[x] += gph;
+}
+
+// per column: the level pieces of the column-mass gradient and the
+// damper's top height; this substep's dmdt_rf adjoint joins grf (the stage
+// reference u, v and mass it weights are mapped once per stage)
+__global__ __launch_bounds__(256) void k_ref_col(Dom d, Sc s, Fw f, RefAcc r)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nz = d.nz, st = d.ny * d.nx;
+    if (c >= st) return;
+    real m = R(0.0), top = R(0.0);
+    for (int k = 0; k <= nz; ++k) {
+        m += f.s[W_RM][k * st + c];
+        top += f.s[W_RPH][k * st + c];
+    }
+    r.ph[nz * st + c] += top;
+    r.mut[c] += m;
+    r.grf[c] += s.dtau * f.s[W_LM1][c];
 }
 
 // ---- c2a and alpha_t through the stage EOS (dry, hypsometric_opt = 1) ----
@@ -972,15 +953,60 @@ __global__ void k_refg_php(Dom d, RefAcc r, real* gphp, const real* w)
     gphp[t] = g;
 }
 
-__global__ void k_refg_mu(Dom d, RefAcc r, real* gmup, const real* w)
+// the stage's dmdt_rf = sum_k dnw (divergence of the coupled u, v): its
+// share for the substep-summed grf, on the faces (u, v) ...
+__global__ __launch_bounds__(256) void k_refg_dmdt(Dom d, Sc s, RefAcc r)
 {
-    const int nz = d.nz, st = d.ny * d.nx;
+    FaceGeo fg, dg;
+    bool has_dup, isu;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (!face_geo(d, t, fg, dg, has_dup, isu)) return;
+    const real half = R(0.5);
+    const real rd = isu ? s.rdx : s.rdy;
+    const real c1 = d.c1h[k], c2 = d.c2h[k];
+    real v = R(0.0);
+    if (fg.hasE)
+        v += rd * (c1 * (half * (mut_of(d, fg.cE) + mut_of(d, fg.cEp))) + c2)
+             * (d.dnw[k] * r.grf[fg.cE]);
+    if (fg.hasW)
+        v -= rd * (c1 * (half * (mut_of(d, fg.cWm) + mut_of(d, fg.cW))) + c2)
+             * (d.dnw[k] * r.grf[fg.cW]);
+    (isu ? r.u : r.v)[fg.out0 + k * fg.ostride] += v;
+}
+
+// ... and in the column mass (with the EOS's mu' column sum)
+__global__ void k_refg_mu(Dom d, Sc sc, RefAcc r, real* gmup, const real* w)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= st) return;
-    const size_t F = (size_t)(nz + 1) * (d.ny + 1) * (d.nx + 1);
+    const size_t F = (size_t)(nz + 1) * (ny + 1) * (nx + 1);
     real s = R(0.0);
     for (int k = 0; k < nz; ++k) s += w[G_MU3 * F + (size_t)k * st + c];
-    gmup[c] = r.mut[c] + s;
+    const int j = c / nx, i = c - j * nx;
+    auto su = [&](int ff) {
+        real v = R(0.0);
+        for (int k = 0; k < nz; ++k)
+            v += d.c1h[k] * d.u[iU(k, j, ff, ny, nx)] * d.dnw[k];
+        return v;
+    };
+    auto sv = [&](int gg) {
+        real v = R(0.0);
+        for (int k = 0; k < nz; ++k)
+            v += d.c1h[k] * d.v[iV(k, gg, i, ny, nx)] * d.dnw[k];
+        return v;
+    };
+    const real half = R(0.5);
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i + 1 == nx ? 0 : i + 1;
+    const int jm = j == 0 ? ny - 1 : j - 1, jp = j + 1 == ny ? 0 : j + 1;
+    const real G = r.grf[c];
+    real m = half * (sc.rdx * G * su(i + 1)
+                     + sc.rdx * r.grf[j * nx + im] * su(i == 0 ? nx : i))
+             + half * (-sc.rdx * G * su(i) - sc.rdx * r.grf[j * nx + ip] * su(ip));
+    m += half * (sc.rdy * G * sv(j + 1)
+                 + sc.rdy * r.grf[jm * nx + i] * sv(j == 0 ? ny : j))
+         + half * (-sc.rdy * G * sv(j) - sc.rdy * r.grf[jp * nx + i] * sv(jp));
+    gmup[c] = (r.mut[c] + m) + s;
 }
 
 // ---- the kinematic lower boundary w(sfc) = u . grad(ht) ----------------
@@ -1028,10 +1054,45 @@ __global__ void k_wsfc_adj(Dom d, real hdx, real hdy, const real* l0,
 
 using namespace ARW_NS;
 
+// The forward substep on copies of X in the work area (arw_acoustic_
+// substep's sequence): the outputs the transpose is taken about.
+static int32_t recompute(const Dom* d, const Sc& s, Fw& fw, hipStream_t st)
+{
+    int32_t rc;
+    const int nz = d->nz, ny = d->ny, nx = d->nx;
+    Dom dc = *d;
+    real* src[] = {d->u_pp, d->v_pp, d->w_pp, d->th_pp, d->ph_pp, d->mu_pp,
+                   d->p_pp, d->p_pp_old, d->al_pp};
+    const size_t n3 = (size_t)nz * ny * nx, nw = (size_t)(nz + 1) * ny * nx;
+    const size_t sizes[] = {(size_t)nz * ny * (nx + 1), (size_t)nz * (ny + 1) * nx,
+                            nw, n3, nw, (size_t)ny * nx, n3, n3, n3};
+    for (int q = 0; q < 9; ++q)
+        if (hipMemcpyAsync(fw.s[q], src[q], sizes[q] * RB,
+                           hipMemcpyDeviceToDevice, st) != hipSuccess)
+            return lib_fail(ARW_E_LAUNCH, "adjoint: state copy failed");
+    dc.u_pp = fw.s[W_U]; dc.v_pp = fw.s[W_V]; dc.w_pp = fw.s[W_W];
+    dc.th_pp = fw.s[W_TH]; dc.ph_pp = fw.s[W_PH]; dc.mu_pp = fw.s[W_MU];
+    dc.p_pp = fw.s[W_P]; dc.p_pp_old = fw.s[W_PO]; dc.al_pp = fw.s[W_AL];
+    dc.ww_pp = fw.s[W_WW];
+    if ((rc = lib_launched(run_uv(dc, s, st), "k_uv (adjoint recompute)")))
+        return rc;
+    if (hipMemcpyAsync(dc.p_pp_old, dc.p_pp, n3 * RB, hipMemcpyDeviceToDevice,
+                       st) != hipSuccess)
+        return lib_fail(ARW_E_LAUNCH, "adjoint: p_pp_old copy failed");
+    if ((rc = lib_launched(run_column(dc, s, dc.p_pp, 1, 1, st),
+                           "k_column (adjoint recompute)")))
+        return rc;
+    fw.M1 = dc.mu_pp; fw.T1 = dc.th_pp; fw.W1 = dc.w_pp; fw.AL1 = dc.al_pp;
+    fw.U1 = dc.u_pp; fw.V1 = dc.v_pp; fw.WW1 = dc.ww_pp;
+    return ARW_OK;
+}
+
 static int32_t substep_adjoint(const Dom* d, const arw_params* prm,
                                double dtau, int32_t first,
                                const ARW_API(acoustic_adj)* adj,
-                               const RefAcc* ref, real* work, void* stream)
+                               const RefAcc* ref,
+                               const ARW_API(acoustic_fwd)* fwd, real* work,
+                               void* stream)
 {
     int32_t rc = lib_check_domain(d);
     if (rc) return rc;
@@ -1053,32 +1114,15 @@ static int32_t substep_adjoint(const Dom* d, const arw_params* prm,
     const size_t F = (size_t)(nz + 1) * (ny + 1) * (nx + 1);
     Fw fw;
     for (int w = 0; w < W_COUNT; ++w) fw.s[w] = work + (size_t)w * F;
-    // forward recompute on copies of X (arw_acoustic_substep's sequence)
-    Dom dc = *d;
-    real* src[] = {d->u_pp, d->v_pp, d->w_pp, d->th_pp, d->ph_pp, d->mu_pp,
-                   d->p_pp, d->p_pp_old, d->al_pp};
-    const size_t n3 = (size_t)nz * ny * nx, nw = (size_t)(nz + 1) * ny * nx;
-    const size_t sizes[] = {(size_t)nz * ny * (nx + 1), (size_t)nz * (ny + 1) * nx,
-                            nw, n3, nw, (size_t)ny * nx, n3, n3, n3};
-    for (int q = 0; q < 9; ++q)
-        if (hipMemcpyAsync(fw.s[q], src[q], sizes[q] * RB,
-                           hipMemcpyDeviceToDevice, st) != hipSuccess)
-            return lib_fail(ARW_E_LAUNCH, "adjoint: state copy failed");
-    dc.u_pp = fw.s[W_U]; dc.v_pp = fw.s[W_V]; dc.w_pp = fw.s[W_W];
-    dc.th_pp = fw.s[W_TH]; dc.ph_pp = fw.s[W_PH]; dc.mu_pp = fw.s[W_MU];
-    dc.p_pp = fw.s[W_P]; dc.p_pp_old = fw.s[W_PO]; dc.al_pp = fw.s[W_AL];
-    dc.ww_pp = fw.s[W_WW];
     Sc s = make_sc(prm, dtau, first);
     s.use_pold = 1;
-    if ((rc = lib_launched(run_uv(dc, s, st), "k_uv (adjoint recompute)")))
+    if (fwd) {            // the substep's outputs, kept by the caller
+        fw.U1 = fwd->u_pp; fw.V1 = fwd->v_pp; fw.W1 = fwd->w_pp;
+        fw.T1 = fwd->th_pp; fw.M1 = fwd->mu_pp; fw.AL1 = fwd->al_pp;
+        fw.WW1 = fwd->ww_pp;
+    } else if ((rc = recompute(d, s, fw, st))) {
         return rc;
-    if (hipMemcpyAsync(dc.p_pp_old, dc.p_pp, n3 * RB, hipMemcpyDeviceToDevice,
-                       st) != hipSuccess)
-        return lib_fail(ARW_E_LAUNCH, "adjoint: p_pp_old copy failed");
-    if ((rc = lib_launched(run_column(dc, s, dc.p_pp, 1, 1, st),
-                           "k_column (adjoint recompute)")))
-        return rc;
-    fw.M1 = dc.mu_pp; fw.T1 = dc.th_pp; fw.W1 = dc.w_pp; fw.AL1 = dc.al_pp;
+    }
     Adj a;
     a.lu = adj->l_u_pp; a.lv = adj->l_v_pp; a.lw = adj->l_w_pp;
     a.lth = adj->l_th_pp; a.lph = adj->l_ph_pp; a.lmu = adj->l_mu_pp;
@@ -1101,21 +1145,19 @@ static int32_t substep_adjoint(const Dom* d, const arw_params* prm,
     hipLaunchKernelGGL(k_adj_p3, lev, dim3(256), 0, st, *d, s, a, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_p3"))) return rc;
     if (ref) {       // reads the solve's lambda before the face pass reuses it
+        hipLaunchKernelGGL(k_ref_pre, lev, dim3(256), 0, st, *d, s, fw);
+        if ((rc = lib_launched(hipGetLastError(), "k_ref_pre"))) return rc;
         hipLaunchKernelGGL(k_ref_lev, lev, dim3(256), 0, st, *d, s, a, fw, *ref);
         if ((rc = lib_launched(hipGetLastError(), "k_ref_lev"))) return rc;
     }
     hipLaunchKernelGGL(k_adj_s2, colg, dim3(64), 0, st, *d, s, a, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_s2"))) return rc;
-    if (ref) {
-        hipLaunchKernelGGL(k_ref_col, colg, dim3(64), 0, st, *d, s, fw, *ref);
-        if ((rc = lib_launched(hipGetLastError(), "k_ref_col"))) return rc;
-    }
     const int nf = ny * (nx + 1) + (ny + 1) * nx;
     const dim3 flev((nf + 255) / 256, nz);
     hipLaunchKernelGGL(k_adj_f1, flev, dim3(256), 0, st, *d, s, a, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_f1"))) return rc;
     if (ref) {
-        hipLaunchKernelGGL(k_ref_face, flev, dim3(256), 0, st, *d, s, fw, *ref);
+        hipLaunchKernelGGL(k_ref_face, flev, dim3(256), 0, st, *d, s, fw);
         if ((rc = lib_launched(hipGetLastError(), "k_ref_face"))) return rc;
     }
     hipLaunchKernelGGL(k_adj_f2, flev, dim3(256), 0, st, *d, s, fw);
@@ -1124,22 +1166,24 @@ static int32_t substep_adjoint(const Dom* d, const arw_params* prm,
                        *d, s, a, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_cells"))) return rc;
     if (!ref) return ARW_OK;
-    hipLaunchKernelGGL(k_ref_cells, dim3((cols + 127) / 128), dim3(128), 0, st,
-                       *d, fw, *ref);
-    return lib_launched(hipGetLastError(), "k_ref_cells");
+    hipLaunchKernelGGL(k_ref_cells, lev, dim3(256), 0, st, *d, fw, *ref);
+    if ((rc = lib_launched(hipGetLastError(), "k_ref_cells"))) return rc;
+    hipLaunchKernelGGL(k_ref_col, dim3((cols + 255) / 256), dim3(256), 0, st,
+                       *d, s, fw, *ref);
+    return lib_launched(hipGetLastError(), "k_ref_col");
 }
 
 static bool ref_ok(const ARW_API(acoustic_ref)* acc)
 {
     return acc && acc->mut && acc->c2a && acc->alt && acc->thr && acc->ph
-           && acc->w && acc->u && acc->v;
+           && acc->w && acc->u && acc->v && acc->grf;
 }
 
 static RefAcc ref_of(const ARW_API(acoustic_ref)* acc)
 {
     RefAcc r;
     r.mut = acc->mut; r.c2a = acc->c2a; r.alt = acc->alt; r.thr = acc->thr;
-    r.ph = acc->ph; r.w = acc->w; r.u = acc->u; r.v = acc->v;
+    r.ph = acc->ph; r.w = acc->w; r.u = acc->u; r.v = acc->v; r.grf = acc->grf;
     return r;
 }
 
@@ -1150,28 +1194,32 @@ int32_t ARW_API(acoustic_substep_adjoint)(const Dom* d, const arw_params* prm,
                                           const ARW_API(acoustic_adj)* adj,
                                           real* work, void* stream)
 {
-    return substep_adjoint(d, prm, dtau, first, adj, nullptr, work, stream);
+    return substep_adjoint(d, prm, dtau, first, adj, nullptr, nullptr, work,
+                           stream);
 }
 
 int32_t ARW_API(acoustic_substep_adjoint_ref)(
     const Dom* d, const arw_params* prm, double dtau, int32_t first,
     const ARW_API(acoustic_adj)* adj, const ARW_API(acoustic_ref)* acc,
-    real* work, void* stream)
+    const ARW_API(acoustic_fwd)* fwd, real* work, void* stream)
 {
     if (!ref_ok(acc)) return lib_fail(ARW_E_ARG, "null reference accumulator");
+    if (fwd && !(fwd->u_pp && fwd->v_pp && fwd->w_pp && fwd->th_pp
+                 && fwd->mu_pp && fwd->al_pp && fwd->ww_pp))
+        return lib_fail(ARW_E_ARG, "null substep output");
     if (d && d->qv)
         return lib_fail(ARW_E_UNSUPPORTED, "reference adjoint: dry states only");
     const RefAcc r = ref_of(acc);
-    return substep_adjoint(d, prm, dtau, first, adj, &r, work, stream);
+    return substep_adjoint(d, prm, dtau, first, adj, &r, fwd, work, stream);
 }
 
 int32_t ARW_API(acoustic_reference_gradients)(
-    const Dom* d, const ARW_API(acoustic_ref)* acc, real* g_thp, real* g_php,
-    real* g_mup, real* work, void* stream)
+    const Dom* d, const arw_params* prm, const ARW_API(acoustic_ref)* acc,
+    real* g_thp, real* g_php, real* g_mup, real* work, void* stream)
 {
     int32_t rc = lib_check_domain(d);
     if (rc) return rc;
-    if (!ref_ok(acc) || !g_thp || !g_php || !g_mup || !work)
+    if (!prm || !ref_ok(acc) || !g_thp || !g_php || !g_mup || !work)
         return lib_fail(ARW_E_ARG, "null argument");
     if (d->qv) return lib_fail(ARW_E_UNSUPPORTED, "reference adjoint: dry states only");
     const RefAcc r = ref_of(acc);
@@ -1184,8 +1232,13 @@ int32_t ARW_API(acoustic_reference_gradients)(
     hipLaunchKernelGGL(k_refg_php, dim3((nw + 255) / 256), dim3(256), 0, st,
                        *d, r, g_php, work);
     if ((rc = lib_launched(hipGetLastError(), "k_refg_php"))) return rc;
+    const Sc sc = make_sc(prm, prm->dt, 1);
+    const int nf = d->ny * (d->nx + 1) + (d->ny + 1) * d->nx;
+    hipLaunchKernelGGL(k_refg_dmdt, dim3((nf + 255) / 256, nz), dim3(256), 0,
+                       st, *d, sc, r);
+    if ((rc = lib_launched(hipGetLastError(), "k_refg_dmdt"))) return rc;
     hipLaunchKernelGGL(k_refg_mu, dim3((cols + 255) / 256), dim3(256), 0, st,
-                       *d, r, g_mup, work);
+                       *d, sc, r, g_mup, work);
     return lib_launched(hipGetLastError(), "k_refg_mu");
 }
 

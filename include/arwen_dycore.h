@@ -250,27 +250,36 @@ int32_t arw_acoustic_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
  * substep through c2a = gamma p / alpha_t and alpha_t of the stage EOS,
  * the column mass mu_t, theta, the full phi, w (the damper) and u, v
  * (dmdt_rf): the call ADDS J^T lam with respect to those to the
- * accumulators of acc (mut: (ny,nx); c2a, alt, thr: mass points; ph, w:
- * w levels; u, v: faces), which the caller clears at the stage start and
- * keeps over the stage's substeps.  Otherwise as
- * arw_acoustic_substep_adjoint (same work size).  Dry states only.
- * arw_acoustic_reference_gradients maps the accumulated c2a / alt through
- * the stage EOS (d->p, d->alt must hold it): g_thp, g_php, g_mup are
- * written, the u, v, w gradients are acc->u, v, w themselves; work: 2
- * fields of (nz+1)(ny+1)(nx+1), e.g. the substep adjoint's work.
+ * accumulators of acc (mut, grf: (ny,nx); c2a, alt, thr: mass points; ph,
+ * w: w levels; u, v: faces), which the caller clears at the stage start
+ * and keeps over the stage's substeps.  fwd (optional): the substep's
+ * outputs from d's small-step fields (u''..al'' and Omega'' after
+ * arw_acoustic_substep), which the call then does not recompute.
+ * Otherwise as arw_acoustic_substep_adjoint (same work size).  Dry states
+ * only.  arw_acoustic_reference_gradients first adds dmdt_rf's share (grf
+ * times the stage u, v and face masses) to acc->u, v and then maps the
+ * accumulated c2a / alt through the stage EOS (d->p, d->alt must hold
+ * it): g_thp, g_php, g_mup are written, the u, v, w gradients are acc->u,
+ * v, w themselves; work: 2 fields of (nz+1)(ny+1)(nx+1), e.g. the substep
+ * adjoint's work.
  * arw_set_w_surface_adjoint ADDS the (u, v) gradients of the kinematic
  * lower boundary w(sfc) = u . grad(ht) (replaces nothing: the transpose of
  * dycore.py:1999-2026) for the adjoint lam_w0 (ny,nx) of w(sfc). */
 typedef struct arw_acoustic_ref {
-    float *mut, *c2a, *alt, *thr, *ph, *w, *u, *v;
+    float *mut, *c2a, *alt, *thr, *ph, *w, *u, *v, *grf;
 } arw_acoustic_ref;
+typedef struct arw_acoustic_fwd {
+    const float *u_pp, *v_pp, *w_pp, *th_pp, *mu_pp, *al_pp, *ww_pp;
+} arw_acoustic_fwd;
 int32_t arw_acoustic_substep_adjoint_ref(const arw_domain *d,
                                          const arw_params *prm, double dtau,
                                          int32_t first,
                                          const arw_acoustic_adj *adj,
                                          const arw_acoustic_ref *acc,
+                                         const arw_acoustic_fwd *fwd,
                                          float *work, void *stream);
 int32_t arw_acoustic_reference_gradients(const arw_domain *d,
+                                         const arw_params *prm,
                                          const arw_acoustic_ref *acc,
                                          float *g_thp, float *g_php, float *g_mup,
                                          float *work, void *stream);
@@ -564,15 +573,20 @@ int32_t arw64_acoustic_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
                                     int64_t *elements);
 
 typedef struct arw64_acoustic_ref {
-    double *mut, *c2a, *alt, *thr, *ph, *w, *u, *v;
+    double *mut, *c2a, *alt, *thr, *ph, *w, *u, *v, *grf;
 } arw64_acoustic_ref;
+typedef struct arw64_acoustic_fwd {
+    const double *u_pp, *v_pp, *w_pp, *th_pp, *mu_pp, *al_pp, *ww_pp;
+} arw64_acoustic_fwd;
 int32_t arw64_acoustic_substep_adjoint_ref(const arw64_domain *d,
                                          const arw_params *prm, double dtau,
                                          int32_t first,
                                          const arw64_acoustic_adj *adj,
                                          const arw64_acoustic_ref *acc,
+                                         const arw64_acoustic_fwd *fwd,
                                          double *work, void *stream);
 int32_t arw64_acoustic_reference_gradients(const arw64_domain *d,
+                                         const arw_params *prm,
                                          const arw64_acoustic_ref *acc,
                                          double *g_thp, double *g_php, double *g_mup,
                                          double *work, void *stream);

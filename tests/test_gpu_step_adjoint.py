@@ -47,7 +47,7 @@ def test_substep_reference_share_matches_oracle(hill, damp, top_lid, first):
     plain = adjoint.acoustic_substep_adjoint(g, cfg, dtau, first, tl)
     for n in A.SUBSTEP_STATE + A.SUBSTEP_TEND:
         assert torch.equal(got[n], plain[n]), n
-    rg = adjoint.reference_gradients(g, acc)
+    rg = adjoint.reference_gradients(g, cfg, acc)
     bad = {n: _rel(rg[n], ref["ref"][n]) for n in A.STAGE_IN}
     assert max(bad.values()) < ORACLE_TOL, bad
 
@@ -71,6 +71,32 @@ def test_reference_accumulators_add_over_substeps():
         s = single[0][n] + single[1][n]
         scale = float(s.abs().max()) or 1.0
         assert float((both[n] - s).abs().max()) <= 1e-13 * scale, n
+
+
+def test_given_substep_outputs_equal_recompute():
+    """The reference-share adjoint given the forward substep's outputs
+    (as the whole-step driver passes its checkpoints) is bit-identical to
+    the one that recomputes them, with the inputs read through views."""
+    from arwen_amd import _lib, adjoint, dycore
+    o, coefs, dtau, cfg, rng, g = _device_pair(hill=300.0, damp=True)
+    print("substep adjoint with given outputs", flush=True)
+    dev = g.u.device
+    x0 = {n: getattr(g, n).clone() for n in adjoint.ACOUSTIC_STATE}
+    dycore.acoustic_substep(g, cfg, dtau, False)
+    fwd = {n: getattr(g, n).clone() for n in _lib.ACOUSTIC_FWD}
+    lam = {n: torch.as_tensor(rng.standard_normal(getattr(o, n).shape),
+                              device=dev) for n in NAMES_OUT}
+    a1 = adjoint.reference_accumulators(g)
+    got = adjoint.acoustic_substep_adjoint(g, cfg, dtau, False, lam, a1,
+                                           fwd=fwd, views=x0)
+    for n, t in x0.items():
+        getattr(g, n).copy_(t)
+    a2 = adjoint.reference_accumulators(g)
+    ref = adjoint.acoustic_substep_adjoint(g, cfg, dtau, False, lam, a2)
+    for n in adjoint.ACOUSTIC_STATE + adjoint.ACOUSTIC_TEND:
+        assert torch.equal(got[n], ref[n]), n
+    for n in adjoint.ACOUSTIC_REF:
+        assert torch.equal(a1[n], a2[n]), n
 
 
 def test_set_w_surface_adjoint_matches_oracle():
