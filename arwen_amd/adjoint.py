@@ -547,39 +547,41 @@ def _step_stages(cfg: RunConfig):
 _PREP = ("p", "al", "alt", "ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t")
 
 
-def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict) -> dict:
-    """J^T lam of one dycore step (``dycore.step``; reference dycore.py:
-    2161-2503) at the state's prognostic fields: ``lam`` maps the new
-    state's STAGE_STATE fields to adjoints (missing = 0); returns the
-    gradients of the step's input fields (STAGE_STATE names).  Dry states
-    without constant-K diffusion, plain form.
+class StepTape:
+    """What the reverse sweep of one step needs, recorded by ``step_taped``:
+    per RK stage the stage state, what its preparation leaves (EOS, slow
+    tendencies, Omega, dmdt_rf) and every substep's small-step fields and
+    Omega''; and the step's output fields."""
 
-    The forward is re-run stage by stage on the device with checkpoints
-    (each stage's state and what its preparation leaves -- EOS, slow
-    tendencies, Omega, dmdt_rf -- and every substep's small-step fields and
-    Omega''), then reversed: per stage the finish adjoint, the substep
-    adjoints in reverse order (given each substep's outputs, adding the
-    stage-reference share), the small-step initialisation adjoint (to the
-    time-t and stage fields) and the tendency-pass adjoint; the stages
-    chain through their states and the kinematic lower boundary of the
-    step's end comes first.  The state's prognostic fields are left as
-    they were (their EOS recomputed); its other fields are overwritten."""
-    from . import dycore as dyn
-    _check_state(state)
+    def __init__(self, cfg: RunConfig):
+        self.cfg = cfg
+        self.stages = []
+        self.out = None
+
+
+def _check_adjoint_cfg(state, cfg):
     if state.qv is not None or _uses_diffusion(cfg) \
             or getattr(cfg, "precision_form", False):
         raise _lib.ArwenError("step adjoint: dry states, no constant-K "
                               "diffusion, plain form")
-    names = STAGE_STATE
+
+
+def step_taped(state: DomainState, cfg: RunConfig) -> StepTape:
+    """One dycore step of the state (as ``dycore.step``; reference
+    dycore.py:2161-2503), run stage by stage through the piecewise entry
+    points so that it records the tape its adjoint needs.  Dry states
+    without constant-K diffusion, plain form."""
+    from . import dycore as dyn
+    _check_state(state)
+    _check_adjoint_cfg(state, cfg)
     nz, ny, nx = state.shape
-    x_in = {n: getattr(state, n).clone() for n in names}
-    for n in names:
+    tape = StepTape(cfg)
+    for n in STAGE_STATE:
         getattr(state, n + "0").copy_(getattr(state, n))
     ww = state.scratch((nz + 1, ny, nx), "arw_ww")
     dmdt = state.scratch((ny, nx), "arw_dmdt_rf")
-    ck = []
     for nsub, dtau in _step_stages(cfg):
-        xs = {n: getattr(state, n).clone() for n in names}
+        xs = {n: getattr(state, n).clone() for n in STAGE_STATE}
         dyn.stage_prepare(state, cfg, dtau, True)
         prep = {n: getattr(state, n).clone() for n in _PREP}
         prep["ww"] = ww.clone()
@@ -590,8 +592,31 @@ def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict) -> dict:
             x = {n: getattr(state, n).clone() for n in ACOUSTIC_STATE}
             x["ww_pp"] = state.ww_pp.clone()
             subs.append(x)
-        ck.append((xs, prep, subs, dtau))
+        tape.stages.append((xs, prep, subs, dtau))
         dyn.small_step_finish(state)
+    dyn.set_w_surface(state, cfg)
+    dyn.update_diagnostics(state)
+    state.elapsed_seconds += cfg.dt
+    tape.out = {n: getattr(state, n).clone() for n in STAGE_STATE}
+    return tape
+
+
+def step_adjoint_taped(state: DomainState, tape: StepTape,
+                       lam: dict) -> dict:
+    """The reverse sweep of one step: J^T lam from the tape ``step_taped``
+    recorded (``lam`` maps the step's output STAGE_STATE fields to
+    adjoints, missing = 0; returns the gradients of its input fields).
+    Per stage, last first: the finish adjoint, the substep adjoints in
+    reverse order (given each substep's outputs, adding the stage-reference
+    share), the small-step initialisation adjoint (to the time-t and stage
+    fields) and the tendency-pass adjoint; the stages chain through their
+    states and the kinematic lower boundary of the step's end comes first.
+    The state is the workspace: its prognostic fields end as the step's
+    output, its other fields are overwritten."""
+    _check_state(state)
+    cfg = tape.cfg
+    _check_adjoint_cfg(state, cfg)
+    names = STAGE_STATE
     lx = {}
     for n in names:
         t = lam.get(n)
@@ -599,8 +624,11 @@ def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict) -> dict:
                  else _as_like(t, getattr(state, n), n).clone())
     set_w_surface_adjoint(state, cfg, lx["w"][0], lx["u"], lx["v"])
     lx["w"][0].zero_()
+    x0 = tape.stages[0][0]                     # the step's input
+    for n in names:
+        getattr(state, n + "0").copy_(x0[n])
     g0 = {n: torch.zeros_like(getattr(state, n)) for n in names}
-    for xs, prep, subs, dtau in reversed(ck):
+    for xs, prep, subs, dtau in reversed(tape.stages):
         for n in names:
             getattr(state, n).copy_(xs[n])
         nsub = len(subs) - 1
@@ -625,39 +653,56 @@ def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict) -> dict:
         gt = stage_tendencies_adjoint(state, cfg, lt)
         for n in names:
             g0[n] += gi[n + "0"]
-            gs[n] = gs[n] + ref[n] + gi[n] + gt[n]
+            gs[n].add_(ref[n]).add_(gi[n]).add_(gt[n])
         lx = gs
     for n in names:
         g0[n] += lx[n]
-        getattr(state, n).copy_(x_in[n])
-    dyn.update_diagnostics(state)
+        getattr(state, n).copy_(tape.out[n])
     return g0
 
 
+def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict) -> dict:
+    """J^T lam of one dycore step (``dycore.step``; reference dycore.py:
+    2161-2503) at the state's prognostic fields: ``lam`` maps the new
+    state's STAGE_STATE fields to adjoints (missing = 0); returns the
+    gradients of the step's input fields (STAGE_STATE names).  Dry states
+    without constant-K diffusion, plain form.  ``step_taped`` then
+    ``step_adjoint_taped``; the state's prognostic fields are left as they
+    were (their EOS recomputed), its other fields are overwritten."""
+    from . import dycore as dyn
+    _check_state(state)
+    _check_adjoint_cfg(state, cfg)
+    x_in = {n: getattr(state, n).clone() for n in STAGE_STATE}
+    t0 = state.elapsed_seconds
+    tape = step_taped(state, cfg)
+    g = step_adjoint_taped(state, tape, lam)
+    for n in STAGE_STATE:
+        getattr(state, n).copy_(x_in[n])
+    state.elapsed_seconds = t0
+    dyn.update_diagnostics(state)
+    return g
+
+
 class DycoreStep(torch.autograd.Function):
-    """(u, v, w, thp, php, mu') -> the same fields one dycore step later
-    (the device step, ``dycore.step``), with ``step_adjoint`` as its
-    backward.  ``apply(state, cfg, *fields)`` writes the fields into the
-    state and steps it there; the backward re-runs the forward from the
-    saved inputs."""
+    """(u, v, w, thp, php, mu') -> the same fields one dycore step later,
+    with the hand-written adjoint as its backward.  ``apply(state, cfg,
+    *fields)`` writes the fields into the state and steps it there with
+    ``step_taped``; the backward is ``step_adjoint_taped`` on that tape
+    (the state is its workspace)."""
 
     @staticmethod
     def forward(ctx, state, cfg, *fields):
-        from .dycore import step
         if len(fields) != len(STAGE_STATE):
             raise ValueError(f"expected {len(STAGE_STATE)} fields")
         for name, x in zip(STAGE_STATE, fields):
             getattr(state, name).copy_(x)
-        ctx.state, ctx.cfg = state, cfg
-        ctx.save_for_backward(*[x.detach() for x in fields])
-        step(state, cfg)
+        ctx.state = state
+        ctx.tape = step_taped(state, cfg)
         return tuple(getattr(state, n).clone() for n in STAGE_STATE)
 
     @staticmethod
     def backward(ctx, *grads):
-        st = ctx.state
-        for name, x in zip(STAGE_STATE, ctx.saved_tensors):
-            getattr(st, name).copy_(x)
         lam = {n: g for n, g in zip(STAGE_STATE, grads) if g is not None}
-        g = step_adjoint(st, ctx.cfg, lam)
+        g = step_adjoint_taped(ctx.state, ctx.tape, lam)
+        ctx.tape = None
         return (None, None) + tuple(g[n] for n in STAGE_STATE)

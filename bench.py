@@ -457,9 +457,22 @@ def main_adjoint(args):
                  for n in adjoint.STAGE_STATE}
         t_da = _time_ms(lambda: adjoint.step_adjoint(sd, cfg_dry, lam_s),
                         reps)
+        holder = {}
+
+        def taped():
+            restore_d()
+            holder["tape"] = adjoint.step_taped(sd, cfg_dry)
+
+        t_tp = _time_ms(taped, reps) - t_rd
+        t_rev = _time_ms(lambda: adjoint.step_adjoint_taped(
+            sd, holder["tape"], lam_s), reps)
+        holder.clear()
         rec.update({"dry_step_ms": round(t_ds, 4),
                     "step_adjoint_ms": round(t_da, 4),
-                    "step_adjoint_to_forward": round(t_da / t_ds, 3)})
+                    "step_adjoint_to_forward": round(t_da / t_ds, 3),
+                    "taped_step_ms": round(t_tp, 4),
+                    "reverse_sweep_ms": round(t_rev, 4),
+                    "reverse_to_forward": round(t_rev / t_ds, 3)})
         del sd
         # the four advection kernels at the stage mass fluxes of the
         # state: forward tendency against the adjoint (field and all three

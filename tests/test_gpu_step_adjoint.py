@@ -193,6 +193,30 @@ def test_step_adjoint_dot_product_vs_device_step(name):
     assert abs(jv - vjt) <= 1e-6 * abs(jv), (jv, vjt)
 
 
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12),
+                                       (torch.float32, 1e-5)])
+def test_taped_step_equals_fused_step(dtype, tol):
+    """The piecewise taped forward advances the state like the fused
+    device step (to rounding: the stage EOS of stages 2 and 3 is formed by
+    its own pass instead of inside the finish)."""
+    from arwen_amd import adjoint, dycore
+    case, s, g, rng = _step_states("terrain", dtype=dtype, seed=47)
+    print(f"taped step vs fused step {dtype}", flush=True)
+    x0 = {n: getattr(g, n).clone() for n in A.STAGE_IN}
+    dycore.step(g, case.cfg)
+    fused = {n: getattr(g, n).clone() for n in A.STAGE_IN}
+    for n, t in x0.items():
+        getattr(g, n).copy_(t)
+    dycore.update_diagnostics(g)
+    tape = adjoint.step_taped(g, case.cfg)
+    for n in A.STAGE_IN:
+        a = getattr(g, n).double()
+        b = fused[n].double()
+        assert torch.equal(tape.out[n], getattr(g, n)), n
+        scale = float(b.abs().max()) or 1.0
+        assert float((a - b).abs().max()) <= tol * scale, n
+
+
 def test_dycore_step_autograd():
     from arwen_amd import adjoint
     case, s, g, rng = _step_states("terrain", seed=41)
