@@ -328,46 +328,65 @@ __global__ __launch_bounds__(256) void k_adj_p3(Dom d, Sc s, Adj a, Fw f)
 }
 
 // S2: column sums of the mu'' pieces, Omega'' suffix sums, and the
-// divergence adjoint.
-__global__ __launch_bounds__(64) void k_adj_s2(Dom d, Sc s, Adj a, Fw f)
+// divergence adjoint.  Split over KG level groups (contiguous level
+// chunks for the suffix sums, whose chunk totals meet in LDS).
+__global__ __launch_bounds__(64 * KG) void k_adj_s2(Dom d, Sc s, Adj a, Fw f)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= d.ny * d.nx) return;
+    const int c = blockIdx.x * 64 + threadIdx.x, ty = threadIdx.y;
     const int st = d.ny * d.nx, nz = d.nz;
+    const bool on = c < st;
     const real half = R(0.5), one = R(1.0), dt = s.dtau, eps = s.epssm;
     const real* __restrict__ MC1 = f.s[W_MC1];
     const real* __restrict__ MC2 = f.s[W_MC2];
     const real* __restrict__ MC3 = f.s[W_MC3];
     const real* __restrict__ LWW = f.s[W_LWW];
     real* __restrict__ LDVP = f.s[W_LDVP];
-    real lM1 = a.lmu[c], lmuave = R(0.0);
-    for (int k = 0; k <= nz; ++k) {
-        lM1 += MC1[k * st + c];
-        lmuave += MC2[k * st + c];
-        if (k < nz) lM1 += MC3[k * st + c];
-    }
-    const real lM0old = half * (one - eps) * lmuave;
-    lM1 += half * (one + eps) * lmuave;
-    real acc = R(0.0), ldmdt = R(0.0), gr = R(0.0);
-    for (int k = nz - 1; k >= 0; --k) {
-        real linc = R(0.0);
-        if (k < nz - 1) {
-            acc += LWW[(k + 1) * st + c];
-            linc = -acc;
+    real m[2] = {R(0.0), R(0.0)};                   // sum MC1 + MC3, sum MC2
+    if (on)
+        for (int k = ty; k <= nz; k += KG) {
+            m[0] += MC1[k * st + c];
+            m[1] += MC2[k * st + c];
+            if (k < nz) m[0] += MC3[k * st + c];
         }
-        LDVP[k * st + c] = d.dnw[k] * linc;
-        ldmdt += d.dnw[k] * d.c1h[k] * linc;
-        gr += d.dnw[k] * d.c1h[k] * linc;
+    group_sums(m);
+    const real lM1 = (on ? a.lmu[c] : R(0.0)) + m[0]
+                     + half * (one + eps) * m[1];
+    const real lM0old = half * (one - eps) * m[1];
+    // linc[k] = -sum_{m = k+1}^{nz-1} LWW[m]: chunk [k0, k1) of this group,
+    // offset by the totals of the chunks above it
+    const int ch = (nz + KG - 1) / KG;
+    const int k0 = ty * ch < nz ? ty * ch : nz;
+    const int k1 = k0 + ch < nz ? k0 + ch : nz;
+    real tot[KG] = {};
+    real own = R(0.0);
+    if (on)
+        for (int k = k0; k < k1; ++k) own += LWW[k * st + c];
+    {
+        __shared__ real sh[KG][64];
+        sh[ty][threadIdx.x] = own;
+        __syncthreads();
+        for (int g = 0; g < KG; ++g) tot[g] = sh[g][threadIdx.x];
+        __syncthreads();
     }
+    real run = R(0.0);
+    for (int g = KG - 1; g > ty; --g) run += tot[g];
+    real p[1] = {R(0.0)};                           // sum dnw c1h linc
+    if (on)
+        for (int k = k1 - 1; k >= k0; --k) {
+            const real linc = k < nz - 1 ? -run : R(0.0);
+            LDVP[k * st + c] = d.dnw[k] * linc;
+            p[0] += d.dnw[k] * d.c1h[k] * linc;
+            run += LWW[k * st + c];
+        }
+    group_sums(p);
+    const real ldmdt = p[0] + dt * lM1;
+    if (!on) return;
+    for (int k = k0; k < k1; ++k)
+        LDVP[k * st + c] = LDVP[k * st + c] + d.dnw[k] * ldmdt;
+    if (ty != 0) return;
     f.s[W_LM1][c] = lM1;
-    ldmdt += dt * lM1;
-    gr += dt * lM1;
-    a.grmu[c] = gr;
+    a.grmu[c] = ldmdt;
     a.gmu[c] = lM1 + lM0old;
-    for (int k = 0; k < nz; ++k) {
-        const int x = k * st + c;
-        LDVP[x] = LDVP[x] + d.dnw[k] * ldmdt;
-    }
 }
 
 // Face pass: u faces (j, fc), fc = 0..nx, then v faces (g, i), g = 0..ny.
@@ -513,13 +532,16 @@ __global__ __launch_bounds__(256) void k_adj_f2(Dom d, Sc s, Fw f)
     f.s[isu ? W_LSX : W_LSY][k * st + fg.slot] = v;
 }
 
-// Cell pass: gather the face pieces of the advance_uv transpose.
-__global__ void k_adj_cells(Dom d, Sc s, Adj a, Fw f)
+// Cell pass: gather the face pieces of the advance_uv transpose.  Split
+// over KG level groups; phi'' (w levels) gathers the bracket terms of the
+// two mass levels it bounds.
+__global__ __launch_bounds__(64 * KG) void k_adj_cells(Dom d, Sc s, Adj a, Fw f)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= st) return;
-    const int j = c / nx, i = c - j * nx;
+    const int c = blockIdx.x * 64 + threadIdx.x, ty = threadIdx.y;
+    const bool on = c < st;
+    const int cc = on ? c : 0;
+    const int j = cc / nx, i = cc - j * nx;
     const int ip = j * nx + (i + 1 == nx ? 0 : i + 1);
     const int im = j * nx + (i == 0 ? nx - 1 : i - 1);
     const int jp = (j + 1 == ny ? 0 : j + 1) * nx + i;
@@ -531,31 +553,42 @@ __global__ void k_adj_cells(Dom d, Sc s, Adj a, Fw f)
     const real one = R(1.0), sm = s.smdiv;
     const real* LTX = f.s[W_LPH1];
     const real* LTY = f.s[W_LW];
-    real lms = R(0.0);
-    for (int k = 0; k < nz; ++k) {
+    auto lb = [&](int k) {
         const int x = k * st;
-        // the face mass sums: -1/2 c1h t of the faces the cell touches
-        lms += -R(0.5) * d.c1h[k] * ((LTX[x + c] + LTX[x + ip])
-                                     + (LTY[x + c] + LTY[x + jp]));
-        auto alt = [&](int cc) { return d.alt[x + cc]; };
-        // face c is the cell's own (A side), face ip/jp has it as B side
-        real lpd = (alt(c) + alt(im)) * LBX[x + c] - (alt(ip) + alt(c)) * LBX[x + ip]
-                   + (alt(c) + alt(jm)) * LBY[x + c] - (alt(jp) + alt(c)) * LBY[x + jp];
-        lpd += LSX[x + c] + LSX[x + ip] + LSY[x + c] + LSY[x + jp];
-        const real lb = (LBX[x + c] - LBX[x + ip]) + (LBY[x + c] - LBY[x + jp]);
-        a.gph[x + c] += lb;
-        a.gph[x + st + c] += lb;
-        real gal = R(0.0);
-        if (d.base3d) {
-            auto pb = [&](int cc) { return d.pb[x + cc]; };
-            gal = (pb(c) - pb(im)) * LBX[x + c] + (pb(ip) - pb(c)) * LBX[x + ip]
-                  + (pb(c) - pb(jm)) * LBY[x + c] + (pb(jp) - pb(c)) * LBY[x + jp];
+        return (LBX[x + cc] - LBX[x + ip]) + (LBY[x + cc] - LBY[x + jp]);
+    };
+    real lms[1] = {R(0.0)};
+    if (on)
+        for (int k = ty; k <= nz; k += KG) {
+            const int x = k * st;
+            real gph = R(0.0);
+            if (k < nz) {
+                // the face mass sums: -1/2 c1h t of the faces the cell touches
+                lms[0] += -R(0.5) * d.c1h[k] * ((LTX[x + c] + LTX[x + ip])
+                                                + (LTY[x + c] + LTY[x + jp]));
+                auto alt = [&](int q) { return d.alt[x + q]; };
+                // face c is the cell's own (A side), face ip/jp has it as B
+                real lpd = (alt(c) + alt(im)) * LBX[x + c]
+                           - (alt(ip) + alt(c)) * LBX[x + ip]
+                           + (alt(c) + alt(jm)) * LBY[x + c]
+                           - (alt(jp) + alt(c)) * LBY[x + jp];
+                lpd += LSX[x + c] + LSX[x + ip] + LSY[x + c] + LSY[x + jp];
+                gph += lb(k);
+                real gal = R(0.0);
+                if (d.base3d) {
+                    auto pb = [&](int q) { return d.pb[x + q]; };
+                    gal = (pb(c) - pb(im)) * LBX[x + c] + (pb(ip) - pb(c)) * LBX[x + ip]
+                          + (pb(c) - pb(jm)) * LBY[x + c] + (pb(jp) - pb(c)) * LBY[x + jp];
+                }
+                a.gal[x + c] = gal;
+                a.gp[x + c] = (one + sm) * lpd + a.lpo[x + c];
+                a.gpo[x + c] = -sm * lpd;
+            }
+            if (k >= 1) gph += lb(k - 1);
+            a.gph[x + c] += gph;
         }
-        a.gal[x + c] = gal;
-        a.gp[x + c] = (one + sm) * lpd + a.lpo[x + c];
-        a.gpo[x + c] = -sm * lpd;
-    }
-    a.gmu[c] += lms;
+    group_sums(lms);
+    if (on && ty == 0) a.gmu[c] += lms[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -904,18 +937,21 @@ __global__ __launch_bounds__(256) void k_ref_cells(Dom d, Fw f, RefAcc r)
 // per column: the level pieces of the column-mass gradient and the
 // damper's top height; this substep's dmdt_rf adjoint joins grf (the stage
 // reference u, v and mass it weights are mapped once per stage)
-__global__ __launch_bounds__(256) void k_ref_col(Dom d, Sc s, Fw f, RefAcc r)
+__global__ __launch_bounds__(64 * KG) void k_ref_col(Dom d, Sc s, Fw f, RefAcc r)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int nz = d.nz, st = d.ny * d.nx;
-    if (c >= st) return;
-    real m = R(0.0), top = R(0.0);
-    for (int k = 0; k <= nz; ++k) {
-        m += f.s[W_RM][k * st + c];
-        top += f.s[W_RPH][k * st + c];
-    }
-    r.ph[nz * st + c] += top;
-    r.mut[c] += m;
+    const int c = blockIdx.x * 64 + threadIdx.x, ty = threadIdx.y;
+    const bool on = c < st;
+    real v[2] = {R(0.0), R(0.0)};                   // mass, top height
+    if (on)
+        for (int k = ty; k <= nz; k += KG) {
+            v[0] += f.s[W_RM][k * st + c];
+            v[1] += f.s[W_RPH][k * st + c];
+        }
+    group_sums(v);
+    if (!on || ty != 0) return;
+    r.ph[nz * st + c] += v[1];
+    r.mut[c] += v[0];
     r.grf[c] += s.dtau * f.s[W_LM1][c];
 }
 
@@ -975,38 +1011,40 @@ __global__ __launch_bounds__(256) void k_refg_dmdt(Dom d, Sc s, RefAcc r)
 }
 
 // ... and in the column mass (with the EOS's mu' column sum)
-__global__ void k_refg_mu(Dom d, Sc sc, RefAcc r, real* gmup, const real* w)
+__global__ __launch_bounds__(64 * KG) void k_refg_mu(Dom d, Sc sc, RefAcc r,
+                                                    real* gmup, const real* w)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= st) return;
+    const int c = blockIdx.x * 64 + threadIdx.x, ty = threadIdx.y;
+    const bool on = c < st;
+    const int cc = on ? c : 0;
     const size_t F = (size_t)(nz + 1) * (ny + 1) * (nx + 1);
-    real s = R(0.0);
-    for (int k = 0; k < nz; ++k) s += w[G_MU3 * F + (size_t)k * st + c];
-    const int j = c / nx, i = c - j * nx;
-    auto su = [&](int ff) {
-        real v = R(0.0);
-        for (int k = 0; k < nz; ++k)
-            v += d.c1h[k] * d.u[iU(k, j, ff, ny, nx)] * d.dnw[k];
-        return v;
-    };
-    auto sv = [&](int gg) {
-        real v = R(0.0);
-        for (int k = 0; k < nz; ++k)
-            v += d.c1h[k] * d.v[iV(k, gg, i, ny, nx)] * d.dnw[k];
-        return v;
-    };
-    const real half = R(0.5);
+    const int j = cc / nx, i = cc - j * nx;
     const int im = i == 0 ? nx - 1 : i - 1, ip = i + 1 == nx ? 0 : i + 1;
     const int jm = j == 0 ? ny - 1 : j - 1, jp = j + 1 == ny ? 0 : j + 1;
+    // the EOS's mu' terms, then the column sums of c1h dnw u (v) on the
+    // faces i, i+1 and the neighbours' faces (i == 0 ? nx : i), ip
+    const int fu[4] = {i, i + 1, i == 0 ? nx : i, ip};
+    const int fv[4] = {j, j + 1, j == 0 ? ny : j, jp};
+    real v[9] = {};
+    if (on)
+        for (int k = ty; k < nz; k += KG) {
+            v[0] += w[G_MU3 * F + (size_t)k * st + c];
+            const real cd = d.c1h[k];
+            for (int q = 0; q < 4; ++q) {
+                v[1 + q] += cd * d.u[iU(k, j, fu[q], ny, nx)] * d.dnw[k];
+                v[5 + q] += cd * d.v[iV(k, fv[q], i, ny, nx)] * d.dnw[k];
+            }
+        }
+    group_sums(v);
+    if (!on || ty != 0) return;
+    const real half = R(0.5);
     const real G = r.grf[c];
-    real m = half * (sc.rdx * G * su(i + 1)
-                     + sc.rdx * r.grf[j * nx + im] * su(i == 0 ? nx : i))
-             + half * (-sc.rdx * G * su(i) - sc.rdx * r.grf[j * nx + ip] * su(ip));
-    m += half * (sc.rdy * G * sv(j + 1)
-                 + sc.rdy * r.grf[jm * nx + i] * sv(j == 0 ? ny : j))
-         + half * (-sc.rdy * G * sv(j) - sc.rdy * r.grf[jp * nx + i] * sv(jp));
-    gmup[c] = (r.mut[c] + m) + s;
+    real m = half * (sc.rdx * G * v[2] + sc.rdx * r.grf[j * nx + im] * v[3])
+             + half * (-sc.rdx * G * v[1] - sc.rdx * r.grf[j * nx + ip] * v[4]);
+    m += half * (sc.rdy * G * v[6] + sc.rdy * r.grf[jm * nx + i] * v[7])
+         + half * (-sc.rdy * G * v[5] - sc.rdy * r.grf[jp * nx + i] * v[8]);
+    gmup[c] = (r.mut[c] + m) + v[0];
 }
 
 // ---- the kinematic lower boundary w(sfc) = u . grad(ht) ----------------
@@ -1150,7 +1188,7 @@ static int32_t substep_adjoint(const Dom* d, const arw_params* prm,
         hipLaunchKernelGGL(k_ref_lev, lev, dim3(256), 0, st, *d, s, a, fw, *ref);
         if ((rc = lib_launched(hipGetLastError(), "k_ref_lev"))) return rc;
     }
-    hipLaunchKernelGGL(k_adj_s2, colg, dim3(64), 0, st, *d, s, a, fw);
+    hipLaunchKernelGGL(k_adj_s2, colg, dim3(64, KG), 0, st, *d, s, a, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_s2"))) return rc;
     const int nf = ny * (nx + 1) + (ny + 1) * nx;
     const dim3 flev((nf + 255) / 256, nz);
@@ -1162,13 +1200,12 @@ static int32_t substep_adjoint(const Dom* d, const arw_params* prm,
     }
     hipLaunchKernelGGL(k_adj_f2, flev, dim3(256), 0, st, *d, s, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_f2"))) return rc;
-    hipLaunchKernelGGL(k_adj_cells, dim3((cols + 127) / 128), dim3(128), 0, st,
-                       *d, s, a, fw);
+    hipLaunchKernelGGL(k_adj_cells, colg, dim3(64, KG), 0, st, *d, s, a, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_cells"))) return rc;
     if (!ref) return ARW_OK;
     hipLaunchKernelGGL(k_ref_cells, lev, dim3(256), 0, st, *d, fw, *ref);
     if ((rc = lib_launched(hipGetLastError(), "k_ref_cells"))) return rc;
-    hipLaunchKernelGGL(k_ref_col, dim3((cols + 255) / 256), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_ref_col, dim3((cols + 63) / 64), dim3(64, KG), 0, st,
                        *d, s, fw, *ref);
     return lib_launched(hipGetLastError(), "k_ref_col");
 }
@@ -1237,7 +1274,7 @@ int32_t ARW_API(acoustic_reference_gradients)(
     hipLaunchKernelGGL(k_refg_dmdt, dim3((nf + 255) / 256, nz), dim3(256), 0,
                        st, *d, sc, r);
     if ((rc = lib_launched(hipGetLastError(), "k_refg_dmdt"))) return rc;
-    hipLaunchKernelGGL(k_refg_mu, dim3((cols + 255) / 256), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_refg_mu, dim3((cols + 63) / 64), dim3(64, KG), 0, st,
                        *d, sc, r, g_mup, work);
     return lib_launched(hipGetLastError(), "k_refg_mu");
 }

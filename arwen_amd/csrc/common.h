@@ -276,3 +276,24 @@ struct Kahan {
         s = t;
     }
 };
+
+// Column sums split over KG level groups: thread (tx, ty) of a (64, KG)
+// block sums levels ty, ty + KG, ... of column blockIdx.x * 64 + tx (each
+// level's loads coalesced over tx), then the groups meet in LDS.  Four
+// times the waves of a thread-per-column loop, which at a few tens of
+// thousands of columns leaves most SIMDs without a second wave to hide
+// load latency behind.
+constexpr int KG = 4;
+
+template <int N>
+__device__ __forceinline__ void group_sums(real (&v)[N])
+{
+    static_assert(KG == 4, "group_sums adds four level groups");
+    __shared__ real sh[N][KG][64];
+    for (int q = 0; q < N; ++q) sh[q][threadIdx.y][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int q = 0; q < N; ++q)
+        v[q] = (sh[q][0][threadIdx.x] + sh[q][1][threadIdx.x])
+               + (sh[q][2][threadIdx.x] + sh[q][3][threadIdx.x]);
+    __syncthreads();                 // sh may be reused by a later call
+}

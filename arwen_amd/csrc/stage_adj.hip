@@ -662,29 +662,34 @@ __device__ __forceinline__ real face_pair_v(const real* P, int k, int j,
 }
 
 // per column: the time-t and stage mass gradients
-__global__ void k_ia_cols(IA a)
+__global__ __launch_bounds__(64 * KG) void k_ia_cols(IA a)
 {
     const Dom& d = a.d;
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= st) return;
-    const int j = c / nx, i = c - j * nx;
-    real gmt = R(0.0), gms = R(0.0), gmupp = a.lmu[c], geos = R(0.0);
-    for (int k = 0; k < nz; ++k) {
-        const int t = k * st + c;
-        gmt += a.w[I_M3][t] + face_pair_u(a.w[I_PU0], k, j, i, ny, nx)
-               + face_pair_v(a.w[I_PV0], k, j, i, ny, nx);
-        gms += a.w[I_S3][t] + face_pair_u(a.w[I_PU], k, j, i, ny, nx)
-               + face_pair_v(a.w[I_PV], k, j, i, ny, nx);
-        gmupp += a.w[I_U3][t];
-        geos += a.w[I_U3B][t];
+    const int c = blockIdx.x * 64 + threadIdx.x, ty = threadIdx.y;
+    const bool on = c < st;
+    const int cc = on ? c : 0;
+    const int j = cc / nx, i = cc - j * nx;
+    real v[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};   // gmt, gms, gmupp, geos
+    if (on) {
+        for (int k = ty; k < nz; k += KG) {
+            const int t = k * st + c;
+            v[0] += a.w[I_M3][t] + face_pair_u(a.w[I_PU0], k, j, i, ny, nx)
+                    + face_pair_v(a.w[I_PV0], k, j, i, ny, nx);
+            v[1] += a.w[I_S3][t] + face_pair_u(a.w[I_PU], k, j, i, ny, nx)
+                    + face_pair_v(a.w[I_PV], k, j, i, ny, nx);
+            v[2] += a.w[I_U3][t];
+            v[3] += a.w[I_U3B][t];
+        }
+        for (int k = ty; k <= nz; k += KG) {
+            v[0] += a.w[I_MTW][k * st + c];
+            v[1] += a.w[I_MSW][k * st + c];
+        }
     }
-    for (int k = 0; k <= nz; ++k) {
-        gmt += a.w[I_MTW][k * st + c];
-        gms += a.w[I_MSW][k * st + c];
-    }
-    a.gmup0[c] = gmupp + gmt;
-    a.gmup[c] = (gms - gmupp) + geos;
+    group_sums(v);
+    if (!on || ty != 0) return;
+    a.gmup0[c] = (a.lmu[c] + v[2]) + v[0];
+    a.gmup[c] = (v[1] - (a.lmu[c] + v[2])) + v[3];
 }
 
 struct FA {
@@ -764,27 +769,32 @@ __global__ void k_fa_pts(FA a)
     }
 }
 
-__global__ void k_fa_cols(FA a)
+__global__ __launch_bounds__(64 * KG) void k_fa_cols(FA a)
 {
     const Dom& d = a.d;
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= st) return;
-    const int j = c / nx, i = c - j * nx;
-    real gms = R(0.0), gmn = R(0.0);
-    for (int k = 0; k < nz; ++k) {
-        gms += a.w[F_S3][k * st + c] + face_pair_u(a.w[F_PSU], k, j, i, ny, nx)
-               + face_pair_v(a.w[F_PSV], k, j, i, ny, nx);
-        gmn += a.w[F_N3][k * st + c] + face_pair_u(a.w[F_PNU], k, j, i, ny, nx)
-               + face_pair_v(a.w[F_PNV], k, j, i, ny, nx);
+    const int c = blockIdx.x * 64 + threadIdx.x, ty = threadIdx.y;
+    const bool on = c < st;
+    const int cc = on ? c : 0;
+    const int j = cc / nx, i = cc - j * nx;
+    real v[2] = {R(0.0), R(0.0)};                   // gms, gmn
+    if (on) {
+        for (int k = ty; k < nz; k += KG) {
+            v[0] += a.w[F_S3][k * st + c] + face_pair_u(a.w[F_PSU], k, j, i, ny, nx)
+                    + face_pair_v(a.w[F_PSV], k, j, i, ny, nx);
+            v[1] += a.w[F_N3][k * st + c] + face_pair_u(a.w[F_PNU], k, j, i, ny, nx)
+                    + face_pair_v(a.w[F_PNV], k, j, i, ny, nx);
+        }
+        for (int k = ty; k <= nz; k += KG) {
+            v[0] += a.w[F_SW][k * st + c];
+            v[1] += a.w[F_NW][k * st + c];
+        }
     }
-    for (int k = 0; k <= nz; ++k) {
-        gms += a.w[F_SW][k * st + c];
-        gmn += a.w[F_NW][k * st + c];
-    }
+    group_sums(v);
+    if (!on || ty != 0) return;
     const real L = a.lmup[c];
-    a.gmup[c] = (L + gms) + gmn;
-    a.gmupp[c] = L + gmn;
+    a.gmup[c] = (L + v[0]) + v[1];
+    a.gmupp[c] = L + v[1];
 }
 
 }  // namespace
@@ -845,7 +855,7 @@ extern "C" int32_t ARW_API(small_step_init_adjoint)(
     if ((rc = lib_launched(hipGetLastError(), "k_ia_eos"))) return rc;
     hipLaunchKernelGGL(k_ia_php, dim3((nwl + 255) / 256), dim3(256), 0, sm, a);
     if ((rc = lib_launched(hipGetLastError(), "k_ia_php"))) return rc;
-    hipLaunchKernelGGL(k_ia_cols, dim3((st + 255) / 256), dim3(256), 0, sm, a);
+    hipLaunchKernelGGL(k_ia_cols, dim3((st + 63) / 64), dim3(64, KG), 0, sm, a);
     return lib_launched(hipGetLastError(), "k_ia_cols");
 }
 
@@ -881,7 +891,7 @@ extern "C" int32_t ARW_API(small_step_finish_adjoint)(
     const dim3 gu((nx + 1 + 63) / 64, (ny + 1 + 3) / 4, nz + 1);
     hipLaunchKernelGGL(k_fa_pts, gu, blk, 0, sm, a);
     if ((rc = lib_launched(hipGetLastError(), "k_fa_pts"))) return rc;
-    hipLaunchKernelGGL(k_fa_cols, dim3((st + 255) / 256), dim3(256), 0, sm, a);
+    hipLaunchKernelGGL(k_fa_cols, dim3((st + 63) / 64), dim3(64, KG), 0, sm, a);
     return lib_launched(hipGetLastError(), "k_fa_cols");
 }
 
