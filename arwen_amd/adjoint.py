@@ -549,12 +549,15 @@ _PREP = ("p", "al", "alt", "ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t")
 
 class StepTape:
     """What the reverse sweep of one step needs, recorded by ``step_taped``:
-    per RK stage the stage state, what its preparation leaves (EOS, slow
-    tendencies, Omega, dmdt_rf) and every substep's small-step fields and
-    Omega''; and the step's output fields."""
+    per RK stage the stage state and, with ``keep="substeps"``, what its
+    preparation leaves (EOS, slow tendencies, Omega, dmdt_rf) and every
+    substep's small-step fields and Omega'' (``keep="stages"`` keeps only
+    the stage states, and the reverse sweep recomputes each stage from
+    its checkpoint); and the step's output fields."""
 
-    def __init__(self, cfg: RunConfig):
+    def __init__(self, cfg: RunConfig, keep: str):
         self.cfg = cfg
+        self.keep = keep
         self.stages = []
         self.out = None
 
@@ -566,33 +569,50 @@ def _check_adjoint_cfg(state, cfg):
                               "diffusion, plain form")
 
 
-def step_taped(state: DomainState, cfg: RunConfig) -> StepTape:
-    """One dycore step of the state (as ``dycore.step``; reference
-    dycore.py:2161-2503), run stage by stage through the piecewise entry
-    points so that it records the tape its adjoint needs.  Dry states
-    without constant-K diffusion, plain form."""
+def _run_stage(state, cfg, nsub, dtau, record):
+    """One stage's preparation and substeps from the state's stage and
+    time-t fields; with ``record``, returns what the stage's adjoint reads
+    (the preparation's outputs and every substep's small-step fields)."""
     from . import dycore as dyn
-    _check_state(state)
-    _check_adjoint_cfg(state, cfg)
     nz, ny, nx = state.shape
-    tape = StepTape(cfg)
-    for n in STAGE_STATE:
-        getattr(state, n + "0").copy_(getattr(state, n))
-    ww = state.scratch((nz + 1, ny, nx), "arw_ww")
-    dmdt = state.scratch((ny, nx), "arw_dmdt_rf")
-    for nsub, dtau in _step_stages(cfg):
-        xs = {n: getattr(state, n).clone() for n in STAGE_STATE}
-        dyn.stage_prepare(state, cfg, dtau, True)
+    dyn.stage_prepare(state, cfg, dtau, True)
+    prep = subs = None
+    if record:
         prep = {n: getattr(state, n).clone() for n in _PREP}
-        prep["ww"] = ww.clone()
-        prep["dmdt_rf"] = dmdt.clone()
+        prep["ww"] = state.scratch((nz + 1, ny, nx), "arw_ww").clone()
+        prep["dmdt_rf"] = state.scratch((ny, nx), "arw_dmdt_rf").clone()
         subs = [{n: getattr(state, n).clone() for n in ACOUSTIC_STATE}]
-        for i in range(nsub):
-            dyn.acoustic_substep(state, cfg, dtau, first=(i == 0))
+    for i in range(nsub):
+        dyn.acoustic_substep(state, cfg, dtau, first=(i == 0))
+        if record:
             x = {n: getattr(state, n).clone() for n in ACOUSTIC_STATE}
             x["ww_pp"] = state.ww_pp.clone()
             subs.append(x)
-        tape.stages.append((xs, prep, subs, dtau))
+    return prep, subs
+
+
+def step_taped(state: DomainState, cfg: RunConfig,
+               keep: str = "substeps") -> StepTape:
+    """One dycore step of the state (as ``dycore.step``; reference
+    dycore.py:2161-2503), run stage by stage through the piecewise entry
+    points so that it records the tape its adjoint needs.  ``keep``:
+    "substeps" records every stage's preparation and substep states (the
+    faster reverse sweep; at 1000x1000x50 FP32 about 30 GB), "stages" only
+    the three stage states (3.6 GB there; the reverse sweep recomputes each
+    stage from its checkpoint).  Dry states without constant-K diffusion,
+    plain form."""
+    from . import dycore as dyn
+    _check_state(state)
+    _check_adjoint_cfg(state, cfg)
+    if keep not in ("substeps", "stages"):
+        raise ValueError("keep must be 'substeps' or 'stages'")
+    tape = StepTape(cfg, keep)
+    for n in STAGE_STATE:
+        getattr(state, n + "0").copy_(getattr(state, n))
+    for nsub, dtau in _step_stages(cfg):
+        xs = {n: getattr(state, n).clone() for n in STAGE_STATE}
+        prep, subs = _run_stage(state, cfg, nsub, dtau, keep == "substeps")
+        tape.stages.append((xs, prep, subs, nsub, dtau))
         dyn.small_step_finish(state)
     dyn.set_w_surface(state, cfg)
     dyn.update_diagnostics(state)
@@ -606,7 +626,9 @@ def step_adjoint_taped(state: DomainState, tape: StepTape,
     """The reverse sweep of one step: J^T lam from the tape ``step_taped``
     recorded (``lam`` maps the step's output STAGE_STATE fields to
     adjoints, missing = 0; returns the gradients of its input fields).
-    Per stage, last first: the finish adjoint, the substep adjoints in
+    Per stage, last first (recomputing the stage's preparation and
+    substeps from its checkpoint when the tape kept only stage states):
+    the finish adjoint, the substep adjoints in
     reverse order (given each substep's outputs, adding the stage-reference
     share), the small-step initialisation adjoint (to the time-t and stage
     fields) and the tendency-pass adjoint; the stages chain through their
@@ -628,10 +650,11 @@ def step_adjoint_taped(state: DomainState, tape: StepTape,
     for n in names:
         getattr(state, n + "0").copy_(x0[n])
     g0 = {n: torch.zeros_like(getattr(state, n)) for n in names}
-    for xs, prep, subs, dtau in reversed(tape.stages):
+    for xs, prep, subs, nsub, dtau in reversed(tape.stages):
         for n in names:
             getattr(state, n).copy_(xs[n])
-        nsub = len(subs) - 1
+        if prep is None:                 # recompute the stage's tape
+            prep, subs = _run_stage(state, cfg, nsub, dtau, True)
         fin = small_step_finish_adjoint(
             state, lx, views={n: subs[nsub][n] for n in FINISH_PP})
         gs = {n: fin[n] for n in names}
@@ -661,7 +684,8 @@ def step_adjoint_taped(state: DomainState, tape: StepTape,
     return g0
 
 
-def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict) -> dict:
+def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict,
+                 keep: str = "substeps") -> dict:
     """J^T lam of one dycore step (``dycore.step``; reference dycore.py:
     2161-2503) at the state's prognostic fields: ``lam`` maps the new
     state's STAGE_STATE fields to adjoints (missing = 0); returns the
@@ -674,7 +698,7 @@ def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict) -> dict:
     _check_adjoint_cfg(state, cfg)
     x_in = {n: getattr(state, n).clone() for n in STAGE_STATE}
     t0 = state.elapsed_seconds
-    tape = step_taped(state, cfg)
+    tape = step_taped(state, cfg, keep)
     g = step_adjoint_taped(state, tape, lam)
     for n in STAGE_STATE:
         getattr(state, n).copy_(x_in[n])
@@ -686,18 +710,23 @@ def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict) -> dict:
 class DycoreStep(torch.autograd.Function):
     """(u, v, w, thp, php, mu') -> the same fields one dycore step later,
     with the hand-written adjoint as its backward.  ``apply(state, cfg,
-    *fields)`` writes the fields into the state and steps it there with
-    ``step_taped``; the backward is ``step_adjoint_taped`` on that tape
-    (the state is its workspace)."""
+    *fields)`` (or ``apply(state, cfg, keep, *fields)``, keep as for
+    ``step_taped``) writes the fields into the state and steps it there
+    with ``step_taped``; the backward is ``step_adjoint_taped`` on that
+    tape (the state is its workspace)."""
 
     @staticmethod
     def forward(ctx, state, cfg, *fields):
+        ctx.keep_arg = bool(fields) and isinstance(fields[0], str)
+        keep = fields[0] if ctx.keep_arg else "substeps"
+        if ctx.keep_arg:
+            fields = fields[1:]
         if len(fields) != len(STAGE_STATE):
             raise ValueError(f"expected {len(STAGE_STATE)} fields")
         for name, x in zip(STAGE_STATE, fields):
             getattr(state, name).copy_(x)
         ctx.state = state
-        ctx.tape = step_taped(state, cfg)
+        ctx.tape = step_taped(state, cfg, keep)
         return tuple(getattr(state, n).clone() for n in STAGE_STATE)
 
     @staticmethod
@@ -705,4 +734,5 @@ class DycoreStep(torch.autograd.Function):
         lam = {n: g for n, g in zip(STAGE_STATE, grads) if g is not None}
         g = step_adjoint_taped(ctx.state, ctx.tape, lam)
         ctx.tape = None
-        return (None, None) + tuple(g[n] for n in STAGE_STATE)
+        lead = (None, None, None) if ctx.keep_arg else (None, None)
+        return lead + tuple(g[n] for n in STAGE_STATE)

@@ -466,13 +466,24 @@ def main_adjoint(args):
         t_tp = _time_ms(taped, reps) - t_rd
         t_rev = _time_ms(lambda: adjoint.step_adjoint_taped(
             sd, holder["tape"], lam_s), reps)
+
+        def taped_lean():
+            restore_d()
+            holder["tape"] = adjoint.step_taped(sd, cfg_dry, keep="stages")
+
+        t_tl = _time_ms(taped_lean, reps) - t_rd
+        t_rl = _time_ms(lambda: adjoint.step_adjoint_taped(
+            sd, holder["tape"], lam_s), reps)
         holder.clear()
         rec.update({"dry_step_ms": round(t_ds, 4),
                     "step_adjoint_ms": round(t_da, 4),
                     "step_adjoint_to_forward": round(t_da / t_ds, 3),
                     "taped_step_ms": round(t_tp, 4),
                     "reverse_sweep_ms": round(t_rev, 4),
-                    "reverse_to_forward": round(t_rev / t_ds, 3)})
+                    "reverse_to_forward": round(t_rev / t_ds, 3),
+                    "stage_ckpt_taped_step_ms": round(t_tl, 4),
+                    "stage_ckpt_reverse_ms": round(t_rl, 4),
+                    "stage_ckpt_reverse_to_forward": round(t_rl / t_ds, 3)})
         del sd
         # the four advection kernels at the stage mass fluxes of the
         # state: forward tendency against the adjoint (field and all three
