@@ -126,6 +126,9 @@ ArwFinishAdj64 = ArwFinishAdj
 
 #: accumulators of the substep adjoint's stage-reference share
 ACOUSTIC_REF = ("mut", "c2a", "alt", "thr", "ph", "w", "u", "v", "grf")
+#: flags of the accumulating adjoint entry points
+ADJ_CLEAR = 1
+ADJ_ACCUMULATE = 2
 #: outputs of a substep the reference-share adjoint can take as given
 ACOUSTIC_FWD = ("u_pp", "v_pp", "w_pp", "th_pp", "mu_pp", "al_pp", "ww_pp")
 
@@ -194,7 +197,8 @@ def load():
                                               ctypes.POINTER(ArwAcousticAdj),
                                               ctypes.POINTER(ArwAcousticRef),
                                               ctypes.POINTER(ArwAcousticFwd),
-                                              _P, _P], _I),
+                                              _P, _I, _P, _P], _I),
+        "arw_acoustic_adjoint_factors": ([dp, pp, _D, _P, _P], _I),
         "arw_acoustic_reference_gradients": ([dp, pp,
                                               ctypes.POINTER(ArwAcousticRef),
                                               _P, _P, _P, _P, _P], _I),
@@ -274,7 +278,7 @@ FP64_ENTRY_POINTS = (
     "arw64_ensemble_destroy", "arw64_stage_tendencies_adjoint",
     "arw64_stage_adjoint_work", "arw64_small_step_init_adjoint",
     "arw64_small_step_finish_adjoint", "arw64_small_step_adjoint_work",
-    "arw64_acoustic_substep_adjoint_ref",
+    "arw64_acoustic_substep_adjoint_ref", "arw64_acoustic_adjoint_factors",
     "arw64_acoustic_reference_gradients",
     "arw64_set_w_surface_adjoint") + tuple(
         "arw64_" + n for n in FLUX_DIV_ADJOINTS)
@@ -298,6 +302,7 @@ def exported_symbols() -> list[str]:
             "arw_small_step_init_adjoint", "arw_small_step_finish_adjoint",
             "arw_small_step_adjoint_work",
             "arw_acoustic_substep_adjoint_ref",
+            "arw_acoustic_adjoint_factors",
             "arw_acoustic_reference_gradients",
             "arw_set_w_surface_adjoint"] + [
             "arw_" + n for n in FLUX_DIV_ADJOINTS]

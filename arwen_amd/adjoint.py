@@ -183,7 +183,11 @@ def acoustic_substep_adjoint(state: DomainState, cfg: RunConfig,
                              dtau: float, first: bool,
                              lam: dict, ref: dict | None = None,
                              fwd: dict | None = None,
-                             views: dict | None = None) -> dict:
+                             views: dict | None = None,
+                             lu: torch.Tensor | None = None,
+                             clear_ref: bool = False,
+                             tend_out: dict | None = None,
+                             accumulate: bool = False) -> dict:
     """J^T lam for one acoustic substep taken from the state's current
     small-step fields (coefficients prepared as for the forward substep,
     ``dycore.prepare_acoustic_coefficients``).  ``lam`` maps the outputs
@@ -194,9 +198,13 @@ def acoustic_substep_adjoint(state: DomainState, cfg: RunConfig,
     ``reference_gradients`` once the stage's substeps are done); with
     ``ref``, ``fwd`` may give the substep's outputs (ACOUSTIC_FWD names:
     the small-step fields and Omega'' after the forward substep), which
-    are then not recomputed.  ``views``: tensors read in place of the
-    state's arrays of those names (e.g. a checkpointed substep input).
-    The state is not modified."""
+    are then not recomputed, ``lu`` the stage's Thomas factors
+    (``adjoint_factors``), ``clear_ref`` makes this call write the
+    accumulators instead of adding (a stage's first call), ``tend_out``
+    gives the tensors the tendency adjoints go to and ``accumulate`` adds
+    them there.  ``views``: tensors read in place of the state's arrays of
+    those names (e.g. a checkpointed substep input).  The state is not
+    modified."""
     _check_state(state)
     lib = _lib.load()
     pre = "arw64_" if state.dtype == torch.float64 else "arw_"
@@ -220,12 +228,17 @@ def acoustic_substep_adjoint(state: DomainState, cfg: RunConfig,
         setattr(a, "l_" + name, t.data_ptr())
     out = {}
     for name in ACOUSTIC_STATE + ACOUSTIC_TEND:
-        out[name] = torch.empty_like(getattr(state, name))
+        if tend_out is not None and name in ACOUSTIC_TEND:
+            out[name] = _as_like(tend_out[name], getattr(state, name), name)
+            if out[name] is not tend_out[name]:
+                raise ValueError(f"tend_out[{name}] must be contiguous")
+        else:
+            out[name] = torch.empty_like(getattr(state, name))
         setattr(a, "g_" + name, out[name].data_ptr())
     d = _dom_views(state, cfg, views)
     if ref is None:
-        if fwd is not None:
-            raise ValueError("fwd needs ref")
+        if fwd is not None or lu is not None or clear_ref or accumulate:
+            raise ValueError("fwd, lu, clear_ref and accumulate need ref")
         fn = getattr(lib, pre + "acoustic_substep_adjoint")
         _lib.check(fn(d, params(cfg), float(dtau), int(bool(first)),
                       ctypes.byref(a), work.data_ptr(), _stream()),
@@ -244,11 +257,37 @@ def acoustic_substep_adjoint(state: DomainState, cfg: RunConfig,
                                  f"{tuple(like.shape)} {like.dtype} tensor")
             setattr(fs, n, t.data_ptr())
         fp = ctypes.byref(fs)
+    if lu is not None:
+        nz, ny, nx = state.shape
+        if tuple(lu.shape) != (3, nz + 1, ny, nx) or lu.dtype != state.dtype \
+                or not lu.is_contiguous():
+            raise ValueError("lu: expected adjoint_factors' tensor")
+    flags = (_lib.ADJ_CLEAR if clear_ref else 0) \
+        | (_lib.ADJ_ACCUMULATE if accumulate else 0)
     fn = getattr(lib, pre + "acoustic_substep_adjoint_ref")
     _lib.check(fn(d, params(cfg), float(dtau), int(bool(first)),
-                  ctypes.byref(a), ctypes.byref(r), fp, work.data_ptr(),
-                  _stream()), pre + "acoustic_substep_adjoint_ref")
+                  ctypes.byref(a), ctypes.byref(r), fp,
+                  None if lu is None else lu.data_ptr(), flags,
+                  work.data_ptr(), _stream()),
+               pre + "acoustic_substep_adjoint_ref")
     return out
+
+
+def adjoint_factors(state: DomainState, cfg: RunConfig, dtau: float,
+                    views: dict | None = None) -> torch.Tensor:
+    """The stage's implicit-w Thomas factors (3, nz+1, ny, nx) the substep
+    adjoints of a stage can share (they depend on the stage state, its EOS
+    and dtau only)."""
+    _check_state(state)
+    lib = _lib.load()
+    pre = "arw64_" if state.dtype == torch.float64 else "arw_"
+    nz, ny, nx = state.shape
+    lu = torch.empty((3, nz + 1, ny, nx), dtype=state.dtype,
+                     device=state.u.device)
+    _lib.check(getattr(lib, pre + "acoustic_adjoint_factors")(
+        _dom_views(state, cfg, views), params(cfg), float(dtau),
+        lu.data_ptr(), _stream()), pre + "acoustic_adjoint_factors")
+    return lu
 
 
 ACOUSTIC_REF = _lib.ACOUSTIC_REF
@@ -659,18 +698,23 @@ def step_adjoint_taped(state: DomainState, tape: StepTape,
             state, lx, views={n: subs[nsub][n] for n in FINISH_PP})
         gs = {n: fin[n] for n in names}
         lp = {n: fin[n] for n in FINISH_PP}
-        lt = {n: torch.zeros_like(getattr(state, n)) for n in STAGE_TEND}
-        acc = reference_accumulators(state)
+        # the stage's substep adjoints write (first) then add to one set of
+        # tendency adjoints and reference accumulators, and share the
+        # stage's Thomas factors
+        lt = {n: torch.empty_like(getattr(state, n)) for n in ACOUSTIC_TEND}
+        acc = {n: torch.empty(shp, dtype=state.dtype, device=state.u.device)
+               for n, shp in _ref_shapes(state).items()}
+        lu = adjoint_factors(state, cfg, dtau, views=prep)
         for i in range(nsub - 1, -1, -1):
             views = dict(prep)
             views.update(subs[i])
             views.pop("ww_pp", None)
             fwd = {n: subs[i + 1][n] for n in _lib.ACOUSTIC_FWD}
-            out = acoustic_substep_adjoint(state, cfg, dtau, i == 0, lp, acc,
-                                           fwd=fwd, views=views)
+            last = i == nsub - 1
+            out = acoustic_substep_adjoint(
+                state, cfg, dtau, i == 0, lp, acc, fwd=fwd, views=views,
+                lu=lu, clear_ref=last, tend_out=lt, accumulate=not last)
             lp = {n: out[n] for n in ACOUSTIC_STATE}
-            for n in STAGE_TEND:
-                lt[n] += out[n]
         ref = reference_gradients(state, cfg, acc, views=prep)
         gi = small_step_init_adjoint(state, lp)
         gt = stage_tendencies_adjoint(state, cfg, lt)

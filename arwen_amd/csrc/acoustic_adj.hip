@@ -47,7 +47,14 @@ struct Adj {
     const real *lu, *lv, *lw, *lth, *lph, *lmu, *lp, *lpo, *lal, *lww;
     real *gu, *gv, *gw, *gth, *gph, *gmu, *gp, *gpo, *gal;
     real *gru, *grv, *grw, *grth, *grph, *grmu;
+    int acc;                 // tendency adjoints add to their outputs
 };
+
+// a tendency adjoint: written, or added to what the output holds
+__device__ __forceinline__ void put_t(const Adj& a, real* g, int x, real v)
+{
+    g[x] = (a.acc ? g[x] : R(0.0)) + v;
+}
 
 struct Fw {                 // the substep's outputs, and scratch
     const real *M1, *T1, *W1, *AL1, *U1, *V1, *WW1;
@@ -190,6 +197,70 @@ __global__ __launch_bounds__(256) void k_adj_p1(Dom d, Sc s, Adj a, Fw f)
     f.s[W_MC1][x] = mcon;
 }
 
+// The implicit-w system's Thomas factors of a stage (they depend on the
+// stage state and dtau only): al = 1/(b - a gprev) and a of rows 1..nz,
+// and gprev = c al of rows 1..nz-1 -- the same expressions k_adj_s1
+// evaluates, so a sweep reading them is bit-identical to one forming them.
+enum { LU_AL, LU_A, LU_GP, LU_COUNT };
+
+__global__ __launch_bounds__(64) void k_adj_lu(Dom d, Sc s, real* lu)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.ny * d.nx) return;
+    const ColCtx C(d, s, c, R(0.0));
+    const int st = C.st, nz = d.nz;
+    const size_t F = (size_t)(nz + 1) * st;
+    const real one = R(1.0);
+    real gprev = R(0.0);
+    real c2a_k = C.c2a(0), c2a_m;
+    for (int k = 1; k < nz; ++k) {
+        c2a_m = c2a_k;
+        c2a_k = C.c2a(k);
+        const real b = one + C.cof * d.rdn[k] * (
+            d.rdnw[k] * c2a_k / (C.chm(k) * C.cfm(k))
+            + d.rdnw[k - 1] * c2a_m / (C.chm(k - 1) * C.cfm(k)));
+        const real cc = -C.cof * d.rdn[k] * d.rdnw[k] * c2a_k
+                        / (C.chm(k) * C.cfm(k + 1));
+        const real ak = C.a(k);
+        const real al = one / (b - ak * gprev);
+        gprev = cc * al;
+        lu[LU_AL * F + k * st + c] = al;
+        lu[LU_A * F + k * st + c] = ak;
+        lu[LU_GP * F + k * st + c] = gprev;
+    }
+    const real bn = one + R(2.0) * C.cof * d.rdnw[nz - 1] * d.rdnw[nz - 1]
+                    * c2a_k / (C.chm(nz - 1) * C.cfm(nz));
+    const real an = C.a(nz);
+    lu[LU_AL * F + nz * st + c] = one / (bn - an * gprev);
+    lu[LU_A * F + nz * st + c] = an;
+}
+
+// S1 with the stage's factors: the two sweeps only.
+__global__ __launch_bounds__(64) void k_adj_s1_lu(Dom d, Fw f, const real* lu)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int st = d.ny * d.nx, nz = d.nz;
+    if (c >= st) return;
+    const size_t F = (size_t)(nz + 1) * st;
+    const real* __restrict__ LWIN = f.s[W_LPH1];
+    real* __restrict__ LW = f.s[W_LW];
+    real vprev = R(0.0);
+    for (int k = 1; k <= nz; ++k) {
+        const real gp = k >= 2 ? lu[LU_GP * F + (k - 1) * st + c] : R(0.0);
+        const real v = LWIN[k * st + c] - gp * vprev;
+        vprev = v;
+        LW[k * st + c] = v;
+    }
+    real carry = R(0.0);
+    for (int k = nz; k >= 1; --k) {
+        const int x = k * st + c;
+        const real v = lu[LU_AL * F + x] * (LW[x] + carry);
+        LW[x] = v;
+        carry = -lu[LU_A * F + x] * v;
+    }
+    f.s[W_L0][c] = LWIN[c] + carry;
+}
+
 // S1: the transposed Thomas solve of one column.
 __global__ __launch_bounds__(64) void k_adj_s1(Dom d, Sc s, Fw f)
 {
@@ -256,7 +327,7 @@ __global__ __launch_bounds__(256) void k_adj_p2(Dom d, Sc s, Adj a, Fw f)
     if (k >= 1) f.s[W_LRHS][x] = f.s[W_LRHS][x] + (one + eps) * T;
     const bool row = C.row(k);
     a.gw[x] = row ? la_k : R(0.0);
-    a.grw[x] = row ? dt * la_k : R(0.0);
+    put_t(a, a.grw, x, row ? dt * la_k : R(0.0));
     f.s[W_MC2][x] = row ? -dt * g * d.c1f[k] * la_k : R(0.0);
     if (k < nz) {
         real t2 = R(0.0);
@@ -302,7 +373,7 @@ __global__ __launch_bounds__(256) void k_adj_p3(Dom d, Sc s, Adj a, Fw f)
                + half * (one + eps) / (chs * th_ref(d, kk, c)) * LT2[kk * st + c];
     };
     a.gph[x] = f.s[W_LWD][x] + (k == 0 ? f.s[W_G0][c] : lrhs(k));
-    a.grph[x] = k >= 1 ? dt * lr2(k) : R(0.0);
+    put_t(a, a.grph, x, k >= 1 ? dt * lr2(k) : R(0.0));
     if (k >= 1) a.gw[x] = a.gw[x] + dt * half * g * (one - eps) * lr2(k);
     real ww = a.lww ? a.lww[x] : R(0.0);
     if (k < nz) ww += cwd(k) * lwd(k);
@@ -315,7 +386,7 @@ __global__ __launch_bounds__(256) void k_adj_p3(Dom d, Sc s, Adj a, Fw f)
         const real lt2 = LT2[x];
         const real l1 = lT1(k);
         a.gth[x] = half * (one - eps) / (chs * thr) * lt2 + l1;
-        a.grth[x] = dt * l1;
+        put_t(a, a.grth, x, dt * l1);
         f.s[W_GGF][x] = -dt * l1;
         f.s[W_MC3][x] = -t2 * d.c1h[k] / chs * lt2;
         if (k >= 1) {
@@ -385,7 +456,7 @@ __global__ __launch_bounds__(64 * KG) void k_adj_s2(Dom d, Sc s, Adj a, Fw f)
         LDVP[k * st + c] = LDVP[k * st + c] + d.dnw[k] * ldmdt;
     if (ty != 0) return;
     f.s[W_LM1][c] = lM1;
-    a.grmu[c] = ldmdt;
+    put_t(a, a.grmu, c, ldmdt);
     a.gmu[c] = lM1 + lM0old;
 }
 
@@ -488,7 +559,7 @@ __global__ __launch_bounds__(256) void k_adj_f1(Dom d, Sc s, Adj a, Fw f)
     };
     const real l1 = lam1(fg);
     (isu ? a.gu : a.gv)[fg.out0 + k * fg.ostride] = l1;
-    (isu ? a.gru : a.grv)[fg.out0 + k * fg.ostride] = dt * l1;
+    put_t(a, isu ? a.gru : a.grv, fg.out0 + k * fg.ostride, dt * l1);
     if (!fg.own) return;
     const real ldup = has_dup ? lam1(dg) : R(0.0);
     const real ld = -dt * l1 - dt * ldup;
@@ -609,7 +680,13 @@ __global__ __launch_bounds__(64 * KG) void k_adj_cells(Dom d, Sc s, Adj a, Fw f)
 
 struct RefAcc {
     real *mut, *c2a, *alt, *thr, *ph, *w, *u, *v, *grf;
+    int clear;               // this call writes the accumulators
 };
+
+__device__ __forceinline__ void put_r(const RefAcc& r, real* g, int x, real v)
+{
+    g[x] = (r.clear ? R(0.0) : g[x]) + v;
+}
 
 // the damper at w level k: inside the layer?, dw, its sine argument
 __device__ __forceinline__ bool damp_at(const Dom& d, const Sc& s, int k,
@@ -800,9 +877,9 @@ __global__ __launch_bounds__(256) void k_ref_lev(Dom d, Sc s, Adj a, Fw f,
                 - half * s.rdx * U1[iU(k, j, i + 1 == nx ? 0 : i + 1, ny, nx)] * gg(k, cip)
                 + half * s.rdy * V1[iV(k, j == 0 ? ny : j, i, ny, nx)] * gg(k, cjm)
                 - half * s.rdy * V1[iV(k, j + 1 == ny ? 0 : j + 1, i, ny, nx)] * gg(k, cjp);
-        r.c2a[x] += gc2a;
-        r.alt[x] += galt;
-        r.thr[x] += gthr;
+        put_r(r, r.c2a, x, gc2a);
+        put_r(r, r.alt, x, galt);
+        put_r(r, r.thr, x, gthr);
         mpc += d.c1h[k] * gchm;
     }
     // w level k: C(mu_t) of the Thomas rows, the rhs, the phi'' update and
@@ -859,8 +936,8 @@ __global__ __launch_bounds__(256) void k_ref_lev(Dom d, Sc s, Adj a, Fw f,
     if (k >= 1) gph += gdp(k - 1);
     if (k <= nz - 1) gph -= gdp(k);
     mpc += d.c1f[k] * gcfm;
-    r.ph[x] += gph;
-    r.w[x] += gw;
+    put_r(r, r.ph, x, gph);
+    put_r(r, r.w, x, gw);
     f.s[W_RM][x] = mpc;
     f.s[W_RPH][x] = top;
 }
@@ -951,8 +1028,8 @@ __global__ __launch_bounds__(64 * KG) void k_ref_col(Dom d, Sc s, Fw f, RefAcc r
     group_sums(v);
     if (!on || ty != 0) return;
     r.ph[nz * st + c] += v[1];
-    r.mut[c] += v[0];
-    r.grf[c] += s.dtau * f.s[W_LM1][c];
+    put_r(r, r.mut, c, v[0]);
+    put_r(r, r.grf, c, s.dtau * f.s[W_LM1][c]);
 }
 
 // ---- c2a and alpha_t through the stage EOS (dry, hypsometric_opt = 1) ----
@@ -1007,7 +1084,7 @@ __global__ __launch_bounds__(256) void k_refg_dmdt(Dom d, Sc s, RefAcc r)
     if (fg.hasW)
         v -= rd * (c1 * (half * (mut_of(d, fg.cWm) + mut_of(d, fg.cW))) + c2)
              * (d.dnw[k] * r.grf[fg.cW]);
-    (isu ? r.u : r.v)[fg.out0 + k * fg.ostride] += v;
+    (isu ? r.u : r.v)[fg.out0 + k * fg.ostride] = v;
 }
 
 // ... and in the column mass (with the EOS's mu' column sum)
@@ -1129,7 +1206,8 @@ static int32_t substep_adjoint(const Dom* d, const arw_params* prm,
                                double dtau, int32_t first,
                                const ARW_API(acoustic_adj)* adj,
                                const RefAcc* ref,
-                               const ARW_API(acoustic_fwd)* fwd, real* work,
+                               const ARW_API(acoustic_fwd)* fwd,
+                               const real* lu, int acc, real* work,
                                void* stream)
 {
     int32_t rc = lib_check_domain(d);
@@ -1171,12 +1249,16 @@ static int32_t substep_adjoint(const Dom* d, const arw_params* prm,
     a.gp = adj->g_p_pp; a.gpo = adj->g_p_pp_old; a.gal = adj->g_al_pp;
     a.gru = adj->g_ru_t; a.grv = adj->g_rv_t; a.grw = adj->g_rw_t;
     a.grth = adj->g_rth_t; a.grph = adj->g_rph_t; a.grmu = adj->g_rmu_t;
+    a.acc = acc;
     const int cols = ny * nx;
     // the damper and dt follow the forward launcher's Sc
     const dim3 lev((cols + 255) / 256, nz + 1), colg((cols + 63) / 64);
     hipLaunchKernelGGL(k_adj_p1, lev, dim3(256), 0, st, *d, s, a, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_p1"))) return rc;
-    hipLaunchKernelGGL(k_adj_s1, colg, dim3(64), 0, st, *d, s, fw);
+    if (lu)
+        hipLaunchKernelGGL(k_adj_s1_lu, colg, dim3(64), 0, st, *d, fw, lu);
+    else
+        hipLaunchKernelGGL(k_adj_s1, colg, dim3(64), 0, st, *d, s, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_s1"))) return rc;
     hipLaunchKernelGGL(k_adj_p2, lev, dim3(256), 0, st, *d, s, a, fw);
     if ((rc = lib_launched(hipGetLastError(), "k_adj_p2"))) return rc;
@@ -1221,6 +1303,7 @@ static RefAcc ref_of(const ARW_API(acoustic_ref)* acc)
     RefAcc r;
     r.mut = acc->mut; r.c2a = acc->c2a; r.alt = acc->alt; r.thr = acc->thr;
     r.ph = acc->ph; r.w = acc->w; r.u = acc->u; r.v = acc->v; r.grf = acc->grf;
+    r.clear = 0;
     return r;
 }
 
@@ -1231,23 +1314,42 @@ int32_t ARW_API(acoustic_substep_adjoint)(const Dom* d, const arw_params* prm,
                                           const ARW_API(acoustic_adj)* adj,
                                           real* work, void* stream)
 {
-    return substep_adjoint(d, prm, dtau, first, adj, nullptr, nullptr, work,
-                           stream);
+    return substep_adjoint(d, prm, dtau, first, adj, nullptr, nullptr, nullptr,
+                           0, work, stream);
 }
 
 int32_t ARW_API(acoustic_substep_adjoint_ref)(
     const Dom* d, const arw_params* prm, double dtau, int32_t first,
     const ARW_API(acoustic_adj)* adj, const ARW_API(acoustic_ref)* acc,
-    const ARW_API(acoustic_fwd)* fwd, real* work, void* stream)
+    const ARW_API(acoustic_fwd)* fwd, const real* lu, int32_t flags,
+    real* work, void* stream)
 {
+    if (flags & ~(ARW_ADJ_CLEAR | ARW_ADJ_ACCUMULATE))
+        return lib_fail(ARW_E_ARG, "unknown adjoint flags");
     if (!ref_ok(acc)) return lib_fail(ARW_E_ARG, "null reference accumulator");
     if (fwd && !(fwd->u_pp && fwd->v_pp && fwd->w_pp && fwd->th_pp
                  && fwd->mu_pp && fwd->al_pp && fwd->ww_pp))
         return lib_fail(ARW_E_ARG, "null substep output");
     if (d && d->qv)
         return lib_fail(ARW_E_UNSUPPORTED, "reference adjoint: dry states only");
-    const RefAcc r = ref_of(acc);
-    return substep_adjoint(d, prm, dtau, first, adj, &r, fwd, work, stream);
+    RefAcc r = ref_of(acc);
+    r.clear = (flags & ARW_ADJ_CLEAR) != 0;
+    return substep_adjoint(d, prm, dtau, first, adj, &r, fwd, lu,
+                           (flags & ARW_ADJ_ACCUMULATE) != 0, work, stream);
+}
+
+int32_t ARW_API(acoustic_adjoint_factors)(const Dom* d, const arw_params* prm,
+                                          double dtau, real* lu, void* stream)
+{
+    int32_t rc = lib_check_domain(d);
+    if (rc) return rc;
+    if (!prm || !lu) return lib_fail(ARW_E_ARG, "null argument");
+    if ((rc = lib_check_prec(prm, d))) return rc;
+    const Sc s = make_sc(prm, dtau, 1);
+    const int cols = d->ny * d->nx;
+    hipLaunchKernelGGL(k_adj_lu, dim3((cols + 63) / 64), dim3(64), 0,
+                       (hipStream_t)stream, *d, s, lu);
+    return lib_launched(hipGetLastError(), "k_adj_lu");
 }
 
 int32_t ARW_API(acoustic_reference_gradients)(

@@ -25,6 +25,10 @@ extern "C" {
 
 #define ARW_ABI_VERSION 5
 
+/* flags of the adjoint entry points that accumulate */
+#define ARW_ADJ_CLEAR 1
+#define ARW_ADJ_ACCUMULATE 2
+
 #define ARW_OK 0
 #define ARW_E_ARG (-1)      /* invalid argument (null pointer, bad extent) */
 #define ARW_E_LAUNCH (-2)   /* a kernel launch or runtime call failed */
@@ -254,10 +258,17 @@ int32_t arw_acoustic_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
  * w: w levels; u, v: faces), which the caller clears at the stage start
  * and keeps over the stage's substeps.  fwd (optional): the substep's
  * outputs from d's small-step fields (u''..al'' and Omega'' after
- * arw_acoustic_substep), which the call then does not recompute.
- * Otherwise as arw_acoustic_substep_adjoint (same work size).  Dry states
- * only.  arw_acoustic_reference_gradients first adds dmdt_rf's share (grf
- * times the stage u, v and face masses) to acc->u, v and then maps the
+ * arw_acoustic_substep), which the call then does not recompute.  lu
+ * (optional): the stage's implicit-w Thomas factors from
+ * arw_acoustic_adjoint_factors (3 fields of (nz+1)*ny*nx; they depend on
+ * the stage state and dtau only), read instead of formed.  flags:
+ * ARW_ADJ_CLEAR -- this call writes the accumulators instead of adding
+ * (no clearing needed before a stage's first call); ARW_ADJ_ACCUMULATE
+ * -- the tendency adjoints g_r*_t are added to what they hold (a stage's
+ * substeps summed in place).  Otherwise as arw_acoustic_substep_adjoint
+ * (same work size).  Dry states only.  arw_acoustic_reference_gradients
+ * first writes dmdt_rf's share (grf times the stage u, v and face
+ * masses) to acc->u, v and then maps the
  * accumulated c2a / alt through the stage EOS (d->p, d->alt must hold
  * it): g_thp, g_php, g_mup are written, the u, v, w gradients are acc->u,
  * v, w themselves; work: 2 fields of (nz+1)(ny+1)(nx+1), e.g. the substep
@@ -277,7 +288,11 @@ int32_t arw_acoustic_substep_adjoint_ref(const arw_domain *d,
                                          const arw_acoustic_adj *adj,
                                          const arw_acoustic_ref *acc,
                                          const arw_acoustic_fwd *fwd,
+                                         const float *lu, int32_t flags,
                                          float *work, void *stream);
+int32_t arw_acoustic_adjoint_factors(const arw_domain *d,
+                                     const arw_params *prm, double dtau,
+                                     float *lu, void *stream);
 int32_t arw_acoustic_reference_gradients(const arw_domain *d,
                                          const arw_params *prm,
                                          const arw_acoustic_ref *acc,
@@ -584,7 +599,11 @@ int32_t arw64_acoustic_substep_adjoint_ref(const arw64_domain *d,
                                          const arw64_acoustic_adj *adj,
                                          const arw64_acoustic_ref *acc,
                                          const arw64_acoustic_fwd *fwd,
+                                         const double *lu, int32_t flags,
                                          double *work, void *stream);
+int32_t arw64_acoustic_adjoint_factors(const arw64_domain *d,
+                                     const arw_params *prm, double dtau,
+                                     double *lu, void *stream);
 int32_t arw64_acoustic_reference_gradients(const arw64_domain *d,
                                          const arw_params *prm,
                                          const arw64_acoustic_ref *acc,

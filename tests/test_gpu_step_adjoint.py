@@ -99,6 +99,44 @@ def test_given_substep_outputs_equal_recompute():
         assert torch.equal(a1[n], a2[n]), n
 
 
+def test_stage_factors_and_flags_equal_plain_calls():
+    """Two substep adjoints with the stage's precomputed Thomas factors,
+    the first writing (clear) the accumulators and the tendency adjoints
+    and the second adding to them, equal bit for bit the plain calls on
+    zeroed accumulators with the tendency adjoints summed afterwards."""
+    from arwen_amd import adjoint
+    o, coefs, dtau, cfg, rng, g = _device_pair(hill=300.0, damp=True,
+                                               top_lid=False)
+    print("substep adjoint flags and stage factors", flush=True)
+    dev = g.u.device
+    lams = [{n: torch.as_tensor(rng.standard_normal(getattr(o, n).shape),
+                                device=dev) for n in NAMES_OUT}
+            for _ in range(2)]
+    a_ref = adjoint.reference_accumulators(g)
+    t_ref = {n: torch.zeros_like(getattr(g, n)) for n in adjoint.ACOUSTIC_TEND}
+    outs_ref = []
+    for lam in lams:
+        out = adjoint.acoustic_substep_adjoint(g, cfg, dtau, False, lam, a_ref)
+        for n in adjoint.ACOUSTIC_TEND:
+            t_ref[n] += out[n]
+        outs_ref.append(out)
+    lu = adjoint.adjoint_factors(g, cfg, dtau)
+    a_new = {n: torch.full_like(t, float("nan")) for n, t in a_ref.items()}
+    t_new = {n: torch.full_like(t, float("nan")) for n, t in t_ref.items()}
+    for q, lam in enumerate(lams):
+        out = adjoint.acoustic_substep_adjoint(
+            g, cfg, dtau, False, lam, a_new, lu=lu, clear_ref=(q == 0),
+            tend_out=t_new, accumulate=(q > 0))
+        for n in adjoint.ACOUSTIC_STATE:
+            assert torch.equal(out[n], outs_ref[q][n]), (q, n)
+    for n in adjoint.ACOUSTIC_TEND:
+        assert torch.equal(t_new[n], t_ref[n]), n
+    for n in adjoint.ACOUSTIC_REF:
+        if n in ("u", "v"):          # written by reference_gradients only
+            continue
+        assert torch.equal(a_new[n], a_ref[n]), n
+
+
 def test_set_w_surface_adjoint_matches_oracle():
     from arwen_amd import adjoint
     o, coefs, dtau, cfg, rng, g = _device_pair(hill=300.0)
