@@ -96,7 +96,8 @@ STAGE_TEND = ("ru_t", "rv_t", "rw_t", "rth_t", "rph_t")
 class ArwStageAdj(ctypes.Structure):
     """arw_stage_adj (and arw64_stage_adj: same layout)."""
     _fields_ = ([("l_" + n, _P) for n in STAGE_TEND]
-                + [("g_" + n, _P) for n in STAGE_STATE])
+                + [("g_" + n, _P) for n in STAGE_STATE]
+                + [("flags", ctypes.c_int32)])
 
 
 ArwStageAdj64 = ArwStageAdj
@@ -111,7 +112,8 @@ class ArwInitAdj(ctypes.Structure):
     """arw_init_adj (and arw64_init_adj: same layout)."""
     _fields_ = ([("l_" + n, _P) for n in INIT_OUT]
                 + [("g_" + n + "0", _P) for n in STAGE_STATE]
-                + [("g_" + n, _P) for n in STAGE_STATE])
+                + [("g_" + n, _P) for n in STAGE_STATE]
+                + [("flags", ctypes.c_int32)])
 
 
 class ArwFinishAdj(ctypes.Structure):
@@ -125,7 +127,7 @@ ArwInitAdj64 = ArwInitAdj
 ArwFinishAdj64 = ArwFinishAdj
 
 #: accumulators of the substep adjoint's stage-reference share
-ACOUSTIC_REF = ("mut", "c2a", "alt", "thr", "ph", "w", "u", "v", "grf")
+ACOUSTIC_REF = ("mut", "c2a", "alt", "thr", "ph", "w", "grf")
 #: flags of the accumulating adjoint entry points
 ADJ_CLEAR = 1
 ADJ_ACCUMULATE = 2
@@ -201,7 +203,8 @@ def load():
         "arw_acoustic_adjoint_factors": ([dp, pp, _D, _P, _P], _I),
         "arw_acoustic_reference_gradients": ([dp, pp,
                                               ctypes.POINTER(ArwAcousticRef),
-                                              _P, _P, _P, _P, _P], _I),
+                                              _P, _P, _P, _P, _P, _P, _I,
+                                              _P, _P], _I),
         "arw_set_w_surface_adjoint": ([dp, pp, _P, _P, _P, _P], _I),
         "arw_acoustic_adjoint_work": ([_I, _I, _I,
                                        ctypes.POINTER(ctypes.c_int64)], _I),

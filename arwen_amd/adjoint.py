@@ -297,14 +297,13 @@ def _ref_shapes(state: DomainState) -> dict:
     nz, ny, nx = state.shape
     return {"mut": (ny, nx), "c2a": (nz, ny, nx), "alt": (nz, ny, nx),
             "thr": (nz, ny, nx), "ph": (nz + 1, ny, nx),
-            "w": (nz + 1, ny, nx), "u": (nz, ny, nx + 1),
-            "v": (nz, ny + 1, nx), "grf": (ny, nx)}
+            "w": (nz + 1, ny, nx), "grf": (ny, nx)}
 
 
 def reference_accumulators(state: DomainState) -> dict:
     """Zeroed accumulators of the substep adjoint's stage-reference
     share: mut, grf (ny,nx); c2a, alt, thr (mass points); ph, w (w
-    levels); u, v (faces)."""
+    levels)."""
     return {n: torch.zeros(shp, dtype=state.dtype, device=state.u.device)
             for n, shp in _ref_shapes(state).items()}
 
@@ -323,14 +322,31 @@ def _ref_struct(state, ref):
     return r
 
 
+def _out_like(state, out, names, accumulate):
+    """Output tensors for names: the caller's (to write or add to) or
+    fresh ones."""
+    if out is None:
+        if accumulate:
+            raise ValueError("accumulate needs out")
+        return {n: torch.empty_like(getattr(state, n)) for n in names}
+    res = {}
+    for n in names:
+        t = _as_like(out[n], getattr(state, n), n)
+        if t is not out[n]:
+            raise ValueError(f"out[{n}] must be contiguous")
+        res[n] = t
+    return res
+
+
 def reference_gradients(state: DomainState, cfg: RunConfig, ref: dict,
-                        views: dict | None = None) -> dict:
+                        views: dict | None = None, out: dict | None = None,
+                        accumulate: bool = False) -> dict:
     """The stage-state gradients (STAGE_STATE names) of the substep
-    adjoints accumulated in ``ref`` over one stage: dmdt_rf's share added
-    to ref's u, v, then c2a and alpha_t mapped through the stage EOS (the
+    adjoints accumulated in ``ref`` over one stage: dmdt_rf's share to u,
+    v, ref's w to w, and c2a and alpha_t through the stage EOS (the
     state's p and alt, or those of ``views``, must hold it, as after
-    ``dycore.stage_prepare``).  u, v, w are the accumulators
-    themselves."""
+    ``dycore.stage_prepare``) to thp, php, mup.  Written to ``out`` (or
+    fresh tensors), or added to ``out`` with ``accumulate``."""
     _check_state(state)
     lib = _lib.load()
     pre = "arw64_" if state.dtype == torch.float64 else "arw_"
@@ -338,16 +354,13 @@ def reference_gradients(state: DomainState, cfg: RunConfig, ref: dict,
     nz, ny, nx = state.shape
     work = state.scratch((2 * (nz + 1) * (ny + 1) * (nx + 1),),
                          "arw_reference_gradients_work")
-    out = {"u": ref["u"], "v": ref["v"], "w": ref["w"],
-           "thp": torch.empty_like(state.thp),
-           "php": torch.empty_like(state.php),
-           "mup": torch.empty_like(state.mup)}
+    o = _out_like(state, out, STAGE_STATE, accumulate)
     _lib.check(getattr(lib, pre + "acoustic_reference_gradients")(
         _dom_views(state, cfg, views), params(cfg), ctypes.byref(r),
-        out["thp"].data_ptr(),
-        out["php"].data_ptr(), out["mup"].data_ptr(), work.data_ptr(),
+        *[o[n].data_ptr() for n in STAGE_STATE],
+        _lib.ADJ_ACCUMULATE if accumulate else 0, work.data_ptr(),
         _stream()), pre + "acoustic_reference_gradients")
-    return out
+    return o
 
 
 def set_w_surface_adjoint(state: DomainState, cfg: RunConfig,
@@ -431,11 +444,13 @@ def stage_tendencies(state: DomainState, cfg: RunConfig) -> dict:
 
 
 def stage_tendencies_adjoint(state: DomainState, cfg: RunConfig,
-                             lam: dict) -> dict:
+                             lam: dict, out: dict | None = None,
+                             accumulate: bool = False) -> dict:
     """J^T lam for the tendency pass at the state's stage fields: ``lam``
     maps STAGE_TEND names to adjoints (missing = 0); returns the
-    gradients of STAGE_STATE.  Dry states without constant-K diffusion,
-    plain form.  The state is not modified."""
+    gradients of STAGE_STATE (written to ``out`` or fresh tensors, or
+    added to ``out`` with ``accumulate``).  Dry states without constant-K
+    diffusion, plain form.  The state is not modified."""
     _check_state(state)
     lib = _lib.load()
     pre = "arw64_" if state.dtype == torch.float64 else "arw_"
@@ -453,10 +468,10 @@ def stage_tendencies_adjoint(state: DomainState, cfg: RunConfig,
         t = _as_like(t, getattr(state, name), name)
         keep.append(t)
         setattr(a, "l_" + name, t.data_ptr())
-    out = {}
+    out = _out_like(state, out, STAGE_STATE, accumulate)
     for name in STAGE_STATE:
-        out[name] = torch.empty_like(getattr(state, name))
         setattr(a, "g_" + name, out[name].data_ptr())
+    a.flags = _lib.ADJ_ACCUMULATE if accumulate else 0
     fn = getattr(lib, pre + "stage_tendencies_adjoint")
     _lib.check(fn(state.domain_struct(), params(cfg), ctypes.byref(a),
                   work.data_ptr(), _stream()),
@@ -510,12 +525,16 @@ def _small_step_work(state):
     return lib, pre, state.scratch((n.value,), "arw_small_step_adjoint_work")
 
 
-def small_step_init_adjoint(state: DomainState, lam: dict) -> dict:
+def small_step_init_adjoint(state: DomainState, lam: dict,
+                            out: dict | None = None,
+                            accumulate: bool = False) -> dict:
     """J^T lam of the small-step initialisation (reference dycore.cu:
     947-1085) at the state's time-t fields (u0..mup0) and stage fields
     (u..mup): ``lam`` maps INIT_OUT names to adjoints (missing = 0);
     returns gradients named u0..mup0 and u..mup (the latter including the
-    stage EOS's share).  Dry, plain form; the state is not modified."""
+    stage EOS's share), written to ``out`` or fresh tensors, or added to
+    ``out`` with ``accumulate``.  Dry, plain form; the state is not
+    modified."""
     _check_state(state)
     lib, pre, work = _small_step_work(state)
     a = _lib.ArwInitAdj()
@@ -527,11 +546,20 @@ def small_step_init_adjoint(state: DomainState, lam: dict) -> dict:
         t = _as_like(t, getattr(state, name), name)
         keep.append(t)
         setattr(a, "l_" + name, t.data_ptr())
-    out = {}
-    for name in STAGE_STATE:
-        for n in (name + "0", name):
-            out[n] = torch.empty_like(getattr(state, name))
-            setattr(a, "g_" + n, out[n].data_ptr())
+    names = [n for name in STAGE_STATE for n in (name + "0", name)]
+    if out is None:
+        if accumulate:
+            raise ValueError("accumulate needs out")
+        out = {n: torch.empty_like(getattr(state, n.rstrip("0")))
+               for n in names}
+    else:
+        out = {n: _as_like(out[n], getattr(state, n.rstrip("0")), n)
+               if out[n].is_contiguous() else None for n in names}
+        if any(t is None for t in out.values()):
+            raise ValueError("out tensors must be contiguous")
+    for n in names:
+        setattr(a, "g_" + n, out[n].data_ptr())
+    a.flags = _lib.ADJ_ACCUMULATE if accumulate else 0
     _lib.check(getattr(lib, pre + "small_step_init_adjoint")(
         state.domain_struct(), ctypes.byref(a), work.data_ptr(), _stream()),
         pre + "small_step_init_adjoint")
@@ -715,12 +743,14 @@ def step_adjoint_taped(state: DomainState, tape: StepTape,
                 state, cfg, dtau, i == 0, lp, acc, fwd=fwd, views=views,
                 lu=lu, clear_ref=last, tend_out=lt, accumulate=not last)
             lp = {n: out[n] for n in ACOUSTIC_STATE}
-        ref = reference_gradients(state, cfg, acc, views=prep)
-        gi = small_step_init_adjoint(state, lp)
-        gt = stage_tendencies_adjoint(state, cfg, lt)
-        for n in names:
-            g0[n] += gi[n + "0"]
-            gs[n].add_(ref[n]).add_(gi[n]).add_(gt[n])
+        # every share adds into the stage's gradients gs (the finish's)
+        # and the time-t gradients g0
+        reference_gradients(state, cfg, acc, views=prep, out=gs,
+                            accumulate=True)
+        gio = dict(gs)
+        gio.update({n + "0": g0[n] for n in names})
+        small_step_init_adjoint(state, lp, out=gio, accumulate=True)
+        stage_tendencies_adjoint(state, cfg, lt, out=gs, accumulate=True)
         lx = gs
     for n in names:
         g0[n] += lx[n]

@@ -679,9 +679,20 @@ __global__ __launch_bounds__(64 * KG) void k_adj_cells(Dom d, Sc s, Adj a, Fw f)
 // terms as the mass-column kernels above, one thread per point it writes.
 
 struct RefAcc {
-    real *mut, *c2a, *alt, *thr, *ph, *w, *u, *v, *grf;
+    real *mut, *c2a, *alt, *thr, *ph, *w, *grf;
     int clear;               // this call writes the accumulators
 };
+
+// the stage-field gradients the accumulators map to
+struct RefOut {
+    real *gu, *gv, *gw, *gthp, *gphp, *gmup;
+    int acc;                 // add to them instead of writing
+};
+
+__device__ __forceinline__ void put_o(const RefOut& o, real* g, int x, real v)
+{
+    g[x] = (o.acc ? g[x] : R(0.0)) + v;
+}
 
 __device__ __forceinline__ void put_r(const RefAcc& r, real* g, int x, real v)
 {
@@ -1035,7 +1046,7 @@ __global__ __launch_bounds__(64 * KG) void k_ref_col(Dom d, Sc s, Fw f, RefAcc r
 // ---- c2a and alpha_t through the stage EOS (dry, hypsometric_opt = 1) ----
 enum { G_DPH, G_MU3, G_COUNT };
 
-__global__ void k_refg_eos(Dom d, RefAcc r, real* gthp, real* w)
+__global__ void k_refg_eos(Dom d, RefAcc r, RefOut o, real* w)
 {
     const int nz = d.nz, st = d.ny * d.nx;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1049,12 +1060,12 @@ __global__ void k_refg_eos(Dom d, RefAcc r, real* gthp, real* w)
     const real th = th_ref(d, k, c);
     const real mh = d.c1h[k] * (d.mub2d[c] + d.mup[c]) + d.c2h[k];
     const real ga = galt - gp * (GAMMA_F * p / alt);
-    gthp[t] = r.thr[t] + gp * (GAMMA_F * p / th);
+    put_o(o, o.gthp, t, r.thr[t] + gp * (GAMMA_F * p / th));
     w[G_DPH * F + t] = ga * (-d.rdnw[k] / mh);
     w[G_MU3 * F + t] = ga * (-alt * d.c1h[k] / mh);
 }
 
-__global__ void k_refg_php(Dom d, RefAcc r, real* gphp, const real* w)
+__global__ void k_refg_php(Dom d, RefAcc r, RefOut o, const real* w)
 {
     const int nz = d.nz, st = d.ny * d.nx;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1063,12 +1074,14 @@ __global__ void k_refg_php(Dom d, RefAcc r, real* gphp, const real* w)
     real g = r.ph[t];
     if (k >= 1) g += w[t - st];
     if (k <= nz - 1) g -= w[t];
-    gphp[t] = g;
+    put_o(o, o.gphp, t, g);
+    put_o(o, o.gw, t, r.w[t]);
 }
 
 // the stage's dmdt_rf = sum_k dnw (divergence of the coupled u, v): its
 // share for the substep-summed grf, on the faces (u, v) ...
-__global__ __launch_bounds__(256) void k_refg_dmdt(Dom d, Sc s, RefAcc r)
+__global__ __launch_bounds__(256) void k_refg_dmdt(Dom d, Sc s, RefAcc r,
+                                                   RefOut o)
 {
     FaceGeo fg, dg;
     bool has_dup, isu;
@@ -1084,12 +1097,12 @@ __global__ __launch_bounds__(256) void k_refg_dmdt(Dom d, Sc s, RefAcc r)
     if (fg.hasW)
         v -= rd * (c1 * (half * (mut_of(d, fg.cWm) + mut_of(d, fg.cW))) + c2)
              * (d.dnw[k] * r.grf[fg.cW]);
-    (isu ? r.u : r.v)[fg.out0 + k * fg.ostride] = v;
+    put_o(o, isu ? o.gu : o.gv, fg.out0 + k * fg.ostride, v);
 }
 
 // ... and in the column mass (with the EOS's mu' column sum)
 __global__ __launch_bounds__(64 * KG) void k_refg_mu(Dom d, Sc sc, RefAcc r,
-                                                    real* gmup, const real* w)
+                                                    RefOut o, const real* w)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
     const int c = blockIdx.x * 64 + threadIdx.x, ty = threadIdx.y;
@@ -1121,7 +1134,7 @@ __global__ __launch_bounds__(64 * KG) void k_refg_mu(Dom d, Sc sc, RefAcc r,
              + half * (-sc.rdx * G * v[1] - sc.rdx * r.grf[j * nx + ip] * v[4]);
     m += half * (sc.rdy * G * v[6] + sc.rdy * r.grf[jm * nx + i] * v[7])
          + half * (-sc.rdy * G * v[5] - sc.rdy * r.grf[jp * nx + i] * v[8]);
-    gmup[c] = (r.mut[c] + m) + v[0];
+    put_o(o, o.gmup, c, (r.mut[c] + m) + v[0]);
 }
 
 // ---- the kinematic lower boundary w(sfc) = u . grad(ht) ----------------
@@ -1295,14 +1308,14 @@ static int32_t substep_adjoint(const Dom* d, const arw_params* prm,
 static bool ref_ok(const ARW_API(acoustic_ref)* acc)
 {
     return acc && acc->mut && acc->c2a && acc->alt && acc->thr && acc->ph
-           && acc->w && acc->u && acc->v && acc->grf;
+           && acc->w && acc->grf;
 }
 
 static RefAcc ref_of(const ARW_API(acoustic_ref)* acc)
 {
     RefAcc r;
     r.mut = acc->mut; r.c2a = acc->c2a; r.alt = acc->alt; r.thr = acc->thr;
-    r.ph = acc->ph; r.w = acc->w; r.u = acc->u; r.v = acc->v; r.grf = acc->grf;
+    r.ph = acc->ph; r.w = acc->w; r.grf = acc->grf;
     r.clear = 0;
     return r;
 }
@@ -1354,30 +1367,36 @@ int32_t ARW_API(acoustic_adjoint_factors)(const Dom* d, const arw_params* prm,
 
 int32_t ARW_API(acoustic_reference_gradients)(
     const Dom* d, const arw_params* prm, const ARW_API(acoustic_ref)* acc,
-    real* g_thp, real* g_php, real* g_mup, real* work, void* stream)
+    real* g_u, real* g_v, real* g_w, real* g_thp, real* g_php, real* g_mup,
+    int32_t flags, real* work, void* stream)
 {
     int32_t rc = lib_check_domain(d);
     if (rc) return rc;
-    if (!prm || !ref_ok(acc) || !g_thp || !g_php || !g_mup || !work)
+    if (!prm || !ref_ok(acc) || !g_u || !g_v || !g_w || !g_thp || !g_php
+        || !g_mup || !work)
         return lib_fail(ARW_E_ARG, "null argument");
+    if (flags & ~ARW_ADJ_ACCUMULATE) return lib_fail(ARW_E_ARG, "unknown adjoint flags");
     if (d->qv) return lib_fail(ARW_E_UNSUPPORTED, "reference adjoint: dry states only");
     const RefAcc r = ref_of(acc);
+    RefOut o;
+    o.gu = g_u; o.gv = g_v; o.gw = g_w; o.gthp = g_thp; o.gphp = g_php;
+    o.gmup = g_mup; o.acc = (flags & ARW_ADJ_ACCUMULATE) != 0;
     hipStream_t st = (hipStream_t)stream;
     const int nz = d->nz, cols = d->ny * d->nx;
     const int nm = nz * cols, nw = (nz + 1) * cols;
     hipLaunchKernelGGL(k_refg_eos, dim3((nm + 255) / 256), dim3(256), 0, st,
-                       *d, r, g_thp, work);
+                       *d, r, o, work);
     if ((rc = lib_launched(hipGetLastError(), "k_refg_eos"))) return rc;
     hipLaunchKernelGGL(k_refg_php, dim3((nw + 255) / 256), dim3(256), 0, st,
-                       *d, r, g_php, work);
+                       *d, r, o, work);
     if ((rc = lib_launched(hipGetLastError(), "k_refg_php"))) return rc;
     const Sc sc = make_sc(prm, prm->dt, 1);
     const int nf = d->ny * (d->nx + 1) + (d->ny + 1) * d->nx;
     hipLaunchKernelGGL(k_refg_dmdt, dim3((nf + 255) / 256, nz), dim3(256), 0,
-                       st, *d, sc, r);
+                       st, *d, sc, r, o);
     if ((rc = lib_launched(hipGetLastError(), "k_refg_dmdt"))) return rc;
     hipLaunchKernelGGL(k_refg_mu, dim3((cols + 63) / 64), dim3(64, KG), 0, st,
-                       *d, sc, r, g_mup, work);
+                       *d, sc, r, o, work);
     return lib_launched(hipGetLastError(), "k_refg_mu");
 }
 

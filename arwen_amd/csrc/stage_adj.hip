@@ -47,6 +47,7 @@ struct SA {
     real* w[S_COUNT];
     real rdx, rdy, hrdx, hrdy, qrdx, qrdy;
     int top_lid;
+    int acc;                 // the outputs hold values to add to
 };
 
 __device__ __forceinline__ int wrapn(int i, int n)
@@ -485,7 +486,7 @@ __global__ void k_sa_mu(SA a)
     if (c >= st) return;
     real s = R(0.0);
     for (int k = 0; k <= nz; ++k) s += a.w[S_GMU3][k * st + c];
-    a.gmup[c] = s;
+    a.gmup[c] = (a.acc ? a.gmup[c] : R(0.0)) + s;
 }
 
 // ---------------------------------------------------------------------------
@@ -504,7 +505,14 @@ struct IA {
     real *gu0, *gv0, *gw0, *gthp0, *gphp0, *gmup0;
     real *gu, *gv, *gw, *gthp, *gphp, *gmup;
     real* w[I_COUNT];
+    int acc;                 // the outputs hold values to add to
 };
+
+// an init-adjoint output: written, or added to what it holds
+__device__ __forceinline__ void put_i(const IA& a, real* g, int x, real v)
+{
+    g[x] = (a.acc ? g[x] : R(0.0)) + v;
+}
 
 __device__ __forceinline__ real mu0_c(const Dom& d, int c)
 {
@@ -557,8 +565,8 @@ __global__ void k_ia_pts(IA a)
             g_cht = g_cht + g_thpp * tht;
             const real g_chs = -g_thpp * ths;
             g_ths = g_ths - g_thpp * chs;
-            a.gthp0[t] = g_thpp * cht;
-            a.gthp[t] = g_ths;
+            put_i(a, a.gthp0, t, g_thpp * cht);
+            put_i(a, a.gthp, t, g_ths);
             a.w[I_GP][t] = g_p;
             a.w[I_GALT][t] = g_alt;
             a.w[I_GDPH][t] = g_dph;
@@ -568,8 +576,8 @@ __global__ void k_ia_pts(IA a)
         }
         const int t = k * st + c;              // w level k
         const real lw = a.lw[t];
-        a.gw0[t] = (d.c1f[k] * mt + d.c2f[k]) * lw;
-        a.gw[t] = -((d.c1f[k] * ms + d.c2f[k]) * lw);
+        put_i(a, a.gw0, t, (d.c1f[k] * mt + d.c2f[k]) * lw);
+        put_i(a, a.gw, t, -((d.c1f[k] * ms + d.c2f[k]) * lw));
         a.w[I_MTW][t] = d.c1f[k] * d.w0[t] * lw;
         a.w[I_MSW][t] = -(d.c1f[k] * d.w[t] * lw);
     }
@@ -580,8 +588,8 @@ __global__ void k_ia_pts(IA a)
         const real mtf = R(0.5) * (mu0_c(d, ca) + mu0_c(d, cb));
         const real msf = R(0.5) * (mut_c(d, ca) + mut_c(d, cb));
         const real c1 = d.c1h[k], c2 = d.c2h[k], L = a.lu[x];
-        a.gu0[x] = (c1 * mtf + c2) * L;
-        a.gu[x] = -((c1 * msf + c2) * L);
+        put_i(a, a.gu0, x, (c1 * mtf + c2) * L);
+        put_i(a, a.gu, x, -((c1 * msf + c2) * L));
         a.w[I_PU0][x] = c1 * d.u0[x] * L;
         a.w[I_PU][x] = -(c1 * d.u[x] * L);
     }
@@ -592,8 +600,8 @@ __global__ void k_ia_pts(IA a)
         const real mtf = R(0.5) * (mu0_c(d, ca) + mu0_c(d, cb));
         const real msf = R(0.5) * (mut_c(d, ca) + mut_c(d, cb));
         const real c1 = d.c1h[k], c2 = d.c2h[k], L = a.lv[x];
-        a.gv0[x] = (c1 * mtf + c2) * L;
-        a.gv[x] = -((c1 * msf + c2) * L);
+        put_i(a, a.gv0, x, (c1 * mtf + c2) * L);
+        put_i(a, a.gv, x, -((c1 * msf + c2) * L));
         a.w[I_PV0][x] = c1 * d.v0[x] * L;
         a.w[I_PV][x] = -(c1 * d.v[x] * L);
     }
@@ -634,8 +642,8 @@ __global__ void k_ia_php(IA a)
         g -= a.w[I_GDPH][t];
         e -= a.w[I_GDPH2][t];
     }
-    a.gphp0[t] = g;
-    a.gphp[t] = -g + e;
+    put_i(a, a.gphp0, t, g);
+    put_i(a, a.gphp, t, -g + e);
 }
 
 // the face products of a column folded onto the periodic faces c, c+1
@@ -688,8 +696,8 @@ __global__ __launch_bounds__(64 * KG) void k_ia_cols(IA a)
     }
     group_sums(v);
     if (!on || ty != 0) return;
-    a.gmup0[c] = (a.lmu[c] + v[2]) + v[0];
-    a.gmup[c] = (v[1] - (a.lmu[c] + v[2])) + v[3];
+    put_i(a, a.gmup0, c, (a.lmu[c] + v[2]) + v[0]);
+    put_i(a, a.gmup, c, (v[1] - (a.lmu[c] + v[2])) + v[3]);
 }
 
 struct FA {
@@ -844,6 +852,9 @@ extern "C" int32_t ARW_API(small_step_init_adjoint)(
     a.gthp0 = adj->g_thp0; a.gphp0 = adj->g_php0; a.gmup0 = adj->g_mup0;
     a.gu = adj->g_u; a.gv = adj->g_v; a.gw = adj->g_w; a.gthp = adj->g_thp;
     a.gphp = adj->g_php; a.gmup = adj->g_mup;
+    if (adj->flags & ~ARW_ADJ_ACCUMULATE)
+        return lib_fail(ARW_E_ARG, "unknown adjoint flags");
+    a.acc = (adj->flags & ARW_ADJ_ACCUMULATE) != 0;
     a.d.p = a.w[I_P]; a.d.al = a.w[I_AL]; a.d.alt = a.w[I_ALT];
     if ((rc = lib_launched(run_eos(a.d, sm), "k_eos (init adjoint)"))) return rc;
     const dim3 blk(64, 4);
@@ -940,6 +951,9 @@ extern "C" int32_t ARW_API(stage_tendencies_adjoint)(
     a.hrdx = (real)(0.5 * rdx); a.hrdy = (real)(0.5 * rdy);
     a.qrdx = (real)(0.25 * rdx); a.qrdy = (real)(0.25 * rdy);
     a.top_lid = prm->top_lid != 0;
+    if (adj->flags & ~ARW_ADJ_ACCUMULATE)
+        return lib_fail(ARW_E_ARG, "unknown adjoint flags");
+    a.acc = (adj->flags & ARW_ADJ_ACCUMULATE) != 0;
     // forward recompute into the work area: EOS + Omega, ru, rv, theta
     Dom& dc = a.d;
     dc.p = a.w[S_P]; dc.al = a.w[S_AL]; dc.alt = a.w[S_ALT];
@@ -956,7 +970,7 @@ extern "C" int32_t ARW_API(stage_tendencies_adjoint)(
     // zeroed accumulators: the outputs and the flux gradients
     const size_t sz[] = {(size_t)nz * ny * (nx + 1), (size_t)nz * (ny + 1) * nx,
                          (size_t)nwl, (size_t)nm, (size_t)nwl, (size_t)st};
-    for (int q = 0; q < 6; ++q)
+    for (int q = 0; q < 6 && !a.acc; ++q)
         if (hipMemsetAsync(gout[q], 0, sz[q] * RB, sm) != hipSuccess)
             return lib_fail(ARW_E_LAUNCH, "stage adjoint: clearing outputs");
     for (int w : {S_GRU, S_GRV, S_GWW})

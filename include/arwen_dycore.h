@@ -255,8 +255,9 @@ int32_t arw_acoustic_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
  * the column mass mu_t, theta, the full phi, w (the damper) and u, v
  * (dmdt_rf): the call ADDS J^T lam with respect to those to the
  * accumulators of acc (mut, grf: (ny,nx); c2a, alt, thr: mass points; ph,
- * w: w levels; u, v: faces), which the caller clears at the stage start
- * and keeps over the stage's substeps.  fwd (optional): the substep's
+ * w: w levels), which the caller clears at the stage start (or lets the
+ * stage's first call write, ARW_ADJ_CLEAR) and keeps over the stage's
+ * substeps.  fwd (optional): the substep's
  * outputs from d's small-step fields (u''..al'' and Omega'' after
  * arw_acoustic_substep), which the call then does not recompute.  lu
  * (optional): the stage's implicit-w Thomas factors from
@@ -267,17 +268,17 @@ int32_t arw_acoustic_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
  * -- the tendency adjoints g_r*_t are added to what they hold (a stage's
  * substeps summed in place).  Otherwise as arw_acoustic_substep_adjoint
  * (same work size).  Dry states only.  arw_acoustic_reference_gradients
- * first writes dmdt_rf's share (grf times the stage u, v and face
- * masses) to acc->u, v and then maps the
- * accumulated c2a / alt through the stage EOS (d->p, d->alt must hold
- * it): g_thp, g_php, g_mup are written, the u, v, w gradients are acc->u,
- * v, w themselves; work: 2 fields of (nz+1)(ny+1)(nx+1), e.g. the substep
- * adjoint's work.
+ * maps the accumulators to the stage fields' gradients g_u .. g_mup:
+ * dmdt_rf's share (grf times the stage u, v and face masses) to g_u,
+ * g_v, acc->w to g_w, and c2a / alt through the stage EOS (d->p, d->alt
+ * must hold it) to g_thp, g_php, g_mup; written, or with
+ * ARW_ADJ_ACCUMULATE added to what they hold; work: 2 fields of
+ * (nz+1)(ny+1)(nx+1), e.g. the substep adjoint's work.
  * arw_set_w_surface_adjoint ADDS the (u, v) gradients of the kinematic
  * lower boundary w(sfc) = u . grad(ht) (replaces nothing: the transpose of
  * dycore.py:1999-2026) for the adjoint lam_w0 (ny,nx) of w(sfc). */
 typedef struct arw_acoustic_ref {
-    float *mut, *c2a, *alt, *thr, *ph, *w, *u, *v, *grf;
+    float *mut, *c2a, *alt, *thr, *ph, *w, *grf;
 } arw_acoustic_ref;
 typedef struct arw_acoustic_fwd {
     const float *u_pp, *v_pp, *w_pp, *th_pp, *mu_pp, *al_pp, *ww_pp;
@@ -296,8 +297,10 @@ int32_t arw_acoustic_adjoint_factors(const arw_domain *d,
 int32_t arw_acoustic_reference_gradients(const arw_domain *d,
                                          const arw_params *prm,
                                          const arw_acoustic_ref *acc,
+                                         float *g_u, float *g_v, float *g_w,
                                          float *g_thp, float *g_php, float *g_mup,
-                                         float *work, void *stream);
+                                         int32_t flags, float *work,
+                                         void *stream);
 int32_t arw_set_w_surface_adjoint(const arw_domain *d, const arw_params *prm,
                                   const float *lam_w0, float *g_u, float *g_v,
                                   void *stream);
@@ -350,6 +353,7 @@ int32_t arw_flux_div_w(const float *w, const float *ru, const float *rv,
 typedef struct arw_stage_adj {
     const float *l_ru_t, *l_rv_t, *l_rw_t, *l_rth_t, *l_rph_t;
     float *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+    int32_t flags;   /* ARW_ADJ_ACCUMULATE: add to g_* instead of writing */
 } arw_stage_adj;
 int32_t arw_stage_tendencies_adjoint(const arw_domain *d,
                                      const arw_params *prm,
@@ -371,6 +375,7 @@ typedef struct arw_init_adj {
         *l_al_pp, *l_p_pp, *l_p_pp_old;
     float *g_u0, *g_v0, *g_w0, *g_thp0, *g_php0, *g_mup0;
     float *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+    int32_t flags;   /* ARW_ADJ_ACCUMULATE: add to g_* instead of writing */
 } arw_init_adj;
 typedef struct arw_finish_adj {
     const float *l_u, *l_v, *l_w, *l_thp, *l_php, *l_mup;
@@ -588,7 +593,7 @@ int32_t arw64_acoustic_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
                                     int64_t *elements);
 
 typedef struct arw64_acoustic_ref {
-    double *mut, *c2a, *alt, *thr, *ph, *w, *u, *v, *grf;
+    double *mut, *c2a, *alt, *thr, *ph, *w, *grf;
 } arw64_acoustic_ref;
 typedef struct arw64_acoustic_fwd {
     const double *u_pp, *v_pp, *w_pp, *th_pp, *mu_pp, *al_pp, *ww_pp;
@@ -607,8 +612,10 @@ int32_t arw64_acoustic_adjoint_factors(const arw64_domain *d,
 int32_t arw64_acoustic_reference_gradients(const arw64_domain *d,
                                          const arw_params *prm,
                                          const arw64_acoustic_ref *acc,
+                                         double *g_u, double *g_v, double *g_w,
                                          double *g_thp, double *g_php, double *g_mup,
-                                         double *work, void *stream);
+                                         int32_t flags, double *work,
+                                         void *stream);
 int32_t arw64_set_w_surface_adjoint(const arw64_domain *d, const arw_params *prm,
                                   const double *lam_w0, double *g_u, double *g_v,
                                   void *stream);
@@ -616,6 +623,7 @@ int32_t arw64_set_w_surface_adjoint(const arw64_domain *d, const arw_params *prm
 typedef struct arw64_stage_adj {
     const double *l_ru_t, *l_rv_t, *l_rw_t, *l_rth_t, *l_rph_t;
     double *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+    int32_t flags;   /* ARW_ADJ_ACCUMULATE: add to g_* instead of writing */
 } arw64_stage_adj;
 int32_t arw64_stage_tendencies_adjoint(const arw64_domain *d,
                                        const arw_params *prm,
@@ -629,6 +637,7 @@ typedef struct arw64_init_adj {
         *l_al_pp, *l_p_pp, *l_p_pp_old;
     double *g_u0, *g_v0, *g_w0, *g_thp0, *g_php0, *g_mup0;
     double *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+    int32_t flags;   /* ARW_ADJ_ACCUMULATE: add to g_* instead of writing */
 } arw64_init_adj;
 typedef struct arw64_finish_adj {
     const double *l_u, *l_v, *l_w, *l_thp, *l_php, *l_mup;

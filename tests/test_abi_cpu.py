@@ -113,7 +113,12 @@ def _c_layout(struct, fields, tmp_path):
     ("arw_acoustic_adj", _lib.ArwAcousticAdj),
     ("arw64_acoustic_adj", _lib.ArwAcousticAdj64),
     ("arw_domain", _lib.ArwDomain), ("arw64_domain", _lib.ArwDomain64),
-    ("arw_params", _lib.ArwParams)])
+    ("arw_params", _lib.ArwParams),
+    ("arw_stage_adj", _lib.ArwStageAdj), ("arw_init_adj", _lib.ArwInitAdj),
+    ("arw_acoustic_ref", _lib.ArwAcousticRef),
+    ("arw64_acoustic_ref", _lib.ArwAcousticRef64),
+    ("arw_acoustic_fwd", _lib.ArwAcousticFwd),
+    ("arw64_acoustic_fwd", _lib.ArwAcousticFwd64)])
 def test_ctypes_offsets_match_the_c_compiler(cname, pyt, tmp_path):
     names = [n for n, _ in pyt._fields_]
     got = _c_layout(cname, names, tmp_path)
