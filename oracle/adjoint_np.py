@@ -575,6 +575,8 @@ def acoustic_substep_adjoint(s, coefs, dtau, c, first, lam, ref=False):
         out["ref"] = {"u": R["u"], "v": R["v"], "w": R["w"],
                       "thp": R["thr"] + gt, "php": R["ph"] + gph,
                       "mup": R["mut"] + gm}
+        if s.qv is not None:
+            out["ref"]["qv"] = eos_q_adjoint(s, gp)
     return out
 
 
@@ -864,6 +866,17 @@ def eos_adjoint(s, g_p, g_al, g_alt):
 
 
 GAMMA = D.GAMMA
+SPECIES = ("qv", "qc", "qr")
+
+
+def eos_q_adjoint(s, g_p):
+    """qv gradient of the moist EOS (calc_p_alpha with theta_m = theta
+    (1 + Rv/Rd qv), reference diagnostics.cu:71) given that of p: dp/dqv =
+    gamma p / theta_m * theta Rv/Rd.  The theta' gradient of the moist EOS
+    is eos_adjoint's gamma p / theta unchanged."""
+    th = D._b3(s.thb) + s.thp
+    thm = th * (1.0 + D.RVOVRD * s.qv)
+    return g_p * (GAMMA * s.p / thm) * th * D.RVOVRD
 
 
 def coupling_adjoint(s, g_ru, g_rv):
@@ -983,6 +996,42 @@ def pgf_adjoint(s, lam_u, lam_v, dx, dy, top_lid=True):
     return g
 
 
+def buoyancy_adjoint_moist(s, lam_w):
+    """(p, mup, q_total) gradients of slow_buoyancy with the moist loading
+    (reference dycore.cu:231-308, 289-300): rw_t += G ((cq1 rdn) dp' - c1f
+    mu' - cqw cq1 (c1f mub + c2f)) with cqw the face mean of qv + qc + qr
+    and cq1 = 1 / (1 + cqw); d(term)/d(cqw) = -cq1^2 (rdn dp' + c1f mub +
+    c2f)."""
+    nz = s.thp.shape[0]
+    pp = s.p - D._b3(s.pb)
+    qt = (s.qv + s.qc) + s.qr
+    c1f, c2f = s.c1f[:, None, None], s.c2f[:, None, None]
+    g_pp = np.zeros_like(s.p)
+    g_qt = np.zeros_like(s.p)
+    lw = lam_w[1:nz]
+    cqw = 0.5 * (qt[1:] + qt[:-1])
+    cq1 = 1.0 / (1.0 + cqw)
+    rdn = s.rdn[1:, None, None]
+    g_pp[1:] += D.G * cq1 * rdn * lw
+    g_pp[:-1] -= D.G * cq1 * rdn * lw
+    dp = pp[1:] - pp[:-1]
+    g_cqw = D.G * lw * (-cq1 * cq1) * (rdn * dp + (c1f[1:nz] * s.mub2d
+                                                   + c2f[1:nz]))
+    g_qt[1:] += 0.5 * g_cqw
+    g_qt[:-1] += 0.5 * g_cqw
+    g_mup = -D.G * np.sum(c1f[1:nz] * lw, axis=0)
+    lt = lam_w[nz]
+    cqw = 0.5 * (qt[nz - 1] + qt[nz - 2])
+    cq1 = 1.0 / (1.0 + cqw)
+    g_pp[nz - 1] -= D.G * cq1 * 2.0 * s.rdnw[nz - 1] * lt
+    g_cqw = D.G * lt * (-cq1 * cq1) * (2.0 * s.rdnw[nz - 1] * (0.0 - pp[nz - 1])
+                                       + (s.c1f[nz] * s.mub2d + s.c2f[nz]))
+    g_qt[nz - 1] += 0.5 * g_cqw
+    g_qt[nz - 2] += 0.5 * g_cqw
+    g_mup = g_mup - D.G * s.c1f[nz] * lt
+    return g_pp, g_mup, g_qt
+
+
 def buoyancy_adjoint(s, lam_w):
     """(p, mup) gradients of slow_buoyancy (dry; reference dycore.cu:
     231-308)."""
@@ -1059,10 +1108,13 @@ def geopotential_adjoint(s, ww, lam, dx, dy):
     return g
 
 
-def stage_tendencies_adjoint(s, lam, dx, dy, top_lid=True):
+def stage_tendencies_adjoint(s, lam, dx, dy, top_lid=True, lam_flux=None):
     """Transpose of stage_tendencies' Jacobian at the state s applied to
-    lam = {STAGE_OUT name: adjoint}: returns {STAGE_IN name: gradient}.
-    s must hold the stage state (its diagnostics are recomputed)."""
+    lam = {STAGE_OUT name: adjoint}: returns {STAGE_IN name: gradient}
+    (and qv, qc, qr for a moist state).  ``lam_flux``: adjoints of the
+    stage fluxes (ru, rv, ww) the stage also hands on (the moist scalars'
+    sumflux).  s must hold the stage state (its diagnostics are
+    recomputed)."""
     work = D.OState(**{k: (v.copy() if isinstance(v, np.ndarray) else v)
                        for k, v in vars(s).items()})
     _, (ru, rv, ww) = stage_tendencies(work, dx, dy, top_lid)
@@ -1071,6 +1123,11 @@ def stage_tendencies_adjoint(s, lam, dx, dy, top_lid=True):
     dxi, dyi = 1.0 / dx, 1.0 / dy
     g = {n: np.zeros_like(getattr(s, n)) for n in STAGE_IN}
     g_ru, g_rv, g_ww = np.zeros_like(ru), np.zeros_like(rv), np.zeros_like(ww)
+    if lam_flux is not None:
+        g_ru += lam_flux["ru"]
+        g_rv += lam_flux["rv"]
+        g_ww += lam_flux["ww"]
+    moist = work.qv is not None
     theta = D._b3(work.thb) + work.thp
     for op, q, l, rdz, name in (("scalar", theta, l_rth, work.rdnw, "thp"),
                                 ("u", work.u, l_ru, work.rdnw, "u"),
@@ -1083,7 +1140,10 @@ def stage_tendencies_adjoint(s, lam, dx, dy, top_lid=True):
         g_rv += a["rv"]
         g_ww += a["rw"]
     gp = pgf_adjoint(work, l_ru, l_rv, dx, dy, top_lid)
-    g_pp, g_mup_b = buoyancy_adjoint(work, l_rw)
+    if moist:
+        g_pp, g_mup_b, g_qt = buoyancy_adjoint_moist(work, l_rw)
+    else:
+        g_pp, g_mup_b = buoyancy_adjoint(work, l_rw)
     gg = geopotential_adjoint(work, ww, l_rph, dx, dy)
     g_ww += gg["ww"]
     for n in ("w", "u", "v", "php", "mup"):
@@ -1102,6 +1162,10 @@ def stage_tendencies_adjoint(s, lam, dx, dy, top_lid=True):
     g["thp"] += gt
     g["php"] += gph
     g["mup"] += gm
+    if moist:
+        g["qv"] = eos_q_adjoint(work, g_p) + g_qt
+        g["qc"] = g_qt.copy()
+        g["qr"] = g_qt.copy()
     return g
 
 
@@ -1213,6 +1277,8 @@ def init_adjoint(s, lam):
     g["thp"] += gt
     g["php"] += gph
     g["mup"] += gm
+    if w.qv is not None:
+        g["qv"] = eos_q_adjoint(w, g_p)
     return g
 
 
@@ -1302,22 +1368,66 @@ def _copy_state(s):
                        for k, v in vars(s).items()})
 
 
+def scalar_update_adjoint(w, q, q0, fl, mu0, mu, dt_stage, clamp, lam, dx,
+                          dy):
+    """Transpose of one species' stage update (moist_np.advance_scalars_
+    stage without the PD limiter: q_new = (C(mu0) q0 + dt flux_div_scalar
+    (q; ru_m, rv_m, ww_m)) / C(mu), clamped at zero on the final stage;
+    reference moist.py:385-398) for lam = adjoint of q_new.  Returns the
+    gradients of q (the stage scalar), q0, the three averaged fluxes and
+    the column masses mu0, mu (2-D).  The clamp is differentiated as
+    1 where the update is positive, 0 elsewhere."""
+    f = q.dtype.type
+    ru_m, rv_m, ww_m = fl
+    dxi, dyi = float(f(1.0 / dx)), float(f(1.0 / dy))
+    c1, c2 = w.c1h[:, None, None], w.c2h[:, None, None]
+    chm0 = c1 * mu0[None] + c2
+    chm = c1 * mu[None] + c2
+    tend = f(0.0) + D.flux_div_scalar(q, ru_m, rv_m, ww_m, w.rdnw, w.fnm,
+                                      w.fnp, dxi, dyi)
+    v = (chm0 * q0 + f(dt_stage) * tend) / chm
+    lv = np.where(np.real(v) > 0.0, lam, 0.0) if clamp else lam
+    g_tend = f(dt_stage) * lv / chm
+    a = flux_div_adjoint("scalar", g_tend, q, ru_m, rv_m, ww_m, w.rdnw,
+                         w.fnm, w.fnp, dxi, dyi)
+    return {"q": a["q"], "q0": chm0 * lv / chm,
+            "ru": a["ru"], "rv": a["rv"], "ww": a["rw"],
+            "mu0": np.sum(c1 * q0 * lv / chm, axis=0),
+            "mu": -np.sum(c1 * v * lv / chm, axis=0)}
+
+
 def step_adjoint(s, cfg, lam):
     """J^T lam for one dycore step (D.step) at the state s: ``lam`` maps
-    the new state's STAGE_IN fields to adjoints; returns the gradients of
-    the step's input STAGE_IN fields.  Dry, plain form, no diffusion.
-    The forward is re-run on a copy, checkpointing each stage's state and
-    each substep's input; s is not modified."""
+    the new state's STAGE_IN fields (and qv, qc, qr on a moist state) to
+    adjoints; returns the gradients of the step's input fields.  Plain
+    form, no diffusion; a moist state's scalars without the PD limiter
+    (moist_adv_opt = 0).  The forward is re-run on a copy, checkpointing
+    each stage's state and each substep's input; s is not modified.
+
+    Moist: the scalars' stage update runs after the finish with the
+    substep-averaged fluxes (sumflux: the mean of the substeps' u'', v'',
+    Omega'' plus the stage fluxes), so its transpose comes first in the
+    stage's reverse: it adds to the adjoint of the finish's new mu', to
+    every substep's output u'', v'', Omega'' (1/nsub each) and to the stage
+    fluxes, which the tendency-pass adjoint takes with the tendencies; the
+    moist EOS and buoyancy loading add the stage scalars' share."""
     diffusion = cfg.km_opt == 1 and (cfg.khdif > 0.0 or cfg.kvdif > 0.0)
-    if s.qv is not None or diffusion or getattr(cfg, "precision_form",
-                                                 False):
-        raise NotImplementedError("step adjoint: dry, plain, no diffusion")
+    moist = s.qv is not None
+    if diffusion or getattr(cfg, "precision_form", False):
+        raise NotImplementedError("step adjoint: plain form, no diffusion")
+    if moist and getattr(cfg, "moist_adv_opt", 1) != 0:
+        raise NotImplementedError("step adjoint: scalars without the PD "
+                                  "limiter (moist_adv_opt = 0)")
+    names = STAGE_IN + (SPECIES if moist else ())
     ns = cfg.time_step_sound
     stages = ((1, cfg.dt / 3.0), (max(ns // 2, 1), cfg.dt / ns),
               (ns, cfg.dt / ns))
     w = _copy_state(s)
     for n in D.PROGNOSTICS:
         getattr(w, n + "0")[...] = getattr(w, n)
+    if moist:
+        for q in SPECIES:
+            getattr(w, q + "0")[...] = getattr(w, q)
     ck = []                                   # per stage: state, P_i list
     for istage, (nsub, dtau) in enumerate(stages):
         for n in STAGE_OUT + ("rmu_t",):
@@ -1328,44 +1438,88 @@ def step_adjoint(s, cfg, lam):
         D.small_step_init(w)
         coefs = D.calc_coefs(w, dtau, cfg.epssm, cfg.top_lid)
         subs = []
+        ex = None
+        if moist:
+            qst = {q: getattr(w, q).copy() for q in SPECIES}
+            sm = [np.zeros_like(w.u), np.zeros_like(w.v), np.zeros_like(w.w)]
         for i in range(nsub):
             subs.append(_copy_state(w))       # substep i's input
             D.acoustic_substep(w, coefs, dtau, cfg, first=(i == 0))
-        ck.append((_copy_state(w), subs, coefs, dtau, nsub))
+            if moist:
+                sm[0] += w.u_pp
+                sm[1] += w.v_pp
+                sm[2] += w.ww_pp
+        after = _copy_state(w)
         D.small_step_finish(w)
+        if moist:
+            fl = D.sumflux_finish(sm[0], sm[1], sm[2], ru, rv, ww, nsub)
+            ex = {"q": qst, "fl": fl, "mu": w.mub2d + w.mup,
+                  "final": istage == 2, "dt": nsub * dtau}
+            D.advance_scalars_stage(w, fl[0], fl[1], fl[2], nsub * dtau,
+                                    istage == 2, cfg.dx, cfg.dy, 0)
+        ck.append((after, subs, coefs, dtau, nsub, ex))
     # reverse
-    g0 = {n: np.zeros_like(getattr(s, n)) for n in STAGE_IN}   # time-t
+    g0 = {n: np.zeros_like(getattr(s, n)) for n in names}   # time-t
     lx = {n: np.array(lam.get(n, np.zeros_like(getattr(s, n))), copy=True)
-          for n in STAGE_IN}
+          for n in names}
     gu, gv = set_w_surface_adjoint(s, lx["w"][0], cfg.dx, cfg.dy)
     lx["u"] += gu
     lx["v"] += gv
     lx["w"][0] = 0.0
+    mu0 = s.mub2d + s.mup
     for istage in range(2, -1, -1):
-        after_subs, subs, coefs, dtau, nsub = ck[istage]
+        after_subs, subs, coefs, dtau, nsub, ex = ck[istage]
+        lam_flux = None
+        inj = None
+        gq = {}
+        if moist:
+            lam_flux = {"ru": 0.0, "rv": 0.0, "ww": 0.0}
+            g_mu = 0.0
+            for q in SPECIES:
+                r = scalar_update_adjoint(after_subs, ex["q"][q],
+                                          getattr(s, q), ex["fl"], mu0,
+                                          ex["mu"], ex["dt"], ex["final"],
+                                          lx[q], cfg.dx, cfg.dy)
+                gq[q] = r["q"]
+                g0[q] += r["q0"]
+                g0["mup"] += r["mu0"]
+                g_mu = g_mu + r["mu"]
+                for k in ("ru", "rv", "ww"):
+                    lam_flux[k] = lam_flux[k] + r[k]
+            lx["mup"] = lx["mup"] + g_mu
+            fn = s.dtype.type(nsub)
+            inj = (lam_flux["ru"] / fn, lam_flux["rv"] / fn,
+                   lam_flux["ww"] / fn)
         # the stage state Xs is the finish's input: after_subs holds it
         fin = finish_adjoint(after_subs, lx)
         gs = {n: fin[n] for n in STAGE_IN}
+        gs.update(gq)
         lp = {n: fin[n] for n in FINISH_IN}
         lt = {n: np.zeros_like(getattr(s, n)) for n in SUBSTEP_TEND}
         for i in range(nsub - 1, -1, -1):
             st = subs[i]
+            if moist:
+                lp["u_pp"] = lp["u_pp"] + inj[0]
+                lp["v_pp"] = lp["v_pp"] + inj[1]
+                lp["ww_pp"] = inj[2]
             out = acoustic_substep_adjoint(st, coefs, dtau, cfg, i == 0, lp,
                                            ref=True)
             lp = {n: out[n] for n in SUBSTEP_STATE}
             for n in SUBSTEP_TEND:
                 lt[n] += out[n]
-            for n in STAGE_IN:
-                gs[n] += out["ref"][n]
+            for n, v in out["ref"].items():
+                gs[n] = gs[n] + v
         gi = init_adjoint(subs[0], lp)
         for n in STAGE_IN:
             g0[n] += gi[n + "0"]
             gs[n] += gi[n]
+        if moist:
+            gs["qv"] = gs["qv"] + gi["qv"]
         gt = stage_tendencies_adjoint(subs[0], {n: lt[n] for n in STAGE_OUT},
-                                      cfg.dx, cfg.dy, cfg.top_lid)
-        for n in STAGE_IN:
-            gs[n] += gt[n]
+                                      cfg.dx, cfg.dy, cfg.top_lid, lam_flux)
+        for n, v in gt.items():
+            gs[n] = gs[n] + v
         lx = gs                                # the previous stage's output
-    for n in STAGE_IN:
+    for n in names:
         g0[n] += lx[n]
     return g0

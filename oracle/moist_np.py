@@ -152,7 +152,8 @@ def advance_scalars_stage(s, ru_m, rv_m, ww_m, dt_stage, final, dx, dy,
     f = s.dtype.type
     mu0 = s.mub2d + s.mup0
     mu = s.mub2d + s.mup
-    dx_inv, dy_inv = float(f(1.0 / dx)), float(f(1.0 / dy))
+    dx_inv = float(np.real(f(1.0 / dx)))
+    dy_inv = float(np.real(f(1.0 / dy)))
     pd = final and moist_adv_opt == 1
     for name in SPECIES:
         q, q0 = getattr(s, name), getattr(s, name + "0")
