@@ -97,7 +97,9 @@ class ArwStageAdj(ctypes.Structure):
     """arw_stage_adj (and arw64_stage_adj: same layout)."""
     _fields_ = ([("l_" + n, _P) for n in STAGE_TEND]
                 + [("g_" + n, _P) for n in STAGE_STATE]
-                + [("flags", ctypes.c_int32)])
+                + [("g_qv", _P), ("g_qc", _P), ("g_qr", _P),
+                   ("l_ru", _P), ("l_rv", _P), ("l_ww", _P),
+                   ("flags", ctypes.c_int32)])
 
 
 ArwStageAdj64 = ArwStageAdj
@@ -113,7 +115,7 @@ class ArwInitAdj(ctypes.Structure):
     _fields_ = ([("l_" + n, _P) for n in INIT_OUT]
                 + [("g_" + n + "0", _P) for n in STAGE_STATE]
                 + [("g_" + n, _P) for n in STAGE_STATE]
-                + [("flags", ctypes.c_int32)])
+                + [("g_qv", _P), ("flags", ctypes.c_int32)])
 
 
 class ArwFinishAdj(ctypes.Structure):
@@ -203,9 +205,11 @@ def load():
         "arw_acoustic_adjoint_factors": ([dp, pp, _D, _P, _P], _I),
         "arw_acoustic_reference_gradients": ([dp, pp,
                                               ctypes.POINTER(ArwAcousticRef),
-                                              _P, _P, _P, _P, _P, _P, _I,
+                                              _P, _P, _P, _P, _P, _P, _P, _I,
                                               _P, _P], _I),
         "arw_set_w_surface_adjoint": ([dp, pp, _P, _P, _P, _P], _I),
+        "arw_scalar_stage": ([dp, pp, _D, _I] + [_P] * 9, _I),
+        "arw_scalar_stage_adjoint": ([dp, pp, _D, _I] + [_P] * 17, _I),
         "arw_acoustic_adjoint_work": ([_I, _I, _I,
                                        ctypes.POINTER(ctypes.c_int64)], _I),
         "arw_stage_tendencies_adjoint": ([dp, pp, ctypes.POINTER(ArwStageAdj),
@@ -283,7 +287,8 @@ FP64_ENTRY_POINTS = (
     "arw64_small_step_finish_adjoint", "arw64_small_step_adjoint_work",
     "arw64_acoustic_substep_adjoint_ref", "arw64_acoustic_adjoint_factors",
     "arw64_acoustic_reference_gradients",
-    "arw64_set_w_surface_adjoint") + tuple(
+    "arw64_set_w_surface_adjoint", "arw64_scalar_stage",
+    "arw64_scalar_stage_adjoint") + tuple(
         "arw64_" + n for n in FLUX_DIV_ADJOINTS)
 
 
@@ -307,7 +312,8 @@ def exported_symbols() -> list[str]:
             "arw_acoustic_substep_adjoint_ref",
             "arw_acoustic_adjoint_factors",
             "arw_acoustic_reference_gradients",
-            "arw_set_w_surface_adjoint"] + [
+            "arw_set_w_surface_adjoint", "arw_scalar_stage",
+            "arw_scalar_stage_adjoint"] + [
             "arw_" + n for n in FLUX_DIV_ADJOINTS]
 
 

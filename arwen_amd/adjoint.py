@@ -354,10 +354,12 @@ def reference_gradients(state: DomainState, cfg: RunConfig, ref: dict,
     nz, ny, nx = state.shape
     work = state.scratch((2 * (nz + 1) * (ny + 1) * (nx + 1),),
                          "arw_reference_gradients_work")
-    o = _out_like(state, out, STAGE_STATE, accumulate)
+    o = _out_like(state, out, STAGE_STATE + (("qv",) if _moist(state)
+                                            else ()), accumulate)
     _lib.check(getattr(lib, pre + "acoustic_reference_gradients")(
         _dom_views(state, cfg, views), params(cfg), ctypes.byref(r),
         *[o[n].data_ptr() for n in STAGE_STATE],
+        o["qv"].data_ptr() if _moist(state) else None,
         _lib.ADJ_ACCUMULATE if accumulate else 0, work.data_ptr(),
         _stream()), pre + "acoustic_reference_gradients")
     return o
@@ -443,14 +445,25 @@ def stage_tendencies(state: DomainState, cfg: RunConfig) -> dict:
     return {n: getattr(state, n).clone() for n in STAGE_TEND}
 
 
+SPECIES = ("qv", "qc", "qr")
+
+
+def _moist(state):
+    return state.qv is not None
+
+
 def stage_tendencies_adjoint(state: DomainState, cfg: RunConfig,
                              lam: dict, out: dict | None = None,
-                             accumulate: bool = False) -> dict:
+                             accumulate: bool = False,
+                             lam_flux: dict | None = None) -> dict:
     """J^T lam for the tendency pass at the state's stage fields: ``lam``
     maps STAGE_TEND names to adjoints (missing = 0); returns the
-    gradients of STAGE_STATE (written to ``out`` or fresh tensors, or
-    added to ``out`` with ``accumulate``).  Dry states without constant-K
-    diffusion, plain form.  The state is not modified."""
+    gradients of STAGE_STATE, and of qv, qc, qr on a moist state (the
+    moist EOS and buoyancy loading), written to ``out`` or fresh tensors,
+    or added to ``out`` with ``accumulate``.  ``lam_flux``: adjoints of
+    the stage fluxes ru, rv, ww handed on besides the tendencies (the
+    scalars' sumflux).  No constant-K diffusion, plain form.  The state
+    is not modified."""
     _check_state(state)
     lib = _lib.load()
     pre = "arw64_" if state.dtype == torch.float64 else "arw_"
@@ -468,9 +481,21 @@ def stage_tendencies_adjoint(state: DomainState, cfg: RunConfig,
         t = _as_like(t, getattr(state, name), name)
         keep.append(t)
         setattr(a, "l_" + name, t.data_ptr())
-    out = _out_like(state, out, STAGE_STATE, accumulate)
-    for name in STAGE_STATE:
+    names = STAGE_STATE + (SPECIES if _moist(state) else ())
+    out = _out_like(state, out, names, accumulate)
+    for name in names:
         setattr(a, "g_" + name, out[name].data_ptr())
+    if lam_flux is not None:
+        nz, ny, nx = state.shape
+        want = {"ru": (nz, ny, nx + 1), "rv": (nz, ny + 1, nx),
+                "ww": (nz + 1, ny, nx)}
+        for n, shp in want.items():
+            t = lam_flux[n]
+            if tuple(t.shape) != shp or t.dtype != state.dtype \
+                    or not t.is_contiguous():
+                raise ValueError(f"lam_flux[{n}]: expected contiguous {shp}")
+            keep.append(t)
+            setattr(a, "l_" + n, t.data_ptr())
     a.flags = _lib.ADJ_ACCUMULATE if accumulate else 0
     fn = getattr(lib, pre + "stage_tendencies_adjoint")
     _lib.check(fn(state.domain_struct(), params(cfg), ctypes.byref(a),
@@ -547,6 +572,8 @@ def small_step_init_adjoint(state: DomainState, lam: dict,
         keep.append(t)
         setattr(a, "l_" + name, t.data_ptr())
     names = [n for name in STAGE_STATE for n in (name + "0", name)]
+    if _moist(state):
+        names.append("qv")
     if out is None:
         if accumulate:
             raise ValueError("accumulate needs out")
@@ -630,20 +657,30 @@ class StepTape:
 
 
 def _check_adjoint_cfg(state, cfg):
-    if state.qv is not None or _uses_diffusion(cfg) \
-            or getattr(cfg, "precision_form", False):
-        raise _lib.ArwenError("step adjoint: dry states, no constant-K "
-                              "diffusion, plain form")
+    if _uses_diffusion(cfg) or getattr(cfg, "precision_form", False):
+        raise _lib.ArwenError("step adjoint: no constant-K diffusion, "
+                              "plain form")
+    if _moist(state) and getattr(cfg, "moist_adv_opt", 1) != 0:
+        raise _lib.ArwenError("step adjoint: moist states need the scalars "
+                              "without the PD limiter (moist_adv_opt = 0)")
 
 
 def _run_stage(state, cfg, nsub, dtau, record):
     """One stage's preparation and substeps from the state's stage and
     time-t fields; with ``record``, returns what the stage's adjoint reads
-    (the preparation's outputs and every substep's small-step fields)."""
+    (the preparation's outputs and every substep's small-step fields).  On
+    a moist state also the stage fluxes and the substep sums of u'', v'',
+    Omega'' (sumflux)."""
     from . import dycore as dyn
     nz, ny, nx = state.shape
     dyn.stage_prepare(state, cfg, dtau, True)
     prep = subs = None
+    flx = None
+    if _moist(state):
+        ru, rv, ww = dyn.stage_fluxes(state, cfg)
+        flx = {"ru": ru.clone(), "rv": rv.clone(), "ww": ww.clone(),
+               "su": torch.zeros_like(state.u), "sv": torch.zeros_like(state.v),
+               "sw": torch.zeros_like(state.w)}
     if record:
         prep = {n: getattr(state, n).clone() for n in _PREP}
         prep["ww"] = state.scratch((nz + 1, ny, nx), "arw_ww").clone()
@@ -651,11 +688,49 @@ def _run_stage(state, cfg, nsub, dtau, record):
         subs = [{n: getattr(state, n).clone() for n in ACOUSTIC_STATE}]
     for i in range(nsub):
         dyn.acoustic_substep(state, cfg, dtau, first=(i == 0))
+        if flx is not None:
+            flx["su"] += state.u_pp
+            flx["sv"] += state.v_pp
+            flx["sw"] += state.ww_pp
         if record:
             x = {n: getattr(state, n).clone() for n in ACOUSTIC_STATE}
             x["ww_pp"] = state.ww_pp.clone()
             subs.append(x)
-    return prep, subs
+    return prep, subs, flx
+
+
+def scalar_stage(state: DomainState, cfg: RunConfig, dt_stage: float,
+                 clamp: bool, q, q0, fl, mup_t, mup_new) -> torch.Tensor:
+    """One stage of a scalar's unlimited update with explicit fluxes fl =
+    (ru, rv, ww) and masses (``arw_scalar_stage``)."""
+    lib = _lib.load()
+    pre = "arw64_" if state.dtype == torch.float64 else "arw_"
+    q_new = torch.empty_like(q)
+    _lib.check(getattr(lib, pre + "scalar_stage")(
+        state.domain_struct(), params(cfg), float(dt_stage), int(bool(clamp)),
+        q.data_ptr(), q0.data_ptr(), *[t.data_ptr() for t in fl],
+        mup_t.data_ptr(), mup_new.data_ptr(), q_new.data_ptr(), _stream()),
+        pre + "scalar_stage")
+    return q_new
+
+
+def scalar_stage_adjoint(state: DomainState, cfg: RunConfig,
+                         dt_stage: float, clamp: bool, q, q0, fl, mup_t,
+                         mup_new, lam, g_q0, g_fl, g_mup_t, g_mup_new):
+    """Transpose of ``scalar_stage`` for lam: returns the stage scalar's
+    gradient; adds to g_q0, g_fl (ru, rv, ww), g_mup_t, g_mup_new."""
+    lib = _lib.load()
+    pre = "arw64_" if state.dtype == torch.float64 else "arw_"
+    g_q = torch.empty_like(q)
+    work = state.scratch((3 * q.numel(),), "arw_scalar_adjoint_work")
+    _lib.check(getattr(lib, pre + "scalar_stage_adjoint")(
+        state.domain_struct(), params(cfg), float(dt_stage), int(bool(clamp)),
+        q.data_ptr(), q0.data_ptr(), *[t.data_ptr() for t in fl],
+        mup_t.data_ptr(), mup_new.data_ptr(), lam.data_ptr(), g_q.data_ptr(),
+        g_q0.data_ptr(), *[t.data_ptr() for t in g_fl], g_mup_t.data_ptr(),
+        g_mup_new.data_ptr(), work.data_ptr(), _stream()),
+        pre + "scalar_stage_adjoint")
+    return g_q
 
 
 def step_taped(state: DomainState, cfg: RunConfig,
@@ -674,18 +749,39 @@ def step_taped(state: DomainState, cfg: RunConfig,
     if keep not in ("substeps", "stages"):
         raise ValueError("keep must be 'substeps' or 'stages'")
     tape = StepTape(cfg, keep)
-    for n in STAGE_STATE:
+    names = _step_fields(state)
+    for n in names:
         getattr(state, n + "0").copy_(getattr(state, n))
-    for nsub, dtau in _step_stages(cfg):
-        xs = {n: getattr(state, n).clone() for n in STAGE_STATE}
-        prep, subs = _run_stage(state, cfg, nsub, dtau, keep == "substeps")
-        tape.stages.append((xs, prep, subs, nsub, dtau))
+    for istage, (nsub, dtau) in enumerate(_step_stages(cfg)):
+        xs = {n: getattr(state, n).clone() for n in names}
+        prep, subs, flx = _run_stage(state, cfg, nsub, dtau,
+                                     keep == "substeps")
         dyn.small_step_finish(state)
+        ex = None
+        if flx is not None:
+            # sumflux, then the scalars' stage update from the stage's
+            # scalars and the time-t ones
+            fl = (flx["su"] / float(nsub) + flx["ru"],
+                  flx["sv"] / float(nsub) + flx["rv"],
+                  flx["sw"] / float(nsub) + flx["ww"])
+            ex = {"fl": fl, "mu_new": state.mup.clone(),
+                  "dt": nsub * dtau, "final": istage == 2}
+            for q in SPECIES:
+                q_new = scalar_stage(state, cfg, ex["dt"], ex["final"],
+                                     getattr(state, q),
+                                     getattr(state, q + "0"), fl,
+                                     state.mup0, state.mup)
+                getattr(state, q).copy_(q_new)
+        tape.stages.append((xs, prep, subs, nsub, dtau, ex))
     dyn.set_w_surface(state, cfg)
     dyn.update_diagnostics(state)
     state.elapsed_seconds += cfg.dt
-    tape.out = {n: getattr(state, n).clone() for n in STAGE_STATE}
+    tape.out = {n: getattr(state, n).clone() for n in names}
     return tape
+
+
+def _step_fields(state):
+    return STAGE_STATE + (SPECIES if _moist(state) else ())
 
 
 def step_adjoint_taped(state: DomainState, tape: StepTape,
@@ -705,7 +801,8 @@ def step_adjoint_taped(state: DomainState, tape: StepTape,
     _check_state(state)
     cfg = tape.cfg
     _check_adjoint_cfg(state, cfg)
-    names = STAGE_STATE
+    names = _step_fields(state)
+    moist = _moist(state)
     lx = {}
     for n in names:
         t = lam.get(n)
@@ -717,14 +814,32 @@ def step_adjoint_taped(state: DomainState, tape: StepTape,
     for n in names:
         getattr(state, n + "0").copy_(x0[n])
     g0 = {n: torch.zeros_like(getattr(state, n)) for n in names}
-    for xs, prep, subs, nsub, dtau in reversed(tape.stages):
+    for xs, prep, subs, nsub, dtau, ex in reversed(tape.stages):
         for n in names:
             getattr(state, n).copy_(xs[n])
         if prep is None:                 # recompute the stage's tape
-            prep, subs = _run_stage(state, cfg, nsub, dtau, True)
+            prep, subs, _ = _run_stage(state, cfg, nsub, dtau, True)
+        gq, inj = {}, None
+        if moist:
+            # the scalars' stage update (last in the stage): its adjoint
+            # feeds the finish's new mu', the stage fluxes and, through
+            # sumflux, every substep's output u'', v'', Omega''
+            lf = (torch.zeros_like(ex["fl"][0]), torch.zeros_like(ex["fl"][1]),
+                  torch.zeros_like(ex["fl"][2]))
+            g_mu = torch.zeros_like(state.mup)
+            for q in SPECIES:
+                gq[q] = scalar_stage_adjoint(
+                    state, cfg, ex["dt"], ex["final"], xs[q], x0[q],
+                    ex["fl"], x0["mup"], ex["mu_new"], lx[q], g0[q], lf,
+                    g0["mup"], g_mu)
+            lx["mup"] = lx["mup"] + g_mu
+            inj = [t / float(nsub) for t in lf]
+            lam_flux = {"ru": lf[0], "rv": lf[1], "ww": lf[2]}
         fin = small_step_finish_adjoint(
-            state, lx, views={n: subs[nsub][n] for n in FINISH_PP})
-        gs = {n: fin[n] for n in names}
+            state, {n: lx[n] for n in STAGE_STATE},
+            views={n: subs[nsub][n] for n in FINISH_PP})
+        gs = {n: fin[n] for n in STAGE_STATE}
+        gs.update(gq)
         lp = {n: fin[n] for n in FINISH_PP}
         # the stage's substep adjoints write (first) then add to one set of
         # tendency adjoints and reference accumulators, and share the
@@ -739,6 +854,10 @@ def step_adjoint_taped(state: DomainState, tape: StepTape,
             views.pop("ww_pp", None)
             fwd = {n: subs[i + 1][n] for n in _lib.ACOUSTIC_FWD}
             last = i == nsub - 1
+            if inj is not None:
+                lp["u_pp"] = lp["u_pp"] + inj[0]
+                lp["v_pp"] = lp["v_pp"] + inj[1]
+                lp["ww_pp"] = inj[2]
             out = acoustic_substep_adjoint(
                 state, cfg, dtau, i == 0, lp, acc, fwd=fwd, views=views,
                 lu=lu, clear_ref=last, tend_out=lt, accumulate=not last)
@@ -747,10 +866,11 @@ def step_adjoint_taped(state: DomainState, tape: StepTape,
         # and the time-t gradients g0
         reference_gradients(state, cfg, acc, views=prep, out=gs,
                             accumulate=True)
-        gio = dict(gs)
-        gio.update({n + "0": g0[n] for n in names})
+        gio = {n: gs[n] for n in STAGE_STATE + (("qv",) if moist else ())}
+        gio.update({n + "0": g0[n] for n in STAGE_STATE})
         small_step_init_adjoint(state, lp, out=gio, accumulate=True)
-        stage_tendencies_adjoint(state, cfg, lt, out=gs, accumulate=True)
+        stage_tendencies_adjoint(state, cfg, lt, out=gs, accumulate=True,
+                                 lam_flux=lam_flux if moist else None)
         lx = gs
     for n in names:
         g0[n] += lx[n]
@@ -762,27 +882,29 @@ def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict,
                  keep: str = "substeps") -> dict:
     """J^T lam of one dycore step (``dycore.step``; reference dycore.py:
     2161-2503) at the state's prognostic fields: ``lam`` maps the new
-    state's STAGE_STATE fields to adjoints (missing = 0); returns the
-    gradients of the step's input fields (STAGE_STATE names).  Dry states
-    without constant-K diffusion, plain form.  ``step_taped`` then
+    state's STAGE_STATE fields (and qv, qc, qr on a moist state) to
+    adjoints (missing = 0); returns the gradients of the step's input
+    fields.  No constant-K diffusion, plain form; moist states with the
+    scalars' unlimited update (moist_adv_opt = 0).  ``step_taped`` then
     ``step_adjoint_taped``; the state's prognostic fields are left as they
     were (their EOS recomputed), its other fields are overwritten."""
     from . import dycore as dyn
     _check_state(state)
     _check_adjoint_cfg(state, cfg)
-    x_in = {n: getattr(state, n).clone() for n in STAGE_STATE}
+    x_in = {n: getattr(state, n).clone() for n in _step_fields(state)}
     t0 = state.elapsed_seconds
     tape = step_taped(state, cfg, keep)
     g = step_adjoint_taped(state, tape, lam)
-    for n in STAGE_STATE:
-        getattr(state, n).copy_(x_in[n])
+    for n, t in x_in.items():
+        getattr(state, n).copy_(t)
     state.elapsed_seconds = t0
     dyn.update_diagnostics(state)
     return g
 
 
 class DycoreStep(torch.autograd.Function):
-    """(u, v, w, thp, php, mu') -> the same fields one dycore step later,
+    """(u, v, w, thp, php, mu'[, qv, qc, qr]) -> the same fields one dycore
+    step later (the scalars on a moist state),
     with the hand-written adjoint as its backward.  ``apply(state, cfg,
     *fields)`` (or ``apply(state, cfg, keep, *fields)``, keep as for
     ``step_taped``) writes the fields into the state and steps it there
@@ -795,18 +917,19 @@ class DycoreStep(torch.autograd.Function):
         keep = fields[0] if ctx.keep_arg else "substeps"
         if ctx.keep_arg:
             fields = fields[1:]
-        if len(fields) != len(STAGE_STATE):
-            raise ValueError(f"expected {len(STAGE_STATE)} fields")
-        for name, x in zip(STAGE_STATE, fields):
+        names = _step_fields(state)
+        if len(fields) != len(names):
+            raise ValueError(f"expected {len(names)} fields {names}")
+        for name, x in zip(names, fields):
             getattr(state, name).copy_(x)
-        ctx.state = state
+        ctx.state, ctx.names = state, names
         ctx.tape = step_taped(state, cfg, keep)
-        return tuple(getattr(state, n).clone() for n in STAGE_STATE)
+        return tuple(getattr(state, n).clone() for n in names)
 
     @staticmethod
     def backward(ctx, *grads):
-        lam = {n: g for n, g in zip(STAGE_STATE, grads) if g is not None}
+        lam = {n: g for n, g in zip(ctx.names, grads) if g is not None}
         g = step_adjoint_taped(ctx.state, ctx.tape, lam)
         ctx.tape = None
         lead = (None, None, None) if ctx.keep_arg else (None, None)
-        return lead + tuple(g[n] for n in STAGE_STATE)
+        return lead + tuple(g[n] for n in ctx.names)

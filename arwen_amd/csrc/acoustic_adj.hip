@@ -686,6 +686,7 @@ struct RefAcc {
 // the stage-field gradients the accumulators map to
 struct RefOut {
     real *gu, *gv, *gw, *gthp, *gphp, *gmup;
+    real *gqv;               // moist: the stage qv's gradient
     int acc;                 // add to them instead of writing
 };
 
@@ -1061,6 +1062,10 @@ __global__ void k_refg_eos(Dom d, RefAcc r, RefOut o, real* w)
     const real mh = d.c1h[k] * (d.mub2d[c] + d.mup[c]) + d.c2h[k];
     const real ga = galt - gp * (GAMMA_F * p / alt);
     put_o(o, o.gthp, t, r.thr[t] + gp * (GAMMA_F * p / th));
+    if (d.qv) {            // theta_m = theta (1 + Rv/Rd qv) of the moist EOS
+        const real thm = th * (R(1.0) + RVOVRD_F * d.qv[t]);
+        put_o(o, o.gqv, t, gp * (GAMMA_F * p / thm) * th * RVOVRD_F);
+    }
     w[G_DPH * F + t] = ga * (-d.rdnw[k] / mh);
     w[G_MU3 * F + t] = ga * (-alt * d.c1h[k] / mh);
 }
@@ -1343,8 +1348,6 @@ int32_t ARW_API(acoustic_substep_adjoint_ref)(
     if (fwd && !(fwd->u_pp && fwd->v_pp && fwd->w_pp && fwd->th_pp
                  && fwd->mu_pp && fwd->al_pp && fwd->ww_pp))
         return lib_fail(ARW_E_ARG, "null substep output");
-    if (d && d->qv)
-        return lib_fail(ARW_E_UNSUPPORTED, "reference adjoint: dry states only");
     RefAcc r = ref_of(acc);
     r.clear = (flags & ARW_ADJ_CLEAR) != 0;
     return substep_adjoint(d, prm, dtau, first, adj, &r, fwd, lu,
@@ -1368,7 +1371,7 @@ int32_t ARW_API(acoustic_adjoint_factors)(const Dom* d, const arw_params* prm,
 int32_t ARW_API(acoustic_reference_gradients)(
     const Dom* d, const arw_params* prm, const ARW_API(acoustic_ref)* acc,
     real* g_u, real* g_v, real* g_w, real* g_thp, real* g_php, real* g_mup,
-    int32_t flags, real* work, void* stream)
+    real* g_qv, int32_t flags, real* work, void* stream)
 {
     int32_t rc = lib_check_domain(d);
     if (rc) return rc;
@@ -1376,9 +1379,11 @@ int32_t ARW_API(acoustic_reference_gradients)(
         || !g_mup || !work)
         return lib_fail(ARW_E_ARG, "null argument");
     if (flags & ~ARW_ADJ_ACCUMULATE) return lib_fail(ARW_E_ARG, "unknown adjoint flags");
-    if (d->qv) return lib_fail(ARW_E_UNSUPPORTED, "reference adjoint: dry states only");
+    if (d->qv && !g_qv)
+        return lib_fail(ARW_E_ARG, "reference adjoint: a moist state needs g_qv");
     const RefAcc r = ref_of(acc);
     RefOut o;
+    o.gqv = g_qv;
     o.gu = g_u; o.gv = g_v; o.gw = g_w; o.gthp = g_thp; o.gphp = g_php;
     o.gmup = g_mup; o.acc = (flags & ARW_ADJ_ACCUMULATE) != 0;
     hipStream_t st = (hipStream_t)stream;

@@ -655,6 +655,68 @@ __global__ void k_tracer_stage(Dom d, const real* __restrict__ qcur,
     qdst[x] = v;
 }
 
+// One scalar stage's transpose with the dynamics' share: for lam (adjoint
+// of q_new) the per-point pieces -- the adjoint of the flux divergence's
+// output (g_tend), the time-t scalar's (added to gq0) and the two column
+// masses' per-level terms (P0: mu0, P1: mu_new) -- from the forward's own
+// tendency (the clamp's mask and d q_new / d C(mu_new) need q_new).
+__global__ void k_ss_adj_pts(Dom d, const real* __restrict__ q,
+                             const real* __restrict__ q0,
+                             const real* __restrict__ ru,
+                             const real* __restrict__ rv,
+                             const real* __restrict__ ww,
+                             const real* __restrict__ mup_t,
+                             const real* __restrict__ mup_new,
+                             const real* __restrict__ lam,
+                             real* __restrict__ gq0, real* __restrict__ wk,
+                             real dt, int clamp, real rdx, real rdy)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    Arr3 Q{q, ny, nx}, RU{ru, ny, nx + 1}, RV{rv, ny + 1, nx}, RW{ww, ny, nx};
+    const real tend = R(0.0) + adv_scalar(Q, RU, RV, RW, k, j, i, nz, ny, nx,
+                                         d.rdnw, d.fnm, d.fnp, rdx, rdy);
+    const int x = iM(k, j, i, ny, nx), c = j * nx + i;
+    const size_t n = (size_t)nz * ny * nx;
+    const real mu0 = d.mub2d[c] + mup_t[c];
+    const real mu = d.mub2d[c] + mup_new[c];
+    const real chm0 = d.c1h[k] * mu0 + d.c2h[k];
+    const real chm = d.c1h[k] * mu + d.c2h[k];
+    real v = chm0 * q0[x];
+    const real t = dt * tend;
+    v = v + t;
+    v = v / chm;
+    const real lv = (clamp && !(v > R(0.0))) ? R(0.0) : lam[x];
+    wk[x] = dt * lv / chm;
+    gq0[x] += chm0 * lv / chm;
+    wk[n + x] = d.c1h[k] * q0[x] * lv / chm;
+    wk[2 * n + x] = -(d.c1h[k] * v * lv / chm);
+}
+
+__global__ __launch_bounds__(64 * KG) void k_ss_adj_cols(Dom d,
+                                                          const real* wk,
+                                                          real* gmu0,
+                                                          real* gmu)
+{
+    const int nz = d.nz, st = d.ny * d.nx;
+    const int c = blockIdx.x * 64 + threadIdx.x, ty = threadIdx.y;
+    const bool on = c < st;
+    const size_t n = (size_t)nz * st;
+    real v[2] = {R(0.0), R(0.0)};
+    if (on)
+        for (int k = ty; k < nz; k += KG) {
+            v[0] += wk[n + (size_t)k * st + c];
+            v[1] += wk[2 * n + (size_t)k * st + c];
+        }
+    group_sums(v);
+    if (!on || ty != 0) return;
+    gmu0[c] += v[0];
+    gmu[c] += v[1];
+}
+
 // Adjoint stage, part 1: m = g / C(mu_new) (the adjoint of the division)
 // and gq (+)= C(mu_t) m (the adjoint of the time-t term).
 __global__ void k_tracer_adj_scale(Dom d, const real* __restrict__ g,
@@ -1478,6 +1540,65 @@ int32_t ARW_API(tracer_adjoint)(const Dom* d, const arw_params* prm,
     hipLaunchKernelGGL(k_add3, dim3(blocks), dim3(256), 0, st, lam_out, gq, g,
                        (int)n);
     return launched("k_add3");
+}
+
+// One stage of the unlimited scalar update with explicit fluxes and
+// masses (the arithmetic of k_scalar_update; reference moist.py:385-398):
+// q_new = (C(mu_t) q0 + dt flux_div(q; ru, rv, ww)) / C(mu_new), clamped
+// at zero when clamp.
+int32_t ARW_API(scalar_stage)(const Dom* d, const arw_params* prm,
+                              double dt_stage, int32_t clamp, const real* q,
+                              const real* q0, const real* ru, const real* rv,
+                              const real* ww, const real* mup_t,
+                              const real* mup_new, real* q_new, void* stream)
+{
+    int32_t rc = check_domain(d);
+    if (rc) return rc;
+    if (!prm || !q || !q0 || !ru || !rv || !ww || !mup_t || !mup_new || !q_new)
+        return fail(ARW_E_ARG, "null argument");
+    if (q_new == q) return fail(ARW_E_ARG, "q_new must not alias q");
+    const real rdx = (real)(1.0 / prm->dx), rdy = (real)(1.0 / prm->dy);
+    const dim3 cells((d->nx + 63) / 64, (d->ny + 3) / 4, d->nz);
+    hipLaunchKernelGGL(k_tracer_stage, cells, dim3(64, 4), 0,
+                       (hipStream_t)stream, *d, q, q0, q_new, ru, rv, ww,
+                       mup_t, mup_new, (real)dt_stage, clamp ? 1 : 0, rdx,
+                       rdy);
+    return launched("k_tracer_stage");
+}
+
+// Its transpose for lam = adjoint of q_new: g_q is written; g_q0, the flux
+// gradients g_ru, g_rv, g_ww and the column-mass gradients g_mup_t,
+// g_mup_new (mu' of time t and after the stage) are added to.  The clamp
+// is differentiated as 1 where q_new > 0, 0 elsewhere.  work: 3 nz*ny*nx.
+int32_t ARW_API(scalar_stage_adjoint)(
+    const Dom* d, const arw_params* prm, double dt_stage, int32_t clamp,
+    const real* q, const real* q0, const real* ru, const real* rv,
+    const real* ww, const real* mup_t, const real* mup_new, const real* lam,
+    real* g_q, real* g_q0, real* g_ru, real* g_rv, real* g_ww, real* g_mup_t,
+    real* g_mup_new, real* work, void* stream)
+{
+    int32_t rc = check_domain(d);
+    if (rc) return rc;
+    if (!prm || !q || !q0 || !ru || !rv || !ww || !mup_t || !mup_new || !lam
+        || !g_q || !g_q0 || !g_ru || !g_rv || !g_ww || !g_mup_t || !g_mup_new
+        || !work)
+        return fail(ARW_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int nz = d->nz, ny = d->ny, nx = d->nx;
+    const real rdx = (real)(1.0 / prm->dx), rdy = (real)(1.0 / prm->dy);
+    const dim3 cells((nx + 63) / 64, (ny + 3) / 4, nz);
+    hipLaunchKernelGGL(k_ss_adj_pts, cells, dim3(64, 4), 0, st, *d, q, q0, ru,
+                       rv, ww, mup_t, mup_new, lam, g_q0, work,
+                       (real)dt_stage, clamp ? 1 : 0, rdx, rdy);
+    if ((rc = launched("k_ss_adj_pts"))) return rc;
+    hipLaunchKernelGGL(k_ss_adj_cols, dim3((ny * nx + 63) / 64), dim3(64, KG),
+                       0, st, *d, work, g_mup_t, g_mup_new);
+    if ((rc = launched("k_ss_adj_cols"))) return rc;
+    if (hipMemsetAsync(g_q, 0, (size_t)nz * ny * nx * RB, st) != hipSuccess)
+        return fail(ARW_E_LAUNCH, "scalar adjoint: clearing g_q");
+    return lib_flux_div_adjoint(0, q, ru, rv, ww, work, g_q, g_ru, g_rv, g_ww,
+                                d->rdnw, d->fnm, d->fnp, rdx, rdy, nz, ny, nx,
+                                st);
 }
 
 int32_t ARW_API(ensemble_create)(const Dom* members, int32_t n,

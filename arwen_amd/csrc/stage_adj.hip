@@ -44,6 +44,7 @@ struct SA {
     Dom d;                   // state, diagnostics pointing into the work area
     const real *lru, *lrv, *lrw, *lrth, *lrph;
     real *gu, *gv, *gw, *gthp, *gphp, *gmup;
+    real *gqv, *gqc, *gqr;   // moist: the stage scalars' gradients
     real* w[S_COUNT];
     real rdx, rdy, hrdx, hrdy, qrdx, qrdy;
     int top_lid;
@@ -287,9 +288,49 @@ __global__ void k_sa_pts(SA a)
                 if (k == nz - 2) gpp += d.cfn1 * g_dpn(a, nz, c, cE, cN);
             }
             // buoyancy
-            if (k >= 1) gpp += G * d.rdn[k] * a.lrw[h + c];
-            if (k + 1 <= nz - 1) gpp -= G * d.rdn[k + 1] * a.lrw[h + st + c];
-            if (k == nz - 1) gpp -= G * R(2.0) * d.rdnw[nz - 1] * a.lrw[nz * st + c];
+            if (!d.qv) {
+                if (k >= 1) gpp += G * d.rdn[k] * a.lrw[h + c];
+                if (k + 1 <= nz - 1) gpp -= G * d.rdn[k + 1] * a.lrw[h + st + c];
+                if (k == nz - 1) gpp -= G * R(2.0) * d.rdnw[nz - 1] * a.lrw[nz * st + c];
+            } else {
+                // with the moist loading (dycore.cu:289-300): the p' term
+                // scaled by cq1 = 1 / (1 + cqw), cqw the face mean of
+                // qv + qc + qr, and d(term)/d(cqw) = -cq1^2 (rdn dp' + c1f
+                // mub + c2f); each face mean takes half of it
+                auto qt = [&](int kk) {
+                    const int y = kk * st + c;
+                    return (d.qv[y] + d.qc[y]) + d.qr[y];
+                };
+                const real mub = d.mub2d[c];
+                real gqt = R(0.0);
+                if (k >= 1) {
+                    const real q1 = R(1.0) / (R(1.0) + R(0.5) * (qt(k) + qt(k - 1)));
+                    const real lw = a.lrw[h + c];
+                    gpp += G * q1 * d.rdn[k] * lw;
+                    const real dp = ppv(a, k, c) - ppv(a, k - 1, c);
+                    gqt += R(0.5) * (G * lw * (-q1 * q1)
+                                     * (d.rdn[k] * dp + (d.c1f[k] * mub + d.c2f[k])));
+                }
+                if (k + 1 <= nz - 1) {
+                    const real q1 = R(1.0) / (R(1.0) + R(0.5) * (qt(k + 1) + qt(k)));
+                    const real lw = a.lrw[h + st + c];
+                    gpp -= G * q1 * d.rdn[k + 1] * lw;
+                    const real dp = ppv(a, k + 1, c) - ppv(a, k, c);
+                    gqt += R(0.5) * (G * lw * (-q1 * q1)
+                                     * (d.rdn[k + 1] * dp + (d.c1f[k + 1] * mub + d.c2f[k + 1])));
+                }
+                if (k >= nz - 2) {
+                    const real q1 = R(1.0) / (R(1.0) + R(0.5) * (qt(nz - 1) + qt(nz - 2)));
+                    const real lw = a.lrw[nz * st + c];
+                    if (k == nz - 1) gpp -= G * q1 * R(2.0) * d.rdnw[nz - 1] * lw;
+                    gqt += R(0.5) * (G * lw * (-q1 * q1)
+                                     * (R(2.0) * d.rdnw[nz - 1] * (R(0.0) - ppv(a, nz - 1, c))
+                                        + (d.c1f[nz] * mub + d.c2f[nz])));
+                }
+                a.gqv[h + c] = (a.acc ? a.gqv[h + c] : R(0.0)) + gqt;
+                a.gqc[h + c] = (a.acc ? a.gqc[h + c] : R(0.0)) + gqt;
+                a.gqr[h + c] = (a.acc ? a.gqr[h + c] : R(0.0)) + gqt;
+            }
             a.w[S_GPP][h + c] = gpp;
             a.w[S_GALT][h + c] = galt;
             a.w[S_GAL][h + c] = gal;
@@ -460,6 +501,10 @@ __global__ void k_sa_eos(SA a)
     const real gp = a.w[S_GPP][t];
     const real ga = a.w[S_GALT][t] + a.w[S_GAL][t] - gp * (GAMMA_F * p / alt);
     a.gthp[t] += gp * (GAMMA_F * p / th);
+    if (d.qv) {            // theta_m = theta (1 + Rv/Rd qv) of the moist EOS
+        const real thm = th * (R(1.0) + RVOVRD_F * d.qv[t]);
+        a.gqv[t] += gp * (GAMMA_F * p / thm) * th * RVOVRD_F;
+    }
     a.w[S_GDPH][t] = ga * (-d.rdnw[k] / mh);
     gmu += ga * (-alt * d.c1h[k] / mh);
     a.w[S_GMU3][t] += gmu;
@@ -504,6 +549,7 @@ struct IA {
     const real *lu, *lv, *lw, *lth, *lph, *lmu, *lal, *lp, *lpo;
     real *gu0, *gv0, *gw0, *gthp0, *gphp0, *gmup0;
     real *gu, *gv, *gw, *gthp, *gphp, *gmup;
+    real *gqv;               // moist: the stage qv's gradient
     real* w[I_COUNT];
     int acc;                 // the outputs hold values to add to
 };
@@ -621,6 +667,10 @@ __global__ void k_ia_eos(IA a)
     const real gp = a.w[I_GP][t];
     const real ga = a.w[I_GALT][t] - gp * (GAMMA_F * p / alt);
     a.gthp[t] += gp * (GAMMA_F * p / th);
+    if (d.qv) {
+        const real thm = th * (R(1.0) + RVOVRD_F * d.qv[t]);
+        put_i(a, a.gqv, t, gp * (GAMMA_F * p / thm) * th * RVOVRD_F);
+    }
     a.w[I_GDPH2][t] = ga * (-d.rdnw[k] / mh);
     a.w[I_U3B][t] = ga * (-alt * d.c1h[k] / mh);
 }
@@ -837,7 +887,8 @@ extern "C" int32_t ARW_API(small_step_init_adjoint)(
         if (!p) return lib_fail(ARW_E_ARG, "null output adjoint");
     for (real* p : gout)
         if (!p) return lib_fail(ARW_E_ARG, "null input adjoint");
-    if (d->qv) return lib_fail(ARW_E_UNSUPPORTED, "init adjoint: dry states only");
+    if (d->qv && !adj->g_qv)
+        return lib_fail(ARW_E_ARG, "init adjoint: a moist state needs g_qv");
     const int nz = d->nz, ny = d->ny, nx = d->nx, st = ny * nx;
     if (nz < 3) return lib_fail(ARW_E_ARG, "init adjoint needs nz >= 3");
     hipStream_t sm = (hipStream_t)stream;
@@ -851,7 +902,7 @@ extern "C" int32_t ARW_API(small_step_init_adjoint)(
     a.gu0 = adj->g_u0; a.gv0 = adj->g_v0; a.gw0 = adj->g_w0;
     a.gthp0 = adj->g_thp0; a.gphp0 = adj->g_php0; a.gmup0 = adj->g_mup0;
     a.gu = adj->g_u; a.gv = adj->g_v; a.gw = adj->g_w; a.gthp = adj->g_thp;
-    a.gphp = adj->g_php; a.gmup = adj->g_mup;
+    a.gphp = adj->g_php; a.gmup = adj->g_mup; a.gqv = adj->g_qv;
     if (adj->flags & ~ARW_ADJ_ACCUMULATE)
         return lib_fail(ARW_E_ARG, "unknown adjoint flags");
     a.acc = (adj->flags & ARW_ADJ_ACCUMULATE) != 0;
@@ -930,7 +981,8 @@ extern "C" int32_t ARW_API(stage_tendencies_adjoint)(
         if (!p) return lib_fail(ARW_E_ARG, "null output adjoint");
     for (real* p : gout)
         if (!p) return lib_fail(ARW_E_ARG, "null input adjoint");
-    if (d->qv) return lib_fail(ARW_E_UNSUPPORTED, "stage adjoint: dry states only");
+    if (d->qv && !(adj->g_qv && adj->g_qc && adj->g_qr))
+        return lib_fail(ARW_E_ARG, "stage adjoint: a moist state needs g_qv, g_qc, g_qr");
     if (prm->khdif > 0.0 || prm->kvdif > 0.0)
         return lib_fail(ARW_E_UNSUPPORTED, "stage adjoint: no constant-K diffusion");
     if (prm->precision_form)
@@ -946,6 +998,7 @@ extern "C" int32_t ARW_API(stage_tendencies_adjoint)(
     a.lrth = adj->l_rth_t; a.lrph = adj->l_rph_t;
     a.gu = adj->g_u; a.gv = adj->g_v; a.gw = adj->g_w; a.gthp = adj->g_thp;
     a.gphp = adj->g_php; a.gmup = adj->g_mup;
+    a.gqv = adj->g_qv; a.gqc = adj->g_qc; a.gqr = adj->g_qr;
     const double rdx = 1.0 / prm->dx, rdy = 1.0 / prm->dy;
     a.rdx = (real)rdx; a.rdy = (real)rdy;
     a.hrdx = (real)(0.5 * rdx); a.hrdy = (real)(0.5 * rdy);
@@ -976,6 +1029,16 @@ extern "C" int32_t ARW_API(stage_tendencies_adjoint)(
     for (int w : {S_GRU, S_GRV, S_GWW})
         if (hipMemsetAsync(a.w[w], 0, F * RB, sm) != hipSuccess)
             return lib_fail(ARW_E_LAUNCH, "stage adjoint: clearing work");
+    // adjoints of the stage fluxes handed on (the scalars' sumflux) seed
+    // the flux gradients the advection transposes add to
+    const real* lfl[3] = {adj->l_ru, adj->l_rv, adj->l_ww};
+    const size_t nfl[3] = {(size_t)nz * ny * (nx + 1), (size_t)nz * (ny + 1) * nx,
+                           (size_t)nwl};
+    const int sfl[3] = {S_GRU, S_GRV, S_GWW};
+    for (int q = 0; q < 3; ++q)
+        if (lfl[q] && hipMemcpyAsync(a.w[sfl[q]], lfl[q], nfl[q] * RB,
+                                     hipMemcpyDeviceToDevice, sm) != hipSuccess)
+            return lib_fail(ARW_E_LAUNCH, "stage adjoint: flux adjoints");
     // advection of theta, u, v, w with the stage fluxes
     const real dxi = a.rdx, dyi = a.rdy;
     struct { int op; const real* q; const real* lam; real* g; const real* rdz; }

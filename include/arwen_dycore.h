@@ -299,11 +299,35 @@ int32_t arw_acoustic_reference_gradients(const arw_domain *d,
                                          const arw_acoustic_ref *acc,
                                          float *g_u, float *g_v, float *g_w,
                                          float *g_thp, float *g_php, float *g_mup,
-                                         int32_t flags, float *work,
+                                         float *g_qv, int32_t flags, float *work,
                                          void *stream);
 int32_t arw_set_w_surface_adjoint(const arw_domain *d, const arw_params *prm,
                                   const float *lam_w0, float *g_u, float *g_v,
                                   void *stream);
+
+/* One stage of the scalars' unlimited update with explicit fluxes and
+ * masses (no counterpart in the reference as an entry point; its
+ * arithmetic is moist.py:385-398 _update_scalar_kernel with the
+ * substep-averaged fluxes): q_new = (C(mu_t) q0 + dt flux_div(q; ru, rv,
+ * ww)) / C(mu_new), clamped at zero when clamp.  mup_t, mup_new: mu' of
+ * time t and after the stage's finish.  arw_scalar_stage_adjoint: its
+ * transpose for lam = adjoint of q_new; g_q is written, g_q0, g_ru, g_rv,
+ * g_ww, g_mup_t, g_mup_new are added to; the clamp is differentiated as 1
+ * where q_new > 0, 0 elsewhere; work: 3*nz*ny*nx elements. */
+int32_t arw_scalar_stage(const arw_domain *d, const arw_params *prm,
+                         double dt_stage, int32_t clamp, const float *q,
+                         const float *q0, const float *ru, const float *rv,
+                         const float *ww, const float *mup_t,
+                         const float *mup_new, float *q_new, void *stream);
+int32_t arw_scalar_stage_adjoint(const arw_domain *d, const arw_params *prm,
+                                 double dt_stage, int32_t clamp,
+                                 const float *q, const float *q0, const float *ru,
+                                 const float *rv, const float *ww,
+                                 const float *mup_t, const float *mup_new,
+                                 const float *lam, float *g_q, float *g_q0,
+                                 float *g_ru, float *g_rv, float *g_ww,
+                                 float *g_mup_t, float *g_mup_new, float *work,
+                                 void *stream);
 
 /* EOS p, al, alt from thp, php, mup (replaces diagnostics.cu:37-89
  * calc_p_alpha, launched by diagnostics.py:14-57; hypsometric_opt = 1). */
@@ -353,6 +377,8 @@ int32_t arw_flux_div_w(const float *w, const float *ru, const float *rv,
 typedef struct arw_stage_adj {
     const float *l_ru_t, *l_rv_t, *l_rw_t, *l_rth_t, *l_rph_t;
     float *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+    float *g_qv, *g_qc, *g_qr;     /* moist states: the stage scalars' */
+    const float *l_ru, *l_rv, *l_ww; /* optional: adjoints of the stage fluxes */
     int32_t flags;   /* ARW_ADJ_ACCUMULATE: add to g_* instead of writing */
 } arw_stage_adj;
 int32_t arw_stage_tendencies_adjoint(const arw_domain *d,
@@ -375,6 +401,7 @@ typedef struct arw_init_adj {
         *l_al_pp, *l_p_pp, *l_p_pp_old;
     float *g_u0, *g_v0, *g_w0, *g_thp0, *g_php0, *g_mup0;
     float *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+    float *g_qv;                   /* moist states: the stage qv's */
     int32_t flags;   /* ARW_ADJ_ACCUMULATE: add to g_* instead of writing */
 } arw_init_adj;
 typedef struct arw_finish_adj {
@@ -614,15 +641,31 @@ int32_t arw64_acoustic_reference_gradients(const arw64_domain *d,
                                          const arw64_acoustic_ref *acc,
                                          double *g_u, double *g_v, double *g_w,
                                          double *g_thp, double *g_php, double *g_mup,
-                                         int32_t flags, double *work,
+                                         double *g_qv, int32_t flags, double *work,
                                          void *stream);
 int32_t arw64_set_w_surface_adjoint(const arw64_domain *d, const arw_params *prm,
                                   const double *lam_w0, double *g_u, double *g_v,
                                   void *stream);
+int32_t arw64_scalar_stage(const arw64_domain *d, const arw_params *prm,
+                         double dt_stage, int32_t clamp, const double *q,
+                         const double *q0, const double *ru, const double *rv,
+                         const double *ww, const double *mup_t,
+                         const double *mup_new, double *q_new, void *stream);
+int32_t arw64_scalar_stage_adjoint(const arw64_domain *d, const arw_params *prm,
+                                 double dt_stage, int32_t clamp,
+                                 const double *q, const double *q0, const double *ru,
+                                 const double *rv, const double *ww,
+                                 const double *mup_t, const double *mup_new,
+                                 const double *lam, double *g_q, double *g_q0,
+                                 double *g_ru, double *g_rv, double *g_ww,
+                                 double *g_mup_t, double *g_mup_new, double *work,
+                                 void *stream);
 
 typedef struct arw64_stage_adj {
     const double *l_ru_t, *l_rv_t, *l_rw_t, *l_rth_t, *l_rph_t;
     double *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+    double *g_qv, *g_qc, *g_qr;     /* moist states: the stage scalars' */
+    const double *l_ru, *l_rv, *l_ww; /* optional: adjoints of the stage fluxes */
     int32_t flags;   /* ARW_ADJ_ACCUMULATE: add to g_* instead of writing */
 } arw64_stage_adj;
 int32_t arw64_stage_tendencies_adjoint(const arw64_domain *d,
@@ -637,6 +680,7 @@ typedef struct arw64_init_adj {
         *l_al_pp, *l_p_pp, *l_p_pp_old;
     double *g_u0, *g_v0, *g_w0, *g_thp0, *g_php0, *g_mup0;
     double *g_u, *g_v, *g_w, *g_thp, *g_php, *g_mup;
+    double *g_qv;                   /* moist states: the stage qv's */
     int32_t flags;   /* ARW_ADJ_ACCUMULATE: add to g_* instead of writing */
 } arw64_init_adj;
 typedef struct arw64_finish_adj {

@@ -326,6 +326,171 @@ def test_step_adjoint_rejects_unsupported():
     from arwen_amd._lib import ArwenError
     case, s, g, rng = _step_states("dry_flat")
     lam = {"u": torch.ones_like(g.u)}
-    with pytest.raises(ArwenError, match="dry states"):
+    with pytest.raises(ArwenError, match="constant-K"):
         adjoint.step_adjoint(g, case.cfg.with_(khdif=75.0, kvdif=75.0,
                                                km_opt=1), lam)
+
+
+# ---------------------------------------------------------------------------
+# Moist states: qv, qc, qr through the moist EOS, the buoyancy loading and
+# the sumflux-coupled scalar update (moist_adv_opt = 0)
+# ---------------------------------------------------------------------------
+
+MOIST_NAMES = A.STAGE_IN + A.SPECIES
+
+
+def _moist_states(name, dtype=torch.float64, seed=59):
+    """The oracle's moist whole-step test state and a device state of dtype
+    holding the same fields (scalars kept positive)."""
+    from arwen_amd import dycore
+    from arwen_amd.state import DomainState
+    from tests import pins
+    case = pins.CASES[name]()
+    case.cfg = case.cfg.with_(moist=True, moist_adv_opt=0)
+    cfg = case.cfg
+    s = O.from_case(case, np.float64)
+    rng = np.random.default_rng(seed)
+    s.u[...] += rng.standard_normal(s.u.shape) * 0.5
+    s.v[...] += rng.standard_normal(s.v.shape) * 0.5
+    s.w[1:-1] += rng.standard_normal(s.w[1:-1].shape) * 0.3
+    s.mup[...] += rng.standard_normal(s.mup.shape) * 20.0
+    s.php[1:] += rng.standard_normal(s.php[1:].shape) * 30.0
+    shp = s.thp.shape
+    s.qv = 0.008 + 0.002 * rng.standard_normal(shp)
+    s.qc = 5e-4 + 1e-4 * rng.standard_normal(shp)
+    s.qr = 5e-4 + 1e-4 * rng.standard_normal(shp)
+    for q in A.SPECIES:
+        setattr(s, q + "0", np.zeros(shp))
+    O.set_w_surface(s, cfg.dx, cfg.dy)
+    if dtype == torch.float32:
+        for n in MOIST_NAMES:
+            x = getattr(s, n)
+            x[...] = x.astype(np.float32).astype(np.float64)
+    O.calc_p_alpha(s)
+    g = DomainState.from_host(case, dtype=dtype)
+    assert g.qv is not None
+    for n in MOIST_NAMES:
+        getattr(g, n).copy_(torch.as_tensor(getattr(s, n)))
+    dycore.update_diagnostics(g)
+    return case, s, g, rng
+
+
+def test_scalar_stage_and_adjoint_match_oracle():
+    from arwen_amd import adjoint
+    case, s, g, rng = _moist_states("terrain", seed=61)
+    cfg = case.cfg
+    print("scalar stage and its adjoint vs oracle", flush=True)
+    dev = g.u.device
+    nz, ny, nx = s.thp.shape
+    q, q0 = s.qv, s.qv * (1.0 + 0.01 * rng.standard_normal(s.qv.shape))
+    fl = (rng.standard_normal(s.u.shape) * 3e3,
+          rng.standard_normal(s.v.shape) * 3e3,
+          rng.standard_normal(s.w.shape) * 1e1)
+    mup_t = s.mup * 0.9
+    mup_new = s.mup * 1.1
+    mu0, mu = s.mub2d + mup_t, s.mub2d + mup_new
+    lam = rng.standard_normal(q.shape)
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+    for clamp in (False, True):
+        r = A.scalar_update_adjoint(s, q, q0, fl, mu0, mu, 9.0, clamp, lam,
+                                    cfg.dx, cfg.dy)
+        g_q0 = torch.zeros_like(g.qv)
+        g_fl = [torch.zeros_like(g.u), torch.zeros_like(g.v),
+                torch.zeros_like(g.w)]
+        g_mt = torch.zeros_like(g.mup)
+        g_mn = torch.zeros_like(g.mup)
+        g_q = adjoint.scalar_stage_adjoint(
+            g, cfg, 9.0, clamp, t(q), t(q0), [t(x) for x in fl], t(mup_t),
+            t(mup_new), t(lam), g_q0, g_fl, g_mt, g_mn)
+        got = {"q": g_q, "q0": g_q0, "ru": g_fl[0], "rv": g_fl[1],
+               "ww": g_fl[2], "mu0": g_mt, "mu": g_mn}
+        bad = {k: _rel(v, r[k]) for k, v in got.items()}
+        assert max(bad.values()) < 1e-10, (clamp, bad)
+
+
+@pytest.mark.parametrize("name", ["dry_flat", "terrain"])
+def test_moist_step_adjoint_matches_oracle(name):
+    from arwen_amd import adjoint
+    case, s, g, rng = _moist_states(name)
+    print(f"moist step adjoint vs oracle {name}", flush=True)
+    lam = {n: rng.standard_normal(getattr(s, n).shape) for n in MOIST_NAMES}
+    ref = A.step_adjoint(s, case.cfg, lam)
+    before = {n: getattr(g, n).clone() for n in MOIST_NAMES}
+    got = adjoint.step_adjoint(
+        g, case.cfg, {n: torch.as_tensor(v, device=g.u.device)
+                      for n, v in lam.items()})
+    for n in MOIST_NAMES:
+        assert torch.equal(getattr(g, n), before[n]), n
+    bad = {n: _rel(got[n], ref[n]) for n in MOIST_NAMES}
+    assert max(bad.values()) < ORACLE_TOL, bad
+
+
+def test_moist_step_adjoint_dot_product_vs_device_step():
+    from arwen_amd import adjoint, dycore
+    case, s, g, rng = _moist_states("terrain", seed=67)
+    cfg = case.cfg
+    print("moist step adjoint FD dot product", flush=True)
+    dev = g.u.device
+    base = {n: getattr(g, n).clone() for n in MOIST_NAMES}
+    lam = {n: torch.as_tensor(rng.standard_normal(getattr(s, n).shape),
+                              device=dev) for n in MOIST_NAMES}
+    lam.update({q: lam[q] * 1e3 for q in A.SPECIES})
+    gt = adjoint.step_adjoint(g, cfg, lam)
+    scale = {"u": 1.0, "v": 1.0, "w": 0.1, "thp": 0.5, "php": 20.0,
+             "mup": 20.0, "qv": 1e-3, "qc": 1e-4, "qr": 1e-4}
+    v = {n: torch.as_tensor(rng.standard_normal(getattr(s, n).shape)
+                            * scale[n], device=dev) for n in MOIST_NAMES}
+    h = 1e-6
+
+    def run(sign):
+        for n in MOIST_NAMES:
+            getattr(g, n).copy_(base[n] + sign * h * v[n])
+        dycore.update_diagnostics(g)
+        dycore.step(g, cfg)
+        return {n: getattr(g, n).clone() for n in MOIST_NAMES}
+
+    fp, fm = run(1.0), run(-1.0)
+    jv = sum(float(torch.sum((fp[n] - fm[n]) / (2 * h) * lam[n]))
+             for n in MOIST_NAMES)
+    vjt = sum(float(torch.sum(v[n] * gt[n])) for n in MOIST_NAMES)
+    assert abs(jv - vjt) <= 1e-6 * abs(jv), (jv, vjt)
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12),
+                                       (torch.float32, 1e-5)])
+def test_taped_moist_step_equals_fused_step(dtype, tol):
+    from arwen_amd import adjoint, dycore
+    case, s, g, rng = _moist_states("terrain", dtype=dtype, seed=71)
+    print(f"taped moist step vs fused step {dtype}", flush=True)
+    x0 = {n: getattr(g, n).clone() for n in MOIST_NAMES}
+    dycore.step(g, case.cfg)
+    fused = {n: getattr(g, n).clone() for n in MOIST_NAMES}
+    for n, t in x0.items():
+        getattr(g, n).copy_(t)
+    dycore.update_diagnostics(g)
+    tape = adjoint.step_taped(g, case.cfg)
+    for n in MOIST_NAMES:
+        a = getattr(g, n).double()
+        b = fused[n].double()
+        scale = float(b.abs().max()) or 1.0
+        assert float((a - b).abs().max()) <= tol * scale, n
+
+
+def test_moist_dycore_step_autograd_and_stage_checkpoints():
+    from arwen_amd import adjoint
+    case, s, g, rng = _moist_states("dry_flat", seed=73)
+    cfg = case.cfg
+    print("moist dycore step autograd", flush=True)
+    dev = g.u.device
+    lam = {n: torch.as_tensor(rng.standard_normal(getattr(s, n).shape),
+                              device=dev) for n in MOIST_NAMES}
+    full = adjoint.step_adjoint(g, cfg, lam)
+    lean = adjoint.step_adjoint(g, cfg, lam, keep="stages")
+    for n in MOIST_NAMES:
+        assert torch.equal(full[n], lean[n]), n
+    xs = [getattr(g, n).clone().requires_grad_() for n in MOIST_NAMES]
+    outs = adjoint.DycoreStep.apply(g, cfg, *xs)
+    loss = sum(torch.sum(o * lam[n]) for o, n in zip(outs, MOIST_NAMES))
+    grads = torch.autograd.grad(loss, xs)
+    for n, gr in zip(MOIST_NAMES, grads):
+        assert torch.equal(gr, full[n]), n
