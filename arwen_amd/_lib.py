@@ -222,7 +222,7 @@ def load():
                                            _P, _P], _I),
         "arw_small_step_adjoint_work": ([_I, _I, _I,
                                          ctypes.POINTER(ctypes.c_int64)], _I),
-        "arw_ensemble_create": ([dp, _I, pp, _P, _I,
+        "arw_ensemble_create": ([dp, _I, pp, _P, _I, _I,
                                  ctypes.POINTER(ctypes.c_void_p)], _I),
         "arw_ensemble_step": ([_P, _I, _P], _I),
         "arw_ensemble_capture": ([_P, _I, _I], _I),

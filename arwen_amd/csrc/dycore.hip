@@ -1603,7 +1603,8 @@ int32_t ARW_API(scalar_stage_adjoint)(
 
 int32_t ARW_API(ensemble_create)(const Dom* members, int32_t n,
                                  const arw_params* prm, const int32_t* order,
-                                 int32_t groups, ARW_API(ensemble)** out)
+                                 int32_t groups, int32_t col_tile,
+                                 ARW_API(ensemble)** out)
 {
     if (!out) return fail(ARW_E_ARG, "null output handle");
     *out = nullptr;
@@ -1611,6 +1612,8 @@ int32_t ARW_API(ensemble_create)(const Dom* members, int32_t n,
         return fail(ARW_E_ARG, "need 1 <= n <= 65536 members");
     if (groups < 1 || groups > n)
         return fail(ARW_E_ARG, "need 1 <= groups <= n");
+    if (col_tile != 64 && col_tile != 128 && col_tile != 256)
+        return fail(ARW_E_ARG, "col_tile must be 64, 128 or 256");
     int32_t rc;
     for (int m = 0; m < n; ++m) {
         if ((rc = check_domain(&members[m]))) return rc;
@@ -1659,7 +1662,7 @@ int32_t ARW_API(ensemble_create)(const Dom* members, int32_t n,
         off[g].fin = bytes; bytes = align(bytes + 3 * ng * sizeof(FinishDst));
         for (int k = 0; k < BK_COUNT; ++k) {
             long long t = 0;
-            for (int m : gm[g]) t += batch_blocks(members[m], k);
+            for (int m : gm[g]) t += batch_blocks(members[m], k, col_tile);
             if (t >= (1LL << 30)) return fail(ARW_E_ARG, "ensemble too large for one launch");
             off[g].total[k] = (int)t;
             off[g].start[k] = bytes; bytes = align(bytes + (ng + 1) * sizeof(int));
@@ -1684,7 +1687,7 @@ int32_t ARW_API(ensemble_create)(const Dom* members, int32_t n,
             int t = 0;
             for (size_t i = 0; i < ng; ++i) {
                 start[i] = t;
-                const int nb = batch_blocks(members[gm[g][i]], k);
+                const int nb = batch_blocks(members[gm[g][i]], k, col_tile);
                 for (int b = 0; b < nb; ++b) owner[t + b] = (int)i;
                 t += nb;
             }
@@ -1720,6 +1723,7 @@ int32_t ARW_API(ensemble_create)(const Dom* members, int32_t n,
                 b.owner = (const int*)(dev + off[g].owner[k]);
                 b.start = (const int*)(dev + off[g].start[k]);
                 b.total = off[g].total[k];
+                b.tile = k == BK_COLT ? col_tile : 256;
             }
         }
     }

@@ -20,11 +20,14 @@ struct Batch {
     const int* owner;
     const int* start;
     int total;
+    int tile;          // threads per workgroup of this launch shape
 };
 
-// Workgroups one member needs in each batched launch shape.
-enum BatchKind { BK_COLS, BK_ROWS, BK_UV, BK_TFE, BK_FE, BK_COUNT };
-int batch_blocks(const Dom& d, int kind);
+// Workgroups one member needs in each batched launch shape.  BK_COLT is
+// the column solve's: col_tile columns per workgroup (64, 128 or 256,
+// chosen per ensemble); the other column launches keep 256.
+enum BatchKind { BK_COLS, BK_ROWS, BK_UV, BK_TFE, BK_FE, BK_COLT, BK_COUNT };
+int batch_blocks(const Dom& d, int kind, int col_tile);
 
 // The members' views for one launch role, with the per-shape tables.
 struct BatchSet {

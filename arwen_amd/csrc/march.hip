@@ -2009,11 +2009,12 @@ hipError_t run_finish(const Dom& d, const FinishDst& o, bool prec,
 // Batched launchers (Many)
 // ---------------------------------------------------------------------------
 
-int batch_blocks(const Dom& d, int kind)
+int batch_blocks(const Dom& d, int kind, int col_tile)
 {
     const int rows = ((d.nx + 63) / 64) * ((d.ny + 3) / 4);
     switch (kind) {
     case BK_COLS: return (d.ny * d.nx + 255) / 256;
+    case BK_COLT: return (d.ny * d.nx + col_tile - 1) / col_tile;
     case BK_ROWS: return rows;
     case BK_UV: return (d.ny + d.nx + 255) / 256 + rows;
     case BK_TFE: return (d.nz * d.ny + d.nz * d.nx + 255) / 256;
@@ -2089,10 +2090,10 @@ static void launch_column_b(const Batch& r, const Sc& s, hipStream_t st)
 {
     if (s.prec)
         hipLaunchKernelGGL((k_column<Many, B3, DAMP, AL, false, true>),
-                           bgrid(r), dim3(256), 0, st, r, s, nullptr);
+                           bgrid(r), dim3(r.tile), 0, st, r, s, nullptr);
     else
         hipLaunchKernelGGL((k_column<Many, B3, DAMP, AL, false, false>),
-                           bgrid(r), dim3(256), 0, st, r, s, nullptr);
+                           bgrid(r), dim3(r.tile), 0, st, r, s, nullptr);
 }
 
 template <bool B3, bool DAMP>
@@ -2106,7 +2107,7 @@ static void launch_column_b_al(const Batch& r, const Sc& s, int al,
 hipError_t run_column_b(const BatchSet& b, const Sc& s, bool b3, int write_al,
                         hipStream_t st)
 {
-    const Batch& r = b.k[BK_COLS];
+    const Batch& r = b.k[BK_COLT];
     const bool damp = s.damp != 0;
     if (b3 && damp) launch_column_b_al<true, true>(r, s, write_al, st);
     else if (b3) launch_column_b_al<true, false>(r, s, write_al, st);

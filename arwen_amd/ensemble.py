@@ -49,7 +49,7 @@ class Ensemble:
 
     def __init__(self, states: list[DomainState], cfg: RunConfig,
                  order: list[int] | None = None, groups: int = 1,
-                 join_every: int = 0):
+                 join_every: int = 0, col_tile: int = 256):
         if not states:
             raise ValueError("an ensemble needs at least one member")
         validate_for_step(cfg, acoustic=True)
@@ -69,6 +69,7 @@ class Ensemble:
         self.order = list(order) if order is not None else None
         self.groups = int(groups)
         self.join_every = int(join_every)
+        self.col_tile = int(col_tile)
         n = len(states)
         Dom = _lib.ArwDomain64 if dt0 == torch.float64 else _lib.ArwDomain
         arr = (Dom * n)()
@@ -86,7 +87,7 @@ class Ensemble:
         lib = _lib.load()
         h = ctypes.c_void_p()
         fn = getattr(lib, self._pre + "ensemble_create")
-        _lib.check(fn(arr, n, params(cfg), ordp, self.groups,
+        _lib.check(fn(arr, n, params(cfg), ordp, self.groups, self.col_tile,
                       ctypes.byref(h)), self._pre + "ensemble_create")
         self._h = h
         self.graph_steps = 0
@@ -176,9 +177,10 @@ def member_orders(states) -> dict[str, list[int]]:
 _TUNED: dict = {}
 
 
-def _time_layout(states, cfg, saved, order, groups, join_every, steps, reps):
+def _time_layout(states, cfg, saved, order, groups, join_every, steps, reps,
+                 col_tile=256):
     with Ensemble(states, cfg, order=order, groups=groups,
-                  join_every=join_every) as ens:
+                  join_every=join_every, col_tile=col_tile) as ens:
         ens.capture(steps)
         ens.launch()                               # warm-up replay
         torch.cuda.synchronize()
@@ -193,16 +195,19 @@ def _time_layout(states, cfg, saved, order, groups, join_every, steps, reps):
 
 def autotune(states: list[DomainState], cfg: RunConfig, *, steps: int = 10,
              reps: int = 2, groups=(1, 2, 4, 8), joins=(1, 5, 0),
-             orders=None, verbose: bool = False) -> dict:
+             col_tiles=(256, 128, 64), orders=None,
+             verbose: bool = False) -> dict:
     """Pick the launch layout by timing it on the ensemble itself: first
     the member order of the work tables (one group), then the number of
     groups that run as parallel graph branches and how often the branches
-    rejoin.  Returns {"order", "order_name", "groups", "join_every",
+    rejoin (the substep schedule across the branches), then the column
+    solve's tile (columns per workgroup) for that layout.  Returns
+    {"order", "order_name", "groups", "join_every", "col_tile",
     "ms_per_step", "trials"}; the members' state is restored afterwards
     and the result is cached per member shapes and step parameters."""
     key = (tuple(st.shape for st in states), states[0].dtype,
            tuple(getattr(cfg, f) for f in SHARED), tuple(groups),
-           tuple(joins), steps)
+           tuple(joins), tuple(col_tiles), steps)
     if key in _TUNED:
         return _TUNED[key]
     saved = save_state(states)
@@ -211,14 +216,15 @@ def autotune(states: list[DomainState], cfg: RunConfig, *, steps: int = 10,
         cands = {k: v for k, v in cands.items() if k in orders}
     trials = []
 
-    def run(oname, g, j):
+    def run(oname, g, j, t=256):
         ms = _time_layout(states, cfg, saved, cands[oname], g, j, steps,
-                          reps)
+                          reps, t)
         trials.append({"order": oname, "groups": g, "join_every": j,
-                       "ms_per_step": ms})
+                       "col_tile": t, "ms_per_step": ms})
         if verbose:
-            print(f"autotune order={oname} groups={g} join_every={j}: "
-                  f"{ms:.3f} ms/step", file=sys.stderr, flush=True)
+            print(f"autotune order={oname} groups={g} join_every={j} "
+                  f"col_tile={t}: {ms:.3f} ms/step", file=sys.stderr,
+                  flush=True)
         return ms
 
     best_order = min(cands, key=lambda o: run(o, 1, 0))
@@ -227,9 +233,18 @@ def autotune(states: list[DomainState], cfg: RunConfig, *, steps: int = 10,
             continue
         for j in joins:
             run(best_order, g, j)
+    lay = min(trials, key=lambda t: t["ms_per_step"])
+    for t in col_tiles:
+        if t != 256:
+            run(lay["order"], lay["groups"], lay["join_every"], t)
     best = min(trials, key=lambda t: t["ms_per_step"])
+    # a trial graph of `steps` steps with join_every = 0 rejoins at its end,
+    # i.e. every `steps` steps: that is what was measured, and what a longer
+    # capture should do (never rejoining over 100 steps is slower)
+    join = best["join_every"] if best["join_every"] > 0 else steps
     out = {"order": cands[best["order"]], "order_name": best["order"],
-           "groups": best["groups"], "join_every": best["join_every"],
-           "ms_per_step": best["ms_per_step"], "trials": trials}
+           "groups": best["groups"], "join_every": join,
+           "col_tile": best["col_tile"], "ms_per_step": best["ms_per_step"],
+           "trials": trials}
     _TUNED[key] = out
     return out

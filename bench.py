@@ -591,7 +591,7 @@ def main_ensemble(args):
 
     tune = E.autotune(states, cfg, steps=10, reps=2, verbose=True)
     ens = E.Ensemble(states, cfg, order=tune["order"], groups=tune["groups"],
-                     join_every=tune["join_every"])
+                     join_every=tune["join_every"], col_tile=tune["col_tile"])
     ens.capture(n)
     # warm-up: the first replay of a fresh graph pays one-off costs
     ens.launch()
@@ -679,7 +679,8 @@ def main_ensemble(args):
                    "points": pts, "parallelism": "single GPU"},
         "ensemble": {
             "layout": {"order": tune["order_name"], "groups": tune["groups"],
-                       "join_every": tune["join_every"]},
+                       "join_every": tune["join_every"],
+                       "col_tile": tune["col_tile"]},
             "autotune_trials": tune["trials"],
             "batched_ms": round(t_batched * 1e3, 2),
             "sequential_eager_ms": round(t_seq * 1e3, 2),

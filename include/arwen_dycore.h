@@ -202,12 +202,15 @@ int32_t arw_tracer_adjoint(const arw_domain *d, const arw_params *prm,
  * groups: split the ordered members into this many groups of about equal
  *         work; each group is its own launch sequence, and a captured
  *         graph runs the groups as parallel branches (1 <= groups <= n).
+ * col_tile: columns per workgroup of the batched column solve (64, 128
+ *         or 256; results do not depend on it).
  * The handle owns its device tables until arw_ensemble_destroy.
  */
 typedef struct arw_ensemble arw_ensemble;
 int32_t arw_ensemble_create(const arw_domain *members, int32_t n,
                             const arw_params *prm, const int32_t *order,
-                            int32_t groups, arw_ensemble **out);
+                            int32_t groups, int32_t col_tile,
+                            arw_ensemble **out);
 /* nsteps steps, launched on stream. */
 int32_t arw_ensemble_step(arw_ensemble *e, int32_t nsteps, void *stream);
 /* Capture nsteps steps into one HIP graph held by the handle (replacing
@@ -700,7 +703,8 @@ int32_t arw64_small_step_adjoint_work(int32_t nz, int32_t ny, int32_t nx,
 typedef struct arw64_ensemble arw64_ensemble;
 int32_t arw64_ensemble_create(const arw64_domain *members, int32_t n,
                               const arw_params *prm, const int32_t *order,
-                              int32_t groups, arw64_ensemble **out);
+                              int32_t groups, int32_t col_tile,
+                              arw64_ensemble **out);
 int32_t arw64_ensemble_step(arw64_ensemble *e, int32_t nsteps,
                             void *stream);
 int32_t arw64_ensemble_capture(arw64_ensemble *e, int32_t nsteps,

@@ -54,18 +54,18 @@ def test_eager_ensemble_matches_member_steps_bitwise(terrain):
     _assert_same(seq, ens_states)
 
 
-@pytest.mark.parametrize("groups,order,join", [(1, None, 0),
-                                               (2, "largest_first", 0),
-                                               (3, "smallest_first", 1)])
-def test_graph_replays_match_member_steps_bitwise(groups, order, join):
+@pytest.mark.parametrize("groups,order,join,tile", [
+    (1, None, 0, 256), (2, "largest_first", 0, 128),
+    (3, "smallest_first", 1, 64)])
+def test_graph_replays_match_member_steps_bitwise(groups, order, join, tile):
     from arwen_amd import dycore
     from arwen_amd.ensemble import Ensemble, member_orders
     cfg, seq, ens_states = _pair()
-    print(f"ensemble graph groups={groups} order={order} join={join}",
-          flush=True)
+    print(f"ensemble graph groups={groups} order={order} join={join} "
+          f"col_tile={tile}", flush=True)
     ordl = member_orders(ens_states)[order] if order else None
     with Ensemble(ens_states, cfg, order=ordl, groups=groups,
-                  join_every=join) as ens:
+                  join_every=join, col_tile=tile) as ens:
         ens.capture(2)
         for rep in range(3):
             ens.launch()
@@ -103,8 +103,10 @@ def test_autotune_leaves_state_and_picks_a_layout():
         for n, t in sv.items():
             assert torch.equal(getattr(st, n), t), n
     assert res["groups"] in (1, 2) and sorted(res["order"]) == list(range(5))
-    assert res["join_every"] in (0, 1, 5)
-    assert len(res["trials"]) == 6          # 3 orders, then 3 joins x 2 groups
+    assert res["join_every"] in (1, 5)      # 0 measured as every `steps`
+    assert res["col_tile"] in (64, 128, 256)
+    # 3 orders, then 3 joins x 2 groups, then 2 more column tiles
+    assert len(res["trials"]) == 8
 
 
 def test_ensemble_rejects_bad_members():
@@ -120,6 +122,8 @@ def test_ensemble_rejects_bad_members():
     from arwen_amd import _lib
     with pytest.raises(_lib.ArwenError):
         Ensemble(states, cfg, groups=9)
+    with pytest.raises(_lib.ArwenError, match="col_tile"):
+        Ensemble(states, cfg, col_tile=96)
 
 
 @pytest.mark.parametrize("terrain", [False, True])
