@@ -811,17 +811,77 @@ STAGE_IN = ("u", "v", "w", "thp", "php", "mup")
 STAGE_OUT = ("ru_t", "rv_t", "rw_t", "rth_t", "rph_t")
 
 
-def stage_tendencies(s, dx, dy, top_lid=True):
+def stage_tendencies(s, dx, dy, top_lid=True, diff=None):
     """The slow tendencies of one RK stage at the state s (reference
-    dycore.py:2378-2389 with km_opt = 0: calc_p_alpha, stage_fluxes,
-    _add_slow_tendencies).  Mutates s's diagnostics and tendencies;
-    returns {name: tendency} and the stage fluxes (ru, rv, ww)."""
+    dycore.py:2378-2389: calc_p_alpha, stage_fluxes, _add_slow_tendencies,
+    and with ``diff`` = (khdif, kvdif) the constant-K diffusion).  Mutates
+    s's diagnostics and tendencies; returns {name: tendency} and the stage
+    fluxes (ru, rv, ww)."""
     for name in STAGE_OUT + ("rmu_t",):
         getattr(s, name)[...] = 0.0
     D.calc_p_alpha(s)
     ru, rv, ww = D.stage_fluxes(s, dx, dy)
     D.add_slow_tendencies(s, ru, rv, ww, dx, dy, top_lid)
+    if diff is not None:
+        D.add_diffusion_tendencies(s, diff[0], diff[1], dx, dy)
     return {n: getattr(s, n).copy() for n in STAGE_OUT}, (ru, rv, ww)
+
+
+def diff2_adjoint(lam, kh, kv, dx, dy, zf, stagger):
+    """Transpose of D.add_diff2 (the constant-K Laplacian on the periodic
+    core, the duplicate face copying the first, w's end levels zeroed) for
+    lam of its output: the horizontal 3-point operator is symmetric; the
+    vertical one maps mu_k = kv lam_k rdzc_k to rdzf_(k-1) (mu_(k-1) -
+    mu_k) + rdzf_k (mu_(k+1) - mu_k)."""
+    nlev, nys, nxs = lam.shape
+    nx = nxs - 1 if stagger == "x" else nxs
+    ny = nys - 1 if stagger == "y" else nys
+    rdzf, rdzc = D._diff2_spacings(zf, stagger)
+    L = np.array(lam[:, :ny, :nx], copy=True)
+    if nxs > nx:
+        L[:, :, 0] += lam[:, :ny, nx]
+    if nys > ny:
+        L[:, 0, :] += lam[:, ny, :nx]
+    if stagger == "z":
+        L[0] = 0.0
+        L[-1] = 0.0
+    g = kh * ((1.0 / dx ** 2) * (np.roll(L, 1, -1) - 2.0 * L
+                                 + np.roll(L, -1, -1))
+              + (1.0 / dy ** 2) * (np.roll(L, 1, -2) - 2.0 * L
+                                   + np.roll(L, -1, -2)))
+    mu = kv * L * rdzc[:, None, None]
+    g[1:] += rdzf[:, None, None] * (mu[:-1] - mu[1:])
+    g[:-1] += rdzf[:, None, None] * (mu[1:] - mu[:-1])
+    out = np.zeros_like(lam)
+    out[:, :ny, :nx] = g
+    return out
+
+
+def diffusion_adjoint(s, lam, kh, kv, dx, dy):
+    """(u, v, w, thp, mup) gradients of D.add_diffusion_tendencies (flat
+    base; reference diffusion.py:100-130) for lam over ru_t, rv_t, rw_t,
+    rth_t: tend += C(mu_face) diff2(field)."""
+    zf = s.phb.astype(np.float64) / D.G
+    mu = s.mub2d + s.mup
+    mux, muy = D.mu_at_u_faces(mu), D.mu_at_v_faces(mu)
+    c1h, c2h = s.c1h[:, None, None], s.c2h[:, None, None]
+    c1f, c2f = s.c1f[:, None, None], s.c2f[:, None, None]
+    g = {}
+    g_mu = np.zeros_like(mu)
+    for name, tname, muf, c1, c2, stag, ax in (
+            ("u", "ru_t", mux, c1h, c2h, "x", -1),
+            ("v", "rv_t", muy, c1h, c2h, "y", -2),
+            ("w", "rw_t", mu, c1f, c2f, "z", None),
+            ("thp", "rth_t", mu, c1h, c2h, "", None)):
+        lt = lam[tname]
+        fld = getattr(s, name)
+        tmp = D.add_diff2(fld, kh, kv, dx, dy, zf, stag, s.dtype)
+        g[name] = diff2_adjoint((c1 * muf[None] + c2) * lt, kh, kv, dx, dy,
+                                zf, stag)
+        gm = np.sum(c1 * tmp * lt, axis=0)
+        g_mu += gm if ax is None else _faces_to_mass(gm, ax)
+    g["mup"] = g_mu
+    return g
 
 
 def stage_tendencies_cs(s, dx, dy, top_lid=True):
@@ -1108,7 +1168,8 @@ def geopotential_adjoint(s, ww, lam, dx, dy):
     return g
 
 
-def stage_tendencies_adjoint(s, lam, dx, dy, top_lid=True, lam_flux=None):
+def stage_tendencies_adjoint(s, lam, dx, dy, top_lid=True, lam_flux=None,
+                             diff=None):
     """Transpose of stage_tendencies' Jacobian at the state s applied to
     lam = {STAGE_OUT name: adjoint}: returns {STAGE_IN name: gradient}
     (and qv, qc, qr for a moist state).  ``lam_flux``: adjoints of the
@@ -1119,6 +1180,10 @@ def stage_tendencies_adjoint(s, lam, dx, dy, top_lid=True, lam_flux=None):
                        for k, v in vars(s).items()})
     _, (ru, rv, ww) = stage_tendencies(work, dx, dy, top_lid)
     z = lambda n: lam.get(n, np.zeros_like(getattr(s, n)))
+    g_diff = None
+    if diff is not None and (diff[0] > 0.0 or diff[1] > 0.0):
+        g_diff = diffusion_adjoint(work, {n: z(n) for n in STAGE_OUT},
+                                   diff[0], diff[1], dx, dy)
     l_ru, l_rv, l_rw, l_rth, l_rph = (z(n) for n in STAGE_OUT)
     dxi, dyi = 1.0 / dx, 1.0 / dy
     g = {n: np.zeros_like(getattr(s, n)) for n in STAGE_IN}
@@ -1166,6 +1231,9 @@ def stage_tendencies_adjoint(s, lam, dx, dy, top_lid=True, lam_flux=None):
         g["qv"] = eos_q_adjoint(work, g_p) + g_qt
         g["qc"] = g_qt.copy()
         g["qr"] = g_qt.copy()
+    if g_diff is not None:
+        for n, v in g_diff.items():
+            g[n] += v
     return g
 
 
@@ -1413,8 +1481,11 @@ def step_adjoint(s, cfg, lam):
     moist EOS and buoyancy loading add the stage scalars' share."""
     diffusion = cfg.km_opt == 1 and (cfg.khdif > 0.0 or cfg.kvdif > 0.0)
     moist = s.qv is not None
-    if diffusion or getattr(cfg, "precision_form", False):
-        raise NotImplementedError("step adjoint: plain form, no diffusion")
+    if getattr(cfg, "precision_form", False):
+        raise NotImplementedError("step adjoint: plain form")
+    if diffusion and s.base3d:
+        raise NotImplementedError("step adjoint: diffusion needs a flat base")
+    diff = (cfg.khdif, cfg.kvdif) if diffusion else None
     if moist and getattr(cfg, "moist_adv_opt", 1) != 0:
         raise NotImplementedError("step adjoint: scalars without the PD "
                                   "limiter (moist_adv_opt = 0)")
@@ -1435,6 +1506,8 @@ def step_adjoint(s, cfg, lam):
         D.calc_p_alpha(w)
         ru, rv, ww = D.stage_fluxes(w, cfg.dx, cfg.dy)
         D.add_slow_tendencies(w, ru, rv, ww, cfg.dx, cfg.dy, cfg.top_lid)
+        if diff is not None:
+            D.add_diffusion_tendencies(w, diff[0], diff[1], cfg.dx, cfg.dy)
         D.small_step_init(w)
         coefs = D.calc_coefs(w, dtau, cfg.epssm, cfg.top_lid)
         subs = []
@@ -1516,7 +1589,8 @@ def step_adjoint(s, cfg, lam):
         if moist:
             gs["qv"] = gs["qv"] + gi["qv"]
         gt = stage_tendencies_adjoint(subs[0], {n: lt[n] for n in STAGE_OUT},
-                                      cfg.dx, cfg.dy, cfg.top_lid, lam_flux)
+                                      cfg.dx, cfg.dy, cfg.top_lid, lam_flux,
+                                      diff)
         for n, v in gt.items():
             gs[n] = gs[n] + v
         lx = gs                                # the previous stage's output

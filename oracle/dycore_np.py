@@ -577,7 +577,7 @@ def add_diffusion_tendencies(s, khdif, kvdif, dx, dy):
         return
     if s.base3d:
         raise NotImplementedError("constant-K diffusion needs a flat base")
-    zf = s.phb.astype(np.float64) / G
+    zf = np.real(s.phb).astype(np.float64) / G
     mu = s.mub2d + s.mup
     mux, muy = mu_at_u_faces(mu), mu_at_v_faces(mu)
     c1h, c2h = s.c1h[:, None, None], s.c2h[:, None, None]

@@ -513,3 +513,54 @@ def test_moist_step_adjoint_vs_complex_step(name):
         rhs = sum(float(np.sum(dx[n] * g[n])) for n in which)
         assert abs(lhs - rhs) <= 1e-9 * max(abs(lhs), 1e-30), \
             (which, lhs, rhs)
+
+
+@pytest.mark.parametrize("moist", [False, True])
+def test_step_adjoint_with_diffusion_vs_complex_step(moist):
+    """The whole-step adjoint with constant-K diffusion (km_opt = 1, K =
+    75, the Straka configuration's; flat base) against complex-step JVPs
+    of the oracle's own step, dry and moist."""
+    from tests import pins
+    case = pins.CASES["dry_flat"]()
+    cfg = case.cfg.with_(km_opt=1, khdif=75.0, kvdif=75.0, moist=moist,
+                         moist_adv_opt=0)
+    s = D.from_case(case, np.float64)
+    rng = np.random.default_rng(79)
+    s.u[...] += rng.standard_normal(s.u.shape) * 0.5
+    s.v[...] += rng.standard_normal(s.v.shape) * 0.5
+    s.w[1:-1] += rng.standard_normal(s.w[1:-1].shape) * 0.3
+    s.mup[...] += rng.standard_normal(s.mup.shape) * 20.0
+    s.php[1:] += rng.standard_normal(s.php[1:].shape) * 30.0
+    s.thp[...] += rng.standard_normal(s.thp.shape) * 0.5
+    names = A.STAGE_IN
+    if moist:
+        shp = s.thp.shape
+        s.qv = 0.008 + 0.002 * rng.standard_normal(shp)
+        s.qc = 5e-4 + 1e-4 * rng.standard_normal(shp)
+        s.qr = 5e-4 + 1e-4 * rng.standard_normal(shp)
+        for q in A.SPECIES:
+            setattr(s, q + "0", np.zeros(shp))
+        names = names + A.SPECIES
+    D.calc_p_alpha(s)
+    lam = {n: rng.standard_normal(getattr(s, n).shape) for n in names}
+    g = A.step_adjoint(s, cfg, lam)
+    h = 1e-30
+    scale = {"u": 1.0, "v": 1.0, "w": 0.1, "thp": 0.5, "php": 20.0,
+             "mup": 20.0, "qv": 1e-3, "qc": 1e-4, "qr": 1e-4}
+    saved = (D.flux5, D.flux3)
+    for which in (names,) + tuple((n,) for n in names):
+        dx = {n: rng.standard_normal(getattr(s, n).shape) * scale[n]
+              for n in which}
+        c = _cplx(s)
+        for n, d in dx.items():
+            getattr(c, n)[...] += 1j * h * d
+        D.flux5, D.flux3 = A._flux5_cs, A._flux3_cs
+        try:
+            D.step(c, cfg)
+        finally:
+            D.flux5, D.flux3 = saved
+        lhs = sum(float(np.sum(np.imag(getattr(c, n)) / h * lam[n]))
+                  for n in names)
+        rhs = sum(float(np.sum(dx[n] * g[n])) for n in which)
+        assert abs(lhs - rhs) <= 1e-9 * max(abs(lhs), 1e-30), \
+            (which, lhs, rhs)
