@@ -462,8 +462,8 @@ def stage_tendencies_adjoint(state: DomainState, cfg: RunConfig,
     moist EOS and buoyancy loading), written to ``out`` or fresh tensors,
     or added to ``out`` with ``accumulate``.  ``lam_flux``: adjoints of
     the stage fluxes ru, rv, ww handed on besides the tendencies (the
-    scalars' sumflux).  No constant-K diffusion, plain form.  The state
-    is not modified."""
+    scalars' sumflux).  Constant-K diffusion (km_opt = 1) on a flat base;
+    plain form.  The state is not modified."""
     _check_state(state)
     lib = _lib.load()
     pre = "arw64_" if state.dtype == torch.float64 else "arw_"
@@ -498,7 +498,7 @@ def stage_tendencies_adjoint(state: DomainState, cfg: RunConfig,
             setattr(a, "l_" + n, t.data_ptr())
     a.flags = _lib.ADJ_ACCUMULATE if accumulate else 0
     fn = getattr(lib, pre + "stage_tendencies_adjoint")
-    _lib.check(fn(state.domain_struct(), params(cfg), ctypes.byref(a),
+    _lib.check(fn(_dom(state, cfg), params(cfg), ctypes.byref(a),
                   work.data_ptr(), _stream()),
                pre + "stage_tendencies_adjoint")
     return out
@@ -657,9 +657,11 @@ class StepTape:
 
 
 def _check_adjoint_cfg(state, cfg):
-    if _uses_diffusion(cfg) or getattr(cfg, "precision_form", False):
-        raise _lib.ArwenError("step adjoint: no constant-K diffusion, "
-                              "plain form")
+    if getattr(cfg, "precision_form", False):
+        raise _lib.ArwenError("step adjoint: plain form only")
+    if _uses_diffusion(cfg) and state.thb.dim() == 3:
+        raise _lib.ArwenError("step adjoint: constant-K diffusion needs a "
+                              "flat base")
     if _moist(state) and getattr(cfg, "moist_adv_opt", 1) != 0:
         raise _lib.ArwenError("step adjoint: moist states need the scalars "
                               "without the PD limiter (moist_adv_opt = 0)")

@@ -510,6 +510,139 @@ __global__ void k_sa_eos(SA a)
     a.w[S_GMU3][t] += gmu;
 }
 
+// Constant-K diffusion (reference diffusion.cu:22-51, coupled in
+// diffusion.py:100-130; flat base): tend += C(mu_face) diff2(field), with
+// diff2 the Laplacian kh (fxx + fyy) + kv (fz)z on each field's periodic
+// core (the duplicate face copies the first; w's end levels are zero).
+// One thread per cell and w level gathers each field's transpose at its
+// core point -- the horizontal 3-point operator is symmetric, the vertical
+// one maps mu_k = kv L_k rdzc_k to rdzf_(k-1) (mu_(k-1) - mu_k) + rdzf_k
+// (mu_(k+1) - mu_k) -- and adds the column-mass share C'(mu) diff2 lam
+// (the u / v faces' halved onto their two cells) to the level's mass piece.
+struct DiffField {
+    const real* f;       // the field
+    const real* lam;     // adjoint of its tendency
+    const real* rdzf;    // inverse spacings of its levels
+    const real* rdzc;
+    int nlev, zero_ends;
+};
+
+__device__ __forceinline__ real mu_at(const Dom& d, int jj, int ii)
+{
+    const int ny = d.ny, nx = d.nx;
+    jj = jj < 0 ? jj + ny : (jj >= ny ? jj - ny : jj);
+    ii = ii < 0 ? ii + nx : (ii >= nx ? ii - nx : ii);
+    return mut_c(d, jj * nx + ii);
+}
+
+__global__ void k_sa_diff(SA a, Sc s)
+{
+    const Dom& d = a.d;
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny || k > nz) return;
+    const real half = R(0.5), two = R(2.0);
+    auto wi = [&](int q) { return q < 0 ? q + nx : (q >= nx ? q - nx : q); };
+    auto wj = [&](int q) { return q < 0 ? q + ny : (q >= ny ? q - ny : q); };
+    // face masses: u face f between cells f-1 and f of row jj, v face g
+    // between rows g-1 and g of column ii (periodic; face nx / ny = face 0)
+    auto mux = [&](int jj, int f) {
+        return half * (mu_at(d, jj, f) + mu_at(d, jj, f - 1));
+    };
+    auto muy = [&](int g, int ii) {
+        return half * (mu_at(d, g, ii) + mu_at(d, g - 1, ii));
+    };
+    // field values on the periodic core
+    auto U = [&](int kk, int jj, int ii) { return d.u[iU(kk, wj(jj), wi(ii), ny, nx)]; };
+    auto V = [&](int kk, int jj, int ii) { return d.v[iV(kk, wj(jj), wi(ii), ny, nx)]; };
+    auto M = [&](const real* f, int kk, int jj, int ii) {
+        return f[(kk * ny + wj(jj)) * nx + wi(ii)];
+    };
+    // the forward Laplacian of a field at a core point
+    auto lap = [&](auto F, int kk, int jj, int ii, const real* rdzf,
+                   const real* rdzc, int nlev, int zero_ends) -> real {
+        if (zero_ends && (kk == 0 || kk == nlev - 1)) return R(0.0);
+        const real fc = F(kk, jj, ii);
+        real r = s.kh * (s.dxi2 * (F(kk, jj, ii - 1) - two * fc + F(kk, jj, ii + 1))
+                         + s.dyi2 * (F(kk, jj - 1, ii) - two * fc + F(kk, jj + 1, ii)));
+        const real fup = kk <= nlev - 2 ? (F(kk + 1, jj, ii) - fc) * rdzf[kk] : R(0.0);
+        const real fdn = kk >= 1 ? (fc - F(kk - 1, jj, ii)) * rdzf[kk - 1] : R(0.0);
+        return r + s.kv * (fup - fdn) * rdzc[kk];
+    };
+    // the transpose at a core point for L(kk, jj, ii) = lambda of the
+    // Laplacian (coupled, folded, ends zeroed)
+    auto lapT = [&](auto L, int kk, int jj, int ii, const real* rdzf,
+                    const real* rdzc, int nlev) -> real {
+        const real lc = L(kk, jj, ii);
+        real g = s.kh * (s.dxi2 * (L(kk, jj, ii - 1) - two * lc + L(kk, jj, ii + 1))
+                         + s.dyi2 * (L(kk, jj - 1, ii) - two * lc + L(kk, jj + 1, ii)));
+        const real m0 = s.kv * lc * rdzc[kk];
+        if (kk >= 1) g += rdzf[kk - 1] * (s.kv * L(kk - 1, jj, ii) * rdzc[kk - 1] - m0);
+        if (kk <= nlev - 2) g += rdzf[kk] * (s.kv * L(kk + 1, jj, ii) * rdzc[kk + 1] - m0);
+        return g;
+    };
+    const int c = j * nx + i;
+    real piece = R(0.0);
+    // w (levels 0..nz)
+    {
+        auto LW = [&](int kk, int jj, int ii) -> real {
+            if (kk == 0 || kk == nz) return R(0.0);
+            const int y = (kk * ny + wj(jj)) * nx + wi(ii);
+            return (d.c1f[kk] * mu_at(d, jj, ii) + d.c2f[kk]) * a.lrw[y];
+        };
+        auto FW = [&](int kk, int jj, int ii) { return M(d.w, kk, jj, ii); };
+        a.gw[k * st + c] += lapT(LW, k, j, i, d.rdzf_w, d.rdzc_w, nz + 1);
+        piece += d.c1f[k] * lap(FW, k, j, i, d.rdzf_w, d.rdzc_w, nz + 1, 1)
+                 * a.lrw[k * st + c];
+    }
+    if (k < nz) {
+        // theta'
+        auto LT = [&](int kk, int jj, int ii) -> real {
+            const int y = (kk * ny + wj(jj)) * nx + wi(ii);
+            return (d.c1h[kk] * mu_at(d, jj, ii) + d.c2h[kk]) * a.lrth[y];
+        };
+        auto FT = [&](int kk, int jj, int ii) { return M(d.thp, kk, jj, ii); };
+        a.gthp[k * st + c] += lapT(LT, k, j, i, d.rdzf_h, d.rdzc_h, nz);
+        piece += d.c1h[k] * lap(FT, k, j, i, d.rdzf_h, d.rdzc_h, nz, 0)
+                 * a.lrth[k * st + c];
+        // u (core faces 0..nx-1; face nx folds into face 0)
+        auto LU = [&](int kk, int jj, int ii) -> real {
+            const int jr = wj(jj), f = wi(ii);
+            real v = (d.c1h[kk] * mux(jr, f) + d.c2h[kk]) * a.lru[iU(kk, jr, f, ny, nx)];
+            if (f == 0)
+                v += (d.c1h[kk] * mux(jr, nx) + d.c2h[kk]) * a.lru[iU(kk, jr, nx, ny, nx)];
+            return v;
+        };
+        a.gu[iU(k, j, i, ny, nx)] += lapT(LU, k, j, i, d.rdzf_h, d.rdzc_h, nz);
+        auto gfu = [&](int f) {        // folded mass share of u face f
+            const real t = lap(U, k, j, f, d.rdzf_h, d.rdzc_h, nz, 0);
+            real v = d.c1h[k] * t * a.lru[iU(k, j, f, ny, nx)];
+            if (f == 0) v += d.c1h[k] * t * a.lru[iU(k, j, nx, ny, nx)];
+            return v;
+        };
+        piece += half * (gfu(i) + gfu(wi(i + 1)));
+        // v (core rows 0..ny-1; row ny folds into row 0)
+        auto LV = [&](int kk, int jj, int ii) -> real {
+            const int g = wj(jj), ic = wi(ii);
+            real v = (d.c1h[kk] * muy(g, ic) + d.c2h[kk]) * a.lrv[iV(kk, g, ic, ny, nx)];
+            if (g == 0)
+                v += (d.c1h[kk] * muy(ny, ic) + d.c2h[kk]) * a.lrv[iV(kk, ny, ic, ny, nx)];
+            return v;
+        };
+        a.gv[iV(k, j, i, ny, nx)] += lapT(LV, k, j, i, d.rdzf_h, d.rdzc_h, nz);
+        auto gfv = [&](int g) {
+            const real t = lap(V, k, g, i, d.rdzf_h, d.rdzc_h, nz, 0);
+            real v = d.c1h[k] * t * a.lrv[iV(k, g, i, ny, nx)];
+            if (g == 0) v += d.c1h[k] * t * a.lrv[iV(k, ny, i, ny, nx)];
+            return v;
+        };
+        piece += half * (gfv(j) + gfv(wj(j + 1)));
+    }
+    a.w[S_GMU3][k * st + c] += piece;
+}
+
 __global__ void k_sa_php(SA a)
 {
     const Dom& d = a.d;
@@ -983,8 +1116,12 @@ extern "C" int32_t ARW_API(stage_tendencies_adjoint)(
         if (!p) return lib_fail(ARW_E_ARG, "null input adjoint");
     if (d->qv && !(adj->g_qv && adj->g_qc && adj->g_qr))
         return lib_fail(ARW_E_ARG, "stage adjoint: a moist state needs g_qv, g_qc, g_qr");
-    if (prm->khdif > 0.0 || prm->kvdif > 0.0)
-        return lib_fail(ARW_E_UNSUPPORTED, "stage adjoint: no constant-K diffusion");
+    const bool diff = prm->khdif > 0.0 || prm->kvdif > 0.0;
+    if (diff && (d->base3d || !d->rdzf_h || !d->rdzc_h || !d->rdzf_w
+                 || !d->rdzc_w))
+        return lib_fail(ARW_E_UNSUPPORTED,
+                        "stage adjoint: constant-K diffusion needs a flat base "
+                        "and the diffusion spacings");
     if (prm->precision_form)
         return lib_fail(ARW_E_UNSUPPORTED, "stage adjoint: plain form only");
     const int nz = d->nz, ny = d->ny, nx = d->nx, st = ny * nx;
@@ -1066,6 +1203,11 @@ extern "C" int32_t ARW_API(stage_tendencies_adjoint)(
     if ((rc = lib_launched(hipGetLastError(), "k_sa_flux"))) return rc;
     hipLaunchKernelGGL(k_sa_eos, dim3((nm + 255) / 256), dim3(256), 0, sm, a);
     if ((rc = lib_launched(hipGetLastError(), "k_sa_eos"))) return rc;
+    if (diff) {
+        const dim3 gd((nx + 63) / 64, (ny + 3) / 4, nz + 1);
+        hipLaunchKernelGGL(k_sa_diff, gd, blk, 0, sm, a, s);
+        if ((rc = lib_launched(hipGetLastError(), "k_sa_diff"))) return rc;
+    }
     hipLaunchKernelGGL(k_sa_php, dim3((nwl + 255) / 256), dim3(256), 0, sm, a);
     if ((rc = lib_launched(hipGetLastError(), "k_sa_php"))) return rc;
     hipLaunchKernelGGL(k_sa_mu, dim3((st + 255) / 256), dim3(256), 0, sm, a);

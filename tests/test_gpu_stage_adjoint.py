@@ -142,12 +142,33 @@ def test_stage_autograd_and_state_untouched():
 def test_stage_adjoint_rejects_unsupported():
     from arwen_amd import adjoint
     from arwen_amd._lib import ArwenError
-    case, _, g = _states("dry_flat", seed=1)
+    case, _, g = _states("terrain", seed=1)
     lam = _lam(g, 2)
-    with pytest.raises(ArwenError, match="diffusion"):
+    # refused before device work (the state's diffusion spacings need a
+    # flat base), as the forward refuses it
+    with pytest.raises((ArwenError, NotImplementedError), match="flat base"):
         adjoint.stage_tendencies_adjoint(g, case.cfg.with_(khdif=75.0,
                                                            kvdif=75.0,
                                                            km_opt=1), lam)
+
+
+def test_stage_adjoint_with_diffusion_matches_oracle():
+    """Constant-K diffusion (K = 75, flat base) in the tendency pass:
+    the device transpose equals the oracle's (itself pinned by complex-step
+    JVPs of the whole step with diffusion)."""
+    from arwen_amd import adjoint
+    case, o, g = _states("dry_flat", seed=29)
+    cfg = case.cfg.with_(km_opt=1, khdif=75.0, kvdif=75.0)
+    print("stage adjoint with diffusion vs oracle", flush=True)
+    lam = _lam(g, 31)
+    got = adjoint.stage_tendencies_adjoint(g, cfg, lam)
+    ref = A.stage_tendencies_adjoint(
+        o, {n: t.cpu().numpy() for n, t in lam.items()}, cfg.dx, cfg.dy,
+        cfg.top_lid, diff=(75.0, 75.0))
+    for n in A.STAGE_IN:
+        r = ref[n]
+        err = np.abs(got[n].cpu().numpy() - r).max()
+        assert err <= ORACLE_TOL * np.abs(r).max(), (n, err, np.abs(r).max())
 
 
 def _with_small_step_fields(o, g, seed):

@@ -326,9 +326,8 @@ def test_step_adjoint_rejects_unsupported():
     from arwen_amd._lib import ArwenError
     case, s, g, rng = _step_states("dry_flat")
     lam = {"u": torch.ones_like(g.u)}
-    with pytest.raises(ArwenError, match="constant-K"):
-        adjoint.step_adjoint(g, case.cfg.with_(khdif=75.0, kvdif=75.0,
-                                               km_opt=1), lam)
+    with pytest.raises(ArwenError, match="plain form"):
+        adjoint.step_adjoint(g, case.cfg.with_(precision_form=True), lam)
 
 
 # ---------------------------------------------------------------------------
@@ -494,3 +493,46 @@ def test_moist_dycore_step_autograd_and_stage_checkpoints():
     grads = torch.autograd.grad(loss, xs)
     for n, gr in zip(MOIST_NAMES, grads):
         assert torch.equal(gr, full[n]), n
+
+
+@pytest.mark.parametrize("moist", [False, True])
+def test_step_adjoint_with_diffusion_matches_oracle(moist):
+    """Constant-K diffusion (K = 75, km_opt = 1, the Straka
+    configuration's) in the whole-step adjoint, dry and moist, against the
+    oracle's (pinned by complex-step JVPs); and central differences of the
+    fused device step with diffusion."""
+    from arwen_amd import adjoint, dycore
+    if moist:
+        case, s, g, rng = _moist_states("dry_flat", seed=83)
+    else:
+        case, s, g, rng = _step_states("dry_flat", seed=83)
+    case.cfg = case.cfg.with_(km_opt=1, khdif=75.0, kvdif=75.0)
+    cfg = case.cfg
+    names = MOIST_NAMES if moist else A.STAGE_IN
+    print(f"step adjoint with diffusion moist={moist}", flush=True)
+    lam = {n: rng.standard_normal(getattr(s, n).shape) for n in names}
+    ref = A.step_adjoint(s, cfg, lam)
+    dev = g.u.device
+    tl = {n: torch.as_tensor(v, device=dev) for n, v in lam.items()}
+    got = adjoint.step_adjoint(g, cfg, tl)
+    bad = {n: _rel(got[n], ref[n]) for n in names}
+    assert max(bad.values()) < ORACLE_TOL, bad
+    base = {n: getattr(g, n).clone() for n in names}
+    scale = {"u": 1.0, "v": 1.0, "w": 0.1, "thp": 0.5, "php": 20.0,
+             "mup": 20.0, "qv": 1e-3, "qc": 1e-4, "qr": 1e-4}
+    v = {n: torch.as_tensor(rng.standard_normal(getattr(s, n).shape)
+                            * scale[n], device=dev) for n in names}
+    h = 1e-6
+
+    def run(sign):
+        for n in names:
+            getattr(g, n).copy_(base[n] + sign * h * v[n])
+        dycore.update_diagnostics(g)
+        dycore.step(g, cfg)
+        return {n: getattr(g, n).clone() for n in names}
+
+    fp, fm = run(1.0), run(-1.0)
+    jv = sum(float(torch.sum((fp[n] - fm[n]) / (2 * h) * tl[n]))
+             for n in names)
+    vjt = sum(float(torch.sum(v[n] * got[n])) for n in names)
+    assert abs(jv - vjt) <= 1e-6 * abs(jv), (jv, vjt)
