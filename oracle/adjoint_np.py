@@ -657,13 +657,19 @@ def _h_adjoint(q, vel, lam, ax, rd, dup):
     0..n with face n a separate copy of face 0 (its own velocity, its flux
     entering only cell n-1); else n periodic faces.  Returns (dL/dq,
     dL/dvel)."""
-    n = q.shape[ax]
     if dup:
         z = np.zeros_like(np.take(lam, [0], axis=ax))
         L = rd * (np.concatenate([lam, z], axis=ax)
                   - np.concatenate([z, lam], axis=ax))
     else:
         L = rd * (lam - np.roll(lam, 1, axis=ax))
+    return _h_faces_adjoint(q, vel, L, ax, dup)
+
+
+def _h_faces_adjoint(q, vel, L, ax, dup):
+    """Transpose of the flux5 face fluxes along axis ax for the face
+    adjoints L (same layout as vel): (dL/dq, dL/dvel)."""
+    n = q.shape[ax]
     a, sv = np.abs(vel), np.sign(vel)
     gq = np.zeros_like(q)
     dv = np.zeros_like(vel)
@@ -686,19 +692,28 @@ def _zhalf_adjoint(q, vel, lam, rdz, fnm, fnp):
     """Vertical transport of a half-level field (faces kf = 1..nz-1, 2nd
     order one face in from each boundary, flux3 inside)."""
     nz = q.shape[0]
+    L = np.zeros_like(vel)
+    for kf in range(1, nz):
+        L[kf] = rdz[kf] * lam[kf] - rdz[kf - 1] * lam[kf - 1]
+    return _zhalf_faces_adjoint(q, vel, L, fnm, fnp)
+
+
+def _zhalf_faces_adjoint(q, vel, L, fnm, fnp):
+    """Transpose of the half-level vertical face fluxes (dycore_np.
+    _zfaces_half) for the face adjoints L (w levels; 0 and nz unused)."""
+    nz = q.shape[0]
     gq = np.zeros_like(q)
     dv = np.zeros_like(vel)
     for kf in range(1, nz):
-        L = rdz[kf] * lam[kf] - rdz[kf - 1] * lam[kf - 1]
         v = vel[kf]
         if kf == 1 or kf == nz - 1:
-            gq[kf] += v * fnm[kf] * L
-            gq[kf - 1] += v * fnp[kf] * L
-            dv[kf] = L * (fnm[kf] * q[kf] + fnp[kf] * q[kf - 1])
+            gq[kf] += v * fnm[kf] * L[kf]
+            gq[kf - 1] += v * fnp[kf] * L[kf]
+            dv[kf] = L[kf] * (fnm[kf] * q[kf] + fnp[kf] * q[kf - 1])
         else:
             for o in range(4):               # face kf reads cell kf + o - 2
-                gq[kf + o - 2] += (C3[o] * v + S3[o] * np.abs(v)) * L
-                dv[kf] += L * (C3[o] + S3[o] * np.sign(v)) * q[kf + o - 2]
+                gq[kf + o - 2] += (C3[o] * v + S3[o] * np.abs(v)) * L[kf]
+                dv[kf] += L[kf] * (C3[o] + S3[o] * np.sign(v)) * q[kf + o - 2]
     return gq, dv
 
 
@@ -1464,12 +1479,181 @@ def scalar_update_adjoint(w, q, q0, fl, mu0, mu, dt_stage, clamp, lam, dx,
             "mu": -np.sum(c1 * v * lv / chm, axis=0)}
 
 
+def _upwind_adjoint(qm1, q0v, cr, g):
+    """Transpose of moist_np.flux_upwind for g = adjoint of its output:
+    (dqm1, dq0v, dcr); |cr| differentiated as sign(cr), the CFL caps at
+    +-1/2 as constants."""
+    a = cr + np.abs(cr)
+    b = cr - np.abs(cr)
+    sg = np.sign(cr)
+    da = np.where(a < 1.0, 1.0 + sg, 0.0)
+    db = np.where(b > -1.0, 1.0 - sg, 0.0)
+    return (0.5 * np.fmin(1.0, a) * g, 0.5 * np.fmax(-1.0, b) * g,
+            0.5 * g * (qm1 * da + q0v * db))
+
+
+def _dup_fold(g, ax):
+    """Transpose of appending a copy of index 0 along ax (n+1 -> n)."""
+    n = g.shape[ax] - 1
+    out = np.take(g, range(n), axis=ax).copy()
+    idx = [slice(None)] * g.ndim
+    idx[ax] = slice(0, 1)
+    out[tuple(idx)] += np.take(g, [n], axis=ax)
+    return out
+
+
+def pd_update_adjoint(w, q, q0, fl, mu0, mu, dt_stage, lam, dx, dy):
+    """Transpose of one species' final-stage update with WRF's
+    positive-definite limiter (moist_np.advance_scalars_stage with
+    moist_adv_opt = 1: pd_fluxes -> pd_scale -> pd_apply -> the clamped
+    coupled update; reference pd_advection.cu, moist.py:385-398,
+    :493-494) for lam = adjoint of q_new.  Returns the gradients of q,
+    q0, the averaged fluxes ru, rv, ww and the column masses mu0 (time t)
+    and mu (after the stage).  Piecewise: the clamp at zero, each
+    correction's donor choice (the sign of the correction flux), the
+    factor's clamps to [0, 1] and the upwind CFL caps are differentiated
+    as the branch taken; |cr| as sign(cr)."""
+    from oracle import moist_np as M
+    f = q.dtype.type
+    ru, rv, rw = fl
+    nz, ny, nx = q.shape
+    dxi = float(np.real(f(1.0 / dx)))
+    dyi = float(np.real(f(1.0 / dy)))
+    dt = float(dt_stage)
+    c1, c2 = w.c1h[:, None, None], w.c2h[:, None, None]
+    rz = w.rdnw[:, None, None]
+    fx = M.pd_fluxes(q, q0, ru, rv, rw, mu, w.c1h, w.c2h, w.rdnw, w.fnm,
+                     w.fnp, dx, dy, dt)
+    fxl, fxc, fyl, fyc, fzl, fzc = fx
+    sc = M.pd_scale(q0, mu0, w.c1h, w.c2h, w.rdnw, fx, dxi, dyi, dt)
+    tend = f(0.0) + M.pd_apply(sc, fx, w.rdnw, dxi, dyi)
+    chm0 = c1 * mu0[None] + c2
+    chm = c1 * mu[None] + c2
+    v = (chm0 * q0 + f(dt) * tend) / chm
+    lv = np.where(v > 0.0, lam, 0.0)
+    g_tend = dt * lv / chm
+    g_q0 = chm0 * lv / chm
+    g_mu0 = np.sum(c1 * q0 * lv / chm, axis=0)
+    g_mu = -np.sum(c1 * v * lv / chm, axis=0)
+    # pd_apply
+    sx_l, sx_r, sy_l, sy_r, sz_b, sz_t = M.pd_donor_scales(sc, fx)
+    gfxl, gfxc = np.zeros_like(fxl), np.zeros_like(fxc)
+    gfyl, gfyc = np.zeros_like(fyl), np.zeros_like(fyc)
+    gfzl, gfzc = np.zeros_like(fzl), np.zeros_like(fzc)
+    ggx, ggy, ggz = -dxi * g_tend, -dyi * g_tend, -rz * g_tend
+    gfxc[..., 1:] += ggx * sx_r
+    gfxc[..., :-1] -= ggx * sx_l
+    gfxl[..., 1:] += ggx
+    gfxl[..., :-1] -= ggx
+    gfyc[:, 1:] += ggy * sy_r
+    gfyc[:, :-1] -= ggy * sy_l
+    gfyl[:, 1:] += ggy
+    gfyl[:, :-1] -= ggy
+    gfzc[1:] += ggz * sz_t
+    gfzc[:nz] -= ggz * sz_b
+    gfzl[1:] += ggz
+    gfzl[:nz] -= ggz
+    # each factor's adjoint back to its donor cell
+    g_sc = np.zeros_like(sc)
+    gs = ggx * fxc[..., 1:]                  # sx_r
+    g_sc += np.where(fxc[..., 1:] > 0.0, gs, 0.0)
+    g_sc += M._pm(np.where(fxc[..., 1:] < 0.0, gs, 0.0), -1, 2)
+    gs = -ggx * fxc[..., :-1]                # sx_l
+    g_sc += M._pm(np.where(fxc[..., :-1] > 0.0, gs, 0.0), 1, 2)
+    g_sc += np.where(fxc[..., :-1] < 0.0, gs, 0.0)
+    gs = ggy * fyc[:, 1:]                    # sy_r
+    g_sc += np.where(fyc[:, 1:] > 0.0, gs, 0.0)
+    g_sc += M._pm(np.where(fyc[:, 1:] < 0.0, gs, 0.0), -1, 1)
+    gs = -ggy * fyc[:, :-1]                  # sy_l
+    g_sc += M._pm(np.where(fyc[:, :-1] > 0.0, gs, 0.0), 1, 1)
+    g_sc += np.where(fyc[:, :-1] < 0.0, gs, 0.0)
+    gs = ggz * fzc[1:]                       # sz_t: self, or above
+    g_sc += np.where(fzc[1:] < 0.0, gs, 0.0)
+    ga = np.where(fzc[1:] > 0.0, gs, 0.0)
+    g_sc[1:] += ga[:-1]
+    g_sc[-1] += ga[-1]
+    gs = -ggz * fzc[:nz]                     # sz_b: self, or below
+    g_sc += np.where(fzc[:nz] > 0.0, gs, 0.0)
+    gb = np.where(fzc[:nz] < 0.0, gs, 0.0)
+    g_sc[:-1] += gb[1:]
+    g_sc[0] += gb[0]
+    # pd_scale: the factor inside its clamps
+    ph_low, fo = M.pd_budget(q0, mu0, w.c1h, w.c2h, w.rdnw, fx, dxi, dyi,
+                             dt)
+    den = fo + 1e-20
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ratio = ph_low / den
+        inside = (fo > ph_low) & (ratio > 0.0) & (ratio < 1.0)
+        g_pl = np.where(inside, g_sc / den, 0.0)
+        g_fo = np.where(inside, -g_sc * ratio / den, 0.0)
+    g_q0 += chm0 * g_pl
+    g_mu0 += np.sum(c1 * q0 * g_pl, axis=0)
+    g_dl = -dt * g_pl
+    gfxl[..., 1:] += dxi * g_dl
+    gfxl[..., :-1] -= dxi * g_dl
+    gfyl[:, 1:] += dyi * g_dl
+    gfyl[:, :-1] -= dyi * g_dl
+    gfzl[1:] += rz * g_dl
+    gfzl[:-1] -= rz * g_dl
+    a = dt * g_fo
+    gfxc[..., 1:] += np.where(fxc[..., 1:] > 0.0, dxi * a, 0.0)
+    gfxc[..., :-1] -= np.where(fxc[..., :-1] < 0.0, dxi * a, 0.0)
+    gfyc[:, 1:] += np.where(fyc[:, 1:] > 0.0, dyi * a, 0.0)
+    gfyc[:, :-1] -= np.where(fyc[:, :-1] < 0.0, dyi * a, 0.0)
+    gfzc[1:] += np.where(fzc[1:] < 0.0, rz * a, 0.0)
+    gfzc[:-1] -= np.where(fzc[:-1] > 0.0, rz * a, 0.0)
+    # pd_fluxes: correction = high-order flux of q - upwind flux of q0
+    g_q = np.zeros_like(q)
+    gq_h, g_ru = _h_faces_adjoint(q, ru, gfxc, 2, True)
+    g_q += gq_h
+    gq_h, g_rv = _h_faces_adjoint(q, rv, gfyc, 1, True)
+    g_q += gq_h
+    gzc = gfzc.copy()
+    gzc[0] = 0.0
+    gzc[nz] = 0.0
+    gq_h, g_rw = _zhalf_faces_adjoint(q, rw, gzc, w.fnm, w.fnp)
+    g_q += gq_h
+    glx, gly, glz = gfxl - gfxc, gfyl - gfyc, gfzl - gfzc
+    for ax, rvel, gl, dd, m_ax in ((2, ru, glx, dx, 1), (1, rv, gly, dy, 0)):
+        mb = (c1 * 0.5) * (mu + M._pm(mu, -1, m_ax))[None] + c2
+        mf = np.concatenate([mb, np.take(mb, [0], axis=ax)], axis=ax)
+        cr = ((rvel * dt) / dd) / mf
+        qm1 = M._pm(q0, -1, ax)
+        qm1 = np.concatenate([qm1, np.take(qm1, [0], axis=ax)], axis=ax)
+        qc0 = np.concatenate([q0, np.take(q0, [0], axis=ax)], axis=ax)
+        fu = M.flux_upwind(qm1, qc0, cr)
+        g_fu = mf * (dd / dt) * gl
+        dqm1, dqc0, dcr = _upwind_adjoint(qm1, qc0, cr, g_fu)
+        g_m = (dd / dt) * fu * gl - dcr * cr / mf
+        gvel = dcr * (dt / dd) / mf
+        if ax == 2:
+            g_ru = g_ru + gvel
+        else:
+            g_rv = g_rv + gvel
+        g_q0 += M._pm(_dup_fold(dqm1, ax), 1, ax) + _dup_fold(dqc0, ax)
+        sm = np.sum((c1 * 0.5) * _dup_fold(g_m, ax), axis=0)
+        g_mu += sm + M._pm(sm, 1, m_ax)
+    for k in range(1, nz):
+        dz = 2.0 / (w.rdnw[k] + w.rdnw[k - 1])
+        muf = w.c1h[k] * mu + w.c2h[k]
+        crz = (rw[k] * dt) / (dz * muf)
+        fu = M.flux_upwind(q0[k - 1], q0[k], crz)
+        g_fu = muf * (dz / dt) * glz[k]
+        dqm1, dqc0, dcr = _upwind_adjoint(q0[k - 1], q0[k], crz, g_fu)
+        g_q0[k - 1] += dqm1
+        g_q0[k] += dqc0
+        g_rw[k] += dcr * dt / (dz * muf)
+        g_mu += w.c1h[k] * ((dz / dt) * fu * glz[k] - dcr * crz / muf)
+    return {"q": g_q, "q0": g_q0, "ru": g_ru, "rv": g_rv, "ww": g_rw,
+            "mu0": g_mu0, "mu": g_mu}
+
+
 def step_adjoint(s, cfg, lam):
     """J^T lam for one dycore step (D.step) at the state s: ``lam`` maps
     the new state's STAGE_IN fields (and qv, qc, qr on a moist state) to
     adjoints; returns the gradients of the step's input fields.  Plain
-    form, no diffusion; a moist state's scalars without the PD limiter
-    (moist_adv_opt = 0).  The forward is re-run on a copy, checkpointing
+    form; constant-K diffusion on a flat base; a moist state's scalars with
+    or without the final stage's PD limiter (moist_adv_opt 1 or 0).  The forward is re-run on a copy, checkpointing
     each stage's state and each substep's input; s is not modified.
 
     Moist: the scalars' stage update runs after the finish with the
@@ -1486,9 +1670,7 @@ def step_adjoint(s, cfg, lam):
     if diffusion and s.base3d:
         raise NotImplementedError("step adjoint: diffusion needs a flat base")
     diff = (cfg.khdif, cfg.kvdif) if diffusion else None
-    if moist and getattr(cfg, "moist_adv_opt", 1) != 0:
-        raise NotImplementedError("step adjoint: scalars without the PD "
-                                  "limiter (moist_adv_opt = 0)")
+    pd_opt = getattr(cfg, "moist_adv_opt", 1)
     names = STAGE_IN + (SPECIES if moist else ())
     ns = cfg.time_step_sound
     stages = ((1, cfg.dt / 3.0), (max(ns // 2, 1), cfg.dt / ns),
@@ -1529,7 +1711,7 @@ def step_adjoint(s, cfg, lam):
             ex = {"q": qst, "fl": fl, "mu": w.mub2d + w.mup,
                   "final": istage == 2, "dt": nsub * dtau}
             D.advance_scalars_stage(w, fl[0], fl[1], fl[2], nsub * dtau,
-                                    istage == 2, cfg.dx, cfg.dy, 0)
+                                    istage == 2, cfg.dx, cfg.dy, pd_opt)
         ck.append((after, subs, coefs, dtau, nsub, ex))
     # reverse
     g0 = {n: np.zeros_like(getattr(s, n)) for n in names}   # time-t
@@ -1549,10 +1731,17 @@ def step_adjoint(s, cfg, lam):
             lam_flux = {"ru": 0.0, "rv": 0.0, "ww": 0.0}
             g_mu = 0.0
             for q in SPECIES:
-                r = scalar_update_adjoint(after_subs, ex["q"][q],
+                if ex["final"] and pd_opt == 1:
+                    r = pd_update_adjoint(after_subs, ex["q"][q],
                                           getattr(s, q), ex["fl"], mu0,
-                                          ex["mu"], ex["dt"], ex["final"],
-                                          lx[q], cfg.dx, cfg.dy)
+                                          ex["mu"], ex["dt"], lx[q], cfg.dx,
+                                          cfg.dy)
+                else:
+                    r = scalar_update_adjoint(after_subs, ex["q"][q],
+                                              getattr(s, q), ex["fl"], mu0,
+                                              ex["mu"], ex["dt"],
+                                              ex["final"], lx[q], cfg.dx,
+                                              cfg.dy)
                 gq[q] = r["q"]
                 g0[q] += r["q0"]
                 g0["mup"] += r["mu0"]

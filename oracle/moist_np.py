@@ -27,6 +27,30 @@ def _pm(a, shift, axis):
     return np.roll(a, -shift, axis=axis)
 
 
+# min, max, |x| and ordering tests that also carry complex-step
+# perturbations: on real arrays they are np.fmin, np.fmax, np.abs and the
+# plain comparisons; on complex arrays the branch follows the real part
+# and the derivative rides in the imaginary part.
+def _re(x):
+    return np.real(x) if np.iscomplexobj(x) else x
+
+
+def _abs(x):
+    return np.sign(np.real(x)) * x if np.iscomplexobj(x) else np.abs(x)
+
+
+def _fmin(a, b):
+    if np.iscomplexobj(a) or np.iscomplexobj(b):
+        return np.where(_re(a) <= _re(b), a, b)
+    return np.fmin(a, b)
+
+
+def _fmax(a, b):
+    if np.iscomplexobj(a) or np.iscomplexobj(b):
+        return np.where(_re(a) >= _re(b), a, b)
+    return np.fmax(a, b)
+
+
 def coupled_update(q0, tend, c1h, c2h, mu0, mu, dt_eff, clamp):
     """WRF rk_update_scalar (reference moist.py:385-398): q = (C(mu_t) q_t
     + dt tend) / C(mu_new); the final stage clamps at zero (NaN kept)."""
@@ -47,8 +71,8 @@ def flux_upwind(qm1, q0v, cr):
     """WRF flux_upwind (reference pd_advection.cu:101-108)."""
     f = cr.dtype.type
     half, one = f(0.5), f(1.0)
-    return ((half * np.fmin(one, cr + np.abs(cr))) * qm1
-            + (half * np.fmax(-one, cr - np.abs(cr))) * q0v)
+    return ((half * _fmin(one, cr + _abs(cr))) * qm1
+            + (half * _fmax(-one, cr - _abs(cr))) * q0v)
 
 
 def pd_fluxes(q, q0, ru, rv, rw, mut, c1h, c2h, rdnw, fnm, fnp, dx, dy,
@@ -90,13 +114,13 @@ def pd_fluxes(q, q0, ru, rv, rw, mut, c1h, c2h, rdnw, fnm, fnp, dx, dy,
     return fxl, fxc, fyl, fyc, fzl, fzc
 
 
-def pd_scale(q0, mu_old, c1h, c2h, rdnw, fl, dx_inv, dy_inv, dt):
-    """Per-cell renormalisation factor (reference pd_advection.cu
-    pd_scale, periodic no-msf branch): where the outgoing correction flux
-    exceeds the upwind-updated coupled scalar, scale it down to fit."""
+def pd_budget(q0, mu_old, c1h, c2h, rdnw, fl, dx_inv, dy_inv, dt):
+    """(ph_low, fo) of every cell (reference pd_advection.cu pd_scale,
+    periodic no-msf branch): the coupled scalar after the upwind update,
+    and the outgoing correction flux over the stage."""
     fxl, fxc, fyl, fyc, fzl, fzc = fl
     f = q0.dtype.type
-    zero, one = f(0.0), f(1.0)
+    zero = f(0.0)
     dxi, dyi, dtf = f(dx_inv), f(dy_inv), f(dt)
     rz = rdnw[:, None, None]
     chm0 = c1h[:, None, None] * mu_old[None] + c2h[:, None, None]
@@ -104,14 +128,51 @@ def pd_scale(q0, mu_old, c1h, c2h, rdnw, fl, dx_inv, dy_inv, dt):
           + dyi * (fyl[:, 1:] - fyl[:, :-1])) + rz * (fzl[1:] - fzl[:-1])
     ph_low = chm0 * q0 - dtf * dl
     # fminf/fmaxf semantics (a NaN operand yields the other one)
-    fo = dtf * ((dxi * (np.fmax(zero, fxc[..., 1:])
-                        - np.fmin(zero, fxc[..., :-1]))
-                 + dyi * (np.fmax(zero, fyc[:, 1:])
-                          - np.fmin(zero, fyc[:, :-1])))
-                + rz * (np.fmin(zero, fzc[1:]) - np.fmax(zero, fzc[:-1])))
+    fo = dtf * ((dxi * (_fmax(zero, fxc[..., 1:])
+                        - _fmin(zero, fxc[..., :-1]))
+                 + dyi * (_fmax(zero, fyc[:, 1:])
+                          - _fmin(zero, fyc[:, :-1])))
+                + rz * (_fmin(zero, fzc[1:]) - _fmax(zero, fzc[:-1])))
+    return ph_low, fo
+
+
+def pd_scale(q0, mu_old, c1h, c2h, rdnw, fl, dx_inv, dy_inv, dt):
+    """Per-cell renormalisation factor (reference pd_advection.cu
+    pd_scale, periodic no-msf branch): where the outgoing correction flux
+    exceeds the upwind-updated coupled scalar, scale it down to fit."""
+    f = q0.dtype.type
+    zero, one = f(0.0), f(1.0)
+    ph_low, fo = pd_budget(q0, mu_old, c1h, c2h, rdnw, fl, dx_inv, dy_inv,
+                           dt)
     with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
-        r = np.fmin(one, np.fmax(zero, ph_low / (fo + f(1e-20))))
-    return np.where(fo > ph_low, r, one)
+        r = _fmin(one, _fmax(zero, ph_low / (fo + f(1e-20))))
+    return np.where(_re(fo) > _re(ph_low), r, one)
+
+
+def pd_donor_scales(sc, fl):
+    """The donor cell's factor for each correction flux as each cell sees
+    it (reference pd_advection.cu pd_renorm_apply): (sx_l, sx_r, sy_l,
+    sy_r, sz_b, sz_t), 1 where the correction vanishes."""
+    _, fxc, _, fyc, _, fzc = fl
+    f = sc.dtype.type
+    zero, one = f(0.0), f(1.0)
+    nz = sc.shape[0]
+    fxc_l, fxc_r = _re(fxc[..., :-1]), _re(fxc[..., 1:])
+    sx_l = np.where(fxc_l > zero, _pm(sc, -1, 2),
+                    np.where(fxc_l < zero, sc, one))
+    sx_r = np.where(fxc_r > zero, sc,
+                    np.where(fxc_r < zero, _pm(sc, 1, 2), one))
+    fyc_l, fyc_r = _re(fyc[:, :-1]), _re(fyc[:, 1:])
+    sy_l = np.where(fyc_l > zero, _pm(sc, -1, 1),
+                    np.where(fyc_l < zero, sc, one))
+    sy_r = np.where(fyc_r > zero, sc,
+                    np.where(fyc_r < zero, _pm(sc, 1, 1), one))
+    below = np.concatenate([sc[:1], sc[:-1]], axis=0)      # scale(k-1)
+    above = np.concatenate([sc[1:], sc[-1:]], axis=0)      # scale(k+1)
+    fzc_b, fzc_t = _re(fzc[:nz]), _re(fzc[1:])
+    sz_b = np.where(fzc_b > zero, sc, np.where(fzc_b < zero, below, one))
+    sz_t = np.where(fzc_t < zero, sc, np.where(fzc_t > zero, above, one))
+    return sx_l, sx_r, sy_l, sy_r, sz_b, sz_t
 
 
 def pd_apply(sc, fl, rdnw, dx_inv, dy_inv):
@@ -120,27 +181,14 @@ def pd_apply(sc, fl, rdnw, dx_inv, dy_inv):
     scaled by its donor cell's factor."""
     fxl, fxc, fyl, fyc, fzl, fzc = fl
     f = sc.dtype.type
-    zero, one = f(0.0), f(1.0)
     dxi, dyi = f(dx_inv), f(dy_inv)
     nz = sc.shape[0]
-    fxc_l, fxc_r = fxc[..., :-1], fxc[..., 1:]
-    sx_l = np.where(fxc_l > zero, _pm(sc, -1, 2),
-                    np.where(fxc_l < zero, sc, one))
-    sx_r = np.where(fxc_r > zero, sc,
-                    np.where(fxc_r < zero, _pm(sc, 1, 2), one))
-    fyc_l, fyc_r = fyc[:, :-1], fyc[:, 1:]
-    sy_l = np.where(fyc_l > zero, _pm(sc, -1, 1),
-                    np.where(fyc_l < zero, sc, one))
-    sy_r = np.where(fyc_r > zero, sc,
-                    np.where(fyc_r < zero, _pm(sc, 1, 1), one))
-    below = np.concatenate([sc[:1], sc[:-1]], axis=0)      # scale(k-1)
-    above = np.concatenate([sc[1:], sc[-1:]], axis=0)      # scale(k+1)
-    fzc_b, fzc_t = fzc[:nz], fzc[1:]
-    sz_b = np.where(fzc_b > zero, sc, np.where(fzc_b < zero, below, one))
-    sz_t = np.where(fzc_t < zero, sc, np.where(fzc_t > zero, above, one))
-    gx = (sx_r * fxc_r + fxl[..., 1:]) - (sx_l * fxc_l + fxl[..., :-1])
-    gy = (sy_r * fyc_r + fyl[:, 1:]) - (sy_l * fyc_l + fyl[:, :-1])
-    gz = (sz_t * fzc_t + fzl[1:]) - (sz_b * fzc_b + fzl[:nz])
+    sx_l, sx_r, sy_l, sy_r, sz_b, sz_t = pd_donor_scales(sc, fl)
+    gx = (sx_r * fxc[..., 1:] + fxl[..., 1:]) - (sx_l * fxc[..., :-1]
+                                                 + fxl[..., :-1])
+    gy = (sy_r * fyc[:, 1:] + fyl[:, 1:]) - (sy_l * fyc[:, :-1]
+                                             + fyl[:, :-1])
+    gz = (sz_t * fzc[1:] + fzl[1:]) - (sz_b * fzc[:nz] + fzl[:nz])
     return ((-dxi) * gx - dyi * gy) - rdnw[:, None, None] * gz
 
 

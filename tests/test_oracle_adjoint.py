@@ -464,16 +464,17 @@ def test_whole_step_adjoint_vs_complex_step(name):
             (which, lhs, rhs)
 
 
-@pytest.mark.parametrize("name", ["dry_flat", "terrain"])
-def test_moist_step_adjoint_vs_complex_step(name):
+@pytest.mark.parametrize("name,pd", [("dry_flat", 0), ("terrain", 0),
+                                     ("dry_flat", 1), ("terrain", 1)])
+def test_moist_step_adjoint_vs_complex_step(name, pd):
     """The hand adjoint of one moist RK3 step (qv/qc/qr through the moist
     EOS, the buoyancy loading and the sumflux-coupled scalar update,
-    moist_adv_opt = 0) against complex-step JVPs of the oracle's own step.
-    The scalars stay positive, away from the final stage's clamp at zero
-    (not differentiable there)."""
+    without (moist_adv_opt = 0) and with (1) the final stage's PD limiter)
+    against complex-step JVPs of the oracle's own step.  With the limiter,
+    qv is rough (a third of the cells nearly empty) so that it engages."""
     from tests import pins
     case = pins.CASES[name]()
-    cfg = case.cfg.with_(moist=True, moist_adv_opt=0)
+    cfg = case.cfg.with_(moist=True, moist_adv_opt=pd)
     s = D.from_case(case, np.float64)
     rng = np.random.default_rng(59)
     s.u[...] += rng.standard_normal(s.u.shape) * 0.5
@@ -485,6 +486,9 @@ def test_moist_step_adjoint_vs_complex_step(name):
     s.qv = 0.008 + 0.002 * rng.standard_normal(shp)
     s.qc = 5e-4 + 1e-4 * rng.standard_normal(shp)
     s.qr = 5e-4 + 1e-4 * rng.standard_normal(shp)
+    if pd:
+        s.qv = np.abs(s.qv - 0.008) * 3.0
+        s.qv[rng.random(shp) < 0.3] *= 1e-3
     for q in A.SPECIES:
         setattr(s, q + "0", np.zeros(shp))
     D.set_w_surface(s, cfg.dx, cfg.dy)
@@ -564,3 +568,145 @@ def test_step_adjoint_with_diffusion_vs_complex_step(moist):
         rhs = sum(float(np.sum(dx[n] * g[n])) for n in which)
         assert abs(lhs - rhs) <= 1e-9 * max(abs(lhs), 1e-30), \
             (which, lhs, rhs)
+
+
+# ---------------------------------------------------------------------------
+# The final stage's positive-definite limiter
+# ---------------------------------------------------------------------------
+
+def _pd_case(seed):
+    """A flat-base state after its acoustic substeps, with a rough
+    non-negative scalar (some cells nearly empty beside full ones) and
+    averaged fluxes strong enough that the PD limiter scales some
+    corrections (0 < factor < 1), zeroes others and leaves the rest."""
+    from oracle import moist_np as M
+    from tests import pins
+    case = pins.CASES["dry_flat"]()
+    s = D.from_case(case, np.float64)
+    rng = np.random.default_rng(seed)
+    shp = s.thp.shape
+    q0 = np.abs(rng.standard_normal(shp)) * 4e-3
+    q0[rng.random(shp) < 0.3] *= 1e-3
+    s.qv0 = q0
+    s.qv = q0 * (1.0 + 0.2 * rng.standard_normal(shp))
+    s.qc = s.qc0 = np.zeros(shp)
+    s.qr = s.qr0 = np.zeros(shp)
+    s.mup0 = s.mup + rng.standard_normal(s.mup.shape) * 30.0
+    mu = s.mub2d + s.mup
+    cfl = 0.35
+    cu = (np.concatenate([mu, mu[:, :1]], axis=1)[None] * s.c1h[:, None, None]
+          + s.c2h[:, None, None])
+    ru = rng.standard_normal(s.u.shape) * cfl * cu * case.cfg.dx / case.cfg.dt
+    cv = (np.concatenate([mu, mu[:1]], axis=0)[None] * s.c1h[:, None, None]
+          + s.c2h[:, None, None])
+    rv = rng.standard_normal(s.v.shape) * cfl * cv * case.cfg.dy / case.cfg.dt
+    ww = rng.standard_normal(s.w.shape) * 0.05 * s.mub2d.mean()
+    ww[0] = 0.0
+    ww[-1] = 0.0
+    return case, s, (ru, rv, ww), M
+
+
+def _pd_forward(s, fl, cfg, M):
+    """q_new of the final stage with the limiter (moist_np's own forward
+    over the three species; only qv is nonzero)."""
+    c = D.OState(**{k: (v.copy() if isinstance(v, np.ndarray) else v)
+                    for k, v in vars(s).items()})
+    M.advance_scalars_stage(c, fl[0], fl[1], fl[2], cfg.dt, True, cfg.dx,
+                            cfg.dy, 1)
+    return c.qv
+
+
+@pytest.mark.parametrize("seed", [3, 17])
+def test_pd_update_adjoint_vs_complex_step(seed):
+    """The hand transpose of the limited final-stage update (pd_fluxes ->
+    pd_scale -> pd_apply -> clamped coupled update) against complex-step
+    JVPs of moist_np's own forward, in every input at once and one at a
+    time, on states where the limiter is partly active."""
+    case, s, fl, M = _pd_case(seed)
+    cfg = case.cfg
+    mu0 = s.mub2d + s.mup0
+    mu = s.mub2d + s.mup
+    # the limiter is exercised: some factors strictly inside (0, 1)
+    fx = M.pd_fluxes(s.qv, s.qv0, fl[0], fl[1], fl[2], mu, s.c1h, s.c2h,
+                     s.rdnw, s.fnm, s.fnp, cfg.dx, cfg.dy, cfg.dt)
+    sc = M.pd_scale(s.qv0, mu0, s.c1h, s.c2h, s.rdnw, fx, 1.0 / cfg.dx,
+                    1.0 / cfg.dy, cfg.dt)
+    inside = np.count_nonzero((sc > 0.0) & (sc < 1.0))
+    assert inside > 10 and np.count_nonzero(sc == 1.0) > 10, inside
+    qn = _pd_forward(s, fl, cfg, M)
+    assert np.count_nonzero(qn > 0.0) > qn.size // 2
+    rng = np.random.default_rng(seed + 1)
+    lam = rng.standard_normal(s.qv.shape)
+    g = A.pd_update_adjoint(s, s.qv, s.qv0, fl, mu0, mu, cfg.dt, lam,
+                            cfg.dx, cfg.dy)
+    h = 1e-30
+    names = ("q", "q0", "ru", "rv", "ww", "mu0", "mu")
+    scale = {"q": 1e-4, "q0": 1e-4, "ru": 1e3, "rv": 1e3, "ww": 30.0,
+             "mu0": 20.0, "mu": 20.0}
+    shapes = {"q": s.qv.shape, "q0": s.qv.shape, "ru": fl[0].shape,
+              "rv": fl[1].shape, "ww": fl[2].shape, "mu0": s.mup.shape,
+              "mu": s.mup.shape}
+    saved = (D.flux5, D.flux3)
+    for which in (names,) + tuple((n,) for n in names):
+        dx = {n: rng.standard_normal(shapes[n]) * scale[n] for n in which}
+        if "ww" in dx:
+            dx["ww"][0] = dx["ww"][-1] = 0.0
+        c = _cplx(s)
+        cfl = [a.astype(np.complex128) for a in fl]
+        for n, d in dx.items():
+            if n == "q":
+                c.qv = c.qv + 1j * h * d
+            elif n == "q0":
+                c.qv0 = c.qv0 + 1j * h * d
+            elif n == "mu0":
+                c.mup0 = c.mup0 + 1j * h * d
+            elif n == "mu":
+                c.mup = c.mup + 1j * h * d
+            else:
+                cfl[("ru", "rv", "ww").index(n)] += 1j * h * d
+        D.flux5, D.flux3 = A._flux5_cs, A._flux3_cs
+        try:
+            qc = _pd_forward(c, cfl, cfg, M)
+        finally:
+            D.flux5, D.flux3 = saved
+        lhs = float(np.sum(np.imag(qc) / h * lam))
+        rhs = sum(float(np.sum(dx[n] * g[n])) for n in which)
+        assert abs(lhs - rhs) <= 1e-9 * max(abs(lhs), 1e-30), \
+            (which, lhs, rhs)
+
+
+def test_pd_update_adjoint_planted_mistake_fails():
+    """The check above is sharp: the same transpose with the upwind flux's
+    Courant-number derivative dropped (its mass and velocity share)
+    fails the dot-product identity."""
+    case, s, fl, M = _pd_case(3)
+    cfg = case.cfg
+    mu0 = s.mub2d + s.mup0
+    mu = s.mub2d + s.mup
+    rng = np.random.default_rng(4)
+    lam = rng.standard_normal(s.qv.shape)
+    real = A._upwind_adjoint
+
+    def dropped(qm1, q0v, cr, g):
+        a, b, _ = real(qm1, q0v, cr, g)
+        return a, b, np.zeros_like(g)
+    A._upwind_adjoint = dropped
+    try:
+        g = A.pd_update_adjoint(s, s.qv, s.qv0, fl, mu0, mu, cfg.dt, lam,
+                                cfg.dx, cfg.dy)
+    finally:
+        A._upwind_adjoint = real
+    h = 1e-30
+    d = rng.standard_normal(fl[0].shape) * 1e3
+    c = _cplx(s)
+    cfl = [a.astype(np.complex128) for a in fl]
+    cfl[0] += 1j * h * d
+    saved = (D.flux5, D.flux3)
+    D.flux5, D.flux3 = A._flux5_cs, A._flux3_cs
+    try:
+        qc = _pd_forward(c, cfl, cfg, M)
+    finally:
+        D.flux5, D.flux3 = saved
+    lhs = float(np.sum(np.imag(qc) / h * lam))
+    rhs = float(np.sum(d * g["ru"]))
+    assert abs(lhs - rhs) > 1e-6 * abs(lhs), (lhs, rhs)
