@@ -210,6 +210,8 @@ def load():
         "arw_set_w_surface_adjoint": ([dp, pp, _P, _P, _P, _P], _I),
         "arw_scalar_stage": ([dp, pp, _D, _I] + [_P] * 9, _I),
         "arw_scalar_stage_adjoint": ([dp, pp, _D, _I] + [_P] * 17, _I),
+        "arw_pd_stage": ([dp, pp, _D] + [_P] * 10, _I),
+        "arw_pd_stage_adjoint": ([dp, pp, _D] + [_P] * 17, _I),
         "arw_acoustic_adjoint_work": ([_I, _I, _I,
                                        ctypes.POINTER(ctypes.c_int64)], _I),
         "arw_stage_tendencies_adjoint": ([dp, pp, ctypes.POINTER(ArwStageAdj),
@@ -288,8 +290,15 @@ FP64_ENTRY_POINTS = (
     "arw64_acoustic_substep_adjoint_ref", "arw64_acoustic_adjoint_factors",
     "arw64_acoustic_reference_gradients",
     "arw64_set_w_surface_adjoint", "arw64_scalar_stage",
-    "arw64_scalar_stage_adjoint") + tuple(
+    "arw64_scalar_stage_adjoint", "arw64_pd_stage",
+    "arw64_pd_stage_adjoint") + tuple(
         "arw64_" + n for n in FLUX_DIV_ADJOINTS)
+
+
+def pd_work_elems(nz: int, ny: int, nx: int) -> int:
+    """Work elements of arw_pd_stage / arw_pd_stage_adjoint
+    (ARW_PD_WORK in the header)."""
+    return 26 * (nz + 1) * (ny + 1) * (nx + 1)
 
 
 def exported_symbols() -> list[str]:
@@ -313,7 +322,8 @@ def exported_symbols() -> list[str]:
             "arw_acoustic_adjoint_factors",
             "arw_acoustic_reference_gradients",
             "arw_set_w_surface_adjoint", "arw_scalar_stage",
-            "arw_scalar_stage_adjoint"] + [
+            "arw_scalar_stage_adjoint", "arw_pd_stage",
+            "arw_pd_stage_adjoint"] + [
             "arw_" + n for n in FLUX_DIV_ADJOINTS]
 
 

@@ -662,9 +662,6 @@ def _check_adjoint_cfg(state, cfg):
     if _uses_diffusion(cfg) and state.thb.dim() == 3:
         raise _lib.ArwenError("step adjoint: constant-K diffusion needs a "
                               "flat base")
-    if _moist(state) and getattr(cfg, "moist_adv_opt", 1) != 0:
-        raise _lib.ArwenError("step adjoint: moist states need the scalars "
-                              "without the PD limiter (moist_adv_opt = 0)")
 
 
 def _run_stage(state, cfg, nsub, dtau, record):
@@ -735,6 +732,49 @@ def scalar_stage_adjoint(state: DomainState, cfg: RunConfig,
     return g_q
 
 
+def _pd_work(state):
+    nz, ny, nx = state.shape
+    return state.scratch((_lib.pd_work_elems(nz, ny, nx),), "arw_pd_work")
+
+
+def pd_stage(state: DomainState, cfg: RunConfig, dt_stage: float, q, q0,
+             fl, mup_t, mup_new) -> torch.Tensor:
+    """The final stage's update of a scalar with the positive-definite
+    limiter, explicit fluxes fl = (ru, rv, ww) and masses
+    (``arw_pd_stage``; the step's moist_adv_opt = 1 arithmetic)."""
+    lib = _lib.load()
+    pre = "arw64_" if state.dtype == torch.float64 else "arw_"
+    q_new = torch.empty_like(q)
+    _lib.check(getattr(lib, pre + "pd_stage")(
+        state.domain_struct(), params(cfg), float(dt_stage), q.data_ptr(),
+        q0.data_ptr(), *[t.data_ptr() for t in fl], mup_t.data_ptr(),
+        mup_new.data_ptr(), q_new.data_ptr(), _pd_work(state).data_ptr(),
+        _stream()), pre + "pd_stage")
+    return q_new
+
+
+def pd_stage_adjoint(state: DomainState, cfg: RunConfig, dt_stage: float,
+                     q, q0, fl, mup_t, mup_new, lam, g_q0, g_fl, g_mup_t,
+                     g_mup_new):
+    """Transpose of ``pd_stage`` for lam: returns the stage scalar's
+    gradient; adds to g_q0, g_fl (ru, rv, ww), g_mup_t, g_mup_new."""
+    lib = _lib.load()
+    pre = "arw64_" if state.dtype == torch.float64 else "arw_"
+    g_q = torch.empty_like(q)
+    _lib.check(getattr(lib, pre + "pd_stage_adjoint")(
+        state.domain_struct(), params(cfg), float(dt_stage), q.data_ptr(),
+        q0.data_ptr(), *[t.data_ptr() for t in fl], mup_t.data_ptr(),
+        mup_new.data_ptr(), lam.data_ptr(), g_q.data_ptr(), g_q0.data_ptr(),
+        *[t.data_ptr() for t in g_fl], g_mup_t.data_ptr(),
+        g_mup_new.data_ptr(), _pd_work(state).data_ptr(), _stream()),
+        pre + "pd_stage_adjoint")
+    return g_q
+
+
+def _limited(cfg, ex):
+    return ex["final"] and getattr(cfg, "moist_adv_opt", 1) == 1
+
+
 def step_taped(state: DomainState, cfg: RunConfig,
                keep: str = "substeps") -> StepTape:
     """One dycore step of the state (as ``dycore.step``; reference
@@ -743,8 +783,8 @@ def step_taped(state: DomainState, cfg: RunConfig,
     "substeps" records every stage's preparation and substep states (the
     faster reverse sweep; at 1000x1000x50 FP32 about 30 GB), "stages" only
     the three stage states (3.6 GB there; the reverse sweep recomputes each
-    stage from its checkpoint).  Dry states without constant-K diffusion,
-    plain form."""
+    stage from its checkpoint).  Plain form; constant-K diffusion on a flat
+    base; moist states with or without the final stage's PD limiter."""
     from . import dycore as dyn
     _check_state(state)
     _check_adjoint_cfg(state, cfg)
@@ -769,10 +809,15 @@ def step_taped(state: DomainState, cfg: RunConfig,
             ex = {"fl": fl, "mu_new": state.mup.clone(),
                   "dt": nsub * dtau, "final": istage == 2}
             for q in SPECIES:
-                q_new = scalar_stage(state, cfg, ex["dt"], ex["final"],
-                                     getattr(state, q),
+                if _limited(cfg, ex):
+                    q_new = pd_stage(state, cfg, ex["dt"], getattr(state, q),
                                      getattr(state, q + "0"), fl,
                                      state.mup0, state.mup)
+                else:
+                    q_new = scalar_stage(state, cfg, ex["dt"], ex["final"],
+                                         getattr(state, q),
+                                         getattr(state, q + "0"), fl,
+                                         state.mup0, state.mup)
                 getattr(state, q).copy_(q_new)
         tape.stages.append((xs, prep, subs, nsub, dtau, ex))
     dyn.set_w_surface(state, cfg)
@@ -830,10 +875,16 @@ def step_adjoint_taped(state: DomainState, tape: StepTape,
                   torch.zeros_like(ex["fl"][2]))
             g_mu = torch.zeros_like(state.mup)
             for q in SPECIES:
-                gq[q] = scalar_stage_adjoint(
-                    state, cfg, ex["dt"], ex["final"], xs[q], x0[q],
-                    ex["fl"], x0["mup"], ex["mu_new"], lx[q], g0[q], lf,
-                    g0["mup"], g_mu)
+                if _limited(cfg, ex):
+                    gq[q] = pd_stage_adjoint(
+                        state, cfg, ex["dt"], xs[q], x0[q], ex["fl"],
+                        x0["mup"], ex["mu_new"], lx[q], g0[q], lf,
+                        g0["mup"], g_mu)
+                else:
+                    gq[q] = scalar_stage_adjoint(
+                        state, cfg, ex["dt"], ex["final"], xs[q], x0[q],
+                        ex["fl"], x0["mup"], ex["mu_new"], lx[q], g0[q], lf,
+                        g0["mup"], g_mu)
             lx["mup"] = lx["mup"] + g_mu
             inj = [t / float(nsub) for t in lf]
             lam_flux = {"ru": lf[0], "rv": lf[1], "ww": lf[2]}
@@ -886,8 +937,9 @@ def step_adjoint(state: DomainState, cfg: RunConfig, lam: dict,
     2161-2503) at the state's prognostic fields: ``lam`` maps the new
     state's STAGE_STATE fields (and qv, qc, qr on a moist state) to
     adjoints (missing = 0); returns the gradients of the step's input
-    fields.  No constant-K diffusion, plain form; moist states with the
-    scalars' unlimited update (moist_adv_opt = 0).  ``step_taped`` then
+    fields.  Plain form; constant-K diffusion on a flat base; moist states
+    with or without the final stage's PD limiter (moist_adv_opt 1 or 0,
+    each branch of the limiter differentiated as taken).  ``step_taped`` then
     ``step_adjoint_taped``; the state's prognostic fields are left as they
     were (their EOS recomputed), its other fields are overwritten."""
     from . import dycore as dyn

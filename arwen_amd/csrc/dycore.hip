@@ -805,6 +805,365 @@ __global__ void k_tracer_adj_div(Dom d, const real* __restrict__ m,
     gout[iM(k, j, i, ny, nx)] = dt * acc;
 }
 
+// ---------------------------------------------------------------------------
+// Transpose of the final stage's PD-limited update (reference
+// pd_advection.cu pd_fluxes -> pd_scale -> pd_renorm_apply, moist.py:
+// 385-398).  The forward is re-run by k_pd_fluxes / k_pd_scale / k_pd_apply
+// on a domain copy whose ru_m/rv_m/ww_m are the given fluxes, mup the
+// post-stage mu' (mu_pp a zero field) and pd_* work slices.  Then, all in
+// gather form:
+//   k_pda_update  the clamped coupled update -> tg (adjoint of the
+//                 limited divergence) and the time-t / mass shares;
+//   k_pda_scale   each cell's factor adjoint, gathered from the correction
+//                 fluxes it donates to, through the factor's clamps to the
+//                 upwind budget (ph_low) and the outgoing correction (fo);
+//   k_pda_faces   per face: the correction flux's and the upwind flux's
+//                 adjoints, the velocity gradient, the upwind flux's q0
+//                 shares and its face-mass adjoint;
+//   k_pda_gather  per cell: the high-order flux transpose to q, the upwind
+//                 shares to q0, the face masses to the column mass;
+//   k_ss_adj_cols the two column-mass sums.
+// Non-smooth points are differentiated as the branch taken: the clamp at
+// zero, each correction's donor (the flux sign), the factor's [0, 1]
+// clamps, the upwind flux's CFL caps; |cr| as sign(cr) cr.
+// ---------------------------------------------------------------------------
+
+struct PdWork {
+    real *fxl, *fxc, *fyl, *fyc, *fzl, *fzc, *sc, *qn, *zero;
+    real *tg;                    // tg, then P0 (mu_t), P1 (mu_new), gdl, gfo
+    real *hx, *bx, *ax, *mx;     // x faces: correction adj, q0 shares, mass
+    real *hy, *by, *ay, *my;
+    real *hz, *bz, *az, *mz;
+};
+
+__device__ __forceinline__ real sgnr(real v)
+{
+    return v > R(0.0) ? R(1.0) : (v < R(0.0) ? -R(1.0) : R(0.0));
+}
+
+__global__ void k_pda_update(Dom d, PdWork w, const real* __restrict__ q0,
+                             const real* __restrict__ mup_t,
+                             const real* __restrict__ mup_new,
+                             const real* __restrict__ lam,
+                             real* __restrict__ gq0, real dt)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    const int x = iM(k, j, i, ny, nx), c = j * nx + i;
+    const size_t n = (size_t)nz * ny * nx;
+    const real chm0 = d.c1h[k] * (d.mub2d[c] + mup_t[c]) + d.c2h[k];
+    const real chm = d.c1h[k] * (d.mub2d[c] + mup_new[c]) + d.c2h[k];
+    const real v = w.qn[x];
+    const real lv = (v > R(0.0)) ? lam[x] : R(0.0);
+    w.tg[x] = dt * lv / chm;
+    gq0[x] += chm0 * lv / chm;
+    w.tg[n + x] = d.c1h[k] * q0[x] * lv / chm;
+    w.tg[2 * n + x] = -(d.c1h[k] * v * lv / chm);
+}
+
+__global__ void k_pda_scale(Dom d, PdWork w, const real* __restrict__ q0,
+                            const real* __restrict__ mup_t,
+                            real* __restrict__ gq0, real dxi, real dyi,
+                            real dt)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    const size_t n = (size_t)nz * st;
+    const int z0 = iM(k, j, i, ny, nx), c = j * nx + i;
+    const int u0 = iU(k, j, i, ny, nx), v0 = iV(k, j, i, ny, nx);
+    const real* tg = w.tg;
+    // face jumps of tg: x face f sees cells f-1 (if f >= 1) and f (if f < nx)
+    auto jx = [&](int f) {
+        const real r = f < nx ? tg[iM(k, j, f, ny, nx)] : R(0.0);
+        const real l = f >= 1 ? tg[iM(k, j, f - 1, ny, nx)] : R(0.0);
+        return dxi * (r - l);
+    };
+    auto jy = [&](int g) {
+        const real r = g < ny ? tg[iM(k, g, i, ny, nx)] : R(0.0);
+        const real l = g >= 1 ? tg[iM(k, g - 1, i, ny, nx)] : R(0.0);
+        return dyi * (r - l);
+    };
+    auto jz = [&](int kf) {
+        const real u = kf < nz ? d.rdnw[kf] * tg[iM(kf, j, i, ny, nx)] : R(0.0);
+        const real b = kf >= 1 ? d.rdnw[kf - 1] * tg[iM(kf - 1, j, i, ny, nx)]
+                               : R(0.0);
+        return u - b;
+    };
+    // this cell's factor adjoint: the faces it donates to
+    real gs = R(0.0);
+    {
+        const real fl = w.fxc[u0], fr = w.fxc[u0 + 1];
+        if (fl < R(0.0)) gs += fl * jx(i);
+        if (fr > R(0.0)) gs += fr * jx(i + 1);
+        if (i == nx - 1) {
+            const real f0 = w.fxc[iU(k, j, 0, ny, nx)];
+            if (f0 > R(0.0)) gs += f0 * jx(0);
+        }
+        if (i == 0) {
+            const real fn = w.fxc[iU(k, j, nx, ny, nx)];
+            if (fn < R(0.0)) gs += fn * jx(nx);
+        }
+    }
+    {
+        const real fl = w.fyc[v0], fr = w.fyc[v0 + nx];
+        if (fl < R(0.0)) gs += fl * jy(j);
+        if (fr > R(0.0)) gs += fr * jy(j + 1);
+        if (j == ny - 1) {
+            const real f0 = w.fyc[iV(k, 0, i, ny, nx)];
+            if (f0 > R(0.0)) gs += f0 * jy(0);
+        }
+        if (j == 0) {
+            const real fn = w.fyc[iV(k, ny, i, ny, nx)];
+            if (fn < R(0.0)) gs += fn * jy(ny);
+        }
+    }
+    {
+        const real fb = w.fzc[z0], ft = w.fzc[z0 + st];
+        if (fb > R(0.0)) gs += fb * jz(k);
+        if (ft < R(0.0)) gs += ft * jz(k + 1);
+    }
+    // the factor: min(1, max(0, ph_low / (fo + 1e-20))) where fo > ph_low
+    const real chm0 = d.c1h[k] * (d.mub2d[c] + mup_t[c]) + d.c2h[k];
+    const real dl = (dxi * (w.fxl[u0 + 1] - w.fxl[u0])
+                      + dyi * (w.fyl[v0 + nx] - w.fyl[v0]))
+                     + d.rdnw[k] * (w.fzl[z0 + st] - w.fzl[z0]);
+    const real ph_low = chm0 * q0[z0] - dt * dl;
+    const real fo = dt * ((dxi * (fmax(R(0.0), w.fxc[u0 + 1])
+                                   - fmin(R(0.0), w.fxc[u0]))
+                            + dyi * (fmax(R(0.0), w.fyc[v0 + nx])
+                                     - fmin(R(0.0), w.fyc[v0])))
+                           + d.rdnw[k] * (fmin(R(0.0), w.fzc[z0 + st])
+                                          - fmax(R(0.0), w.fzc[z0])));
+    real gpl = R(0.0), gfo = R(0.0);
+    if (fo > ph_low) {
+        const real den = fo + R(1e-20);
+        const real ratio = ph_low / den;
+        if (ratio > R(0.0) && ratio < R(1.0)) {
+            gpl = gs / den;
+            gfo = -(gs * ratio / den);
+        }
+    }
+    gq0[z0] += chm0 * gpl;
+    w.tg[n + z0] += d.c1h[k] * q0[z0] * gpl;
+    w.tg[3 * n + z0] = -(dt * gpl);
+    w.tg[4 * n + z0] = gfo;
+}
+
+// WRF flux_upwind's transpose: (dqm1, dq0v) coefficients and d/dcr.
+__device__ __forceinline__ void upwind_adj(real qm1, real q0v, real cr,
+                                           real g, real& dqm1, real& dq0v,
+                                           real& dcr)
+{
+    const real a = cr + fabs(cr), b = cr - fabs(cr), sg = sgnr(cr);
+    dqm1 = R(0.5) * fmin(R(1.0), a) * g;
+    dq0v = R(0.5) * fmax(-R(1.0), b) * g;
+    const real da = a < R(1.0) ? R(1.0) + sg : R(0.0);
+    const real db = b > -R(1.0) ? R(1.0) - sg : R(0.0);
+    dcr = R(0.5) * g * (qm1 * da + q0v * db);
+}
+
+__global__ void k_pda_faces(Dom d, PdWork w, const real* __restrict__ q,
+                            const real* __restrict__ q0,
+                            const real* __restrict__ ru,
+                            const real* __restrict__ rv,
+                            const real* __restrict__ ww,
+                            const real* __restrict__ mup_new,
+                            real* __restrict__ gru, real* __restrict__ grv,
+                            real* __restrict__ gww, real dx, real dy,
+                            real dxi, real dyi, real dt)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    const size_t n = (size_t)nz * ny * nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    auto mu = [&](int jj, int ii) { return d.mub2d[jj * nx + ii] + mup_new[jj * nx + ii]; };
+    auto cellv = [&](const real* a, int kk, int jj, int ii) {
+        return a[iM(kk, jj, ii, ny, nx)];
+    };
+    const real* tg = w.tg;
+    const real* gdl = w.tg + 3 * n;
+    const real* gfo = w.tg + 4 * n;
+    auto d5 = [&](const real* x, real vel) {   // d flux5 / d vel
+        return ((R(37.0) * (x[3] + x[2]) - R(8.0) * (x[4] + x[1]) + (x[5] + x[0]))
+                - sgnr(vel) * (R(10.0) * (x[3] - x[2]) - R(5.0) * (x[4] - x[1])
+                               + (x[5] - x[0]))) / R(60.0);
+    };
+    if (j < ny && k < nz) {                    // x face f = i
+        const int f = i, a = (f >= nx) ? f - nx : f, b = (f == 0) ? nx - 1 : f - 1;
+        const int xf = iU(k, j, f, ny, nx);
+        const bool hr = f < nx, hl = f >= 1;
+        const real fc = w.fxc[xf];
+        const real jmp = dxi * ((hr ? cellv(tg, k, j, f) : R(0.0))
+                                - (hl ? cellv(tg, k, j, f - 1) : R(0.0)));
+        const real sf = fc > R(0.0) ? cellv(w.sc, k, j, b)
+                        : (fc < R(0.0) ? cellv(w.sc, k, j, a) : R(1.0));
+        real gc = sf * jmp;
+        if (fc > R(0.0) && hl) gc += dt * dxi * cellv(gfo, k, j, f - 1);
+        if (fc < R(0.0) && hr) gc -= dt * dxi * cellv(gfo, k, j, f);
+        const real glw = jmp + dxi * ((hl ? cellv(gdl, k, j, f - 1) : R(0.0))
+                                      - (hr ? cellv(gdl, k, j, f) : R(0.0)));
+        const real G = glw - gc;
+        const real vel = ru[xf];
+        const real muf = (d.c1h[k] * R(0.5)) * (mu(j, a) + mu(j, b)) + d.c2h[k];
+        const real cr = ((vel * dt) / dx) / muf;
+        const real qb = cellv(q0, k, j, b), qa = cellv(q0, k, j, a);
+        const real fu = flux_upwind(qb, qa, cr);
+        real dqb, dqa, dcr;
+        upwind_adj(qb, qa, cr, muf * (dx / dt) * G, dqb, dqa, dcr);
+        real x[6];
+#pragma unroll
+        for (int o = 0; o < 6; ++o) x[o] = cellv(q, k, j, pmod(f + o - 3, nx));
+        gru[xf] += d5(x, vel) * gc + dcr * (dt / dx) / muf;
+        w.hx[xf] = gc;
+        w.bx[xf] = dqb;
+        w.ax[xf] = dqa;
+        w.mx[xf] = (dx / dt) * fu * G - dcr * cr / muf;
+    }
+    if (i < nx && k < nz) {                    // y face g = j
+        const int g = j, a = (g >= ny) ? g - ny : g, b = (g == 0) ? ny - 1 : g - 1;
+        const int yf = iV(k, g, i, ny, nx);
+        const bool hr = g < ny, hl = g >= 1;
+        const real fc = w.fyc[yf];
+        const real jmp = dyi * ((hr ? cellv(tg, k, g, i) : R(0.0))
+                                - (hl ? cellv(tg, k, g - 1, i) : R(0.0)));
+        const real sf = fc > R(0.0) ? cellv(w.sc, k, b, i)
+                        : (fc < R(0.0) ? cellv(w.sc, k, a, i) : R(1.0));
+        real gc = sf * jmp;
+        if (fc > R(0.0) && hl) gc += dt * dyi * cellv(gfo, k, g - 1, i);
+        if (fc < R(0.0) && hr) gc -= dt * dyi * cellv(gfo, k, g, i);
+        const real glw = jmp + dyi * ((hl ? cellv(gdl, k, g - 1, i) : R(0.0))
+                                      - (hr ? cellv(gdl, k, g, i) : R(0.0)));
+        const real G = glw - gc;
+        const real vel = rv[yf];
+        const real muf = (d.c1h[k] * R(0.5)) * (mu(a, i) + mu(b, i)) + d.c2h[k];
+        const real cr = ((vel * dt) / dy) / muf;
+        const real qb = cellv(q0, k, b, i), qa = cellv(q0, k, a, i);
+        const real fu = flux_upwind(qb, qa, cr);
+        real dqb, dqa, dcr;
+        upwind_adj(qb, qa, cr, muf * (dy / dt) * G, dqb, dqa, dcr);
+        real x[6];
+#pragma unroll
+        for (int o = 0; o < 6; ++o) x[o] = cellv(q, k, pmod(g + o - 3, ny), i);
+        grv[yf] += d5(x, vel) * gc + dcr * (dt / dy) / muf;
+        w.hy[yf] = gc;
+        w.by[yf] = dqb;
+        w.ay[yf] = dqa;
+        w.my[yf] = (dy / dt) * fu * G - dcr * cr / muf;
+    }
+    if (i < nx && j < ny) {                    // eta face kf = k
+        const int kf = k, zf = iM(kf, j, i, ny, nx);
+        real gc = R(0.0), dqb = R(0.0), dqa = R(0.0), gm = R(0.0);
+        if (kf > 0 && kf < nz) {
+            const real fc = w.fzc[zf];
+            const real jmp = d.rdnw[kf] * cellv(tg, kf, j, i)
+                             - d.rdnw[kf - 1] * cellv(tg, kf - 1, j, i);
+            const real sf = fc > R(0.0) ? cellv(w.sc, kf, j, i)
+                            : (fc < R(0.0) ? cellv(w.sc, kf - 1, j, i) : R(1.0));
+            gc = sf * jmp;
+            if (fc < R(0.0)) gc += dt * d.rdnw[kf - 1] * cellv(gfo, kf - 1, j, i);
+            if (fc > R(0.0)) gc -= dt * d.rdnw[kf] * cellv(gfo, kf, j, i);
+            const real glw = jmp + (d.rdnw[kf - 1] * cellv(gdl, kf - 1, j, i)
+                                    - d.rdnw[kf] * cellv(gdl, kf, j, i));
+            const real G = glw - gc;
+            const real vel = ww[zf];
+            const real dz = R(2.0) / (d.rdnw[kf] + d.rdnw[kf - 1]);
+            const real muf = d.c1h[kf] * mu(j, i) + d.c2h[kf];
+            const real cr = (vel * dt) / (dz * muf);
+            const real qb = cellv(q0, kf - 1, j, i), qa = cellv(q0, kf, j, i);
+            const real fu = flux_upwind(qb, qa, cr);
+            real dcr;
+            upwind_adj(qb, qa, cr, muf * (dz / dt) * G, dqb, dqa, dcr);
+            real dv;
+            if (kf == 1 || kf == nz - 1) {
+                dv = d.fnm[kf] * cellv(q, kf, j, i) + d.fnp[kf] * cellv(q, kf - 1, j, i);
+            } else {
+                const real qm2 = cellv(q, kf - 2, j, i), qm1 = cellv(q, kf - 1, j, i);
+                const real qc0 = cellv(q, kf, j, i), qp1 = cellv(q, kf + 1, j, i);
+                dv = ((R(7.0) * (qc0 + qm1) - (qp1 + qm2))
+                      + sgnr(vel) * (R(3.0) * (qc0 - qm1) - (qp1 - qm2))) / R(12.0);
+            }
+            gww[zf] += dv * gc + dcr * dt / (dz * muf);
+            gm = d.c1h[kf] * ((dz / dt) * fu * G - dcr * cr / muf);
+        }
+        w.hz[zf] = gc;
+        w.bz[zf] = dqb;
+        w.az[zf] = dqa;
+        w.mz[zf] = gm;
+    }
+}
+
+__global__ void k_pda_gather(Dom d, PdWork w, const real* __restrict__ ru,
+                             const real* __restrict__ rv,
+                             const real* __restrict__ ww,
+                             real* __restrict__ gq, real* __restrict__ gq0)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    const size_t n = (size_t)nz * st;
+    const int z0 = iM(k, j, i, ny, nx);
+    real acc = R(0.0);
+    for (int o = 0; o < 6; ++o) {
+        // x face f reads cell f + o - 3 == i; face nx copies face 0's stencil
+        const int f = pmod(i + 3 - o, nx);
+        acc += w5(o, ru[iU(k, j, f, ny, nx)]) * w.hx[iU(k, j, f, ny, nx)];
+        if (f == 0)
+            acc += w5(o, ru[iU(k, j, nx, ny, nx)]) * w.hx[iU(k, j, nx, ny, nx)];
+        const int g = pmod(j + 3 - o, ny);
+        acc += w5(o, rv[iV(k, g, i, ny, nx)]) * w.hy[iV(k, g, i, ny, nx)];
+        if (g == 0)
+            acc += w5(o, rv[iV(k, ny, i, ny, nx)]) * w.hy[iV(k, ny, i, ny, nx)];
+    }
+    for (int kf = k - 1; kf <= k + 2; ++kf) {
+        if (kf < 1 || kf > nz - 1) continue;
+        const int zf = iM(kf, j, i, ny, nx);
+        const real vel = ww[zf], h = w.hz[zf];
+        if (kf == 1 || kf == nz - 1) {
+            if (k == kf) acc += vel * d.fnm[kf] * h;
+            else if (k == kf - 1) acc += vel * d.fnp[kf] * h;
+        } else {
+            acc += w3(k - kf + 2, vel) * h;
+        }
+    }
+    gq[z0] = acc;
+    // upwind fluxes of q0: cell i is the east cell of face i and the west
+    // cell of face i+1 (periodic copies: face nx east = 0, face 0 west = nx-1)
+    const int u0 = iU(k, j, i, ny, nx), v0 = iV(k, j, i, ny, nx);
+    real g0 = w.ax[u0] + w.bx[u0 + 1] + w.ay[v0] + w.by[v0 + nx];
+    real gm = w.mx[u0] + w.mx[u0 + 1] + w.my[v0] + w.my[v0 + nx];
+    if (i == 0) {
+        g0 += w.ax[iU(k, j, nx, ny, nx)];
+        gm += w.mx[iU(k, j, nx, ny, nx)];
+    }
+    if (i == nx - 1) {
+        g0 += w.bx[iU(k, j, 0, ny, nx)];
+        gm += w.mx[iU(k, j, 0, ny, nx)];
+    }
+    if (j == 0) {
+        g0 += w.ay[iV(k, ny, i, ny, nx)];
+        gm += w.my[iV(k, ny, i, ny, nx)];
+    }
+    if (j == ny - 1) {
+        g0 += w.by[iV(k, 0, i, ny, nx)];
+        gm += w.my[iV(k, 0, i, ny, nx)];
+    }
+    real gmz = R(0.0);
+    if (k >= 1) { g0 += w.az[z0]; gmz = w.mz[z0]; }
+    if (k + 1 <= nz - 1) g0 += w.bz[z0 + st];
+    gq0[z0] += g0;
+    w.tg[2 * n + z0] += (d.c1h[k] * R(0.5)) * gm + gmz;
+}
+
 __global__ void k_add3(real* __restrict__ out, const real* __restrict__ a,
                        const real* __restrict__ b, int n)
 {
@@ -1135,6 +1494,73 @@ int32_t lib_check_domain(const Dom* d) { return check_domain(d); }
 int32_t lib_check_prec(const arw_params* p, const Dom* d)
 {
     return check_prec(p, d);
+}
+
+// Work layout of the PD stage entries (elements of real): the six face
+// flux arrays, the factors, the recomputed q_new and a zero mu'' field
+// (forward), then tg and four cell adjoints, then four adjoint arrays per
+// face direction.  6 (X + Y + Z) + 7 n + ny nx <= 26 (nz+1)(ny+1)(nx+1).
+PdWork pd_layout(const Dom& d, real* work)
+{
+    const size_t nz = d.nz, ny = d.ny, nx = d.nx;
+    const size_t X = nz * ny * (nx + 1), Y = nz * (ny + 1) * nx;
+    const size_t Z = (nz + 1) * ny * nx, n = nz * ny * nx;
+    real* p = work;
+    auto take = [&](size_t m) { real* r = p; p += m; return r; };
+    PdWork w;
+    w.fxl = take(X); w.fxc = take(X);
+    w.fyl = take(Y); w.fyc = take(Y);
+    w.fzl = take(Z); w.fzc = take(Z);
+    w.sc = take(n); w.qn = take(n); w.zero = take(ny * nx);
+    w.tg = take(5 * n);
+    w.hx = take(X); w.bx = take(X); w.ax = take(X); w.mx = take(X);
+    w.hy = take(Y); w.by = take(Y); w.ay = take(Y); w.my = take(Y);
+    w.hz = take(Z); w.bz = take(Z); w.az = take(Z); w.mz = take(Z);
+    return w;
+}
+
+__global__ void k_zero_real(real* p, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = R(0.0);
+}
+
+// The fused step's PD kernels on explicit inputs: a domain copy reads
+// the fluxes as its ru_m/rv_m/ww_m and the post-stage mu' as mup (with a
+// zero mu''; x + 0 is exact), so q_new is bit-identical to the step's.
+int32_t pd_forward(const Dom* d, const arw_params* prm, double dt_stage,
+                   const real* q, const real* q0, const real* ru,
+                   const real* rv, const real* ww, const real* mup_t,
+                   const real* mup_new, real* q_new, const PdWork& w,
+                   hipStream_t st)
+{
+    int32_t rc;
+    Dom dd = *d;
+    dd.mup = const_cast<real*>(mup_new);
+    dd.mu_pp = w.zero;
+    dd.ru_m = const_cast<real*>(ru);
+    dd.rv_m = const_cast<real*>(rv);
+    dd.ww_m = const_cast<real*>(ww);
+    dd.pd_fxl = w.fxl; dd.pd_fxc = w.fxc;
+    dd.pd_fyl = w.fyl; dd.pd_fyc = w.fyc;
+    dd.pd_fzl = w.fzl; dd.pd_fzc = w.fzc;
+    dd.pd_sc = w.sc;
+    const int s2 = d->ny * d->nx;
+    hipLaunchKernelGGL(k_zero_real, dim3((s2 + 255) / 256), dim3(256), 0, st,
+                       w.zero, s2);
+    if ((rc = launched("k_zero_real"))) return rc;
+    const real dt = (real)dt_stage;
+    const real rdx = (real)(1.0 / prm->dx), rdy = (real)(1.0 / prm->dy);
+    const dim3 cells((d->nx + 63) / 64, (d->ny + 3) / 4, d->nz);
+    hipLaunchKernelGGL(k_pd_fluxes, union_grid(dd, dd.nz + 1), dim3(64, 4),
+                       0, st, dd, q, q0, (real)prm->dx, (real)prm->dy, dt);
+    if ((rc = launched("k_pd_fluxes"))) return rc;
+    hipLaunchKernelGGL(k_pd_scale, cells, dim3(64, 4), 0, st, dd, q0, mup_t,
+                       rdx, rdy, dt);
+    if ((rc = launched("k_pd_scale"))) return rc;
+    hipLaunchKernelGGL(k_pd_apply, cells, dim3(64, 4), 0, st, dd, q0, q_new,
+                       mup_t, rdx, rdy, dt);
+    return launched("k_pd_apply");
 }
 
 }  // namespace ARW_NS
@@ -1599,6 +2025,71 @@ int32_t ARW_API(scalar_stage_adjoint)(
     return lib_flux_div_adjoint(0, q, ru, rv, ww, work, g_q, g_ru, g_rv, g_ww,
                                 d->rdnw, d->fnm, d->fnp, rdx, rdy, nz, ny, nx,
                                 st);
+}
+
+// The final stage's update of one scalar with WRF's positive-definite
+// limiter and explicit fluxes and masses (the arithmetic of the step's
+// moist_adv_opt = 1 path; reference pd_advection.cu, moist.py:385-398,
+// :493-494).  work: 26 (nz+1)(ny+1)(nx+1) elements.
+int32_t ARW_API(pd_stage)(const Dom* d, const arw_params* prm,
+                          double dt_stage, const real* q, const real* q0,
+                          const real* ru, const real* rv, const real* ww,
+                          const real* mup_t, const real* mup_new,
+                          real* q_new, real* work, void* stream)
+{
+    int32_t rc = check_domain(d);
+    if (rc) return rc;
+    if (!prm || !q || !q0 || !ru || !rv || !ww || !mup_t || !mup_new
+        || !q_new || !work)
+        return fail(ARW_E_ARG, "null argument");
+    if (q_new == q || q_new == q0)
+        return fail(ARW_E_ARG, "q_new must not alias q or q0");
+    return pd_forward(d, prm, dt_stage, q, q0, ru, rv, ww, mup_t, mup_new,
+                      q_new, pd_layout(*d, work), (hipStream_t)stream);
+}
+
+// Its transpose for lam = adjoint of q_new: g_q is written; g_q0, the flux
+// gradients and the column-mass gradients g_mup_t, g_mup_new are added
+// to.  Piecewise: each branch (clamp, donor choice, factor clamps, CFL
+// caps) is differentiated as taken.  work: 26 (nz+1)(ny+1)(nx+1).
+int32_t ARW_API(pd_stage_adjoint)(
+    const Dom* d, const arw_params* prm, double dt_stage, const real* q,
+    const real* q0, const real* ru, const real* rv, const real* ww,
+    const real* mup_t, const real* mup_new, const real* lam, real* g_q,
+    real* g_q0, real* g_ru, real* g_rv, real* g_ww, real* g_mup_t,
+    real* g_mup_new, real* work, void* stream)
+{
+    int32_t rc = check_domain(d);
+    if (rc) return rc;
+    if (!prm || !q || !q0 || !ru || !rv || !ww || !mup_t || !mup_new || !lam
+        || !g_q || !g_q0 || !g_ru || !g_rv || !g_ww || !g_mup_t || !g_mup_new
+        || !work)
+        return fail(ARW_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int nz = d->nz, ny = d->ny, nx = d->nx;
+    const PdWork w = pd_layout(*d, work);
+    if ((rc = pd_forward(d, prm, dt_stage, q, q0, ru, rv, ww, mup_t, mup_new,
+                         w.qn, w, st)))
+        return rc;
+    const real dt = (real)dt_stage;
+    const real rdx = (real)(1.0 / prm->dx), rdy = (real)(1.0 / prm->dy);
+    const dim3 cells((nx + 63) / 64, (ny + 3) / 4, nz);
+    hipLaunchKernelGGL(k_pda_update, cells, dim3(64, 4), 0, st, *d, w, q0,
+                       mup_t, mup_new, lam, g_q0, dt);
+    if ((rc = launched("k_pda_update"))) return rc;
+    hipLaunchKernelGGL(k_pda_scale, cells, dim3(64, 4), 0, st, *d, w, q0,
+                       mup_t, g_q0, rdx, rdy, dt);
+    if ((rc = launched("k_pda_scale"))) return rc;
+    hipLaunchKernelGGL(k_pda_faces, union_grid(*d, nz + 1), dim3(64, 4), 0,
+                       st, *d, w, q, q0, ru, rv, ww, mup_new, g_ru, g_rv,
+                       g_ww, (real)prm->dx, (real)prm->dy, rdx, rdy, dt);
+    if ((rc = launched("k_pda_faces"))) return rc;
+    hipLaunchKernelGGL(k_pda_gather, cells, dim3(64, 4), 0, st, *d, w, ru,
+                       rv, ww, g_q, g_q0);
+    if ((rc = launched("k_pda_gather"))) return rc;
+    hipLaunchKernelGGL(k_ss_adj_cols, dim3((ny * nx + 63) / 64), dim3(64, KG),
+                       0, st, *d, w.tg, g_mup_t, g_mup_new);
+    return launched("k_ss_adj_cols");
 }
 
 int32_t ARW_API(ensemble_create)(const Dom* members, int32_t n,

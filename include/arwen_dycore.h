@@ -17,6 +17,7 @@
 #ifndef ARWEN_DYCORE_H
 #define ARWEN_DYCORE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -331,6 +332,32 @@ int32_t arw_scalar_stage_adjoint(const arw_domain *d, const arw_params *prm,
                                  float *g_ru, float *g_rv, float *g_ww,
                                  float *g_mup_t, float *g_mup_new, float *work,
                                  void *stream);
+
+/* The final RK stage's update of one scalar with WRF's positive-definite
+ * limiter (replaces the moist_adv_opt = 1 branch of moist.py:434-698:
+ * pd_advection.cu pd_fluxes -> pd_scale -> pd_renorm_apply, then the
+ * clamped coupled update moist.py:385-398), with explicit averaged fluxes
+ * and masses; bit-identical to the step's own final-stage scalar.
+ * arw_pd_stage_adjoint: its transpose for lam = adjoint of q_new; g_q is
+ * written, g_q0, g_ru, g_rv, g_ww, g_mup_t, g_mup_new are added to; each
+ * branch (the clamp, a correction's donor cell, the factor's [0, 1]
+ * clamps, the upwind CFL caps) is differentiated as taken.  work (both):
+ * ARW_PD_WORK(nz, ny, nx) elements. */
+#define ARW_PD_WORK(nz, ny, nx) \
+    (26 * (size_t)((nz) + 1) * (size_t)((ny) + 1) * (size_t)((nx) + 1))
+int32_t arw_pd_stage(const arw_domain *d, const arw_params *prm,
+                     double dt_stage, const float *q, const float *q0,
+                     const float *ru, const float *rv, const float *ww,
+                     const float *mup_t, const float *mup_new, float *q_new,
+                     float *work, void *stream);
+int32_t arw_pd_stage_adjoint(const arw_domain *d, const arw_params *prm,
+                             double dt_stage, const float *q, const float *q0,
+                             const float *ru, const float *rv, const float *ww,
+                             const float *mup_t, const float *mup_new,
+                             const float *lam, float *g_q, float *g_q0,
+                             float *g_ru, float *g_rv, float *g_ww,
+                             float *g_mup_t, float *g_mup_new, float *work,
+                             void *stream);
 
 /* EOS p, al, alt from thp, php, mup (replaces diagnostics.cu:37-89
  * calc_p_alpha, launched by diagnostics.py:14-57; hypsometric_opt = 1). */
@@ -663,6 +690,20 @@ int32_t arw64_scalar_stage_adjoint(const arw64_domain *d, const arw_params *prm,
                                  double *g_ru, double *g_rv, double *g_ww,
                                  double *g_mup_t, double *g_mup_new, double *work,
                                  void *stream);
+int32_t arw64_pd_stage(const arw64_domain *d, const arw_params *prm,
+                       double dt_stage, const double *q, const double *q0,
+                       const double *ru, const double *rv, const double *ww,
+                       const double *mup_t, const double *mup_new,
+                       double *q_new, double *work, void *stream);
+int32_t arw64_pd_stage_adjoint(const arw64_domain *d, const arw_params *prm,
+                               double dt_stage, const double *q,
+                               const double *q0, const double *ru,
+                               const double *rv, const double *ww,
+                               const double *mup_t, const double *mup_new,
+                               const double *lam, double *g_q, double *g_q0,
+                               double *g_ru, double *g_rv, double *g_ww,
+                               double *g_mup_t, double *g_mup_new,
+                               double *work, void *stream);
 
 typedef struct arw64_stage_adj {
     const double *l_ru_t, *l_rv_t, *l_rw_t, *l_rth_t, *l_rph_t;

@@ -332,20 +332,23 @@ def test_step_adjoint_rejects_unsupported():
 
 # ---------------------------------------------------------------------------
 # Moist states: qv, qc, qr through the moist EOS, the buoyancy loading and
-# the sumflux-coupled scalar update (moist_adv_opt = 0)
+# the sumflux-coupled scalar update, without (moist_adv_opt = 0) and with
+# (1) the final stage's positive-definite limiter
 # ---------------------------------------------------------------------------
 
 MOIST_NAMES = A.STAGE_IN + A.SPECIES
 
 
-def _moist_states(name, dtype=torch.float64, seed=59):
+def _moist_states(name, dtype=torch.float64, seed=59, pd=0):
     """The oracle's moist whole-step test state and a device state of dtype
-    holding the same fields (scalars kept positive)."""
+    holding the same fields (scalars kept positive).  pd = 1: the final
+    stage's PD limiter on, with a rough qv (a third of the cells nearly
+    empty) so that it engages."""
     from arwen_amd import dycore
     from arwen_amd.state import DomainState
     from tests import pins
     case = pins.CASES[name]()
-    case.cfg = case.cfg.with_(moist=True, moist_adv_opt=0)
+    case.cfg = case.cfg.with_(moist=True, moist_adv_opt=pd)
     cfg = case.cfg
     s = O.from_case(case, np.float64)
     rng = np.random.default_rng(seed)
@@ -358,6 +361,9 @@ def _moist_states(name, dtype=torch.float64, seed=59):
     s.qv = 0.008 + 0.002 * rng.standard_normal(shp)
     s.qc = 5e-4 + 1e-4 * rng.standard_normal(shp)
     s.qr = 5e-4 + 1e-4 * rng.standard_normal(shp)
+    if pd:
+        s.qv = np.abs(s.qv - 0.008) * 3.0
+        s.qv[rng.random(shp) < 0.3] *= 1e-3
     for q in A.SPECIES:
         setattr(s, q + "0", np.zeros(shp))
     O.set_w_surface(s, cfg.dx, cfg.dy)
@@ -407,11 +413,97 @@ def test_scalar_stage_and_adjoint_match_oracle():
         assert max(bad.values()) < 1e-10, (clamp, bad)
 
 
+def _pd_inputs(s, cfg, rng):
+    """A rough non-negative scalar pair and averaged fluxes strong enough
+    (Courant numbers ~0.35) that the PD limiter scales some corrections."""
+    shp = s.thp.shape
+    q0 = np.abs(rng.standard_normal(shp)) * 4e-3
+    q0[rng.random(shp) < 0.3] *= 1e-3
+    q = q0 * (1.0 + 0.2 * rng.standard_normal(shp))
+    mup_t = s.mup + rng.standard_normal(s.mup.shape) * 30.0
+    mup_new = s.mup.copy()
+    mu = s.mub2d + mup_new
+    c1, c2 = s.c1h[:, None, None], s.c2h[:, None, None]
+    cu = np.concatenate([mu, mu[:, :1]], axis=1)[None] * c1 + c2
+    cv = np.concatenate([mu, mu[:1]], axis=0)[None] * c1 + c2
+    fl = (rng.standard_normal(s.u.shape) * 0.35 * cu * cfg.dx / cfg.dt,
+          rng.standard_normal(s.v.shape) * 0.35 * cv * cfg.dy / cfg.dt,
+          rng.standard_normal(s.w.shape) * 0.05 * s.mub2d.mean())
+    fl[2][0] = 0.0
+    fl[2][-1] = 0.0
+    return q, q0, fl, mup_t, mup_new
+
+
 @pytest.mark.parametrize("name", ["dry_flat", "terrain"])
-def test_moist_step_adjoint_matches_oracle(name):
+def test_pd_stage_and_adjoint_match_oracle(name):
+    """arw_pd_stage equals the oracle's limited final-stage update (FP64 to
+    round-off, FP32 bit for bit) and arw_pd_stage_adjoint its transpose
+    (pinned by complex-step JVPs), on inputs where the limiter engages."""
     from arwen_amd import adjoint
-    case, s, g, rng = _moist_states(name)
-    print(f"moist step adjoint vs oracle {name}", flush=True)
+    from arwen_amd.state import DomainState
+    from oracle import moist_np as M
+    case, s, g, rng = _moist_states(name, seed=89, pd=1)
+    cfg = case.cfg
+    print(f"pd stage and its adjoint vs oracle {name}", flush=True)
+    dev = g.u.device
+    q, q0, fl, mup_t, mup_new = _pd_inputs(s, cfg, rng)
+    mu0, mu = s.mub2d + mup_t, s.mub2d + mup_new
+    dt = cfg.dt
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+
+    def oracle_forward(st, dtype):
+        c = lambda a: np.asarray(a).astype(dtype)
+        f = [c(x) for x in fl]
+        m0, mn = c(st.mub2d) + c(mup_t), c(st.mub2d) + c(mup_new)
+        fx = M.pd_fluxes(c(q), c(q0), f[0], f[1], f[2], mn, c(st.c1h),
+                         c(st.c2h), c(st.rdnw), c(st.fnm), c(st.fnp),
+                         cfg.dx, cfg.dy, dt)
+        dxi = float(np.real(dtype(1.0 / cfg.dx)))
+        dyi = float(np.real(dtype(1.0 / cfg.dy)))
+        sc = M.pd_scale(c(q0), m0, c(st.c1h), c(st.c2h), c(st.rdnw), fx,
+                        dxi, dyi, dt)
+        assert np.count_nonzero((sc > 0) & (sc < 1)) > 10
+        tend = dtype(0.0) + M.pd_apply(sc, fx, c(st.rdnw), dxi, dyi)
+        return M.coupled_update(c(q0), tend, c(st.c1h), c(st.c2h), m0, mn,
+                                dt, True)
+
+    ref_q = oracle_forward(s, np.float64)
+    got_q = adjoint.pd_stage(g, cfg, dt, t(q), t(q0), [t(x) for x in fl],
+                             t(mup_t), t(mup_new))
+    assert _rel(got_q, ref_q) < 1e-13
+    # FP32: bit-identical to the oracle's float32 arithmetic
+    g32 = DomainState.from_host(case, dtype=torch.float32)
+    s32 = O.from_case(case, np.float32)
+    f32 = lambda a: torch.as_tensor(np.ascontiguousarray(
+        np.asarray(a).astype(np.float32)), device=dev)
+    got32 = adjoint.pd_stage(g32, cfg, dt, f32(q), f32(q0),
+                             [f32(x) for x in fl], f32(mup_t), f32(mup_new))
+    ref32 = oracle_forward(s32, np.float32)
+    assert np.array_equal(got32.cpu().numpy(), ref32)
+    # the transpose
+    lam = rng.standard_normal(q.shape)
+    r = A.pd_update_adjoint(s, q, q0, fl, mu0, mu, dt, lam, cfg.dx, cfg.dy)
+    g_q0 = torch.zeros_like(g.qv)
+    g_fl = [torch.zeros_like(g.u), torch.zeros_like(g.v),
+            torch.zeros_like(g.w)]
+    g_mt = torch.zeros_like(g.mup)
+    g_mn = torch.zeros_like(g.mup)
+    g_q = adjoint.pd_stage_adjoint(
+        g, cfg, dt, t(q), t(q0), [t(x) for x in fl], t(mup_t), t(mup_new),
+        t(lam), g_q0, g_fl, g_mt, g_mn)
+    got = {"q": g_q, "q0": g_q0, "ru": g_fl[0], "rv": g_fl[1],
+           "ww": g_fl[2], "mu0": g_mt, "mu": g_mn}
+    bad = {k: _rel(v, r[k]) for k, v in got.items()}
+    print("pd adjoint relative errors", bad, flush=True)
+    assert max(bad.values()) < 1e-10, bad
+
+
+@pytest.mark.parametrize("name,pd", [("dry_flat", 0), ("terrain", 0),
+                                     ("dry_flat", 1), ("terrain", 1)])
+def test_moist_step_adjoint_matches_oracle(name, pd):
+    from arwen_amd import adjoint
+    case, s, g, rng = _moist_states(name, pd=pd)
+    print(f"moist step adjoint vs oracle {name} pd={pd}", flush=True)
     lam = {n: rng.standard_normal(getattr(s, n).shape) for n in MOIST_NAMES}
     ref = A.step_adjoint(s, case.cfg, lam)
     before = {n: getattr(g, n).clone() for n in MOIST_NAMES}
@@ -424,11 +516,12 @@ def test_moist_step_adjoint_matches_oracle(name):
     assert max(bad.values()) < ORACLE_TOL, bad
 
 
-def test_moist_step_adjoint_dot_product_vs_device_step():
+@pytest.mark.parametrize("pd", [0, 1])
+def test_moist_step_adjoint_dot_product_vs_device_step(pd):
     from arwen_amd import adjoint, dycore
-    case, s, g, rng = _moist_states("terrain", seed=67)
+    case, s, g, rng = _moist_states("terrain", seed=67, pd=pd)
     cfg = case.cfg
-    print("moist step adjoint FD dot product", flush=True)
+    print(f"moist step adjoint FD dot product pd={pd}", flush=True)
     dev = g.u.device
     base = {n: getattr(g, n).clone() for n in MOIST_NAMES}
     lam = {n: torch.as_tensor(rng.standard_normal(getattr(s, n).shape),
@@ -455,12 +548,13 @@ def test_moist_step_adjoint_dot_product_vs_device_step():
     assert abs(jv - vjt) <= 1e-6 * abs(jv), (jv, vjt)
 
 
+@pytest.mark.parametrize("pd", [0, 1])
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12),
                                        (torch.float32, 1e-5)])
-def test_taped_moist_step_equals_fused_step(dtype, tol):
+def test_taped_moist_step_equals_fused_step(dtype, tol, pd):
     from arwen_amd import adjoint, dycore
-    case, s, g, rng = _moist_states("terrain", dtype=dtype, seed=71)
-    print(f"taped moist step vs fused step {dtype}", flush=True)
+    case, s, g, rng = _moist_states("terrain", dtype=dtype, seed=71, pd=pd)
+    print(f"taped moist step vs fused step {dtype} pd={pd}", flush=True)
     x0 = {n: getattr(g, n).clone() for n in MOIST_NAMES}
     dycore.step(g, case.cfg)
     fused = {n: getattr(g, n).clone() for n in MOIST_NAMES}
@@ -475,11 +569,12 @@ def test_taped_moist_step_equals_fused_step(dtype, tol):
         assert float((a - b).abs().max()) <= tol * scale, n
 
 
-def test_moist_dycore_step_autograd_and_stage_checkpoints():
+@pytest.mark.parametrize("pd", [0, 1])
+def test_moist_dycore_step_autograd_and_stage_checkpoints(pd):
     from arwen_amd import adjoint
-    case, s, g, rng = _moist_states("dry_flat", seed=73)
+    case, s, g, rng = _moist_states("dry_flat", seed=73, pd=pd)
     cfg = case.cfg
-    print("moist dycore step autograd", flush=True)
+    print(f"moist dycore step autograd pd={pd}", flush=True)
     dev = g.u.device
     lam = {n: torch.as_tensor(rng.standard_normal(getattr(s, n).shape),
                               device=dev) for n in MOIST_NAMES}
