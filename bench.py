@@ -395,65 +395,78 @@ def main_adjoint(args):
             ad = float(torch.sum(grad * x))
             rec["fd_rel_err"] = abs(fd - ad) / abs(ad)
         # the whole moist step (qv/qc/qr through the moist EOS, buoyancy
-        # loading and the sumflux-coupled scalar update) against its
-        # adjoint: the taped step, then the reverse sweep
-        names = adjoint.STAGE_STATE + adjoint.SPECIES
-        holder = {}
+        # loading and the sumflux-coupled scalar update), without and with
+        # ("_pd" keys; the reference's default moist_adv_opt = 1) the final
+        # stage's positive-definite limiter, against its adjoint: the taped
+        # step, then the reverse sweep
+        cfg_pd = dataclasses.replace(cfg, moist_adv_opt=1)
+        for tag, mcfg in (("", cfg), ("_pd", cfg_pd)):
+            names = adjoint.STAGE_STATE + adjoint.SPECIES
+            holder = {}
 
-        def moist_taped():
-            for a, t in saved.items():
-                getattr(st, a).copy_(t)
-            holder["tape"] = adjoint.step_taped(st, cfg)
-
-        gen = torch.Generator(device=st.device).manual_seed(7)
-        lam_m = {n: torch.randn(getattr(st, n).shape, dtype=dtype,
-                                device=st.device, generator=gen)
-                 for n in names}
-        t_mt = _time_ms(moist_taped, reps) - t_copy
-        t_mr = _time_ms(lambda: adjoint.step_adjoint_taped(
-            st, holder["tape"], lam_m), reps)
-        holder.clear()
-        rec.update({"moist_taped_step_ms": round(t_mt, 4),
-                    "moist_reverse_sweep_ms": round(t_mr, 4),
-                    "moist_reverse_to_forward": round(t_mr / t_step, 3)})
-        if dtype == torch.float64:
-            # <J v, lam> by central differences of the fused step against
-            # <v, J^T lam>, v scaled per field, about the state after three
-            # steps (the initial state is at rest vertically, where the
-            # vertical upwind switch sign(Omega) is not differentiable)
-            for a, t in saved.items():
-                getattr(st, a).copy_(t)
-            for _ in range(3):
-                dycore.step(st, cfg)
-            base = {a: getattr(st, a).clone() for a in saved}
-            gm = adjoint.step_adjoint(st, cfg, lam_m)
-            scl = {"u": 1.0, "v": 1.0, "w": 0.1, "thp": 0.5, "php": 20.0,
-                   "mup": 20.0, "qv": 1e-4, "qc": 1e-5, "qr": 1e-5}
-            vv = {n: torch.randn(getattr(st, n).shape, dtype=dtype,
-                                 device=st.device, generator=gen) * scl[n]
-                  for n in names}
-            for n in adjoint.SPECIES:      # keep the scalars non-negative
-                vv[n] = vv[n] * (base[n] > 0)
-
-            def moved(hh):
-                for a, t in base.items():
+            def moist_taped():
+                for a, t in saved.items():
                     getattr(st, a).copy_(t)
-                for n in names:
-                    getattr(st, n).add_(hh * vv[n])
-                dycore.update_diagnostics(st)
-                dycore.step(st, cfg)
-                return {n: getattr(st, n).clone() for n in names}
+                holder["tape"] = adjoint.step_taped(st, mcfg)
 
-            vjt = sum(float(torch.sum(vv[n] * gm[n])) for n in names)
-            fd = {}
-            for hh in (1e-3, 1e-4, 1e-5, 1e-6):
-                fp, fm = moved(hh), moved(-hh)
-                jv = sum(float(torch.sum((fp[n] - fm[n]) / (2 * hh)
-                                         * lam_m[n])) for n in names)
-                fd[f"{hh:g}"] = abs(jv - vjt) / abs(jv)
-                del fp, fm
-            rec["moist_step_fd_rel_err_by_h"] = fd
-            del gm, base
+            gen = torch.Generator(device=st.device).manual_seed(7)
+            lam_m = {n: torch.randn(getattr(st, n).shape, dtype=dtype,
+                                    device=st.device, generator=gen)
+                     for n in names}
+
+            def moist_plain():
+                for a, t in saved.items():
+                    getattr(st, a).copy_(t)
+                dycore.step(st, mcfg)
+
+            t_ms = _time_ms(moist_plain, reps) - t_copy
+            t_mt = _time_ms(moist_taped, reps) - t_copy
+            t_mr = _time_ms(lambda: adjoint.step_adjoint_taped(
+                st, holder["tape"], lam_m), reps)
+            holder.clear()
+            rec.update({f"moist{tag}_step_ms": round(t_ms, 4),
+                        f"moist{tag}_taped_step_ms": round(t_mt, 4),
+                        f"moist{tag}_reverse_sweep_ms": round(t_mr, 4),
+                        f"moist{tag}_reverse_to_forward": round(t_mr / t_ms,
+                                                                3)})
+            if dtype == torch.float64:
+                # <J v, lam> by central differences of the fused step against
+                # <v, J^T lam>, v scaled per field, about the state after three
+                # steps (the initial state is at rest vertically, where the
+                # vertical upwind switch sign(Omega) is not differentiable)
+                for a, t in saved.items():
+                    getattr(st, a).copy_(t)
+                for _ in range(3):
+                    dycore.step(st, mcfg)
+                base = {a: getattr(st, a).clone() for a in saved}
+                gm = adjoint.step_adjoint(st, mcfg, lam_m)
+                scl = {"u": 1.0, "v": 1.0, "w": 0.1, "thp": 0.5, "php": 20.0,
+                       "mup": 20.0, "qv": 1e-4, "qc": 1e-5, "qr": 1e-5}
+                vv = {n: torch.randn(getattr(st, n).shape, dtype=dtype,
+                                     device=st.device, generator=gen) * scl[n]
+                      for n in names}
+                for n in adjoint.SPECIES:      # keep the scalars non-negative
+                    vv[n] = vv[n] * (base[n] > 0)
+
+                def moved(hh):
+                    for a, t in base.items():
+                        getattr(st, a).copy_(t)
+                    for n in names:
+                        getattr(st, n).add_(hh * vv[n])
+                    dycore.update_diagnostics(st)
+                    dycore.step(st, mcfg)
+                    return {n: getattr(st, n).clone() for n in names}
+
+                vjt = sum(float(torch.sum(vv[n] * gm[n])) for n in names)
+                fd = {}
+                for hh in (1e-3, 1e-4, 1e-5, 1e-6):
+                    fp, fm = moved(hh), moved(-hh)
+                    jv = sum(float(torch.sum((fp[n] - fm[n]) / (2 * hh)
+                                             * lam_m[n])) for n in names)
+                    fd[f"{hh:g}"] = abs(jv - vjt) / abs(jv)
+                    del fp, fm
+                rec[f"moist{tag}_step_fd_rel_err_by_h"] = fd
+                del gm, base
         for a, t in saved.items():
             getattr(st, a).copy_(t)
         dycore.update_diagnostics(st)
