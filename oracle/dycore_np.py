@@ -20,6 +20,8 @@ from __future__ import annotations
 
 from types import SimpleNamespace
 
+import math
+
 import numpy as np
 
 G = 9.81
@@ -1048,6 +1050,142 @@ def advance_scalars_stage(s, ru_m, rv_m, ww_m, dt_stage, final, dx, dy,
 # The step
 # ---------------------------------------------------------------------------
 
+def diff6_factor(cfg) -> float:
+    """Per-step 6th-order filter factor (reference dycore.py:1767-1786):
+    the namelist factor, or, when the model clock step clock_dt differs
+    from dt, the factor whose dt/clock_dt-fold composition removes the
+    namelist fraction of a 2dx mode per clock step:
+    1 - (1 - factor)^(dt/clock_dt)."""
+    clock = cfg.clock_dt if cfg.clock_dt > 0.0 else cfg.dt
+    factor = float(cfg.diff_6th_factor)
+    if clock == cfg.dt or factor == 1.0:
+        return factor
+    return -math.expm1((cfg.dt / clock) * math.log1p(-factor))
+
+
+# Xue (2000) eq. 3 flux weights of the 6th-order filter: 10, 5, 1
+DIFF6_W = (10.0, 5.0)
+
+
+def diff6(f, mut, c1, c2, coef, mono, variant, wstag, ny, nx, slope=False,
+          phb=None, dzthr_x=1.0, dzthr_y=1.0):
+    """WRF sixth_order_diffusion's coupled tendency of one periodic field
+    (reference kernels/diff6.cu; Xue 2000 eq. 3 fluxes, the mono = 2
+    up-gradient knock-down, the face masses of the u (variant 1), v (2)
+    and mass/w (0) positions, the diff_6th_slopeopt taper when ``slope``;
+    msf = 1).  f: (nlev, nys, nxs); returns 0 + (x part + y part), zero on
+    a w field's boundary levels (wstag)."""
+    t = f.dtype.type
+    nlev, nys, nxs = f.shape
+    jj = np.arange(nys) % ny
+    ii = np.arange(nxs) % nx
+    core = f[:, :ny, :nx]
+
+    def fx(m):
+        return core[:, jj][:, :, (ii + m) % nx]
+
+    def fy(m):
+        return core[:, (jj + m) % ny][:, :, ii]
+
+    def mu(dj, di):
+        return mut[((jj + dj) % ny)[:, None], ((ii + di) % nx)[None, :]][None]
+
+    ten, five = t(DIFF6_W[0]), t(DIFF6_W[1])
+    q, h, one, zero = t(0.25), t(0.5), t(1.0), t(0.0)
+    C1, C2 = c1[:, None, None], c2[:, None, None]
+
+    def fluxes(g):
+        p0 = ten * (g(0) - g(-1)) - five * (g(1) - g(-2)) + (g(2) - g(-3))
+        p1 = ten * (g(1) - g(0)) - five * (g(2) - g(-1)) + (g(3) - g(-2))
+        if mono == 2:
+            p0 = np.where(p0 * (g(0) - g(-1)) <= zero, zero, p0)
+            p1 = np.where(p1 * (g(1) - g(0)) <= zero, zero, p1)
+        return p0, p1
+
+    def ph(dj, di):
+        lev = np.minimum(np.arange(nlev), phb.shape[0] - 1)
+        return phb[lev][:, ((jj + dj) % ny)[:, None], ((ii + di) % nx)[None, :]]
+
+    def taper(dz, thr):
+        return np.maximum(one - dz / t(thr), zero)
+
+    px0, px1 = fluxes(fx)
+    if variant == 1:
+        mx0, mx1 = mu(0, -1), mu(0, 0)
+    elif variant == 2:
+        mx0 = q * (mu(-1, -1) + mu(-1, 0) + mu(0, -1) + mu(0, 0))
+        mx1 = q * (mu(-1, 0) + mu(-1, 1) + mu(0, 0) + mu(0, 1))
+    else:
+        mx0 = h * (mu(0, -1) + mu(0, 0))
+        mx1 = h * (mu(0, 0) + mu(0, 1))
+    sx0 = sx1 = one
+    sy0 = sy1 = one
+    if slope:
+        a0 = np.abs(ph(0, 0) - ph(0, -1))
+        a1 = np.abs(ph(0, 1) - ph(0, 0))
+        d0, d1 = a0, a1
+        if variant == 1:
+            d0 = np.fmax(a0, np.abs(ph(0, -1) - ph(0, -2)))
+            d1 = np.fmax(a1, a0)
+        elif variant == 2:
+            d0 = np.fmax(a0, np.abs(ph(-1, 0) - ph(-1, -1)))
+            d1 = np.fmax(a1, np.abs(ph(-1, 1) - ph(-1, 0)))
+        sx0, sx1 = taper(d0, dzthr_x), taper(d1, dzthr_x)
+        b0 = np.abs(ph(0, 0) - ph(-1, 0))
+        b1 = np.abs(ph(1, 0) - ph(0, 0))
+        d0, d1 = b0, b1
+        if variant == 1:
+            d0 = np.fmax(b0, np.abs(ph(0, -1) - ph(-1, -1)))
+            d1 = np.fmax(b1, np.abs(ph(1, -1) - ph(0, -1)))
+        elif variant == 2:
+            d0 = np.fmax(b0, np.abs(ph(-1, 0) - ph(-2, 0)))
+            d1 = np.fmax(b1, b0)
+        sy0, sy1 = taper(d0, dzthr_y), taper(d1, dzthr_y)
+    cf = t(coef)
+    tx = cf * (sx1 * (C1 * mx1 + C2) * px1 - sx0 * (C1 * mx0 + C2) * px0)
+    py0, py1 = fluxes(fy)
+    if variant == 1:
+        my0 = q * (mu(-1, -1) + mu(-1, 0) + mu(0, -1) + mu(0, 0))
+        my1 = q * (mu(0, -1) + mu(0, 0) + mu(1, -1) + mu(1, 0))
+    elif variant == 2:
+        my0, my1 = mu(-1, 0), mu(0, 0)
+    else:
+        my0 = h * (mu(-1, 0) + mu(0, 0))
+        my1 = h * (mu(0, 0) + mu(1, 0))
+    ty = cf * (sy1 * (C1 * my1 + C2) * py1 - sy0 * (C1 * my0 + C2) * py0)
+    out = zero + (tx + ty)
+    if wstag:
+        out[0] = zero
+        out[-1] = zero
+    return out
+
+
+def diff6_held(s, cfg):
+    """The once-per-step 6th-order tendencies of u, v, w, theta' from the
+    time-t fields (reference dycore.py:1345-1420 with km_opt != 2..4:
+    prepare_fixed_tendencies; launch_diff6 dycore.py:1647-1740 for the
+    coefficient and the slope threshold), coupled with the time-t column
+    mass; added to every RK stage's tendencies."""
+    t = s.dtype.type
+    ny, nx = s.mub2d.shape
+    mut = s.mub2d + s.mup0
+    coef = float(diff6_factor(cfg)) * 0.015625 / (2.0 * cfg.dt)
+    slope = int(getattr(cfg, "diff_6th_slopeopt", 0)) >= 1 and s.base3d
+    thr = float(getattr(cfg, "diff_6th_thresh", 0.10))
+    kw = dict(slope=slope, phb=s.phb if slope else None,
+              dzthr_x=float(t(thr * 9.81 * cfg.dx)),
+              dzthr_y=float(t(thr * 9.81 * cfg.dy)))
+    mono = int(cfg.diff_6th_opt)
+    return {"ru_t": diff6(s.u0, mut, s.c1h, s.c2h, coef, mono, 1, False,
+                          ny, nx, **kw),
+            "rv_t": diff6(s.v0, mut, s.c1h, s.c2h, coef, mono, 2, False,
+                          ny, nx, **kw),
+            "rw_t": diff6(s.w0, mut, s.c1f, s.c2f, coef, mono, 0, True,
+                          ny, nx, **kw),
+            "rth_t": diff6(s.thp0, mut, s.c1h, s.c2h, coef, mono, 0, False,
+                           ny, nx, **kw)}
+
+
 def step(s, cfg) -> None:
     """One RK3 split-explicit step (reference dycore.py:2161-2503, dry or
     qv-tracer, periodic, msf = 1)."""
@@ -1065,6 +1203,8 @@ def step(s, cfg) -> None:
               (ns, cfg.dt / ns))
     prec = bool(getattr(cfg, "precision_form", False)) \
         and s.dtype == np.float32
+    held = diff6_held(s, cfg) if getattr(cfg, "diff_6th_opt", 0) > 0 \
+        else {}
     for istage, (nsub, dtau) in enumerate(stages):
         for name in ("ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t"):
             getattr(s, name)[...] = 0.0
@@ -1073,6 +1213,8 @@ def step(s, cfg) -> None:
         add_slow_tendencies(s, ru, rv, ww, cfg.dx, cfg.dy, cfg.top_lid, prec)
         if cfg.km_opt == 1:
             add_diffusion_tendencies(s, cfg.khdif, cfg.kvdif, cfg.dx, cfg.dy)
+        for name, h in held.items():       # held 6th-order tendencies
+            getattr(s, name)[...] = getattr(s, name) + h
         small_step_init(s, prec)
         coefs = calc_coefs(s, dtau, cfg.epssm, cfg.top_lid, prec)
         if moist:

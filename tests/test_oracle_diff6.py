@@ -1,0 +1,105 @@
+"""The oracle's 6th-order filter (oracle/dycore_np.diff6, restating the
+reference's kernels/diff6.cu) pinned by its closed forms: on a uniform
+column mass the plain operator (mono = 1) maps the Fourier mode
+exp(i(ax + by)) to -(factor C / (2 dt)) (sin^6(a/2) + sin^6(b/2)) times
+itself, so one step removes `factor` of a 2-D 2dx checkerboard; the w
+field's boundary levels get nothing; the slope taper scales a uniform
+slope's fluxes by max(1 - dz/dzthr, 0); the monotonic form (mono = 2)
+never creates new extrema on a front.  A planted stencil mistake fails."""
+import numpy as np
+import pytest
+
+from oracle import dycore_np as D
+
+NZ, NY, NX = 3, 12, 16
+
+
+def _mode(a, b, shape):
+    nlev, nys, nxs = shape
+    j = np.arange(nys)[:, None]
+    i = np.arange(nxs)[None, :]
+    return np.broadcast_to(np.cos(a * i + b * j), shape).copy()
+
+
+@pytest.mark.parametrize("variant,stag", [(0, ""), (1, "x"), (2, "y"),
+                                          (0, "z")])
+@pytest.mark.parametrize("a,b", [(np.pi, np.pi), (np.pi, 0.0),
+                                 (2 * np.pi * 3 / 16, 2 * np.pi / 12),
+                                 (2 * np.pi / 16, 2 * np.pi * 5 / 12)])
+def test_fourier_symbol(variant, stag, a, b):
+    nlev = NZ + 1 if stag == "z" else NZ
+    shape = (nlev, NY + (stag == "y"), NX + (stag == "x"))
+    f = _mode(a, b, shape)
+    mu = np.full((NY, NX), 8192.0)
+    c1 = np.linspace(0.2, 1.0, nlev)
+    c2 = np.linspace(50.0, 0.0, nlev)
+    factor, dt = 0.12, 6.0
+    coef = factor * 0.015625 / (2.0 * dt)
+    t = D.diff6(f, mu, c1, c2, coef, 1, variant, stag == "z", NY, NX)
+    C = (c1 * 8192.0 + c2)[:, None, None]
+    want = -(factor / (2.0 * dt)) * C * (np.sin(a / 2) ** 6
+                                         + np.sin(b / 2) ** 6) * f
+    if stag == "z":
+        want[0] = want[-1] = 0.0
+    assert np.max(np.abs(t - want)) <= 1e-12 * np.max(np.abs(want)) + 1e-300
+
+
+def test_checkerboard_loses_factor_per_step():
+    f = _mode(np.pi, np.pi, (NZ, NY, NX))
+    mu = np.full((NY, NX), 6000.0)
+    c1, c2 = np.ones(NZ), np.zeros(NZ)
+    factor, dt = 0.12, 10.0
+    coef = factor * 0.015625 / (2.0 * dt)
+    t = D.diff6(f, mu, c1, c2, coef, 2, 0, False, NY, NX)
+    new = f + dt * t / 6000.0
+    assert np.allclose(new, (1.0 - factor) * f, rtol=0, atol=1e-13)
+
+
+def test_monotonic_front_keeps_bounds_and_plain_overshoots():
+    x = np.arange(NX)
+    f = np.broadcast_to(np.where((x >= 4) & (x < 10), 1.0, 0.0),
+                        (NZ, NY, NX)).copy()
+    mu = np.full((NY, NX), 5000.0)
+    c1, c2 = np.ones(NZ), np.zeros(NZ)
+    dt = 5.0
+    coef = 1.0 * 0.015625 / (2.0 * dt)
+    new = {m: f + dt * D.diff6(f, mu, c1, c2, coef, m, 0, False, NY,
+                               NX) / 5000.0 for m in (1, 2)}
+    assert new[2].min() >= 0.0 and new[2].max() <= 1.0
+    assert new[1].min() < -1e-3 or new[1].max() > 1.0 + 1e-3
+
+
+def test_slope_taper_on_uniform_slope():
+    f = _mode(np.pi, 0.0, (NZ, NY, NX))
+    mu = np.full((NY, NX), 5000.0)
+    c1, c2 = np.ones(NZ), np.zeros(NZ)
+    # a base geopotential rising 300 m^2 s^-2 per column in x only: every
+    # x face jump is 300 (except across the periodic seam), y jumps are 0
+    phb = np.broadcast_to(300.0 * np.arange(NX)[None, None, :],
+                          (NZ + 1, NY, NX)).copy()
+    phb[..., :] = 300.0 * np.minimum(np.arange(NX), 4)[None, None, :]
+    dt = 6.0
+    coef = 0.12 * 0.015625 / (2.0 * dt)
+    base = D.diff6(f, mu, c1, c2, coef, 1, 0, False, NY, NX)
+    tap = D.diff6(f, mu, c1, c2, coef, 1, 0, False, NY, NX, slope=True,
+                  phb=phb, dzthr_x=1000.0, dzthr_y=1000.0)
+    # columns 6..10 see flat faces on both sides: untapered
+    assert np.array_equal(tap[..., 6:11], base[..., 6:11])
+    # column 2: both x faces jump 300 -> x fluxes scaled by 0.7
+    fl = tap[..., 2] / base[..., 2]
+    assert np.allclose(fl, 0.7, rtol=1e-12)
+
+
+def test_planted_stencil_mistake_fails_the_symbol(monkeypatch):
+    """A 4 in place of the 5 in Xue's flux breaks the Fourier symbol."""
+    a = 2 * np.pi * 3 / 16
+    f = _mode(a, 0.0, (NZ, NY, NX))
+    mu = np.full((NY, NX), 4096.0)
+    c1, c2 = np.ones(NZ), np.zeros(NZ)
+    coef = 0.12 * 0.015625 / 12.0
+    want = -(0.12 / 12.0) * 4096.0 * np.sin(a / 2) ** 6 * f
+    ok = D.diff6(f, mu, c1, c2, coef, 1, 0, False, NY, NX)
+    assert np.max(np.abs(ok - want)) < 1e-12 * np.max(np.abs(want))
+    monkeypatch.setattr(D, "DIFF6_W", (10.0, 4.0))
+    bad = D.diff6(f, mu, c1, c2, coef, 1, 0, False, NY, NX)
+    assert np.max(np.abs(bad - want)) > 1e-3 * np.max(np.abs(want))
