@@ -12,7 +12,7 @@ from pathlib import Path
 LIB_NAME = "libarwen_dycore.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -31,6 +31,7 @@ DOMAIN_ARRAYS = (
 )
 SCRATCH_ARRAYS = ("ww", "gam", "dmdt_rf", "mup_next")
 DIFF_ARRAYS = ("rdzf_h", "rdzc_h", "rdzf_w", "rdzc_w")
+DIFF6_ARRAYS = ("d6_u", "d6_v", "d6_w", "d6_t")
 #: moist scalars: state arrays, then the third stage buffer (scratch)
 MOIST_ARRAYS = ("qv", "qc", "qr", "qv0", "qc0", "qr0", "qsv", "qsc", "qsr")
 #: moist scratch: WRF sumflux averages and the PD limiter's buffers
@@ -46,7 +47,8 @@ class ArwDomain(ctypes.Structure):
                 + [(n, _P) for n in DIFF_ARRAYS]
                 + [(n, _P) for n in MOIST_ARRAYS]
                 + [(n, _P) for n in MOIST_SCRATCH]
-                + [("tri64", _P), ("pp", _P)])
+                + [("tri64", _P), ("pp", _P)]
+                + [(n, _P) for n in DIFF6_ARRAYS])
 
 
 class ArwDomain64(ctypes.Structure):
@@ -61,7 +63,9 @@ class ArwParams(ctypes.Structure):
                 ("khdif", _D), ("kvdif", _D),
                 ("damp_opt", _I), ("zdamp", _D), ("dampcoef", _D),
                 ("top_lid", _I), ("moist_adv_opt", _I),
-                ("precision_form", _I)]
+                ("precision_form", _I), ("diff_6th_opt", _I),
+                ("diff_6th_factor", _D), ("diff_6th_slopeopt", _I),
+                ("diff_6th_thresh", _D)]
 
 
 class ArwTracerCkpt(ctypes.Structure):

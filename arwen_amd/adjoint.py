@@ -659,6 +659,9 @@ class StepTape:
 def _check_adjoint_cfg(state, cfg):
     if getattr(cfg, "precision_form", False):
         raise _lib.ArwenError("step adjoint: plain form only")
+    if getattr(cfg, "diff_6th_opt", 0):
+        raise _lib.ArwenError("step adjoint: no 6th-order filter "
+                              "(diff_6th_opt = 0)")
     if _uses_diffusion(cfg) and state.thb.dim() == 3:
         raise _lib.ArwenError("step adjoint: constant-K diffusion needs a "
                               "flat base")

@@ -10,6 +10,8 @@ unchanged; :func:`validate_for_step` refuses any non-default value with a
 """
 from __future__ import annotations
 
+import math
+
 from dataclasses import dataclass, fields, replace
 
 
@@ -42,6 +44,8 @@ class RunConfig:
     moist_adv_opt: int = 1
     diff_6th_opt: int = 0
     diff_6th_factor: float = 0.12
+    diff_6th_slopeopt: int = 0
+    diff_6th_thresh: float = 0.10
     km_opt: int = 1
     w_damping: int = 0
     open_x: bool = False
@@ -113,9 +117,15 @@ def validate_for_step(cfg: RunConfig, *, acoustic: bool = True) -> None:
     for name in ("open_x", "open_y", "specified", "nested", "moist_cq"):
         if getattr(cfg, name):
             raise NotImplementedError(f"{name}=True is not part of this port")
-    if cfg.emdiv > 0.0 or cfg.w_damping or cfg.diff_6th_opt:
+    if cfg.emdiv > 0.0 or cfg.w_damping:
         raise NotImplementedError(
-            "emdiv / w_damping / diff_6th_opt are not part of this port")
+            "emdiv / w_damping are not part of this port")
+    if cfg.diff_6th_opt not in (0, 1, 2):
+        raise ValueError(f"diff_6th_opt must be 0, 1 or 2, got "
+                         f"{cfg.diff_6th_opt}")
+    if cfg.diff_6th_opt and (cfg.moist or cfg.precision_form):
+        raise NotImplementedError(
+            "diff_6th_opt is ported for dry states in the plain form")
     if cfg.mp_physics != 0 or cfg.bl_pbl_physics or cfg.ra_physics \
             or cfg.cu_physics or cfg.sf_sfclay_physics \
             or cfg.sf_surface_physics:
@@ -128,3 +138,21 @@ def validate_for_step(cfg: RunConfig, *, acoustic: bool = True) -> None:
             f"moist_adv_opt={cfg.moist_adv_opt}: only 0 (unlimited) and 1 "
             "(positive definite) are ported")
 
+
+
+def diff6_factor(cfg: RunConfig) -> float:
+    """Per-step 6th-order filter factor (reference dycore.py:1767-1786):
+    the namelist factor, or, when the model clock step ``clock_dt``
+    differs from dt, 1 - (1 - factor)^(dt/clock_dt), so that the steps of
+    one clock interval together remove the namelist fraction of a 2dx
+    mode."""
+    clock = cfg.clock_dt if cfg.clock_dt > 0.0 else cfg.dt
+    factor = float(cfg.diff_6th_factor)
+    if clock == cfg.dt:
+        return factor
+    if not 0.0 <= factor <= 1.0:
+        raise ValueError("clock-scaled diff_6th_factor must lie in [0, 1], "
+                         f"got {factor}")
+    if factor == 1.0:
+        return 1.0
+    return -math.expm1((cfg.dt / clock) * math.log1p(-factor))

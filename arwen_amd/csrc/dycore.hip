@@ -1563,6 +1563,158 @@ int32_t pd_forward(const Dom* d, const arw_params* prm, double dt_stage,
     return launched("k_pd_apply");
 }
 
+// WRF sixth_order_diffusion (reference kernels/diff6.cu, launched by
+// dycore.py:1647-1740 from prepare_fixed_tendencies :1345-1420): the
+// coupled tendency of one periodic field f (nlev, nys, nxs) from Xue's
+// (2000) eq. 3 face fluxes, the mono = 2 up-gradient knock-down, the face
+// masses of the u (variant 1), v (2) and mass/w (0) positions and the
+// diff_6th_slopeopt taper on a terrain base (msf = 1).  out = 0 + (x part
+// + y part); a w field's boundary levels (wstag) get 0.
+__global__ void k_diff6(Dom d, const real* __restrict__ f,
+                        real* __restrict__ out, const real* __restrict__ c1,
+                        const real* __restrict__ c2, real coef, int mono,
+                        int slope, real dzthr_x, real dzthr_y, int nlev,
+                        int nys, int nxs, int variant, int wstag)
+{
+    const int ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nxs || j >= nys || k >= nlev) return;
+    const size_t o = ((size_t)k * nys + j) * nxs + i;
+    if (wstag && (k == 0 || k == nlev - 1)) { out[o] = R(0.0); return; }
+    const int ic = i % nx, jc = j % ny;
+    auto FX = [&](int m) { return f[((size_t)k * nys + jc) * nxs + pmod(ic + m, nx)]; };
+    auto FY = [&](int m) { return f[((size_t)k * nys + pmod(jc + m, ny)) * nxs + ic]; };
+    auto MU = [&](int jj, int ii) {
+        const int c = pmod(jj, ny) * nx + pmod(ii, nx);
+        return d.mub2d[c] + d.mup[c];
+    };
+    auto PHB = [&](int jj, int ii) {
+        return d.phb[((size_t)k * ny + pmod(jj, ny)) * nx + pmod(ii, nx)];
+    };
+    const real ten = R(10.0), five = R(5.0), zero = R(0.0), one = R(1.0);
+    real px0 = ten * (FX(0) - FX(-1)) - five * (FX(1) - FX(-2)) + (FX(2) - FX(-3));
+    real px1 = ten * (FX(1) - FX(0)) - five * (FX(2) - FX(-1)) + (FX(3) - FX(-2));
+    if (mono == 2) {
+        if (px0 * (FX(0) - FX(-1)) <= zero) px0 = zero;
+        if (px1 * (FX(1) - FX(0)) <= zero) px1 = zero;
+    }
+    real mx0, mx1;
+    if (variant == 1) {
+        mx0 = MU(jc, ic - 1);
+        mx1 = MU(jc, ic);
+    } else if (variant == 2) {
+        mx0 = R(0.25) * (MU(jc - 1, ic - 1) + MU(jc - 1, ic) + MU(jc, ic - 1) + MU(jc, ic));
+        mx1 = R(0.25) * (MU(jc - 1, ic) + MU(jc - 1, ic + 1) + MU(jc, ic) + MU(jc, ic + 1));
+    } else {
+        mx0 = R(0.5) * (MU(jc, ic - 1) + MU(jc, ic));
+        mx1 = R(0.5) * (MU(jc, ic) + MU(jc, ic + 1));
+    }
+    real sx0 = one, sx1 = one, sy0 = one, sy1 = one;
+    if (slope) {
+        const real a0 = fabs(PHB(jc, ic) - PHB(jc, ic - 1));
+        const real a1 = fabs(PHB(jc, ic + 1) - PHB(jc, ic));
+        real d0 = a0, d1 = a1;
+        if (variant == 1) {
+            d0 = fmax(a0, fabs(PHB(jc, ic - 1) - PHB(jc, ic - 2)));
+            d1 = fmax(a1, a0);
+        } else if (variant == 2) {
+            d0 = fmax(a0, fabs(PHB(jc - 1, ic) - PHB(jc - 1, ic - 1)));
+            d1 = fmax(a1, fabs(PHB(jc - 1, ic + 1) - PHB(jc - 1, ic)));
+        }
+        sx0 = fmax(one - d0 / dzthr_x, zero);
+        sx1 = fmax(one - d1 / dzthr_x, zero);
+        const real b0 = fabs(PHB(jc, ic) - PHB(jc - 1, ic));
+        const real b1 = fabs(PHB(jc + 1, ic) - PHB(jc, ic));
+        d0 = b0; d1 = b1;
+        if (variant == 1) {
+            d0 = fmax(b0, fabs(PHB(jc, ic - 1) - PHB(jc - 1, ic - 1)));
+            d1 = fmax(b1, fabs(PHB(jc + 1, ic - 1) - PHB(jc, ic - 1)));
+        } else if (variant == 2) {
+            d0 = fmax(b0, fabs(PHB(jc - 1, ic) - PHB(jc - 2, ic)));
+            d1 = fmax(b1, b0);
+        }
+        sy0 = fmax(one - d0 / dzthr_y, zero);
+        sy1 = fmax(one - d1 / dzthr_y, zero);
+    }
+    const real tx = coef * (sx1 * (c1[k] * mx1 + c2[k]) * px1
+                            - sx0 * (c1[k] * mx0 + c2[k]) * px0);
+    real py0 = ten * (FY(0) - FY(-1)) - five * (FY(1) - FY(-2)) + (FY(2) - FY(-3));
+    real py1 = ten * (FY(1) - FY(0)) - five * (FY(2) - FY(-1)) + (FY(3) - FY(-2));
+    if (mono == 2) {
+        if (py0 * (FY(0) - FY(-1)) <= zero) py0 = zero;
+        if (py1 * (FY(1) - FY(0)) <= zero) py1 = zero;
+    }
+    real my0, my1;
+    if (variant == 1) {
+        my0 = R(0.25) * (MU(jc - 1, ic - 1) + MU(jc - 1, ic) + MU(jc, ic - 1) + MU(jc, ic));
+        my1 = R(0.25) * (MU(jc, ic - 1) + MU(jc, ic) + MU(jc + 1, ic - 1) + MU(jc + 1, ic));
+    } else if (variant == 2) {
+        my0 = MU(jc - 1, ic);
+        my1 = MU(jc, ic);
+    } else {
+        my0 = R(0.5) * (MU(jc - 1, ic) + MU(jc, ic));
+        my1 = R(0.5) * (MU(jc, ic) + MU(jc + 1, ic));
+    }
+    const real ty = coef * (sy1 * (c1[k] * my1 + c2[k]) * py1
+                            - sy0 * (c1[k] * my0 + c2[k]) * py0);
+    out[o] = zero + (tx + ty);
+}
+
+// The held tendencies join every RK stage's slow tendencies, after the
+// constant-K diffusion (reference dycore.py:2389-2390,
+// add_fixed_dry_tendencies :1422-1430).
+__global__ void k_add_held(Dom d)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    if (j < ny && k < nz) {
+        const int x = iU(k, j, i, ny, nx);
+        d.ru_t[x] = d.ru_t[x] + d.d6_u[x];
+    }
+    if (i < nx && k < nz) {
+        const int x = iV(k, j, i, ny, nx);
+        d.rv_t[x] = d.rv_t[x] + d.d6_v[x];
+    }
+    if (i < nx && j < ny) {
+        const int x = iM(k, j, i, ny, nx);
+        d.rw_t[x] = d.rw_t[x] + d.d6_w[x];
+        if (k < nz) d.rth_t[x] = d.rth_t[x] + d.d6_t[x];
+    }
+}
+
+// The four held tendencies from the time-t fields of d (u, v, w, thp and
+// mub2d + mup), once per step.
+int32_t held_diff6(const Dom& d, const arw_params* prm, hipStream_t st)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    const double clock_factor = prm->diff_6th_factor;
+    const real coef = (real)(clock_factor * 0.015625 / (2.0 * prm->dt));
+    const int slope = (prm->diff_6th_slopeopt >= 1 && d.base3d) ? 1 : 0;
+    const real tx = (real)(prm->diff_6th_thresh * 9.81 * prm->dx);
+    const real ty = (real)(prm->diff_6th_thresh * 9.81 * prm->dy);
+    const int mono = prm->diff_6th_opt;
+    struct Row { const real* f; real* out; const real *c1, *c2; int nlev, nys, nxs, var, wst; };
+    const Row rows[4] = {
+        {d.u, d.d6_u, d.c1h, d.c2h, nz, ny, nx + 1, 1, 0},
+        {d.v, d.d6_v, d.c1h, d.c2h, nz, ny + 1, nx, 2, 0},
+        {d.w, d.d6_w, d.c1f, d.c2f, nz + 1, ny, nx, 0, 1},
+        {d.thp, d.d6_t, d.c1h, d.c2h, nz, ny, nx, 0, 0}};
+    for (const Row& r : rows) {
+        hipLaunchKernelGGL(k_diff6, dim3((r.nxs + 63) / 64, (r.nys + 3) / 4, r.nlev),
+                           dim3(64, 4), 0, st, d, r.f, r.out, r.c1, r.c2, coef,
+                           mono, slope, tx, ty, r.nlev, r.nys, r.nxs, r.var,
+                           r.wst);
+        int32_t rc = launched("k_diff6");
+        if (rc) return rc;
+    }
+    return ARW_OK;
+}
+
 }  // namespace ARW_NS
 
 using namespace ARW_NS;
@@ -1773,6 +1925,14 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
     if (rc) return rc;
     if ((rc = check_params(prm, d))) return rc;
     const bool prec = (!ARW_F64) && prm->precision_form != 0;
+    if (prm->diff_6th_opt != 0) {
+        if (prm->diff_6th_opt != 1 && prm->diff_6th_opt != 2)
+            return fail(ARW_E_ARG, "diff_6th_opt must be 0, 1 or 2");
+        if (!d->d6_u || !d->d6_v || !d->d6_w || !d->d6_t)
+            return fail(ARW_E_ARG, "diff_6th_opt needs the d6_* buffers");
+        if (d->qv || prec)
+            return fail(ARW_E_ARG, "diff_6th_opt: dry states, plain form");
+    }
     if (ck) {
         if (!ck->mup_t) return fail(ARW_E_ARG, "checkpoint: null mup_t");
         for (int s = 0; s < 3; ++s)
@@ -1795,6 +1955,8 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
     if (ck && hipMemcpyAsync(ck->mup_t, A[5], (size_t)d->ny * d->nx * RB,
                              hipMemcpyDeviceToDevice, st) != hipSuccess)
         return fail(ARW_E_LAUNCH, "checkpoint mu' copy failed");
+    const bool d6 = prm->diff_6th_opt > 0;
+    if (d6 && (rc = held_diff6(*d, prm, st))) return rc;
     for (int stage = 0; stage < 3; ++stage) {
         Dom dd = *d;
         // time-t fields: always A
@@ -1821,6 +1983,11 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
         }
         Sc s = make_sc(prm, dtaus[stage], 1);
         if ((rc = stage_prepare(dd, s, stage == 0, st))) return rc;
+        if (d6) {
+            hipLaunchKernelGGL(k_add_held, union_grid(dd, dd.nz + 1),
+                               dim3(64, 4), 0, st, dd);
+            if ((rc = launched("k_add_held"))) return rc;
+        }
         Dom ds = dd;
         for (int n = 0; n < nsub[stage]; ++n) {
             Sc sn = make_sc(prm, dtaus[stage], n == 0);
@@ -2099,6 +2266,8 @@ int32_t ARW_API(ensemble_create)(const Dom* members, int32_t n,
 {
     if (!out) return fail(ARW_E_ARG, "null output handle");
     *out = nullptr;
+    if (prm && prm->diff_6th_opt != 0)
+        return fail(ARW_E_ARG, "ensemble: diff_6th_opt is not supported");
     if (!members || n < 1 || n > 65536)
         return fail(ARW_E_ARG, "need 1 <= n <= 65536 members");
     if (groups < 1 || groups > n)

@@ -14,7 +14,7 @@ import math
 import torch
 
 from . import _lib
-from .config import RunConfig, validate_for_step
+from .config import RunConfig, diff6_factor, validate_for_step
 from .state import DomainState, mu_at_u_faces, mu_at_v_faces
 
 
@@ -50,6 +50,10 @@ def params(cfg: RunConfig) -> _lib.ArwParams:
     p.top_lid = int(bool(cfg.top_lid))
     p.moist_adv_opt = int(cfg.moist_adv_opt)
     p.precision_form = int(bool(cfg.precision_form))
+    p.diff_6th_opt = int(cfg.diff_6th_opt)
+    p.diff_6th_factor = float(diff6_factor(cfg)) if cfg.diff_6th_opt else 0.0
+    p.diff_6th_slopeopt = int(cfg.diff_6th_slopeopt)
+    p.diff_6th_thresh = float(cfg.diff_6th_thresh)
     return p
 
 
@@ -89,7 +93,8 @@ def step(state: DomainState, cfg: RunConfig, *, acoustic: bool = True,
             "allocated with the qv/qc/qr scalars, and a dry one none")
     _check_state(state)
     d = state.domain_struct(diffusion=_uses_diffusion(cfg),
-                            precision_form=bool(cfg.precision_form))
+                            precision_form=bool(cfg.precision_form),
+                            diff6=cfg.diff_6th_opt > 0)
     p = params(cfg)
     name, fn = _fn(state, "step")
     _lib.check(fn(d, p, _stream()), name)

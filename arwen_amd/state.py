@@ -206,7 +206,8 @@ class DomainState:
         return self._diff_rdz
 
     def domain_struct(self, diffusion: bool = False,
-                      precision_form: bool = False) -> ArwDomain:
+                      precision_form: bool = False,
+                      diff6: bool = False) -> ArwDomain:
         """Raw-pointer view of the state for the C-ABI."""
         nz, ny, nx = self.shape
         d = ArwDomain64() if self.dtype == torch.float64 else ArwDomain()
@@ -248,6 +249,11 @@ class DomainState:
             d.tri64 = self.scratch((2, nz + 1, ny, nx), "arw_tri64",
                                    dtype=torch.float64).data_ptr()
             d.pp = self.scratch((nz, ny, nx), "arw_pp").data_ptr()
+        if diff6:
+            shapes = {"d6_u": (nz, ny, nx + 1), "d6_v": (nz, ny + 1, nx),
+                      "d6_w": (nz + 1, ny, nx), "d6_t": (nz, ny, nx)}
+            for name, shp in shapes.items():
+                setattr(d, name, self.scratch(shp, "arw_" + name).data_ptr())
         return d
 
     def numpy_fields(self, names=("u", "v", "w", "thp", "php", "mup")):

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define ARW_ABI_VERSION 5
+#define ARW_ABI_VERSION 6
 
 /* flags of the adjoint entry points that accumulate */
 #define ARW_ADJ_CLEAR 1
@@ -84,6 +84,10 @@ typedef struct arw_domain {
      * pressure-gradient terms */
     double *tri64;
     float *pp;
+    /* diff_6th_opt > 0 only (else NULL): the held 6th-order tendencies of
+     * u, v, w, theta' (shapes of u, v, w, thp), formed once per step from
+     * the time-t fields (reference dycore.py:1345-1420) */
+    float *d6_u, *d6_v, *d6_w, *d6_t;
 } arw_domain;
 
 /*
@@ -113,6 +117,7 @@ typedef struct arw64_domain {
     double *pd_fxl, *pd_fxc, *pd_fyl, *pd_fyc, *pd_fzl, *pd_fzc, *pd_sc;
     double *tri64;
     double *pp;                /* unused by the FP64 build */
+    double *d6_u, *d6_v, *d6_w, *d6_t;
 } arw64_domain;
 
 /* Step configuration (the RunConfig fields the step reads,
@@ -136,6 +141,15 @@ typedef struct arw_params {
      * w/phi tridiagonal factored and solved in FP64 (needs d->tri64).  No
      * reference counterpart; ignored by the FP64 build. */
     int32_t precision_form;
+    /* WRF sixth_order_diffusion held once per step (reference dycore.py:
+     * 1345-1420, kernels/diff6.cu): 0 off, 1 plain, 2 monotonic (no
+     * up-gradient flux); the per-step factor (the caller applies the
+     * model-clock scaling of dycore.py:1767-1786); the terrain-slope taper
+     * switch and threshold.  arw_step only; dry states, plain form. */
+    int32_t diff_6th_opt;
+    double diff_6th_factor;
+    int32_t diff_6th_slopeopt;
+    double diff_6th_thresh;
 } arw_params;
 
 int32_t arw_abi_version(void);
