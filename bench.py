@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--ens-members", type=int, default=32)
     ap.add_argument("--no-fp64", action="store_true",
                     help="c5: skip the FP64 error reference")
+    ap.add_argument("--diff6", type=int, default=0, choices=(0, 1, 2),
+                    help="c3/c2: run with WRF's held 6th-order filter "
+                         "(diff_6th_opt; factor 0.12) to time its cost")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=160,
                     help="y rows of the CPU-baseline strip")
@@ -140,6 +143,9 @@ def main():
 
     case, desc = build_case(args.workload)
     cfg = case.cfg
+    if args.diff6:
+        cfg = cfg.with_(diff_6th_opt=args.diff6)
+        desc += f", held 6th-order filter diff_6th_opt={args.diff6}"
     state = DomainState.from_host(case)
     del case
     torch.cuda.synchronize()
