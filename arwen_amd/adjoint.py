@@ -82,7 +82,8 @@ def step_with_checkpoints(state: DomainState, cfg: RunConfig,
     if ckpt.shape != state.shape or ckpt.dtype != state.dtype:
         raise ValueError("checkpoint buffers do not match the state")
     d = state.domain_struct(diffusion=_uses_diffusion(cfg),
-                            precision_form=bool(cfg.precision_form))
+                            precision_form=bool(cfg.precision_form),
+                            diff6=cfg.diff_6th_opt > 0)
     name, fn = _fn(state, "step_ckpt")
     _lib.check(fn(d, params(cfg), ctypes.byref(ckpt.c), _stream()), name)
     state.elapsed_seconds += cfg.dt

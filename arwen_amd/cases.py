@@ -5,9 +5,16 @@ vertical coordinate, the base state and the initial prognostic fields.
 ``arwen_amd.state.DomainState.from_host`` uploads it to the GPU; the test
 oracle builds its own state from the same object.
 
+The reference's own initial-state API, :func:`init_at_rest` and
+:func:`init_theta_perturbation` (gpuwm/core/state.py:922-993), returns a
+device :class:`~arwen_amd.state.DomainState`, as the reference's does; it
+is built from the host-side :func:`host_at_rest` /
+:func:`host_theta_perturbation`, which the oracle and the workload builders
+below share.
+
 Builders:
 
-* :func:`init_theta_perturbation` -- at-rest state plus a hydrostatically
+* :func:`host_theta_perturbation` -- at-rest state plus a hydrostatically
   rebalanced theta perturbation (reference state.py:961-993);
 * :func:`straka` -- Straka et al. (1993) density current (reference
   verify/cases/straka.py:42-75), at the resolution and time step given;
@@ -66,19 +73,60 @@ def cell_y(cfg: RunConfig) -> np.ndarray:
     return (np.arange(cfg.ny) + 0.5) * cfg.dy - 0.5 * cfg.ny * cfg.dy
 
 
-def init_at_rest(cfg: RunConfig, coord: VerticalCoord,
-                 base: BaseState) -> HostCase:
+def _same_terrain(base: BaseState, terrain_z) -> None:
+    """An explicit ``terrain_z`` is a consistency check against the
+    terrain the base state was built over, never a second source of it
+    (reference state.py:900-919)."""
+    if terrain_z is None:
+        return
+    if base.terrain_z is None:
+        raise ValueError("terrain_z was passed for a base state built "
+                         "without terrain; pass the terrain to "
+                         "make_base_state instead")
+    if np.shape(terrain_z) != np.shape(base.terrain_z) or not np.array_equal(
+            np.asarray(terrain_z, np.float64),
+            np.asarray(base.terrain_z, np.float64)):
+        raise ValueError("terrain_z differs from base.terrain_z, the "
+                         "terrain the base state was built over")
+
+
+def host_at_rest(cfg: RunConfig, coord: VerticalCoord, base: BaseState,
+                 terrain_z=None) -> HostCase:
+    """Host-side at-rest state: zero perturbations on ``base``."""
+    _same_terrain(base, terrain_z)
     case = HostCase(cfg, coord, base)
     case.fields = case.zeros_like_state()
     return case
 
 
+def init_at_rest(cfg: RunConfig, coord: VerticalCoord, base: BaseState,
+                 terrain_z=None, *, device="cuda", dtype=None):
+    """Device state with zero perturbations on ``base`` (reference
+    state.py:922-932)."""
+    from .state import DTYPE, DomainState
+    return DomainState.from_host(host_at_rest(cfg, coord, base, terrain_z),
+                                 device=device, dtype=dtype or DTYPE)
+
+
 def init_theta_perturbation(cfg: RunConfig, coord: VerticalCoord,
-                            base: BaseState, thp_func) -> HostCase:
+                            base: BaseState, thp_func, terrain_z=None, *,
+                            device="cuda", dtype=None):
+    """Device state at rest plus theta' = thp_func(x, z), hydrostatically
+    rebalanced; winds zero, mu' = 0 (reference state.py:961-993).  ``x`` is
+    the domain-centred cell-centre coordinate (nx,), ``z`` the base
+    half-level heights, (nz,) flat or (nz, ny, nx) over terrain."""
+    from .state import DTYPE, DomainState
+    case = host_theta_perturbation(cfg, coord, base, thp_func, terrain_z)
+    return DomainState.from_host(case, device=device, dtype=dtype or DTYPE)
+
+
+def host_theta_perturbation(cfg: RunConfig, coord: VerticalCoord,
+                            base: BaseState, thp_func,
+                            terrain_z=None) -> HostCase:
     """At-rest state plus theta' = thp_func(x, z), with phi' chosen so
     every column is discretely hydrostatic (reference state.py:961-993);
     winds zero, mu' = 0."""
-    case = init_at_rest(cfg, coord, base)
+    case = host_at_rest(cfg, coord, base, terrain_z)
     z = height_half(base)
     thp = np.asarray(thp_func(cell_x(cfg), z), dtype=np.float64)
     thp = np.broadcast_to(thp, (cfg.nz, cfg.ny, cfg.nx)).copy()
@@ -132,7 +180,7 @@ def straka(nx=512, nz=64, dx=100.0, dt=1.0, time_step_sound=6,
         tp = np.where(L <= 1.0, -15.0 * (np.cos(np.pi * L) + 1.0) / 2.0, 0.0)
         return tp / exner[:, None, None]
 
-    return init_theta_perturbation(cfg, coord, base, thp)
+    return host_theta_perturbation(cfg, coord, base, thp)
 
 
 def smooth_noise(rng, shape, dx, dy, scale, sigma, dz_levels=None):
@@ -174,7 +222,7 @@ def stratified_hill(nx=250, ny=200, nz=49, dx=12000.0, dt=60.0,
                            ztop, terrain_z=tz)
     rng = np.random.default_rng(seed)
     th_noise = smooth_noise(rng, (nz, ny, nx), dx, dx, 200e3, 0.5, 3)
-    case = init_theta_perturbation(cfg, coord, base, lambda x, z: th_noise)
+    case = host_theta_perturbation(cfg, coord, base, lambda x, z: th_noise)
     u = u0 + smooth_noise(rng, (nz, ny, nx), dx, dx, 200e3, 1.0, 3)
     v = smooth_noise(rng, (nz, ny, nx), dx, dx, 200e3, 1.0, 3)
     case.fields["u"][..., :nx] = u
@@ -222,7 +270,7 @@ def warm_bubbles(nx=1000, ny=1000, nz=50, dx=1000.0, dt=6.0,
                 L < 1.0, amp * np.cos(0.5 * np.pi * L) ** 2, 0.0)
         return out
 
-    case = init_theta_perturbation(cfg, coord, base, thp)
+    case = host_theta_perturbation(cfg, coord, base, thp)
     case.fields["u"][...] = u0
     case.fields["v"][...] = v0
     return case
@@ -243,7 +291,7 @@ def ensemble_member(nx, ny, nz, seed, dx=3000.0, dt=18.0,
                            ztop)
     rng = np.random.default_rng(seed)
     th = smooth_noise(rng, (nz, ny, nx), dx, dx, scale, 0.5, 3)
-    case = init_theta_perturbation(cfg, coord, base, lambda x, z: th)
+    case = host_theta_perturbation(cfg, coord, base, lambda x, z: th)
     case.fields["u"][..., :nx] = u0 + smooth_noise(
         rng, (nz, ny, nx), dx, dx, scale, 1.0, 3)
     case.fields["v"][:, :ny, :] = smooth_noise(
@@ -283,7 +331,7 @@ def init_moist_balanced(cfg: RunConfig, coord: VerticalCoord,
     if base.terrain_z is not None:
         raise NotImplementedError("moist balance is built on flat bases only")
     nz, ny, nx = cfg.nz, cfg.ny, cfg.nx
-    case = init_at_rest(cfg, coord, base)
+    case = host_at_rest(cfg, coord, base)
     qv3 = np.broadcast_to(np.asarray(qv3, np.float64), (nz, ny, nx))
     thp = np.zeros((nz, ny, nx)) if thp is None else np.broadcast_to(
         np.asarray(thp, np.float64), (nz, ny, nx))

@@ -3,7 +3,7 @@ step reads (gpuwm/config.py:27-190), with identical field names and
 defaults, plus the validation the step performs before any device work.
 
 Fields for subsystems this build does not implement (physics schemes,
-nesting, open lateral boundaries, Smagorinsky, sixth-order filter) are
+nesting, open lateral boundaries, Smagorinsky) are
 accepted with their reference defaults so a reference config constructs
 unchanged; :func:`validate_for_step` refuses any non-default value with a
 ``NotImplementedError`` instead of silently ignoring it.
@@ -108,6 +108,13 @@ def validate_for_step(cfg: RunConfig, *, acoustic: bool = True) -> None:
         raise ValueError(f"damp_opt must be 0 or 3, got {cfg.damp_opt}")
     if cfg.damp_opt == 3 and cfg.zdamp <= 0.0:
         raise ValueError("damp_opt=3 needs zdamp > 0")
+    if not cfg.dampcoef >= 0.0:
+        raise ValueError(f"dampcoef = {cfg.dampcoef!r}: a negative damping "
+                         "rate would feed energy into the upper layer")
+    if not cfg.diff_6th_factor >= 0.0:
+        raise ValueError(f"diff_6th_factor = {cfg.diff_6th_factor!r}: a "
+                         "negative factor makes the 6th-order filter "
+                         "anti-diffusive")
     if cfg.h_sca_adv_order != 2:
         raise NotImplementedError(
             "only h_sca_adv_order=2 (the reference default) is ported")

@@ -1249,11 +1249,13 @@ namespace {
 // created by arw_profile_begin, so the timed step allocates nothing.
 enum KernelId {
     KID_OMEGA = 0, KID_TEND_FACES, KID_TEND_COLS, KID_UV, KID_COLUMN,
-    KID_FINISH, KID_SET_W_SURFACE, KID_SCALARS, KID_COUNT
+    KID_FINISH, KID_SET_W_SURFACE, KID_SCALARS, KID_DIFF6, KID_ADD_HELD,
+    KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {
     "k_omega", "k_tend_faces", "k_tend_cols", "k_uv", "k_column",
-    "k_finish", "k_set_w_surface", "moist_scalars"};
+    "k_finish", "k_set_w_surface", "moist_scalars", "k_diff6",
+    "k_add_held"};
 
 struct Profiler {
     bool active = false;
@@ -1709,7 +1711,7 @@ int32_t held_diff6(const Dom& d, const arw_params* prm, hipStream_t st)
                            dim3(64, 4), 0, st, d, r.f, r.out, r.c1, r.c2, coef,
                            mono, slope, tx, ty, r.nlev, r.nys, r.nxs, r.var,
                            r.wst);
-        int32_t rc = launched("k_diff6");
+        int32_t rc = launched(hipGetLastError(), "k_diff6", KID_DIFF6, st);
         if (rc) return rc;
     }
     return ARW_OK;
@@ -1837,6 +1839,11 @@ int32_t ARW_API(stage_prepare)(const Dom* d, const arw_params* prm,
     int32_t rc = check_domain(d);
     if (rc) return rc;
     if ((rc = check_params(prm, d))) return rc;
+    // the held 6th-order tendencies are formed and added by the whole step
+    // only; a piecewise caller would silently miss them
+    if (prm->diff_6th_opt != 0)
+        return fail(ARW_E_UNSUPPORTED,
+                    "stage_prepare: diff_6th_opt is applied by arw_step only");
     Sc s = make_sc(prm, dtau, 1);
     hipStream_t st = (hipStream_t)stream;
     if ((rc = stage_prepare(*d, s, do_eos, st))) return rc;
@@ -1986,7 +1993,9 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
         if (d6) {
             hipLaunchKernelGGL(k_add_held, union_grid(dd, dd.nz + 1),
                                dim3(64, 4), 0, st, dd);
-            if ((rc = launched("k_add_held"))) return rc;
+            if ((rc = launched(hipGetLastError(), "k_add_held", KID_ADD_HELD,
+                               st)))
+                return rc;
         }
         Dom ds = dd;
         for (int n = 0; n < nsub[stage]; ++n) {

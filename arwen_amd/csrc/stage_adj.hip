@@ -1106,6 +1106,10 @@ extern "C" int32_t ARW_API(stage_tendencies_adjoint)(
     int32_t rc = lib_check_domain(d);
     if (rc) return rc;
     if (!prm || !adj || !work) return lib_fail(ARW_E_ARG, "null argument");
+    if (prm->diff_6th_opt != 0)
+        return lib_fail(ARW_E_UNSUPPORTED,
+                        "stage adjoint: the held 6th-order tendencies are not "
+                        "differentiated");
     const real* lin[] = {adj->l_ru_t, adj->l_rv_t, adj->l_rw_t, adj->l_rth_t,
                          adj->l_rph_t};
     real* gout[] = {adj->g_u, adj->g_v, adj->g_w, adj->g_thp, adj->g_php,

@@ -132,6 +132,7 @@ class DomainState:
         else:
             self.ht.zero_()
         self._phb_host = np.array(base.phb, dtype=np.float64, copy=True)
+        self._coord, self._base = coord, base
         for name, val in surface_extrapolation(coord).items():
             setattr(self, name, float(self.np_dtype.type(val)))
 
@@ -150,6 +151,20 @@ class DomainState:
             getattr(s, name).copy_(torch.as_tensor(
                 np.asarray(arr, dtype=s.np_dtype)))
         return s
+
+    def host_case(self):
+        """The state's FP64 coordinate and base objects with its current
+        prognostic (and moist) fields downloaded, as an
+        ``arwen_amd.cases.HostCase``."""
+        from .cases import HostCase
+        if getattr(self, "_coord", None) is None:
+            raise RuntimeError("host_case() called before load_base()")
+        names = ["u", "v", "w", "thp", "php", "mup"]
+        if self.qv is not None:
+            names += ["qv", "qc", "qr"]
+        fields = {n: getattr(self, n).detach().cpu().numpy().astype(
+            np.float64) for n in names}
+        return HostCase(self.cfg, self._coord, self._base, fields)
 
     # -- reference helpers ----------------------------------------------------
 
@@ -259,3 +274,21 @@ class DomainState:
     def numpy_fields(self, names=("u", "v", "w", "thp", "php", "mup")):
         return {n: getattr(self, n).detach().cpu().numpy().copy()
                 for n in names}
+
+
+def init_at_rest(cfg: RunConfig, coord: VerticalCoord, base: BaseState,
+                 terrain_z=None, *, device="cuda",
+                 dtype=DTYPE) -> DomainState:
+    """Zero perturbations on ``base`` (reference state.py:922-932)."""
+    from .cases import init_at_rest as _init
+    return _init(cfg, coord, base, terrain_z, device=device, dtype=dtype)
+
+
+def init_theta_perturbation(cfg: RunConfig, coord: VerticalCoord,
+                            base: BaseState, thp_func, terrain_z=None, *,
+                            device="cuda", dtype=DTYPE) -> DomainState:
+    """At rest plus a hydrostatically rebalanced theta perturbation
+    (reference state.py:961-993)."""
+    from .cases import init_theta_perturbation as _init
+    return _init(cfg, coord, base, thp_func, terrain_z, device=device,
+                 dtype=dtype)

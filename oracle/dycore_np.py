@@ -922,18 +922,26 @@ def advance_w_phi(s, coefs, dtau, epssm, dx, dy, top_lid=True,
         hbot = htop - f(zdamp)
         for k in range(1, nz + 1):
             hk = phf[k] / g
-            arg = f(1.5707963) * (hk - hbot) / f(zdamp)
-            if np.iscomplexobj(arg):        # complex-step derivatives
-                sn = np.sin(arg)
-            else:
-                sn = np.sin(arg.astype(np.float64)).astype(s.dtype)
-            dw = f(dampmag) * sn * sn
+            dw = damper_weight(hk, hbot, zdamp, dampmag, s.dtype)
             new = (w[k] - dw * cfm[k] * s.w[k]) / (one + dw)
             w[k] = np.where(np.real(hk) >= np.real(hbot), new, w[k])
     s.w_pp[...] = w
     cfs = s.c1f[:, None, None] * muts + s.c2f[:, None, None]
     s.ph_pp[1:] = rhs[1:] + half * dtau * g * (one + eps) * w[1:] / cfs[1:]
     diagnose_p(s, c2a)
+
+
+def damper_weight(hk, hbot, zdamp, dampmag, dtype):
+    """damp_opt = 3 weight dtau*dampcoef*sin^2(pi/2 (z - z_bot)/zdamp) at
+    height ``hk`` (reference acoustic.cu:950-965: the kernel's 1.5707963f
+    literal; sin evaluated in double and rounded once)."""
+    f = np.dtype(dtype).type
+    arg = f(1.5707963) * (hk - hbot) / f(zdamp)
+    if np.iscomplexobj(arg):        # complex-step derivatives
+        sn = np.sin(arg)
+    else:
+        sn = np.sin(np.asarray(arg).astype(np.float64)).astype(dtype)
+    return f(dampmag) * sn * sn
 
 
 def diagnose_p(s, c2a):

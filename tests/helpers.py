@@ -44,7 +44,7 @@ def random_state_case(seed=0, nz=8, ny=6, nx=12, hybrid_opt=0, hill=0.0):
     """Small consistent state like the reference's acoustic test builder
     (gpuwm/verify/npref.py:2217-2324): isothermal 300 K base, seeded
     reference fields; returns (HostCase, rng) for further filling."""
-    from arwen_amd.cases import HostCase, init_at_rest
+    from arwen_amd.cases import HostCase, host_at_rest
     from arwen_amd.config import RunConfig
     from arwen_amd.grid import bell_hill, make_base_state, make_vertical_coord
     cfg = RunConfig(nx=nx, ny=ny, nz=nz, dx=500.0, dy=500.0, ztop=8000.0,
@@ -54,7 +54,7 @@ def random_state_case(seed=0, nz=8, ny=6, nx=12, hybrid_opt=0, hill=0.0):
     tz = bell_hill(nx, ny, 500.0, hill, 1500.0) if hill > 0 else None
     base = make_base_state(coord, lambda z: np.full_like(z, 300.0),
                            cfg.p_surf, cfg.ztop, terrain_z=tz)
-    case = init_at_rest(cfg, coord, base)
+    case = host_at_rest(cfg, coord, base)
     rng = np.random.default_rng(seed)
     f = case.fields
     f["u"] = 0.05 * rng.standard_normal(f["u"].shape)

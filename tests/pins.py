@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from arwen_amd.cases import HostCase, cell_x, height_half, init_theta_perturbation
+from arwen_amd.cases import HostCase, cell_x, height_half, host_theta_perturbation
 from arwen_amd.config import RunConfig
 from arwen_amd.grid import bell_hill, make_base_state, make_vertical_coord
 
@@ -51,7 +51,7 @@ def build_dry_flat() -> HostCase:
                     dt=2.0, run_seconds=0.0)
     vc = make_vertical_coord(cfg.nz)
     b = make_base_state(vc, _theta, cfg.p_surf, cfg.ztop)
-    case = init_theta_perturbation(cfg, vc, b, _bubble3(cfg))
+    case = host_theta_perturbation(cfg, vc, b, _bubble3(cfg))
     _shear_u(case)
     return case
 
@@ -66,7 +66,7 @@ def build_terrain() -> HostCase:
     tz = bell_hill(cfg.nx, cfg.ny, cfg.dx, cfg.hill_height,
                    cfg.hill_halfwidth)
     b = make_base_state(vc, _theta, cfg.p_surf, cfg.ztop, terrain_z=tz)
-    case = init_theta_perturbation(cfg, vc, b, _bubble3(cfg, zc=3000.0))
+    case = host_theta_perturbation(cfg, vc, b, _bubble3(cfg, zc=3000.0))
     _shear_u(case, u0=3.0)
     return case
 

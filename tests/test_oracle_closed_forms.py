@@ -59,14 +59,14 @@ def test_thomas_planted_index_mistake_is_caught():
 
 def _rest_case(nx=64, nz=32, n_bv=0.01):
     """The reference's rest setup (tests/test_dycore_full.py:74-82)."""
-    from arwen_amd.cases import init_at_rest, stratified_sounding
+    from arwen_amd.cases import host_at_rest, stratified_sounding
     from arwen_amd.config import RunConfig
     from arwen_amd.grid import make_base_state, make_vertical_coord
     cfg = RunConfig(nx=nx, ny=1, nz=nz, dx=1000.0, dy=1000.0, ztop=10000.0,
                     dt=6.0)
     vc = make_vertical_coord(nz)
     b = make_base_state(vc, stratified_sounding(n_bv), cfg.p_surf, cfg.ztop)
-    return init_at_rest(cfg, vc, b)
+    return host_at_rest(cfg, vc, b)
 
 
 def test_rest_state_stays_at_rest():
@@ -104,7 +104,7 @@ def _isothermal_pulse(nx=400, nz=16, dx=100.0, T=300.0, amp=1.0, x0=20000.0,
                       hw=500.0, dtype=np.float64):
     """Isothermal atmosphere at rest with a Gaussian p'' pulse planted in
     (mu theta)'' (reference verify/npref.py:2971-3016)."""
-    from arwen_amd.cases import init_at_rest
+    from arwen_amd.cases import host_at_rest
     from arwen_amd.config import RunConfig
     from arwen_amd.grid import make_base_state, make_vertical_coord
     from arwen_amd import constants as c
@@ -113,7 +113,7 @@ def _isothermal_pulse(nx=400, nz=16, dx=100.0, T=300.0, amp=1.0, x0=20000.0,
     b = make_base_state(vc, lambda z: T * np.exp(c.G * np.asarray(z, float)
                                                  / (c.CP * T)),
                         cfg.p_surf, cfg.ztop)
-    s = O.from_case(init_at_rest(cfg, vc, b), dtype)
+    s = O.from_case(host_at_rest(cfg, vc, b), dtype)
     O.calc_p_alpha(s)
     x = (np.arange(nx) + 0.5) * dx
     g = amp * np.exp(-((x - x0) / hw) ** 2)
