@@ -180,6 +180,7 @@ def load():
         "arw_abi_version": ([], _I),
         "arw_last_error": ([], ctypes.c_char_p),
         "arw_step": ([dp, pp, _P], _I),
+        "arw_step_noalloc": ([dp, pp, _P], _I),
         "arw_calc_p_alpha": ([dp, _P], _I),
         "arw_stage_fluxes": ([dp, pp, _P, _P, _P], _I),
         "arw_stage_prepare": ([dp, pp, _D, _I, _P], _I),
@@ -295,7 +296,7 @@ FP64_ENTRY_POINTS = (
     "arw64_acoustic_reference_gradients",
     "arw64_set_w_surface_adjoint", "arw64_scalar_stage",
     "arw64_scalar_stage_adjoint", "arw64_pd_stage",
-    "arw64_pd_stage_adjoint") + tuple(
+    "arw64_pd_stage_adjoint", "arw64_step_noalloc") + tuple(
         "arw64_" + n for n in FLUX_DIV_ADJOINTS)
 
 
@@ -327,7 +328,7 @@ def exported_symbols() -> list[str]:
             "arw_acoustic_reference_gradients",
             "arw_set_w_surface_adjoint", "arw_scalar_stage",
             "arw_scalar_stage_adjoint", "arw_pd_stage",
-            "arw_pd_stage_adjoint"] + [
+            "arw_pd_stage_adjoint", "arw_step_noalloc"] + [
             "arw_" + n for n in FLUX_DIV_ADJOINTS]
 
 

@@ -20,6 +20,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -2071,6 +2072,77 @@ int32_t ARW_API(step_ckpt)(const Dom* d, const arw_params* prm, Ckpt* ck,
 {
     if (!ck) return fail(ARW_E_ARG, "null checkpoint");
     return step_impl(d, prm, ck, stream);
+}
+
+// ---------------------------------------------------------------------------
+// The step with library-held scratch, for callers that cannot allocate.
+// Its four scratch arrays (Omega, the column solve's factors, the
+// reference mass divergence, the next stage's mu') live in one file-scope
+// device array of kStepScratchBytes per GPU, held until the process ends.
+// Every call on a GPU shares that GPU's array: calls that overlap on one
+// GPU race.  Each array is written before it is read within a step, so
+// nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+constexpr size_t kStepScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kStepScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static char* g_scratch_dev[kMaxDevices];          // per device, set once
+static std::mutex g_scratch_mu;
+
+// This device's address of `scratch` (the host-side name is only a
+// stand-in for the symbol).
+static int32_t step_scratch(char** out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return fail(ARW_E_LAUNCH, "step_noalloc: no current device");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            snprintf(g_err, sizeof(g_err),
+                     "step_noalloc: hipGetSymbolAddress(scratch): %s",
+                     hipGetErrorString(e));
+            return ARW_E_LAUNCH;
+        }
+        g_scratch_dev[dev] = static_cast<char*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return ARW_OK;
+}
+
+int32_t ARW_API(step_noalloc)(const Dom* d, const arw_params* prm,
+                              void* stream)
+{
+    if (!d) return fail(ARW_E_ARG, "null domain");
+    if (!prm) return fail(ARW_E_ARG, "null params");
+    if (d->qv || (!ARW_F64 && prm->precision_form) || prm->diff_6th_opt)
+        return fail(ARW_E_UNSUPPORTED,
+                    "step_noalloc: dry plain-form steps without the "
+                    "6th-order filter only");
+    if (d->nz < 3 || d->ny < 1 || d->nx < 1)
+        return fail(ARW_E_ARG, "need nz >= 3, ny >= 1, nx >= 1");
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t col = (size_t)d->ny * d->nx * sizeof(real);
+    const size_t lev = (size_t)(d->nz + 1) * col;
+    const size_t need = 2 * al(lev) + 2 * al(col);
+    if (need > kStepScratchBytes) {
+        snprintf(g_err, sizeof(g_err),
+                 "step_noalloc: this domain needs %zu bytes of step scratch; "
+                 "the library holds %zu per GPU",
+                 need, kStepScratchBytes);
+        return ARW_E_ARG;
+    }
+    char* base = nullptr;
+    if (int32_t rc = step_scratch(&base)) return rc;
+    Dom dd = *d;
+    dd.ww = reinterpret_cast<real*>(base);
+    dd.gam = reinterpret_cast<real*>(base + al(lev));
+    dd.dmdt_rf = reinterpret_cast<real*>(base + 2 * al(lev));
+    dd.mup_next = reinterpret_cast<real*>(base + 2 * al(lev) + al(col));
+    return step_impl(&dd, prm, nullptr, stream);
 }
 
 static int32_t check_tracer(const Dom* d, const arw_params* prm,

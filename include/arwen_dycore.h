@@ -164,6 +164,19 @@ const char *arw_last_error(void);
  * like the reference's closing update_diagnostics (dycore.py:2482). */
 int32_t arw_step(const arw_domain *d, const arw_params *prm, void *stream);
 
+/* arw_step with no workspace and no allocation at all.  No counterpart
+ * in the reference: it is for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The step's scratch (d->ww, d->gam, d->dmdt_rf,
+ * d->mup_next are ignored and may be null) comes from one array of
+ * 64 MiB per GPU that the library holds until the process ends; a domain
+ * needing more (2*(nz+1)*ny*nx + 2*ny*nx values, each part rounded up to
+ * 256 bytes) returns ARW_E_ARG.  Dry plain-form steps without the
+ * 6th-order filter (ARW_E_UNSUPPORTED otherwise).  Calls that overlap on
+ * one GPU share the array and race.  Bit-identical to arw_step. */
+int32_t arw_step_noalloc(const arw_domain *d, const arw_params *prm,
+                         void *stream);
+
 /*
  * Stage checkpoints of one step's scalar transport: per RK stage the
  * substep-averaged mass fluxes (WRF sumflux: ru_m (nz,ny,nx+1), rv_m
@@ -560,6 +573,11 @@ int32_t arw64_abi_version(void);
 const char *arw64_last_error(void);
 int32_t arw64_step(const arw64_domain *d, const arw_params *prm,
                    void *stream);
+/* arw_step_noalloc of the FP64 build: no counterpart in the reference,
+ * for callers that cannot allocate; its own 64 MiB array per GPU (half
+ * the points of the FP32 build's limit). */
+int32_t arw64_step_noalloc(const arw64_domain *d, const arw_params *prm,
+                           void *stream);
 int32_t arw64_calc_p_alpha(const arw64_domain *d, void *stream);
 int32_t arw64_stage_fluxes(const arw64_domain *d, const arw_params *prm,
                            double *ru, double *rv, void *stream);
