@@ -4,7 +4,8 @@
   dt = 60 s): the first step bit-identical to the float32 oracle, then an
   hour of steps with bounded winds and dry mass conserved;
 - C3 (1000x1000x50 warm bubbles): the first step bit-identical to the
-  float32 oracle at full size;
+  float32 oracle at full size, in the plain form and in the precision
+  form;
 - a long run of the small dry pin case: 50 steps still bit-identical to
   the oracle (the drift between the device step and the restatement of
   the reference step stays exactly zero).
@@ -66,5 +67,20 @@ def test_fifty_steps_stay_bitwise():
     for n in range(50):
         O.step(o, case.cfg)
         dycore.step(g, case.cfg)
+    torch.cuda.synchronize()
+    assert not _bitwise(o, g)
+
+
+def test_c3_precision_form_first_step_bitwise_full_size():
+    """The precision form (compensated column sums, FP64 vertical
+    tridiagonal, perturbation-form EOS) at the C3 size: the first step
+    bit-identical to the oracle's restatement of the same form."""
+    from arwen_amd import dycore
+    case = cases.warm_bubbles()
+    cfg = case.cfg.with_(precision_form=True)
+    print("C3 1000x1000x50 precision form full size", flush=True)
+    o, g = oracle_and_gpu(case)
+    O.step(o, cfg)
+    dycore.step(g, cfg)
     torch.cuda.synchronize()
     assert not _bitwise(o, g)

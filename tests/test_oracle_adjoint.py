@@ -414,7 +414,7 @@ def test_acoustic_substep_reference_adjoint_vs_complex_step(hill, damp,
         jv = _complex_ref_jvp(o, dtau, cfg, first, dx)
         lhs = sum(float(np.vdot(jv[n], y[n])) for n in y)
         rhs = sum(float(np.vdot(dx[n], g[n])) for n in which)
-        assert abs(lhs - rhs) <= 1e-9 * max(abs(lhs), 1e-30), \
+        assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1e-30), \
             (which, lhs, rhs)
 
 
@@ -460,7 +460,7 @@ def test_whole_step_adjoint_vs_complex_step(name):
         lhs = sum(float(np.sum(np.imag(getattr(c, n)) / h * lam[n]))
                   for n in A.STAGE_IN)
         rhs = sum(float(np.sum(dx[n] * g[n])) for n in which)
-        assert abs(lhs - rhs) <= 1e-9 * max(abs(lhs), 1e-30), \
+        assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1e-30), \
             (which, lhs, rhs)
 
 
@@ -515,7 +515,7 @@ def test_moist_step_adjoint_vs_complex_step(name, pd):
         lhs = sum(float(np.sum(np.imag(getattr(c, n)) / h * lam[n]))
                   for n in names)
         rhs = sum(float(np.sum(dx[n] * g[n])) for n in which)
-        assert abs(lhs - rhs) <= 1e-9 * max(abs(lhs), 1e-30), \
+        assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1e-30), \
             (which, lhs, rhs)
 
 
@@ -566,7 +566,7 @@ def test_step_adjoint_with_diffusion_vs_complex_step(moist):
         lhs = sum(float(np.sum(np.imag(getattr(c, n)) / h * lam[n]))
                   for n in names)
         rhs = sum(float(np.sum(dx[n] * g[n])) for n in which)
-        assert abs(lhs - rhs) <= 1e-9 * max(abs(lhs), 1e-30), \
+        assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1e-30), \
             (which, lhs, rhs)
 
 
@@ -671,7 +671,7 @@ def test_pd_update_adjoint_vs_complex_step(seed):
             D.flux5, D.flux3 = saved
         lhs = float(np.sum(np.imag(qc) / h * lam))
         rhs = sum(float(np.sum(dx[n] * g[n])) for n in which)
-        assert abs(lhs - rhs) <= 1e-9 * max(abs(lhs), 1e-30), \
+        assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1e-30), \
             (which, lhs, rhs)
 
 
