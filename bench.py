@@ -58,6 +58,12 @@ def parse():
     ap.add_argument("--diff6", type=int, default=0, choices=(0, 1, 2),
                     help="c3/c2: run with WRF's held 6th-order filter "
                          "(diff_6th_opt; factor 0.12) to time its cost")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="c3/c2: leave out the compact adjoint (C4) and "
+                         "ensemble (C5) objects measured after the timed "
+                         "region")
+    ap.add_argument("--extra-ens-steps", type=int, default=20,
+                    help="steps of the compact C5 ensemble comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=160,
                     help="y rows of the CPU-baseline strip")
@@ -99,29 +105,216 @@ def read_traffic(path, kernel):
     return None
 
 
+def host_cpu() -> str:
+    """The host's CPU model and the cores this process may use."""
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count()
+    return f"{model}, {ncpu} cores visible"
+
+
 def cpu_baseline(workload: str, rows: int, steps: int, dtype_name="float32"):
     """NumPy oracle (float32, or float64) on a (nz, rows, nx) strip of the
-    workload."""
+    workload: one untimed warm-up step, then ``steps`` timed steps.  A
+    strip, not the full grid: a full-size step of the oracle takes about
+    half a minute (float32) to a minute (float64) per step, and the
+    baseline is bounded to a few tens of seconds so that the default run
+    stays within minutes.  Every column of the strip is a full-height
+    column of the workload, and NumPy's ufuncs are single-threaded, so the
+    rate per point is the full grid's."""
     import numpy as np
     from oracle import dycore_np as O
     case, _ = build_case(workload, ny_override=rows)
     s = O.from_case(case, np.dtype(dtype_name).type)
     cfg = case.cfg
+    O.step(s, cfg)                              # warm-up
     t0 = time.perf_counter()
     for _ in range(steps):
         O.step(s, cfg)
     dt = time.perf_counter() - t0
     pts = cfg.nx * cfg.ny * cfg.nz
-    try:
-        ncpu = len(os.sched_getaffinity(0))
-    except AttributeError:
-        ncpu = os.cpu_count()
     return {"value": pts * steps / dt / 1e6, "unit": "Mpoint-steps/s",
             "cores": 1, "kind": "port",
-            "sample": f"{dtype_name} NumPy oracle, {steps} steps on a "
-                      f"{cfg.nx}x{cfg.ny}x{cfg.nz} strip of the same "
-                      f"workload ({dt:.1f} s, single-threaded ufuncs; "
-                      f"{ncpu} host cores visible)"}
+            "sample": f"{dtype_name} NumPy oracle, 1 warm-up + {steps} "
+                      f"timed steps on a {cfg.nx}x{cfg.ny}x{cfg.nz} strip "
+                      f"of the same workload ({dt:.1f} s timed, "
+                      f"single-threaded ufuncs; host: {host_cpu()})"}
+
+
+def extras_adjoint(reps: int = 5) -> dict:
+    """Compact C4 adjoint record: the 192x160x50 moist-tracer case with the
+    final stage's PD limiter (the reference's default moist_adv_opt = 1),
+    FP32 and FP64.  The fused step, the taped step and the reverse sweep
+    in both tape modes (every substep kept, or stage checkpoints only with
+    each stage recomputed in the reverse sweep); the FP64 dot-product
+    residual of the scalar transport (exactly linear); and the FP64
+    central-difference agreement of the whole step's adjoint (the step is
+    nonlinear, so this is bounded by the differencing error; the exact
+    whole-step check is against complex-step JVPs of the oracle, in the
+    tests)."""
+    import torch
+    from arwen_amd import adjoint, cases, dycore
+    from arwen_amd.state import DomainState
+
+    case = cases.moist_tracer()
+    cfg = case.cfg
+    names = adjoint.STAGE_STATE + adjoint.SPECIES
+    out = {"workload": "C4 moist tracer 192x160x50 + qv/qc/qr, dx=3 km, "
+                       "dt=18 s, 4 substeps, PD limiter on the final stage",
+           "ratio": "reverse sweep ms / fused forward step ms"}
+    for tag, dtype in (("fp32", torch.float32), ("fp64", torch.float64)):
+        st = DomainState.from_host(case, dtype=dtype)
+        for _ in range(3):                  # leave the state at rest
+            dycore.step(st, cfg)
+        saved = {a: getattr(st, a).clone() for a in names}
+
+        def restore():
+            for a, t in saved.items():
+                getattr(st, a).copy_(t)
+            dycore.update_diagnostics(st)
+
+        gen = torch.Generator(device=st.device).manual_seed(11)
+        lam = {n: torch.randn(getattr(st, n).shape, dtype=dtype,
+                              device=st.device, generator=gen) for n in names}
+        t_rest = _time_ms(restore, reps)
+
+        def fused():
+            restore()
+            dycore.step(st, cfg)
+
+        t_f = _time_ms(fused, reps) - t_rest
+        rec = {"forward_step_ms": round(t_f, 4)}
+        for keep in ("stages", "substeps"):
+            holder = {}
+
+            def taped():
+                restore()
+                holder["tape"] = adjoint.step_taped(st, cfg, keep=keep)
+
+            t_t = _time_ms(taped, reps) - t_rest
+            t_r = _time_ms(lambda: adjoint.step_adjoint_taped(
+                st, holder["tape"], lam), reps)
+            holder.clear()
+            rec[keep] = {"taped_step_ms": round(t_t, 4),
+                         "reverse_ms": round(t_r, 4),
+                         "reverse_to_forward": round(t_r / t_f, 3),
+                         "taped_plus_reverse_to_forward":
+                             round((t_t + t_r) / t_f, 3)}
+        if dtype == torch.float64:
+            ck = adjoint.TracerCheckpoints(st)
+            restore()
+            adjoint.step_with_checkpoints(st, cfg, ck)
+            x = torch.randn(st.qv.shape, dtype=dtype, device=st.device,
+                            generator=gen)
+            y = torch.randn(st.qv.shape, dtype=dtype, device=st.device,
+                            generator=gen)
+            lhs = float(torch.sum(adjoint.tracer_step(x, st, cfg, ck) * y))
+            rhs = float(torch.sum(x * adjoint.tracer_adjoint(y, st, cfg, ck)))
+            rec["transport_dot_product_rel_residual"] = abs(lhs - rhs) / abs(lhs)
+            restore()
+            gm = adjoint.step_adjoint(st, cfg, lam, keep="stages")
+            scl = {"u": 1.0, "v": 1.0, "w": 0.1, "thp": 0.5, "php": 20.0,
+                   "mup": 20.0, "qv": 1e-4, "qc": 1e-5, "qr": 1e-5}
+            vv = {n: torch.randn(getattr(st, n).shape, dtype=dtype,
+                                 device=st.device, generator=gen) * scl[n]
+                  for n in names}
+            for n in adjoint.SPECIES:
+                vv[n] = vv[n] * (saved[n] > 0)
+            h = 1e-6
+
+            def moved(hh):
+                restore()
+                for n in names:
+                    getattr(st, n).add_(hh * vv[n])
+                dycore.update_diagnostics(st)
+                dycore.step(st, cfg)
+                return {n: getattr(st, n).clone() for n in names}
+
+            fp, fm = moved(h), moved(-h)
+            jv = sum(float(torch.sum((fp[n] - fm[n]) / (2 * h) * lam[n]))
+                     for n in names)
+            vjt = sum(float(torch.sum(vv[n] * gm[n])) for n in names)
+            rec["step_fd_rel_err_h1e-6"] = abs(jv - vjt) / abs(jv)
+            del fp, fm, gm, ck
+        out[tag] = rec
+        del st, saved
+        torch.cuda.empty_cache()
+    return out
+
+
+def extras_ensemble(steps: int) -> dict:
+    """Compact C5 record: the 32 uneven members (nx, ny ~ U{96..256}, nz ~
+    U{40..60}, dx = 3 km, dt = 18 s) advanced ``steps`` steps in one
+    captured graph (4 member groups rejoining every 5 steps, column tile
+    256: the autotuner's choice on this ensemble) against the same members
+    stepped one after another (eagerly, the reference's launch model);
+    bit-identity of the two; and the worst member's error against the FP64
+    build of the same discretisation after the same steps."""
+    import torch
+    from arwen_amd import cases, dycore
+    from arwen_amd import ensemble as E
+    from arwen_amd.state import DomainState
+
+    shapes = cases.ensemble_shapes()
+    st32, st64 = [], []
+    for i, (nx, ny, nz) in enumerate(shapes):
+        h = cases.ensemble_member(nx, ny, nz, 20260731 + 1 + i)
+        st32.append(DomainState.from_host(h))
+        st64.append(DomainState.from_host(h, dtype=torch.float64))
+        del h
+    cfg = st32[0].cfg
+    pts = sum(a * b * c for a, b, c in shapes)
+    print(f"extras: C5 {len(st32)} members built", file=sys.stderr,
+          flush=True)
+    init = E.save_state(st32)
+    with E.Ensemble(st32, cfg, groups=4, join_every=5, col_tile=256) as ens:
+        ens.capture(steps)
+        ens.launch()                        # first replay: one-off costs
+        torch.cuda.synchronize()
+        E.restore_state(st32, init)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ens.launch()
+        torch.cuda.synchronize()
+        t_b = time.perf_counter() - t0
+    batched = E.save_state(st32)
+    E.restore_state(st32, init)
+    for st in st32:                         # warm each member's launch path
+        dycore.step(st, cfg)
+    E.restore_state(st32, init)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E.sequential_steps(st32, cfg, steps)
+    torch.cuda.synchronize()
+    t_s = time.perf_counter() - t0
+    same = all(torch.equal(getattr(st, a), sv[0][a])
+               for st, sv in zip(st32, batched) for a in E.STATE_ARRAYS)
+    del batched, init
+    with E.Ensemble(st64, cfg, groups=1) as e64:
+        e64.step(steps)
+    torch.cuda.synchronize()
+    worst = {}
+    for a, b in zip(st32, st64):
+        for f in ("u", "v", "w", "thp", "php", "mup", "p"):
+            x, y = getattr(a, f).double(), getattr(b, f)
+            e = float((x - y).abs().max()) / float(y.abs().max())
+            worst[f] = max(worst.get(f, 0.0), e)
+    return {"workload": f"C5 32 members, {pts} points, {steps} steps",
+            "batched_graph_ms_per_step": round(t_b * 1e3 / steps, 3),
+            "sequential_ms_per_step": round(t_s * 1e3 / steps, 3),
+            "speedup": round(t_s / t_b, 3),
+            "bitwise_equal_to_sequential": same,
+            "max_rel_err_vs_fp64_worst_member": worst}
 
 
 def main():
@@ -227,6 +420,13 @@ def main():
         "kernel_ms_per_step": {k: round(v / args.steps, 4)
                                for k, v in timer.ms.items() if v > 0},
     }
+    del state
+    torch.cuda.empty_cache()
+    if not args.no_extras:
+        out["adjoint"] = extras_adjoint()
+        torch.cuda.empty_cache()
+        out["ensemble"] = extras_ensemble(args.extra_ens_steps)
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_rows,
                                            args.cpu_steps)
