@@ -49,6 +49,11 @@ struct Sc {
     real dtau, epssm, smdiv;
     real dampmag, zdamp;
     int top_lid, diff, damp, first, use_pold, prec;
+    // k_column: the stage's Thomas factors gamma are already stored (they
+    // depend on the stage reference only, so later substeps of a stage
+    // skip storing them); park = levels at the top of each column whose
+    // gamma and eliminated w'' stay in LDS between the two sweeps
+    int gam_ready, park;
 };
 
 static inline Sc make_sc(const arw_params* p, double dtau, int first)
@@ -76,6 +81,8 @@ static inline Sc make_sc(const arw_params* p, double dtau, int first)
     s.top_lid = p->top_lid != 0;
     s.diff = (p->khdif > 0.0 || p->kvdif > 0.0);
     s.prec = (!ARW_F64) && p->precision_form != 0;
+    s.gam_ready = 0;
+    s.park = 0;
     return s;
 }
 

@@ -1469,6 +1469,7 @@ int32_t ens_issue(const EnsGroup& G, const arw_params* prm, bool b3,
         for (int n = 0; n < nsub[stage]; ++n) {
             Sc sn = make_sc(prm, dtaus[stage], n == 0);
             sn.use_pold = (n > 0);
+            sn.gam_ready = (n > 0);
             const bool last = (n + 1 == nsub[stage]);
             const BatchSet& DS = G.sets[3 * stage + 1 + (n & 1)];
             if ((rc = launched(run_uv_b(DS, sn, b3, st), "k_uv (ensemble)", -1, st)))
@@ -2002,6 +2003,7 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
         for (int n = 0; n < nsub[stage]; ++n) {
             Sc sn = make_sc(prm, dtaus[stage], n == 0);
             sn.use_pold = (n > 0);
+            sn.gam_ready = (n > 0);             // stored by the first
             const bool last = (n + 1 == nsub[stage]);
             // new p'' goes to the buffer holding the dead p''(n-1); the two
             // names swap afterwards

@@ -1224,10 +1224,15 @@ struct ColDn {
 // B3: terrain (3-D) base state; DAMP: damp_opt = 3 w damper; AL: store
 // alpha''; WW: store Omega'' (standalone substep API).  nz >= 3.
 // Batched (Many): the new p'' goes to each member's p_pp_old.
+// Rows r >= nz + 1 - s.park keep gamma and the eliminated w'' in LDS
+// (2 s.park values per column, dynamic shared memory) instead of parking
+// them in HBM between the sweeps; rows below go to d.gam / d.w_pp, gamma
+// only when the stage has not stored it yet (s.gam_ready = 0).
 template <class SRC, bool B3, bool DAMP, bool AL, bool WW, bool PREC>
 __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
                                                 real* RS pout_arg)
 {
+    extern __shared__ real lpark[];
     int m, g;
     if (!SRC::locate(a, m, g)) return;
     const Dom& d = SRC::dom(a, m);
@@ -1288,6 +1293,10 @@ __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
     real cof = R(0.5) * dtau * G_F * one_p;
     cof = cof * cof;
     auto cfm_at = [&](int kk) { return c1f[kk] * mut + c2f[kk]; };
+    // first row parked in LDS; slot (r - r0) holds gamma at 2 (r - r0),
+    // the eliminated w'' at 2 (r - r0) + 1, each a row of blockDim.x
+    const int r0 = PREC ? nz + 2 : nz + 1 - s.park;
+    const int lt = (int)blockDim.x, lx = (int)threadIdx.x;
     kreal* thb1 = kview(d.thb);     // 1-D base profiles (flat base)
     kreal* phb1 = kview(d.phb);
     auto thbv = [&](real loaded, int kk) { return B3 ? loaded : thb1[kk]; };
@@ -1397,8 +1406,13 @@ __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
                  / ((tri)chmc * (tri)cfm_at(r + 1));
         alpha_r = (tri)R(1.0) / (b - a_r * gam_prev);
         gam_prev = cc * alpha_r;
-        if (PREC) d.tri64[r * st + c] = (double)gam_prev;
-        else gst(d.gam, r * st, bc, (real)gam_prev);
+        if (PREC) {
+            if (!s.gam_ready) d.tri64[r * st + c] = (double)gam_prev;
+        } else if (r >= r0) {
+            lpark[(2 * (r - r0)) * lt + lx] = (real)gam_prev;
+        } else if (!s.gam_ready) {
+            gst(d.gam, r * st, bc, (real)gam_prev);
+        }
         return a_r;
     };
 
@@ -1469,6 +1483,7 @@ __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
                                  alt_a, alt_b, chm_a, chm_b);
             tri wf = ((tri)wr - a_r * wf_prev) * alpha_r;
             if (PREC) d.tri64[stt + r * st + c] = (double)wf;
+            else if (r >= r0) lpark[(2 * (r - r0) + 1) * lt + lx] = (real)wf;
             else gst(d.w_pp, r * st, bc, (real)wf);
             wf_prev = wf;
         }
@@ -1507,6 +1522,7 @@ __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
                              alt_b, chm_a, chm_b);
         tri wfr = ((tri)wr - a_r * wf_prev) * alpha_r;
         if (PREC) d.tri64[stt + rr * st + c] = (double)wfr;
+        else if (rr >= r0) lpark[(2 * (rr - r0) + 1) * lt + lx] = (real)wfr;
         else gst(d.w_pp, rr * st, bc, (real)wfr);
         tri a_top = (tri)R(0.0);
         real wtop;
@@ -1529,6 +1545,7 @@ __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
         tri alpha_top = (tri)R(1.0) / (b_top - a_top * gam_prev);
         const tri wtf = ((tri)wtop - a_top * wfr) * alpha_top;
         if (PREC) d.tri64[stt + f + c] = (double)wtf;
+        else if (nz >= r0) lpark[(2 * (nz - r0) + 1) * lt + lx] = (real)wtf;
         else gst(d.w_pp, f, bc, (real)wtf);
     }
 
@@ -1545,6 +1562,9 @@ __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
         if (PREC) {
             L.wpp64 = d.tri64[stt + ks + c];
             L.gam64 = d.tri64[ks + c];
+        } else if (k >= r0) {
+            L.wpp = lpark[(2 * (k - r0) + 1) * lt + lx];
+            L.gam = k < nz ? lpark[(2 * (k - r0)) * lt + lx] : R(0.0);
         } else {
             L.wpp = gld(d.w_pp, ks, bc);
             L.gam = gld(d.gam, ks, bc);
@@ -1958,16 +1978,33 @@ hipError_t run_uv(const Dom& d, const Sc& s, hipStream_t st)
     return hipGetLastError();
 }
 
+// Levels per column whose Thomas factors and eliminated w'' k_column keeps
+// in LDS: 128 bytes per column (FP32), which leaves its VGPR-bound
+// occupancy unchanged; the precision form parks its FP64 pair in HBM.
+static inline Sc column_sc(const Sc& s)
+{
+    Sc c = s;
+    c.park = s.prec ? 0 : (int)(128 / (2 * sizeof(real)));
+    return c;
+}
+
+static inline size_t park_bytes(const Sc& c, int cols_per_block)
+{
+    return (size_t)2 * c.park * cols_per_block * sizeof(real);
+}
+
 template <bool B3, bool DAMP, bool AL, bool WW>
-static void launch_column(const Dom& d, const Sc& s, real* pout,
+static void launch_column(const Dom& d, const Sc& s0, real* pout,
                           hipStream_t st)
 {
+    const Sc s = column_sc(s0);
+    const size_t lds = park_bytes(s, 256);
     if (s.prec)
         hipLaunchKernelGGL((k_column<One, B3, DAMP, AL, WW, true>), cols(d),
-                           dim3(256), 0, st, d, s, pout);
+                           dim3(256), lds, st, d, s, pout);
     else
         hipLaunchKernelGGL((k_column<One, B3, DAMP, AL, WW, false>), cols(d),
-                           dim3(256), 0, st, d, s, pout);
+                           dim3(256), lds, st, d, s, pout);
 }
 
 template <bool B3, bool DAMP>
@@ -2086,14 +2123,16 @@ hipError_t run_uv_b(const BatchSet& b, const Sc& s, bool b3, hipStream_t st)
 }
 
 template <bool B3, bool DAMP, bool AL>
-static void launch_column_b(const Batch& r, const Sc& s, hipStream_t st)
+static void launch_column_b(const Batch& r, const Sc& s0, hipStream_t st)
 {
+    const Sc s = column_sc(s0);
+    const size_t lds = park_bytes(s, r.tile);
     if (s.prec)
         hipLaunchKernelGGL((k_column<Many, B3, DAMP, AL, false, true>),
-                           bgrid(r), dim3(r.tile), 0, st, r, s, nullptr);
+                           bgrid(r), dim3(r.tile), lds, st, r, s, nullptr);
     else
         hipLaunchKernelGGL((k_column<Many, B3, DAMP, AL, false, false>),
-                           bgrid(r), dim3(r.tile), 0, st, r, s, nullptr);
+                           bgrid(r), dim3(r.tile), lds, st, r, s, nullptr);
 }
 
 template <bool B3, bool DAMP>
