@@ -157,6 +157,18 @@ class ArwAcousticFwd(ctypes.Structure):
 ArwAcousticFwd64 = ArwAcousticFwd
 
 
+#: flags of arw_health_field
+HEALTH_LOWER, HEALTH_UPPER, HEALTH_STRICT_LOWER = 1, 2, 4
+HEALTH_ADD_AUX, HEALTH_AUX_LEVEL = 8, 16
+
+
+class ArwHealthField(ctypes.Structure):
+    """arw_health_field: one descriptor of the full-state health gate."""
+    _fields_ = [("values", _P), ("aux", _P), ("size", ctypes.c_uint64),
+                ("lower", _D), ("upper", _D), ("flags", ctypes.c_uint32),
+                ("plane", ctypes.c_uint32), ("status_bit", ctypes.c_uint64)]
+
+
 class ArwenError(RuntimeError):
     """A library entry point returned a nonzero status."""
 
@@ -181,6 +193,8 @@ def load():
         "arw_last_error": ([], ctypes.c_char_p),
         "arw_step": ([dp, pp, _P], _I),
         "arw_step_noalloc": ([dp, pp, _P], _I),
+        "arw_health_gate": ([_P, _I, ctypes.c_uint64, ctypes.c_uint32, _P,
+                             _P], _I),
         "arw_calc_p_alpha": ([dp, _P], _I),
         "arw_stage_fluxes": ([dp, pp, _P, _P, _P], _I),
         "arw_stage_prepare": ([dp, pp, _D, _I, _P], _I),
@@ -296,7 +310,8 @@ FP64_ENTRY_POINTS = (
     "arw64_acoustic_reference_gradients",
     "arw64_set_w_surface_adjoint", "arw64_scalar_stage",
     "arw64_scalar_stage_adjoint", "arw64_pd_stage",
-    "arw64_pd_stage_adjoint", "arw64_step_noalloc") + tuple(
+    "arw64_pd_stage_adjoint", "arw64_step_noalloc",
+    "arw64_health_gate") + tuple(
         "arw64_" + n for n in FLUX_DIV_ADJOINTS)
 
 
@@ -328,7 +343,8 @@ def exported_symbols() -> list[str]:
             "arw_acoustic_reference_gradients",
             "arw_set_w_surface_adjoint", "arw_scalar_stage",
             "arw_scalar_stage_adjoint", "arw_pd_stage",
-            "arw_pd_stage_adjoint", "arw_step_noalloc"] + [
+            "arw_pd_stage_adjoint", "arw_step_noalloc",
+            "arw_health_gate"] + [
             "arw_" + n for n in FLUX_DIV_ADJOINTS]
 
 

@@ -551,6 +551,39 @@ int32_t arw_set_w_surface(const arw_domain *d, const arw_params *prm,
  * geopotential.  NaNs propagate.  out is a device array of 4 floats. */
 int32_t arw_health(const arw_domain *d, float *out, void *stream);
 
+/*
+ * Full-state health gate (replaces the reference's StateHealthValidator,
+ * gpuwm/core/health.py:611-770, and its kernel validate_full_state,
+ * kernels/health.cu:224-297).  One descriptor per checked field, in
+ * DEVICE memory: the field (an array of the build's precision, float for
+ * arw_, double for arw64_), an optional auxiliary added before the check
+ * (same shape, or with ARW_HEALTH_AUX_LEVEL one value per `plane`
+ * elements), its bounds and the status bit its failure sets.  A value
+ * fails when it is not finite, below `lower` (at or below with
+ * ARW_HEALTH_STRICT_LOWER) or above `upper`.  result (device, two
+ * uint64): the OR of the failing fields' status bits, and the smallest
+ * (field number << 48 | flat index) of a failing value (all ones when
+ * none fails).  Launch: field x chunk pairs, nchunks chunks of `chunk`
+ * elements per field.
+ */
+#define ARW_HEALTH_LOWER 1u
+#define ARW_HEALTH_UPPER 2u
+#define ARW_HEALTH_STRICT_LOWER 4u
+#define ARW_HEALTH_ADD_AUX 8u
+#define ARW_HEALTH_AUX_LEVEL 16u
+typedef struct arw_health_field {
+    const void *values;
+    const void *aux;
+    uint64_t size;
+    double lower, upper;
+    uint32_t flags;
+    uint32_t plane;
+    uint64_t status_bit;
+} arw_health_field;
+int32_t arw_health_gate(const arw_health_field *fields, int32_t nfields,
+                        uint64_t chunk, uint32_t nchunks, uint64_t *result,
+                        void *stream);
+
 /* FP64 domain dry-mass sum(mub2d + mup) (replaces dycore.py:209-218
  * domain_mass_measure, identity map factors).  out is one device
  * double. */
@@ -578,6 +611,9 @@ int32_t arw64_step(const arw64_domain *d, const arw_params *prm,
  * the points of the FP32 build's limit). */
 int32_t arw64_step_noalloc(const arw64_domain *d, const arw_params *prm,
                            void *stream);
+int32_t arw64_health_gate(const arw_health_field *fields, int32_t nfields,
+                          uint64_t chunk, uint32_t nchunks, uint64_t *result,
+                          void *stream);
 int32_t arw64_calc_p_alpha(const arw64_domain *d, void *stream);
 int32_t arw64_stage_fluxes(const arw64_domain *d, const arw_params *prm,
                            double *ru, double *rv, void *stream);

@@ -118,7 +118,8 @@ def _c_layout(struct, fields, tmp_path):
     ("arw_acoustic_ref", _lib.ArwAcousticRef),
     ("arw64_acoustic_ref", _lib.ArwAcousticRef64),
     ("arw_acoustic_fwd", _lib.ArwAcousticFwd),
-    ("arw64_acoustic_fwd", _lib.ArwAcousticFwd64)])
+    ("arw64_acoustic_fwd", _lib.ArwAcousticFwd64),
+    ("arw_health_field", _lib.ArwHealthField)])
 def test_ctypes_offsets_match_the_c_compiler(cname, pyt, tmp_path):
     names = [n for n, _ in pyt._fields_]
     got = _c_layout(cname, names, tmp_path)
