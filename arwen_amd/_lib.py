@@ -12,7 +12,9 @@ from pathlib import Path
 LIB_NAME = "libarwen_dycore.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
-ABI_VERSION = 6
+ABI_VERSION = 7
+#: transported species beyond qv/qc/qr a domain can carry (ARW_MAX_EXTRA)
+MAX_EXTRA = 13
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -48,7 +50,10 @@ class ArwDomain(ctypes.Structure):
                 + [(n, _P) for n in MOIST_ARRAYS]
                 + [(n, _P) for n in MOIST_SCRATCH]
                 + [("tri64", _P), ("pp", _P)]
-                + [(n, _P) for n in DIFF6_ARRAYS])
+                + [(n, _P) for n in DIFF6_ARRAYS]
+                + [("n_extra", _I), ("n_load", _I)]
+                + [(n, _P * MAX_EXTRA) for n in ("qx", "qx0", "qxs")]
+                + [("h_diabatic", _P)])
 
 
 class ArwDomain64(ctypes.Structure):
@@ -193,6 +198,7 @@ def load():
         "arw_last_error": ([], ctypes.c_char_p),
         "arw_step": ([dp, pp, _P], _I),
         "arw_step_noalloc": ([dp, pp, _P], _I),
+        "arw_step_advective": ([dp, pp, _P], _I),
         "arw_health_gate": ([_P, _I, ctypes.c_uint64, ctypes.c_uint32, _P,
                              _P], _I),
         "arw_calc_p_alpha": ([dp, _P], _I),
@@ -311,7 +317,7 @@ FP64_ENTRY_POINTS = (
     "arw64_set_w_surface_adjoint", "arw64_scalar_stage",
     "arw64_scalar_stage_adjoint", "arw64_pd_stage",
     "arw64_pd_stage_adjoint", "arw64_step_noalloc",
-    "arw64_health_gate") + tuple(
+    "arw64_health_gate", "arw64_step_advective") + tuple(
         "arw64_" + n for n in FLUX_DIV_ADJOINTS)
 
 
@@ -344,7 +350,7 @@ def exported_symbols() -> list[str]:
             "arw_set_w_surface_adjoint", "arw_scalar_stage",
             "arw_scalar_stage_adjoint", "arw_pd_stage",
             "arw_pd_stage_adjoint", "arw_step_noalloc",
-            "arw_health_gate"] + [
+            "arw_health_gate", "arw_step_advective"] + [
             "arw_" + n for n in FLUX_DIV_ADJOINTS]
 
 

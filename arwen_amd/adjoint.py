@@ -663,9 +663,25 @@ def _check_adjoint_cfg(state, cfg):
     if getattr(cfg, "diff_6th_opt", 0):
         raise _lib.ArwenError("step adjoint: no 6th-order filter "
                               "(diff_6th_opt = 0)")
+    if getattr(cfg, "mp_physics", 0) != 0:
+        raise _lib.ArwenError("step adjoint: qv/qc/qr states only "
+                              "(mp_physics = 0, no h_diabatic)")
     if _uses_diffusion(cfg) and state.thb.dim() == 3:
         raise _lib.ArwenError("step adjoint: constant-K diffusion needs a "
                               "flat base")
+
+
+def _snapshot(tensors: dict) -> dict:
+    """Copies of the named tensors, made by one batched copy launch."""
+    out = {n: torch.empty_like(t) for n, t in tensors.items()}
+    torch._foreach_copy_(list(out.values()), list(tensors.values()))
+    return out
+
+
+def _restore(state, names, src: dict, suffix: str = "") -> None:
+    """state.<name><suffix> = src[name] for every name, one batched copy."""
+    torch._foreach_copy_([getattr(state, n + suffix) for n in names],
+                         [src[n] for n in names])
 
 
 def _run_stage(state, cfg, nsub, dtau, record):
@@ -681,24 +697,29 @@ def _run_stage(state, cfg, nsub, dtau, record):
     flx = None
     if _moist(state):
         ru, rv, ww = dyn.stage_fluxes(state, cfg)
-        flx = {"ru": ru.clone(), "rv": rv.clone(), "ww": ww.clone(),
-               "su": torch.zeros_like(state.u), "sv": torch.zeros_like(state.v),
-               "sw": torch.zeros_like(state.w)}
+        flx = _snapshot({"ru": ru, "rv": rv, "ww": ww})
     if record:
-        prep = {n: getattr(state, n).clone() for n in _PREP}
-        prep["ww"] = state.scratch((nz + 1, ny, nx), "arw_ww").clone()
-        prep["dmdt_rf"] = state.scratch((ny, nx), "arw_dmdt_rf").clone()
-        subs = [{n: getattr(state, n).clone() for n in ACOUSTIC_STATE}]
+        src = {n: getattr(state, n) for n in _PREP}
+        src["ww"] = state.scratch((nz + 1, ny, nx), "arw_ww")
+        src["dmdt_rf"] = state.scratch((ny, nx), "arw_dmdt_rf")
+        src.update({"sub0_" + n: getattr(state, n) for n in ACOUSTIC_STATE})
+        snap = _snapshot(src)
+        subs = [{n: snap.pop("sub0_" + n) for n in ACOUSTIC_STATE}]
+        prep = snap
     for i in range(nsub):
         dyn.acoustic_substep(state, cfg, dtau, first=(i == 0))
         if flx is not None:
-            flx["su"] += state.u_pp
-            flx["sv"] += state.v_pp
-            flx["sw"] += state.ww_pp
+            # the substep sums of u'', v'', Omega'' (the first one copies)
+            pp = [state.u_pp, state.v_pp, state.ww_pp]
+            if i == 0:                   # 0 + x, as the sums start at zero
+                flx.update(zip(("su", "sv", "sw"),
+                               torch._foreach_add(pp, 0.0)))
+            else:
+                torch._foreach_add_([flx["su"], flx["sv"], flx["sw"]], pp)
         if record:
-            x = {n: getattr(state, n).clone() for n in ACOUSTIC_STATE}
-            x["ww_pp"] = state.ww_pp.clone()
-            subs.append(x)
+            src = {n: getattr(state, n) for n in ACOUSTIC_STATE}
+            src["ww_pp"] = state.ww_pp
+            subs.append(_snapshot(src))
     return prep, subs, flx
 
 
@@ -796,10 +817,9 @@ def step_taped(state: DomainState, cfg: RunConfig,
         raise ValueError("keep must be 'substeps' or 'stages'")
     tape = StepTape(cfg, keep)
     names = _step_fields(state)
-    for n in names:
-        getattr(state, n + "0").copy_(getattr(state, n))
+    _restore(state, names, {n: getattr(state, n) for n in names}, "0")
     for istage, (nsub, dtau) in enumerate(_step_stages(cfg)):
-        xs = {n: getattr(state, n).clone() for n in names}
+        xs = _snapshot({n: getattr(state, n) for n in names})
         prep, subs, flx = _run_stage(state, cfg, nsub, dtau,
                                      keep == "substeps")
         dyn.small_step_finish(state)
@@ -807,9 +827,10 @@ def step_taped(state: DomainState, cfg: RunConfig,
         if flx is not None:
             # sumflux, then the scalars' stage update from the stage's
             # scalars and the time-t ones
-            fl = (flx["su"] / float(nsub) + flx["ru"],
-                  flx["sv"] / float(nsub) + flx["rv"],
-                  flx["sw"] / float(nsub) + flx["ww"])
+            fl = tuple(torch._foreach_add(
+                torch._foreach_div([flx["su"], flx["sv"], flx["sw"]],
+                                   float(nsub)),
+                [flx["ru"], flx["rv"], flx["ww"]]))
             ex = {"fl": fl, "mu_new": state.mup.clone(),
                   "dt": nsub * dtau, "final": istage == 2}
             for q in SPECIES:
@@ -827,7 +848,7 @@ def step_taped(state: DomainState, cfg: RunConfig,
     dyn.set_w_surface(state, cfg)
     dyn.update_diagnostics(state)
     state.elapsed_seconds += cfg.dt
-    tape.out = {n: getattr(state, n).clone() for n in names}
+    tape.out = _snapshot({n: getattr(state, n) for n in names})
     return tape
 
 
@@ -862,12 +883,10 @@ def step_adjoint_taped(state: DomainState, tape: StepTape,
     set_w_surface_adjoint(state, cfg, lx["w"][0], lx["u"], lx["v"])
     lx["w"][0].zero_()
     x0 = tape.stages[0][0]                     # the step's input
-    for n in names:
-        getattr(state, n + "0").copy_(x0[n])
+    _restore(state, names, x0, "0")
     g0 = {n: torch.zeros_like(getattr(state, n)) for n in names}
     for xs, prep, subs, nsub, dtau, ex in reversed(tape.stages):
-        for n in names:
-            getattr(state, n).copy_(xs[n])
+        _restore(state, names, xs)
         if prep is None:                 # recompute the stage's tape
             prep, subs, _ = _run_stage(state, cfg, nsub, dtau, True)
         gq, inj = {}, None
@@ -929,9 +948,8 @@ def step_adjoint_taped(state: DomainState, tape: StepTape,
         stage_tendencies_adjoint(state, cfg, lt, out=gs, accumulate=True,
                                  lam_flux=lam_flux if moist else None)
         lx = gs
-    for n in names:
-        g0[n] += lx[n]
-        getattr(state, n).copy_(tape.out[n])
+    torch._foreach_add_([g0[n] for n in names], [lx[n] for n in names])
+    _restore(state, names, tape.out)
     return g0
 
 

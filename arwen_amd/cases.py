@@ -392,6 +392,52 @@ def moist_tracer(nx=192, ny=160, nz=50, dx=3000.0, dt=18.0,
     return case
 
 
+def add_species(case: HostCase, mp_physics: int, seed: int = 20261203,
+                heating: float = 2.0e-3, qv_max: float = 0.0) -> HostCase:
+    """Give ``case`` a mixed-phase scheme's species and a retained heating
+    (in place; returns the case).  Mass species are drawn in [0, 1e-3]
+    kg/kg, number moments in [0, 1e6] /kg (NSSL's CCN qnn about its
+    408163264 /kg start), particle volumes in [0, 1e-9] m3/kg; a third of
+    the cells of every species hold zero, so the final stage's
+    positive-definite limiter acts.  h_diabatic is a seeded field in
+    [-heating, heating] K/s.  A dry case becomes moist with qv drawn in
+    [0, qv_max] (its balance is the dry one; for throughput runs)."""
+    from .config import EXTRA_SPECIES
+    cfg = case.cfg
+    shape = (cfg.nz, cfg.ny, cfg.nx)
+    rng = np.random.default_rng(seed + mp_physics)
+    if "qv" not in case.fields:
+        case.fields["qv"] = rng.uniform(0.0, qv_max, shape)
+    case.cfg = cfg.with_(moist=True, mp_physics=mp_physics)
+    for name in ("qc", "qr") + EXTRA_SPECIES[mp_physics]:
+        if name == "qnn":
+            x = 408163264.0 * rng.uniform(0.5, 1.5, shape)
+        elif name.startswith("qvol"):
+            x = rng.uniform(0.0, 1.0e-9, shape)
+        elif name.startswith("n") or name.startswith("qn"):
+            x = rng.uniform(0.0, 1.0e6, shape)
+        else:
+            x = rng.uniform(0.0, 1.0e-3, shape)
+        x[rng.uniform(size=shape) < 1.0 / 3.0] = 0.0
+        case.fields[name] = x
+    if mp_physics != 0:
+        case.fields["h_diabatic"] = rng.uniform(-heating, heating, shape)
+    return case
+
+
+def mixed_phase(mp_physics=10, nx=64, ny=48, nz=30, dx=3000.0, dt=18.0,
+                time_step_sound=4, seed=20261203, ztop=20000.0,
+                moist_adv_opt=1, heating=2.0e-3) -> HostCase:
+    """:func:`moist_tracer` carrying a mixed-phase scheme's species and a
+    retained microphysics heating (:func:`add_species`)."""
+    case = moist_tracer(nx=nx, ny=ny, nz=nz, dx=dx, dt=dt,
+                        time_step_sound=time_step_sound, seed=seed, ztop=ztop,
+                        moist_adv_opt=moist_adv_opt)
+    case = add_species(case, mp_physics, seed=seed, heating=heating)
+    case.cfg = case.cfg.with_(case="mixed_phase")
+    return case
+
+
 def set_w_surface_host(case: HostCase) -> None:
     """w(sfc) = u . grad(ht) on the host (reference dycore.py:1999-2026)."""
     cfg, f = case.cfg, case.fields

@@ -2,8 +2,9 @@
 step reads (gpuwm/config.py:27-190), with identical field names and
 defaults, plus the validation the step performs before any device work.
 
-Fields for subsystems this build does not implement (physics schemes,
-nesting, open lateral boundaries, Smagorinsky) are
+Fields for subsystems this build does not implement (physics schemes --
+whose species the step still transports --, nesting, open lateral
+boundaries, Smagorinsky) are
 accepted with their reference defaults so a reference config constructs
 unchanged; :func:`validate_for_step` refuses any non-default value with a
 ``NotImplementedError`` instead of silently ignoring it.
@@ -81,6 +82,42 @@ def config_fields() -> tuple[str, ...]:
     return tuple(f.name for f in fields(RunConfig))
 
 
+# ---------------------------------------------------------------------------
+# Transported scalar species of a moist state (reference moist.py:107-240,
+# allocation state.py:428-470).  The step advects every species of the
+# state with the acoustic time-averaged fluxes (positive definite on the
+# final stage); the microphysics scheme itself runs outside the step, in
+# the caller's physics driver.  The condensate masses listed first in a
+# scheme's extras also load the buoyancy (dycore.cu:213-230, q_total).
+# ---------------------------------------------------------------------------
+
+#: the qv/qc/qr base set of every moist state
+BASE_SPECIES = ("qv", "qc", "qr")
+_ICE_MASS = ("qi", "qs", "qg")
+#: transported species beyond qv/qc/qr per mp_physics
+EXTRA_SPECIES = {
+    0: (), 1: (),
+    6: _ICE_MASS,                                         # WSM6
+    8: _ICE_MASS + ("nr", "ni"),                          # Thompson
+    10: _ICE_MASS + ("nr", "ni", "ns", "ng"),             # Morrison
+    18: ("qi", "qs", "qg", "qh", "qndrop", "qnr", "qni", "qns", "qng",
+         "qnh", "qnn", "qvolg", "qvolh"),                 # NSSL
+    28: _ICE_MASS + ("nr", "ni", "nc", "nwfa", "nifa"),   # Thompson aerosol
+}
+#: how many of a scheme's extra species (the first ones) load the
+#: buoyancy with qv, qc, qr
+LOADING_EXTRA = {0: 0, 1: 0, 6: 3, 8: 3, 10: 3, 18: 4, 28: 3}
+#: the largest extra set any scheme carries (the C ABI's table size)
+MAX_EXTRA_SPECIES = 13
+
+
+def moist_species(cfg: RunConfig) -> tuple[str, ...]:
+    """The transported species of a moist state under cfg.mp_physics."""
+    if not cfg.moist:
+        return ()
+    return BASE_SPECIES + EXTRA_SPECIES[cfg.mp_physics]
+
+
 def validate_for_step(cfg: RunConfig, *, acoustic: bool = True) -> None:
     """Fail-closed checks made before any device work.
 
@@ -133,9 +170,18 @@ def validate_for_step(cfg: RunConfig, *, acoustic: bool = True) -> None:
     if cfg.diff_6th_opt and (cfg.moist or cfg.precision_form):
         raise NotImplementedError(
             "diff_6th_opt is ported for dry states in the plain form")
-    if cfg.mp_physics != 0 or cfg.bl_pbl_physics or cfg.ra_physics \
-            or cfg.cu_physics or cfg.sf_sfclay_physics \
-            or cfg.sf_surface_physics:
+    if cfg.mp_physics not in EXTRA_SPECIES:
+        raise NotImplementedError(
+            f"mp_physics={cfg.mp_physics}: the step transports the species "
+            f"of mp_physics {sorted(EXTRA_SPECIES)}")
+    if cfg.mp_physics != 0 and not cfg.moist:
+        raise ValueError("mp_physics != 0 needs moist=True")
+    if cfg.mp_physics != 0 and cfg.precision_form:
+        raise NotImplementedError(
+            "the precision form is ported for mp_physics = 0 (no retained "
+            "microphysics heating, no mixed-phase species)")
+    if cfg.bl_pbl_physics or cfg.ra_physics or cfg.cu_physics \
+            or cfg.sf_sfclay_physics or cfg.sf_surface_physics:
         raise NotImplementedError(
             "physics schemes are outside the dycore step port")
     if cfg.khdif < 0.0 or cfg.kvdif < 0.0:

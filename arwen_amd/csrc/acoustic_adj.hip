@@ -1228,7 +1228,7 @@ static int32_t substep_adjoint(const Dom* d, const arw_params* prm,
                                const real* lu, int acc, real* work,
                                void* stream)
 {
-    int32_t rc = lib_check_domain(d);
+    int32_t rc = lib_check_adj_domain(d);
     if (rc) return rc;
     if (!prm || !adj || !work) return lib_fail(ARW_E_ARG, "null argument");
     if ((rc = lib_check_prec(prm, d))) return rc;
@@ -1357,7 +1357,7 @@ int32_t ARW_API(acoustic_substep_adjoint_ref)(
 int32_t ARW_API(acoustic_adjoint_factors)(const Dom* d, const arw_params* prm,
                                           double dtau, real* lu, void* stream)
 {
-    int32_t rc = lib_check_domain(d);
+    int32_t rc = lib_check_adj_domain(d);
     if (rc) return rc;
     if (!prm || !lu) return lib_fail(ARW_E_ARG, "null argument");
     if ((rc = lib_check_prec(prm, d))) return rc;
@@ -1373,7 +1373,7 @@ int32_t ARW_API(acoustic_reference_gradients)(
     real* g_u, real* g_v, real* g_w, real* g_thp, real* g_php, real* g_mup,
     real* g_qv, int32_t flags, real* work, void* stream)
 {
-    int32_t rc = lib_check_domain(d);
+    int32_t rc = lib_check_adj_domain(d);
     if (rc) return rc;
     if (!prm || !ref_ok(acc) || !g_u || !g_v || !g_w || !g_thp || !g_php
         || !g_mup || !work)
@@ -1409,7 +1409,7 @@ int32_t ARW_API(set_w_surface_adjoint)(const Dom* d, const arw_params* prm,
                                        const real* lam_w0, real* g_u,
                                        real* g_v, void* stream)
 {
-    int32_t rc = lib_check_domain(d);
+    int32_t rc = lib_check_adj_domain(d);
     if (rc) return rc;
     if (!prm || !lam_w0 || !g_u || !g_v) return lib_fail(ARW_E_ARG, "null argument");
     if (d->nz < 3) return lib_fail(ARW_E_ARG, "set_w_surface adjoint needs nz >= 3");

@@ -54,6 +54,10 @@ struct Sc {
     // skip storing them); park = levels at the top of each column whose
     // gamma and eliminated w'' stay in LDS between the two sweeps
     int gam_ready, park;
+    // WRF sumflux fused into the substep pass: 0 off, 1 the stage's first
+    // substep (the sums start from zero), 2 later substeps (accumulate);
+    // k_uv adds u'', v'' into ru_m, rv_m and k_column Omega'' into ww_m
+    int sum;
 };
 
 static inline Sc make_sc(const arw_params* p, double dtau, int first)
@@ -83,6 +87,7 @@ static inline Sc make_sc(const arw_params* p, double dtau, int first)
     s.prec = (!ARW_F64) && p->precision_form != 0;
     s.gam_ready = 0;
     s.park = 0;
+    s.sum = 0;
     return s;
 }
 

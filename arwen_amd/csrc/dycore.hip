@@ -372,33 +372,12 @@ hipError_t run_faces(const Dom& d, real* ru, real* rv, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
-// Moist scalar transport (qv/qc/qr), periodic, msf = 1
+// Moist scalar transport (every species), periodic, msf = 1
 // ---------------------------------------------------------------------------
 
-// WRF sumflux (reference acoustic.cu:1437-1467): after every substep the
-// perturbation fluxes u'', v'', Omega'' are summed; the first substep
-// starts from zero.  One thread per (k, j, i) of the union of staggers.
-__global__ void k_sumflux(Dom d, int first)
-{
-    const int nz = d.nz, ny = d.ny, nx = d.nx;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
-    if (i > nx || j > ny || k > nz) return;
-    if (j < ny && k < nz) {
-        const int x = iU(k, j, i, ny, nx);
-        d.ru_m[x] = (first ? R(0.0) : d.ru_m[x]) + d.u_pp[x];
-    }
-    if (i < nx && k < nz) {
-        const int x = iV(k, j, i, ny, nx);
-        d.rv_m[x] = (first ? R(0.0) : d.rv_m[x]) + d.v_pp[x];
-    }
-    if (i < nx && j < ny) {
-        const int x = iM(k, j, i, ny, nx);
-        d.ww_m[x] = (first ? R(0.0) : d.ww_m[x]) + d.ww_pp[x];
-    }
-}
-
+// WRF sumflux (reference acoustic.cu:1437-1467), the per-substep sums of
+// u'', v'', Omega'', runs inside the substep pass (k_uv, k_column with
+// Sc::sum set).
 // WRF sumflux at the last substep (reference acoustic.cu:1469-1489): the
 // substep mean plus the stage-reference coupled fluxes ru, rv (from the
 // stage state d.u/d.v/d.mup) and Omega (d.ww).
@@ -449,27 +428,6 @@ __device__ __forceinline__ real scalar_update(const Dom& d, int k, int c,
     v = v / chm;
     if (clamp && !(v > R(0.0)) && v == v) v = R(0.0);
     return v;
-}
-
-// Unlimited stage update of one scalar: tend = div of qcur with the
-// averaged fluxes ru_m, rv_m, ww_m; qdst = update from q0.
-__global__ void k_scalar_update(Dom d, const real* __restrict__ qcur,
-                                const real* __restrict__ q0,
-                                real* __restrict__ qdst,
-                                const real* __restrict__ mup0, real dt,
-                                int clamp, real rdx, real rdy)
-{
-    const int nz = d.nz, ny = d.ny, nx = d.nx;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
-    if (i >= nx || j >= ny || k >= nz) return;
-    Arr3 Q{qcur, ny, nx}, RU{d.ru_m, ny, nx + 1}, RV{d.rv_m, ny + 1, nx},
-        RW{d.ww_m, ny, nx};
-    const real tend = R(0.0) + adv_scalar(Q, RU, RV, RW, k, j, i, nz, ny, nx,
-                                         d.rdnw, d.fnm, d.fnp, rdx, rdy);
-    const int x = iM(k, j, i, ny, nx);
-    qdst[x] = scalar_update(d, k, j * nx + i, q0[x], tend, mup0, dt, clamp);
 }
 
 // WRF flux_upwind (reference pd_advection.cu:101-108).
@@ -1299,6 +1257,16 @@ int32_t check_domain(const Dom* d)
                          d->mup_next};
     for (const void* p : req)
         if (!p) return fail(ARW_E_ARG, "null array in arw_domain");
+    if (d->n_extra < 0 || d->n_extra > ARW_MAX_EXTRA || d->n_load < 0
+        || d->n_load > 4 || d->n_load > d->n_extra)
+        return fail(ARW_E_ARG, "need 0 <= n_load <= min(4, n_extra), "
+                               "n_extra <= ARW_MAX_EXTRA");
+    if ((d->n_extra || d->h_diabatic) && !d->qv)
+        return fail(ARW_E_ARG, "extra species and h_diabatic need a moist "
+                               "domain");
+    for (int e = 0; e < d->n_extra; ++e)
+        if (!d->qx[e] || !d->qx0[e] || !d->qxs[e])
+            return fail(ARW_E_ARG, "null extra species array");
     if (d->qv) {
         const void* moist[] = {d->qc, d->qr, d->qv0, d->qc0, d->qr0, d->qsv,
                                d->qsc, d->qsr, d->ru_m, d->rv_m, d->ww_m,
@@ -1314,6 +1282,9 @@ int32_t check_domain(const Dom* d)
 int32_t check_prec(const arw_params* p, const Dom* d)
 {
     if (ARW_F64 || !p || !p->precision_form) return ARW_OK;
+    if (d->h_diabatic)
+        return fail(ARW_E_UNSUPPORTED, "the precision form carries no "
+                                       "h_diabatic removal");
     if (!d->tri64)
         return fail(ARW_E_ARG, "precision form needs the FP64 tridiagonal scratch tri64");
     if (!d->pp)
@@ -1377,33 +1348,47 @@ int32_t substep(const Dom& d, const Sc& s, real* pout, int write_al,
                     KID_COLUMN, st);
 }
 
-// qv/qc/qr one RK stage (reference moist.py:434-698): the unlimited flux
-// form, or the positive-definite limiter on the final stage when
+// Every scalar's RK stage (reference moist.py:434-698): the unlimited
+// flux form, or the positive-definite limiter on the final stage when
 // moist_adv_opt = 1.  cur: stage-state scalars; q0: time t; dst: new.
+// The limited update reads q0's neighbours while it runs, so it goes to
+// spare (a dead buffer of the stage rotation) and is copied to dst.
 int32_t scalar_stage(const Dom& dd, const arw_params* prm, real* const* cur,
                      real* const* q0, real* const* dst, const real* mup0,
-                     double dt_stage, bool final, hipStream_t st)
+                     double dt_stage, bool final, hipStream_t st, int nq,
+                     real* const* spare)
 {
     const real dt = (real)dt_stage;
     const real rdx = (real)(1.0 / prm->dx), rdy = (real)(1.0 / prm->dy);
     const bool pd = final && prm->moist_adv_opt == 1;
-    const dim3 cells((dd.nx + 63) / 64, (dd.ny + 3) / 4, dd.nz);
-    for (int q = 0; q < 3; ++q) {
-        if (pd) {
-            hipLaunchKernelGGL(k_pd_fluxes, union_grid(dd, dd.nz + 1),
-                               dim3(64, 4), 0, st, dd, cur[q], q0[q],
-                               (real)prm->dx, (real)prm->dy, dt);
-            hipLaunchKernelGGL(k_pd_scale, cells, dim3(64, 4), 0, st, dd,
-                               q0[q], mup0, rdx, rdy, dt);
-            hipLaunchKernelGGL(k_pd_apply, cells, dim3(64, 4), 0, st, dd,
-                               q0[q], dst[q], mup0, rdx, rdy, dt);
-        } else {
-            hipLaunchKernelGGL(k_scalar_update, cells, dim3(64, 4), 0, st, dd,
-                               cur[q], q0[q], dst[q], mup0, dt, final ? 1 : 0,
-                               rdx, rdy);
+    int32_t rc;
+    if (pd) {
+        const size_t bytes = (size_t)dd.nz * dd.ny * dd.nx * sizeof(real);
+        for (int q = 0; q < nq; ++q) {
+            if ((rc = launched(run_pd_cols(dd, cur[q], q0[q], spare[q], mup0,
+                                           (real)prm->dx, (real)prm->dy, rdx,
+                                           rdy, dt, st),
+                               "k_pd_cols", KID_SCALARS, st)))
+                return rc;
+            if (hipMemcpyAsync(dst[q], spare[q], bytes,
+                               hipMemcpyDeviceToDevice, st) != hipSuccess)
+                return fail(ARW_E_LAUNCH, "limited scalar copy failed");
         }
-        int32_t rc = launched(hipGetLastError(), "scalar stage", KID_SCALARS, st);
-        if (rc) return rc;
+        return ARW_OK;
+    }
+    for (int q = 0; q < nq; q += 4) {
+        QGroup g;
+        g.n = nq - q < 4 ? nq - q : 4;
+        for (int e = 0; e < 4; ++e) {
+            const int k = e < g.n ? q + e : q;
+            g.cur[e] = cur[k];
+            g.q0[e] = q0[k];
+            g.dst[e] = dst[k];
+        }
+        if ((rc = launched(run_scalar_cols(dd, g, mup0, dt, rdx, rdy,
+                                           final ? 1 : 0, st),
+                           "k_scalar_cols", KID_SCALARS, st)))
+            return rc;
     }
     return ARW_OK;
 }
@@ -1495,6 +1480,17 @@ int32_t lib_launched(hipError_t e, const char* what)
     return launched(e, what, -1, nullptr);
 }
 int32_t lib_check_domain(const Dom* d) { return check_domain(d); }
+// The adjoint kernels differentiate the qv/qc/qr step without retained
+// microphysics heating.
+int32_t lib_check_adj_domain(const Dom* d)
+{
+    int32_t rc = check_domain(d);
+    if (rc) return rc;
+    if (d->n_extra || d->h_diabatic)
+        return fail(ARW_E_UNSUPPORTED, "the adjoint covers qv/qc/qr states "
+                                       "without h_diabatic");
+    return ARW_OK;
+}
 int32_t lib_check_prec(const arw_params* p, const Dom* d)
 {
     return check_prec(p, d);
@@ -1732,6 +1728,160 @@ struct ARW_API(ensemble) {
     hipGraphExec_t exec = nullptr;
 };
 
+// ---------------------------------------------------------------------------
+// The advection-only step (reference dycore.py:2318-2343, step(acoustic=
+// False)): per stage of length dt/3, dt/2, dt the EOS, the coupled flux-
+// form advection of theta, u, v, w with Omega = 0 (advection.py:134-182),
+// the constant-K diffusion, and the uncoupled update from the time-t
+// copies with w, phi' and mu' frozen (dycore.py:1602-1643).  Dry,
+// periodic, msf = 1; the caller guarantees w == 0.  A test path of the
+// reference: thread-per-point kernels over the union of the staggers.
+// ---------------------------------------------------------------------------
+
+// Stage preparation: tendencies zeroed, the total theta into th_pp, the
+// coupled fluxes (c1h <mu> + c2h) u, v into u_pp, v_pp, Omega = 0 in ww_pp.
+__global__ void k_adv_prep(Dom d)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    if (i < nx && j < ny && k < nz) {
+        const int x = iM(k, j, i, ny, nx), c = j * nx + i;
+        d.th_pp[x] = basev(d.thb, k, c, st, d.base3d) + d.thp[x];
+        d.rth_t[x] = R(0.0);
+    }
+    if (j < ny && k < nz) {
+        const int x = iU(k, j, i, ny, nx);
+        d.u_pp[x] = ru_at(d, k, j, i);
+        d.ru_t[x] = R(0.0);
+    }
+    if (i < nx && k < nz) {
+        const int x = iV(k, j, i, ny, nx);
+        d.v_pp[x] = rv_at(d, k, j, i);
+        d.rv_t[x] = R(0.0);
+    }
+    if (i < nx && j < ny) {
+        const int x = iM(k, j, i, ny, nx);
+        d.ww_pp[x] = R(0.0);
+        d.rw_t[x] = R(0.0);
+        d.rph_t[x] = R(0.0);
+        if (k == 0) d.rmu_t[j * nx + i] = R(0.0);
+    }
+}
+
+// Constant-K Laplacian of field f at (k, j, i) of a (nlev, ny', nx') array
+// with row pitch px: kh (f_xx + f_yy) + kv (f_z)_z (reference
+// diffusion.cu:22-51), the same float32 expressions as the stage kernels'
+// fused diffusion.  Vertical spacings rdzf/rdzc of the field's levels.
+__device__ __forceinline__ real lap2(const real* __restrict__ f, int k, int j, int i,
+                                     int nlev, int ny, int nx, int px,
+                                     int plane, const Sc& s,
+                                     const real* __restrict__ rdzf,
+                                     const real* __restrict__ rdzc)
+{
+    const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j + 1 == ny) ? 0 : j + 1;
+    const int im = (i == 0) ? nx - 1 : i - 1, ip = (i + 1 == nx) ? 0 : i + 1;
+    const int r = k * plane;
+    const real fc = f[r + j * px + i];
+    real lap = s.kh * (s.dxi2 * ((f[r + j * px + ip] - R(2.0) * fc)
+                                 + f[r + j * px + im])
+                       + s.dyi2 * ((f[r + jp * px + i] - R(2.0) * fc)
+                                   + f[r + jm * px + i]));
+    const real fup = (k < nlev - 1)
+        ? (f[r + plane + j * px + i] - fc) * rdzf[k] : R(0.0);
+    const real fdn = (k > 0)
+        ? (fc - f[r - plane + j * px + i]) * rdzf[k - 1] : R(0.0);
+    return lap + (s.kv * (fup - fdn)) * rdzc[k];
+}
+
+// The diffusion tendencies coupled with C(mu) at each staggering (mass
+// at the face for u, v), added to ru_t, rv_t, rw_t, rth_t.
+__global__ void k_adv_diff(Dom d, Sc s)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    const real c1 = k < nz ? d.c1h[k] : R(0.0), c2 = k < nz ? d.c2h[k] : R(0.0);
+    if (j < ny && k < nz) {              // u faces, the duplicate included
+        const int ic = (i >= nx) ? i - nx : i, ib = (ic == 0) ? nx - 1 : ic - 1;
+        const real mux = R(0.5) * (mut_at(d, j * nx + ic) + mut_at(d, j * nx + ib));
+        const real lap = lap2(d.u, k, j, ic, nz, ny, nx, nx + 1,
+                              ny * (nx + 1), s, d.rdzf_h, d.rdzc_h);
+        const int x = iU(k, j, i, ny, nx);
+        d.ru_t[x] = d.ru_t[x] + (c1 * mux + c2) * lap;
+    }
+    if (i < nx && k < nz) {              // v faces, the duplicate row included
+        const int jc = (j >= ny) ? j - ny : j, jb = (jc == 0) ? ny - 1 : jc - 1;
+        const real muy = R(0.5) * (mut_at(d, jc * nx + i) + mut_at(d, jb * nx + i));
+        const real lap = lap2(d.v, k, jc, i, nz, ny, nx, nx, (ny + 1) * nx, s,
+                              d.rdzf_h, d.rdzc_h);
+        const int x = iV(k, j, i, ny, nx);
+        d.rv_t[x] = d.rv_t[x] + (c1 * muy + c2) * lap;
+    }
+    if (i < nx && j < ny) {
+        const int c = j * nx + i;
+        const real mu = mut_at(d, c);
+        if (k < nz) {
+            const real lap = lap2(d.thp, k, j, i, nz, ny, nx, nx, st, s,
+                                  d.rdzf_h, d.rdzc_h);
+            const int x = iM(k, j, i, ny, nx);
+            d.rth_t[x] = d.rth_t[x] + (c1 * mu + c2) * lap;
+        }
+        // w: interior levels only; the surface and lid get zero
+        real lap = R(0.0);
+        if (k > 0 && k < nz)
+            lap = lap2(d.w, k, j, i, nz + 1, ny, nx, nx, st, s, d.rdzf_w,
+                       d.rdzc_w);
+        const int x = iM(k, j, i, ny, nx);
+        d.rw_t[x] = d.rw_t[x] + (d.c1f[k] * mu + d.c2f[k]) * lap;
+    }
+}
+
+// Uncoupled update from the time-t copies (reference dycore.py:1602-1643):
+// q = (C(mu_t) q_t + dt r_q) / C(mu), face masses for u, v.
+__global__ void k_adv_advance(Dom d, real dt)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    auto m0 = [&](int c) { return d.mub2d[c] + d.mup0[c]; };
+    auto m1 = [&](int c) { return d.mub2d[c] + d.mup[c]; };
+    if (i < nx && j < ny && k < nz) {
+        const int x = iM(k, j, i, ny, nx), c = j * nx + i;
+        const real thb = basev(d.thb, k, c, st, d.base3d);
+        d.thp[x] = (((d.c1h[k] * m0(c) + d.c2h[k]) * (thb + d.thp0[x])
+                     + dt * d.rth_t[x]) / (d.c1h[k] * m1(c) + d.c2h[k]))
+                   - thb;
+    }
+    if (j < ny && k < nz) {
+        const int a = (i >= nx) ? i - nx : i, b = (a == 0) ? nx - 1 : a - 1;
+        const real x0 = R(0.5) * (m0(j * nx + a) + m0(j * nx + b));
+        const real x1 = R(0.5) * (m1(j * nx + a) + m1(j * nx + b));
+        const int x = iU(k, j, i, ny, nx);
+        d.u[x] = ((d.c1h[k] * x0 + d.c2h[k]) * d.u0[x] + dt * d.ru_t[x])
+                 / (d.c1h[k] * x1 + d.c2h[k]);
+    }
+    if (i < nx && k < nz) {
+        const int a = (j >= ny) ? j - ny : j, b = (a == 0) ? ny - 1 : a - 1;
+        const real y0 = R(0.5) * (m0(a * nx + i) + m0(b * nx + i));
+        const real y1 = R(0.5) * (m1(a * nx + i) + m1(b * nx + i));
+        const int x = iV(k, j, i, ny, nx);
+        d.v[x] = ((d.c1h[k] * y0 + d.c2h[k]) * d.v0[x] + dt * d.rv_t[x])
+                 / (d.c1h[k] * y1 + d.c2h[k]);
+    }
+    if (i < nx && j < ny) {
+        const int x = iM(k, j, i, ny, nx), c = j * nx + i;
+        d.w[x] = ((d.c1f[k] * m0(c) + d.c2f[k]) * d.w0[x] + dt * d.rw_t[x])
+                 / (d.c1f[k] * m1(c) + d.c2f[k]);
+    }
+}
+
 extern "C" {
 
 int32_t ARW_API(abi_version)(void) { return ARW_ABI_VERSION; }
@@ -1957,9 +2107,16 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
     real* B[6] = {d->u0, d->v0, d->w0, d->thp0, d->php0, d->mup0};
     real* mup_c = d->mup_next;
     const bool moist = d->qv != nullptr;
-    real* QA[3] = {d->qv, d->qc, d->qr};
-    real* QB[3] = {d->qv0, d->qc0, d->qr0};
-    real* QC[3] = {d->qsv, d->qsc, d->qsr};
+    // every transported species: qv, qc, qr, then the scheme's extras
+    const int nq = 3 + (moist ? d->n_extra : 0);
+    real* QA[3 + ARW_MAX_EXTRA] = {d->qv, d->qc, d->qr};
+    real* QB[3 + ARW_MAX_EXTRA] = {d->qv0, d->qc0, d->qr0};
+    real* QC[3 + ARW_MAX_EXTRA] = {d->qsv, d->qsc, d->qsr};
+    for (int e = 3; e < nq; ++e) {
+        QA[e] = d->qx[e - 3];
+        QB[e] = d->qx0[e - 3];
+        QC[e] = d->qxs[e - 3];
+    }
     const bool sum = moist || ck;                // substep flux averages
     if (ck && hipMemcpyAsync(ck->mup_t, A[5], (size_t)d->ny * d->nx * RB,
                              hipMemcpyDeviceToDevice, st) != hipSuccess)
@@ -1984,6 +2141,7 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
             // scalars of the stage state: time t, then the stage buffers
             real* const* Q = (stage == 0) ? QA : (stage == 1 ? QB : QC);
             dd.qv = Q[0]; dd.qc = Q[1]; dd.qr = Q[2];
+            for (int e = 3; e < nq; ++e) dd.qx[e - 3] = Q[e];
         }
         if (ck) {
             dd.ru_m = ck->ru_m[stage]; dd.rv_m = ck->rv_m[stage];
@@ -2004,18 +2162,15 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
             Sc sn = make_sc(prm, dtaus[stage], n == 0);
             sn.use_pold = (n > 0);
             sn.gam_ready = (n > 0);             // stored by the first
+            // WRF sumflux (reference acoustic.cu:1437-1468) inside the
+            // substep pass: the first substep starts the sums
+            sn.sum = sum ? (n == 0 ? 1 : 2) : 0;
             const bool last = (n + 1 == nsub[stage]);
             // new p'' goes to the buffer holding the dead p''(n-1); the two
             // names swap afterwards
             if ((rc = substep(ds, sn, ds.p_pp_old, ds.base3d || last, sum,
                               st)))
                 return rc;
-            if (sum) {
-                hipLaunchKernelGGL(k_sumflux, union_grid(*d, d->nz + 1),
-                                   dim3(64, 4), 0, st, ds, n == 0);
-                if ((rc = launched(hipGetLastError(), "k_sumflux", KID_SCALARS, st)))
-                    return rc;
-            }
             real* t = ds.p_pp;
             ds.p_pp = ds.p_pp_old;
             ds.p_pp_old = t;
@@ -2038,11 +2193,17 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
         if (moist) {
             real* const* Q = (stage == 0) ? QA : (stage == 1 ? QB : QC);
             real* const* Dst = (stage == 0) ? QB : (stage == 1 ? QC : QA);
+            // QB (stage 1's output) is dead during the final stage
             if ((rc = scalar_stage(dd, prm, Q, QA, Dst, A[5],
-                                   nsub[stage] * dtaus[stage], stage == 2, st)))
+                                   nsub[stage] * dtaus[stage], stage == 2, st,
+                                   nq, QB)))
                 return rc;
         }
-        if ((rc = launched(run_finish(dd, o, prec, st), "k_finish", KID_FINISH, st)))
+        // the final stage removes the heating its substeps integrated
+        const double hdiab_dt = (stage == 2 && d->h_diabatic)
+                                    ? nsub[stage] * dtaus[stage] : 0.0;
+        if ((rc = launched(run_finish(dd, o, prec, st, hdiab_dt), "k_finish",
+                           KID_FINISH, st)))
             return rc;
         if (moist) {
             // moist EOS of the new stage state (needs its updated qv)
@@ -2062,6 +2223,67 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
                        s);
     return launched(hipGetLastError(), "k_set_w_surface", KID_SET_W_SURFACE,
                     st);
+}
+
+// The advection-only step (reference dycore.py:2318-2343); see
+// k_adv_prep.  Uses u_pp, v_pp, ww_pp, th_pp as its flux and total-theta
+// work arrays.
+int32_t ARW_API(step_advective)(const Dom* d, const arw_params* prm,
+                                void* stream)
+{
+    int32_t rc = check_domain(d);
+    if (rc) return rc;
+    if ((rc = check_params(prm, d))) return rc;
+    if (d->qv)
+        return fail(ARW_E_UNSUPPORTED, "step_advective: dry states only");
+    if ((!ARW_F64 && prm->precision_form) || prm->diff_6th_opt)
+        return fail(ARW_E_UNSUPPORTED, "step_advective: plain form, no "
+                                       "6th-order filter");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t m3 = (size_t)d->nz * d->ny * d->nx * sizeof(real);
+    const size_t u3 = (size_t)d->nz * d->ny * (d->nx + 1) * sizeof(real);
+    const size_t v3 = (size_t)d->nz * (d->ny + 1) * d->nx * sizeof(real);
+    const size_t w3 = (size_t)(d->nz + 1) * d->ny * d->nx * sizeof(real);
+    const size_t m2 = (size_t)d->ny * d->nx * sizeof(real);
+    const struct { real* dst; const real* src; size_t n; } cp[] = {
+        {d->u0, d->u, u3}, {d->v0, d->v, v3}, {d->w0, d->w, w3},
+        {d->thp0, d->thp, m3}, {d->php0, d->php, w3}, {d->mup0, d->mup, m2}};
+    for (const auto& c : cp)
+        if (hipMemcpyAsync(c.dst, c.src, c.n, hipMemcpyDeviceToDevice, st)
+            != hipSuccess)
+            return fail(ARW_E_LAUNCH, "step_advective: time-t copy failed");
+    const Sc s = make_sc(prm, prm->dt, 1);
+    const bool diff = s.diff;
+    const dim3 g = union_grid(*d, d->nz + 1), b(64, 4);
+    const real rdx = (real)(1.0 / prm->dx), rdy = (real)(1.0 / prm->dy);
+    const double lens[3] = {prm->dt / 3.0, prm->dt / 2.0, prm->dt};
+    const int nz = d->nz, ny = d->ny, nx = d->nx;
+    for (int stage = 0; stage < 3; ++stage) {
+        if ((rc = launched(run_eos(*d, st), "k_eos", -1, st))) return rc;
+        hipLaunchKernelGGL(k_adv_prep, g, b, 0, st, *d);
+        if ((rc = launched("k_adv_prep"))) return rc;
+        if ((rc = flux_div(0, d->th_pp, d->u_pp, d->v_pp, d->ww_pp, d->rth_t,
+                           d->rdnw, d->fnm, d->fnp, rdx, rdy, nz, ny, nx, st)))
+            return rc;
+        if ((rc = flux_div(1, d->u, d->u_pp, d->v_pp, d->ww_pp, d->ru_t,
+                           d->rdnw, d->fnm, d->fnp, rdx, rdy, nz, ny, nx, st)))
+            return rc;
+        if ((rc = flux_div(2, d->v, d->u_pp, d->v_pp, d->ww_pp, d->rv_t,
+                           d->rdnw, d->fnm, d->fnp, rdx, rdy, nz, ny, nx, st)))
+            return rc;
+        if ((rc = flux_div(3, d->w, d->u_pp, d->v_pp, d->ww_pp, d->rw_t,
+                           d->rdn, d->fnm, d->fnp, rdx, rdy, nz, ny, nx, st)))
+            return rc;
+        if (diff) {
+            hipLaunchKernelGGL(k_adv_diff, g, b, 0, st, *d, s);
+            if ((rc = launched("k_adv_diff"))) return rc;
+        }
+        hipLaunchKernelGGL(k_adv_advance, g, b, 0, st, *d, (real)lens[stage]);
+        if ((rc = launched("k_adv_advance"))) return rc;
+    }
+    hipLaunchKernelGGL(k_set_w_surface<One>, col_grid(*d), dim3(256), 0, st,
+                       *d, s);
+    return launched("k_set_w_surface");
 }
 
 int32_t ARW_API(step)(const Dom* d, const arw_params* prm, void* stream)

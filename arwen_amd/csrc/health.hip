@@ -50,22 +50,40 @@ __global__ __launch_bounds__(256) void k_health_gate(
     const unsigned long long begin = (unsigned long long)blockIdx.x * chunk;
     if (begin >= size) return;
     const unsigned long long end = begin + chunk < size ? begin + chunk : size;
-    const real* __restrict__ v = static_cast<const real*>(f.values);
+    // offsets within the chunk are 32-bit; the level of an element (for a
+    // per-level auxiliary) is carried along instead of divided out
+    const real* __restrict__ v = static_cast<const real*>(f.values) + begin;
     const real* __restrict__ aux = static_cast<const real*>(f.aux);
     const real lo = (real)f.lower, hi = (real)f.upper;
     const uint32_t flags = f.flags;
-    const unsigned long long plane = f.plane ? f.plane : 1ull;
-    unsigned long long first = ~0ull;
-    for (unsigned long long ix = begin + threadIdx.x; ix < end;
-         ix += blockDim.x) {
-        real x = v[ix];
-        if (flags & kAddAux) x = x + aux[(flags & kAuxLevel) ? ix / plane : ix];
+    const unsigned n = (unsigned)(end - begin), step = blockDim.x;
+    const unsigned plane = f.plane ? f.plane : 1u;
+    const bool level = (flags & kAuxLevel) != 0, add = (flags & kAddAux) != 0;
+    unsigned lev = 0, rem = 0;
+    if (level) {
+        const unsigned long long ix0 = begin + threadIdx.x;
+        lev = (unsigned)(ix0 / plane);
+        rem = (unsigned)(ix0 - (unsigned long long)lev * plane);
+    }
+    const real* __restrict__ auxd = add && !level ? aux + begin : aux;
+    unsigned first = ~0u;
+    for (unsigned t = threadIdx.x; t < n; t += step) {
+        real x = v[t];
+        if (add) x = x + (level ? aux[lev] : auxd[t]);
         bool bad = !isfinite(x);
         if (flags & kLower) bad |= (flags & kStrictLower) ? x <= lo : x < lo;
         if (flags & kUpper) bad |= x > hi;
-        if (bad && ix < first) first = ix;
+        if (bad && t < first) first = t;
+        if (level) {
+            rem += step;
+            while (rem >= plane) {
+                rem -= plane;
+                ++lev;
+            }
+        }
     }
-    const unsigned long long m = wave_min(first);
+    const unsigned long long m =
+        wave_min(first == ~0u ? ~0ull : begin + first);
     if ((threadIdx.x & 63) == 0 && m != ~0ull) {
         atomicOr(result, (unsigned long long)f.status_bit);
         atomicMin(result + 1, ((unsigned long long)fi << 48) | m);

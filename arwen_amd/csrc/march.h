@@ -103,6 +103,7 @@ __device__ __forceinline__ const FinishDst& fin_of(const Batch& b, int m,
 int32_t lib_fail(int32_t code, const char* msg);
 int32_t lib_launched(hipError_t e, const char* what);
 int32_t lib_check_domain(const Dom* d);
+int32_t lib_check_adj_domain(const Dom* d);
 int32_t lib_check_prec(const arw_params* p, const Dom* d);
 
 hipError_t run_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st);
@@ -118,13 +119,31 @@ int32_t lib_flux_div_adjoint(int op, const real* q, const real* ru,
                              const real* rdz, const real* fnm,
                              const real* fnp, real dx_inv, real dy_inv,
                              int nz, int ny, int nx, hipStream_t st);
+// Up to four scalars of one stage update that share the advecting fluxes.
+struct QGroup {
+    const real* cur[4];
+    const real* q0[4];
+    real* dst[4];
+    int n;
+};
+
+// Scalar transport of one RK stage as column marches: the unlimited
+// coupled update of a group, and the final stage's positive-definite
+// limiter of one scalar (factors in d.pd_sc, the update into out).
+hipError_t run_scalar_cols(const Dom& d, const QGroup& g, const real* mup0,
+                           real dt, real rdx, real rdy, int clamp,
+                           hipStream_t st);
+hipError_t run_pd_cols(const Dom& d, const real* cur, const real* q0,
+                       real* out, const real* mup0, real dx, real dy,
+                       real dxi, real dyi, real dt, hipStream_t st);
+
 hipError_t run_tend_faces(const Dom& d, const Sc& s, hipStream_t st);
 hipError_t run_tend_cols(const Dom& d, const Sc& s, hipStream_t st);
 hipError_t run_uv(const Dom& d, const Sc& s, hipStream_t st);
 hipError_t run_column(const Dom& d, const Sc& s, real* pout, int write_al,
                       int write_ww, hipStream_t st);
 hipError_t run_finish(const Dom& d, const FinishDst& o, bool prec,
-                      hipStream_t st);
+                      hipStream_t st, double hdiab_dt = 0.0);
 
 // Batched forms (members share base3d and every flag in Sc).
 hipError_t run_omega_b(const BatchSet& b, const Sc& s, int do_eos, bool b3,

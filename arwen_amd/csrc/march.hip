@@ -392,8 +392,10 @@ __global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
         uz[2 + kk] = gld(d.u, kk * stu + j * pu, cC);
         vz[2 + kk] = gld(d.v, kk * stv + j * nx, cC);
     }
-    real wA0 = gld(d.ww, rA, cC), wW0 = gld(d.ww, rA, cW);
-    real wS0 = gld(d.ww, rS, cC);
+    // the vertical flux through w level k+1 is the next level's flux
+    // through its lower face: carried, not recomputed (level 0's lower
+    // face flux is zero)
+    real fzu = R(0.0), fzv = R(0.0);
 
     for (int k = 0; k < nz; ++k) {
         opaque_all();
@@ -437,9 +439,10 @@ __global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
             real v0 = R(0.5) * (rv_jW + rv_jA), v1 = R(0.5) * (rv_pW + rv_pA);
             real fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], v0);
             real fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], v1);
-            real wz0 = R(0.5) * (wW0 + wA0), wz1 = R(0.5) * (wW1 + wA1);
-            real fz0 = zf(uz[0], uz[1], uz[2], uz[3], wz0, k, nz, fnm, fnp);
-            real fz1 = zf(uz[1], uz[2], uz[3], uz[4], wz1, k + 1, nz, fnm, fnp);
+            const real wz1 = R(0.5) * (wW1 + wA1);
+            const real fz0 = fzu;
+            const real fz1 = zf(uz[1], uz[2], uz[3], uz[4], wz1, k + 1, nz, fnm, fnp);
+            fzu = fz1;
             real t = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy,
                            rdnw[k]);
             t = t - pgf_val(s.rdx, s.hrdx, c1, c2, rdnw[k], mux_c, dmu_u,
@@ -490,9 +493,10 @@ __global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
             real vl = R(0.5) * (rv_a + rv_b), vr = R(0.5) * (rv_c + rv_d);
             real fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], vl);
             real fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], vr);
-            real wz0 = R(0.5) * (wS0 + wA0), wz1 = R(0.5) * (wS1 + wA1);
-            real fz0 = zf(vz[0], vz[1], vz[2], vz[3], wz0, k, nz, fnm, fnp);
-            real fz1 = zf(vz[1], vz[2], vz[3], vz[4], wz1, k + 1, nz, fnm, fnp);
+            const real wz1 = R(0.5) * (wS1 + wA1);
+            const real fz0 = fzv;
+            const real fz1 = zf(vz[1], vz[2], vz[3], vz[4], wz1, k + 1, nz, fnm, fnp);
+            fzv = fz1;
             real t = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy,
                            rdnw[k]);
             t = t - pgf_val(s.rdy, s.hrdy, c1, c2, rdnw[k], muy_c, dmu_v,
@@ -519,7 +523,6 @@ __global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
             const real vn = gld(d.v, (k + 3 < nz ? k + 3 : nz - 1) * stv + j * nx, cC);
             vz[4] = (k + 3 < nz) ? vn : R(0.0);
         }
-        wA0 = wA1; wW0 = wW1; wS0 = wS1;
         win_shift(A, k, rA, cC);
         win_shift(W, k, rA, cW);
         win_shift(S, k, rS, cC);
@@ -692,6 +695,10 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
         return -(dnw[k] * (c1h[k] * mubc + c2h[k])) * albk;
     };
 
+    // vertical fluxes through the upper face of the level below, carried
+    // up the column: theta's at w level k, w's at mass level k-1
+    real fzt = R(0.0), fzw = R(0.0);
+
     // kinds: 0 = surface level (w level 0 + mass level 0), 1 = interior,
     // 2 = lid (w level nz only)
     auto level = [&](int k, auto kind) {
@@ -711,9 +718,14 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
                 pp_0 = gld(d.pp, lm + rA, cC);
             else
                 pp_0 = p_k - (B3 ? gld(d.pb, lm + rA, cC) : pb1[k]);
-            if (MOIST)
+            if (MOIST) {
                 qt_0 = (gld(d.qv, lm + rA, cC) + gld(d.qc, lm + rA, cC))
                        + gld(d.qr, lm + rA, cC);
+                // a mixed-phase scheme's condensate masses, in order
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (e < d.n_load) qt_0 = qt_0 + gld(d.qx[e], lm + rA, cC);
+            }
         }
         real ru_i = R(0.0), ru_ip = R(0.0), rv_j = R(0.0), rv_jp = R(0.0);
         real u_i = R(0.0), u_ip = R(0.0), v_j = R(0.0), v_jp = R(0.0);
@@ -750,18 +762,14 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
             real fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], u1v);
             real fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], v0v);
             real fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], v1v);
-            real fz[2];
-            const real wwv[3] = {ww_m, ww_0, ww_p};
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const int m = k - 1 + q;
-                const real vel = R(0.5) * (wwv[q] + wwv[q + 1]);
-                const real lin = R(0.5) * vel * (wz[q + 2] + wz[q + 1]);
-                real cub = flux3(wz[q], wz[q + 1], wz[q + 2], wz[q + 3], vel);
-                keep(cub);
-                fz[q] = (m == 0 || m == nz - 1) ? lin : cub;
-            }
-            t = div3(fx1 - fx0, fy1 - fy0, fz[1] - fz[0], s.rdx, s.rdy, rdn[k]);
+            // flux through mass level k (between w levels k and k+1)
+            const real vel = R(0.5) * (ww_0 + ww_p);
+            const real lin = R(0.5) * vel * (wz[3] + wz[2]);
+            real cub = flux3(wz[1], wz[2], wz[3], wz[4], vel);
+            keep(cub);
+            const real fz1 = (k == nz - 1) ? lin : cub;
+            t = div3(fx1 - fx0, fy1 - fy0, fz1 - fzw, s.rdx, s.rdy, rdn[k]);
+            fzw = fz1;
             real term;
             if (MOIST) {
                 // moist pg_buoy_w (reference dycore.cu:279-289)
@@ -773,6 +781,11 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
                 term = rdn[k] * (pp_0 - pp_m) - c1f[k] * mupc;
             }
             t = t + G_F * term;
+        } else if (!top) {
+            // surface: the flux through mass level 0, which w level 1
+            // takes as its lower one (the linear form)
+            const real vel = R(0.5) * (ww_0 + ww_p);
+            fzw = R(0.5) * vel * (wz[3] + wz[2]);
         } else if (top) {
             if (MOIST) {
                 // lid: the top two layers (reference dycore.cu:258-273);
@@ -871,9 +884,14 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
             real fx1 = flux5(qxv[1], qxv[2], qxv[3], qxv[4], qxv[5], qxv[6], ru_ip);
             real fy0 = flux5(qyv[0], qyv[1], qyv[2], qyv[3], qyv[4], qyv[5], rv_j);
             real fy1 = flux5(qyv[1], qyv[2], qyv[3], qyv[4], qyv[5], qyv[6], rv_jp);
-            real fz0 = zf(thz[0], thz[1], thz[2], thz[3], ww_0, k, nz, fnm, fnp);
-            real fz1 = zf(thz[1], thz[2], thz[3], thz[4], ww_p, k + 1, nz, fnm, fnp);
-            real t2 = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy, rdnw[k]);
+            const real fz1 = zf(thz[1], thz[2], thz[3], thz[4], ww_p, k + 1, nz, fnm, fnp);
+            real t2 = div3(fx1 - fx0, fy1 - fy0, fz1 - fzt, s.rdx, s.rdy, rdnw[k]);
+            fzt = fz1;
+            if (MOIST && d.h_diabatic) {
+                // the retained microphysics heating, coupled with the
+                // stage's column mass (reference dycore.py:572-592)
+                t2 = t2 + (c1h[k] * mA + c2h[k]) * gld(d.h_diabatic, lm + rA, cC);
+            }
             if (DIFF) {
                 real fc = thp_0;
                 real lap = s.kh * (s.dxi2 * ((tpx[4] - R(2.0) * fc) + tpx[2])
@@ -962,7 +980,7 @@ struct UvIn {
 // and north of the v face, W its west and S its south neighbour.  The
 // damped pressure p'' + smdiv (p'' - p''_old) rides in 4-level windows for
 // the surface and lid extrapolations of dpn.
-template <bool B3, bool USEP, bool DU, bool DV>
+template <bool B3, bool USEP, bool DU, bool DV, bool SUM = false>
 __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
                                          int rS, int rU, unsigned bA,
                                          unsigned bW, unsigned bS, unsigned bu,
@@ -1098,7 +1116,13 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
             if (B3) dphp = dphp + R(0.5) * ((gbA0 + gbA1) - (gbW0 + L.gbW));
             dpxy = dpxy + (s.rdx * dphp) * (rdnw[k] * (dn_up - dnu_0)
                                             - (R(0.5) * c1) * dmu_u);
-            gst(d.u_pp, k * stu + rU, bu, L.u + dtau * (L.ru - dpxy));
+            const real un = L.u + dtau * (L.ru - dpxy);
+            gst(d.u_pp, k * stu + rU, bu, un);
+            if (SUM) {
+                const real acc = (s.sum == 1) ? R(0.0)
+                                              : gld(d.ru_m, k * stu + rU, bu);
+                gst(d.ru_m, k * stu + rU, bu, acc + un);
+            }
             dnu_0 = dn_up;
             phW0 = L.phW; hpW0 = L.hpW; gbW0 = L.gbW;
         }
@@ -1121,7 +1145,13 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
             if (B3) dphp = dphp + R(0.5) * ((gbA0 + gbA1) - (gbS0 + L.gbS));
             dpxy = dpxy + (s.rdy * dphp) * (rdnw[k] * (dn_up - dnv_0)
                                             - (R(0.5) * c1) * dmu_v);
-            gst(d.v_pp, k * stv + rA, bv, L.v + dtau * (L.rv - dpxy));
+            const real vn = L.v + dtau * (L.rv - dpxy);
+            gst(d.v_pp, k * stv + rA, bv, vn);
+            if (SUM) {
+                const real acc = (s.sum == 1) ? R(0.0)
+                                              : gld(d.rv_m, k * stv + rA, bv);
+                gst(d.rv_m, k * stv + rA, bv, acc + vn);
+            }
             dnv_0 = dn_up;
             phS0 = L.phS; hpS0 = L.hpS; gbS0 = L.gbS;
         }
@@ -1150,7 +1180,7 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
 // column (j, i) advances u'' on face (j, i) and v'' on face (j, i).
 // Batched (Many): each member's logical range holds its edge workgroups
 // first; ne_arg is unused.
-template <class SRC, bool B3, bool USEP>
+template <class SRC, bool B3, bool USEP, bool SUM = false>
 __global__ __launch_bounds__(256) void k_uv(typename SRC::Arg a, Sc s, int ne_arg)
 {
     int m, g;
@@ -1162,12 +1192,12 @@ __global__ __launch_bounds__(256) void k_uv(typename SRC::Arg a, Sc s, int ne_ar
         const int t = g * 256 + threadIdx.y * 64 + threadIdx.x;
         if (t < ny) {
             const int j = t;
-            uv_march<B3, USEP, true, false>(
+            uv_march<B3, USEP, true, false, SUM>(
                 d, s, 0, 0, 0, RB * (j * nx), RB * (j * nx + nx - 1), 0u,
                 RB * (j * pu + nx), 0u);
         } else if (t < ny + nx) {
             const int i = t - ny;
-            uv_march<B3, USEP, false, true>(
+            uv_march<B3, USEP, false, true, SUM>(
                 d, s, 0, 0, 0, RB * i, 0u, RB * ((ny - 1) * nx + i), 0u,
                 RB * (ny * nx + i));
         }
@@ -1179,7 +1209,7 @@ __global__ __launch_bounds__(256) void k_uv(typename SRC::Arg a, Sc s, int ne_ar
     const int i = rs.i, j = rs.j;
     const int jm = (j == 0) ? ny - 1 : j - 1;
     const unsigned bc = RB * i, bw = RB * ((i == 0) ? nx - 1 : i - 1);
-    uv_march<B3, USEP, true, true>(d, s, j * nx, jm * nx, j * pu, bc, bw, bc,
+    uv_march<B3, USEP, true, true, SUM>(d, s, j * nx, jm * nx, j * pu, bc, bw, bc,
                                    bc, bc);
 }
 
@@ -1355,7 +1385,18 @@ __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
     };
 
     // upward sweep
-    if (WW) gst(d.ww_pp, 0, bc, R(0.0));
+    // Omega'' at w level q: stored (standalone substep), or summed into
+    // ww_m (the step's fused sumflux)
+    auto put_ww = [&](int q, real x) {
+        if (!WW) return;
+        if (s.sum == 0) {
+            gst(d.ww_pp, q * st, bc, x);
+        } else {
+            const real acc = (s.sum == 1) ? R(0.0) : gld(d.ww_m, q * st, bc);
+            gst(d.ww_m, q * st, bc, acc + x);
+        }
+    };
+    put_ww(0, R(0.0));
     real ww_k = R(0.0), wwk = R(0.0), wdtn_k = R(0.0);
     Kahan kww;
     real th_m = (B3 ? gld(d.thb, 0, bc) : thb1[0]) + gld(d.thp, 0, bc);
@@ -1444,7 +1485,7 @@ __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
         } else {
             ww_kp1 = R(0.0);
         }
-        if (WW) gst(d.ww_pp, ms + st, bc, ww_kp1);
+        put_ww(m + 1, ww_kp1);
         real th_up = has_up ? thbv(L.bhu, m + 1) + L.thu : R(0.0);
         real wdtn_kp1 = has_up
             ? ww_kp1 * (fnm[m + 1] * th_up + fnp[m + 1] * th_m) : R(0.0);
@@ -1681,9 +1722,13 @@ struct FinIn {
 // state (diagnostics.cu:37-89).  PREC (precision form): the new theta' is
 // (C(mu_s) thp + theta'' - (c1h mu'') thb) / C(mu_n), free of the base
 // theta's cancellation, and the EOS is eos_pert.
+// hdt > 0 (the final stage of a step with retained microphysics heating,
+// d.h_diabatic set): the theta fold removes hdt C(mu_s) h_diabatic, what
+// the substeps integrated from the stage tendency's heating term
+// (reference dycore.cu:1186-1194, module_small_step_em.F:408-426).
 template <class SRC, bool B3, bool EOS, bool PREC>
 __global__ __launch_bounds__(256) void k_finish(typename SRC::Arg a,
-                                                FinishDst o_arg)
+                                                FinishDst o_arg, real hdt)
 {
     int m, g;
     if (!SRC::locate(a, m, g)) return;
@@ -1790,7 +1835,10 @@ __global__ __launch_bounds__(256) void k_finish(typename SRC::Arg a,
                            - (c1h[k] * muppA) * thbk;
                 thn = num / (c1h[k] * mnA + c2h[k]);
             } else {
-                real num = (c1h[k] * msA + c2h[k]) * (thbk + L.thp) + L.thpp;
+                const real cs = c1h[k] * msA + c2h[k];
+                real num = cs * (thbk + L.thp) + L.thpp;
+                if (hdt > R(0.0))
+                    num = num - (hdt * cs) * gld(d.h_diabatic, f, bc);
                 thn = num / (c1h[k] * mnA + c2h[k]) - thbk;
             }
             gst(o.thp, f, bc, thn);
@@ -1834,6 +1882,376 @@ __global__ __launch_bounds__(256) void k_finish_edge(typename SRC::Arg a,
         const real msB = mub[cB] + mup[cB], mnB = msB + mu_pp[cB];
         finish_face<1>(d, o.v, d.v, d.v_pp, ny * nx + i,
                        R(0.5) * (msA + msB), R(0.5) * (mnA + mnB));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Scalar transport of one RK stage as column marches (reference moist.py:
+// 434-698 with the update kernel :353-402 and pd_advection.cu's periodic
+// path).  Row-segment waves as in the stencil kernels: each lane marches
+// one mass column upward, the y stencil rows are wave-uniform, and the
+// vertical flux through a level's upper face is carried to the next level
+// as its lower one.
+// ---------------------------------------------------------------------------
+
+// WRF flux_upwind (reference pd_advection.cu:101-108).
+__device__ __forceinline__ real upwind_flux(real qm1, real q0v, real cr)
+{
+    return (R(0.5) * fmin(R(1.0), cr + fabs(cr))) * qm1
+           + (R(0.5) * fmax(-R(1.0), cr - fabs(cr))) * q0v;
+}
+
+// Per-lane view of a mass column for the scalar marches: the 7 x columns
+// (periodic), the 7 y rows (wave-uniform), the u/v face columns.
+struct ScalCol {
+    int i, j, rA, yo[7];
+    unsigned cx[7], cu1;
+};
+
+__device__ __forceinline__ bool scal_col(const Dom& d, ScalCol& s)
+{
+    const RowSeg rs = row_seg(d.nx, d.ny);
+    if (!rs.live) return false;
+    const int nx = d.nx, ny = d.ny;
+    s.i = rs.i;
+    s.j = rs.j;
+    s.rA = rs.j * nx;
+#pragma unroll
+    for (int o = 0; o < 7; ++o) {
+        s.yo[o] = pmod(rs.j + o - 3, ny) * nx;
+        s.cx[o] = RB * pmod(rs.i + o - 3, nx);
+    }
+    s.cu1 = RB * (rs.i + 1);
+    return true;
+}
+
+// Unlimited coupled stage update of up to NS species sharing the
+// substep-averaged fluxes ru_m, rv_m, ww_m (the final stage clamps at
+// zero when clamp is set):
+//   q = (C(mu_t) q_t + dt div(q_cur; ru_m, rv_m, ww_m)) / C(mu_new).
+// Same float32 expressions, in the same order, as the reference's flux
+// divergence and update, so the result is bit-identical.
+template <int NS>
+__global__ __launch_bounds__(256) void k_scalar_cols(Dom d, QGroup g,
+                                                     const real* RS mup0,
+                                                     real dt, real rdx,
+                                                     real rdy, int clamp)
+{
+    ScalCol q;
+    if (!scal_col(d, q)) return;
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    const int pu = nx + 1, stu = ny * pu, stv = (ny + 1) * nx;
+    kreal* c1h = kview(d.c1h);
+    kreal* c2h = kview(d.c2h);
+    kreal* rdnw = kview(d.rdnw);
+    kreal* fnm = kview(d.fnm);
+    kreal* fnp = kview(d.fnp);
+    auto opaque_all = [&]() {
+#pragma unroll
+        for (int o = 0; o < 7; ++o) colb_opaque(q.cx[o]);
+        colb_opaque(q.cu1);
+    };
+    const unsigned cC = q.cx[3];
+    const int c = q.rA + q.i;
+    const real mu0 = d.mub2d[c] + mup0[c];
+    const real mu = d.mub2d[c] + (d.mup[c] + d.mu_pp[c]);
+    real qz[NS][5], fz0[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        qz[s][0] = qz[s][1] = R(0.0);
+        qz[s][2] = gld(g.cur[s], q.rA, cC);
+        qz[s][3] = gld(g.cur[s], st + q.rA, cC);
+        qz[s][4] = gld(g.cur[s], 2 * st + q.rA, cC);
+        fz0[s] = R(0.0);                    // no flux through the surface
+    }
+    for (int k = 0; k < nz; ++k) {
+        opaque_all();
+        const int lm = k * st, lu = k * stu + q.j * pu, lv = k * stv;
+        const real ru0 = gld(d.ru_m, lu, cC), ru1 = gld(d.ru_m, lu, q.cu1);
+        const real rv0 = gld(d.rv_m, lv + q.rA, cC);
+        const real rv1 = gld(d.rv_m, lv + q.rA + nx, cC);
+        const real ww1 = gld(d.ww_m, lm + st + q.rA, cC);
+        const real chm0 = c1h[k] * mu0 + c2h[k];
+        const real chm = c1h[k] * mu + c2h[k];
+        const int kn = k + 3 < nz ? k + 3 : nz - 1;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            real x[7], y[7];
+#pragma unroll
+            for (int o = 0; o < 7; ++o) {
+                x[o] = (o == 3) ? qz[s][2] : gld(g.cur[s], lm + q.rA, q.cx[o]);
+                y[o] = (o == 3) ? qz[s][2] : gld(g.cur[s], lm + q.yo[o], cC);
+            }
+            const real fx0 = flux5(x[0], x[1], x[2], x[3], x[4], x[5], ru0);
+            const real fx1 = flux5(x[1], x[2], x[3], x[4], x[5], x[6], ru1);
+            const real fy0 = flux5(y[0], y[1], y[2], y[3], y[4], y[5], rv0);
+            const real fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], rv1);
+            const real fz1 = zf(qz[s][1], qz[s][2], qz[s][3], qz[s][4], ww1,
+                                k + 1, nz, fnm, fnp);
+            const real tend = R(0.0) + div3(fx1 - fx0, fy1 - fy0,
+                                            fz1 - fz0[s], rdx, rdy, rdnw[k]);
+            fz0[s] = fz1;
+            real v = chm0 * gld(g.q0[s], lm + q.rA, cC);
+            const real t = dt * tend;
+            v = v + t;
+            v = v / chm;
+            if (clamp && !(v > R(0.0)) && v == v) v = R(0.0);
+            gst(g.dst[s], lm + q.rA, cC, v);
+            const real qn = gld(g.cur[s], kn * st + q.rA, cC);
+            qz[s][0] = qz[s][1];
+            qz[s][1] = qz[s][2];
+            qz[s][2] = qz[s][3];
+            qz[s][3] = qz[s][4];
+            qz[s][4] = (k + 3 < nz) ? qn : R(0.0);
+        }
+    }
+}
+
+// The positive-definite limiter's face fluxes of one cell (reference
+// pd_advection.cu pd_fluxes, periodic, no msf): at each face the
+// CFL-clamped upwind flux fl of the time-t scalar and the correction
+// fc = (unlimited flux of the stage scalar) - fl.  x: faces i (l), i+1
+// (r); y: rows j (l), j+1 (r).  The vertical faces are carried up the
+// column by the caller.
+struct PdFaces {
+    real xl_l, xl_c, xr_l, xr_c, yl_l, yl_c, yr_l, yr_c;
+};
+
+struct PdCol {
+    real mW, mA, mE, mS, mN;     // post-acoustic stage mass of the 5 columns
+};
+
+__device__ __forceinline__ PdFaces pd_faces(const Dom& d, const ScalCol& q,
+                                            const PdCol& m,
+                                            const real* RS cur,
+                                            const real* RS q0, int k,
+                                            real dx, real dy, real dt,
+                                            real c1k, real c2k, real q0A,
+                                            real qcA)
+{
+    const int ny = d.ny, nx = d.nx, st = ny * nx;
+    const int pu = nx + 1, stu = ny * pu, stv = (ny + 1) * nx;
+    const int lm = k * st, lu = k * stu + q.j * pu, lv = k * stv;
+    const unsigned cC = q.cx[3];
+    PdFaces f;
+    real x[7], y[7];
+#pragma unroll
+    for (int o = 0; o < 7; ++o) {
+        x[o] = (o == 3) ? qcA : gld(cur, lm + q.rA, q.cx[o]);
+        y[o] = (o == 3) ? qcA : gld(cur, lm + q.yo[o], cC);
+    }
+    const real q0W = gld(q0, lm + q.rA, q.cx[2]);
+    const real q0E = gld(q0, lm + q.rA, q.cx[4]);
+    const real q0S = gld(q0, lm + q.yo[2], cC);
+    const real q0N = gld(q0, lm + q.yo[4], cC);
+    {   // x face i: cells i-1 (upwind left) and i
+        const real vel = gld(d.ru_m, lu, cC);
+        const real muf = (c1k * R(0.5)) * (m.mA + m.mW) + c2k;
+        const real cr = ((vel * dt) / dx) / muf;
+        f.xl_l = (muf * (dx / dt)) * upwind_flux(q0W, q0A, cr);
+        f.xl_c = flux5(x[0], x[1], x[2], x[3], x[4], x[5], vel) - f.xl_l;
+    }
+    {   // x face i+1: cells i and i+1
+        const real vel = gld(d.ru_m, lu, q.cu1);
+        const real muf = (c1k * R(0.5)) * (m.mE + m.mA) + c2k;
+        const real cr = ((vel * dt) / dx) / muf;
+        f.xr_l = (muf * (dx / dt)) * upwind_flux(q0A, q0E, cr);
+        f.xr_c = flux5(x[1], x[2], x[3], x[4], x[5], x[6], vel) - f.xr_l;
+    }
+    {   // y face j
+        const real vel = gld(d.rv_m, lv + q.rA, cC);
+        const real muf = (c1k * R(0.5)) * (m.mA + m.mS) + c2k;
+        const real cr = ((vel * dt) / dy) / muf;
+        f.yl_l = (muf * (dy / dt)) * upwind_flux(q0S, q0A, cr);
+        f.yl_c = flux5(y[0], y[1], y[2], y[3], y[4], y[5], vel) - f.yl_l;
+    }
+    {   // y face j+1
+        const real vel = gld(d.rv_m, lv + q.rA + nx, cC);
+        const real muf = (c1k * R(0.5)) * (m.mN + m.mA) + c2k;
+        const real cr = ((vel * dt) / dy) / muf;
+        f.yr_l = (muf * (dy / dt)) * upwind_flux(q0A, q0N, cr);
+        f.yr_c = flux5(y[1], y[2], y[3], y[4], y[5], y[6], vel) - f.yr_l;
+    }
+    return f;
+}
+
+// Vertical face kf (1 <= kf <= nz-1) of the limiter: (fl, fc) from the
+// time-t scalar below/above and the stage scalar's window qm2..qp1 around
+// the face; zero at the surface and the lid.
+__device__ __forceinline__ void pd_zface(const Dom& d, kreal* rdnw,
+                                         kreal* c1h, kreal* c2h, kreal* fnm,
+                                         kreal* fnp, int kf, real vel,
+                                         real mA, real q0b, real q0t,
+                                         real qm2, real qm1, real q0v,
+                                         real qp1, real dt, real& fl,
+                                         real& fc)
+{
+    const int nz = d.nz;
+    const int kc = kf < nz ? kf : nz - 1, kb = kf > 0 ? kf - 1 : 0;
+    const real dz = R(2.0) / (rdnw[kc] + rdnw[kb]);
+    const real muf = c1h[kc] * mA + c2h[kc];
+    const real cr = (vel * dt) / (dz * muf);
+    const real l = (muf * (dz / dt)) * upwind_flux(q0b, q0t, cr);
+    const real h = zf(qm2, qm1, q0v, qp1, vel, kf, nz, fnm, fnp);
+    const bool in = kf > 0 && kf < nz;
+    fl = in ? l : R(0.0);
+    fc = in ? h - l : R(0.0);
+}
+
+__device__ __forceinline__ PdCol pd_col(const Dom& d, const ScalCol& q)
+{
+    auto mut = [&](int row, unsigned cb) {
+        return gld(d.mub2d, row, cb)
+               + (gld(d.mup, row, cb) + gld(d.mu_pp, row, cb));
+    };
+    PdCol m;
+    m.mA = mut(q.rA, q.cx[3]);
+    m.mW = mut(q.rA, q.cx[2]);
+    m.mE = mut(q.rA, q.cx[4]);
+    m.mS = mut(q.yo[2], q.cx[3]);
+    m.mN = mut(q.yo[4], q.cx[3]);
+    return m;
+}
+
+// Pass 1 of the final-stage limiter (reference pd_advection.cu pd_scale,
+// periodic no-msf branch): the renormalisation factor of every cell from
+// its face fluxes, recomputed here instead of stored.
+__global__ __launch_bounds__(256) void k_pd_scale_cols(
+    Dom d, const real* RS cur, const real* RS q0, const real* RS mup0,
+    real* RS sc, real dx, real dy, real dxi, real dyi, real dt)
+{
+    ScalCol q;
+    if (!scal_col(d, q)) return;
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    kreal* c1h = kview(d.c1h);
+    kreal* c2h = kview(d.c2h);
+    kreal* rdnw = kview(d.rdnw);
+    kreal* fnm = kview(d.fnm);
+    kreal* fnp = kview(d.fnp);
+    auto opaque_all = [&]() {
+#pragma unroll
+        for (int o = 0; o < 7; ++o) colb_opaque(q.cx[o]);
+        colb_opaque(q.cu1);
+    };
+    const unsigned cC = q.cx[3];
+    const int c = q.rA + q.i;
+    const PdCol m = pd_col(d, q);
+    const real mu0 = d.mub2d[c] + mup0[c];
+    // stage scalar window q(k-2..k+2), time-t scalar at k, k+1
+    real w[5] = {R(0.0), R(0.0), gld(cur, q.rA, cC), gld(cur, st + q.rA, cC),
+                 gld(cur, 2 * st + q.rA, cC)};
+    real t0 = gld(q0, q.rA, cC), t1 = gld(q0, st + q.rA, cC);
+    real zb_l = R(0.0), zb_c = R(0.0);      // surface face
+    for (int k = 0; k < nz; ++k) {
+        opaque_all();
+        const real c1k = c1h[k], c2k = c2h[k];
+        const PdFaces f = pd_faces(d, q, m, cur, q0, k, dx, dy, dt, c1k, c2k,
+                                   t0, w[2]);
+        real zt_l, zt_c;
+        const real vel = gld(d.ww_m, (k + 1) * st + q.rA, cC);
+        pd_zface(d, rdnw, c1h, c2h, fnm, fnp, k + 1, vel, m.mA, t0, t1, w[1],
+                 w[2], w[3], w[4], dt, zt_l, zt_c);
+        const real chm0 = c1k * mu0 + c2k;
+        const real dl = (dxi * (f.xr_l - f.xl_l) + dyi * (f.yr_l - f.yl_l))
+                        + rdnw[k] * (zt_l - zb_l);
+        const real ph_low = chm0 * t0 - dt * dl;
+        const real fo = dt * ((dxi * (fmax(R(0.0), f.xr_c) - fmin(R(0.0), f.xl_c))
+                               + dyi * (fmax(R(0.0), f.yr_c) - fmin(R(0.0), f.yl_c)))
+                              + rdnw[k] * (fmin(R(0.0), zt_c) - fmax(R(0.0), zb_c)));
+        real r = R(1.0);
+        if (fo > ph_low) r = fmin(R(1.0), fmax(R(0.0), ph_low / (fo + R(1e-20))));
+        gst(sc, k * st + q.rA, cC, r);
+        zb_l = zt_l;
+        zb_c = zt_c;
+        const int kn = k + 3 < nz ? k + 3 : nz - 1;
+        const real wn = gld(cur, kn * st + q.rA, cC);
+        w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4];
+        w[4] = (k + 3 < nz) ? wn : R(0.0);
+        const int kt = k + 2 < nz ? k + 2 : nz - 1;
+        const real tn = gld(q0, kt * st + q.rA, cC);
+        t0 = t1;
+        t1 = (k + 2 < nz) ? tn : R(0.0);
+    }
+}
+
+// Pass 2 (reference pd_advection.cu pd_renorm_apply + moist.py:385-398,
+// final stage, clamped): every correction flux takes its donor cell's
+// factor; the coupled update goes to out (not q0's array, whose
+// neighbours the face fluxes still read).
+__global__ __launch_bounds__(256) void k_pd_apply_cols(
+    Dom d, const real* RS cur, const real* RS q0, const real* RS mup0,
+    const real* RS sc, real* RS out, real dx, real dy, real dxi, real dyi,
+    real dt)
+{
+    ScalCol q;
+    if (!scal_col(d, q)) return;
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    kreal* c1h = kview(d.c1h);
+    kreal* c2h = kview(d.c2h);
+    kreal* rdnw = kview(d.rdnw);
+    kreal* fnm = kview(d.fnm);
+    kreal* fnp = kview(d.fnp);
+    auto opaque_all = [&]() {
+#pragma unroll
+        for (int o = 0; o < 7; ++o) colb_opaque(q.cx[o]);
+        colb_opaque(q.cu1);
+    };
+    const unsigned cC = q.cx[3];
+    const int c = q.rA + q.i;
+    const PdCol m = pd_col(d, q);
+    const real mu0 = d.mub2d[c] + mup0[c];
+    const real mu = d.mub2d[c] + (d.mup[c] + d.mu_pp[c]);
+    real w[5] = {R(0.0), R(0.0), gld(cur, q.rA, cC), gld(cur, st + q.rA, cC),
+                 gld(cur, 2 * st + q.rA, cC)};
+    real t0 = gld(q0, q.rA, cC), t1 = gld(q0, st + q.rA, cC);
+    real zb_l = R(0.0), zb_c = R(0.0);
+    real s_b = R(1.0), s0 = gld(sc, q.rA, cC);   // factors at k-1, k
+    for (int k = 0; k < nz; ++k) {
+        opaque_all();
+        const int lm = k * st;
+        const real c1k = c1h[k], c2k = c2h[k];
+        const PdFaces f = pd_faces(d, q, m, cur, q0, k, dx, dy, dt, c1k, c2k,
+                                   t0, w[2]);
+        real zt_l, zt_c;
+        const real vel = gld(d.ww_m, (k + 1) * st + q.rA, cC);
+        pd_zface(d, rdnw, c1h, c2h, fnm, fnp, k + 1, vel, m.mA, t0, t1, w[1],
+                 w[2], w[3], w[4], dt, zt_l, zt_c);
+        const int ku = k + 1 < nz ? k + 1 : nz - 1;
+        const real s_t = gld(sc, ku * st + q.rA, cC);
+        const real sW = gld(sc, lm + q.rA, q.cx[2]);
+        const real sE = gld(sc, lm + q.rA, q.cx[4]);
+        const real sS = gld(sc, lm + q.yo[2], cC);
+        const real sN = gld(sc, lm + q.yo[4], cC);
+        const real sx_l = (f.xl_c > R(0.0)) ? sW : ((f.xl_c < R(0.0)) ? s0 : R(1.0));
+        const real sx_r = (f.xr_c > R(0.0)) ? s0 : ((f.xr_c < R(0.0)) ? sE : R(1.0));
+        const real sy_l = (f.yl_c > R(0.0)) ? sS : ((f.yl_c < R(0.0)) ? s0 : R(1.0));
+        const real sy_r = (f.yr_c > R(0.0)) ? s0 : ((f.yr_c < R(0.0)) ? sN : R(1.0));
+        const real sz_b = (zb_c > R(0.0)) ? s0 : ((zb_c < R(0.0)) ? s_b : R(1.0));
+        const real sz_t = (zt_c < R(0.0)) ? s0 : ((zt_c > R(0.0)) ? s_t : R(1.0));
+        const real gx = (sx_r * f.xr_c + f.xr_l) - (sx_l * f.xl_c + f.xl_l);
+        const real gy = (sy_r * f.yr_c + f.yr_l) - (sy_l * f.yl_c + f.yl_l);
+        const real gz = (sz_t * zt_c + zt_l) - (sz_b * zb_c + zb_l);
+        const real tend = R(0.0) + (((-dxi) * gx - dyi * gy) - rdnw[k] * gz);
+        const real chm0 = c1k * mu0 + c2k;
+        const real chm = c1k * mu + c2k;
+        real v = chm0 * t0;
+        const real tt = dt * tend;
+        v = v + tt;
+        v = v / chm;
+        if (!(v > R(0.0)) && v == v) v = R(0.0);
+        gst(out, lm + q.rA, cC, v);
+        zb_l = zt_l;
+        zb_c = zt_c;
+        s_b = s0;
+        s0 = s_t;
+        const int kn = k + 3 < nz ? k + 3 : nz - 1;
+        const real wn = gld(cur, kn * st + q.rA, cC);
+        w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4];
+        w[4] = (k + 3 < nz) ? wn : R(0.0);
+        const int kt = k + 2 < nz ? k + 2 : nz - 1;
+        const real tn = gld(q0, kt * st + q.rA, cC);
+        t0 = t1;
+        t1 = (k + 2 < nz) ? tn : R(0.0);
     }
 }
 
@@ -1891,19 +2309,26 @@ static void with_prec(bool prec, F&& f)
     f(std::false_type{});
 }
 
+// Unused LDS reserved by every k_tend_faces launch: 40 KB of the CU's
+// 160 KB admits at most 4 workgroups (4 waves per SIMD).  At 5 waves per
+// SIMD the 7-row stencils' L2 reuse drops (13.4 instead of 11.3 fetched
+// words per point at 1000x1000x50) and the kernel runs slower.
+constexpr size_t kTendFacesLds = 40 * 1024;
+
 template <class SRC, bool PREC>
 static void launch_tend_faces(const typename SRC::Arg& a, dim3 g, const Sc& s,
                               bool b3, hipStream_t st)
 {
     const dim3 b(64, 4);
+    const size_t l = kTendFacesLds;
     if (b3 && s.diff)
-        hipLaunchKernelGGL((k_tend_faces<SRC, true, true, PREC>), g, b, 0, st, a, s);
+        hipLaunchKernelGGL((k_tend_faces<SRC, true, true, PREC>), g, b, l, st, a, s);
     else if (b3)
-        hipLaunchKernelGGL((k_tend_faces<SRC, true, false, PREC>), g, b, 0, st, a, s);
+        hipLaunchKernelGGL((k_tend_faces<SRC, true, false, PREC>), g, b, l, st, a, s);
     else if (s.diff)
-        hipLaunchKernelGGL((k_tend_faces<SRC, false, true, PREC>), g, b, 0, st, a, s);
+        hipLaunchKernelGGL((k_tend_faces<SRC, false, true, PREC>), g, b, l, st, a, s);
     else
-        hipLaunchKernelGGL((k_tend_faces<SRC, false, false, PREC>), g, b, 0, st, a, s);
+        hipLaunchKernelGGL((k_tend_faces<SRC, false, false, PREC>), g, b, l, st, a, s);
 }
 
 template <class SRC, bool MOIST, bool PREC>
@@ -1924,16 +2349,16 @@ static void launch_tend_cols(const typename SRC::Arg& a, dim3 g, const Sc& s,
 template <class SRC, bool PREC>
 static void launch_finish(const typename SRC::Arg& a, dim3 g,
                           const FinishDst& o, bool b3, bool eos,
-                          hipStream_t st)
+                          hipStream_t st, real hdt = R(0.0))
 {
     if (b3 && eos)
-        hipLaunchKernelGGL((k_finish<SRC, true, true, PREC>), g, dim3(256), 0, st, a, o);
+        hipLaunchKernelGGL((k_finish<SRC, true, true, PREC>), g, dim3(256), 0, st, a, o, hdt);
     else if (b3)
-        hipLaunchKernelGGL((k_finish<SRC, true, false, PREC>), g, dim3(256), 0, st, a, o);
+        hipLaunchKernelGGL((k_finish<SRC, true, false, PREC>), g, dim3(256), 0, st, a, o, hdt);
     else if (eos)
-        hipLaunchKernelGGL((k_finish<SRC, false, true, PREC>), g, dim3(256), 0, st, a, o);
+        hipLaunchKernelGGL((k_finish<SRC, false, true, PREC>), g, dim3(256), 0, st, a, o, hdt);
     else
-        hipLaunchKernelGGL((k_finish<SRC, false, false, PREC>), g, dim3(256), 0, st, a, o);
+        hipLaunchKernelGGL((k_finish<SRC, false, false, PREC>), g, dim3(256), 0, st, a, o, hdt);
 }
 
 hipError_t run_tend_faces(const Dom& d, const Sc& s, hipStream_t st)
@@ -1965,7 +2390,12 @@ static void launch_uv(const Dom& d, const Sc& s, hipStream_t st)
 {
     const int ne = (int)xcd_grid((unsigned)((d.ny + d.nx + 255) / 256));
     const dim3 g(rowseg_grid(d.nx, d.ny).x + ne);
-    hipLaunchKernelGGL((k_uv<One, B3, USEP>), g, dim3(64, 4), 0, st, d, s, ne);
+    if (s.sum)
+        hipLaunchKernelGGL((k_uv<One, B3, USEP, true>), g, dim3(64, 4), 0, st,
+                           d, s, ne);
+    else
+        hipLaunchKernelGGL((k_uv<One, B3, USEP>), g, dim3(64, 4), 0, st, d, s,
+                           ne);
 }
 
 hipError_t run_uv(const Dom& d, const Sc& s, hipStream_t st)
@@ -2029,16 +2459,57 @@ hipError_t run_column(const Dom& d, const Sc& s, real* pout, int write_al,
 }
 
 hipError_t run_finish(const Dom& d, const FinishDst& o, bool prec,
-                      hipStream_t st)
+                      hipStream_t st, double hdiab_dt)
 {
     // a moist state's EOS waits for the scalar update of the stage
     const bool eos = d.qv == nullptr;
+    const real hdt = d.h_diabatic ? (real)hdiab_dt : R(0.0);
     with_prec(prec, [&](auto p) {
         launch_finish<One, decltype(p)::value>(d, cols(d), o, d.base3d != 0,
-                                               eos, st);
+                                               eos, st, hdt);
     });
     hipLaunchKernelGGL(k_finish_edge<One>, dim3((d.ny + d.nx + 255) / 256),
                        dim3(256), 0, st, d, o);
+    return hipGetLastError();
+}
+
+hipError_t run_scalar_cols(const Dom& d, const QGroup& g, const real* mup0,
+                           real dt, real rdx, real rdy, int clamp,
+                           hipStream_t st)
+{
+    const dim3 grid = rowseg_grid(d.nx, d.ny), blk(64, 4);
+    switch (g.n) {
+    case 1:
+        hipLaunchKernelGGL(k_scalar_cols<1>, grid, blk, 0, st, d, g, mup0, dt,
+                           rdx, rdy, clamp);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_scalar_cols<2>, grid, blk, 0, st, d, g, mup0, dt,
+                           rdx, rdy, clamp);
+        break;
+    case 3:
+        hipLaunchKernelGGL(k_scalar_cols<3>, grid, blk, 0, st, d, g, mup0, dt,
+                           rdx, rdy, clamp);
+        break;
+    case 4:
+        hipLaunchKernelGGL(k_scalar_cols<4>, grid, blk, 0, st, d, g, mup0, dt,
+                           rdx, rdy, clamp);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t run_pd_cols(const Dom& d, const real* cur, const real* q0,
+                       real* out, const real* mup0, real dx, real dy,
+                       real dxi, real dyi, real dt, hipStream_t st)
+{
+    const dim3 grid = rowseg_grid(d.nx, d.ny), blk(64, 4);
+    hipLaunchKernelGGL(k_pd_scale_cols, grid, blk, 0, st, d, cur, q0, mup0,
+                       d.pd_sc, dx, dy, dxi, dyi, dt);
+    hipLaunchKernelGGL(k_pd_apply_cols, grid, blk, 0, st, d, cur, q0, mup0,
+                       d.pd_sc, out, dx, dy, dxi, dyi, dt);
     return hipGetLastError();
 }
 

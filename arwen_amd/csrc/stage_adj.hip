@@ -1007,7 +1007,7 @@ extern "C" int32_t ARW_API(small_step_adjoint_work)(int32_t nz, int32_t ny,
 extern "C" int32_t ARW_API(small_step_init_adjoint)(
     const Dom* d, const ARW_API(init_adj)* adj, real* work, void* stream)
 {
-    int32_t rc = lib_check_domain(d);
+    int32_t rc = lib_check_adj_domain(d);
     if (rc) return rc;
     if (!adj || !work) return lib_fail(ARW_E_ARG, "null argument");
     const real* lin[] = {adj->l_u_pp, adj->l_v_pp, adj->l_w_pp, adj->l_th_pp,
@@ -1057,7 +1057,7 @@ extern "C" int32_t ARW_API(small_step_init_adjoint)(
 extern "C" int32_t ARW_API(small_step_finish_adjoint)(
     const Dom* d, const ARW_API(finish_adj)* adj, real* work, void* stream)
 {
-    int32_t rc = lib_check_domain(d);
+    int32_t rc = lib_check_adj_domain(d);
     if (rc) return rc;
     if (!adj || !work) return lib_fail(ARW_E_ARG, "null argument");
     const real* lin[] = {adj->l_u, adj->l_v, adj->l_w, adj->l_thp, adj->l_php,
@@ -1103,7 +1103,7 @@ extern "C" int32_t ARW_API(stage_tendencies_adjoint)(
     const Dom* d, const arw_params* prm, const ARW_API(stage_adj)* adj,
     real* work, void* stream)
 {
-    int32_t rc = lib_check_domain(d);
+    int32_t rc = lib_check_adj_domain(d);
     if (rc) return rc;
     if (!prm || !adj || !work) return lib_fail(ARW_E_ARG, "null argument");
     if (prm->diff_6th_opt != 0)

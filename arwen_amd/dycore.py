@@ -72,16 +72,22 @@ def _check_state(state: DomainState) -> None:
 
 def step(state: DomainState, cfg: RunConfig, *, acoustic: bool = True,
          mass_flux_observer=None, mass_flux_accumulator=None,
-         refl_10cm_due: bool = False) -> None:
-    """Advance ``state`` one RK3 split-explicit step of length cfg.dt."""
+         refl_10cm_due: bool = False, microphysics=None) -> None:
+    """Advance ``state`` one RK3 split-explicit step of length cfg.dt.
+
+    With ``cfg.mp_physics != 0`` the reference's step ends with its
+    microphysics scheme and a diagnostics refresh (dycore.py:2491-2502).
+    The scheme is not part of this library: the RK3 dynamics here carry
+    the scheme's species and apply and remove the retained heating, and
+    ``microphysics`` (the reference's ``apply_microphysics``, called as
+    ``microphysics(state, cfg, cfg.dt, refl_10cm_due=...)``) runs in that
+    slot when given, followed by the EOS refresh."""
     if mass_flux_observer is not None and mass_flux_accumulator is not None:
         raise ValueError(
             "mass_flux_observer and mass_flux_accumulator are mutually "
             "exclusive")
     if not acoustic:
-        raise NotImplementedError(
-            "step(acoustic=False) (the advection-only test path) is not "
-            "part of this port")
+        return _step_advective(state, cfg)
     if mass_flux_observer is not None or mass_flux_accumulator is not None:
         raise NotImplementedError(
             "boundary mass-flux observers apply to open boundaries, which "
@@ -98,6 +104,41 @@ def step(state: DomainState, cfg: RunConfig, *, acoustic: bool = True,
     p = params(cfg)
     name, fn = _fn(state, "step")
     _lib.check(fn(d, p, _stream()), name)
+    if microphysics is not None and cfg.mp_physics != 0:
+        microphysics(state, cfg, cfg.dt, refl_10cm_due=refl_10cm_due)
+        update_diagnostics(state, cfg.hypsometric_opt)
+    state.elapsed_seconds += cfg.dt
+
+
+def _step_advective(state: DomainState, cfg: RunConfig) -> None:
+    """The reference's advection-only step, step(acoustic=False)
+    (dycore.py:2318-2343): coupled flux-form advection of theta, u, v, w
+    with Omega = 0 plus constant-K diffusion, three stages from the
+    time-t fields with w, phi' and mu' frozen.  Dry states with w == 0,
+    like the reference (advection.py:151-156)."""
+    if state.qv is not None or cfg.moist:
+        raise NotImplementedError(
+            "moist transport requires the acoustic dycore path "
+            "(step(acoustic=False) is the dry advection test path)")
+    if cfg.km_opt in (2, 3, 4):
+        raise NotImplementedError(
+            f"km_opt={cfg.km_opt} Smagorinsky mixing requires the acoustic "
+            "dycore path (step(acoustic=False) is the dry advection test "
+            "path)")
+    validate_for_step(cfg, acoustic=False)
+    if cfg.precision_form or cfg.diff_6th_opt:
+        raise NotImplementedError(
+            "step(acoustic=False) is the plain-form advection test path "
+            "(no precision form, no 6th-order filter)")
+    _check_state(state)
+    if bool(state.w.any()):
+        raise ValueError(
+            "step(acoustic=False) (advection-only path) requires w == 0: "
+            "its eta mass flux is zero, which is the flux only for w == 0; "
+            "the full step advects with the diagnosed Omega")
+    d = state.domain_struct(diffusion=_uses_diffusion(cfg))
+    name, fn = _fn(state, "step_advective")
+    _lib.check(fn(d, params(cfg), _stream()), name)
     state.elapsed_seconds += cfg.dt
 
 

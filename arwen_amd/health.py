@@ -39,8 +39,10 @@ STATUS_BITS = {
     "specific_volume": 1 << 4,
     "theta": 1 << 5,
     "moisture": 1 << 6,
+    "moment": 1 << 7,
     "held_tendency": 1 << 9,
     "generic": 1 << 15,
+    "particle_volume": 1 << 16,
 }
 
 MAX_HEALTH_FIELDS = 1024
@@ -137,8 +139,15 @@ def rule_for_field(name: str) -> FieldRule:
         return FieldRule("specific_volume", -1.0e4, 1.0e4)
     if leaf == "alt":
         return FieldRule("specific_volume", 0.0, 1.0e4, strict_lower=True)
-    if leaf in ("qv", "qc", "qr"):
+    if leaf in ("qv", "qc", "qr", "qi", "qs", "qg", "qh"):
         return FieldRule("moisture", 0.0, 1.0)
+    if leaf in ("nc", "nr", "ni", "ns", "ng", "qndrop", "qnr", "qni", "qns",
+                "qng", "qnh", "qnn", "nwfa", "nifa"):
+        # number moments per kilogram: a blow-up ceiling, not the
+        # schemes' own bounds
+        return FieldRule("moment", 0.0, 1.0e15)
+    if leaf in ("qvolg", "qvolh"):
+        return FieldRule("particle_volume", 0.0, 1.0)
     return _FINITE
 
 
@@ -152,7 +161,9 @@ def field_from_array(name: str, values: Any, *, auxiliary: Any = None,
 #: the cross-step fields of the state (the restart manifest's serialized
 #: set), in gate order
 STATE_FIELDS = ("u", "v", "w", "thp", "php", "mup", "p", "al", "alt",
-                "qv", "qc", "qr", "h_diabatic")
+                "qv", "qc", "qr", "qi", "qs", "qg", "nc", "nr", "ni", "ns",
+                "ng", "nwfa", "nifa", "qh", "qndrop", "qnr", "qni", "qns",
+                "qng", "qnh", "qnn", "qvolg", "qvolh", "h_diabatic")
 
 
 def collect_state_fields(state: Any) -> tuple:

@@ -41,22 +41,26 @@ from pathlib import Path
 import numpy as np
 import torch
 
-from .config import RunConfig
+from .config import EXTRA_SPECIES, RunConfig
 from .state import DomainState
 
 #: Bump on any change to the member layout or the classification tables;
 #: readers refuse other versions.
-RESTART_FORMAT_VERSION = 1
+RESTART_FORMAT_VERSION = 2
 
 _HEADER_KEY = "__header__"
 
 #: Cross-step state written to the file (present ones only: the moist
 #: scalars exist on moist states).
+#: every mixed-phase scheme's transported species, in registry order
+_EXTRA = tuple(dict.fromkeys(q for mp in sorted(EXTRA_SPECIES)
+                             for q in EXTRA_SPECIES[mp]))
+
 STATE_SERIALIZED_ATTRS = (
     "u", "v", "w", "thp", "php", "mup",
     "p", "al", "alt",
     "qv", "qc", "qr", "h_diabatic",
-)
+) + _EXTRA
 
 #: Overwritten before every read.  The step rotates its stage buffers
 #: through the *0 arrays, writes every tendency and perturbation of a stage
@@ -66,7 +70,7 @@ STATE_REBUILT_ATTRS = frozenset({
     "ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t",
     "u_pp", "v_pp", "w_pp", "th_pp", "ph_pp", "mu_pp",
     "p_pp", "p_pp_old", "ww_pp", "al_pp",
-})
+}) | frozenset(q + "0" for q in _EXTRA)
 
 #: Deterministic setup arrays and scalars covered by the fingerprint.
 STATE_SETUP_ARRAYS = (
@@ -84,7 +88,7 @@ STATE_SETUP_SCALARS = (
 #: by the preparation (host copies of the setup, caches, scratch pool).
 STATE_INFRA_ATTRS = frozenset({
     "cfg", "device", "dtype", "elapsed_seconds", "_scratch", "_phb_host",
-    "_coord", "_base", "_diff_rdz",
+    "_coord", "_base", "_diff_rdz", "_dom_memo",
 })
 
 #: Scratch slots outside the library's own work areas (prefix ``arw_``,

@@ -12,9 +12,9 @@ import numpy as np
 import torch
 
 from . import constants as c
-from ._lib import (DIFF_ARRAYS, DOMAIN_ARRAYS, MOIST_ARRAYS, MOIST_SCRATCH,
-                   SCRATCH_ARRAYS, ArwDomain, ArwDomain64)
-from .config import RunConfig
+from ._lib import (DIFF_ARRAYS, DOMAIN_ARRAYS, MAX_EXTRA, MOIST_ARRAYS,
+                   MOIST_SCRATCH, SCRATCH_ARRAYS, ArwDomain, ArwDomain64)
+from .config import EXTRA_SPECIES, LOADING_EXTRA, RunConfig
 from .grid import BaseState, VerticalCoord, surface_extrapolation
 
 DTYPE = torch.float32          # the step's precision; FP64 is the reference build
@@ -60,6 +60,15 @@ class DomainState:
                 setattr(self, q, z(nz, ny, nx))
                 setattr(self, q + "0", z(nz, ny, nx))
             self.h_diabatic = z(nz, ny, nx)
+            # a mixed-phase scheme's transported species (reference
+            # state.py:428-470; the scheme runs in the caller's physics)
+            for q in EXTRA_SPECIES[cfg.mp_physics]:
+                setattr(self, q, z(nz, ny, nx))
+                setattr(self, q + "0", z(nz, ny, nx))
+            if cfg.mp_physics == 18:
+                # NSSL predicted CCN starts at nssl_cccn / 1.225 (reference
+                # state.py:447-452)
+                self.qnn.fill_(408163264.0)
         else:
             self.qv = self.qc = self.qr = None
             self.qv0 = self.qc0 = self.qr0 = None
@@ -137,6 +146,13 @@ class DomainState:
             setattr(self, name, float(self.np_dtype.type(val)))
 
     @property
+    def extra_species(self) -> tuple:
+        """The transported species beyond qv/qc/qr this state carries."""
+        if self.qv is None:
+            return ()
+        return EXTRA_SPECIES[self.cfg.mp_physics]
+
+    @property
     def np_dtype(self) -> np.dtype:
         return np.dtype(np.float64 if self.dtype == torch.float64
                         else np.float32)
@@ -161,7 +177,9 @@ class DomainState:
             raise RuntimeError("host_case() called before load_base()")
         names = ["u", "v", "w", "thp", "php", "mup"]
         if self.qv is not None:
-            names += ["qv", "qc", "qr"]
+            names += ["qv", "qc", "qr"] + list(self.extra_species)
+            if self.cfg.mp_physics != 0:
+                names.append("h_diabatic")
         fields = {n: getattr(self, n).detach().cpu().numpy().astype(
             np.float64) for n in names}
         return HostCase(self.cfg, self._coord, self._base, fields)
@@ -176,6 +194,7 @@ class DomainState:
             buf = torch.zeros(shape, dtype=dtype or self.dtype,
                               device=self.device)
             self._scratch[slot] = buf
+            self.__dict__.pop("_dom_memo", None)
         elif tuple(buf.shape) != shape:
             raise ValueError(f"scratch slot {slot!r} has shape "
                              f"{tuple(buf.shape)}, requested {shape}")
@@ -220,10 +239,30 @@ class DomainState:
             for a in (rdzf_h, rdzc_h, rdzf_w, rdzc_w)]
         return self._diff_rdz
 
+    def __setattr__(self, name, value):
+        # rebinding an array invalidates the cached raw-pointer views
+        if isinstance(value, torch.Tensor) or value is None:
+            self.__dict__.pop("_dom_memo", None)
+        object.__setattr__(self, name, value)
+
     def domain_struct(self, diffusion: bool = False,
                       precision_form: bool = False,
                       diff6: bool = False) -> ArwDomain:
-        """Raw-pointer view of the state for the C-ABI."""
+        """Raw-pointer view of the state for the C-ABI (a fresh copy the
+        caller may modify).  Built once per option set and cached until an
+        array attribute is rebound or a scratch slot is created; arrays
+        updated in place keep their pointers."""
+        key = (bool(diffusion), bool(precision_form), bool(diff6))
+        memo = self.__dict__.get("_dom_memo")
+        if memo is not None and key in memo:
+            m = memo[key]
+            return type(m).from_buffer_copy(m)
+        d = self._build_domain_struct(*key)
+        self.__dict__.setdefault("_dom_memo", {})[key] = d
+        return type(d).from_buffer_copy(d)
+
+    def _build_domain_struct(self, diffusion: bool, precision_form: bool,
+                             diff6: bool) -> ArwDomain:
         nz, ny, nx = self.shape
         d = ArwDomain64() if self.dtype == torch.float64 else ArwDomain()
         d.nz, d.ny, d.nx = nz, ny, nx
@@ -260,6 +299,18 @@ class DomainState:
             for name in MOIST_SCRATCH:
                 setattr(d, name, self.scratch(shapes[name],
                                               "arw_" + name).data_ptr())
+        extra = self.extra_species
+        if len(extra) > MAX_EXTRA:
+            raise ValueError(f"{len(extra)} extra species exceed the "
+                             f"ABI's {MAX_EXTRA}")
+        d.n_extra = len(extra)
+        d.n_load = LOADING_EXTRA[self.cfg.mp_physics] if extra else 0
+        for k, name in enumerate(extra):
+            d.qx[k] = getattr(self, name).data_ptr()
+            d.qx0[k] = getattr(self, name + "0").data_ptr()
+            d.qxs[k] = self.scratch((nz, ny, nx), "arw_qs_" + name).data_ptr()
+        if self.qv is not None and self.cfg.mp_physics != 0:
+            d.h_diabatic = self.h_diabatic.data_ptr()
         if precision_form and self.dtype == torch.float32:
             d.tri64 = self.scratch((2, nz + 1, ny, nx), "arw_tri64",
                                    dtype=torch.float64).data_ptr()

@@ -58,6 +58,12 @@ def parse():
     ap.add_argument("--diff6", type=int, default=0, choices=(0, 1, 2),
                     help="c3/c2: run with WRF's held 6th-order filter "
                          "(diff_6th_opt; factor 0.12) to time its cost")
+    ap.add_argument("--species", type=int, default=-1,
+                    choices=(-1, 0, 6, 8, 10, 18, 28),
+                    help="c3/c2: run moist, carrying the species of this "
+                         "mp_physics scheme (and, for mp > 0, a retained "
+                         "heating); a throughput line for the scalar "
+                         "family, without extras or CPU baseline")
     ap.add_argument("--no-extras", action="store_true",
                     help="c3/c2: leave out the compact adjoint (C4) and "
                          "ensemble (C5) objects measured after the timed "
@@ -335,6 +341,16 @@ def main():
     from arwen_amd.state import DomainState
 
     case, desc = build_case(args.workload)
+    if args.species >= 0:
+        from arwen_amd import cases as _cases
+        from arwen_amd.config import moist_species
+        _cases.add_species(case, args.species, qv_max=1.0e-3)
+        n_q = len(moist_species(case.cfg))
+        desc += (f", moist: mp_physics={args.species} species ({n_q} "
+                 "transported, PD on the final stage)"
+                 + (" and a retained heating" if args.species else ""))
+        args.no_extras = True
+        args.no_cpu_baseline = True
     cfg = case.cfg
     if args.diff6:
         cfg = cfg.with_(diff_6th_opt=args.diff6)
@@ -346,7 +362,7 @@ def main():
         dycore.step(state, cfg)
     torch.cuda.synchronize()
 
-    timer = _lib.KernelTimer(max_events=64 * max(args.steps, 1) + 64)
+    timer = _lib.KernelTimer(max_events=320 * max(args.steps, 1) + 64)
     t0 = time.perf_counter()
     with timer:
         for _ in range(args.steps):
@@ -361,6 +377,12 @@ def main():
     rep = dycore.stability_report(state, cfg)
     if rep["nan"]:
         raise SystemExit("bench.py: the state went NaN")
+    # the full-state health gate on the same state (outside the timed
+    # region): one pass over every checked field
+    from arwen_amd.health import StateHealthValidator
+    gate = StateHealthValidator(state)
+    gate.require_healthy(phase="bench")
+    t_gate = _time_ms(lambda: gate.require_healthy(phase="bench"), 5)
 
     pts = cfg.nx * cfg.ny * cfg.nz
     sec = (t1 - t0) / args.steps
@@ -377,7 +399,7 @@ def main():
     u_gbs = u_bytes / (u_avg_ms * 1e-3) / 1e9 if n_sub else None
     traffic = None
     tfile = args.traffic_file
-    if tfile is None and args.workload == "c3":
+    if tfile is None and args.workload == "c3" and args.species < 0:
         tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                              "profiles",
                              "dycore_step_1000x1000x50_pmc_traffic.txt")
@@ -419,6 +441,7 @@ def main():
         },
         "kernel_ms_per_step": {k: round(v / args.steps, 4)
                                for k, v in timer.ms.items() if v > 0},
+        "health_gate_ms": round(t_gate, 4),
     }
     del state
     torch.cuda.empty_cache()

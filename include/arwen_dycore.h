@@ -24,7 +24,10 @@
 extern "C" {
 #endif
 
-#define ARW_ABI_VERSION 6
+#define ARW_ABI_VERSION 7
+
+/* transported species beyond qv/qc/qr a domain can carry (NSSL's set) */
+#define ARW_MAX_EXTRA 13
 
 /* flags of the adjoint entry points that accumulate */
 #define ARW_ADJ_CLEAR 1
@@ -88,6 +91,20 @@ typedef struct arw_domain {
      * u, v, w, theta' (shapes of u, v, w, thp), formed once per step from
      * the time-t fields (reference dycore.py:1345-1420) */
     float *d6_u, *d6_v, *d6_w, *d6_t;
+    /* ABI 7: a moist state's transported species beyond qv/qc/qr (the
+     * mixed-phase schemes' ice masses and number moments, reference
+     * gpuwm/core/moist.py:107-240): their count, how many of them (the
+     * first, in order) add to the condensate loading of the buoyancy
+     * (reference dycore.cu:213-230: 3 for qi, qs, qg, 4 with qh), and per
+     * species the state array (time t on entry, new on exit), its time-t
+     * copy and a third stage buffer, all (nz,ny,nx). */
+    int32_t n_extra, n_load;
+    float *qx[ARW_MAX_EXTRA], *qx0[ARW_MAX_EXTRA], *qxs[ARW_MAX_EXTRA];
+    /* the retained microphysics heating rate (K/s, (nz,ny,nx)); NULL when
+     * mp_physics = 0.  Its coupled form is added to every stage's theta
+     * tendency and the final stage's finish removes what the substeps
+     * integrated (reference dycore.py:572-592, dycore.cu:1186-1194). */
+    float *h_diabatic;
 } arw_domain;
 
 /*
@@ -118,6 +135,9 @@ typedef struct arw64_domain {
     double *tri64;
     double *pp;                /* unused by the FP64 build */
     double *d6_u, *d6_v, *d6_w, *d6_t;
+    int32_t n_extra, n_load;
+    double *qx[ARW_MAX_EXTRA], *qx0[ARW_MAX_EXTRA], *qxs[ARW_MAX_EXTRA];
+    double *h_diabatic;
 } arw64_domain;
 
 /* Step configuration (the RunConfig fields the step reads,
@@ -157,12 +177,27 @@ const char *arw_last_error(void);
 
 /* Whole RK3 split-explicit step, in place (replaces gpuwm/core/dycore.py:
  * 2161-2503 ``step`` for periodic, msf = 1 states, dry or carrying the
- * qv/qc/qr scalars with mp_physics = 0: moist EOS and buoyancy, WRF
+ * qv/qc/qr scalars and any mixed-phase scheme's extra species (n_extra,
+ * with h_diabatic for mp_physics != 0): moist EOS and buoyancy, WRF
  * sumflux, the coupled scalar stage update of moist.py:434-698 and the
  * positive-definite limiter of pd_advection.cu on the final stage when
  * moist_adv_opt = 1).  Leaves p/al/alt consistent with the new state,
  * like the reference's closing update_diagnostics (dycore.py:2482). */
 int32_t arw_step(const arw_domain *d, const arw_params *prm, void *stream);
+
+/* The advection-only step, step(acoustic=False) (replaces
+ * gpuwm/core/dycore.py:2318-2343 with advection.py:134-182 and the stage
+ * update dycore.py:1602-1643): three stages of length dt/3, dt/2, dt of
+ * coupled flux-form advection of theta, u, v, w with Omega = 0 and the
+ * constant-K diffusion, each advanced from the time-t copies (u0..mup0,
+ * written here) with w, phi' and mu' frozen; then the kinematic lower
+ * boundary.  Dry, plain form, no 6th-order filter (ARW_E_UNSUPPORTED
+ * otherwise); the caller guarantees w == 0 (the reference raises
+ * ValueError otherwise).  u_pp, v_pp, ww_pp and th_pp serve as work
+ * arrays; p/al/alt are left at the last stage's EOS, as in the
+ * reference. */
+int32_t arw_step_advective(const arw_domain *d, const arw_params *prm,
+                           void *stream);
 
 /* arw_step with no workspace and no allocation at all.  No counterpart
  * in the reference: it is for callers that cannot allocate (code under
@@ -609,6 +644,8 @@ int32_t arw64_step(const arw64_domain *d, const arw_params *prm,
 /* arw_step_noalloc of the FP64 build: no counterpart in the reference,
  * for callers that cannot allocate; its own 64 MiB array per GPU (half
  * the points of the FP32 build's limit). */
+int32_t arw64_step_advective(const arw64_domain *d, const arw_params *prm,
+                             void *stream);
 int32_t arw64_step_noalloc(const arw64_domain *d, const arw_params *prm,
                            void *stream);
 int32_t arw64_health_gate(const arw_health_field *fields, int32_t nfields,

@@ -49,13 +49,18 @@ class OState(SimpleNamespace):
 
 
 def make_state(nz, ny, nx, coord, base, *, dtype=np.float32, moist=False,
-               cf=None) -> OState:
+               cf=None, extra=(), n_load=0, h_diabatic=False) -> OState:
     """Allocate an at-rest oracle state and load coordinate/base arrays
     (reference state.py:359-771 ``DomainState.__init__`` + ``load_base``).
 
     ``coord``/``base`` are FP64 host objects with the WRF names (znw, dnw,
     c1h, ..., thb, pb, alb, phb, mub, p_top, terrain_z); ``cf`` holds the
-    cf1..cf3/cfn/cfn1 extrapolation weights.
+    cf1..cf3/cfn/cfn1 extrapolation weights.  ``extra`` names a
+    mixed-phase scheme's transported species beyond qv/qc/qr, of which the
+    first ``n_load`` load the buoyancy (reference state.py:428-470,
+    moist.py:107-240); ``h_diabatic`` allocates the retained microphysics
+    heating that every stage adds to the theta tendency (reference
+    dycore.py:572-592).
     """
     f = np.dtype(dtype).type
     z = lambda *sh: np.zeros(sh, dtype=dtype)
@@ -74,12 +79,17 @@ def make_state(nz, ny, nx, coord, base, *, dtype=np.float32, moist=False,
     s.p_pp, s.p_pp_old, s.al_pp = z(nz, ny, nx), z(nz, ny, nx), z(nz, ny, nx)
     s.ww_pp = z(nz + 1, ny, nx)
     s.pp = z(nz, ny, nx)       # precision form: p' of the last EOS
+    if not moist and (extra or h_diabatic):
+        raise ValueError("extra species and h_diabatic need moist=True")
     if moist:
-        for q in ("qv", "qc", "qr"):
+        for q in ("qv", "qc", "qr") + tuple(extra):
             setattr(s, q, z(nz, ny, nx))
             setattr(s, q + "0", z(nz, ny, nx))
     else:
         s.qv = s.qc = s.qr = None
+    s.extra = tuple(extra)
+    s.n_load = int(n_load)
+    s.h_diabatic = z(nz, ny, nx) if h_diabatic else None
     for name in ("dnw", "rdnw", "dn", "rdn", "fnp", "fnm", "znu", "znw",
                  "c1h", "c2h", "c1f", "c2f", "c3h", "c4h", "c3f", "c4f"):
         setattr(s, name, np.asarray(getattr(coord, name), dtype=dtype))
@@ -435,7 +445,12 @@ def slow_pgf(s, dx, dy, top_lid=True, prec=False):
 
 
 def _q_total(s):
-    return (s.qv + s.qc) + s.qr
+    """qv + qc + qr, then the loading condensates in order (reference
+    dycore.cu:213-230 q_total: qi, qs, qg and NSSL's qh)."""
+    q = (s.qv + s.qc) + s.qr
+    for name in getattr(s, "extra", ())[:getattr(s, "n_load", 0)]:
+        q = q + getattr(s, name)
+    return q
 
 
 def slow_buoyancy(s, prec=False):
@@ -607,6 +622,16 @@ def add_slow_tendencies(s, ru, rv, ww, dx, dy, top_lid=True, prec=False):
     slow_pgf(s, dx, dy, top_lid, prec)
     slow_buoyancy(s, prec)
     slow_geopotential(s, ww, dx, dy, prec)
+
+
+def add_h_diabatic_tendency(s) -> None:
+    """The retained microphysics heating enters the coupled theta tendency
+    of every stage, rth_t += (c1h mu + c2h) h_diabatic with the stage
+    state's column mass (reference dycore.py:572-592, after the slow
+    tendencies and before diffusion, dycore.py:2381-2389)."""
+    mu = s.mub2d + s.mup
+    s.rth_t += (s.c1h[:, None, None] * mu[None]
+                + s.c2h[:, None, None]) * s.h_diabatic
 
 
 # ---------------------------------------------------------------------------
@@ -984,11 +1009,15 @@ def run_acoustic_only(s, cfg_like, dtau, n):
         acoustic_substep(s, coefs, dtau, cfg_like, first=(i == 0))
 
 
-def small_step_finish(s, prec=False) -> None:
+def small_step_finish(s, prec=False, hdiab_dt=0.0) -> None:
     """WRF small_step_finish (reference dycore.cu:1087-1198).  ``prec``
     (precision form): theta' = (C(mu_s) theta_s + theta'') / C(mu_n) - thb
     is formed without the base theta's cancellation, as (C(mu_s) thp +
-    theta'' - (c1h mu'') thb) / C(mu_n)."""
+    theta'' - (c1h mu'') thb) / C(mu_n).  A nonzero ``hdiab_dt`` (the final
+    stage of a step carrying h_diabatic) removes what the substeps
+    integrated of the heating: the numerator drops
+    (hdiab_dt C(mu_s)) h_diabatic (reference dycore.cu:1187-1195,
+    dycore.py:1583-1600)."""
     f = s.dtype.type
     half = f(0.5)
     ms = s.mub2d + s.mup
@@ -1012,7 +1041,10 @@ def small_step_finish(s, prec=False) -> None:
         num = ((c1h * ms + c2h) * s.thp + s.th_pp) - (c1h * s.mu_pp) * thb
         s.thp[...] = num / (c1h * mn + c2h)
     else:
-        num = (c1h * ms + c2h) * (thb + s.thp) + s.th_pp
+        cs = c1h * ms + c2h
+        num = cs * (thb + s.thp) + s.th_pp
+        if hdiab_dt:
+            num = num - (f(hdiab_dt) * cs) * s.h_diabatic
         s.thp[...] = num / (c1h * mn + c2h) - thb
     s.mup[...] = s.mup + s.mu_pp
 
@@ -1204,13 +1236,16 @@ def step(s, cfg) -> None:
         getattr(s, name + "0")[...] = getattr(s, name)
     moist = s.qv is not None
     if moist:
-        for q in ("qv", "qc", "qr"):
+        for q in ("qv", "qc", "qr") + getattr(s, "extra", ()):
             getattr(s, q + "0")[...] = getattr(s, q)
+    hdiab = getattr(s, "h_diabatic", None)
     ns = cfg.time_step_sound
     stages = ((1, cfg.dt / 3.0), (max(ns // 2, 1), cfg.dt / ns),
               (ns, cfg.dt / ns))
     prec = bool(getattr(cfg, "precision_form", False)) \
         and s.dtype == np.float32
+    if hdiab is not None and prec:
+        raise ValueError("the precision form carries no h_diabatic removal")
     held = diff6_held(s, cfg) if getattr(cfg, "diff_6th_opt", 0) > 0 \
         else {}
     for istage, (nsub, dtau) in enumerate(stages):
@@ -1219,6 +1254,8 @@ def step(s, cfg) -> None:
         calc_p_alpha(s, prec)
         ru, rv, ww = stage_fluxes(s, cfg.dx, cfg.dy, prec)
         add_slow_tendencies(s, ru, rv, ww, cfg.dx, cfg.dy, cfg.top_lid, prec)
+        if hdiab is not None:
+            add_h_diabatic_tendency(s)
         if cfg.km_opt == 1:
             add_diffusion_tendencies(s, cfg.khdif, cfg.kvdif, cfg.dx, cfg.dy)
         for name, h in held.items():       # held 6th-order tendencies
@@ -1235,7 +1272,10 @@ def step(s, cfg) -> None:
                 ru_m += s.u_pp
                 rv_m += s.v_pp
                 ww_m += s.ww_pp
-        small_step_finish(s, prec)
+        # the final stage removes the heating its substeps integrated,
+        # over its length nsub * dtau (= dt)
+        small_step_finish(s, prec, nsub * dtau
+                          if hdiab is not None and istage == 2 else 0.0)
         if moist:
             ru_m, rv_m, ww_m = sumflux_finish(ru_m, rv_m, ww_m, ru, rv, ww,
                                               nsub)
@@ -1244,6 +1284,66 @@ def step(s, cfg) -> None:
                                   getattr(cfg, "moist_adv_opt", 1))
     set_w_surface(s, cfg.dx, cfg.dy)
     calc_p_alpha(s, prec)
+    s.elapsed_seconds += cfg.dt
+
+
+def advance_stage_advective(s, dt_eff) -> None:
+    """Uncoupled update of theta, u, v, w from the time-t copies with the
+    stage's coupled tendencies, mu' frozen (reference dycore.py:1602-1643,
+    msf = 1)."""
+    f = s.dtype.type
+    dt_eff = f(dt_eff)
+    c1h, c2h = s.c1h[:, None, None], s.c2h[:, None, None]
+    c1f, c2f = s.c1f[:, None, None], s.c2f[:, None, None]
+    mu0 = s.mub2d + s.mup0
+    mu = s.mub2d + s.mup
+    thb = _b3(s.thb)
+    s.thp[...] = (((c1h * mu0[None] + c2h) * (thb + s.thp0)
+                   + dt_eff * s.rth_t) / (c1h * mu[None] + c2h)) - thb
+    mux0, mux = mu_at_u_faces(mu0), mu_at_u_faces(mu)
+    s.u[...] = (((c1h * mux0[None] + c2h) * s.u0 + dt_eff * s.ru_t)
+                / (c1h * mux[None] + c2h))
+    muy0, muy = mu_at_v_faces(mu0), mu_at_v_faces(mu)
+    s.v[...] = (((c1h * muy0[None] + c2h) * s.v0 + dt_eff * s.rv_t)
+                / (c1h * muy[None] + c2h))
+    s.w[...] = (((c1f * mu0[None] + c2f) * s.w0 + dt_eff * s.rw_t)
+                / (c1f * mu[None] + c2f))
+
+
+def step_advective(s, cfg) -> None:
+    """The reference's advection-only step, step(acoustic=False)
+    (dycore.py:2318-2343): three stages of length dt/3, dt/2, dt, each the
+    EOS, the coupled flux-form advection of theta, u, v, w with Omega = 0
+    (advection.py:134-182; w must be zero) and the constant-K diffusion,
+    advanced from the time-t fields with w, phi' and mu' frozen; then the
+    kinematic lower boundary.  Dry periodic states only."""
+    if s.qv is not None:
+        raise NotImplementedError("the advection-only step is dry")
+    f = s.dtype.type
+    for name in PROGNOSTICS:
+        getattr(s, name + "0")[...] = getattr(s, name)
+    nz = s.thp.shape[0]
+    dxi, dyi = f(1.0 / cfg.dx), f(1.0 / cfg.dy)
+    c1h, c2h = s.c1h[:, None, None], s.c2h[:, None, None]
+    for dt_eff in (cfg.dt / 3.0, cfg.dt / 2.0, cfg.dt):
+        for name in ("ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t"):
+            getattr(s, name)[...] = 0.0
+        calc_p_alpha(s)
+        if np.any(s.w):
+            raise ValueError("the advection-only step needs w == 0")
+        mu = s.mub2d + s.mup
+        ru = (c1h * mu_at_u_faces(mu)[None] + c2h) * s.u
+        rv = (c1h * mu_at_v_faces(mu)[None] + c2h) * s.v
+        rw = np.zeros((nz + 1,) + mu.shape, dtype=s.dtype)
+        s.rth_t += flux_div_scalar(_b3(s.thb) + s.thp, ru, rv, rw, s.rdnw,
+                                   s.fnm, s.fnp, dxi, dyi)
+        s.ru_t += flux_div_u(s.u, ru, rv, rw, s.rdnw, s.fnm, s.fnp, dxi, dyi)
+        s.rv_t += flux_div_v(s.v, ru, rv, rw, s.rdnw, s.fnm, s.fnp, dxi, dyi)
+        s.rw_t += flux_div_w(s.w, ru, rv, rw, s.rdn, s.fnm, s.fnp, dxi, dyi)
+        if cfg.km_opt == 1:
+            add_diffusion_tendencies(s, cfg.khdif, cfg.kvdif, cfg.dx, cfg.dy)
+        advance_stage_advective(s, dt_eff)
+    set_w_surface(s, cfg.dx, cfg.dy)
     s.elapsed_seconds += cfg.dt
 
 
@@ -1282,8 +1382,13 @@ def from_case(case, dtype=np.float32) -> OState:
     """Oracle state from a host initial condition (``arwen_amd.cases``
     HostCase): allocate, load base, cast the initial fields."""
     cfg = case.cfg
+    from oracle.moist_np import EXTRA_SPECIES, LOADING_EXTRA
+    moist = "qv" in case.fields
+    mp = getattr(cfg, "mp_physics", 0) if moist else 0
     s = make_state(cfg.nz, cfg.ny, cfg.nx, case.coord, case.base,
-                   dtype=dtype, moist="qv" in case.fields, cf=case.cf)
+                   dtype=dtype, moist=moist, cf=case.cf,
+                   extra=EXTRA_SPECIES[mp], n_load=LOADING_EXTRA[mp],
+                   h_diabatic=mp != 0)
     for name, arr in case.fields.items():
         getattr(s, name)[...] = np.asarray(arr, dtype=dtype)
     return s

@@ -21,6 +21,23 @@ from oracle import dycore_np as D
 
 SPECIES = ("qv", "qc", "qr")
 
+# The species each mixed-phase scheme transports beyond qv/qc/qr, in the
+# reference's registry order (moist.py:107-240 extra_moist_species; the
+# allocation state.py:428-470), and how many of them (the leading condensate
+# masses) enter the buoyancy's total water (dycore.cu:213-230 q_total and
+# dycore.py:398-414: qi, qs, qg for moist_mode 2, plus qh for NSSL).
+_ICE = ("qi", "qs", "qg")
+EXTRA_SPECIES = {
+    0: (), 1: (),
+    6: _ICE,
+    8: _ICE + ("nr", "ni"),
+    10: _ICE + ("nr", "ni", "ns", "ng"),
+    18: ("qi", "qs", "qg", "qh", "qndrop", "qnr", "qni", "qns", "qng",
+         "qnh", "qnn", "qvolg", "qvolh"),
+    28: _ICE + ("nr", "ni", "nc", "nwfa", "nifa"),
+}
+LOADING_EXTRA = {0: 0, 1: 0, 6: 3, 8: 3, 10: 3, 18: 4, 28: 3}
+
 
 def _pm(a, shift, axis):
     """a[(i + shift) mod n] along axis (periodic neighbour)."""
@@ -203,7 +220,7 @@ def advance_scalars_stage(s, ru_m, rv_m, ww_m, dt_stage, final, dx, dy,
     dx_inv = float(np.real(f(1.0 / dx)))
     dy_inv = float(np.real(f(1.0 / dy)))
     pd = final and moist_adv_opt == 1
-    for name in SPECIES:
+    for name in SPECIES + tuple(getattr(s, "extra", ())):
         q, q0 = getattr(s, name), getattr(s, name + "0")
         if pd:
             fl = pd_fluxes(q, q0, ru_m, rv_m, ww_m, mu, s.c1h, s.c2h,

@@ -49,6 +49,25 @@ def test_abi_version_matches_binding():
     assert lib.arw_abi_version() == ver == _lib.ABI_VERSION
 
 
+def test_species_table_size_matches_header_and_schemes():
+    """The domain's extra-species pointer tables hold every scheme's set."""
+    from arwen_amd.config import EXTRA_SPECIES, MAX_EXTRA_SPECIES
+    text = HEADER.read_text()
+    n = int(re.search(r"#define ARW_MAX_EXTRA (\d+)", text).group(1))
+    assert n == _lib.MAX_EXTRA == MAX_EXTRA_SPECIES
+    assert max(len(v) for v in EXTRA_SPECIES.values()) <= n
+
+
+def test_adjoint_refuses_mixed_phase_configs():
+    from arwen_amd import adjoint
+    from arwen_amd.state import DomainState
+    from arwen_amd import cases
+    case = cases.mixed_phase(6, nx=8, ny=6, nz=5)
+    st = DomainState.from_host(case, device="cpu")
+    with pytest.raises(_lib.ArwenError, match="mp_physics"):
+        adjoint._check_adjoint_cfg(st, case.cfg)
+
+
 def _struct_fields(name):
     text = HEADER.read_text()
     body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name),
@@ -65,8 +84,12 @@ def _struct_fields(name):
         for part in rest.split(","):
             part = part.strip()
             ptr = part.startswith("*")
-            fields.append((part.lstrip("*").strip(),
-                           "ptr" if ptr else base))
+            name = part.lstrip("*").strip()
+            arr = re.match(r"(\w+)\[(\w+)\]", name)
+            if arr:                       # pointer table: name[ARW_MAX_EXTRA]
+                fields.append((arr.group(1), "ptr[]"))
+                continue
+            fields.append((name, "ptr" if ptr else base))
     return fields
 
 
@@ -81,7 +104,8 @@ def _struct_fields(name):
                                        ("arw64_finish_adj", _lib.ArwFinishAdj64)])
 def test_ctypes_layout_matches_header(cname, pyt):
     kinds = {"ptr": ctypes.c_void_p, "int32_t": ctypes.c_int32,
-             "float": ctypes.c_float, "double": ctypes.c_double}
+             "float": ctypes.c_float, "double": ctypes.c_double,
+             "ptr[]": ctypes.c_void_p * _lib.MAX_EXTRA}
     want = [(n, kinds[k]) for n, k in _struct_fields(cname)]
     have = [(n, t) for n, t in pyt._fields_]
     assert [n for n, _ in have] == [n for n, _ in want]

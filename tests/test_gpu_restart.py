@@ -2,8 +2,8 @@
 prepared state restored from it and N more steps equal 2N uninterrupted
 steps bit for bit on every serialized field and the clock (reference
 gpuwm/io/restart.py:1-10, its 6 h + restart + 6 h == 12 h gate), dry on a
-flat and a terrain base with the upper damper, and moist with the PD
-limiter."""
+flat and a terrain base with the upper damper, moist with the PD
+limiter, and an NSSL state with every species and retained heating."""
 import numpy as np
 import pytest
 import torch
@@ -21,10 +21,12 @@ def _cases():
         "terrain_damped": cases.stratified_hill(nx=36, ny=30, nz=24,
                                                 seed=9, radius=40000.0),
         "moist_pd": cases.moist_tracer(nx=32, ny=24, nz=20),
+        "nssl_heated": cases.mixed_phase(18, nx=32, ny=24, nz=20),
     }
 
 
-@pytest.mark.parametrize("name", ["dry_flat", "terrain_damped", "moist_pd"])
+@pytest.mark.parametrize("name", ["dry_flat", "terrain_damped", "moist_pd",
+                                  "nssl_heated"])
 def test_restart_continues_bitwise(name, tmp_path):
     from arwen_amd import dycore
     case = _cases()[name]
@@ -42,7 +44,7 @@ def test_restart_continues_bitwise(name, tmp_path):
     torch.cuda.synchronize()
     assert resumed.elapsed_seconds == whole.elapsed_seconds
     for n in R.STATE_SERIALIZED_ATTRS:
-        a, b = getattr(whole, n), getattr(resumed, n)
+        a, b = getattr(whole, n, None), getattr(resumed, n, None)
         if a is None:
             assert b is None
             continue

@@ -92,7 +92,22 @@ def test_state_inventory_follows_the_restart_manifest():
     names = [f.name for f in collect_state_fields(dry)]
     assert names == ["u", "v", "w", "thp", "php", "mup", "p", "al", "alt"]
     mnames = [f.name for f in collect_state_fields(moist)]
-    assert set(mnames) == {n for n in restart.STATE_SERIALIZED_ATTRS}
+    assert set(mnames) == {n for n in restart.STATE_SERIALIZED_ATTRS
+                           if getattr(moist, n, None) is not None}
+    assert set(mnames) == {"u", "v", "w", "thp", "php", "mup", "p", "al",
+                           "alt", "qv", "qc", "qr", "h_diabatic"}
+    # an NSSL state: every transported species is gated, with the
+    # reference's moisture / moment / particle-volume classes
+    nssl = DomainState.from_host(cases.mixed_phase(18, nx=8, ny=6, nz=5),
+                                 device="cpu")
+    nf = {f.name: f for f in collect_state_fields(nssl)}
+    assert set(nf) == {n for n in restart.STATE_SERIALIZED_ATTRS
+                       if getattr(nssl, n, None) is not None}
+    assert set(nssl.extra_species) <= set(nf)
+    assert nf["qh"].rule.status_class == "moisture"
+    assert nf["qnn"].rule.status_class == "moment"
+    assert nf["qnn"].rule.upper == 1.0e15
+    assert nf["qvolg"].rule.status_class == "particle_volume"
     by = {f.name: f for f in collect_state_fields(dry)}
     assert by["thp"].aux_mode == "level" and by["thp"].plane_size == 48
     assert by["mup"].aux_mode == "direct"

@@ -15,15 +15,20 @@ from arwen_amd import restart as R
 from arwen_amd.state import DomainState
 
 
-def _state(moist=False, seed=3):
-    case = (cases.moist_tracer(nx=12, ny=10, nz=8) if moist
-            else cases.ensemble_member(12, 10, 8, seed))
+def _state(moist=False, seed=3, mp=0):
+    if mp:
+        case = cases.mixed_phase(mp, nx=12, ny=10, nz=8)
+    elif moist:
+        case = cases.moist_tracer(nx=12, ny=10, nz=8)
+    else:
+        case = cases.ensemble_member(12, 10, 8, seed)
     return case, DomainState.from_host(case, device="cpu")
 
 
 def test_every_attribute_is_classified():
-    for moist in (False, True):
-        _, st = _state(moist)
+    for moist, mp in ((False, 0), (True, 0), (True, 10), (True, 18),
+                      (True, 28)):
+        _, st = _state(moist, mp=mp)
         for name in vars(st):
             assert R.classify_state_attr(name) in (
                 "serialized", "rebuilt", "setup", "infrastructure")
@@ -40,8 +45,9 @@ def test_unclassified_attribute_and_slot_refuse_the_write(tmp_path):
         R.write_restart(tmp_path / "a.npz", st, case.cfg)
 
 
-def test_round_trip_is_bit_exact(tmp_path):
-    case, a = _state(moist=True)
+@pytest.mark.parametrize("mp", [0, 18])
+def test_round_trip_is_bit_exact(tmp_path, mp):
+    case, a = _state(moist=True, mp=mp)
     special = np.array([-0.0, 1e-45, np.float32(np.nan), -np.inf],
                        dtype=np.float32)
     # a NaN with a payload: a round trip through arithmetic would lose it
@@ -52,11 +58,14 @@ def test_round_trip_is_bit_exact(tmp_path):
     a.elapsed_seconds = 1234.5
     path = R.write_restart(tmp_path / "rst.npz", a, case.cfg)
     assert not (tmp_path / "rst.npz.tmp").exists()
-    _, b = _state(moist=True)
+    _, b = _state(moist=True, mp=mp, seed=4)
     info = R.restore_restart(path, b, case.cfg)
     assert info.elapsed_seconds == 1234.5 == b.elapsed_seconds
-    assert set(info.members) == {"state/" + n for n in R.STATE_SERIALIZED_ATTRS}
-    for n in R.STATE_SERIALIZED_ATTRS:
+    present = [n for n in R.STATE_SERIALIZED_ATTRS
+               if getattr(a, n, None) is not None]
+    assert set(info.members) == {"state/" + n for n in present}
+    assert set(a.extra_species) <= set(present)
+    for n in present:
         x, y = getattr(a, n).numpy(), getattr(b, n).numpy()
         assert x.tobytes() == y.tobytes(), n
     hdr = R.read_restart_header(path)
