@@ -12,7 +12,7 @@ from pathlib import Path
 LIB_NAME = "libarwen_dycore.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 #: transported species beyond qv/qc/qr a domain can carry (ARW_MAX_EXTRA)
 MAX_EXTRA = 13
 
@@ -53,7 +53,10 @@ class ArwDomain(ctypes.Structure):
                 + [(n, _P) for n in DIFF6_ARRAYS]
                 + [("n_extra", _I), ("n_load", _I)]
                 + [(n, _P * MAX_EXTRA) for n in ("qx", "qx0", "qxs")]
-                + [("h_diabatic", _P)])
+                + [("h_diabatic", _P), ("hypsometric_opt", _I)]
+                + [(n, _P) for n in ("c3h", "c4h", "c3f", "c4f")]
+                + [("p_top", _D)]
+                + [(n, _P) for n in ("cqu", "cqv", "cqw")])
 
 
 class ArwDomain64(ctypes.Structure):
@@ -70,7 +73,8 @@ class ArwParams(ctypes.Structure):
                 ("top_lid", _I), ("moist_adv_opt", _I),
                 ("precision_form", _I), ("diff_6th_opt", _I),
                 ("diff_6th_factor", _D), ("diff_6th_slopeopt", _I),
-                ("diff_6th_thresh", _D)]
+                ("diff_6th_thresh", _D), ("h_sca_adv_order", _I),
+                ("moist_cq", _I)]
 
 
 class ArwTracerCkpt(ctypes.Structure):

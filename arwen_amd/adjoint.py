@@ -666,6 +666,11 @@ def _check_adjoint_cfg(state, cfg):
     if getattr(cfg, "mp_physics", 0) != 0:
         raise _lib.ArwenError("step adjoint: qv/qc/qr states only "
                               "(mp_physics = 0, no h_diabatic)")
+    if getattr(cfg, "h_sca_adv_order", 2) != 2 \
+            or getattr(cfg, "hypsometric_opt", 1) != 1 \
+            or getattr(cfg, "moist_cq", False):
+        raise _lib.ArwenError("step adjoint: h_sca_adv_order = 2, "
+                              "hypsometric_opt = 1, moist_cq off only")
     if _uses_diffusion(cfg) and state.thb.dim() == 3:
         raise _lib.ArwenError("step adjoint: constant-K diffusion needs a "
                               "flat base")

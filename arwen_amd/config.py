@@ -152,13 +152,23 @@ def validate_for_step(cfg: RunConfig, *, acoustic: bool = True) -> None:
         raise ValueError(f"diff_6th_factor = {cfg.diff_6th_factor!r}: a "
                          "negative factor makes the 6th-order filter "
                          "anti-diffusive")
-    if cfg.h_sca_adv_order != 2:
+    if cfg.h_sca_adv_order not in (2, 5):
+        raise ValueError(
+            f"h_sca_adv_order must be 2 or 5 (the rhs_ph horizontal "
+            f"geopotential advection), got {cfg.h_sca_adv_order}")
+    if cfg.hypsometric_opt not in (1, 2):
+        raise ValueError(
+            "hypsometric_opt must be 1 (d(phi)/d(eta) hydrostatic alpha) or "
+            f"2 (log-pressure hypsometric alpha), got {cfg.hypsometric_opt}")
+    if cfg.hypsometric_opt == 2 and cfg.precision_form:
         raise NotImplementedError(
-            "only h_sca_adv_order=2 (the reference default) is ported")
-    if cfg.hypsometric_opt != 1:
+            "the precision form's perturbation EOS is the hypsometric_opt=1 "
+            "one")
+    if cfg.moist_cq and cfg.precision_form:
         raise NotImplementedError(
-            "only hypsometric_opt=1 (the reference default) is ported")
-    for name in ("open_x", "open_y", "specified", "nested", "moist_cq"):
+            "moist_cq is ported for the plain form (the precision form's "
+            "implicit solve runs in FP64)")
+    for name in ("open_x", "open_y", "specified", "nested"):
         if getattr(cfg, name):
             raise NotImplementedError(f"{name}=True is not part of this port")
     if cfg.emdiv > 0.0 or cfg.w_damping:

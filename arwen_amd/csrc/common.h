@@ -58,6 +58,10 @@ struct Sc {
     // substep (the sums start from zero), 2 later substeps (accumulate);
     // k_uv adds u'', v'' into ru_m, rv_m and k_column Omega'' into ww_m
     int sum;
+    // rhs_ph's horizontal geopotential advection: the 5th-order form
+    int h5;
+    // moist_cq: the pressure terms take d.cqu / d.cqv / d.cqw
+    int cq;
 };
 
 static inline Sc make_sc(const arw_params* p, double dtau, int first)
@@ -88,6 +92,8 @@ static inline Sc make_sc(const arw_params* p, double dtau, int first)
     s.gam_ready = 0;
     s.park = 0;
     s.sum = 0;
+    s.h5 = p->h_sca_adv_order == 5;
+    s.cq = p->moist_cq != 0;
     return s;
 }
 
@@ -191,6 +197,32 @@ __device__ __forceinline__ EosPert eos_pert(real thp, real thb, real qv,
     o.pp = (real)((double)pb * expm1((double)GAMMA_F * (x - y)));
     o.p = pb + o.pp;
     return o;
+}
+
+// The EOS's specific volumes of layer k with column mass mu and
+// geopotential thickness dph (reference diagnostics.cu:66-84):
+// hypsometric_opt 1 (or 0) alt = -dph rdnw / C(mu), al = alt - alb;
+// hypsometric_opt 2 al = dph / p_mid / log(p_bot / p_top_of_layer) - alb,
+// alt = al + alb, with the layer's dry pressures from the hybrid
+// coefficients c3/c4 and p_top.  The logarithm runs in double and is
+// rounded once.
+__device__ __forceinline__ void eos_alpha(const Dom& d, int k, real mu,
+                                          real dph, real albk, real c1k,
+                                          real c2k, real rdnwk, real& alt,
+                                          real& al)
+{
+    if (d.hypsometric_opt == 2) {
+        const real pt = (real)d.p_top;
+        const real pfu = (d.c3f[k + 1] * mu + d.c4f[k + 1]) + pt;
+        const real pfd = (d.c3f[k] * mu + d.c4f[k]) + pt;
+        const real phm = (d.c3h[k] * mu + d.c4h[k]) + pt;
+        const real lg = (real)log((double)(pfd / pfu));
+        al = ((dph / phm) / lg) - albk;
+        alt = al + albk;
+    } else {
+        alt = (-dph * rdnwk) / (c1k * mu + c2k);
+        al = alt - albk;
+    }
 }
 
 // Base-profile value: 1-D column (base3d = 0) or per-point 3-D field.

@@ -285,9 +285,11 @@ __device__ __forceinline__ void eos_level(const Dom& d, int k, int c, int st,
                   + d.php[(k + 1) * st + c];
     real ph_lo = basev(d.phb, k, c, st, d.base3d) + d.php[k * st + c];
     real dph = ph_hi - ph_lo;
-    real alt = (-dph * d.rdnw[k]) / (d.c1h[k] * mu + d.c2h[k]);
+    real alt, al;
+    eos_alpha(d, k, mu, dph, basev(d.alb, k, c, st, d.base3d), d.c1h[k],
+              d.c2h[k], d.rdnw[k], alt, al);
     d.alt[k * st + c] = alt;
-    d.al[k * st + c] = alt - basev(d.alb, k, c, st, d.base3d);
+    d.al[k * st + c] = al;
     d.p[k * st + c] = P0_F * pow_gamma((RD_F * th) / (P0_F * alt));
 }
 
@@ -402,6 +404,47 @@ __global__ void k_sumflux_finish(Dom d, real nsub)
     }
 }
 
+// WRF calc_cq at the stage state (reference acoustic.cu:84-158): cq =
+// 1 / (1 + 0.5 qtot), with qtot the first n moist mass species (qv, qc,
+// qr, then the loading extras) summed over the two cells around each u
+// face, v face and interior w level; 1 at the surface and the lid.
+__global__ void k_calc_cq(Dom d, int n)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    auto pair = [&](int a, int b) {
+        real q = R(0.0);
+#pragma unroll
+        for (int e = 0; e < 3 + 4; ++e) {
+            if (e >= n) break;
+            const real* x = e == 0 ? d.qv : (e == 1 ? d.qc : (e == 2 ? d.qr
+                                                  : d.qx[e - 3]));
+            q = q + x[a];
+            q = q + x[b];
+        }
+        return R(1.0) / (R(1.0) + R(0.5) * q);
+    };
+    if (j < ny && k < nz) {
+        const int a = (i >= nx) ? i - nx : i, b = (a == 0) ? nx - 1 : a - 1;
+        d.cqu[iU(k, j, i, ny, nx)] = pair(k * st + j * nx + a,
+                                          k * st + j * nx + b);
+    }
+    if (i < nx && k < nz) {
+        const int a = (j >= ny) ? j - ny : j, b = (a == 0) ? ny - 1 : a - 1;
+        d.cqv[iV(k, j, i, ny, nx)] = pair(k * st + a * nx + i,
+                                          k * st + b * nx + i);
+    }
+    if (i < nx && j < ny) {
+        const int c = j * nx + i;
+        d.cqw[k * st + c] = (k == 0 || k == nz)
+                                ? R(1.0)
+                                : pair(k * st + c, (k - 1) * st + c);
+    }
+}
+
 // Post-acoustic stage mu' of column c: the value small_step_finish
 // stores (mup + mu''), formed here so the scalar update can run before
 // the finish overwrites the time-t mass on the last stage.
@@ -428,6 +471,28 @@ __device__ __forceinline__ real scalar_update(const Dom& d, int k, int c,
     v = v / chm;
     if (clamp && !(v > R(0.0)) && v == v) v = R(0.0);
     return v;
+}
+
+// Unlimited stage update of one scalar, one thread per point: tend = div
+// of qcur with the averaged fluxes ru_m, rv_m, ww_m; qdst = update from
+// q0.  Used where the domain has too few columns for the column marches.
+__global__ void k_scalar_update(Dom d, const real* __restrict__ qcur,
+                                const real* __restrict__ q0,
+                                real* __restrict__ qdst,
+                                const real* __restrict__ mup0, real dt,
+                                int clamp, real rdx, real rdy)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    Arr3 Q{qcur, ny, nx}, RU{d.ru_m, ny, nx + 1}, RV{d.rv_m, ny + 1, nx},
+        RW{d.ww_m, ny, nx};
+    const real tend = R(0.0) + adv_scalar(Q, RU, RV, RW, k, j, i, nz, ny, nx,
+                                         d.rdnw, d.fnm, d.fnp, rdx, rdy);
+    const int x = iM(k, j, i, ny, nx);
+    qdst[x] = scalar_update(d, k, j * nx + i, q0[x], tend, mup0, dt, clamp);
 }
 
 // WRF flux_upwind (reference pd_advection.cu:101-108).
@@ -1257,6 +1322,12 @@ int32_t check_domain(const Dom* d)
                          d->mup_next};
     for (const void* p : req)
         if (!p) return fail(ARW_E_ARG, "null array in arw_domain");
+    if (d->hypsometric_opt < 0 || d->hypsometric_opt > 2)
+        return fail(ARW_E_ARG, "hypsometric_opt must be 1 or 2");
+    if (d->hypsometric_opt == 2
+        && (!d->c3h || !d->c4h || !d->c3f || !d->c4f))
+        return fail(ARW_E_ARG, "hypsometric_opt = 2 needs c3h, c4h, c3f, "
+                               "c4f");
     if (d->n_extra < 0 || d->n_extra > ARW_MAX_EXTRA || d->n_load < 0
         || d->n_load > 4 || d->n_load > d->n_extra)
         return fail(ARW_E_ARG, "need 0 <= n_load <= min(4, n_extra), "
@@ -1285,6 +1356,9 @@ int32_t check_prec(const arw_params* p, const Dom* d)
     if (d->h_diabatic)
         return fail(ARW_E_UNSUPPORTED, "the precision form carries no "
                                        "h_diabatic removal");
+    if (d->hypsometric_opt == 2)
+        return fail(ARW_E_UNSUPPORTED, "the precision form's EOS is the "
+                                       "hypsometric_opt = 1 one");
     if (!d->tri64)
         return fail(ARW_E_ARG, "precision form needs the FP64 tridiagonal scratch tri64");
     if (!d->pp)
@@ -1303,6 +1377,27 @@ int32_t check_params(const arw_params* p, const Dom* d)
     if ((p->khdif > 0.0 || p->kvdif > 0.0)
         && (!d->rdzf_h || !d->rdzc_h || !d->rdzf_w || !d->rdzc_w))
         return fail(ARW_E_ARG, "diffusion spacings required when khdif/kvdif > 0");
+    if (p->moist_cq) {
+        if (!d->qv) return fail(ARW_E_ARG, "moist_cq needs a moist domain");
+        if (p->moist_cq < 1 || p->moist_cq > 3 + d->n_load
+            || (p->moist_cq > 1 && p->moist_cq < 3))
+            return fail(ARW_E_ARG, "moist_cq: 1, 3 or 3 + n_load species");
+        if (!d->cqu || !d->cqv || !d->cqw)
+            return fail(ARW_E_ARG, "moist_cq needs the cqu, cqv, cqw arrays");
+        if (!ARW_F64 && p->precision_form)
+            return fail(ARW_E_UNSUPPORTED, "moist_cq in the plain form only");
+    }
+    return ARW_OK;
+}
+
+// The stage-piece entry points take the stage cq as arw_step computes it
+// internally; they run with moist_cq = 0 only.
+static int32_t refuse_cq(const arw_params* p)
+{
+    if (p && p->moist_cq)
+        return fail(ARW_E_UNSUPPORTED, "moist_cq: arw_step forms the stage "
+                                       "cq itself; the stage-piece entry "
+                                       "points take moist_cq = 0");
     return ARW_OK;
 }
 
@@ -1348,6 +1443,11 @@ int32_t substep(const Dom& d, const Sc& s, real* pout, int write_al,
                     KID_COLUMN, st);
 }
 
+// Below this many columns the column marches leave most SIMDs with less
+// than four waves, and the thread-per-point scalar kernels are faster
+// (both are bit-identical).
+constexpr long kScalarColumnsMin = 512L * 512;
+
 // Every scalar's RK stage (reference moist.py:434-698): the unlimited
 // flux form, or the positive-definite limiter on the final stage when
 // moist_adv_opt = 1.  cur: stage-state scalars; q0: time t; dst: new.
@@ -1362,6 +1462,29 @@ int32_t scalar_stage(const Dom& dd, const arw_params* prm, real* const* cur,
     const real rdx = (real)(1.0 / prm->dx), rdy = (real)(1.0 / prm->dy);
     const bool pd = final && prm->moist_adv_opt == 1;
     int32_t rc;
+    if ((long)dd.ny * dd.nx < kScalarColumnsMin) {
+        // few columns: one thread per point (stored limiter fluxes)
+        const dim3 cells((dd.nx + 63) / 64, (dd.ny + 3) / 4, dd.nz);
+        for (int q = 0; q < nq; ++q) {
+            if (pd) {
+                hipLaunchKernelGGL(k_pd_fluxes, union_grid(dd, dd.nz + 1),
+                                   dim3(64, 4), 0, st, dd, cur[q], q0[q],
+                                   (real)prm->dx, (real)prm->dy, dt);
+                hipLaunchKernelGGL(k_pd_scale, cells, dim3(64, 4), 0, st, dd,
+                                   q0[q], mup0, rdx, rdy, dt);
+                hipLaunchKernelGGL(k_pd_apply, cells, dim3(64, 4), 0, st, dd,
+                                   q0[q], dst[q], mup0, rdx, rdy, dt);
+            } else {
+                hipLaunchKernelGGL(k_scalar_update, cells, dim3(64, 4), 0, st,
+                                   dd, cur[q], q0[q], dst[q], mup0, dt,
+                                   final ? 1 : 0, rdx, rdy);
+            }
+            if ((rc = launched(hipGetLastError(), "scalar stage", KID_SCALARS,
+                               st)))
+                return rc;
+        }
+        return ARW_OK;
+    }
     if (pd) {
         const size_t bytes = (size_t)dd.nz * dd.ny * dd.nx * sizeof(real);
         for (int q = 0; q < nq; ++q) {
@@ -1489,6 +1612,9 @@ int32_t lib_check_adj_domain(const Dom* d)
     if (d->n_extra || d->h_diabatic)
         return fail(ARW_E_UNSUPPORTED, "the adjoint covers qv/qc/qr states "
                                        "without h_diabatic");
+    if (d->hypsometric_opt == 2)
+        return fail(ARW_E_UNSUPPORTED, "the adjoint covers hypsometric_opt "
+                                       "= 1");
     return ARW_OK;
 }
 int32_t lib_check_prec(const arw_params* p, const Dom* d)
@@ -1991,6 +2117,7 @@ int32_t ARW_API(stage_prepare)(const Dom* d, const arw_params* prm,
     int32_t rc = check_domain(d);
     if (rc) return rc;
     if ((rc = check_params(prm, d))) return rc;
+    if ((rc = refuse_cq(prm))) return rc;
     // the held 6th-order tendencies are formed and added by the whole step
     // only; a piecewise caller would silently miss them
     if (prm->diff_6th_opt != 0)
@@ -2014,6 +2141,7 @@ int32_t ARW_API(calc_coefs)(const Dom* d, const arw_params* prm,
     if (rc) return rc;
     if (!prm) return fail(ARW_E_ARG, "null params");
     if ((rc = check_prec(prm, d))) return rc;
+    if ((rc = refuse_cq(prm))) return rc;
     // The implicit-w factors are formed inside every substep from the
     // stage reference p/alt; what persists across substeps is the
     // reference mass divergence (and Omega), formed here.
@@ -2029,6 +2157,7 @@ int32_t ARW_API(acoustic_substep)(const Dom* d, const arw_params* prm,
     if (rc) return rc;
     if (!prm) return fail(ARW_E_ARG, "null params");
     if ((rc = check_prec(prm, d))) return rc;
+    if ((rc = refuse_cq(prm))) return rc;
     Sc s = make_sc(prm, dtau, first);
     s.use_pold = 1;
     hipStream_t st = (hipStream_t)stream;
@@ -2149,6 +2278,14 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
             ck->dt[stage] = nsub[stage] * dtaus[stage];
         }
         Sc s = make_sc(prm, dtaus[stage], 1);
+        if (s.cq) {
+            // moist_cq: the stage's cq from its scalars, before the slow
+            // pressure gradient reads it
+            hipLaunchKernelGGL(k_calc_cq, union_grid(dd, dd.nz + 1),
+                               dim3(64, 4), 0, st, dd, prm->moist_cq);
+            if ((rc = launched(hipGetLastError(), "k_calc_cq", -1, st)))
+                return rc;
+        }
         if ((rc = stage_prepare(dd, s, stage == 0, st))) return rc;
         if (d6) {
             hipLaunchKernelGGL(k_add_held, union_grid(dd, dd.nz + 1),

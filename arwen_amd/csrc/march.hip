@@ -143,9 +143,11 @@ __global__ __launch_bounds__(256) void k_omega(typename SRC::Arg a, Sc s)
                 real th = (B3 ? gld(d.thb, h, bc) : thb1[k]) + gld(d.thp, h, bc);
                 if (d.qv) th = th * (R(1.0) + RVOVRD_F * gld(d.qv, h, bc));
                 real dph = (gb_hi + ph_hi) - (gb_lo + ph_lo);
-                real alt = (-dph * rdnw[k]) / (c1h[k] * ms + c2h[k]);
+                real alt, al;
+                eos_alpha(d, k, ms, dph, albk, c1h[k], c2h[k], rdnw[k], alt,
+                          al);
                 gst(d.alt, h, bc, alt);
-                gst(d.al, h, bc, alt - albk);
+                gst(d.al, h, bc, al);
                 gst(d.p, h, bc, P0_F * pow_gamma((RD_F * th) / (P0_F * alt)));
             }
             ph_lo = ph_hi;
@@ -445,11 +447,13 @@ __global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
             fzu = fz1;
             real t = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy,
                            rdnw[k]);
-            t = t - pgf_val(s.rdx, s.hrdx, c1, c2, rdnw[k], mux_c, dmu_u,
-                            A.ph_0, A.ph_p, W.ph_0, W.ph_p, altA, altW,
-                            A.pp_0, W.pp_0, alA, alW, pbA, pbW, B3,
-                            phh(A), phh(W), A.dpn_0, A.dpn_p, W.dpn_0,
-                            W.dpn_p);
+            real pgu = pgf_val(s.rdx, s.hrdx, c1, c2, rdnw[k], mux_c, dmu_u,
+                               A.ph_0, A.ph_p, W.ph_0, W.ph_p, altA, altW,
+                               A.pp_0, W.pp_0, alA, alW, pbA, pbW, B3,
+                               phh(A), phh(W), A.dpn_0, A.dpn_p, W.dpn_0,
+                               W.dpn_p);
+            if (s.cq) pgu = gld(d.cqu, lu + j * pu, cC) * pgu;   // moist_cq
+            t = t - pgu;
             if (DIFF) {
                 real fc = x[3];
                 real lap = s.kh * (s.dxi2 * ((x[4] - R(2.0) * fc) + x[2])
@@ -499,11 +503,13 @@ __global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
             fzv = fz1;
             real t = div3(fx1 - fx0, fy1 - fy0, fz1 - fz0, s.rdx, s.rdy,
                            rdnw[k]);
-            t = t - pgf_val(s.rdy, s.hrdy, c1, c2, rdnw[k], muy_c, dmu_v,
-                            A.ph_0, A.ph_p, S.ph_0, S.ph_p, altA, altS,
-                            A.pp_0, S.pp_0, alA, alS, pbA, pbS, B3,
-                            phh(A), phh(S), A.dpn_0, A.dpn_p, S.dpn_0,
-                            S.dpn_p);
+            real pgv = pgf_val(s.rdy, s.hrdy, c1, c2, rdnw[k], muy_c, dmu_v,
+                               A.ph_0, A.ph_p, S.ph_0, S.ph_p, altA, altS,
+                               A.pp_0, S.pp_0, alA, alS, pbA, pbS, B3,
+                               phh(A), phh(S), A.dpn_0, A.dpn_p, S.dpn_0,
+                               S.dpn_p);
+            if (s.cq) pgv = gld(d.cqv, lv + j * nx, cC) * pgv;    // moist_cq
+            t = t - pgv;
             if (DIFF) {
                 real fc = y[3];
                 real lap = s.kh * (s.dxi2 * ((x[4] - R(2.0) * fc) + x[2])
@@ -851,12 +857,32 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
             real fcy_jp = (c1f[k] * muy_jp + c2f[k]) * wjp;
             // the window holds php alone in the precision form
             real p0 = (PREC && B3) ? phb_at(k, rA, cC) + ph_0 : ph_0;
-            real fx = fcx_i * (p0 - phh_at(k, rA, cx[2]));
-            real fxp = fcx_ip * (phh_at(k, rA, cx[4]) - p0);
-            r = r - qxk * (fxp + fx);
-            real fy = fcy_j * (p0 - phh_at(k, rS, cC));
-            real fyp = fcy_jp * (phh_at(k, rN, cC) - p0);
-            r = r - qyk * (fyp + fy);
+            if (s.h5) {
+                // h_sca_adv_order = 5 (reference dycore.cu:668-748): the
+                // face-flux sum times (45 d1 - 9 d2 + d3) / 60 of the
+                // centred differences d_o = ph(+o) - ph(-o)
+                auto cen = [&](real d1, real d2, real d3) {
+                    return div_const((R(45.0) * d1 + R(-9.0) * d2)
+                                         + R(1.0) * d3,
+                                     1.0 / 60.0, R(60.0));
+                };
+                const real dx5 = cen(phh_at(k, rA, cx[4]) - phh_at(k, rA, cx[2]),
+                                     phh_at(k, rA, cx[5]) - phh_at(k, rA, cx[1]),
+                                     phh_at(k, rA, cx[6]) - phh_at(k, rA, cx[0]));
+                const real dy5 = cen(phh_at(k, yo[4], cC) - phh_at(k, yo[2], cC),
+                                     phh_at(k, yo[5], cC) - phh_at(k, yo[1], cC),
+                                     phh_at(k, yo[6], cC) - phh_at(k, yo[0], cC));
+                const real hx = (qxk * (fcx_i + fcx_ip)) * dx5;
+                const real hy = (qyk * (fcy_j + fcy_jp)) * dy5;
+                r = r - (hx + hy);
+            } else {
+                real fx = fcx_i * (p0 - phh_at(k, rA, cx[2]));
+                real fxp = fcx_ip * (phh_at(k, rA, cx[4]) - p0);
+                r = r - qxk * (fxp + fx);
+                real fy = fcy_j * (p0 - phh_at(k, rS, cC));
+                real fyp = fcy_jp * (phh_at(k, rN, cC) - p0);
+                r = r - qyk * (fyp + fy);
+            }
         }
         gst(d.rph_t, lm + rA, cC, r);
 
@@ -1116,6 +1142,7 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
             if (B3) dphp = dphp + R(0.5) * ((gbA0 + gbA1) - (gbW0 + L.gbW));
             dpxy = dpxy + (s.rdx * dphp) * (rdnw[k] * (dn_up - dnu_0)
                                             - (R(0.5) * c1) * dmu_u);
+            if (s.cq) dpxy = gld(d.cqu, k * stu + rU, bu) * dpxy;   // moist_cq
             const real un = L.u + dtau * (L.ru - dpxy);
             gst(d.u_pp, k * stu + rU, bu, un);
             if (SUM) {
@@ -1145,6 +1172,7 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
             if (B3) dphp = dphp + R(0.5) * ((gbA0 + gbA1) - (gbS0 + L.gbS));
             dpxy = dpxy + (s.rdy * dphp) * (rdnw[k] * (dn_up - dnv_0)
                                             - (R(0.5) * c1) * dmu_v);
+            if (s.cq) dpxy = gld(d.cqv, k * stv + rA, bv) * dpxy;   // moist_cq
             const real vn = L.v + dtau * (L.rv - dpxy);
             gst(d.v_pp, k * stv + rA, bv, vn);
             if (SUM) {
@@ -1422,10 +1450,11 @@ __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
                          real chmc) {
         real dph_up = one_p * (rhs_up - rhs_c) + one_m * (ph_up - ph_c);
         real dph_dn = one_p * (rhs_c - rhs_dn) + one_m * (ph_c - ph_dn);
-        real t1 = dtau * rwr
-                   + (c_force * rdn[r])
-                         * ((c2c * rdnw[r]) / chmc * dph_up
-                            - (c2dn * rdnw[r - 1]) / chmdn * dph_dn);
+        real pg = (c_force * rdn[r])
+                  * ((c2c * rdnw[r]) / chmc * dph_up
+                     - (c2dn * rdnw[r - 1]) / chmdn * dph_dn);
+        if (s.cq) pg = gld(d.cqw, r * st, bc) * pg;            // moist_cq
+        real t1 = dtau * rwr + pg;
         real t3 = c_buoy * (rdn[r] * ((c2c * altc) * t2c - (c2dn * altdn) * t2dn)
                              - c1f[r] * muave);
         return wold + (t1 + t3);
@@ -1433,7 +1462,10 @@ __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
     // LU factors of row r (1 <= r <= nz-1) from the windowed c2a/chm
     auto lu_row = [&](int r, real c2dn, real c2c, real chmdn, real chmc,
                       tri& alpha_r, bool r_ge2) {
-        const tri tcof = cof, rdn_r = rdn[r], rdnw_r = rdnw[r];
+        // moist_cq: the row's coefficient takes cqw(r) first
+        const tri tcof = s.cq ? (tri)gld(d.cqw, r * st, bc) * (tri)cof
+                              : (tri)cof;
+        const tri rdn_r = rdn[r], rdnw_r = rdnw[r];
         const tri rdnw_m = rdnw[r - 1];
         tri a_r = (tri)R(0.0);
         if (r_ge2)
@@ -1821,9 +1853,11 @@ __global__ __launch_bounds__(256) void k_finish(typename SRC::Arg a,
             } else {
                 real th = thb_lo + thp_lo;
                 real dph = (phb_k + php_new) - (phb_lo + php_lo);
-                real alt = (-dph * rdnw[k - 1]) / (c1h[k - 1] * mu_eos + c2h[k - 1]);
+                real alt, al;
+                eos_alpha(d, k - 1, mu_eos, dph, alb_lo, c1h[k - 1],
+                          c2h[k - 1], rdnw[k - 1], alt, al);
                 gst(d.alt, h, bc, alt);
-                gst(d.al, h, bc, alt - alb_lo);
+                gst(d.al, h, bc, al);
                 gst(d.p, h, bc, P0_F * pow_gamma((RD_F * th) / (P0_F * alt)));
             }
         }

@@ -14,7 +14,7 @@ import math
 import torch
 
 from . import _lib
-from .config import RunConfig, diff6_factor, validate_for_step
+from .config import LOADING_EXTRA, RunConfig, diff6_factor, validate_for_step
 from .state import DomainState, mu_at_u_faces, mu_at_v_faces
 
 
@@ -33,7 +33,8 @@ def _dom(state: DomainState, cfg: RunConfig):
     """The state's C view with the arrays cfg's step options need
     (diffusion spacings, the precision form's pp and FP64 scratch)."""
     return state.domain_struct(diffusion=_uses_diffusion(cfg),
-                               precision_form=bool(cfg.precision_form))
+                               precision_form=bool(cfg.precision_form),
+                               hypsometric_opt=cfg.hypsometric_opt)
 
 
 def params(cfg: RunConfig) -> _lib.ArwParams:
@@ -54,7 +55,20 @@ def params(cfg: RunConfig) -> _lib.ArwParams:
     p.diff_6th_factor = float(diff6_factor(cfg)) if cfg.diff_6th_opt else 0.0
     p.diff_6th_slopeopt = int(cfg.diff_6th_slopeopt)
     p.diff_6th_thresh = float(cfg.diff_6th_thresh)
+    p.h_sca_adv_order = int(cfg.h_sca_adv_order)
+    p.moist_cq = moist_cq_species(cfg)
     return p
+
+
+def moist_cq_species(cfg: RunConfig) -> int:
+    """arw_params.moist_cq: 0 off, else how many moist mass species WRF
+    calc_cq sums (reference acoustic.py:61-110): qv (mp_physics 0), qv qc
+    qr (1), plus the ice and hail masses (the mixed-phase schemes)."""
+    if not (cfg.moist and cfg.moist_cq):
+        return 0
+    if cfg.mp_physics == 0:
+        return 1
+    return 3 + LOADING_EXTRA[cfg.mp_physics]
 
 
 def _uses_diffusion(cfg: RunConfig) -> bool:
@@ -100,7 +114,9 @@ def step(state: DomainState, cfg: RunConfig, *, acoustic: bool = True,
     _check_state(state)
     d = state.domain_struct(diffusion=_uses_diffusion(cfg),
                             precision_form=bool(cfg.precision_form),
-                            diff6=cfg.diff_6th_opt > 0)
+                            diff6=cfg.diff_6th_opt > 0,
+                            hypsometric_opt=cfg.hypsometric_opt,
+                            moist_cq=bool(cfg.moist_cq))
     p = params(cfg)
     name, fn = _fn(state, "step")
     _lib.check(fn(d, p, _stream()), name)
@@ -136,7 +152,8 @@ def _step_advective(state: DomainState, cfg: RunConfig) -> None:
             "step(acoustic=False) (advection-only path) requires w == 0: "
             "its eta mass flux is zero, which is the flux only for w == 0; "
             "the full step advects with the diagnosed Omega")
-    d = state.domain_struct(diffusion=_uses_diffusion(cfg))
+    d = state.domain_struct(diffusion=_uses_diffusion(cfg),
+                            hypsometric_opt=cfg.hypsometric_opt)
     name, fn = _fn(state, "step_advective")
     _lib.check(fn(d, params(cfg), _stream()), name)
     state.elapsed_seconds += cfg.dt
@@ -150,12 +167,14 @@ def run_steps(state: DomainState, cfg: RunConfig, n: int, *,
 
 def update_diagnostics(state: DomainState, hypsometric_opt: int = 1) -> None:
     """Recompute p, al, alt from (thp, php, mup) (reference
-    diagnostics.py:14-57)."""
-    if hypsometric_opt != 1:
-        raise NotImplementedError("only hypsometric_opt=1 is ported")
+    diagnostics.py:14-57; hypsometric_opt 1 or 2)."""
+    if hypsometric_opt not in (1, 2):
+        raise ValueError(
+            f"hypsometric_opt must be 1 or 2, got {hypsometric_opt}")
     _check_state(state)
     name, fn = _fn(state, "calc_p_alpha")
-    _lib.check(fn(state.domain_struct(), _stream()), name)
+    _lib.check(fn(state.domain_struct(hypsometric_opt=hypsometric_opt),
+                  _stream()), name)
 
 
 def stage_fluxes(state: DomainState, cfg: RunConfig):

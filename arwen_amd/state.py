@@ -247,12 +247,17 @@ class DomainState:
 
     def domain_struct(self, diffusion: bool = False,
                       precision_form: bool = False,
-                      diff6: bool = False) -> ArwDomain:
+                      diff6: bool = False,
+                      hypsometric_opt: int | None = None,
+                      moist_cq: bool = False) -> ArwDomain:
         """Raw-pointer view of the state for the C-ABI (a fresh copy the
         caller may modify).  Built once per option set and cached until an
         array attribute is rebound or a scratch slot is created; arrays
         updated in place keep their pointers."""
-        key = (bool(diffusion), bool(precision_form), bool(diff6))
+        if hypsometric_opt is None:
+            hypsometric_opt = getattr(self.cfg, "hypsometric_opt", 1)
+        key = (bool(diffusion), bool(precision_form), bool(diff6),
+               int(hypsometric_opt), bool(moist_cq))
         memo = self.__dict__.get("_dom_memo")
         if memo is not None and key in memo:
             m = memo[key]
@@ -262,7 +267,8 @@ class DomainState:
         return type(d).from_buffer_copy(d)
 
     def _build_domain_struct(self, diffusion: bool, precision_form: bool,
-                             diff6: bool) -> ArwDomain:
+                             diff6: bool, hypsometric_opt: int,
+                             moist_cq: bool) -> ArwDomain:
         nz, ny, nx = self.shape
         d = ArwDomain64() if self.dtype == torch.float64 else ArwDomain()
         d.nz, d.ny, d.nx = nz, ny, nx
@@ -311,6 +317,20 @@ class DomainState:
             d.qxs[k] = self.scratch((nz, ny, nx), "arw_qs_" + name).data_ptr()
         if self.qv is not None and self.cfg.mp_physics != 0:
             d.h_diabatic = self.h_diabatic.data_ptr()
+        d.hypsometric_opt = int(hypsometric_opt)
+        if hypsometric_opt == 2:
+            if self.p_top is None:
+                raise ValueError("hypsometric_opt=2 needs the base state's "
+                                 "p_top: call load_base first")
+            for name in ("c3h", "c4h", "c3f", "c4f"):
+                d_arr = getattr(self, name)
+                setattr(d, name, d_arr.data_ptr())
+            d.p_top = float(self.p_top)
+        if moist_cq and self.qv is not None:
+            for name, shp in (("cqu", (nz, ny, nx + 1)),
+                              ("cqv", (nz, ny + 1, nx)),
+                              ("cqw", (nz + 1, ny, nx))):
+                setattr(d, name, self.scratch(shp, "arw_" + name).data_ptr())
         if precision_form and self.dtype == torch.float32:
             d.tri64 = self.scratch((2, nz + 1, ny, nx), "arw_tri64",
                                    dtype=torch.float64).data_ptr()

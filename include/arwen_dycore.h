@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define ARW_ABI_VERSION 7
+#define ARW_ABI_VERSION 8
 
 /* transported species beyond qv/qc/qr a domain can carry (NSSL's set) */
 #define ARW_MAX_EXTRA 13
@@ -105,6 +105,16 @@ typedef struct arw_domain {
      * tendency and the final stage's finish removes what the substeps
      * integrated (reference dycore.py:572-592, dycore.cu:1186-1194). */
     float *h_diabatic;
+    /* WRF hypsometric_opt of the EOS's specific volume (reference
+     * diagnostics.cu:37-89): 1 (or 0) from the layer's geopotential
+     * thickness, 2 from the log of its dry-pressure ratio, which reads the
+     * hybrid coefficients c3h, c4h (nz) and c3f, c4f (nz+1) and the model
+     * top pressure p_top.  Plain form only. */
+    int32_t hypsometric_opt;
+    const float *c3h, *c4h, *c3f, *c4f;
+    double p_top;
+    /* moist_cq's stage coefficients (u, v and w shapes; scratch) */
+    float *cqu, *cqv, *cqw;
 } arw_domain;
 
 /*
@@ -138,6 +148,10 @@ typedef struct arw64_domain {
     int32_t n_extra, n_load;
     double *qx[ARW_MAX_EXTRA], *qx0[ARW_MAX_EXTRA], *qxs[ARW_MAX_EXTRA];
     double *h_diabatic;
+    int32_t hypsometric_opt;
+    const double *c3h, *c4h, *c3f, *c4f;
+    double p_top;
+    double *cqu, *cqv, *cqw;
 } arw64_domain;
 
 /* Step configuration (the RunConfig fields the step reads,
@@ -170,6 +184,18 @@ typedef struct arw_params {
     double diff_6th_factor;
     int32_t diff_6th_slopeopt;
     double diff_6th_thresh;
+    /* WRF h_sca_adv_order of rhs_ph's horizontal geopotential advection
+     * (reference dycore.cu:541-748): 2, or 5 (the face-flux sum times the
+     * centred difference (45 d1 - 9 d2 + d3) / 60); 0 means 2. */
+    int32_t h_sca_adv_order;
+    /* WRF moist_cq (reference acoustic.py:61-110, acoustic.cu:84-158 and
+     * the uses :287, :307, :700-744, :899-905, dycore.cu:195, :208): 0
+     * off; otherwise the number of moist mass species summed into
+     * cq = 1 / (1 + 0.5 qtot) at faces and w levels -- 1 (qv), 3 (qv, qc,
+     * qr) or 3 + n_load (the ice and hail masses).  cq scales the slow and
+     * acoustic horizontal pressure gradients and the implicit w solve's
+     * pressure terms; needs d->cqu, cqv, cqw.  Moist plain-form steps. */
+    int32_t moist_cq;
 } arw_params;
 
 int32_t arw_abi_version(void);

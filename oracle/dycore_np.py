@@ -126,10 +126,13 @@ def _c(v, s):
 # EOS diagnostics
 # ---------------------------------------------------------------------------
 
-def calc_p_alpha(s, prec=False) -> None:
-    """Full pressure, alpha', total alpha (reference diagnostics.cu:37-89,
-    hypso=1; launcher diagnostics.py:14-57).  ``prec``: the precision
-    form's perturbation EOS (calc_p_alpha_pert)."""
+def calc_p_alpha(s, prec=False, hypso=1) -> None:
+    """Full pressure, alpha', total alpha (reference diagnostics.cu:37-89;
+    launcher diagnostics.py:14-57).  ``hypso`` is hypsometric_opt: 1 takes
+    alpha from the layer's geopotential thickness, 2 from the log of its
+    dry-pressure ratio (diagnostics.cu:75-80), the logarithm evaluated in
+    double and rounded once.  ``prec``: the precision form's perturbation
+    EOS (calc_p_alpha_pert, hypso 1)."""
     if prec:
         calc_p_alpha_pert(s)
         return
@@ -140,9 +143,21 @@ def calc_p_alpha(s, prec=False) -> None:
     ph = _b3(s.phb) + s.php
     dph = ph[1:] - ph[:-1]
     c1h, c2h = s.c1h[:, None, None], s.c2h[:, None, None]
-    alt = (-dph * s.rdnw[:, None, None]) / (c1h * mu + c2h)
-    s.alt[...] = alt
-    s.al[...] = alt - _b3(s.alb)
+    if hypso == 2:
+        lev = lambda a: a[:, None, None]
+        pt = s.p_top
+        pfu = (lev(s.c3f[1:]) * mu + lev(s.c4f[1:])) + pt
+        pfd = (lev(s.c3f[:-1]) * mu + lev(s.c4f[:-1])) + pt
+        phm = (lev(s.c3h) * mu + lev(s.c4h)) + pt
+        lg = np.log((pfd / pfu).astype(np.float64)).astype(s.dtype)
+        al = ((dph / phm) / lg) - _b3(s.alb)
+        alt = al + _b3(s.alb)
+        s.alt[...] = alt
+        s.al[...] = al
+    else:
+        alt = (-dph * s.rdnw[:, None, None]) / (c1h * mu + c2h)
+        s.alt[...] = alt
+        s.al[...] = alt - _b3(s.alb)
     s.p[...] = _c(P0, s) * _pow_gamma((_c(RD, s) * th) / (_c(P0, s) * alt))
 
 
@@ -387,6 +402,42 @@ def _pp(s, prec=False):
     return s.pp if prec else s.p - _b3(s.pb)
 
 
+def cq_mass_species(s, mp_physics):
+    """The moist mass species WRF calc_cq sums (reference acoustic.py:
+    61-110): qv for mp_physics 0, qv/qc/qr for 1, plus qi/qs/qg (and qh
+    for NSSL) for the mixed-phase schemes; number moments never."""
+    if mp_physics == 0:
+        return ("qv",)
+    return ("qv", "qc", "qr") + tuple(getattr(s, "extra", ())[
+        :getattr(s, "n_load", 0)])
+
+
+def calc_cq(s, mp_physics):
+    """WRF calc_cq at the stage state (reference acoustic.cu:84-158):
+    1 / (1 + 0.5 qtot) with qtot the species summed over the two cells
+    around each u face, v face and interior w level (cqw = 1 at the
+    surface and the lid).  Returns (cqu, cqv, cqw)."""
+    f = s.dtype.type
+    names = cq_mass_species(s, mp_physics)
+
+    def pair(a_of, b_of):
+        q = None
+        for n in names:
+            x = getattr(s, n)
+            q = (f(0.0) + a_of(x)) if q is None else q + a_of(x)
+            q = q + b_of(x)
+        return f(1.0) / (f(1.0) + f(0.5) * q)
+
+    cqu = pair(lambda x: x, lambda x: np.roll(x, 1, axis=2))
+    cqu = np.concatenate([cqu, cqu[..., :1]], axis=2)
+    cqv = pair(lambda x: x, lambda x: np.roll(x, 1, axis=1))
+    cqv = np.concatenate([cqv, cqv[:, :1]], axis=1)
+    nz = s.thp.shape[0]
+    cqw = np.ones((nz + 1,) + s.thp.shape[1:], dtype=s.dtype)
+    cqw[1:nz] = pair(lambda x: x[1:], lambda x: x[:-1])
+    return cqu, cqv, cqw
+
+
 def _dpn_levels(s, pp, cfn, cfn1, top_lid):
     """Perturbation pressure at full levels 0..nz per column (WRF dpn,
     reference dycore.cu:59-85)."""
@@ -440,8 +491,14 @@ def slow_pgf(s, dx, dy, top_lid=True, prec=False):
 
     tu = face(-1, rdx, hrdx)
     tv = face(-2, rdy, hrdy)
-    s.ru_t -= np.concatenate([tu, tu[..., :1]], axis=-1)
-    s.rv_t -= np.concatenate([tv, tv[:, :1, :]], axis=-2)
+    tu = np.concatenate([tu, tu[..., :1]], axis=-1)
+    tv = np.concatenate([tv, tv[:, :1, :]], axis=-2)
+    cq = getattr(s, "cq", None)
+    if cq is not None:                   # moist_cq (dycore.cu:195, :208)
+        tu = cq[0] * tu
+        tv = cq[1] * tv
+    s.ru_t -= tu
+    s.rv_t -= tv
 
 
 def _q_total(s):
@@ -481,9 +538,12 @@ def slow_buoyancy(s, prec=False):
     s.rw_t[nz] += f(G) * term
 
 
-def slow_geopotential(s, ww, dx, dy, prec=False):
-    """WRF rhs_ph, order-2 horizontal branch with the vertical Omega and
-    g*w terms (reference dycore.cu:541-667; launcher dycore.py:443-462).
+def slow_geopotential(s, ww, dx, dy, prec=False, order=2):
+    """WRF rhs_ph with the vertical Omega and g*w terms (reference
+    dycore.cu:541-667; launcher dycore.py:443-462), horizontal branch of
+    h_sca_adv_order ``order``: 2, or 5 (dycore.cu:668-748, periodic: the
+    face-flux sum times the centred 6th-order difference
+    (45 d1 - 9 d2 + d3) / 60 of the full geopotential).
     Writes rph_t at levels 1..nz (the top row is overwritten, not added).
     ``prec`` (precision form): the layer thickness of the full geopotential
     is dphb + dphp with dphb = -(dnw (c1h mub + c2h)) alb (the base
@@ -531,6 +591,16 @@ def slow_geopotential(s, ww, dx, dy, prec=False):
         fcx = mx * wx                       # face i (between i-1 and i)
         fcy = my * wy
         p = s.php[kk] if (prec and not s.base3d) else ph[kk]
+        if order == 5:
+            def cen(ax):
+                d = [np.roll(p, -o, axis=ax) - np.roll(p, o, axis=ax)
+                     for o in (1, 2, 3)]
+                return ((f(45.0) * d[0] + f(-9.0) * d[1]) + f(1.0) * d[2]) \
+                    / f(60.0)
+            hx = (qx * (fcx + np.roll(fcx, -1, axis=1))) * cen(1)
+            hy = (qy * (fcy + np.roll(fcy, -1, axis=0))) * cen(0)
+            tend[kk] = tend[kk] - (hx + hy)
+            continue
         fx = fcx * (p - np.roll(p, 1, axis=1))
         fxp = np.roll(fx, -1, axis=1)
         tend[kk] = tend[kk] - qx * (fxp + fx)
@@ -608,7 +678,8 @@ def add_diffusion_tendencies(s, khdif, kvdif, dx, dy):
         tend += (c1 * muf[None] + c2) * tmp
 
 
-def add_slow_tendencies(s, ru, rv, ww, dx, dy, top_lid=True, prec=False):
+def add_slow_tendencies(s, ru, rv, ww, dx, dy, top_lid=True, prec=False,
+                        h_order=2):
     """Advection of theta/u/v/w plus PGF, buoyancy and rhs_ph (reference
     dycore.py:502-569)."""
     f = s.dtype.type
@@ -621,7 +692,7 @@ def add_slow_tendencies(s, ru, rv, ww, dx, dy, top_lid=True, prec=False):
     s.rw_t += flux_div_w(s.w, ru, rv, ww, s.rdn, s.fnm, s.fnp, dxi, dyi)
     slow_pgf(s, dx, dy, top_lid, prec)
     slow_buoyancy(s, prec)
-    slow_geopotential(s, ww, dx, dy, prec)
+    slow_geopotential(s, ww, dx, dy, prec, h_order)
 
 
 def add_h_diabatic_tendency(s) -> None:
@@ -706,9 +777,14 @@ def calc_coefs(s, dtau, epssm, top_lid=True, prec=False):
     else:
         c2a = c2a_s
     shp = (nz + 1,) + mut.shape
+    # moist_cq: row k's coefficient cof takes the factor cqw(k)
+    # (acoustic.cu:700-744, (cq cof) then the same products)
+    cq = getattr(s, "cq", None)
+    qcof = [cof if cq is None else cq[2][k].astype(c2a.dtype) * cof
+            for k in range(nz + 1)]
     a = np.zeros(shp, c2a.dtype)
     for k in range(2, nz):
-        a[k] = (-cof * rdn[k] * rdnw[k - 1] * c2a[k - 1]
+        a[k] = (-qcof[k] * rdn[k] * rdnw[k - 1] * c2a[k - 1]
                 / (chm[k - 1] * cfm[k - 1]))
     if not top_lid:
         a[nz] = (f(-2.0) * cof * rdnw[nz - 1] * rdnw[nz - 1]
@@ -716,10 +792,10 @@ def calc_coefs(s, dtau, epssm, top_lid=True, prec=False):
     b = np.zeros(shp, c2a.dtype)
     cc = np.zeros(shp, c2a.dtype)
     for k in range(1, nz):
-        b[k] = f(1.0) + cof * rdn[k] * (
+        b[k] = f(1.0) + qcof[k] * rdn[k] * (
             rdnw[k] * c2a[k] / (chm[k] * cfm[k])
             + rdnw[k - 1] * c2a[k - 1] / (chm[k - 1] * cfm[k]))
-        cc[k] = -cof * rdn[k] * rdnw[k] * c2a[k] / (chm[k] * cfm[k + 1])
+        cc[k] = -qcof[k] * rdn[k] * rdnw[k] * c2a[k] / (chm[k] * cfm[k + 1])
     b[nz] = f(1.0) + f(2.0) * cof * rdnw[nz - 1] * rdnw[nz - 1] \
         * c2a[nz - 1] / (chm[nz - 1] * cfm[nz])
     alpha, gam = lu_factors(a, b, cc)
@@ -807,6 +883,9 @@ def advance_uv(s, dtau, smdiv, dx, dy, top_lid=True):
             dpxy = np.concatenate([dpxy, dpxy[..., :1]], axis=-1)
         else:
             dpxy = np.concatenate([dpxy, dpxy[:, :1, :]], axis=-2)
+        cq = getattr(s, "cq", None)
+        if cq is not None:               # moist_cq (acoustic.cu:287, :307)
+            dpxy = cq[0 if ax == -1 else 1] * dpxy
         pp += dtau * (rt - dpxy)
 
 
@@ -913,16 +992,18 @@ def advance_w_phi(s, coefs, dtau, epssm, dx, dy, top_lid=True,
     t2 = half * ((one + eps) * s.th_pp + (one - eps) * s.th_pp_old) \
         / (chs * thr)
     chm = s.c1h[:, None, None] * mut + s.c2h[:, None, None]
+    cq = getattr(s, "cq", None)
     for k in range(1, nz):
         dph_up = (one + eps) * (rhs[k + 1] - rhs[k]) \
             + (one - eps) * (s.ph_pp[k + 1] - s.ph_pp[k])
         dph_dn = (one + eps) * (rhs[k] - rhs[k - 1]) \
             + (one - eps) * (s.ph_pp[k] - s.ph_pp[k - 1])
-        w[k] = w[k] + ((dtau * s.rw_t[k]
-                        + half * dtau * g * s.rdn[k] * (
-                            c2a[k] * s.rdnw[k] / chm[k] * dph_up
-                            - c2a[k - 1] * s.rdnw[k - 1] / chm[k - 1]
-                            * dph_dn))
+        pg = half * dtau * g * s.rdn[k] * (
+            c2a[k] * s.rdnw[k] / chm[k] * dph_up
+            - c2a[k - 1] * s.rdnw[k - 1] / chm[k - 1] * dph_dn)
+        if cq is not None:               # moist_cq (acoustic.cu:899-905)
+            pg = cq[2][k] * pg
+        w[k] = w[k] + ((dtau * s.rw_t[k] + pg)
                        + dtau * g * (s.rdn[k] * (
                            c2a[k] * s.alt[k] * t2[k]
                            - c2a[k - 1] * s.alt[k - 1] * t2[k - 1])
@@ -1239,6 +1320,7 @@ def step(s, cfg) -> None:
         for q in ("qv", "qc", "qr") + getattr(s, "extra", ()):
             getattr(s, q + "0")[...] = getattr(s, q)
     hdiab = getattr(s, "h_diabatic", None)
+    hypso = getattr(cfg, "hypsometric_opt", 1)
     ns = cfg.time_step_sound
     stages = ((1, cfg.dt / 3.0), (max(ns // 2, 1), cfg.dt / ns),
               (ns, cfg.dt / ns))
@@ -1251,9 +1333,13 @@ def step(s, cfg) -> None:
     for istage, (nsub, dtau) in enumerate(stages):
         for name in ("ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t"):
             getattr(s, name)[...] = 0.0
-        calc_p_alpha(s, prec)
+        calc_p_alpha(s, prec, hypso)
         ru, rv, ww = stage_fluxes(s, cfg.dx, cfg.dy, prec)
-        add_slow_tendencies(s, ru, rv, ww, cfg.dx, cfg.dy, cfg.top_lid, prec)
+        # moist_cq: WRF calc_cq once per stage from the stage state
+        s.cq = (calc_cq(s, getattr(cfg, "mp_physics", 0))
+                if moist and getattr(cfg, "moist_cq", False) else None)
+        add_slow_tendencies(s, ru, rv, ww, cfg.dx, cfg.dy, cfg.top_lid, prec,
+                            getattr(cfg, "h_sca_adv_order", 2))
         if hdiab is not None:
             add_h_diabatic_tendency(s)
         if cfg.km_opt == 1:
@@ -1282,8 +1368,9 @@ def step(s, cfg) -> None:
             advance_scalars_stage(s, ru_m, rv_m, ww_m, nsub * dtau,
                                   istage == 2, cfg.dx, cfg.dy,
                                   getattr(cfg, "moist_adv_opt", 1))
+    s.cq = None
     set_w_surface(s, cfg.dx, cfg.dy)
-    calc_p_alpha(s, prec)
+    calc_p_alpha(s, prec, hypso)
     s.elapsed_seconds += cfg.dt
 
 
@@ -1328,7 +1415,7 @@ def step_advective(s, cfg) -> None:
     for dt_eff in (cfg.dt / 3.0, cfg.dt / 2.0, cfg.dt):
         for name in ("ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t"):
             getattr(s, name)[...] = 0.0
-        calc_p_alpha(s)
+        calc_p_alpha(s, False, getattr(cfg, "hypsometric_opt", 1))
         if np.any(s.w):
             raise ValueError("the advection-only step needs w == 0")
         mu = s.mub2d + s.mup

@@ -104,3 +104,21 @@ def test_microphysics_slot_runs_after_the_dynamics():
     dycore.update_diagnostics(g)
     assert torch.equal(p, g.p)
     assert not torch.equal(g.p, plain.p)
+
+
+@pytest.mark.parametrize("opt", [1, 0])
+def test_column_march_scalars_match_oracle_bitwise(opt):
+    """At 512 x 512 columns and more the step's scalar transport runs as
+    column marches (k_scalar_cols, and k_pd_scale_cols / k_pd_apply_cols
+    on the final stage); below it, one thread per point.  One WSM6 step at
+    that size against the oracle, with and without the PD limiter."""
+    from arwen_amd import dycore
+    case = cases.mixed_phase(6, nx=512, ny=512, nz=6, dx=3000.0,
+                             moist_adv_opt=opt)
+    o, g = oracle_and_gpu(case)
+    print(f"column-march scalars 512x512x6 moist_adv_opt={opt}", flush=True)
+    O.step(o, case.cfg)
+    dycore.step(g, case.cfg)
+    res = compare(o, g, BASE + EXTRA_SPECIES[6])
+    bad = {k: v for k, v in res.items() if v[1] < 1.0}
+    assert not bad, bad

@@ -114,7 +114,7 @@ MISTAKES = {
     "buoyancy dropped": lambda mp: mp.setattr(O, "slow_buoyancy",
                                               lambda s, prec=False: None),
     "rhs_ph dropped": lambda mp: mp.setattr(
-        O, "slow_geopotential", lambda s, ww, dx, dy, prec=False: None),
+        O, "slow_geopotential", lambda s, ww, dx, dy, prec=False, order=2: None),
     "large-step PGF dropped": lambda mp: mp.setattr(
         O, "slow_pgf", lambda *a, **k: None),
     "theta'' update dropped": lambda mp: mp.setattr(
