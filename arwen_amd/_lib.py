@@ -12,7 +12,7 @@ from pathlib import Path
 LIB_NAME = "libarwen_dycore.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 #: transported species beyond qv/qc/qr a domain can carry (ARW_MAX_EXTRA)
 MAX_EXTRA = 13
 
@@ -56,7 +56,7 @@ class ArwDomain(ctypes.Structure):
                 + [("h_diabatic", _P), ("hypsometric_opt", _I)]
                 + [(n, _P) for n in ("c3h", "c4h", "c3f", "c4f")]
                 + [("p_top", _D)]
-                + [(n, _P) for n in ("cqu", "cqv", "cqw")])
+                + [(n, _P) for n in ("cqu", "cqv", "cqw", "sm_buf")])
 
 
 class ArwDomain64(ctypes.Structure):
@@ -74,7 +74,9 @@ class ArwParams(ctypes.Structure):
                 ("precision_form", _I), ("diff_6th_opt", _I),
                 ("diff_6th_factor", _D), ("diff_6th_slopeopt", _I),
                 ("diff_6th_thresh", _D), ("h_sca_adv_order", _I),
-                ("moist_cq", _I)]
+                ("moist_cq", _I), ("km_opt", _I), ("c_s", _D),
+                ("bl_pbl_physics", _I), ("isfflx", _I),
+                ("tke_heat_flux", _D), ("tke_drag_coefficient", _D)]
 
 
 class ArwTracerCkpt(ctypes.Structure):

@@ -660,9 +660,9 @@ class StepTape:
 def _check_adjoint_cfg(state, cfg):
     if getattr(cfg, "precision_form", False):
         raise _lib.ArwenError("step adjoint: plain form only")
-    if getattr(cfg, "diff_6th_opt", 0):
-        raise _lib.ArwenError("step adjoint: no 6th-order filter "
-                              "(diff_6th_opt = 0)")
+    if getattr(cfg, "diff_6th_opt", 0) or getattr(cfg, "km_opt", 1) == 4:
+        raise _lib.ArwenError("step adjoint: no held mixing tendencies "
+                              "(diff_6th_opt = 0, km_opt != 4)")
     if getattr(cfg, "mp_physics", 0) != 0:
         raise _lib.ArwenError("step adjoint: qv/qc/qr states only "
                               "(mp_physics = 0, no h_diabatic)")

@@ -18,7 +18,8 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 SOURCES = [PKG / "csrc" / "dycore.hip", PKG / "csrc" / "march.hip",
            PKG / "csrc" / "acoustic_adj.hip", PKG / "csrc" / "advect_adj.hip",
-           PKG / "csrc" / "stage_adj.hip", PKG / "csrc" / "health.hip"]
+           PKG / "csrc" / "stage_adj.hip", PKG / "csrc" / "health.hip",
+           PKG / "csrc" / "smag.hip"]
 HEADERS = [PKG / "csrc" / "common.h", PKG / "csrc" / "march.h",
            ROOT / "include" / "arwen_dycore.h"]
 OUT = PKG / "libarwen_dycore.so"

@@ -48,6 +48,13 @@ class RunConfig:
     diff_6th_slopeopt: int = 0
     diff_6th_thresh: float = 0.10
     km_opt: int = 1
+    # km_opt = 4 (2-D Smagorinsky): the Smagorinsky constant and the
+    # surface policy of the dycore's vertical mixing (reference
+    # config.py:83, :345, :380-381)
+    c_s: float = 0.25
+    isfflx: int = 1
+    tke_heat_flux: float = 0.0
+    tke_drag_coefficient: float = 0.0
     w_damping: int = 0
     open_x: bool = False
     open_y: bool = False
@@ -133,10 +140,23 @@ def validate_for_step(cfg: RunConfig, *, acoustic: bool = True) -> None:
             "values.")
     if cfg.time_step_sound < 1:
         raise ValueError("time_step_sound must be positive")
-    if cfg.km_opt not in (0, 1):
+    if cfg.km_opt in (2, 3):
         raise NotImplementedError(
-            f"km_opt={cfg.km_opt} (Smagorinsky) is not part of this port; "
-            "only km_opt=1 constant-K diffusion is implemented")
+            f"km_opt={cfg.km_opt} (prognostic TKE / 3-D Smagorinsky) is not "
+            "part of this port; km_opt 0, 1 and 4 are")
+    if cfg.km_opt not in (0, 1, 4):
+        raise ValueError(f"km_opt must be 0, 1, 2, 3 or 4, got {cfg.km_opt}")
+    if cfg.km_opt == 4:
+        if cfg.moist or cfg.precision_form:
+            raise NotImplementedError(
+                "km_opt=4 is ported for dry states in the plain form")
+        if cfg.nz < 3:
+            raise ValueError("km_opt=4 needs nz >= 3 (the surface "
+                             "extrapolation reads three levels)")
+        if cfg.isfflx not in (0, 1, 2):
+            raise ValueError(f"isfflx must be 0, 1 or 2, got {cfg.isfflx}")
+        if not cfg.c_s >= 0.0:
+            raise ValueError(f"c_s = {cfg.c_s!r} must be non-negative")
     if not 0.0 < cfg.epssm <= 1.0:
         raise ValueError(f"epssm must lie in (0, 1], got {cfg.epssm}")
     if cfg.smdiv < 0.0:
@@ -190,7 +210,11 @@ def validate_for_step(cfg: RunConfig, *, acoustic: bool = True) -> None:
         raise NotImplementedError(
             "the precision form is ported for mp_physics = 0 (no retained "
             "microphysics heating, no mixed-phase species)")
-    if cfg.bl_pbl_physics or cfg.ra_physics or cfg.cu_physics \
+    # km_opt = 4 with a PBL scheme: the step mixes horizontally only and
+    # leaves the vertical to the scheme, which runs outside the step
+    # (reference dycore.py:1285)
+    pbl = cfg.bl_pbl_physics and cfg.km_opt != 4
+    if pbl or cfg.ra_physics or cfg.cu_physics \
             or cfg.sf_sfclay_physics or cfg.sf_surface_physics:
         raise NotImplementedError(
             "physics schemes are outside the dycore step port")

@@ -1274,12 +1274,12 @@ namespace {
 enum KernelId {
     KID_OMEGA = 0, KID_TEND_FACES, KID_TEND_COLS, KID_UV, KID_COLUMN,
     KID_FINISH, KID_SET_W_SURFACE, KID_SCALARS, KID_DIFF6, KID_ADD_HELD,
-    KID_COUNT
+    KID_SMAG, KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {
     "k_omega", "k_tend_faces", "k_tend_cols", "k_uv", "k_column",
     "k_finish", "k_set_w_surface", "moist_scalars", "k_diff6",
-    "k_add_held"};
+    "k_add_held", "smag_held"};
 
 struct Profiler {
     bool active = false;
@@ -1386,6 +1386,17 @@ int32_t check_params(const arw_params* p, const Dom* d)
             return fail(ARW_E_ARG, "moist_cq needs the cqu, cqv, cqw arrays");
         if (!ARW_F64 && p->precision_form)
             return fail(ARW_E_UNSUPPORTED, "moist_cq in the plain form only");
+    }
+    if (p->km_opt == 4) {
+        if (d->qv || (!ARW_F64 && p->precision_form))
+            return fail(ARW_E_UNSUPPORTED, "km_opt = 4: dry states, plain form");
+        if (d->nz < 3) return fail(ARW_E_ARG, "km_opt = 4 needs nz >= 3");
+        if (!d->sm_buf || !d->d6_u || !d->d6_v || !d->d6_w || !d->d6_t)
+            return fail(ARW_E_ARG, "km_opt = 4 needs sm_buf and the d6_* "
+                                   "held buffers");
+        if (p->isfflx < 0 || p->isfflx > 2)
+            return fail(ARW_E_ARG, "isfflx must be 0, 1 or 2");
+        if (!(p->c_s >= 0.0)) return fail(ARW_E_ARG, "c_s must be >= 0");
     }
     return ARW_OK;
 }
@@ -1700,7 +1711,7 @@ __global__ void k_diff6(Dom d, const real* __restrict__ f,
                         real* __restrict__ out, const real* __restrict__ c1,
                         const real* __restrict__ c2, real coef, int mono,
                         int slope, real dzthr_x, real dzthr_y, int nlev,
-                        int nys, int nxs, int variant, int wstag)
+                        int nys, int nxs, int variant, int wstag, int acc)
 {
     const int ny = d.ny, nx = d.nx;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1708,7 +1719,10 @@ __global__ void k_diff6(Dom d, const real* __restrict__ f,
     int k = blockIdx.z;
     if (i >= nxs || j >= nys || k >= nlev) return;
     const size_t o = ((size_t)k * nys + j) * nxs + i;
-    if (wstag && (k == 0 || k == nlev - 1)) { out[o] = R(0.0); return; }
+    if (wstag && (k == 0 || k == nlev - 1)) {
+        out[o] = acc ? out[o] + R(0.0) : R(0.0);
+        return;
+    }
     const int ic = i % nx, jc = j % ny;
     auto FX = [&](int m) { return f[((size_t)k * nys + jc) * nxs + pmod(ic + m, nx)]; };
     auto FY = [&](int m) { return f[((size_t)k * nys + pmod(jc + m, ny)) * nxs + ic]; };
@@ -1785,7 +1799,9 @@ __global__ void k_diff6(Dom d, const real* __restrict__ f,
     }
     const real ty = coef * (sy1 * (c1[k] * my1 + c2[k]) * py1
                             - sy0 * (c1[k] * my0 + c2[k]) * py0);
-    out[o] = zero + (tx + ty);
+    // a held Smagorinsky tendency in out takes the filter's on top
+    // (reference dycore.py:1415, slot += tmp)
+    out[o] = acc ? out[o] + (zero + (tx + ty)) : zero + (tx + ty);
 }
 
 // The held tendencies join every RK stage's slow tendencies, after the
@@ -1815,7 +1831,8 @@ __global__ void k_add_held(Dom d)
 
 // The four held tendencies from the time-t fields of d (u, v, w, thp and
 // mub2d + mup), once per step.
-int32_t held_diff6(const Dom& d, const arw_params* prm, hipStream_t st)
+int32_t held_diff6(const Dom& d, const arw_params* prm, hipStream_t st,
+                   int acc)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx;
     const double clock_factor = prm->diff_6th_factor;
@@ -1834,7 +1851,7 @@ int32_t held_diff6(const Dom& d, const arw_params* prm, hipStream_t st)
         hipLaunchKernelGGL(k_diff6, dim3((r.nxs + 63) / 64, (r.nys + 3) / 4, r.nlev),
                            dim3(64, 4), 0, st, d, r.f, r.out, r.c1, r.c2, coef,
                            mono, slope, tx, ty, r.nlev, r.nys, r.nxs, r.var,
-                           r.wst);
+                           r.wst, acc);
         int32_t rc = launched(hipGetLastError(), "k_diff6", KID_DIFF6, st);
         if (rc) return rc;
     }
@@ -2120,9 +2137,9 @@ int32_t ARW_API(stage_prepare)(const Dom* d, const arw_params* prm,
     if ((rc = refuse_cq(prm))) return rc;
     // the held 6th-order tendencies are formed and added by the whole step
     // only; a piecewise caller would silently miss them
-    if (prm->diff_6th_opt != 0)
-        return fail(ARW_E_UNSUPPORTED,
-                    "stage_prepare: diff_6th_opt is applied by arw_step only");
+    if (prm->diff_6th_opt != 0 || prm->km_opt == 4)
+        return fail(ARW_E_UNSUPPORTED, "stage_prepare: diff_6th_opt and "
+                                       "km_opt = 4 are applied by arw_step only");
     Sc s = make_sc(prm, dtau, 1);
     hipStream_t st = (hipStream_t)stream;
     if ((rc = stage_prepare(*d, s, do_eos, st))) return rc;
@@ -2250,8 +2267,19 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
     if (ck && hipMemcpyAsync(ck->mup_t, A[5], (size_t)d->ny * d->nx * RB,
                              hipMemcpyDeviceToDevice, st) != hipSuccess)
         return fail(ARW_E_LAUNCH, "checkpoint mu' copy failed");
-    const bool d6 = prm->diff_6th_opt > 0;
-    if (d6 && (rc = held_diff6(*d, prm, st))) return rc;
+    const bool smag = prm->km_opt == 4;
+    if (smag) {
+        // the Smagorinsky density 1/alt is the time-t state's: refresh the
+        // EOS first (reference dycore.py:2353-2354)
+        if ((rc = launched(run_eos(*d, st), "k_eos", -1, st))) return rc;
+        if ((rc = launched(run_smag(*d, prm, d->sm_buf, d->d6_u, d->d6_v,
+                                    d->d6_w, d->d6_t, st),
+                           "smag_held", KID_SMAG, st)))
+            return rc;
+    }
+    const bool d6 = smag || prm->diff_6th_opt > 0;
+    if (prm->diff_6th_opt > 0 && (rc = held_diff6(*d, prm, st, smag)))
+        return rc;
     for (int stage = 0; stage < 3; ++stage) {
         Dom dd = *d;
         // time-t fields: always A
@@ -2373,9 +2401,10 @@ int32_t ARW_API(step_advective)(const Dom* d, const arw_params* prm,
     if ((rc = check_params(prm, d))) return rc;
     if (d->qv)
         return fail(ARW_E_UNSUPPORTED, "step_advective: dry states only");
-    if ((!ARW_F64 && prm->precision_form) || prm->diff_6th_opt)
+    if ((!ARW_F64 && prm->precision_form) || prm->diff_6th_opt
+        || prm->km_opt == 4)
         return fail(ARW_E_UNSUPPORTED, "step_advective: plain form, no "
-                                       "6th-order filter");
+                                       "6th-order filter, no km_opt = 4");
     hipStream_t st = (hipStream_t)stream;
     const size_t m3 = (size_t)d->nz * d->ny * d->nx * sizeof(real);
     const size_t u3 = (size_t)d->nz * d->ny * (d->nx + 1) * sizeof(real);
@@ -2479,10 +2508,11 @@ int32_t ARW_API(step_noalloc)(const Dom* d, const arw_params* prm,
 {
     if (!d) return fail(ARW_E_ARG, "null domain");
     if (!prm) return fail(ARW_E_ARG, "null params");
-    if (d->qv || (!ARW_F64 && prm->precision_form) || prm->diff_6th_opt)
+    if (d->qv || (!ARW_F64 && prm->precision_form) || prm->diff_6th_opt
+        || prm->km_opt == 4)
         return fail(ARW_E_UNSUPPORTED,
                     "step_noalloc: dry plain-form steps without the "
-                    "6th-order filter only");
+                    "6th-order filter or km_opt = 4 only");
     if (d->nz < 3 || d->ny < 1 || d->nx < 1)
         return fail(ARW_E_ARG, "need nz >= 3, ny >= 1, nx >= 1");
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
@@ -2708,8 +2738,9 @@ int32_t ARW_API(ensemble_create)(const Dom* members, int32_t n,
 {
     if (!out) return fail(ARW_E_ARG, "null output handle");
     *out = nullptr;
-    if (prm && prm->diff_6th_opt != 0)
-        return fail(ARW_E_ARG, "ensemble: diff_6th_opt is not supported");
+    if (prm && (prm->diff_6th_opt != 0 || prm->km_opt == 4))
+        return fail(ARW_E_ARG, "ensemble: diff_6th_opt and km_opt = 4 are "
+                               "not supported");
     if (!members || n < 1 || n > 65536)
         return fail(ARW_E_ARG, "need 1 <= n <= 65536 members");
     if (groups < 1 || groups > n)

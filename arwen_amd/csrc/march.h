@@ -101,6 +101,13 @@ __device__ __forceinline__ const FinishDst& fin_of(const Batch& b, int m,
 // Error reporting and argument checks shared by the library's sources
 // (defined in dycore.hip).
 int32_t lib_fail(int32_t code, const char* msg);
+
+// WRF km_opt = 4 held tendencies of the time-t fields of d into out_u,
+// out_v, out_w, out_t (smag.hip); buf: 6 nz ny nx + 2 (nz+1) ny nx
+// elements of scratch
+hipError_t run_smag(const Dom& d, const arw_params* prm, real* buf,
+                    real* out_u, real* out_v, real* out_w, real* out_t,
+                    hipStream_t st);
 int32_t lib_launched(hipError_t e, const char* what);
 int32_t lib_check_domain(const Dom* d);
 int32_t lib_check_adj_domain(const Dom* d);

@@ -57,6 +57,12 @@ def params(cfg: RunConfig) -> _lib.ArwParams:
     p.diff_6th_thresh = float(cfg.diff_6th_thresh)
     p.h_sca_adv_order = int(cfg.h_sca_adv_order)
     p.moist_cq = moist_cq_species(cfg)
+    p.km_opt = int(cfg.km_opt)
+    p.c_s = float(cfg.c_s)
+    p.bl_pbl_physics = int(cfg.bl_pbl_physics)
+    p.isfflx = int(cfg.isfflx)
+    p.tke_heat_flux = float(cfg.tke_heat_flux)
+    p.tke_drag_coefficient = float(cfg.tke_drag_coefficient)
     return p
 
 
@@ -116,7 +122,8 @@ def step(state: DomainState, cfg: RunConfig, *, acoustic: bool = True,
                             precision_form=bool(cfg.precision_form),
                             diff6=cfg.diff_6th_opt > 0,
                             hypsometric_opt=cfg.hypsometric_opt,
-                            moist_cq=bool(cfg.moist_cq))
+                            moist_cq=bool(cfg.moist_cq),
+                            smag=cfg.km_opt == 4)
     p = params(cfg)
     name, fn = _fn(state, "step")
     _lib.check(fn(d, p, _stream()), name)

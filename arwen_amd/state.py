@@ -249,7 +249,8 @@ class DomainState:
                       precision_form: bool = False,
                       diff6: bool = False,
                       hypsometric_opt: int | None = None,
-                      moist_cq: bool = False) -> ArwDomain:
+                      moist_cq: bool = False,
+                      smag: bool = False) -> ArwDomain:
         """Raw-pointer view of the state for the C-ABI (a fresh copy the
         caller may modify).  Built once per option set and cached until an
         array attribute is rebound or a scratch slot is created; arrays
@@ -257,7 +258,7 @@ class DomainState:
         if hypsometric_opt is None:
             hypsometric_opt = getattr(self.cfg, "hypsometric_opt", 1)
         key = (bool(diffusion), bool(precision_form), bool(diff6),
-               int(hypsometric_opt), bool(moist_cq))
+               int(hypsometric_opt), bool(moist_cq), bool(smag))
         memo = self.__dict__.get("_dom_memo")
         if memo is not None and key in memo:
             m = memo[key]
@@ -268,7 +269,7 @@ class DomainState:
 
     def _build_domain_struct(self, diffusion: bool, precision_form: bool,
                              diff6: bool, hypsometric_opt: int,
-                             moist_cq: bool) -> ArwDomain:
+                             moist_cq: bool, smag: bool) -> ArwDomain:
         nz, ny, nx = self.shape
         d = ArwDomain64() if self.dtype == torch.float64 else ArwDomain()
         d.nz, d.ny, d.nx = nz, ny, nx
@@ -335,7 +336,12 @@ class DomainState:
             d.tri64 = self.scratch((2, nz + 1, ny, nx), "arw_tri64",
                                    dtype=torch.float64).data_ptr()
             d.pp = self.scratch((nz, ny, nx), "arw_pp").data_ptr()
-        if diff6:
+        if smag:
+            # km_opt = 4: the deformation, K and stress work areas
+            n, nw = nz * ny * nx, (nz + 1) * ny * nx
+            d.sm_buf = self.scratch((6 * n + 2 * nw,),
+                                    "arw_sm_buf").data_ptr()
+        if diff6 or smag:
             shapes = {"d6_u": (nz, ny, nx + 1), "d6_v": (nz, ny + 1, nx),
                       "d6_w": (nz + 1, ny, nx), "d6_t": (nz, ny, nx)}
             for name, shp in shapes.items():

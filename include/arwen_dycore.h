@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define ARW_ABI_VERSION 8
+#define ARW_ABI_VERSION 9
 
 /* transported species beyond qv/qc/qr a domain can carry (NSSL's set) */
 #define ARW_MAX_EXTRA 13
@@ -115,6 +115,9 @@ typedef struct arw_domain {
     double p_top;
     /* moist_cq's stage coefficients (u, v and w shapes; scratch) */
     float *cqu, *cqv, *cqw;
+    /* km_opt = 4 (arw_params): scratch of 6 nz ny nx + 2 (nz+1) ny nx
+     * elements for the deformation, the coefficients and the stresses */
+    float *sm_buf;
 } arw_domain;
 
 /*
@@ -152,6 +155,7 @@ typedef struct arw64_domain {
     const double *c3h, *c4h, *c3f, *c4f;
     double p_top;
     double *cqu, *cqv, *cqw;
+    double *sm_buf;
 } arw64_domain;
 
 /* Step configuration (the RunConfig fields the step reads,
@@ -196,6 +200,24 @@ typedef struct arw_params {
      * acoustic horizontal pressure gradients and the implicit w solve's
      * pressure terms; needs d->cqu, cqv, cqw.  Moist plain-form steps. */
     int32_t moist_cq;
+    /* WRF diff_opt = 2 eddy viscosity (reference dycore.py:1223-1343,
+     * kernels/smag2d.cu:272-1340): km_opt = 4 holds the 2-D Smagorinsky
+     * stress divergences of u, v, w and the theta flux divergence of the
+     * time-t fields, once per step, and adds them to every RK stage's
+     * tendencies (before a 6th-order filter's); needs d->sm_buf and the
+     * d6_* held buffers.  c_s is the Smagorinsky constant.  bl_pbl_physics
+     * 0: the step also mixes vertically, with the surface policy isfflx
+     * (0: the constant drag coefficient tke_drag_coefficient and the
+     * kinematic heat flux tke_heat_flux, 2: the heat flux, 1: neither --
+     * no surface-layer scheme runs inside the step); otherwise a PBL
+     * scheme outside the step owns the vertical.  Other km_opt values
+     * leave this off (km_opt = 1's constant K is khdif / kvdif).
+     * arw_step only; dry states, plain form, nz >= 3. */
+    int32_t km_opt;
+    double c_s;
+    int32_t bl_pbl_physics;
+    int32_t isfflx;
+    double tke_heat_flux, tke_drag_coefficient;
 } arw_params;
 
 int32_t arw_abi_version(void);

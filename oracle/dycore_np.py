@@ -1328,8 +1328,16 @@ def step(s, cfg) -> None:
         and s.dtype == np.float32
     if hdiab is not None and prec:
         raise ValueError("the precision form carries no h_diabatic removal")
-    held = diff6_held(s, cfg) if getattr(cfg, "diff_6th_opt", 0) > 0 \
-        else {}
+    held = {}
+    if getattr(cfg, "km_opt", 1) == 4:
+        # the Smagorinsky density 1/alt of the time-t state (reference
+        # dycore.py:2353-2354 refreshes the EOS first)
+        from .smag_np import smag_held
+        calc_p_alpha(s, prec, hypso)
+        held = smag_held(s, cfg)
+    if getattr(cfg, "diff_6th_opt", 0) > 0:
+        d6 = diff6_held(s, cfg)
+        held = {n: held[n] + d6[n] for n in d6} if held else d6
     for istage, (nsub, dtau) in enumerate(stages):
         for name in ("ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t"):
             getattr(s, name)[...] = 0.0
@@ -1344,7 +1352,7 @@ def step(s, cfg) -> None:
             add_h_diabatic_tendency(s)
         if cfg.km_opt == 1:
             add_diffusion_tendencies(s, cfg.khdif, cfg.kvdif, cfg.dx, cfg.dy)
-        for name, h in held.items():       # held 6th-order tendencies
+        for name, h in held.items():       # held Smagorinsky / 6th-order
             getattr(s, name)[...] = getattr(s, name) + h
         small_step_init(s, prec)
         coefs = calc_coefs(s, dtau, cfg.epssm, cfg.top_lid, prec)
