@@ -56,7 +56,8 @@ class ArwDomain(ctypes.Structure):
                 + [("h_diabatic", _P), ("hypsometric_opt", _I)]
                 + [(n, _P) for n in ("c3h", "c4h", "c3f", "c4f")]
                 + [("p_top", _D)]
-                + [(n, _P) for n in ("cqu", "cqv", "cqw", "sm_buf")])
+                + [(n, _P) for n in ("cqu", "cqv", "cqw", "sm_buf",
+                                     "held_q")])
 
 
 class ArwDomain64(ctypes.Structure):
@@ -76,7 +77,8 @@ class ArwParams(ctypes.Structure):
                 ("diff_6th_thresh", _D), ("h_sca_adv_order", _I),
                 ("moist_cq", _I), ("km_opt", _I), ("c_s", _D),
                 ("bl_pbl_physics", _I), ("isfflx", _I),
-                ("tke_heat_flux", _D), ("tke_drag_coefficient", _D)]
+                ("tke_heat_flux", _D), ("tke_drag_coefficient", _D),
+                ("diff6_species", ctypes.c_uint32)]
 
 
 class ArwTracerCkpt(ctypes.Structure):

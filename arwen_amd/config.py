@@ -47,6 +47,10 @@ class RunConfig:
     diff_6th_factor: float = 0.12
     diff_6th_slopeopt: int = 0
     diff_6th_thresh: float = 0.10
+    # WRF moist_mix6_off: the filter skips the moist array (qv, qc, qr,
+    # qi, qs, qg, qh); number and volume tracers keep it (reference
+    # config.py:529-551)
+    moist_mix6_off: bool = False
     km_opt: int = 1
     # km_opt = 4 (2-D Smagorinsky): the Smagorinsky constant and the
     # surface policy of the dycore's vertical mixing (reference
@@ -147,9 +151,9 @@ def validate_for_step(cfg: RunConfig, *, acoustic: bool = True) -> None:
     if cfg.km_opt not in (0, 1, 4):
         raise ValueError(f"km_opt must be 0, 1, 2, 3 or 4, got {cfg.km_opt}")
     if cfg.km_opt == 4:
-        if cfg.moist or cfg.precision_form:
+        if cfg.precision_form:
             raise NotImplementedError(
-                "km_opt=4 is ported for dry states in the plain form")
+                "km_opt=4 is ported for the plain form")
         if cfg.nz < 3:
             raise ValueError("km_opt=4 needs nz >= 3 (the surface "
                              "extrapolation reads three levels)")
@@ -197,9 +201,9 @@ def validate_for_step(cfg: RunConfig, *, acoustic: bool = True) -> None:
     if cfg.diff_6th_opt not in (0, 1, 2):
         raise ValueError(f"diff_6th_opt must be 0, 1 or 2, got "
                          f"{cfg.diff_6th_opt}")
-    if cfg.diff_6th_opt and (cfg.moist or cfg.precision_form):
+    if cfg.diff_6th_opt and cfg.precision_form:
         raise NotImplementedError(
-            "diff_6th_opt is ported for dry states in the plain form")
+            "diff_6th_opt is ported for the plain form")
     if cfg.mp_physics not in EXTRA_SPECIES:
         raise NotImplementedError(
             f"mp_physics={cfg.mp_physics}: the step transports the species "

@@ -480,7 +480,8 @@ __global__ void k_scalar_update(Dom d, const real* __restrict__ qcur,
                                 const real* __restrict__ q0,
                                 real* __restrict__ qdst,
                                 const real* __restrict__ mup0, real dt,
-                                int clamp, real rdx, real rdy)
+                                int clamp, real rdx, real rdy,
+                                const real* __restrict__ fixed)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -489,10 +490,28 @@ __global__ void k_scalar_update(Dom d, const real* __restrict__ qcur,
     if (i >= nx || j >= ny || k >= nz) return;
     Arr3 Q{qcur, ny, nx}, RU{d.ru_m, ny, nx + 1}, RV{d.rv_m, ny + 1, nx},
         RW{d.ww_m, ny, nx};
-    const real tend = R(0.0) + adv_scalar(Q, RU, RV, RW, k, j, i, nz, ny, nx,
-                                         d.rdnw, d.fnm, d.fnp, rdx, rdy);
+    real tend = R(0.0) + adv_scalar(Q, RU, RV, RW, k, j, i, nz, ny, nx,
+                                   d.rdnw, d.fnm, d.fnp, rdx, rdy);
     const int x = iM(k, j, i, ny, nx);
+    if (fixed) tend = tend + fixed[x];       // held mixing (moist.py:384-386)
     qdst[x] = scalar_update(d, k, j * nx + i, q0[x], tend, mup0, dt, clamp);
+}
+
+// The held tendency folded into the time-t scalar ahead of the final
+// stage's limiter, in place: h = q0 + dt h / C(mu_t) (reference moist.py:
+// 293-346 _pd_fold_sources).
+__global__ void k_pd_fold(Dom d, const real* __restrict__ q0,
+                          real* __restrict__ h, const real* __restrict__ mup0,
+                          real dt)
+{
+    const int nz = d.nz, ny = d.ny, nx = d.nx;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    const int c = j * nx + i, x = iM(k, j, i, ny, nx);
+    const real chm0 = d.c1h[k] * (d.mub2d[c] + mup0[c]) + d.c2h[k];
+    h[x] = q0[x] + (dt * h[x]) / chm0;
 }
 
 // WRF flux_upwind (reference pd_advection.cu:101-108).
@@ -1388,8 +1407,10 @@ int32_t check_params(const arw_params* p, const Dom* d)
             return fail(ARW_E_UNSUPPORTED, "moist_cq in the plain form only");
     }
     if (p->km_opt == 4) {
-        if (d->qv || (!ARW_F64 && p->precision_form))
-            return fail(ARW_E_UNSUPPORTED, "km_opt = 4: dry states, plain form");
+        if (!ARW_F64 && p->precision_form)
+            return fail(ARW_E_UNSUPPORTED, "km_opt = 4: plain form only");
+        if (d->qv && !d->held_q)
+            return fail(ARW_E_ARG, "km_opt = 4 on a moist state needs held_q");
         if (d->nz < 3) return fail(ARW_E_ARG, "km_opt = 4 needs nz >= 3");
         if (!d->sm_buf || !d->d6_u || !d->d6_v || !d->d6_w || !d->d6_t)
             return fail(ARW_E_ARG, "km_opt = 4 needs sm_buf and the d6_* "
@@ -1465,14 +1486,33 @@ constexpr long kScalarColumnsMin = 512L * 512;
 // The limited update reads q0's neighbours while it runs, so it goes to
 // spare (a dead buffer of the stage rotation) and is copied to dst.
 int32_t scalar_stage(const Dom& dd, const arw_params* prm, real* const* cur,
-                     real* const* q0, real* const* dst, const real* mup0,
+                     real* const* q0_in, real* const* dst, const real* mup0,
                      double dt_stage, bool final, hipStream_t st, int nq,
-                     real* const* spare)
+                     real* const* spare, real* const* fixed)
 {
     const real dt = (real)dt_stage;
     const real rdx = (real)(1.0 / prm->dx), rdy = (real)(1.0 / prm->dy);
     const bool pd = final && prm->moist_adv_opt == 1;
     int32_t rc;
+    const real* q0[3 + ARW_MAX_EXTRA];
+    const real* fx[3 + ARW_MAX_EXTRA];
+    for (int q = 0; q < nq; ++q) {
+        q0[q] = q0_in[q];
+        fx[q] = fixed ? fixed[q] : nullptr;
+        if (pd && fx[q]) {
+            // the limiter advects q_t + dt fixed / C(mu_t): fold the held
+            // tendency into the time-t scalar in its own slot (its last use)
+            hipLaunchKernelGGL(k_pd_fold, dim3((dd.nx + 63) / 64,
+                                               (dd.ny + 3) / 4, dd.nz),
+                               dim3(64, 4), 0, st, dd, q0_in[q], fixed[q],
+                               mup0, dt);
+            if ((rc = launched(hipGetLastError(), "k_pd_fold", KID_SCALARS,
+                               st)))
+                return rc;
+            q0[q] = fixed[q];
+            fx[q] = nullptr;
+        }
+    }
     if ((long)dd.ny * dd.nx < kScalarColumnsMin) {
         // few columns: one thread per point (stored limiter fluxes)
         const dim3 cells((dd.nx + 63) / 64, (dd.ny + 3) / 4, dd.nz);
@@ -1488,7 +1528,7 @@ int32_t scalar_stage(const Dom& dd, const arw_params* prm, real* const* cur,
             } else {
                 hipLaunchKernelGGL(k_scalar_update, cells, dim3(64, 4), 0, st,
                                    dd, cur[q], q0[q], dst[q], mup0, dt,
-                                   final ? 1 : 0, rdx, rdy);
+                                   final ? 1 : 0, rdx, rdy, fx[q]);
             }
             if ((rc = launched(hipGetLastError(), "scalar stage", KID_SCALARS,
                                st)))
@@ -1518,6 +1558,7 @@ int32_t scalar_stage(const Dom& dd, const arw_params* prm, real* const* cur,
             g.cur[e] = cur[k];
             g.q0[e] = q0[k];
             g.dst[e] = dst[k];
+            g.fixed[e] = fx[k];
         }
         if ((rc = launched(run_scalar_cols(dd, g, mup0, dt, rdx, rdy,
                                            final ? 1 : 0, st),
@@ -1832,7 +1873,7 @@ __global__ void k_add_held(Dom d)
 // The four held tendencies from the time-t fields of d (u, v, w, thp and
 // mub2d + mup), once per step.
 int32_t held_diff6(const Dom& d, const arw_params* prm, hipStream_t st,
-                   int acc)
+                   int acc, real* const* q_src, int nq)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx;
     const double clock_factor = prm->diff_6th_factor;
@@ -1852,6 +1893,18 @@ int32_t held_diff6(const Dom& d, const arw_params* prm, hipStream_t st,
                            dim3(64, 4), 0, st, d, r.f, r.out, r.c1, r.c2, coef,
                            mono, slope, tx, ty, r.nlev, r.nys, r.nxs, r.var,
                            r.wst, acc);
+        int32_t rc = launched(hipGetLastError(), "k_diff6", KID_DIFF6, st);
+        if (rc) return rc;
+    }
+    // the transported species the filter acts on (mass rows, reference
+    // dycore.py:1387-1415), into their held slots
+    const size_t n = (size_t)nz * ny * nx;
+    for (int e = 0; e < nq; ++e) {
+        if (!((prm->diff6_species >> e) & 1u)) continue;
+        hipLaunchKernelGGL(k_diff6, dim3((nx + 63) / 64, (ny + 3) / 4, nz),
+                           dim3(64, 4), 0, st, d, q_src[e], d.held_q + e * n,
+                           d.c1h, d.c2h, coef, mono, slope, tx, ty, nz, ny, nx,
+                           0, 0, acc);
         int32_t rc = launched(hipGetLastError(), "k_diff6", KID_DIFF6, st);
         if (rc) return rc;
     }
@@ -2235,8 +2288,10 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
             return fail(ARW_E_ARG, "diff_6th_opt must be 0, 1 or 2");
         if (!d->d6_u || !d->d6_v || !d->d6_w || !d->d6_t)
             return fail(ARW_E_ARG, "diff_6th_opt needs the d6_* buffers");
-        if (d->qv || prec)
-            return fail(ARW_E_ARG, "diff_6th_opt: dry states, plain form");
+        if (prec) return fail(ARW_E_ARG, "diff_6th_opt: plain form only");
+        if (d->qv && !d->held_q)
+            return fail(ARW_E_ARG, "diff_6th_opt on a moist state needs "
+                                   "held_q");
     }
     if (ck) {
         if (!ck->mup_t) return fail(ARW_E_ARG, "checkpoint: null mup_t");
@@ -2273,13 +2328,21 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
         // EOS first (reference dycore.py:2353-2354)
         if ((rc = launched(run_eos(*d, st), "k_eos", -1, st))) return rc;
         if ((rc = launched(run_smag(*d, prm, d->sm_buf, d->d6_u, d->d6_v,
-                                    d->d6_w, d->d6_t, st),
+                                    d->d6_w, d->d6_t, QA, d->held_q,
+                                    moist ? nq : 0, st),
                            "smag_held", KID_SMAG, st)))
             return rc;
     }
     const bool d6 = smag || prm->diff_6th_opt > 0;
-    if (prm->diff_6th_opt > 0 && (rc = held_diff6(*d, prm, st, smag)))
+    if (prm->diff_6th_opt > 0
+        && (rc = held_diff6(*d, prm, st, smag, QA, moist ? nq : 0)))
         return rc;
+    // the species' held tendencies: Smagorinsky's for every species, the
+    // filter's for those it acts on; null where nothing is held
+    real* fixed[3 + ARW_MAX_EXTRA] = {};
+    for (int e = 0; moist && e < nq; ++e)
+        if (smag || (prm->diff_6th_opt > 0 && ((prm->diff6_species >> e) & 1u)))
+            fixed[e] = d->held_q + (size_t)e * d->nz * d->ny * d->nx;
     for (int stage = 0; stage < 3; ++stage) {
         Dom dd = *d;
         // time-t fields: always A
@@ -2361,7 +2424,7 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
             // QB (stage 1's output) is dead during the final stage
             if ((rc = scalar_stage(dd, prm, Q, QA, Dst, A[5],
                                    nsub[stage] * dtaus[stage], stage == 2, st,
-                                   nq, QB)))
+                                   nq, QB, fixed)))
                 return rc;
         }
         // the final stage removes the heating its substeps integrated

@@ -107,6 +107,7 @@ int32_t lib_fail(int32_t code, const char* msg);
 // elements of scratch
 hipError_t run_smag(const Dom& d, const arw_params* prm, real* buf,
                     real* out_u, real* out_v, real* out_w, real* out_t,
+                    const real* const* q_src, real* held_q, int nq,
                     hipStream_t st);
 int32_t lib_launched(hipError_t e, const char* what);
 int32_t lib_check_domain(const Dom* d);
@@ -131,6 +132,7 @@ struct QGroup {
     const real* cur[4];
     const real* q0[4];
     real* dst[4];
+    const real* fixed[4];   // held mixing tendencies, or null
     int n;
 };
 

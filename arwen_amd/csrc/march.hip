@@ -2022,8 +2022,10 @@ __global__ __launch_bounds__(256) void k_scalar_cols(Dom d, QGroup g,
             const real fy1 = flux5(y[1], y[2], y[3], y[4], y[5], y[6], rv1);
             const real fz1 = zf(qz[s][1], qz[s][2], qz[s][3], qz[s][4], ww1,
                                 k + 1, nz, fnm, fnp);
-            const real tend = R(0.0) + div3(fx1 - fx0, fy1 - fy0,
-                                            fz1 - fz0[s], rdx, rdy, rdnw[k]);
+            real tend = R(0.0) + div3(fx1 - fx0, fy1 - fy0,
+                                      fz1 - fz0[s], rdx, rdy, rdnw[k]);
+            if (g.fixed[s])              // held mixing (moist.py:384-386)
+                tend = tend + gld(g.fixed[s], lm + q.rA, cC);
             fz0[s] = fz1;
             real v = chm0 * gld(g.q0[s], lm + q.rA, cC);
             const real t = dt * tend;

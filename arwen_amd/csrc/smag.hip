@@ -75,10 +75,12 @@ __device__ __forceinline__ real zy(const Dom& d, real rdy, int kw, int j,
     return (rdy * (phi(d, kw, j, i) - phi(d, kw, j - 1, i))) / G_F;
 }
 
-// dry density 1 / alt (smag2d.cu:394-404)
+// density with the vapour loading, (1 + qv) / alt, and exactly 1 / alt
+// for a dry state (smag2d.cu:394-404)
 __device__ __forceinline__ real rho(const Dom& d, int k, int j, int i)
 {
-    return R(1.0) / d.alt[mix(d, k, j, i)];
+    const int x = mix(d, k, j, i);
+    return d.qv ? (R(1.0) + d.qv[x]) / d.alt[x] : R(1.0) / d.alt[x];
 }
 
 __device__ __forceinline__ real U(const Dom& d, int k, int j, int i)
@@ -402,22 +404,27 @@ __device__ __forceinline__ real theta2(const Dom& d, int k, int j, int i)
     return (( d.base3d ? d.thb[x] : d.thb[k]) + d.thp[x]) - R(300.0);
 }
 
-// the theta face fluxes H1 (u face i) and H2 (v face j)
-// (smag2d.cu:1164-1200, wrf_smag_flux_s :1236-1251)
-__global__ void k_smag_flux(Dom d, SmagSc c, SmagBuf b)
+// the scalar face fluxes H1 (u face i) and H2 (v face j) of theta's t_2
+// (src null) or of a species' mixing ratio src (smag2d.cu:1164-1200,
+// wrf_smag_flux_s :1236-1251)
+__global__ void k_smag_flux(Dom d, SmagSc c, SmagBuf b,
+                            const real* __restrict__ src)
 {
     SMAG_POINT(d.nx, d.ny, d.nz)
     const real h = R(0.5), one = R(1.0), two = R(2.0);
     const int x = iM(k, j, i, d.ny, d.nx);
-    const real s0 = theta2(d, k, j, i);
+    auto sv = [&](int kk, int jj, int ii) {
+        return src ? src[mix(d, kk, jj, ii)] : theta2(d, kk, jj, ii);
+    };
+    const real s0 = sv(k, j, i);
     const real rz = rdzw(d, k, j, i);
     {
         const real ra = h * (rho(d, k, j, i - 1) + rho(d, k, j, i));
         const real ka = h * (at(d, b.kh, k, j, i - 1) + b.kh[x]);
         const real tz = h * (zx(d, c.rdx, k, j, i) + zx(d, c.rdx, k + 1, j, i));
         const real rdzu = two / (one / rz + one / rdzw(d, k, j, i - 1));
-        auto p = [&](int kk) { return theta2(d, kk, j, i - 1) + theta2(d, kk, j, i); };
-        const real grad = c.rdx * (s0 - theta2(d, k, j, i - 1))
+        auto p = [&](int kk) { return sv(kk, j, i - 1) + sv(kk, j, i); };
+        const real grad = c.rdx * (s0 - sv(k, j, i - 1))
                           - (tz * (full(d, k + 1, p) - full(d, k, p))) * rdzu;
         b.t11[x] = ((-ra) * ka) * grad;
     }
@@ -426,16 +433,18 @@ __global__ void k_smag_flux(Dom d, SmagSc c, SmagBuf b)
         const real ka = h * (at(d, b.kh, k, j - 1, i) + b.kh[x]);
         const real tz = h * (zy(d, c.rdy, k, j, i) + zy(d, c.rdy, k + 1, j, i));
         const real rdzv = two / (one / rz + one / rdzw(d, k, j - 1, i));
-        auto p = [&](int kk) { return theta2(d, kk, j - 1, i) + theta2(d, kk, j, i); };
-        const real grad = c.rdy * (s0 - theta2(d, k, j - 1, i))
+        auto p = [&](int kk) { return sv(kk, j - 1, i) + sv(kk, j, i); };
+        const real grad = c.rdy * (s0 - sv(k, j - 1, i))
                           - (tz * (full(d, k + 1, p) - full(d, k, p))) * rdzv;
         b.t22[x] = ((-ra) * ka) * grad;
     }
 }
 
-// theta: the flux divergence (wrf_smag_hd_s, smag2d.cu:1254-1292) and the
-// constant surface heat flux under isfflx 0 / 2 (:1658-1668)
-__global__ void k_smag_theta(Dom d, SmagSc c, SmagBuf b, real* __restrict__ out)
+// a scalar's flux divergence (wrf_smag_hd_s, smag2d.cu:1254-1292), with
+// theta's constant surface heat flux under isfflx 0 / 2 when heat is set
+// (:1658-1668)
+__global__ void k_smag_theta(Dom d, SmagSc c, SmagBuf b, real* __restrict__ out,
+                             int heat)
 {
     SMAG_POINT(d.nx, d.ny, d.nz)
     const real q = R(0.25);
@@ -454,7 +463,7 @@ __global__ void k_smag_theta(Dom d, SmagSc c, SmagBuf b, real* __restrict__ out)
                       + (zym * (mavg(d, fy, k + 1, j, i, 1, 0)
                                 - mavg(d, fy, k, j, i, 1, 0))) * rz;
     real r = R(0.0) + (G_F / (d.dnw[k] * rz)) * (divh - divz);
-    if (c.heatflux && k == 0)
+    if (heat && c.heatflux && k == 0)
         r = r - ((G_F * c.heat) * rho(d, 0, j, i)) / d.dnw[0];
     out[x] = r;
 }
@@ -462,10 +471,12 @@ __global__ void k_smag_theta(Dom d, SmagSc c, SmagBuf b, real* __restrict__ out)
 }  // namespace
 
 // The held km_opt = 4 tendencies of the time-t fields of d (u, v, w, thp,
-// php and the EOS's alt) into out_u, out_v, out_w, out_t, written (not
+// php, the EOS's alt and, moist, qv) into out_u, out_v, out_w, out_t and
+// of the nq species q_src[e] into held_q + e nz ny nx, written (not
 // added); buf holds 6 nz ny nx + 2 (nz+1) ny nx elements of scratch.
 hipError_t run_smag(const Dom& d, const arw_params* prm, real* buf,
                     real* out_u, real* out_v, real* out_w, real* out_t,
+                    const real* const* q_src, real* held_q, int nq,
                     hipStream_t st)
 {
     const int nz = d.nz, ny = d.ny, nx = d.nx;
@@ -502,8 +513,16 @@ hipError_t run_smag(const Dom& d, const arw_params* prm, real* buf,
     hipLaunchKernelGGL(k_smag_u, grid(nx + 1, ny, nz), blk, 0, st, d, c, b, out_u);
     hipLaunchKernelGGL(k_smag_v, grid(nx, ny + 1, nz), blk, 0, st, d, c, b, out_v);
     hipLaunchKernelGGL(k_smag_w, grid(nx, ny, nz + 1), blk, 0, st, d, c, b, out_w);
-    hipLaunchKernelGGL(k_smag_flux, grid(nx, ny, nz), blk, 0, st, d, c, b);
-    hipLaunchKernelGGL(k_smag_theta, grid(nx, ny, nz), blk, 0, st, d, c, b, out_t);
+    hipLaunchKernelGGL(k_smag_flux, grid(nx, ny, nz), blk, 0, st, d, c, b,
+                       (const real*)nullptr);
+    hipLaunchKernelGGL(k_smag_theta, grid(nx, ny, nz), blk, 0, st, d, c, b,
+                       out_t, 1);
+    for (int e = 0; e < nq; ++e) {
+        hipLaunchKernelGGL(k_smag_flux, grid(nx, ny, nz), blk, 0, st, d, c, b,
+                           q_src[e]);
+        hipLaunchKernelGGL(k_smag_theta, grid(nx, ny, nz), blk, 0, st, d, c, b,
+                           held_q + (size_t)e * n, 0);
+    }
     return hipGetLastError();
 }
 

@@ -14,7 +14,8 @@ import math
 import torch
 
 from . import _lib
-from .config import LOADING_EXTRA, RunConfig, diff6_factor, validate_for_step
+from .config import (EXTRA_SPECIES, LOADING_EXTRA, RunConfig, diff6_factor,
+                     validate_for_step)
 from .state import DomainState, mu_at_u_faces, mu_at_v_faces
 
 
@@ -63,7 +64,26 @@ def params(cfg: RunConfig) -> _lib.ArwParams:
     p.isfflx = int(cfg.isfflx)
     p.tke_heat_flux = float(cfg.tke_heat_flux)
     p.tke_drag_coefficient = float(cfg.tke_drag_coefficient)
+    p.diff6_species = diff6_species_mask(cfg)
     return p
+
+
+#: WRF's moist array: moist_mix6_off exempts these species from the
+#: 6th-order filter (reference moist.py:172-173, dycore.py:1200-1220)
+MOIST_ARRAY = frozenset(("qv", "qc", "qr", "qi", "qs", "qg", "qh"))
+
+
+def diff6_species_mask(cfg: RunConfig) -> int:
+    """arw_params.diff6_species: bit e for species e (qv, qc, qr, then the
+    scheme's extras) the 6th-order filter acts on."""
+    if not (cfg.moist and cfg.diff_6th_opt):
+        return 0
+    names = ("qv", "qc", "qr") + EXTRA_SPECIES[cfg.mp_physics]
+    mask = 0
+    for e, n in enumerate(names):
+        if not (cfg.moist_mix6_off and n in MOIST_ARRAY):
+            mask |= 1 << e
+    return mask
 
 
 def moist_cq_species(cfg: RunConfig) -> int:

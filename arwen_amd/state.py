@@ -341,6 +341,11 @@ class DomainState:
             n, nw = nz * ny * nx, (nz + 1) * ny * nx
             d.sm_buf = self.scratch((6 * n + 2 * nw,),
                                     "arw_sm_buf").data_ptr()
+        if (diff6 or smag) and self.qv is not None:
+            # every transported species' held mixing tendency
+            nq = 3 + len(self.extra_species)
+            d.held_q = self.scratch((nq, nz, ny, nx),
+                                    "arw_held_q").data_ptr()
         if diff6 or smag:
             shapes = {"d6_u": (nz, ny, nx + 1), "d6_v": (nz, ny + 1, nx),
                       "d6_w": (nz + 1, ny, nx), "d6_t": (nz, ny, nx)}

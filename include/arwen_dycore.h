@@ -118,6 +118,10 @@ typedef struct arw_domain {
     /* km_opt = 4 (arw_params): scratch of 6 nz ny nx + 2 (nz+1) ny nx
      * elements for the deformation, the coefficients and the stresses */
     float *sm_buf;
+    /* moist states with held mixing (km_opt = 4 or diff_6th_opt): the held
+     * tendency of every transported species, nz ny nx elements each, in
+     * the order qv, qc, qr, then the extras */
+    float *held_q;
 } arw_domain;
 
 /*
@@ -156,6 +160,7 @@ typedef struct arw64_domain {
     double p_top;
     double *cqu, *cqv, *cqw;
     double *sm_buf;
+    double *held_q;
 } arw64_domain;
 
 /* Step configuration (the RunConfig fields the step reads,
@@ -218,6 +223,13 @@ typedef struct arw_params {
     int32_t bl_pbl_physics;
     int32_t isfflx;
     double tke_heat_flux, tke_drag_coefficient;
+    /* moist states under diff_6th_opt: bit e set filters species e (qv,
+     * qc, qr, then the extras); WRF's moist_mix6_off clears the moist
+     * array's bits (reference dycore.py:1200-1220).  The held tendencies
+     * of the species (d->held_q) join the unlimited scalar updates of
+     * every RK stage and are folded into the time-t scalar ahead of the
+     * final stage's positive-definite limiter (moist.py:293-346). */
+    uint32_t diff6_species;
 } arw_params;
 
 int32_t arw_abi_version(void);

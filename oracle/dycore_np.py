@@ -1158,13 +1158,14 @@ def sumflux_finish(ru_m, rv_m, ww_m, ru, rv, ww, nsub):
 
 
 def advance_scalars_stage(s, ru_m, rv_m, ww_m, dt_stage, final, dx, dy,
-                          moist_adv_opt=1):
+                          moist_adv_opt=1, fixed=None):
     """Coupled qv/qc/qr stage update with the substep-averaged fluxes
     (reference moist.py:434-698, _update_scalar_kernel :353-402) and the
-    positive-definite limiter on the final stage (pd_advection.cu)."""
+    positive-definite limiter on the final stage (pd_advection.cu);
+    ``fixed`` holds the held mixing tendencies per species."""
     from oracle import moist_np
     moist_np.advance_scalars_stage(s, ru_m, rv_m, ww_m, dt_stage, final,
-                                   dx, dy, moist_adv_opt)
+                                   dx, dy, moist_adv_opt, fixed)
 
 
 # ---------------------------------------------------------------------------
@@ -1297,14 +1298,30 @@ def diff6_held(s, cfg):
               dzthr_x=float(t(thr * 9.81 * cfg.dx)),
               dzthr_y=float(t(thr * 9.81 * cfg.dy)))
     mono = int(cfg.diff_6th_opt)
-    return {"ru_t": diff6(s.u0, mut, s.c1h, s.c2h, coef, mono, 1, False,
-                          ny, nx, **kw),
-            "rv_t": diff6(s.v0, mut, s.c1h, s.c2h, coef, mono, 2, False,
-                          ny, nx, **kw),
-            "rw_t": diff6(s.w0, mut, s.c1f, s.c2f, coef, mono, 0, True,
-                          ny, nx, **kw),
-            "rth_t": diff6(s.thp0, mut, s.c1h, s.c2h, coef, mono, 0, False,
-                           ny, nx, **kw)}
+    out = {"ru_t": diff6(s.u0, mut, s.c1h, s.c2h, coef, mono, 1, False,
+                         ny, nx, **kw),
+           "rv_t": diff6(s.v0, mut, s.c1h, s.c2h, coef, mono, 2, False,
+                         ny, nx, **kw),
+           "rw_t": diff6(s.w0, mut, s.c1f, s.c2f, coef, mono, 0, True,
+                         ny, nx, **kw),
+           "rth_t": diff6(s.thp0, mut, s.c1h, s.c2h, coef, mono, 0, False,
+                          ny, nx, **kw)}
+    if getattr(s, "qv", None) is not None:
+        # every transported species, less WRF's moist array under
+        # moist_mix6_off (reference dycore.py:1200-1220, :1391-1393)
+        off = DIFF6_MOIST_ARRAY if getattr(cfg, "moist_mix6_off", False) \
+            else ()
+        out["q"] = {n: diff6(getattr(s, n + "0"), mut, s.c1h, s.c2h, coef,
+                             mono, 0, False, ny, nx, **kw)
+                    for n in ("qv", "qc", "qr") + tuple(getattr(s, "extra",
+                                                              ()))
+                    if n not in off}
+    return out
+
+
+#: WRF's moist array: the species moist_mix6_off exempts from the filter
+#: (reference moist.py:172-173); number and volume tracers keep it
+DIFF6_MOIST_ARRAY = ("qv", "qc", "qr", "qi", "qs", "qg", "qh")
 
 
 def step(s, cfg) -> None:
@@ -1337,7 +1354,16 @@ def step(s, cfg) -> None:
         held = smag_held(s, cfg)
     if getattr(cfg, "diff_6th_opt", 0) > 0:
         d6 = diff6_held(s, cfg)
-        held = {n: held[n] + d6[n] for n in d6} if held else d6
+        if held:
+            for n, h in d6.items():
+                if n == "q":
+                    for q, hq in h.items():
+                        held["q"][q] = held["q"][q] + hq
+                else:
+                    held[n] = held[n] + h
+        else:
+            held = d6
+    held_q = held.pop("q", None)
     for istage, (nsub, dtau) in enumerate(stages):
         for name in ("ru_t", "rv_t", "rw_t", "rth_t", "rph_t", "rmu_t"):
             getattr(s, name)[...] = 0.0
@@ -1375,7 +1401,7 @@ def step(s, cfg) -> None:
                                               nsub)
             advance_scalars_stage(s, ru_m, rv_m, ww_m, nsub * dtau,
                                   istage == 2, cfg.dx, cfg.dy,
-                                  getattr(cfg, "moist_adv_opt", 1))
+                                  getattr(cfg, "moist_adv_opt", 1), held_q)
     s.cq = None
     set_w_surface(s, cfg.dx, cfg.dy)
     calc_p_alpha(s, prec, hypso)

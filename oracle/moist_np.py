@@ -210,10 +210,14 @@ def pd_apply(sc, fl, rdnw, dx_inv, dy_inv):
 
 
 def advance_scalars_stage(s, ru_m, rv_m, ww_m, dt_stage, final, dx, dy,
-                          moist_adv_opt=1):
+                          moist_adv_opt=1, fixed=None):
     """qv/qc/qr one RK stage from their time-t copies (reference moist.py:
     434-698): unlimited flux form, or the PD limiter on the final stage
-    when moist_adv_opt = 1.  ``dt_stage`` is the host double nsub*dtau."""
+    when moist_adv_opt = 1.  ``dt_stage`` is the host double nsub*dtau.
+    ``fixed`` ({name: tendency}) holds held mixing tendencies: the
+    unlimited update adds them to the advective tendency (moist.py:
+    384-386), the limited one folds them into the time-t scalar first,
+    q0 + dt fixed / C(mu_t) (``_pd_fold_sources`` moist.py:293-346)."""
     f = s.dtype.type
     mu0 = s.mub2d + s.mup0
     mu = s.mub2d + s.mup
@@ -222,7 +226,12 @@ def advance_scalars_stage(s, ru_m, rv_m, ww_m, dt_stage, final, dx, dy,
     pd = final and moist_adv_opt == 1
     for name in SPECIES + tuple(getattr(s, "extra", ())):
         q, q0 = getattr(s, name), getattr(s, name + "0")
+        fx = fixed.get(name) if fixed else None
         if pd:
+            if fx is not None:
+                chm0 = s.c1h[:, None, None] * mu0[None] \
+                    + s.c2h[:, None, None]
+                q0 = q0 + (f(dt_stage) * fx) / chm0
             fl = pd_fluxes(q, q0, ru_m, rv_m, ww_m, mu, s.c1h, s.c2h,
                            s.rdnw, s.fnm, s.fnp, dx, dy, dt_stage)
             sc = pd_scale(q0, mu0, s.c1h, s.c2h, s.rdnw, fl, dx_inv,
@@ -233,5 +242,7 @@ def advance_scalars_stage(s, ru_m, rv_m, ww_m, dt_stage, final, dx, dy,
         else:
             tend = f(0.0) + D.flux_div_scalar(q, ru_m, rv_m, ww_m, s.rdnw,
                                               s.fnm, s.fnp, dx_inv, dy_inv)
+            if fx is not None:
+                tend = tend + fx
             q[...] = coupled_update(q0, tend, s.c1h, s.c2h, mu0, mu,
                                     dt_stage, clamp=final)

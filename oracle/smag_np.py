@@ -327,7 +327,14 @@ def theta_tendency(gr, kh, thp, thb, heat_flux):
     constant surface heat flux when ``heat_flux`` is given (isfflx 0 or 2
     with the dycore's vertical mixing, :1658-1668)."""
     t = gr.t
-    S = (_b3(thb) + thp) - t(WRF_T0)
+    return scalar_tendency(gr, kh, (_b3(thb) + thp) - t(WRF_T0), heat_flux)
+
+
+def scalar_tendency(gr, kh, S, heat_flux=None):
+    """The flux divergence of WRF horizontal_diffusion_s_2 for the scalar
+    S (reference smag2d.cu:1254-1292): theta's t_2, or a moist species'
+    mixing ratio as stored (dycore.py:1259-1264)."""
+    t = gr.t
     zx, zy, rdzw, g = gr.zx, gr.zy, gr.rdzw, gr.g
     rho = gr.rho
     fx, fy = scalar_fluxes(gr, kh, S)
@@ -350,8 +357,12 @@ def smag_held(s, cfg):
     """The held km_opt = 4 tendencies {ru_t, rv_t, rw_t, rth_t} of the
     time-t fields (u0 ... php0 and the refreshed alt), as
     ``_compute_wrf_smag_tendencies`` (reference dycore.py:1223-1343) leaves
-    them in the carrying buffers for a dry periodic state."""
-    gr = Grid(s, cfg.dx, cfg.dy, s.php0, s.alt)
+    them in the carrying buffers for a periodic state; a moist state's
+    density carries the vapour, rho = (1 + qv) / alt (smag2d.cu:394-404),
+    and every transported species takes K_h's flux divergence under
+    ``"q"`` ({name: tendency}, dycore.py:1193-1199)."""
+    moist = getattr(s, "qv", None) is not None
+    gr = Grid(s, cfg.dx, cfg.dy, s.php0, s.alt, s.qv0 if moist else None)
     d11, d22, d12 = deformation(gr, s.u0, s.v0)
     km, kh = smag_km(gr, d11, d22, d12, cfg.c_s)
     tau = stresses(gr, km, d11, d22, d12, s.u0, s.v0, s.w0)
@@ -361,4 +372,9 @@ def smag_held(s, cfg):
     heat = cfg.tke_heat_flux if (vertical and isfflx in (0, 2)) else None
     ru, rv, rw = momentum_tendencies(gr, tau, vertical, cd0, s.u0, s.v0)
     rth = theta_tendency(gr, kh, s.thp0, s.thb, heat)
-    return {"ru_t": ru, "rv_t": rv, "rw_t": rw, "rth_t": rth}
+    out = {"ru_t": ru, "rv_t": rv, "rw_t": rw, "rth_t": rth}
+    if moist:
+        out["q"] = {n: scalar_tendency(gr, kh, getattr(s, n + "0"))
+                    for n in ("qv", "qc", "qr") + tuple(getattr(s, "extra",
+                                                              ()))}
+    return out

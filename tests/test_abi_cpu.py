@@ -78,8 +78,8 @@ def _struct_fields(name):
         decl = decl.strip()
         if not decl:
             continue
-        m = re.match(r"(?:const\s+)?(int32_t|float|double)\s+(.*)", decl,
-                     flags=re.S)
+        m = re.match(r"(?:const\s+)?(int32_t|uint32_t|float|double)\s+(.*)",
+                     decl, flags=re.S)
         base, rest = m.group(1), m.group(2)
         for part in rest.split(","):
             part = part.strip()
@@ -104,6 +104,7 @@ def _struct_fields(name):
                                        ("arw64_finish_adj", _lib.ArwFinishAdj64)])
 def test_ctypes_layout_matches_header(cname, pyt):
     kinds = {"ptr": ctypes.c_void_p, "int32_t": ctypes.c_int32,
+             "uint32_t": ctypes.c_uint32,
              "float": ctypes.c_float, "double": ctypes.c_double,
              "ptr[]": ctypes.c_void_p * _lib.MAX_EXTRA}
     want = [(n, kinds[k]) for n, k in _struct_fields(cname)]

@@ -400,3 +400,107 @@ def test_slope_factor_alpha_and_its_square_beyond_the_limit(rise, want):
     km, _ = S.smag_km(gr, *S.deformation(gr, u, np.zeros((nz, ny + 1, nx))),
                       0.25)
     np.testing.assert_allclose(km[INT], want, rtol=1e-13)
+
+
+# --- moist states: the held species tendencies ----------------------------
+
+def _sheared_moist(mp=10, nx=20, ny=14, nz=12, **opts):
+    from arwen_amd import cases
+    case = cases.mixed_phase(mp, nx=nx, ny=ny, nz=nz) if mp \
+        else cases.moist_tracer(nx=nx, ny=ny, nz=nz)
+    y = (np.arange(ny) + 0.5) / ny
+    x = (np.arange(nx) + 0.5) / nx
+    case.fields["u"] = case.fields["u"] + 3.0 * np.sin(2 * np.pi * y)[
+        None, :, None]
+    case.fields["v"] = case.fields["v"] + 2.0 * np.cos(2 * np.pi * x)[
+        None, None, :]
+    case.cfg = case.cfg.with_(**opts)
+    return case
+
+
+def test_vapour_loads_the_smagorinsky_density():
+    """rho = (1 + qv) / alt: a uniform qv = c scales every momentum stress
+    by exactly (1 + c) against the dry density (smag2d.cu:394-404)."""
+    nz, ny, nx = 5, 6, 8
+    gr_d, u, v, w, rng = _flat_random(5, nz, ny, nx)
+    c = 0.0125
+    s = _state(nz, ny, nx, gr_d.ph)
+    alt = 1.0 / gr_d.rho
+    gr_m = S.Grid(s, 900.0, 1100.0, np.zeros_like(gr_d.ph), alt,
+                  np.full((nz, ny, nx), c))
+    out = []
+    for gr in (gr_d, gr_m):
+        d = S.deformation(gr, u, v)
+        km, _ = S.smag_km(gr, *d, 0.25)
+        tau = S.stresses(gr, km, *d, u, v, w)
+        out.append(S.momentum_tendencies(gr, tau, True, None, u, v))
+    for a, b in zip(*out):
+        np.testing.assert_allclose(b, (1.0 + c) * a, rtol=1e-12,
+                                   atol=1e-12 * np.abs(a).max())
+
+
+def test_species_rows_conserve_and_mix():
+    """Every species takes K_h's flux divergence of its own mixing ratio:
+    on the flat sheared moist case each row's coupled sum is zero and the
+    rows are live."""
+    from oracle import dycore_np as D
+    case = _sheared_moist(10, km_opt=4)
+    s = D.from_case(case, np.float64)
+    s.php[...] = 0.0
+    for n in D.PROGNOSTICS:
+        getattr(s, n + "0")[...] = getattr(s, n)
+    for q in ("qv", "qc", "qr") + s.extra:
+        getattr(s, q + "0")[...] = getattr(s, q)
+    D.calc_p_alpha(s)
+    held = S.smag_held(s, case.cfg)
+    dnw = s.dnw[:, None, None]
+    assert set(held["q"]) == {"qv", "qc", "qr"} | set(s.extra)
+    for n, h in held["q"].items():
+        rdzw = D.G / (s.phb[1:] - s.phb[:-1])
+        w = dnw / rdzw[:, None, None] if rdzw.ndim == 1 else dnw / rdzw
+        assert abs((h * w).sum()) < 1e-12 * np.abs(h * w).sum(), n
+        assert np.abs(h).max() > 0.0, n
+
+
+@pytest.mark.parametrize("pd", [True, False])
+def test_held_species_tendency_at_rest(pd):
+    """With no advecting flux a stage moves q by dt fixed / C(mu): folded
+    into q_t ahead of the limiter, or added to the unlimited update
+    (reference moist.py:293-346, :384-386); leaving the fold out fails."""
+    from oracle import dycore_np as D
+    from oracle import moist_np as M
+    case = _sheared_moist(6)
+    s = D.from_case(case, np.float64)
+    for q in ("qv", "qc", "qr") + s.extra:
+        getattr(s, q + "0")[...] = getattr(s, q)
+    s.mup0[...] = s.mup
+    z = (np.zeros_like(s.u), np.zeros_like(s.v), np.zeros_like(s.w))
+    rng = np.random.default_rng(3)
+    fixed = {n: 1e-3 * rng.standard_normal(s.qv.shape)
+             for n in ("qv", "qc", "qr") + s.extra}
+    dt = 18.0
+    M.advance_scalars_stage(s, *z, dt, True, case.cfg.dx, case.cfg.dy,
+                            1 if pd else 0, fixed)
+    chm = s.c1h[:, None, None] * (s.mub2d + s.mup)[None] \
+        + s.c2h[:, None, None]
+    for n, fx in fixed.items():
+        want = np.maximum(getattr(s, n + "0") + dt * fx / chm, 0.0)
+        np.testing.assert_allclose(getattr(s, n), want, rtol=1e-13,
+                                   atol=1e-18)
+        assert np.abs(getattr(s, n) - getattr(s, n + "0")).max() > 1e-9
+
+
+def test_moist_mix6_off_exempts_the_moist_array_only():
+    from oracle import dycore_np as D
+    case = _sheared_moist(10, diff_6th_opt=1, moist_mix6_off=True)
+    s = D.from_case(case, np.float32)
+    for n in D.PROGNOSTICS:
+        getattr(s, n + "0")[...] = getattr(s, n)
+    for q in ("qv", "qc", "qr") + s.extra:
+        getattr(s, q + "0")[...] = getattr(s, q)
+    held = D.diff6_held(s, case.cfg)
+    kept = set(held["q"])
+    assert not kept & {"qv", "qc", "qr", "qi", "qs", "qg"}
+    assert "nr" in kept and kept <= set(s.extra)
+    all_rows = D.diff6_held(s, case.cfg.with_(moist_mix6_off=False))
+    assert set(all_rows["q"]) == {"qv", "qc", "qr"} | set(s.extra)
