@@ -33,11 +33,20 @@ struct SmagSc {
 // tau11/tau22's slots then hold the theta fluxes
 struct SmagBuf {
     real *t11, *t22, *t12, *km, *kh, *t33, *t13, *t23;
+    // the metrics of the time-t geopotential and the density, stored once
+    real *rdzw, *rho, *rdz, *zx, *zy;
 };
+
+// periodic wrap of an index within [-1, n]: every neighbour these kernels
+// read lies one cell (or the duplicate face) away
+__device__ __forceinline__ int wrap(int i, int n)
+{
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+}
 
 __device__ __forceinline__ int mix(const Dom& d, int k, int j, int i)
 {
-    return iM(k, pmod(j, d.ny), pmod(i, d.nx), d.ny, d.nx);
+    return iM(k, wrap(j, d.ny), wrap(i, d.nx), d.ny, d.nx);
 }
 
 __device__ __forceinline__ real phi(const Dom& d, int kw, int j, int i)
@@ -50,12 +59,12 @@ __device__ __forceinline__ real phi(const Dom& d, int kw, int j, int i)
 // G / dz of mass level k, 2 G / (phi(k+1) - phi(k-1)) of w level kw
 // (one-sided at the surface and the lid), the coordinate slopes at the
 // u face i / v face j of w level kw (smag2d.cu:355-392)
-__device__ __forceinline__ real rdzw(const Dom& d, int k, int j, int i)
+__device__ __forceinline__ real rdzw_of(const Dom& d, int k, int j, int i)
 {
     return G_F / (phi(d, k + 1, j, i) - phi(d, k, j, i));
 }
 
-__device__ __forceinline__ real rdz(const Dom& d, int kw, int j, int i)
+__device__ __forceinline__ real rdz_of(const Dom& d, int kw, int j, int i)
 {
     const real g2 = R(2.0) * G_F;
     if (kw <= 0) return g2 / (phi(d, 1, j, i) - phi(d, 0, j, i));
@@ -63,34 +72,65 @@ __device__ __forceinline__ real rdz(const Dom& d, int kw, int j, int i)
     return g2 / (phi(d, kw + 1, j, i) - phi(d, kw - 1, j, i));
 }
 
-__device__ __forceinline__ real zx(const Dom& d, real rdx, int kw, int j,
-                                   int i)
+__device__ __forceinline__ real zx_of(const Dom& d, real rdx, int kw, int j,
+                                      int i)
 {
     return (rdx * (phi(d, kw, j, i) - phi(d, kw, j, i - 1))) / G_F;
 }
 
-__device__ __forceinline__ real zy(const Dom& d, real rdy, int kw, int j,
-                                   int i)
+__device__ __forceinline__ real zy_of(const Dom& d, real rdy, int kw, int j,
+                                      int i)
 {
     return (rdy * (phi(d, kw, j, i) - phi(d, kw, j - 1, i))) / G_F;
 }
 
 // density with the vapour loading, (1 + qv) / alt, and exactly 1 / alt
 // for a dry state (smag2d.cu:394-404)
-__device__ __forceinline__ real rho(const Dom& d, int k, int j, int i)
+__device__ __forceinline__ real rho_of(const Dom& d, int k, int j, int i)
 {
     const int x = mix(d, k, j, i);
     return d.qv ? (R(1.0) + d.qv[x]) / d.alt[x] : R(1.0) / d.alt[x];
 }
 
+// the stored metrics and density
+__device__ __forceinline__ real rdzw(const Dom& d, const SmagBuf& b, int k,
+                                     int j, int i)
+{
+    return b.rdzw[mix(d, k, j, i)];
+}
+
+__device__ __forceinline__ real rdz(const Dom& d, const SmagBuf& b, int kw,
+                                    int j, int i)
+{
+    return b.rdz[mix(d, kw, j, i)];
+}
+
+__device__ __forceinline__ real zx(const Dom& d, const SmagBuf& b, int kw,
+                                   int j, int i)
+{
+    return b.zx[mix(d, kw, j, i)];
+}
+
+__device__ __forceinline__ real zy(const Dom& d, const SmagBuf& b, int kw,
+                                   int j, int i)
+{
+    return b.zy[mix(d, kw, j, i)];
+}
+
+__device__ __forceinline__ real rho(const Dom& d, const SmagBuf& b, int k,
+                                    int j, int i)
+{
+    return b.rho[mix(d, k, j, i)];
+}
+
 __device__ __forceinline__ real U(const Dom& d, int k, int j, int i)
 {
-    return d.u[iU(k, pmod(j, d.ny), pmod(i, d.nx), d.ny, d.nx)];
+    return d.u[iU(k, wrap(j, d.ny), wrap(i, d.nx), d.ny, d.nx)];
 }
 
 __device__ __forceinline__ real V(const Dom& d, int k, int j, int i)
 {
-    return d.v[iV(k, pmod(j, d.ny), pmod(i, d.nx), d.ny, d.nx)];
+    return d.v[iV(k, wrap(j, d.ny), wrap(i, d.nx), d.ny, d.nx)];
 }
 
 __device__ __forceinline__ real W(const Dom& d, int k, int j, int i)
@@ -136,56 +176,82 @@ __device__ __forceinline__ real mavg(const Dom& d, const real* a, int kw,
                                     + at(d, a, kw - 1, j, i)));
 }
 
-#define SMAG_POINT(NXS, NYS, NLEV)                                          \
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;                   \
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;                   \
-    const int k = blockIdx.z;                                              \
-    if (i >= (NXS) || j >= (NYS) || k >= (NLEV)) return;
+// Each pass is a per-point body (k, j, i) launched one thread per point,
+// 64 x 4 points of one level per workgroup.  (A column march of the same
+// bodies, one thread per column looping over the levels, measured no
+// faster and the theta flux pass 1.6x slower: without register carries
+// the march only re-orders the same loads.)
+#define SMAG_BODY(NAME, ...)                                                 \
+    __device__ __forceinline__ void NAME(__VA_ARGS__, int k, int j, int i)
+#define SMAG_POINTS(KERNEL, BODY, NXS, NYS, NLEV, PARAMS, ...)                \
+    __global__ void KERNEL PARAMS                                            \
+    {                                                                        \
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;                 \
+        const int j = blockIdx.y * blockDim.y + threadIdx.y;                 \
+        const int k = blockIdx.z;                                            \
+        if (i >= (NXS) || j >= (NYS) || k >= (NLEV)) return;                \
+        BODY(__VA_ARGS__, k, j, i);                                          \
+    }
+
+// rdzw and the density at mass level k, rdz, zx and zy at w level k
+SMAG_BODY(smag_metric_at, const Dom& d, const SmagSc& c, const SmagBuf& b)
+{
+    const int x = iM(k, j, i, d.ny, d.nx);
+    if (k < d.nz) {
+        b.rdzw[x] = rdzw_of(d, k, j, i);
+        b.rho[x] = rho_of(d, k, j, i);
+    }
+    b.rdz[x] = rdz_of(d, k, j, i);
+    b.zx[x] = zx_of(d, c.rdx, k, j, i);
+    b.zy[x] = zy_of(d, c.rdy, k, j, i);
+}
+SMAG_POINTS(k_smag_metric, smag_metric_at, d.nx, d.ny, d.nz + 1,
+           (Dom d, SmagSc c, SmagBuf b), d, c, b)
 
 // WRF cal_deform_and_div (smag2d.cu:484-595)
-__global__ void k_smag_deform(Dom d, SmagSc c, SmagBuf b)
+SMAG_BODY(smag_deform_at, const Dom& d, const SmagSc& c, const SmagBuf& b)
 {
-    SMAG_POINT(d.nx, d.ny, d.nz)
     const real q = R(0.25);
     const real rdx = c.rdx, rdy = c.rdy;
     const int x = iM(k, j, i, d.ny, d.nx);
-    const real rz = rdzw(d, k, j, i);
+    const real rz = rdzw(d, b, k, j, i);
     {
-        const real tmpzx = q * (((zx(d, rdx, k, j, i) + zx(d, rdx, k, j, i + 1))
-                                 + zx(d, rdx, k + 1, j, i))
-                                + zx(d, rdx, k + 1, j, i + 1));
+        const real tmpzx = q * (((zx(d, b, k, j, i) + zx(d, b, k, j, i + 1))
+                                 + zx(d, b, k + 1, j, i))
+                                + zx(d, b, k + 1, j, i + 1));
         auto p = [&](int kk) { return U(d, kk, j, i) + U(d, kk, j, i + 1); };
         const real slope = ((full(d, k + 1, p) - full(d, k, p)) * tmpzx) * rz;
         b.t11[x] = R(2.0) * (rdx * (U(d, k, j, i + 1) - U(d, k, j, i)) - slope);
     }
     {
-        const real tmpzy = q * (((zy(d, rdy, k, j, i) + zy(d, rdy, k, j + 1, i))
-                                 + zy(d, rdy, k + 1, j, i))
-                                + zy(d, rdy, k + 1, j + 1, i));
+        const real tmpzy = q * (((zy(d, b, k, j, i) + zy(d, b, k, j + 1, i))
+                                 + zy(d, b, k + 1, j, i))
+                                + zy(d, b, k + 1, j + 1, i));
         auto p = [&](int kk) { return V(d, kk, j, i) + V(d, kk, j + 1, i); };
         const real slope = ((full(d, k + 1, p) - full(d, k, p)) * tmpzy) * rz;
         b.t22[x] = R(2.0) * (rdy * (V(d, k, j + 1, i) - V(d, k, j, i)) - slope);
     }
-    const real rr = ((rz + rdzw(d, k, j, i - 1)) + rdzw(d, k, j - 1, i - 1))
-                    + rdzw(d, k, j - 1, i);
-    const real tzy = q * (((zy(d, rdy, k, j, i - 1) + zy(d, rdy, k, j, i))
-                           + zy(d, rdy, k + 1, j, i - 1))
-                          + zy(d, rdy, k + 1, j, i));
+    const real rr = ((rz + rdzw(d, b, k, j, i - 1)) + rdzw(d, b, k, j - 1, i - 1))
+                    + rdzw(d, b, k, j - 1, i);
+    const real tzy = q * (((zy(d, b, k, j, i - 1) + zy(d, b, k, j, i))
+                           + zy(d, b, k + 1, j, i - 1))
+                          + zy(d, b, k + 1, j, i));
     auto pu = [&](int kk) { return U(d, kk, j - 1, i) + U(d, kk, j, i); };
     const real uslope = (((full(d, k + 1, pu) - full(d, k, pu)) * q) * tzy) * rr;
-    const real tzx = q * (((zx(d, rdx, k, j - 1, i) + zx(d, rdx, k, j, i))
-                           + zx(d, rdx, k + 1, j - 1, i))
-                          + zx(d, rdx, k + 1, j, i));
+    const real tzx = q * (((zx(d, b, k, j - 1, i) + zx(d, b, k, j, i))
+                           + zx(d, b, k + 1, j - 1, i))
+                          + zx(d, b, k + 1, j, i));
     auto pv = [&](int kk) { return V(d, kk, j, i - 1) + V(d, kk, j, i); };
     const real vslope = (((full(d, k + 1, pv) - full(d, k, pv)) * q) * tzx) * rr;
     b.t12[x] = ((rdy * (U(d, k, j, i) - U(d, k, j - 1, i)) - uslope)
                 + rdx * (V(d, k, j, i) - V(d, k, j, i - 1))) - vslope;
 }
+SMAG_POINTS(k_smag_deform, smag_deform_at, d.nx, d.ny, d.nz,
+           (Dom d, SmagSc c, SmagBuf b), d, c, b)
 
 // WRF smag2d_km (smag2d.cu:604-652)
-__global__ void k_smag_km(Dom d, SmagSc c, SmagBuf b)
+SMAG_BODY(smag_km_at, const Dom& d, const SmagSc& c, const SmagBuf& b)
 {
-    SMAG_POINT(d.nx, d.ny, d.nz)
     const real q = R(0.25);
     const int x = iM(k, j, i, d.ny, d.nx);
     const real d11 = b.t11[x], d22 = b.t22[x];
@@ -195,51 +261,53 @@ __global__ void k_smag_km(Dom d, SmagSc c, SmagBuf b)
     const real strain = sqrt((q * (d11 - d22)) * (d11 - d22) + d12 * d12);
     const real mlen = sqrt(c.dx * c.dy);
     real km = fmin((((c.cs * c.cs) * mlen) * mlen) * strain, R(10.0) * mlen);
-    const real rz = rdzw(d, k, j, i);
-    const real sx = ((q * (((fabs(zx(d, c.rdx, k, j, i))
-                             + fabs(zx(d, c.rdx, k, j, i + 1)))
-                            + fabs(zx(d, c.rdx, k + 1, j, i)))
-                           + fabs(zx(d, c.rdx, k + 1, j, i + 1)))) * rz) * c.dx;
-    const real sy = ((q * (((fabs(zy(d, c.rdy, k, j, i))
-                             + fabs(zy(d, c.rdy, k, j + 1, i)))
-                            + fabs(zy(d, c.rdy, k + 1, j, i)))
-                           + fabs(zy(d, c.rdy, k + 1, j + 1, i)))) * rz) * c.dy;
+    const real rz = rdzw(d, b, k, j, i);
+    const real sx = ((q * (((fabs(zx(d, b, k, j, i))
+                             + fabs(zx(d, b, k, j, i + 1)))
+                            + fabs(zx(d, b, k + 1, j, i)))
+                           + fabs(zx(d, b, k + 1, j, i + 1)))) * rz) * c.dx;
+    const real sy = ((q * (((fabs(zy(d, b, k, j, i))
+                             + fabs(zy(d, b, k, j + 1, i)))
+                            + fabs(zy(d, b, k + 1, j, i)))
+                           + fabs(zy(d, b, k + 1, j + 1, i)))) * rz) * c.dy;
     const real alpha = fmax(sqrt(sx * sx + sy * sy), R(1.0));
     const real limit = fmax(R(10.0) / mlen, R(1.0e-3));
     km = km / (strain > limit ? alpha * alpha : alpha);
     b.km[x] = km;
     b.kh[x] = km / c.prandtl;
 }
+SMAG_POINTS(k_smag_km, smag_km_at, d.nx, d.ny, d.nz,
+           (Dom d, SmagSc c, SmagBuf b), d, c, b)
 
 // tau11, tau22, tau12 = -rho K D in place (smag2d.cu:659-686)
-__global__ void k_smag_tau(Dom d, SmagBuf b)
+SMAG_BODY(smag_tau_at, const Dom& d, const SmagBuf& b)
 {
-    SMAG_POINT(d.nx, d.ny, d.nz)
     const real q = R(0.25);
     const int x = iM(k, j, i, d.ny, d.nx);
-    const real r = rho(d, k, j, i), km = b.km[x];
+    const real r = rho(d, b, k, j, i), km = b.km[x];
     b.t11[x] = ((-r) * km) * b.t11[x];
     b.t22[x] = ((-r) * km) * b.t22[x];
-    const real ra = q * (((r + rho(d, k, j, i - 1)) + rho(d, k, j - 1, i - 1))
-                         + rho(d, k, j - 1, i));
+    const real ra = q * (((r + rho(d, b, k, j, i - 1)) + rho(d, b, k, j - 1, i - 1))
+                         + rho(d, b, k, j - 1, i));
     const real ka = q * (((km + at(d, b.km, k, j, i - 1))
                           + at(d, b.km, k, j - 1, i - 1))
                          + at(d, b.km, k, j - 1, i));
     b.t12[x] = ((-ra) * ka) * b.t12[x];
 }
+SMAG_POINTS(k_smag_tau, smag_tau_at, d.nx, d.ny, d.nz,
+           (Dom d, SmagBuf b), d, b)
 
 // tau13 / tau23 at the u / v faces of w level kw (0 at the surface and the
 // lid) and tau33 at mass level kw (smag2d.cu:826-899, :945-956)
-__global__ void k_smag_tauw(Dom d, SmagSc c, SmagBuf b)
+SMAG_BODY(smag_tauw_at, const Dom& d, const SmagSc& c, const SmagBuf& b)
 {
-    SMAG_POINT(d.nx, d.ny, d.nz + 1)
     const int kw = k, nz = d.nz;
     const real q = R(0.25), h = R(0.5);
     const int x = iM(kw, j, i, d.ny, d.nx);
     if (kw < nz) {
         const real dw = (R(2.0) * (W(d, kw + 1, j, i) - W(d, kw, j, i)))
-                        * rdzw(d, kw, j, i);
-        b.t33[x] = ((-rho(d, kw, j, i)) * b.km[x]) * dw;
+                        * rdzw(d, b, kw, j, i);
+        b.t33[x] = ((-rho(d, b, kw, j, i)) * b.km[x]) * dw;
     }
     if (kw == 0 || kw == nz) {
         b.t13[x] = R(0.0);
@@ -248,17 +316,17 @@ __global__ void k_smag_tauw(Dom d, SmagSc c, SmagBuf b)
     }
     const real fm = d.fnm[kw], fp = d.fnp[kw];
     {
-        const real rz = h * (rdz(d, kw, j, i) + rdz(d, kw, j, i - 1));
+        const real rz = h * (rdz(d, b, kw, j, i) + rdz(d, b, kw, j, i - 1));
         auto wa = [&](int kk) {
             return q * (((W(d, kk, j, i) + W(d, kk + 1, j, i))
                          + W(d, kk, j, i - 1)) + W(d, kk + 1, j, i - 1));
         };
-        const real slope = ((wa(kw) - wa(kw - 1)) * zx(d, c.rdx, kw, j, i)) * rz;
+        const real slope = ((wa(kw) - wa(kw - 1)) * zx(d, b, kw, j, i)) * rz;
         const real dw = c.rdx * (W(d, kw, j, i) - W(d, kw, j, i - 1)) - slope;
         const real d13 = dw + (U(d, kw, j, i) - U(d, kw - 1, j, i)) * rz;
-        const real ra = h * (fm * (rho(d, kw, j, i - 1) + rho(d, kw, j, i))
-                             + fp * (rho(d, kw - 1, j, i - 1)
-                                     + rho(d, kw - 1, j, i)));
+        const real ra = h * (fm * (rho(d, b, kw, j, i - 1) + rho(d, b, kw, j, i))
+                             + fp * (rho(d, b, kw - 1, j, i - 1)
+                                     + rho(d, b, kw - 1, j, i)));
         const real ka = h * (fm * (at(d, b.km, kw, j, i - 1)
                                    + at(d, b.km, kw, j, i))
                              + fp * (at(d, b.km, kw - 1, j, i - 1)
@@ -266,17 +334,17 @@ __global__ void k_smag_tauw(Dom d, SmagSc c, SmagBuf b)
         b.t13[x] = ((-ra) * ka) * d13;
     }
     {
-        const real rz = h * (rdz(d, kw, j, i) + rdz(d, kw, j - 1, i));
+        const real rz = h * (rdz(d, b, kw, j, i) + rdz(d, b, kw, j - 1, i));
         auto wa = [&](int kk) {
             return q * (((W(d, kk, j, i) + W(d, kk + 1, j, i))
                          + W(d, kk, j - 1, i)) + W(d, kk + 1, j - 1, i));
         };
-        const real slope = ((wa(kw) - wa(kw - 1)) * zy(d, c.rdy, kw, j, i)) * rz;
+        const real slope = ((wa(kw) - wa(kw - 1)) * zy(d, b, kw, j, i)) * rz;
         const real dw = c.rdy * (W(d, kw, j, i) - W(d, kw, j - 1, i)) - slope;
         const real d23 = dw + (V(d, kw, j, i) - V(d, kw - 1, j, i)) * rz;
-        const real ra = h * (fm * (rho(d, kw, j - 1, i) + rho(d, kw, j, i))
-                             + fp * (rho(d, kw - 1, j - 1, i)
-                                     + rho(d, kw - 1, j, i)));
+        const real ra = h * (fm * (rho(d, b, kw, j - 1, i) + rho(d, b, kw, j, i))
+                             + fp * (rho(d, b, kw - 1, j - 1, i)
+                                     + rho(d, b, kw - 1, j, i)));
         const real ka = h * (fm * (at(d, b.km, kw, j - 1, i)
                                    + at(d, b.km, kw, j, i))
                              + fp * (at(d, b.km, kw - 1, j - 1, i)
@@ -284,24 +352,25 @@ __global__ void k_smag_tauw(Dom d, SmagSc c, SmagBuf b)
         b.t23[x] = ((-ra) * ka) * d23;
     }
 }
+SMAG_POINTS(k_smag_tauw, smag_tauw_at, d.nx, d.ny, d.nz + 1,
+           (Dom d, SmagSc c, SmagBuf b), d, c, b)
 
 // u: horizontal_diffusion_u_2 (smag2d.cu:765-795), vertical_diffusion_u_2
 // (:916-928) and the isfflx = 0 wall stress (:1608-1628); every u face,
 // the periodic duplicate included
-__global__ void k_smag_u(Dom d, SmagSc c, SmagBuf b, real* __restrict__ out)
+SMAG_BODY(smag_u_at, const Dom& d, const SmagSc& c, const SmagBuf& b, real* __restrict__ out)
 {
-    SMAG_POINT(d.nx + 1, d.ny, d.nz)
     const real q = R(0.25), h = R(0.5), e = R(0.125), one = R(1.0);
     const real rdx = c.rdx, rdy = c.rdy;
-    const real tmpdz = h * (one / rdzw(d, k, j, i) + one / rdzw(d, k, j, i - 1));
-    const real zxu = h * (zx(d, rdx, k, j, i) + zx(d, rdx, k + 1, j, i));
-    const real zyu = e * (((((((zy(d, rdy, k, j, i - 1) + zy(d, rdy, k, j, i))
-                               + zy(d, rdy, k, j + 1, i - 1))
-                              + zy(d, rdy, k, j + 1, i))
-                             + zy(d, rdy, k + 1, j, i - 1))
-                            + zy(d, rdy, k + 1, j, i))
-                           + zy(d, rdy, k + 1, j + 1, i - 1))
-                          + zy(d, rdy, k + 1, j + 1, i));
+    const real tmpdz = h * (one / rdzw(d, b, k, j, i) + one / rdzw(d, b, k, j, i - 1));
+    const real zxu = h * (zx(d, b, k, j, i) + zx(d, b, k + 1, j, i));
+    const real zyu = e * (((((((zy(d, b, k, j, i - 1) + zy(d, b, k, j, i))
+                               + zy(d, b, k, j + 1, i - 1))
+                              + zy(d, b, k, j + 1, i))
+                             + zy(d, b, k + 1, j, i - 1))
+                            + zy(d, b, k + 1, j, i))
+                           + zy(d, b, k + 1, j + 1, i - 1))
+                          + zy(d, b, k + 1, j + 1, i));
     const real divh = rdx * (at(d, b.t11, k, j, i) - at(d, b.t11, k, j, i - 1))
                       + rdy * (at(d, b.t12, k, j + 1, i) - at(d, b.t12, k, j, i));
     const real divz = (zxu * (mavg(d, b.t11, k + 1, j, i, 0, -1)
@@ -318,29 +387,30 @@ __global__ void k_smag_u(Dom d, SmagSc c, SmagBuf b, real* __restrict__ out)
             const real uu = U(d, 0, j, i);
             const real speed = sqrt(uu * uu + vv * vv) + R(1.0e-15);
             const real stress = (c.cd0 * speed) * uu;
-            const real ra = h * (rho(d, 0, j, i) + rho(d, 0, j, i - 1));
+            const real ra = h * (rho(d, b, 0, j, i) + rho(d, b, 0, j, i - 1));
             r = r + ((G_F * stress) * ra) / d.dnw[0];
         }
     }
     out[iU(k, j, i, d.ny, d.nx)] = r;
 }
+SMAG_POINTS(k_smag_u, smag_u_at, d.nx + 1, d.ny, d.nz,
+           (Dom d, SmagSc c, SmagBuf b, real* __restrict__ out), d, c, b, out)
 
 // v: horizontal_diffusion_v_2 (smag2d.cu:798-824), vertical_diffusion_v_2
 // (:931-943) and the isfflx = 0 wall stress (:1630-1654)
-__global__ void k_smag_v(Dom d, SmagSc c, SmagBuf b, real* __restrict__ out)
+SMAG_BODY(smag_v_at, const Dom& d, const SmagSc& c, const SmagBuf& b, real* __restrict__ out)
 {
-    SMAG_POINT(d.nx, d.ny + 1, d.nz)
     const real q = R(0.25), h = R(0.5), e = R(0.125), one = R(1.0);
     const real rdx = c.rdx, rdy = c.rdy;
-    const real tmpdz = h * (one / rdzw(d, k, j, i) + one / rdzw(d, k, j - 1, i));
-    const real zxv = e * (((((((zx(d, rdx, k, j - 1, i) + zx(d, rdx, k, j - 1, i + 1))
-                               + zx(d, rdx, k, j, i))
-                              + zx(d, rdx, k, j, i + 1))
-                             + zx(d, rdx, k + 1, j - 1, i))
-                            + zx(d, rdx, k + 1, j - 1, i + 1))
-                           + zx(d, rdx, k + 1, j, i))
-                          + zx(d, rdx, k + 1, j, i + 1));
-    const real zyv = h * (zy(d, rdy, k, j, i) + zy(d, rdy, k + 1, j, i));
+    const real tmpdz = h * (one / rdzw(d, b, k, j, i) + one / rdzw(d, b, k, j - 1, i));
+    const real zxv = e * (((((((zx(d, b, k, j - 1, i) + zx(d, b, k, j - 1, i + 1))
+                               + zx(d, b, k, j, i))
+                              + zx(d, b, k, j, i + 1))
+                             + zx(d, b, k + 1, j - 1, i))
+                            + zx(d, b, k + 1, j - 1, i + 1))
+                           + zx(d, b, k + 1, j, i))
+                          + zx(d, b, k + 1, j, i + 1));
+    const real zyv = h * (zy(d, b, k, j, i) + zy(d, b, k + 1, j, i));
     const real divh = rdy * (at(d, b.t22, k, j, i) - at(d, b.t22, k, j - 1, i))
                       + rdx * (at(d, b.t12, k, j, i + 1) - at(d, b.t12, k, j, i));
     const real divz = (zxv * (mavg(d, b.t12, k + 1, j, i, 0, 1)
@@ -357,18 +427,19 @@ __global__ void k_smag_v(Dom d, SmagSc c, SmagBuf b, real* __restrict__ out)
             const real vv = V(d, 0, j, i);
             const real speed = sqrt(vv * vv + uu * uu) + R(1.0e-15);
             const real stress = (c.cd0 * speed) * vv;
-            const real ra = h * (rho(d, 0, j, i) + rho(d, 0, j - 1, i));
+            const real ra = h * (rho(d, b, 0, j, i) + rho(d, b, 0, j - 1, i));
             r = r + ((G_F * stress) * ra) / d.dnw[0];
         }
     }
     out[iV(k, j, i, d.ny, d.nx)] = r;
 }
+SMAG_POINTS(k_smag_v, smag_v_at, d.nx, d.ny + 1, d.nz,
+           (Dom d, SmagSc c, SmagBuf b, real* __restrict__ out), d, c, b, out)
 
 // w: horizontal_diffusion_w_2 (smag2d.cu:1071-1093) and
 // vertical_diffusion_w_2 (:958-973); 0 at the surface and the lid
-__global__ void k_smag_w(Dom d, SmagSc c, SmagBuf b, real* __restrict__ out)
+SMAG_BODY(smag_w_at, const Dom& d, const SmagSc& c, const SmagBuf& b, real* __restrict__ out)
 {
-    SMAG_POINT(d.nx, d.ny, d.nz + 1)
     const int kw = k, nz = d.nz;
     const int x = iM(kw, j, i, d.ny, d.nx);
     if (kw == 0 || kw == nz) {
@@ -376,9 +447,9 @@ __global__ void k_smag_w(Dom d, SmagSc c, SmagBuf b, real* __restrict__ out)
         return;
     }
     const real q = R(0.25), h = R(0.5);
-    const real zxw = h * (zx(d, c.rdx, kw, j, i) + zx(d, c.rdx, kw, j, i + 1));
-    const real zyw = h * (zy(d, c.rdy, kw, j, i) + zy(d, c.rdy, kw, j + 1, i));
-    const real rz = rdz(d, kw, j, i);
+    const real zxw = h * (zx(d, b, kw, j, i) + zx(d, b, kw, j, i + 1));
+    const real zyw = h * (zy(d, b, kw, j, i) + zy(d, b, kw, j + 1, i));
+    const real rz = rdz(d, b, kw, j, i);
     const real divh = c.rdx * (at(d, b.t13, kw, j, i + 1) - at(d, b.t13, kw, j, i))
                       + c.rdy * (at(d, b.t23, kw, j + 1, i) - at(d, b.t23, kw, j, i));
     auto m13 = [&](int kk) {
@@ -396,6 +467,8 @@ __global__ void k_smag_w(Dom d, SmagSc c, SmagBuf b, real* __restrict__ out)
         r = r + (G_F * (b.t33[x] - at(d, b.t33, kw - 1, j, i))) / d.dn[kw];
     out[x] = r;
 }
+SMAG_POINTS(k_smag_w, smag_w_at, d.nx, d.ny, d.nz + 1,
+           (Dom d, SmagSc c, SmagBuf b, real* __restrict__ out), d, c, b, out)
 
 // t_2 = theta - 300 from the stored perturbation (smag2d.cu:1105-1119)
 __device__ __forceinline__ real theta2(const Dom& d, int k, int j, int i)
@@ -407,55 +480,53 @@ __device__ __forceinline__ real theta2(const Dom& d, int k, int j, int i)
 // the scalar face fluxes H1 (u face i) and H2 (v face j) of theta's t_2
 // (src null) or of a species' mixing ratio src (smag2d.cu:1164-1200,
 // wrf_smag_flux_s :1236-1251)
-__global__ void k_smag_flux(Dom d, SmagSc c, SmagBuf b,
-                            const real* __restrict__ src)
+SMAG_BODY(smag_flux_at, const Dom& d, const SmagSc& c, const SmagBuf& b, const real* __restrict__ src)
 {
-    SMAG_POINT(d.nx, d.ny, d.nz)
     const real h = R(0.5), one = R(1.0), two = R(2.0);
     const int x = iM(k, j, i, d.ny, d.nx);
     auto sv = [&](int kk, int jj, int ii) {
         return src ? src[mix(d, kk, jj, ii)] : theta2(d, kk, jj, ii);
     };
     const real s0 = sv(k, j, i);
-    const real rz = rdzw(d, k, j, i);
+    const real rz = rdzw(d, b, k, j, i);
     {
-        const real ra = h * (rho(d, k, j, i - 1) + rho(d, k, j, i));
+        const real ra = h * (rho(d, b, k, j, i - 1) + rho(d, b, k, j, i));
         const real ka = h * (at(d, b.kh, k, j, i - 1) + b.kh[x]);
-        const real tz = h * (zx(d, c.rdx, k, j, i) + zx(d, c.rdx, k + 1, j, i));
-        const real rdzu = two / (one / rz + one / rdzw(d, k, j, i - 1));
+        const real tz = h * (zx(d, b, k, j, i) + zx(d, b, k + 1, j, i));
+        const real rdzu = two / (one / rz + one / rdzw(d, b, k, j, i - 1));
         auto p = [&](int kk) { return sv(kk, j, i - 1) + sv(kk, j, i); };
         const real grad = c.rdx * (s0 - sv(k, j, i - 1))
                           - (tz * (full(d, k + 1, p) - full(d, k, p))) * rdzu;
         b.t11[x] = ((-ra) * ka) * grad;
     }
     {
-        const real ra = h * (rho(d, k, j - 1, i) + rho(d, k, j, i));
+        const real ra = h * (rho(d, b, k, j - 1, i) + rho(d, b, k, j, i));
         const real ka = h * (at(d, b.kh, k, j - 1, i) + b.kh[x]);
-        const real tz = h * (zy(d, c.rdy, k, j, i) + zy(d, c.rdy, k + 1, j, i));
-        const real rdzv = two / (one / rz + one / rdzw(d, k, j - 1, i));
+        const real tz = h * (zy(d, b, k, j, i) + zy(d, b, k + 1, j, i));
+        const real rdzv = two / (one / rz + one / rdzw(d, b, k, j - 1, i));
         auto p = [&](int kk) { return sv(kk, j - 1, i) + sv(kk, j, i); };
         const real grad = c.rdy * (s0 - sv(k, j - 1, i))
                           - (tz * (full(d, k + 1, p) - full(d, k, p))) * rdzv;
         b.t22[x] = ((-ra) * ka) * grad;
     }
 }
+SMAG_POINTS(k_smag_flux, smag_flux_at, d.nx, d.ny, d.nz,
+           (Dom d, SmagSc c, SmagBuf b, const real* __restrict__ src), d, c, b, src)
 
 // a scalar's flux divergence (wrf_smag_hd_s, smag2d.cu:1254-1292), with
 // theta's constant surface heat flux under isfflx 0 / 2 when heat is set
 // (:1658-1668)
-__global__ void k_smag_theta(Dom d, SmagSc c, SmagBuf b, real* __restrict__ out,
-                             int heat)
+SMAG_BODY(smag_theta_at, const Dom& d, const SmagSc& c, const SmagBuf& b, real* __restrict__ out, int heat)
 {
-    SMAG_POINT(d.nx, d.ny, d.nz)
     const real q = R(0.25);
     const int x = iM(k, j, i, d.ny, d.nx);
     const real* fx = b.t11;
     const real* fy = b.t22;
-    const real zxm = q * (((zx(d, c.rdx, k, j, i) + zx(d, c.rdx, k, j, i + 1))
-                           + zx(d, c.rdx, k + 1, j, i)) + zx(d, c.rdx, k + 1, j, i + 1));
-    const real zym = q * (((zy(d, c.rdy, k, j, i) + zy(d, c.rdy, k, j + 1, i))
-                           + zy(d, c.rdy, k + 1, j, i)) + zy(d, c.rdy, k + 1, j + 1, i));
-    const real rz = rdzw(d, k, j, i);
+    const real zxm = q * (((zx(d, b, k, j, i) + zx(d, b, k, j, i + 1))
+                           + zx(d, b, k + 1, j, i)) + zx(d, b, k + 1, j, i + 1));
+    const real zym = q * (((zy(d, b, k, j, i) + zy(d, b, k, j + 1, i))
+                           + zy(d, b, k + 1, j, i)) + zy(d, b, k + 1, j + 1, i));
+    const real rz = rdzw(d, b, k, j, i);
     const real divh = c.rdx * (at(d, fx, k, j, i + 1) - fx[x])
                       + c.rdy * (at(d, fy, k, j + 1, i) - fy[x]);
     const real divz = (zxm * (mavg(d, fx, k + 1, j, i, 0, 1)
@@ -464,16 +535,18 @@ __global__ void k_smag_theta(Dom d, SmagSc c, SmagBuf b, real* __restrict__ out,
                                 - mavg(d, fy, k, j, i, 1, 0))) * rz;
     real r = R(0.0) + (G_F / (d.dnw[k] * rz)) * (divh - divz);
     if (heat && c.heatflux && k == 0)
-        r = r - ((G_F * c.heat) * rho(d, 0, j, i)) / d.dnw[0];
+        r = r - ((G_F * c.heat) * rho(d, b, 0, j, i)) / d.dnw[0];
     out[x] = r;
 }
+SMAG_POINTS(k_smag_theta, smag_theta_at, d.nx, d.ny, d.nz,
+           (Dom d, SmagSc c, SmagBuf b, real* __restrict__ out, int heat), d, c, b, out, heat)
 
 }  // namespace
 
 // The held km_opt = 4 tendencies of the time-t fields of d (u, v, w, thp,
 // php, the EOS's alt and, moist, qv) into out_u, out_v, out_w, out_t and
 // of the nq species q_src[e] into held_q + e nz ny nx, written (not
-// added); buf holds 6 nz ny nx + 2 (nz+1) ny nx elements of scratch.
+// added); buf holds 8 nz ny nx + 5 (nz+1) ny nx elements of scratch.
 hipError_t run_smag(const Dom& d, const arw_params* prm, real* buf,
                     real* out_u, real* out_v, real* out_w, real* out_t,
                     const real* const* q_src, real* held_q, int nq,
@@ -488,8 +561,13 @@ hipError_t run_smag(const Dom& d, const arw_params* prm, real* buf,
     b.km = buf + 3 * n;
     b.kh = buf + 4 * n;
     b.t33 = buf + 5 * n;
-    b.t13 = buf + 6 * n;
-    b.t23 = buf + 6 * n + nw;
+    b.rdzw = buf + 6 * n;
+    b.rho = buf + 7 * n;
+    b.t13 = buf + 8 * n;
+    b.t23 = buf + 8 * n + nw;
+    b.rdz = buf + 8 * n + 2 * nw;
+    b.zx = buf + 8 * n + 3 * nw;
+    b.zy = buf + 8 * n + 4 * nw;
     SmagSc c;
     c.rdx = (real)(1.0 / prm->dx);
     c.rdy = (real)(1.0 / prm->dy);
@@ -506,6 +584,7 @@ hipError_t run_smag(const Dom& d, const arw_params* prm, real* buf,
     auto grid = [&](int nxs, int nys, int nlev) {
         return dim3((nxs + 63) / 64, (nys + 3) / 4, nlev);
     };
+    hipLaunchKernelGGL(k_smag_metric, grid(nx, ny, nz + 1), blk, 0, st, d, c, b);
     hipLaunchKernelGGL(k_smag_deform, grid(nx, ny, nz), blk, 0, st, d, c, b);
     hipLaunchKernelGGL(k_smag_km, grid(nx, ny, nz), blk, 0, st, d, c, b);
     hipLaunchKernelGGL(k_smag_tau, grid(nx, ny, nz), blk, 0, st, d, b);

@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--diff6", type=int, default=0, choices=(0, 1, 2),
                     help="c3/c2: run with WRF's held 6th-order filter "
                          "(diff_6th_opt; factor 0.12) to time its cost")
+    ap.add_argument("--km-opt", type=int, default=1, choices=(1, 4),
+                    help="c3/c2: 4 runs with WRF's held 2-D Smagorinsky "
+                         "mixing (km_opt=4, c_s=0.25) to time its cost")
     ap.add_argument("--species", type=int, default=-1,
                     choices=(-1, 0, 6, 8, 10, 18, 28),
                     help="c3/c2: run moist, carrying the species of this "
@@ -355,6 +358,9 @@ def main():
     if args.diff6:
         cfg = cfg.with_(diff_6th_opt=args.diff6)
         desc += f", held 6th-order filter diff_6th_opt={args.diff6}"
+    if args.km_opt == 4:
+        cfg = cfg.with_(km_opt=4)
+        desc += ", held 2-D Smagorinsky mixing km_opt=4"
     state = DomainState.from_host(case)
     del case
     torch.cuda.synchronize()

@@ -115,8 +115,8 @@ typedef struct arw_domain {
     double p_top;
     /* moist_cq's stage coefficients (u, v and w shapes; scratch) */
     float *cqu, *cqv, *cqw;
-    /* km_opt = 4 (arw_params): scratch of 6 nz ny nx + 2 (nz+1) ny nx
-     * elements for the deformation, the coefficients and the stresses */
+    /* km_opt = 4 (arw_params): scratch of 8 nz ny nx + 5 (nz+1) ny nx
+     * elements for the metrics, deformation, coefficients and stresses */
     float *sm_buf;
     /* moist states with held mixing (km_opt = 4 or diff_6th_opt): the held
      * tendency of every transported species, nz ny nx elements each, in
