@@ -689,18 +689,19 @@ def _restore(state, names, src: dict, suffix: str = "") -> None:
                          [src[n] for n in names])
 
 
-def _run_stage(state, cfg, nsub, dtau, record):
+def _run_stage(state, cfg, nsub, dtau, record, sums=True):
     """One stage's preparation and substeps from the state's stage and
     time-t fields; with ``record``, returns what the stage's adjoint reads
     (the preparation's outputs and every substep's small-step fields).  On
-    a moist state also the stage fluxes and the substep sums of u'', v'',
-    Omega'' (sumflux)."""
+    a moist state, with ``sums``, also the stage fluxes and the substep sums
+    of u'', v'', Omega'' (sumflux; the reverse sweep's recompute has them
+    on the tape already)."""
     from . import dycore as dyn
     nz, ny, nx = state.shape
     dyn.stage_prepare(state, cfg, dtau, True)
     prep = subs = None
     flx = None
-    if _moist(state):
+    if sums and _moist(state):
         ru, rv, ww = dyn.stage_fluxes(state, cfg)
         flx = _snapshot({"ru": ru, "rv": rv, "ww": ww})
     if record:
@@ -893,7 +894,8 @@ def step_adjoint_taped(state: DomainState, tape: StepTape,
     for xs, prep, subs, nsub, dtau, ex in reversed(tape.stages):
         _restore(state, names, xs)
         if prep is None:                 # recompute the stage's tape
-            prep, subs, _ = _run_stage(state, cfg, nsub, dtau, True)
+            prep, subs, _ = _run_stage(state, cfg, nsub, dtau, True,
+                                       sums=False)
         gq, inj = {}, None
         if moist:
             # the scalars' stage update (last in the stage): its adjoint
