@@ -62,6 +62,9 @@ struct Sc {
     int h5;
     // moist_cq: the pressure terms take d.cqu / d.cqv / d.cqw
     int cq;
+    // held mixing tendencies (d.d6_*) join the stage tendencies as they are
+    // stored (arw_step only)
+    int held;
 };
 
 static inline Sc make_sc(const arw_params* p, double dtau, int first)
@@ -94,6 +97,7 @@ static inline Sc make_sc(const arw_params* p, double dtau, int first)
     s.sum = 0;
     s.h5 = p->h_sca_adv_order == 5;
     s.cq = p->moist_cq != 0;
+    s.held = 0;
     return s;
 }
 
@@ -205,13 +209,15 @@ __device__ __forceinline__ EosPert eos_pert(real thp, real thb, real qv,
 // hypsometric_opt 2 al = dph / p_mid / log(p_bot / p_top_of_layer) - alb,
 // alt = al + alb, with the layer's dry pressures from the hybrid
 // coefficients c3/c4 and p_top.  The logarithm runs in double and is
-// rounded once.
+// rounded once.  XO = false compiles the hypsometric_opt = 1 form alone
+// (the step's plain variants, launched when no option needs the others).
+template <bool XO = true>
 __device__ __forceinline__ void eos_alpha(const Dom& d, int k, real mu,
                                           real dph, real albk, real c1k,
                                           real c2k, real rdnwk, real& alt,
                                           real& al)
 {
-    if (d.hypsometric_opt == 2) {
+    if (XO && d.hypsometric_opt == 2) {
         const real pt = (real)d.p_top;
         const real pfu = (d.c3f[k + 1] * mu + d.c4f[k + 1]) + pt;
         const real pfd = (d.c3f[k] * mu + d.c4f[k]) + pt;

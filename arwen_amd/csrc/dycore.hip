@@ -1292,13 +1292,13 @@ namespace {
 // created by arw_profile_begin, so the timed step allocates nothing.
 enum KernelId {
     KID_OMEGA = 0, KID_TEND_FACES, KID_TEND_COLS, KID_UV, KID_COLUMN,
-    KID_FINISH, KID_SET_W_SURFACE, KID_SCALARS, KID_DIFF6, KID_ADD_HELD,
-    KID_SMAG, KID_COUNT
+    KID_FINISH, KID_SET_W_SURFACE, KID_SCALARS, KID_DIFF6, KID_SMAG,
+    KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {
     "k_omega", "k_tend_faces", "k_tend_cols", "k_uv", "k_column",
     "k_finish", "k_set_w_surface", "moist_scalars", "k_diff6",
-    "k_add_held", "smag_held"};
+    "smag_held"};
 
 struct Profiler {
     bool active = false;
@@ -1845,31 +1845,6 @@ __global__ void k_diff6(Dom d, const real* __restrict__ f,
     out[o] = acc ? out[o] + (zero + (tx + ty)) : zero + (tx + ty);
 }
 
-// The held tendencies join every RK stage's slow tendencies, after the
-// constant-K diffusion (reference dycore.py:2389-2390,
-// add_fixed_dry_tendencies :1422-1430).
-__global__ void k_add_held(Dom d)
-{
-    const int nz = d.nz, ny = d.ny, nx = d.nx;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
-    if (i > nx || j > ny || k > nz) return;
-    if (j < ny && k < nz) {
-        const int x = iU(k, j, i, ny, nx);
-        d.ru_t[x] = d.ru_t[x] + d.d6_u[x];
-    }
-    if (i < nx && k < nz) {
-        const int x = iV(k, j, i, ny, nx);
-        d.rv_t[x] = d.rv_t[x] + d.d6_v[x];
-    }
-    if (i < nx && j < ny) {
-        const int x = iM(k, j, i, ny, nx);
-        d.rw_t[x] = d.rw_t[x] + d.d6_w[x];
-        if (k < nz) d.rth_t[x] = d.rth_t[x] + d.d6_t[x];
-    }
-}
-
 // The four held tendencies from the time-t fields of d (u, v, w, thp and
 // mub2d + mup), once per step.
 int32_t held_diff6(const Dom& d, const arw_params* prm, hipStream_t st,
@@ -2377,14 +2352,11 @@ static int32_t step_impl(const Dom* d, const arw_params* prm, Ckpt* ck,
             if ((rc = launched(hipGetLastError(), "k_calc_cq", -1, st)))
                 return rc;
         }
+        // the held tendencies join the stage's as the stage kernels store
+        // them, after the constant-K diffusion (reference dycore.py:
+        // 2389-2390, add_fixed_dry_tendencies :1422-1430)
+        s.held = d6;
         if ((rc = stage_prepare(dd, s, stage == 0, st))) return rc;
-        if (d6) {
-            hipLaunchKernelGGL(k_add_held, union_grid(dd, dd.nz + 1),
-                               dim3(64, 4), 0, st, dd);
-            if ((rc = launched(hipGetLastError(), "k_add_held", KID_ADD_HELD,
-                               st)))
-                return rc;
-        }
         Dom ds = dd;
         for (int n = 0; n < nsub[stage]; ++n) {
             Sc sn = make_sc(prm, dtaus[stage], n == 0);

@@ -92,7 +92,7 @@ __device__ __forceinline__ real pgf_val(
 // calc_p_alpha (diagnostics.cu:37-89) when the stage state is new; in the
 // precision form (PREC) its perturbation form, eos_pert, which also keeps
 // p' in d.pp.
-template <class SRC, bool EOS, bool B3, bool PREC>
+template <class SRC, bool EOS, bool B3, bool PREC, bool XO = true>
 __global__ __launch_bounds__(256) void k_omega(typename SRC::Arg a, Sc s)
 {
     int m, g;
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_omega(typename SRC::Arg a, Sc s)
                 if (d.qv) th = th * (R(1.0) + RVOVRD_F * gld(d.qv, h, bc));
                 real dph = (gb_hi + ph_hi) - (gb_lo + ph_lo);
                 real alt, al;
-                eos_alpha(d, k, ms, dph, albk, c1h[k], c2h[k], rdnw[k], alt,
+                eos_alpha<XO>(d, k, ms, dph, albk, c1h[k], c2h[k], rdnw[k], alt,
                           al);
                 gst(d.alt, h, bc, alt);
                 gst(d.al, h, bc, al);
@@ -275,7 +275,8 @@ struct PgfWin {
 // u''/v'' small-step initialisation (dycore.cu:947-1010).  The periodic
 // duplicate faces are finished by k_tend_faces_edge.  PREC (precision
 // form): the pressure gradient takes p' from the perturbation-form EOS.
-template <class SRC, bool B3, bool DIFF, bool PREC>
+template <class SRC, bool B3, bool DIFF, bool PREC, bool HELD = false,
+          bool XO = true>
 __global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
 {
     int m, g;
@@ -452,7 +453,7 @@ __global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
                                A.pp_0, W.pp_0, alA, alW, pbA, pbW, B3,
                                phh(A), phh(W), A.dpn_0, A.dpn_p, W.dpn_0,
                                W.dpn_p);
-            if (s.cq) pgu = gld(d.cqu, lu + j * pu, cC) * pgu;   // moist_cq
+            if (XO && s.cq) pgu = gld(d.cqu, lu + j * pu, cC) * pgu;   // moist_cq
             t = t - pgu;
             if (DIFF) {
                 real fc = x[3];
@@ -465,6 +466,9 @@ __global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
                 lap = lap + (s.kv * (fup - fdn)) * rdzc_h[k];
                 t = t + (c1 * msf_u + c2) * lap;
             }
+            // the held mixing tendency comes last (reference dycore.py:
+            // 2389-2390, add_fixed_dry_tendencies)
+            if (HELD) t = t + gld(d.d6_u, lu + j * pu, cC);
             gst(d.ru_t, lu + j * pu, cC, t);
             gst(d.u_pp, lu + j * pu, cC,
                 (c1 * mtf_u + c2) * gld(d.u0, lu + j * pu, cC)
@@ -508,7 +512,7 @@ __global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
                                A.pp_0, S.pp_0, alA, alS, pbA, pbS, B3,
                                phh(A), phh(S), A.dpn_0, A.dpn_p, S.dpn_0,
                                S.dpn_p);
-            if (s.cq) pgv = gld(d.cqv, lv + j * nx, cC) * pgv;    // moist_cq
+            if (XO && s.cq) pgv = gld(d.cqv, lv + j * nx, cC) * pgv;    // moist_cq
             t = t - pgv;
             if (DIFF) {
                 real fc = y[3];
@@ -521,6 +525,7 @@ __global__ __launch_bounds__(256) void k_tend_faces(typename SRC::Arg a, Sc s)
                 lap = lap + (s.kv * (fup - fdn)) * rdzc_h[k];
                 t = t + (c1 * msf_v + c2) * lap;
             }
+            if (HELD) t = t + gld(d.d6_v, lv + j * nx, cC);
             gst(d.rv_t, lv + j * nx, cC, t);
             gst(d.v_pp, lv + j * nx, cC,
                 (c1 * mtf_v + c2) * gld(d.v0, lv + j * nx, cC)
@@ -594,7 +599,8 @@ __global__ __launch_bounds__(256) void k_tend_faces_edge(typename SRC::Arg a)
 // its horizontal differences are those of php; theta'' is formed as
 // (c1h mu'') thb + (C(mu_t) thp_t - C(mu_s) thp_s), free of the base
 // theta's cancellation.
-template <class SRC, bool B3, bool DIFF, bool MOIST, bool PREC>
+template <class SRC, bool B3, bool DIFF, bool MOIST, bool PREC,
+          bool HELD = false, bool XO = true>
 __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
 {
     int m, g;
@@ -730,7 +736,7 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
                 // a mixed-phase scheme's condensate masses, in order
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
-                    if (e < d.n_load) qt_0 = qt_0 + gld(d.qx[e], lm + rA, cC);
+                    if (XO && e < d.n_load) qt_0 = qt_0 + gld(d.qx[e], lm + rA, cC);
             }
         }
         real ru_i = R(0.0), ru_ip = R(0.0), rv_j = R(0.0), rv_jp = R(0.0);
@@ -820,6 +826,7 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
             }
             t = t + (c1f[k] * mA + c2f[k]) * lap;
         }
+        if (HELD) t = t + gld(d.d6_w, lm + rA, cC);   // held, last
         gst(d.rw_t, lm + rA, cC, t);
 
         // --- rph_t: WRF rhs_ph ---
@@ -857,7 +864,7 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
             real fcy_jp = (c1f[k] * muy_jp + c2f[k]) * wjp;
             // the window holds php alone in the precision form
             real p0 = (PREC && B3) ? phb_at(k, rA, cC) + ph_0 : ph_0;
-            if (s.h5) {
+            if (XO && s.h5) {
                 // h_sca_adv_order = 5 (reference dycore.cu:668-748): the
                 // face-flux sum times (45 d1 - 9 d2 + d3) / 60 of the
                 // centred differences d_o = ph(+o) - ph(-o)
@@ -913,7 +920,7 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
             const real fz1 = zf(thz[1], thz[2], thz[3], thz[4], ww_p, k + 1, nz, fnm, fnp);
             real t2 = div3(fx1 - fx0, fy1 - fy0, fz1 - fzt, s.rdx, s.rdy, rdnw[k]);
             fzt = fz1;
-            if (MOIST && d.h_diabatic) {
+            if (XO && MOIST && d.h_diabatic) {
                 // the retained microphysics heating, coupled with the
                 // stage's column mass (reference dycore.py:572-592)
                 t2 = t2 + (c1h[k] * mA + c2h[k]) * gld(d.h_diabatic, lm + rA, cC);
@@ -929,6 +936,7 @@ __global__ __launch_bounds__(256) void k_tend_cols(typename SRC::Arg a, Sc s)
                 lap = lap + (s.kv * (fup - fdn)) * rdzc_h[k];
                 t2 = t2 + (c1h[k] * mA + c2h[k]) * lap;
             }
+            if (HELD) t2 = t2 + gld(d.d6_t, lm + rA, cC);
             gst(d.rth_t, lm + rA, cC, t2);
 
             phpp_p = gld(d.php0, lm + st + rA, cC) - gld(d.php, lm + st + rA, cC);
@@ -1006,7 +1014,8 @@ struct UvIn {
 // and north of the v face, W its west and S its south neighbour.  The
 // damped pressure p'' + smdiv (p'' - p''_old) rides in 4-level windows for
 // the surface and lid extrapolations of dpn.
-template <bool B3, bool USEP, bool DU, bool DV, bool SUM = false>
+template <bool B3, bool USEP, bool DU, bool DV, bool SUM = false,
+          bool XO = true>
 __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
                                          int rS, int rU, unsigned bA,
                                          unsigned bW, unsigned bS, unsigned bu,
@@ -1142,7 +1151,7 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
             if (B3) dphp = dphp + R(0.5) * ((gbA0 + gbA1) - (gbW0 + L.gbW));
             dpxy = dpxy + (s.rdx * dphp) * (rdnw[k] * (dn_up - dnu_0)
                                             - (R(0.5) * c1) * dmu_u);
-            if (s.cq) dpxy = gld(d.cqu, k * stu + rU, bu) * dpxy;   // moist_cq
+            if (XO && s.cq) dpxy = gld(d.cqu, k * stu + rU, bu) * dpxy;   // moist_cq
             const real un = L.u + dtau * (L.ru - dpxy);
             gst(d.u_pp, k * stu + rU, bu, un);
             if (SUM) {
@@ -1172,7 +1181,7 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
             if (B3) dphp = dphp + R(0.5) * ((gbA0 + gbA1) - (gbS0 + L.gbS));
             dpxy = dpxy + (s.rdy * dphp) * (rdnw[k] * (dn_up - dnv_0)
                                             - (R(0.5) * c1) * dmu_v);
-            if (s.cq) dpxy = gld(d.cqv, k * stv + rA, bv) * dpxy;   // moist_cq
+            if (XO && s.cq) dpxy = gld(d.cqv, k * stv + rA, bv) * dpxy;   // moist_cq
             const real vn = L.v + dtau * (L.rv - dpxy);
             gst(d.v_pp, k * stv + rA, bv, vn);
             if (SUM) {
@@ -1208,7 +1217,7 @@ __device__ __forceinline__ void uv_march(const Dom& d, const Sc& s, int rA,
 // column (j, i) advances u'' on face (j, i) and v'' on face (j, i).
 // Batched (Many): each member's logical range holds its edge workgroups
 // first; ne_arg is unused.
-template <class SRC, bool B3, bool USEP, bool SUM = false>
+template <class SRC, bool B3, bool USEP, bool SUM = false, bool XO = true>
 __global__ __launch_bounds__(256) void k_uv(typename SRC::Arg a, Sc s, int ne_arg)
 {
     int m, g;
@@ -1220,12 +1229,12 @@ __global__ __launch_bounds__(256) void k_uv(typename SRC::Arg a, Sc s, int ne_ar
         const int t = g * 256 + threadIdx.y * 64 + threadIdx.x;
         if (t < ny) {
             const int j = t;
-            uv_march<B3, USEP, true, false, SUM>(
+            uv_march<B3, USEP, true, false, SUM, XO>(
                 d, s, 0, 0, 0, RB * (j * nx), RB * (j * nx + nx - 1), 0u,
                 RB * (j * pu + nx), 0u);
         } else if (t < ny + nx) {
             const int i = t - ny;
-            uv_march<B3, USEP, false, true, SUM>(
+            uv_march<B3, USEP, false, true, SUM, XO>(
                 d, s, 0, 0, 0, RB * i, 0u, RB * ((ny - 1) * nx + i), 0u,
                 RB * (ny * nx + i));
         }
@@ -1237,7 +1246,7 @@ __global__ __launch_bounds__(256) void k_uv(typename SRC::Arg a, Sc s, int ne_ar
     const int i = rs.i, j = rs.j;
     const int jm = (j == 0) ? ny - 1 : j - 1;
     const unsigned bc = RB * i, bw = RB * ((i == 0) ? nx - 1 : i - 1);
-    uv_march<B3, USEP, true, true, SUM>(d, s, j * nx, jm * nx, j * pu, bc, bw, bc,
+    uv_march<B3, USEP, true, true, SUM, XO>(d, s, j * nx, jm * nx, j * pu, bc, bw, bc,
                                    bc, bc);
 }
 
@@ -1286,7 +1295,8 @@ struct ColDn {
 // (2 s.park values per column, dynamic shared memory) instead of parking
 // them in HBM between the sweeps; rows below go to d.gam / d.w_pp, gamma
 // only when the stage has not stored it yet (s.gam_ready = 0).
-template <class SRC, bool B3, bool DAMP, bool AL, bool WW, bool PREC>
+template <class SRC, bool B3, bool DAMP, bool AL, bool WW, bool PREC,
+          bool XO = true>
 __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
                                                 real* RS pout_arg)
 {
@@ -1453,7 +1463,7 @@ __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
         real pg = (c_force * rdn[r])
                   * ((c2c * rdnw[r]) / chmc * dph_up
                      - (c2dn * rdnw[r - 1]) / chmdn * dph_dn);
-        if (s.cq) pg = gld(d.cqw, r * st, bc) * pg;            // moist_cq
+        if (XO && s.cq) pg = gld(d.cqw, r * st, bc) * pg;            // moist_cq
         real t1 = dtau * rwr + pg;
         real t3 = c_buoy * (rdn[r] * ((c2c * altc) * t2c - (c2dn * altdn) * t2dn)
                              - c1f[r] * muave);
@@ -1463,7 +1473,7 @@ __global__ __launch_bounds__(256) void k_column(typename SRC::Arg a, Sc s,
     auto lu_row = [&](int r, real c2dn, real c2c, real chmdn, real chmc,
                       tri& alpha_r, bool r_ge2) {
         // moist_cq: the row's coefficient takes cqw(r) first
-        const tri tcof = s.cq ? (tri)gld(d.cqw, r * st, bc) * (tri)cof
+        const tri tcof = (XO && s.cq) ? (tri)gld(d.cqw, r * st, bc) * (tri)cof
                               : (tri)cof;
         const tri rdn_r = rdn[r], rdnw_r = rdnw[r];
         const tri rdnw_m = rdnw[r - 1];
@@ -1758,7 +1768,7 @@ struct FinIn {
 // d.h_diabatic set): the theta fold removes hdt C(mu_s) h_diabatic, what
 // the substeps integrated from the stage tendency's heating term
 // (reference dycore.cu:1186-1194, module_small_step_em.F:408-426).
-template <class SRC, bool B3, bool EOS, bool PREC>
+template <class SRC, bool B3, bool EOS, bool PREC, bool XO = true>
 __global__ __launch_bounds__(256) void k_finish(typename SRC::Arg a,
                                                 FinishDst o_arg, real hdt)
 {
@@ -1854,7 +1864,7 @@ __global__ __launch_bounds__(256) void k_finish(typename SRC::Arg a,
                 real th = thb_lo + thp_lo;
                 real dph = (phb_k + php_new) - (phb_lo + php_lo);
                 real alt, al;
-                eos_alpha(d, k - 1, mu_eos, dph, alb_lo, c1h[k - 1],
+                eos_alpha<XO>(d, k - 1, mu_eos, dph, alb_lo, c1h[k - 1],
                           c2h[k - 1], rdnw[k - 1], alt, al);
                 gst(d.alt, h, bc, alt);
                 gst(d.al, h, bc, al);
@@ -1871,7 +1881,7 @@ __global__ __launch_bounds__(256) void k_finish(typename SRC::Arg a,
             } else {
                 const real cs = c1h[k] * msA + c2h[k];
                 real num = cs * (thbk + L.thp) + L.thpp;
-                if (hdt > R(0.0))
+                if (XO && hdt > R(0.0))
                     num = num - (hdt * cs) * gld(d.h_diabatic, f, bc);
                 thn = num / (c1h[k] * mnA + c2h[k]) - thbk;
             }
@@ -2310,22 +2320,42 @@ static inline dim3 masses(const Dom& d)
     return dim3((d.nx + 63) / 64, (d.ny + 3) / 4);
 }
 
-template <bool PREC>
+// Whether a step option reaches the stage and substep kernels (moist_cq,
+// h_sca_adv_order = 5, hypsometric_opt = 2, the retained heating, loading
+// extra species): the plain step launches the variants compiled without
+// them (XO = false), which keeps their registers and instruction count
+// those of the step without options.
+static inline bool xo_active(const Dom& d, const Sc& s)
+{
+    return s.prec || s.cq || s.h5 || d.hypsometric_opt == 2
+           || d.h_diabatic != nullptr || d.n_load > 0;
+}
+
+// Calls f(std::true_type) when the options are live, else f(false_type).
+template <class F>
+static void with_xo(bool xo, F&& f)
+{
+    if (xo) f(std::true_type{});
+    else f(std::false_type{});
+}
+
+template <bool PREC, bool XO>
 static void launch_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st)
 {
     const bool b3 = d.base3d != 0;
     if (do_eos && b3)
-        hipLaunchKernelGGL((k_omega<One, true, true, PREC>), cols(d), dim3(256), 0, st, d, s);
+        hipLaunchKernelGGL((k_omega<One, true, true, PREC, XO>), cols(d), dim3(256), 0, st, d, s);
     else if (do_eos)
-        hipLaunchKernelGGL((k_omega<One, true, false, PREC>), cols(d), dim3(256), 0, st, d, s);
+        hipLaunchKernelGGL((k_omega<One, true, false, PREC, XO>), cols(d), dim3(256), 0, st, d, s);
     else
-        hipLaunchKernelGGL((k_omega<One, false, false, PREC>), cols(d), dim3(256), 0, st, d, s);
+        hipLaunchKernelGGL((k_omega<One, false, false, PREC, XO>), cols(d), dim3(256), 0, st, d, s);
 }
 
 hipError_t run_omega(const Dom& d, const Sc& s, int do_eos, hipStream_t st)
 {
-    if (s.prec) launch_omega<true>(d, s, do_eos, st);
-    else launch_omega<false>(d, s, do_eos, st);
+    if (s.prec) launch_omega<true, true>(d, s, do_eos, st);
+    else if (xo_active(d, s)) launch_omega<false, true>(d, s, do_eos, st);
+    else launch_omega<false, false>(d, s, do_eos, st);
     return hipGetLastError();
 }
 
@@ -2351,58 +2381,90 @@ static void with_prec(bool prec, F&& f)
 // words per point at 1000x1000x50) and the kernel runs slower.
 constexpr size_t kTendFacesLds = 40 * 1024;
 
-template <class SRC, bool PREC>
+template <class SRC, bool PREC, bool XO = true>
 static void launch_tend_faces(const typename SRC::Arg& a, dim3 g, const Sc& s,
                               bool b3, hipStream_t st)
 {
     const dim3 b(64, 4);
     const size_t l = kTendFacesLds;
     if (b3 && s.diff)
-        hipLaunchKernelGGL((k_tend_faces<SRC, true, true, PREC>), g, b, l, st, a, s);
+        hipLaunchKernelGGL((k_tend_faces<SRC, true, true, PREC, false, XO>), g, b, l, st, a, s);
     else if (b3)
-        hipLaunchKernelGGL((k_tend_faces<SRC, true, false, PREC>), g, b, l, st, a, s);
+        hipLaunchKernelGGL((k_tend_faces<SRC, true, false, PREC, false, XO>), g, b, l, st, a, s);
     else if (s.diff)
-        hipLaunchKernelGGL((k_tend_faces<SRC, false, true, PREC>), g, b, l, st, a, s);
+        hipLaunchKernelGGL((k_tend_faces<SRC, false, true, PREC, false, XO>), g, b, l, st, a, s);
     else
-        hipLaunchKernelGGL((k_tend_faces<SRC, false, false, PREC>), g, b, l, st, a, s);
+        hipLaunchKernelGGL((k_tend_faces<SRC, false, false, PREC, false, XO>), g, b, l, st, a, s);
 }
 
-template <class SRC, bool MOIST, bool PREC>
+template <class SRC, bool MOIST, bool PREC, bool XO = true>
 static void launch_tend_cols(const typename SRC::Arg& a, dim3 g, const Sc& s,
                              bool b3, hipStream_t st)
 {
     const dim3 b(64, 4);
     if (b3 && s.diff)
-        hipLaunchKernelGGL((k_tend_cols<SRC, true, true, MOIST, PREC>), g, b, 0, st, a, s);
+        hipLaunchKernelGGL((k_tend_cols<SRC, true, true, MOIST, PREC, false, XO>), g, b, 0, st, a, s);
     else if (b3)
-        hipLaunchKernelGGL((k_tend_cols<SRC, true, false, MOIST, PREC>), g, b, 0, st, a, s);
+        hipLaunchKernelGGL((k_tend_cols<SRC, true, false, MOIST, PREC, false, XO>), g, b, 0, st, a, s);
     else if (s.diff)
-        hipLaunchKernelGGL((k_tend_cols<SRC, false, true, MOIST, PREC>), g, b, 0, st, a, s);
+        hipLaunchKernelGGL((k_tend_cols<SRC, false, true, MOIST, PREC, false, XO>), g, b, 0, st, a, s);
     else
-        hipLaunchKernelGGL((k_tend_cols<SRC, false, false, MOIST, PREC>), g, b, 0, st, a, s);
+        hipLaunchKernelGGL((k_tend_cols<SRC, false, false, MOIST, PREC, false, XO>), g, b, 0, st, a, s);
 }
 
-template <class SRC, bool PREC>
+template <class SRC, bool PREC, bool XO = true>
 static void launch_finish(const typename SRC::Arg& a, dim3 g,
                           const FinishDst& o, bool b3, bool eos,
                           hipStream_t st, real hdt = R(0.0))
 {
     if (b3 && eos)
-        hipLaunchKernelGGL((k_finish<SRC, true, true, PREC>), g, dim3(256), 0, st, a, o, hdt);
+        hipLaunchKernelGGL((k_finish<SRC, true, true, PREC, XO>), g, dim3(256), 0, st, a, o, hdt);
     else if (b3)
-        hipLaunchKernelGGL((k_finish<SRC, true, false, PREC>), g, dim3(256), 0, st, a, o, hdt);
+        hipLaunchKernelGGL((k_finish<SRC, true, false, PREC, XO>), g, dim3(256), 0, st, a, o, hdt);
     else if (eos)
-        hipLaunchKernelGGL((k_finish<SRC, false, true, PREC>), g, dim3(256), 0, st, a, o, hdt);
+        hipLaunchKernelGGL((k_finish<SRC, false, true, PREC, XO>), g, dim3(256), 0, st, a, o, hdt);
     else
-        hipLaunchKernelGGL((k_finish<SRC, false, false, PREC>), g, dim3(256), 0, st, a, o, hdt);
+        hipLaunchKernelGGL((k_finish<SRC, false, false, PREC, XO>), g, dim3(256), 0, st, a, o, hdt);
+}
+
+// The held-mixing variants: one domain, plain form (arw_step's held
+// tendencies refuse the precision form and the ensemble).
+template <bool B3, bool DIFF>
+static void launch_tend_faces_held(const Dom& d, dim3 g, const Sc& s,
+                                   hipStream_t st)
+{
+    hipLaunchKernelGGL((k_tend_faces<One, B3, DIFF, false, true>), g,
+                       dim3(64, 4), kTendFacesLds, st, d, s);
+}
+
+template <bool B3, bool DIFF>
+static void launch_tend_cols_held(const Dom& d, dim3 g, const Sc& s,
+                                  hipStream_t st)
+{
+    if (d.qv)
+        hipLaunchKernelGGL((k_tend_cols<One, B3, DIFF, true, false, true>), g,
+                           dim3(64, 4), 0, st, d, s);
+    else
+        hipLaunchKernelGGL((k_tend_cols<One, B3, DIFF, false, false, true>), g,
+                           dim3(64, 4), 0, st, d, s);
 }
 
 hipError_t run_tend_faces(const Dom& d, const Sc& s, hipStream_t st)
 {
     const dim3 g = rowseg_grid(d.nx, d.ny);
-    with_prec(s.prec, [&](auto p) {
-        launch_tend_faces<One, decltype(p)::value>(d, g, s, d.base3d != 0, st);
-    });
+    const bool b3 = d.base3d != 0;
+    if (s.held && !s.prec) {
+        if (b3 && s.diff) launch_tend_faces_held<true, true>(d, g, s, st);
+        else if (b3) launch_tend_faces_held<true, false>(d, g, s, st);
+        else if (s.diff) launch_tend_faces_held<false, true>(d, g, s, st);
+        else launch_tend_faces_held<false, false>(d, g, s, st);
+    } else if (xo_active(d, s)) {
+        with_prec(s.prec, [&](auto p) {
+            launch_tend_faces<One, decltype(p)::value>(d, g, s, b3, st);
+        });
+    } else {
+        launch_tend_faces<One, false, false>(d, g, s, b3, st);
+    }
     const int n = d.nz * d.ny + d.nz * d.nx;
     hipLaunchKernelGGL(k_tend_faces_edge<One>, dim3((n + 255) / 256), dim3(256), 0,
                        st, d);
@@ -2413,6 +2475,18 @@ hipError_t run_tend_cols(const Dom& d, const Sc& s, hipStream_t st)
 {
     const dim3 g = rowseg_grid(d.nx, d.ny);
     const bool b3 = d.base3d != 0;
+    if (s.held && !s.prec) {
+        if (b3 && s.diff) launch_tend_cols_held<true, true>(d, g, s, st);
+        else if (b3) launch_tend_cols_held<true, false>(d, g, s, st);
+        else if (s.diff) launch_tend_cols_held<false, true>(d, g, s, st);
+        else launch_tend_cols_held<false, false>(d, g, s, st);
+        return hipGetLastError();
+    }
+    if (!xo_active(d, s)) {
+        if (d.qv) launch_tend_cols<One, true, false, false>(d, g, s, b3, st);
+        else launch_tend_cols<One, false, false, false>(d, g, s, b3, st);
+        return hipGetLastError();
+    }
     with_prec(s.prec, [&](auto p) {
         constexpr bool P = decltype(p)::value;
         if (d.qv) launch_tend_cols<One, true, P>(d, g, s, b3, st);
@@ -2421,26 +2495,29 @@ hipError_t run_tend_cols(const Dom& d, const Sc& s, hipStream_t st)
     return hipGetLastError();
 }
 
-template <bool B3, bool USEP>
+template <bool B3, bool USEP, bool XO>
 static void launch_uv(const Dom& d, const Sc& s, hipStream_t st)
 {
     const int ne = (int)xcd_grid((unsigned)((d.ny + d.nx + 255) / 256));
     const dim3 g(rowseg_grid(d.nx, d.ny).x + ne);
     if (s.sum)
-        hipLaunchKernelGGL((k_uv<One, B3, USEP, true>), g, dim3(64, 4), 0, st,
-                           d, s, ne);
+        hipLaunchKernelGGL((k_uv<One, B3, USEP, true, XO>), g, dim3(64, 4), 0,
+                           st, d, s, ne);
     else
-        hipLaunchKernelGGL((k_uv<One, B3, USEP>), g, dim3(64, 4), 0, st, d, s,
-                           ne);
+        hipLaunchKernelGGL((k_uv<One, B3, USEP, false, XO>), g, dim3(64, 4), 0,
+                           st, d, s, ne);
 }
 
 hipError_t run_uv(const Dom& d, const Sc& s, hipStream_t st)
 {
     const bool b3 = d.base3d != 0, usep = s.use_pold != 0;
-    if (b3 && usep) launch_uv<true, true>(d, s, st);
-    else if (b3) launch_uv<true, false>(d, s, st);
-    else if (usep) launch_uv<false, true>(d, s, st);
-    else launch_uv<false, false>(d, s, st);
+    with_xo(xo_active(d, s), [&](auto x) {
+        constexpr bool X = decltype(x)::value;
+        if (b3 && usep) launch_uv<true, true, X>(d, s, st);
+        else if (b3) launch_uv<true, false, X>(d, s, st);
+        else if (usep) launch_uv<false, true, X>(d, s, st);
+        else launch_uv<false, false, X>(d, s, st);
+    });
     return hipGetLastError();
 }
 
@@ -2468,9 +2545,12 @@ static void launch_column(const Dom& d, const Sc& s0, real* pout,
     if (s.prec)
         hipLaunchKernelGGL((k_column<One, B3, DAMP, AL, WW, true>), cols(d),
                            dim3(256), lds, st, d, s, pout);
-    else
+    else if (xo_active(d, s))
         hipLaunchKernelGGL((k_column<One, B3, DAMP, AL, WW, false>), cols(d),
                            dim3(256), lds, st, d, s, pout);
+    else
+        hipLaunchKernelGGL((k_column<One, B3, DAMP, AL, WW, false, false>),
+                           cols(d), dim3(256), lds, st, d, s, pout);
 }
 
 template <bool B3, bool DAMP>
@@ -2500,10 +2580,16 @@ hipError_t run_finish(const Dom& d, const FinishDst& o, bool prec,
     // a moist state's EOS waits for the scalar update of the stage
     const bool eos = d.qv == nullptr;
     const real hdt = d.h_diabatic ? (real)hdiab_dt : R(0.0);
-    with_prec(prec, [&](auto p) {
-        launch_finish<One, decltype(p)::value>(d, cols(d), o, d.base3d != 0,
-                                               eos, st, hdt);
-    });
+    const bool xo = prec || d.hypsometric_opt == 2 || d.h_diabatic != nullptr;
+    if (xo) {
+        with_prec(prec, [&](auto p) {
+            launch_finish<One, decltype(p)::value>(d, cols(d), o,
+                                                   d.base3d != 0, eos, st, hdt);
+        });
+    } else {
+        launch_finish<One, false, false>(d, cols(d), o, d.base3d != 0, eos, st,
+                                         hdt);
+    }
     hipLaunchKernelGGL(k_finish_edge<One>, dim3((d.ny + d.nx + 255) / 256),
                        dim3(256), 0, st, d, o);
     return hipGetLastError();
