@@ -83,7 +83,10 @@ def step_with_checkpoints(state: DomainState, cfg: RunConfig,
         raise ValueError("checkpoint buffers do not match the state")
     d = state.domain_struct(diffusion=_uses_diffusion(cfg),
                             precision_form=bool(cfg.precision_form),
-                            diff6=cfg.diff_6th_opt > 0)
+                            diff6=cfg.diff_6th_opt > 0,
+                            hypsometric_opt=cfg.hypsometric_opt,
+                            moist_cq=bool(cfg.moist_cq),
+                            smag=cfg.km_opt == 4)
     name, fn = _fn(state, "step_ckpt")
     _lib.check(fn(d, params(cfg), ctypes.byref(ckpt.c), _stream()), name)
     state.elapsed_seconds += cfg.dt

@@ -150,3 +150,30 @@ def test_ctypes_offsets_match_the_c_compiler(cname, pyt, tmp_path):
     got = _c_layout(cname, names, tmp_path)
     assert got[0] == ctypes.sizeof(pyt)
     assert got[1:] == [getattr(pyt, n).offset for n in names]
+
+
+def test_held_mixing_scratch_and_params():
+    """km_opt = 4 and the filter on a moist state: the domain view carries
+    the Smagorinsky work area (8 nz ny nx + 5 (nz+1) ny nx) and one held
+    slot per species; the filter's species mask follows moist_mix6_off."""
+    from arwen_amd import cases, dycore
+    from arwen_amd.state import DomainState
+    case = cases.mixed_phase(10, nx=8, ny=6, nz=5)
+    cfg = case.cfg.with_(km_opt=4, diff_6th_opt=2)
+    st = DomainState.from_host(case, device="cpu")
+    d = st.domain_struct(diff6=True, smag=True)
+    nz, ny, nx = 5, 6, 8
+    n, nw = nz * ny * nx, (nz + 1) * ny * nx
+    assert st._scratch["arw_sm_buf"].numel() == 8 * n + 5 * nw
+    nq = 3 + len(st.extra_species)
+    assert tuple(st._scratch["arw_held_q"].shape) == (nq, nz, ny, nx)
+    assert d.sm_buf and d.held_q and d.d6_u and d.d6_t
+    p = dycore.params(cfg)
+    assert p.km_opt == 4 and p.isfflx == 1 and abs(p.c_s - 0.25) < 1e-15
+    assert p.diff6_species == (1 << nq) - 1
+    off = dycore.params(cfg.with_(moist_mix6_off=True)).diff6_species
+    names = ("qv", "qc", "qr") + st.extra_species
+    for e, name in enumerate(names):
+        kept = bool((off >> e) & 1)
+        assert kept == (name not in dycore.MOIST_ARRAY), name
+    assert dycore.params(case.cfg).diff6_species == 0
