@@ -541,6 +541,81 @@ SMAG_BODY(smag_theta_at, const Dom& d, const SmagSc& c, const SmagBuf& b, real* 
 SMAG_POINTS(k_smag_theta, smag_theta_at, d.nx, d.ny, d.nz,
            (Dom d, SmagSc c, SmagBuf b, real* __restrict__ out, int heat), d, c, b, out, heat)
 
+// The species rows in groups of up to three (the work area's free slots
+// hold their fluxes): one pass forms every species' face fluxes, sharing
+// the density, K_h and metric loads of the point, and one pass their
+// divergences.  Per species the arithmetic is smag_flux_at's and
+// smag_theta_at's, in their order.
+struct SmagQ {
+    const real* src[3];
+    real* fx[3];
+    real* fy[3];
+    real* out[3];
+};
+
+template <int NS>
+__global__ void k_smag_fluxq(Dom d, SmagSc c, SmagBuf b, SmagQ q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny || k >= d.nz) return;
+    const real h = R(0.5), one = R(1.0), two = R(2.0);
+    const int x = iM(k, j, i, d.ny, d.nx);
+    const real rz = rdzw(d, b, k, j, i);
+    const real rax = h * (rho(d, b, k, j, i - 1) + rho(d, b, k, j, i));
+    const real kax = h * (at(d, b.kh, k, j, i - 1) + b.kh[x]);
+    const real tzx = h * (zx(d, b, k, j, i) + zx(d, b, k + 1, j, i));
+    const real rdzu = two / (one / rz + one / rdzw(d, b, k, j, i - 1));
+    const real ray = h * (rho(d, b, k, j - 1, i) + rho(d, b, k, j, i));
+    const real kay = h * (at(d, b.kh, k, j - 1, i) + b.kh[x]);
+    const real tzy = h * (zy(d, b, k, j, i) + zy(d, b, k + 1, j, i));
+    const real rdzv = two / (one / rz + one / rdzw(d, b, k, j - 1, i));
+#pragma unroll
+    for (int e = 0; e < NS; ++e) {
+        const real* S = q.src[e];
+        auto sv = [&](int kk, int jj, int ii) { return S[mix(d, kk, jj, ii)]; };
+        const real s0 = S[x];
+        auto px = [&](int kk) { return sv(kk, j, i - 1) + sv(kk, j, i); };
+        const real gx = c.rdx * (s0 - sv(k, j, i - 1))
+                        - (tzx * (full(d, k + 1, px) - full(d, k, px))) * rdzu;
+        q.fx[e][x] = ((-rax) * kax) * gx;
+        auto py = [&](int kk) { return sv(kk, j - 1, i) + sv(kk, j, i); };
+        const real gy = c.rdy * (s0 - sv(k, j - 1, i))
+                        - (tzy * (full(d, k + 1, py) - full(d, k, py))) * rdzv;
+        q.fy[e][x] = ((-ray) * kay) * gy;
+    }
+}
+
+template <int NS>
+__global__ void k_smag_divq(Dom d, SmagSc c, SmagBuf b, SmagQ q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny || k >= d.nz) return;
+    const real qq = R(0.25);
+    const int x = iM(k, j, i, d.ny, d.nx);
+    const real zxm = qq * (((zx(d, b, k, j, i) + zx(d, b, k, j, i + 1))
+                            + zx(d, b, k + 1, j, i)) + zx(d, b, k + 1, j, i + 1));
+    const real zym = qq * (((zy(d, b, k, j, i) + zy(d, b, k, j + 1, i))
+                            + zy(d, b, k + 1, j, i)) + zy(d, b, k + 1, j + 1, i));
+    const real rz = rdzw(d, b, k, j, i);
+    const real fac = G_F / (d.dnw[k] * rz);
+#pragma unroll
+    for (int e = 0; e < NS; ++e) {
+        const real* fx = q.fx[e];
+        const real* fy = q.fy[e];
+        const real divh = c.rdx * (at(d, fx, k, j, i + 1) - fx[x])
+                          + c.rdy * (at(d, fy, k, j + 1, i) - fy[x]);
+        const real divz = (zxm * (mavg(d, fx, k + 1, j, i, 0, 1)
+                                  - mavg(d, fx, k, j, i, 0, 1))) * rz
+                          + (zym * (mavg(d, fy, k + 1, j, i, 1, 0)
+                                    - mavg(d, fy, k, j, i, 1, 0))) * rz;
+        q.out[e][x] = R(0.0) + fac * (divh - divz);
+    }
+}
+
 }  // namespace
 
 // The held km_opt = 4 tendencies of the time-t fields of d (u, v, w, thp,
@@ -596,11 +671,31 @@ hipError_t run_smag(const Dom& d, const arw_params* prm, real* buf,
                        (const real*)nullptr);
     hipLaunchKernelGGL(k_smag_theta, grid(nx, ny, nz), blk, 0, st, d, c, b,
                        out_t, 1);
-    for (int e = 0; e < nq; ++e) {
-        hipLaunchKernelGGL(k_smag_flux, grid(nx, ny, nz), blk, 0, st, d, c, b,
-                           q_src[e]);
-        hipLaunchKernelGGL(k_smag_theta, grid(nx, ny, nz), blk, 0, st, d, c, b,
-                           held_q + (size_t)e * n, 0);
+    // the species in groups of three; their fluxes take the work area's
+    // slots the momentum passes are done with
+    real* fxs[3] = {b.t11, b.t12, b.t13};
+    real* fys[3] = {b.t22, b.t33, b.t23};
+    for (int e0 = 0; e0 < nq; e0 += 3) {
+        SmagQ q;
+        const int ns = nq - e0 < 3 ? nq - e0 : 3;
+        for (int e = 0; e < 3; ++e) {
+            const int s = e0 + (e < ns ? e : 0);
+            q.src[e] = q_src[s];
+            q.fx[e] = fxs[e];
+            q.fy[e] = fys[e];
+            q.out[e] = held_q + (size_t)s * n;
+        }
+        const dim3 g = grid(nx, ny, nz);
+        if (ns == 3) {
+            hipLaunchKernelGGL(k_smag_fluxq<3>, g, blk, 0, st, d, c, b, q);
+            hipLaunchKernelGGL(k_smag_divq<3>, g, blk, 0, st, d, c, b, q);
+        } else if (ns == 2) {
+            hipLaunchKernelGGL(k_smag_fluxq<2>, g, blk, 0, st, d, c, b, q);
+            hipLaunchKernelGGL(k_smag_divq<2>, g, blk, 0, st, d, c, b, q);
+        } else {
+            hipLaunchKernelGGL(k_smag_fluxq<1>, g, blk, 0, st, d, c, b, q);
+            hipLaunchKernelGGL(k_smag_divq<1>, g, blk, 0, st, d, c, b, q);
+        }
     }
     return hipGetLastError();
 }
