@@ -1845,6 +1845,69 @@ __global__ void k_diff6(Dom d, const real* __restrict__ f,
     out[o] = acc ? out[o] + (zero + (tx + ty)) : zero + (tx + ty);
 }
 
+// The filter's species rows (mass points, plain periodic) in groups of
+// up to four: the column masses, slope taper and C(mu) face factors of
+// the point are formed once and shared; per species the arithmetic is
+// k_diff6's, in its order.
+struct D6Q {
+    const real* f[4];
+    real* out[4];
+};
+
+template <int NS>
+__global__ void k_diff6q(Dom d, D6Q q, real coef, int mono, int slope,
+                         real dzthr_x, real dzthr_y, int acc)
+{
+    const int ny = d.ny, nx = d.nx;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= d.nz) return;
+    const size_t o = ((size_t)k * ny + j) * nx + i;
+    auto MU = [&](int jj, int ii) {
+        const int c = pmod(jj, ny) * nx + pmod(ii, nx);
+        return d.mub2d[c] + d.mup[c];
+    };
+    auto PHB = [&](int jj, int ii) {
+        return d.phb[((size_t)k * ny + pmod(jj, ny)) * nx + pmod(ii, nx)];
+    };
+    const real ten = R(10.0), five = R(5.0), zero = R(0.0), one = R(1.0);
+    const real mx0 = R(0.5) * (MU(j, i - 1) + MU(j, i));
+    const real mx1 = R(0.5) * (MU(j, i) + MU(j, i + 1));
+    const real my0 = R(0.5) * (MU(j - 1, i) + MU(j, i));
+    const real my1 = R(0.5) * (MU(j, i) + MU(j + 1, i));
+    real sx0 = one, sx1 = one, sy0 = one, sy1 = one;
+    if (slope) {
+        sx0 = fmax(one - fabs(PHB(j, i) - PHB(j, i - 1)) / dzthr_x, zero);
+        sx1 = fmax(one - fabs(PHB(j, i + 1) - PHB(j, i)) / dzthr_x, zero);
+        sy0 = fmax(one - fabs(PHB(j, i) - PHB(j - 1, i)) / dzthr_y, zero);
+        sy1 = fmax(one - fabs(PHB(j + 1, i) - PHB(j, i)) / dzthr_y, zero);
+    }
+    const real c1 = d.c1h[k], c2 = d.c2h[k];
+    const real cx1 = sx1 * (c1 * mx1 + c2), cx0 = sx0 * (c1 * mx0 + c2);
+    const real cy1 = sy1 * (c1 * my1 + c2), cy0 = sy0 * (c1 * my0 + c2);
+#pragma unroll
+    for (int e = 0; e < NS; ++e) {
+        const real* f = q.f[e];
+        auto FX = [&](int m) { return f[((size_t)k * ny + j) * nx + pmod(i + m, nx)]; };
+        auto FY = [&](int m) { return f[((size_t)k * ny + pmod(j + m, ny)) * nx + i]; };
+        real px0 = ten * (FX(0) - FX(-1)) - five * (FX(1) - FX(-2)) + (FX(2) - FX(-3));
+        real px1 = ten * (FX(1) - FX(0)) - five * (FX(2) - FX(-1)) + (FX(3) - FX(-2));
+        real py0 = ten * (FY(0) - FY(-1)) - five * (FY(1) - FY(-2)) + (FY(2) - FY(-3));
+        real py1 = ten * (FY(1) - FY(0)) - five * (FY(2) - FY(-1)) + (FY(3) - FY(-2));
+        if (mono == 2) {
+            if (px0 * (FX(0) - FX(-1)) <= zero) px0 = zero;
+            if (px1 * (FX(1) - FX(0)) <= zero) px1 = zero;
+            if (py0 * (FY(0) - FY(-1)) <= zero) py0 = zero;
+            if (py1 * (FY(1) - FY(0)) <= zero) py1 = zero;
+        }
+        const real tx = coef * (cx1 * px1 - cx0 * px0);
+        const real ty = coef * (cy1 * py1 - cy0 * py0);
+        real* out = q.out[e];
+        out[o] = acc ? out[o] + (zero + (tx + ty)) : zero + (tx + ty);
+    }
+}
+
 // The four held tendencies from the time-t fields of d (u, v, w, thp and
 // mub2d + mup), once per step.
 int32_t held_diff6(const Dom& d, const arw_params* prm, hipStream_t st,
@@ -1874,12 +1937,24 @@ int32_t held_diff6(const Dom& d, const arw_params* prm, hipStream_t st,
     // the transported species the filter acts on (mass rows, reference
     // dycore.py:1387-1415), into their held slots
     const size_t n = (size_t)nz * ny * nx;
-    for (int e = 0; e < nq; ++e) {
-        if (!((prm->diff6_species >> e) & 1u)) continue;
-        hipLaunchKernelGGL(k_diff6, dim3((nx + 63) / 64, (ny + 3) / 4, nz),
-                           dim3(64, 4), 0, st, d, q_src[e], d.held_q + e * n,
-                           d.c1h, d.c2h, coef, mono, slope, tx, ty, nz, ny, nx,
-                           0, 0, acc);
+    int sel[3 + ARW_MAX_EXTRA], ns = 0;
+    for (int e = 0; e < nq; ++e)
+        if ((prm->diff6_species >> e) & 1u) sel[ns++] = e;
+    const dim3 g((nx + 63) / 64, (ny + 3) / 4, nz), b(64, 4);
+    for (int s0 = 0; s0 < ns; s0 += 4) {
+        const int m = ns - s0 < 4 ? ns - s0 : 4;
+        D6Q q;
+        for (int e = 0; e < 4; ++e) {
+            const int sp = sel[s0 + (e < m ? e : 0)];
+            q.f[e] = q_src[sp];
+            q.out[e] = d.held_q + sp * n;
+        }
+        switch (m) {
+        case 4: hipLaunchKernelGGL(k_diff6q<4>, g, b, 0, st, d, q, coef, mono, slope, tx, ty, acc); break;
+        case 3: hipLaunchKernelGGL(k_diff6q<3>, g, b, 0, st, d, q, coef, mono, slope, tx, ty, acc); break;
+        case 2: hipLaunchKernelGGL(k_diff6q<2>, g, b, 0, st, d, q, coef, mono, slope, tx, ty, acc); break;
+        default: hipLaunchKernelGGL(k_diff6q<1>, g, b, 0, st, d, q, coef, mono, slope, tx, ty, acc); break;
+        }
         int32_t rc = launched(hipGetLastError(), "k_diff6", KID_DIFF6, st);
         if (rc) return rc;
     }
