@@ -1845,6 +1845,134 @@ __global__ void k_diff6(Dom d, const real* __restrict__ f,
     out[o] = acc ? out[o] + (zero + (tx + ty)) : zero + (tx + ty);
 }
 
+// The filter's four rows of the step's own fields -- u (variant 1), v (2),
+// w (mass, zero on the boundary levels) and theta' (mass) -- in one launch
+// over the union grid: each point gathers the column masses of its 3 x 3
+// neighbourhood and, on a tapered terrain base, the geopotential taps
+// once for every row it owns.  Per row the arithmetic is k_diff6's, in
+// its order.
+__global__ void k_diff6_rows(Dom d, real coef, int mono, int slope,
+                             real dzthr_x, real dzthr_y, int acc)
+{
+    const int ny = d.ny, nx = d.nx, nz = d.nz;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    const bool du = j < ny && k < nz;               // u face (i <= nx)
+    const bool dv = i < nx && k < nz;               // v face (j <= ny)
+    const bool dw = i < nx && j < ny;               // w level (k <= nz)
+    const bool dt = dw && k < nz;                   // mass point
+    if (!(du || dv || dw)) return;
+    const int ic = (i == nx) ? 0 : i, jc = (j == ny) ? 0 : j;
+    const real ten = R(10.0), five = R(5.0), zero = R(0.0), one = R(1.0);
+    const real q = R(0.25), h = R(0.5);
+    real m[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const int c = pmod(jc + a - 1, ny) * nx + pmod(ic + b - 1, nx);
+            m[a][b] = d.mub2d[c] + d.mup[c];
+        }
+    // geopotential taps (jc + a - 2, ic + b - 2), a, b in 0..3
+    real ph[4][4];
+    const bool tap = slope && k < nz;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            ph[a][b] = tap ? d.phb[((size_t)k * ny + pmod(jc + a - 2, ny)) * nx
+                                   + pmod(ic + b - 2, nx)]
+                           : zero;
+#define MU(dj, di) m[(dj) + 1][(di) + 1]
+#define PHB(dj, di) ph[(dj) + 2][(di) + 2]
+    auto row = [&](const real* f, real* out, size_t o, int nys, int nxs,
+                   real cc1, real cc2, real mx0, real mx1, real my0, real my1,
+                   real sx0, real sx1, real sy0, real sy1) {
+        auto FX = [&](int mm) {
+            return f[((size_t)k * nys + jc) * nxs + pmod(ic + mm, nx)];
+        };
+        auto FY = [&](int mm) {
+            return f[((size_t)k * nys + pmod(jc + mm, ny)) * nxs + ic];
+        };
+        real px0 = ten * (FX(0) - FX(-1)) - five * (FX(1) - FX(-2)) + (FX(2) - FX(-3));
+        real px1 = ten * (FX(1) - FX(0)) - five * (FX(2) - FX(-1)) + (FX(3) - FX(-2));
+        if (mono == 2) {
+            if (px0 * (FX(0) - FX(-1)) <= zero) px0 = zero;
+            if (px1 * (FX(1) - FX(0)) <= zero) px1 = zero;
+        }
+        const real tx = coef * (sx1 * (cc1 * mx1 + cc2) * px1
+                                - sx0 * (cc1 * mx0 + cc2) * px0);
+        real py0 = ten * (FY(0) - FY(-1)) - five * (FY(1) - FY(-2)) + (FY(2) - FY(-3));
+        real py1 = ten * (FY(1) - FY(0)) - five * (FY(2) - FY(-1)) + (FY(3) - FY(-2));
+        if (mono == 2) {
+            if (py0 * (FY(0) - FY(-1)) <= zero) py0 = zero;
+            if (py1 * (FY(1) - FY(0)) <= zero) py1 = zero;
+        }
+        const real ty = coef * (sy1 * (cc1 * my1 + cc2) * py1
+                                - sy0 * (cc1 * my0 + cc2) * py0);
+        out[o] = acc ? out[o] + (zero + (tx + ty)) : zero + (tx + ty);
+    };
+    // the slope taper of a row: a0/a1 (b0/b1) the x (y) face steps, d0/d1
+    // their variant's maxima (k_diff6)
+    const real a0 = fabs(PHB(0, 0) - PHB(0, -1));
+    const real a1 = fabs(PHB(0, 1) - PHB(0, 0));
+    const real b0 = fabs(PHB(0, 0) - PHB(-1, 0));
+    const real b1 = fabs(PHB(1, 0) - PHB(0, 0));
+    auto taper = [&](real dd, real thr) { return fmax(one - dd / thr, zero); };
+    if (du) {
+        real sx0 = one, sx1 = one, sy0 = one, sy1 = one;
+        if (slope) {
+            sx0 = taper(fmax(a0, fabs(PHB(0, -1) - PHB(0, -2))), dzthr_x);
+            sx1 = taper(fmax(a1, a0), dzthr_x);
+            sy0 = taper(fmax(b0, fabs(PHB(0, -1) - PHB(-1, -1))), dzthr_y);
+            sy1 = taper(fmax(b1, fabs(PHB(1, -1) - PHB(0, -1))), dzthr_y);
+        }
+        row(d.u, d.d6_u, ((size_t)k * ny + j) * (nx + 1) + i, ny, nx + 1,
+            d.c1h[k], d.c2h[k], MU(0, -1), MU(0, 0),
+            q * (MU(-1, -1) + MU(-1, 0) + MU(0, -1) + MU(0, 0)),
+            q * (MU(0, -1) + MU(0, 0) + MU(1, -1) + MU(1, 0)),
+            sx0, sx1, sy0, sy1);
+    }
+    if (dv) {
+        real sx0 = one, sx1 = one, sy0 = one, sy1 = one;
+        if (slope) {
+            sx0 = taper(fmax(a0, fabs(PHB(-1, 0) - PHB(-1, -1))), dzthr_x);
+            sx1 = taper(fmax(a1, fabs(PHB(-1, 1) - PHB(-1, 0))), dzthr_x);
+            sy0 = taper(fmax(b0, fabs(PHB(-1, 0) - PHB(-2, 0))), dzthr_y);
+            sy1 = taper(fmax(b1, b0), dzthr_y);
+        }
+        row(d.v, d.d6_v, ((size_t)k * (ny + 1) + j) * nx + i, ny + 1, nx,
+            d.c1h[k], d.c2h[k],
+            q * (MU(-1, -1) + MU(-1, 0) + MU(0, -1) + MU(0, 0)),
+            q * (MU(-1, 0) + MU(-1, 1) + MU(0, 0) + MU(0, 1)),
+            MU(-1, 0), MU(0, 0), sx0, sx1, sy0, sy1);
+    }
+    if (dw || dt) {
+        real sx0 = one, sx1 = one, sy0 = one, sy1 = one;
+        if (slope) {
+            sx0 = taper(a0, dzthr_x);
+            sx1 = taper(a1, dzthr_x);
+            sy0 = taper(b0, dzthr_y);
+            sy1 = taper(b1, dzthr_y);
+        }
+        const real mx0 = h * (MU(0, -1) + MU(0, 0)), mx1 = h * (MU(0, 0) + MU(0, 1));
+        const real my0 = h * (MU(-1, 0) + MU(0, 0)), my1 = h * (MU(0, 0) + MU(1, 0));
+        const size_t o = ((size_t)k * ny + j) * nx + i;
+        if (k == 0 || k == nz)
+            d.d6_w[o] = acc ? d.d6_w[o] + R(0.0) : R(0.0);
+        else
+            row(d.w, d.d6_w, o, ny, nx, d.c1f[k], d.c2f[k], mx0, mx1, my0,
+                my1, sx0, sx1, sy0, sy1);
+        if (dt)
+            row(d.thp, d.d6_t, o, ny, nx, d.c1h[k], d.c2h[k], mx0, mx1, my0,
+                my1, sx0, sx1, sy0, sy1);
+    }
+#undef MU
+#undef PHB
+}
+
 // The filter's species rows (mass points, plain periodic) in groups of
 // up to four: the column masses, slope taper and C(mu) face factors of
 // the point are formed once and shared; per species the arithmetic is
@@ -1920,20 +2048,11 @@ int32_t held_diff6(const Dom& d, const arw_params* prm, hipStream_t st,
     const real tx = (real)(prm->diff_6th_thresh * 9.81 * prm->dx);
     const real ty = (real)(prm->diff_6th_thresh * 9.81 * prm->dy);
     const int mono = prm->diff_6th_opt;
-    struct Row { const real* f; real* out; const real *c1, *c2; int nlev, nys, nxs, var, wst; };
-    const Row rows[4] = {
-        {d.u, d.d6_u, d.c1h, d.c2h, nz, ny, nx + 1, 1, 0},
-        {d.v, d.d6_v, d.c1h, d.c2h, nz, ny + 1, nx, 2, 0},
-        {d.w, d.d6_w, d.c1f, d.c2f, nz + 1, ny, nx, 0, 1},
-        {d.thp, d.d6_t, d.c1h, d.c2h, nz, ny, nx, 0, 0}};
-    for (const Row& r : rows) {
-        hipLaunchKernelGGL(k_diff6, dim3((r.nxs + 63) / 64, (r.nys + 3) / 4, r.nlev),
-                           dim3(64, 4), 0, st, d, r.f, r.out, r.c1, r.c2, coef,
-                           mono, slope, tx, ty, r.nlev, r.nys, r.nxs, r.var,
-                           r.wst, acc);
-        int32_t rc = launched(hipGetLastError(), "k_diff6", KID_DIFF6, st);
-        if (rc) return rc;
-    }
+    hipLaunchKernelGGL(k_diff6_rows, dim3((nx + 1 + 63) / 64, (ny + 1 + 3) / 4,
+                                          nz + 1),
+                       dim3(64, 4), 0, st, d, coef, mono, slope, tx, ty, acc);
+    if (int32_t rc = launched(hipGetLastError(), "k_diff6", KID_DIFF6, st))
+        return rc;
     // the transported species the filter acts on (mass rows, reference
     // dycore.py:1387-1415), into their held slots
     const size_t n = (size_t)nz * ny * nx;
