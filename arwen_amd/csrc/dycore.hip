@@ -1743,114 +1743,15 @@ int32_t pd_forward(const Dom* d, const arw_params* prm, double dt_stage,
 
 // WRF sixth_order_diffusion (reference kernels/diff6.cu, launched by
 // dycore.py:1647-1740 from prepare_fixed_tendencies :1345-1420): the
-// coupled tendency of one periodic field f (nlev, nys, nxs) from Xue's
-// (2000) eq. 3 face fluxes, the mono = 2 up-gradient knock-down, the face
-// masses of the u (variant 1), v (2) and mass/w (0) positions and the
-// diff_6th_slopeopt taper on a terrain base (msf = 1).  out = 0 + (x part
-// + y part); a w field's boundary levels (wstag) get 0.
-__global__ void k_diff6(Dom d, const real* __restrict__ f,
-                        real* __restrict__ out, const real* __restrict__ c1,
-                        const real* __restrict__ c2, real coef, int mono,
-                        int slope, real dzthr_x, real dzthr_y, int nlev,
-                        int nys, int nxs, int variant, int wstag, int acc)
-{
-    const int ny = d.ny, nx = d.nx;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
-    if (i >= nxs || j >= nys || k >= nlev) return;
-    const size_t o = ((size_t)k * nys + j) * nxs + i;
-    if (wstag && (k == 0 || k == nlev - 1)) {
-        out[o] = acc ? out[o] + R(0.0) : R(0.0);
-        return;
-    }
-    const int ic = i % nx, jc = j % ny;
-    auto FX = [&](int m) { return f[((size_t)k * nys + jc) * nxs + pmod(ic + m, nx)]; };
-    auto FY = [&](int m) { return f[((size_t)k * nys + pmod(jc + m, ny)) * nxs + ic]; };
-    auto MU = [&](int jj, int ii) {
-        const int c = pmod(jj, ny) * nx + pmod(ii, nx);
-        return d.mub2d[c] + d.mup[c];
-    };
-    auto PHB = [&](int jj, int ii) {
-        return d.phb[((size_t)k * ny + pmod(jj, ny)) * nx + pmod(ii, nx)];
-    };
-    const real ten = R(10.0), five = R(5.0), zero = R(0.0), one = R(1.0);
-    real px0 = ten * (FX(0) - FX(-1)) - five * (FX(1) - FX(-2)) + (FX(2) - FX(-3));
-    real px1 = ten * (FX(1) - FX(0)) - five * (FX(2) - FX(-1)) + (FX(3) - FX(-2));
-    if (mono == 2) {
-        if (px0 * (FX(0) - FX(-1)) <= zero) px0 = zero;
-        if (px1 * (FX(1) - FX(0)) <= zero) px1 = zero;
-    }
-    real mx0, mx1;
-    if (variant == 1) {
-        mx0 = MU(jc, ic - 1);
-        mx1 = MU(jc, ic);
-    } else if (variant == 2) {
-        mx0 = R(0.25) * (MU(jc - 1, ic - 1) + MU(jc - 1, ic) + MU(jc, ic - 1) + MU(jc, ic));
-        mx1 = R(0.25) * (MU(jc - 1, ic) + MU(jc - 1, ic + 1) + MU(jc, ic) + MU(jc, ic + 1));
-    } else {
-        mx0 = R(0.5) * (MU(jc, ic - 1) + MU(jc, ic));
-        mx1 = R(0.5) * (MU(jc, ic) + MU(jc, ic + 1));
-    }
-    real sx0 = one, sx1 = one, sy0 = one, sy1 = one;
-    if (slope) {
-        const real a0 = fabs(PHB(jc, ic) - PHB(jc, ic - 1));
-        const real a1 = fabs(PHB(jc, ic + 1) - PHB(jc, ic));
-        real d0 = a0, d1 = a1;
-        if (variant == 1) {
-            d0 = fmax(a0, fabs(PHB(jc, ic - 1) - PHB(jc, ic - 2)));
-            d1 = fmax(a1, a0);
-        } else if (variant == 2) {
-            d0 = fmax(a0, fabs(PHB(jc - 1, ic) - PHB(jc - 1, ic - 1)));
-            d1 = fmax(a1, fabs(PHB(jc - 1, ic + 1) - PHB(jc - 1, ic)));
-        }
-        sx0 = fmax(one - d0 / dzthr_x, zero);
-        sx1 = fmax(one - d1 / dzthr_x, zero);
-        const real b0 = fabs(PHB(jc, ic) - PHB(jc - 1, ic));
-        const real b1 = fabs(PHB(jc + 1, ic) - PHB(jc, ic));
-        d0 = b0; d1 = b1;
-        if (variant == 1) {
-            d0 = fmax(b0, fabs(PHB(jc, ic - 1) - PHB(jc - 1, ic - 1)));
-            d1 = fmax(b1, fabs(PHB(jc + 1, ic - 1) - PHB(jc, ic - 1)));
-        } else if (variant == 2) {
-            d0 = fmax(b0, fabs(PHB(jc - 1, ic) - PHB(jc - 2, ic)));
-            d1 = fmax(b1, b0);
-        }
-        sy0 = fmax(one - d0 / dzthr_y, zero);
-        sy1 = fmax(one - d1 / dzthr_y, zero);
-    }
-    const real tx = coef * (sx1 * (c1[k] * mx1 + c2[k]) * px1
-                            - sx0 * (c1[k] * mx0 + c2[k]) * px0);
-    real py0 = ten * (FY(0) - FY(-1)) - five * (FY(1) - FY(-2)) + (FY(2) - FY(-3));
-    real py1 = ten * (FY(1) - FY(0)) - five * (FY(2) - FY(-1)) + (FY(3) - FY(-2));
-    if (mono == 2) {
-        if (py0 * (FY(0) - FY(-1)) <= zero) py0 = zero;
-        if (py1 * (FY(1) - FY(0)) <= zero) py1 = zero;
-    }
-    real my0, my1;
-    if (variant == 1) {
-        my0 = R(0.25) * (MU(jc - 1, ic - 1) + MU(jc - 1, ic) + MU(jc, ic - 1) + MU(jc, ic));
-        my1 = R(0.25) * (MU(jc, ic - 1) + MU(jc, ic) + MU(jc + 1, ic - 1) + MU(jc + 1, ic));
-    } else if (variant == 2) {
-        my0 = MU(jc - 1, ic);
-        my1 = MU(jc, ic);
-    } else {
-        my0 = R(0.5) * (MU(jc - 1, ic) + MU(jc, ic));
-        my1 = R(0.5) * (MU(jc, ic) + MU(jc + 1, ic));
-    }
-    const real ty = coef * (sy1 * (c1[k] * my1 + c2[k]) * py1
-                            - sy0 * (c1[k] * my0 + c2[k]) * py0);
-    // a held Smagorinsky tendency in out takes the filter's on top
-    // (reference dycore.py:1415, slot += tmp)
-    out[o] = acc ? out[o] + (zero + (tx + ty)) : zero + (tx + ty);
-}
-
-// The filter's four rows of the step's own fields -- u (variant 1), v (2),
-// w (mass, zero on the boundary levels) and theta' (mass) -- in one launch
-// over the union grid: each point gathers the column masses of its 3 x 3
-// neighbourhood and, on a tapered terrain base, the geopotential taps
-// once for every row it owns.  Per row the arithmetic is k_diff6's, in
-// its order.
+// coupled tendency of a periodic field from Xue's (2000) eq. 3 face
+// fluxes, the mono = 2 up-gradient knock-down, the face masses of the u
+// (variant 1), v (variant 2) and mass/w positions and the
+// diff_6th_slopeopt taper on a terrain base (msf = 1); out = 0 + (x part
+// + y part), or added to a held Smagorinsky tendency already in out.
+// The four rows of the step's own fields -- u, v, w (zero on the boundary
+// levels) and theta' -- run in one launch over the union grid: each point
+// gathers the column masses of its 3 x 3 neighbourhood and, tapered, the
+// geopotential taps once for every row it owns.
 __global__ void k_diff6_rows(Dom d, real coef, int mono, int slope,
                              real dzthr_x, real dzthr_y, int acc)
 {
@@ -1914,8 +1815,8 @@ __global__ void k_diff6_rows(Dom d, real coef, int mono, int slope,
                                 - sy0 * (cc1 * my0 + cc2) * py0);
         out[o] = acc ? out[o] + (zero + (tx + ty)) : zero + (tx + ty);
     };
-    // the slope taper of a row: a0/a1 (b0/b1) the x (y) face steps, d0/d1
-    // their variant's maxima (k_diff6)
+    // the slope taper of a row: a0/a1 (b0/b1) the x (y) face steps, taken
+    // with the variant's neighbouring steps (diff6.cu's d0/d1)
     const real a0 = fabs(PHB(0, 0) - PHB(0, -1));
     const real a1 = fabs(PHB(0, 1) - PHB(0, 0));
     const real b0 = fabs(PHB(0, 0) - PHB(-1, 0));
@@ -1976,7 +1877,7 @@ __global__ void k_diff6_rows(Dom d, real coef, int mono, int slope,
 // The filter's species rows (mass points, plain periodic) in groups of
 // up to four: the column masses, slope taper and C(mu) face factors of
 // the point are formed once and shared; per species the arithmetic is
-// k_diff6's, in its order.
+// k_diff6_rows' mass row, in its order.
 struct D6Q {
     const real* f[4];
     real* out[4];
