@@ -301,6 +301,14 @@ __device__ __forceinline__ int xcd_block()
     return (b % ARW_NXCD) * (n / ARW_NXCD) + b / ARW_NXCD;
 }
 
+// The same numbering over the first n workgroups of a launch whose later
+// workgroups do other work (n a multiple of ARW_NXCD).
+__device__ __forceinline__ int xcd_block_n(int n)
+{
+    const int b = (int)blockIdx.x;
+    return (b % ARW_NXCD) * (n / ARW_NXCD) + b / ARW_NXCD;
+}
+
 static inline unsigned xcd_grid(unsigned nblocks)
 {
     return (nblocks + ARW_NXCD - 1) / ARW_NXCD * ARW_NXCD;

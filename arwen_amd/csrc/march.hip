@@ -1759,6 +1759,43 @@ struct FinIn {
     real pb = R(0.0);                                // terrain, PREC
 };
 
+// Periodic duplicate faces of the finish: u at i = nx (t < ny, row t),
+// v at j = ny (ny <= t < ny + nx, column t - ny).  They read their own
+// face's inputs only, so they run beside the interior columns.
+__device__ __forceinline__ void finish_edges(const Dom& d, const FinishDst& o,
+                                             int t)
+{
+    const int ny = d.ny, nx = d.nx;
+    const real* RS mub = d.mub2d;
+    const real* RS mup = d.mup;
+    const real* RS mu_pp = d.mu_pp;
+    if (t < ny) {
+        const int j = t, cA = j * nx, cB = j * nx + nx - 1;
+        const real msA = mub[cA] + mup[cA], mnA = msA + mu_pp[cA];
+        const real msB = mub[cB] + mup[cB], mnB = msB + mu_pp[cB];
+        finish_face<0>(d, o.u, d.u, d.u_pp, j * (nx + 1) + nx,
+                       R(0.5) * (msA + msB), R(0.5) * (mnA + mnB));
+    } else if (t < ny + nx) {
+        const int i = t - ny, cA = i, cB = (ny - 1) * nx + i;
+        const real msA = mub[cA] + mup[cA], mnA = msA + mu_pp[cA];
+        const real msB = mub[cB] + mup[cB], mnB = msB + mu_pp[cB];
+        finish_face<1>(d, o.v, d.v, d.v_pp, ny * nx + i,
+                       R(0.5) * (msA + msB), R(0.5) * (mnA + mnB));
+    }
+}
+
+// The batched launch's duplicate faces (one domain's run inside k_finish).
+template <class SRC>
+__global__ __launch_bounds__(256) void k_finish_edge(typename SRC::Arg a,
+                                                     FinishDst o_arg)
+{
+    int m, g;
+    if (!SRC::locate(a, m, g)) return;
+    const Dom& d = SRC::dom(a, m);
+    const FinishDst& o = fin_of<SRC>(a, m, o_arg);
+    finish_edges(d, o, g * blockDim.x + threadIdx.x);
+}
+
 // Small-step finish (reference dycore.cu:1087-1198) of the mass column
 // (j, i), its u face (j, i) and v face (j, i), then the EOS of the new
 // state (diagnostics.cu:37-89).  PREC (precision form): the new theta' is
@@ -1770,15 +1807,21 @@ struct FinIn {
 // (reference dycore.cu:1186-1194, module_small_step_em.F:408-426).
 template <class SRC, bool B3, bool EOS, bool PREC, bool XO = true>
 __global__ __launch_bounds__(256) void k_finish(typename SRC::Arg a,
-                                                FinishDst o_arg, real hdt)
+                                                FinishDst o_arg, real hdt,
+                                                int nmain)
 {
     int m, g;
     if (!SRC::locate(a, m, g)) return;
     const Dom& d = SRC::dom(a, m);
     const FinishDst& o = fin_of<SRC>(a, m, o_arg);
+    if (!SRC::many && (int)blockIdx.x >= nmain) {
+        // the launch's last workgroups: the periodic duplicate faces
+        finish_edges(d, o, ((int)blockIdx.x - nmain) * blockDim.x + threadIdx.x);
+        return;
+    }
     const int nz = d.nz, ny = d.ny, nx = d.nx, st = ny * nx;
     const int stu = ny * (nx + 1), stv = (ny + 1) * nx;
-    const int c = (SRC::many ? g : xcd_block()) * blockDim.x + threadIdx.x;
+    const int c = (SRC::many ? g : xcd_block_n(nmain)) * blockDim.x + threadIdx.x;
     if (c >= st) return;
     const int j = c / nx, i = c - j * nx;
     const int im = (i == 0) ? nx - 1 : i - 1;
@@ -1900,34 +1943,6 @@ __global__ __launch_bounds__(256) void k_finish(typename SRC::Arg a,
     o.mup[c] = mup_new;
 }
 
-// Periodic duplicate faces of the finish: u at i = nx, v at j = ny.
-template <class SRC>
-__global__ __launch_bounds__(256) void k_finish_edge(typename SRC::Arg a,
-                                                     FinishDst o_arg)
-{
-    int m, g;
-    if (!SRC::locate(a, m, g)) return;
-    const Dom& d = SRC::dom(a, m);
-    const FinishDst& o = fin_of<SRC>(a, m, o_arg);
-    const int ny = d.ny, nx = d.nx;
-    const int t = g * blockDim.x + threadIdx.x;
-    const real* RS mub = d.mub2d;
-    const real* RS mup = d.mup;
-    const real* RS mu_pp = d.mu_pp;
-    if (t < ny) {
-        const int j = t, cA = j * nx, cB = j * nx + nx - 1;
-        const real msA = mub[cA] + mup[cA], mnA = msA + mu_pp[cA];
-        const real msB = mub[cB] + mup[cB], mnB = msB + mu_pp[cB];
-        finish_face<0>(d, o.u, d.u, d.u_pp, j * (nx + 1) + nx,
-                       R(0.5) * (msA + msB), R(0.5) * (mnA + mnB));
-    } else if (t < ny + nx) {
-        const int i = t - ny, cA = i, cB = (ny - 1) * nx + i;
-        const real msA = mub[cA] + mup[cA], mnA = msA + mu_pp[cA];
-        const real msB = mub[cB] + mup[cB], mnB = msB + mu_pp[cB];
-        finish_face<1>(d, o.v, d.v, d.v_pp, ny * nx + i,
-                       R(0.5) * (msA + msB), R(0.5) * (mnA + mnB));
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Scalar transport of one RK stage as column marches (reference moist.py:
@@ -2412,19 +2427,21 @@ static void launch_tend_cols(const typename SRC::Arg& a, dim3 g, const Sc& s,
         hipLaunchKernelGGL((k_tend_cols<SRC, false, false, MOIST, PREC, false, XO>), g, b, 0, st, a, s);
 }
 
+// g: the whole launch; nmain: its leading column workgroups (one domain:
+// the duplicate-face workgroups follow them; batched: unused)
 template <class SRC, bool PREC, bool XO = true>
 static void launch_finish(const typename SRC::Arg& a, dim3 g,
                           const FinishDst& o, bool b3, bool eos,
-                          hipStream_t st, real hdt = R(0.0))
+                          hipStream_t st, real hdt = R(0.0), int nmain = 0)
 {
     if (b3 && eos)
-        hipLaunchKernelGGL((k_finish<SRC, true, true, PREC, XO>), g, dim3(256), 0, st, a, o, hdt);
+        hipLaunchKernelGGL((k_finish<SRC, true, true, PREC, XO>), g, dim3(256), 0, st, a, o, hdt, nmain);
     else if (b3)
-        hipLaunchKernelGGL((k_finish<SRC, true, false, PREC, XO>), g, dim3(256), 0, st, a, o, hdt);
+        hipLaunchKernelGGL((k_finish<SRC, true, false, PREC, XO>), g, dim3(256), 0, st, a, o, hdt, nmain);
     else if (eos)
-        hipLaunchKernelGGL((k_finish<SRC, false, true, PREC, XO>), g, dim3(256), 0, st, a, o, hdt);
+        hipLaunchKernelGGL((k_finish<SRC, false, true, PREC, XO>), g, dim3(256), 0, st, a, o, hdt, nmain);
     else
-        hipLaunchKernelGGL((k_finish<SRC, false, false, PREC, XO>), g, dim3(256), 0, st, a, o, hdt);
+        hipLaunchKernelGGL((k_finish<SRC, false, false, PREC, XO>), g, dim3(256), 0, st, a, o, hdt, nmain);
 }
 
 // The held-mixing variants: one domain, plain form (arw_step's held
@@ -2581,17 +2598,18 @@ hipError_t run_finish(const Dom& d, const FinishDst& o, bool prec,
     const bool eos = d.qv == nullptr;
     const real hdt = d.h_diabatic ? (real)hdiab_dt : R(0.0);
     const bool xo = prec || d.hypsometric_opt == 2 || d.h_diabatic != nullptr;
+    // the column workgroups, then those of the duplicate faces
+    const int nmain = (int)cols(d).x;
+    const dim3 g(nmain + (d.ny + d.nx + 255) / 256);
     if (xo) {
         with_prec(prec, [&](auto p) {
-            launch_finish<One, decltype(p)::value>(d, cols(d), o,
-                                                   d.base3d != 0, eos, st, hdt);
+            launch_finish<One, decltype(p)::value>(d, g, o, d.base3d != 0,
+                                                   eos, st, hdt, nmain);
         });
     } else {
-        launch_finish<One, false, false>(d, cols(d), o, d.base3d != 0, eos, st,
-                                         hdt);
+        launch_finish<One, false, false>(d, g, o, d.base3d != 0, eos, st,
+                                         hdt, nmain);
     }
-    hipLaunchKernelGGL(k_finish_edge<One>, dim3((d.ny + d.nx + 255) / 256),
-                       dim3(256), 0, st, d, o);
     return hipGetLastError();
 }
 
