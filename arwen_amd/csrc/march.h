@@ -104,7 +104,7 @@ int32_t lib_fail(int32_t code, const char* msg);
 
 // WRF km_opt = 4 held tendencies of the time-t fields of d into out_u,
 // out_v, out_w, out_t and of the nq species q_src into held_q (smag.hip);
-// buf: 8 nz ny nx + 5 (nz+1) ny nx elements of scratch
+// buf: 9 nz ny nx + 5 (nz+1) ny nx elements of scratch
 hipError_t run_smag(const Dom& d, const arw_params* prm, real* buf,
                     real* out_u, real* out_v, real* out_w, real* out_t,
                     const real* const* q_src, real* held_q, int nq,

@@ -9,8 +9,9 @@
 //   tau       tau11, tau22, tau12 in place of D   (pointwise)
 //   tauw      tau13, tau23 (u / v faces of w levels), tau33 (mass)
 //   u, v, w   stress divergences + vertical divergences + wall stress
-//   flux      the theta face fluxes H1, H2 (into tau11 / tau22's slots)
-//   theta     the flux divergence + the surface heat flux
+//   fluxq     the scalar rows' face fluxes H1, H2: theta's t_2 (staged by
+//             the metric pass) and each species, three rows per pass
+//   divq      their divergences, theta's with the surface heat flux
 // The stresses are stored once instead of being re-derived at every
 // neighbour as the reference's on-demand device functions do; each
 // stored value is the same expression in the same order, so the results
@@ -35,6 +36,7 @@ struct SmagBuf {
     real *t11, *t22, *t12, *km, *kh, *t33, *t13, *t23;
     // the metrics of the time-t geopotential and the density, stored once
     real *rdzw, *rho, *rdz, *zx, *zy;
+    real* t2;    // theta's t_2 = theta - 300
 };
 
 // periodic wrap of an index within [-1, n]: every neighbour these kernels
@@ -200,6 +202,7 @@ SMAG_BODY(smag_metric_at, const Dom& d, const SmagSc& c, const SmagBuf& b)
     if (k < d.nz) {
         b.rdzw[x] = rdzw_of(d, k, j, i);
         b.rho[x] = rho_of(d, k, j, i);
+        b.t2[x] = ((d.base3d ? d.thb[x] : d.thb[k]) + d.thp[x]) - R(300.0);
     }
     b.rdz[x] = rdz_of(d, k, j, i);
     b.zx[x] = zx_of(d, c.rdx, k, j, i);
@@ -470,87 +473,20 @@ SMAG_BODY(smag_w_at, const Dom& d, const SmagSc& c, const SmagBuf& b, real* __re
 SMAG_POINTS(k_smag_w, smag_w_at, d.nx, d.ny, d.nz + 1,
            (Dom d, SmagSc c, SmagBuf b, real* __restrict__ out), d, c, b, out)
 
-// t_2 = theta - 300 from the stored perturbation (smag2d.cu:1105-1119)
-__device__ __forceinline__ real theta2(const Dom& d, int k, int j, int i)
-{
-    const int x = mix(d, k, j, i);
-    return (( d.base3d ? d.thb[x] : d.thb[k]) + d.thp[x]) - R(300.0);
-}
-
-// the scalar face fluxes H1 (u face i) and H2 (v face j) of theta's t_2
-// (src null) or of a species' mixing ratio src (smag2d.cu:1164-1200,
-// wrf_smag_flux_s :1236-1251)
-SMAG_BODY(smag_flux_at, const Dom& d, const SmagSc& c, const SmagBuf& b, const real* __restrict__ src)
-{
-    const real h = R(0.5), one = R(1.0), two = R(2.0);
-    const int x = iM(k, j, i, d.ny, d.nx);
-    auto sv = [&](int kk, int jj, int ii) {
-        return src ? src[mix(d, kk, jj, ii)] : theta2(d, kk, jj, ii);
-    };
-    const real s0 = sv(k, j, i);
-    const real rz = rdzw(d, b, k, j, i);
-    {
-        const real ra = h * (rho(d, b, k, j, i - 1) + rho(d, b, k, j, i));
-        const real ka = h * (at(d, b.kh, k, j, i - 1) + b.kh[x]);
-        const real tz = h * (zx(d, b, k, j, i) + zx(d, b, k + 1, j, i));
-        const real rdzu = two / (one / rz + one / rdzw(d, b, k, j, i - 1));
-        auto p = [&](int kk) { return sv(kk, j, i - 1) + sv(kk, j, i); };
-        const real grad = c.rdx * (s0 - sv(k, j, i - 1))
-                          - (tz * (full(d, k + 1, p) - full(d, k, p))) * rdzu;
-        b.t11[x] = ((-ra) * ka) * grad;
-    }
-    {
-        const real ra = h * (rho(d, b, k, j - 1, i) + rho(d, b, k, j, i));
-        const real ka = h * (at(d, b.kh, k, j - 1, i) + b.kh[x]);
-        const real tz = h * (zy(d, b, k, j, i) + zy(d, b, k + 1, j, i));
-        const real rdzv = two / (one / rz + one / rdzw(d, b, k, j - 1, i));
-        auto p = [&](int kk) { return sv(kk, j - 1, i) + sv(kk, j, i); };
-        const real grad = c.rdy * (s0 - sv(k, j - 1, i))
-                          - (tz * (full(d, k + 1, p) - full(d, k, p))) * rdzv;
-        b.t22[x] = ((-ra) * ka) * grad;
-    }
-}
-SMAG_POINTS(k_smag_flux, smag_flux_at, d.nx, d.ny, d.nz,
-           (Dom d, SmagSc c, SmagBuf b, const real* __restrict__ src), d, c, b, src)
-
-// a scalar's flux divergence (wrf_smag_hd_s, smag2d.cu:1254-1292), with
-// theta's constant surface heat flux under isfflx 0 / 2 when heat is set
-// (:1658-1668)
-SMAG_BODY(smag_theta_at, const Dom& d, const SmagSc& c, const SmagBuf& b, real* __restrict__ out, int heat)
-{
-    const real q = R(0.25);
-    const int x = iM(k, j, i, d.ny, d.nx);
-    const real* fx = b.t11;
-    const real* fy = b.t22;
-    const real zxm = q * (((zx(d, b, k, j, i) + zx(d, b, k, j, i + 1))
-                           + zx(d, b, k + 1, j, i)) + zx(d, b, k + 1, j, i + 1));
-    const real zym = q * (((zy(d, b, k, j, i) + zy(d, b, k, j + 1, i))
-                           + zy(d, b, k + 1, j, i)) + zy(d, b, k + 1, j + 1, i));
-    const real rz = rdzw(d, b, k, j, i);
-    const real divh = c.rdx * (at(d, fx, k, j, i + 1) - fx[x])
-                      + c.rdy * (at(d, fy, k, j + 1, i) - fy[x]);
-    const real divz = (zxm * (mavg(d, fx, k + 1, j, i, 0, 1)
-                              - mavg(d, fx, k, j, i, 0, 1))) * rz
-                      + (zym * (mavg(d, fy, k + 1, j, i, 1, 0)
-                                - mavg(d, fy, k, j, i, 1, 0))) * rz;
-    real r = R(0.0) + (G_F / (d.dnw[k] * rz)) * (divh - divz);
-    if (heat && c.heatflux && k == 0)
-        r = r - ((G_F * c.heat) * rho(d, b, 0, j, i)) / d.dnw[0];
-    out[x] = r;
-}
-SMAG_POINTS(k_smag_theta, smag_theta_at, d.nx, d.ny, d.nz,
-           (Dom d, SmagSc c, SmagBuf b, real* __restrict__ out, int heat), d, c, b, out, heat)
-
-// The species rows in groups of up to three (the work area's free slots
-// hold their fluxes): one pass forms every species' face fluxes, sharing
-// the density, K_h and metric loads of the point, and one pass their
-// divergences.  Per species the arithmetic is smag_flux_at's and
-// smag_theta_at's, in their order.
+// The scalar rows -- theta's t_2 = theta - 300 (staged by the metric pass
+// from the stored perturbation, smag2d.cu:1105-1119) and every species'
+// mixing ratio -- in groups of up to three, the work area's free slots
+// holding their fluxes: one pass forms each row's face fluxes H1 (u face
+// i), H2 (v face j) (smag2d.cu:1164-1200, wrf_smag_flux_s :1236-1251),
+// sharing the density, K_h and metric loads of the point, and one pass
+// their divergences (wrf_smag_hd_s :1254-1292), theta's with the constant
+// surface heat flux under isfflx 0 / 2 (:1658-1668).
 struct SmagQ {
     const real* src[3];
     real* fx[3];
     real* fy[3];
     real* out[3];
+    int heat0;               // row 0 is theta: add the surface heat flux
 };
 
 template <int NS>
@@ -612,7 +548,10 @@ __global__ void k_smag_divq(Dom d, SmagSc c, SmagBuf b, SmagQ q)
                                   - mavg(d, fx, k, j, i, 0, 1))) * rz
                           + (zym * (mavg(d, fy, k + 1, j, i, 1, 0)
                                     - mavg(d, fy, k, j, i, 1, 0))) * rz;
-        q.out[e][x] = R(0.0) + fac * (divh - divz);
+        real r = R(0.0) + fac * (divh - divz);
+        if (e == 0 && q.heat0 && c.heatflux && k == 0)
+            r = r - ((G_F * c.heat) * rho(d, b, 0, j, i)) / d.dnw[0];
+        q.out[e][x] = r;
     }
 }
 
@@ -621,7 +560,7 @@ __global__ void k_smag_divq(Dom d, SmagSc c, SmagBuf b, SmagQ q)
 // The held km_opt = 4 tendencies of the time-t fields of d (u, v, w, thp,
 // php, the EOS's alt and, moist, qv) into out_u, out_v, out_w, out_t and
 // of the nq species q_src[e] into held_q + e nz ny nx, written (not
-// added); buf holds 8 nz ny nx + 5 (nz+1) ny nx elements of scratch.
+// added); buf holds 9 nz ny nx + 5 (nz+1) ny nx elements of scratch.
 hipError_t run_smag(const Dom& d, const arw_params* prm, real* buf,
                     real* out_u, real* out_v, real* out_w, real* out_t,
                     const real* const* q_src, real* held_q, int nq,
@@ -643,6 +582,7 @@ hipError_t run_smag(const Dom& d, const arw_params* prm, real* buf,
     b.rdz = buf + 8 * n + 2 * nw;
     b.zx = buf + 8 * n + 3 * nw;
     b.zy = buf + 8 * n + 4 * nw;
+    b.t2 = buf + 8 * n + 5 * nw;
     SmagSc c;
     c.rdx = (real)(1.0 / prm->dx);
     c.rdy = (real)(1.0 / prm->dy);
@@ -667,24 +607,24 @@ hipError_t run_smag(const Dom& d, const arw_params* prm, real* buf,
     hipLaunchKernelGGL(k_smag_u, grid(nx + 1, ny, nz), blk, 0, st, d, c, b, out_u);
     hipLaunchKernelGGL(k_smag_v, grid(nx, ny + 1, nz), blk, 0, st, d, c, b, out_v);
     hipLaunchKernelGGL(k_smag_w, grid(nx, ny, nz + 1), blk, 0, st, d, c, b, out_w);
-    hipLaunchKernelGGL(k_smag_flux, grid(nx, ny, nz), blk, 0, st, d, c, b,
-                       (const real*)nullptr);
-    hipLaunchKernelGGL(k_smag_theta, grid(nx, ny, nz), blk, 0, st, d, c, b,
-                       out_t, 1);
-    // the species in groups of three; their fluxes take the work area's
-    // slots the momentum passes are done with
+    // theta and the species in groups of three; their fluxes take the
+    // work area's slots the momentum passes are done with
     real* fxs[3] = {b.t11, b.t12, b.t13};
     real* fys[3] = {b.t22, b.t33, b.t23};
-    for (int e0 = 0; e0 < nq; e0 += 3) {
+    const int nr = 1 + nq;                      // row 0: theta
+    auto src_of = [&](int r) { return r == 0 ? (const real*)b.t2 : q_src[r - 1]; };
+    auto out_of = [&](int r) { return r == 0 ? out_t : held_q + (size_t)(r - 1) * n; };
+    for (int e0 = 0; e0 < nr; e0 += 3) {
         SmagQ q;
-        const int ns = nq - e0 < 3 ? nq - e0 : 3;
+        const int ns = nr - e0 < 3 ? nr - e0 : 3;
         for (int e = 0; e < 3; ++e) {
-            const int s = e0 + (e < ns ? e : 0);
-            q.src[e] = q_src[s];
+            const int r = e0 + (e < ns ? e : 0);
+            q.src[e] = src_of(r);
             q.fx[e] = fxs[e];
             q.fy[e] = fys[e];
-            q.out[e] = held_q + (size_t)s * n;
+            q.out[e] = out_of(r);
         }
+        q.heat0 = e0 == 0;
         const dim3 g = grid(nx, ny, nz);
         if (ns == 3) {
             hipLaunchKernelGGL(k_smag_fluxq<3>, g, blk, 0, st, d, c, b, q);

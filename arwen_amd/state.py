@@ -339,7 +339,7 @@ class DomainState:
         if smag:
             # km_opt = 4: the deformation, K and stress work areas
             n, nw = nz * ny * nx, (nz + 1) * ny * nx
-            d.sm_buf = self.scratch((8 * n + 5 * nw,),
+            d.sm_buf = self.scratch((9 * n + 5 * nw,),
                                     "arw_sm_buf").data_ptr()
         if (diff6 or smag) and self.qv is not None:
             # every transported species' held mixing tendency

@@ -115,7 +115,7 @@ typedef struct arw_domain {
     double p_top;
     /* moist_cq's stage coefficients (u, v and w shapes; scratch) */
     float *cqu, *cqv, *cqw;
-    /* km_opt = 4 (arw_params): scratch of 8 nz ny nx + 5 (nz+1) ny nx
+    /* km_opt = 4 (arw_params): scratch of 9 nz ny nx + 5 (nz+1) ny nx
      * elements for the metrics, deformation, coefficients and stresses */
     float *sm_buf;
     /* moist states with held mixing (km_opt = 4 or diff_6th_opt): the held

@@ -154,7 +154,7 @@ def test_ctypes_offsets_match_the_c_compiler(cname, pyt, tmp_path):
 
 def test_held_mixing_scratch_and_params():
     """km_opt = 4 and the filter on a moist state: the domain view carries
-    the Smagorinsky work area (8 nz ny nx + 5 (nz+1) ny nx) and one held
+    the Smagorinsky work area (9 nz ny nx + 5 (nz+1) ny nx) and one held
     slot per species; the filter's species mask follows moist_mix6_off."""
     from arwen_amd import cases, dycore
     from arwen_amd.state import DomainState
@@ -164,7 +164,7 @@ def test_held_mixing_scratch_and_params():
     d = st.domain_struct(diff6=True, smag=True)
     nz, ny, nx = 5, 6, 8
     n, nw = nz * ny * nx, (nz + 1) * ny * nx
-    assert st._scratch["arw_sm_buf"].numel() == 8 * n + 5 * nw
+    assert st._scratch["arw_sm_buf"].numel() == 9 * n + 5 * nw
     nq = 3 + len(st.extra_species)
     assert tuple(st._scratch["arw_held_q"].shape) == (nq, nz, ny, nx)
     assert d.sm_buf and d.held_q and d.d6_u and d.d6_t
