@@ -6,8 +6,8 @@
 // Passes, each a level-parallel point kernel in 64 x 4 workgroups:
 //   deform    D11, D22 (mass), D12 (u-v corner), slope-corrected
 //   km        K_m, K_h = K_m / Pr                 (needs D12 at 4 corners)
-//   tau       tau11, tau22, tau12 in place of D   (pointwise)
-//   tauw      tau13, tau23 (u / v faces of w levels), tau33 (mass)
+//   stress    tau11, tau22, tau12 in place of D (pointwise), tau13, tau23
+//             (u / v faces of w levels), tau33 (mass)
 //   u, v, w   stress divergences + vertical divergences + wall stress
 //   fluxq     the scalar rows' face fluxes H1, H2: theta's t_2 (staged by
 //             the metric pass) and each species, three rows per pass
@@ -297,8 +297,6 @@ SMAG_BODY(smag_tau_at, const Dom& d, const SmagBuf& b)
                          + at(d, b.km, k, j - 1, i));
     b.t12[x] = ((-ra) * ka) * b.t12[x];
 }
-SMAG_POINTS(k_smag_tau, smag_tau_at, d.nx, d.ny, d.nz,
-           (Dom d, SmagBuf b), d, b)
 
 // tau13 / tau23 at the u / v faces of w level kw (0 at the surface and the
 // lid) and tau33 at mass level kw (smag2d.cu:826-899, :945-956)
@@ -355,7 +353,14 @@ SMAG_BODY(smag_tauw_at, const Dom& d, const SmagSc& c, const SmagBuf& b)
         b.t23[x] = ((-ra) * ka) * d23;
     }
 }
-SMAG_POINTS(k_smag_tauw, smag_tauw_at, d.nx, d.ny, d.nz + 1,
+// one launch for every stress: the horizontal ones of mass level k (in
+// place of the deformation) and the vertical ones of w level k
+SMAG_BODY(smag_stress_at, const Dom& d, const SmagSc& c, const SmagBuf& b)
+{
+    if (k < d.nz) smag_tau_at(d, b, k, j, i);
+    smag_tauw_at(d, c, b, k, j, i);
+}
+SMAG_POINTS(k_smag_stress, smag_stress_at, d.nx, d.ny, d.nz + 1,
            (Dom d, SmagSc c, SmagBuf b), d, c, b)
 
 // u: horizontal_diffusion_u_2 (smag2d.cu:765-795), vertical_diffusion_u_2
@@ -602,8 +607,7 @@ hipError_t run_smag(const Dom& d, const arw_params* prm, real* buf,
     hipLaunchKernelGGL(k_smag_metric, grid(nx, ny, nz + 1), blk, 0, st, d, c, b);
     hipLaunchKernelGGL(k_smag_deform, grid(nx, ny, nz), blk, 0, st, d, c, b);
     hipLaunchKernelGGL(k_smag_km, grid(nx, ny, nz), blk, 0, st, d, c, b);
-    hipLaunchKernelGGL(k_smag_tau, grid(nx, ny, nz), blk, 0, st, d, b);
-    hipLaunchKernelGGL(k_smag_tauw, grid(nx, ny, nz + 1), blk, 0, st, d, c, b);
+    hipLaunchKernelGGL(k_smag_stress, grid(nx, ny, nz + 1), blk, 0, st, d, c, b);
     hipLaunchKernelGGL(k_smag_u, grid(nx + 1, ny, nz), blk, 0, st, d, c, b, out_u);
     hipLaunchKernelGGL(k_smag_v, grid(nx, ny + 1, nz), blk, 0, st, d, c, b, out_v);
     hipLaunchKernelGGL(k_smag_w, grid(nx, ny, nz + 1), blk, 0, st, d, c, b, out_w);
